@@ -1,0 +1,148 @@
+/*
+ * lbdem.h — C ABI of the coupled D3Q19 free-surface lattice-Boltzmann / DEM timestep.
+ *
+ * Two libraries export exactly this interface:
+ *   lbm-dem_amd/lib/liblbdem.so        — the MI355X (gfx950) implementation (HIP kernels)
+ *   oracle/_build/liblbdem_refcpu.so   — the FP64 CPU oracle (test infrastructure only)
+ *
+ * The entry points replace the method surface that the reference's main loop calls
+ * (reference src/main.cpp:38-67 goCycle, src/main.cpp:262-286 initialisation):
+ *   lbdem_init            <- LB::latticeDefinition (LB.cpp:85-126),
+ *                            DEM::discreteElementInit (DEM.cpp:199-357),
+ *                            LB::latticeBolzmannInit (LB.cpp:340-398)
+ *   lbdem_dem_step        <- DEM::discreteElementStep (DEM.cpp:359-401)
+ *   lbdem_coupling_step   <- LB::latticeBoltzmannCouplingStep (LB.cpp:432-467)
+ *   lbdem_lb_step         <- LB::latticeBolzmannStep (LB.cpp:400-418)
+ *   lbdem_fs_step         <- LB::latticeBoltzmannFreeSurfaceStep (LB.cpp:422-430)
+ *   lbdem_cycle           <- goCycle (main.cpp:38-67) repeated, minus IO::outputStep
+ * Configuration values are the ones LB::latticeBoltzmannGet (LB.cpp:128-338),
+ * DEM::discreteElementGet (DEM.cpp:17-197) and parseConfigFile (main.cpp:128-185) read
+ * from the getpot file; particle/object arrays are the contents of input/particles.dat and
+ * input/objects.dat after the reader's transformations (DEM.cpp:81-165).
+ *
+ * Conventions: plain pointers and sizes; every call returns 0 on success and a non-zero
+ * LBDEM_E* code otherwise (lbdem_last_error gives the message). All arrays are host memory
+ * owned by the caller and copied before the call returns. Axes follow the reference:
+ * y is vertical. Cell index = x + y*X + z*X*Y (LB.cpp:2113-2137).
+ */
+#ifndef LBDEM_H
+#define LBDEM_H
+
+#include <stdint.h>
+
+#ifdef __cplusplus
+extern "C" {
+#endif
+
+#define LBDEM_OK 0
+#define LBDEM_E_ARG 1          /* invalid argument / unsupported configuration */
+#define LBDEM_E_STATE 2        /* call out of order (e.g. step before init) */
+#define LBDEM_E_DEVICE 3       /* HIP runtime error */
+#define LBDEM_E_CAPACITY 4     /* a fixed capacity (neighbours, lists, scratch) overflowed */
+#define LBDEM_E_TYPE 5         /* streaming met a neighbour type it cannot handle (LB.cpp:1524) */
+#define LBDEM_E_NOMEM 6
+
+#define LBDEM_PREC_FP64 0      /* populations stored and computed in FP64 */
+#define LBDEM_PREC_FP32 1      /* populations stored as f - w_i in FP32, FP32 arithmetic */
+#define LBDEM_PREC_FP16 2      /* populations stored as scaled (f - w_i) in FP16, FP32 arithmetic */
+
+/* Parsed configuration. Physical units exactly as in the getpot file. */
+typedef struct lbdem_params {
+    /* main.cpp:131-175 */
+    double demInitialRepeat;
+    double maxTime;
+    int demSolve, lbSolve, freeSurface, forceField;
+    /* LB.cpp:137-336 */
+    double unitLength, unitTime, unitDensity;
+    double lbSize[3];            /* lbSizeX/Y/Z (physical) */
+    double initVelocity[3];      /* physical */
+    double lbF[3];               /* physical acceleration */
+    int boundary[6];             /* boundary0..5: 4 periodic, 7 no-slip static wall */
+    double fluidHeight;          /* physical */
+    double initVisc;             /* physical dynamic viscosity (NEWTONIAN) */
+    /* DEM.cpp:19-75 */
+    double density, objectDensity, bodyDensity, linearStiff, frictionCoefPart;
+    double lubNormal, lubTangential, lubRotation, lubCutoff, kinVisc;
+    double electroForce, debyeLength;
+    int objWaitTime;
+    /* implementation options (no reference counterpart) */
+    int precision;               /* LBDEM_PREC_* (the CPU oracle always uses FP64) */
+    int device;                  /* HIP device ordinal */
+} lbdem_params;
+
+/* Per-particle record layout used by lbdem_get_particles (doubles). */
+enum {
+    LBDEM_P_X0 = 0, LBDEM_P_X1 = 3, LBDEM_P_W1 = 6, LBDEM_P_FHYDRO = 9, LBDEM_P_MHYDRO = 12,
+    LBDEM_P_FCONTACT = 15, LBDEM_P_MCONTACT = 18, LBDEM_P_FLUB = 21, LBDEM_P_MLUB = 24,
+    LBDEM_P_FREP = 27, LBDEM_P_FTOTAL = 30, LBDEM_P_MTOTAL = 33, LBDEM_P_FGRAV = 36,
+    LBDEM_P_FLUIDVOL = 39, LBDEM_P_XIW = 40, LBDEM_P_XIO = 43, LBDEM_P_R = 46, LBDEM_P_M = 47,
+    LBDEM_P_REC = 48
+};
+/* Per-object (impactor) record layout used by lbdem_get_objects (doubles). */
+enum {
+    LBDEM_O_X0 = 0, LBDEM_O_X1 = 3, LBDEM_O_W1 = 6, LBDEM_O_FHYDRO = 9, LBDEM_O_MHYDRO = 12,
+    LBDEM_O_FCONTACT = 15, LBDEM_O_MCONTACT = 18, LBDEM_O_FLUB = 21, LBDEM_O_MLUB = 24,
+    LBDEM_O_FTOTAL = 27, LBDEM_O_MTOTAL = 30, LBDEM_O_FGRAV = 33, LBDEM_O_R = 36, LBDEM_O_M = 37,
+    LBDEM_O_REC = 38
+};
+/* Per-wall record layout used by lbdem_get_walls (doubles). */
+enum {
+    LBDEM_W_FHYDRO = 0, LBDEM_W_FCONTACT = 3, LBDEM_W_FLUB = 6, LBDEM_W_FTOTAL = 9,
+    LBDEM_W_REC = 12
+};
+
+typedef struct lbdem_info {
+    int size[3];                 /* lbSize in cells (LB.cpp:170-172) */
+    int64_t totNodes;
+    int nParticles, nObjects, nWalls;
+    int nActive, nFluid, nInterface, nParticleNodes, nObjectNodes;
+    unsigned int lbTime;         /* LB::time */
+    unsigned int demTimeStep;    /* DEM::demTimeStep */
+    double demTime, realTime;
+    int neighborListBuilds;      /* DEM::neighborList calls so far (incl. init) */
+    int newNeighborList;         /* DEM::newNeighborList */
+    double totalMassInit;        /* LB::totalMass (LB.cpp:388-395) */
+    double lastExtraMass;        /* streaming extraMass of the last LB step (LB.cpp:1533) */
+    double lastSurplus;          /* massSurplus of the last free-surface step (LB.cpp:1644) */
+    int precision;
+} lbdem_info;
+
+typedef struct lbdem_ctx lbdem_ctx;
+
+int lbdem_create(const lbdem_params* params, lbdem_ctx** out);
+int lbdem_destroy(lbdem_ctx* ctx);
+const char* lbdem_last_error(const lbdem_ctx* ctx);
+
+/* particles.dat rows (DEM.cpp:88-134) after scaling: size, r*scale, x0*scale-0.05 (3n). */
+int lbdem_set_particles(lbdem_ctx* ctx, int n, const int* size, const double* r, const double* x0);
+/* objects.dat rows (DEM.cpp:138-165) after scaling: r, x0 (3n), x1 (3n). */
+int lbdem_set_objects(lbdem_ctx* ctx, int n, const double* r, const double* x0, const double* x1);
+
+int lbdem_init(lbdem_ctx* ctx);
+int lbdem_dem_step(lbdem_ctx* ctx);
+int lbdem_coupling_step(lbdem_ctx* ctx);
+int lbdem_lb_step(lbdem_ctx* ctx);
+int lbdem_fs_step(lbdem_ctx* ctx);
+/* nsteps repetitions of goCycle's solver part (main.cpp:38-64), including the
+ * demInitialRepeat gating and the realTime/lb.time counters. */
+int lbdem_cycle(lbdem_ctx* ctx, int nsteps);
+int lbdem_synchronize(lbdem_ctx* ctx);
+
+int lbdem_get_info(lbdem_ctx* ctx, lbdem_info* info);
+/* type: node type code (node.h:122-135); pflag: inside-particle flag; solid: solidIndex. */
+int lbdem_get_types(lbdem_ctx* ctx, uint8_t* type, uint8_t* pflag, int32_t* solid);
+/* f: 19*N post-streaming populations, SoA (f[q*N+i]), 0 where the node does not exist;
+ * mass, n: N values (0 where the node does not exist). Any pointer may be NULL. */
+int lbdem_get_fields(lbdem_ctx* ctx, double* f, double* mass, double* n);
+int lbdem_get_particles(lbdem_ctx* ctx, double* rec /* nParticles*LBDEM_P_REC */);
+int lbdem_get_objects(lbdem_ctx* ctx, double* rec /* nObjects*LBDEM_O_REC */);
+int lbdem_get_walls(lbdem_ctx* ctx, double* rec /* nWalls*LBDEM_W_REC */);
+/* Verlet list of every particle, partners sorted ascending: count[n], list[n*cap + k]. */
+int lbdem_get_neighbors(lbdem_ctx* ctx, int cap, int* count, int* list);
+/* Total fluid mass: sum of mass over active nodes not inside a particle (LB.cpp:388-395). */
+int lbdem_total_mass(lbdem_ctx* ctx, double* mass);
+
+#ifdef __cplusplus
+}
+#endif
+#endif /* LBDEM_H */

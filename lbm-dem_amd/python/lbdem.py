@@ -1,0 +1,421 @@
+"""ctypes binding of the lbdem C ABI (include/lbdem.h) plus the reference's input formats.
+
+The same binding drives any library exporting the ABI (the MI355X library
+``lbm-dem_amd/lib/liblbdem.so``; tests also point it at the CPU oracle).
+
+Input formats follow the reference:
+  * getpot config  ``name = value  # comment`` (getpot.h:110-117, macros.h:14-34),
+    keys and defaults as read by LB::latticeBoltzmannGet (LB.cpp:128-338),
+    DEM::discreteElementGet (DEM.cpp:17-75) and parseConfigFile (main.cpp:128-185);
+  * ``input/particles.dat``: count, then rows ``index size r x y z vx vy vz wx wy wz q0..q3 q1(4)``
+    read with ``x0 = x*scale - 0.05`` and ``r *= scale`` (DEM.cpp:81-134);
+  * ``input/objects.dat``: count, then rows ``index r x y z vx vy vz`` (DEM.cpp:136-165).
+"""
+from __future__ import annotations
+
+import ctypes as C
+import os
+import re
+from dataclasses import dataclass, field
+from typing import Dict, List, Optional, Sequence
+
+import numpy as np
+
+HERE = os.path.dirname(os.path.abspath(__file__))
+PKG = os.path.dirname(HERE)
+REPO = os.path.dirname(PKG)
+LIB_PATH = os.path.join(PKG, "lib", "liblbdem.so")
+
+PREC_FP64, PREC_FP32, PREC_FP16 = 0, 1, 2
+Q = 19
+
+# record layouts (include/lbdem.h)
+P = dict(X0=0, X1=3, W1=6, FHYDRO=9, MHYDRO=12, FCONTACT=15, MCONTACT=18, FLUB=21, MLUB=24, FREP=27,
+         FTOTAL=30, MTOTAL=33, FGRAV=36, FLUIDVOL=39, XIW=40, XIO=43, R=46, M=47, REC=48)
+O = dict(X0=0, X1=3, W1=6, FHYDRO=9, MHYDRO=12, FCONTACT=15, MCONTACT=18, FLUB=21, MLUB=24, FTOTAL=27,
+         MTOTAL=30, FGRAV=33, R=36, M=37, REC=38)
+W = dict(FHYDRO=0, FCONTACT=3, FLUB=6, FTOTAL=9, REC=12)
+
+
+class Params(C.Structure):
+    _fields_ = [
+        ("demInitialRepeat", C.c_double), ("maxTime", C.c_double),
+        ("demSolve", C.c_int), ("lbSolve", C.c_int), ("freeSurface", C.c_int), ("forceField", C.c_int),
+        ("unitLength", C.c_double), ("unitTime", C.c_double), ("unitDensity", C.c_double),
+        ("lbSize", C.c_double * 3), ("initVelocity", C.c_double * 3), ("lbF", C.c_double * 3),
+        ("boundary", C.c_int * 6), ("fluidHeight", C.c_double), ("initVisc", C.c_double),
+        ("density", C.c_double), ("objectDensity", C.c_double), ("bodyDensity", C.c_double),
+        ("linearStiff", C.c_double), ("frictionCoefPart", C.c_double),
+        ("lubNormal", C.c_double), ("lubTangential", C.c_double), ("lubRotation", C.c_double),
+        ("lubCutoff", C.c_double), ("kinVisc", C.c_double),
+        ("electroForce", C.c_double), ("debyeLength", C.c_double),
+        ("objWaitTime", C.c_int),
+        ("precision", C.c_int), ("device", C.c_int),
+    ]
+
+
+class Info(C.Structure):
+    _fields_ = [
+        ("size", C.c_int * 3), ("totNodes", C.c_int64),
+        ("nParticles", C.c_int), ("nObjects", C.c_int), ("nWalls", C.c_int),
+        ("nActive", C.c_int), ("nFluid", C.c_int), ("nInterface", C.c_int), ("nParticleNodes", C.c_int),
+        ("nObjectNodes", C.c_int),
+        ("lbTime", C.c_uint), ("demTimeStep", C.c_uint), ("demTime", C.c_double), ("realTime", C.c_double),
+        ("neighborListBuilds", C.c_int), ("newNeighborList", C.c_int),
+        ("totalMassInit", C.c_double), ("lastExtraMass", C.c_double), ("lastSurplus", C.c_double),
+        ("precision", C.c_int),
+    ]
+
+    def as_dict(self) -> Dict:
+        d = {}
+        for name, _ in self._fields_:
+            v = getattr(self, name)
+            d[name] = list(v) if name == "size" else v
+        return d
+
+
+_dp = C.POINTER(C.c_double)
+_ip = C.POINTER(C.c_int)
+
+
+class LbdemError(RuntimeError):
+    pass
+
+
+class Library:
+    """Loads a shared library that exports the lbdem C ABI."""
+
+    _cache: Dict[str, "Library"] = {}
+
+    def __init__(self, path: str):
+        if not os.path.exists(path):
+            raise LbdemError(f"lbdem library not found: {path} (build it with __graft_entry__.build())")
+        self.path = path
+        lib = C.CDLL(path, mode=os.RTLD_LOCAL | getattr(os, "RTLD_NOW", 2))
+        vp = C.c_void_p
+        sig = {
+            "lbdem_create": ([C.POINTER(Params), C.POINTER(vp)], C.c_int),
+            "lbdem_destroy": ([vp], C.c_int),
+            "lbdem_last_error": ([vp], C.c_char_p),
+            "lbdem_set_particles": ([vp, C.c_int, _ip, _dp, _dp], C.c_int),
+            "lbdem_set_objects": ([vp, C.c_int, _dp, _dp, _dp], C.c_int),
+            "lbdem_init": ([vp], C.c_int),
+            "lbdem_dem_step": ([vp], C.c_int),
+            "lbdem_coupling_step": ([vp], C.c_int),
+            "lbdem_lb_step": ([vp], C.c_int),
+            "lbdem_fs_step": ([vp], C.c_int),
+            "lbdem_cycle": ([vp, C.c_int], C.c_int),
+            "lbdem_synchronize": ([vp], C.c_int),
+            "lbdem_get_info": ([vp, C.POINTER(Info)], C.c_int),
+            "lbdem_get_types": ([vp, C.c_void_p, C.c_void_p, C.c_void_p], C.c_int),
+            "lbdem_get_fields": ([vp, C.c_void_p, C.c_void_p, C.c_void_p], C.c_int),
+            "lbdem_get_particles": ([vp, _dp], C.c_int),
+            "lbdem_get_objects": ([vp, _dp], C.c_int),
+            "lbdem_get_walls": ([vp, _dp], C.c_int),
+            "lbdem_get_neighbors": ([vp, C.c_int, _ip, _ip], C.c_int),
+            "lbdem_total_mass": ([vp, _dp], C.c_int),
+        }
+        for name, (args, res) in sig.items():
+            fn = getattr(lib, name)
+            fn.argtypes = args
+            fn.restype = res
+        self.lib = lib
+
+    @classmethod
+    def load(cls, path: Optional[str] = None) -> "Library":
+        path = os.path.abspath(path or LIB_PATH)
+        if path not in cls._cache:
+            cls._cache[path] = Library(path)
+        return cls._cache[path]
+
+
+def _ptr(a: np.ndarray, ctype):
+    return a.ctypes.data_as(C.POINTER(ctype))
+
+
+class Simulation:
+    """One lbdem context: configuration, particles and objects, then steps and downloads."""
+
+    def __init__(self, lib: Library, params: Params, particles: "ParticleSet", objects: "ObjectSet"):
+        self.lib = lib
+        self.ctx = C.c_void_p()
+        self.params = params
+        rc = lib.lib.lbdem_create(C.byref(params), C.byref(self.ctx))
+        self._check(rc, "lbdem_create")
+        n = len(particles.r)
+        size = np.ascontiguousarray(particles.size, dtype=np.int32)
+        r = np.ascontiguousarray(particles.r, dtype=np.float64)
+        x0 = np.ascontiguousarray(particles.x0, dtype=np.float64).reshape(-1)
+        self._check(lib.lib.lbdem_set_particles(self.ctx, n, _ptr(size, C.c_int), _ptr(r, C.c_double),
+                                                _ptr(x0, C.c_double)), "lbdem_set_particles")
+        m = len(objects.r)
+        orr = np.ascontiguousarray(objects.r, dtype=np.float64)
+        ox0 = np.ascontiguousarray(objects.x0, dtype=np.float64).reshape(-1)
+        ox1 = np.ascontiguousarray(objects.x1, dtype=np.float64).reshape(-1)
+        self._check(lib.lib.lbdem_set_objects(self.ctx, m, _ptr(orr, C.c_double), _ptr(ox0, C.c_double),
+                                              _ptr(ox1, C.c_double)), "lbdem_set_objects")
+
+    def _check(self, rc: int, what: str):
+        if rc != 0:
+            msg = self.lib.lib.lbdem_last_error(self.ctx) if self.ctx else b""
+            raise LbdemError(f"{what} failed with code {rc}: {msg.decode(errors='replace') if msg else ''}")
+
+    def close(self):
+        if self.ctx:
+            self.lib.lib.lbdem_destroy(self.ctx)
+            self.ctx = C.c_void_p()
+
+    def __del__(self):
+        try:
+            self.close()
+        except Exception:
+            pass
+
+    def init(self):
+        self._check(self.lib.lib.lbdem_init(self.ctx), "lbdem_init")
+
+    def dem_step(self):
+        self._check(self.lib.lib.lbdem_dem_step(self.ctx), "lbdem_dem_step")
+
+    def coupling_step(self):
+        self._check(self.lib.lib.lbdem_coupling_step(self.ctx), "lbdem_coupling_step")
+
+    def lb_step(self):
+        self._check(self.lib.lib.lbdem_lb_step(self.ctx), "lbdem_lb_step")
+
+    def fs_step(self):
+        self._check(self.lib.lib.lbdem_fs_step(self.ctx), "lbdem_fs_step")
+
+    def cycle(self, n: int = 1):
+        self._check(self.lib.lib.lbdem_cycle(self.ctx, int(n)), "lbdem_cycle")
+
+    def synchronize(self):
+        self._check(self.lib.lib.lbdem_synchronize(self.ctx), "lbdem_synchronize")
+
+    def info(self) -> Dict:
+        inf = Info()
+        self._check(self.lib.lib.lbdem_get_info(self.ctx, C.byref(inf)), "lbdem_get_info")
+        return inf.as_dict()
+
+    def types(self):
+        N = self.info()["totNodes"]
+        t = np.zeros(N, np.uint8)
+        p = np.zeros(N, np.uint8)
+        s = np.zeros(N, np.int32)
+        self._check(self.lib.lib.lbdem_get_types(self.ctx, t.ctypes.data, p.ctypes.data, s.ctypes.data),
+                    "lbdem_get_types")
+        return t, p, s
+
+    def fields(self, want_f: bool = True):
+        N = self.info()["totNodes"]
+        f = np.zeros(Q * N, np.float64) if want_f else None
+        m = np.zeros(N, np.float64)
+        n = np.zeros(N, np.float64)
+        self._check(self.lib.lib.lbdem_get_fields(self.ctx, f.ctypes.data if want_f else None, m.ctypes.data,
+                                                  n.ctypes.data), "lbdem_get_fields")
+        return (f.reshape(Q, N) if want_f else None), m, n
+
+    def particles(self) -> np.ndarray:
+        n = self.info()["nParticles"]
+        rec = np.zeros((max(n, 1), P["REC"]), np.float64)
+        self._check(self.lib.lib.lbdem_get_particles(self.ctx, _ptr(rec, C.c_double)), "lbdem_get_particles")
+        return rec[:n]
+
+    def objects(self) -> np.ndarray:
+        n = self.info()["nObjects"]
+        rec = np.zeros((max(n, 1), O["REC"]), np.float64)
+        self._check(self.lib.lib.lbdem_get_objects(self.ctx, _ptr(rec, C.c_double)), "lbdem_get_objects")
+        return rec[:n]
+
+    def walls(self) -> np.ndarray:
+        n = self.info()["nWalls"]
+        rec = np.zeros((max(n, 1), W["REC"]), np.float64)
+        self._check(self.lib.lib.lbdem_get_walls(self.ctx, _ptr(rec, C.c_double)), "lbdem_get_walls")
+        return rec[:n]
+
+    def neighbors(self, cap: int = 64):
+        n = self.info()["nParticles"]
+        cnt = np.zeros(max(n, 1), np.int32)
+        lst = np.full((max(n, 1), cap), -1, np.int32)
+        self._check(self.lib.lib.lbdem_get_neighbors(self.ctx, cap, _ptr(cnt, C.c_int), _ptr(lst, C.c_int)),
+                    "lbdem_get_neighbors")
+        return cnt[:n], lst[:n]
+
+    def total_mass(self) -> float:
+        v = C.c_double()
+        self._check(self.lib.lib.lbdem_total_mass(self.ctx, C.byref(v)), "lbdem_total_mass")
+        return v.value
+
+
+# ----------------------------------------------------------------------------------------------
+# getpot configuration (reference keys and defaults)
+# ----------------------------------------------------------------------------------------------
+DEFAULTS: Dict[str, object] = {
+    # main.cpp:131-175
+    "demInitialRepeat": 0.0, "lbmInitialRepeat": 0, "maxTime": 0.0, "maximumTimeSteps": 0,
+    "screenExpTime": 0.0, "fluidExpTime": 0.0, "fluidLagrangianExpTime": 0.0, "partExpTime": 0.0,
+    "fluidRecycleExpTime": 0.0, "partRecycleExpTime": 0.0,
+    "demSolve": 0, "lbSolve": 0, "freeSurfaceSolve": 0, "forceFieldSolve": 0,
+    # LB.cpp:137-336
+    "lbRestart": 0, "lbTopography": 0, "unitLength": 1.0, "unitTime": 1.0, "unitDensity": 1.0,
+    "lbSizeX": 0.0, "lbSizeY": 0.0, "lbSizeZ": 0.0, "rheologyModel": "none", "initVisc": 1.0,
+    "turbulenceSolve": 0, "turbConst": 0.0,
+    "initVelocityX": 0.0, "initVelocityY": 0.0, "initVelocityZ": 0.0,
+    "lbFX": 0.0, "lbFY": 0.0, "lbFZ": 0.0,
+    "boundary0": 1, "boundary1": 1, "boundary2": 1, "boundary3": 1, "boundary4": 1, "boundary5": 1,
+    "slipCoefficient": 0.0, "fluidHeight": 0.0,
+    # DEM.cpp:19-75
+    "density": 0.0, "objectDensity": 0.0, "bodyDensity": 0.0, "linearStiff": 0.0, "frictionCoefPart": 0.0,
+    "lubNormal": 0.0, "lubTangential": 0.0, "lubRotation": 0.0, "lubCutoff": 0.0, "kinVisc": 0.0,
+    "electroForce": 0.0, "debyeLength": 0.0, "objWaitTime": 0, "intervalTime": 0, "springDamp": 0.0,
+    "epsilon": 0.0, "nonLinType": 0, "k13": 0.0, "scale": 1.0,
+}
+
+_LINE = re.compile(r"^\s*([A-Za-z_][A-Za-z0-9_]*)\s*=\s*(.*?)\s*$")
+
+
+def parse_getpot(text: str) -> Dict[str, str]:
+    """Variables of a getpot file: ``name = value``, ``#`` starts a comment, quotes allowed."""
+    out: Dict[str, str] = {}
+    for raw in text.splitlines():
+        line = raw.split("#", 1)[0]
+        m = _LINE.match(line)
+        if not m:
+            continue
+        val = m.group(2).strip()
+        if len(val) >= 2 and val[0] == val[-1] and val[0] in "'\"":
+            val = val[1:-1]
+        out[m.group(1)] = val
+    return out
+
+
+def _num(v: str, like):
+    if isinstance(like, str):
+        return v
+    if isinstance(like, int) and not isinstance(like, bool):
+        s = v.strip().lower()
+        if s in ("true", "false"):
+            return 1 if s == "true" else 0
+        return int(float(s))
+    return float(v)
+
+
+def config_values(text: str, overrides: Optional[Dict[str, object]] = None) -> Dict[str, object]:
+    vals = dict(DEFAULTS)
+    for k, v in parse_getpot(text).items():
+        vals[k] = _num(v, DEFAULTS.get(k, 0.0))
+    if overrides:
+        vals.update(overrides)
+    return vals
+
+
+def params_from_values(v: Dict[str, object], precision: int = PREC_FP64, device: int = 0) -> Params:
+    if str(v.get("rheologyModel", "NEWTONIAN")) not in ("NEWTONIAN",):
+        raise LbdemError("only rheologyModel = NEWTONIAN is supported")
+    if int(v.get("turbulenceSolve", 0)):
+        raise LbdemError("turbulenceSolve is not supported")
+    p = Params()
+    p.demInitialRepeat = float(v["demInitialRepeat"])
+    p.maxTime = float(v["maxTime"])
+    p.demSolve, p.lbSolve = int(v["demSolve"]), int(v["lbSolve"])
+    p.freeSurface, p.forceField = int(v["freeSurfaceSolve"]), int(v["forceFieldSolve"])
+    p.unitLength, p.unitTime, p.unitDensity = float(v["unitLength"]), float(v["unitTime"]), float(v["unitDensity"])
+    p.lbSize[:] = [float(v["lbSizeX"]), float(v["lbSizeY"]), float(v["lbSizeZ"])]
+    p.initVelocity[:] = [float(v["initVelocityX"]), float(v["initVelocityY"]), float(v["initVelocityZ"])]
+    p.lbF[:] = [float(v["lbFX"]), float(v["lbFY"]), float(v["lbFZ"])]
+    p.boundary[:] = [int(v[f"boundary{i}"]) for i in range(6)]
+    p.fluidHeight = float(v["fluidHeight"])
+    p.initVisc = float(v["initVisc"])
+    for k in ("density", "objectDensity", "bodyDensity", "linearStiff", "frictionCoefPart", "lubNormal",
+              "lubTangential", "lubRotation", "lubCutoff", "kinVisc", "electroForce", "debyeLength"):
+        setattr(p, k, float(v[k]))
+    p.objWaitTime = int(v["objWaitTime"])
+    p.precision = int(precision)
+    p.device = int(device)
+    return p
+
+
+def write_getpot(path: str, values: Dict[str, object]):
+    with open(path, "w") as fh:
+        for k, v in values.items():
+            if isinstance(v, (float, np.floating)):
+                fh.write(f"{k} = {float(v)!r}\n")
+            else:
+                fh.write(f"{k} = {v}\n")
+
+
+# ----------------------------------------------------------------------------------------------
+# particles.dat / objects.dat
+# ----------------------------------------------------------------------------------------------
+@dataclass
+class ParticleSet:
+    size: np.ndarray = field(default_factory=lambda: np.zeros(0, np.int32))
+    r: np.ndarray = field(default_factory=lambda: np.zeros(0))
+    x0: np.ndarray = field(default_factory=lambda: np.zeros((0, 3)))
+
+
+@dataclass
+class ObjectSet:
+    r: np.ndarray = field(default_factory=lambda: np.zeros(0))
+    x0: np.ndarray = field(default_factory=lambda: np.zeros((0, 3)))
+    x1: np.ndarray = field(default_factory=lambda: np.zeros((0, 3)))
+
+
+def _tokens(path: str) -> List[str]:
+    with open(path) as fh:
+        return fh.read().split()
+
+
+def read_particles(path: str, scale: float = 1.0) -> ParticleSet:
+    """input/particles.dat as DEM::discreteElementGet reads it (DEM.cpp:81-134)."""
+    tok = _tokens(path)
+    n = int(tok[0])
+    rows = np.array([float(x) for x in tok[1:1 + 20 * n]], dtype=np.float64).reshape(n, 20) if n else np.zeros((0, 20))
+    ps = ParticleSet()
+    ps.size = rows[:, 1].astype(np.int32)
+    ps.r = rows[:, 2] * scale
+    ps.x0 = rows[:, 3:6] * scale - 0.05
+    return ps
+
+
+def read_objects(path: str, scale: float = 1.0) -> ObjectSet:
+    """input/objects.dat as DEM::discreteElementGet reads it (DEM.cpp:136-165)."""
+    tok = _tokens(path)
+    n = int(tok[0])
+    rows = np.array([float(x) for x in tok[1:1 + 8 * n]], dtype=np.float64).reshape(n, 8) if n else np.zeros((0, 8))
+    os_ = ObjectSet()
+    os_.r = rows[:, 1] * scale
+    os_.x0 = rows[:, 2:5] * scale
+    os_.x1 = rows[:, 5:8].copy()
+    return os_
+
+
+def write_particles(path: str, r: Sequence[float], centers: np.ndarray, size: Optional[Sequence[int]] = None):
+    """Rows in particles.dat format; ``centers`` are written as given (the reader subtracts 0.05)."""
+    centers = np.asarray(centers, dtype=np.float64).reshape(-1, 3)
+    with open(path, "w") as fh:
+        fh.write(f"{len(r)}\n")
+        for i in range(len(r)):
+            s = 1 if size is None else int(size[i])
+            x, y, z = (float(c) for c in centers[i])
+            fh.write(f"{i} {s} {float(r[i])!r} {x!r} {y!r} {z!r} 0 0 0 0 0 0 1 0 0 0 0 0 0 0\n")
+
+
+def write_objects(path: str, r: Sequence[float], x0: np.ndarray, x1: np.ndarray):
+    x0 = np.asarray(x0, dtype=np.float64).reshape(-1, 3)
+    x1 = np.asarray(x1, dtype=np.float64).reshape(-1, 3)
+    with open(path, "w") as fh:
+        fh.write(f"{len(r)}\n")
+        for i in range(len(r)):
+            a = [float(c) for c in (*x0[i], *x1[i])]
+            fh.write(f"{i} {float(r[i])!r} " + " ".join(repr(c) for c in a) + "\n")
+
+
+def load_case(case_dir: str, cfg_name: str = "config.cfg", precision: int = PREC_FP64, overrides=None):
+    """Reads a reference-style case directory: cfg plus input/particles.dat and input/objects.dat."""
+    with open(os.path.join(case_dir, cfg_name)) as fh:
+        vals = config_values(fh.read(), overrides)
+    scale = float(vals.get("scale", 1.0))
+    ps = read_particles(os.path.join(case_dir, "input", "particles.dat"), scale)
+    obs = read_objects(os.path.join(case_dir, "input", "objects.dat"), scale)
+    return vals, params_from_values(vals, precision), ps, obs

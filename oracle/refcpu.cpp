@@ -1,0 +1,1428 @@
+// ORACLE — test infrastructure only.
+//
+// FP64 CPU restatement of the reference's coupled free-surface LB / DEM timestep, written
+// from scratch to the reference's semantics (including its order-dependent rules), behind
+// the same C ABI as the MI355X library (include/lbdem.h). Only tests/, the smoke check and
+// bench.py's cpu_baseline leg load this library; the product never does.
+//
+// Every function cites the reference function it restates (paths relative to the
+// reference repository). Quantities used only by output-only diagnostics (stress tensors,
+// FElas/FVisc, contact-point lists, strains) are not computed.
+//
+// Pinned behaviour where the reference is undefined (each also used by the GPU library):
+//  * xPbcs/yPbcs/zPbcs loop index starts at 0 (DEM.cpp:2771, 2792, 2817 leave it uninitialised).
+//  * object-particle tangential damping dampCoeffTanw = dampCoeff if kinVisc==0 else 0
+//    (DEM.cpp:916 reads it uninitialised; rule taken from DEM.cpp:2015-2020).
+//  * wall bounce-back force goes to walls[solidIndex] when solidIndex < nWalls, else walls[0]
+//    (LB.cpp:1425-1429 indexes walls[] with a particle index after findNewSolid floods a wall node).
+//  * a particle-flagged node that has no node object (gas) takes the active-node neighbour
+//    rule in findNewSolid (LB.cpp:2046 dereferences a null node there).
+//  * neighbour capacity is 63 partners per particle (reference: 49, unchecked, elmt.h:44-51);
+//    exceeding it is reported as LBDEM_E_CAPACITY.
+#include "refcpu_common.h"
+#include "../include/lbdem.h"
+
+#include <algorithm>
+#include <cstdio>
+#include <cstring>
+#include <string>
+#include <vector>
+
+using namespace refcpu;
+
+namespace {
+
+constexpr int MAXL = 64;  // list[0] holds the count, entries 1..63
+
+struct Units {  // measureUnits (node.h:192-225, node.cpp:512-525)
+    double Length = 1, Time = 1, Density = 1, Volume = 1, Speed = 1, Accel = 1, AngVel = 1, KinVisc = 1,
+           DynVisc = 1, Force = 1, Torque = 1, Mass = 1, Stress = 1, Pressure = 1, invLength = 1;
+    void setComposite() {
+        Volume = Length * Length * Length;
+        Speed = Length / Time;
+        Accel = Length / Time / Time;
+        AngVel = 1.0 / Time;
+        KinVisc = Length * Length / Time;
+        DynVisc = Density * Length * Length / Time;
+        Force = Density * Length * Length * Length * Length / Time / Time;
+        Torque = Density * Length * Length * Length * Length * Length / Time / Time;
+        Mass = Density * Length * Length * Length;
+        Stress = Density * Length * Length / Time / Time;
+        Pressure = Density * Length * Length / Time / Time;
+        invLength = 1.0 / Length;
+    }
+};
+
+struct Particle {  // particle (elmt.h:15-158), fields used on the path
+    unsigned size = 1, particleIndex = 0;
+    double r = 1, m = 1, I = 1, _m = 1, _I = 1, dens = 1;
+    V3 x0, x1, w1, x0_lst;
+    V3 FHydro, FContact, FTotal, FLub, FGrav, FRep, MHydro, MContact, MTotal, MLub;
+    double fluidVolume = 0;
+    int list[MAXL], _list[MAXL];
+    double xix[MAXL], xiy[MAXL], xiz[MAXL], _xix[MAXL], _xiy[MAXL], _xiz[MAXL];
+    V3 xiw, xio;
+    Particle() {
+        std::memset(list, 0, sizeof(list));
+        std::memset(_list, 0, sizeof(_list));
+        for (int i = 0; i < MAXL; ++i) xix[i] = xiy[i] = xiz[i] = _xix[i] = _xiy[i] = _xiz[i] = 0.0;
+    }
+};
+
+struct Object {  // object (elmt.h:160-228)
+    unsigned index = 0;
+    double r = 0, m = 1, _m = 1, I = 1, _I = 1, rho = 0;
+    V3 x0, x1, w1;
+    V3 FContact, FHydro, FLub, FGrav, FSpring, FTotal, MContact, MHydro, MTotal, MLub;
+};
+
+struct Wall {  // wall (utils.h:19-101)
+    unsigned index = 0;
+    V3 n = V3(1, 0, 0), p;
+    V3 FHydro, FContact, FLub, FTotal, FRep;
+};
+
+}  // namespace
+
+struct lbdem_ctx {
+    lbdem_params prm;
+    std::string err;
+    bool initialised = false;
+    // ---------------- LB (LB.h:28-150) ----------------
+    Units unit;
+    unsigned lbSize[3] = {1, 1, 1};
+    double lbPhysicalSize[3];
+    V3 lbBoundaryLocation[6];
+    int boundary[6];
+    double fluidHeight = 0;
+    double initDynVisc = 0, initDensity = 1.0;
+    V3 initVelocity, lbF;
+    bool freeSurface = false, forceField = false;
+    unsigned lbTime = 0;
+    int64_t shift[3], domain[3], ne[Q];
+    int64_t N = 0;
+    double totalMass = 0;
+    // node types (node.h:116-175)
+    std::vector<uint8_t> t, p;
+    std::vector<uint16_t> solid;
+    // node objects (node.h:54-114) held as SoA; exists[i] == (nodes[i] != 0)
+    std::vector<uint8_t> exists;
+    std::vector<double> f, fs, n, u, h, mass, newMass, visc;
+    std::vector<uint32_t> d;
+    std::vector<uint32_t> activeNodes, fluidNodes, interfaceNodes, particleNodes, objectNodes, wallNodes;
+    double lastExtraMass = 0, lastSurplus = 0;
+    // ---------------- DEM (DEM.h:22-118) ----------------
+    double demSize[3] = {1, 1, 1};
+    std::vector<Particle> particles;
+    std::vector<Object> objects;
+    std::vector<Wall> walls;
+    double demInitialRepeat = 0;
+    unsigned demTimeStep = 0;
+    double deltat = 1.0, demTime = 0, maxDisp = 0;
+    V3 demF;
+    double dampCoeff = 0;
+    double minPartRadius = 0, maxPartRadius = 0, meanPartRadius = 0;
+    bool newNeighborList = false;
+    int nlBuilds = 0;
+    // ---------------- IO counters (main.cpp:40-43) ----------------
+    double realTime = 0;
+    unsigned currentTimeStep = 0;
+    int fail(int code, const std::string& m) { err = m; return code; }
+
+    // ================= index helpers (LB.cpp:2113-2169) =================
+    unsigned getX(int64_t i) const { div_t a = div(int(i), int(lbSize[0] * lbSize[1])); div_t b = div(a.rem, int(lbSize[0])); return b.rem; }
+    unsigned getY(int64_t i) const { div_t a = div(int(i), int(lbSize[0] * lbSize[1])); div_t b = div(a.rem, int(lbSize[0])); return b.quot; }
+    unsigned getZ(int64_t i) const { div_t a = div(int(i), int(lbSize[0] * lbSize[1])); return a.quot; }
+    V3 getPosition(int64_t i) const { return V3(double(getX(i)), double(getY(i)), double(getZ(i))); }
+
+    bool isActive(int64_t i) const { return t[i] == 0 || t[i] == 3; }
+    bool isWall(int64_t i) const { return t[i] >= 5 && t[i] <= 11; }
+
+    // ================= node methods (node.cpp) =================
+    double* F(int64_t i) { return &f[size_t(i) * Q]; }
+    double* FS(int64_t i) { return &fs[size_t(i) * Q]; }
+    V3 U(int64_t i) const { return V3(u[3 * i], u[3 * i + 1], u[3 * i + 2]); }
+    void setU(int64_t i, const V3& v) { u[3 * i] = v.x; u[3 * i + 1] = v.y; u[3 * i + 2] = v.z; }
+    V3 H(int64_t i) const { return V3(h[3 * i], h[3 * i + 1], h[3 * i + 2]); }
+    void setH(int64_t i, const V3& v) { h[3 * i] = v.x; h[3 * i + 1] = v.y; h[3 * i + 2] = v.z; }
+
+    void equilibrium(double nn, const V3& uu, double* feq) const {  // node.cpp:100-111
+        const double usq = uu.norm2();
+        for (int j = 0; j < Q; ++j) {
+            const double vu = uu.dot(vdir(j));
+            feq[j] = coeff(j) * nn * (1.0 + 3.0 * vu + 4.5 * vu * vu - 1.5 * usq);
+        }
+    }
+    // node::initialize (node.cpp:26-37) + setEquilibrium (node.cpp:52-73)
+    void nodeInitialize(int64_t i, double density, const V3& velocity, double massFunction, double viscosity, const V3& F_) {
+        n[i] = density;
+        mass[i] = massFunction;
+        V3 uu = F_ / 2.0 / n[i] + velocity;
+        setU(i, uu);
+        visc[i] = viscosity;
+        double feq[Q];
+        equilibrium(n[i], uu, feq);
+        for (int j = 0; j < Q; ++j) F(i)[j] = FS(i)[j] = feq[j];
+    }
+    // LB::createNode (LB.cpp:474-481): fresh node (node() constructor, node.h:75-88) + findNeighbors
+    void createNode(int64_t it) {
+        exists[it] = 1;
+        n[it] = 0; mass[it] = newMass[it] = 0; visc[it] = 0;
+        setU(it, V3()); setH(it, V3());
+        for (int j = 0; j < Q; ++j) F(it)[j] = FS(it)[j] = 0.0;
+        findNeighbors(it, &d[size_t(it) * Q]);
+    }
+    void deleteNode(int64_t it) { exists[it] = 0; }
+
+    // LB::findNeighbors (LB.cpp:567-680)
+    void findNeighbors(int64_t it, uint32_t* dd) const {
+        int64_t tmp[Q];
+        for (int j = 0; j < Q; ++j) tmp[j] = it + ne[j];
+        if (isWall(it)) {
+            const unsigned xh = getX(it), yh = getY(it), zh = getZ(it);
+            for (int j = 1; j < Q; ++j) {
+                if (xh == 0 && CX[j] < 0) tmp[j] = it;
+                if (xh == lbSize[0] - 1 && CX[j] > 0) tmp[j] = it;
+                if (yh == 0 && CY[j] < 0) tmp[j] = it;
+                if (yh == lbSize[1] - 1 && CY[j] > 0) tmp[j] = it;
+                if (zh == 0 && CZ[j] < 0) tmp[j] = it;
+                if (zh == lbSize[2] - 1 && CZ[j] > 0) tmp[j] = it;
+            }
+        } else if (isActive(it)) {
+            int64_t pbc[Q] = {0};
+            auto per = [&](int64_t k) { return k >= 0 && k < N && t[k] == 4; };
+            if (per(tmp[1])) for (int j = 1; j < Q; ++j) if (CX[j] > 0) pbc[j] -= domain[0];
+            if (per(tmp[2])) for (int j = 1; j < Q; ++j) if (CX[j] < 0) pbc[j] += domain[0];
+            if (per(tmp[3])) for (int j = 1; j < Q; ++j) if (CY[j] > 0) pbc[j] -= domain[1];
+            if (per(tmp[4])) for (int j = 1; j < Q; ++j) if (CY[j] < 0) pbc[j] += domain[1];
+            if (per(tmp[5])) for (int j = 1; j < Q; ++j) if (CZ[j] > 0) pbc[j] -= domain[2];
+            if (per(tmp[6])) for (int j = 1; j < Q; ++j) if (CZ[j] < 0) pbc[j] += domain[2];
+            for (int j = 1; j < Q; ++j) tmp[j] += pbc[j];
+        }
+        for (int j = 0; j < Q; ++j) dd[j] = uint32_t(tmp[j]);
+    }
+    // neighbour of a node: the stored link when the node object exists, the active-node
+    // rule otherwise (pinned, see header)
+    uint32_t link(int64_t i, int j) {
+        if (exists[i]) return d[size_t(i) * Q + j];
+        uint8_t save = t[i];
+        uint32_t dd[Q];
+        if (!isActive(i) && !isWall(i)) t[i] = 0;
+        findNeighbors(i, dd);
+        t[i] = save;
+        return dd[j];
+    }
+    bool inRange(uint32_t k) const { return int64_t(k) < N; }
+
+    // ================= parameters (LB.cpp:128-338, DEM.cpp:17-75) =================
+    int getParams() {
+        const lbdem_params& P = prm;
+        unit.Length = P.unitLength; unit.Time = P.unitTime; unit.Density = P.unitDensity;
+        if (!(unit.Length > 0) || !(unit.Time > 0) || !(unit.Density > 0)) return fail(LBDEM_E_ARG, "unit lengths/times must be > 0");
+        for (int a = 0; a < 3; ++a) {
+            if (!(P.lbSize[a] > 0)) return fail(LBDEM_E_ARG, "lbSize must be > 0");
+            lbSize[a] = unsigned(int(std::floor(P.lbSize[a] / unit.Length + 0.5)));
+            lbPhysicalSize[a] = double(lbSize[a]) * unit.Length;
+        }
+        lbBoundaryLocation[0] = V3(0.5 * unit.Length, 0.0, 0.0);
+        lbBoundaryLocation[1] = V3(double(lbSize[0] - 1.5) * unit.Length, 0.0, 0.0);
+        lbBoundaryLocation[2] = V3(0.0, 0.5 * unit.Length, 0.0);
+        lbBoundaryLocation[3] = V3(0.0, double(lbSize[1] - 1.5) * unit.Length, 0.0);
+        lbBoundaryLocation[4] = V3(0.0, 0.0, 0.5 * unit.Length);
+        lbBoundaryLocation[5] = V3(0.0, 0.0, double(lbSize[2] - 1.5) * unit.Length);
+        initDynVisc = P.initVisc;
+        if (!(initDynVisc > 0)) return fail(LBDEM_E_ARG, "initVisc must be > 0");
+        unit.setComposite();
+        initVelocity = V3(P.initVelocity[0], P.initVelocity[1], P.initVelocity[2]);
+        initVelocity /= unit.Speed;
+        lbF = V3(P.lbF[0], P.lbF[1], P.lbF[2]);
+        lbF /= unit.Accel;
+        for (int i = 0; i < 6; ++i) {
+            boundary[i] = P.boundary[i];
+            if (boundary[i] != 4 && boundary[i] != 7) return fail(LBDEM_E_ARG, "only boundary types 4 (periodic) and 7 (static no-slip wall) are supported");
+        }
+        fluidHeight = P.fluidHeight;
+        initDynVisc /= unit.DynVisc;
+        initDensity = 1.0;
+        freeSurface = P.freeSurface != 0;
+        forceField = P.forceField != 0;
+        demInitialRepeat = P.demInitialRepeat;
+        if (!P.lbSolve) return fail(LBDEM_E_ARG, "lbSolve=0 is not supported");
+        if (!(P.density > 0) || !(P.objectDensity > 0)) return fail(LBDEM_E_ARG, "density/objectDensity must be > 0");
+        return 0;
+    }
+
+    // ================= LB::latticeDefinition (LB.cpp:85-126) =================
+    void latticeDefinition() {
+        shift[2] = int64_t(lbSize[0]) * lbSize[1];
+        shift[1] = lbSize[0];
+        shift[0] = 1;
+        domain[2] = int64_t(lbSize[2]) * lbSize[1] * lbSize[0] - 2 * shift[2];
+        domain[1] = int64_t(lbSize[1]) * lbSize[0] - 2 * shift[1];
+        domain[0] = int64_t(lbSize[0]) - 2 * shift[0];
+        for (int j = 0; j < Q; ++j) ne[j] = CX[j] * shift[0] + CY[j] * shift[1] + CZ[j] * shift[2];
+    }
+
+    // ================= DEM init (DEM.cpp:199-357, elmt.cpp:17-53, DEM.cpp:405-458) =================
+    void discreteElementInit() {
+        demSize[0] = lbPhysicalSize[0];
+        demSize[1] = freeSurface ? fluidHeight : lbPhysicalSize[1];
+        demSize[2] = lbPhysicalSize[2];
+        const double partDensity = prm.density, objDensity = prm.objectDensity;
+        demF = lbF * unit.Accel;
+        unsigned globalIndex = 0;
+        for (auto& P : particles) {
+            const double singleMass = 4.0 / 3.0 * partDensity * M_PI * P.r * P.r * P.r;
+            P.m = P.size * singleMass;
+            P.dens = partDensity;
+            P.I = P.size * 2.0 / 5.0 * singleMass * P.r * P.r;
+            P.x1.reset(); P.w1.reset();
+            P.particleIndex = globalIndex;
+            P.FTotal.reset(); P.FRep.reset(); P.FHydro.reset(); P.FContact.reset(); P.FLub.reset();
+            P.FGrav = demF * P.m;
+            P.MTotal.reset(); P.MHydro.reset(); P.MContact.reset(); P.MLub.reset();
+            globalIndex++;
+        }
+        for (auto& O : objects) {
+            O.m = 4.0 / 3.0 * objDensity * M_PI * O.r * O.r * O.r;
+            O.FGrav = demF * O.m;
+            O.I = 2.0 / 5.0 * O.m * O.r * O.r;
+            O._m = 1.0 / O.m;
+            O._I = 1.0 / O.I;
+            O.rho = objDensity;
+        }
+        // DEM::initializeWalls (DEM.cpp:405-458); walls are static no-slip (type 7 only)
+        walls.clear();
+        const V3 dir[6] = {V3(1, 0, 0), V3(-1, 0, 0), V3(0, 1, 0), V3(0, -1, 0), V3(0, 0, 1), V3(0, 0, -1)};
+        unsigned index = 0;
+        for (int i = 0; i < 6; ++i) {
+            if (boundary[i] >= 5 && boundary[i] <= 8) {
+                Wall w; w.p = lbBoundaryLocation[i]; w.n = dir[i]; w.index = index++;
+                walls.push_back(w);
+            }
+        }
+        for (auto& P : particles) { P._m = 1.0 / P.m; P._I = 1.0 / P.I; }
+        neighborList();
+        newNeighborList = true;
+        minPartRadius = 1.0e99; maxPartRadius = 0.0; meanPartRadius = 0.0;
+        for (auto& P : particles) {
+            meanPartRadius += P.r;
+            if (P.r > maxPartRadius) maxPartRadius = P.r;
+            if (P.r < minPartRadius) minPartRadius = P.r;
+        }
+        if (particles.size() > 0) meanPartRadius /= double(particles.size());
+        deltat = unit.Time / double(1);
+        const double averageMass = 4.0 / 3.0 * partDensity * M_PI * meanPartRadius * meanPartRadius * meanPartRadius;
+        dampCoeff = std::sqrt(averageMass * prm.linearStiff);
+    }
+
+    // ================= LB init (LB.cpp:340-398 and helpers) =================
+    void latticeBolzmannInit() {
+        // initializeNodes (LB.cpp:500-536)
+        N = int64_t(lbSize[0]) * lbSize[1] * lbSize[2];
+        t.assign(N, 0); p.assign(N, 0); solid.assign(N, 0); exists.assign(N, 0);
+        f.assign(size_t(N) * Q, 0.0); fs.assign(size_t(N) * Q, 0.0); d.assign(size_t(N) * Q, 0);
+        n.assign(N, 0.0); u.assign(3 * N, 0.0); h.assign(3 * N, 0.0);
+        mass.assign(N, 0.0); newMass.assign(N, 0.0); visc.assign(N, 0.0);
+        // initializeLatticeBoundaries (LB.cpp:538-564)
+        for (int64_t it = 0; it < N; ++it) {
+            if (isWall(it)) continue;
+            const unsigned x = getX(it), y = getY(it), z = getZ(it);
+            if (x == 0) t[it] = boundary[0];
+            else if (x == lbSize[0] - 1) t[it] = boundary[1];
+            else if (y == 0) t[it] = boundary[2];
+            else if (y == lbSize[1] - 1) t[it] = boundary[3];
+            else if (z == 0) t[it] = boundary[4];
+            else if (z == lbSize[2] - 1) t[it] = boundary[5];
+        }
+        // initializeTypes (LB.cpp:487-498): particle boundaries (activeNodes empty: only reset)
+        initializeParticleBoundaries();
+        // initializeWallBoundaries (LB.cpp:713-748)
+        for (auto& w : walls) {
+            const V3 wp = w.p / unit.Length;
+            for (int64_t it = 0; it < N; ++it)
+                if (getPosition(it).insidePlane(wp, w.n)) { solid[it] = uint16_t(w.index); t[it] = 7; }
+        }
+        // initializeObjectBoundaries (LB.cpp:751-795)
+        objectNodes.clear();
+        for (auto& O : objects) {
+            const V3 c = O.x0 / unit.Length; const double rr = O.r / unit.Length;
+            for (int64_t it = 0; it < N; ++it)
+                if (getPosition(it).insideSphere(c, rr)) { solid[it] = uint16_t(O.index); t[it] = 12; }
+        }
+        for (int64_t ip = 0; ip < N; ++ip) {
+            if (t[ip] != 12) continue;
+            createNode(ip);
+            const Object& O = objects[solid[ip]];
+            if (getPosition(ip).insideSphere(O.x0 / unit.Length, O.r / unit.Length)) {
+                nodeInitialize(ip, initDensity, O.x1 / unit.Speed, 0.0, 0.0, V3());
+                objectNodes.push_back(uint32_t(ip));
+            }
+        }
+        // initializeInterface (LB.cpp:974-992)
+        if (freeSurface) {
+            const double h0 = fluidHeight / unit.Length;
+            for (int64_t it = 0; it < N; ++it)
+                if (getY(it) > h0 && t[it] == 0) t[it] = 2;
+        }
+        // initializeVariables (LB.cpp:1042-1096)
+        for (int64_t it = 0; it < N; ++it) {
+            if (t[it] == 0) {
+                createNode(it);
+                for (int j = 1; j < Q; ++j) {
+                    const uint32_t l = d[size_t(it) * Q + j];
+                    if (inRange(l) && t[l] == 2) { t[it] = 3; break; }
+                }
+            }
+            if (t[it] == 0) nodeInitialize(it, initDensity, initVelocity, initDensity, initDynVisc, lbF);
+            else if (t[it] == 3) nodeInitialize(it, initDensity, initVelocity, 0.5 * initDensity, initDynVisc, lbF);
+        }
+        // initializeWalls (LB.cpp:1140-1176)
+        wallNodes.clear();
+        for (int64_t it = 0; it < N; ++it) {
+            if (isWall(it)) {
+                createNode(it);
+                if (t[it] == 7) nodeInitialize(it, initDensity, V3(), 0.0, 0.0, V3());
+                wallNodes.push_back(uint32_t(it));
+            }
+        }
+        initializeLists();
+        cleanLists();
+        totalMass = 0.0;
+        for (uint32_t idx : activeNodes) if (!p[idx]) totalMass += mass[idx];
+    }
+
+    // LB::initializeLists (LB.cpp:1011-1040)
+    void initializeLists() {
+        fluidNodes.clear(); particleNodes.clear(); interfaceNodes.clear();
+        for (int64_t it = 0; it < N; ++it) {
+            if (t[it] == 0) fluidNodes.push_back(uint32_t(it));
+            else if (t[it] == 3) interfaceNodes.push_back(uint32_t(it));
+            if (p[it]) particleNodes.push_back(uint32_t(it));
+        }
+    }
+    static void sortDedupe(std::vector<uint32_t>& v) {
+        std::sort(v.begin(), v.end());
+        for (int ind = int(v.size()) - 1; ind > 0; --ind)
+            if (v[ind] == v[ind - 1]) v.erase(v.begin() + ind);
+    }
+    // LB::cleanLists (LB.cpp:1181-1259)
+    int cleanLists() {
+        sortDedupe(fluidNodes); sortDedupe(interfaceNodes); sortDedupe(particleNodes); sortDedupe(objectNodes);
+        activeNodes.clear();
+        activeNodes.insert(activeNodes.end(), fluidNodes.begin(), fluidNodes.end());
+        activeNodes.insert(activeNodes.end(), interfaceNodes.begin(), interfaceNodes.end());
+        std::sort(activeNodes.begin(), activeNodes.end());
+        for (int ind = int(activeNodes.size()) - 1; ind > 0; --ind) {
+            const uint32_t i = activeNodes[ind], j = activeNodes[ind - 1];
+            if (t[i] == 2 || t[i] == 12 || !exists[i] || i == j) activeNodes.erase(activeNodes.begin() + ind);
+        }
+        for (uint32_t a : activeNodes) if (t[a] == 2) return fail(LBDEM_E_STATE, "GAS ERROR: gas node in active list");
+        return 0;
+    }
+
+    // ================= coupling (LB.cpp:432-467, 683-711, 797-943, 1875-1899, 1980-2100) =================
+    void initializeParticleBoundaries() {  // LB.cpp:683-711
+        for (int64_t it = 0; it < N; ++it) { p[it] = 0; solid[it] = 0; }
+        for (auto& P : particles) {
+            const V3 c = P.x0 / unit.Length; const double rr = P.r / unit.Length;
+            const unsigned idx = P.particleIndex;
+            for (uint32_t index : activeNodes)
+                if (getPosition(index).insideSphere(c, rr)) { p[index] = 1; solid[index] = uint16_t(idx); }
+        }
+    }
+    void findNewActiveObject() {  // LB.cpp:846-943
+        for (size_t ip = 0; ip < objectNodes.size(); ++ip) {
+            const uint32_t index = objectNodes[ip];
+            const V3 pos = getPosition(index);
+            const Object& O = objects[0];
+            bool newActive = true, surroundedGas = true;
+            for (int j = 1; j < Q; ++j) {
+                const uint32_t l = d[size_t(index) * Q + j];
+                if (inRange(l) && t[l] == 0) { surroundedGas = false; break; }
+            }
+            if (pos.insideSphere(O.x0 / unit.Length, O.r / unit.Length)) newActive = false;
+            if (newActive && surroundedGas) { t[index] = 2; deleteNode(index); newActive = false; }
+            if (newActive) {
+                createNode(index);  // created while still of object type: unwrapped links
+                t[index] = 0;
+                for (int j = 1; j < Q; ++j) {
+                    const uint32_t l = d[size_t(index) * Q + j];
+                    if (inRange(l) && t[l] == 2) { t[index] = 3; break; }
+                }
+                if (t[index] == 0) nodeInitialize(index, initDensity, initVelocity, initDensity, initDynVisc, lbF);
+                else if (t[index] == 3) nodeInitialize(index, initDensity, V3(), 0.5 * initDensity, initDynVisc, lbF);
+            }
+        }
+    }
+    void updateObjectBoundaries() {  // LB.cpp:797-843
+        findNewActiveObject();
+        objectNodes.clear();
+        for (auto& O : objects) {
+            const V3 c = O.x0 / unit.Length; const double rr = O.r / unit.Length;
+            for (int64_t it = 0; it < N; ++it)
+                if (getPosition(it).insideSphere(c, rr)) { solid[it] = uint16_t(O.index); t[it] = 12; }
+        }
+        for (int64_t ip = 0; ip < N; ++ip) {
+            if (t[ip] != 12) continue;
+            createNode(ip);
+            const Object& O = objects[solid[ip]];
+            if (getPosition(ip).insideSphere(O.x0 / unit.Length, O.r / unit.Length)) {
+                nodeInitialize(ip, initDensity, O.x1 / unit.Speed, 0.0, 0.0, V3());
+                objectNodes.push_back(uint32_t(ip));
+            }
+        }
+    }
+    void findNewActive() {  // LB.cpp:1980-2029
+        std::vector<uint32_t> popUp;
+        for (size_t ip = 0; ip < particleNodes.size(); ++ip) {
+            const uint32_t index = particleNodes[ip];
+            const Particle& P = particles[solid[index]];
+            if (getPosition(index).insideSphere(P.x0 / unit.Length, P.r / unit.Length)) break;
+            popUp.push_back(index);
+            p[index] = 0; solid[index] = 0;
+        }
+        for (uint32_t q : popUp)
+            for (int ind = int(particleNodes.size()) - 1; ind >= 0; --ind)
+                if (q == particleNodes[ind]) particleNodes.erase(particleNodes.begin() + ind);
+    }
+    // LB::findNewSolid (LB.cpp:2032-2100). The `break` at LB.cpp:2062 leaves the k-loop as
+    // soon as a covered, unflagged neighbour is found, before the flagging block at
+    // LB.cpp:2069-2094 can run; that block is therefore unreachable with newSolid == true and
+    // the routine never changes a flag. Restated literally: scan, stop at the first hit.
+    void findNewSolid() {
+        for (size_t it = 0; it < particleNodes.size(); ++it) {
+            const uint32_t index = particleNodes[it];
+            const Particle& P = particles[solid[index]];
+            const V3 c = P.x0 / unit.Length; const double rr = P.r / unit.Length;
+            for (int k = 1; k < 7; ++k) {
+                const uint32_t l = link(index, k);
+                if (!inRange(l) || p[l]) continue;
+                if (getPosition(l).insideSphere(c, rr)) break;
+            }
+        }
+    }
+    void couplingStep() {  // LB.cpp:432-467
+        updateObjectBoundaries();
+        if (newNeighborList) {
+            initializeParticleBoundaries();  // updateIndices (LB.cpp:1875-1899)
+            initializeLists();
+            newNeighborList = false;
+        }
+        findNewActive();
+        findNewSolid();
+    }
+
+    // ================= LB step (LB.cpp:400-418, 1261-1544, 1901-1978) =================
+    void computeHydroForces() {  // LB.cpp:1901-1978
+        for (auto& P : particles) { P.FHydro = V3(); P.MHydro = V3(); P.fluidVolume = 0.0; }
+        for (uint32_t index : activeNodes) {
+            setH(index, V3());
+            if (!p[index]) continue;
+            Particle& P = particles[solid[index]];
+            const V3 radius = getPosition(index) - P.x0 / unit.Length;
+            const V3 localVel = P.x1 / unit.Speed + (P.w1.cross(radius)) / unit.AngVel;
+            const V3 diffVel = (mass[index] / n[index]) * (U(index) - localVel);
+            V3 hh = H(index); hh += -1.0 * diffVel; setH(index, hh);
+            P.fluidVolume += mass[index];
+            P.FHydro += 1.0 * diffVel;
+            P.MHydro += 1.0 * radius.cross(diffVel);
+        }
+        for (auto& P : particles) { P.FHydro *= unit.Force; P.MHydro *= unit.Torque; P.fluidVolume *= unit.Volume; }
+    }
+    void collision() {  // LB.cpp:1301-1332 with node.cpp:95-98, 181-207
+        if (!forceField) lbF.reset();
+        const int64_t na = int64_t(activeNodes.size());
+#pragma omp parallel for schedule(static)
+        for (int64_t it = 0; it < na; ++it) {
+            const uint32_t index = activeNodes[it];
+            double* fi = F(index);
+            const V3 totalForce = lbF + H(index);
+            V3 uu = U(index);
+            uu += 0.5 * totalForce / n[index];
+            setU(index, uu);
+            double feq[Q];
+            equilibrium(n[index], uu, feq);
+            const double omega = 1.0 / (0.5 + 3.0 * visc[index]);
+            for (int j = 0; j < Q; j++) fi[j] += omega * (feq[j] - fi[j]);
+            const double omegaf = 1.0 - 1.0 / (1.0 + 6.0 * visc[index]);
+            for (int j = 0; j < Q; j++) {
+                const V3 vj = vdir(j);
+                const double vu = uu.dot(vj);
+                const V3 vmu = vj - uu;
+                const V3 forcePart = 9.0 * vu * vj + 3.0 * vmu;
+                fi[j] += omegaf * coeff(j) * forcePart.dot(totalForce);
+            }
+        }
+    }
+    int streaming() {  // LB.cpp:1334-1544 (types 0/3 active, 2 gas, 7 static wall, 12 object)
+        double staticPres[Q];
+        for (int j = 0; j < Q; ++j) staticPres[j] = initDensity * coeff(j);
+        for (auto& w : walls) w.FHydro.reset();
+        for (auto& o : objects) { o.FHydro.reset(); o.MHydro.reset(); }
+        const int64_t na = int64_t(activeNodes.size());
+#pragma omp parallel for schedule(static)
+        for (int64_t it = 0; it < na; ++it) {
+            const uint32_t i = activeNodes[it];
+            for (int j = 0; j < Q; ++j) FS(i)[j] = F(i)[j];
+        }
+        int bad = 0;
+#pragma omp parallel for schedule(static) reduction(+ : bad)
+        for (int64_t in = 0; in < na; ++in) {
+            const uint32_t it = activeNodes[in];
+            const double* fsi = FS(it);
+            double* fi = F(it);
+            for (int j = 1; j < Q; ++j) {
+                const uint32_t l = d[size_t(it) * Q + j];
+                const uint8_t tl = t[l];
+                if (tl == 0 || tl == 3) fi[OPP[j]] = FS(l)[OPP[j]];
+                else if (tl == 2) {
+                    const V3 uu = U(it);
+                    const double usq = uu.norm2(), vuj = uu.dot(vdir(j));
+                    fi[OPP[j]] = -fsi[j] + coeff(j) * initDensity * (2.0 + 9.0 * (vuj * vuj) - 3.0 * usq);
+                } else if (tl == 7) fi[OPP[j]] = fsi[j];
+                else if (tl == 12) {
+                    const V3 vel = U(l);
+                    const double BBi = 6.0 * n[it] * coeff(j) * vel.dot(vdir(j));
+                    fi[OPP[j]] = fsi[j] - BBi;
+                } else bad++;
+            }
+        }
+        if (bad) return fail(LBDEM_E_TYPE, "TYPE ERROR: streaming met an unsupported neighbour type");
+        // momentum exchange and extra mass accumulated in ascending (node, direction) order,
+        // the order the reference's `omp ordered` sections impose (LB.cpp:1414-1453)
+        double extraMass = 0.0;
+        for (int64_t in = 0; in < na; ++in) {
+            const uint32_t it = activeNodes[in];
+            const double* fsi = FS(it);
+            for (int j = 1; j < Q; ++j) {
+                const uint32_t l = d[size_t(it) * Q + j];
+                if (t[l] == 7) {
+                    const V3 BBforce = (2.0 * (fsi[j] - staticPres[j]) - 0.0) * vdir(j);
+                    const unsigned si = solid[l];
+                    walls[si < walls.size() ? si : 0].FHydro += BBforce;
+                } else if (t[l] == 12) {
+                    const unsigned si = solid[l];
+                    const V3 vel = U(l);
+                    const double BBi = 6.0 * n[it] * coeff(j) * vel.dot(vdir(j));
+                    const V3 BBforce = (2.0 * (fsi[j] - staticPres[j]) - BBi) * vdir(j);
+                    const V3 rad = getPosition(l) - objects[si].x0 / unit.Length;
+                    objects[si].FHydro += BBforce;
+                    objects[si].MHydro += rad.cross(BBforce);
+                    extraMass += BBi * mass[it];
+                }
+            }
+        }
+        lastExtraMass = extraMass;
+        redistributeMass(extraMass);
+        for (auto& w : walls) w.FHydro *= unit.Force;
+        for (auto& o : objects) { o.FHydro *= unit.Force; o.MHydro *= unit.Torque; }
+        return 0;
+    }
+    int lbStep() {  // LB.cpp:400-418
+        initializeLists();
+        int rc = cleanLists();
+        if (rc) return rc;
+        const int64_t na = int64_t(activeNodes.size());
+#pragma omp parallel for schedule(static)
+        for (int64_t it = 0; it < na; ++it) {  // reconstruct (node.cpp:75-93)
+            const uint32_t i = activeNodes[it];
+            const double* fi = F(i);
+            double nn = 0.0;
+            for (int j = 0; j < Q; ++j) nn += fi[j];
+            V3 uu;
+            for (int j = 0; j < Q; ++j) uu += fi[j] * vdir(j);
+            uu /= nn;
+            n[i] = nn; setU(i, uu);
+        }
+        computeHydroForces();
+        collision();
+        return streaming();
+    }
+
+    // ================= free surface (LB.cpp:422-430, 1548-1852) =================
+    void redistributeMass(double massSurplus) {  // LB.cpp:1844-1852
+        const double addMass = massSurplus / double(interfaceNodes.size());
+        for (uint32_t i : interfaceNodes) mass[i] += addMass;
+    }
+    void updateMass() {  // LB.cpp:1548-1639
+        for (uint32_t i : interfaceNodes) newMass[i] = mass[i];
+        for (uint32_t index : interfaceNodes) {
+            double deltaMass = 0.0;
+            const double* fi = F(index);
+            const double* fsi = FS(index);
+            for (int j = 1; j < Q; ++j) {
+                const uint32_t l = d[size_t(index) * Q + j];
+                const double ms = fi[OPP[j]] - fsi[j];
+                if (t[l] == 3) deltaMass += 0.5 * (mass[l] + mass[index]) * ms;
+                else if (t[l] == 0) deltaMass += 1.0 * ms;
+                else if (t[l] == 8 || t[l] == 12) deltaMass += 1.0 * mass[index] * ms;
+            }
+            newMass[index] += deltaMass;
+        }
+        for (uint32_t i : fluidNodes) mass[i] = n[i];
+        for (uint32_t i : interfaceNodes) mass[i] = newMass[i];
+    }
+    void updateInterface() {  // LB.cpp:1641-1667
+        double massSurplus = 0.0;
+        std::vector<uint32_t> filled, emptied, newIf;
+        // findInterfaceMutants (LB.cpp:1669-1698)
+        for (int ind = int(interfaceNodes.size()) - 1; ind >= 0; --ind) {
+            const uint32_t i = interfaceNodes[ind];
+            if (mass[i] > n[i]) {
+                filled.push_back(i); t[i] = 0; fluidNodes.push_back(i);
+                interfaceNodes.erase(interfaceNodes.begin() + ind);
+            } else if (mass[i] < 0.0) {
+                emptied.push_back(i); t[i] = 2;
+                interfaceNodes.erase(interfaceNodes.begin() + ind);
+            }
+        }
+        // smoothenInterface (LB.cpp:1700-1749)
+        for (uint32_t index : filled) {
+            for (int j = 1; j < Q; ++j) {
+                const uint32_t l = d[size_t(index) * Q + j];
+                if (t[l] == 2) {
+                    t[l] = 3;
+                    newIf.push_back(l);
+                    createNode(l);
+                    nodeInitialize(l, initDensity, U(index), 0.01 * initDensity, visc[index], H(index) + lbF);
+                    massSurplus -= 0.01 * initDensity;
+                }
+            }
+        }
+        for (uint32_t index : emptied) {
+            for (int j = 0; j < Q; ++j) {
+                const uint32_t l = d[size_t(index) * Q + j];
+                if (t[l] == 0) {
+                    t[l] = 3;
+                    newIf.push_back(l);
+                    mass[l] = 0.99 * n[l];
+                    massSurplus += 0.01 * n[l];
+                    for (int ind = int(fluidNodes.size()) - 1; ind >= 0; --ind)
+                        if (fluidNodes[ind] == l) fluidNodes.erase(fluidNodes.begin() + ind);
+                }
+            }
+        }
+        // updateMutants (LB.cpp:1751-1790)
+        for (uint32_t e : emptied) {
+            if (std::find(newIf.begin(), newIf.end(), e) == newIf.end()) { massSurplus += mass[e]; deleteNode(e); }
+        }
+        for (uint32_t fl : filled) {
+            if (std::find(newIf.begin(), newIf.end(), fl) == newIf.end()) { massSurplus += mass[fl] - n[fl]; mass[fl] = n[fl]; }
+        }
+        for (uint32_t q : newIf) interfaceNodes.push_back(q);
+        // removeIsolated (LB.cpp:1792-1842)
+        for (int ind = int(interfaceNodes.size()) - 1; ind >= 0; --ind) {
+            const uint32_t i = interfaceNodes[ind];
+            bool surroundedFluid = true;
+            for (int j = 1; j < Q; ++j) if (t[d[size_t(i) * Q + j]] == 2) { surroundedFluid = false; break; }
+            if (surroundedFluid) {
+                massSurplus += mass[i] - n[i];
+                mass[i] = n[i];
+                t[i] = 0;
+                interfaceNodes.erase(interfaceNodes.begin() + ind);
+                fluidNodes.push_back(i);
+            }
+        }
+        for (int ind = int(interfaceNodes.size()) - 1; ind >= 0; --ind) {
+            const uint32_t i = interfaceNodes[ind];
+            bool surroundedGas = true;
+            for (int j = 1; j < Q; ++j) if (t[d[size_t(i) * Q + j]] == 0) { surroundedGas = false; break; }
+            if (surroundedGas) {
+                massSurplus += mass[i];
+                interfaceNodes.erase(interfaceNodes.begin() + ind);
+                t[i] = 2;
+                deleteNode(i);
+            }
+        }
+        lastSurplus = massSurplus;
+        redistributeMass(massSurplus);
+    }
+    int fsStep() {  // LB.cpp:422-430
+        updateMass();
+        updateInterface();
+        return cleanLists();
+    }
+
+    // ================= DEM (DEM.cpp:359-2836) =================
+    int neighborList() {  // DEM.cpp:473-652
+        nlBuilds++;
+        const int num = int(particles.size());
+        for (auto& P : particles) {
+            std::memcpy(P._list, P.list, sizeof(P.list));
+            std::memcpy(P._xix, P.xix, sizeof(P.xix));
+            std::memcpy(P._xiy, P.xiy, sizeof(P.xiy));
+            std::memcpy(P._xiz, P.xiz, sizeof(P.xiz));
+        }
+        double r_max = 0.0;
+        for (auto& P : particles) {
+            r_max = dmax(r_max, P.r);
+            P.x0_lst = P.x0;
+            P.list[0] = 0; P.xix[0] = P.xiy[0] = P.xiz[0] = 0.0;
+        }
+        double cut = 2.0 * r_max + 2.0;
+        const int nlx = int(demSize[0] / cut), nly = int(demSize[1] / cut), nlz = int(demSize[2] / cut);
+        bool overflow = false;
+        auto add = [&](int n1, int n2) {
+            Particle& A = particles[n1]; Particle& B = particles[n2];
+            if (A.list[0] + 1 >= MAXL || B.list[0] + 1 >= MAXL) { overflow = true; return false; }
+            A.list[0]++; B.list[0]++;
+            A.xix[0] += 1.0; A.xiy[0] += 1.0; A.xiz[0] += 1.0;
+            B.xix[0] += 1.0; B.xiy[0] += 1.0; B.xiz[0] += 1.0;
+            A.list[A.list[0]] = n2; B.list[B.list[0]] = n1;
+            return true;
+        };
+        auto within = [&](int n1, int n2) {
+            const Particle& A = particles[n1]; const Particle& B = particles[n2];
+            const double x12 = n_image(A.x0.x - B.x0.x, demSize[0]);
+            const double y12 = n_image(A.x0.y - B.x0.y, demSize[1]);
+            const double z12 = n_image(A.x0.z - B.x0.z, demSize[2]);
+            const double r12 = x12 * x12 + y12 * y12 + z12 * z12;
+            const double sig = A.r + B.r;
+            const double c = sig + 2.0;
+            return r12 <= c * c;
+        };
+        if ((nlx < nly ? nlx : nly) < 3) {
+            for (int n1 = 1; n1 < num; n1++)
+                for (int n2 = 0; n2 < n1; n2++) {
+                    if (!within(n1, n2)) continue;
+                    if (!add(n1, n2)) continue;
+                    Particle& A = particles[n1]; Particle& B = particles[n2];
+                    A.xix[A.list[0]] = B.xix[B.list[0]]; B.xix[B.list[0]] = A.xix[A.list[0]];
+                    A.xiy[A.list[0]] = B.xiy[B.list[0]]; B.xiy[B.list[0]] = A.xiy[A.list[0]];
+                    A.xiz[A.list[0]] = B.xiz[B.list[0]]; B.xiz[B.list[0]] = A.xiz[A.list[0]];
+                }
+        } else {
+            const int ncell = nlx * nly * nlz;
+            std::vector<std::vector<int>> cell(ncell);
+            for (int n1 = 0; n1 < num; n1++) {
+                Particle& P = particles[n1];
+                const int lx = int(dbox(P.x0.x / demSize[0], 1) * nlx);
+                const int ly = int(dbox(P.x0.y / demSize[1], 1) * nly);
+                const int lz = int(dbox(P.x0.z / demSize[2], 1) * nlz);
+                cell[lx * nly * nlz + ly * nlz + lz].push_back(n1);
+                P.list[0] = 0; P.xix[0] = P.xiy[0] = P.xiz[0] = 0.0;
+            }
+            for (int lx1 = 0; lx1 < nlx; lx1++)
+                for (int ly1 = 0; ly1 < nly; ly1++)
+                    for (int lz1 = 0; lz1 < nlz; lz1++)
+                        for (int lx2 = lx1 - 1; lx2 <= lx1 + 1; lx2++)
+                            for (int ly2 = ly1 - 1; ly2 <= ly1 + 1; ly2++)
+                                for (int lz2 = lz1 - 1; lz2 <= lz1 + 1; lz2++) {
+                                    const int l1 = lx1 * nly * nlz + ly1 * nlz + lz1;
+                                    const int l2 = ((lx2 + nlx) % nlx) * nly * nlz + ((ly2 + nly) % nly) * nlz + (lz2 + nlz) % nlz;
+                                    for (int a : cell[l1])
+                                        for (int b : cell[l2])
+                                            if (b < a && within(a, b)) add(a, b);
+                                }
+        }
+        if (overflow) { err = "neighbour list capacity (63) exceeded"; }
+        for (auto& P : particles)
+            for (int nn = 1; nn <= P.list[0]; nn++) {
+                const int k = P.list[nn];
+                for (int l = 1; l <= P._list[0]; l++)
+                    if (P._list[l] == k) { P.xix[nn] = P._xix[l]; P.xiy[nn] = P._xiy[l]; P.xiz[nn] = P._xiz[l]; }
+            }
+        return overflow ? LBDEM_E_CAPACITY : 0;
+    }
+
+    // Jeffrey-Onishi / Nguyen-Ladd lubrication coefficients shared by the particle-particle
+    // and object-particle routines (DEM.cpp:1181-1257, 2349-2416)
+    struct LubOut { V3 F, T1, T2; };
+    LubOut pairLub(const V3& xa, const V3& xb, double ra, double rb, const V3& va, const V3& vb, const V3& wa, const V3& wb,
+                   double beta1_num, double beta1_den, bool& active) const {
+        LubOut o;
+        const double lub_N = prm.lubNormal, lub_T = prm.lubTangential, lub_R = prm.lubRotation;
+        const double cut = prm.lubCutoff;
+        const double rad = 0.5 * (ra + rb);
+        double x12 = xa.x - xb.x, y12 = xa.y - xb.y, z12 = xa.z - xb.z;
+        const double r12 = std::sqrt(x12 * x12 + y12 * y12 + z12 * z12);
+        const double umx = va.x - vb.x, umy = va.y - vb.y, umz = va.z - vb.z;
+        const double w1x = wa.x, w1y = wa.y, w1z = wa.z, w2x = wb.x, w2y = wb.y, w2z = wb.z;
+        const double dr = r12 - 2.0 * rad;
+        const double h = dmax(dr, cut);
+        active = dr < dmax(cut, dmax(lub_N, dmax(lub_T, lub_R)));
+        if (!active) return o;
+        double Xa, Ya, Yb1, Yb2, Yc11, Yc12, Yc21, Yc22, h_c, beta;
+        const double nu = prm.kinVisc;
+        const double alpha = M_PI * nu * rad * deltat;
+        const double beta1 = beta1_num / beta1_den;
+        const double beta2 = beta1 * beta1, beta3 = beta1 * beta2;
+        const double beta4 = std::pow((1 + beta1), 4.0);
+        if (dr < dmax(cut, lub_N)) {
+            Xa = 4.0 * beta2 / beta4;
+            h_c = dmax(dr, lub_N);
+            Xa *= rad / h - rad / h_c;
+            Xa *= 6.0 * alpha;
+        } else Xa = 0.0;
+        if (dr < dmax(cut, lub_T)) {
+            Ya = 8.0 * beta1 * (2.0 + beta1 + 2.0 * beta2) / (15.0 * beta4);
+            Yb1 = -4.0 * beta1 * (4.0 + beta1) / (5.0 * beta4);
+            Yb2 = 4.0 * beta1 * (4.0 * beta2 + beta1) / (5.0 * beta4);
+            h_c = dmax(dr, lub_T);
+            beta = alpha * std::log(h_c / h);
+            Ya *= 6.0 * beta;
+            Yb1 *= 4.0 * beta * rad;
+            Yb2 *= 4.0 * beta * rad;
+        } else Ya = Yb1 = Yb2 = 0.0;
+        if (dr < dmax(cut, lub_R)) {
+            Yc11 = 16.0 * beta1 / (5.0 * beta4);
+            Yc12 = 4.0 * beta2 / (5.0 * beta4);
+            Yc21 = 4.0 * beta2 / (5.0 * beta4);
+            Yc22 = 16.0 * beta3 / (5.0 * beta4);
+            h_c = dmax(dr, lub_R);
+            beta = alpha * std::log(h_c / h);
+            Yc11 *= 8.0 * beta * rad * rad;
+            Yc12 *= 8.0 * beta * rad * rad;
+            Yc21 *= 8.0 * beta * rad * rad;
+            Yc22 *= 8.0 * beta * rad * rad;
+        } else Yc11 = Yc12 = Yc21 = Yc22 = 0.0;
+        x12 /= r12; y12 /= r12; z12 /= r12;
+        const double um_r = umx * x12 + umy * y12 + umz * z12;
+        const double w1_r = w1x * x12 + w1y * y12 + w1z * z12;
+        const double w2_r = w2x * x12 + w2y * y12 + w2z * z12;
+        const double umxrx = umy * z12 - umz * y12, umxry = umz * x12 - umx * z12, umxrz = umx * y12 - umy * x12;
+        const double w1xrx = w1y * z12 - w1z * y12, w1xry = w1z * x12 - w1x * z12, w1xrz = w1x * y12 - w1y * x12;
+        const double w2xrx = w2y * z12 - w2z * y12, w2xry = w2z * x12 - w2x * z12, w2xrz = w2x * y12 - w2y * x12;
+        double Fx = -Xa * um_r * x12 - Ya * (umx - um_r * x12);
+        double Fy = -Xa * um_r * y12 - Ya * (umy - um_r * y12);
+        double Fz = -Xa * um_r * z12 - Ya * (umz - um_r * z12);
+        Fx += -Yb1 * w1xrx + Yb2 * w2xrx;
+        Fy += -Yb1 * w1xry + Yb2 * w2xry;
+        Fz += -Yb1 * w1xrz + Yb2 * w2xrz;
+        double T1x = Yb1 * umxrx, T1y = Yb1 * umxry, T1z = Yb1 * umxrz;
+        double T2x = -Yb2 * umxrx, T2y = -Yb2 * umxry, T2z = -Yb2 * umxrz;
+        T1x += -Yc11 * (w1x - w1_r * x12) - Yc12 * (w2x - w2_r * x12);
+        T1y += -Yc11 * (w1y - w1_r * y12) - Yc12 * (w2y - w2_r * y12);
+        T1z += -Yc11 * (w1z - w1_r * z12) - Yc12 * (w2z - w2_r * z12);
+        T2x += -Yc21 * (w1x - w1_r * x12) - Yc22 * (w2x - w2_r * x12);
+        T2y += -Yc21 * (w1y - w1_r * y12) - Yc22 * (w2y - w2_r * y12);
+        T2z += -Yc21 * (w1z - w1_r * z12) - Yc22 * (w2z - w2_r * z12);
+        o.F = V3(Fx, Fy, Fz); o.T1 = V3(T1x, T1y, T1z); o.T2 = V3(T2x, T2y, T2z);
+        return o;
+    }
+    void particleLubrication() {  // DEM.cpp:2302-2521
+        for (int n1 = 0; n1 < int(particles.size()); n1++) {
+            Particle& A = particles[n1];
+            for (int nn = 1; nn <= A.list[0]; nn++) {
+                const int n2 = A.list[nn];
+                if (n2 >= n1) continue;
+                Particle& B = particles[n2];
+                bool act;
+                LubOut o = pairLub(A.x0, B.x0, A.r, B.r, A.x1, B.x1, A.w1, B.w1, B.r, A.r, act);
+                if (!act) continue;
+                A.FLub += o.F; B.FLub -= o.F; A.MLub += o.T1; B.MLub += o.T2;
+            }
+        }
+    }
+    void wallLubrication() {  // DEM.cpp:2525-2672
+        const double lub_N = prm.lubNormal, lub_T = prm.lubTangential, lub_R = prm.lubRotation, cut = prm.lubCutoff;
+        for (auto& P : particles)
+            for (auto& W : walls) {
+                const double rad = P.r;
+                double x12 = (P.x0.x - W.p.x) * W.n.x * W.n.x;
+                double y12 = (P.x0.y - W.p.y) * W.n.y * W.n.y;
+                double z12 = (P.x0.z - W.p.z) * W.n.z * W.n.z;
+                const double r12 = std::sqrt(x12 * x12 + y12 * y12 + z12 * z12);
+                const double umx = P.x1.x, umy = P.x1.y, umz = P.x1.z;
+                const double w1x = P.w1.x, w1y = P.w1.y, w1z = P.w1.z;
+                const double dr = r12 - rad;
+                const double h = dmax(dr, cut);
+                if (!(dr < dmax(cut, dmax(lub_N, dmax(lub_T, lub_R))))) continue;
+                const double alpha = M_PI * prm.kinVisc * rad * deltat;
+                double Xa, Ya, Yb1, Yc11, h_c, beta;
+                if (dr < dmax(cut, lub_N)) { Xa = 1.0; h_c = dmax(dr, lub_N); Xa *= rad / h - rad / h_c; Xa *= 6.0 * alpha; }
+                else Xa = 0.0;
+                if (dr < dmax(cut, lub_T)) {
+                    Ya = 8.0 / 15.0; Yb1 = -1.0 / 5.0; h_c = dmax(dr, lub_T); beta = alpha * std::log(h_c / h);
+                    Ya *= 6.0 * beta; Yb1 *= 4.0 * beta * rad;
+                } else Ya = Yb1 = 0.0;
+                if (dr < dmax(cut, lub_R)) { Yc11 = 2.0 / 5.0; h_c = dmax(dr, lub_R); beta = alpha * std::log(h_c / h); Yc11 *= 8.0 * beta * rad * rad; }
+                else Yc11 = 0.0;
+                x12 /= r12; y12 /= r12; z12 /= r12;
+                const double um_r = umx * x12 + umy * y12 + umz * z12;
+                const double w1_r = w1x * x12 + w1y * y12 + w1z * z12;
+                const double umxrx = umy * z12 - umz * y12, umxry = umz * x12 - umx * z12, umxrz = umx * y12 - umy * x12;
+                const double w1xrx = w1y * z12 - w1z * y12, w1xry = w1z * x12 - w1x * z12, w1xrz = w1x * y12 - w1y * x12;
+                double Fx = -Xa * x12 * um_r - Ya * (umx - um_r * x12);
+                double Fy = -Xa * y12 * um_r - Ya * (umy - um_r * y12);
+                double Fz = -Xa * z12 * um_r - Ya * (umz - um_r * z12);
+                Fx += -Yb1 * w1xrx; Fy += -Yb1 * w1xry; Fz += -Yb1 * w1xrz;
+                double T1x = Yb1 * umxrx, T1y = Yb1 * umxry, T1z = Yb1 * umxrz;
+                T1x += -Yc11 * (w1x - w1_r * x12); T1y += -Yc11 * (w1y - w1_r * y12); T1z += -Yc11 * (w1z - w1_r * z12);
+                P.FLub += V3(Fx, Fy, Fz); P.MLub += V3(T1x, T1y, T1z); W.FLub -= V3(Fx, Fy, Fz);
+            }
+    }
+    // tangential spring-dashpot with Coulomb cap, shared by the wall and fictional-lid contacts
+    // (DEM.cpp:1962-2063, 2175-2254); xi is the per-particle wall history xiw.
+    void wallLikeContact(Particle& P, double x1w, double y1w, double z1w, V3& Fout, V3& Mout) {
+        const double radw = P.r;
+        const double r1w = std::sqrt(x1w * x1w + y1w * y1w + z1w * z1w);
+        const double urxw = P.x1.x, uryw = P.x1.y, urzw = P.x1.z;
+        const double delw = radw - r1w;
+        double fnx = 0, fny = 0, fnz = 0, ftx = 0, fty = 0, ftz = 0, n1wx = 0, n1wy = 0, n1wz = 0;
+        if (delw > 0.0) {
+            n1wx = x1w / r1w; n1wy = y1w / r1w; n1wz = z1w / r1w;
+            const double gn1w = urxw * n1wx + uryw * n1wy + urzw * n1wz;
+            const double muw = prm.frictionCoefPart, k = prm.linearStiff;
+            fnx = k * delw * n1wx - dampCoeff * gn1w * n1wx;
+            fny = k * delw * n1wy - dampCoeff * gn1w * n1wy;
+            fnz = k * delw * n1wz - dampCoeff * gn1w * n1wz;
+            if (muw != 0.0) {
+                const double o1wx = P.r * P.w1.x, o1wy = P.r * P.w1.y, o1wz = P.r * P.w1.z;
+                const double v1wx = urxw + n1wy * o1wz - n1wz * o1wy - gn1w * n1wx;
+                const double v1wy = uryw + n1wz * o1wx - n1wx * o1wz - gn1w * n1wy;
+                const double v1wz = urzw + n1wx * o1wy - n1wy * o1wx - gn1w * n1wz;
+                const double xi1w = P.xiw.x * n1wx + P.xiw.y * n1wy + P.xiw.z * n1wz;
+                P.xiw.x -= xi1w * n1wx; P.xiw.y -= xi1w * n1wy; P.xiw.z -= xi1w * n1wz;
+                const double ktw = 0.2 * k;
+                const double dct = prm.kinVisc == 0.0 ? dampCoeff : 0.0;
+                ftx = -ktw * P.xiw.x - dct * v1wx;
+                fty = -ktw * P.xiw.y - dct * v1wy;
+                ftz = -ktw * P.xiw.z - dct * v1wz;
+                const double fnnw = std::sqrt(fnx * fnx + fny * fny + fnz * fnz);
+                const double fttw = std::sqrt(ftx * ftx + fty * fty + ftz * ftz);
+                const double friw = muw * fnnw;
+                const double t1wx = ftx / fttw, t1wy = fty / fttw, t1wz = ftz / fttw;
+                P.xiw.x += v1wx * deltat; P.xiw.y += v1wy * deltat; P.xiw.z += v1wz * deltat;
+                if (friw < fttw) {
+                    ftx = friw * t1wx; fty = friw * t1wy; ftz = friw * t1wz;
+                    P.xiw.x = -(1 / ktw) * (ftx + dct * v1wx);
+                    P.xiw.y = -(1 / ktw) * (fty + dct * v1wy);
+                    P.xiw.z = -(1 / ktw) * (ftz + dct * v1wz);
+                }
+            }
+        } else {
+            P.xiw = V3();
+        }
+        const double dfx = fnx + ftx, dfy = fny + fty, dfz = fnz + ftz;
+        const double dtx = fty * n1wz - ftz * n1wy, dty = ftz * n1wx - ftx * n1wz, dtz = ftx * n1wy - fty * n1wx;
+        Fout = V3(dfx, dfy, dfz);
+        Mout = V3(P.r * dtx, P.r * dty, P.r * dtz);
+    }
+    void wallContacts() {  // DEM.cpp:1946-2153
+        for (auto& P : particles)
+            for (auto& W : walls) {
+                const double x1w = (P.x0.x - W.p.x) * W.n.x * W.n.x;
+                const double y1w = (P.x0.y - W.p.y) * W.n.y * W.n.y;
+                const double z1w = (P.x0.z - W.p.z) * W.n.z * W.n.z;
+                V3 Fc, Mc;
+                wallLikeContact(P, x1w, y1w, z1w, Fc, Mc);
+                P.FContact += Fc; W.FContact -= Fc; P.MContact += Mc;
+            }
+    }
+    void fictionalWallContacts() {  // DEM.cpp:2157-2296
+        for (auto& P : particles) {
+            V3 Fc, Mc;
+            wallLikeContact(P, 0.0, P.x0.y - (demSize[1] - 0.3), 0.0, Fc, Mc);
+            P.FContact += Fc; P.MContact += Mc;
+        }
+    }
+    void objectParticleContacts() {  // DEM.cpp:912-1138
+        for (auto& P : particles)
+            for (auto& O : objects) {
+                const double radw = 0.5 * (P.r + O.r);
+                const double x1w = P.x0.x - O.x0.x, y1w = P.x0.y - O.x0.y, z1w = P.x0.z - O.x0.z;
+                const double r1w = std::sqrt(x1w * x1w + y1w * y1w + z1w * z1w);
+                const double n1wx = x1w / r1w, n1wy = y1w / r1w, n1wz = z1w / r1w;
+                const double urxw = P.x1.x - O.x1.x, uryw = P.x1.y - O.x1.y, urzw = P.x1.z - O.x1.z;
+                const double delw = (2 * radw) - r1w;
+                double fnx = 0, fny = 0, fnz = 0, ftx = 0, fty = 0, ftz = 0;
+                if (delw > 0.0) {
+                    const double gn1w = urxw * n1wx + uryw * n1wy + urzw * n1wz;
+                    const double muw = prm.frictionCoefPart, k = prm.linearStiff;
+                    fnx = k * delw * n1wx - dampCoeff * gn1w * n1wx;
+                    fny = k * delw * n1wy - dampCoeff * gn1w * n1wy;
+                    fnz = k * delw * n1wz - dampCoeff * gn1w * n1wz;
+                    if (muw != 0.0) {
+                        const double o1wx = P.r * P.w1.x + O.r * O.w1.x;
+                        const double o1wy = P.r * P.w1.y + O.r * O.w1.y;
+                        const double o1wz = P.r * P.w1.z + O.r * O.w1.z;
+                        const double v1wx = urxw + n1wy * o1wz - n1wz * o1wy - gn1w * n1wx;
+                        const double v1wy = uryw + n1wz * o1wx - n1wx * o1wz - gn1w * n1wy;
+                        const double v1wz = urzw + n1wx * o1wy - n1wy * o1wx - gn1w * n1wz;
+                        const double xi1w = P.xio.x * n1wx + P.xio.y * n1wy + P.xio.z * n1wz;
+                        P.xio.x -= xi1w * n1wx; P.xio.y -= xi1w * n1wy; P.xio.z -= xi1w * n1wz;
+                        const double ktw = 0.2 * k;
+                        const double dct = prm.kinVisc == 0.0 ? dampCoeff : 0.0;  // pinned (see header)
+                        ftx = -ktw * P.xio.x; fty = -ktw * P.xio.y; ftz = -ktw * P.xio.z;
+                        const double fnnw = std::sqrt(fnx * fnx + fny * fny + fnz * fnz);
+                        const double fttw = std::sqrt(ftx * ftx + fty * fty + ftz * ftz);
+                        const double friw = muw * fnnw;
+                        const double t1wx = ftx / fttw, t1wy = fty / fttw, t1wz = ftz / fttw;
+                        if (friw < fttw) {
+                            ftx = friw * t1wx; fty = friw * t1wy; ftz = friw * t1wz;
+                            P.xio.x = -(1 / ktw) * (ftx + dct * v1wx);
+                            P.xio.y = -(1 / ktw) * (fty + dct * v1wy);
+                            P.xio.z = -(1 / ktw) * (ftz + dct * v1wz);
+                        } else {
+                            P.xio.x += v1wx * deltat; P.xio.y += v1wy * deltat; P.xio.z += v1wz * deltat;
+                        }
+                    }
+                } else {
+                    P.xio = V3();
+                }
+                const double dfx = fnx + ftx, dfy = fny + fty, dfz = fnz + ftz;
+                const double dtx = fty * n1wz - ftz * n1wy, dty = ftz * n1wx - ftx * n1wz, dtz = ftx * n1wy - fty * n1wx;
+                P.FContact += V3(dfx, dfy, dfz);
+                O.FContact -= V3(dfx, dfy, dfz);
+                P.MContact += V3(P.r * dtx, P.r * dty, P.r * dtz);
+                O.MContact += V3(O.r * dtx, O.r * dty, O.r * dtz);
+            }
+    }
+    void objectParticleLubrication() {  // DEM.cpp:1140-1340
+        for (auto& P : particles)
+            for (auto& O : objects) {
+                bool act;
+                LubOut o = pairLub(P.x0, O.x0, P.r, O.r, P.x1, O.x1, P.w1, O.w1, O.r, P.r, act);
+                if (!act) continue;
+                P.FLub += o.F; O.FLub -= o.F; P.MLub += o.T1; O.MLub += o.T2;
+            }
+    }
+    void particleContacts() {  // DEM.cpp:1487-1791
+        const double k = prm.linearStiff, mu = prm.frictionCoefPart;
+        for (int n1 = 0; n1 < int(particles.size()); n1++) {
+            Particle& A = particles[n1];
+            for (int nn = 1; nn <= A.list[0]; nn++) {
+                const int n2 = A.list[nn];
+                if (n2 >= n1) continue;
+                Particle& B = particles[n2];
+                const double rad = 0.5 * (A.r + B.r);
+                const double x12 = A.x0.x - B.x0.x, y12 = A.x0.y - B.x0.y, z12 = A.x0.z - B.x0.z;
+                const double r12 = std::sqrt(x12 * x12 + y12 * y12 + z12 * z12);
+                const double urx = A.x1.x - B.x1.x, ury = A.x1.y - B.x1.y, urz = A.x1.z - B.x1.z;
+                const double n12x = x12 / r12, n12y = y12 / r12, n12z = z12 / r12;
+                const double gn12 = urx * n12x + ury * n12y + urz * n12z;
+                const double del = 2 * rad - r12;
+                double fnx = 0, fny = 0, fnz = 0, ftx = 0, fty = 0, ftz = 0;
+                if (del > 0.0) {
+                    fnx = k * del * n12x - dampCoeff * gn12 * n12x;
+                    fny = k * del * n12y - dampCoeff * gn12 * n12y;
+                    fnz = k * del * n12z - dampCoeff * gn12 * n12z;
+                    if (mu != 0.0) {
+                        const double o12x = A.r * A.w1.x + B.r * B.w1.x;
+                        const double o12y = A.r * A.w1.y + B.r * B.w1.y;
+                        const double o12z = A.r * A.w1.z + B.r * B.w1.z;
+                        const double v12x = urx + n12y * o12z - n12z * o12y - gn12 * n12x;
+                        const double v12y = ury + n12z * o12x - n12x * o12z - gn12 * n12y;
+                        const double v12z = urz + n12x * o12y - n12y * o12x - gn12 * n12z;
+                        const double xi12 = A.xix[nn] * n12x + A.xiy[nn] * n12y + A.xiz[nn] * n12z;
+                        A.xix[nn] -= xi12 * n12x; A.xiy[nn] -= xi12 * n12y; A.xiz[nn] -= xi12 * n12z;
+                        const double kt = 0.2 * k;
+                        const double dct = prm.kinVisc == 0.0 ? dampCoeff : 0.0;
+                        ftx = -kt * A.xix[nn] - dct * v12x;
+                        fty = -kt * A.xiy[nn] - dct * v12y;
+                        ftz = -kt * A.xiz[nn] - dct * v12z;
+                        const double fnn = std::sqrt(fnx * fnx + fny * fny + fnz * fnz);
+                        const double ftt = std::sqrt(ftx * ftx + fty * fty + ftz * ftz);
+                        const double fri = mu * fnn;
+                        const double t12x = ftx / ftt, t12y = fty / ftt, t12z = ftz / ftt;
+                        A.xix[nn] += v12x * deltat; A.xiy[nn] += v12y * deltat; A.xiz[nn] += v12z * deltat;
+                        if (fri < ftt) {
+                            ftx = fri * t12x; fty = fri * t12y; ftz = fri * t12z;
+                            A.xix[nn] = -(1 / kt) * (ftx + dct * v12x);
+                            A.xiy[nn] = -(1 / kt) * (fty + dct * v12y);
+                            A.xiz[nn] = -(1 / kt) * (ftz + dct * v12z);
+                        }
+                    }
+                } else {
+                    A.xix[nn] = A.xiy[nn] = A.xiz[nn] = 0.0;
+                }
+                const double dfx = fnx + ftx, dfy = fny + fty, dfz = fnz + ftz;
+                const double dtx = fty * n12z - ftz * n12y, dty = ftz * n12x - ftx * n12z, dtz = ftx * n12y - fty * n12x;
+                A.FContact += V3(dfx, dfy, dfz);
+                B.FContact -= V3(dfx, dfy, dfz);
+                A.MContact += V3(A.r * dtx, A.r * dty, A.r * dtz);
+                B.MContact += V3(B.r * dtx, B.r * dty, B.r * dtz);
+            }
+        }
+    }
+    void particleElectro() {  // DEM.cpp:1404-1484
+        const double fer = prm.electroForce, lamda = prm.debyeLength;
+        for (int n1 = 0; n1 < int(particles.size()); n1++) {
+            Particle& A = particles[n1];
+            for (int nn = 1; nn <= A.list[0]; nn++) {
+                const int n2 = A.list[nn];
+                if (n2 >= n1) continue;
+                Particle& B = particles[n2];
+                const double rad = 0.5 * (A.r + B.r);
+                const double x12 = A.x0.x - B.x0.x, y12 = A.x0.y - B.x0.y, z12 = A.x0.z - B.x0.z;
+                const double r12 = std::sqrt(x12 * x12 + y12 * y12 + z12 * z12);
+                const double n12x = x12 / r12, n12y = y12 / r12, n12z = z12 / r12;
+                const double del = 2 * rad - r12;
+                double frx, fry, frz;
+                if (del <= 0.0) {
+                    frx = fer * std::exp(del / lamda) * n12x;
+                    fry = fer * std::exp(del / lamda) * n12y;
+                    frz = fer * std::exp(del / lamda) * n12z;
+                } else {
+                    frx = fer * n12x; fry = fer * n12y; frz = fer * n12z;
+                }
+                A.FRep += V3(frx, fry, frz);
+                B.FRep -= V3(frx, fry, frz);
+            }
+        }
+    }
+    void evaluateForces() {  // DEM.cpp:655-796
+        for (auto& P : particles) {
+            P.FTotal.reset(); P.MTotal.reset(); P.FContact.reset(); P.MContact.reset();
+            P.FLub.reset(); P.MLub.reset(); P.FRep.reset();
+        }
+        for (auto& O : objects) {
+            O.FTotal.reset(); O.FContact.reset(); O.FLub.reset(); O.FSpring.reset();
+            O.MTotal.reset(); O.MContact.reset(); O.MLub.reset();
+        }
+        for (auto& W : walls) { W.FTotal.reset(); W.FContact.reset(); W.FLub.reset(); W.FRep.reset(); }
+        if (int(demTimeStep) >= prm.objWaitTime) {
+            particleLubrication();
+            wallLubrication();
+            objectParticleContacts();
+            objectParticleLubrication();
+        } else {
+            fictionalWallContacts();
+        }
+        particleContacts();
+        particleElectro();
+        wallContacts();
+        for (auto& P : particles) {
+            P.FTotal += P.FHydro + P.FContact + P.FLub + P.FGrav + P.FRep;
+            P.MTotal += P.MHydro + P.MContact + P.MLub;
+        }
+        for (auto& W : walls) W.FTotal += W.FHydro + W.FContact + W.FLub + W.FRep;
+        if (int(demTimeStep) <= prm.objWaitTime)
+            for (auto& P : particles) P.FGrav = V3();
+        for (auto& O : objects) {
+            O.FTotal += O.FHydro + O.FContact + O.FLub + O.FGrav + O.FSpring;
+            O.MTotal += O.MHydro + O.MContact + O.MLub;
+        }
+    }
+    void integration() {  // DEM.cpp:2676-2765
+        for (auto& P : particles) {
+            P.x0.x += deltat * P.x1.x + deltat * deltat * P.FTotal.x * P._m;
+            P.x0.y += deltat * P.x1.y + deltat * deltat * P.FTotal.y * P._m;
+            P.x0.z += deltat * P.x1.z + deltat * deltat * P.FTotal.z * P._m;
+            P.x1.x += deltat * P.FTotal.x * P._m;
+            P.x1.y += deltat * P.FTotal.y * P._m;
+            P.x1.z += deltat * P.FTotal.z * P._m;
+            P.w1.x += deltat * P.MTotal.x * P._I;
+            P.w1.y += deltat * P.MTotal.y * P._I;
+            P.w1.z += deltat * P.MTotal.z * P._I;
+        }
+        if (int(demTimeStep) >= prm.objWaitTime)
+            for (auto& O : objects) {
+                O.x0.x += deltat * O.x1.x + deltat * deltat * O.FTotal.x * O._m;
+                O.x0.y += deltat * O.x1.y + deltat * deltat * O.FTotal.y * O._m;
+                O.x0.z += deltat * O.x1.z + deltat * deltat * O.FTotal.z * O._m;
+                O.x1.x += deltat * O.FTotal.x * O._m;
+                O.x1.y += deltat * O.FTotal.y * O._m;
+                O.x1.z += deltat * O.FTotal.z * O._m;
+                O.w1.x += deltat * O.MTotal.x * O._I;
+                O.w1.y += deltat * O.MTotal.y * O._I;
+                O.w1.z += deltat * O.MTotal.z * O._I;
+            }
+    }
+    void pbcs() {  // DEM.cpp:385-398, 2769-2836 (loop index pinned to start at 0)
+        double* sz = demSize;
+        auto wrap = [&](int a) {
+            for (auto& P : particles) {
+                double& c = a == 0 ? P.x0.x : (a == 1 ? P.x0.y : P.x0.z);
+                if (c > sz[a]) c -= sz[a];
+                if (c < 0.0) c += sz[a];
+            }
+        };
+        if (boundary[0] == 4 && boundary[1] == 4) wrap(0);
+        if (boundary[2] == 4 && boundary[3] == 4) wrap(1);
+        if (boundary[4] == 4 && boundary[5] == 4) wrap(2);
+    }
+    int demStep() {  // DEM.cpp:359-401
+        demTimeStep++;
+        demTime += deltat;
+        double maxVel = 0.0;  // evalMaxDisp (DEM.cpp:460-470)
+        for (auto& P : particles) if (P.x1.norm2() > maxVel) maxVel = P.x1.norm2();
+        maxVel = std::sqrt(maxVel);
+        maxDisp += maxVel * deltat;
+        int rc = 0;
+        if (maxDisp > 0.05) { maxDisp = 0.0; rc = neighborList(); newNeighborList = true; }
+        evaluateForces();
+        integration();
+        pbcs();
+        return rc;
+    }
+    // goCycle (main.cpp:38-67) without IO::outputStep
+    int cycleOnce() {
+        realTime += unit.Time;
+        ++currentTimeStep;
+        ++lbTime;
+        int rc = 0;
+        if (prm.demSolve) { rc = demStep(); if (rc) return rc; }
+        if (demTime >= demInitialRepeat) {
+            if (prm.demSolve) couplingStep();
+            rc = lbStep(); if (rc) return rc;
+            if (freeSurface) rc = fsStep();
+        } else if (demTime <= demInitialRepeat) {
+            if (prm.demSolve) couplingStep();
+        }
+        return rc;
+    }
+};
+
+extern "C" {
+
+int lbdem_create(const lbdem_params* params, lbdem_ctx** out) {
+    if (!params || !out) return LBDEM_E_ARG;
+    lbdem_ctx* c = new lbdem_ctx();
+    c->prm = *params;
+    int rc = c->getParams();
+    if (rc) { *out = c; return rc; }
+    *out = c;
+    return 0;
+}
+int lbdem_destroy(lbdem_ctx* ctx) { delete ctx; return 0; }
+const char* lbdem_last_error(const lbdem_ctx* ctx) { return ctx ? ctx->err.c_str() : "null context"; }
+
+int lbdem_set_particles(lbdem_ctx* c, int n, const int* size, const double* r, const double* x0) {
+    if (!c || n < 0 || (n > 0 && (!r || !x0))) return LBDEM_E_ARG;
+    if (n > 65535) return c->fail(LBDEM_E_ARG, "more than 65535 particles (solidIndex is 16-bit, node.h:135)");
+    c->particles.assign(n, Particle());
+    for (int i = 0; i < n; ++i) {
+        c->particles[i].size = size ? unsigned(size[i]) : 1u;
+        c->particles[i].r = r[i];
+        c->particles[i].x0 = V3(x0[3 * i], x0[3 * i + 1], x0[3 * i + 2]);
+    }
+    return 0;
+}
+int lbdem_set_objects(lbdem_ctx* c, int n, const double* r, const double* x0, const double* x1) {
+    if (!c || n < 0 || (n > 0 && (!r || !x0 || !x1))) return LBDEM_E_ARG;
+    c->objects.assign(n, Object());
+    for (int i = 0; i < n; ++i) {
+        c->objects[i].index = unsigned(i);
+        c->objects[i].r = r[i];
+        c->objects[i].x0 = V3(x0[3 * i], x0[3 * i + 1], x0[3 * i + 2]);
+        c->objects[i].x1 = V3(x1[3 * i], x1[3 * i + 1], x1[3 * i + 2]);
+    }
+    return 0;
+}
+int lbdem_init(lbdem_ctx* c) {
+    if (!c) return LBDEM_E_ARG;
+    c->latticeDefinition();
+    c->discreteElementInit();
+    c->latticeBolzmannInit();
+    c->lbTime = 0; c->demTime = 0.0; c->realTime = 0.0; c->demTimeStep = 0;
+    c->initialised = true;
+    return c->err.empty() ? 0 : LBDEM_E_CAPACITY;
+}
+#define NEED_INIT if (!c || !c->initialised) return LBDEM_E_STATE
+int lbdem_dem_step(lbdem_ctx* c) { NEED_INIT; return c->demStep(); }
+int lbdem_coupling_step(lbdem_ctx* c) { NEED_INIT; c->couplingStep(); return 0; }
+int lbdem_lb_step(lbdem_ctx* c) { NEED_INIT; return c->lbStep(); }
+int lbdem_fs_step(lbdem_ctx* c) { NEED_INIT; return c->fsStep(); }
+int lbdem_cycle(lbdem_ctx* c, int nsteps) {
+    NEED_INIT;
+    for (int s = 0; s < nsteps; ++s) { int rc = c->cycleOnce(); if (rc) return rc; }
+    return 0;
+}
+int lbdem_synchronize(lbdem_ctx* c) { return c ? 0 : LBDEM_E_ARG; }
+
+int lbdem_get_info(lbdem_ctx* c, lbdem_info* in) {
+    NEED_INIT;
+    std::memset(in, 0, sizeof(*in));
+    for (int a = 0; a < 3; ++a) in->size[a] = int(c->lbSize[a]);
+    in->totNodes = c->N;
+    in->nParticles = int(c->particles.size()); in->nObjects = int(c->objects.size()); in->nWalls = int(c->walls.size());
+    int na = 0, nf = 0, ni = 0, np = 0, no = 0;
+    for (int64_t i = 0; i < c->N; ++i) {
+        if (c->t[i] == 0) nf++;
+        if (c->t[i] == 3) ni++;
+        if (c->t[i] == 12) no++;
+        if (c->p[i]) np++;
+    }
+    na = nf + ni;
+    in->nActive = na; in->nFluid = nf; in->nInterface = ni; in->nParticleNodes = np; in->nObjectNodes = no;
+    in->lbTime = c->lbTime; in->demTimeStep = c->demTimeStep; in->demTime = c->demTime; in->realTime = c->realTime;
+    in->neighborListBuilds = c->nlBuilds; in->newNeighborList = c->newNeighborList ? 1 : 0;
+    in->totalMassInit = c->totalMass; in->lastExtraMass = c->lastExtraMass; in->lastSurplus = c->lastSurplus;
+    in->precision = LBDEM_PREC_FP64;
+    return 0;
+}
+int lbdem_get_types(lbdem_ctx* c, uint8_t* type, uint8_t* pflag, int32_t* solid) {
+    NEED_INIT;
+    for (int64_t i = 0; i < c->N; ++i) {
+        if (type) type[i] = c->t[i];
+        if (pflag) pflag[i] = c->p[i];
+        if (solid) solid[i] = c->solid[i];
+    }
+    return 0;
+}
+int lbdem_get_fields(lbdem_ctx* c, double* f, double* mass, double* n) {
+    NEED_INIT;
+    const int64_t N = c->N;
+    for (int64_t i = 0; i < N; ++i) {
+        const bool e = c->exists[i];
+        if (f) for (int q = 0; q < Q; ++q) f[q * N + i] = e ? c->f[size_t(i) * Q + q] : 0.0;
+        if (mass) mass[i] = e ? c->mass[i] : 0.0;
+        if (n) n[i] = e ? c->n[i] : 0.0;
+    }
+    return 0;
+}
+static void put3(double* dst, const V3& v) { dst[0] = v.x; dst[1] = v.y; dst[2] = v.z; }
+int lbdem_get_particles(lbdem_ctx* c, double* rec) {
+    NEED_INIT;
+    for (size_t i = 0; i < c->particles.size(); ++i) {
+        const Particle& P = c->particles[i];
+        double* r = rec + i * LBDEM_P_REC;
+        put3(r + LBDEM_P_X0, P.x0); put3(r + LBDEM_P_X1, P.x1); put3(r + LBDEM_P_W1, P.w1);
+        put3(r + LBDEM_P_FHYDRO, P.FHydro); put3(r + LBDEM_P_MHYDRO, P.MHydro);
+        put3(r + LBDEM_P_FCONTACT, P.FContact); put3(r + LBDEM_P_MCONTACT, P.MContact);
+        put3(r + LBDEM_P_FLUB, P.FLub); put3(r + LBDEM_P_MLUB, P.MLub); put3(r + LBDEM_P_FREP, P.FRep);
+        put3(r + LBDEM_P_FTOTAL, P.FTotal); put3(r + LBDEM_P_MTOTAL, P.MTotal); put3(r + LBDEM_P_FGRAV, P.FGrav);
+        r[LBDEM_P_FLUIDVOL] = P.fluidVolume;
+        put3(r + LBDEM_P_XIW, P.xiw); put3(r + LBDEM_P_XIO, P.xio);
+        r[LBDEM_P_R] = P.r; r[LBDEM_P_M] = P.m;
+    }
+    return 0;
+}
+int lbdem_get_objects(lbdem_ctx* c, double* rec) {
+    NEED_INIT;
+    for (size_t i = 0; i < c->objects.size(); ++i) {
+        const Object& O = c->objects[i];
+        double* r = rec + i * LBDEM_O_REC;
+        put3(r + LBDEM_O_X0, O.x0); put3(r + LBDEM_O_X1, O.x1); put3(r + LBDEM_O_W1, O.w1);
+        put3(r + LBDEM_O_FHYDRO, O.FHydro); put3(r + LBDEM_O_MHYDRO, O.MHydro);
+        put3(r + LBDEM_O_FCONTACT, O.FContact); put3(r + LBDEM_O_MCONTACT, O.MContact);
+        put3(r + LBDEM_O_FLUB, O.FLub); put3(r + LBDEM_O_MLUB, O.MLub);
+        put3(r + LBDEM_O_FTOTAL, O.FTotal); put3(r + LBDEM_O_MTOTAL, O.MTotal); put3(r + LBDEM_O_FGRAV, O.FGrav);
+        r[LBDEM_O_R] = O.r; r[LBDEM_O_M] = O.m;
+    }
+    return 0;
+}
+int lbdem_get_walls(lbdem_ctx* c, double* rec) {
+    NEED_INIT;
+    for (size_t i = 0; i < c->walls.size(); ++i) {
+        const Wall& W = c->walls[i];
+        double* r = rec + i * LBDEM_W_REC;
+        put3(r + LBDEM_W_FHYDRO, W.FHydro); put3(r + LBDEM_W_FCONTACT, W.FContact);
+        put3(r + LBDEM_W_FLUB, W.FLub); put3(r + LBDEM_W_FTOTAL, W.FTotal);
+    }
+    return 0;
+}
+int lbdem_get_neighbors(lbdem_ctx* c, int cap, int* count, int* list) {
+    NEED_INIT;
+    for (size_t i = 0; i < c->particles.size(); ++i) {
+        const Particle& P = c->particles[i];
+        std::vector<int> v(P.list + 1, P.list + 1 + P.list[0]);
+        std::sort(v.begin(), v.end());
+        count[i] = int(v.size());
+        for (int k = 0; k < int(v.size()) && k < cap; ++k) list[i * cap + k] = v[k];
+    }
+    return 0;
+}
+int lbdem_total_mass(lbdem_ctx* c, double* m) {
+    NEED_INIT;
+    double s = 0.0;
+    for (int64_t i = 0; i < c->N; ++i)
+        if ((c->t[i] == 0 || c->t[i] == 3) && !c->p[i] && c->exists[i]) s += c->mass[i];
+    *m = s;
+    return 0;
+}
+
+}  // extern "C"

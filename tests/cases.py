@@ -1,0 +1,136 @@
+"""Small reference-style cases (getpot config + input/*.dat) shared by the tests, the golden
+fixture script and the smoke check, and a reader for the reference dump files written by
+oracle/ref_driver.cpp."""
+from __future__ import annotations
+
+import os
+import sys
+from typing import Dict, Tuple
+
+import numpy as np
+
+REPO = os.path.dirname(os.path.dirname(os.path.abspath(__file__)))
+sys.path.insert(0, os.path.join(REPO, "lbm-dem_amd", "python"))
+import lbdem  # noqa: E402
+
+BASE: Dict[str, object] = {
+    "demSolve": 1, "lbSolve": 1, "freeSurfaceSolve": 1, "forceFieldSolve": 1,
+    "demInitialRepeat": 0.0, "lbmInitialRepeat": 0, "maximumTimeSteps": 0, "maxTime": 100.0,
+    "screenExpTime": 0.0, "fluidExpTime": 0.0, "fluidLagrangianExpTime": 0.0, "partExpTime": 0.0,
+    "partRecycleExpTime": 0.0, "fluidRecycleExpTime": 0.0,
+    "unitLength": 1.0, "unitTime": 1.0, "unitDensity": 1.0,
+    "lbRestart": "false", "lbTopography": "false",
+    "initVelocityX": 0.0, "initVelocityY": 0.0, "initVelocityZ": 0.0,
+    "lbFX": 0.0, "lbFY": -2.0e-4, "lbFZ": 0.0,
+    "lbRotX": 0.0, "lbRotY": 0.0, "lbRotZ": 0.0, "lbRotCenterX": 0.0, "lbRotCenterY": 0.0, "lbRotCenterZ": 0.0,
+    "boundary0": 7, "boundary1": 7, "boundary2": 7, "boundary3": 7, "boundary4": 7, "boundary5": 7,
+    "slipCoefficient": 0.0, "rheologyModel": "NEWTONIAN", "initVisc": 0.1, "plasticVisc": 0.0,
+    "yieldStress": 0.0, "frictionCoefFluid": 0.0, "voellmyConstant": 0.0, "turbulenceSolve": 0, "turbConst": 0.0,
+    "density": 1.0, "objectDensity": 3.0, "bodyDensity": 12.0, "linearStiff": 0.3, "frictionCoefPart": 0.3,
+    "lubNormal": 0.5, "lubTangential": 0.3, "lubRotation": 0.2, "lubCutoff": 0.1, "kinVisc": 0.02,
+    "debyeLength": 0.5, "electroForce": 0.01, "objWaitTime": 3, "intervalTime": 0, "springDamp": 0.0,
+    "epsilon": 1.0, "nonLinType": 0, "k13": 1.0, "scale": 1.0,
+}
+
+
+def _case_values(name: str) -> Tuple[Dict[str, object], np.ndarray, np.ndarray, np.ndarray, np.ndarray, np.ndarray]:
+    """Returns (config values, particle radii, particle centres as written to particles.dat,
+    object radii, object centres, object velocities)."""
+    v = dict(BASE)
+    empty = np.zeros((0, 3))
+    if name == "fs_impactor":
+        # closed box, free surface, impactor entering the surface; no suspended spheres
+        v.update(lbSizeX=14.0, lbSizeY=20.0, lbSizeZ=14.0, fluidHeight=11.0, objWaitTime=0)
+        return v, np.zeros(0), empty, np.array([3.0]), np.array([[7.2, 14.6, 6.9]]), np.array([[0.0, -0.08, 0.0]])
+    if name == "suspension":
+        # closed box, free surface, impactor above a small column of spheres
+        v.update(lbSizeX=16.0, lbSizeY=22.0, lbSizeZ=16.0, fluidHeight=14.0)
+        pr = np.array([2.0, 2.0, 2.0, 1.6, 1.6, 2.0])
+        pc = np.array([[8.05, 11.25, 8.05], [8.05, 6.95, 8.05], [3.4, 2.5, 3.5], [12.6, 5.0, 12.2],
+                       [12.3, 12.0, 3.8], [4.0, 9.0, 12.0]])
+        return v, pr, pc, np.array([3.0]), np.array([[8.0, 16.55, 8.0]]), np.array([[0.0, -0.06, 0.0]])
+    if name == "periodic":
+        # x and z periodic, floor and lid walls, spheres away from the periodic seams
+        v.update(lbSizeX=16.0, lbSizeY=18.0, lbSizeZ=16.0, fluidHeight=12.0,
+                 boundary0=4, boundary1=4, boundary4=4, boundary5=4, lbFX=2.0e-5)
+        pr = np.array([2.0, 1.8, 2.0, 1.5, 1.5])
+        pc = np.array([[8.0, 6.0, 8.0], [5.0, 3.0, 10.0], [10.5, 8.5, 5.0], [1.6, 5.0, 4.0], [14.5, 5.2, 4.3]])
+        return v, pr, pc, np.array([2.5]), np.array([[8.0, 13.0, 8.0]]), np.array([[0.0, -0.05, 0.0]])
+    if name == "units":
+        # physical units (unitLength 0.5, unitTime 0.02): exercises every conversion
+        v.update(unitLength=0.5, unitTime=0.02, lbSizeX=8.0, lbSizeY=10.0, lbSizeZ=8.0, fluidHeight=6.5,
+                 lbFY=-0.5, initVisc=2.5, linearStiff=40.0, kinVisc=2.5, lubNormal=0.25, lubTangential=0.15,
+                 lubRotation=0.1, lubCutoff=0.05, debyeLength=0.25, electroForce=0.02, maxTime=1000.0)
+        pr = np.array([1.0, 0.9, 1.0])
+        pc = np.array([[4.0, 3.0, 4.0], [2.0, 1.6, 5.5], [5.8, 4.4, 2.5]])
+        return v, pr, pc, np.array([1.4]), np.array([[4.1, 7.55, 3.9]]), np.array([[0.0, -1.5, 0.0]])
+    if name == "lb_closed":
+        # no free surface, no spheres, no impactor: gravity-driven closed box
+        v.update(lbSizeX=10.0, lbSizeY=12.0, lbSizeZ=9.0, freeSurfaceSolve=0, fluidHeight=0.0, lbFX=1.0e-4)
+        return v, np.zeros(0), empty, np.zeros(0), empty, empty
+    raise KeyError(name)
+
+
+CASES = ("fs_impactor", "suspension", "periodic", "units", "lb_closed")
+
+
+def make_case(root: str, name: str) -> str:
+    """Writes config.cfg and input/{particles,objects,body}.dat for a named case; returns the dir."""
+    v, pr, pc, orr, ox0, ox1 = _case_values(name)
+    d = os.path.join(root, name)
+    os.makedirs(os.path.join(d, "input"), exist_ok=True)
+    lbdem.write_getpot(os.path.join(d, "config.cfg"), v)
+    lbdem.write_particles(os.path.join(d, "input", "particles.dat"), pr, pc)
+    lbdem.write_objects(os.path.join(d, "input", "objects.dat"), orr, ox0, ox1)
+    with open(os.path.join(d, "input", "body.dat"), "w") as fh:
+        fh.write("0\n")
+    return d
+
+
+def read_dump(path: str) -> Dict[str, object]:
+    """Parses a binary dump written by oracle/ref_driver.cpp."""
+    with open(path, "rb") as fh:
+        buf = fh.read()
+    assert buf[:8] == b"LBDMDUMP", path
+    off = 8
+    hdr = np.frombuffer(buf, np.int32, 8, off); off += 32
+    X, Y, Z, npart, nobj, nwall, lbt, dts = [int(x) for x in hdr]
+    N = X * Y * Z
+
+    def take(dtype, count):
+        nonlocal off
+        a = np.frombuffer(buf, dtype, count, off)
+        off += a.nbytes
+        return a.copy()
+
+    out: Dict[str, object] = dict(size=(X, Y, Z), N=N, lbTime=lbt, demTimeStep=dts)
+    out["type"] = take(np.uint8, N)
+    out["pflag"] = take(np.uint8, N)
+    out["solid"] = take(np.int32, N)
+    out["exists"] = take(np.uint8, N)
+    out["f"] = take(np.float64, 19 * N).reshape(19, N)
+    out["n"] = take(np.float64, N)
+    out["mass"] = take(np.float64, N)
+    out["particles"] = take(np.float64, npart * lbdem.P["REC"]).reshape(npart, lbdem.P["REC"])
+    out["objects"] = take(np.float64, nobj * lbdem.O["REC"]).reshape(nobj, lbdem.O["REC"])
+    out["walls"] = take(np.float64, nwall * lbdem.W["REC"]).reshape(nwall, lbdem.W["REC"])
+    out["ncount"] = take(np.int32, npart)
+    out["nlist"] = take(np.int32, npart * 64).reshape(npart, 64)
+    sc = take(np.float64, 6)
+    out["totalMassInit"], out["totalMass"], out["demTime"], out["realTime"] = sc[0], sc[1], sc[2], sc[3]
+    return out
+
+
+def open_sim(lib: "lbdem.Library", case_dir: str, precision: int = lbdem.PREC_FP64) -> "lbdem.Simulation":
+    vals, prm, ps, obs = lbdem.load_case(case_dir, precision=precision)
+    sim = lbdem.Simulation(lib, prm, ps, obs)
+    sim.init()
+    return sim
+
+
+def state(sim: "lbdem.Simulation", with_f: bool = True) -> Dict[str, object]:
+    t, p, s = sim.types()
+    f, m, n = sim.fields(with_f)
+    cnt, lst = sim.neighbors()
+    return dict(type=t, pflag=p, solid=s, f=f, mass=m, n=n, particles=sim.particles(), objects=sim.objects(),
+                walls=sim.walls(), ncount=cnt, nlist=lst, totalMass=sim.total_mass(), info=sim.info())
