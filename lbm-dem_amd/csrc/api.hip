@@ -1,0 +1,711 @@
+// C ABI of the MI355X library (include/lbdem.h): configuration, initial node types (the
+// reference's initialisation order, LB.cpp:340-398 and DEM.cpp:199-357), device allocation,
+// the goCycle step sequence (main.cpp:38-67) and downloads.
+#include "lbdem_internal.hpp"
+
+#include <algorithm>
+#include <array>
+#include <cmath>
+#include <cstdio>
+#include <cstring>
+#include <string>
+#include <vector>
+
+using namespace lbdem;
+
+namespace {
+
+struct Units {  // conversion units (node.cpp:512-525)
+    double Length = 1, Time = 1, Density = 1, Volume = 1, Speed = 1, Accel = 1, AngVel = 1, KinVisc = 1, DynVisc = 1,
+           Force = 1, Torque = 1, Mass = 1;
+    void setComposite() {
+        Volume = Length * Length * Length;
+        Speed = Length / Time;
+        Accel = Length / Time / Time;
+        AngVel = 1.0 / Time;
+        KinVisc = Length * Length / Time;
+        DynVisc = Density * Length * Length / Time;
+        Force = Density * Length * Length * Length * Length / Time / Time;
+        Torque = Density * Length * Length * Length * Length * Length / Time / Time;
+        Mass = Density * Length * Length * Length;
+    }
+};
+
+struct HostVec { double x = 0, y = 0, z = 0; };
+
+}  // namespace
+
+struct lbdem_ctx {
+    lbdem_params prm{};
+    std::string err;
+    bool initialised = false;
+    Units unit;
+    int lbSize[3] = {1, 1, 1};
+    double physSize[3] = {1, 1, 1};
+    double bLoc[6][3];
+    double lbF[3] = {0, 0, 0}, initVel[3] = {0, 0, 0};
+    double initDynVisc = 0, rho0 = 1.0, fluidHeight = 0;
+    bool freeSurface = false, forceField = false, lbFReset = false;
+    // inputs
+    std::vector<int> psize;
+    std::vector<double> pr, px0, orad, ox0, ox1;
+    // device
+    hipStream_t stream = nullptr;
+    Box box{};
+    Consts k{};
+    DemParams dp{};
+    DemState ds{};
+    DevScalars* sc = nullptr;
+    double* dTmp = nullptr;
+    size_t dTmpBytes = 0;
+    std::vector<void*> allocs;
+    // host counters (goCycle, main.cpp:40-43; DEM.cpp:365-366)
+    unsigned lbTime = 0, demTimeStep = 0, currentTimeStep = 0;
+    double demTime = 0, realTime = 0, deltat = 1.0, totalMass = 0;
+
+    int fail(int code, const std::string& m) { err = m; return code; }
+    template <typename T> T* dalloc(size_t n) {
+        void* p = nullptr;
+        if (n == 0) n = 1;
+        if (hipMalloc(&p, n * sizeof(T)) != hipSuccess) return nullptr;
+        allocs.push_back(p);
+        return (T*)p;
+    }
+    ~lbdem_ctx() {
+        for (void* p : allocs) hipFree(p);
+        if (dTmp) hipFree(dTmp);
+        if (stream) hipStreamDestroy(stream);
+    }
+    double* tmp(size_t bytes) {
+        if (bytes > dTmpBytes) {
+            if (dTmp) hipFree(dTmp);
+            dTmp = nullptr;
+            if (hipMalloc(&dTmp, bytes) != hipSuccess) { dTmpBytes = 0; return nullptr; }
+            dTmpBytes = bytes;
+        }
+        return dTmp;
+    }
+};
+
+#define HIPCHK(x)                                                                                    \
+    do {                                                                                             \
+        hipError_t e_ = (x);                                                                         \
+        if (e_ != hipSuccess) return c->fail(LBDEM_E_DEVICE, std::string("HIP error: ") + hipGetErrorString(e_) + " at " #x); \
+    } while (0)
+
+static int read_params(lbdem_ctx* c) {
+    const lbdem_params& P = c->prm;
+    Units& u = c->unit;
+    u.Length = P.unitLength; u.Time = P.unitTime; u.Density = P.unitDensity;
+    if (!(u.Length > 0) || !(u.Time > 0) || !(u.Density > 0)) return c->fail(LBDEM_E_ARG, "unit lengths/times must be > 0");
+    for (int a = 0; a < 3; ++a) {
+        if (!(P.lbSize[a] > 0)) return c->fail(LBDEM_E_ARG, "lbSize must be > 0");
+        c->lbSize[a] = int(std::floor(P.lbSize[a] / u.Length + 0.5));  // LB.cpp:170-172
+        if (c->lbSize[a] < 3) return c->fail(LBDEM_E_ARG, "lattice must have at least 3 nodes per axis");
+        c->physSize[a] = double(unsigned(c->lbSize[a])) * u.Length;
+    }
+    const double L = u.Length;
+    const double loc[6][3] = {{0.5 * L, 0, 0}, {double(unsigned(c->lbSize[0]) - 1.5) * L, 0, 0},
+                              {0, 0.5 * L, 0}, {0, double(unsigned(c->lbSize[1]) - 1.5) * L, 0},
+                              {0, 0, 0.5 * L}, {0, 0, double(unsigned(c->lbSize[2]) - 1.5) * L}};
+    std::memcpy(c->bLoc, loc, sizeof(loc));
+    if (!(P.initVisc > 0)) return c->fail(LBDEM_E_ARG, "initVisc must be > 0");
+    u.setComposite();
+    for (int a = 0; a < 3; ++a) {
+        c->initVel[a] = P.initVelocity[a] / u.Speed;
+        c->lbF[a] = P.lbF[a] / u.Accel;
+    }
+    for (int i = 0; i < 6; ++i)
+        if (P.boundary[i] != 4 && P.boundary[i] != 7)
+            return c->fail(LBDEM_E_ARG, "only boundary types 4 (periodic) and 7 (static no-slip wall) are supported");
+    c->fluidHeight = P.fluidHeight;
+    c->initDynVisc = P.initVisc / u.DynVisc;
+    c->freeSurface = P.freeSurface != 0;
+    c->forceField = P.forceField != 0;
+    if (!P.lbSolve) return c->fail(LBDEM_E_ARG, "lbSolve=0 is not supported");
+    if (!(P.density > 0) || !(P.objectDensity > 0)) return c->fail(LBDEM_E_ARG, "density/objectDensity must be > 0");
+    if (P.precision < 0 || P.precision > 2) return c->fail(LBDEM_E_ARG, "unknown precision");
+    return 0;
+}
+
+extern "C" int lbdem_create(const lbdem_params* params, lbdem_ctx** out) {
+    if (!params || !out) return LBDEM_E_ARG;
+    lbdem_ctx* c = new lbdem_ctx();
+    *out = c;
+    c->prm = *params;
+    return read_params(c);
+}
+
+extern "C" int lbdem_destroy(lbdem_ctx* ctx) {
+    if (ctx) {
+        if (ctx->stream) hipStreamSynchronize(ctx->stream);
+        delete ctx;
+    }
+    return 0;
+}
+
+extern "C" const char* lbdem_last_error(const lbdem_ctx* ctx) { return ctx ? ctx->err.c_str() : "null context"; }
+
+extern "C" int lbdem_set_particles(lbdem_ctx* c, int n, const int* size, const double* r, const double* x0) {
+    if (!c || n < 0 || (n > 0 && (!r || !x0))) return LBDEM_E_ARG;
+    if (c->initialised) return c->fail(LBDEM_E_STATE, "particles must be set before lbdem_init");
+    c->psize.assign(n, 1);
+    if (size) for (int i = 0; i < n; ++i) c->psize[i] = size[i];
+    c->pr.assign(r, r + n);
+    c->px0.assign(x0, x0 + 3 * size_t(n));
+    return 0;
+}
+
+extern "C" int lbdem_set_objects(lbdem_ctx* c, int n, const double* r, const double* x0, const double* x1) {
+    if (!c || n < 0 || (n > 0 && (!r || !x0 || !x1))) return LBDEM_E_ARG;
+    if (n > MAX_OBJECTS) return c->fail(LBDEM_E_ARG, "too many objects");
+    if (n > 2) return c->fail(LBDEM_E_ARG, "at most 2 objects are supported by the lattice kernels");
+    if (c->initialised) return c->fail(LBDEM_E_STATE, "objects must be set before lbdem_init");
+    c->orad.assign(r, r + n);
+    c->ox0.assign(x0, x0 + 3 * size_t(n));
+    c->ox1.assign(x1, x1 + 3 * size_t(n));
+    return 0;
+}
+
+// initial node types on the host: lattice boundaries (LB.cpp:538-564), DEM walls
+// (LB.cpp:713-748), objects (LB.cpp:751-795), gas above the fluid height (LB.cpp:974-992) and
+// interface nodes next to gas (LB.cpp:1058-1068)
+static void host_types(lbdem_ctx* c, const Geom& g, std::vector<uint8_t>& t, std::vector<int32_t>& s,
+                       const std::vector<std::array<double, 6>>& walls) {
+    const int X = g.X, Y = g.Y, Z = g.Z;
+    t.assign(g.N, T_FLUID);
+    s.assign(g.N, 0);
+    const int* B = c->prm.boundary;
+    for (int z = 0; z < Z; ++z)
+        for (int y = 0; y < Y; ++y)
+            for (int x = 0; x < X; ++x) {
+                const int64_t i = cidx(g, x, y, z);
+                int v = -1;
+                if (x == 0) v = B[0]; else if (x == X - 1) v = B[1];
+                else if (y == 0) v = B[2]; else if (y == Y - 1) v = B[3];
+                else if (z == 0) v = B[4]; else if (z == Z - 1) v = B[5];
+                if (v >= 0) t[i] = uint8_t(v);
+            }
+    for (size_t w = 0; w < walls.size(); ++w) {
+        const double px = walls[w][0] / c->unit.Length, py = walls[w][1] / c->unit.Length, pz = walls[w][2] / c->unit.Length;
+        const double nx = walls[w][3], ny = walls[w][4], nz = walls[w][5];
+        for (int z = 0; z < Z; ++z)
+            for (int y = 0; y < Y; ++y)
+                for (int x = 0; x < X; ++x) {
+                    const double d = nx * (x - px) + ny * (y - py) + nz * (z - pz);
+                    if (d < 0.0) { const int64_t i = cidx(g, x, y, z); s[i] = int32_t(w); t[i] = T_WALL; }
+                }
+    }
+    for (size_t o = 0; o < c->orad.size(); ++o) {
+        const double cx = c->ox0[3 * o] / c->unit.Length, cy = c->ox0[3 * o + 1] / c->unit.Length, cz = c->ox0[3 * o + 2] / c->unit.Length;
+        const double r = c->orad[o] / c->unit.Length;
+        for (int z = 0; z < Z; ++z)
+            for (int y = 0; y < Y; ++y)
+                for (int x = 0; x < X; ++x) {
+                    const double dx = x - cx, dy = y - cy, dz = z - cz;
+                    if (dx * dx + dy * dy + dz * dz < r * r) { const int64_t i = cidx(g, x, y, z); s[i] = int32_t(o); t[i] = T_OBJ; }
+                }
+    }
+    if (c->freeSurface) {
+        const double h0 = c->fluidHeight / c->unit.Length;
+        for (int z = 0; z < Z; ++z)
+            for (int y = 0; y < Y; ++y)
+                for (int x = 0; x < X; ++x) {
+                    const int64_t i = cidx(g, x, y, z);
+                    if (unsigned(y) > h0 && t[i] == T_FLUID) t[i] = T_GAS;
+                }
+    }
+    for (int z = 0; z < Z; ++z)
+        for (int y = 0; y < Y; ++y)
+            for (int x = 0; x < X; ++x) {
+                const int64_t i = cidx(g, x, y, z);
+                if (t[i] != T_FLUID) continue;
+                for (int q = 1; q < Q; ++q)
+                    if (t[nbr_wrapped(g, x, y, z, q)] == T_GAS) { t[i] = T_IF; break; }
+            }
+}
+
+extern "C" int lbdem_init(lbdem_ctx* c) {
+    if (!c) return LBDEM_E_ARG;
+    if (c->initialised) return c->fail(LBDEM_E_STATE, "already initialised");
+    HIPCHK(hipSetDevice(c->prm.device));
+    HIPCHK(hipStreamCreateWithFlags(&c->stream, hipStreamNonBlocking));
+    const Units& u = c->unit;
+    const int np = int(c->pr.size()), no = int(c->orad.size());
+    if (np > (1 << 30)) return c->fail(LBDEM_E_ARG, "too many particles");
+
+    // ---------------- DEM set-up (DEM.cpp:199-357, elmt.cpp:17-53) ----------------
+    DemParams& P = c->dp;
+    std::memset(&P, 0, sizeof(P));
+    P.demSize[0] = c->physSize[0];
+    P.demSize[1] = c->freeSurface ? c->fluidHeight : c->physSize[1];
+    P.demSize[2] = c->physSize[2];
+    double demF[3];
+    for (int a = 0; a < 3; ++a) demF[a] = c->lbF[a] * u.Accel;
+    std::vector<double> m(np), I(np), im(np), iI(np), fg(3 * size_t(np));
+    double meanR = 0.0, rmax = 0.0;
+    for (int i = 0; i < np; ++i) {
+        const double r = c->pr[i];
+        const double single = 4.0 / 3.0 * c->prm.density * M_PI * r * r * r;
+        m[i] = c->psize[i] * single;
+        I[i] = c->psize[i] * 2.0 / 5.0 * single * r * r;
+        im[i] = 1.0 / m[i];
+        iI[i] = 1.0 / I[i];
+        for (int a = 0; a < 3; ++a) fg[3 * i + a] = demF[a] * m[i];
+        meanR += r;
+        rmax = r > rmax ? r : rmax;
+    }
+    if (np > 0) meanR /= double(np);
+    std::vector<double> om(no), oI(no), oim(no), oiI(no), ofg(3 * size_t(no));
+    for (int o = 0; o < no; ++o) {
+        const double r = c->orad[o];
+        om[o] = 4.0 / 3.0 * c->prm.objectDensity * M_PI * r * r * r;
+        for (int a = 0; a < 3; ++a) ofg[3 * o + a] = demF[a] * om[o];
+        oI[o] = 2.0 / 5.0 * om[o] * r * r;
+        oim[o] = 1.0 / om[o];
+        oiI[o] = 1.0 / oI[o];
+    }
+    std::vector<std::array<double, 6>> walls;
+    const double dirs[6][3] = {{1, 0, 0}, {-1, 0, 0}, {0, 1, 0}, {0, -1, 0}, {0, 0, 1}, {0, 0, -1}};
+    for (int i = 0; i < 6; ++i)
+        if (c->prm.boundary[i] >= 5 && c->prm.boundary[i] <= 8)
+            walls.push_back({c->bLoc[i][0], c->bLoc[i][1], c->bLoc[i][2], dirs[i][0], dirs[i][1], dirs[i][2]});
+    c->deltat = u.Time / double(1);
+    const double avgMass = 4.0 / 3.0 * c->prm.density * M_PI * meanR * meanR * meanR;
+    P.deltat = c->deltat;
+    P.linearStiff = c->prm.linearStiff;
+    P.dampCoeff = std::sqrt(avgMass * c->prm.linearStiff);
+    P.frictionCoefPart = c->prm.frictionCoefPart;
+    P.kinVisc = c->prm.kinVisc;
+    P.lubNormal = c->prm.lubNormal; P.lubTangential = c->prm.lubTangential;
+    P.lubRotation = c->prm.lubRotation; P.lubCutoff = c->prm.lubCutoff;
+    P.electroForce = c->prm.electroForce; P.debyeLength = c->prm.debyeLength;
+    P.rmax = rmax;
+    P.objWaitTime = c->prm.objWaitTime;
+    for (int a = 0; a < 3; ++a) P.periodic[a] = c->prm.boundary[2 * a] == 4 && c->prm.boundary[2 * a + 1] == 4;
+    const double cut = 2.0 * rmax + 2.0;
+    P.nlx = int(P.demSize[0] / cut); P.nly = int(P.demSize[1] / cut); P.nlz = int(P.demSize[2] / cut);
+    if (std::min(P.nlx, P.nly) < 3 || P.nlz < 3) P.nlx = P.nly = P.nlz = 0;  // direct sum (DEM.cpp:515-554)
+    P.nPart = np; P.nObj = no; P.nWall = int(walls.size());
+    for (size_t w = 0; w < walls.size(); ++w)
+        for (int a = 0; a < 3; ++a) { P.wallP[w][a] = walls[w][a]; P.wallN[w][a] = walls[w][3 + a]; }
+    P.unitLength = u.Length; P.unitSpeed = u.Speed; P.unitAngVel = u.AngVel;
+
+    // ---------------- lattice ----------------
+    Geom g;
+    g.X = c->lbSize[0]; g.Y = c->lbSize[1]; g.Z = c->lbSize[2];
+    g.N = int64_t(g.X) * g.Y * g.Z;
+    for (int i = 0; i < 6; ++i) g.per[i] = c->prm.boundary[i] == 4;
+    std::vector<uint8_t> t;
+    std::vector<int32_t> s;
+    host_types(c, g, t, s, walls);
+
+    Box& b = c->box;
+    b.g = g;
+    b.prec = c->prm.precision;
+    const size_t selt = b.prec == LBDEM_PREC_FP64 ? 8 : (b.prec == LBDEM_PREC_FP32 ? 4 : 2);
+    const size_t celt = b.prec == LBDEM_PREC_FP64 ? 8 : 4;
+    b.f = c->dalloc<uint8_t>(size_t(Q) * g.N * selt);
+    b.type = c->dalloc<uint8_t>(g.N);
+    b.solid = c->dalloc<int32_t>(g.N);
+    b.mass = c->dalloc<uint8_t>(g.N * celt);
+    b.aux = c->dalloc<uint8_t>(7 * g.N * celt);
+    b.ifCap = int(std::max<int64_t>(65536, g.N / 4));
+    b.ifList = c->dalloc<int32_t>(b.ifCap);
+    b.ifList2 = c->dalloc<int32_t>(b.ifCap + 16);
+    b.ifNewMass = c->dalloc<uint8_t>(size_t(b.ifCap) * celt);
+    b.filled = c->dalloc<int32_t>(b.ifCap);
+    b.emptied = c->dalloc<int32_t>(b.ifCap);
+    b.newIf = c->dalloc<int32_t>(b.ifCap);
+    b.donor = c->dalloc<int32_t>(g.N);
+    b.flagNew = c->dalloc<int32_t>(g.N);
+    int64_t objCells = 0;
+    for (int o = 0; o < no; ++o) {
+        const double r = c->orad[o] / u.Length + 4.0;
+        objCells += int64_t(std::ceil(8.0 * r * r * r));
+    }
+    b.objCap = int(std::min<int64_t>(std::max<int64_t>(objCells, 1024), g.N));
+    b.objList = c->dalloc<int32_t>(b.objCap);
+    b.blockPartCap = fused_grid_blocks(g) * PART_STRIDE_;
+    b.blockPart = c->dalloc<double>(b.blockPartCap);
+    b.hacc = c->dalloc<double>(size_t(std::max(np, 1)) * 8);
+    b.parity = 0;
+    c->sc = c->dalloc<DevScalars>(1);
+    if (!b.f || !b.type || !b.solid || !b.mass || !b.aux || !b.ifList || !b.ifList2 || !b.ifNewMass || !b.filled ||
+        !b.emptied || !b.newIf || !b.donor || !b.flagNew || !b.objList || !b.blockPart || !b.hacc || !c->sc)
+        return c->fail(LBDEM_E_NOMEM, "device allocation failed");
+    hipStream_t st = c->stream;
+    HIPCHK(hipMemsetAsync(b.f, 0, size_t(Q) * g.N * selt, st));
+    HIPCHK(hipMemsetAsync(b.mass, 0, g.N * celt, st));
+    HIPCHK(hipMemsetAsync(b.aux, 0, 7 * g.N * celt, st));
+    HIPCHK(hipMemsetAsync(b.donor, 0xff, g.N * 4, st));
+    HIPCHK(hipMemsetAsync(b.flagNew, 0xff, g.N * 4, st));
+    HIPCHK(hipMemcpyAsync(b.type, t.data(), g.N, hipMemcpyHostToDevice, st));
+    HIPCHK(hipMemcpyAsync(b.solid, s.data(), g.N * 4, hipMemcpyHostToDevice, st));
+    std::vector<int32_t> objList;
+    for (int64_t i = 0; i < g.N; ++i) if (t[i] == T_OBJ) objList.push_back(int32_t(i));
+    if (int64_t(objList.size()) > b.objCap) return c->fail(LBDEM_E_CAPACITY, "object node list capacity");
+    if (!objList.empty()) HIPCHK(hipMemcpyAsync(b.objList, objList.data(), objList.size() * 4, hipMemcpyHostToDevice, st));
+    DevScalars hs;
+    std::memset(&hs, 0, sizeof(hs));
+    hs.nObjList = int(objList.size());
+    hs.kminP = -1;
+    hs.newNeighborList = 1;
+
+    // DEM device state
+    DemState& d = c->ds;
+    const size_t n3 = 3 * size_t(std::max(np, 1));
+    auto z3 = [&](double*& p) { p = c->dalloc<double>(n3); return p != nullptr; };
+    double** vecs[] = {&d.x0, &d.x1, &d.w1, &d.FHydro, &d.MHydro, &d.FContact, &d.MContact, &d.FLub, &d.MLub, &d.FRep,
+                       &d.FTotal, &d.MTotal, &d.FGrav, &d.xiw, &d.xio};
+    for (double** v : vecs) if (!z3(*v)) return c->fail(LBDEM_E_NOMEM, "device allocation failed");
+    for (double** v : vecs) HIPCHK(hipMemsetAsync(*v, 0, n3 * 8, st));
+    d.r = c->dalloc<double>(std::max(np, 1)); d.m = c->dalloc<double>(std::max(np, 1));
+    d.I = c->dalloc<double>(std::max(np, 1)); d.im = c->dalloc<double>(std::max(np, 1));
+    d.iI = c->dalloc<double>(std::max(np, 1)); d.fluidVol = c->dalloc<double>(std::max(np, 1));
+    HIPCHK(hipMemsetAsync(d.fluidVol, 0, std::max(np, 1) * 8, st));
+    const size_t slots = size_t(std::max(np, 1)) * NBR_CAP;
+    d.nbrCount = c->dalloc<int>(std::max(np, 1)); d.nbrCountOld = c->dalloc<int>(std::max(np, 1));
+    d.nbrList = c->dalloc<int>(slots); d.nbrListOld = c->dalloc<int>(slots); d.nbrSlot = c->dalloc<int>(slots);
+    d.xiCur = c->dalloc<double>(3 * slots); d.xiNext = c->dalloc<double>(3 * slots); d.xiTmp = c->dalloc<double>(3 * slots);
+    const int ncell = std::max(1, P.nlx * P.nly * P.nlz);
+    d.cellCount = c->dalloc<int>(ncell); d.cellStart = c->dalloc<int>(ncell); d.cellFill = c->dalloc<int>(ncell);
+    d.cellItems = c->dalloc<int>(std::max(np, 1)); d.cellKey = c->dalloc<int>(std::max(np, 1));
+    d.objContrib = c->dalloc<double>(size_t(std::max(np, 1)) * 12 * std::max(no, 1));
+    d.wallContrib = c->dalloc<double>(size_t(std::max(np, 1)) * 6 * std::max(int(walls.size()), 1));
+    d.objSum = c->dalloc<double>(12 * MAX_OBJECTS);
+    d.wallSum = c->dalloc<double>(6 * MAX_WALLS);
+    const size_t o3 = 3 * size_t(std::max(no, 1));
+    double** ovecs[] = {&d.ox0, &d.ox1, &d.ow1, &d.oFHydro, &d.oMHydro, &d.oFContact, &d.oMContact, &d.oFLub,
+                        &d.oMLub, &d.oFTotal, &d.oMTotal, &d.oFGrav};
+    for (double** v : ovecs) { *v = c->dalloc<double>(o3); if (!*v) return c->fail(LBDEM_E_NOMEM, "device allocation failed"); HIPCHK(hipMemsetAsync(*v, 0, o3 * 8, st)); }
+    d.or_ = c->dalloc<double>(std::max(no, 1)); d.om = c->dalloc<double>(std::max(no, 1));
+    d.oI = c->dalloc<double>(std::max(no, 1)); d.oim = c->dalloc<double>(std::max(no, 1)); d.oiI = c->dalloc<double>(std::max(no, 1));
+    const size_t w3 = 3 * size_t(MAX_WALLS);
+    double** wvecs[] = {&d.wFHydro, &d.wFContact, &d.wFLub, &d.wFTotal};
+    for (double** v : wvecs) { *v = c->dalloc<double>(w3); if (!*v) return c->fail(LBDEM_E_NOMEM, "device allocation failed"); HIPCHK(hipMemsetAsync(*v, 0, w3 * 8, st)); }
+    d.plb = c->dalloc<PartLB>(std::max(np, 1));
+    d.olb = c->dalloc<ObjLB>(std::max(no, 1));
+    if (!d.nbrCount || !d.nbrList || !d.xiCur || !d.xiTmp || !d.plb || !d.olb || !d.objContrib || !d.wallContrib)
+        return c->fail(LBDEM_E_NOMEM, "device allocation failed");
+    HIPCHK(hipMemsetAsync(d.nbrCount, 0, std::max(np, 1) * 4, st));
+    HIPCHK(hipMemsetAsync(d.nbrCountOld, 0, std::max(np, 1) * 4, st));
+    HIPCHK(hipMemsetAsync(d.xiCur, 0, 3 * slots * 8, st));
+    HIPCHK(hipMemsetAsync(d.xiNext, 0, 3 * slots * 8, st));
+    HIPCHK(hipMemsetAsync(d.xiTmp, 0, 3 * slots * 8, st));
+    if (np > 0) {
+        HIPCHK(hipMemcpyAsync(d.x0, c->px0.data(), 3 * np * 8, hipMemcpyHostToDevice, st));
+        HIPCHK(hipMemcpyAsync(d.r, c->pr.data(), np * 8, hipMemcpyHostToDevice, st));
+        HIPCHK(hipMemcpyAsync(d.m, m.data(), np * 8, hipMemcpyHostToDevice, st));
+        HIPCHK(hipMemcpyAsync(d.I, I.data(), np * 8, hipMemcpyHostToDevice, st));
+        HIPCHK(hipMemcpyAsync(d.im, im.data(), np * 8, hipMemcpyHostToDevice, st));
+        HIPCHK(hipMemcpyAsync(d.iI, iI.data(), np * 8, hipMemcpyHostToDevice, st));
+        HIPCHK(hipMemcpyAsync(d.FGrav, fg.data(), 3 * np * 8, hipMemcpyHostToDevice, st));
+    }
+    if (no > 0) {
+        HIPCHK(hipMemcpyAsync(d.ox0, c->ox0.data(), 3 * no * 8, hipMemcpyHostToDevice, st));
+        HIPCHK(hipMemcpyAsync(d.ox1, c->ox1.data(), 3 * no * 8, hipMemcpyHostToDevice, st));
+        HIPCHK(hipMemcpyAsync(d.or_, c->orad.data(), no * 8, hipMemcpyHostToDevice, st));
+        HIPCHK(hipMemcpyAsync(d.om, om.data(), no * 8, hipMemcpyHostToDevice, st));
+        HIPCHK(hipMemcpyAsync(d.oI, oI.data(), no * 8, hipMemcpyHostToDevice, st));
+        HIPCHK(hipMemcpyAsync(d.oim, oim.data(), no * 8, hipMemcpyHostToDevice, st));
+        HIPCHK(hipMemcpyAsync(d.oiI, oiI.data(), no * 8, hipMemcpyHostToDevice, st));
+        HIPCHK(hipMemcpyAsync(d.oFGrav, ofg.data(), 3 * no * 8, hipMemcpyHostToDevice, st));
+    }
+    HIPCHK(hipMemcpyAsync(c->sc, &hs, sizeof(hs), hipMemcpyHostToDevice, st));
+
+    // constants of the lattice kernels
+    Consts& k = c->k;
+    for (int a = 0; a < 3; ++a) { k.lbF[a] = c->lbF[a]; k.initVel[a] = c->initVel[a]; }
+    k.visc = c->initDynVisc;
+    k.rho0 = c->rho0;
+    k.omega = 1.0 / (0.5 + 3.0 * k.visc);
+    k.omegaf = 1.0 - 1.0 / (1.0 + 6.0 * k.visc);
+    k.freeSurface = c->freeSurface ? 1 : 0;
+    k.nPart = np; k.nObj = no; k.nWall = int(walls.size());
+    k.unitLength = u.Length; k.unitSpeed = u.Speed; k.unitAngVel = u.AngVel;
+    k.unitForce = u.Force; k.unitTorque = u.Torque; k.unitVolume = u.Volume;
+
+    dem_refresh_views(P, d, st);
+    dem_neighbor_list(P, d, c->sc, true, st);   // DEM.cpp:294
+    lb_init_fields(b, k, st);
+    HIPCHK(hipGetLastError());
+    HIPCHK(hipStreamSynchronize(st));
+    // total mass (LB.cpp:388-395): no node is flagged yet, all active nodes count
+    double total = 0.0;
+    {
+        const int nb = 1024;
+        double* part = c->tmp(nb * sizeof(double));
+        if (!part) return c->fail(LBDEM_E_NOMEM, "device allocation failed");
+        lb_total_mass(b, part, st);
+        std::vector<double> h(nb);
+        HIPCHK(hipMemcpyAsync(h.data(), part, nb * 8, hipMemcpyDeviceToHost, st));
+        HIPCHK(hipStreamSynchronize(st));
+        for (double v : h) total += v;
+    }
+    c->totalMass = total;
+    c->lbTime = 0; c->demTime = 0.0; c->realTime = 0.0; c->demTimeStep = 0;
+    c->initialised = true;
+    return 0;
+}
+
+static int check_errors(lbdem_ctx* c) {
+    DevScalars hs;
+    HIPCHK(hipMemcpyAsync(&hs, c->sc, sizeof(hs), hipMemcpyDeviceToHost, c->stream));
+    HIPCHK(hipStreamSynchronize(c->stream));
+    if (hs.typeError) return c->fail(LBDEM_E_TYPE, "TYPE ERROR: streaming met an unsupported neighbour type");
+    if (hs.listOverflow) return c->fail(LBDEM_E_CAPACITY, "an interface/object list overflowed its capacity");
+    if (hs.nbrOverflow) return c->fail(LBDEM_E_CAPACITY, "neighbour list capacity (64) exceeded");
+    return 0;
+}
+
+// --------------------------- steps ---------------------------
+static void enqueue_dem(lbdem_ctx* c) {  // DEM::discreteElementStep (DEM.cpp:359-401)
+    c->demTimeStep++;
+    c->demTime += c->deltat;
+    dem_neighbor_list(c->dp, c->ds, c->sc, false, c->stream);
+    dem_step(c->dp, c->ds, c->sc, int(c->demTimeStep), c->stream);
+}
+static void enqueue_coupling(lbdem_ctx* c) {  // LB::latticeBoltzmannCouplingStep (LB.cpp:432-467)
+    lb_flags_rebuild_compute(c->box, c->k, c->ds.plb, c->sc, c->stream);
+    lb_object_update(c->box, c->k, c->ds.olb, c->sc, c->stream);
+    lb_flags_rebuild_apply(c->box, c->sc, c->stream);
+    lb_find_new_active(c->box, c->k, c->ds.plb, c->sc, c->stream);
+}
+static void enqueue_lb(lbdem_ctx* c) {  // LB::latticeBolzmannStep (LB.cpp:400-418)
+    if (!c->forceField && !c->lbFReset) {  // LB.cpp:1303-1305
+        c->k.lbF[0] = c->k.lbF[1] = c->k.lbF[2] = 0.0;
+        c->lbFReset = true;
+    }
+    lb_fused_step(c->box, c->k, c->ds.plb, c->ds.olb, c->sc, true, c->stream);
+    lb_hydro_finalize(c->box, c->k, c->ds.FHydro, c->ds.MHydro, c->ds.fluidVol, c->sc, c->ds.oFHydro, c->ds.oMHydro,
+                      c->ds.wFHydro, true, c->stream);
+}
+static void enqueue_fs(lbdem_ctx* c) { lb_free_surface(c->box, c->k, c->sc, c->stream); }
+
+#define NEED_INIT                                                           \
+    if (!c) return LBDEM_E_ARG;                                             \
+    if (!c->initialised) return c->fail(LBDEM_E_STATE, "call lbdem_init first")
+
+extern "C" int lbdem_dem_step(lbdem_ctx* c) { NEED_INIT; enqueue_dem(c); HIPCHK(hipGetLastError()); return 0; }
+extern "C" int lbdem_coupling_step(lbdem_ctx* c) { NEED_INIT; enqueue_coupling(c); HIPCHK(hipGetLastError()); return 0; }
+extern "C" int lbdem_lb_step(lbdem_ctx* c) { NEED_INIT; enqueue_lb(c); HIPCHK(hipGetLastError()); return 0; }
+extern "C" int lbdem_fs_step(lbdem_ctx* c) {
+    NEED_INIT;
+    if (c->freeSurface) enqueue_fs(c);
+    HIPCHK(hipGetLastError());
+    return 0;
+}
+
+extern "C" int lbdem_cycle(lbdem_ctx* c, int nsteps) {
+    NEED_INIT;
+    for (int s = 0; s < nsteps; ++s) {  // goCycle (main.cpp:38-67)
+        c->realTime += c->unit.Time;
+        ++c->currentTimeStep;
+        ++c->lbTime;
+        if (c->prm.demSolve) enqueue_dem(c);
+        if (c->demTime >= c->prm.demInitialRepeat) {
+            if (c->prm.demSolve) enqueue_coupling(c);
+            enqueue_lb(c);
+            if (c->freeSurface) enqueue_fs(c);
+        } else if (c->demTime <= c->prm.demInitialRepeat) {
+            if (c->prm.demSolve) enqueue_coupling(c);
+        }
+    }
+    HIPCHK(hipGetLastError());
+    return 0;
+}
+
+extern "C" int lbdem_synchronize(lbdem_ctx* c) {
+    NEED_INIT;
+    HIPCHK(hipStreamSynchronize(c->stream));
+    return check_errors(c);
+}
+
+// --------------------------- downloads ---------------------------
+extern "C" int lbdem_get_info(lbdem_ctx* c, lbdem_info* in) {
+    NEED_INIT;
+    std::memset(in, 0, sizeof(*in));
+    for (int a = 0; a < 3; ++a) in->size[a] = c->lbSize[a];
+    in->totNodes = c->box.g.N;
+    in->nParticles = c->dp.nPart; in->nObjects = c->dp.nObj; in->nWalls = c->dp.nWall;
+    int* cnt = (int*)c->tmp(64);
+    if (!cnt) return c->fail(LBDEM_E_NOMEM, "device allocation failed");
+    HIPCHK(hipMemsetAsync(cnt, 0, 5 * sizeof(int), c->stream));
+    lb_count_types(c->box, cnt, c->stream);
+    int h[5];
+    DevScalars hs;
+    HIPCHK(hipMemcpyAsync(h, cnt, sizeof(h), hipMemcpyDeviceToHost, c->stream));
+    HIPCHK(hipMemcpyAsync(&hs, c->sc, sizeof(hs), hipMemcpyDeviceToHost, c->stream));
+    HIPCHK(hipStreamSynchronize(c->stream));
+    in->nFluid = h[0]; in->nInterface = h[1]; in->nObjectNodes = h[2]; in->nParticleNodes = h[3];
+    in->nActive = h[0] + h[1];
+    in->lbTime = c->lbTime; in->demTimeStep = c->demTimeStep; in->demTime = c->demTime; in->realTime = c->realTime;
+    in->neighborListBuilds = hs.builds;
+    in->newNeighborList = hs.newNeighborList;
+    in->totalMassInit = c->totalMass;
+    in->lastExtraMass = hs.lastExtraMass;
+    in->lastSurplus = hs.lastSurplus;
+    in->precision = c->box.prec;
+    return check_errors(c);
+}
+
+extern "C" int lbdem_get_types(lbdem_ctx* c, uint8_t* type, uint8_t* pflag, int32_t* solid) {
+    NEED_INIT;
+    const int64_t N = c->box.g.N;
+    std::vector<uint8_t> t(N);
+    HIPCHK(hipMemcpyAsync(t.data(), c->box.type, N, hipMemcpyDeviceToHost, c->stream));
+    if (solid) HIPCHK(hipMemcpyAsync(solid, c->box.solid, N * 4, hipMemcpyDeviceToHost, c->stream));
+    HIPCHK(hipStreamSynchronize(c->stream));
+    for (int64_t i = 0; i < N; ++i) {
+        if (type) type[i] = t[i] & T_BASE;
+        if (pflag) pflag[i] = (t[i] & T_PMASK) ? 1 : 0;
+    }
+    return 0;
+}
+
+extern "C" int lbdem_get_fields(lbdem_ctx* c, double* f, double* mass, double* n) {
+    NEED_INIT;
+    const int64_t N = c->box.g.N;
+    if (f) {
+        double* d = c->tmp(size_t(Q) * N * 8);
+        if (!d) return c->fail(LBDEM_E_NOMEM, "device allocation failed");
+        lb_download_f(c->box, d, c->stream);
+        HIPCHK(hipMemcpyAsync(f, d, size_t(Q) * N * 8, hipMemcpyDeviceToHost, c->stream));
+        HIPCHK(hipStreamSynchronize(c->stream));
+    }
+    std::vector<uint8_t> t(N);
+    HIPCHK(hipMemcpyAsync(t.data(), c->box.type, N, hipMemcpyDeviceToHost, c->stream));
+    if (mass) {
+        if (c->box.prec == LBDEM_PREC_FP64) {
+            HIPCHK(hipMemcpyAsync(mass, c->box.mass, N * 8, hipMemcpyDeviceToHost, c->stream));
+            HIPCHK(hipStreamSynchronize(c->stream));
+        } else {
+            std::vector<float> mf(N);
+            HIPCHK(hipMemcpyAsync(mf.data(), c->box.mass, N * 4, hipMemcpyDeviceToHost, c->stream));
+            HIPCHK(hipStreamSynchronize(c->stream));
+            for (int64_t i = 0; i < N; ++i) mass[i] = mf[i];
+        }
+        for (int64_t i = 0; i < N; ++i) {
+            const uint8_t b = t[i] & T_BASE;
+            if (b != T_FLUID && b != T_IF) mass[i] = 0.0;
+        }
+    }
+    if (n) {
+        double* d = c->tmp(size_t(N) * 8);
+        if (!d) return c->fail(LBDEM_E_NOMEM, "device allocation failed");
+        lb_download_n(c->box, c->k, d, c->stream);
+        HIPCHK(hipMemcpyAsync(n, d, N * 8, hipMemcpyDeviceToHost, c->stream));
+    }
+    HIPCHK(hipStreamSynchronize(c->stream));
+    return 0;
+}
+
+static int fetch(lbdem_ctx* c, double* dst, const double* src, size_t n) {
+    HIPCHK(hipMemcpyAsync(dst, src, n * 8, hipMemcpyDeviceToHost, c->stream));
+    return 0;
+}
+
+extern "C" int lbdem_get_particles(lbdem_ctx* c, double* rec) {
+    NEED_INIT;
+    const int np = c->dp.nPart;
+    if (np == 0) return 0;
+    const DemState& d = c->ds;
+    const double* srcs[] = {d.x0, d.x1, d.w1, d.FHydro, d.MHydro, d.FContact, d.MContact, d.FLub, d.MLub, d.FRep,
+                            d.FTotal, d.MTotal, d.FGrav};
+    const int offs[] = {LBDEM_P_X0, LBDEM_P_X1, LBDEM_P_W1, LBDEM_P_FHYDRO, LBDEM_P_MHYDRO, LBDEM_P_FCONTACT,
+                        LBDEM_P_MCONTACT, LBDEM_P_FLUB, LBDEM_P_MLUB, LBDEM_P_FREP, LBDEM_P_FTOTAL, LBDEM_P_MTOTAL,
+                        LBDEM_P_FGRAV};
+    std::vector<double> buf(3 * size_t(np));
+    for (int k = 0; k < 13; ++k) {
+        if (fetch(c, buf.data(), srcs[k], 3 * np)) return LBDEM_E_DEVICE;
+        HIPCHK(hipStreamSynchronize(c->stream));
+        for (int i = 0; i < np; ++i) for (int a = 0; a < 3; ++a) rec[size_t(i) * LBDEM_P_REC + offs[k] + a] = buf[3 * i + a];
+    }
+    const double* s2[] = {d.xiw, d.xio};
+    const int o2[] = {LBDEM_P_XIW, LBDEM_P_XIO};
+    for (int k = 0; k < 2; ++k) {
+        if (fetch(c, buf.data(), s2[k], 3 * np)) return LBDEM_E_DEVICE;
+        HIPCHK(hipStreamSynchronize(c->stream));
+        for (int i = 0; i < np; ++i) for (int a = 0; a < 3; ++a) rec[size_t(i) * LBDEM_P_REC + o2[k] + a] = buf[3 * i + a];
+    }
+    const double* s1[] = {d.fluidVol, d.r, d.m};
+    const int o1[] = {LBDEM_P_FLUIDVOL, LBDEM_P_R, LBDEM_P_M};
+    for (int k = 0; k < 3; ++k) {
+        if (fetch(c, buf.data(), s1[k], np)) return LBDEM_E_DEVICE;
+        HIPCHK(hipStreamSynchronize(c->stream));
+        for (int i = 0; i < np; ++i) rec[size_t(i) * LBDEM_P_REC + o1[k]] = buf[i];
+    }
+    return 0;
+}
+
+extern "C" int lbdem_get_objects(lbdem_ctx* c, double* rec) {
+    NEED_INIT;
+    const int no = c->dp.nObj;
+    if (no == 0) return 0;
+    const DemState& d = c->ds;
+    const double* srcs[] = {d.ox0, d.ox1, d.ow1, d.oFHydro, d.oMHydro, d.oFContact, d.oMContact, d.oFLub, d.oMLub,
+                            d.oFTotal, d.oMTotal, d.oFGrav};
+    const int offs[] = {LBDEM_O_X0, LBDEM_O_X1, LBDEM_O_W1, LBDEM_O_FHYDRO, LBDEM_O_MHYDRO, LBDEM_O_FCONTACT,
+                        LBDEM_O_MCONTACT, LBDEM_O_FLUB, LBDEM_O_MLUB, LBDEM_O_FTOTAL, LBDEM_O_MTOTAL, LBDEM_O_FGRAV};
+    std::vector<double> buf(3 * size_t(no));
+    for (int k = 0; k < 12; ++k) {
+        if (fetch(c, buf.data(), srcs[k], 3 * no)) return LBDEM_E_DEVICE;
+        HIPCHK(hipStreamSynchronize(c->stream));
+        for (int i = 0; i < no; ++i) for (int a = 0; a < 3; ++a) rec[size_t(i) * LBDEM_O_REC + offs[k] + a] = buf[3 * i + a];
+    }
+    const double* s1[] = {d.or_, d.om};
+    const int o1[] = {LBDEM_O_R, LBDEM_O_M};
+    for (int k = 0; k < 2; ++k) {
+        if (fetch(c, buf.data(), s1[k], no)) return LBDEM_E_DEVICE;
+        HIPCHK(hipStreamSynchronize(c->stream));
+        for (int i = 0; i < no; ++i) rec[size_t(i) * LBDEM_O_REC + o1[k]] = buf[i];
+    }
+    return 0;
+}
+
+extern "C" int lbdem_get_walls(lbdem_ctx* c, double* rec) {
+    NEED_INIT;
+    const int nw = c->dp.nWall;
+    if (nw == 0) return 0;
+    const DemState& d = c->ds;
+    const double* srcs[] = {d.wFHydro, d.wFContact, d.wFLub, d.wFTotal};
+    const int offs[] = {LBDEM_W_FHYDRO, LBDEM_W_FCONTACT, LBDEM_W_FLUB, LBDEM_W_FTOTAL};
+    std::vector<double> buf(3 * size_t(nw));
+    for (int k = 0; k < 4; ++k) {
+        if (fetch(c, buf.data(), srcs[k], 3 * nw)) return LBDEM_E_DEVICE;
+        HIPCHK(hipStreamSynchronize(c->stream));
+        for (int i = 0; i < nw; ++i) for (int a = 0; a < 3; ++a) rec[size_t(i) * LBDEM_W_REC + offs[k] + a] = buf[3 * i + a];
+    }
+    return 0;
+}
+
+extern "C" int lbdem_get_neighbors(lbdem_ctx* c, int cap, int* count, int* list) {
+    NEED_INIT;
+    const int np = c->dp.nPart;
+    if (np == 0) return 0;
+    std::vector<int> cnt(np), lst(size_t(np) * NBR_CAP);
+    HIPCHK(hipMemcpyAsync(cnt.data(), c->ds.nbrCount, np * 4, hipMemcpyDeviceToHost, c->stream));
+    HIPCHK(hipMemcpyAsync(lst.data(), c->ds.nbrList, size_t(np) * NBR_CAP * 4, hipMemcpyDeviceToHost, c->stream));
+    HIPCHK(hipStreamSynchronize(c->stream));
+    for (int i = 0; i < np; ++i) {
+        count[i] = cnt[i];
+        for (int k = 0; k < cnt[i] && k < cap; ++k) list[size_t(i) * cap + k] = lst[size_t(i) * NBR_CAP + k];
+    }
+    return 0;
+}
+
+extern "C" int lbdem_total_mass(lbdem_ctx* c, double* m) {
+    NEED_INIT;
+    const int nb = 1024;
+    double* part = c->tmp(nb * sizeof(double));
+    if (!part) return c->fail(LBDEM_E_NOMEM, "device allocation failed");
+    lb_total_mass(c->box, part, c->stream);
+    std::vector<double> h(nb);
+    HIPCHK(hipMemcpyAsync(h.data(), part, nb * 8, hipMemcpyDeviceToHost, c->stream));
+    HIPCHK(hipStreamSynchronize(c->stream));
+    double s = 0.0;
+    for (double v : h) s += v;
+    *m = s;
+    return 0;
+}

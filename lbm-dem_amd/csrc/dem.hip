@@ -1,0 +1,737 @@
+// DEM kernels of the coupled step on gfx950, all FP64 (compiled without contraction so the
+// arithmetic rounds like the reference's scalar code):
+//   * neighbour list by cell sort (DEM::neighborList, DEM.cpp:473-652): counting sort of
+//     particles into link cells, per-particle gather over the 27 neighbour cells, partners
+//     sorted ascending, tangential history carried over by partner id
+//   * forces (DEM::evaluateForces, DEM.cpp:655-796): one thread per particle evaluates every
+//     pair from its side with the reference's (n1 > n2) orientation, so each pair's force is
+//     computed bit-identically twice and no atomics are needed; the pair's tangential history
+//     is read from the previous buffer and written once, by the thread of the higher index
+//   * integration and periodic wrap (DEM.cpp:2676-2836)
+#include "lbdem_internal.hpp"
+
+#include <climits>
+
+namespace lbdem {
+
+__device__ inline double dmaxd(double a, double b) { return a > b ? a : b; }
+__device__ inline int n_int(double x) { return x >= 0 ? int(x + 0.5) : int(x - 0.5); }
+__device__ inline int s_int(double x) { return x >= 0 ? int(x) : int(x - 1.0); }
+__device__ inline double dbox(double a, double b) { return a - s_int(a / b) * b; }
+__device__ inline double n_image(double a, double b) { return a - n_int(a / b) * b; }
+
+struct V { double x, y, z; };
+__device__ inline V ld3(const double* p, int i) { return V{p[3 * i], p[3 * i + 1], p[3 * i + 2]}; }
+__device__ inline void st3(double* p, int i, V v) { p[3 * i] = v.x; p[3 * i + 1] = v.y; p[3 * i + 2] = v.z; }
+__device__ inline V add(V a, V b) { return V{a.x + b.x, a.y + b.y, a.z + b.z}; }
+__device__ inline V sub(V a, V b) { return V{a.x - b.x, a.y - b.y, a.z - b.z}; }
+
+// ------------------------------------------------------------------------------------------
+// neighbour-list trigger (DEM.cpp:366-378, 460-470)
+// ------------------------------------------------------------------------------------------
+__global__ void k_maxdisp(const double* x1, int n, double deltat, DevScalars* sc, int force) {
+    __shared__ double red[256];
+    double m = 0.0;
+    for (int p = threadIdx.x; p < n; p += blockDim.x) {
+        const double v2 = x1[3 * p] * x1[3 * p] + x1[3 * p + 1] * x1[3 * p + 1] + x1[3 * p + 2] * x1[3 * p + 2];
+        if (v2 > m) m = v2;
+    }
+    red[threadIdx.x] = m;
+    __syncthreads();
+    for (int w = 128; w > 0; w >>= 1) {
+        if (threadIdx.x < w && red[threadIdx.x + w] > red[threadIdx.x]) red[threadIdx.x] = red[threadIdx.x + w];
+        __syncthreads();
+    }
+    if (threadIdx.x == 0) {
+        if (force) {
+            sc->rebuild = 1;
+            sc->builds += 1;
+            return;
+        }
+        const double maxVel = sqrt(red[0]);
+        sc->maxDisp += maxVel * deltat;
+        if (sc->maxDisp > 0.05) {
+            sc->maxDisp = 0.0;
+            sc->rebuild = 1;
+            sc->newNeighborList = 1;
+            sc->builds += 1;
+        } else {
+            sc->rebuild = 0;
+        }
+    }
+}
+
+// ------------------------------------------------------------------------------------------
+// cell list (counting sort by link cell)
+// ------------------------------------------------------------------------------------------
+struct NbrArgs {
+    DemParams p;
+    const double* x0;
+    const double* r;
+    int* cellCount;
+    int* cellStart;
+    int* cellFill;
+    int* cellItems;
+    int* cellKey;
+    int* nbrCount;
+    int* nbrList;
+    int* nbrSlot;
+    const int* oldCount;
+    const int* oldList;
+    const double* xiCur;
+    double* xiTmp;
+    DevScalars* sc;
+};
+
+__global__ void k_cells_clear(NbrArgs a, int ncell) {
+    if (!a.sc->rebuild) return;
+    for (int c = blockIdx.x * blockDim.x + threadIdx.x; c < ncell; c += gridDim.x * blockDim.x) {
+        a.cellCount[c] = 0;
+        a.cellFill[c] = 0;
+    }
+}
+__global__ void k_cells_assign(NbrArgs a) {
+    if (!a.sc->rebuild) return;
+    const int p = blockIdx.x * blockDim.x + threadIdx.x;
+    if (p >= a.p.nPart) return;
+    const int lx = int(dbox(a.x0[3 * p] / a.p.demSize[0], 1) * a.p.nlx);
+    const int ly = int(dbox(a.x0[3 * p + 1] / a.p.demSize[1], 1) * a.p.nly);
+    const int lz = int(dbox(a.x0[3 * p + 2] / a.p.demSize[2], 1) * a.p.nlz);
+    int key = lx * a.p.nly * a.p.nlz + ly * a.p.nlz + lz;
+    const int ncell = a.p.nlx * a.p.nly * a.p.nlz;
+    key = key < 0 ? 0 : (key >= ncell ? ncell - 1 : key);
+    a.cellKey[p] = key;
+    atomicAdd(&a.cellCount[key], 1);
+}
+__global__ void k_cells_scan(NbrArgs a, int ncell) {
+    if (!a.sc->rebuild) return;
+    __shared__ int part[1024];
+    const int per = (ncell + blockDim.x - 1) / blockDim.x;
+    const int b = threadIdx.x * per, e = min(ncell, b + per);
+    int s = 0;
+    for (int c = b; c < e; ++c) s += a.cellCount[c];
+    part[threadIdx.x] = s;
+    __syncthreads();
+    if (threadIdx.x == 0) {
+        int acc = 0;
+        for (int t = 0; t < int(blockDim.x); ++t) { const int v = part[t]; part[t] = acc; acc += v; }
+    }
+    __syncthreads();
+    int acc = part[threadIdx.x];
+    for (int c = b; c < e; ++c) { a.cellStart[c] = acc; acc += a.cellCount[c]; }
+}
+__global__ void k_cells_fill(NbrArgs a) {
+    if (!a.sc->rebuild) return;
+    const int p = blockIdx.x * blockDim.x + threadIdx.x;
+    if (p >= a.p.nPart) return;
+    const int key = a.cellKey[p];
+    const int pos = atomicAdd(&a.cellFill[key], 1);
+    a.cellItems[a.cellStart[key] + pos] = p;
+}
+
+__device__ inline bool within_cut(const NbrArgs& a, int n1, int n2) {
+#pragma clang fp contract(off)
+    const double x12 = n_image(a.x0[3 * n1] - a.x0[3 * n2], a.p.demSize[0]);
+    const double y12 = n_image(a.x0[3 * n1 + 1] - a.x0[3 * n2 + 1], a.p.demSize[1]);
+    const double z12 = n_image(a.x0[3 * n1 + 2] - a.x0[3 * n2 + 2], a.p.demSize[2]);
+    const double r12 = x12 * x12 + y12 * y12 + z12 * z12;
+    const double sig = a.r[n1] + a.r[n2];
+    const double cut = sig + 2.0;
+    return r12 <= cut * cut;
+}
+
+// per-particle partner gather (pairs with minimum-image distance within sig + 2 in all axes)
+__global__ void k_nbr_build(NbrArgs a) {
+    if (!a.sc->rebuild) return;
+    const int p = blockIdx.x * blockDim.x + threadIdx.x;
+    if (p >= a.p.nPart) return;
+    int lst[NBR_CAP];
+    int cnt = 0;
+    bool over = false;
+    auto consider = [&](int q) {
+        if (q == p) return;
+        const int n1 = p > q ? p : q, n2 = p > q ? q : p;
+        if (!within_cut(a, n1, n2)) return;
+        if (cnt < NBR_CAP) lst[cnt++] = q; else over = true;
+    };
+    if (a.p.nlx == 0) {
+        for (int q = 0; q < a.p.nPart; ++q) consider(q);
+    } else {
+        const int nlx = a.p.nlx, nly = a.p.nly, nlz = a.p.nlz;
+        const int key = a.cellKey[p];
+        const int lx = key / (nly * nlz), ly = (key / nlz) % nly, lz = key % nlz;
+        for (int dx = -1; dx <= 1; ++dx)
+            for (int dy = -1; dy <= 1; ++dy)
+                for (int dz = -1; dz <= 1; ++dz) {
+                    const int c = ((lx + dx + nlx) % nlx) * nly * nlz + ((ly + dy + nly) % nly) * nlz + (lz + dz + nlz) % nlz;
+                    const int b = a.cellStart[c], e = b + a.cellCount[c];
+                    for (int t = b; t < e; ++t) consider(a.cellItems[t]);
+                }
+    }
+    // ascending partners (insertion sort, lists are short)
+    for (int i = 1; i < cnt; ++i) {
+        const int v = lst[i];
+        int j = i - 1;
+        while (j >= 0 && lst[j] > v) { lst[j + 1] = lst[j]; --j; }
+        lst[j + 1] = v;
+    }
+    a.nbrCount[p] = cnt;
+    for (int i = 0; i < cnt; ++i) a.nbrList[p * NBR_CAP + i] = lst[i];
+    if (over) atomicOr(&a.sc->nbrOverflow, 1);
+}
+
+// slot of every entry (history lives in the list of the higher index), history carried over by
+// partner id (DEM.cpp:633-651); a partner absent from the old list starts with zero history
+__global__ void k_nbr_slots(NbrArgs a) {
+    if (!a.sc->rebuild) return;
+    const int p = blockIdx.x * blockDim.x + threadIdx.x;
+    if (p >= a.p.nPart) return;
+    const int cnt = a.nbrCount[p];
+    for (int k = 0; k < cnt; ++k) {
+        const int q = a.nbrList[p * NBR_CAP + k];
+        int slot;
+        if (p > q) {
+            slot = p * NBR_CAP + k;
+            double h0 = 0.0, h1 = 0.0, h2 = 0.0;
+            const int oc = a.oldCount[p];
+            for (int l = 0; l < oc; ++l)
+                if (a.oldList[p * NBR_CAP + l] == q) {
+                    const int os = p * NBR_CAP + l;
+                    h0 = a.xiCur[3 * os]; h1 = a.xiCur[3 * os + 1]; h2 = a.xiCur[3 * os + 2];
+                }
+            a.xiTmp[3 * slot] = h0; a.xiTmp[3 * slot + 1] = h1; a.xiTmp[3 * slot + 2] = h2;
+        } else {
+            // binary search p in q's (sorted) list
+            const int qc = a.nbrCount[q];
+            int lo = 0, hi = qc - 1, pos = -1;
+            while (lo <= hi) {
+                const int mid = (lo + hi) >> 1;
+                const int v = a.nbrList[q * NBR_CAP + mid];
+                if (v == p) { pos = mid; break; }
+                if (v < p) lo = mid + 1; else hi = mid - 1;
+            }
+            slot = pos >= 0 ? q * NBR_CAP + pos : -1;
+        }
+        a.nbrSlot[p * NBR_CAP + k] = slot;
+    }
+}
+__global__ void k_nbr_commit(NbrArgs a, double* xiCurMut, int* oldCount, int* oldList) {
+    if (!a.sc->rebuild) return;
+    const int p = blockIdx.x * blockDim.x + threadIdx.x;
+    if (p >= a.p.nPart) return;
+    const int cnt = a.nbrCount[p];
+    for (int k = 0; k < cnt; ++k) {
+        const int s = p * NBR_CAP + k;
+        xiCurMut[3 * s] = a.xiTmp[3 * s]; xiCurMut[3 * s + 1] = a.xiTmp[3 * s + 1]; xiCurMut[3 * s + 2] = a.xiTmp[3 * s + 2];
+        oldList[s] = a.nbrList[s];
+    }
+    oldCount[p] = cnt;
+}
+
+// ------------------------------------------------------------------------------------------
+// forces
+// ------------------------------------------------------------------------------------------
+struct ForceArgs {
+    DemParams p;
+    const double *x0, *x1, *w1, *r;
+    double *FContact, *MContact, *FLub, *MLub, *FRep;
+    double *xiw, *xio;
+    const int* nbrCount;
+    const int* nbrList;
+    const int* nbrSlot;
+    const double* xiOld;
+    double* xiNew;
+    const double *ox0, *ox1, *ow1, *or_;
+    double* objContrib;   // per particle, per object: Fc(3) Mc(3) Fl(3) Ml(3)
+    double* wallContrib;  // per particle, per wall: Fc(3) Fl(3)
+    int demTimeStep;
+};
+
+struct LubOut { V F, T1, T2; bool act; };
+
+// pair lubrication (DEM.cpp:1140-1340, 2302-2521) for the ordered pair (a = n1, b = n2)
+__device__ LubOut pair_lub(const DemParams& P, V xa, V xb, double ra, double rb, V va, V vb, V wa, V wb, double bnum,
+                           double bden) {
+#pragma clang fp contract(off)
+    LubOut o{};
+    const double lub_N = P.lubNormal, lub_T = P.lubTangential, lub_R = P.lubRotation, cut = P.lubCutoff;
+    const double rad = 0.5 * (ra + rb);
+    double x12 = xa.x - xb.x, y12 = xa.y - xb.y, z12 = xa.z - xb.z;
+    const double r12 = sqrt(x12 * x12 + y12 * y12 + z12 * z12);
+    const double umx = va.x - vb.x, umy = va.y - vb.y, umz = va.z - vb.z;
+    const double dr = r12 - 2.0 * rad;
+    const double h = dmaxd(dr, cut);
+    o.act = dr < dmaxd(cut, dmaxd(lub_N, dmaxd(lub_T, lub_R)));
+    if (!o.act) return o;
+    double Xa, Ya, Yb1, Yb2, Yc11, Yc12, Yc21, Yc22, h_c, beta;
+    const double alpha = M_PI * P.kinVisc * rad * P.deltat;
+    const double beta1 = bnum / bden;
+    const double beta2 = beta1 * beta1, beta3 = beta1 * beta2;
+    const double beta4 = pow((1 + beta1), 4.0);
+    if (dr < dmaxd(cut, lub_N)) {
+        Xa = 4.0 * beta2 / beta4;
+        h_c = dmaxd(dr, lub_N);
+        Xa *= rad / h - rad / h_c;
+        Xa *= 6.0 * alpha;
+    } else Xa = 0.0;
+    if (dr < dmaxd(cut, lub_T)) {
+        Ya = 8.0 * beta1 * (2.0 + beta1 + 2.0 * beta2) / (15.0 * beta4);
+        Yb1 = -4.0 * beta1 * (4.0 + beta1) / (5.0 * beta4);
+        Yb2 = 4.0 * beta1 * (4.0 * beta2 + beta1) / (5.0 * beta4);
+        h_c = dmaxd(dr, lub_T);
+        beta = alpha * log(h_c / h);
+        Ya *= 6.0 * beta; Yb1 *= 4.0 * beta * rad; Yb2 *= 4.0 * beta * rad;
+    } else Ya = Yb1 = Yb2 = 0.0;
+    if (dr < dmaxd(cut, lub_R)) {
+        Yc11 = 16.0 * beta1 / (5.0 * beta4);
+        Yc12 = 4.0 * beta2 / (5.0 * beta4);
+        Yc21 = 4.0 * beta2 / (5.0 * beta4);
+        Yc22 = 16.0 * beta3 / (5.0 * beta4);
+        h_c = dmaxd(dr, lub_R);
+        beta = alpha * log(h_c / h);
+        Yc11 *= 8.0 * beta * rad * rad; Yc12 *= 8.0 * beta * rad * rad;
+        Yc21 *= 8.0 * beta * rad * rad; Yc22 *= 8.0 * beta * rad * rad;
+    } else Yc11 = Yc12 = Yc21 = Yc22 = 0.0;
+    x12 /= r12; y12 /= r12; z12 /= r12;
+    const double w1x = wa.x, w1y = wa.y, w1z = wa.z, w2x = wb.x, w2y = wb.y, w2z = wb.z;
+    const double um_r = umx * x12 + umy * y12 + umz * z12;
+    const double w1_r = w1x * x12 + w1y * y12 + w1z * z12;
+    const double w2_r = w2x * x12 + w2y * y12 + w2z * z12;
+    const double umxrx = umy * z12 - umz * y12, umxry = umz * x12 - umx * z12, umxrz = umx * y12 - umy * x12;
+    const double w1xrx = w1y * z12 - w1z * y12, w1xry = w1z * x12 - w1x * z12, w1xrz = w1x * y12 - w1y * x12;
+    const double w2xrx = w2y * z12 - w2z * y12, w2xry = w2z * x12 - w2x * z12, w2xrz = w2x * y12 - w2y * x12;
+    double Fx = -Xa * um_r * x12 - Ya * (umx - um_r * x12);
+    double Fy = -Xa * um_r * y12 - Ya * (umy - um_r * y12);
+    double Fz = -Xa * um_r * z12 - Ya * (umz - um_r * z12);
+    Fx += -Yb1 * w1xrx + Yb2 * w2xrx;
+    Fy += -Yb1 * w1xry + Yb2 * w2xry;
+    Fz += -Yb1 * w1xrz + Yb2 * w2xrz;
+    double T1x = Yb1 * umxrx, T1y = Yb1 * umxry, T1z = Yb1 * umxrz;
+    double T2x = -Yb2 * umxrx, T2y = -Yb2 * umxry, T2z = -Yb2 * umxrz;
+    T1x += -Yc11 * (w1x - w1_r * x12) - Yc12 * (w2x - w2_r * x12);
+    T1y += -Yc11 * (w1y - w1_r * y12) - Yc12 * (w2y - w2_r * y12);
+    T1z += -Yc11 * (w1z - w1_r * z12) - Yc12 * (w2z - w2_r * z12);
+    T2x += -Yc21 * (w1x - w1_r * x12) - Yc22 * (w2x - w2_r * x12);
+    T2y += -Yc21 * (w1y - w1_r * y12) - Yc22 * (w2y - w2_r * y12);
+    T2z += -Yc21 * (w1z - w1_r * z12) - Yc22 * (w2z - w2_r * z12);
+    o.F = V{Fx, Fy, Fz}; o.T1 = V{T1x, T1y, T1z}; o.T2 = V{T2x, T2y, T2z};
+    return o;
+}
+
+// wall-type contact with the shared per-particle history xiw (DEM.cpp:1946-2153, 2157-2296)
+__device__ void wall_like_contact(const DemParams& P, double dampCoeff, double r, V x1, V w1, V& xi, double x1w,
+                                  double y1w, double z1w, V& Fout, V& Mout) {
+#pragma clang fp contract(off)
+    const double r1w = sqrt(x1w * x1w + y1w * y1w + z1w * z1w);
+    const double delw = r - r1w;
+    double fnx = 0, fny = 0, fnz = 0, ftx = 0, fty = 0, ftz = 0, n1wx = 0, n1wy = 0, n1wz = 0;
+    if (delw > 0.0) {
+        n1wx = x1w / r1w; n1wy = y1w / r1w; n1wz = z1w / r1w;
+        const double gn = x1.x * n1wx + x1.y * n1wy + x1.z * n1wz;
+        const double k = P.linearStiff, mu = P.frictionCoefPart;
+        fnx = k * delw * n1wx - dampCoeff * gn * n1wx;
+        fny = k * delw * n1wy - dampCoeff * gn * n1wy;
+        fnz = k * delw * n1wz - dampCoeff * gn * n1wz;
+        if (mu != 0.0) {
+            const double ox = r * w1.x, oy = r * w1.y, oz = r * w1.z;
+            const double vx = x1.x + n1wy * oz - n1wz * oy - gn * n1wx;
+            const double vy = x1.y + n1wz * ox - n1wx * oz - gn * n1wy;
+            const double vz = x1.z + n1wx * oy - n1wy * ox - gn * n1wz;
+            const double xin = xi.x * n1wx + xi.y * n1wy + xi.z * n1wz;
+            xi.x -= xin * n1wx; xi.y -= xin * n1wy; xi.z -= xin * n1wz;
+            const double kt = 0.2 * k;
+            const double dct = P.kinVisc == 0.0 ? dampCoeff : 0.0;
+            ftx = -kt * xi.x - dct * vx; fty = -kt * xi.y - dct * vy; ftz = -kt * xi.z - dct * vz;
+            const double fnn = sqrt(fnx * fnx + fny * fny + fnz * fnz);
+            const double ftt = sqrt(ftx * ftx + fty * fty + ftz * ftz);
+            const double fri = mu * fnn;
+            const double tx = ftx / ftt, ty = fty / ftt, tz = ftz / ftt;
+            xi.x += vx * P.deltat; xi.y += vy * P.deltat; xi.z += vz * P.deltat;
+            if (fri < ftt) {
+                ftx = fri * tx; fty = fri * ty; ftz = fri * tz;
+                xi.x = -(1 / kt) * (ftx + dct * vx);
+                xi.y = -(1 / kt) * (fty + dct * vy);
+                xi.z = -(1 / kt) * (ftz + dct * vz);
+            }
+        }
+    } else {
+        xi = V{0, 0, 0};
+    }
+    Fout = V{fnx + ftx, fny + fty, fnz + ftz};
+    const double dtx = fty * n1wz - ftz * n1wy, dty = ftz * n1wx - ftx * n1wz, dtz = ftx * n1wy - fty * n1wx;
+    Mout = V{r * dtx, r * dty, r * dtz};
+}
+
+__global__ void k_dem_forces(ForceArgs a, double dampCoeff) {
+#pragma clang fp contract(off)
+    const int p = blockIdx.x * blockDim.x + threadIdx.x;
+    const DemParams& P = a.p;
+    if (p >= P.nPart) return;
+    const V xp = ld3(a.x0, p), vp = ld3(a.x1, p), wp = ld3(a.w1, p);
+    const double rp = a.r[p];
+    V Fc{0, 0, 0}, Mc{0, 0, 0}, Fl{0, 0, 0}, Ml{0, 0, 0}, Fr{0, 0, 0};
+    V xiw = ld3(a.xiw, p), xio = ld3(a.xio, p);
+    const int cnt = a.nbrCount[p];
+    const bool waited = a.demTimeStep >= P.objWaitTime;
+    double* oc = a.objContrib + int64_t(p) * 12 * P.nObj;
+    double* wc = a.wallContrib + int64_t(p) * 6 * P.nWall;
+    for (int c = 0; c < 12 * P.nObj; ++c) oc[c] = 0.0;
+    for (int c = 0; c < 6 * P.nWall; ++c) wc[c] = 0.0;
+    if (waited) {
+        // particle lubrication (DEM.cpp:2302-2521)
+        for (int k = 0; k < cnt; ++k) {
+            const int q = a.nbrList[p * NBR_CAP + k];
+            const int n1 = p > q ? p : q, n2 = p > q ? q : p;
+            const LubOut o = pair_lub(P, ld3(a.x0, n1), ld3(a.x0, n2), a.r[n1], a.r[n2], ld3(a.x1, n1), ld3(a.x1, n2),
+                                      ld3(a.w1, n1), ld3(a.w1, n2), a.r[n2], a.r[n1]);
+            if (!o.act) continue;
+            if (p == n1) { Fl = add(Fl, o.F); Ml = add(Ml, o.T1); }
+            else { Fl = sub(Fl, o.F); Ml = add(Ml, o.T2); }
+        }
+        // wall lubrication (DEM.cpp:2525-2672)
+        for (int w = 0; w < P.nWall; ++w) {
+            const double rad = rp;
+            double x12 = (xp.x - P.wallP[w][0]) * P.wallN[w][0] * P.wallN[w][0];
+            double y12 = (xp.y - P.wallP[w][1]) * P.wallN[w][1] * P.wallN[w][1];
+            double z12 = (xp.z - P.wallP[w][2]) * P.wallN[w][2] * P.wallN[w][2];
+            const double r12 = sqrt(x12 * x12 + y12 * y12 + z12 * z12);
+            const double dr = r12 - rad;
+            const double cut = P.lubCutoff, lub_N = P.lubNormal, lub_T = P.lubTangential, lub_R = P.lubRotation;
+            const double h = dmaxd(dr, cut);
+            if (!(dr < dmaxd(cut, dmaxd(lub_N, dmaxd(lub_T, lub_R))))) continue;
+            const double alpha = M_PI * P.kinVisc * rad * P.deltat;
+            double Xa, Ya, Yb1, Yc11, h_c, beta;
+            if (dr < dmaxd(cut, lub_N)) { Xa = 1.0; h_c = dmaxd(dr, lub_N); Xa *= rad / h - rad / h_c; Xa *= 6.0 * alpha; }
+            else Xa = 0.0;
+            if (dr < dmaxd(cut, lub_T)) {
+                Ya = 8.0 / 15.0; Yb1 = -1.0 / 5.0; h_c = dmaxd(dr, lub_T); beta = alpha * log(h_c / h);
+                Ya *= 6.0 * beta; Yb1 *= 4.0 * beta * rad;
+            } else Ya = Yb1 = 0.0;
+            if (dr < dmaxd(cut, lub_R)) { Yc11 = 2.0 / 5.0; h_c = dmaxd(dr, lub_R); beta = alpha * log(h_c / h); Yc11 *= 8.0 * beta * rad * rad; }
+            else Yc11 = 0.0;
+            x12 /= r12; y12 /= r12; z12 /= r12;
+            const double umx = vp.x, umy = vp.y, umz = vp.z, w1x = wp.x, w1y = wp.y, w1z = wp.z;
+            const double um_r = umx * x12 + umy * y12 + umz * z12;
+            const double w1_r = w1x * x12 + w1y * y12 + w1z * z12;
+            const double umxrx = umy * z12 - umz * y12, umxry = umz * x12 - umx * z12, umxrz = umx * y12 - umy * x12;
+            const double w1xrx = w1y * z12 - w1z * y12, w1xry = w1z * x12 - w1x * z12, w1xrz = w1x * y12 - w1y * x12;
+            double Fx = -Xa * x12 * um_r - Ya * (umx - um_r * x12);
+            double Fy = -Xa * y12 * um_r - Ya * (umy - um_r * y12);
+            double Fz = -Xa * z12 * um_r - Ya * (umz - um_r * z12);
+            Fx += -Yb1 * w1xrx; Fy += -Yb1 * w1xry; Fz += -Yb1 * w1xrz;
+            double T1x = Yb1 * umxrx, T1y = Yb1 * umxry, T1z = Yb1 * umxrz;
+            T1x += -Yc11 * (w1x - w1_r * x12); T1y += -Yc11 * (w1y - w1_r * y12); T1z += -Yc11 * (w1z - w1_r * z12);
+            Fl = add(Fl, V{Fx, Fy, Fz}); Ml = add(Ml, V{T1x, T1y, T1z});
+            wc[6 * w + 3] -= Fx; wc[6 * w + 4] -= Fy; wc[6 * w + 5] -= Fz;
+        }
+        // object-particle contact (DEM.cpp:912-1138); history xio shared by all objects
+        for (int o = 0; o < P.nObj; ++o) {
+            const V ox = ld3(a.ox0, o), ov = ld3(a.ox1, o), ow = ld3(a.ow1, o);
+            const double orr = a.or_[o];
+            const double radw = 0.5 * (rp + orr);
+            const double x1w = xp.x - ox.x, y1w = xp.y - ox.y, z1w = xp.z - ox.z;
+            const double r1w = sqrt(x1w * x1w + y1w * y1w + z1w * z1w);
+            const double n1wx = x1w / r1w, n1wy = y1w / r1w, n1wz = z1w / r1w;
+            const double urx = vp.x - ov.x, ury = vp.y - ov.y, urz = vp.z - ov.z;
+            const double delw = (2 * radw) - r1w;
+            double fnx = 0, fny = 0, fnz = 0, ftx = 0, fty = 0, ftz = 0;
+            if (delw > 0.0) {
+                const double gn = urx * n1wx + ury * n1wy + urz * n1wz;
+                const double k = P.linearStiff, mu = P.frictionCoefPart;
+                fnx = k * delw * n1wx - dampCoeff * gn * n1wx;
+                fny = k * delw * n1wy - dampCoeff * gn * n1wy;
+                fnz = k * delw * n1wz - dampCoeff * gn * n1wz;
+                if (mu != 0.0) {
+                    const double o1x = rp * wp.x + orr * ow.x, o1y = rp * wp.y + orr * ow.y, o1z = rp * wp.z + orr * ow.z;
+                    const double vx = urx + n1wy * o1z - n1wz * o1y - gn * n1wx;
+                    const double vy = ury + n1wz * o1x - n1wx * o1z - gn * n1wy;
+                    const double vz = urz + n1wx * o1y - n1wy * o1x - gn * n1wz;
+                    const double xin = xio.x * n1wx + xio.y * n1wy + xio.z * n1wz;
+                    xio.x -= xin * n1wx; xio.y -= xin * n1wy; xio.z -= xin * n1wz;
+                    const double kt = 0.2 * k;
+                    const double dct = P.kinVisc == 0.0 ? dampCoeff : 0.0;
+                    ftx = -kt * xio.x; fty = -kt * xio.y; ftz = -kt * xio.z;
+                    const double fnn = sqrt(fnx * fnx + fny * fny + fnz * fnz);
+                    const double ftt = sqrt(ftx * ftx + fty * fty + ftz * ftz);
+                    const double fri = mu * fnn;
+                    const double tx = ftx / ftt, ty = fty / ftt, tz = ftz / ftt;
+                    if (fri < ftt) {
+                        ftx = fri * tx; fty = fri * ty; ftz = fri * tz;
+                        xio.x = -(1 / kt) * (ftx + dct * vx);
+                        xio.y = -(1 / kt) * (fty + dct * vy);
+                        xio.z = -(1 / kt) * (ftz + dct * vz);
+                    } else {
+                        xio.x += vx * P.deltat; xio.y += vy * P.deltat; xio.z += vz * P.deltat;
+                    }
+                }
+            } else {
+                xio = V{0, 0, 0};
+            }
+            const V df{fnx + ftx, fny + fty, fnz + ftz};
+            const double dtx = fty * n1wz - ftz * n1wy, dty = ftz * n1wx - ftx * n1wz, dtz = ftx * n1wy - fty * n1wx;
+            Fc = add(Fc, df);
+            Mc = add(Mc, V{rp * dtx, rp * dty, rp * dtz});
+            double* c = oc + 12 * o;
+            c[0] = -df.x; c[1] = -df.y; c[2] = -df.z;
+            c[3] = orr * dtx; c[4] = orr * dty; c[5] = orr * dtz;
+        }
+        // object-particle lubrication (DEM.cpp:1140-1340)
+        for (int o = 0; o < P.nObj; ++o) {
+            const LubOut lo = pair_lub(P, xp, ld3(a.ox0, o), rp, a.or_[o], vp, ld3(a.ox1, o), wp, ld3(a.ow1, o), a.or_[o], rp);
+            if (!lo.act) continue;
+            Fl = add(Fl, lo.F); Ml = add(Ml, lo.T1);
+            double* c = oc + 12 * o;
+            c[6] = -lo.F.x; c[7] = -lo.F.y; c[8] = -lo.F.z;
+            c[9] = lo.T2.x; c[10] = lo.T2.y; c[11] = lo.T2.z;
+        }
+    } else {
+        // fictional lid at demSize[1] - 0.3 (DEM.cpp:2157-2296)
+        V F, M;
+        wall_like_contact(P, dampCoeff, rp, vp, wp, xiw, 0.0, xp.y - (P.demSize[1] - 0.3), 0.0, F, M);
+        Fc = add(Fc, F); Mc = add(Mc, M);
+    }
+    // particle-particle contact (DEM.cpp:1487-1791)
+    for (int k = 0; k < cnt; ++k) {
+        const int q = a.nbrList[p * NBR_CAP + k];
+        const int n1 = p > q ? p : q, n2 = p > q ? q : p;
+        const int slot = a.nbrSlot[p * NBR_CAP + k];
+        V xi{0, 0, 0};
+        if (slot >= 0) xi = ld3(a.xiOld, slot);
+        const V A = ld3(a.x0, n1), B = ld3(a.x0, n2), vA = ld3(a.x1, n1), vB = ld3(a.x1, n2);
+        const V wA = ld3(a.w1, n1), wB = ld3(a.w1, n2);
+        const double rA = a.r[n1], rB = a.r[n2];
+        const double rad = 0.5 * (rA + rB);
+        const double x12 = A.x - B.x, y12 = A.y - B.y, z12 = A.z - B.z;
+        const double r12 = sqrt(x12 * x12 + y12 * y12 + z12 * z12);
+        const double urx = vA.x - vB.x, ury = vA.y - vB.y, urz = vA.z - vB.z;
+        const double nx = x12 / r12, ny = y12 / r12, nz = z12 / r12;
+        const double gn = urx * nx + ury * ny + urz * nz;
+        const double del = 2 * rad - r12;
+        double fnx = 0, fny = 0, fnz = 0, ftx = 0, fty = 0, ftz = 0;
+        if (del > 0.0) {
+            const double k_ = P.linearStiff, mu = P.frictionCoefPart;
+            fnx = k_ * del * nx - dampCoeff * gn * nx;
+            fny = k_ * del * ny - dampCoeff * gn * ny;
+            fnz = k_ * del * nz - dampCoeff * gn * nz;
+            if (mu != 0.0) {
+                const double ox = rA * wA.x + rB * wB.x, oy = rA * wA.y + rB * wB.y, oz = rA * wA.z + rB * wB.z;
+                const double vx = urx + ny * oz - nz * oy - gn * nx;
+                const double vy = ury + nz * ox - nx * oz - gn * ny;
+                const double vz = urz + nx * oy - ny * ox - gn * nz;
+                const double xin = xi.x * nx + xi.y * ny + xi.z * nz;
+                xi.x -= xin * nx; xi.y -= xin * ny; xi.z -= xin * nz;
+                const double kt = 0.2 * k_;
+                const double dct = P.kinVisc == 0.0 ? dampCoeff : 0.0;
+                ftx = -kt * xi.x - dct * vx; fty = -kt * xi.y - dct * vy; ftz = -kt * xi.z - dct * vz;
+                const double fnn = sqrt(fnx * fnx + fny * fny + fnz * fnz);
+                const double ftt = sqrt(ftx * ftx + fty * fty + ftz * ftz);
+                const double fri = mu * fnn;
+                const double tx = ftx / ftt, ty = fty / ftt, tz = ftz / ftt;
+                xi.x += vx * P.deltat; xi.y += vy * P.deltat; xi.z += vz * P.deltat;
+                if (fri < ftt) {
+                    ftx = fri * tx; fty = fri * ty; ftz = fri * tz;
+                    xi.x = -(1 / kt) * (ftx + dct * vx);
+                    xi.y = -(1 / kt) * (fty + dct * vy);
+                    xi.z = -(1 / kt) * (ftz + dct * vz);
+                }
+            }
+        } else {
+            xi = V{0, 0, 0};
+        }
+        if (p == n1 && slot >= 0) st3(a.xiNew, slot, xi);
+        const double dfx = fnx + ftx, dfy = fny + fty, dfz = fnz + ftz;
+        const double dtx = fty * nz - ftz * ny, dty = ftz * nx - ftx * nz, dtz = ftx * ny - fty * nx;
+        if (p == n1) { Fc = add(Fc, V{dfx, dfy, dfz}); Mc = add(Mc, V{rA * dtx, rA * dty, rA * dtz}); }
+        else { Fc = sub(Fc, V{dfx, dfy, dfz}); Mc = add(Mc, V{rB * dtx, rB * dty, rB * dtz}); }
+    }
+    // electrostatic repulsion over every listed pair (DEM.cpp:1404-1484)
+    for (int k = 0; k < cnt; ++k) {
+        const int q = a.nbrList[p * NBR_CAP + k];
+        const int n1 = p > q ? p : q, n2 = p > q ? q : p;
+        const V A = ld3(a.x0, n1), B = ld3(a.x0, n2);
+        const double rad = 0.5 * (a.r[n1] + a.r[n2]);
+        const double x12 = A.x - B.x, y12 = A.y - B.y, z12 = A.z - B.z;
+        const double r12 = sqrt(x12 * x12 + y12 * y12 + z12 * z12);
+        const double nx = x12 / r12, ny = y12 / r12, nz = z12 / r12;
+        const double del = 2 * rad - r12;
+        double fx, fy, fz;
+        if (del <= 0.0) {
+            const double e = P.electroForce * exp(del / P.debyeLength);
+            fx = e * nx; fy = e * ny; fz = e * nz;
+        } else {
+            fx = P.electroForce * nx; fy = P.electroForce * ny; fz = P.electroForce * nz;
+        }
+        if (p == n1) Fr = add(Fr, V{fx, fy, fz}); else Fr = sub(Fr, V{fx, fy, fz});
+    }
+    // wall contacts (DEM.cpp:1946-2153), shared history xiw
+    for (int w = 0; w < P.nWall; ++w) {
+        const double x1w = (xp.x - P.wallP[w][0]) * P.wallN[w][0] * P.wallN[w][0];
+        const double y1w = (xp.y - P.wallP[w][1]) * P.wallN[w][1] * P.wallN[w][1];
+        const double z1w = (xp.z - P.wallP[w][2]) * P.wallN[w][2] * P.wallN[w][2];
+        V F, M;
+        wall_like_contact(P, dampCoeff, rp, vp, wp, xiw, x1w, y1w, z1w, F, M);
+        Fc = add(Fc, F); Mc = add(Mc, M);
+        wc[6 * w + 0] -= F.x; wc[6 * w + 1] -= F.y; wc[6 * w + 2] -= F.z;
+    }
+    st3(a.FContact, p, Fc); st3(a.MContact, p, Mc); st3(a.FLub, p, Fl); st3(a.MLub, p, Ml); st3(a.FRep, p, Fr);
+    st3(a.xiw, p, xiw); st3(a.xio, p, xio);
+}
+
+// deterministic column sums of the object and wall contributions (one block per column)
+__global__ void k_dem_reduce(const double* contrib, int nPart, int ncol, double* out) {
+    __shared__ double red[256];
+    const int c = blockIdx.x;
+    if (c >= ncol) return;
+    double acc = 0.0;
+    for (int p = threadIdx.x; p < nPart; p += blockDim.x) acc += contrib[int64_t(p) * ncol + c];
+    red[threadIdx.x] = acc;
+    __syncthreads();
+    for (int w = 128; w > 0; w >>= 1) {
+        if (threadIdx.x < w) red[threadIdx.x] += red[threadIdx.x + w];
+        __syncthreads();
+    }
+    if (threadIdx.x == 0) out[c] = red[0];
+}
+
+// objects: totals and integration (DEM.cpp:786-790, 2742-2758); walls: totals (DEM.cpp:773-775)
+__global__ void k_dem_objects(DemParams P, DemState d, const double* objSum, const double* wallSum, int demTimeStep) {
+#pragma clang fp contract(off)
+    if (threadIdx.x != 0 || blockIdx.x != 0) return;
+    for (int o = 0; o < P.nObj; ++o) {
+        const double* s = objSum + 12 * o;
+        const V Fc{s[0], s[1], s[2]}, Mc{s[3], s[4], s[5]}, Fl{s[6], s[7], s[8]}, Ml{s[9], s[10], s[11]};
+        st3(d.oFContact, o, Fc); st3(d.oMContact, o, Mc); st3(d.oFLub, o, Fl); st3(d.oMLub, o, Ml);
+        const V FH = ld3(d.oFHydro, o), MH = ld3(d.oMHydro, o), FG = ld3(d.oFGrav, o);
+        const V FT = add(add(add(add(FH, Fc), Fl), FG), V{0, 0, 0});
+        const V MT = add(add(MH, Mc), Ml);
+        st3(d.oFTotal, o, FT); st3(d.oMTotal, o, MT);
+        if (demTimeStep >= P.objWaitTime) {
+            const double dt = P.deltat, im = d.oim[o], iI = d.oiI[o];
+            double* x0 = d.ox0 + 3 * o; double* x1 = d.ox1 + 3 * o; double* w1 = d.ow1 + 3 * o;
+            const double F[3] = {FT.x, FT.y, FT.z}, M[3] = {MT.x, MT.y, MT.z};
+            for (int c = 0; c < 3; ++c) x0[c] += dt * x1[c] + dt * dt * F[c] * im;
+            for (int c = 0; c < 3; ++c) x1[c] += dt * F[c] * im;
+            for (int c = 0; c < 3; ++c) w1[c] += dt * M[c] * iI;
+        }
+        ObjLB ol;
+        for (int c = 0; c < 3; ++c) { ol.c[c] = d.ox0[3 * o + c] / P.unitLength; ol.u[c] = d.ox1[3 * o + c] / P.unitSpeed; }
+        ol.r = d.or_[o] / P.unitLength;
+        ol.pad = 0.0;
+        d.olb[o] = ol;
+    }
+    for (int w = 0; w < P.nWall; ++w) {
+        const double* s = wallSum + 6 * w;
+        const V Fc{s[0], s[1], s[2]}, Fl{s[3], s[4], s[5]};
+        st3(d.wFContact, w, Fc); st3(d.wFLub, w, Fl);
+        st3(d.wFTotal, w, add(add(add(ld3(d.wFHydro, w), Fc), Fl), V{0, 0, 0}));
+    }
+}
+
+// particles: totals, gravity switch-off, integration, periodic wrap, lattice view
+__global__ void k_dem_integrate(DemParams P, DemState d, int demTimeStep) {
+#pragma clang fp contract(off)
+    const int p = blockIdx.x * blockDim.x + threadIdx.x;
+    if (p >= P.nPart) return;
+    const V FT = add(add(add(add(ld3(d.FHydro, p), ld3(d.FContact, p)), ld3(d.FLub, p)), ld3(d.FGrav, p)), ld3(d.FRep, p));
+    const V MT = add(add(ld3(d.MHydro, p), ld3(d.MContact, p)), ld3(d.MLub, p));
+    st3(d.FTotal, p, FT); st3(d.MTotal, p, MT);
+    if (demTimeStep <= P.objWaitTime) st3(d.FGrav, p, V{0, 0, 0});
+    const double dt = P.deltat, im = d.im[p], iI = d.iI[p];
+    double* x0 = d.x0 + 3 * p; double* x1 = d.x1 + 3 * p; double* w1 = d.w1 + 3 * p;
+    const double F[3] = {FT.x, FT.y, FT.z}, M[3] = {MT.x, MT.y, MT.z};
+    for (int c = 0; c < 3; ++c) x0[c] += dt * x1[c] + dt * dt * F[c] * im;
+    for (int c = 0; c < 3; ++c) x1[c] += dt * F[c] * im;
+    for (int c = 0; c < 3; ++c) w1[c] += dt * M[c] * iI;
+    for (int c = 0; c < 3; ++c)
+        if (P.periodic[c]) {
+            if (x0[c] > P.demSize[c]) x0[c] -= P.demSize[c];
+            if (x0[c] < 0.0) x0[c] += P.demSize[c];
+        }
+    PartLB v;
+    for (int c = 0; c < 3; ++c) {
+        v.c[c] = x0[c] / P.unitLength;
+        v.v[c] = x1[c] / P.unitSpeed;
+        v.w[c] = w1[c] / P.unitAngVel;
+    }
+    v.r = d.r[p] / P.unitLength;
+    d.plb[p] = v;
+}
+
+__global__ void k_dem_views(DemParams P, DemState d) {
+    const int p = blockIdx.x * blockDim.x + threadIdx.x;
+    if (p < P.nPart) {
+        PartLB v;
+        for (int c = 0; c < 3; ++c) {
+            v.c[c] = d.x0[3 * p + c] / P.unitLength;
+            v.v[c] = d.x1[3 * p + c] / P.unitSpeed;
+            v.w[c] = d.w1[3 * p + c] / P.unitAngVel;
+        }
+        v.r = d.r[p] / P.unitLength;
+        d.plb[p] = v;
+    }
+    if (p < P.nObj) {
+        ObjLB ol;
+        for (int c = 0; c < 3; ++c) { ol.c[c] = d.ox0[3 * p + c] / P.unitLength; ol.u[c] = d.ox1[3 * p + c] / P.unitSpeed; }
+        ol.r = d.or_[p] / P.unitLength;
+        ol.pad = 0.0;
+        d.olb[p] = ol;
+    }
+}
+
+__global__ void k_zero(double* p, int n) {
+    for (int i = threadIdx.x; i < n; i += blockDim.x) p[i] = 0.0;
+}
+
+void dem_refresh_views(const DemParams& p, DemState& d, hipStream_t s) {
+    const int n = p.nPart > p.nObj ? p.nPart : p.nObj;
+    if (n > 0) hipLaunchKernelGGL(k_dem_views, (n + 255) / 256, 256, 0, s, p, d);
+}
+
+static NbrArgs nbr_args(const DemParams& p, DemState& d, DevScalars* sc) {
+    NbrArgs a;
+    a.p = p; a.x0 = d.x0; a.r = d.r;
+    a.cellCount = d.cellCount; a.cellStart = d.cellStart; a.cellFill = d.cellFill; a.cellItems = d.cellItems;
+    a.cellKey = d.cellKey; a.nbrCount = d.nbrCount; a.nbrList = d.nbrList; a.nbrSlot = d.nbrSlot;
+    a.oldCount = d.nbrCountOld; a.oldList = d.nbrListOld; a.xiCur = d.xiCur; a.xiTmp = d.xiTmp; a.sc = sc;
+    return a;
+}
+
+void dem_neighbor_list(const DemParams& p, DemState& d, DevScalars* sc, bool force, hipStream_t s) {
+    hipLaunchKernelGGL(k_maxdisp, 1, 256, 0, s, d.x1, p.nPart, p.deltat, sc, force ? 1 : 0);
+    if (p.nPart <= 0) return;
+    NbrArgs a = nbr_args(p, d, sc);
+    const int pb = (p.nPart + 127) / 128;
+    if (p.nlx > 0) {
+        const int ncell = p.nlx * p.nly * p.nlz;
+        hipLaunchKernelGGL(k_cells_clear, (ncell + 255) / 256, 256, 0, s, a, ncell);
+        hipLaunchKernelGGL(k_cells_assign, pb, 128, 0, s, a);
+        hipLaunchKernelGGL(k_cells_scan, 1, 1024, 0, s, a, ncell);
+        hipLaunchKernelGGL(k_cells_fill, pb, 128, 0, s, a);
+    }
+    hipLaunchKernelGGL(k_nbr_build, pb, 128, 0, s, a);
+    hipLaunchKernelGGL(k_nbr_slots, pb, 128, 0, s, a);
+    hipLaunchKernelGGL(k_nbr_commit, pb, 128, 0, s, a, d.xiCur, d.nbrCountOld, d.nbrListOld);
+}
+
+void dem_step(const DemParams& p, DemState& d, DevScalars* sc, int demTimeStep, hipStream_t s) {
+    if (p.nPart > 0) {
+        ForceArgs f;
+        f.p = p; f.x0 = d.x0; f.x1 = d.x1; f.w1 = d.w1; f.r = d.r;
+        f.FContact = d.FContact; f.MContact = d.MContact; f.FLub = d.FLub; f.MLub = d.MLub; f.FRep = d.FRep;
+        f.xiw = d.xiw; f.xio = d.xio; f.nbrCount = d.nbrCount; f.nbrList = d.nbrList; f.nbrSlot = d.nbrSlot;
+        f.xiOld = d.xiCur; f.xiNew = d.xiNext;
+        f.ox0 = d.ox0; f.ox1 = d.ox1; f.ow1 = d.ow1; f.or_ = d.or_;
+        f.objContrib = d.objContrib; f.wallContrib = d.wallContrib; f.demTimeStep = demTimeStep;
+        hipLaunchKernelGGL(k_dem_forces, (p.nPart + 63) / 64, 64, 0, s, f, p.dampCoeff);
+        if (p.nObj > 0) hipLaunchKernelGGL(k_dem_reduce, 12 * p.nObj, 256, 0, s, d.objContrib, p.nPart, 12 * p.nObj, d.objSum);
+        if (p.nWall > 0) hipLaunchKernelGGL(k_dem_reduce, 6 * p.nWall, 256, 0, s, d.wallContrib, p.nPart, 6 * p.nWall, d.wallSum);
+        double* t = d.xiCur; d.xiCur = d.xiNext; d.xiNext = t;
+    } else {
+        hipLaunchKernelGGL(k_zero, 1, 128, 0, s, d.objSum, 12 * MAX_OBJECTS);
+        hipLaunchKernelGGL(k_zero, 1, 128, 0, s, d.wallSum, 6 * MAX_WALLS);
+    }
+    hipLaunchKernelGGL(k_dem_objects, 1, 64, 0, s, p, d, d.objSum, d.wallSum, demTimeStep);
+    if (p.nPart > 0) hipLaunchKernelGGL(k_dem_integrate, (p.nPart + 127) / 128, 128, 0, s, p, d, demTimeStep);
+}
+
+}  // namespace lbdem

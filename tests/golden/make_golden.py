@@ -1,0 +1,60 @@
+"""Writes the golden fixtures in this directory from the reference program.
+
+Runs the reference compiled from its own sources (oracle/_ref/ref_driver, built by
+``make -C oracle ref`` in a checkout that has the reference at /root/reference) on the small
+cases of tests/cases.py, single-threaded, and stores the solver state after selected steps as
+compressed .npz files: node types / particle flags / solid indices, post-streaming populations
+of active nodes, density and mass fields, particle / object / wall records, Verlet lists and
+the total fluid mass. Only data is stored.
+
+usage: python tests/golden/make_golden.py
+"""
+from __future__ import annotations
+
+import json
+import os
+import subprocess
+import sys
+import tempfile
+
+import numpy as np
+
+HERE = os.path.dirname(os.path.abspath(__file__))
+sys.path.insert(0, os.path.dirname(HERE))
+import cases  # noqa: E402
+
+DRIVER = os.path.join(cases.REPO, "oracle", "_ref", "ref_driver")
+PLAN = {"lb_closed": [1, 12], "fs_impactor": [1, 40], "suspension": [1, 40, 120], "periodic": [40], "units": [40]}
+
+
+def main():
+    if not os.path.exists(DRIVER):
+        sys.exit(f"{DRIVER} missing: run `make -C oracle ref` where the reference sources are present")
+    manifest = {}
+    with tempfile.TemporaryDirectory() as tmp:
+        for name, steps in PLAN.items():
+            d = cases.make_case(tmp, name)
+            env = dict(os.environ, OMP_NUM_THREADS="1")
+            r = subprocess.run([DRIVER, "config.cfg", "out", str(max(steps)), "dump"] + [str(s) for s in steps],
+                               cwd=d, env=env, capture_output=True, text=True)
+            if r.returncode:
+                sys.exit(f"reference run failed for {name}:\n{r.stdout[-2000:]}\n{r.stderr[-2000:]}")
+            for s in steps:
+                dump = cases.read_dump(os.path.join(d, f"dump_{s}.bin"))
+                act = (dump["type"] == 0) | (dump["type"] == 3)
+                out = os.path.join(HERE, f"{name}_s{s}.npz")
+                np.savez_compressed(
+                    out, type=dump["type"], pflag=dump["pflag"], solid=dump["solid"], active=np.nonzero(act)[0],
+                    f_active=dump["f"][:, act], n_active=dump["n"][act], mass_active=dump["mass"][act],
+                    particles=dump["particles"], objects=dump["objects"], walls=dump["walls"],
+                    ncount=dump["ncount"], nlist=dump["nlist"], totalMass=dump["totalMass"],
+                    totalMassInit=dump["totalMassInit"], size=np.array(dump["size"]))
+                manifest[f"{name}_s{s}"] = dict(case=name, step=s, size=list(dump["size"]),
+                                                 active=int(act.sum()), bytes=os.path.getsize(out))
+    with open(os.path.join(HERE, "manifest.json"), "w") as fh:
+        json.dump(manifest, fh, indent=1, sort_keys=True)
+    print(json.dumps(manifest, indent=1))
+
+
+if __name__ == "__main__":
+    main()
