@@ -290,14 +290,19 @@ def parse_getpot(text: str) -> Dict[str, str]:
 
 
 def _num(v: str, like):
+    """Value of a getpot variable, typed like the reference default (strings stay strings)."""
     if isinstance(like, str):
         return v
+    s = v.strip().lower()
+    if s in ("true", "false"):
+        return 1 if s == "true" else 0
+    try:
+        x = float(s)
+    except ValueError:
+        return v
     if isinstance(like, int) and not isinstance(like, bool):
-        s = v.strip().lower()
-        if s in ("true", "false"):
-            return 1 if s == "true" else 0
-        return int(float(s))
-    return float(v)
+        return int(x)
+    return x
 
 
 def config_values(text: str, overrides: Optional[Dict[str, object]] = None) -> Dict[str, object]:

@@ -1,0 +1,126 @@
+"""MI355X library against the FP64 oracle on identical inputs, step by step, through the C ABI.
+
+Tolerances (absolute unless marked relative; populations are O(1e-1), masses O(1)):
+  FP64 storage  : node types / particle flags / solid indices / Verlet lists identical;
+                  populations 1e-13, mass and density 1e-11, particle / impactor records 1e-10 relative
+  FP32 offsets  : flags identical over the window; populations 2e-7, mass 5e-6,
+                  particle records 1e-4 relative, impactor 1e-6 relative
+  FP16 storage  : flags identical over the window; populations 5e-5, mass 2e-4,
+                  particle records 1e-3 relative
+The reference's own golden fixtures are also compared with the FP64 library directly."""
+import json
+import os
+
+import numpy as np
+import pytest
+
+import cases
+import lbdem
+
+pytestmark = pytest.mark.gpu
+
+TOL = {
+    lbdem.PREC_FP64: dict(f=1e-13, mass=1e-11, n=1e-11, part=1e-10, obj=1e-10, wall=1e-8),
+    lbdem.PREC_FP32: dict(f=2e-7, mass=5e-6, n=5e-6, part=1e-4, obj=1e-6, wall=1e-3),
+    lbdem.PREC_FP16: dict(f=5e-5, mass=2e-4, n=2e-4, part=1e-3, obj=1e-4, wall=1e-2),
+}
+
+
+def rel(a, b):
+    if a.size == 0:
+        return 0.0
+    return float(np.abs(a - b).max() / max(np.abs(a).max(), 1e-300))
+
+
+def compare(A, B, prec, where, free_surface=True):
+    t = TOL[prec]
+    for k in ("type", "pflag", "solid"):
+        bad = np.nonzero(A[k] != B[k])[0]
+        assert bad.size == 0, f"{where}: {k} differs at {bad[:8]}"
+    act = (A["type"] == 0) | (A["type"] == 3)
+    df = np.abs(A["f"][:, act] - B["f"][:, act]).max()
+    assert df <= t["f"], f"{where}: populations differ by {df}"
+    dm = np.abs(A["mass"][act] - B["mass"][act]).max()
+    assert dm <= t["mass"], f"{where}: mass differs by {dm}"
+    if free_surface:
+        dn = np.abs(A["n"][act] - B["n"][act]).max()
+        assert dn <= t["n"], f"{where}: density differs by {dn}"
+    assert rel(A["particles"], B["particles"]) <= t["part"], f"{where}: particle records"
+    assert rel(A["objects"], B["objects"]) <= t["obj"], f"{where}: impactor records"
+    assert rel(A["walls"], B["walls"]) <= t["wall"], f"{where}: wall records"
+    assert np.array_equal(A["ncount"], B["ncount"]), f"{where}: neighbour counts"
+    for p in range(len(A["ncount"])):
+        c = A["ncount"][p]
+        assert np.array_equal(A["nlist"][p, :c], B["nlist"][p, :c]), f"{where}: neighbours of {p}"
+
+
+def run_pair(oracle_lib, gpu_lib, case_root, name, prec, steps, every=1):
+    d = cases.make_case(case_root, name)
+    a = cases.open_sim(oracle_lib, d)
+    b = cases.open_sim(gpu_lib, d, precision=prec)
+    fs = name != "lb_closed"
+    print(f"case={name} prec={prec} steps={steps}", flush=True)
+    compare(cases.state(a), cases.state(b), prec, f"{name} init", fs)
+    done = 0
+    while done < steps:
+        n = min(every, steps - done)
+        a.cycle(n)
+        b.cycle(n)
+        b.synchronize()
+        done += n
+        compare(cases.state(a), cases.state(b), prec, f"{name} step {done}", fs)
+    a.close()
+    b.close()
+
+
+@pytest.mark.parametrize("name", cases.CASES)
+def test_fp64_matches_oracle(oracle_lib, gpu_lib, case_root, name):
+    run_pair(oracle_lib, gpu_lib, case_root, name, lbdem.PREC_FP64, 40)
+
+
+def test_fp64_long_window(oracle_lib, gpu_lib, case_root):
+    run_pair(oracle_lib, gpu_lib, case_root, "suspension", lbdem.PREC_FP64, 300, every=25)
+
+
+@pytest.mark.parametrize("name", ["suspension", "units", "periodic"])
+def test_fp32_offsets_within_tolerance(oracle_lib, gpu_lib, case_root, name):
+    run_pair(oracle_lib, gpu_lib, case_root, name, lbdem.PREC_FP32, 40, every=5)
+
+
+def test_fp16_storage_within_tolerance(oracle_lib, gpu_lib, case_root):
+    run_pair(oracle_lib, gpu_lib, case_root, "suspension", lbdem.PREC_FP16, 30, every=5)
+
+
+GOLD = os.path.join(os.path.dirname(os.path.abspath(__file__)), "golden")
+
+
+@pytest.mark.parametrize("name", ["suspension", "fs_impactor", "periodic", "units"])
+def test_fp64_matches_reference_fixtures(gpu_lib, case_root, name):
+    with open(os.path.join(GOLD, "manifest.json")) as fh:
+        steps = sorted(m["step"] for m in json.load(fh).values() if m["case"] == name)
+    d = cases.make_case(case_root, name)
+    sim = cases.open_sim(gpu_lib, d, precision=lbdem.PREC_FP64)
+    done = 0
+    for step in steps:
+        sim.cycle(step - done)
+        done = step
+        g = np.load(os.path.join(GOLD, f"{name}_s{step}.npz"), allow_pickle=False)
+        st = cases.state(sim)
+        assert np.array_equal(st["type"], g["type"]) and np.array_equal(st["pflag"], g["pflag"])
+        assert np.array_equal(st["solid"], g["solid"])
+        act = g["active"]
+        assert np.abs(st["f"][:, act] - g["f_active"]).max() <= 1e-13
+        assert np.abs(st["mass"][act] - g["mass_active"]).max() <= 1e-11
+        assert rel(g["particles"], st["particles"]) <= 1e-10
+        assert rel(g["objects"], st["objects"]) <= 1e-10
+        assert abs(st["totalMass"] - float(g["totalMass"])) <= 1e-11 * abs(float(g["totalMass"]))
+    sim.close()
+
+
+def test_device_errors_are_reported(gpu_lib, case_root):
+    """An unsupported configuration fails at create time with a message, before any launch."""
+    d = cases.make_case(case_root, "suspension")
+    vals, prm, ps, obs = lbdem.load_case(d)
+    prm.boundary[0] = 5
+    with pytest.raises(lbdem.LbdemError, match="boundary"):
+        lbdem.Simulation(gpu_lib, prm, ps, obs)
