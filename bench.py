@@ -1,0 +1,182 @@
+#!/usr/bin/env python3
+"""Benchmark of the coupled LB/DEM step on one MI355X.
+
+Workload (N=1): the headline impact run — a 256 x 256 x 320 closed box (x, z horizontal,
+y vertical), fluid to height 256 with a free surface, 31 000 spheres of radius 4 (phi ~ 0.5,
+seeded packing), an impactor of radius 16 at height 290 falling at 0.01 lattice units per
+step, tau = 0.8. A step is one full coupled step (DEM, coupling, fused LB collide-stream,
+free surface). Inputs are resident on the device when the timed region starts.
+
+Prints one JSON line: MLUPS over active (fluid + interface) cells for the FP32-offset step, the
+FP64 step on the same state for comparison, the fused kernel's HBM roofline fraction and a CPU
+baseline (the reference program compiled from its sources, on a bounded sample).
+"""
+import argparse
+import json
+import os
+import subprocess
+import sys
+import tempfile
+import time
+
+HERE = os.path.dirname(os.path.abspath(__file__))
+sys.path.insert(0, os.path.join(HERE, "lbm-dem_amd", "python"))
+
+import numpy as np  # noqa: E402
+
+import lbdem  # noqa: E402
+import workloads  # noqa: E402
+
+PEAK_HBM_GBS = 8000.0
+BYTES_PER_CELL = {lbdem.PREC_FP64: 38 * 8 + 1, lbdem.PREC_FP32: 38 * 4 + 1, lbdem.PREC_FP16: 38 * 2 + 1}
+PREC_NAME = {lbdem.PREC_FP64: "fp64", lbdem.PREC_FP32: "fp32", lbdem.PREC_FP16: "fp16"}
+
+
+def log(msg):
+    print(msg, file=sys.stderr, flush=True)
+
+
+def run_gpu(vals, ps, obs, prec, steps, warmup):
+    lib = lbdem.Library.load()
+    sim = lbdem.Simulation(lib, lbdem.params_from_values(vals, prec), ps, obs)
+    t0 = time.time()
+    sim.init()
+    sim.synchronize()
+    log(f"[bench] {PREC_NAME[prec]}: init {time.time() - t0:.2f} s")
+    sim.cycle(warmup)
+    sim.synchronize()
+    a0 = sim.info()["nActive"]
+    sim.profile(True)
+    sim.synchronize()
+    t0 = time.perf_counter()
+    sim.cycle(steps)
+    sim.synchronize()
+    t1 = time.perf_counter()
+    fused_ms, launches = sim.profile_read()
+    sim.profile(False)
+    inf = sim.info()
+    a1 = inf["nActive"]
+    mass = sim.total_mass()
+    obj = sim.objects()
+    sim.close()
+    wall = t1 - t0
+    active = 0.5 * (a0 + a1)
+    res = dict(prec=PREC_NAME[prec], steps=steps, seconds=wall, ms_per_step=1e3 * wall / steps,
+               active_cells=active, mlups_active=active * steps / wall / 1e6,
+               mlups_box=inf["totNodes"] * steps / wall / 1e6,
+               particle_steps_per_s=len(ps.r) * steps / wall,
+               fused_ms_avg=fused_ms / max(launches, 1), fused_launches=launches,
+               total_mass=mass, total_mass_init=inf["totalMassInit"],
+               impactor_y=float(obj[0, 1]) if len(obj) else None, builds=inf["neighborListBuilds"])
+    b = BYTES_PER_CELL[prec]
+    res["fused_achieved_gbs"] = active * b / (res["fused_ms_avg"] * 1e-3) / 1e9
+    log(f"[bench] {PREC_NAME[prec]}: {res['ms_per_step']:.3f} ms/step, fused {res['fused_ms_avg']:.3f} ms, "
+        f"{res['mlups_active']:.0f} MLUPS active, {res['fused_achieved_gbs']:.0f} GB/s")
+    return res
+
+
+def cpu_baseline(seconds_budget=25.0):
+    """The reference program (oracle/_ref/ref_driver) on a bounded sample of the workload: a
+    64 x 80 x 64 box with the same sphere radius and volume fraction, all host cores."""
+    drv = os.path.join(HERE, "oracle", "_ref", "ref_driver")
+    kind = "reference"
+    if not os.path.exists(drv):
+        return None
+    with tempfile.TemporaryDirectory() as tmp:
+        size = (64, 80, 64)
+        n = int(round(0.5 * (size[0] - 2) * (64 - 1) * (size[2] - 2) / (4.0 / 3.0 * np.pi * 64)))
+        vals, ps, obs = workloads.impact_box(size=size, fluid_height=64.0, n_spheres=n, r=4.0, r_imp=8.0,
+                                             y_imp=74.0, speed=0.01, seed=7, relax=300)
+        d = os.path.join(tmp, "case")
+        os.makedirs(os.path.join(d, "input"))
+        lbdem.write_getpot(os.path.join(d, "config.cfg"), vals)
+        lbdem.write_particles(os.path.join(d, "input", "particles.dat"), ps.r, ps.x0 + 0.05)
+        lbdem.write_objects(os.path.join(d, "input", "objects.dat"), obs.r, obs.x0, obs.x1)
+        with open(os.path.join(d, "input", "body.dat"), "w") as fh:
+            fh.write("0\n")
+        threads = os.cpu_count() or 1
+        env = dict(os.environ, OMP_NUM_THREADS=str(threads), LBDEM_WARMUP="3")
+        steps = 23
+        t0 = time.time()
+        r = subprocess.run([drv, "config.cfg", "out", str(steps), "dump"], cwd=d, env=env, capture_output=True,
+                           text=True, timeout=600)
+        if r.returncode:
+            log(f"[bench] cpu baseline failed: {r.stdout[-500:]} {r.stderr[-500:]}")
+            return None
+        line = [l for l in r.stdout.splitlines() if l.startswith("LBDEM_TIMING")][-1]
+        kv = dict(x.split("=") for x in line.split()[1:])
+        sec = float(kv["seconds"])
+        nst = int(kv["steps"])
+        active = int(kv["active"])
+        log(f"[bench] cpu baseline: {nst} steps in {sec:.2f} s ({time.time() - t0:.1f} s total), {active} active cells")
+        return {"value": active * nst / sec / 1e6, "unit": "MLUPS", "cores": int(kv["threads"]), "kind": kind,
+                "sample": f"reference program (g++ -O2 -fopenmp), 64x80x64 box, {n} spheres r=4 (phi~0.5), impactor r=8, "
+                          f"{nst} timed steps after 3 warm-up steps, {active} active cells"}
+
+
+def main():
+    ap = argparse.ArgumentParser()
+    ap.add_argument("--gpus", type=int, default=1)
+    ap.add_argument("--steps", type=int, default=200)
+    ap.add_argument("--warmup", type=int, default=20)
+    ap.add_argument("--precision", default="fp32", choices=["fp32", "fp64", "fp16"])
+    ap.add_argument("--no-fp64", action="store_true", help="skip the FP64 comparison run")
+    ap.add_argument("--no-cpu-baseline", action="store_true")
+    ap.add_argument("--small", action="store_true", help="quarter-size box (for quick checks)")
+    args = ap.parse_args()
+    if args.gpus != 1:
+        print(f"bench.py runs on one GPU only (asked for {args.gpus}); the coupled step has no multi-GPU path",
+              file=sys.stderr)
+        sys.exit(2)
+    prec = {"fp32": lbdem.PREC_FP32, "fp64": lbdem.PREC_FP64, "fp16": lbdem.PREC_FP16}[args.precision]
+    wl = dict(workloads.HEADLINE)
+    if args.small:
+        wl.update(size=(128, 160, 128), fluid_height=128.0, n_spheres=3800, r_imp=8.0, y_imp=145.0)
+    t0 = time.time()
+    vals, ps, obs = workloads.impact_box(**wl)
+    phi = workloads.volume_fraction(ps, wl["size"], wl["fluid_height"])
+    log(f"[bench] packing {len(ps.r)} spheres phi={phi:.3f} in {time.time() - t0:.1f} s")
+    main_res = run_gpu(vals, ps, obs, prec, args.steps, args.warmup)
+    fp64 = None
+    if not args.no_fp64 and prec != lbdem.PREC_FP64:
+        fp64 = run_gpu(vals, ps, obs, lbdem.PREC_FP64, max(20, args.steps // 4), args.warmup)
+    cpu = None if args.no_cpu_baseline else cpu_baseline()
+    b = BYTES_PER_CELL[prec]
+    ach = main_res["fused_achieved_gbs"]
+    out = {
+        "metric": "coupled MLUPS at 256²×320 φ=0.5 vs FP64 plain; p50/p99 served-run latency",
+        "value": round(main_res["mlups_active"], 1),
+        "unit": "MLUPS",
+        "n_gpus": 1,
+        "steps": args.steps,
+        "warmup": args.warmup,
+        "ms_per_step": round(main_res["ms_per_step"], 4),
+        "higher_is_better": True,
+        "scaling": "weak",
+        "vs_baseline": None,
+        "dtype": PREC_NAME[prec],
+        "data": "synthetic (seeded sphere packing and impactor, lattice units)",
+        "config": {
+            "workload": "impact run, box 256x256 horizontal x 320 vertical, fluid to 256, 31000 spheres r=4 "
+                        f"(phi={phi:.3f}), impactor r=16 at 290, vy=-0.01, tau=0.8, closed box"
+                        if not args.small else "quarter-size impact run (128x128x160, 3800 spheres)",
+            "precision": PREC_NAME[prec],
+            "active_cells": int(main_res["active_cells"]),
+            "mlups_box": round(main_res["mlups_box"], 1),
+            "particle_steps_per_s": round(main_res["particle_steps_per_s"], 1),
+            "fp64_plain_mlups": round(fp64["mlups_active"], 1) if fp64 else None,
+            "speedup_vs_fp64_plain": round(main_res["mlups_active"] / fp64["mlups_active"], 3) if fp64 else None,
+            "fused_kernel_ms": round(main_res["fused_ms_avg"], 4),
+            "neighbor_list_builds": main_res["builds"],
+            "total_mass_drift_rel": (main_res["total_mass"] - main_res["total_mass_init"]) / main_res["total_mass_init"],
+        },
+        "roofline": {"bound": "hbm", "achieved": round(ach, 1), "peak": PEAK_HBM_GBS, "unit": "GB/s",
+                     "frac": round(ach / PEAK_HBM_GBS, 4), "traffic": None,
+                     "bytes_per_active_cell": b},
+        "cpu_baseline": cpu,
+    }
+    print(json.dumps(out), flush=True)
+
+
+if __name__ == "__main__":
+    main()

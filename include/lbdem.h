@@ -142,6 +142,12 @@ int lbdem_get_neighbors(lbdem_ctx* ctx, int cap, int* count, int* list);
 /* Total fluid mass: sum of mass over active nodes not inside a particle (LB.cpp:388-395). */
 int lbdem_total_mass(lbdem_ctx* ctx, double* mass);
 
+/* Kernel timing (no reference counterpart; used by benchmarks): when enabled, HIP events on the
+ * library's stream bracket every fused collide-stream launch; lbdem_profile_read returns the
+ * summed duration in milliseconds and the number of launches since the last enable. */
+int lbdem_profile_enable(lbdem_ctx* ctx, int enable);
+int lbdem_profile_read(lbdem_ctx* ctx, double* fused_ms, int* fused_launches);
+
 #ifdef __cplusplus
 }
 #endif

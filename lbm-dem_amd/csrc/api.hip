@@ -59,6 +59,10 @@ struct lbdem_ctx {
     double* dTmp = nullptr;
     size_t dTmpBytes = 0;
     std::vector<void*> allocs;
+    // kernel timing
+    bool profiling = false;
+    std::vector<hipEvent_t> evPool;
+    size_t evUsed = 0;
     // host counters (goCycle, main.cpp:40-43; DEM.cpp:365-366)
     unsigned lbTime = 0, demTimeStep = 0, currentTimeStep = 0;
     double demTime = 0, realTime = 0, deltat = 1.0, totalMass = 0;
@@ -72,6 +76,7 @@ struct lbdem_ctx {
         return (T*)p;
     }
     ~lbdem_ctx() {
+        for (hipEvent_t e : evPool) (void)hipEventDestroy(e);
         for (void* p : allocs) hipFree(p);
         if (dTmp) hipFree(dTmp);
         if (stream) hipStreamDestroy(stream);
@@ -477,7 +482,16 @@ static void enqueue_lb(lbdem_ctx* c) {  // LB::latticeBolzmannStep (LB.cpp:400-4
         c->k.lbF[0] = c->k.lbF[1] = c->k.lbF[2] = 0.0;
         c->lbFReset = true;
     }
-    lb_fused_step(c->box, c->k, c->ds.plb, c->ds.olb, c->sc, true, c->stream);
+    hipEvent_t e0 = nullptr, e1 = nullptr;
+    if (c->profiling) {
+        while (c->evPool.size() < c->evUsed + 2) {
+            hipEvent_t e;
+            if (hipEventCreate(&e) != hipSuccess) break;
+            c->evPool.push_back(e);
+        }
+        if (c->evPool.size() >= c->evUsed + 2) { e0 = c->evPool[c->evUsed]; e1 = c->evPool[c->evUsed + 1]; c->evUsed += 2; }
+    }
+    lb_fused_step(c->box, c->k, c->ds.plb, c->ds.olb, c->sc, true, c->stream, e0, e1);
     lb_hydro_finalize(c->box, c->k, c->ds.FHydro, c->ds.MHydro, c->ds.fluidVol, c->sc, c->ds.oFHydro, c->ds.oMHydro,
                       c->ds.wFHydro, true, c->stream);
 }
@@ -707,5 +721,27 @@ extern "C" int lbdem_total_mass(lbdem_ctx* c, double* m) {
     double s = 0.0;
     for (double v : h) s += v;
     *m = s;
+    return 0;
+}
+
+extern "C" int lbdem_profile_enable(lbdem_ctx* c, int enable) {
+    NEED_INIT;
+    HIPCHK(hipStreamSynchronize(c->stream));
+    c->profiling = enable != 0;
+    c->evUsed = 0;
+    return 0;
+}
+
+extern "C" int lbdem_profile_read(lbdem_ctx* c, double* fused_ms, int* fused_launches) {
+    NEED_INIT;
+    HIPCHK(hipStreamSynchronize(c->stream));
+    double tot = 0.0;
+    for (size_t i = 0; i + 1 < c->evUsed; i += 2) {
+        float ms = 0.0f;
+        HIPCHK(hipEventElapsedTime(&ms, c->evPool[i], c->evPool[i + 1]));
+        tot += ms;
+    }
+    if (fused_ms) *fused_ms = tot;
+    if (fused_launches) *fused_launches = int(c->evUsed / 2);
     return 0;
 }

@@ -404,12 +404,14 @@ __global__ void k_begin_lb(DevScalars* sc, double* hacc, int nPart) {
 }
 
 void lb_fused_step(Box& b, const Consts& k, const PartLB* plb, const ObjLB* olb, DevScalars* sc, bool diag,
-                   hipStream_t s) {
+                   hipStream_t s, hipEvent_t ev0, hipEvent_t ev1) {
     const int n = k.nPart * 8 + 1;
     hipLaunchKernelGGL(k_begin_lb, (n + 255) / 256, 256, 0, s, sc, b.hacc, k.nPart);
+    if (ev0) (void)hipEventRecord(ev0, s);
     if (b.prec == LBDEM_PREC_FP64) launch_fused<double>(b, k, plb, olb, sc, diag, s);
     else if (b.prec == LBDEM_PREC_FP32) launch_fused<float>(b, k, plb, olb, sc, diag, s);
     else launch_fused<__half>(b, k, plb, olb, sc, diag, s);
+    if (ev1) (void)hipEventRecord(ev1, s);
 }
 
 // deterministic sum of the per-block partials, unit conversion of all hydrodynamic results

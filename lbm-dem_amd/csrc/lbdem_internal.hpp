@@ -173,7 +173,7 @@ struct Consts {
 // ---------------- launchers (lb.hip) ----------------
 void lb_init_fields(const Box& b, const Consts& k, hipStream_t s);
 void lb_fused_step(Box& b, const Consts& k, const PartLB* plb, const ObjLB* olb, DevScalars* sc, bool diag,
-                   hipStream_t s);
+                   hipStream_t s, hipEvent_t ev0 = nullptr, hipEvent_t ev1 = nullptr);
 void lb_hydro_finalize(const Box& b, const Consts& k, double* FHydro, double* MHydro, double* fluidVol,
                        DevScalars* sc, double* objFH, double* objMH, double* wallFH, bool diag, hipStream_t s);
 void lb_free_surface(Box& b, const Consts& k, DevScalars* sc, hipStream_t s);

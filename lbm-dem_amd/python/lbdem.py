@@ -114,6 +114,8 @@ class Library:
             "lbdem_get_walls": ([vp, _dp], C.c_int),
             "lbdem_get_neighbors": ([vp, C.c_int, _ip, _ip], C.c_int),
             "lbdem_total_mass": ([vp, _dp], C.c_int),
+            "lbdem_profile_enable": ([vp, C.c_int], C.c_int),
+            "lbdem_profile_read": ([vp, _dp, _ip], C.c_int),
         }
         for name, (args, res) in sig.items():
             fn = getattr(lib, name)
@@ -240,6 +242,15 @@ class Simulation:
         self._check(self.lib.lib.lbdem_get_neighbors(self.ctx, cap, _ptr(cnt, C.c_int), _ptr(lst, C.c_int)),
                     "lbdem_get_neighbors")
         return cnt[:n], lst[:n]
+
+    def profile(self, enable: bool = True):
+        self._check(self.lib.lib.lbdem_profile_enable(self.ctx, 1 if enable else 0), "lbdem_profile_enable")
+
+    def profile_read(self):
+        ms = C.c_double()
+        n = C.c_int()
+        self._check(self.lib.lib.lbdem_profile_read(self.ctx, C.byref(ms), C.byref(n)), "lbdem_profile_read")
+        return ms.value, n.value
 
     def total_mass(self) -> float:
         v = C.c_double()

@@ -19,6 +19,8 @@
 #include <string>
 #include <vector>
 
+#include <omp.h>
+
 #include "DEM.h"
 #include "IO.h"
 #include "LB.h"
@@ -149,9 +151,16 @@ int main(int argc, char** argv) {
     dem.demTimeStep = 0;
     io.outputStep(lb, dem);
     if (steps.count(0)) dump(prefix + "_0.bin", lb, dem, io);
+    const double t0 = omp_get_wtime();
+    double tWarm = t0;
+    const int warm = std::getenv("LBDEM_WARMUP") ? std::atoi(std::getenv("LBDEM_WARMUP")) : 0;
     for (int s = 1; s <= nsteps; ++s) {
         goCycle(io, dem, lb);
+        if (s == warm) tWarm = omp_get_wtime();
         if (steps.count(s)) dump(prefix + "_" + std::to_string(s) + ".bin", lb, dem, io);
     }
+    const double t1 = omp_get_wtime();
+    std::printf("LBDEM_TIMING steps=%d warmup=%d seconds=%.6f active=%zu threads=%d\n", nsteps - warm, warm, t1 - tWarm,
+                lb.activeNodes.size(), omp_get_max_threads());
     return 0;
 }

@@ -1416,6 +1416,13 @@ int lbdem_get_neighbors(lbdem_ctx* c, int cap, int* count, int* list) {
     }
     return 0;
 }
+int lbdem_profile_enable(lbdem_ctx* c, int) { NEED_INIT; return 0; }
+int lbdem_profile_read(lbdem_ctx* c, double* fused_ms, int* n) {
+    NEED_INIT;
+    if (fused_ms) *fused_ms = 0.0;
+    if (n) *n = 0;
+    return 0;
+}
 int lbdem_total_mass(lbdem_ctx* c, double* m) {
     NEED_INIT;
     double s = 0.0;
