@@ -300,6 +300,7 @@ extern "C" int lbdem_init(lbdem_ctx* c) {
     Geom g;
     g.X = c->lbSize[0]; g.Y = c->lbSize[1]; g.Z = c->lbSize[2];
     g.N = int64_t(g.X) * g.Y * g.Z;
+    if (g.N >= (int64_t(1) << 31) / Q) return c->fail(LBDEM_E_ARG, "box too large for 32-bit population indexing");
     for (int i = 0; i < 6; ++i) g.per[i] = c->prm.boundary[i] == 4;
     std::vector<uint8_t> t;
     std::vector<int32_t> s;
@@ -331,13 +332,14 @@ extern "C" int lbdem_init(lbdem_ctx* c) {
     }
     b.objCap = int(std::min<int64_t>(std::max<int64_t>(objCells, 1024), g.N));
     b.objList = c->dalloc<int32_t>(b.objCap);
+    b.objCand = c->dalloc<int32_t>(b.objCap);
     b.blockPartCap = fused_grid_blocks(g) * PART_STRIDE_;
     b.blockPart = c->dalloc<double>(b.blockPartCap);
     b.hacc = c->dalloc<double>(size_t(std::max(np, 1)) * 8);
     b.parity = 0;
     c->sc = c->dalloc<DevScalars>(1);
     if (!b.f || !b.type || !b.solid || !b.mass || !b.aux || !b.ifList || !b.ifList2 || !b.ifNewMass || !b.filled ||
-        !b.emptied || !b.newIf || !b.donor || !b.flagNew || !b.objList || !b.blockPart || !b.hacc || !c->sc)
+        !b.emptied || !b.newIf || !b.donor || !b.flagNew || !b.objList || !b.objCand || !b.blockPart || !b.hacc || !c->sc)
         return c->fail(LBDEM_E_NOMEM, "device allocation failed");
     hipStream_t st = c->stream;
     HIPCHK(hipMemsetAsync(b.f, 0, size_t(Q) * g.N * selt, st));
@@ -380,6 +382,7 @@ extern "C" int lbdem_init(lbdem_ctx* c) {
     d.wallContrib = c->dalloc<double>(size_t(std::max(np, 1)) * 6 * std::max(int(walls.size()), 1));
     d.objSum = c->dalloc<double>(12 * MAX_OBJECTS);
     d.wallSum = c->dalloc<double>(6 * MAX_WALLS);
+    d.blockMax = c->dalloc<unsigned long long>(64);
     const size_t o3 = 3 * size_t(std::max(no, 1));
     double** ovecs[] = {&d.ox0, &d.ox1, &d.ow1, &d.oFHydro, &d.oMHydro, &d.oFContact, &d.oMContact, &d.oFLub,
                         &d.oMLub, &d.oFTotal, &d.oMTotal, &d.oFGrav};
@@ -491,6 +494,7 @@ static void enqueue_lb(lbdem_ctx* c) {  // LB::latticeBolzmannStep (LB.cpp:400-4
         }
         if (c->evPool.size() >= c->evUsed + 2) { e0 = c->evPool[c->evUsed]; e1 = c->evPool[c->evUsed + 1]; c->evUsed += 2; }
     }
+    lb_mark_edges(c->box, c->stream);
     lb_fused_step(c->box, c->k, c->ds.plb, c->ds.olb, c->sc, true, c->stream, e0, e1);
     lb_hydro_finalize(c->box, c->k, c->ds.FHydro, c->ds.MHydro, c->ds.fluidVol, c->sc, c->ds.oFHydro, c->ds.oMHydro,
                       c->ds.wFHydro, true, c->stream);

@@ -10,6 +10,7 @@
 //   * integration and periodic wrap (DEM.cpp:2676-2836)
 #include "lbdem_internal.hpp"
 
+#include <algorithm>
 #include <climits>
 
 namespace lbdem {
@@ -29,10 +30,12 @@ __device__ inline V sub(V a, V b) { return V{a.x - b.x, a.y - b.y, a.z - b.z}; }
 // ------------------------------------------------------------------------------------------
 // neighbour-list trigger (DEM.cpp:366-378, 460-470)
 // ------------------------------------------------------------------------------------------
-__global__ void k_maxdisp(const double* x1, int n, double deltat, DevScalars* sc, int force) {
+// maximum |v|^2 over particles: per-block maxima into blockMax (non-negative doubles order like
+// their bit patterns), then the trigger update by one thread
+__global__ void k_maxdisp_partial(const double* x1, int n, unsigned long long* blockMax) {
     __shared__ double red[256];
     double m = 0.0;
-    for (int p = threadIdx.x; p < n; p += blockDim.x) {
+    for (int p = blockIdx.x * blockDim.x + threadIdx.x; p < n; p += gridDim.x * blockDim.x) {
         const double v2 = x1[3 * p] * x1[3 * p] + x1[3 * p + 1] * x1[3 * p + 1] + x1[3 * p + 2] * x1[3 * p + 2];
         if (v2 > m) m = v2;
     }
@@ -42,22 +45,26 @@ __global__ void k_maxdisp(const double* x1, int n, double deltat, DevScalars* sc
         if (threadIdx.x < w && red[threadIdx.x + w] > red[threadIdx.x]) red[threadIdx.x] = red[threadIdx.x + w];
         __syncthreads();
     }
-    if (threadIdx.x == 0) {
-        if (force) {
-            sc->rebuild = 1;
-            sc->builds += 1;
-            return;
-        }
-        const double maxVel = sqrt(red[0]);
-        sc->maxDisp += maxVel * deltat;
-        if (sc->maxDisp > 0.05) {
-            sc->maxDisp = 0.0;
-            sc->rebuild = 1;
-            sc->newNeighborList = 1;
-            sc->builds += 1;
-        } else {
-            sc->rebuild = 0;
-        }
+    if (threadIdx.x == 0) blockMax[blockIdx.x] = __double_as_longlong(red[0]);
+}
+__global__ void k_maxdisp(const unsigned long long* blockMax, int nb, double deltat, DevScalars* sc, int force) {
+    if (threadIdx.x != 0) return;
+    if (force) {
+        sc->rebuild = 1;
+        sc->builds += 1;
+        return;
+    }
+    double m = 0.0;
+    for (int b = 0; b < nb; ++b) { const double v = __longlong_as_double(blockMax[b]); if (v > m) m = v; }
+    const double maxVel = sqrt(m);
+    sc->maxDisp += maxVel * deltat;
+    if (sc->maxDisp > 0.05) {
+        sc->maxDisp = 0.0;
+        sc->rebuild = 1;
+        sc->newNeighborList = 1;
+        sc->builds += 1;
+    } else {
+        sc->rebuild = 0;
     }
 }
 
@@ -697,7 +704,9 @@ static NbrArgs nbr_args(const DemParams& p, DemState& d, DevScalars* sc) {
 }
 
 void dem_neighbor_list(const DemParams& p, DemState& d, DevScalars* sc, bool force, hipStream_t s) {
-    hipLaunchKernelGGL(k_maxdisp, 1, 256, 0, s, d.x1, p.nPart, p.deltat, sc, force ? 1 : 0);
+    const int nbm = p.nPart > 0 ? std::min(64, (p.nPart + 255) / 256) : 1;
+    hipLaunchKernelGGL(k_maxdisp_partial, nbm, 256, 0, s, d.x1, p.nPart, d.blockMax);
+    hipLaunchKernelGGL(k_maxdisp, 1, 64, 0, s, d.blockMax, nbm, p.deltat, sc, force ? 1 : 0);
     if (p.nPart <= 0) return;
     NbrArgs a = nbr_args(p, d, sc);
     const int pb = (p.nPart + 127) / 128;

@@ -153,6 +153,83 @@ __device__ inline C seg_sum(C v, int start, int lane) {
     return v;
 }
 
+// boundary link of the storing cell (LB.cpp:1390-1453): value reflected into the slot the cell
+// itself loads next step; momentum exchange into the block partials
+template <typename S, bool DIAG>
+__device__ inline typename Types<S>::C boundary_value(const FusedArgs<S>& a, int j, int32_t yj, uint8_t ty,
+                                                     typename Types<S>::C gj, typename Types<S>::C n,
+                                                     typename Types<S>::C sx, typename Types<S>::C sy,
+                                                     typename Types<S>::C sz, typename Types<S>::C usq, double mi,
+                                                     double* sPart) {
+    using C = typename Types<S>::C;
+    const Consts& k = a.k;
+    if (ty == T_GAS) {
+        const C cu = C(cxq(j)) * sx + C(cyq(j)) * sy + C(czq(j)) * sz;
+        return -gj + C(wq(j)) * (C(9) * (cu * cu) - C(3) * usq);
+    }
+    if (ty == T_WALL) {
+        if (DIAG && k.nWall > 0) {
+            int wi = a.solid[yj];
+            wi = (wi >= 0 && wi < k.nWall) ? wi : 0;
+            const double v2 = 2.0 * double(gj);
+            if (cxq(j)) atomicAdd(&sPart[7 * MAX_OBJ_PART + 3 * wi + 0], v2 * cxq(j));
+            if (cyq(j)) atomicAdd(&sPart[7 * MAX_OBJ_PART + 3 * wi + 1], v2 * cyq(j));
+            if (czq(j)) atomicAdd(&sPart[7 * MAX_OBJ_PART + 3 * wi + 2], v2 * czq(j));
+        }
+        return gj;
+    }
+    if (ty == T_OBJ) {
+        const int nObj = k.nObj;
+        int ov = 0;
+        if (nObj > 1) {
+            int yx, yy, yz;
+            cell_xyz(a.g, yj, yx, yy, yz);
+            for (int oo = 0; oo < nObj; ++oo)
+                if (inside_sphere(yx, yy, yz, a.olb[oo].c[0], a.olb[oo].c[1], a.olb[oo].c[2], a.olb[oo].r)) ov = oo;
+        }
+        const ObjLB& OV = a.olb[ov];
+        const double cuo = OV.u[0] * cxq(j) + OV.u[1] * cyq(j) + OV.u[2] * czq(j);
+        const C BBi = C(6.0) * n * C(wq(j)) * C(cuo);
+        const int oi = a.solid[yj];
+        if (oi >= 0 && oi < nObj && oi < MAX_OBJ_PART) {
+            int yx, yy, yz;
+            cell_xyz(a.g, yj, yx, yy, yz);
+            const double fm = 2.0 * double(gj) - double(BBi);
+            const double Fb[3] = {fm * cxq(j), fm * cyq(j), fm * czq(j)};
+            const double rr[3] = {yx - a.olb[oi].c[0], yy - a.olb[oi].c[1], yz - a.olb[oi].c[2]};
+            double* sp = &sPart[7 * oi];
+            atomicAdd(sp + 0, Fb[0]); atomicAdd(sp + 1, Fb[1]); atomicAdd(sp + 2, Fb[2]);
+            atomicAdd(sp + 3, rr[1] * Fb[2] - rr[2] * Fb[1]);
+            atomicAdd(sp + 4, rr[2] * Fb[0] - rr[0] * Fb[2]);
+            atomicAdd(sp + 5, rr[0] * Fb[1] - rr[1] * Fb[0]);
+        }
+        atomicAdd(&sPart[6], double(BBi) * mi);
+        return gj - BBi;
+    }
+    atomicOr(&a.sc->typeError, 1);
+    return gj;
+}
+
+// plain index offset of direction q (valid for cells without the edge bit)
+__device__ inline int32_t plain_off(int q, int32_t sy, int32_t sz) { return cxq(q) + cyq(q) * sy + czq(q) * sz; }
+
+template <typename S, int PAR>
+__device__ inline void load_cell(const FusedArgs<S>& a, int32_t i, uint8_t t, int x, int y, int z,
+                                 typename Types<S>::C* gq) {
+    const int32_t N = int32_t(a.g.N), sy = a.g.X, sz = a.g.X * a.g.Y;
+    if (PAR == 0) {
+#pragma unroll
+        for (int q = 0; q < Q; ++q) gq[q] = StoreT<S>::ld(a.f + q * N + i);
+    } else if (!(t & T_EDGE)) {
+#pragma unroll
+        for (int q = 0; q < Q; ++q) gq[q] = StoreT<S>::ld(a.f + oppq(q) * N + i + plain_off(oppq(q), sy, sz));
+    } else {
+        const Offs o(a.g, x, y, z);
+#pragma unroll
+        for (int q = 0; q < Q; ++q) gq[q] = StoreT<S>::ld(a.f + oppq(q) * N + i + int32_t(o.d(oppq(q))));
+    }
+}
+
 template <typename S, int PAR, bool FSURF, bool DIAG>
 __global__ __launch_bounds__(FX* FY) void k_fused(FusedArgs<S> a) {
     using C = typename Types<S>::C;
@@ -177,39 +254,54 @@ __global__ __launch_bounds__(FX* FY) void k_fused(FusedArgs<S> a) {
     const int bx = tile % a.nbx, by = (tile / a.nbx) % a.nby, bz = tile / (a.nbx * a.nby);
     const int x = bx * FX + threadIdx.x, y = by * FY + threadIdx.y;
     const Geom& g = a.g;
-    const int64_t N = g.N;
+    const int32_t N = int32_t(g.N), SY = g.X, SZ = g.X * g.Y;
     const int lane = __lane_id();
     const Consts& k = a.k;
-    const int nObj = k.nObj;
+    const bool col_ok = tile_ok && x < g.X && y < g.Y;
+    const int z0 = bz * FZ;
 
+    // prefetch of the first cell of the column
+    C gc[Q], gn[Q];
+    uint8_t tc = 0;
+    int32_t ic = 0;
+    bool actc = false;
+    if (col_ok && z0 < g.Z) {
+        ic = int32_t(cidx(g, x, y, z0));
+        tc = a.type[ic];
+        actc = is_active(tc);
+        if (actc) load_cell<S, PAR>(a, ic, tc, x, y, z0, gc);
+    }
     for (int zz = 0; zz < FZ; ++zz) {
-        const int z = bz * FZ + zz;
-        bool act = false;
-        int64_t i = 0;
-        uint8_t t = 0;
-        if (tile_ok && x < g.X && y < g.Y && z < g.Z) {
-            i = cidx(g, x, y, z);
-            t = a.type[i];
-            act = is_active(t);
+        const int z = z0 + zz;
+        // issue the loads of the next cell before this cell's stores
+        uint8_t tn = 0;
+        int32_t in = 0;
+        bool actn = false;
+        if (zz + 1 < FZ && col_ok && z + 1 < g.Z) {
+            in = ic + SZ;
+            tn = a.type[in];
+            actn = is_active(tn);
+            if (actn) load_cell<S, PAR>(a, in, tn, x, y, z + 1, gn);
         }
+        const bool act = actc;
+        const int32_t i = ic;
+        const uint8_t t = tc;
         int hkeyv = -1;
         C dfx = 0, dfy = 0, dfz = 0, dmx = 0, dmy = 0, dmz = 0, dvol = 0;
         if (act) {
-            const Offs o(g, x, y, z);
-            C gq[Q];
-#pragma unroll
-            for (int q = 0; q < Q; ++q) gq[q] = StoreT<S>::ld(a.f + load_slot(g, i, o, q, PAR));
             // reconstruct (node.cpp:75-93)
             C sum = 0, jx = 0, jy = 0, jz = 0;
 #pragma unroll
             for (int q = 0; q < Q; ++q) {
-                sum += gq[q];
-                jx += C(cxq(q)) * gq[q]; jy += C(cyq(q)) * gq[q]; jz += C(czq(q)) * gq[q];
+                sum += gc[q];
+                jx += C(cxq(q)) * gc[q]; jy += C(cyq(q)) * gc[q]; jz += C(czq(q)) * gc[q];
             }
             const C n = C(1) + sum;
             const C ux = jx / n, uy = jy / n, uz = jz / n;
             // interior forcing of particle nodes (LB.cpp:1919-1962)
             C hx = 0, hy = 0, hz = 0;
+            // mass is needed by particle nodes (LB.cpp:1935) and object links (LB.cpp:1453) only
+            const double mi = (t & (T_PMASK | T_EDGE)) ? double(a.mass[i]) : 0.0;
             if (t & T_PMASK) {
                 const int sp = a.solid[i];
                 const PartLB P = a.plb[sp];
@@ -218,12 +310,11 @@ __global__ __launch_bounds__(FX* FY) void k_fused(FusedArgs<S> a) {
                 const C lvx = C(P.v[0]) + (wy * rz - wz * ry);
                 const C lvy = C(P.v[1]) + (wz * rx - wx * rz);
                 const C lvz = C(P.v[2]) + (wx * ry - wy * rx);
-                const C mi = C(a.mass[i]);
-                const C lf = mi / n;
+                const C lf = C(mi) / n;
                 dfx = lf * (ux - lvx); dfy = lf * (uy - lvy); dfz = lf * (uz - lvz);
                 hx = -dfx; hy = -dfy; hz = -dfz;
                 dmx = ry * dfz - rz * dfy; dmy = rz * dfx - rx * dfz; dmz = rx * dfy - ry * dfx;
-                dvol = mi;
+                dvol = C(mi);
                 hkeyv = sp;
             }
             // collision (node.cpp:95-111, 181-207)
@@ -237,67 +328,29 @@ __global__ __launch_bounds__(FX* FY) void k_fused(FusedArgs<S> a) {
                 const C cu = C(cxq(q)) * sx + C(cyq(q)) * sy + C(czq(q)) * sz;
                 const C cF = C(cxq(q)) * Fx + C(cyq(q)) * Fy + C(czq(q)) * Fz;
                 const C geq = C(wq(q)) * ((n - C(1)) + n * (C(3) * cu + C(4.5) * cu * cu - C(1.5) * usq));
-                gq[q] = gq[q] + om * (geq - gq[q]) + omf * C(wq(q)) * (C(9) * cu * cF + C(3) * (cF - uF));
+                gc[q] = gc[q] + om * (geq - gc[q]) + omf * C(wq(q)) * (C(9) * cu * cF + C(3) * (cF - uF));
             }
-            // streaming with boundary conditions at the storing cell (LB.cpp:1375-1528)
-            StoreT<S>::st(a.f + i, gq[0]);
-            const M mi = a.mass[i];
+            // streaming (LB.cpp:1375-1528)
+            StoreT<S>::st(a.f + i, gc[0]);
+            if (!(t & T_EDGE)) {
 #pragma unroll
-            for (int j = 1; j < Q; ++j) {
-                const int64_t yj = i + o.d(j);
-                if (PAR == 0) StoreT<S>::st(a.f + int64_t(oppq(j)) * N + i, gq[j]);
-                else StoreT<S>::st(a.f + int64_t(j) * N + yj, gq[j]);
-                const uint8_t ty = a.type[yj] & T_BASE;
-                if (ty == T_FLUID || ty == T_IF) continue;
-                C bc;
-                if (ty == T_GAS) {
-                    const C cu = C(cxq(j)) * sx + C(cyq(j)) * sy + C(czq(j)) * sz;
-                    bc = -gq[j] + C(wq(j)) * (C(9) * (cu * cu) - C(3) * usq);
-                } else if (ty == T_WALL) {
-                    bc = gq[j];
-                    if (DIAG) {
-                        int wi = a.solid[yj];
-                        wi = (wi >= 0 && wi < k.nWall) ? wi : 0;
-                        if (wi < MAX_WALLS && k.nWall > 0) {
-                            const double v2 = 2.0 * double(gq[j]);
-                            if (cxq(j)) atomicAdd(&sPart[7 * MAX_OBJ_PART + 3 * wi + 0], v2 * cxq(j));
-                            if (cyq(j)) atomicAdd(&sPart[7 * MAX_OBJ_PART + 3 * wi + 1], v2 * cyq(j));
-                            if (czq(j)) atomicAdd(&sPart[7 * MAX_OBJ_PART + 3 * wi + 2], v2 * czq(j));
-                        }
-                    }
-                } else if (ty == T_OBJ) {
-                    // object node velocity: the object whose sphere holds the node
-                    int ov = 0;
-                    if (nObj > 1) {
-                        int yx, yy, yz;
-                        cell_xyz(g, yj, yx, yy, yz);
-                        for (int oo = 0; oo < nObj; ++oo)
-                            if (inside_sphere(yx, yy, yz, a.olb[oo].c[0], a.olb[oo].c[1], a.olb[oo].c[2], a.olb[oo].r)) ov = oo;
-                    }
-                    const ObjLB& OV = a.olb[ov];
-                    const double cuo = OV.u[0] * cxq(j) + OV.u[1] * cyq(j) + OV.u[2] * czq(j);
-                    const C BBi = C(6.0) * n * C(wq(j)) * C(cuo);
-                    bc = gq[j] - BBi;
-                    const int oi = a.solid[yj];
-                    if (oi >= 0 && oi < nObj && oi < MAX_OBJ_PART) {
-                        int yx, yy, yz;
-                        cell_xyz(g, yj, yx, yy, yz);
-                        const double fm = 2.0 * double(gq[j]) - double(BBi);
-                        const double Fb[3] = {fm * cxq(j), fm * cyq(j), fm * czq(j)};
-                        const double rr[3] = {yx - a.olb[oi].c[0], yy - a.olb[oi].c[1], yz - a.olb[oi].c[2]};
-                        double* sp = &sPart[7 * oi];
-                        atomicAdd(sp + 0, Fb[0]); atomicAdd(sp + 1, Fb[1]); atomicAdd(sp + 2, Fb[2]);
-                        atomicAdd(sp + 3, rr[1] * Fb[2] - rr[2] * Fb[1]);
-                        atomicAdd(sp + 4, rr[2] * Fb[0] - rr[0] * Fb[2]);
-                        atomicAdd(sp + 5, rr[0] * Fb[1] - rr[1] * Fb[0]);
-                    }
-                    atomicAdd(&sPart[6], double(BBi) * double(mi));
-                } else {
-                    atomicOr(&a.sc->typeError, 1);
-                    bc = gq[j];
+                for (int j = 1; j < Q; ++j) {
+                    if (PAR == 0) StoreT<S>::st(a.f + oppq(j) * N + i, gc[j]);
+                    else StoreT<S>::st(a.f + j * N + i + plain_off(j, SY, SZ), gc[j]);
                 }
-                if (PAR == 0) StoreT<S>::st(a.f + int64_t(j) * N + yj, bc);
-                else StoreT<S>::st(a.f + int64_t(oppq(j)) * N + i, bc);
+            } else {
+                const Offs o(g, x, y, z);
+#pragma unroll
+                for (int j = 1; j < Q; ++j) {
+                    const int32_t yj = i + int32_t(o.d(j));
+                    if (PAR == 0) StoreT<S>::st(a.f + oppq(j) * N + i, gc[j]);
+                    else StoreT<S>::st(a.f + j * N + yj, gc[j]);
+                    const uint8_t ty = a.type[yj] & T_BASE;
+                    if (ty == T_FLUID || ty == T_IF) continue;
+                    const C bc = boundary_value<S, DIAG>(a, j, yj, ty, gc[j], n, sx, sy, sz, usq, mi, sPart);
+                    if (PAR == 0) StoreT<S>::st(a.f + j * N + yj, bc);
+                    else StoreT<S>::st(a.f + oppq(j) * N + i, bc);
+                }
             }
             if (FSURF) {
                 if ((t & T_BASE) == T_FLUID) {
@@ -320,7 +373,7 @@ __global__ __launch_bounds__(FX* FY) void k_fused(FusedArgs<S> a) {
                 base = __shfl(base, leader, 64);
                 if (isIf) {
                     const int pos = base + __popcll(m & ((1ull << lane) - 1ull));
-                    if (pos < a.ifCap) a.ifList[pos] = int32_t(i);
+                    if (pos < a.ifCap) a.ifList[pos] = i;
                     else atomicOr(&a.sc->listOverflow, 1);
                 }
             }
@@ -355,6 +408,12 @@ __global__ __launch_bounds__(FX* FY) void k_fused(FusedArgs<S> a) {
                 }
             }
         }
+        // advance the column
+        actc = actn;
+        ic = in;
+        tc = tn;
+#pragma unroll
+        for (int q = 0; q < Q; ++q) gc[q] = gn[q];
     }
     __syncthreads();
     for (int s = tid; s < HS; s += FX * FY) {
@@ -365,6 +424,30 @@ __global__ __launch_bounds__(FX* FY) void k_fused(FusedArgs<S> a) {
         }
     }
     if (tid < PART_STRIDE) a.blockPart[int64_t(tid) * gridDim.x + b0] = sPart[tid];
+}
+
+// edge bit of active nodes: a neighbour (wrapped) is not active, or the node sits next to a
+// periodic face (its links wrap)
+__global__ void k_mark_edges(Geom g, uint8_t* type) {
+    const int32_t i = int32_t(blockIdx.x) * blockDim.x + threadIdx.x;
+    if (i >= int32_t(g.N)) return;
+    const uint8_t t = type[i];
+    if (!is_active(t)) return;
+    int x, y, z;
+    cell_xyz(g, i, x, y, z);
+    bool edge = (x == 1 && g.per[0]) || (x == g.X - 2 && g.per[1]) || (y == 1 && g.per[2]) ||
+                (y == g.Y - 2 && g.per[3]) || (z == 1 && g.per[4]) || (z == g.Z - 2 && g.per[5]);
+    if (!edge) {
+        const int32_t sy = g.X, sz = g.X * g.Y;
+#pragma unroll
+        for (int q = 1; q < Q; ++q) edge |= !is_active(type[i + plain_off(q, sy, sz)]);
+    }
+    const uint8_t nt = edge ? (t | T_EDGE) : (t & uint8_t(~T_EDGE));
+    if (nt != t) type[i] = nt;
+}
+
+void lb_mark_edges(const Box& b, hipStream_t s) {
+    hipLaunchKernelGGL(k_mark_edges, int((b.g.N + 255) / 256), 256, 0, s, b.g, b.type);
 }
 
 template <typename S>
@@ -796,18 +879,25 @@ void lb_flags_rebuild_compute(const Box& b, const Consts& k, const PartLB* plb, 
 // clear every flag and solid index, apply the new flags (initializeParticleBoundaries)
 __global__ void k_flags_apply(int64_t N, uint8_t* type, int32_t* solid, int32_t* flagNew, DevScalars* sc) {
     if (!sc->newNeighborList) return;
+    __shared__ int bmin;
+    if (threadIdx.x == 0) bmin = INT_MAX;
+    __syncthreads();
+    int mymin = INT_MAX;
     for (int64_t i = int64_t(blockIdx.x) * blockDim.x + threadIdx.x; i < N; i += int64_t(gridDim.x) * blockDim.x) {
         const int32_t s = flagNew[i];
         if (s >= 0) {
             type[i] = (type[i] & T_BASE) | T_PMASK;
             solid[i] = s;
             flagNew[i] = -1;
-            atomicMin(&sc->kminP, int(i));
+            mymin = min(mymin, int(i));
         } else {
             type[i] &= T_BASE;
             solid[i] = 0;
         }
     }
+    if (mymin != INT_MAX) atomicMin(&bmin, mymin);
+    __syncthreads();
+    if (threadIdx.x == 0 && bmin != INT_MAX) atomicMin(&sc->kminP, bmin);
 }
 
 __global__ void k_flags_apply_begin(DevScalars* sc) {
@@ -878,21 +968,54 @@ void lb_find_new_active(const Box& b, const Consts& k, const PartLB* plb, DevSca
 // ------------------------------------------------------------------------------------------
 // impactor nodes (updateObjectBoundaries + findNewActiveObject, LB.cpp:797-943)
 // ------------------------------------------------------------------------------------------
-// uncovered object nodes, in ascending index order, by one thread (each decision reads the
-// types its predecessors wrote)
+// uncovered object nodes (LB.cpp:846-943). All threads of one block test the previous object
+// nodes (ascending list) against the moved sphere and compact the uncovered ones in ascending
+// order; one thread then converts them in that order, because each decision reads the node
+// types its predecessors wrote.
+constexpr int UNCOVER_THREADS = 1024;
 template <typename S, int L>
-__global__ void k_obj_uncover(Geom g, S* f, uint8_t* type, typename Types<S>::M* mass, typename Types<S>::C* aux,
-                              const int32_t* objList, DevScalars* sc, const ObjLB* olb, Consts k) {
+__global__ __launch_bounds__(UNCOVER_THREADS) void k_obj_uncover(Geom g, S* f, uint8_t* type,
+                                                                 typename Types<S>::M* mass,
+                                                                 typename Types<S>::C* aux, const int32_t* objList,
+                                                                 int32_t* cand, DevScalars* sc, const ObjLB* olb,
+                                                                 Consts k) {
     using C = typename Types<S>::C;
-    if (threadIdx.x != 0 || blockIdx.x != 0) return;
+    __shared__ int counts[UNCOVER_THREADS];
+    __shared__ int total;
     const int n = sc->nObjList;
     const ObjLB O = olb[0];
-    volatile uint8_t* vt = type;
-    for (int kk = 0; kk < n; ++kk) {
-        const int64_t i = objList[kk];
+    const int per = (n + blockDim.x - 1) / blockDim.x;
+    const int beg = threadIdx.x * per, end = min(n, beg + per);
+    int cnt = 0;
+    for (int kk = beg; kk < end; ++kk) {
+        int x, y, z;
+        cell_xyz(g, objList[kk], x, y, z);
+        cnt += !inside_sphere(x, y, z, O.c[0], O.c[1], O.c[2], O.r);
+    }
+    counts[threadIdx.x] = cnt;
+    __syncthreads();
+    if (threadIdx.x == 0) {
+        int acc = 0;
+        for (int t = 0; t < int(blockDim.x); ++t) { const int c = counts[t]; counts[t] = acc; acc += c; }
+        total = acc;
+    }
+    __syncthreads();
+    int pos = counts[threadIdx.x];
+    for (int kk = beg; kk < end; ++kk) {
+        const int32_t i = objList[kk];
         int x, y, z;
         cell_xyz(g, i, x, y, z);
-        if (inside_sphere(x, y, z, O.c[0], O.c[1], O.c[2], O.r)) continue;
+        if (!inside_sphere(x, y, z, O.c[0], O.c[1], O.c[2], O.r)) cand[pos++] = i;
+    }
+    __threadfence_block();
+    __syncthreads();
+    if (threadIdx.x != 0) return;
+    volatile uint8_t* vt = type;
+    const int m = total;
+    for (int kk = 0; kk < m; ++kk) {
+        const int64_t i = cand[kk];
+        int x, y, z;
+        cell_xyz(g, i, x, y, z);
         bool anyFluid = false;
         for (int j = 1; j < Q; ++j) anyFluid |= (vt[nbr_plain(g, x, y, z, j)] & T_BASE) == T_FLUID;
         const uint8_t pbit = vt[i] & T_PMASK;
@@ -976,9 +1099,11 @@ static void obj_impl(Box& b, const Consts& k, const ObjLB* olb, DevScalars* sc, 
     using M = typename Types<S>::M;
     using C = typename Types<S>::C;
     if (b.parity == 0)
-        hipLaunchKernelGGL((k_obj_uncover<S, 0>), 1, 64, 0, s, b.g, (S*)b.f, b.type, (M*)b.mass, (C*)b.aux, b.objList, sc, olb, k);
+        hipLaunchKernelGGL((k_obj_uncover<S, 0>), 1, UNCOVER_THREADS, 0, s, b.g, (S*)b.f, b.type, (M*)b.mass, (C*)b.aux,
+                           b.objList, b.objCand, sc, olb, k);
     else
-        hipLaunchKernelGGL((k_obj_uncover<S, 1>), 1, 64, 0, s, b.g, (S*)b.f, b.type, (M*)b.mass, (C*)b.aux, b.objList, sc, olb, k);
+        hipLaunchKernelGGL((k_obj_uncover<S, 1>), 1, UNCOVER_THREADS, 0, s, b.g, (S*)b.f, b.type, (M*)b.mass, (C*)b.aux,
+                           b.objList, b.objCand, sc, olb, k);
     hipLaunchKernelGGL(k_obj_cover, 1, 1024, 0, s, b.g, b.type, b.solid, olb, k.nObj, b.objList, b.objCap, sc);
 }
 

@@ -47,7 +47,10 @@ __host__ __device__ constexpr int oppq(int q) {
 }
 __host__ __device__ constexpr double wq(int q) { return q == 0 ? 12.0 / 36.0 : (q < 7 ? 2.0 / 36.0 : 1.0 / 36.0); }
 
-enum : uint8_t { T_FLUID = 0, T_GAS = 2, T_IF = 3, T_PER = 4, T_WALL = 7, T_OBJ = 12, T_PMASK = 0x80, T_BASE = 0x7F };
+// type byte: bits 0-5 node type, bit 6 'edge' (some neighbour is not active or the node wraps
+// periodically; refreshed before every lattice step), bit 7 inside-particle flag
+enum : uint8_t { T_FLUID = 0, T_GAS = 2, T_IF = 3, T_PER = 4, T_WALL = 7, T_OBJ = 12, T_EDGE = 0x40, T_PMASK = 0x80,
+                 T_BASE = 0x3F };
 
 // Geometry of one box: sizes and which faces wrap (reference LB.cpp:85-126, 632-678:
 // an active node wraps along an axis when its face neighbour on that side is periodic).
@@ -151,6 +154,7 @@ struct Box {
     int32_t* donor;            // N, -1 = none
     int32_t* flagNew;          // N, -1 = none (particle flags computed at a rebuild)
     int32_t* objList;          // object nodes, ascending
+    int32_t* objCand;          // uncovered object nodes of the current step, ascending
     int ifCap, objCap;
     double* blockPart;         // per-block partials of the fused kernel (object F/M, extra mass, walls)
     int blockPartCap;
@@ -172,6 +176,7 @@ struct Consts {
 
 // ---------------- launchers (lb.hip) ----------------
 void lb_init_fields(const Box& b, const Consts& k, hipStream_t s);
+void lb_mark_edges(const Box& b, hipStream_t s);
 void lb_fused_step(Box& b, const Consts& k, const PartLB* plb, const ObjLB* olb, DevScalars* sc, bool diag,
                    hipStream_t s, hipEvent_t ev0 = nullptr, hipEvent_t ev1 = nullptr);
 void lb_hydro_finalize(const Box& b, const Consts& k, double* FHydro, double* MHydro, double* fluidVol,
@@ -225,6 +230,7 @@ struct DemState {
     double* wallContrib;       // per particle, per wall: Fc(3) Fl(3)
     double* objSum;            // 12*MAX_OBJECTS
     double* wallSum;           // 6*MAX_WALLS
+    unsigned long long* blockMax;  // per-block maxima of |v|^2
     // objects
     double *ox0, *ox1, *ow1, *or_, *om, *oI, *oim, *oiI;
     double *oFHydro, *oMHydro, *oFContact, *oMContact, *oFLub, *oMLub, *oFTotal, *oMTotal, *oFGrav;

@@ -6,7 +6,9 @@
   * fluid up to ``fluid_height`` with a free surface;
   * ``n_spheres`` spheres of radius ``r`` from a seeded random-sequential-addition + growth /
     relaxation packing below the surface (lbm-dem_amd/host/packing.cpp);
-  * one impactor of radius ``r_imp`` at (x/2, y_imp, z/2) moving with vy = -speed, density 3.
+  * one impactor of radius ``r_imp`` at (x/2, y_imp, z/2) moving with vy = -speed, density 3;
+  * sphere density 2.5 (the reference's explicit interior forcing, LB.cpp:1901-1978 used one
+    step late by the DEM, amplifies velocity differences when spheres are as dense as the fluid).
 DEM parameters in lattice units (dt = 1): linearStiff 2.0 (contact time sqrt(m/k) ~ 12 steps for
 r = 4), friction 0.3, lubrication with kinVisc = 0.1 (the fluid's), lubNormal 1.0,
 lubTangential 0.5, lubRotation 0.4, lubCutoff 0.2, electrostatic repulsion 0.01 with Debye
@@ -48,7 +50,7 @@ def impact_box(size=(256, 320, 256), fluid_height=256.0, n_spheres=31000, r=4.0,
         "lbSizeX": float(X), "lbSizeY": float(Y), "lbSizeZ": float(Z),
         "rheologyModel": "NEWTONIAN", "initVisc": 0.1, "lbFY": -1.0e-5, "fluidHeight": float(fluid_height),
         "boundary0": 7, "boundary1": 7, "boundary2": 7, "boundary3": 7, "boundary4": 7, "boundary5": 7,
-        "density": 1.0, "objectDensity": 3.0, "bodyDensity": 12.0, "linearStiff": 2.0, "frictionCoefPart": 0.3,
+        "density": 2.5, "objectDensity": 3.0, "bodyDensity": 12.0, "linearStiff": 2.0, "frictionCoefPart": 0.3,
         "lubNormal": 1.0, "lubTangential": 0.5, "lubRotation": 0.4, "lubCutoff": 0.2, "kinVisc": 0.1,
         "electroForce": 0.01, "debyeLength": 0.5, "objWaitTime": 0,
     })

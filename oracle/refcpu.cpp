@@ -14,7 +14,11 @@
 //  * object-particle tangential damping dampCoeffTanw = dampCoeff if kinVisc==0 else 0
 //    (DEM.cpp:916 reads it uninitialised; rule taken from DEM.cpp:2015-2020).
 //  * wall bounce-back force goes to walls[solidIndex] when solidIndex < nWalls, else walls[0]
-//    (LB.cpp:1425-1429 indexes walls[] with a particle index after findNewSolid floods a wall node).
+//    (LB.cpp:1425-1429 would index walls[] with a particle index).
+//  * object bounce-back force and torque are dropped when the object node's solidIndex is not an
+//    object index (LB.cpp:1441-1449 indexes objects[] with the particle index that
+//    updateIndices, LB.cpp:1875-1899, gives an object node that was active on the previous list);
+//    the extra mass of those links is kept (it does not depend on the index).
 //  * a particle-flagged node that has no node object (gas) takes the active-node neighbour
 //    rule in findNewSolid (LB.cpp:2046 dereferences a null node there).
 //  * neighbour capacity is 63 partners per particle (reference: 49, unchecked, elmt.h:44-51);
@@ -606,9 +610,11 @@ struct lbdem_ctx {
                     const V3 vel = U(l);
                     const double BBi = 6.0 * n[it] * coeff(j) * vel.dot(vdir(j));
                     const V3 BBforce = (2.0 * (fsi[j] - staticPres[j]) - BBi) * vdir(j);
-                    const V3 rad = getPosition(l) - objects[si].x0 / unit.Length;
-                    objects[si].FHydro += BBforce;
-                    objects[si].MHydro += rad.cross(BBforce);
+                    if (si < objects.size()) {  // pinned: see header
+                        const V3 rad = getPosition(l) - objects[si].x0 / unit.Length;
+                        objects[si].FHydro += BBforce;
+                        objects[si].MHydro += rad.cross(BBforce);
+                    }
                     extraMass += BBi * mass[it];
                 }
             }
@@ -795,6 +801,11 @@ struct lbdem_ctx {
         } else {
             const int ncell = nlx * nly * nlz;
             std::vector<std::vector<int>> cell(ncell);
+            for (auto& P : particles)
+                if (!std::isfinite(P.x0.x) || !std::isfinite(P.x0.y) || !std::isfinite(P.x0.z)) {
+                    err = "particle position is not finite";
+                    return LBDEM_E_STATE;
+                }
             for (int n1 = 0; n1 < num; n1++) {
                 Particle& P = particles[n1];
                 const int lx = int(dbox(P.x0.x / demSize[0], 1) * nlx);
