@@ -77,13 +77,13 @@ struct lbdem_ctx {
     }
     ~lbdem_ctx() {
         for (hipEvent_t e : evPool) (void)hipEventDestroy(e);
-        for (void* p : allocs) hipFree(p);
-        if (dTmp) hipFree(dTmp);
-        if (stream) hipStreamDestroy(stream);
+        for (void* p : allocs) (void)hipFree(p);
+        if (dTmp) (void)hipFree(dTmp);
+        if (stream) (void)hipStreamDestroy(stream);
     }
     double* tmp(size_t bytes) {
         if (bytes > dTmpBytes) {
-            if (dTmp) hipFree(dTmp);
+            if (dTmp) (void)hipFree(dTmp);
             dTmp = nullptr;
             if (hipMalloc(&dTmp, bytes) != hipSuccess) { dTmpBytes = 0; return nullptr; }
             dTmpBytes = bytes;
@@ -143,7 +143,7 @@ extern "C" int lbdem_create(const lbdem_params* params, lbdem_ctx** out) {
 
 extern "C" int lbdem_destroy(lbdem_ctx* ctx) {
     if (ctx) {
-        if (ctx->stream) hipStreamSynchronize(ctx->stream);
+        if (ctx->stream) (void)hipStreamSynchronize(ctx->stream);
         delete ctx;
     }
     return 0;
@@ -228,6 +228,13 @@ static void host_types(lbdem_ctx* c, const Geom& g, std::vector<uint8_t>& t, std
                 for (int q = 1; q < Q; ++q)
                     if (t[nbr_wrapped(g, x, y, z, q)] == T_GAS) { t[i] = T_IF; break; }
             }
+    // wall nodes carry the wall that receives their link forces (LB.cpp:1416-1420)
+    const int nw = int(walls.size());
+    for (int64_t i = 0; i < g.N; ++i)
+        if (t[i] == T_WALL) {
+            const int w = (s[i] >= 0 && s[i] < nw) ? s[i] : 0;
+            t[i] = uint8_t(T_WALL | (w << T_WSHIFT));
+        }
 }
 
 extern "C" int lbdem_init(lbdem_ctx* c) {
@@ -333,13 +340,16 @@ extern "C" int lbdem_init(lbdem_ctx* c) {
     b.objCap = int(std::min<int64_t>(std::max<int64_t>(objCells, 1024), g.N));
     b.objList = c->dalloc<int32_t>(b.objCap);
     b.objCand = c->dalloc<int32_t>(b.objCap);
+    b.coverCount = c->dalloc<int>(256);
     b.blockPartCap = fused_grid_blocks(g) * PART_STRIDE_;
     b.blockPart = c->dalloc<double>(b.blockPartCap);
+    b.blockFlag = c->dalloc<int32_t>(fused_grid_blocks(g));
+    b.bcm = c->dalloc<uint64_t>(g.N);
     b.hacc = c->dalloc<double>(size_t(std::max(np, 1)) * 8);
     b.parity = 0;
     c->sc = c->dalloc<DevScalars>(1);
     if (!b.f || !b.type || !b.solid || !b.mass || !b.aux || !b.ifList || !b.ifList2 || !b.ifNewMass || !b.filled ||
-        !b.emptied || !b.newIf || !b.donor || !b.flagNew || !b.objList || !b.objCand || !b.blockPart || !b.hacc || !c->sc)
+        !b.emptied || !b.newIf || !b.donor || !b.flagNew || !b.objList || !b.objCand || !b.coverCount || !b.blockFlag || !b.bcm || !b.blockPart || !b.hacc || !c->sc)
         return c->fail(LBDEM_E_NOMEM, "device allocation failed");
     hipStream_t st = c->stream;
     HIPCHK(hipMemsetAsync(b.f, 0, size_t(Q) * g.N * selt, st));

@@ -34,20 +34,45 @@ __device__ inline void cell_xyz(const Geom& g, int64_t i, int& x, int& y, int& z
 }
 
 // Per-axis index offsets of the wrapped neighbour rule: nb(q) = i + ox[cx+1] + oy[cy+1] + oz[cz+1]
+// (offsets fit 32 bits: initialisation rejects boxes with 19*N >= 2^31)
 struct Offs {
-    int64_t ox[3], oy[3], oz[3];
+    int32_t ox[3], oy[3], oz[3];
     __device__ Offs(const Geom& g, int x, int y, int z) {
-        const int64_t sy = g.X, sz = int64_t(g.X) * g.Y;
+        const int32_t sy = g.X, sz = g.X * g.Y;
         ox[1] = 0; oy[1] = 0; oz[1] = 0;
-        ox[0] = (x - 1 == 0 && g.per[0]) ? int64_t(g.X - 3) : -1;
-        ox[2] = (x + 1 == g.X - 1 && g.per[1]) ? -int64_t(g.X - 3) : 1;
-        oy[0] = (y - 1 == 0 && g.per[2]) ? int64_t(g.Y - 3) * sy : -sy;
-        oy[2] = (y + 1 == g.Y - 1 && g.per[3]) ? -int64_t(g.Y - 3) * sy : sy;
-        oz[0] = (z - 1 == 0 && g.per[4]) ? int64_t(g.Z - 3) * sz : -sz;
-        oz[2] = (z + 1 == g.Z - 1 && g.per[5]) ? -int64_t(g.Z - 3) * sz : sz;
+        ox[0] = (x - 1 == 0 && g.per[0]) ? g.X - 3 : -1;
+        ox[2] = (x + 1 == g.X - 1 && g.per[1]) ? -(g.X - 3) : 1;
+        oy[0] = (y - 1 == 0 && g.per[2]) ? (g.Y - 3) * sy : -sy;
+        oy[2] = (y + 1 == g.Y - 1 && g.per[3]) ? -(g.Y - 3) * sy : sy;
+        oz[0] = (z - 1 == 0 && g.per[4]) ? (g.Z - 3) * sz : -sz;
+        oz[2] = (z + 1 == g.Z - 1 && g.per[5]) ? -(g.Z - 3) * sz : sz;
     }
-    __device__ int64_t d(int q) const { return ox[cxq(q) + 1] + oy[cyq(q) + 1] + oz[czq(q) + 1]; }
+    __device__ int32_t d(int q) const { return ox[cxq(q) + 1] + oy[cyq(q) + 1] + oz[czq(q) + 1]; }
 };
+
+// lattice vectors for a direction index known only at run time: 2-bit fields of packed words
+constexpr uint64_t pack_dir(int axis) {
+    uint64_t k = 0;
+    for (int q = 0; q < Q; ++q) k |= uint64_t((axis == 0 ? cxq(q) : axis == 1 ? cyq(q) : czq(q)) + 1) << (2 * q);
+    return k;
+}
+constexpr uint64_t PACK_CX = pack_dir(0), PACK_CY = pack_dir(1), PACK_CZ = pack_dir(2);
+constexpr bool opp_rule_ok() {
+    for (int q = 1; q < Q; ++q)
+        if (oppq(q) != ((q - 1) ^ 1) + 1) return false;
+    return true;
+}
+static_assert(opp_rule_ok(), "opposite-direction rule");
+__device__ inline int cx_rt(int q) { return int((PACK_CX >> (2 * q)) & 3u) - 1; }
+__device__ inline int cy_rt(int q) { return int((PACK_CY >> (2 * q)) & 3u) - 1; }
+__device__ inline int cz_rt(int q) { return int((PACK_CZ >> (2 * q)) & 3u) - 1; }
+__device__ inline int opp_rt(int q) { return q == 0 ? 0 : ((q - 1) ^ 1) + 1; }
+__device__ inline double w_rt(int q) { return q == 0 ? 12.0 / 36.0 : (q < 7 ? 2.0 / 36.0 : 1.0 / 36.0); }
+__device__ inline int32_t offs_rt(const Offs& o, int q) {
+    const int cx = cx_rt(q), cy = cy_rt(q), cz = cz_rt(q);
+    return (cx < 0 ? o.ox[0] : (cx > 0 ? o.ox[2] : 0)) + (cy < 0 ? o.oy[0] : (cy > 0 ? o.oy[2] : 0)) +
+           (cz < 0 ? o.oz[0] : (cz > 0 ? o.oz[2] : 0));
+}
 
 // slot holding the post-streaming population q of cell i in layout L (0 natural, 1 swapped)
 __device__ inline int64_t load_slot(const Geom& g, int64_t i, const Offs& o, int q, int L) {
@@ -126,6 +151,8 @@ struct FusedArgs {
     const ObjLB* olb;
     double* hacc;
     double* blockPart;
+    int32_t* blockFlag;
+    const uint64_t* bcm;
     DevScalars* sc;
     int nbx, nby, nbz;
     Consts k;
@@ -164,17 +191,17 @@ __device__ inline typename Types<S>::C boundary_value(const FusedArgs<S>& a, int
     using C = typename Types<S>::C;
     const Consts& k = a.k;
     if (ty == T_GAS) {
-        const C cu = C(cxq(j)) * sx + C(cyq(j)) * sy + C(czq(j)) * sz;
-        return -gj + C(wq(j)) * (C(9) * (cu * cu) - C(3) * usq);
+        const C cu = C(cx_rt(j)) * sx + C(cy_rt(j)) * sy + C(cz_rt(j)) * sz;
+        return -gj + C(w_rt(j)) * (C(9) * (cu * cu) - C(3) * usq);
     }
     if (ty == T_WALL) {
         if (DIAG && k.nWall > 0) {
             int wi = a.solid[yj];
             wi = (wi >= 0 && wi < k.nWall) ? wi : 0;
             const double v2 = 2.0 * double(gj);
-            if (cxq(j)) atomicAdd(&sPart[7 * MAX_OBJ_PART + 3 * wi + 0], v2 * cxq(j));
-            if (cyq(j)) atomicAdd(&sPart[7 * MAX_OBJ_PART + 3 * wi + 1], v2 * cyq(j));
-            if (czq(j)) atomicAdd(&sPart[7 * MAX_OBJ_PART + 3 * wi + 2], v2 * czq(j));
+            if (cx_rt(j)) atomicAdd(&sPart[7 * MAX_OBJ_PART + 3 * wi + 0], v2 * cx_rt(j));
+            if (cy_rt(j)) atomicAdd(&sPart[7 * MAX_OBJ_PART + 3 * wi + 1], v2 * cy_rt(j));
+            if (cz_rt(j)) atomicAdd(&sPart[7 * MAX_OBJ_PART + 3 * wi + 2], v2 * cz_rt(j));
         }
         return gj;
     }
@@ -188,14 +215,14 @@ __device__ inline typename Types<S>::C boundary_value(const FusedArgs<S>& a, int
                 if (inside_sphere(yx, yy, yz, a.olb[oo].c[0], a.olb[oo].c[1], a.olb[oo].c[2], a.olb[oo].r)) ov = oo;
         }
         const ObjLB& OV = a.olb[ov];
-        const double cuo = OV.u[0] * cxq(j) + OV.u[1] * cyq(j) + OV.u[2] * czq(j);
-        const C BBi = C(6.0) * n * C(wq(j)) * C(cuo);
+        const double cuo = OV.u[0] * cx_rt(j) + OV.u[1] * cy_rt(j) + OV.u[2] * cz_rt(j);
+        const C BBi = C(6.0) * n * C(w_rt(j)) * C(cuo);
         const int oi = a.solid[yj];
         if (oi >= 0 && oi < nObj && oi < MAX_OBJ_PART) {
             int yx, yy, yz;
             cell_xyz(a.g, yj, yx, yy, yz);
             const double fm = 2.0 * double(gj) - double(BBi);
-            const double Fb[3] = {fm * cxq(j), fm * cyq(j), fm * czq(j)};
+            const double Fb[3] = {fm * cx_rt(j), fm * cy_rt(j), fm * cz_rt(j)};
             const double rr[3] = {yx - a.olb[oi].c[0], yy - a.olb[oi].c[1], yz - a.olb[oi].c[2]};
             double* sp = &sPart[7 * oi];
             atomicAdd(sp + 0, Fb[0]); atomicAdd(sp + 1, Fb[1]); atomicAdd(sp + 2, Fb[2]);
@@ -213,36 +240,201 @@ __device__ inline typename Types<S>::C boundary_value(const FusedArgs<S>& a, int
 // plain index offset of direction q (valid for cells without the edge bit)
 __device__ inline int32_t plain_off(int q, int32_t sy, int32_t sz) { return cxq(q) + cyq(q) * sy + czq(q) * sz; }
 
+// populations of node i before collision (its post-streaming values). PAR 0: the node's own
+// slots; PAR 1: the slots its neighbours wrote, at wrapped offsets when the wave holds a node
+// next to a periodic face (for every other node the wrapped offsets equal the plain ones).
+// Called by all lanes (the wave-wide vote decides the offset form).
 template <typename S, int PAR>
-__device__ inline void load_cell(const FusedArgs<S>& a, int32_t i, uint8_t t, int x, int y, int z,
-                                 typename Types<S>::C* gq) {
+__device__ inline void load_pop(const FusedArgs<S>& a, int32_t i, bool act, bool wrap, int x, int y, int z,
+                                typename Types<S>::C* gq) {
     const int32_t N = int32_t(a.g.N), sy = a.g.X, sz = a.g.X * a.g.Y;
     if (PAR == 0) {
+        if (act) {
 #pragma unroll
-        for (int q = 0; q < Q; ++q) gq[q] = StoreT<S>::ld(a.f + q * N + i);
-    } else if (!(t & T_EDGE)) {
+            for (int q = 0; q < Q; ++q) gq[q] = StoreT<S>::ld(a.f + q * N + i);
+        }
+    } else if (__ballot(act && wrap)) {
+        if (act) {
+            const Offs o(a.g, x, y, z);
+#pragma unroll
+            for (int q = 0; q < Q; ++q) gq[q] = StoreT<S>::ld(a.f + oppq(q) * N + i + o.d(oppq(q)));
+        }
+    } else if (act) {
 #pragma unroll
         for (int q = 0; q < Q; ++q) gq[q] = StoreT<S>::ld(a.f + oppq(q) * N + i + plain_off(oppq(q), sy, sz));
-    } else {
-        const Offs o(a.g, x, y, z);
-#pragma unroll
-        for (int q = 0; q < Q; ++q) gq[q] = StoreT<S>::ld(a.f + oppq(q) * N + i + int32_t(o.d(oppq(q))));
     }
 }
 
+// streaming of the post-collision values (LB.cpp:1375-1389): PAR 0 into the node's own slots
+// (opposite direction), PAR 1 into the neighbour's slots
+template <typename S, int PAR>
+__device__ inline void store_pop(const FusedArgs<S>& a, int32_t i, bool act, bool wrap, int x, int y, int z,
+                                 const typename Types<S>::C* gq) {
+    const int32_t N = int32_t(a.g.N), sy = a.g.X, sz = a.g.X * a.g.Y;
+    if (PAR == 0) {
+        if (act) {
+            StoreT<S>::st(a.f + i, gq[0]);
+#pragma unroll
+            for (int j = 1; j < Q; ++j) StoreT<S>::st(a.f + oppq(j) * N + i, gq[j]);
+        }
+    } else if (__ballot(act && wrap)) {
+        if (act) {
+            const Offs o(a.g, x, y, z);
+            StoreT<S>::st(a.f + i, gq[0]);
+#pragma unroll
+            for (int j = 1; j < Q; ++j) StoreT<S>::st(a.f + j * N + i + o.d(j), gq[j]);
+        }
+    } else if (act) {
+        StoreT<S>::st(a.f + i, gq[0]);
+#pragma unroll
+        for (int j = 1; j < Q; ++j) StoreT<S>::st(a.f + j * N + i + plain_off(j, sy, sz), gq[j]);
+    }
+}
+
+// one node's collision (node.cpp:75-207): reconstruct, interior forcing of particle nodes
+// (LB.cpp:1919-1962), BGK with Guo forcing. gc holds the post-streaming populations on entry
+// and the post-collision ones on return.
+template <typename S>
+struct CellOut {
+    typename Types<S>::C n, sx, sy, sz, usq, hx, hy, hz;
+    typename Types<S>::C d[7];  // particle contribution: F(3), M(3), fluid volume
+    int key;                    // particle index or -1
+};
+template <typename S>
+__device__ inline void collide_cell(const FusedArgs<S>& a, int32_t sp, uint8_t t, int x, int y, int z, double mi,
+                                    typename Types<S>::C* gc, CellOut<S>& r) {
+    using C = typename Types<S>::C;
+    const Consts& k = a.k;
+    C sum = 0, jx = 0, jy = 0, jz = 0;
+#pragma unroll
+    for (int q = 0; q < Q; ++q) {
+        sum += gc[q];
+        jx += C(cxq(q)) * gc[q]; jy += C(cyq(q)) * gc[q]; jz += C(czq(q)) * gc[q];
+    }
+    const C n = C(1) + sum;
+    const C ux = jx / n, uy = jy / n, uz = jz / n;
+    C hx = 0, hy = 0, hz = 0;
+    if (t & T_PMASK) {
+        const PartLB P = a.plb[sp];
+        const C rx = C(double(x) - P.c[0]), ry = C(double(y) - P.c[1]), rz = C(double(z) - P.c[2]);
+        const C wx = C(P.w[0]), wy = C(P.w[1]), wz = C(P.w[2]);
+        const C lvx = C(P.v[0]) + (wy * rz - wz * ry);
+        const C lvy = C(P.v[1]) + (wz * rx - wx * rz);
+        const C lvz = C(P.v[2]) + (wx * ry - wy * rx);
+        const C lf = C(mi) / n;
+        const C dfx = lf * (ux - lvx), dfy = lf * (uy - lvy), dfz = lf * (uz - lvz);
+        hx = -dfx; hy = -dfy; hz = -dfz;
+        r.d[0] = dfx; r.d[1] = dfy; r.d[2] = dfz;
+        r.d[3] = ry * dfz - rz * dfy; r.d[4] = rz * dfx - rx * dfz; r.d[5] = rx * dfy - ry * dfx;
+        r.d[6] = C(mi);
+        r.key = sp;
+    }
+    const C Fx = C(k.lbF[0]) + hx, Fy = C(k.lbF[1]) + hy, Fz = C(k.lbF[2]) + hz;
+    const C sx = ux + C(0.5) * Fx / n, sy = uy + C(0.5) * Fy / n, sz = uz + C(0.5) * Fz / n;
+    const C usq = sx * sx + sy * sy + sz * sz;
+    const C uF = sx * Fx + sy * Fy + sz * Fz;
+    const C om = C(k.omega), omf = C(k.omegaf);
+#pragma unroll
+    for (int q = 0; q < Q; ++q) {
+        const C cu = C(cxq(q)) * sx + C(cyq(q)) * sy + C(czq(q)) * sz;
+        const C cF = C(cxq(q)) * Fx + C(cyq(q)) * Fy + C(czq(q)) * Fz;
+        const C geq = C(wq(q)) * ((n - C(1)) + n * (C(3) * cu + C(4.5) * cu * cu - C(1.5) * usq));
+        gc[q] = gc[q] + om * (geq - gc[q]) + omf * C(wq(q)) * (C(9) * cu * cF + C(3) * (cF - uF));
+    }
+    r.n = n; r.sx = sx; r.sy = sy; r.sz = sz; r.usq = usq; r.hx = hx; r.hy = hy; r.hz = hz;
+}
+
+// particle accumulators of one wavefront: segmented sums over runs of equal particle index,
+// one LDS hash update per run (global atomics when the block's table is full)
+template <typename C>
+__device__ inline void accumulate_particles(const C* d, int key, int lane, int* hkey, double (*hval)[7],
+                                            double* hacc) {
+    const unsigned long long pm = __ballot(key >= 0);
+    if (!pm) return;
+    const int st0 = run_start(key, lane);
+    C s[7];
+#pragma unroll
+    for (int c = 0; c < 7; ++c) s[c] = seg_sum(d[c], st0, lane);
+    const int nextKey = __shfl_down(key, 1, 64);
+    const bool tail = key >= 0 && (lane == 63 || nextKey != key);
+    if (!tail) return;
+    int slot = key & (HS - 1);
+    int probes = 0;
+    for (; probes < HS; ++probes) {
+        const int old = atomicCAS(&hkey[slot], -1, key);
+        if (old == -1 || old == key) break;
+        slot = (slot + 1) & (HS - 1);
+    }
+    if (probes < HS) {
+#pragma unroll
+        for (int c = 0; c < 7; ++c) atomicAdd(&hval[slot][c], double(s[c]));
+    } else {
+        double* hp = hacc + int64_t(key) * 8;
+#pragma unroll
+        for (int c = 0; c < 7; ++c) atomicAdd(hp + c, double(s[c]));
+    }
+}
+
+__device__ inline void flush_particles(int tid, int nthreads, const int* hkey, double (*hval)[7], double* hacc) {
+    for (int s = tid; s < HS; s += nthreads) {
+        const int key = hkey[s];
+        if (key >= 0) {
+            double* hp = hacc + int64_t(key) * 8;
+            for (int c = 0; c < 7; ++c) atomicAdd(hp + c, hval[s][c]);
+        }
+    }
+}
+
+// boundary-link classes of the active node i at (x, y, z), over its wrapped neighbours
+__device__ inline uint64_t link_classes(const Geom& g, const uint8_t* type, int32_t i, int x, int y, int z) {
+    const Offs o(g, x, y, z);
+    uint64_t m = 0;
+    int wprim = -1;
+#pragma unroll
+    for (int j = 1; j < Q; ++j) {
+        const uint8_t tf = type[i + o.d(j)];
+        const uint8_t ty = tf & T_BASE;
+        uint64_t cls = BC_OTHER;
+        if (ty == T_FLUID || ty == T_IF) {
+            cls = BC_NONE;
+        } else if (ty == T_GAS) {
+            cls = BC_GAS;
+        } else if (ty == T_WALL) {
+            const int w = (tf >> T_WSHIFT) & 7;
+            if (wprim < 0) wprim = w;
+            cls = w == wprim ? BC_WALL : BC_OTHER;
+        }
+        m |= cls << (2 * (j - 1));
+    }
+    if (wprim > 0) m |= uint64_t(wprim) << BC_WALL_SHIFT;
+    return m;
+}
+
+// Fused collide-stream over the box. Block = FX x FY columns; each thread walks FZ nodes along z.
+// Per node the loads of the next node (populations, mass, solid index, boundary-link classes)
+// and the type byte of the node after it go out before this node's stores (in either AA parity a
+// node loads and stores only its own slot set, so no store of node z aliases a load of node
+// z+1). Edge nodes are handled by the same waves as their row neighbours, so every population
+// access stays coalesced; their boundary links run in a rolled loop over the precomputed link
+// classes, without further loads for gas and wall links.
 template <typename S, int PAR, bool FSURF, bool DIAG>
 __global__ __launch_bounds__(FX* FY) void k_fused(FusedArgs<S> a) {
     using C = typename Types<S>::C;
     using M = typename Types<S>::M;
+    constexpr int NT = FX * FY;
     __shared__ int hkey[HS];
     __shared__ double hval[HS][7];
     __shared__ double sPart[PART_STRIDE];
+    __shared__ C sg[Q - 1][NT];           // post-collision values of edge nodes (boundary loop)
+    __shared__ int32_t sIf[NT * FZ];      // interface nodes of the block
+    __shared__ int sIfCount, sIfBase;
     const int tid = threadIdx.x + FX * threadIdx.y;
-    for (int s = tid; s < HS; s += FX * FY) {
+    for (int s = tid; s < HS; s += NT) {
         hkey[s] = -1;
         for (int c = 0; c < 7; ++c) hval[s][c] = 0.0;
     }
     if (tid < PART_STRIDE) sPart[tid] = 0.0;
+    if (tid == 0) sIfCount = 0;
     __syncthreads();
 
     // XCD-aware tile order: consecutive tiles of the box go to the same XCD
@@ -254,181 +446,195 @@ __global__ __launch_bounds__(FX* FY) void k_fused(FusedArgs<S> a) {
     const int bx = tile % a.nbx, by = (tile / a.nbx) % a.nby, bz = tile / (a.nbx * a.nby);
     const int x = bx * FX + threadIdx.x, y = by * FY + threadIdx.y;
     const Geom& g = a.g;
-    const int32_t N = int32_t(g.N), SY = g.X, SZ = g.X * g.Y;
+    const int32_t N = int32_t(g.N), SZ = g.X * g.Y;
     const int lane = __lane_id();
-    const Consts& k = a.k;
     const bool col_ok = tile_ok && x < g.X && y < g.Y;
     const int z0 = bz * FZ;
+    // links of nodes next to a periodic face wrap
+    const bool wrapXY = (x == 1 && g.per[0]) || (x == g.X - 2 && g.per[1]) || (y == 1 && g.per[2]) ||
+                        (y == g.Y - 2 && g.per[3]);
+    auto wraps = [&](int z) { return wrapXY || (z == 1 && g.per[4]) || (z == g.Z - 2 && g.per[5]); };
+    auto type_at = [&](int zz) -> uint8_t {
+        return (col_ok && zz < FZ && z0 + zz < g.Z) ? a.type[int32_t(cidx(g, x, y, z0 + zz))] : uint8_t(T_WALL);
+    };
 
-    // prefetch of the first cell of the column
+    // per-node bundle: populations, mass (particle or edge node), solid index (particle node),
+    // boundary-link classes (edge node)
     C gc[Q], gn[Q];
-    uint8_t tc = 0;
-    int32_t ic = 0;
-    bool actc = false;
-    if (col_ok && z0 < g.Z) {
-        ic = int32_t(cidx(g, x, y, z0));
-        tc = a.type[ic];
-        actc = is_active(tc);
-        if (actc) load_cell<S, PAR>(a, ic, tc, x, y, z0, gc);
-    }
+    int32_t ic = col_ok ? int32_t(cidx(g, x, y, z0)) : 0;
+    uint8_t tc = type_at(0), tn = type_at(1);
+    bool actc = is_active(tc);
+    load_pop<S, PAR>(a, ic, actc, wraps(z0), x, y, z0, gc);
+    double mc = (actc && (tc & (T_PMASK | T_EDGE))) ? double(a.mass[ic]) : 0.0;
+    int32_t sc_ = (actc && (tc & T_PMASK)) ? a.solid[ic] : 0;
+    uint64_t bc_ = (actc && (tc & T_EDGE)) ? a.bcm[ic] : 0;
     for (int zz = 0; zz < FZ; ++zz) {
         const int z = z0 + zz;
-        // issue the loads of the next cell before this cell's stores
-        uint8_t tn = 0;
-        int32_t in = 0;
-        bool actn = false;
-        if (zz + 1 < FZ && col_ok && z + 1 < g.Z) {
-            in = ic + SZ;
-            tn = a.type[in];
-            actn = is_active(tn);
-            if (actn) load_cell<S, PAR>(a, in, tn, x, y, z + 1, gn);
-        }
-        const bool act = actc;
-        const int32_t i = ic;
-        const uint8_t t = tc;
-        int hkeyv = -1;
-        C dfx = 0, dfy = 0, dfz = 0, dmx = 0, dmy = 0, dmz = 0, dvol = 0;
-        if (act) {
-            // reconstruct (node.cpp:75-93)
-            C sum = 0, jx = 0, jy = 0, jz = 0;
+        // next node's bundle and the type byte of the node after it
+        const int32_t in = ic + SZ;
+        const bool actn = is_active(tn);
+        const uint8_t tnn = type_at(zz + 2);
+        load_pop<S, PAR>(a, in, actn, wraps(z + 1), x, y, z + 1, gn);
+        const double mn = (actn && (tn & (T_PMASK | T_EDGE))) ? double(a.mass[in]) : 0.0;
+        const int32_t sn = (actn && (tn & T_PMASK)) ? a.solid[in] : 0;
+        const uint64_t bn = (actn && (tn & T_EDGE)) ? a.bcm[in] : 0;
+
+        CellOut<S> r;
+        r.key = -1;
 #pragma unroll
-            for (int q = 0; q < Q; ++q) {
-                sum += gc[q];
-                jx += C(cxq(q)) * gc[q]; jy += C(cyq(q)) * gc[q]; jz += C(czq(q)) * gc[q];
-            }
-            const C n = C(1) + sum;
-            const C ux = jx / n, uy = jy / n, uz = jz / n;
-            // interior forcing of particle nodes (LB.cpp:1919-1962)
-            C hx = 0, hy = 0, hz = 0;
-            // mass is needed by particle nodes (LB.cpp:1935) and object links (LB.cpp:1453) only
-            const double mi = (t & (T_PMASK | T_EDGE)) ? double(a.mass[i]) : 0.0;
-            if (t & T_PMASK) {
-                const int sp = a.solid[i];
-                const PartLB P = a.plb[sp];
-                const C rx = C(double(x) - P.c[0]), ry = C(double(y) - P.c[1]), rz = C(double(z) - P.c[2]);
-                const C wx = C(P.w[0]), wy = C(P.w[1]), wz = C(P.w[2]);
-                const C lvx = C(P.v[0]) + (wy * rz - wz * ry);
-                const C lvy = C(P.v[1]) + (wz * rx - wx * rz);
-                const C lvz = C(P.v[2]) + (wx * ry - wy * rx);
-                const C lf = C(mi) / n;
-                dfx = lf * (ux - lvx); dfy = lf * (uy - lvy); dfz = lf * (uz - lvz);
-                hx = -dfx; hy = -dfy; hz = -dfz;
-                dmx = ry * dfz - rz * dfy; dmy = rz * dfx - rx * dfz; dmz = rx * dfy - ry * dfx;
-                dvol = C(mi);
-                hkeyv = sp;
-            }
-            // collision (node.cpp:95-111, 181-207)
-            const C Fx = C(k.lbF[0]) + hx, Fy = C(k.lbF[1]) + hy, Fz = C(k.lbF[2]) + hz;
-            const C sx = ux + C(0.5) * Fx / n, sy = uy + C(0.5) * Fy / n, sz = uz + C(0.5) * Fz / n;
-            const C usq = sx * sx + sy * sy + sz * sz;
-            const C uF = sx * Fx + sy * Fy + sz * Fz;
-            const C om = C(k.omega), omf = C(k.omegaf);
-#pragma unroll
-            for (int q = 0; q < Q; ++q) {
-                const C cu = C(cxq(q)) * sx + C(cyq(q)) * sy + C(czq(q)) * sz;
-                const C cF = C(cxq(q)) * Fx + C(cyq(q)) * Fy + C(czq(q)) * Fz;
-                const C geq = C(wq(q)) * ((n - C(1)) + n * (C(3) * cu + C(4.5) * cu * cu - C(1.5) * usq));
-                gc[q] = gc[q] + om * (geq - gc[q]) + omf * C(wq(q)) * (C(9) * cu * cF + C(3) * (cF - uF));
-            }
-            // streaming (LB.cpp:1375-1528)
-            StoreT<S>::st(a.f + i, gc[0]);
-            if (!(t & T_EDGE)) {
+        for (int c = 0; c < 7; ++c) r.d[c] = 0;
+        r.n = 1; r.sx = 0; r.sy = 0; r.sz = 0; r.usq = 0;
+        if (actc) collide_cell<S>(a, sc_, tc, x, y, z, mc, gc, r);
+        store_pop<S, PAR>(a, ic, actc, wraps(z), x, y, z, gc);
+
+        // boundary links of edge nodes (LB.cpp:1390-1528): the value reflected into the slot
+        // this node loads next step
+        const bool edge = actc && (tc & T_EDGE) && bc_ != 0;
+        if (__ballot(edge)) {
+            uint32_t lm = 0;
+            if (edge) {
 #pragma unroll
                 for (int j = 1; j < Q; ++j) {
-                    if (PAR == 0) StoreT<S>::st(a.f + oppq(j) * N + i, gc[j]);
-                    else StoreT<S>::st(a.f + j * N + i + plain_off(j, SY, SZ), gc[j]);
-                }
-            } else {
-                const Offs o(g, x, y, z);
-#pragma unroll
-                for (int j = 1; j < Q; ++j) {
-                    const int32_t yj = i + int32_t(o.d(j));
-                    if (PAR == 0) StoreT<S>::st(a.f + oppq(j) * N + i, gc[j]);
-                    else StoreT<S>::st(a.f + j * N + yj, gc[j]);
-                    const uint8_t ty = a.type[yj] & T_BASE;
-                    if (ty == T_FLUID || ty == T_IF) continue;
-                    const C bc = boundary_value<S, DIAG>(a, j, yj, ty, gc[j], n, sx, sy, sz, usq, mi, sPart);
-                    if (PAR == 0) StoreT<S>::st(a.f + j * N + yj, bc);
-                    else StoreT<S>::st(a.f + oppq(j) * N + i, bc);
+                    sg[j - 1][tid] = gc[j];
+                    if ((bc_ >> (2 * (j - 1))) & 3u) lm |= 1u << j;
                 }
             }
-            if (FSURF) {
-                if ((t & T_BASE) == T_FLUID) {
-                    a.mass[i] = M(n);
+            const Offs o(g, x, y, z);
+            const int wprim = int(bc_ >> BC_WALL_SHIFT) & 7;
+            while (lm) {
+                const int j = __ffs(lm) - 1;
+                lm &= lm - 1u;
+                const int cls = int(bc_ >> (2 * (j - 1))) & 3;
+                const C gj = sg[j - 1][tid];
+                const int32_t yj = ic + offs_rt(o, j);
+                C bcv;
+                if (cls == BC_GAS) {
+                    const C cu = C(cx_rt(j)) * r.sx + C(cy_rt(j)) * r.sy + C(cz_rt(j)) * r.sz;
+                    bcv = -gj + C(w_rt(j)) * (C(9) * (cu * cu) - C(3) * r.usq);
+                } else if (cls == BC_WALL) {
+                    if (DIAG && a.k.nWall > 0) {
+                        const double v2 = 2.0 * double(gj);
+                        double* wp = &sPart[7 * MAX_OBJ_PART + 3 * wprim];
+                        if (cx_rt(j)) atomicAdd(wp + 0, v2 * cx_rt(j));
+                        if (cy_rt(j)) atomicAdd(wp + 1, v2 * cy_rt(j));
+                        if (cz_rt(j)) atomicAdd(wp + 2, v2 * cz_rt(j));
+                    }
+                    bcv = gj;
                 } else {
-                    a.aux[i] = n;
-                    a.aux[N + i] = sx; a.aux[2 * N + i] = sy; a.aux[3 * N + i] = sz;
-                    a.aux[4 * N + i] = hx; a.aux[5 * N + i] = hy; a.aux[6 * N + i] = hz;
+                    bcv = boundary_value<S, DIAG>(a, j, yj, uint8_t(a.type[yj] & T_BASE), gj, r.n, r.sx, r.sy,
+                                                  r.sz, r.usq, mc, sPart);
                 }
+                if (PAR == 0) StoreT<S>::st(a.f + j * N + yj, bcv);
+                else StoreT<S>::st(a.f + opp_rt(j) * N + ic, bcv);
             }
         }
-        // interface list append (wave-aggregated)
         if (FSURF) {
-            const bool isIf = act && (t & T_BASE) == T_IF;
+            const bool isIf = actc && (tc & T_BASE) == T_IF;
+            if (actc && !isIf) a.mass[ic] = M(r.n);
+            if (isIf) {
+                a.aux[ic] = r.n;
+                a.aux[N + ic] = r.sx; a.aux[2 * N + ic] = r.sy; a.aux[3 * N + ic] = r.sz;
+                a.aux[4 * N + ic] = r.hx; a.aux[5 * N + ic] = r.hy; a.aux[6 * N + ic] = r.hz;
+            }
+            // interface list: one LDS reservation per wave
             const unsigned long long m = __ballot(isIf);
             if (m) {
-                int base = 0;
                 const int leader = __ffsll((unsigned long long)m) - 1;
-                if (lane == leader) base = atomicAdd(&a.sc->nIf, __popcll(m));
+                int base = 0;
+                if (lane == leader) base = atomicAdd(&sIfCount, __popcll(m));
                 base = __shfl(base, leader, 64);
-                if (isIf) {
-                    const int pos = base + __popcll(m & ((1ull << lane) - 1ull));
-                    if (pos < a.ifCap) a.ifList[pos] = i;
-                    else atomicOr(&a.sc->listOverflow, 1);
-                }
+                if (isIf) sIf[base + __popcll(m & ((1ull << lane) - 1ull))] = ic;
             }
         }
-        // particle accumulators: wave segmented sums by particle, one LDS update per run
-        const unsigned long long pm = __ballot(hkeyv >= 0);
-        if (pm) {
-            const int st0 = run_start(hkeyv, lane);
-            const C s0 = seg_sum(dfx, st0, lane), s1 = seg_sum(dfy, st0, lane), s2 = seg_sum(dfz, st0, lane);
-            const C s3 = seg_sum(dmx, st0, lane), s4 = seg_sum(dmy, st0, lane), s5 = seg_sum(dmz, st0, lane);
-            const C s6 = seg_sum(dvol, st0, lane);
-            const int nextKey = __shfl_down(hkeyv, 1, 64);
-            const bool tail = hkeyv >= 0 && (lane == 63 || nextKey != hkeyv);
-            if (tail) {
-                int slot = hkeyv & (HS - 1);
-                int probes = 0;
-                for (; probes < HS; ++probes) {
-                    const int old = atomicCAS(&hkey[slot], -1, hkeyv);
-                    if (old == -1 || old == hkeyv) break;
-                    slot = (slot + 1) & (HS - 1);
-                }
-                if (probes < HS) {
-                    atomicAdd(&hval[slot][0], double(s0)); atomicAdd(&hval[slot][1], double(s1));
-                    atomicAdd(&hval[slot][2], double(s2)); atomicAdd(&hval[slot][3], double(s3));
-                    atomicAdd(&hval[slot][4], double(s4)); atomicAdd(&hval[slot][5], double(s5));
-                    atomicAdd(&hval[slot][6], double(s6));
-                } else {
-                    double* hp = a.hacc + int64_t(hkeyv) * 8;
-                    atomicAdd(hp + 0, double(s0)); atomicAdd(hp + 1, double(s1)); atomicAdd(hp + 2, double(s2));
-                    atomicAdd(hp + 3, double(s3)); atomicAdd(hp + 4, double(s4)); atomicAdd(hp + 5, double(s5));
-                    atomicAdd(hp + 6, double(s6));
-                }
-            }
-        }
+        accumulate_particles<C>(r.d, r.key, lane, hkey, hval, a.hacc);
         // advance the column
         actc = actn;
         ic = in;
         tc = tn;
+        tn = tnn;
+        mc = mn;
+        sc_ = sn;
+        bc_ = bn;
 #pragma unroll
         for (int q = 0; q < Q; ++q) gc[q] = gn[q];
     }
     __syncthreads();
-    for (int s = tid; s < HS; s += FX * FY) {
-        const int key = hkey[s];
-        if (key >= 0) {
-            double* hp = a.hacc + int64_t(key) * 8;
-            for (int c = 0; c < 7; ++c) atomicAdd(hp + c, hval[s][c]);
+    flush_particles(tid, NT, hkey, hval, a.hacc);
+    if (FSURF) {
+        // interface list: one global reservation per block
+        const int n = sIfCount;
+        if (n > 0) {
+            if (tid == 0) sIfBase = atomicAdd(&a.sc->nIf, n);
+            __syncthreads();
+            const int base = sIfBase;
+            for (int k = tid; k < n; k += NT) {
+                if (base + k < a.ifCap) a.ifList[base + k] = sIf[k];
+                else atomicOr(&a.sc->listOverflow, 1);
+            }
         }
     }
-    if (tid < PART_STRIDE) a.blockPart[int64_t(tid) * gridDim.x + b0] = sPart[tid];
+    // object / wall partials: written only by blocks that have any, flagged for the reduction
+    const int any = __syncthreads_or(tid < PART_STRIDE && sPart[tid] != 0.0);
+    if (any && tid < PART_STRIDE) a.blockPart[int64_t(tid) * gridDim.x + b0] = sPart[tid];
+    if (tid == 0) a.blockFlag[b0] = any;
 }
 
-// edge bit of active nodes: a neighbour (wrapped) is not active, or the node sits next to a
-// periodic face (its links wrap)
-__global__ void k_mark_edges(Geom g, uint8_t* type) {
+// Edge bits (T_EDGE) of active nodes: an interface node, a non-active neighbour, or a node next
+// to a periodic face; for edge nodes also the boundary-link classes. One thread per four nodes
+// of a row (one 32-bit word of type bytes); rows must be a multiple of four bytes long.
+// Neighbour rows are read as words; the test uses three
+// masks of non-active bytes: the node's own row (x +- 1), the four rows sharing one axis with it
+// (x - 1 .. x + 1) and the four diagonal rows (same x) — the D3Q19 stencil.
+__device__ inline uint32_t inactive_bytes(uint32_t w) {
+    uint32_t m = 0;
+#pragma unroll
+    for (int k = 0; k < 4; ++k) m |= uint32_t(!is_active(uint8_t(w >> (8 * k)))) << k;
+    return m;
+}
+__global__ void k_mark_edges_w(Geom g, uint8_t* type, uint64_t* bcm) {
+    const int W = g.X / 4;
+    const int64_t gid = int64_t(blockIdx.x) * blockDim.x + threadIdx.x;
+    if (gid >= int64_t(g.Y) * g.Z * W) return;
+    const int w = int(gid % W);
+    const int64_t row = gid / W;
+    const int y = int(row % g.Y), z = int(row / g.Y);
+    if (y == 0 || y == g.Y - 1 || z == 0 || z == g.Z - 1) return;
+    const uint32_t* t32 = reinterpret_cast<const uint32_t*>(type);
+    const int64_t RW = W, SW = int64_t(W) * g.Y;  // words per row / per plane
+    const int64_t c0 = row * W + w;
+    const uint32_t cw = t32[c0];
+    const uint32_t ca = inactive_bytes(cw);
+    if (ca == 0xFu) return;  // no active node in this word
+    // 6-bit window of row r: bit 0 = byte x-1, bits 1-4 = the word, bit 5 = byte x+4
+    auto window = [&](int64_t r) {
+        const uint32_t mid = inactive_bytes(t32[r]);
+        const uint32_t lo = w > 0 ? uint32_t(!is_active(uint8_t(t32[r - 1] >> 24))) : 1u;
+        const uint32_t hi = w < W - 1 ? uint32_t(!is_active(uint8_t(t32[r + 1]))) : 1u;
+        return lo | (mid << 1) | (hi << 5);
+    };
+    const uint32_t own = window(c0);
+    const uint32_t side = window(c0 - RW) | window(c0 + RW) | window(c0 - SW) | window(c0 + SW);
+    const uint32_t diag = (inactive_bytes(t32[c0 - RW - SW]) | inactive_bytes(t32[c0 - RW + SW]) |
+                           inactive_bytes(t32[c0 + RW - SW]) | inactive_bytes(t32[c0 + RW + SW]));
+    const bool perRow = (y == 1 && g.per[2]) || (y == g.Y - 2 && g.per[3]) || (z == 1 && g.per[4]) ||
+                        (z == g.Z - 2 && g.per[5]);
+    uint32_t nw = cw;
+#pragma unroll
+    for (int k = 0; k < 4; ++k) {
+        const uint8_t t = uint8_t(cw >> (8 * k));
+        if (!is_active(t)) continue;
+        const int x = 4 * w + k;
+        const bool edge = perRow || (x == 1 && g.per[0]) || (x == g.X - 2 && g.per[1]) || (t & T_BASE) == T_IF ||
+                          ((own >> k) & 5u) || ((side >> k) & 7u) || ((diag >> k) & 1u);
+        const uint8_t nt = edge ? uint8_t(t | T_EDGE) : uint8_t(t & ~T_EDGE);
+        nw = (nw & ~(0xFFu << (8 * k))) | (uint32_t(nt) << (8 * k));
+        if (edge) bcm[c0 * 4 + k] = link_classes(g, type, int32_t(c0 * 4 + k), x, y, z);
+    }
+    // other threads read this word concurrently; they only test the type bits, which do not change
+    if (nw != cw) reinterpret_cast<uint32_t*>(type)[c0] = nw;
+}
+// the same for rows whose length is not a multiple of four: one thread per node
+__global__ void k_mark_edges_b(Geom g, uint8_t* type, uint64_t* bcm) {
     const int32_t i = int32_t(blockIdx.x) * blockDim.x + threadIdx.x;
     if (i >= int32_t(g.N)) return;
     const uint8_t t = type[i];
@@ -436,34 +642,44 @@ __global__ void k_mark_edges(Geom g, uint8_t* type) {
     int x, y, z;
     cell_xyz(g, i, x, y, z);
     bool edge = (x == 1 && g.per[0]) || (x == g.X - 2 && g.per[1]) || (y == 1 && g.per[2]) ||
-                (y == g.Y - 2 && g.per[3]) || (z == 1 && g.per[4]) || (z == g.Z - 2 && g.per[5]);
-    if (!edge) {
-        const int32_t sy = g.X, sz = g.X * g.Y;
+                (y == g.Y - 2 && g.per[3]) || (z == 1 && g.per[4]) || (z == g.Z - 2 && g.per[5]) ||
+                (t & T_BASE) == T_IF;
+    const int32_t sy = g.X, sz = g.X * g.Y;
 #pragma unroll
-        for (int q = 1; q < Q; ++q) edge |= !is_active(type[i + plain_off(q, sy, sz)]);
-    }
-    const uint8_t nt = edge ? (t | T_EDGE) : (t & uint8_t(~T_EDGE));
+    for (int q = 1; q < Q; ++q) edge |= !is_active(type[i + plain_off(q, sy, sz)]);
+    const uint8_t nt = edge ? uint8_t(t | T_EDGE) : uint8_t(t & ~T_EDGE);
     if (nt != t) type[i] = nt;
+    if (edge) bcm[i] = link_classes(g, type, i, x, y, z);
 }
 
 void lb_mark_edges(const Box& b, hipStream_t s) {
-    hipLaunchKernelGGL(k_mark_edges, int((b.g.N + 255) / 256), 256, 0, s, b.g, b.type);
+    if (b.g.X % 4 == 0) {
+        const int64_t n = int64_t(b.g.Y) * b.g.Z * (b.g.X / 4);
+        hipLaunchKernelGGL(k_mark_edges_w, int((n + 255) / 256), 256, 0, s, b.g, b.type, b.bcm);
+    } else {
+        hipLaunchKernelGGL(k_mark_edges_b, int((b.g.N + 255) / 256), 256, 0, s, b.g, b.type, b.bcm);
+    }
+}
+
+int fused_grid_blocks(const Geom& g) {
+    const int nbx = (g.X + FX - 1) / FX, nby = (g.Y + FY - 1) / FY, nbz = (g.Z + FZ - 1) / FZ;
+    return ((nbx * nby * nbz + 7) / 8) * 8;
 }
 
 template <typename S>
 static void launch_fused(Box& b, const Consts& k, const PartLB* plb, const ObjLB* olb, DevScalars* sc, bool diag,
-                         hipStream_t s) {
+                         hipStream_t s, hipEvent_t ev0, hipEvent_t ev1) {
     FusedArgs<S> a;
     a.g = b.g; a.f = (S*)b.f; a.type = b.type; a.solid = b.solid;
     a.mass = (typename Types<S>::M*)b.mass; a.aux = (typename Types<S>::C*)b.aux;
-    a.ifList = b.ifList; a.ifCap = b.ifCap; a.plb = plb; a.olb = olb; a.hacc = b.hacc;
-    a.blockPart = b.blockPart; a.sc = sc; a.k = k;
+    a.ifList = b.ifList; a.ifCap = b.ifCap;
+    a.plb = plb; a.olb = olb; a.hacc = b.hacc; a.blockPart = b.blockPart; a.blockFlag = b.blockFlag; a.bcm = b.bcm; a.sc = sc;
+    a.k = k;
     a.nbx = (b.g.X + FX - 1) / FX; a.nby = (b.g.Y + FY - 1) / FY; a.nbz = (b.g.Z + FZ - 1) / FZ;
-    const int nblk = a.nbx * a.nby * a.nbz;
-    const int grid = ((nblk + 7) / 8) * 8;
-    dim3 bs(FX, FY, 1);
+    const int grid = fused_grid_blocks(b.g);
     const bool fs = k.freeSurface != 0;
-#define LBDEM_FUSED(P, F, D) hipLaunchKernelGGL((k_fused<S, P, F, D>), dim3(grid), bs, 0, s, a)
+    if (ev0) (void)hipEventRecord(ev0, s);
+#define LBDEM_FUSED(P, F, D) hipLaunchKernelGGL((k_fused<S, P, F, D>), dim3(grid), dim3(FX, FY, 1), 0, s, a)
     if (b.parity == 0) {
         if (fs) { if (diag) LBDEM_FUSED(0, true, true); else LBDEM_FUSED(0, true, false); }
         else { if (diag) LBDEM_FUSED(0, false, true); else LBDEM_FUSED(0, false, false); }
@@ -472,12 +688,8 @@ static void launch_fused(Box& b, const Consts& k, const PartLB* plb, const ObjLB
         else { if (diag) LBDEM_FUSED(1, false, true); else LBDEM_FUSED(1, false, false); }
     }
 #undef LBDEM_FUSED
+    if (ev1) (void)hipEventRecord(ev1, s);
     b.parity ^= 1;
-}
-
-int fused_grid_blocks(const Geom& g) {
-    const int nbx = (g.X + FX - 1) / FX, nby = (g.Y + FY - 1) / FY, nbz = (g.Z + FZ - 1) / FZ;
-    return ((nbx * nby * nbz + 7) / 8) * 8;
 }
 
 __global__ void k_begin_lb(DevScalars* sc, double* hacc, int nPart) {
@@ -490,16 +702,14 @@ void lb_fused_step(Box& b, const Consts& k, const PartLB* plb, const ObjLB* olb,
                    hipStream_t s, hipEvent_t ev0, hipEvent_t ev1) {
     const int n = k.nPart * 8 + 1;
     hipLaunchKernelGGL(k_begin_lb, (n + 255) / 256, 256, 0, s, sc, b.hacc, k.nPart);
-    if (ev0) (void)hipEventRecord(ev0, s);
-    if (b.prec == LBDEM_PREC_FP64) launch_fused<double>(b, k, plb, olb, sc, diag, s);
-    else if (b.prec == LBDEM_PREC_FP32) launch_fused<float>(b, k, plb, olb, sc, diag, s);
-    else launch_fused<__half>(b, k, plb, olb, sc, diag, s);
-    if (ev1) (void)hipEventRecord(ev1, s);
+    if (b.prec == LBDEM_PREC_FP64) launch_fused<double>(b, k, plb, olb, sc, diag, s, ev0, ev1);
+    else if (b.prec == LBDEM_PREC_FP32) launch_fused<float>(b, k, plb, olb, sc, diag, s, ev0, ev1);
+    else launch_fused<__half>(b, k, plb, olb, sc, diag, s, ev0, ev1);
 }
 
-// deterministic sum of the per-block partials, unit conversion of all hydrodynamic results
+// deterministic sum of the flagged per-block partials, unit conversion of all hydrodynamic results
 // (LB.cpp:1966-1977, 1536-1543)
-__global__ void k_hydro_finalize(int nblk, const double* part, Consts k, const double* hacc, double* FHydro,
+__global__ void k_hydro_finalize(int nblk, const double* part, const int32_t* flag, Consts k, const double* hacc, double* FHydro,
                                  double* MHydro, double* fluidVol, DevScalars* sc, double* objFH, double* objMH,
                                  double* wallFH, int diag) {
     __shared__ double red[256];
@@ -509,7 +719,8 @@ __global__ void k_hydro_finalize(int nblk, const double* part, Consts k, const d
         const int c = blockIdx.x;
         if (c >= 7 * MAX_OBJ_PART && !diag) return;
         double acc = 0.0;
-        for (int b = tid; b < nblk; b += blockDim.x) acc += part[int64_t(c) * nblk + b];
+        for (int b = tid; b < nblk; b += blockDim.x)
+            if (flag[b]) acc += part[int64_t(c) * nblk + b];
         red[tid] = acc;
         __syncthreads();
         for (int w = blockDim.x / 2; w > 0; w >>= 1) {
@@ -547,7 +758,7 @@ void lb_hydro_finalize(const Box& b, const Consts& k, double* FHydro, double* MH
                        DevScalars* sc, double* objFH, double* objMH, double* wallFH, bool diag, hipStream_t s) {
     const int nblk = fused_grid_blocks(b.g);
     const int grid = PART_STRIDE + (k.nPart + 255) / 256;
-    hipLaunchKernelGGL(k_hydro_finalize, grid, 256, 0, s, nblk, b.blockPart, k, b.hacc, FHydro, MHydro, fluidVol,
+    hipLaunchKernelGGL(k_hydro_finalize, grid, 256, 0, s, nblk, b.blockPart, b.blockFlag, k, b.hacc, FHydro, MHydro, fluidVol,
                        sc, objFH, objMH, wallFH, diag ? 1 : 0);
 }
 
@@ -897,7 +1108,10 @@ __global__ void k_flags_apply(int64_t N, uint8_t* type, int32_t* solid, int32_t*
     }
     if (mymin != INT_MAX) atomicMin(&bmin, mymin);
     __syncthreads();
-    if (threadIdx.x == 0 && bmin != INT_MAX) atomicMin(&sc->kminP, bmin);
+    // blocks run roughly in index order: only a smaller minimum needs the global atomic
+    if (threadIdx.x == 0 && bmin != INT_MAX &&
+        bmin < __hip_atomic_load(&sc->kminP, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT))
+        atomicMin(&sc->kminP, bmin);
 }
 
 __global__ void k_flags_apply_begin(DevScalars* sc) {
@@ -932,6 +1146,10 @@ __global__ void k_fna_check(Geom g, const int32_t* solid, const PartLB* plb, int
 __global__ void k_fna_scan(Geom g, const uint8_t* type, const int32_t* solid, const PartLB* plb, int nPart,
                            DevScalars* sc) {
     if (!sc->needScan) return;
+    __shared__ int bmin;
+    if (threadIdx.x == 0) bmin = INT_MAX;
+    __syncthreads();
+    int mymin = INT_MAX;
     for (int64_t i = int64_t(blockIdx.x) * blockDim.x + threadIdx.x; i < g.N; i += int64_t(gridDim.x) * blockDim.x) {
         if (!(type[i] & T_PMASK)) continue;
         const int sp = solid[i];
@@ -939,8 +1157,13 @@ __global__ void k_fna_scan(Geom g, const uint8_t* type, const int32_t* solid, co
         int x, y, z;
         cell_xyz(g, i, x, y, z);
         const PartLB P = plb[sp];
-        if (inside_sphere(x, y, z, P.c[0], P.c[1], P.c[2], P.r)) atomicMin(&sc->kstar, int(i));
+        if (inside_sphere(x, y, z, P.c[0], P.c[1], P.c[2], P.r)) mymin = min(mymin, int(i));
     }
+    if (mymin != INT_MAX) atomicMin(&bmin, mymin);
+    __syncthreads();
+    if (threadIdx.x == 0 && bmin != INT_MAX &&
+        bmin < __hip_atomic_load(&sc->kstar, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT))
+        atomicMin(&sc->kstar, bmin);
 }
 
 __global__ void k_fna_unflag(Geom g, uint8_t* type, int32_t* solid, DevScalars* sc) {
@@ -968,11 +1191,36 @@ void lb_find_new_active(const Box& b, const Consts& k, const PartLB* plb, DevSca
 // ------------------------------------------------------------------------------------------
 // impactor nodes (updateObjectBoundaries + findNewActiveObject, LB.cpp:797-943)
 // ------------------------------------------------------------------------------------------
-// uncovered object nodes (LB.cpp:846-943). All threads of one block test the previous object
-// nodes (ascending list) against the moved sphere and compact the uncovered ones in ascending
-// order; one thread then converts them in that order, because each decision reads the node
-// types its predecessors wrote.
-constexpr int UNCOVER_THREADS = 1024;
+// uncovered object nodes (LB.cpp:846-943). The previous object nodes (ascending list) that the
+// moved sphere no longer covers are compacted in ascending order. Each conversion reads the node
+// types its predecessors in that order wrote, so the decisions are taken in order by one thread,
+// on the neighbour types staged in LDS: for every candidate its 18 neighbour types and, for
+// neighbours that are earlier candidates of the same chunk, their position in the chunk.
+// Candidates are processed in chunks of UNC_CHUNK; a chunk's results are written back before
+// the next chunk stages its neighbourhood.
+constexpr int UNCOVER_THREADS = 1024, UNC_CHUNK = 1024;
+__device__ inline int block_exclusive_scan(int v, int* sWave, int& total) {
+    const int lane = __lane_id(), wave = threadIdx.x / 64, nw = blockDim.x / 64;
+    int incl = v;
+#pragma unroll
+    for (int d = 1; d < 64; d <<= 1) {
+        const int o = __shfl_up(incl, d, 64);
+        if (lane >= d) incl += o;
+    }
+    if (lane == 63) sWave[wave] = incl;
+    __syncthreads();
+    if (threadIdx.x == 0) {
+        int acc = 0;
+        for (int w = 0; w < nw; ++w) { const int c = sWave[w]; sWave[w] = acc; acc += c; }
+        sWave[nw] = acc;
+    }
+    __syncthreads();
+    const int r = sWave[wave] + incl - v;
+    total = sWave[nw];
+    __syncthreads();
+    return r;
+}
+
 template <typename S, int L>
 __global__ __launch_bounds__(UNCOVER_THREADS) void k_obj_uncover(Geom g, S* f, uint8_t* type,
                                                                  typename Types<S>::M* mass,
@@ -980,72 +1228,105 @@ __global__ __launch_bounds__(UNCOVER_THREADS) void k_obj_uncover(Geom g, S* f, u
                                                                  int32_t* cand, DevScalars* sc, const ObjLB* olb,
                                                                  Consts k) {
     using C = typename Types<S>::C;
-    __shared__ int counts[UNCOVER_THREADS];
-    __shared__ int total;
+    __shared__ int sWave[UNCOVER_THREADS / 64 + 1];
+    __shared__ uint8_t nbT[Q - 1][UNC_CHUNK];   // neighbour base types (old values)
+    __shared__ int16_t nbC[Q - 1][UNC_CHUNK];   // neighbour's position among earlier chunk candidates, or -1
+    __shared__ uint8_t newT[UNC_CHUNK];
+    __shared__ int32_t sCand[UNC_CHUNK];
+    const int tid = threadIdx.x;
     const int n = sc->nObjList;
     const ObjLB O = olb[0];
-    const int per = (n + blockDim.x - 1) / blockDim.x;
-    const int beg = threadIdx.x * per, end = min(n, beg + per);
-    int cnt = 0;
-    for (int kk = beg; kk < end; ++kk) {
-        int x, y, z;
-        cell_xyz(g, objList[kk], x, y, z);
-        cnt += !inside_sphere(x, y, z, O.c[0], O.c[1], O.c[2], O.r);
-    }
-    counts[threadIdx.x] = cnt;
-    __syncthreads();
-    if (threadIdx.x == 0) {
-        int acc = 0;
-        for (int t = 0; t < int(blockDim.x); ++t) { const int c = counts[t]; counts[t] = acc; acc += c; }
-        total = acc;
-    }
-    __syncthreads();
-    int pos = counts[threadIdx.x];
-    for (int kk = beg; kk < end; ++kk) {
-        const int32_t i = objList[kk];
-        int x, y, z;
-        cell_xyz(g, i, x, y, z);
-        if (!inside_sphere(x, y, z, O.c[0], O.c[1], O.c[2], O.r)) cand[pos++] = i;
+    // ordered compaction of the uncovered nodes
+    int m = 0;
+    for (int base = 0; base < n; base += UNCOVER_THREADS) {
+        const int kk = base + tid;
+        int32_t i = 0;
+        int flag = 0;
+        if (kk < n) {
+            i = objList[kk];
+            int x, y, z;
+            cell_xyz(g, i, x, y, z);
+            flag = !inside_sphere(x, y, z, O.c[0], O.c[1], O.c[2], O.r);
+        }
+        int tot;
+        const int pos = block_exclusive_scan(flag, sWave, tot);
+        if (flag) cand[m + pos] = i;
+        m += tot;
     }
     __threadfence_block();
     __syncthreads();
-    if (threadIdx.x != 0) return;
-    volatile uint8_t* vt = type;
-    const int m = total;
-    for (int kk = 0; kk < m; ++kk) {
-        const int64_t i = cand[kk];
-        int x, y, z;
-        cell_xyz(g, i, x, y, z);
-        bool anyFluid = false;
-        for (int j = 1; j < Q; ++j) anyFluid |= (vt[nbr_plain(g, x, y, z, j)] & T_BASE) == T_FLUID;
-        const uint8_t pbit = vt[i] & T_PMASK;
-        if (!anyFluid) { vt[i] = pbit | T_GAS; continue; }
-        bool anyGas = false;
-        for (int j = 1; j < Q; ++j) anyGas |= (vt[nbr_plain(g, x, y, z, j)] & T_BASE) == T_GAS;
-        const bool isIf = anyGas;
-        vt[i] = pbit | (isIf ? T_IF : T_FLUID);
-        const double nn = k.rho0;
-        const double ux = k.lbF[0] / 2.0 / nn + (isIf ? 0.0 : k.initVel[0]);
-        const double uy = k.lbF[1] / 2.0 / nn + (isIf ? 0.0 : k.initVel[1]);
-        const double uz = k.lbF[2] / 2.0 / nn + (isIf ? 0.0 : k.initVel[2]);
-        const double usq = ux * ux + uy * uy + uz * uz;
-        const Offs o(g, x, y, z);
-        for (int q = 0; q < Q; ++q) {
-            const double vu = ux * cxq(q) + uy * cyq(q) + uz * czq(q);
-            const double feq = wq(q) * nn * (1.0 + 3.0 * vu + 4.5 * vu * vu - 1.5 * usq);
-            StoreT<S>::st(f + load_slot(g, i, o, q, L), C(feq - wq(q)));
+    for (int c0 = 0; c0 < m; c0 += UNC_CHUNK) {
+        const int cnt = min(UNC_CHUNK, m - c0);
+        if (tid < cnt) sCand[tid] = cand[c0 + tid];
+        __syncthreads();
+        if (tid < cnt) {
+            const int32_t i = sCand[tid];
+            int x, y, z;
+            cell_xyz(g, i, x, y, z);
+            for (int j = 1; j < Q; ++j) {
+                const int32_t nb = int32_t(nbr_plain(g, x, y, z, j));
+                nbT[j - 1][tid] = type[nb] & T_BASE;
+                // earlier candidate of this chunk? (sCand ascending)
+                int lo = 0, hi = tid;
+                while (lo < hi) {
+                    const int mid = (lo + hi) >> 1;
+                    if (sCand[mid] < nb) lo = mid + 1; else hi = mid;
+                }
+                nbC[j - 1][tid] = (lo < tid && sCand[lo] == nb) ? int16_t(lo) : int16_t(-1);
+            }
         }
-        mass[i] = typename Types<S>::M(isIf ? 0.5 * k.rho0 : k.rho0);
-        aux[i] = C(nn);
-        aux[g.N + i] = C(ux); aux[2 * g.N + i] = C(uy); aux[3 * g.N + i] = C(uz);
-        aux[4 * g.N + i] = C(0); aux[5 * g.N + i] = C(0); aux[6 * g.N + i] = C(0);
+        __syncthreads();
+        if (tid == 0) {
+            for (int t = 0; t < cnt; ++t) {
+                bool anyFluid = false, anyGas = false;
+                for (int j = 0; j < Q - 1; ++j) {
+                    const int c = nbC[j][t];
+                    const uint8_t ty = c >= 0 ? newT[c] : nbT[j][t];
+                    anyFluid |= ty == T_FLUID;
+                    anyGas |= ty == T_GAS;
+                }
+                newT[t] = !anyFluid ? uint8_t(T_GAS) : (anyGas ? uint8_t(T_IF) : uint8_t(T_FLUID));
+            }
+        }
+        __syncthreads();
+        if (tid < cnt) {
+            const int32_t i = sCand[tid];
+            const uint8_t nt = newT[tid];
+            type[i] = uint8_t((type[i] & T_PMASK) | nt);
+            if (nt != T_GAS) {
+                const bool isIf = nt == T_IF;
+                int x, y, z;
+                cell_xyz(g, i, x, y, z);
+                const double nn = k.rho0;
+                const double ux = k.lbF[0] / 2.0 / nn + (isIf ? 0.0 : k.initVel[0]);
+                const double uy = k.lbF[1] / 2.0 / nn + (isIf ? 0.0 : k.initVel[1]);
+                const double uz = k.lbF[2] / 2.0 / nn + (isIf ? 0.0 : k.initVel[2]);
+                const double usq = ux * ux + uy * uy + uz * uz;
+                const Offs o(g, x, y, z);
+                for (int q = 0; q < Q; ++q) {
+                    const double vu = ux * cxq(q) + uy * cyq(q) + uz * czq(q);
+                    const double feq = wq(q) * nn * (1.0 + 3.0 * vu + 4.5 * vu * vu - 1.5 * usq);
+                    StoreT<S>::st(f + load_slot(g, i, o, q, L), C(feq - wq(q)));
+                }
+                mass[i] = typename Types<S>::M(isIf ? 0.5 * k.rho0 : k.rho0);
+                aux[i] = C(nn);
+                aux[g.N + i] = C(ux); aux[2 * g.N + i] = C(uy); aux[3 * g.N + i] = C(uz);
+                aux[4 * g.N + i] = C(0); aux[5 * g.N + i] = C(0); aux[6 * g.N + i] = C(0);
+            }
+        }
+        __threadfence_block();
+        __syncthreads();
     }
 }
 
-// cover: every node inside an object sphere becomes an object node; ordered object node list
-__global__ void k_obj_cover(Geom g, uint8_t* type, int32_t* solid, const ObjLB* olb, int nObj, int32_t* objList, int cap,
-                            DevScalars* sc) {
-    __shared__ int counts[1024];
+// cover: every node inside an object sphere becomes an object node; ascending object node list.
+// Pass a covers the union box of the spheres and counts per block (blocks own consecutive
+// ranges of the box); pass b writes the list at the blocks' exclusive prefix offsets.
+constexpr int COVER_GRID = 128, COVER_BS = 256;
+struct CoverBox {
+    int lo[3], n[3], tot;
+};
+__device__ inline CoverBox cover_box(const Geom& g, const ObjLB* olb, int nObj) {
     int lo[3] = {INT_MAX, INT_MAX, INT_MAX}, hi[3] = {INT_MIN, INT_MIN, INT_MIN};
     const int dims[3] = {g.X, g.Y, g.Z};
     for (int o = 0; o < nObj; ++o)
@@ -1054,43 +1335,75 @@ __global__ void k_obj_cover(Geom g, uint8_t* type, int32_t* solid, const ObjLB* 
             lo[a] = min(lo[a], max(l, 0));
             hi[a] = max(hi[a], min(h, dims[a] - 1));
         }
-    const int lo0 = lo[0], lo1 = lo[1], lo2 = lo[2];
-    const int nx = max(hi[0] - lo0 + 1, 0), ny = max(hi[1] - lo1 + 1, 0), nz = max(hi[2] - lo2 + 1, 0);
-    const int tot = nx * ny * nz;
-    const int per = (tot + blockDim.x - 1) / blockDim.x;
-    const int beg = threadIdx.x * per, end = min(tot, beg + per);
+    CoverBox b;
+    for (int a = 0; a < 3; ++a) { b.lo[a] = lo[a]; b.n[a] = max(hi[a] - lo[a] + 1, 0); }
+    b.tot = b.n[0] * b.n[1] * b.n[2];
+    return b;
+}
+__device__ inline int inside_which(int x, int y, int z, const ObjLB* olb, int nObj) {
+    int which = -1;
+    for (int o = 0; o < nObj; ++o)
+        if (inside_sphere(x, y, z, olb[o].c[0], olb[o].c[1], olb[o].c[2], olb[o].r)) which = o;
+    return which;
+}
+__global__ __launch_bounds__(COVER_BS) void k_obj_cover_a(Geom g, uint8_t* type, int32_t* solid, const ObjLB* olb,
+                                                          int nObj, int* blockCount) {
+    const CoverBox cb = cover_box(g, olb, nObj);
+    const int per = (cb.tot + gridDim.x - 1) / gridDim.x;
+    const int beg = blockIdx.x * per, end = min(cb.tot, beg + per);
     int cnt = 0;
-    for (int c = beg; c < end; ++c) {
-        const int x = lo0 + c % nx, y = lo1 + (c / nx) % ny, z = lo2 + c / (nx * ny);
-        int which = -1;
-        for (int o = 0; o < nObj; ++o)
-            if (inside_sphere(x, y, z, olb[o].c[0], olb[o].c[1], olb[o].c[2], olb[o].r)) which = o;
-        if (which >= 0) {
-            const int64_t i = cidx(g, x, y, z);
-            type[i] = (type[i] & T_PMASK) | T_OBJ;
-            solid[i] = which;
-            cnt++;
+    for (int c0 = beg; c0 < end; c0 += COVER_BS) {
+        const int c = c0 + threadIdx.x;
+        int in = 0;
+        if (c < end) {
+            const int x = cb.lo[0] + c % cb.n[0], y = cb.lo[1] + (c / cb.n[0]) % cb.n[1], z = cb.lo[2] + c / (cb.n[0] * cb.n[1]);
+            const int which = inside_which(x, y, z, olb, nObj);
+            if (which >= 0) {
+                const int64_t i = cidx(g, x, y, z);
+                type[i] = uint8_t((type[i] & T_PMASK) | T_OBJ);
+                solid[i] = which;
+                in = 1;
+            }
         }
+        cnt += __syncthreads_count(in);
     }
-    counts[threadIdx.x] = cnt;
-    __syncthreads();
+    if (threadIdx.x == 0) blockCount[blockIdx.x] = cnt;
+}
+__global__ __launch_bounds__(COVER_BS) void k_obj_cover_b(Geom g, const uint8_t* type, const ObjLB* olb, int nObj,
+                                                          const int* blockCount, int32_t* objList, int cap,
+                                                          DevScalars* sc) {
+    __shared__ int sWave[COVER_BS / 64 + 1];
+    __shared__ int sOff;
+    const CoverBox cb = cover_box(g, olb, nObj);
     if (threadIdx.x == 0) {
-        int acc = 0;
-        for (int t = 0; t < int(blockDim.x); ++t) { const int c = counts[t]; counts[t] = acc; acc += c; }
-        sc->nObjList = acc < cap ? acc : cap;
-        if (acc > cap) sc->listOverflow |= 1;
+        int off = 0, tot = 0;
+        for (int b = 0; b < int(gridDim.x); ++b) {
+            if (b == int(blockIdx.x)) off = tot;
+            tot += blockCount[b];
+        }
+        sOff = off;
+        if (blockIdx.x == 0) {
+            sc->nObjList = tot < cap ? tot : cap;
+            if (tot > cap) sc->listOverflow |= 1;
+        }
     }
     __syncthreads();
-    int pos = counts[threadIdx.x];
-    for (int c = beg; c < end; ++c) {
-        const int x = lo0 + c % nx, y = lo1 + (c / nx) % ny, z = lo2 + c / (nx * ny);
-        const int64_t i = cidx(g, x, y, z);
-        if ((type[i] & T_BASE) == T_OBJ) {
-            bool in = false;
-            for (int o = 0; o < nObj && !in; ++o)
-                in = inside_sphere(x, y, z, olb[o].c[0], olb[o].c[1], olb[o].c[2], olb[o].r);
-            if (in) { if (pos < cap) objList[pos] = int32_t(i); pos++; }
+    int off = sOff;
+    const int per = (cb.tot + gridDim.x - 1) / gridDim.x;
+    const int beg = blockIdx.x * per, end = min(cb.tot, beg + per);
+    for (int c0 = beg; c0 < end; c0 += COVER_BS) {
+        const int c = c0 + threadIdx.x;
+        int in = 0;
+        int64_t i = 0;
+        if (c < end) {
+            const int x = cb.lo[0] + c % cb.n[0], y = cb.lo[1] + (c / cb.n[0]) % cb.n[1], z = cb.lo[2] + c / (cb.n[0] * cb.n[1]);
+            i = cidx(g, x, y, z);
+            in = (type[i] & T_BASE) == T_OBJ && inside_which(x, y, z, olb, nObj) >= 0;
         }
+        int tot;
+        const int pos = block_exclusive_scan(in, sWave, tot);
+        if (in && off + pos < cap) objList[off + pos] = int32_t(i);
+        off += tot;
     }
 }
 
@@ -1104,7 +1417,9 @@ static void obj_impl(Box& b, const Consts& k, const ObjLB* olb, DevScalars* sc, 
     else
         hipLaunchKernelGGL((k_obj_uncover<S, 1>), 1, UNCOVER_THREADS, 0, s, b.g, (S*)b.f, b.type, (M*)b.mass, (C*)b.aux,
                            b.objList, b.objCand, sc, olb, k);
-    hipLaunchKernelGGL(k_obj_cover, 1, 1024, 0, s, b.g, b.type, b.solid, olb, k.nObj, b.objList, b.objCap, sc);
+    hipLaunchKernelGGL(k_obj_cover_a, COVER_GRID, COVER_BS, 0, s, b.g, b.type, b.solid, olb, k.nObj, b.coverCount);
+    hipLaunchKernelGGL(k_obj_cover_b, COVER_GRID, COVER_BS, 0, s, b.g, b.type, olb, k.nObj, b.coverCount, b.objList,
+                       b.objCap, sc);
 }
 
 void lb_object_update(Box& b, const Consts& k, const ObjLB* olb, DevScalars* sc, hipStream_t s) {

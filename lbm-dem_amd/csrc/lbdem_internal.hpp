@@ -47,10 +47,18 @@ __host__ __device__ constexpr int oppq(int q) {
 }
 __host__ __device__ constexpr double wq(int q) { return q == 0 ? 12.0 / 36.0 : (q < 7 ? 2.0 / 36.0 : 1.0 / 36.0); }
 
-// type byte: bits 0-5 node type, bit 6 'edge' (some neighbour is not active or the node wraps
-// periodically; refreshed before every lattice step), bit 7 inside-particle flag
+// type byte: bits 0-3 node type; active nodes: bit 6 'edge' (interface node, a neighbour is not
+// active or the node wraps periodically; refreshed before every lattice step), bit 7
+// inside-particle flag; wall nodes: bits 4-6 the wall that receives the node's link forces
+// (its solid index when that is a wall index, else 0; cleared with the solid index when the
+// particle flags are rebuilt)
 enum : uint8_t { T_FLUID = 0, T_GAS = 2, T_IF = 3, T_PER = 4, T_WALL = 7, T_OBJ = 12, T_EDGE = 0x40, T_PMASK = 0x80,
-                 T_BASE = 0x3F };
+                 T_BASE = 0x0F, T_WSHIFT = 4 };
+// Boundary-link classes of an edge node, 2 bits per direction j = 1..18 at bit 2(j-1):
+// 0 active neighbour (no boundary), 1 gas, 2 wall receiving the node's primary wall index
+// (bits 36-38), 3 anything else (object node, another wall, unexpected type: resolved from the
+// neighbour's type and solid index).
+enum : int { BC_NONE = 0, BC_GAS = 1, BC_WALL = 2, BC_OTHER = 3, BC_WALL_SHIFT = 36 };
 
 // Geometry of one box: sizes and which faces wrap (reference LB.cpp:85-126, 632-678:
 // an active node wraps along an axis when its face neighbour on that side is periodic).
@@ -155,8 +163,11 @@ struct Box {
     int32_t* flagNew;          // N, -1 = none (particle flags computed at a rebuild)
     int32_t* objList;          // object nodes, ascending
     int32_t* objCand;          // uncovered object nodes of the current step, ascending
+    int* coverCount;           // per-block counts of the object cover pass
     int ifCap, objCap;
     double* blockPart;         // per-block partials of the fused kernel (object F/M, extra mass, walls)
+    int32_t* blockFlag;        // per block: partials written
+    uint64_t* bcm;             // N, boundary-link classes of edge nodes (valid where T_EDGE is set)
     int blockPartCap;
     double* hacc;              // per-particle hydro accumulators: F3 M3 vol (lattice units)
     int parity;                // 0: populations in natural layout, 1: swapped
