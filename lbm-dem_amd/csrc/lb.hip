@@ -241,54 +241,91 @@ __device__ inline typename Types<S>::C boundary_value(const FusedArgs<S>& a, int
 __device__ inline int32_t plain_off(int q, int32_t sy, int32_t sz) { return cxq(q) + cyq(q) * sy + czq(q) * sz; }
 
 // populations of node i before collision (its post-streaming values). PAR 0: the node's own
-// slots; PAR 1: the slots its neighbours wrote, at wrapped offsets when the wave holds a node
-// next to a periodic face (for every other node the wrapped offsets equal the plain ones).
-// Called by all lanes (the wave-wide vote decides the offset form).
+// slots; PAR 1: the slots its neighbours wrote, at plain offsets (links that wrap through a
+// periodic face reach the face's ghost layer, filled before the step; see k_ghost)
 template <typename S, int PAR>
-__device__ inline void load_pop(const FusedArgs<S>& a, int32_t i, bool act, bool wrap, int x, int y, int z,
-                                typename Types<S>::C* gq) {
+__device__ inline void load_pop(const FusedArgs<S>& a, int32_t i, bool act, typename Types<S>::C* gq) {
+    if (!act) return;
     const int32_t N = int32_t(a.g.N), sy = a.g.X, sz = a.g.X * a.g.Y;
     if (PAR == 0) {
-        if (act) {
 #pragma unroll
-            for (int q = 0; q < Q; ++q) gq[q] = StoreT<S>::ld(a.f + q * N + i);
-        }
-    } else if (__ballot(act && wrap)) {
-        if (act) {
-            const Offs o(a.g, x, y, z);
-#pragma unroll
-            for (int q = 0; q < Q; ++q) gq[q] = StoreT<S>::ld(a.f + oppq(q) * N + i + o.d(oppq(q)));
-        }
-    } else if (act) {
+        for (int q = 0; q < Q; ++q) gq[q] = StoreT<S>::ld(a.f + q * N + i);
+    } else {
 #pragma unroll
         for (int q = 0; q < Q; ++q) gq[q] = StoreT<S>::ld(a.f + oppq(q) * N + i + plain_off(oppq(q), sy, sz));
     }
 }
 
 // streaming of the post-collision values (LB.cpp:1375-1389): PAR 0 into the node's own slots
-// (opposite direction), PAR 1 into the neighbour's slots
+// (opposite direction), PAR 1 into the neighbours' slots
 template <typename S, int PAR>
-__device__ inline void store_pop(const FusedArgs<S>& a, int32_t i, bool act, bool wrap, int x, int y, int z,
-                                 const typename Types<S>::C* gq) {
+__device__ inline void store_pop(const FusedArgs<S>& a, int32_t i, bool act, const typename Types<S>::C* gq) {
+    if (!act) return;
     const int32_t N = int32_t(a.g.N), sy = a.g.X, sz = a.g.X * a.g.Y;
+    StoreT<S>::st(a.f + i, gq[0]);
     if (PAR == 0) {
-        if (act) {
-            StoreT<S>::st(a.f + i, gq[0]);
 #pragma unroll
-            for (int j = 1; j < Q; ++j) StoreT<S>::st(a.f + oppq(j) * N + i, gq[j]);
-        }
-    } else if (__ballot(act && wrap)) {
-        if (act) {
-            const Offs o(a.g, x, y, z);
-            StoreT<S>::st(a.f + i, gq[0]);
-#pragma unroll
-            for (int j = 1; j < Q; ++j) StoreT<S>::st(a.f + j * N + i + o.d(j), gq[j]);
-        }
-    } else if (act) {
-        StoreT<S>::st(a.f + i, gq[0]);
+        for (int j = 1; j < Q; ++j) StoreT<S>::st(a.f + oppq(j) * N + i, gq[j]);
+    } else {
 #pragma unroll
         for (int j = 1; j < Q; ++j) StoreT<S>::st(a.f + j * N + i + plain_off(j, sy, sz), gq[j]);
     }
+}
+
+// Ghost layers of periodic faces for the PAR 1 step. A node next to a periodic face reads and
+// writes, at plain offsets, slots of the face's boundary layer (coordinate 0 or D-1 on that
+// axis); the wrapped partner (coordinate D-2 or 1, other axes wrapped) holds the real slots
+// (LB.cpp:632-678). Before the step (PRE) the boundary layer receives the partner's slots of the
+// directions pointing out through the face (every slot a node can read there; each ghost slot has
+// one wrapped source, so overlapping faces write equal values). After the step (POST) the slots
+// that interior nodes wrote there go back to the partner; only slots whose writer is an active
+// interior node are copied, so every real slot has one source.
+__device__ inline int wrap_coord(int u, int D, bool perLo, bool perHi) {
+    return (u == 0 && perLo) ? D - 2 : ((u == D - 1 && perHi) ? 1 : u);
+}
+template <typename S, bool PRE>
+__global__ void k_ghost(Geom g, S* f, const uint8_t* type, int axis, int side) {
+    const int D[3] = {g.X, g.Y, g.Z};
+    const int a1 = (axis + 1) % 3, a2 = (axis + 2) % 3;
+    const int64_t nFace = int64_t(D[a1]) * D[a2];
+    const int64_t t = int64_t(blockIdx.x) * blockDim.x + threadIdx.x;
+    if (t >= nFace) return;
+    const int u = int(t % D[a1]), v = int(t / D[a1]);
+    int gp[3], pp[3];
+    gp[axis] = side < 0 ? 0 : D[axis] - 1;
+    pp[axis] = side < 0 ? D[axis] - 2 : 1;
+    gp[a1] = u; gp[a2] = v;
+    pp[a1] = wrap_coord(u, D[a1], g.per[2 * a1], g.per[2 * a1 + 1]);
+    pp[a2] = wrap_coord(v, D[a2], g.per[2 * a2], g.per[2 * a2 + 1]);
+    const int64_t gi = cidx(g, gp[0], gp[1], gp[2]), pi = cidx(g, pp[0], pp[1], pp[2]);
+    for (int d = 1; d < Q; ++d) {
+        const int c[3] = {cxq(d), cyq(d), czq(d)};
+        if (c[axis] != side) continue;
+        if (PRE) {
+            f[int64_t(d) * g.N + gi] = f[int64_t(d) * g.N + pi];
+        } else {
+            // writer of ghost slot d: the node at gp - c_d, which must be an interior active node
+            // (otherwise the partner may hold a boundary value of its own in that slot)
+            int wp[3];
+            wp[axis] = gp[axis] - c[axis]; wp[a1] = gp[a1] - c[a1]; wp[a2] = gp[a2] - c[a2];
+            if (wp[a1] < 1 || wp[a1] > D[a1] - 2 || wp[a2] < 1 || wp[a2] > D[a2] - 2) continue;
+            if (!is_active(type[cidx(g, wp[0], wp[1], wp[2])])) continue;
+            f[int64_t(d) * g.N + pi] = f[int64_t(d) * g.N + gi];
+        }
+    }
+}
+template <typename S>
+static void launch_ghost(const Box& b, bool pre, hipStream_t s) {
+    const int D[3] = {b.g.X, b.g.Y, b.g.Z};
+    for (int axis = 0; axis < 3; ++axis)
+        for (int sd = 0; sd < 2; ++sd) {
+            if (!b.g.per[2 * axis + sd]) continue;
+            const int64_t nFace = int64_t(D[(axis + 1) % 3]) * D[(axis + 2) % 3];
+            const int side = sd == 0 ? -1 : 1;
+            const int grid = int((nFace + 255) / 256);
+            if (pre) hipLaunchKernelGGL((k_ghost<S, true>), grid, 256, 0, s, b.g, (S*)b.f, b.type, axis, side);
+            else hipLaunchKernelGGL((k_ghost<S, false>), grid, 256, 0, s, b.g, (S*)b.f, b.type, axis, side);
+        }
 }
 
 // one node's collision (node.cpp:75-207): reconstruct, interior forcing of particle nodes
@@ -450,10 +487,6 @@ __global__ __launch_bounds__(FX* FY) void k_fused(FusedArgs<S> a) {
     const int lane = __lane_id();
     const bool col_ok = tile_ok && x < g.X && y < g.Y;
     const int z0 = bz * FZ;
-    // links of nodes next to a periodic face wrap
-    const bool wrapXY = (x == 1 && g.per[0]) || (x == g.X - 2 && g.per[1]) || (y == 1 && g.per[2]) ||
-                        (y == g.Y - 2 && g.per[3]);
-    auto wraps = [&](int z) { return wrapXY || (z == 1 && g.per[4]) || (z == g.Z - 2 && g.per[5]); };
     auto type_at = [&](int zz) -> uint8_t {
         return (col_ok && zz < FZ && z0 + zz < g.Z) ? a.type[int32_t(cidx(g, x, y, z0 + zz))] : uint8_t(T_WALL);
     };
@@ -464,7 +497,7 @@ __global__ __launch_bounds__(FX* FY) void k_fused(FusedArgs<S> a) {
     int32_t ic = col_ok ? int32_t(cidx(g, x, y, z0)) : 0;
     uint8_t tc = type_at(0), tn = type_at(1);
     bool actc = is_active(tc);
-    load_pop<S, PAR>(a, ic, actc, wraps(z0), x, y, z0, gc);
+    load_pop<S, PAR>(a, ic, actc, gc);
     double mc = (actc && (tc & (T_PMASK | T_EDGE))) ? double(a.mass[ic]) : 0.0;
     int32_t sc_ = (actc && (tc & T_PMASK)) ? a.solid[ic] : 0;
     uint64_t bc_ = (actc && (tc & T_EDGE)) ? a.bcm[ic] : 0;
@@ -474,7 +507,7 @@ __global__ __launch_bounds__(FX* FY) void k_fused(FusedArgs<S> a) {
         const int32_t in = ic + SZ;
         const bool actn = is_active(tn);
         const uint8_t tnn = type_at(zz + 2);
-        load_pop<S, PAR>(a, in, actn, wraps(z + 1), x, y, z + 1, gn);
+        load_pop<S, PAR>(a, in, actn, gn);
         const double mn = (actn && (tn & (T_PMASK | T_EDGE))) ? double(a.mass[in]) : 0.0;
         const int32_t sn = (actn && (tn & T_PMASK)) ? a.solid[in] : 0;
         const uint64_t bn = (actn && (tn & T_EDGE)) ? a.bcm[in] : 0;
@@ -485,7 +518,7 @@ __global__ __launch_bounds__(FX* FY) void k_fused(FusedArgs<S> a) {
         for (int c = 0; c < 7; ++c) r.d[c] = 0;
         r.n = 1; r.sx = 0; r.sy = 0; r.sz = 0; r.usq = 0;
         if (actc) collide_cell<S>(a, sc_, tc, x, y, z, mc, gc, r);
-        store_pop<S, PAR>(a, ic, actc, wraps(z), x, y, z, gc);
+        store_pop<S, PAR>(a, ic, actc, gc);
 
         // boundary links of edge nodes (LB.cpp:1390-1528): the value reflected into the slot
         // this node loads next step
@@ -579,8 +612,9 @@ __global__ __launch_bounds__(FX* FY) void k_fused(FusedArgs<S> a) {
     if (tid == 0) a.blockFlag[b0] = any;
 }
 
-// Edge bits (T_EDGE) of active nodes: an interface node, a non-active neighbour, or a node next
-// to a periodic face; for edge nodes also the boundary-link classes. One thread per four nodes
+// Edge bits (T_EDGE) of active nodes: some plain neighbour is not active (a node next to a
+// periodic face sees the face's boundary layer); for edge nodes also the boundary-link classes,
+// over the wrapped neighbours. One thread per four nodes
 // of a row (one 32-bit word of type bytes); rows must be a multiple of four bytes long.
 // Neighbour rows are read as words; the test uses three
 // masks of non-active bytes: the node's own row (x +- 1), the four rows sharing one axis with it
@@ -616,16 +650,13 @@ __global__ void k_mark_edges_w(Geom g, uint8_t* type, uint64_t* bcm) {
     const uint32_t side = window(c0 - RW) | window(c0 + RW) | window(c0 - SW) | window(c0 + SW);
     const uint32_t diag = (inactive_bytes(t32[c0 - RW - SW]) | inactive_bytes(t32[c0 - RW + SW]) |
                            inactive_bytes(t32[c0 + RW - SW]) | inactive_bytes(t32[c0 + RW + SW]));
-    const bool perRow = (y == 1 && g.per[2]) || (y == g.Y - 2 && g.per[3]) || (z == 1 && g.per[4]) ||
-                        (z == g.Z - 2 && g.per[5]);
     uint32_t nw = cw;
 #pragma unroll
     for (int k = 0; k < 4; ++k) {
         const uint8_t t = uint8_t(cw >> (8 * k));
         if (!is_active(t)) continue;
         const int x = 4 * w + k;
-        const bool edge = perRow || (x == 1 && g.per[0]) || (x == g.X - 2 && g.per[1]) || (t & T_BASE) == T_IF ||
-                          ((own >> k) & 5u) || ((side >> k) & 7u) || ((diag >> k) & 1u);
+        const bool edge = ((own >> k) & 5u) || ((side >> k) & 7u) || ((diag >> k) & 1u);
         const uint8_t nt = edge ? uint8_t(t | T_EDGE) : uint8_t(t & ~T_EDGE);
         nw = (nw & ~(0xFFu << (8 * k))) | (uint32_t(nt) << (8 * k));
         if (edge) bcm[c0 * 4 + k] = link_classes(g, type, int32_t(c0 * 4 + k), x, y, z);
@@ -641,9 +672,7 @@ __global__ void k_mark_edges_b(Geom g, uint8_t* type, uint64_t* bcm) {
     if (!is_active(t)) return;
     int x, y, z;
     cell_xyz(g, i, x, y, z);
-    bool edge = (x == 1 && g.per[0]) || (x == g.X - 2 && g.per[1]) || (y == 1 && g.per[2]) ||
-                (y == g.Y - 2 && g.per[3]) || (z == 1 && g.per[4]) || (z == g.Z - 2 && g.per[5]) ||
-                (t & T_BASE) == T_IF;
+    bool edge = false;
     const int32_t sy = g.X, sz = g.X * g.Y;
 #pragma unroll
     for (int q = 1; q < Q; ++q) edge |= !is_active(type[i + plain_off(q, sy, sz)]);
@@ -678,6 +707,8 @@ static void launch_fused(Box& b, const Consts& k, const PartLB* plb, const ObjLB
     a.nbx = (b.g.X + FX - 1) / FX; a.nby = (b.g.Y + FY - 1) / FY; a.nbz = (b.g.Z + FZ - 1) / FZ;
     const int grid = fused_grid_blocks(b.g);
     const bool fs = k.freeSurface != 0;
+    const bool ghosts = b.parity == 1 && (b.g.per[0] || b.g.per[1] || b.g.per[2] || b.g.per[3] || b.g.per[4] || b.g.per[5]);
+    if (ghosts) launch_ghost<S>(b, true, s);
     if (ev0) (void)hipEventRecord(ev0, s);
 #define LBDEM_FUSED(P, F, D) hipLaunchKernelGGL((k_fused<S, P, F, D>), dim3(grid), dim3(FX, FY, 1), 0, s, a)
     if (b.parity == 0) {
@@ -689,6 +720,7 @@ static void launch_fused(Box& b, const Consts& k, const PartLB* plb, const ObjLB
     }
 #undef LBDEM_FUSED
     if (ev1) (void)hipEventRecord(ev1, s);
+    if (ghosts) launch_ghost<S>(b, false, s);
     b.parity ^= 1;
 }
 
