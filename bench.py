@@ -1,10 +1,10 @@
 #!/usr/bin/env python3
 """Benchmark of the coupled LB/DEM step on one MI355X.
 
-Workload (N=1): the headline impact run — a 256 x 256 x 320 closed box (x, z horizontal,
-y vertical), fluid to height 256 with a free surface, 31 000 spheres of radius 4 (phi ~ 0.5,
-seeded packing), an impactor of radius 16 at height 290 falling at 0.01 lattice units per
-step, tau = 0.8. A step is one full coupled step (DEM, coupling, fused LB collide-stream,
+Workload (N=1): the headline impact run — a 256 x 256 x 320 box (x, z horizontal and periodic,
+y vertical with floor and lid walls), fluid to height 256 with a free surface, 31 000 spheres of
+radius 4 (phi ~ 0.5, seeded packing), an impactor of radius 16 at height 290 falling at 0.01
+lattice units per step, tau = 0.8. A step is one full coupled step (DEM, coupling, fused LB collide-stream,
 free surface). Inputs are resident on the device when the timed region starts.
 
 Prints one JSON line: MLUPS over active (fluid + interface) cells for the FP32-offset step, the
@@ -88,7 +88,8 @@ def cpu_baseline(seconds_budget=25.0):
         size = (64, 80, 64)
         n = int(round(0.5 * (size[0] - 2) * (64 - 1) * (size[2] - 2) / (4.0 / 3.0 * np.pi * 64)))
         vals, ps, obs = workloads.impact_box(size=size, fluid_height=64.0, n_spheres=n, r=4.0, r_imp=8.0,
-                                             y_imp=74.0, speed=0.01, seed=7, relax=300)
+                                             y_imp=74.0, speed=0.01, seed=7, relax=300,
+                                             periodic=True)
         d = os.path.join(tmp, "case")
         os.makedirs(os.path.join(d, "input"))
         lbdem.write_getpot(os.path.join(d, "config.cfg"), vals)
@@ -160,7 +161,7 @@ def main():
         "data": "synthetic (seeded sphere packing and impactor, lattice units)",
         "config": {
             "workload": "impact run, box 256x256 horizontal x 320 vertical, fluid to 256, 31000 spheres r=4 "
-                        f"(phi={phi:.3f}), impactor r=16 at 290, vy=-0.01, tau=0.8, closed box"
+                        f"(phi={phi:.3f}), impactor r=16 at 290, vy=-0.01, tau=0.8, x/z periodic, floor and lid walls"
                         if not args.small else "quarter-size impact run (128x128x160, 3800 spheres)",
             "precision": PREC_NAME[prec],
             "active_cells": int(main_res["active_cells"]),

@@ -341,6 +341,8 @@ extern "C" int lbdem_init(lbdem_ctx* c) {
     b.objList = c->dalloc<int32_t>(b.objCap);
     b.objCand = c->dalloc<int32_t>(b.objCap);
     b.coverCount = c->dalloc<int>(256);
+    b.changedCap = b.ifCap + b.objCap;
+    b.changed = c->dalloc<int32_t>(b.changedCap);
     b.blockPartCap = fused_grid_blocks(g) * PART_STRIDE_;
     b.blockPart = c->dalloc<double>(b.blockPartCap);
     b.blockFlag = c->dalloc<int32_t>(fused_grid_blocks(g));
@@ -349,7 +351,7 @@ extern "C" int lbdem_init(lbdem_ctx* c) {
     b.parity = 0;
     c->sc = c->dalloc<DevScalars>(1);
     if (!b.f || !b.type || !b.solid || !b.mass || !b.aux || !b.ifList || !b.ifList2 || !b.ifNewMass || !b.filled ||
-        !b.emptied || !b.newIf || !b.donor || !b.flagNew || !b.objList || !b.objCand || !b.coverCount || !b.blockFlag || !b.bcm || !b.blockPart || !b.hacc || !c->sc)
+        !b.emptied || !b.newIf || !b.donor || !b.flagNew || !b.objList || !b.objCand || !b.coverCount || !b.changed || !b.blockFlag || !b.bcm || !b.blockPart || !b.hacc || !c->sc)
         return c->fail(LBDEM_E_NOMEM, "device allocation failed");
     hipStream_t st = c->stream;
     HIPCHK(hipMemsetAsync(b.f, 0, size_t(Q) * g.N * selt, st));
@@ -368,6 +370,7 @@ extern "C" int lbdem_init(lbdem_ctx* c) {
     hs.nObjList = int(objList.size());
     hs.kminP = -1;
     hs.newNeighborList = 1;
+    hs.fullMark = 1;
 
     // DEM device state
     DemState& d = c->ds;
@@ -504,7 +507,7 @@ static void enqueue_lb(lbdem_ctx* c) {  // LB::latticeBolzmannStep (LB.cpp:400-4
         }
         if (c->evPool.size() >= c->evUsed + 2) { e0 = c->evPool[c->evUsed]; e1 = c->evPool[c->evUsed + 1]; c->evUsed += 2; }
     }
-    lb_mark_edges(c->box, c->stream);
+    lb_mark_edges(c->box, c->sc, c->stream);
     lb_fused_step(c->box, c->k, c->ds.plb, c->ds.olb, c->sc, true, c->stream, e0, e1);
     lb_hydro_finalize(c->box, c->k, c->ds.FHydro, c->ds.MHydro, c->ds.fluidVol, c->sc, c->ds.oFHydro, c->ds.oMHydro,
                       c->ds.wFHydro, true, c->stream);

@@ -454,8 +454,13 @@ __device__ inline uint64_t link_classes(const Geom& g, const uint8_t* type, int3
 // z+1). Edge nodes are handled by the same waves as their row neighbours, so every population
 // access stays coalesced; their boundary links run in a rolled loop over the precomputed link
 // classes, without further loads for gas and wall links.
+// waves per SIMD the fused kernel is compiled for: 3 for 4- and 2-byte storage (at most 168
+// VGPRs), 2 for FP64
+template <typename S> struct FusedOcc { static constexpr int value = 3; };
+template <> struct FusedOcc<double> { static constexpr int value = 2; };
+
 template <typename S, int PAR, bool FSURF, bool DIAG>
-__global__ __launch_bounds__(FX* FY) void k_fused(FusedArgs<S> a) {
+__global__ __launch_bounds__(FX* FY, FusedOcc<S>::value) void k_fused(FusedArgs<S> a) {
     using C = typename Types<S>::C;
     using M = typename Types<S>::M;
     constexpr int NT = FX * FY;
@@ -554,8 +559,9 @@ __global__ __launch_bounds__(FX* FY) void k_fused(FusedArgs<S> a) {
                     }
                     bcv = gj;
                 } else {
-                    bcv = boundary_value<S, DIAG>(a, j, yj, uint8_t(a.type[yj] & T_BASE), gj, r.n, r.sx, r.sy,
-                                                  r.sz, r.usq, mc, sPart);
+                    const uint8_t ty = a.type[yj] & T_BASE;
+                    if (ty == T_FLUID || ty == T_IF) continue;  // an object node uncovered since marking
+                    bcv = boundary_value<S, DIAG>(a, j, yj, ty, gj, r.n, r.sx, r.sy, r.sz, r.usq, mc, sPart);
                 }
                 if (PAR == 0) StoreT<S>::st(a.f + j * N + yj, bcv);
                 else StoreT<S>::st(a.f + opp_rt(j) * N + ic, bcv);
@@ -612,82 +618,125 @@ __global__ __launch_bounds__(FX* FY) void k_fused(FusedArgs<S> a) {
     if (tid == 0) a.blockFlag[b0] = any;
 }
 
-// Edge bits (T_EDGE) of active nodes: some plain neighbour is not active (a node next to a
-// periodic face sees the face's boundary layer); for edge nodes also the boundary-link classes,
-// over the wrapped neighbours. One thread per four nodes
-// of a row (one 32-bit word of type bytes); rows must be a multiple of four bytes long.
-// Neighbour rows are read as words; the test uses three
-// masks of non-active bytes: the node's own row (x +- 1), the four rows sharing one axis with it
-// (x - 1 .. x + 1) and the four diagonal rows (same x) — the D3Q19 stencil.
+// Edge bits (T_EDGE) and boundary-link classes. An active node is an edge node when some wrapped
+// neighbour (D3Q19 stencil) is not active. The full pass runs at initialisation, after a particle
+// flag rebuild that cleared wall indices, and when the change list overflowed; otherwise only the
+// stencil neighbourhoods of the nodes on the change list are re-marked (only transitions between
+// active and non-active types, or among non-active types, change edge bits or classes).
+//
+// Full pass, rows of a multiple of four bytes: one thread per four nodes of a row (one 32-bit
+// word of type bytes). Neighbour rows (wrapped through periodic faces) are read as words; the test
+// uses three masks of non-active bytes: the node's own row (x - 1, x + 1), the four rows sharing
+// one axis with it (x - 1 .. x + 1) and the four diagonal rows (same x).
 __device__ inline uint32_t inactive_bytes(uint32_t w) {
-    uint32_t m = 0;
-#pragma unroll
-    for (int k = 0; k < 4; ++k) m |= uint32_t(!is_active(uint8_t(w >> (8 * k)))) << k;
-    return m;
+    // a byte is active when its type is 0 (fluid) or 3 (interface); types fit in 4 bits, so the
+    // per-byte additions below cannot carry into the next byte
+    const uint32_t base = w & 0x0F0F0F0Fu;
+    const uint32_t nz0 = (base + 0x7F7F7F7Fu) & 0x80808080u;               // type != 0
+    const uint32_t nz3 = ((base ^ 0x03030303u) + 0x7F7F7F7Fu) & 0x80808080u;  // type != 3
+    uint32_t m = (nz0 & nz3) >> 7;
+    m |= m >> 7;
+    m |= m >> 14;
+    return m & 0xFu;
 }
-__global__ void k_mark_edges_w(Geom g, uint8_t* type, uint64_t* bcm) {
+constexpr int MARK_GRID = 4096;
+__global__ void k_mark_full_w(Geom g, uint8_t* type, uint64_t* bcm, const DevScalars* sc) {
+    if (!sc->fullMark) return;
     const int W = g.X / 4;
-    const int64_t gid = int64_t(blockIdx.x) * blockDim.x + threadIdx.x;
-    if (gid >= int64_t(g.Y) * g.Z * W) return;
-    const int w = int(gid % W);
-    const int64_t row = gid / W;
-    const int y = int(row % g.Y), z = int(row / g.Y);
-    if (y == 0 || y == g.Y - 1 || z == 0 || z == g.Z - 1) return;
     const uint32_t* t32 = reinterpret_cast<const uint32_t*>(type);
-    const int64_t RW = W, SW = int64_t(W) * g.Y;  // words per row / per plane
-    const int64_t c0 = row * W + w;
-    const uint32_t cw = t32[c0];
-    const uint32_t ca = inactive_bytes(cw);
-    if (ca == 0xFu) return;  // no active node in this word
-    // 6-bit window of row r: bit 0 = byte x-1, bits 1-4 = the word, bit 5 = byte x+4
-    auto window = [&](int64_t r) {
-        const uint32_t mid = inactive_bytes(t32[r]);
-        const uint32_t lo = w > 0 ? uint32_t(!is_active(uint8_t(t32[r - 1] >> 24))) : 1u;
-        const uint32_t hi = w < W - 1 ? uint32_t(!is_active(uint8_t(t32[r + 1]))) : 1u;
-        return lo | (mid << 1) | (hi << 5);
-    };
-    const uint32_t own = window(c0);
-    const uint32_t side = window(c0 - RW) | window(c0 + RW) | window(c0 - SW) | window(c0 + SW);
-    const uint32_t diag = (inactive_bytes(t32[c0 - RW - SW]) | inactive_bytes(t32[c0 - RW + SW]) |
-                           inactive_bytes(t32[c0 + RW - SW]) | inactive_bytes(t32[c0 + RW + SW]));
-    uint32_t nw = cw;
+    const int64_t total = int64_t(g.Y) * g.Z * W;
+    for (int64_t gid = int64_t(blockIdx.x) * blockDim.x + threadIdx.x; gid < total;
+         gid += int64_t(gridDim.x) * blockDim.x) {
+        const int w = int(gid % W);
+        const int64_t row = gid / W;
+        const int y = int(row % g.Y), z = int(row / g.Y);
+        if (y == 0 || y == g.Y - 1 || z == 0 || z == g.Z - 1) continue;
+        const int64_t c0 = row * W + w;
+        const uint32_t cw = t32[c0];
+        if (inactive_bytes(cw) == 0xFu) continue;  // no active node in this word
+        const int ym = (y == 1 && g.per[2]) ? g.Y - 2 : y - 1, yp = (y == g.Y - 2 && g.per[3]) ? 1 : y + 1;
+        const int zm = (z == 1 && g.per[4]) ? g.Z - 2 : z - 1, zp = (z == g.Z - 2 && g.per[5]) ? 1 : z + 1;
+        auto rowWord = [&](int yy, int zz) { return (int64_t(zz) * g.Y + yy) * W; };
+        // 6-bit window of a row: bit 0 = byte x-1, bits 1-4 = the word, bit 5 = byte x+4; bytes of
+        // the boundary layer of a periodic x face are replaced by their wrapped partners
+        auto window = [&](int64_t rw) {
+            uint32_t mid = inactive_bytes(t32[rw + w]);
+            if (w == 0 && g.per[0]) {
+                const uint32_t b = (t32[rw + W - 1] >> 16) & 0xFFu;  // x = X - 2
+                mid = (mid & ~1u) | (is_active(uint8_t(b)) ? 0u : 1u);
+            }
+            if (w == W - 1 && g.per[1]) {
+                const uint32_t b = (t32[rw] >> 8) & 0xFFu;  // x = 1
+                mid = (mid & ~8u) | (is_active(uint8_t(b)) ? 0u : 8u);
+            }
+            const uint32_t lo = w > 0 ? uint32_t(!is_active(uint8_t(t32[rw + w - 1] >> 24))) : 1u;
+            const uint32_t hi = w < W - 1 ? uint32_t(!is_active(uint8_t(t32[rw + w + 1]))) : 1u;
+            return lo | (mid << 1) | (hi << 5);
+        };
+        const uint32_t own = window(rowWord(y, z));
+        const uint32_t side = window(rowWord(ym, z)) | window(rowWord(yp, z)) | window(rowWord(y, zm)) |
+                              window(rowWord(y, zp));
+        const uint32_t diag = inactive_bytes(t32[rowWord(ym, zm) + w]) | inactive_bytes(t32[rowWord(ym, zp) + w]) |
+                              inactive_bytes(t32[rowWord(yp, zm) + w]) | inactive_bytes(t32[rowWord(yp, zp) + w]);
+        uint32_t nw = cw;
 #pragma unroll
-    for (int k = 0; k < 4; ++k) {
-        const uint8_t t = uint8_t(cw >> (8 * k));
-        if (!is_active(t)) continue;
-        const int x = 4 * w + k;
-        const bool edge = ((own >> k) & 5u) || ((side >> k) & 7u) || ((diag >> k) & 1u);
-        const uint8_t nt = edge ? uint8_t(t | T_EDGE) : uint8_t(t & ~T_EDGE);
-        nw = (nw & ~(0xFFu << (8 * k))) | (uint32_t(nt) << (8 * k));
-        if (edge) bcm[c0 * 4 + k] = link_classes(g, type, int32_t(c0 * 4 + k), x, y, z);
+        for (int k = 0; k < 4; ++k) {
+            const uint8_t t = uint8_t(cw >> (8 * k));
+            if (!is_active(t)) continue;
+            const bool edge = ((own >> k) & 5u) || ((side >> k) & 7u) || ((diag >> k) & 1u);
+            const uint8_t nt = edge ? uint8_t(t | T_EDGE) : uint8_t(t & ~T_EDGE);
+            nw = (nw & ~(0xFFu << (8 * k))) | (uint32_t(nt) << (8 * k));
+            if (edge) bcm[c0 * 4 + k] = link_classes(g, type, int32_t(c0 * 4 + k), 4 * w + k, y, z);
+        }
+        // other threads read this word concurrently; they only test the type bits, which do not change
+        if (nw != cw) reinterpret_cast<uint32_t*>(type)[c0] = nw;
     }
-    // other threads read this word concurrently; they only test the type bits, which do not change
-    if (nw != cw) reinterpret_cast<uint32_t*>(type)[c0] = nw;
 }
-// the same for rows whose length is not a multiple of four: one thread per node
-__global__ void k_mark_edges_b(Geom g, uint8_t* type, uint64_t* bcm) {
-    const int32_t i = int32_t(blockIdx.x) * blockDim.x + threadIdx.x;
-    if (i >= int32_t(g.N)) return;
+// one active node: edge bit and classes from its wrapped neighbours
+__device__ inline void mark_node(const Geom& g, uint8_t* type, uint64_t* bcm, int32_t i) {
     const uint8_t t = type[i];
     if (!is_active(t)) return;
     int x, y, z;
     cell_xyz(g, i, x, y, z);
-    bool edge = false;
-    const int32_t sy = g.X, sz = g.X * g.Y;
-#pragma unroll
-    for (int q = 1; q < Q; ++q) edge |= !is_active(type[i + plain_off(q, sy, sz)]);
+    const uint64_t m = link_classes(g, type, i, x, y, z);
+    const bool edge = m != 0;
+    if (edge) bcm[i] = m;
     const uint8_t nt = edge ? uint8_t(t | T_EDGE) : uint8_t(t & ~T_EDGE);
     if (nt != t) type[i] = nt;
-    if (edge) bcm[i] = link_classes(g, type, i, x, y, z);
+}
+// full pass for rows whose length is not a multiple of four: one thread per node
+__global__ void k_mark_full_b(Geom g, uint8_t* type, uint64_t* bcm, const DevScalars* sc) {
+    if (!sc->fullMark) return;
+    for (int64_t i = int64_t(blockIdx.x) * blockDim.x + threadIdx.x; i < g.N; i += int64_t(gridDim.x) * blockDim.x)
+        mark_node(g, type, bcm, int32_t(i));
+}
+// incremental pass: the stencil neighbourhood (wrapped) of every node on the change list. A node
+// reached from several changed nodes is marked several times with the same result.
+__global__ void k_remark(Geom g, uint8_t* type, uint64_t* bcm, const int32_t* changed, int cap,
+                         const DevScalars* sc) {
+    if (sc->fullMark) return;
+    const int n = min(sc->nChanged, cap);
+    for (int64_t t = int64_t(blockIdx.x) * blockDim.x + threadIdx.x; t < int64_t(n) * Q;
+         t += int64_t(gridDim.x) * blockDim.x) {
+        const int32_t c = changed[t / Q];
+        const int j = int(t % Q);
+        int x, y, z;
+        cell_xyz(g, c, x, y, z);
+        const Offs o(g, x, y, z);
+        const int64_t p = int64_t(c) + offs_rt(o, j);
+        if (p >= 0 && p < g.N) mark_node(g, type, bcm, int32_t(p));  // (changed boundary-layer nodes)
+    }
+}
+__global__ void k_mark_end(DevScalars* sc) {
+    sc->fullMark = 0;
+    sc->nChanged = 0;
 }
 
-void lb_mark_edges(const Box& b, hipStream_t s) {
-    if (b.g.X % 4 == 0) {
-        const int64_t n = int64_t(b.g.Y) * b.g.Z * (b.g.X / 4);
-        hipLaunchKernelGGL(k_mark_edges_w, int((n + 255) / 256), 256, 0, s, b.g, b.type, b.bcm);
-    } else {
-        hipLaunchKernelGGL(k_mark_edges_b, int((b.g.N + 255) / 256), 256, 0, s, b.g, b.type, b.bcm);
-    }
+void lb_mark_edges(const Box& b, DevScalars* sc, hipStream_t s) {
+    if (b.g.X % 4 == 0) hipLaunchKernelGGL(k_mark_full_w, MARK_GRID, 256, 0, s, b.g, b.type, b.bcm, sc);
+    else hipLaunchKernelGGL(k_mark_full_b, MARK_GRID, 256, 0, s, b.g, b.type, b.bcm, sc);
+    hipLaunchKernelGGL(k_remark, 256, 256, 0, s, b.g, b.type, b.bcm, b.changed, b.changedCap, sc);
+    hipLaunchKernelGGL(k_mark_end, 1, 1, 0, s, sc);
 }
 
 int fused_grid_blocks(const Geom& g) {
@@ -829,6 +878,14 @@ __global__ void k_fs_mass(Geom g, const S* f, const uint8_t* type, const typenam
     }
 }
 
+// a node changed between active and non-active (or among non-active types): its neighbourhood is
+// re-marked before the next LB step; an overflowing list asks for a full re-mark instead
+__device__ inline void push_changed(int32_t* changed, int cap, DevScalars* sc, int32_t i) {
+    const int pos = atomicAdd(&sc->nChanged, 1);
+    if (pos < cap) changed[pos] = i;
+    else atomicOr(&sc->fullMark, 1);
+}
+
 __device__ inline void list_push(int32_t* list, int* count, int cap, int32_t v, int* overflow) {
     const int pos = atomicAdd(count, 1);
     if (pos < cap) list[pos] = v;
@@ -839,7 +896,8 @@ __device__ inline void list_push(int32_t* list, int* count, int cap, int32_t v, 
 template <typename S>
 __global__ void k_fs_mutants(uint8_t* type, typename Types<S>::M* mass, const typename Types<S>::C* aux, int64_t N,
                              const int32_t* ifList, const typename Types<S>::M* newMass, int32_t* filled,
-                             int32_t* emptied, int cap, DevScalars* sc, int32_t* marker) {
+                             int32_t* emptied, int cap, DevScalars* sc, int32_t* marker, int32_t* changed,
+                             int changedCap) {
     const int nIf = sc->nIf;
     for (int k = blockIdx.x * blockDim.x + threadIdx.x; k < nIf; k += gridDim.x * blockDim.x) {
         const int64_t i = ifList[k];
@@ -847,12 +905,13 @@ __global__ void k_fs_mutants(uint8_t* type, typename Types<S>::M* mass, const ty
         mass[i] = m;
         const auto n = aux[i];
         if (m > n) {
-            type[i] = (type[i] & T_PMASK) | T_FLUID;
+            type[i] = (type[i] & (T_PMASK | T_EDGE)) | T_FLUID;  // active to active: edge bit stays
             marker[i] = -3;
             list_push(filled, &sc->nFilled, cap, int32_t(i), &sc->listOverflow);
         } else if (m < 0) {
             type[i] = (type[i] & T_PMASK) | T_GAS;
             list_push(emptied, &sc->nEmptied, cap, int32_t(i), &sc->listOverflow);
+            push_changed(changed, changedCap, sc, int32_t(i));
         }
     }
 }
@@ -880,7 +939,7 @@ __global__ void k_fs_donors(Geom g, const uint8_t* type, const int32_t* filled, 
 template <typename S, int L>
 __global__ void k_fs_new_from_donor(Geom g, S* f, uint8_t* type, typename Types<S>::M* mass,
                                     typename Types<S>::C* aux, const int32_t* newIf, const int32_t* donor,
-                                    DevScalars* sc, Consts k) {
+                                    DevScalars* sc, Consts k, int32_t* changed, int changedCap) {
     using C = typename Types<S>::C;
     const int nN = sc->nNewIf;
     for (int kk = blockIdx.x * blockDim.x + threadIdx.x; kk < nN; kk += gridDim.x * blockDim.x) {
@@ -888,6 +947,7 @@ __global__ void k_fs_new_from_donor(Geom g, S* f, uint8_t* type, typename Types<
         const int64_t d = donor[gi];
         const int64_t N = g.N;
         type[gi] = (type[gi] & T_PMASK) | T_IF;
+        push_changed(changed, changedCap, sc, int32_t(gi));
         const double n = k.rho0;
         const double ux = (double(aux[4 * N + d]) + k.lbF[0]) / 2.0 / n + double(aux[N + d]);
         const double uy = (double(aux[5 * N + d]) + k.lbF[1]) / 2.0 / n + double(aux[2 * N + d]);
@@ -933,7 +993,7 @@ __global__ void k_fs_from_emptied(Geom g, uint8_t* type, typename Types<S>::M* m
             } else {
                 continue;
             }
-            type[l] = (type[l] & T_PMASK) | T_IF;
+            type[l] = (type[l] & (T_PMASK | T_EDGE)) | T_IF;  // active to active: edge bit stays
             mass[l] = M(0.99 * nl);
             aux[l] = C(nl);
             atomicAdd(&sc->surplus, 0.01 * nl);
@@ -995,7 +1055,7 @@ __global__ void k_fs_isolated_a(Geom g, uint8_t* type, typename Types<S>::M* mas
         if (!gasNb) {
             acc += double(mass[i]) - double(aux[i]);
             mass[i] = typename Types<S>::M(aux[i]);
-            type[i] = (type[i] & T_PMASK) | T_FLUID;
+            type[i] = (type[i] & (T_PMASK | T_EDGE)) | T_FLUID;  // active to active: edge bit stays
         }
     }
     if (acc != 0.0) atomicAdd(&sc->surplus, acc);
@@ -1004,7 +1064,7 @@ __global__ void k_fs_isolated_a(Geom g, uint8_t* type, typename Types<S>::M* mas
 // removeIsolated part 2: interface nodes without fluid neighbours become gas (LB.cpp:1821-1841)
 template <typename S>
 __global__ void k_fs_isolated_b(Geom g, uint8_t* type, const typename Types<S>::M* mass, const int32_t* list2,
-                                const int* count2, DevScalars* sc) {
+                                const int* count2, DevScalars* sc, int32_t* changed, int changedCap) {
     const int n2 = *count2;
     double acc = 0.0;
     int keep = 0;
@@ -1019,6 +1079,7 @@ __global__ void k_fs_isolated_b(Geom g, uint8_t* type, const typename Types<S>::
         if (!fluidNb) {
             acc += double(mass[i]);
             type[i] = (type[i] & T_PMASK) | T_GAS;
+            push_changed(changed, changedCap, sc, int32_t(i));
         } else {
             keep++;
         }
@@ -1066,17 +1127,20 @@ static void fs_impl(Box& b, const Consts& k, DevScalars* sc, hipStream_t s) {
     else
         hipLaunchKernelGGL((k_fs_mass<S, 0>), grid, bs, 0, s, b.g, f, b.type, mass, aux, b.ifList, (M*)b.ifNewMass, sc, b.donor);
     hipLaunchKernelGGL(k_fs_mutants<S>, grid, bs, 0, s, b.type, mass, aux, b.g.N, b.ifList, (M*)b.ifNewMass, b.filled,
-                       b.emptied, b.ifCap, sc, b.donor);
+                       b.emptied, b.ifCap, sc, b.donor, b.changed, b.changedCap);
     hipLaunchKernelGGL(k_fs_donors, grid, bs, 0, s, b.g, b.type, b.filled, b.donor, b.newIf, b.ifCap, sc);
     if (b.parity == 1)
-        hipLaunchKernelGGL((k_fs_new_from_donor<S, 1>), grid, bs, 0, s, b.g, f, b.type, mass, aux, b.newIf, b.donor, sc, k);
+        hipLaunchKernelGGL((k_fs_new_from_donor<S, 1>), grid, bs, 0, s, b.g, f, b.type, mass, aux, b.newIf, b.donor, sc, k,
+                           b.changed, b.changedCap);
     else
-        hipLaunchKernelGGL((k_fs_new_from_donor<S, 0>), grid, bs, 0, s, b.g, f, b.type, mass, aux, b.newIf, b.donor, sc, k);
+        hipLaunchKernelGGL((k_fs_new_from_donor<S, 0>), grid, bs, 0, s, b.g, f, b.type, mass, aux, b.newIf, b.donor, sc, k,
+                           b.changed, b.changedCap);
     hipLaunchKernelGGL(k_fs_from_emptied<S>, grid, bs, 0, s, b.g, b.type, mass, aux, b.emptied, b.donor, b.newIf, b.ifCap, sc);
     hipLaunchKernelGGL(k_fs_update_mutants<S>, grid, bs, 0, s, b.type, mass, aux, b.filled, b.emptied, sc);
     hipLaunchKernelGGL(k_fs_list2, grid, bs, 0, s, b.type, b.ifList, b.newIf, b.donor, b.ifList2, b.ifCap, sc, count2);
     hipLaunchKernelGGL(k_fs_isolated_a<S>, grid, bs, 0, s, b.g, b.type, mass, aux, b.ifList2, count2, sc);
-    hipLaunchKernelGGL(k_fs_isolated_b<S>, grid, bs, 0, s, b.g, b.type, mass, b.ifList2, count2, sc);
+    hipLaunchKernelGGL(k_fs_isolated_b<S>, grid, bs, 0, s, b.g, b.type, mass, b.ifList2, count2, sc, b.changed,
+                       b.changedCap);
     hipLaunchKernelGGL(k_fs_redistribute<S>, grid, bs, 0, s, b.type, mass, b.ifList2, count2, b.donor, b.filled, b.newIf, sc);
 }
 
@@ -1126,18 +1190,24 @@ __global__ void k_flags_apply(int64_t N, uint8_t* type, int32_t* solid, int32_t*
     if (threadIdx.x == 0) bmin = INT_MAX;
     __syncthreads();
     int mymin = INT_MAX;
+    bool wallIdx = false;
     for (int64_t i = int64_t(blockIdx.x) * blockDim.x + threadIdx.x; i < N; i += int64_t(gridDim.x) * blockDim.x) {
         const int32_t s = flagNew[i];
+        const uint8_t t = type[i];
         if (s >= 0) {
-            type[i] = (type[i] & T_BASE) | T_PMASK;
+            type[i] = uint8_t(t | T_PMASK);  // flagged nodes are active: the edge bit stays
             solid[i] = s;
             flagNew[i] = -1;
             mymin = min(mymin, int(i));
         } else {
-            type[i] &= T_BASE;
+            // active nodes keep their edge bit; other nodes lose the flag and a wall index
+            const uint8_t nt = is_active(t) ? uint8_t(t & ~T_PMASK) : uint8_t(t & T_BASE);
+            if (!is_active(t) && (t & 0x70)) wallIdx = true;
+            if (nt != t) type[i] = nt;
             solid[i] = 0;
         }
     }
+    if (__syncthreads_or(wallIdx) && threadIdx.x == 0) atomicOr(&sc->fullMark, 1);
     if (mymin != INT_MAX) atomicMin(&bmin, mymin);
     __syncthreads();
     // blocks run roughly in index order: only a smaller minimum needs the global atomic
@@ -1202,7 +1272,7 @@ __global__ void k_fna_unflag(Geom g, uint8_t* type, int32_t* solid, DevScalars* 
     if (!sc->needScan) return;
     const int ks = sc->kstar;
     for (int64_t i = int64_t(blockIdx.x) * blockDim.x + threadIdx.x; i < g.N && i < ks; i += int64_t(gridDim.x) * blockDim.x) {
-        if (type[i] & T_PMASK) { type[i] &= T_BASE; solid[i] = 0; }
+        if (type[i] & T_PMASK) { type[i] &= uint8_t(~T_PMASK); solid[i] = 0; }
     }
 }
 __global__ void k_fna_end(DevScalars* sc) {
@@ -1258,7 +1328,7 @@ __global__ __launch_bounds__(UNCOVER_THREADS) void k_obj_uncover(Geom g, S* f, u
                                                                  typename Types<S>::M* mass,
                                                                  typename Types<S>::C* aux, const int32_t* objList,
                                                                  int32_t* cand, DevScalars* sc, const ObjLB* olb,
-                                                                 Consts k) {
+                                                                 Consts k, int32_t* changed, int changedCap) {
     using C = typename Types<S>::C;
     __shared__ int sWave[UNCOVER_THREADS / 64 + 1];
     __shared__ uint8_t nbT[Q - 1][UNC_CHUNK];   // neighbour base types (old values)
@@ -1325,6 +1395,7 @@ __global__ __launch_bounds__(UNCOVER_THREADS) void k_obj_uncover(Geom g, S* f, u
             const int32_t i = sCand[tid];
             const uint8_t nt = newT[tid];
             type[i] = uint8_t((type[i] & T_PMASK) | nt);
+            push_changed(changed, changedCap, sc, i);
             if (nt != T_GAS) {
                 const bool isIf = nt == T_IF;
                 int x, y, z;
@@ -1379,7 +1450,8 @@ __device__ inline int inside_which(int x, int y, int z, const ObjLB* olb, int nO
     return which;
 }
 __global__ __launch_bounds__(COVER_BS) void k_obj_cover_a(Geom g, uint8_t* type, int32_t* solid, const ObjLB* olb,
-                                                          int nObj, int* blockCount) {
+                                                          int nObj, int* blockCount, int32_t* changed, int changedCap,
+                                                          DevScalars* sc) {
     const CoverBox cb = cover_box(g, olb, nObj);
     const int per = (cb.tot + gridDim.x - 1) / gridDim.x;
     const int beg = blockIdx.x * per, end = min(cb.tot, beg + per);
@@ -1392,7 +1464,9 @@ __global__ __launch_bounds__(COVER_BS) void k_obj_cover_a(Geom g, uint8_t* type,
             const int which = inside_which(x, y, z, olb, nObj);
             if (which >= 0) {
                 const int64_t i = cidx(g, x, y, z);
-                type[i] = uint8_t((type[i] & T_PMASK) | T_OBJ);
+                const uint8_t t = type[i];
+                if ((t & T_BASE) != T_OBJ) push_changed(changed, changedCap, sc, int32_t(i));
+                type[i] = uint8_t((t & T_PMASK) | T_OBJ);
                 solid[i] = which;
                 in = 1;
             }
@@ -1445,11 +1519,12 @@ static void obj_impl(Box& b, const Consts& k, const ObjLB* olb, DevScalars* sc, 
     using C = typename Types<S>::C;
     if (b.parity == 0)
         hipLaunchKernelGGL((k_obj_uncover<S, 0>), 1, UNCOVER_THREADS, 0, s, b.g, (S*)b.f, b.type, (M*)b.mass, (C*)b.aux,
-                           b.objList, b.objCand, sc, olb, k);
+                           b.objList, b.objCand, sc, olb, k, b.changed, b.changedCap);
     else
         hipLaunchKernelGGL((k_obj_uncover<S, 1>), 1, UNCOVER_THREADS, 0, s, b.g, (S*)b.f, b.type, (M*)b.mass, (C*)b.aux,
-                           b.objList, b.objCand, sc, olb, k);
-    hipLaunchKernelGGL(k_obj_cover_a, COVER_GRID, COVER_BS, 0, s, b.g, b.type, b.solid, olb, k.nObj, b.coverCount);
+                           b.objList, b.objCand, sc, olb, k, b.changed, b.changedCap);
+    hipLaunchKernelGGL(k_obj_cover_a, COVER_GRID, COVER_BS, 0, s, b.g, b.type, b.solid, olb, k.nObj, b.coverCount,
+                       b.changed, b.changedCap, sc);
     hipLaunchKernelGGL(k_obj_cover_b, COVER_GRID, COVER_BS, 0, s, b.g, b.type, olb, k.nObj, b.coverCount, b.objList,
                        b.objCap, sc);
 }

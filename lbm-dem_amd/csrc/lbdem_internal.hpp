@@ -47,9 +47,8 @@ __host__ __device__ constexpr int oppq(int q) {
 }
 __host__ __device__ constexpr double wq(int q) { return q == 0 ? 12.0 / 36.0 : (q < 7 ? 2.0 / 36.0 : 1.0 / 36.0); }
 
-// type byte: bits 0-3 node type; active nodes: bit 6 'edge' (interface node, a neighbour is not
-// active or the node wraps periodically; refreshed before every lattice step), bit 7
-// inside-particle flag; wall nodes: bits 4-6 the wall that receives the node's link forces
+// type byte: bits 0-3 node type; active nodes: bit 6 'edge' (some wrapped neighbour is not active;
+// kept up to date before every lattice step), bit 7 inside-particle flag; wall nodes: bits 4-6 the wall that receives the node's link forces
 // (its solid index when that is a wall index, else 0; cleared with the solid index when the
 // particle flags are rebuilt)
 enum : uint8_t { T_FLUID = 0, T_GAS = 2, T_IF = 3, T_PER = 4, T_WALL = 7, T_OBJ = 12, T_EDGE = 0x40, T_PMASK = 0x80,
@@ -121,10 +120,13 @@ struct DevScalars {
     int nFilled, nEmptied, nNewIf;
     int nIfFinal;              // interface cells after removeIsolated
     int nObjList;              // object node list length
+    int nChanged;              // nodes whose type changed between active and non-active (or among
+                               // non-active types) since the last LB step
+    int fullMark;              // edge bits and link classes must be recomputed for the whole box
     int kminP;                 // lowest index of a particle-flagged node (-1: none)
     int needScan;              // findNewActive needs the full scan
     int kstar;                 // first particle node still covered
-    int pad[3];
+    int pad[1];
 };
 
 template <typename Tst> struct StoreT;
@@ -164,6 +166,8 @@ struct Box {
     int32_t* objList;          // object nodes, ascending
     int32_t* objCand;          // uncovered object nodes of the current step, ascending
     int* coverCount;           // per-block counts of the object cover pass
+    int32_t* changed;          // nodes to re-mark (see DevScalars::nChanged)
+    int changedCap;
     int ifCap, objCap;
     double* blockPart;         // per-block partials of the fused kernel (object F/M, extra mass, walls)
     int32_t* blockFlag;        // per block: partials written
@@ -187,7 +191,7 @@ struct Consts {
 
 // ---------------- launchers (lb.hip) ----------------
 void lb_init_fields(const Box& b, const Consts& k, hipStream_t s);
-void lb_mark_edges(const Box& b, hipStream_t s);
+void lb_mark_edges(const Box& b, DevScalars* sc, hipStream_t s);
 void lb_fused_step(Box& b, const Consts& k, const PartLB* plb, const ObjLB* olb, DevScalars* sc, bool diag,
                    hipStream_t s, hipEvent_t ev0 = nullptr, hipEvent_t ev1 = nullptr);
 void lb_hydro_finalize(const Box& b, const Consts& k, double* FHydro, double* MHydro, double* fluidVol,

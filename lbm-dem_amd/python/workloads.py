@@ -1,7 +1,8 @@
 """Synthetic impact-run workloads in the reference's input terms (reference axes: y vertical).
 
 ``impact_box`` builds the getpot values, particles and impactor of one run:
-  * a closed box (no-slip walls on all faces, the reference example's geometry), lattice units
+  * a box with no-slip floor and lid (y faces) and either no-slip walls on the x and z faces (the
+    reference example's geometry) or periodic x and z faces (``periodic=True``); lattice units
     (unitLength = unitTime = unitDensity = 1), tau = 0.8 (initVisc = 0.1), gravity lbFY = -1e-5;
   * fluid up to ``fluid_height`` with a free surface;
   * ``n_spheres`` spheres of radius ``r`` from a seeded random-sequential-addition + growth /
@@ -41,15 +42,17 @@ def pack(n: int, r: float, lo, hi, seed: int, relax: int = 500) -> np.ndarray:
 
 
 def impact_box(size=(256, 320, 256), fluid_height=256.0, n_spheres=31000, r=4.0, r_imp=16.0, y_imp=290.0,
-               speed=0.01, seed=20261202, relax=500) -> Tuple[Dict[str, object], lbdem.ParticleSet, lbdem.ObjectSet]:
+               speed=0.01, seed=20261202, relax=500, periodic=False
+               ) -> Tuple[Dict[str, object], lbdem.ParticleSet, lbdem.ObjectSet]:
     X, Y, Z = size
+    side = 4 if periodic else 7
     v = dict(lbdem.DEFAULTS)
     v.update({
         "demSolve": 1, "lbSolve": 1, "freeSurfaceSolve": 1, "forceFieldSolve": 1, "demInitialRepeat": 0.0,
         "maxTime": 5000.0, "unitLength": 1.0, "unitTime": 1.0, "unitDensity": 1.0,
         "lbSizeX": float(X), "lbSizeY": float(Y), "lbSizeZ": float(Z),
         "rheologyModel": "NEWTONIAN", "initVisc": 0.1, "lbFY": -1.0e-5, "fluidHeight": float(fluid_height),
-        "boundary0": 7, "boundary1": 7, "boundary2": 7, "boundary3": 7, "boundary4": 7, "boundary5": 7,
+        "boundary0": side, "boundary1": side, "boundary2": 7, "boundary3": 7, "boundary4": side, "boundary5": side,
         "density": 2.5, "objectDensity": 3.0, "bodyDensity": 12.0, "linearStiff": 2.0, "frictionCoefPart": 0.3,
         "lubNormal": 1.0, "lubTangential": 0.5, "lubRotation": 0.4, "lubCutoff": 0.2, "kinVisc": 0.1,
         "electroForce": 0.01, "debyeLength": 0.5, "objWaitTime": 0,
@@ -71,7 +74,8 @@ def volume_fraction(ps: lbdem.ParticleSet, size, fluid_height) -> float:
     return float(np.sum(4.0 / 3.0 * math.pi * ps.r ** 3) / ((X - 2.0) * (fluid_height - 1.0) * (Z - 2.0)))
 
 
-# The headline workload: 256 x 256 x 320 box (x, z horizontal; y vertical up to 320), fluid to
-# 256, 31 000 spheres of radius 4 (phi ~ 0.5), impactor r = 16 at height 290, v = -0.01.
+# The headline workload: 256 x 256 x 320 box (x, z horizontal and periodic; y vertical up to 320,
+# floor and lid walls), fluid to 256, 31 000 spheres of radius 4 (phi ~ 0.5), impactor r = 16 at
+# height 290, v = -0.01.
 HEADLINE = dict(size=(256, 320, 256), fluid_height=256.0, n_spheres=31000, r=4.0, r_imp=16.0, y_imp=290.0,
-                speed=0.01, seed=20261202)
+                speed=0.01, seed=20261202, periodic=True)
