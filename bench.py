@@ -28,6 +28,9 @@ import lbdem  # noqa: E402
 import workloads  # noqa: E402
 
 PEAK_HBM_GBS = 8000.0
+# HBM bytes per fused-kernel launch from the rocprofv3 FETCH_SIZE / WRITE_SIZE runs on this
+# workload (scaled by the same counters on a plain copy of known size)
+TRAFFIC_FILE = os.path.join(HERE, "profiles", "k_fused_impact256_fp32_hbm_traffic.json")
 BYTES_PER_CELL = {lbdem.PREC_FP64: 38 * 8 + 1, lbdem.PREC_FP32: 38 * 4 + 1, lbdem.PREC_FP16: 38 * 2 + 1}
 PREC_NAME = {lbdem.PREC_FP64: "fp64", lbdem.PREC_FP32: "fp32", lbdem.PREC_FP16: "fp16"}
 
@@ -79,7 +82,8 @@ def run_gpu(vals, ps, obs, prec, steps, warmup):
 
 def cpu_baseline(seconds_budget=25.0):
     """The reference program (oracle/_ref/ref_driver) on a bounded sample of the workload: a
-    64 x 80 x 64 box with the same sphere radius and volume fraction, all host cores."""
+    64 x 80 x 64 box (x/z periodic) with the same sphere radius and volume fraction, 100 timed
+    steps, up to 16 host threads."""
     drv = os.path.join(HERE, "oracle", "_ref", "ref_driver")
     kind = "reference"
     if not os.path.exists(drv):
@@ -97,9 +101,10 @@ def cpu_baseline(seconds_budget=25.0):
         lbdem.write_objects(os.path.join(d, "input", "objects.dat"), obs.r, obs.x0, obs.x1)
         with open(os.path.join(d, "input", "body.dat"), "w") as fh:
             fh.write("0\n")
-        threads = os.cpu_count() or 1
+        # at most 16 threads: the host's share per GPU
+        threads = max(1, min(os.cpu_count() or 1, 16))
         env = dict(os.environ, OMP_NUM_THREADS=str(threads), LBDEM_WARMUP="3")
-        steps = 23
+        steps = 103
         t0 = time.time()
         r = subprocess.run([drv, "config.cfg", "out", str(steps), "dump"], cwd=d, env=env, capture_output=True,
                            text=True, timeout=600)
@@ -113,8 +118,24 @@ def cpu_baseline(seconds_budget=25.0):
         active = int(kv["active"])
         log(f"[bench] cpu baseline: {nst} steps in {sec:.2f} s ({time.time() - t0:.1f} s total), {active} active cells")
         return {"value": active * nst / sec / 1e6, "unit": "MLUPS", "cores": int(kv["threads"]), "kind": kind,
-                "sample": f"reference program (g++ -O2 -fopenmp), 64x80x64 box, {n} spheres r=4 (phi~0.5), impactor r=8, "
+                "sample": f"reference program (g++ -O2 -fopenmp), 64x80x64 box x/z periodic, {n} spheres r=4 (phi~0.5), impactor r=8, "
                           f"{nst} timed steps after 3 warm-up steps, {active} active cells"}
+
+
+def copy_probe(nbytes=1 << 30, reps=3):
+    """A plain device-to-device copy of known size (torch's copy kernel): reads and writes
+    ``nbytes`` per launch, the reference for scaling the HBM byte counters of rocprofv3."""
+    import torch
+    a = torch.ones(nbytes // 4, dtype=torch.float32, device="cuda")
+    b = torch.empty_like(a)
+    torch.cuda.synchronize()
+    t0 = time.perf_counter()
+    for _ in range(reps):
+        b.copy_(a)
+    torch.cuda.synchronize()
+    dt = (time.perf_counter() - t0) / reps
+    print(json.dumps({"copy_probe_bytes": nbytes, "reps": reps, "ms_per_copy": round(dt * 1e3, 4),
+                      "gbs_read_plus_write": round(2 * nbytes / dt / 1e9, 1)}), flush=True)
 
 
 def main():
@@ -126,7 +147,12 @@ def main():
     ap.add_argument("--no-fp64", action="store_true", help="skip the FP64 comparison run")
     ap.add_argument("--no-cpu-baseline", action="store_true")
     ap.add_argument("--small", action="store_true", help="quarter-size box (for quick checks)")
+    ap.add_argument("--copy-probe", action="store_true",
+                    help="only run a plain 1 GiB device copy three times (calibrates HBM byte counters)")
     args = ap.parse_args()
+    if args.copy_probe:
+        copy_probe()
+        return
     if args.gpus != 1:
         print(f"bench.py runs on one GPU only (asked for {args.gpus}); the coupled step has no multi-GPU path",
               file=sys.stderr)
@@ -146,6 +172,11 @@ def main():
     cpu = None if args.no_cpu_baseline else cpu_baseline()
     b = BYTES_PER_CELL[prec]
     ach = main_res["fused_achieved_gbs"]
+    traffic = None
+    if prec == lbdem.PREC_FP32 and not args.small and os.path.exists(TRAFFIC_FILE):
+        with open(TRAFFIC_FILE) as fh:
+            t = json.load(fh)
+        traffic = t["hbm_read_bytes_per_launch"] + t["hbm_write_bytes_per_launch"]
     out = {
         "metric": "coupled MLUPS at 256²×320 φ=0.5 vs FP64 plain; p50/p99 served-run latency",
         "value": round(main_res["mlups_active"], 1),
@@ -174,7 +205,8 @@ def main():
             "total_mass_drift_rel": (main_res["total_mass"] - main_res["total_mass_start"]) / main_res["total_mass_start"],
         },
         "roofline": {"bound": "hbm", "achieved": round(ach, 1), "peak": PEAK_HBM_GBS, "unit": "GB/s",
-                     "frac": round(ach / PEAK_HBM_GBS, 4), "traffic": None,
+                     "frac": round(ach / PEAK_HBM_GBS, 4), "traffic": traffic,
+                     "algorithmic_bytes_per_launch": int(main_res["active_cells"] * b),
                      "kernel": "k_fused", "bytes_per_active_node": b},
         "cpu_baseline": cpu,
     }
