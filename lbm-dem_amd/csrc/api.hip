@@ -394,6 +394,7 @@ extern "C" int lbdem_init(lbdem_ctx* c) {
     d.objContrib = c->dalloc<double>(size_t(std::max(np, 1)) * 12 * std::max(no, 1));
     d.wallContrib = c->dalloc<double>(size_t(std::max(np, 1)) * 6 * std::max(int(walls.size()), 1));
     d.objSum = c->dalloc<double>(12 * MAX_OBJECTS);
+    d.redPart = c->dalloc<double>(2 * 64 * 12 * MAX_OBJECTS);
     d.wallSum = c->dalloc<double>(6 * MAX_WALLS);
     d.blockMax = c->dalloc<unsigned long long>(64);
     const size_t o3 = 3 * size_t(std::max(no, 1));

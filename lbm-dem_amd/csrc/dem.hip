@@ -369,24 +369,34 @@ __device__ void wall_like_contact(const DemParams& P, double dampCoeff, double r
     Mout = V{r * dtx, r * dty, r * dtz};
 }
 
+constexpr int DEM_LANES = 8;  // lanes per particle in the force kernel
+
+// sum over the DEM_LANES lanes of a particle's group, in a fixed butterfly order
+__device__ inline double group_sum(double v) {
+#pragma unroll
+    for (int m = 1; m < DEM_LANES; m <<= 1) v += __shfl_xor(v, m, 64);
+    return v;
+}
+__device__ inline V group_sum(V v) { return V{group_sum(v.x), group_sum(v.y), group_sum(v.z)}; }
+
+// Forces on particle p: DEM_LANES lanes per particle share its Verlet partners (lane l takes
+// partners l, l + DEM_LANES, ...), their partial sums are combined in a fixed order, and the
+// group's first lane adds the wall and impactor terms and writes the results.
 __global__ void k_dem_forces(ForceArgs a, double dampCoeff) {
 #pragma clang fp contract(off)
-    const int p = blockIdx.x * blockDim.x + threadIdx.x;
+    const int gt = blockIdx.x * blockDim.x + threadIdx.x;
+    const int lane = gt % DEM_LANES;
     const DemParams& P = a.p;
-    if (p >= P.nPart) return;
+    const bool valid = gt / DEM_LANES < P.nPart;
+    const int p = valid ? gt / DEM_LANES : 0;  // every lane takes part in the group sums
     const V xp = ld3(a.x0, p), vp = ld3(a.x1, p), wp = ld3(a.w1, p);
     const double rp = a.r[p];
     V Fc{0, 0, 0}, Mc{0, 0, 0}, Fl{0, 0, 0}, Ml{0, 0, 0}, Fr{0, 0, 0};
-    V xiw = ld3(a.xiw, p), xio = ld3(a.xio, p);
-    const int cnt = a.nbrCount[p];
+    const int cnt = valid ? a.nbrCount[p] : 0;
     const bool waited = a.demTimeStep >= P.objWaitTime;
-    double* oc = a.objContrib + int64_t(p) * 12 * P.nObj;
-    double* wc = a.wallContrib + int64_t(p) * 6 * P.nWall;
-    for (int c = 0; c < 12 * P.nObj; ++c) oc[c] = 0.0;
-    for (int c = 0; c < 6 * P.nWall; ++c) wc[c] = 0.0;
     if (waited) {
         // particle lubrication (DEM.cpp:2302-2521)
-        for (int k = 0; k < cnt; ++k) {
+        for (int k = lane; k < cnt; k += DEM_LANES) {
             const int q = a.nbrList[p * NBR_CAP + k];
             const int n1 = p > q ? p : q, n2 = p > q ? q : p;
             const LubOut o = pair_lub(P, ld3(a.x0, n1), ld3(a.x0, n2), a.r[n1], a.r[n2], ld3(a.x1, n1), ld3(a.x1, n2),
@@ -395,6 +405,88 @@ __global__ void k_dem_forces(ForceArgs a, double dampCoeff) {
             if (p == n1) { Fl = add(Fl, o.F); Ml = add(Ml, o.T1); }
             else { Fl = sub(Fl, o.F); Ml = add(Ml, o.T2); }
         }
+    }
+    // particle-particle contact (DEM.cpp:1487-1791)
+    for (int k = lane; k < cnt; k += DEM_LANES) {
+        const int q = a.nbrList[p * NBR_CAP + k];
+        const int n1 = p > q ? p : q, n2 = p > q ? q : p;
+        const int slot = a.nbrSlot[p * NBR_CAP + k];
+        V xi{0, 0, 0};
+        if (slot >= 0) xi = ld3(a.xiOld, slot);
+        const V A = ld3(a.x0, n1), B = ld3(a.x0, n2), vA = ld3(a.x1, n1), vB = ld3(a.x1, n2);
+        const V wA = ld3(a.w1, n1), wB = ld3(a.w1, n2);
+        const double rA = a.r[n1], rB = a.r[n2];
+        const double rad = 0.5 * (rA + rB);
+        const double x12 = A.x - B.x, y12 = A.y - B.y, z12 = A.z - B.z;
+        const double r12 = sqrt(x12 * x12 + y12 * y12 + z12 * z12);
+        const double urx = vA.x - vB.x, ury = vA.y - vB.y, urz = vA.z - vB.z;
+        const double nx = x12 / r12, ny = y12 / r12, nz = z12 / r12;
+        const double gn = urx * nx + ury * ny + urz * nz;
+        const double del = 2 * rad - r12;
+        double fnx = 0, fny = 0, fnz = 0, ftx = 0, fty = 0, ftz = 0;
+        if (del > 0.0) {
+            const double k_ = P.linearStiff, mu = P.frictionCoefPart;
+            fnx = k_ * del * nx - dampCoeff * gn * nx;
+            fny = k_ * del * ny - dampCoeff * gn * ny;
+            fnz = k_ * del * nz - dampCoeff * gn * nz;
+            if (mu != 0.0) {
+                const double ox = rA * wA.x + rB * wB.x, oy = rA * wA.y + rB * wB.y, oz = rA * wA.z + rB * wB.z;
+                const double vx = urx + ny * oz - nz * oy - gn * nx;
+                const double vy = ury + nz * ox - nx * oz - gn * ny;
+                const double vz = urz + nx * oy - ny * ox - gn * nz;
+                const double xin = xi.x * nx + xi.y * ny + xi.z * nz;
+                xi.x -= xin * nx; xi.y -= xin * ny; xi.z -= xin * nz;
+                const double kt = 0.2 * k_;
+                const double dct = P.kinVisc == 0.0 ? dampCoeff : 0.0;
+                ftx = -kt * xi.x - dct * vx; fty = -kt * xi.y - dct * vy; ftz = -kt * xi.z - dct * vz;
+                const double fnn = sqrt(fnx * fnx + fny * fny + fnz * fnz);
+                const double ftt = sqrt(ftx * ftx + fty * fty + ftz * ftz);
+                const double fri = mu * fnn;
+                const double tx = ftx / ftt, ty = fty / ftt, tz = ftz / ftt;
+                xi.x += vx * P.deltat; xi.y += vy * P.deltat; xi.z += vz * P.deltat;
+                if (fri < ftt) {
+                    ftx = fri * tx; fty = fri * ty; ftz = fri * tz;
+                    xi.x = -(1 / kt) * (ftx + dct * vx);
+                    xi.y = -(1 / kt) * (fty + dct * vy);
+                    xi.z = -(1 / kt) * (ftz + dct * vz);
+                }
+            }
+        } else {
+            xi = V{0, 0, 0};
+        }
+        if (p == n1 && slot >= 0) st3(a.xiNew, slot, xi);
+        const double dfx = fnx + ftx, dfy = fny + fty, dfz = fnz + ftz;
+        const double dtx = fty * nz - ftz * ny, dty = ftz * nx - ftx * nz, dtz = ftx * ny - fty * nx;
+        if (p == n1) { Fc = add(Fc, V{dfx, dfy, dfz}); Mc = add(Mc, V{rA * dtx, rA * dty, rA * dtz}); }
+        else { Fc = sub(Fc, V{dfx, dfy, dfz}); Mc = add(Mc, V{rB * dtx, rB * dty, rB * dtz}); }
+    }
+    // electrostatic repulsion over every listed pair (DEM.cpp:1404-1484)
+    for (int k = lane; k < cnt; k += DEM_LANES) {
+        const int q = a.nbrList[p * NBR_CAP + k];
+        const int n1 = p > q ? p : q, n2 = p > q ? q : p;
+        const V A = ld3(a.x0, n1), B = ld3(a.x0, n2);
+        const double rad = 0.5 * (a.r[n1] + a.r[n2]);
+        const double x12 = A.x - B.x, y12 = A.y - B.y, z12 = A.z - B.z;
+        const double r12 = sqrt(x12 * x12 + y12 * y12 + z12 * z12);
+        const double nx = x12 / r12, ny = y12 / r12, nz = z12 / r12;
+        const double del = 2 * rad - r12;
+        double fx, fy, fz;
+        if (del <= 0.0) {
+            const double e = P.electroForce * exp(del / P.debyeLength);
+            fx = e * nx; fy = e * ny; fz = e * nz;
+        } else {
+            fx = P.electroForce * nx; fy = P.electroForce * ny; fz = P.electroForce * nz;
+        }
+        if (p == n1) Fr = add(Fr, V{fx, fy, fz}); else Fr = sub(Fr, V{fx, fy, fz});
+    }
+    Fc = group_sum(Fc); Mc = group_sum(Mc); Fl = group_sum(Fl); Ml = group_sum(Ml); Fr = group_sum(Fr);
+    if (!valid || lane != 0) return;
+    V xiw = ld3(a.xiw, p), xio = ld3(a.xio, p);
+    double* oc = a.objContrib + int64_t(p) * 12 * P.nObj;
+    double* wc = a.wallContrib + int64_t(p) * 6 * P.nWall;
+    for (int c = 0; c < 12 * P.nObj; ++c) oc[c] = 0.0;
+    for (int c = 0; c < 6 * P.nWall; ++c) wc[c] = 0.0;
+    if (waited) {
         // wall lubrication (DEM.cpp:2525-2672)
         for (int w = 0; w < P.nWall; ++w) {
             const double rad = rp;
@@ -497,79 +589,6 @@ __global__ void k_dem_forces(ForceArgs a, double dampCoeff) {
         wall_like_contact(P, dampCoeff, rp, vp, wp, xiw, 0.0, xp.y - (P.demSize[1] - 0.3), 0.0, F, M);
         Fc = add(Fc, F); Mc = add(Mc, M);
     }
-    // particle-particle contact (DEM.cpp:1487-1791)
-    for (int k = 0; k < cnt; ++k) {
-        const int q = a.nbrList[p * NBR_CAP + k];
-        const int n1 = p > q ? p : q, n2 = p > q ? q : p;
-        const int slot = a.nbrSlot[p * NBR_CAP + k];
-        V xi{0, 0, 0};
-        if (slot >= 0) xi = ld3(a.xiOld, slot);
-        const V A = ld3(a.x0, n1), B = ld3(a.x0, n2), vA = ld3(a.x1, n1), vB = ld3(a.x1, n2);
-        const V wA = ld3(a.w1, n1), wB = ld3(a.w1, n2);
-        const double rA = a.r[n1], rB = a.r[n2];
-        const double rad = 0.5 * (rA + rB);
-        const double x12 = A.x - B.x, y12 = A.y - B.y, z12 = A.z - B.z;
-        const double r12 = sqrt(x12 * x12 + y12 * y12 + z12 * z12);
-        const double urx = vA.x - vB.x, ury = vA.y - vB.y, urz = vA.z - vB.z;
-        const double nx = x12 / r12, ny = y12 / r12, nz = z12 / r12;
-        const double gn = urx * nx + ury * ny + urz * nz;
-        const double del = 2 * rad - r12;
-        double fnx = 0, fny = 0, fnz = 0, ftx = 0, fty = 0, ftz = 0;
-        if (del > 0.0) {
-            const double k_ = P.linearStiff, mu = P.frictionCoefPart;
-            fnx = k_ * del * nx - dampCoeff * gn * nx;
-            fny = k_ * del * ny - dampCoeff * gn * ny;
-            fnz = k_ * del * nz - dampCoeff * gn * nz;
-            if (mu != 0.0) {
-                const double ox = rA * wA.x + rB * wB.x, oy = rA * wA.y + rB * wB.y, oz = rA * wA.z + rB * wB.z;
-                const double vx = urx + ny * oz - nz * oy - gn * nx;
-                const double vy = ury + nz * ox - nx * oz - gn * ny;
-                const double vz = urz + nx * oy - ny * ox - gn * nz;
-                const double xin = xi.x * nx + xi.y * ny + xi.z * nz;
-                xi.x -= xin * nx; xi.y -= xin * ny; xi.z -= xin * nz;
-                const double kt = 0.2 * k_;
-                const double dct = P.kinVisc == 0.0 ? dampCoeff : 0.0;
-                ftx = -kt * xi.x - dct * vx; fty = -kt * xi.y - dct * vy; ftz = -kt * xi.z - dct * vz;
-                const double fnn = sqrt(fnx * fnx + fny * fny + fnz * fnz);
-                const double ftt = sqrt(ftx * ftx + fty * fty + ftz * ftz);
-                const double fri = mu * fnn;
-                const double tx = ftx / ftt, ty = fty / ftt, tz = ftz / ftt;
-                xi.x += vx * P.deltat; xi.y += vy * P.deltat; xi.z += vz * P.deltat;
-                if (fri < ftt) {
-                    ftx = fri * tx; fty = fri * ty; ftz = fri * tz;
-                    xi.x = -(1 / kt) * (ftx + dct * vx);
-                    xi.y = -(1 / kt) * (fty + dct * vy);
-                    xi.z = -(1 / kt) * (ftz + dct * vz);
-                }
-            }
-        } else {
-            xi = V{0, 0, 0};
-        }
-        if (p == n1 && slot >= 0) st3(a.xiNew, slot, xi);
-        const double dfx = fnx + ftx, dfy = fny + fty, dfz = fnz + ftz;
-        const double dtx = fty * nz - ftz * ny, dty = ftz * nx - ftx * nz, dtz = ftx * ny - fty * nx;
-        if (p == n1) { Fc = add(Fc, V{dfx, dfy, dfz}); Mc = add(Mc, V{rA * dtx, rA * dty, rA * dtz}); }
-        else { Fc = sub(Fc, V{dfx, dfy, dfz}); Mc = add(Mc, V{rB * dtx, rB * dty, rB * dtz}); }
-    }
-    // electrostatic repulsion over every listed pair (DEM.cpp:1404-1484)
-    for (int k = 0; k < cnt; ++k) {
-        const int q = a.nbrList[p * NBR_CAP + k];
-        const int n1 = p > q ? p : q, n2 = p > q ? q : p;
-        const V A = ld3(a.x0, n1), B = ld3(a.x0, n2);
-        const double rad = 0.5 * (a.r[n1] + a.r[n2]);
-        const double x12 = A.x - B.x, y12 = A.y - B.y, z12 = A.z - B.z;
-        const double r12 = sqrt(x12 * x12 + y12 * y12 + z12 * z12);
-        const double nx = x12 / r12, ny = y12 / r12, nz = z12 / r12;
-        const double del = 2 * rad - r12;
-        double fx, fy, fz;
-        if (del <= 0.0) {
-            const double e = P.electroForce * exp(del / P.debyeLength);
-            fx = e * nx; fy = e * ny; fz = e * nz;
-        } else {
-            fx = P.electroForce * nx; fy = P.electroForce * ny; fz = P.electroForce * nz;
-        }
-        if (p == n1) Fr = add(Fr, V{fx, fy, fz}); else Fr = sub(Fr, V{fx, fy, fz});
-    }
     // wall contacts (DEM.cpp:1946-2153), shared history xiw
     for (int w = 0; w < P.nWall; ++w) {
         const double x1w = (xp.x - P.wallP[w][0]) * P.wallN[w][0] * P.wallN[w][0];
@@ -584,20 +603,48 @@ __global__ void k_dem_forces(ForceArgs a, double dampCoeff) {
     st3(a.xiw, p, xiw); st3(a.xio, p, xio);
 }
 
-// deterministic column sums of the object and wall contributions (one block per column)
-__global__ void k_dem_reduce(const double* contrib, int nPart, int ncol, double* out) {
-    __shared__ double red[256];
-    const int c = blockIdx.x;
-    if (c >= ncol) return;
+// Deterministic column sums of the per-particle object and wall contributions (row-major,
+// ncol columns), in two levels: block b sums rows [b*rows, (b+1)*rows) of every column
+// (threads stride over the flat range, so reads are coalesced) into part[b][c]; one block then
+// sums the block partials of each column in block order. blockIdx.y selects objects (0) or
+// walls (1).
+constexpr int RED_BLOCKS = 64, RED_BS = 256, RED_MAXCOL = 12 * MAX_OBJECTS;
+__global__ __launch_bounds__(RED_BS) void k_dem_reduce_a(const double* oc, int nco, const double* wcn, int ncw,
+                                                        int nPart, double* part) {
+    __shared__ double red[RED_BS];
+    const double* contrib = blockIdx.y == 0 ? oc : wcn;
+    const int ncol = blockIdx.y == 0 ? nco : ncw;
+    if (ncol == 0) return;
+    const int rows = (nPart + RED_BLOCKS - 1) / RED_BLOCKS;
+    const int64_t beg = int64_t(blockIdx.x) * rows * ncol;
+    const int64_t end = min(int64_t(nPart), int64_t(blockIdx.x + 1) * rows) * ncol;
+    // thread t handles flat indices beg + t + k*step with step a multiple of ncol, so its column
+    // is fixed: c = (t + beg) % ncol
+    const int step = (RED_BS / ncol) * ncol;
     double acc = 0.0;
-    for (int p = threadIdx.x; p < nPart; p += blockDim.x) acc += contrib[int64_t(p) * ncol + c];
-    red[threadIdx.x] = acc;
+    const int t = threadIdx.x;
+    if (t < step)
+        for (int64_t i = beg + t; i < end; i += step) acc += contrib[i];
+    red[t] = t < step ? acc : 0.0;
     __syncthreads();
-    for (int w = 128; w > 0; w >>= 1) {
-        if (threadIdx.x < w) red[threadIdx.x] += red[threadIdx.x + w];
-        __syncthreads();
+    if (t < ncol) {
+        double s = 0.0;
+        for (int u = t; u < step; u += ncol) s += red[u];
+        part[(int64_t(blockIdx.y) * RED_BLOCKS + blockIdx.x) * RED_MAXCOL + t] = s;
     }
-    if (threadIdx.x == 0) out[c] = red[0];
+}
+__global__ void k_dem_reduce_b(const double* part, int nco, int ncw, double* objSum, double* wallSum) {
+    const int t = threadIdx.x;
+    if (t < nco) {
+        double s = 0.0;
+        for (int b = 0; b < RED_BLOCKS; ++b) s += part[int64_t(b) * RED_MAXCOL + t];
+        objSum[t] = s;
+    }
+    if (t < ncw) {
+        double s = 0.0;
+        for (int b = 0; b < RED_BLOCKS; ++b) s += part[(int64_t(RED_BLOCKS) + b) * RED_MAXCOL + t];
+        wallSum[t] = s;
+    }
 }
 
 // objects: totals and integration (DEM.cpp:786-790, 2742-2758); walls: totals (DEM.cpp:773-775)
@@ -731,9 +778,12 @@ void dem_step(const DemParams& p, DemState& d, DevScalars* sc, int demTimeStep, 
         f.xiOld = d.xiCur; f.xiNew = d.xiNext;
         f.ox0 = d.ox0; f.ox1 = d.ox1; f.ow1 = d.ow1; f.or_ = d.or_;
         f.objContrib = d.objContrib; f.wallContrib = d.wallContrib; f.demTimeStep = demTimeStep;
-        hipLaunchKernelGGL(k_dem_forces, (p.nPart + 63) / 64, 64, 0, s, f, p.dampCoeff);
-        if (p.nObj > 0) hipLaunchKernelGGL(k_dem_reduce, 12 * p.nObj, 256, 0, s, d.objContrib, p.nPart, 12 * p.nObj, d.objSum);
-        if (p.nWall > 0) hipLaunchKernelGGL(k_dem_reduce, 6 * p.nWall, 256, 0, s, d.wallContrib, p.nPart, 6 * p.nWall, d.wallSum);
+        hipLaunchKernelGGL(k_dem_forces, (p.nPart * DEM_LANES + 255) / 256, 256, 0, s, f, p.dampCoeff);
+        if (p.nObj > 0 || p.nWall > 0) {
+            hipLaunchKernelGGL(k_dem_reduce_a, dim3(RED_BLOCKS, 2), RED_BS, 0, s, d.objContrib, 12 * p.nObj, d.wallContrib,
+                               6 * p.nWall, p.nPart, d.redPart);
+            hipLaunchKernelGGL(k_dem_reduce_b, 1, 128, 0, s, d.redPart, 12 * p.nObj, 6 * p.nWall, d.objSum, d.wallSum);
+        }
         double* t = d.xiCur; d.xiCur = d.xiNext; d.xiNext = t;
     } else {
         hipLaunchKernelGGL(k_zero, 1, 128, 0, s, d.objSum, 12 * MAX_OBJECTS);

@@ -244,6 +244,7 @@ struct DemState {
     double* objContrib;        // per particle, per object: Fc(3) Mc(3) Fl(3) Ml(3)
     double* wallContrib;       // per particle, per wall: Fc(3) Fl(3)
     double* objSum;            // 12*MAX_OBJECTS
+    double* redPart;           // 2*64*12*MAX_OBJECTS block partials of the object/wall reduction
     double* wallSum;           // 6*MAX_WALLS
     unsigned long long* blockMax;  // per-block maxima of |v|^2
     // objects
