@@ -1,0 +1,403 @@
+#include "facade.hpp"
+
+#include <sys/stat.h>
+
+#include <cmath>
+#include <cstdlib>
+#include <fstream>
+#include <iostream>
+#include <stdexcept>
+
+namespace lbdem {
+
+void tVect::print(std::ostream& os) const { os << x << " " << y << " " << z << " "; }
+
+// ------------------------------------------------------------------------------------------
+// Context
+// ------------------------------------------------------------------------------------------
+Context::Context(const std::string& libraryPath, int precision, int device) : api_(Api::load(libraryPath)) {
+    params.precision = precision;
+    params.device = device;
+}
+
+Context::~Context() {
+    if (ctx_) api_.destroy(ctx_);
+}
+
+void Context::check(int rc, const char* what) const {
+    if (rc == LBDEM_OK) return;
+    std::cout << "ERROR: " << what << " failed (code " << rc << "): "
+              << (ctx_ ? api_.last_error(ctx_) : "no context") << std::endl;
+    std::exit(1);
+}
+
+void Context::create() {
+    if (ctx_) return;
+    check(api_.create(&params, &ctx_), "lbdem_create");
+    const int np = int(pr.size()), no = int(orad.size());
+    check(api_.set_particles(ctx_, np, psize.data(), pr.data(), px0.data()), "lbdem_set_particles");
+    check(api_.set_objects(ctx_, no, orad.data(), ox0.data(), ox1.data()), "lbdem_set_objects");
+}
+
+void Context::initialise() {
+    create();
+    if (initialised_) return;
+    check(api_.init(ctx_), "lbdem_init");
+    initialised_ = true;
+}
+
+// ------------------------------------------------------------------------------------------
+// configuration (main.cpp:128-185)
+// ------------------------------------------------------------------------------------------
+void parseConfigFile(IO& io, DEM& dem, LB& lb, ConfigFile& cfg, CommandLine& cl) {
+    dem.demInitialRepeat = parse_value(cfg, cl, "demInitialRepeat", 0.0);
+    lb.lbmInitialRepeat = parse_value(cfg, cl, "lbmInitialRepeat", 0);
+    io.maxTime = parse_value(cfg, cl, "maxTime", 0.0);
+    io.screenExpTime = parse_value(cfg, cl, "screenExpTime", 0.0);
+    io.demSolve = parse_value(cfg, cl, "demSolve", 0);
+    io.lbmSolve = parse_value(cfg, cl, "lbSolve", 0);
+    lb.freeSurface = parse_value(cfg, cl, "freeSurfaceSolve", 0);
+    lb.forceField = parse_value(cfg, cl, "forceFieldSolve", 0);
+    lb.latticeBoltzmannGet(cfg, cl);
+    dem.discreteElementGet(cfg, cl);
+    lbdem_params& p = lb.context().params;
+    p.demInitialRepeat = dem.demInitialRepeat;
+    p.maxTime = io.maxTime;
+    p.demSolve = io.demSolve;
+    p.lbSolve = io.lbmSolve;
+    p.freeSurface = lb.freeSurface;
+    p.forceField = lb.forceField;
+}
+
+// ------------------------------------------------------------------------------------------
+// LB
+// ------------------------------------------------------------------------------------------
+void LB::latticeBoltzmannGet(ConfigFile& cfg, CommandLine& cl) {
+    // keys and defaults of LB.cpp:137-336 (Newtonian, no restart, no topography, no turbulence)
+    lbdem_params& p = c_.params;
+    p.unitLength = parse_value(cfg, cl, "unitLength", 1.0);
+    p.unitTime = parse_value(cfg, cl, "unitTime", 1.0);
+    p.unitDensity = parse_value(cfg, cl, "unitDensity", 1.0);
+    unitLength = p.unitLength;
+    unitTime = p.unitTime;
+    const char* ax[3] = {"X", "Y", "Z"};
+    for (int a = 0; a < 3; ++a) p.lbSize[a] = parse_value(cfg, cl, std::string("lbSize") + ax[a], 0.0);
+    const std::string restart = parse_value(cfg, cl, "lbRestart", "false");
+    if (to_int(restart, 0)) {
+        std::cout << "ERROR: lbRestart is not supported" << std::endl;
+        std::exit(1);
+    }
+    for (int a = 0; a < 3; ++a) p.initVelocity[a] = parse_value(cfg, cl, std::string("initVelocity") + ax[a], 0.0);
+    for (int a = 0; a < 3; ++a) p.lbF[a] = parse_value(cfg, cl, std::string("lbF") + ax[a], 0.0);
+    for (int b = 0; b < 6; ++b) p.boundary[b] = parse_value(cfg, cl, "boundary" + std::to_string(b), 1);
+    const std::string rheology = parse_value(cfg, cl, "rheologyModel", "none");
+    if (rheology != "NEWTONIAN") {
+        std::cout << "ERROR: rheologyModel " << rheology << " is not supported (NEWTONIAN only)" << std::endl;
+        std::exit(1);
+    }
+    p.initVisc = parse_value(cfg, cl, "initVisc", 1.0);
+    if (parse_value(cfg, cl, "turbulenceSolve", 0)) {
+        std::cout << "ERROR: turbulenceSolve is not supported" << std::endl;
+        std::exit(1);
+    }
+    p.fluidHeight = parse_value(cfg, cl, "fluidHeight", 0.0);
+    fluidHeight = p.fluidHeight;
+}
+
+void LB::latticeDefinition() {
+    for (int a = 0; a < 3; ++a) lbSize[a] = unsigned(c_.params.lbSize[a] / c_.params.unitLength);
+}
+
+void LB::LBShow() const {
+    std::cout << "USED LB PARAMETERS" << std::endl
+              << "lbSize: " << lbSize[0] << " " << lbSize[1] << " " << lbSize[2] << std::endl
+              << "unitLength " << unitLength << ", unitTime " << unitTime << ", initVisc " << c_.params.initVisc
+              << ", fluidHeight " << fluidHeight << ", freeSurface " << freeSurface << ", forceField " << forceField
+              << std::endl;
+}
+
+void LB::latticeBolzmannInit(wallList&, particleList&, objectList&) {
+    c_.initialise();
+    lbdem_info in;
+    c_.check(c_.api().get_info(c_.ctx(), &in), "lbdem_get_info");
+    totalMass = in.totalMassInit;
+}
+
+void LB::latticeBolzmannStep(particleList&, wallList&, objectList&) {
+    c_.initialise();
+    c_.check(c_.api().lb_step(c_.ctx()), "lbdem_lb_step");
+}
+
+void LB::latticeBoltzmannCouplingStep(bool& newNeighborList, particleList&, objectList&) {
+    c_.initialise();
+    c_.check(c_.api().coupling_step(c_.ctx()), "lbdem_coupling_step");
+    newNeighborList = false;  // LB.cpp:445
+}
+
+void LB::latticeBoltzmannFreeSurfaceStep() {
+    c_.initialise();
+    c_.check(c_.api().fs_step(c_.ctx()), "lbdem_fs_step");
+}
+
+double LB::currentMass() const {
+    double m = 0.0;
+    c_.check(c_.api().total_mass(c_.ctx(), &m), "lbdem_total_mass");
+    return m;
+}
+
+// ------------------------------------------------------------------------------------------
+// DEM
+// ------------------------------------------------------------------------------------------
+static std::ifstream open_input(const char* path) {
+    std::ifstream in(path);
+    std::cout << "Reading " << path << std::endl;
+    if (!in.is_open()) {
+        std::cout << "ERROR: Condition " << path << " is_open() not fulfilled!" << std::endl;
+        std::exit(1);
+    }
+    return in;
+}
+
+void DEM::discreteElementGet(ConfigFile& cfg, CommandLine& cl) {
+    // keys and defaults of DEM.cpp:19-75
+    lbdem_params& p = c_.params;
+    p.density = parse_value(cfg, cl, "density", 0.0);
+    p.objectDensity = parse_value(cfg, cl, "objectDensity", 0.0);
+    p.bodyDensity = parse_value(cfg, cl, "bodyDensity", 0.0);
+    p.linearStiff = parse_value(cfg, cl, "linearStiff", 0.0);
+    p.frictionCoefPart = parse_value(cfg, cl, "frictionCoefPart", 0.0);
+    p.lubNormal = parse_value(cfg, cl, "lubNormal", 0.0);
+    p.lubTangential = parse_value(cfg, cl, "lubTangential", 0.0);
+    p.lubRotation = parse_value(cfg, cl, "lubRotation", 0.0);
+    p.lubCutoff = parse_value(cfg, cl, "lubCutoff", 0.0);
+    p.kinVisc = parse_value(cfg, cl, "kinVisc", 0.0);
+    p.electroForce = parse_value(cfg, cl, "electroForce", 0.0);
+    p.debyeLength = parse_value(cfg, cl, "debyeLength", 0.0);
+    p.objWaitTime = parse_value(cfg, cl, "objWaitTime", 0);
+    const double scale = parse_value(cfg, cl, "scale", 1.0);
+
+    // input/particles.dat: count, then index size r x y z vx vy vz wx wy wz q0(4) q1(4) (DEM.cpp:81-134);
+    // the velocity columns are read and dropped: particle::initialize resets x1 and w1
+    // (elmt.cpp:28-31, called from DEM.cpp:232)
+    {
+        std::ifstream in = open_input("./input/particles.dat");
+        unsigned n = 0;
+        in >> n;
+        c_.psize.assign(n, 1);
+        c_.pr.assign(n, 0.0);
+        c_.px0.assign(3 * size_t(n), 0.0);
+        for (unsigned i = 0; i < n; ++i) {
+            double index, size, r, x[3], v[3], w[3], q[8];
+            in >> index >> size >> r >> x[0] >> x[1] >> x[2] >> v[0] >> v[1] >> v[2] >> w[0] >> w[1] >> w[2];
+            for (double& t : q) in >> t;
+            c_.psize[i] = int(size);
+            c_.pr[i] = r * scale;
+            for (int a = 0; a < 3; ++a) {
+                c_.px0[3 * i + a] = x[a] * scale - 0.05;
+            }
+        }
+    }
+    // input/objects.dat: count, then index r x y z vx vy vz (DEM.cpp:136-165)
+    {
+        std::ifstream in = open_input("./input/objects.dat");
+        unsigned n = 0;
+        in >> n;
+        c_.orad.assign(n, 0.0);
+        c_.ox0.assign(3 * size_t(n), 0.0);
+        c_.ox1.assign(3 * size_t(n), 0.0);
+        for (unsigned i = 0; i < n; ++i) {
+            double index, r, x[3], v[3];
+            in >> index >> r >> x[0] >> x[1] >> x[2] >> v[0] >> v[1] >> v[2];
+            c_.orad[i] = r * scale;
+            for (int a = 0; a < 3; ++a) {
+                c_.ox0[3 * i + a] = x[a] * scale;
+                c_.ox1[3 * i + a] = v[a];
+            }
+        }
+    }
+}
+
+void DEM::discreteElementInit(const unsIntList&, const doubleList&, const vecList&, const double& externalTimeStep,
+                              const tVect, const double&, bool&) {
+    deltat = externalTimeStep / double(1);  // one DEM step per LB step (DEM.cpp:327)
+    c_.create();
+    newNeighborList = true;
+}
+
+void DEM::discreteElementStep(const unsIntList&) {
+    c_.initialise();
+    c_.check(c_.api().dem_step(c_.ctx()), "lbdem_dem_step");
+    ++demTimeStep;        // DEM.cpp:365-366
+    demTime += deltat;
+}
+
+static tVect rec3(const double* r, int off) { return tVect(r[off], r[off + 1], r[off + 2]); }
+
+void DEM::refresh() {
+    if (!c_.initialised()) return;
+    lbdem_info in;
+    c_.check(c_.api().get_info(c_.ctx(), &in), "lbdem_get_info");
+    std::vector<double> rec;
+    rec.resize(size_t(std::max(in.nParticles, 1)) * LBDEM_P_REC);
+    c_.check(c_.api().get_particles(c_.ctx(), rec.data()), "lbdem_get_particles");
+    particles.resize(in.nParticles);
+    for (int i = 0; i < in.nParticles; ++i) {
+        const double* r = &rec[size_t(i) * LBDEM_P_REC];
+        particle& P = particles[i];
+        P.index = unsigned(i);
+        P.x0 = rec3(r, LBDEM_P_X0); P.x1 = rec3(r, LBDEM_P_X1); P.w1 = rec3(r, LBDEM_P_W1);
+        P.FHydro = rec3(r, LBDEM_P_FHYDRO); P.MHydro = rec3(r, LBDEM_P_MHYDRO);
+        P.FContact = rec3(r, LBDEM_P_FCONTACT); P.MContact = rec3(r, LBDEM_P_MCONTACT);
+        P.FLub = rec3(r, LBDEM_P_FLUB); P.MLub = rec3(r, LBDEM_P_MLUB); P.FRep = rec3(r, LBDEM_P_FREP);
+        P.FTotal = rec3(r, LBDEM_P_FTOTAL); P.MTotal = rec3(r, LBDEM_P_MTOTAL); P.FGrav = rec3(r, LBDEM_P_FGRAV);
+        P.fluidVolume = r[LBDEM_P_FLUIDVOL]; P.xiw = rec3(r, LBDEM_P_XIW); P.xio = rec3(r, LBDEM_P_XIO);
+        P.r = r[LBDEM_P_R]; P.m = r[LBDEM_P_M];
+    }
+    rec.assign(size_t(std::max(in.nObjects, 1)) * LBDEM_O_REC, 0.0);
+    c_.check(c_.api().get_objects(c_.ctx(), rec.data()), "lbdem_get_objects");
+    objects.resize(in.nObjects);
+    for (int i = 0; i < in.nObjects; ++i) {
+        const double* r = &rec[size_t(i) * LBDEM_O_REC];
+        object& O = objects[i];
+        O.index = unsigned(i);
+        O.x0 = rec3(r, LBDEM_O_X0); O.x1 = rec3(r, LBDEM_O_X1); O.w1 = rec3(r, LBDEM_O_W1);
+        O.FHydro = rec3(r, LBDEM_O_FHYDRO); O.MHydro = rec3(r, LBDEM_O_MHYDRO);
+        O.FContact = rec3(r, LBDEM_O_FCONTACT); O.MContact = rec3(r, LBDEM_O_MCONTACT);
+        O.FLub = rec3(r, LBDEM_O_FLUB); O.MLub = rec3(r, LBDEM_O_MLUB);
+        O.FTotal = rec3(r, LBDEM_O_FTOTAL); O.MTotal = rec3(r, LBDEM_O_MTOTAL); O.FGrav = rec3(r, LBDEM_O_FGRAV);
+        O.FRep = tVect();  // no electrostatic force acts on objects (DEM.cpp:786-790)
+        O.r = r[LBDEM_O_R]; O.m = r[LBDEM_O_M];
+    }
+    rec.assign(size_t(std::max(in.nWalls, 1)) * LBDEM_W_REC, 0.0);
+    c_.check(c_.api().get_walls(c_.ctx(), rec.data()), "lbdem_get_walls");
+    walls.resize(in.nWalls);
+    for (int i = 0; i < in.nWalls; ++i) {
+        const double* r = &rec[size_t(i) * LBDEM_W_REC];
+        wall& W = walls[i];
+        W.index = unsigned(i);
+        W.FHydro = rec3(r, LBDEM_W_FHYDRO); W.FContact = rec3(r, LBDEM_W_FCONTACT);
+        W.FLub = rec3(r, LBDEM_W_FLUB); W.FTotal = rec3(r, LBDEM_W_FTOTAL);
+    }
+}
+
+// ------------------------------------------------------------------------------------------
+// IO: the result files of the path (headers and columns as IO.cpp:50-85, 1239-1338) plus mass.dat
+// ------------------------------------------------------------------------------------------
+static void append_line(const std::string& path, const char* text) {
+    std::ofstream f(path, std::ios::app);
+    f << text;
+}
+
+void IO::initialize() {
+    exportFileName = workDirectory + "/export.dat";
+    if (demSolve) {
+        objectPositionFileName = workDirectory + "/objPosition.dat";
+        append_line(objectPositionFileName,
+                    "position(x) position(y) position(z) velocity(x) velocity(y) velocity(z) depth(y)  \n");
+        objectForceFileName = workDirectory + "/objForce.dat";
+        append_line(objectForceFileName,
+                    "FExerted(x) FExerted(y) FExerted(z) SExerted(x) SExerted(y) SExerted(z) FTotal(x) FTotal(y) "
+                    "FTotal(z) FContact(x) FContact(y) FContact(z) FHydro(x) FHydro(y) FHydro(z) FLub(x) FLub(y) "
+                    "FLub(z) FRep(x) FRep(y) FRep(z) FGrav(x) FGrav(y) FGrav(z) \n");
+        wallForceFileName = workDirectory + "/wallForce.dat";
+        append_line(wallForceFileName,
+                    "0-FTotal(x) 0-FTotal(y) 0-FTotal(z) 1-FTotal(x) 1-FTotal(y) 1-FTotal(z) 2-FTotal(x) "
+                    "2-FTotal(y) 2-FTotal(z) 3-FTotal(x) 3-FTotal(y) 3-FTotal(z) 4-FTotal(x) 4-FTotal(y) "
+                    "4-FTotal(z) 5-FTotal(x) 5-FTotal(y) 5-FTotal(z) \n");
+    }
+    if (lbmSolve) {
+        massFileName = workDirectory + "/mass.dat";
+        append_line(massFileName, "step time totalFluidMass\n");
+    }
+    lastScreenExp = 0;
+}
+
+void IO::outputStep(LB& lb, DEM& dem) {
+    const unsigned screenExpCounter = screenExpTime > 0 ? unsigned(realTime / screenExpTime) + 1 : 0;
+    if (screenExpCounter <= lastScreenExp) return;
+    lastScreenExp = screenExpCounter;
+    std::ofstream exportFile(exportFileName, std::ios::app);
+    std::cout << currentTimeStep << "; time=" << realTime << "\t";
+    exportFile << currentTimeStep << "; time=" << realTime << "\t";
+    dem.refresh();
+    if (demSolve) {
+        if (!dem.objects.empty()) {
+            exportObjectPosition(dem.objects, lb);
+            exportObjectForce(dem.objects, lb);
+        }
+        if (!dem.walls.empty()) exportWallForce(dem.walls);
+    }
+    if (lbmSolve && c_.initialised()) exportMass(lb);
+    std::cout << "\n";
+    exportFile << "\n";
+    std::cout.flush();
+}
+
+void IO::exportObjectPosition(const objectList& objects, const LB& lb) {
+    std::ofstream f(objectPositionFileName, std::ios::app);
+    for (const object& o : objects) {
+        const double depth = (lb.fluidHeight + o.r) - o.x0.y;
+        o.x0.print(f);
+        f << "\t";
+        o.x1.print(f);
+        f << "\t" << depth;
+        f << std::endl;
+    }
+}
+
+void IO::exportObjectForce(const objectList& objects, const LB& lb) {
+    std::ofstream f(objectForceFileName, std::ios::app);
+    for (const object& o : objects) {
+        const double depth = (lb.fluidHeight + o.r) - o.x0.y;
+        const tVect FExerted = o.FTotal - o.FGrav;
+        tVect SExerted;
+        if (depth > 0) {
+            const double area = 4.0 * M_PI * o.r * o.r * (depth / (2.0 * o.r));
+            SExerted = FExerted / area;
+        }
+        FExerted.print(f); f << "\t";
+        SExerted.print(f); f << "\t";
+        o.FTotal.print(f); f << "\t";
+        o.FContact.print(f); f << "\t";
+        o.FHydro.print(f); f << "\t";
+        o.FLub.print(f); f << "\t";
+        o.FRep.print(f);
+        f << std::endl;
+    }
+}
+
+void IO::exportWallForce(const wallList& walls) {
+    std::ofstream f(wallForceFileName, std::ios::app);
+    for (const wall& w : walls) {
+        w.FTotal.print(f); f << "\t";
+        w.FContact.print(f); f << "\t";
+    }
+    f << std::endl;
+}
+
+void IO::exportMass(const LB& lb) {
+    std::ofstream f(massFileName, std::ios::app);
+    f.precision(17);
+    f << currentTimeStep << " " << realTime << " " << lb.currentMass() << "\n";
+}
+
+// ------------------------------------------------------------------------------------------
+// one coupled step (main.cpp:38-67)
+// ------------------------------------------------------------------------------------------
+void goCycle(IO& io, DEM& dem, LB& lb) {
+    static const unsIntList boundary;  // the device holds the boundary codes
+    io.realTime += lb.unitTime;
+    ++io.currentTimeStep;
+    ++lb.time;
+    if (io.demSolve) dem.discreteElementStep(boundary);
+    if (io.lbmSolve && dem.demTime >= dem.demInitialRepeat) {
+        if (io.demSolve) lb.latticeBoltzmannCouplingStep(dem.newNeighborList, dem.particles, dem.objects);
+        lb.latticeBolzmannStep(dem.particles, dem.walls, dem.objects);
+        if (lb.freeSurface) lb.latticeBoltzmannFreeSurfaceStep();
+    } else if (io.lbmSolve && dem.demTime <= dem.demInitialRepeat) {
+        if (io.demSolve) lb.latticeBoltzmannCouplingStep(dem.newNeighborList, dem.particles, dem.objects);
+    }
+    io.outputStep(lb, dem);
+}
+
+}  // namespace lbdem

@@ -1,0 +1,156 @@
+// Host facade of the coupled LB / DEM step with the reference's class and method names
+// (LB.h:166-172, DEM.h:152-154, IO.h:63-64, main.cpp:38-67, 128-185). The solver state lives
+// behind the C ABI (include/lbdem.h) on the device; the public members that the reference's IO
+// reads are host mirrors refreshed by `refresh()` (a download on the output interval only).
+#pragma once
+
+#include <cstdint>
+#include <iosfwd>
+#include <string>
+#include <vector>
+
+#include "abi.hpp"
+#include "config.hpp"
+
+namespace lbdem {
+
+struct tVect {
+    double x = 0.0, y = 0.0, z = 0.0;
+    tVect() = default;
+    tVect(double a, double b, double c) : x(a), y(b), z(c) {}
+    tVect operator+(const tVect& o) const { return {x + o.x, y + o.y, z + o.z}; }
+    tVect operator-(const tVect& o) const { return {x - o.x, y - o.y, z - o.z}; }
+    tVect operator*(double s) const { return {x * s, y * s, z * s}; }
+    tVect operator/(double s) const { return {x / s, y / s, z / s}; }
+    // "x y z " with the stream's current formatting (the reference's tVect::print)
+    void print(std::ostream& os) const;
+};
+
+// host mirrors of the element records (fields on the path; elmt.h:15-228, utils.h:19-101)
+struct particle {
+    unsigned index = 0;
+    int size = 1;
+    double r = 0.0, m = 0.0, fluidVolume = 0.0;
+    tVect x0, x1, w1, FHydro, MHydro, FContact, MContact, FLub, MLub, FRep, FTotal, MTotal, FGrav, xiw, xio;
+};
+struct object {
+    unsigned index = 0;
+    double r = 0.0, m = 0.0;
+    tVect x0, x1, w1, FHydro, MHydro, FContact, MContact, FLub, MLub, FRep, FTotal, MTotal, FGrav;
+};
+struct wall {
+    unsigned index = 0;
+    tVect FHydro, FContact, FLub, FTotal;
+};
+using particleList = std::vector<particle>;
+using objectList = std::vector<object>;
+using wallList = std::vector<wall>;
+using unsIntList = std::vector<unsigned>;
+using doubleList = std::vector<double>;
+using vecList = std::vector<tVect>;
+
+// One solver instance: the loaded ABI, the parameters gathered by the Get methods, the inputs
+// read from input/*.dat and the device context once created.
+class Context {
+public:
+    explicit Context(const std::string& libraryPath, int precision = LBDEM_PREC_FP64, int device = 0);
+    ~Context();
+    Context(const Context&) = delete;
+    Context& operator=(const Context&) = delete;
+
+    const Api& api() const { return api_; }
+    lbdem_params params{};
+    std::vector<int> psize;
+    std::vector<double> pr, px0, orad, ox0, ox1;
+
+    lbdem_ctx* ctx() const { return ctx_; }
+    void create();                    // lbdem_create + inputs (idempotent)
+    void initialise();                // lbdem_init (idempotent)
+    bool initialised() const { return initialised_; }
+    // status check with the reference's behaviour: message and exit(1) (macros.h:10)
+    void check(int rc, const char* what) const;
+
+private:
+    Api api_;
+    lbdem_ctx* ctx_ = nullptr;
+    bool initialised_ = false;
+};
+
+class LB {
+public:
+    explicit LB(Context& c) : c_(c) {}
+    // LB.cpp:128-338 (the keys of the Newtonian path)
+    void latticeBoltzmannGet(ConfigFile& configFile, CommandLine& commandLine);
+    void latticeDefinition();                                                       // LB.cpp:85-126
+    void LBShow() const;
+    void latticeBolzmannInit(wallList& walls, particleList& particles, objectList& objects);  // LB.cpp:340-398
+    void latticeBolzmannStep(particleList& particles, wallList& walls, objectList& objects);  // LB.cpp:400-418
+    void latticeBoltzmannCouplingStep(bool& newNeighborList, particleList& particles, objectList& objects);  // LB.cpp:432-467
+    void latticeBoltzmannFreeSurfaceStep();                                           // LB.cpp:422-430
+    // total fluid mass over active nodes outside particles (LB.cpp:388-395)
+    double currentMass() const;
+
+    unsigned int time = 0;
+    int freeSurface = 0, forceField = 0, lbmInitialRepeat = 0;
+    double fluidHeight = 0.0;   // physical, as configured
+    double unitTime = 1.0, unitLength = 1.0;
+    unsigned lbSize[3] = {0, 0, 0};
+    double totalMass = 0.0;     // at initialisation (LB.cpp:388-395)
+    Context& context() const { return c_; }
+
+private:
+    Context& c_;
+};
+
+class DEM {
+public:
+    explicit DEM(Context& c) : c_(c) {}
+    // DEM.cpp:17-197: parameters, input/particles.dat and input/objects.dat (relative to the
+    // working directory, as the reference reads them)
+    void discreteElementGet(ConfigFile& configFile, CommandLine& commandLine);
+    // DEM.cpp:199-357 (arguments kept for signature parity; the device derives them)
+    void discreteElementInit(const unsIntList& boundary, const doubleList& externalBoundary,
+                             const vecList& boundaryLocation, const double& externalTimeStep,
+                             const tVect externalForce, const double& fluidHeight, bool& freeSurface);
+    void discreteElementStep(const unsIntList& boundary);                            // DEM.cpp:359-401
+    // downloads particle, object and wall records into the mirrors below
+    void refresh();
+
+    double demTime = 0.0, demInitialRepeat = 0.0, deltat = 1.0;
+    unsigned int demTimeStep = 0;
+    bool newNeighborList = false;
+    particleList particles;
+    objectList objects;
+    wallList walls;
+    Context& context() const { return c_; }
+
+private:
+    Context& c_;
+};
+
+class IO {
+public:
+    explicit IO(Context& c) : c_(c) {}
+    void initialize();                             // IO.cpp:9-112 (the result files of the path)
+    void outputStep(LB& lb, DEM& dem);             // IO.cpp:113-168 (screen interval)
+
+    std::string workDirectory = "data", configFileName = "config.cfg";
+    double realTime = 0.0, maxTime = 0.0, screenExpTime = 0.0;
+    unsigned int currentTimeStep = 0;
+    int demSolve = 0, lbmSolve = 0;
+
+private:
+    void exportObjectPosition(const objectList& objects, const LB& lb);   // IO.cpp:1316-1338
+    void exportObjectForce(const objectList& objects, const LB& lb);      // IO.cpp:1239-1293
+    void exportWallForce(const wallList& walls);                          // IO.cpp:1295-1314
+    void exportMass(const LB& lb);                                        // step, time, total fluid mass
+    Context& c_;
+    unsigned lastScreenExp = 0;
+    std::string exportFileName, objectPositionFileName, objectForceFileName, wallForceFileName, massFileName;
+};
+
+// parseConfigFile (main.cpp:128-185) and one goCycle (main.cpp:38-67)
+void parseConfigFile(IO& io, DEM& dem, LB& lb, ConfigFile& configFile, CommandLine& commandLine);
+void goCycle(IO& io, DEM& dem, LB& lb);
+
+}  // namespace lbdem

@@ -368,6 +368,10 @@ class ParticleSet:
     size: np.ndarray = field(default_factory=lambda: np.zeros(0, np.int32))
     r: np.ndarray = field(default_factory=lambda: np.zeros(0))
     x0: np.ndarray = field(default_factory=lambda: np.zeros((0, 3)))
+    # velocity columns as read (particles.dat columns 6-11); the solver starts every sphere at
+    # rest, as particle::initialize resets x1 and w1 (elmt.cpp:28-31, called from DEM.cpp:232)
+    x1: Optional[np.ndarray] = None
+    w1: Optional[np.ndarray] = None
 
 
 @dataclass
@@ -391,6 +395,9 @@ def read_particles(path: str, scale: float = 1.0) -> ParticleSet:
     ps.size = rows[:, 1].astype(np.int32)
     ps.r = rows[:, 2] * scale
     ps.x0 = rows[:, 3:6] * scale - 0.05
+    if n and (np.any(rows[:, 6:12] != 0.0)):
+        ps.x1 = rows[:, 6:9].copy()
+        ps.w1 = rows[:, 9:12].copy()
     return ps
 
 
@@ -406,15 +413,20 @@ def read_objects(path: str, scale: float = 1.0) -> ObjectSet:
     return os_
 
 
-def write_particles(path: str, r: Sequence[float], centers: np.ndarray, size: Optional[Sequence[int]] = None):
+def write_particles(path: str, r: Sequence[float], centers: np.ndarray, size: Optional[Sequence[int]] = None,
+                    x1: Optional[np.ndarray] = None, w1: Optional[np.ndarray] = None):
     """Rows in particles.dat format; ``centers`` are written as given (the reader subtracts 0.05)."""
     centers = np.asarray(centers, dtype=np.float64).reshape(-1, 3)
+    n = len(r)
+    x1 = np.zeros((n, 3)) if x1 is None else np.asarray(x1, dtype=np.float64).reshape(-1, 3)
+    w1 = np.zeros((n, 3)) if w1 is None else np.asarray(w1, dtype=np.float64).reshape(-1, 3)
     with open(path, "w") as fh:
-        fh.write(f"{len(r)}\n")
-        for i in range(len(r)):
+        fh.write(f"{n}\n")
+        for i in range(n):
             s = 1 if size is None else int(size[i])
             x, y, z = (float(c) for c in centers[i])
-            fh.write(f"{i} {s} {float(r[i])!r} {x!r} {y!r} {z!r} 0 0 0 0 0 0 1 0 0 0 0 0 0 0\n")
+            v = " ".join(repr(float(c)) for c in (*x1[i], *w1[i]))
+            fh.write(f"{i} {s} {float(r[i])!r} {x!r} {y!r} {z!r} {v} 1 0 0 0 0 0 0 0\n")
 
 
 def write_objects(path: str, r: Sequence[float], x0: np.ndarray, x1: np.ndarray):

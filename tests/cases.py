@@ -64,6 +64,14 @@ def _case_values(name: str) -> Tuple[Dict[str, object], np.ndarray, np.ndarray, 
         pr = np.array([1.0, 0.9, 1.0])
         pc = np.array([[4.0, 3.0, 4.0], [2.0, 1.6, 5.5], [5.8, 4.4, 2.5]])
         return v, pr, pc, np.array([1.4]), np.array([[4.1, 7.55, 3.9]]), np.array([[0.0, -1.5, 0.0]])
+    if name == "moving":
+        # the suspension geometry with nonzero velocity columns in particles.dat: the reference reads
+        # them (DEM.cpp:102-113) and particle::initialize resets them (elmt.cpp:28-31)
+        v.update(lbSizeX=16.0, lbSizeY=22.0, lbSizeZ=16.0, fluidHeight=14.0)
+        pr = np.array([2.0, 2.0, 2.0, 1.6, 1.6, 2.0])
+        pc = np.array([[8.05, 11.25, 8.05], [8.05, 6.95, 8.05], [3.4, 2.5, 3.5], [12.6, 5.0, 12.2],
+                       [12.3, 12.0, 3.8], [4.0, 9.0, 12.0]])
+        return v, pr, pc, np.array([3.0]), np.array([[8.0, 16.55, 8.0]]), np.array([[0.0, -0.06, 0.0]])
     if name == "lb_closed":
         # no free surface, no spheres, no impactor: gravity-driven closed box
         v.update(lbSizeX=10.0, lbSizeY=12.0, lbSizeZ=9.0, freeSurfaceSolve=0, fluidHeight=0.0, lbFX=1.0e-4)
@@ -71,16 +79,35 @@ def _case_values(name: str) -> Tuple[Dict[str, object], np.ndarray, np.ndarray, 
     raise KeyError(name)
 
 
-CASES = ("fs_impactor", "suspension", "periodic", "units", "lb_closed")
+CASES = ("fs_impactor", "suspension", "periodic", "units", "lb_closed", "moving")
+
+# initial sphere velocities (x1, w1) of the cases that have them
+VELOCITIES = {
+    "moving": (np.array([[0.02, -0.01, 0.0], [-0.015, 0.02, 0.01], [0.0, 0.0, -0.02], [0.01, 0.0, 0.0],
+                         [0.0, -0.03, 0.0], [-0.01, 0.01, 0.02]]),
+               np.array([[0.0, 0.01, 0.0], [0.005, 0.0, -0.005], [0.0, 0.0, 0.0], [0.01, 0.01, 0.0],
+                         [0.0, 0.0, 0.02], [-0.005, 0.0, 0.0]])),
+}
+
+# runs of the host program compared with the reference program's result files: case, config
+# overrides (time window and screen interval)
+HOST_RUNS = {"suspension": {"maxTime": 39.0, "screenExpTime": 4.0},
+             "fs_impactor": {"maxTime": 29.0, "screenExpTime": 3.0},
+             "moving": {"maxTime": 29.0, "screenExpTime": 2.0}}
+HOST_FILES = ("objPosition.dat", "objForce.dat", "wallForce.dat")
 
 
-def make_case(root: str, name: str) -> str:
-    """Writes config.cfg and input/{particles,objects,body}.dat for a named case; returns the dir."""
+def make_case(root: str, name: str, overrides: Dict[str, object] = None, subdir: str = None) -> str:
+    """Writes config.cfg and input/{particles,objects,body}.dat for a named case (config values
+    replaced by ``overrides``) into root/(subdir or name); returns the dir."""
     v, pr, pc, orr, ox0, ox1 = _case_values(name)
-    d = os.path.join(root, name)
+    if overrides:
+        v.update(overrides)
+    d = os.path.join(root, subdir or name)
     os.makedirs(os.path.join(d, "input"), exist_ok=True)
     lbdem.write_getpot(os.path.join(d, "config.cfg"), v)
-    lbdem.write_particles(os.path.join(d, "input", "particles.dat"), pr, pc)
+    x1, w1 = VELOCITIES.get(name, (None, None))
+    lbdem.write_particles(os.path.join(d, "input", "particles.dat"), pr, pc, x1=x1, w1=w1)
     lbdem.write_objects(os.path.join(d, "input", "objects.dat"), orr, ox0, ox1)
     with open(os.path.join(d, "input", "body.dat"), "w") as fh:
         fh.write("0\n")

@@ -5,9 +5,11 @@ Runs the reference compiled from its own sources (oracle/_ref/ref_driver, built 
 cases of tests/cases.py, single-threaded, and stores the solver state after selected steps as
 compressed .npz files: node types / particle flags / solid indices, post-streaming populations
 of active nodes, density and mass fields, particle / object / wall records, Verlet lists and
-the total fluid mass. Only data is stored.
+the total fluid mass. Also runs the reference's own main loop on the host-program cases
+(cases.HOST_RUNS) and keeps the result files it writes (objPosition.dat, objForce.dat,
+wallForce.dat) under host_<case>/. Only data is stored.
 
-usage: python tests/golden/make_golden.py
+usage: python tests/golden/make_golden.py [--host-only]
 """
 from __future__ import annotations
 
@@ -24,12 +26,35 @@ sys.path.insert(0, os.path.dirname(HERE))
 import cases  # noqa: E402
 
 DRIVER = os.path.join(cases.REPO, "oracle", "_ref", "ref_driver")
-PLAN = {"lb_closed": [1, 12], "fs_impactor": [1, 40], "suspension": [1, 40, 120], "periodic": [40], "units": [40]}
+PLAN = {"lb_closed": [1, 12], "fs_impactor": [1, 40], "suspension": [1, 40, 120], "periodic": [40], "units": [40],
+        "moving": [1, 30]}
+
+
+def host_outputs():
+    """Result files of the reference's main loop (IO::outputStep) on the host-program cases."""
+    import shutil
+    with tempfile.TemporaryDirectory() as tmp:
+        for name, over in cases.HOST_RUNS.items():
+            d = cases.make_case(tmp, name, over)
+            steps = int(over["maxTime"]) + 1
+            env = dict(os.environ, OMP_NUM_THREADS="1")
+            r = subprocess.run([DRIVER, "config.cfg", "out", str(steps), "dump"], cwd=d, env=env, capture_output=True,
+                               text=True)
+            if r.returncode:
+                sys.exit(f"reference run failed for {name}:\n{r.stdout[-2000:]}\n{r.stderr[-2000:]}")
+            dst = os.path.join(HERE, f"host_{name}")
+            os.makedirs(dst, exist_ok=True)
+            for f in cases.HOST_FILES:
+                shutil.copy(os.path.join(d, "out", f), os.path.join(dst, f))
+            print("host outputs of", name, "->", dst)
 
 
 def main():
     if not os.path.exists(DRIVER):
         sys.exit(f"{DRIVER} missing: run `make -C oracle ref` where the reference sources are present")
+    host_outputs()
+    if "--host-only" in sys.argv:
+        return
     manifest = {}
     with tempfile.TemporaryDirectory() as tmp:
         for name, steps in PLAN.items():

@@ -94,7 +94,7 @@ def test_fp16_storage_within_tolerance(oracle_lib, gpu_lib, case_root):
 GOLD = os.path.join(os.path.dirname(os.path.abspath(__file__)), "golden")
 
 
-@pytest.mark.parametrize("name", ["suspension", "fs_impactor", "periodic", "units"])
+@pytest.mark.parametrize("name", ["suspension", "fs_impactor", "periodic", "units", "moving"])
 def test_fp64_matches_reference_fixtures(gpu_lib, case_root, name):
     with open(os.path.join(GOLD, "manifest.json")) as fh:
         steps = sorted(m["step"] for m in json.load(fh).values() if m["case"] == name)
