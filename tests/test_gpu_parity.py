@@ -82,6 +82,13 @@ def test_fp64_long_window(oracle_lib, gpu_lib, case_root):
     run_pair(oracle_lib, gpu_lib, case_root, "suspension", lbdem.PREC_FP64, 300, every=25)
 
 
+@pytest.mark.parametrize("name", ["fs_impactor", "periodic", "moving"])
+def test_graph_replays_match_oracle(oracle_lib, gpu_lib, case_root, name):
+    """cycle(n) with n >= 2 replays a captured pair of steady-state steps (HIP graph): 10-step
+    chunks with downloads and synchronisation between replays, every chunk against the oracle."""
+    run_pair(oracle_lib, gpu_lib, case_root, name, lbdem.PREC_FP64, 60, every=10)
+
+
 @pytest.mark.parametrize("name", ["suspension", "units", "periodic"])
 def test_fp32_offsets_within_tolerance(oracle_lib, gpu_lib, case_root, name):
     run_pair(oracle_lib, gpu_lib, case_root, name, lbdem.PREC_FP32, 40, every=5)
