@@ -9,7 +9,11 @@ free surface). Inputs are resident on the device when the timed region starts.
 
 Prints one JSON line: MLUPS over active (fluid + interface) cells for the FP32-offset step, the
 FP64 step on the same state for comparison, the fused kernel's HBM roofline fraction and a CPU
-baseline (the reference program compiled from its sources, on a bounded sample).
+baseline (the reference program compiled from its sources, on a bounded sample); then the
+batched workload (64 boxes of edge 48-128 submitted together to the run pool) and the served
+workload (Poisson arrivals of runs with edge 64-192 and 1000-8000 steps at 0.8 of the batched
+capacity: p50/p99 latency and runs/hour, beside the same requests served one at a time with
+plain launches).
 """
 import argparse
 import json
@@ -126,6 +130,113 @@ def cpu_baseline(seconds_budget=25.0):
                           f"{nst} timed steps after 3 warm-up steps, {active} active cells"}
 
 
+def run_pool(lib, runs, prec, budget, max_concurrent, chunk, arrivals=None):
+    """Submits ``runs`` to a run pool (all at once, or at ``arrivals`` seconds on the pool clock)
+    and waits for all; returns per-run results and the pool statistics."""
+    pool = lbdem.Pool(lib, budget_bytes=budget, max_concurrent=max_concurrent, steps_per_chunk=chunk)
+    params = [lbdem.params_from_values(r.values, prec) for r in runs]
+    ids = []
+    t_base = pool.clock()
+    for i, r in enumerate(runs):
+        if arrivals is not None:
+            wait = t_base + float(arrivals[i]) - pool.clock()
+            if wait > 0:
+                time.sleep(wait)
+        ids.append(pool.submit(params[i], r.particles, r.objects, r.nsteps))
+        if (i + 1) % 16 == 0:
+            log(f"[bench] pool: {i + 1} runs submitted, stats {pool.stats()}")
+    while True:  # a progress line every 20 s while the runs finish
+        try:
+            pool.drain(20.0)
+            break
+        except lbdem.LbdemError:
+            log(f"[bench] pool: draining, stats {pool.stats()}")
+    res = [pool.poll(rid) for rid in ids]
+    st = pool.stats()
+    pool.close()
+    bad = [x["message"] for x in res if x["state"] != lbdem.RUN_DONE]
+    if bad:
+        raise RuntimeError(f"pool runs failed: {bad[:3]}")
+    return res, st
+
+
+def batched(prec, n_boxes=64, nsteps=2000):
+    """Many independent boxes of uneven size submitted together to the run pool: aggregate
+    lattice updates per second over the whole batch."""
+    lib = lbdem.Library.load()
+    t0 = time.time()
+    runs = workloads.batched_runs(n_boxes, nsteps)
+    log(f"[bench] batched: {n_boxes} boxes generated in {time.time() - t0:.1f} s")
+    est = [lbdem.estimate_bytes(lib, lbdem.params_from_values(r.values, prec), r.particles, r.objects) for r in runs]
+    budget = int(1.05 * sum(est)) + (64 << 20)
+    res, st = run_pool(lib, runs, prec, budget, max_concurrent=n_boxes, chunk=32)
+    t_first = min(x["tSubmit"] for x in res)
+    span = max(x["tDone"] for x in res) - t_first
+    node_steps = sum(x["nodeSteps"] for x in res)
+    out = {"boxes": n_boxes, "steps_each": nsteps, "edges": sorted({r.edge for r in runs}),
+           "total_nodes": int(sum(r.nodes for r in runs)), "seconds": round(span, 3),
+           "mlups_box": round(node_steps / span / 1e6, 1),
+           "particle_steps_per_s": round(sum(len(r.particles.r) * r.nsteps for r in runs) / span, 1),
+           "hbm_budget_bytes": budget, "bytes_peak": st["bytesPeak"], "active_peak": st["activePeak"]}
+    log(f"[bench] batched: {out}")
+    return out
+
+
+def served(prec, n_req, capacity_mlups, load=0.8, seed=20261204):
+    """Poisson arrivals of impact runs at ``load`` x the capacity measured on the batched
+    workload, served by the run pool; then the same requests served one at a time with plain
+    per-run launches (no pool, no graph replay). Latency is arrival to result."""
+    lib = lbdem.Library.load()
+    mean_nodes = float(np.mean([e ** 3 for e in workloads.SERVE_EDGES]))
+    mean_steps = 0.5 * (1000 + 8000)
+    rate = load * capacity_mlups * 1e6 / (mean_nodes * mean_steps)
+    t0 = time.time()
+    arrivals, runs = workloads.served_requests(n_req, rate, seed=seed)
+    log(f"[bench] served: {n_req} requests at {rate:.3f}/s generated in {time.time() - t0:.1f} s")
+    budget = 16 << 30
+    res, st = run_pool(lib, runs, prec, budget, max_concurrent=32, chunk=32, arrivals=arrivals)
+    lat = np.array([x["tDone"] - x["tSubmit"] for x in res])
+    t_first = min(x["tSubmit"] for x in res)
+    span = max(x["tDone"] for x in res) - t_first
+    node_steps = sum(x["nodeSteps"] for x in res)
+    pool_out = {"p50_s": round(float(np.percentile(lat, 50)), 3), "p99_s": round(float(np.percentile(lat, 99)), 3),
+                "mean_s": round(float(lat.mean()), 3), "makespan_s": round(span, 3),
+                "mlups_box": round(node_steps / span / 1e6, 1), "runs_per_hour": round(3600.0 * n_req / span, 1),
+                "active_peak": st["activePeak"], "bytes_peak": st["bytesPeak"]}
+    log(f"[bench] served (pool): {pool_out}")
+    # plain sequential service of the same requests: each run alone, kernels launched one by one;
+    # a single server in arrival order (start = max(arrival, previous finish))
+    service = []
+    for i, r in enumerate(runs):
+        sim = lbdem.Simulation(lib, lbdem.params_from_values(r.values, prec), r.particles, r.objects)
+        sim.use_graphs(False)
+        t = time.perf_counter()
+        sim.init()
+        sim.cycle(r.nsteps)
+        sim.synchronize()
+        sim.objects()
+        service.append(time.perf_counter() - t)
+        sim.close()
+        if (i + 1) % 8 == 0:
+            log(f"[bench] served (plain sequential): {i + 1}/{n_req} runs")
+    done = 0.0
+    lat_seq = []
+    for a, s_ in zip(arrivals, service):
+        done = max(float(a), done) + s_
+        lat_seq.append(done - float(a))
+    lat_seq = np.array(lat_seq)
+    span_seq = done - float(arrivals[0])
+    seq_out = {"p50_s": round(float(np.percentile(lat_seq, 50)), 3), "p99_s": round(float(np.percentile(lat_seq, 99)), 3),
+               "mean_s": round(float(lat_seq.mean()), 3), "makespan_s": round(span_seq, 3),
+               "mlups_box": round(node_steps / span_seq / 1e6, 1), "runs_per_hour": round(3600.0 * n_req / span_seq, 1),
+               "service_mlups_box": round(node_steps / sum(service) / 1e6, 1)}
+    log(f"[bench] served (plain sequential): {seq_out}")
+    return {"requests": n_req, "rate_per_s": round(rate, 4), "offered_load": load,
+            "capacity_mlups_box": round(capacity_mlups, 1), "edges": list(workloads.SERVE_EDGES),
+            "steps_range": [1000, 8000], "hbm_budget_bytes": budget, "pool": pool_out, "plain_sequential": seq_out,
+            "runs_per_hour_ratio": round(pool_out["runs_per_hour"] / seq_out["runs_per_hour"], 3)}
+
+
 def copy_probe(nbytes=1 << 30, reps=3):
     """A plain device-to-device copy of known size (torch's copy kernel): reads and writes
     ``nbytes`` per launch, the reference for scaling the HBM byte counters of rocprofv3."""
@@ -151,6 +262,10 @@ def main():
     ap.add_argument("--no-fp64", action="store_true", help="skip the FP64 comparison run")
     ap.add_argument("--no-cpu-baseline", action="store_true")
     ap.add_argument("--small", action="store_true", help="quarter-size box (for quick checks)")
+    ap.add_argument("--batched", type=int, default=64, help="boxes in the batched run (0: skip)")
+    ap.add_argument("--batched-steps", type=int, default=2000)
+    ap.add_argument("--served", type=int, default=24, help="requests in the served run (0: skip)")
+    ap.add_argument("--headline-only", action="store_true", help="skip the batched and served runs")
     ap.add_argument("--copy-probe", action="store_true",
                     help="only run a plain 1 GiB device copy three times (calibrates HBM byte counters)")
     args = ap.parse_args()
@@ -173,6 +288,12 @@ def main():
     fp64 = None
     if not args.no_fp64 and prec != lbdem.PREC_FP64:
         fp64 = run_gpu(vals, ps, obs, lbdem.PREC_FP64, max(20, args.steps // 4), args.warmup)
+    bat = srv = None
+    if not args.headline_only and args.batched > 0:
+        bat = batched(prec, args.batched, args.batched_steps)
+    if not args.headline_only and args.served > 0:
+        cap = bat["mlups_box"] if bat else main_res["mlups_box"]
+        srv = served(prec, args.served, cap)
     cpu = None if args.no_cpu_baseline else cpu_baseline()
     b = BYTES_PER_CELL[prec]
     ach = main_res["fused_achieved_gbs"]
@@ -213,6 +334,8 @@ def main():
                      "algorithmic_bytes_per_launch": int(main_res["active_cells"] * b),
                      "kernel": "k_fused", "bytes_per_active_node": b},
         "cpu_baseline": cpu,
+        "batched": bat,
+        "served": srv,
     }
     print(json.dumps(out), flush=True)
 

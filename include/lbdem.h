@@ -28,6 +28,7 @@
 #ifndef LBDEM_H
 #define LBDEM_H
 
+#include <stddef.h>
 #include <stdint.h>
 
 #ifdef __cplusplus
@@ -147,6 +148,86 @@ int lbdem_total_mass(lbdem_ctx* ctx, double* mass);
  * summed duration in milliseconds and the number of launches since the last enable. */
 int lbdem_profile_enable(lbdem_ctx* ctx, int enable);
 int lbdem_profile_read(lbdem_ctx* ctx, double* fused_ms, int* fused_launches);
+
+/* Device bytes of one context (no reference counterpart). lbdem_estimate_bytes computes, from
+ * the parameters and the particle and impactor radii alone (no device call), the bytes
+ * lbdem_init will allocate; lbdem_device_bytes reports what an initialised context holds. */
+int lbdem_estimate_bytes(const lbdem_params* params, int nParticles, const double* r, int nObjects,
+                         const double* objR, size_t* bytes);
+int lbdem_device_bytes(lbdem_ctx* ctx, size_t* bytes);
+
+/* Replay of captured step pairs in lbdem_cycle (no reference counterpart): on by default
+ * (LBDEM_GRAPHS=0 in the environment turns the default off); 0 launches every step's kernels
+ * one by one. Results are the same either way. */
+int lbdem_use_graphs(lbdem_ctx* ctx, int enable);
+
+/* ---------------------------------------------------------------------------------------------
+ * Run pool (no reference counterpart; batched and served runs of many independent boxes on one
+ * GPU). A worker thread admits submitted runs in arrival order while their estimated device bytes
+ * fit the HBM budget and fewer than maxConcurrent runs are active, gives every run its own context
+ * and stream, advances all active runs in chunks of stepsPerChunk coupled steps (goCycle, as
+ * lbdem_cycle) and releases a run's device memory when its last step is done. Runs never
+ * communicate: the result of a run does not depend on what else runs beside it.
+ * ------------------------------------------------------------------------------------------- */
+typedef struct lbdem_pool lbdem_pool;
+
+typedef struct lbdem_run_desc {
+    lbdem_params params;
+    int nParticles;
+    const int* size;             /* as lbdem_set_particles */
+    const double* r;
+    const double* x0;
+    int nObjects;                /* as lbdem_set_objects */
+    const double* objR;
+    const double* objX0;
+    const double* objX1;
+    int nsteps;                  /* coupled steps to run */
+} lbdem_run_desc;
+
+#define LBDEM_RUN_QUEUED 0
+#define LBDEM_RUN_ACTIVE 1
+#define LBDEM_RUN_DONE 2
+#define LBDEM_RUN_FAILED 3
+#define LBDEM_RUN_CANCELLED 4
+
+typedef struct lbdem_run_result {
+    int state;                   /* LBDEM_RUN_* */
+    int error;                   /* LBDEM_E* of a failed run */
+    int stepsDone;
+    int nObjects;
+    double tSubmit, tStart, tDone;   /* seconds on the pool clock (lbdem_pool_clock) */
+    double objects[2 * LBDEM_O_REC]; /* impactor records after the last step */
+    double totalMass;            /* lbdem_total_mass after the last step */
+    int nActive;                 /* active nodes after the last step */
+    int64_t nodeSteps;           /* sum over steps of the box's node count (totNodes * steps) */
+    size_t deviceBytes;
+    char message[128];           /* error message of a failed run */
+} lbdem_run_result;
+
+/* hbmBudgetBytes: one device allocation the runs are carved from (0: 85% of the free memory);
+ * maxConcurrent: most runs active at once; stepsPerChunk: coupled steps per enqueue (0: 32) */
+int lbdem_pool_create(size_t hbmBudgetBytes, int maxConcurrent, int stepsPerChunk, int device, lbdem_pool** out);
+/* copies the run's inputs; returns its id (0, 1, 2, ... in submission order) */
+int lbdem_pool_submit(lbdem_pool* pool, const lbdem_run_desc* run, int* runId);
+int lbdem_pool_poll(lbdem_pool* pool, int runId, lbdem_run_result* result);
+/* waits until the run is done or failed, or timeoutSeconds passed (LBDEM_E_STATE then) */
+int lbdem_pool_wait(lbdem_pool* pool, int runId, double timeoutSeconds, lbdem_run_result* result);
+/* waits until every submitted run is done or failed */
+int lbdem_pool_drain(lbdem_pool* pool, double timeoutSeconds);
+/* a queued run is dropped; an active run stops after its chunks in flight; both end as
+ * LBDEM_RUN_CANCELLED and release their memory. Other runs are not affected. */
+int lbdem_pool_cancel(lbdem_pool* pool, int runId);
+
+typedef struct lbdem_pool_stats {
+    int submitted, queued, active, done, failed, cancelled;
+    int activePeak;              /* most runs active at once */
+    size_t budget, bytesInUse, bytesPeak;
+} lbdem_pool_stats;
+int lbdem_pool_get_stats(lbdem_pool* pool, lbdem_pool_stats* stats);
+double lbdem_pool_clock(const lbdem_pool* pool);
+const char* lbdem_pool_last_error(const lbdem_pool* pool);
+/* cancels what is queued or active, waits for the device and frees everything */
+int lbdem_pool_destroy(lbdem_pool* pool);
 
 #ifdef __cplusplus
 }

@@ -10,6 +10,7 @@
 #include <cstdlib>
 #include <cstring>
 #include <string>
+#include <type_traits>
 #include <vector>
 
 using namespace lbdem;
@@ -59,7 +60,12 @@ struct lbdem_ctx {
     DevScalars* sc = nullptr;
     double* dTmp = nullptr;
     size_t dTmpBytes = 0;
+    bool tmpPlanned = false;   // dTmp is the planned scratch buffer (freed with the rest)
     std::vector<void*> allocs;
+    // device memory: separate hipMalloc per buffer, or carved from a region the caller owns
+    char* region = nullptr;
+    size_t regionSize = 0, regionUsed = 0;
+    size_t devBytes = 0;
     // steady-state step pairs replayed from a captured HIP graph (see lbdem_cycle)
     // one captured pair per phase: AA parity x which history buffer is current
     bool useGraphs = true, graphFailed = false;
@@ -75,12 +81,20 @@ struct lbdem_ctx {
     double demTime = 0, realTime = 0, deltat = 1.0, totalMass = 0;
 
     int fail(int code, const std::string& m) { err = m; return code; }
-    template <typename T> T* dalloc(size_t n) {
+    // 256-byte granules; from the region when one is set, else one hipMalloc per buffer
+    void* raw_alloc(size_t bytes) {
+        bytes = (std::max<size_t>(bytes, 1) + 255) & ~size_t(255);
         void* p = nullptr;
-        if (n == 0) n = 1;
-        if (hipMalloc(&p, n * sizeof(T)) != hipSuccess) return nullptr;
-        allocs.push_back(p);
-        return (T*)p;
+        if (region) {
+            if (regionUsed + bytes > regionSize) return nullptr;
+            p = region + regionUsed;
+            regionUsed += bytes;
+        } else {
+            if (hipMalloc(&p, bytes) != hipSuccess) return nullptr;
+            allocs.push_back(p);
+        }
+        devBytes += bytes;
+        return p;
     }
     ~lbdem_ctx() {
         for (int q = 0; q < 4; ++q) {
@@ -89,13 +103,16 @@ struct lbdem_ctx {
         }
         for (hipEvent_t e : evPool) (void)hipEventDestroy(e);
         for (void* p : allocs) (void)hipFree(p);
-        if (dTmp) (void)hipFree(dTmp);
+        if (dTmp && !tmpPlanned) (void)hipFree(dTmp);
         if (stream) (void)hipStreamDestroy(stream);
     }
+    // download scratch: the planned buffer, replaced by a hipMalloc of its own when a download
+    // needs more (field downloads only)
     double* tmp(size_t bytes) {
         if (bytes > dTmpBytes) {
-            if (dTmp) (void)hipFree(dTmp);
+            if (dTmp && !tmpPlanned) (void)hipFree(dTmp);
             dTmp = nullptr;
+            tmpPlanned = false;
             if (hipMalloc(&dTmp, bytes) != hipSuccess) { dTmpBytes = 0; return nullptr; }
             dTmpBytes = bytes;
         }
@@ -248,6 +265,170 @@ static void host_types(lbdem_ctx* c, const Geom& g, std::vector<uint8_t>& t, std
         }
 }
 
+// ---------------------------------------------------------------------------------------------
+// device buffers of a context: one list walked both by lbdem_init (allocating) and by
+// lbdem_estimate_bytes (summing), so the estimate is the bytes init takes
+// ---------------------------------------------------------------------------------------------
+constexpr size_t TMP_PLANNED = 8192;  // scratch of lbdem_total_mass / lbdem_get_info
+
+struct Plan {
+    Geom g;
+    int np = 0, no = 0, nw = 0, ncell = 1;
+    int nl[3] = {0, 0, 0};
+    size_t selt = 8, celt = 8;
+    int ifCap = 0, objCap = 0;
+};
+
+static int wall_count(const lbdem_ctx* c) {
+    int n = 0;
+    for (int i = 0; i < 6; ++i) n += c->prm.boundary[i] >= 5 && c->prm.boundary[i] <= 8;
+    return n;
+}
+
+// DEM linked cells (DEM.cpp:515-554): cell edge 2*rmax+2 over the DEM box; direct sum (no
+// cells) when an axis has fewer than 3 cells
+static void dem_cells(const lbdem_ctx* c, double rmax, int nl[3]) {
+    const double demSize[3] = {c->physSize[0], c->freeSurface ? c->fluidHeight : c->physSize[1], c->physSize[2]};
+    const double cut = 2.0 * rmax + 2.0;
+    for (int a = 0; a < 3; ++a) nl[a] = int(demSize[a] / cut);
+    if (std::min(nl[0], nl[1]) < 3 || nl[2] < 3) nl[0] = nl[1] = nl[2] = 0;
+}
+
+static int make_plan(lbdem_ctx* c, Plan& p) {
+    Geom& g = p.g;
+    g.X = c->lbSize[0]; g.Y = c->lbSize[1]; g.Z = c->lbSize[2];
+    g.N = int64_t(g.X) * g.Y * g.Z;
+    if (g.N >= (int64_t(1) << 31) / Q) return c->fail(LBDEM_E_ARG, "box too large for 32-bit population indexing");
+    for (int i = 0; i < 6; ++i) g.per[i] = c->prm.boundary[i] == 4;
+    p.np = int(c->pr.size());
+    p.no = int(c->orad.size());
+    p.nw = wall_count(c);
+    double rmax = 0.0;
+    for (double r : c->pr) rmax = r > rmax ? r : rmax;
+    dem_cells(c, rmax, p.nl);
+    p.ncell = std::max(1, p.nl[0] * p.nl[1] * p.nl[2]);
+    const int prec = c->prm.precision;
+    p.selt = prec == LBDEM_PREC_FP64 ? 8 : (prec == LBDEM_PREC_FP32 ? 4 : 2);
+    p.celt = prec == LBDEM_PREC_FP64 ? 8 : 4;
+    p.ifCap = int(std::max<int64_t>(65536, g.N / 4));
+    int64_t objCells = 0;
+    for (int o = 0; o < p.no; ++o) {
+        const double r = c->orad[o] / c->unit.Length + 4.0;
+        objCells += int64_t(std::ceil(8.0 * r * r * r));
+    }
+    p.objCap = int(std::min<int64_t>(std::max<int64_t>(objCells, 1024), g.N));
+    return 0;
+}
+
+// calls alloc(void*&, bytes) for every device buffer of the context; returns false when one
+// of them came back null
+template <class T> struct EltSize { static constexpr size_t value = sizeof(T); };
+template <> struct EltSize<void> { static constexpr size_t value = 1; };  // byte buffers
+
+template <class F>
+static bool plan_buffers(const Plan& p, Box& b, DemState& d, DevScalars*& sc, double*& tmp, F&& alloc) {
+    bool ok = true;
+    auto A = [&](auto*& ptr, size_t count) {
+        using T = std::remove_pointer_t<std::remove_reference_t<decltype(ptr)>>;
+        void* v = nullptr;
+        alloc(v, std::max<size_t>(count, 1) * EltSize<T>::value);
+        ptr = static_cast<T*>(v);
+        ok = ok && v != nullptr;
+    };
+    const size_t N = size_t(p.g.N);
+    const size_t np1 = size_t(std::max(p.np, 1)), no1 = size_t(std::max(p.no, 1));
+    b.ifCap = p.ifCap;
+    b.objCap = p.objCap;
+    b.changedCap = b.ifCap + b.objCap;
+    b.blockPartCap = fused_grid_blocks(p.g) * PART_STRIDE_;
+    A(b.f, size_t(Q) * N * p.selt);
+    A(b.type, N);
+    A(b.solid, N);
+    A(b.mass, N * p.celt);
+    A(b.aux, 7 * N * p.celt);
+    A(b.ifList, size_t(b.ifCap));
+    A(b.ifList2, size_t(b.ifCap) + 16);
+    A(b.ifNewMass, size_t(b.ifCap) * p.celt);
+    A(b.filled, size_t(b.ifCap));
+    A(b.emptied, size_t(b.ifCap));
+    A(b.newIf, size_t(b.ifCap));
+    A(b.donor, N);
+    A(b.flagNew, N);
+    A(b.objList, size_t(b.objCap));
+    A(b.objCand, size_t(b.objCap));
+    A(b.coverCount, 256);
+    A(b.changed, size_t(b.changedCap));
+    A(b.blockPart, size_t(b.blockPartCap));
+    A(b.blockFlag, size_t(fused_grid_blocks(p.g)));
+    A(b.bcm, N);
+    A(b.hacc, np1 * 8);
+    A(sc, 1);
+    // DEM
+    const size_t n3 = 3 * np1;
+    double** vecs[] = {&d.x0, &d.x1, &d.w1, &d.FHydro, &d.MHydro, &d.FContact, &d.MContact, &d.FLub, &d.MLub, &d.FRep,
+                       &d.FTotal, &d.MTotal, &d.FGrav, &d.xiw, &d.xio};
+    for (double** v : vecs) A(*v, n3);
+    double** pvecs[] = {&d.r, &d.m, &d.I, &d.im, &d.iI, &d.fluidVol};
+    for (double** v : pvecs) A(*v, np1);
+    const size_t slots = np1 * NBR_CAP;
+    A(d.nbrCount, np1); A(d.nbrCountOld, np1);
+    A(d.nbrList, slots); A(d.nbrListOld, slots); A(d.nbrSlot, slots);
+    A(d.xiCur, 3 * slots); A(d.xiNext, 3 * slots); A(d.xiTmp, 3 * slots);
+    A(d.cellCount, size_t(p.ncell)); A(d.cellStart, size_t(p.ncell)); A(d.cellFill, size_t(p.ncell));
+    A(d.cellItems, np1); A(d.cellKey, np1);
+    A(d.objContrib, np1 * 12 * no1);
+    A(d.wallContrib, np1 * 6 * size_t(std::max(p.nw, 1)));
+    A(d.objSum, 12 * MAX_OBJECTS);
+    A(d.redPart, 2 * 64 * 12 * MAX_OBJECTS);
+    A(d.wallSum, 6 * MAX_WALLS);
+    A(d.blockMax, 64);
+    double** ovecs[] = {&d.ox0, &d.ox1, &d.ow1, &d.oFHydro, &d.oMHydro, &d.oFContact, &d.oMContact, &d.oFLub,
+                        &d.oMLub, &d.oFTotal, &d.oMTotal, &d.oFGrav};
+    for (double** v : ovecs) A(*v, 3 * no1);
+    double** opvecs[] = {&d.or_, &d.om, &d.oI, &d.oim, &d.oiI};
+    for (double** v : opvecs) A(*v, no1);
+    double** wvecs[] = {&d.wFHydro, &d.wFContact, &d.wFLub, &d.wFTotal};
+    for (double** v : wvecs) A(*v, 3 * size_t(MAX_WALLS));
+    A(d.plb, np1);
+    A(d.olb, no1);
+    A(tmp, TMP_PLANNED / sizeof(double));
+    return ok;
+}
+
+static size_t plan_bytes(const Plan& p) {
+    Box b{};
+    DemState d{};
+    DevScalars* sc = nullptr;
+    double* tmp = nullptr;
+    size_t total = 0;
+    plan_buffers(p, b, d, sc, tmp, [&](void*& v, size_t bytes) {
+        total += (std::max<size_t>(bytes, 1) + 255) & ~size_t(255);
+        v = &total;  // any non-null marker: nothing is dereferenced
+    });
+    return total;
+}
+
+extern "C" int lbdem_estimate_bytes(const lbdem_params* params, int nParticles, const double* r, int nObjects,
+                                    const double* objR, size_t* bytes) {
+    if (!params || !bytes || nParticles < 0 || nObjects < 0 || (nParticles > 0 && !r) || (nObjects > 0 && !objR))
+        return LBDEM_E_ARG;
+    lbdem_ctx c;
+    c.prm = *params;
+    if (int e = read_params(&c)) return e;
+    c.pr.assign(r, r + nParticles);
+    c.orad.assign(objR, objR + nObjects);
+    Plan p;
+    if (int e = make_plan(&c, p)) return e;
+    *bytes = plan_bytes(p);
+    return 0;
+}
+
+extern "C" int lbdem_device_bytes(lbdem_ctx* c, size_t* bytes) {
+    if (!c || !bytes) return LBDEM_E_ARG;
+    *bytes = c->devBytes;
+    return 0;
+}
+
 extern "C" int lbdem_init(lbdem_ctx* c) {
     if (!c) return LBDEM_E_ARG;
     if (c->initialised) return c->fail(LBDEM_E_STATE, "already initialised");
@@ -307,20 +488,16 @@ extern "C" int lbdem_init(lbdem_ctx* c) {
     P.rmax = rmax;
     P.objWaitTime = c->prm.objWaitTime;
     for (int a = 0; a < 3; ++a) P.periodic[a] = c->prm.boundary[2 * a] == 4 && c->prm.boundary[2 * a + 1] == 4;
-    const double cut = 2.0 * rmax + 2.0;
-    P.nlx = int(P.demSize[0] / cut); P.nly = int(P.demSize[1] / cut); P.nlz = int(P.demSize[2] / cut);
-    if (std::min(P.nlx, P.nly) < 3 || P.nlz < 3) P.nlx = P.nly = P.nlz = 0;  // direct sum (DEM.cpp:515-554)
+    Plan plan;
+    if (int e = make_plan(c, plan)) return e;
+    P.nlx = plan.nl[0]; P.nly = plan.nl[1]; P.nlz = plan.nl[2];
     P.nPart = np; P.nObj = no; P.nWall = int(walls.size());
     for (size_t w = 0; w < walls.size(); ++w)
         for (int a = 0; a < 3; ++a) { P.wallP[w][a] = walls[w][a]; P.wallN[w][a] = walls[w][3 + a]; }
     P.unitLength = u.Length; P.unitSpeed = u.Speed; P.unitAngVel = u.AngVel;
 
     // ---------------- lattice ----------------
-    Geom g;
-    g.X = c->lbSize[0]; g.Y = c->lbSize[1]; g.Z = c->lbSize[2];
-    g.N = int64_t(g.X) * g.Y * g.Z;
-    if (g.N >= (int64_t(1) << 31) / Q) return c->fail(LBDEM_E_ARG, "box too large for 32-bit population indexing");
-    for (int i = 0; i < 6; ++i) g.per[i] = c->prm.boundary[i] == 4;
+    const Geom g = plan.g;
     std::vector<uint8_t> t;
     std::vector<int32_t> s;
     host_types(c, g, t, s, walls);
@@ -328,43 +505,15 @@ extern "C" int lbdem_init(lbdem_ctx* c) {
     Box& b = c->box;
     b.g = g;
     b.prec = c->prm.precision;
-    const size_t selt = b.prec == LBDEM_PREC_FP64 ? 8 : (b.prec == LBDEM_PREC_FP32 ? 4 : 2);
-    const size_t celt = b.prec == LBDEM_PREC_FP64 ? 8 : 4;
-    b.f = c->dalloc<uint8_t>(size_t(Q) * g.N * selt);
-    b.type = c->dalloc<uint8_t>(g.N);
-    b.solid = c->dalloc<int32_t>(g.N);
-    b.mass = c->dalloc<uint8_t>(g.N * celt);
-    b.aux = c->dalloc<uint8_t>(7 * g.N * celt);
-    b.ifCap = int(std::max<int64_t>(65536, g.N / 4));
-    b.ifList = c->dalloc<int32_t>(b.ifCap);
-    b.ifList2 = c->dalloc<int32_t>(b.ifCap + 16);
-    b.ifNewMass = c->dalloc<uint8_t>(size_t(b.ifCap) * celt);
-    b.filled = c->dalloc<int32_t>(b.ifCap);
-    b.emptied = c->dalloc<int32_t>(b.ifCap);
-    b.newIf = c->dalloc<int32_t>(b.ifCap);
-    b.donor = c->dalloc<int32_t>(g.N);
-    b.flagNew = c->dalloc<int32_t>(g.N);
-    int64_t objCells = 0;
-    for (int o = 0; o < no; ++o) {
-        const double r = c->orad[o] / u.Length + 4.0;
-        objCells += int64_t(std::ceil(8.0 * r * r * r));
-    }
-    b.objCap = int(std::min<int64_t>(std::max<int64_t>(objCells, 1024), g.N));
-    b.objList = c->dalloc<int32_t>(b.objCap);
-    b.objCand = c->dalloc<int32_t>(b.objCap);
-    b.coverCount = c->dalloc<int>(256);
-    b.changedCap = b.ifCap + b.objCap;
-    b.changed = c->dalloc<int32_t>(b.changedCap);
-    b.blockPartCap = fused_grid_blocks(g) * PART_STRIDE_;
-    b.blockPart = c->dalloc<double>(b.blockPartCap);
-    b.blockFlag = c->dalloc<int32_t>(fused_grid_blocks(g));
-    b.bcm = c->dalloc<uint64_t>(g.N);
-    b.hacc = c->dalloc<double>(size_t(std::max(np, 1)) * 8);
+    const size_t selt = plan.selt, celt = plan.celt;
     b.parity = 0;
-    c->sc = c->dalloc<DevScalars>(1);
-    if (!b.f || !b.type || !b.solid || !b.mass || !b.aux || !b.ifList || !b.ifList2 || !b.ifNewMass || !b.filled ||
-        !b.emptied || !b.newIf || !b.donor || !b.flagNew || !b.objList || !b.objCand || !b.coverCount || !b.changed || !b.blockFlag || !b.bcm || !b.blockPart || !b.hacc || !c->sc)
+    DemState& d = c->ds;
+    double* tmpBuf = nullptr;
+    if (!plan_buffers(plan, b, d, c->sc, tmpBuf, [&](void*& v, size_t bytes) { v = c->raw_alloc(bytes); }))
         return c->fail(LBDEM_E_NOMEM, "device allocation failed");
+    c->dTmp = tmpBuf;
+    c->dTmpBytes = TMP_PLANNED;
+    c->tmpPlanned = true;
     hipStream_t st = c->stream;
     HIPCHK(hipMemsetAsync(b.f, 0, size_t(Q) * g.N * selt, st));
     HIPCHK(hipMemsetAsync(b.mass, 0, g.N * celt, st));
@@ -385,43 +534,19 @@ extern "C" int lbdem_init(lbdem_ctx* c) {
     hs.fullMark = 1;
 
     // DEM device state
-    DemState& d = c->ds;
     const size_t n3 = 3 * size_t(std::max(np, 1));
-    auto z3 = [&](double*& p) { p = c->dalloc<double>(n3); return p != nullptr; };
-    double** vecs[] = {&d.x0, &d.x1, &d.w1, &d.FHydro, &d.MHydro, &d.FContact, &d.MContact, &d.FLub, &d.MLub, &d.FRep,
-                       &d.FTotal, &d.MTotal, &d.FGrav, &d.xiw, &d.xio};
-    for (double** v : vecs) if (!z3(*v)) return c->fail(LBDEM_E_NOMEM, "device allocation failed");
-    for (double** v : vecs) HIPCHK(hipMemsetAsync(*v, 0, n3 * 8, st));
-    d.r = c->dalloc<double>(std::max(np, 1)); d.m = c->dalloc<double>(std::max(np, 1));
-    d.I = c->dalloc<double>(std::max(np, 1)); d.im = c->dalloc<double>(std::max(np, 1));
-    d.iI = c->dalloc<double>(std::max(np, 1)); d.fluidVol = c->dalloc<double>(std::max(np, 1));
-    HIPCHK(hipMemsetAsync(d.fluidVol, 0, std::max(np, 1) * 8, st));
     const size_t slots = size_t(std::max(np, 1)) * NBR_CAP;
-    d.nbrCount = c->dalloc<int>(std::max(np, 1)); d.nbrCountOld = c->dalloc<int>(std::max(np, 1));
-    d.nbrList = c->dalloc<int>(slots); d.nbrListOld = c->dalloc<int>(slots); d.nbrSlot = c->dalloc<int>(slots);
-    d.xiCur = c->dalloc<double>(3 * slots); d.xiNext = c->dalloc<double>(3 * slots); d.xiTmp = c->dalloc<double>(3 * slots);
-    const int ncell = std::max(1, P.nlx * P.nly * P.nlz);
-    d.cellCount = c->dalloc<int>(ncell); d.cellStart = c->dalloc<int>(ncell); d.cellFill = c->dalloc<int>(ncell);
-    d.cellItems = c->dalloc<int>(std::max(np, 1)); d.cellKey = c->dalloc<int>(std::max(np, 1));
-    d.objContrib = c->dalloc<double>(size_t(std::max(np, 1)) * 12 * std::max(no, 1));
-    d.wallContrib = c->dalloc<double>(size_t(std::max(np, 1)) * 6 * std::max(int(walls.size()), 1));
-    d.objSum = c->dalloc<double>(12 * MAX_OBJECTS);
-    d.redPart = c->dalloc<double>(2 * 64 * 12 * MAX_OBJECTS);
-    d.wallSum = c->dalloc<double>(6 * MAX_WALLS);
-    d.blockMax = c->dalloc<unsigned long long>(64);
     const size_t o3 = 3 * size_t(std::max(no, 1));
-    double** ovecs[] = {&d.ox0, &d.ox1, &d.ow1, &d.oFHydro, &d.oMHydro, &d.oFContact, &d.oMContact, &d.oFLub,
-                        &d.oMLub, &d.oFTotal, &d.oMTotal, &d.oFGrav};
-    for (double** v : ovecs) { *v = c->dalloc<double>(o3); if (!*v) return c->fail(LBDEM_E_NOMEM, "device allocation failed"); HIPCHK(hipMemsetAsync(*v, 0, o3 * 8, st)); }
-    d.or_ = c->dalloc<double>(std::max(no, 1)); d.om = c->dalloc<double>(std::max(no, 1));
-    d.oI = c->dalloc<double>(std::max(no, 1)); d.oim = c->dalloc<double>(std::max(no, 1)); d.oiI = c->dalloc<double>(std::max(no, 1));
     const size_t w3 = 3 * size_t(MAX_WALLS);
-    double** wvecs[] = {&d.wFHydro, &d.wFContact, &d.wFLub, &d.wFTotal};
-    for (double** v : wvecs) { *v = c->dalloc<double>(w3); if (!*v) return c->fail(LBDEM_E_NOMEM, "device allocation failed"); HIPCHK(hipMemsetAsync(*v, 0, w3 * 8, st)); }
-    d.plb = c->dalloc<PartLB>(std::max(np, 1));
-    d.olb = c->dalloc<ObjLB>(std::max(no, 1));
-    if (!d.nbrCount || !d.nbrList || !d.xiCur || !d.xiTmp || !d.plb || !d.olb || !d.objContrib || !d.wallContrib)
-        return c->fail(LBDEM_E_NOMEM, "device allocation failed");
+    double* vecs[] = {d.x0, d.x1, d.w1, d.FHydro, d.MHydro, d.FContact, d.MContact, d.FLub, d.MLub, d.FRep,
+                      d.FTotal, d.MTotal, d.FGrav, d.xiw, d.xio};
+    for (double* v : vecs) HIPCHK(hipMemsetAsync(v, 0, n3 * 8, st));
+    HIPCHK(hipMemsetAsync(d.fluidVol, 0, std::max(np, 1) * 8, st));
+    double* ovecs[] = {d.ox0, d.ox1, d.ow1, d.oFHydro, d.oMHydro, d.oFContact, d.oMContact, d.oFLub,
+                       d.oMLub, d.oFTotal, d.oMTotal, d.oFGrav};
+    for (double* v : ovecs) HIPCHK(hipMemsetAsync(v, 0, o3 * 8, st));
+    double* wvecs[] = {d.wFHydro, d.wFContact, d.wFLub, d.wFTotal};
+    for (double* v : wvecs) HIPCHK(hipMemsetAsync(v, 0, w3 * 8, st));
     HIPCHK(hipMemsetAsync(d.nbrCount, 0, std::max(np, 1) * 4, st));
     HIPCHK(hipMemsetAsync(d.nbrCountOld, 0, std::max(np, 1) * 4, st));
     HIPCHK(hipMemsetAsync(d.xiCur, 0, 3 * slots * 8, st));
@@ -650,6 +775,23 @@ extern "C" int lbdem_cycle(lbdem_ctx* c, int nsteps) {
     HIPCHK(hipGetLastError());
     return 0;
 }
+
+extern "C" int lbdem_use_graphs(lbdem_ctx* c, int enable) {
+    if (!c) return LBDEM_E_ARG;
+    c->useGraphs = enable != 0;
+    return 0;
+}
+
+// hooks of the run pool (pool.cpp)
+int lbdem::ctx_use_region(lbdem_ctx* c, void* base, size_t bytes) {
+    if (!c || !base) return LBDEM_E_ARG;
+    if (c->initialised || c->devBytes) return c->fail(LBDEM_E_STATE, "the memory region must be set before lbdem_init");
+    c->region = static_cast<char*>(base);
+    c->regionSize = bytes;
+    c->regionUsed = 0;
+    return 0;
+}
+hipStream_t lbdem::ctx_stream(const lbdem_ctx* c) { return c ? c->stream : nullptr; }
 
 extern "C" int lbdem_synchronize(lbdem_ctx* c) {
     NEED_INIT;

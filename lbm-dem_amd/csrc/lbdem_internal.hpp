@@ -259,4 +259,9 @@ void dem_neighbor_list(const DemParams& p, DemState& d, DevScalars* sc, bool for
 void dem_step(const DemParams& p, DemState& d, DevScalars* sc, int demTimeStep, hipStream_t s);
 void dem_refresh_views(const DemParams& p, DemState& d, hipStream_t s);
 
+// run pool hooks: a context carves its buffers from [base, base+bytes) instead of allocating
+// them (set before lbdem_init); the stream a context enqueues on
+int ctx_use_region(lbdem_ctx* c, void* base, size_t bytes);
+hipStream_t ctx_stream(const lbdem_ctx* c);
+
 }  // namespace lbdem

@@ -77,6 +77,43 @@ class Info(C.Structure):
 _dp = C.POINTER(C.c_double)
 _ip = C.POINTER(C.c_int)
 
+RUN_QUEUED, RUN_ACTIVE, RUN_DONE, RUN_FAILED, RUN_CANCELLED = 0, 1, 2, 3, 4
+
+
+class RunDesc(C.Structure):
+    _fields_ = [
+        ("params", Params), ("nParticles", C.c_int), ("size", _ip), ("r", _dp), ("x0", _dp),
+        ("nObjects", C.c_int), ("objR", _dp), ("objX0", _dp), ("objX1", _dp), ("nsteps", C.c_int),
+    ]
+
+
+class RunResult(C.Structure):
+    _fields_ = [
+        ("state", C.c_int), ("error", C.c_int), ("stepsDone", C.c_int), ("nObjects", C.c_int),
+        ("tSubmit", C.c_double), ("tStart", C.c_double), ("tDone", C.c_double),
+        ("objects", C.c_double * (2 * 38)), ("totalMass", C.c_double), ("nActive", C.c_int),
+        ("nodeSteps", C.c_int64), ("deviceBytes", C.c_size_t), ("message", C.c_char * 128),
+    ]
+
+    def as_dict(self) -> Dict:
+        no = self.nObjects
+        return dict(state=self.state, error=self.error, stepsDone=self.stepsDone, nObjects=no,
+                    tSubmit=self.tSubmit, tStart=self.tStart, tDone=self.tDone,
+                    objects=np.array(self.objects[:no * 38], dtype=np.float64).reshape(no, 38),
+                    totalMass=self.totalMass, nActive=self.nActive, nodeSteps=self.nodeSteps,
+                    deviceBytes=self.deviceBytes, message=self.message.decode(errors="replace"))
+
+
+class PoolStats(C.Structure):
+    _fields_ = [
+        ("submitted", C.c_int), ("queued", C.c_int), ("active", C.c_int), ("done", C.c_int),
+        ("failed", C.c_int), ("cancelled", C.c_int), ("activePeak", C.c_int),
+        ("budget", C.c_size_t), ("bytesInUse", C.c_size_t), ("bytesPeak", C.c_size_t),
+    ]
+
+    def as_dict(self) -> Dict:
+        return {name: getattr(self, name) for name, _ in self._fields_}
+
 
 class LbdemError(RuntimeError):
     pass
@@ -116,6 +153,19 @@ class Library:
             "lbdem_total_mass": ([vp, _dp], C.c_int),
             "lbdem_profile_enable": ([vp, C.c_int], C.c_int),
             "lbdem_profile_read": ([vp, _dp, _ip], C.c_int),
+            "lbdem_estimate_bytes": ([C.POINTER(Params), C.c_int, _dp, C.c_int, _dp, C.POINTER(C.c_size_t)], C.c_int),
+            "lbdem_device_bytes": ([vp, C.POINTER(C.c_size_t)], C.c_int),
+            "lbdem_use_graphs": ([vp, C.c_int], C.c_int),
+            "lbdem_pool_create": ([C.c_size_t, C.c_int, C.c_int, C.c_int, C.POINTER(vp)], C.c_int),
+            "lbdem_pool_submit": ([vp, C.POINTER(RunDesc), _ip], C.c_int),
+            "lbdem_pool_poll": ([vp, C.c_int, C.POINTER(RunResult)], C.c_int),
+            "lbdem_pool_wait": ([vp, C.c_int, C.c_double, C.POINTER(RunResult)], C.c_int),
+            "lbdem_pool_drain": ([vp, C.c_double], C.c_int),
+            "lbdem_pool_cancel": ([vp, C.c_int], C.c_int),
+            "lbdem_pool_get_stats": ([vp, C.POINTER(PoolStats)], C.c_int),
+            "lbdem_pool_clock": ([vp], C.c_double),
+            "lbdem_pool_last_error": ([vp], C.c_char_p),
+            "lbdem_pool_destroy": ([vp], C.c_int),
         }
         for name, (args, res) in sig.items():
             fn = getattr(lib, name)
@@ -256,6 +306,99 @@ class Simulation:
         v = C.c_double()
         self._check(self.lib.lib.lbdem_total_mass(self.ctx, C.byref(v)), "lbdem_total_mass")
         return v.value
+
+    def use_graphs(self, enable: bool):
+        self._check(self.lib.lib.lbdem_use_graphs(self.ctx, 1 if enable else 0), "lbdem_use_graphs")
+
+    def device_bytes(self) -> int:
+        v = C.c_size_t()
+        self._check(self.lib.lib.lbdem_device_bytes(self.ctx, C.byref(v)), "lbdem_device_bytes")
+        return v.value
+
+
+def estimate_bytes(lib: Library, params: Params, particles: "ParticleSet", objects: "ObjectSet") -> int:
+    """Device bytes lbdem_init will take for this configuration (host-side computation)."""
+    r = np.ascontiguousarray(particles.r, dtype=np.float64)
+    orr = np.ascontiguousarray(objects.r, dtype=np.float64)
+    v = C.c_size_t()
+    rc = lib.lib.lbdem_estimate_bytes(C.byref(params), len(r), _ptr(r, C.c_double), len(orr), _ptr(orr, C.c_double),
+                                      C.byref(v))
+    if rc != 0:
+        raise LbdemError(f"lbdem_estimate_bytes failed with code {rc}")
+    return v.value
+
+
+class Pool:
+    """Run pool of the device library: independent runs admitted against an HBM budget, each on
+    its own stream, stepped concurrently; results are the impactor records, total mass and
+    active-node count after the last step."""
+
+    def __init__(self, lib: Library, budget_bytes: int = 0, max_concurrent: int = 64, steps_per_chunk: int = 32,
+                 device: int = 0):
+        self.lib = lib
+        self.pool = C.c_void_p()
+        rc = lib.lib.lbdem_pool_create(int(budget_bytes), int(max_concurrent), int(steps_per_chunk), int(device),
+                                       C.byref(self.pool))
+        if rc != 0:
+            raise LbdemError(f"lbdem_pool_create failed with code {rc}")
+
+    def _check(self, rc: int, what: str):
+        if rc != 0:
+            msg = self.lib.lib.lbdem_pool_last_error(self.pool) if self.pool else b""
+            raise LbdemError(f"{what} failed with code {rc}: {msg.decode(errors='replace') if msg else ''}")
+
+    def submit(self, params: Params, particles: "ParticleSet", objects: "ObjectSet", nsteps: int) -> int:
+        size = np.ascontiguousarray(particles.size, dtype=np.int32)
+        r = np.ascontiguousarray(particles.r, dtype=np.float64)
+        x0 = np.ascontiguousarray(particles.x0, dtype=np.float64).reshape(-1)
+        orr = np.ascontiguousarray(objects.r, dtype=np.float64)
+        ox0 = np.ascontiguousarray(objects.x0, dtype=np.float64).reshape(-1)
+        ox1 = np.ascontiguousarray(objects.x1, dtype=np.float64).reshape(-1)
+        d = RunDesc()
+        d.params = params
+        d.nParticles = len(r)
+        d.size, d.r, d.x0 = _ptr(size, C.c_int), _ptr(r, C.c_double), _ptr(x0, C.c_double)
+        d.nObjects = len(orr)
+        d.objR, d.objX0, d.objX1 = _ptr(orr, C.c_double), _ptr(ox0, C.c_double), _ptr(ox1, C.c_double)
+        d.nsteps = int(nsteps)
+        rid = C.c_int()
+        self._check(self.lib.lib.lbdem_pool_submit(self.pool, C.byref(d), C.byref(rid)), "lbdem_pool_submit")
+        return rid.value
+
+    def poll(self, rid: int) -> Dict:
+        res = RunResult()
+        self._check(self.lib.lib.lbdem_pool_poll(self.pool, rid, C.byref(res)), "lbdem_pool_poll")
+        return res.as_dict()
+
+    def wait(self, rid: int, timeout: float = 600.0) -> Dict:
+        res = RunResult()
+        self._check(self.lib.lib.lbdem_pool_wait(self.pool, rid, float(timeout), C.byref(res)), "lbdem_pool_wait")
+        return res.as_dict()
+
+    def drain(self, timeout: float = 600.0):
+        self._check(self.lib.lib.lbdem_pool_drain(self.pool, float(timeout)), "lbdem_pool_drain")
+
+    def cancel(self, rid: int):
+        self._check(self.lib.lib.lbdem_pool_cancel(self.pool, rid), "lbdem_pool_cancel")
+
+    def stats(self) -> Dict:
+        st = PoolStats()
+        self._check(self.lib.lib.lbdem_pool_get_stats(self.pool, C.byref(st)), "lbdem_pool_get_stats")
+        return st.as_dict()
+
+    def clock(self) -> float:
+        return self.lib.lib.lbdem_pool_clock(self.pool)
+
+    def close(self):
+        if self.pool:
+            self.lib.lib.lbdem_pool_destroy(self.pool)
+            self.pool = C.c_void_p()
+
+    def __del__(self):
+        try:
+            self.close()
+        except Exception:
+            pass
 
 
 # ----------------------------------------------------------------------------------------------

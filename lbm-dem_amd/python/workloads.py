@@ -20,6 +20,7 @@ from __future__ import annotations
 import ctypes as C
 import math
 import os
+from dataclasses import dataclass
 from typing import Dict, Tuple
 
 import numpy as np
@@ -42,7 +43,7 @@ def pack(n: int, r: float, lo, hi, seed: int, relax: int = 500) -> np.ndarray:
 
 
 def impact_box(size=(256, 320, 256), fluid_height=256.0, n_spheres=31000, r=4.0, r_imp=16.0, y_imp=290.0,
-               speed=0.01, seed=20261202, relax=500, periodic=False
+               speed=0.01, seed=20261202, relax=500, periodic=False, bed_top=None
                ) -> Tuple[Dict[str, object], lbdem.ParticleSet, lbdem.ObjectSet]:
     X, Y, Z = size
     side = 4 if periodic else 7
@@ -60,7 +61,7 @@ def impact_box(size=(256, 320, 256), fluid_height=256.0, n_spheres=31000, r=4.0,
     ps = lbdem.ParticleSet()
     if n_spheres > 0:
         lo = (1.0, 1.0, 1.0)
-        hi = (X - 1.0, fluid_height, Z - 1.0)
+        hi = (X - 1.0, fluid_height if bed_top is None else bed_top, Z - 1.0)
         ps.x0 = pack(n_spheres, r, lo, hi, seed, relax)
         ps.r = np.full(n_spheres, r)
         ps.size = np.ones(n_spheres, np.int32)
@@ -79,3 +80,74 @@ def volume_fraction(ps: lbdem.ParticleSet, size, fluid_height) -> float:
 # height 290, v = -0.01.
 HEADLINE = dict(size=(256, 320, 256), fluid_height=256.0, n_spheres=31000, r=4.0, r_imp=16.0, y_imp=290.0,
                 speed=0.01, seed=20261202, periodic=True)
+
+
+# ----------------------------------------------------------------------------------------------
+# Many independent impact runs (batched and served workloads)
+# ----------------------------------------------------------------------------------------------
+BATCH_EDGES = (48, 64, 96, 128)
+SERVE_EDGES = (64, 96, 128, 192)
+
+
+@dataclass
+class ImpactRun:
+    """One impact run of a cubic box: inputs in the reference's terms plus its draws."""
+    values: Dict[str, object]
+    particles: lbdem.ParticleSet
+    objects: lbdem.ObjectSet
+    nsteps: int
+    edge: int
+    phi: float
+    r: float
+    speed: float
+
+    @property
+    def nodes(self) -> int:
+        return self.edge ** 3
+
+
+def random_impact_run(rng: np.random.Generator, edges, nsteps: int, relax: int = 200) -> ImpactRun:
+    """A cubic box of edge e drawn from ``edges``: x and z periodic, floor and lid walls, fluid to
+    0.75 e with a free surface; an impactor of radius 0.2 e whose top sits 3 lu below the lid
+    (it starts partly immersed: a box this short has no room above the surface) moving down with
+    speed ~ U[0.002, 0.02]; below it a bed of monodisperse spheres of radius ~ U[3, 4] at solid
+    fraction phi ~ U[0.30, 0.55] of the bed region (x, z in [1, e-1], y from 1 to 1 lu under the
+    impactor; in small boxes the packing may hold fewer spheres: ``phi`` records what was placed)."""
+    e = int(rng.choice(np.asarray(edges)))
+    phi = float(rng.uniform(0.30, 0.55))
+    r = float(rng.uniform(3.0, 4.0))
+    speed = float(rng.uniform(0.002, 0.02))
+    fill = 0.75 * e
+    r_imp = 0.2 * e
+    y_imp = e - r_imp - 3.0
+    bed_top = min(fill, y_imp - r_imp - 1.0)
+    vol = (e - 2.0) * (bed_top - 1.0) * (e - 2.0)
+    n = int(round(phi * vol / (4.0 / 3.0 * math.pi * r ** 3)))
+    seed = int(rng.integers(1, 2 ** 62))
+    while True:
+        try:
+            v, ps, obs = impact_box(size=(e, e, e), fluid_height=fill, n_spheres=n, r=r, r_imp=r_imp, y_imp=y_imp,
+                                    speed=speed, seed=seed, relax=relax, periodic=True, bed_top=bed_top)
+            break
+        except RuntimeError:
+            # a thin bed holds fewer spheres than phi asks at this radius: 5% fewer, same seed
+            n = int(0.95 * n)
+    v["maxTime"] = float(nsteps - 1)
+    phi = float(len(ps.r) * 4.0 / 3.0 * math.pi * r ** 3 / vol)
+    return ImpactRun(v, ps, obs, int(nsteps), e, phi, r, speed)
+
+
+def batched_runs(n: int = 64, nsteps: int = 2000, seed: int = 20261203, edges=BATCH_EDGES):
+    """The batched workload: ``n`` boxes with edges from {48, 64, 96, 128}, ``nsteps`` steps each."""
+    rng = np.random.default_rng(seed)
+    return [random_impact_run(rng, edges, nsteps) for _ in range(n)]
+
+
+def served_requests(n: int, rate_per_s: float, seed: int = 20261204, edges=SERVE_EDGES, steps=(1000, 8000)):
+    """The served workload: ``n`` requests with Poisson arrivals at ``rate_per_s`` (exponential
+    gaps), edges from {64, 96, 128, 192} and step counts ~ U[steps]; returns (arrival times, runs)."""
+    rng = np.random.default_rng(seed)
+    gaps = rng.exponential(1.0 / rate_per_s, size=n)
+    arrivals = np.cumsum(gaps) - gaps[0]
+    runs = [random_impact_run(rng, edges, int(rng.integers(steps[0], steps[1] + 1))) for _ in range(n)]
+    return arrivals, runs

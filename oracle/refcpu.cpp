@@ -1434,6 +1434,32 @@ int lbdem_profile_read(lbdem_ctx* c, double* fused_ms, int* n) {
     if (n) *n = 0;
     return 0;
 }
+// device memory, graph replay and the run pool belong to the device library: the oracle holds
+// no device memory, steps one by one and has no pool
+int lbdem_estimate_bytes(const lbdem_params*, int, const double*, int, const double*, size_t* bytes) {
+    if (bytes) *bytes = 0;
+    return LBDEM_E_STATE;
+}
+int lbdem_device_bytes(lbdem_ctx* c, size_t* bytes) {
+    if (!c || !bytes) return LBDEM_E_ARG;
+    *bytes = 0;
+    return 0;
+}
+int lbdem_use_graphs(lbdem_ctx* c, int) { return c ? 0 : LBDEM_E_ARG; }
+int lbdem_pool_create(size_t, int, int, int, lbdem_pool** out) {
+    if (out) *out = nullptr;
+    return LBDEM_E_STATE;
+}
+int lbdem_pool_submit(lbdem_pool*, const lbdem_run_desc*, int*) { return LBDEM_E_STATE; }
+int lbdem_pool_poll(lbdem_pool*, int, lbdem_run_result*) { return LBDEM_E_STATE; }
+int lbdem_pool_wait(lbdem_pool*, int, double, lbdem_run_result*) { return LBDEM_E_STATE; }
+int lbdem_pool_drain(lbdem_pool*, double) { return LBDEM_E_STATE; }
+int lbdem_pool_cancel(lbdem_pool*, int) { return LBDEM_E_STATE; }
+int lbdem_pool_get_stats(lbdem_pool*, lbdem_pool_stats*) { return LBDEM_E_STATE; }
+double lbdem_pool_clock(const lbdem_pool*) { return 0.0; }
+const char* lbdem_pool_last_error(const lbdem_pool*) { return "the CPU oracle has no run pool"; }
+int lbdem_pool_destroy(lbdem_pool*) { return 0; }
+
 int lbdem_total_mass(lbdem_ctx* c, double* m) {
     NEED_INIT;
     double s = 0.0;

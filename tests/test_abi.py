@@ -79,3 +79,40 @@ def test_particle_and_object_readers(tmp_path):
     lbdem.write_objects(str(tmp_path / "o.dat"), [3.0], np.array([[1.0, 2.0, 3.0]]), np.array([[0.0, -0.5, 0.0]]))
     ob = lbdem.read_objects(str(tmp_path / "o.dat"))
     assert np.allclose(ob.x0, [[1.0, 2.0, 3.0]]) and np.allclose(ob.x1, [[0.0, -0.5, 0.0]])
+
+
+def _box_params(n, prec):
+    p = lbdem.Params()
+    p.unitLength = p.unitTime = p.unitDensity = 1.0
+    p.lbSize[:] = [float(n), float(n), float(n)]
+    p.initVisc = 0.1
+    p.boundary[:] = [4, 4, 7, 7, 4, 4]
+    p.lbSolve = p.demSolve = p.freeSurface = p.forceField = 1
+    p.fluidHeight = 0.75 * n
+    p.density = p.objectDensity = 1.0
+    p.precision = prec
+    return p
+
+
+def test_device_byte_estimate_is_host_side():
+    """lbdem_estimate_bytes needs no device: per-node bytes follow the storage precision
+    (19 populations of 8/4/2 bytes plus the per-node fields) and the estimate grows with the
+    particle count; bad parameters are rejected like lbdem_create rejects them."""
+    if not os.path.exists(GPU_LIB):
+        pytest.skip("GPU library not built in this checkout")
+    lib = lbdem.Library.load(GPU_LIB)
+    ps = lbdem.ParticleSet(size=np.ones(100, np.int32), r=np.full(100, 3.0), x0=np.zeros((100, 3)))
+    obs = lbdem.ObjectSet(r=np.array([8.0]), x0=np.zeros((1, 3)), x1=np.zeros((1, 3)))
+    est = {prec: lbdem.estimate_bytes(lib, _box_params(96, prec), ps, obs)
+           for prec in (lbdem.PREC_FP64, lbdem.PREC_FP32, lbdem.PREC_FP16)}
+    N = 96 ** 3
+    assert est[lbdem.PREC_FP64] > est[lbdem.PREC_FP32] > est[lbdem.PREC_FP16]
+    # populations: 19 x (8 - 4) bytes per node, plus mass and 7 auxiliary fields (8 -> 4 bytes)
+    assert est[lbdem.PREC_FP64] - est[lbdem.PREC_FP32] == pytest.approx(N * (19 * 4 + 8 * 4), rel=0.01)
+    assert est[lbdem.PREC_FP32] - est[lbdem.PREC_FP16] == pytest.approx(N * 19 * 2, rel=0.01)
+    more = lbdem.ParticleSet(size=np.ones(1000, np.int32), r=np.full(1000, 3.0), x0=np.zeros((1000, 3)))
+    assert lbdem.estimate_bytes(lib, _box_params(96, lbdem.PREC_FP32), more, obs) > est[lbdem.PREC_FP32]
+    bad = _box_params(96, lbdem.PREC_FP32)
+    bad.boundary[0] = 5
+    with pytest.raises(lbdem.LbdemError):
+        lbdem.estimate_bytes(lib, bad, ps, obs)

@@ -1,0 +1,155 @@
+"""Run pool of the device library: independent impact runs admitted against an HBM budget and
+stepped concurrently, each on its own stream.
+
+Identity: every run's result in the mix equals the same run alone (its own context, lbdem_cycle
+over all steps). The kernels add FP64 partial forces and the free-surface mass surplus with
+atomics, whose order can differ from one execution to the next, so "equal" is to round-off
+growth over the run: impactor records within 1e-9 relative (of each record's largest entry),
+total fluid mass within 1e-12 relative, active node count exact."""
+import numpy as np
+import pytest
+
+import lbdem
+import workloads
+
+pytestmark = pytest.mark.gpu
+
+PRECS = (lbdem.PREC_FP32, lbdem.PREC_FP64, lbdem.PREC_FP16, lbdem.PREC_FP32, lbdem.PREC_FP32, lbdem.PREC_FP64)
+STEPS = (80, 61, 40, 120, 96, 50)
+
+
+@pytest.fixture(scope="module")
+def runs():
+    rng = np.random.default_rng(7)
+    out = []
+    for prec, n in zip(PRECS, STEPS):
+        run = workloads.random_impact_run(rng, (32, 40, 48), n, relax=100)
+        out.append((run, prec))
+    return out
+
+
+def alone(lib, run, prec, graphs=True):
+    prm = lbdem.params_from_values(run.values, prec)
+    sim = lbdem.Simulation(lib, prm, run.particles, run.objects)
+    sim.init()
+    sim.use_graphs(graphs)
+    sim.cycle(run.nsteps)
+    sim.synchronize()
+    res = dict(objects=sim.objects(), totalMass=sim.total_mass(), nActive=sim.info()["nActive"],
+               bytes=sim.device_bytes())
+    sim.close()
+    return res
+
+
+@pytest.fixture(scope="module")
+def alone_results(gpu_lib, runs):
+    return [alone(gpu_lib, run, prec) for run, prec in runs]
+
+
+def estimate(lib, run, prec):
+    return lbdem.estimate_bytes(lib, lbdem.params_from_values(run.values, prec), run.particles, run.objects)
+
+
+def same(a, b, where):
+    obj_a, obj_b = np.asarray(a["objects"]), np.asarray(b["objects"])
+    assert obj_a.shape == obj_b.shape, where
+    for k in range(obj_a.shape[0]):
+        scale = max(np.abs(obj_a[k]).max(), 1e-300)
+        d = np.abs(obj_a[k] - obj_b[k]).max() / scale
+        print(f"{where}: impactor record rel diff {d:.3e}", flush=True)
+        assert d <= 1e-9, f"{where}: impactor record differs by {d}"
+    dm = abs(a["totalMass"] - b["totalMass"]) / abs(b["totalMass"])
+    assert dm <= 1e-12, f"{where}: total mass differs by {dm}"
+    assert a["nActive"] == b["nActive"], where
+
+
+def submit_all(pool, runs, ids=None):
+    out = []
+    for i, (run, prec) in enumerate(runs):
+        if ids is not None and i not in ids:
+            continue
+        out.append(pool.submit(lbdem.params_from_values(run.values, prec), run.particles, run.objects, run.nsteps))
+    return out
+
+
+def test_estimate_equals_allocation(gpu_lib, runs, alone_results):
+    """lbdem_estimate_bytes (host-side) is exactly what lbdem_init allocates."""
+    for (run, prec), res in zip(runs, alone_results):
+        assert estimate(gpu_lib, run, prec) == res["bytes"], (run.edge, prec)
+
+
+def test_graphs_do_not_change_results(gpu_lib, runs, alone_results):
+    run, prec = runs[0]
+    same(alone(gpu_lib, run, prec, graphs=False), alone_results[0], "eager vs graph replay")
+
+
+def test_pool_results_equal_alone(gpu_lib, runs, alone_results):
+    budget = int(1.05 * sum(estimate(gpu_lib, r, p) for r, p in runs)) + (1 << 20) * len(runs)
+    pool = lbdem.Pool(gpu_lib, budget_bytes=budget, max_concurrent=8, steps_per_chunk=16)
+    ids = submit_all(pool, runs)
+    pool.drain(300.0)
+    st = pool.stats()
+    print("pool stats", st, flush=True)
+    for i, rid in enumerate(ids):
+        res = pool.poll(rid)
+        assert res["state"] == lbdem.RUN_DONE, res["message"]
+        assert res["stepsDone"] == runs[i][0].nsteps
+        assert res["nodeSteps"] == runs[i][0].nodes * runs[i][0].nsteps
+        assert res["tSubmit"] <= res["tStart"] <= res["tDone"]
+        same(res, alone_results[i], f"run {i} in the mix")
+    assert st["done"] == len(runs) and st["failed"] == 0
+    assert st["activePeak"] >= 2, "runs did not overlap"
+    assert st["bytesPeak"] <= budget and st["bytesInUse"] == 0
+    pool.close()
+
+
+def test_pool_admits_by_budget(gpu_lib, runs, alone_results):
+    """A budget that holds the largest run but not two of them: runs are admitted one or two at
+    a time, in arrival order, and every result is still the run's own."""
+    sizes = [estimate(gpu_lib, r, p) for r, p in runs]
+    budget = int(1.5 * max(sizes))
+    pool = lbdem.Pool(gpu_lib, budget_bytes=budget, max_concurrent=16, steps_per_chunk=32)
+    ids = submit_all(pool, runs)
+    pool.drain(300.0)
+    st = pool.stats()
+    print("pool stats", st, flush=True)
+    assert st["bytesPeak"] <= budget
+    starts = [pool.poll(rid)["tStart"] for rid in ids]
+    assert starts == sorted(starts), "admission is in arrival order"
+    for i, rid in enumerate(ids):
+        res = pool.poll(rid)
+        assert res["state"] == lbdem.RUN_DONE, res["message"]
+        same(res, alone_results[i], f"run {i} under a tight budget")
+    pool.close()
+
+
+def test_pool_cancel_leaves_other_runs_unchanged(gpu_lib, runs, alone_results):
+    pool = lbdem.Pool(gpu_lib, budget_bytes=int(1.2 * sum(estimate(gpu_lib, r, p) for r, p in runs[:3])),
+                      max_concurrent=4, steps_per_chunk=8)
+    ids = submit_all(pool, runs, ids={0, 1, 2})
+    pool.cancel(ids[1])
+    pool.drain(300.0)
+    assert pool.poll(ids[1])["state"] == lbdem.RUN_CANCELLED
+    for k in (0, 2):
+        res = pool.poll(ids[k])
+        assert res["state"] == lbdem.RUN_DONE, res["message"]
+        same(res, alone_results[k], f"run {k} beside a cancelled run")
+    st = pool.stats()
+    assert st["cancelled"] == 1 and st["bytesInUse"] == 0
+    pool.close()
+
+
+def test_pool_rejects_a_run_above_the_budget(gpu_lib, runs):
+    run, prec = runs[0]
+    pool = lbdem.Pool(gpu_lib, budget_bytes=estimate(gpu_lib, run, prec) // 2, max_concurrent=2)
+    with pytest.raises(lbdem.LbdemError, match="budget"):
+        pool.submit(lbdem.params_from_values(run.values, prec), run.particles, run.objects, run.nsteps)
+    pool.close()
+
+
+def test_pool_destroy_with_work_outstanding(gpu_lib, runs):
+    """Destroying a pool cancels queued and active runs and returns after the device is done."""
+    pool = lbdem.Pool(gpu_lib, budget_bytes=int(1.2 * sum(estimate(gpu_lib, r, p) for r, p in runs)),
+                      max_concurrent=2, steps_per_chunk=8)
+    submit_all(pool, runs)
+    pool.close()
