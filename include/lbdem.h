@@ -161,6 +161,21 @@ int lbdem_device_bytes(lbdem_ctx* ctx, size_t* bytes);
  * one by one. Results are the same either way. */
 int lbdem_use_graphs(lbdem_ctx* ctx, int enable);
 
+/* Tile tuner of the fused collide-stream kernel (no reference counterpart). Tiles are 64 x 4
+ * threads; each thread walks tile[2] nodes along z. Unless fixed, the first LB steps time the
+ * candidates 8, 4, 2, 1 (one launch per AA parity each) on the run's own state and keep the
+ * fastest; results do not depend on the tile beyond FP64 summation order of block partials.
+ * lbdem_set_tile: 0 tunes (default), 1/2/4/8 fixes the z-walk length. */
+typedef struct lbdem_tuning {
+    int tile[3];
+    int tuned;
+    int nCandidates;
+    int candidateZ[4];
+    double candidateMs[4];       /* fused kernel time, both parities, per candidate */
+} lbdem_tuning;
+int lbdem_set_tile(lbdem_ctx* ctx, int tileZ);
+int lbdem_get_tuning(lbdem_ctx* ctx, lbdem_tuning* tuning);
+
 /* ---------------------------------------------------------------------------------------------
  * Run pool (no reference counterpart; batched and served runs of many independent boxes on one
  * GPU). A worker thread admits submitted runs in arrival order while their estimated device bytes

@@ -72,6 +72,14 @@ struct lbdem_ctx {
     hipGraph_t graph[4] = {nullptr, nullptr, nullptr, nullptr};
     hipGraphExec_t graphExec[4] = {nullptr, nullptr, nullptr, nullptr};
     double* xiFirst = nullptr;  // the buffer that is current at initialisation
+    // tile tuner: the first LB steps run the fused kernel once per AA parity with each candidate
+    // z-walk length, timed with events; the fastest candidate is kept (lbdem_set_tile fixes one)
+    static constexpr int NCAND = 4;
+    static constexpr int CAND[NCAND] = {8, 4, 2, 1};
+    bool tuned = false;
+    int tuneLaunches = 0;
+    hipEvent_t tuneEv[2 * NCAND][2] = {};
+    double tuneMs[NCAND] = {0, 0, 0, 0};
     // kernel timing
     bool profiling = false;
     std::vector<hipEvent_t> evPool;
@@ -102,6 +110,9 @@ struct lbdem_ctx {
             if (graph[q]) (void)hipGraphDestroy(graph[q]);
         }
         for (hipEvent_t e : evPool) (void)hipEventDestroy(e);
+        for (auto& pr : tuneEv)
+            for (hipEvent_t e : pr)
+                if (e) (void)hipEventDestroy(e);
         for (void* p : allocs) (void)hipFree(p);
         if (dTmp && !tmpPlanned) (void)hipFree(dTmp);
         if (stream) (void)hipStreamDestroy(stream);
@@ -340,7 +351,7 @@ static bool plan_buffers(const Plan& p, Box& b, DemState& d, DevScalars*& sc, do
     b.ifCap = p.ifCap;
     b.objCap = p.objCap;
     b.changedCap = b.ifCap + b.objCap;
-    b.blockPartCap = fused_grid_blocks(p.g) * PART_STRIDE_;
+    b.blockPartCap = fused_grid_blocks(p.g, 1) * PART_STRIDE_;  // the largest grid (z-walk 1)
     A(b.f, size_t(Q) * N * p.selt);
     A(b.type, N);
     A(b.solid, N);
@@ -356,10 +367,10 @@ static bool plan_buffers(const Plan& p, Box& b, DemState& d, DevScalars*& sc, do
     A(b.flagNew, N);
     A(b.objList, size_t(b.objCap));
     A(b.objCand, size_t(b.objCap));
-    A(b.coverCount, 256);
+    A(b.coverCount, 256 + 1024);
     A(b.changed, size_t(b.changedCap));
     A(b.blockPart, size_t(b.blockPartCap));
-    A(b.blockFlag, size_t(fused_grid_blocks(p.g)));
+    A(b.blockFlag, size_t(fused_grid_blocks(p.g, 1)));
     A(b.bcm, N);
     A(b.hacc, np1 * 8);
     A(sc, 1);
@@ -435,6 +446,9 @@ extern "C" int lbdem_init(lbdem_ctx* c) {
     HIPCHK(hipSetDevice(c->prm.device));
     HIPCHK(hipStreamCreateWithFlags(&c->stream, hipStreamNonBlocking));
     if (const char* g = std::getenv("LBDEM_GRAPHS")) c->useGraphs = std::atoi(g) != 0;
+    if (!c->tuned)
+        for (auto& pr : c->tuneEv)
+            for (hipEvent_t& e : pr) HIPCHK(hipEventCreate(&e));
     const Units& u = c->unit;
     const int np = int(c->pr.size()), no = int(c->orad.size());
     if (np > (1 << 30)) return c->fail(LBDEM_E_ARG, "too many particles");
@@ -632,6 +646,26 @@ static void enqueue_coupling(lbdem_ctx* c) {  // LB::latticeBoltzmannCouplingSte
     lb_flags_rebuild_apply(c->box, c->sc, c->stream);
     lb_find_new_active(c->box, c->k, c->ds.plb, c->sc, c->stream);
 }
+// picks the fastest candidate from the timed launches (waits for the last of them)
+static void finish_tuning(lbdem_ctx* c) {
+    int best = 0;
+    double bestMs = 1e300;
+    bool ok = hipEventSynchronize(c->tuneEv[2 * lbdem_ctx::NCAND - 1][1]) == hipSuccess;
+    for (int q = 0; ok && q < lbdem_ctx::NCAND; ++q) {
+        double ms = 0.0;
+        for (int p = 0; p < 2; ++p) {
+            float t = 0.0f;
+            ok = ok && hipEventElapsedTime(&t, c->tuneEv[2 * q + p][0], c->tuneEv[2 * q + p][1]) == hipSuccess;
+            ms += t;
+        }
+        c->tuneMs[q] = ms;
+        if (ms < bestMs) { bestMs = ms; best = q; }
+    }
+    if (!ok) (void)hipGetLastError();
+    c->box.fz = lbdem_ctx::CAND[ok ? best : 0];
+    c->tuned = true;
+}
+
 static void enqueue_lb(lbdem_ctx* c) {  // LB::latticeBolzmannStep (LB.cpp:400-418)
     if (!c->forceField && !c->lbFReset) {  // LB.cpp:1303-1305
         c->k.lbF[0] = c->k.lbF[1] = c->k.lbF[2] = 0.0;
@@ -645,6 +679,16 @@ static void enqueue_lb(lbdem_ctx* c) {  // LB::latticeBolzmannStep (LB.cpp:400-4
             c->evPool.push_back(e);
         }
         if (c->evPool.size() >= c->evUsed + 2) { e0 = c->evPool[c->evUsed]; e1 = c->evPool[c->evUsed + 1]; c->evUsed += 2; }
+    }
+    if (!c->tuned && !c->profiling) {
+        if (c->tuneLaunches < 2 * lbdem_ctx::NCAND) {  // next candidate launch, timed
+            c->box.fz = lbdem_ctx::CAND[c->tuneLaunches / 2];
+            e0 = c->tuneEv[c->tuneLaunches][0];
+            e1 = c->tuneEv[c->tuneLaunches][1];
+            ++c->tuneLaunches;
+        } else {
+            finish_tuning(c);
+        }
     }
     lb_mark_edges(c->box, c->sc, c->stream);
     lb_fused_step(c->box, c->k, c->ds.plb, c->ds.olb, c->sc, true, c->stream, e0, e1);
@@ -690,7 +734,7 @@ static void enqueue_step(lbdem_ctx* c) {
 // the AA parity and the tangential-history buffers alternate with period two. Every
 // data-dependent branch is a device flag, so the pair can be captured once and replayed.
 static bool graph_ready(const lbdem_ctx* c) {
-    if (!c->useGraphs || c->graphFailed || c->profiling) return false;
+    if (!c->useGraphs || c->graphFailed || c->profiling || !c->tuned) return false;
     if (c->prm.demSolve && c->demTimeStep < unsigned(std::max(c->prm.objWaitTime, 0)) + 1u) return false;
     if (!(c->demTime >= c->prm.demInitialRepeat)) return false;
     if (!c->forceField && !c->lbFReset) return false;
@@ -773,6 +817,50 @@ extern "C" int lbdem_cycle(lbdem_ctx* c, int nsteps) {
         ++s;
     }
     HIPCHK(hipGetLastError());
+    return 0;
+}
+
+static void drop_graphs(lbdem_ctx* c) {
+    for (int q = 0; q < 4; ++q) {
+        if (c->graphExec[q]) (void)hipGraphExecDestroy(c->graphExec[q]);
+        if (c->graph[q]) (void)hipGraphDestroy(c->graph[q]);
+        c->graphExec[q] = nullptr;
+        c->graph[q] = nullptr;
+    }
+}
+
+extern "C" int lbdem_set_tile(lbdem_ctx* c, int tileZ) {
+    if (!c) return LBDEM_E_ARG;
+    if (tileZ != 0 && tileZ != 1 && tileZ != 2 && tileZ != 4 && tileZ != 8)
+        return c->fail(LBDEM_E_ARG, "tile z-walk length must be 0 (tune), 1, 2, 4 or 8");
+    if (c->stream) HIPCHK(hipStreamSynchronize(c->stream));
+    drop_graphs(c);  // captured steps hold the previous tile
+    if (tileZ == 0) {
+        if (c->initialised && c->tuned)
+            for (auto& pr : c->tuneEv)
+                for (hipEvent_t& e : pr)
+                    if (!e) HIPCHK(hipEventCreate(&e));
+        c->tuned = false;
+        c->tuneLaunches = 0;
+    } else {
+        c->box.fz = tileZ;
+        c->tuned = true;
+    }
+    return 0;
+}
+
+extern "C" int lbdem_get_tuning(lbdem_ctx* c, lbdem_tuning* t) {
+    if (!c || !t) return LBDEM_E_ARG;
+    std::memset(t, 0, sizeof(*t));
+    t->tile[0] = FUSED_TILE_X;
+    t->tile[1] = FUSED_TILE_Y;
+    t->tile[2] = c->box.fz;
+    t->tuned = c->tuned ? 1 : 0;
+    t->nCandidates = lbdem_ctx::NCAND;
+    for (int q = 0; q < lbdem_ctx::NCAND; ++q) {
+        t->candidateZ[q] = lbdem_ctx::CAND[q];
+        t->candidateMs[q] = c->tuneMs[q];
+    }
     return 0;
 }
 

@@ -132,10 +132,14 @@ void lb_init_fields(const Box& b, const Consts& k, hipStream_t s) {
 // ------------------------------------------------------------------------------------------
 // fused collide + stream
 // ------------------------------------------------------------------------------------------
-constexpr int FX = 64, FY = 4, FZ = 8;  // tile: 64x4 threads, each walks FZ cells along z
+constexpr int FX = FUSED_TILE_X, FY = FUSED_TILE_Y;  // tile: 64x4 threads, each walks FZ cells
+                                                     // along z (FZ 1, 2, 4 or 8, chosen per box)
 constexpr int HS = 64;                  // per-block particle hash slots
 constexpr int PART_STRIDE = PART_STRIDE_;  // per-block partials: obj (7*MAX_OBJ_PART) + walls 18
 constexpr int MAX_OBJ_PART = 2;
+// per-thread partial sums of the fused kernel: object 0 force and torque, the bounce-back mass
+// term, and the force on the thread's first wall
+constexpr int ACC_OBJ = 0, ACC_BB = 6, ACC_WALL = 7, NACC = 10;
 
 template <typename S>
 struct FusedArgs {
@@ -180,6 +184,42 @@ __device__ inline C seg_sum(C v, int start, int lane) {
     return v;
 }
 
+// moving-object link (LB.cpp:1427-1453): bounce-back with the velocity of the object whose sphere
+// holds the neighbour (the last one, as the reference scans objects in order), momentum exchange
+// into the partials of the object named by the neighbour's solid index oi
+template <typename S>
+__device__ inline typename Types<S>::C object_value(const FusedArgs<S>& a, int j, int32_t yj, int oi,
+                                                   typename Types<S>::C gj, typename Types<S>::C n, double mi,
+                                                   double* sPart, double (*sAcc)[FX * FY], int tid) {
+    using C = typename Types<S>::C;
+    const int nObj = a.k.nObj;
+    int yx, yy, yz;
+    cell_xyz(a.g, yj, yx, yy, yz);
+    int ov = 0;
+    if (nObj > 1)
+        for (int oo = 0; oo < nObj; ++oo)
+            if (inside_sphere(yx, yy, yz, a.olb[oo].c[0], a.olb[oo].c[1], a.olb[oo].c[2], a.olb[oo].r)) ov = oo;
+    const ObjLB& OV = a.olb[ov];
+    const double cuo = OV.u[0] * cx_rt(j) + OV.u[1] * cy_rt(j) + OV.u[2] * cz_rt(j);
+    const C BBi = C(6.0) * n * C(w_rt(j)) * C(cuo);
+    if (oi >= 0 && oi < nObj && oi < MAX_OBJ_PART) {
+        const double fm = 2.0 * double(gj) - double(BBi);
+        const double Fb[3] = {fm * cx_rt(j), fm * cy_rt(j), fm * cz_rt(j)};
+        const double rr[3] = {yx - a.olb[oi].c[0], yy - a.olb[oi].c[1], yz - a.olb[oi].c[2]};
+        const double M[3] = {rr[1] * Fb[2] - rr[2] * Fb[1], rr[2] * Fb[0] - rr[0] * Fb[2], rr[0] * Fb[1] - rr[1] * Fb[0]};
+        if (oi == 0) {  // the thread's own accumulators
+#pragma unroll
+            for (int c = 0; c < 3; ++c) { sAcc[ACC_OBJ + c][tid] += Fb[c]; sAcc[ACC_OBJ + 3 + c][tid] += M[c]; }
+        } else {
+            double* sp = &sPart[7 * oi];
+#pragma unroll
+            for (int c = 0; c < 3; ++c) { atomicAdd(sp + c, Fb[c]); atomicAdd(sp + 3 + c, M[c]); }
+        }
+    }
+    sAcc[ACC_BB][tid] += double(BBi) * mi;
+    return gj - BBi;
+}
+
 // boundary link of the storing cell (LB.cpp:1390-1453): value reflected into the slot the cell
 // itself loads next step; momentum exchange into the block partials
 template <typename S, bool DIAG>
@@ -187,7 +227,7 @@ __device__ inline typename Types<S>::C boundary_value(const FusedArgs<S>& a, int
                                                      typename Types<S>::C gj, typename Types<S>::C n,
                                                      typename Types<S>::C sx, typename Types<S>::C sy,
                                                      typename Types<S>::C sz, typename Types<S>::C usq, double mi,
-                                                     double* sPart) {
+                                                     double* sPart, double (*sAcc)[FX * FY], int tid) {
     using C = typename Types<S>::C;
     const Consts& k = a.k;
     if (ty == T_GAS) {
@@ -205,34 +245,7 @@ __device__ inline typename Types<S>::C boundary_value(const FusedArgs<S>& a, int
         }
         return gj;
     }
-    if (ty == T_OBJ) {
-        const int nObj = k.nObj;
-        int ov = 0;
-        if (nObj > 1) {
-            int yx, yy, yz;
-            cell_xyz(a.g, yj, yx, yy, yz);
-            for (int oo = 0; oo < nObj; ++oo)
-                if (inside_sphere(yx, yy, yz, a.olb[oo].c[0], a.olb[oo].c[1], a.olb[oo].c[2], a.olb[oo].r)) ov = oo;
-        }
-        const ObjLB& OV = a.olb[ov];
-        const double cuo = OV.u[0] * cx_rt(j) + OV.u[1] * cy_rt(j) + OV.u[2] * cz_rt(j);
-        const C BBi = C(6.0) * n * C(w_rt(j)) * C(cuo);
-        const int oi = a.solid[yj];
-        if (oi >= 0 && oi < nObj && oi < MAX_OBJ_PART) {
-            int yx, yy, yz;
-            cell_xyz(a.g, yj, yx, yy, yz);
-            const double fm = 2.0 * double(gj) - double(BBi);
-            const double Fb[3] = {fm * cx_rt(j), fm * cy_rt(j), fm * cz_rt(j)};
-            const double rr[3] = {yx - a.olb[oi].c[0], yy - a.olb[oi].c[1], yz - a.olb[oi].c[2]};
-            double* sp = &sPart[7 * oi];
-            atomicAdd(sp + 0, Fb[0]); atomicAdd(sp + 1, Fb[1]); atomicAdd(sp + 2, Fb[2]);
-            atomicAdd(sp + 3, rr[1] * Fb[2] - rr[2] * Fb[1]);
-            atomicAdd(sp + 4, rr[2] * Fb[0] - rr[0] * Fb[2]);
-            atomicAdd(sp + 5, rr[0] * Fb[1] - rr[1] * Fb[0]);
-        }
-        atomicAdd(&sPart[6], double(BBi) * mi);
-        return gj - BBi;
-    }
+    if (ty == T_OBJ) return object_value<S>(a, j, yj, a.solid[yj], gj, n, mi, sPart, sAcc, tid);
     atomicOr(&a.sc->typeError, 1);
     return gj;
 }
@@ -423,13 +436,15 @@ __device__ inline void flush_particles(int tid, int nthreads, const int* hkey, d
 }
 
 // boundary-link classes of the active node i at (x, y, z), over its wrapped neighbours
-__device__ inline uint64_t link_classes(const Geom& g, const uint8_t* type, int32_t i, int x, int y, int z) {
+__device__ inline uint64_t link_classes(const Geom& g, const uint8_t* type, const int32_t* solid, int32_t i, int x,
+                                        int y, int z) {
     const Offs o(g, x, y, z);
     uint64_t m = 0;
     int wprim = -1;
 #pragma unroll
     for (int j = 1; j < Q; ++j) {
-        const uint8_t tf = type[i + o.d(j)];
+        const int32_t nb = i + o.d(j);
+        const uint8_t tf = type[nb];
         const uint8_t ty = tf & T_BASE;
         uint64_t cls = BC_OTHER;
         if (ty == T_FLUID || ty == T_IF) {
@@ -440,8 +455,11 @@ __device__ inline uint64_t link_classes(const Geom& g, const uint8_t* type, int3
             const int w = (tf >> T_WSHIFT) & 7;
             if (wprim < 0) wprim = w;
             cls = w == wprim ? BC_WALL : BC_OTHER;
+        } else if (ty == T_OBJ) {
+            const int32_t si = solid[nb];
+            cls = si == 0 ? BC_OBJ0 : (si == 1 ? BC_OBJ1 : BC_OTHER);
         }
-        m |= cls << (2 * (j - 1));
+        m |= cls << (BC_BITS * (j - 1));
     }
     if (wprim > 0) m |= uint64_t(wprim) << BC_WALL_SHIFT;
     return m;
@@ -459,7 +477,7 @@ __device__ inline uint64_t link_classes(const Geom& g, const uint8_t* type, int3
 template <typename S> struct FusedOcc { static constexpr int value = 3; };
 template <> struct FusedOcc<double> { static constexpr int value = 2; };
 
-template <typename S, int PAR, bool FSURF, bool DIAG>
+template <typename S, int PAR, bool FSURF, bool DIAG, int FZ>
 __global__ __launch_bounds__(FX* FY, FusedOcc<S>::value) void k_fused(FusedArgs<S> a) {
     using C = typename Types<S>::C;
     using M = typename Types<S>::M;
@@ -470,12 +488,18 @@ __global__ __launch_bounds__(FX* FY, FusedOcc<S>::value) void k_fused(FusedArgs<
     __shared__ C sg[Q - 1][NT];           // post-collision values of edge nodes (boundary loop)
     __shared__ int32_t sIf[NT * FZ];      // interface nodes of the block
     __shared__ int sIfCount, sIfBase;
+    __shared__ double sAcc[NACC][NT];     // per-thread object / wall partials (no atomics)
+    __shared__ int8_t sWtag[NT];          // wall whose force the thread accumulates (-1: none yet)
+    __shared__ double sRed[7 + 3 * MAX_WALLS][NT / 64];
     const int tid = threadIdx.x + FX * threadIdx.y;
     for (int s = tid; s < HS; s += NT) {
         hkey[s] = -1;
         for (int c = 0; c < 7; ++c) hval[s][c] = 0.0;
     }
     if (tid < PART_STRIDE) sPart[tid] = 0.0;
+#pragma unroll
+    for (int c = 0; c < NACC; ++c) sAcc[c][tid] = 0.0;
+    sWtag[tid] = -1;
     if (tid == 0) sIfCount = 0;
     __syncthreads();
 
@@ -534,7 +558,7 @@ __global__ __launch_bounds__(FX* FY, FusedOcc<S>::value) void k_fused(FusedArgs<
 #pragma unroll
                 for (int j = 1; j < Q; ++j) {
                     sg[j - 1][tid] = gc[j];
-                    if ((bc_ >> (2 * (j - 1))) & 3u) lm |= 1u << j;
+                    if ((bc_ >> (BC_BITS * (j - 1))) & 7u) lm |= 1u << j;
                 }
             }
             const Offs o(g, x, y, z);
@@ -542,7 +566,7 @@ __global__ __launch_bounds__(FX* FY, FusedOcc<S>::value) void k_fused(FusedArgs<
             while (lm) {
                 const int j = __ffs(lm) - 1;
                 lm &= lm - 1u;
-                const int cls = int(bc_ >> (2 * (j - 1))) & 3;
+                const int cls = int(bc_ >> (BC_BITS * (j - 1))) & 7;
                 const C gj = sg[j - 1][tid];
                 const int32_t yj = ic + offs_rt(o, j);
                 C bcv;
@@ -552,16 +576,26 @@ __global__ __launch_bounds__(FX* FY, FusedOcc<S>::value) void k_fused(FusedArgs<
                 } else if (cls == BC_WALL) {
                     if (DIAG && a.k.nWall > 0) {
                         const double v2 = 2.0 * double(gj);
-                        double* wp = &sPart[7 * MAX_OBJ_PART + 3 * wprim];
-                        if (cx_rt(j)) atomicAdd(wp + 0, v2 * cx_rt(j));
-                        if (cy_rt(j)) atomicAdd(wp + 1, v2 * cy_rt(j));
-                        if (cz_rt(j)) atomicAdd(wp + 2, v2 * cz_rt(j));
+                        const int tag = sWtag[tid];
+                        if (tag < 0 || tag == wprim) {
+                            sWtag[tid] = int8_t(wprim);
+                            sAcc[ACC_WALL + 0][tid] += v2 * cx_rt(j);
+                            sAcc[ACC_WALL + 1][tid] += v2 * cy_rt(j);
+                            sAcc[ACC_WALL + 2][tid] += v2 * cz_rt(j);
+                        } else {  // a second wall along this column (corners of closed boxes)
+                            double* wp = &sPart[7 * MAX_OBJ_PART + 3 * wprim];
+                            if (cx_rt(j)) atomicAdd(wp + 0, v2 * cx_rt(j));
+                            if (cy_rt(j)) atomicAdd(wp + 1, v2 * cy_rt(j));
+                            if (cz_rt(j)) atomicAdd(wp + 2, v2 * cz_rt(j));
+                        }
                     }
                     bcv = gj;
+                } else if (cls == BC_OBJ0 || cls == BC_OBJ1) {
+                    bcv = object_value<S>(a, j, yj, cls - BC_OBJ0, gj, r.n, mc, sPart, sAcc, tid);
                 } else {
                     const uint8_t ty = a.type[yj] & T_BASE;
                     if (ty == T_FLUID || ty == T_IF) continue;  // an object node uncovered since marking
-                    bcv = boundary_value<S, DIAG>(a, j, yj, ty, gj, r.n, r.sx, r.sy, r.sz, r.usq, mc, sPart);
+                    bcv = boundary_value<S, DIAG>(a, j, yj, ty, gj, r.n, r.sx, r.sy, r.sz, r.usq, mc, sPart, sAcc, tid);
                 }
                 if (PAR == 0) StoreT<S>::st(a.f + j * N + yj, bcv);
                 else StoreT<S>::st(a.f + opp_rt(j) * N + ic, bcv);
@@ -612,6 +646,32 @@ __global__ __launch_bounds__(FX* FY, FusedOcc<S>::value) void k_fused(FusedArgs<
             }
         }
     }
+    // per-thread accumulators into the block partials, in a fixed order (wavefront butterfly,
+    // then the waves in order); skipped by blocks without object or wall links
+    if (__syncthreads_or(sWtag[tid] >= 0 || sAcc[ACC_BB][tid] != 0.0 || sAcc[ACC_OBJ][tid] != 0.0 ||
+                         sAcc[ACC_OBJ + 1][tid] != 0.0 || sAcc[ACC_OBJ + 2][tid] != 0.0)) {
+        const int wv = tid / 64;
+        const int nv = 7 + (DIAG ? 3 * a.k.nWall : 0);  // object 0 (6), BB term, walls (3 each)
+        for (int c = 0; c < nv; ++c) {
+            double v;
+            if (c < 7) v = sAcc[c][tid];
+            else {
+                const int w = (c - 7) / 3;
+                v = sWtag[tid] == w ? sAcc[ACC_WALL + (c - 7) % 3][tid] : 0.0;
+            }
+#pragma unroll
+            for (int off = 32; off > 0; off >>= 1) v += __shfl_xor(v, off, 64);
+            if (lane == 0) sRed[c][wv] = v;
+        }
+        __syncthreads();
+        if (tid < nv) {
+            double t = 0.0;
+#pragma unroll
+            for (int w = 0; w < NT / 64; ++w) t += sRed[tid][w];
+            // partial layout: object o at 7 o (F, M, BB term at 6), wall w at 7 MAX_OBJ_PART + 3 w
+            sPart[tid < 7 ? tid : 7 * MAX_OBJ_PART + (tid - 7)] += t;
+        }
+    }
     // object / wall partials: written only by blocks that have any, flagged for the reduction
     const int any = __syncthreads_or(tid < PART_STRIDE && sPart[tid] != 0.0);
     if (any && tid < PART_STRIDE) a.blockPart[int64_t(tid) * gridDim.x + b0] = sPart[tid];
@@ -640,7 +700,7 @@ __device__ inline uint32_t inactive_bytes(uint32_t w) {
     return m & 0xFu;
 }
 constexpr int MARK_GRID = 4096;
-__global__ void k_mark_full_w(Geom g, uint8_t* type, uint64_t* bcm, const DevScalars* sc) {
+__global__ void k_mark_full_w(Geom g, uint8_t* type, const int32_t* solid, uint64_t* bcm, const DevScalars* sc) {
     if (!sc->fullMark) return;
     const int W = g.X / 4;
     const uint32_t* t32 = reinterpret_cast<const uint32_t*>(type);
@@ -686,33 +746,33 @@ __global__ void k_mark_full_w(Geom g, uint8_t* type, uint64_t* bcm, const DevSca
             const bool edge = ((own >> k) & 5u) || ((side >> k) & 7u) || ((diag >> k) & 1u);
             const uint8_t nt = edge ? uint8_t(t | T_EDGE) : uint8_t(t & ~T_EDGE);
             nw = (nw & ~(0xFFu << (8 * k))) | (uint32_t(nt) << (8 * k));
-            if (edge) bcm[c0 * 4 + k] = link_classes(g, type, int32_t(c0 * 4 + k), 4 * w + k, y, z);
+            if (edge) bcm[c0 * 4 + k] = link_classes(g, type, solid, int32_t(c0 * 4 + k), 4 * w + k, y, z);
         }
         // other threads read this word concurrently; they only test the type bits, which do not change
         if (nw != cw) reinterpret_cast<uint32_t*>(type)[c0] = nw;
     }
 }
 // one active node: edge bit and classes from its wrapped neighbours
-__device__ inline void mark_node(const Geom& g, uint8_t* type, uint64_t* bcm, int32_t i) {
+__device__ inline void mark_node(const Geom& g, uint8_t* type, const int32_t* solid, uint64_t* bcm, int32_t i) {
     const uint8_t t = type[i];
     if (!is_active(t)) return;
     int x, y, z;
     cell_xyz(g, i, x, y, z);
-    const uint64_t m = link_classes(g, type, i, x, y, z);
+    const uint64_t m = link_classes(g, type, solid, i, x, y, z);
     const bool edge = m != 0;
     if (edge) bcm[i] = m;
     const uint8_t nt = edge ? uint8_t(t | T_EDGE) : uint8_t(t & ~T_EDGE);
     if (nt != t) type[i] = nt;
 }
 // full pass for rows whose length is not a multiple of four: one thread per node
-__global__ void k_mark_full_b(Geom g, uint8_t* type, uint64_t* bcm, const DevScalars* sc) {
+__global__ void k_mark_full_b(Geom g, uint8_t* type, const int32_t* solid, uint64_t* bcm, const DevScalars* sc) {
     if (!sc->fullMark) return;
     for (int64_t i = int64_t(blockIdx.x) * blockDim.x + threadIdx.x; i < g.N; i += int64_t(gridDim.x) * blockDim.x)
-        mark_node(g, type, bcm, int32_t(i));
+        mark_node(g, type, solid, bcm, int32_t(i));
 }
 // incremental pass: the stencil neighbourhood (wrapped) of every node on the change list. A node
 // reached from several changed nodes is marked several times with the same result.
-__global__ void k_remark(Geom g, uint8_t* type, uint64_t* bcm, const int32_t* changed, int cap,
+__global__ void k_remark(Geom g, uint8_t* type, const int32_t* solid, uint64_t* bcm, const int32_t* changed, int cap,
                          const DevScalars* sc) {
     if (sc->fullMark) return;
     const int n = min(sc->nChanged, cap);
@@ -724,7 +784,7 @@ __global__ void k_remark(Geom g, uint8_t* type, uint64_t* bcm, const int32_t* ch
         cell_xyz(g, c, x, y, z);
         const Offs o(g, x, y, z);
         const int64_t p = int64_t(c) + offs_rt(o, j);
-        if (p >= 0 && p < g.N) mark_node(g, type, bcm, int32_t(p));  // (changed boundary-layer nodes)
+        if (p >= 0 && p < g.N) mark_node(g, type, solid, bcm, int32_t(p));  // (changed boundary-layer nodes)
     }
 }
 __global__ void k_mark_end(DevScalars* sc) {
@@ -733,14 +793,14 @@ __global__ void k_mark_end(DevScalars* sc) {
 }
 
 void lb_mark_edges(const Box& b, DevScalars* sc, hipStream_t s) {
-    if (b.g.X % 4 == 0) hipLaunchKernelGGL(k_mark_full_w, MARK_GRID, 256, 0, s, b.g, b.type, b.bcm, sc);
-    else hipLaunchKernelGGL(k_mark_full_b, MARK_GRID, 256, 0, s, b.g, b.type, b.bcm, sc);
-    hipLaunchKernelGGL(k_remark, 256, 256, 0, s, b.g, b.type, b.bcm, b.changed, b.changedCap, sc);
+    if (b.g.X % 4 == 0) hipLaunchKernelGGL(k_mark_full_w, MARK_GRID, 256, 0, s, b.g, b.type, b.solid, b.bcm, sc);
+    else hipLaunchKernelGGL(k_mark_full_b, MARK_GRID, 256, 0, s, b.g, b.type, b.solid, b.bcm, sc);
+    hipLaunchKernelGGL(k_remark, 256, 256, 0, s, b.g, b.type, b.solid, b.bcm, b.changed, b.changedCap, sc);
     hipLaunchKernelGGL(k_mark_end, 1, 1, 0, s, sc);
 }
 
-int fused_grid_blocks(const Geom& g) {
-    const int nbx = (g.X + FX - 1) / FX, nby = (g.Y + FY - 1) / FY, nbz = (g.Z + FZ - 1) / FZ;
+int fused_grid_blocks(const Geom& g, int fz) {
+    const int nbx = (g.X + FX - 1) / FX, nby = (g.Y + FY - 1) / FY, nbz = (g.Z + fz - 1) / fz;
     return ((nbx * nby * nbz + 7) / 8) * 8;
 }
 
@@ -753,20 +813,28 @@ static void launch_fused(Box& b, const Consts& k, const PartLB* plb, const ObjLB
     a.ifList = b.ifList; a.ifCap = b.ifCap;
     a.plb = plb; a.olb = olb; a.hacc = b.hacc; a.blockPart = b.blockPart; a.blockFlag = b.blockFlag; a.bcm = b.bcm; a.sc = sc;
     a.k = k;
-    a.nbx = (b.g.X + FX - 1) / FX; a.nby = (b.g.Y + FY - 1) / FY; a.nbz = (b.g.Z + FZ - 1) / FZ;
-    const int grid = fused_grid_blocks(b.g);
+    const int fz = b.fz;
+    a.nbx = (b.g.X + FX - 1) / FX; a.nby = (b.g.Y + FY - 1) / FY; a.nbz = (b.g.Z + fz - 1) / fz;
+    const int grid = fused_grid_blocks(b.g, fz);
     const bool fs = k.freeSurface != 0;
     const bool ghosts = b.parity == 1 && (b.g.per[0] || b.g.per[1] || b.g.per[2] || b.g.per[3] || b.g.per[4] || b.g.per[5]);
     if (ghosts) launch_ghost<S>(b, true, s);
     if (ev0) (void)hipEventRecord(ev0, s);
-#define LBDEM_FUSED(P, F, D) hipLaunchKernelGGL((k_fused<S, P, F, D>), dim3(grid), dim3(FX, FY, 1), 0, s, a)
-    if (b.parity == 0) {
-        if (fs) { if (diag) LBDEM_FUSED(0, true, true); else LBDEM_FUSED(0, true, false); }
-        else { if (diag) LBDEM_FUSED(0, false, true); else LBDEM_FUSED(0, false, false); }
-    } else {
-        if (fs) { if (diag) LBDEM_FUSED(1, true, true); else LBDEM_FUSED(1, true, false); }
-        else { if (diag) LBDEM_FUSED(1, false, true); else LBDEM_FUSED(1, false, false); }
+    (void)diag;  // the wall diagnostics are always accumulated
+#define LBDEM_FUSED(P, F, Z) hipLaunchKernelGGL((k_fused<S, P, F, true, Z>), dim3(grid), dim3(FX, FY, 1), 0, s, a)
+#define LBDEM_FUSED_Z(P, F)                 \
+    switch (fz) {                          \
+        case 1: LBDEM_FUSED(P, F, 1); break; \
+        case 2: LBDEM_FUSED(P, F, 2); break; \
+        case 4: LBDEM_FUSED(P, F, 4); break; \
+        default: LBDEM_FUSED(P, F, 8); break; \
     }
+    if (b.parity == 0) {
+        if (fs) { LBDEM_FUSED_Z(0, true); } else { LBDEM_FUSED_Z(0, false); }
+    } else {
+        if (fs) { LBDEM_FUSED_Z(1, true); } else { LBDEM_FUSED_Z(1, false); }
+    }
+#undef LBDEM_FUSED_Z
 #undef LBDEM_FUSED
     if (ev1) (void)hipEventRecord(ev1, s);
     if (ghosts) launch_ghost<S>(b, false, s);
@@ -837,7 +905,7 @@ __global__ void k_hydro_finalize(int nblk, const double* part, const int32_t* fl
 
 void lb_hydro_finalize(const Box& b, const Consts& k, double* FHydro, double* MHydro, double* fluidVol,
                        DevScalars* sc, double* objFH, double* objMH, double* wallFH, bool diag, hipStream_t s) {
-    const int nblk = fused_grid_blocks(b.g);
+    const int nblk = fused_grid_blocks(b.g, b.fz);
     const int grid = PART_STRIDE + (k.nPart + 255) / 256;
     hipLaunchKernelGGL(k_hydro_finalize, grid, 256, 0, s, nblk, b.blockPart, b.blockFlag, k, b.hacc, FHydro, MHydro, fluidVol,
                        sc, objFH, objMH, wallFH, diag ? 1 : 0);
@@ -1190,7 +1258,7 @@ __global__ void k_flags_apply(int64_t N, uint8_t* type, int32_t* solid, int32_t*
     if (threadIdx.x == 0) bmin = INT_MAX;
     __syncthreads();
     int mymin = INT_MAX;
-    bool wallIdx = false;
+    bool remark = false;  // a cleared wall or object index: link classes need a full pass
     for (int64_t i = int64_t(blockIdx.x) * blockDim.x + threadIdx.x; i < N; i += int64_t(gridDim.x) * blockDim.x) {
         const int32_t s = flagNew[i];
         const uint8_t t = type[i];
@@ -1202,12 +1270,14 @@ __global__ void k_flags_apply(int64_t N, uint8_t* type, int32_t* solid, int32_t*
         } else {
             // active nodes keep their edge bit; other nodes lose the flag and a wall index
             const uint8_t nt = is_active(t) ? uint8_t(t & ~T_PMASK) : uint8_t(t & T_BASE);
-            if (!is_active(t) && (t & 0x70)) wallIdx = true;
+            if (!is_active(t) && (t & 0x70)) remark = true;
+            // an object node's index is part of its neighbours' link classes
+            if ((t & T_BASE) == T_OBJ && solid[i] != 0) remark = true;
             if (nt != t) type[i] = nt;
             solid[i] = 0;
         }
     }
-    if (__syncthreads_or(wallIdx) && threadIdx.x == 0) atomicOr(&sc->fullMark, 1);
+    if (__syncthreads_or(remark) && threadIdx.x == 0) atomicOr(&sc->fullMark, 1);
     if (mymin != INT_MAX) atomicMin(&bmin, mymin);
     __syncthreads();
     // blocks run roughly in index order: only a smaller minimum needs the global atomic
@@ -1301,6 +1371,7 @@ void lb_find_new_active(const Box& b, const Consts& k, const PartLB* plb, DevSca
 // Candidates are processed in chunks of UNC_CHUNK; a chunk's results are written back before
 // the next chunk stages its neighbourhood.
 constexpr int UNCOVER_THREADS = 1024, UNC_CHUNK = 1024;
+constexpr int UNC_MAX_BLOCKS = 1024, UNC_NODES_PER_BLOCK = 4096;
 __device__ inline int block_exclusive_scan(int v, int* sWave, int& total) {
     const int lane = __lane_id(), wave = threadIdx.x / 64, nw = blockDim.x / 64;
     int incl = v;
@@ -1327,39 +1398,75 @@ template <typename S, int L>
 __global__ __launch_bounds__(UNCOVER_THREADS) void k_obj_uncover(Geom g, S* f, uint8_t* type,
                                                                  typename Types<S>::M* mass,
                                                                  typename Types<S>::C* aux, const int32_t* objList,
-                                                                 int32_t* cand, DevScalars* sc, const ObjLB* olb,
-                                                                 Consts k, int32_t* changed, int changedCap) {
+                                                                 int32_t* cand, int* blockCount, DevScalars* sc,
+                                                                 const ObjLB* olb, Consts k, int32_t* changed,
+                                                                 int changedCap) {
     using C = typename Types<S>::C;
     __shared__ int sWave[UNCOVER_THREADS / 64 + 1];
     __shared__ uint8_t nbT[Q - 1][UNC_CHUNK];   // neighbour base types (old values)
     __shared__ int16_t nbC[Q - 1][UNC_CHUNK];   // neighbour's position among earlier chunk candidates, or -1
     __shared__ uint8_t newT[UNC_CHUNK];
     __shared__ int32_t sCand[UNC_CHUNK];
+    __shared__ int sPre[UNC_MAX_BLOCKS + 1];   // exclusive prefix of the blocks' candidate counts
+    __shared__ int sLast;
     const int tid = threadIdx.x;
+    const int G = int(gridDim.x);
     const int n = sc->nObjList;
+    const int per = (n + G - 1) / G;
     const ObjLB O = olb[0];
-    // ordered compaction of the uncovered nodes
-    int m = 0;
-    for (int base = 0; base < n; base += UNCOVER_THREADS) {
-        const int kk = base + tid;
-        int32_t i = 0;
-        int flag = 0;
-        if (kk < n) {
-            i = objList[kk];
-            int x, y, z;
-            cell_xyz(g, i, x, y, z);
-            flag = !inside_sphere(x, y, z, O.c[0], O.c[1], O.c[2], O.r);
+    // pass 1 (every block): ordered compaction of the nodes of its slice that left the sphere,
+    // into the same slice of cand
+    {
+        const int beg = min(n, int(blockIdx.x) * per), end = min(n, beg + per);
+        int m = 0;
+        for (int base = beg; base < end; base += UNCOVER_THREADS) {
+            const int kk = base + tid;
+            int32_t i = 0;
+            int flag = 0;
+            if (kk < end) {
+                i = objList[kk];
+                int x, y, z;
+                cell_xyz(g, i, x, y, z);
+                flag = !inside_sphere(x, y, z, O.c[0], O.c[1], O.c[2], O.r);
+            }
+            int tot;
+            const int pos = block_exclusive_scan(flag, sWave, tot);
+            if (flag) cand[beg + m + pos] = i;
+            m += tot;
         }
-        int tot;
-        const int pos = block_exclusive_scan(flag, sWave, tot);
-        if (flag) cand[m + pos] = i;
-        m += tot;
+        if (tid == 0) blockCount[blockIdx.x] = m;
     }
-    __threadfence_block();
+    // the last block to finish pass 1 does pass 2; no block waits for another
+    __threadfence();
     __syncthreads();
+    if (tid == 0)
+        sLast = __hip_atomic_fetch_add(&sc->uncoverDone, 1, __ATOMIC_ACQ_REL, __HIP_MEMORY_SCOPE_AGENT) == G - 1;
+    __syncthreads();
+    if (!sLast) return;
+    __threadfence();
+    if (tid == 0) sc->uncoverDone = 0;  // every block has counted
+    {
+        const int c = tid < G ? blockCount[tid] : 0;
+        int tot;
+        const int ex = block_exclusive_scan(c, sWave, tot);
+        if (tid < G) sPre[tid] = ex;
+        if (tid == 0) sPre[G] = tot;
+    }
+    __syncthreads();
+    const int m = sPre[G];
+    // pass 2: candidates in ascending node order, in chunks; a candidate's new type depends on
+    // the new types of earlier candidates among its neighbours (the reference's sequential order)
     for (int c0 = 0; c0 < m; c0 += UNC_CHUNK) {
         const int cnt = min(UNC_CHUNK, m - c0);
-        if (tid < cnt) sCand[tid] = cand[c0 + tid];
+        if (tid < cnt) {
+            const int q = c0 + tid;
+            int lo = 0, hi = G - 1;  // block b with sPre[b] <= q < sPre[b + 1]
+            while (lo < hi) {
+                const int mid = (lo + hi + 1) >> 1;
+                if (sPre[mid] <= q) lo = mid; else hi = mid - 1;
+            }
+            sCand[tid] = cand[min(n, lo * per) + (q - sPre[lo])];
+        }
         __syncthreads();
         if (tid < cnt) {
             const int32_t i = sCand[tid];
@@ -1465,7 +1572,8 @@ __global__ __launch_bounds__(COVER_BS) void k_obj_cover_a(Geom g, uint8_t* type,
             if (which >= 0) {
                 const int64_t i = cidx(g, x, y, z);
                 const uint8_t t = type[i];
-                if ((t & T_BASE) != T_OBJ) push_changed(changed, changedCap, sc, int32_t(i));
+                // the link classes of neighbouring edge nodes hold the type and the object index
+                if ((t & T_BASE) != T_OBJ || solid[i] != which) push_changed(changed, changedCap, sc, int32_t(i));
                 type[i] = uint8_t((t & T_PMASK) | T_OBJ);
                 solid[i] = which;
                 in = 1;
@@ -1517,12 +1625,15 @@ template <typename S>
 static void obj_impl(Box& b, const Consts& k, const ObjLB* olb, DevScalars* sc, hipStream_t s) {
     using M = typename Types<S>::M;
     using C = typename Types<S>::C;
+    // the object node list holds at most objCap nodes: one block per UNC_NODES_PER_BLOCK of them
+    const int grid = std::max(1, std::min(UNC_MAX_BLOCKS, (b.objCap + UNC_NODES_PER_BLOCK - 1) / UNC_NODES_PER_BLOCK));
+    int* uncCount = b.coverCount + 256;
     if (b.parity == 0)
-        hipLaunchKernelGGL((k_obj_uncover<S, 0>), 1, UNCOVER_THREADS, 0, s, b.g, (S*)b.f, b.type, (M*)b.mass, (C*)b.aux,
-                           b.objList, b.objCand, sc, olb, k, b.changed, b.changedCap);
+        hipLaunchKernelGGL((k_obj_uncover<S, 0>), grid, UNCOVER_THREADS, 0, s, b.g, (S*)b.f, b.type, (M*)b.mass,
+                           (C*)b.aux, b.objList, b.objCand, uncCount, sc, olb, k, b.changed, b.changedCap);
     else
-        hipLaunchKernelGGL((k_obj_uncover<S, 1>), 1, UNCOVER_THREADS, 0, s, b.g, (S*)b.f, b.type, (M*)b.mass, (C*)b.aux,
-                           b.objList, b.objCand, sc, olb, k, b.changed, b.changedCap);
+        hipLaunchKernelGGL((k_obj_uncover<S, 1>), grid, UNCOVER_THREADS, 0, s, b.g, (S*)b.f, b.type, (M*)b.mass,
+                           (C*)b.aux, b.objList, b.objCand, uncCount, sc, olb, k, b.changed, b.changedCap);
     hipLaunchKernelGGL(k_obj_cover_a, COVER_GRID, COVER_BS, 0, s, b.g, b.type, b.solid, olb, k.nObj, b.coverCount,
                        b.changed, b.changedCap, sc);
     hipLaunchKernelGGL(k_obj_cover_b, COVER_GRID, COVER_BS, 0, s, b.g, b.type, olb, k.nObj, b.coverCount, b.objList,

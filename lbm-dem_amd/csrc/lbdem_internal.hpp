@@ -57,7 +57,11 @@ enum : uint8_t { T_FLUID = 0, T_GAS = 2, T_IF = 3, T_PER = 4, T_WALL = 7, T_OBJ 
 // 0 active neighbour (no boundary), 1 gas, 2 wall receiving the node's primary wall index
 // (bits 36-38), 3 anything else (object node, another wall, unexpected type: resolved from the
 // neighbour's type and solid index).
-enum : int { BC_NONE = 0, BC_GAS = 1, BC_WALL = 2, BC_OTHER = 3, BC_WALL_SHIFT = 36 };
+// boundary-link classes of an edge node: 3 bits per direction 1..18, then the primary wall index.
+// OBJ0/OBJ1: object node whose solid index is 0/1; OTHER: anything else (type reloaded in the
+// fused kernel)
+enum : int { BC_NONE = 0, BC_GAS = 1, BC_WALL = 2, BC_OBJ0 = 3, BC_OBJ1 = 4, BC_OTHER = 7, BC_BITS = 3,
+             BC_WALL_SHIFT = 54 };
 
 // Geometry of one box: sizes and which faces wrap (reference LB.cpp:85-126, 632-678:
 // an active node wraps along an axis when its face neighbour on that side is periodic).
@@ -126,7 +130,7 @@ struct DevScalars {
     int kminP;                 // lowest index of a particle-flagged node (-1: none)
     int needScan;              // findNewActive needs the full scan
     int kstar;                 // first particle node still covered
-    int pad[1];
+    int uncoverDone;           // blocks of the object uncover pass that finished (reset by the last)
 };
 
 template <typename Tst> struct StoreT;
@@ -150,6 +154,7 @@ template <> struct StoreT<__half> {
 struct Box {
     Geom g;
     int prec;
+    int fz = 8;                // z-walk length of the fused kernel's tiles (1, 2, 4 or 8)
     void* f;                   // 19*N storage elements
     uint8_t* type;
     int32_t* solid;
@@ -165,7 +170,7 @@ struct Box {
     int32_t* flagNew;          // N, -1 = none (particle flags computed at a rebuild)
     int32_t* objList;          // object nodes, ascending
     int32_t* objCand;          // uncovered object nodes of the current step, ascending
-    int* coverCount;           // per-block counts of the object cover pass
+    int* coverCount;           // per-block counts of the object cover (256) and uncover (1024) passes
     int32_t* changed;          // nodes to re-mark (see DevScalars::nChanged)
     int changedCap;
     int ifCap, objCap;
@@ -202,7 +207,8 @@ constexpr int PART_STRIDE_ = 32;
 void lb_flags_rebuild_apply(const Box& b, DevScalars* sc, hipStream_t s);
 void lb_find_new_active(const Box& b, const Consts& k, const PartLB* plb, DevScalars* sc, hipStream_t s);
 void lb_object_update(Box& b, const Consts& k, const ObjLB* olb, DevScalars* sc, hipStream_t s);
-int fused_grid_blocks(const Geom& g);
+int fused_grid_blocks(const Geom& g, int fz);  // blocks of the fused kernel for z-walk length fz
+constexpr int FUSED_TILE_X = 64, FUSED_TILE_Y = 4;  // threads of a fused-kernel tile (x, y)
 void lb_download_f(const Box& b, double* out, hipStream_t s);
 void lb_download_n(const Box& b, const Consts& k, double* out, hipStream_t s);
 void lb_total_mass(const Box& b, double* out, hipStream_t s);

@@ -104,6 +104,16 @@ class RunResult(C.Structure):
                     deviceBytes=self.deviceBytes, message=self.message.decode(errors="replace"))
 
 
+class Tuning(C.Structure):
+    _fields_ = [("tile", C.c_int * 3), ("tuned", C.c_int), ("nCandidates", C.c_int), ("candidateZ", C.c_int * 4),
+                ("candidateMs", C.c_double * 4)]
+
+    def as_dict(self) -> Dict:
+        n = self.nCandidates
+        return dict(tile=list(self.tile), tuned=bool(self.tuned),
+                    candidates={int(self.candidateZ[i]): float(self.candidateMs[i]) for i in range(n)})
+
+
 class PoolStats(C.Structure):
     _fields_ = [
         ("submitted", C.c_int), ("queued", C.c_int), ("active", C.c_int), ("done", C.c_int),
@@ -156,6 +166,8 @@ class Library:
             "lbdem_estimate_bytes": ([C.POINTER(Params), C.c_int, _dp, C.c_int, _dp, C.POINTER(C.c_size_t)], C.c_int),
             "lbdem_device_bytes": ([vp, C.POINTER(C.c_size_t)], C.c_int),
             "lbdem_use_graphs": ([vp, C.c_int], C.c_int),
+            "lbdem_set_tile": ([vp, C.c_int], C.c_int),
+            "lbdem_get_tuning": ([vp, C.POINTER(Tuning)], C.c_int),
             "lbdem_pool_create": ([C.c_size_t, C.c_int, C.c_int, C.c_int, C.POINTER(vp)], C.c_int),
             "lbdem_pool_submit": ([vp, C.POINTER(RunDesc), _ip], C.c_int),
             "lbdem_pool_poll": ([vp, C.c_int, C.POINTER(RunResult)], C.c_int),
@@ -309,6 +321,15 @@ class Simulation:
 
     def use_graphs(self, enable: bool):
         self._check(self.lib.lib.lbdem_use_graphs(self.ctx, 1 if enable else 0), "lbdem_use_graphs")
+
+    def set_tile(self, tile_z: int):
+        """0: tune the fused kernel's z-walk length in the first LB steps; 1, 2, 4, 8: fix it."""
+        self._check(self.lib.lib.lbdem_set_tile(self.ctx, int(tile_z)), "lbdem_set_tile")
+
+    def tuning(self) -> Dict:
+        t = Tuning()
+        self._check(self.lib.lib.lbdem_get_tuning(self.ctx, C.byref(t)), "lbdem_get_tuning")
+        return t.as_dict()
 
     def device_bytes(self) -> int:
         v = C.c_size_t()

@@ -1446,6 +1446,15 @@ int lbdem_device_bytes(lbdem_ctx* c, size_t* bytes) {
     return 0;
 }
 int lbdem_use_graphs(lbdem_ctx* c, int) { return c ? 0 : LBDEM_E_ARG; }
+int lbdem_set_tile(lbdem_ctx* c, int tileZ) {
+    if (!c) return LBDEM_E_ARG;
+    return (tileZ == 0 || tileZ == 1 || tileZ == 2 || tileZ == 4 || tileZ == 8) ? 0 : LBDEM_E_ARG;
+}
+int lbdem_get_tuning(lbdem_ctx* c, lbdem_tuning* t) {
+    if (!c || !t) return LBDEM_E_ARG;
+    std::memset(t, 0, sizeof(*t));  // no kernel tiles on the CPU
+    return 0;
+}
 int lbdem_pool_create(size_t, int, int, int, lbdem_pool** out) {
     if (out) *out = nullptr;
     return LBDEM_E_STATE;

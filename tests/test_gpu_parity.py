@@ -89,6 +89,39 @@ def test_graph_replays_match_oracle(oracle_lib, gpu_lib, case_root, name):
     run_pair(oracle_lib, gpu_lib, case_root, name, lbdem.PREC_FP64, 60, every=10)
 
 
+@pytest.mark.parametrize("tile_z", [1, 2, 4, 8])
+@pytest.mark.parametrize("name", ["suspension", "periodic"])
+def test_every_tile_matches_oracle(oracle_lib, gpu_lib, case_root, name, tile_z):
+    """The fused kernel with each z-walk length the tile tuner can pick."""
+    d = cases.make_case(case_root, name)
+    a = cases.open_sim(oracle_lib, d)
+    b = cases.open_sim(gpu_lib, d, precision=lbdem.PREC_FP64)
+    b.set_tile(tile_z)
+    print(f"case={name} tile_z={tile_z}", flush=True)
+    for step in range(3):
+        a.cycle(10)
+        b.cycle(10)
+        b.synchronize()
+        compare(cases.state(a), cases.state(b), lbdem.PREC_FP64, f"{name} tile {tile_z} step {10 * (step + 1)}")
+    assert b.tuning()["tile"] == [64, 4, tile_z]
+    a.close()
+    b.close()
+
+
+def test_tile_tuner_picks_a_candidate(gpu_lib, case_root):
+    d = cases.make_case(case_root, "suspension")
+    sim = cases.open_sim(gpu_lib, d, precision=lbdem.PREC_FP32)
+    assert not sim.tuning()["tuned"]
+    sim.cycle(12)
+    sim.synchronize()
+    t = sim.tuning()
+    print("tuning", t, flush=True)
+    assert t["tuned"] and t["tile"][2] in (1, 2, 4, 8)
+    assert all(ms > 0 for ms in t["candidates"].values())
+    assert t["candidates"][t["tile"][2]] == min(t["candidates"].values())
+    sim.close()
+
+
 @pytest.mark.parametrize("name", ["suspension", "units", "periodic"])
 def test_fp32_offsets_within_tolerance(oracle_lib, gpu_lib, case_root, name):
     run_pair(oracle_lib, gpu_lib, case_root, name, lbdem.PREC_FP32, 40, every=5)
