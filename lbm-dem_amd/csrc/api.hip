@@ -367,7 +367,7 @@ static bool plan_buffers(const Plan& p, Box& b, DemState& d, DevScalars*& sc, do
     A(b.flagNew, N);
     A(b.objList, size_t(b.objCap));
     A(b.objCand, size_t(b.objCap));
-    A(b.coverCount, 256 + 1024);
+    A(b.coverCount, COVER_BLOCKS + UNCOVER_BLOCKS_MAX);
     A(b.changed, size_t(b.changedCap));
     A(b.blockPart, size_t(b.blockPartCap));
     A(b.blockFlag, size_t(fused_grid_blocks(p.g, 1)));

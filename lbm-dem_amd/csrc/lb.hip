@@ -190,7 +190,8 @@ __device__ inline C seg_sum(C v, int start, int lane) {
 template <typename S>
 __device__ inline typename Types<S>::C object_value(const FusedArgs<S>& a, int j, int32_t yj, int oi,
                                                    typename Types<S>::C gj, typename Types<S>::C n, double mi,
-                                                   double* sPart, double (*sAcc)[FX * FY], int tid) {
+                                                   double* sPart, double (*sAcc)[FX * FY], int tid,
+                                                   const ObjLB* sObj) {
     using C = typename Types<S>::C;
     const int nObj = a.k.nObj;
     int yx, yy, yz;
@@ -198,14 +199,14 @@ __device__ inline typename Types<S>::C object_value(const FusedArgs<S>& a, int j
     int ov = 0;
     if (nObj > 1)
         for (int oo = 0; oo < nObj; ++oo)
-            if (inside_sphere(yx, yy, yz, a.olb[oo].c[0], a.olb[oo].c[1], a.olb[oo].c[2], a.olb[oo].r)) ov = oo;
-    const ObjLB& OV = a.olb[ov];
+            if (inside_sphere(yx, yy, yz, sObj[oo].c[0], sObj[oo].c[1], sObj[oo].c[2], sObj[oo].r)) ov = oo;
+    const ObjLB& OV = sObj[ov];
     const double cuo = OV.u[0] * cx_rt(j) + OV.u[1] * cy_rt(j) + OV.u[2] * cz_rt(j);
     const C BBi = C(6.0) * n * C(w_rt(j)) * C(cuo);
     if (oi >= 0 && oi < nObj && oi < MAX_OBJ_PART) {
         const double fm = 2.0 * double(gj) - double(BBi);
         const double Fb[3] = {fm * cx_rt(j), fm * cy_rt(j), fm * cz_rt(j)};
-        const double rr[3] = {yx - a.olb[oi].c[0], yy - a.olb[oi].c[1], yz - a.olb[oi].c[2]};
+        const double rr[3] = {yx - sObj[oi].c[0], yy - sObj[oi].c[1], yz - sObj[oi].c[2]};
         const double M[3] = {rr[1] * Fb[2] - rr[2] * Fb[1], rr[2] * Fb[0] - rr[0] * Fb[2], rr[0] * Fb[1] - rr[1] * Fb[0]};
         if (oi == 0) {  // the thread's own accumulators
 #pragma unroll
@@ -227,7 +228,8 @@ __device__ inline typename Types<S>::C boundary_value(const FusedArgs<S>& a, int
                                                      typename Types<S>::C gj, typename Types<S>::C n,
                                                      typename Types<S>::C sx, typename Types<S>::C sy,
                                                      typename Types<S>::C sz, typename Types<S>::C usq, double mi,
-                                                     double* sPart, double (*sAcc)[FX * FY], int tid) {
+                                                     double* sPart, double (*sAcc)[FX * FY], int tid,
+                                                     const ObjLB* sObj) {
     using C = typename Types<S>::C;
     const Consts& k = a.k;
     if (ty == T_GAS) {
@@ -245,7 +247,7 @@ __device__ inline typename Types<S>::C boundary_value(const FusedArgs<S>& a, int
         }
         return gj;
     }
-    if (ty == T_OBJ) return object_value<S>(a, j, yj, a.solid[yj], gj, n, mi, sPart, sAcc, tid);
+    if (ty == T_OBJ) return object_value<S>(a, j, yj, a.solid[yj], gj, n, mi, sPart, sAcc, tid, sObj);
     atomicOr(&a.sc->typeError, 1);
     return gj;
 }
@@ -491,7 +493,10 @@ __global__ __launch_bounds__(FX* FY, FusedOcc<S>::value) void k_fused(FusedArgs<
     __shared__ double sAcc[NACC][NT];     // per-thread object / wall partials (no atomics)
     __shared__ int8_t sWtag[NT];          // wall whose force the thread accumulates (-1: none yet)
     __shared__ double sRed[7 + 3 * MAX_WALLS][NT / 64];
+    __shared__ ObjLB sObj[MAX_OBJ_PART];  // the impactors' centres, radii and velocities
     const int tid = threadIdx.x + FX * threadIdx.y;
+    if (tid < MAX_OBJ_PART * 8 && tid / 8 < a.k.nObj)
+        reinterpret_cast<double*>(sObj)[tid] = reinterpret_cast<const double*>(a.olb)[tid];
     for (int s = tid; s < HS; s += NT) {
         hkey[s] = -1;
         for (int c = 0; c < 7; ++c) hval[s][c] = 0.0;
@@ -591,11 +596,12 @@ __global__ __launch_bounds__(FX* FY, FusedOcc<S>::value) void k_fused(FusedArgs<
                     }
                     bcv = gj;
                 } else if (cls == BC_OBJ0 || cls == BC_OBJ1) {
-                    bcv = object_value<S>(a, j, yj, cls - BC_OBJ0, gj, r.n, mc, sPart, sAcc, tid);
+                    bcv = object_value<S>(a, j, yj, cls - BC_OBJ0, gj, r.n, mc, sPart, sAcc, tid, sObj);
                 } else {
                     const uint8_t ty = a.type[yj] & T_BASE;
                     if (ty == T_FLUID || ty == T_IF) continue;  // an object node uncovered since marking
-                    bcv = boundary_value<S, DIAG>(a, j, yj, ty, gj, r.n, r.sx, r.sy, r.sz, r.usq, mc, sPart, sAcc, tid);
+                    bcv = boundary_value<S, DIAG>(a, j, yj, ty, gj, r.n, r.sx, r.sy, r.sz, r.usq, mc, sPart, sAcc, tid,
+                                                  sObj);
                 }
                 if (PAR == 0) StoreT<S>::st(a.f + j * N + yj, bcv);
                 else StoreT<S>::st(a.f + opp_rt(j) * N + ic, bcv);
@@ -1364,14 +1370,19 @@ void lb_find_new_active(const Box& b, const Consts& k, const PartLB* plb, DevSca
 // impactor nodes (updateObjectBoundaries + findNewActiveObject, LB.cpp:797-943)
 // ------------------------------------------------------------------------------------------
 // uncovered object nodes (LB.cpp:846-943). The previous object nodes (ascending list) that the
-// moved sphere no longer covers are compacted in ascending order. Each conversion reads the node
-// types its predecessors in that order wrote, so the decisions are taken in order by one thread,
-// on the neighbour types staged in LDS: for every candidate its 18 neighbour types and, for
-// neighbours that are earlier candidates of the same chunk, their position in the chunk.
-// Candidates are processed in chunks of UNC_CHUNK; a chunk's results are written back before
-// the next chunk stages its neighbourhood.
+// moved sphere no longer covers become fluid, interface or gas nodes. Pass 1 (k_obj_uncover_find,
+// many blocks) compacts them in ascending order, each block over its slice of the list. Each
+// conversion reads the node types its predecessors in that order wrote, so pass 2
+// (k_obj_uncover, one block) takes the candidates in chunks of UNC_CHUNK: it stages their neighbour types in LDS, finds
+// neighbours that are candidates of the chunk through a hash table, decides the new types (see
+// below) and writes a chunk's results back before the next chunk stages its neighbourhood.
 constexpr int UNCOVER_THREADS = 1024, UNC_CHUNK = 1024;
-constexpr int UNC_MAX_BLOCKS = 1024, UNC_NODES_PER_BLOCK = 4096;
+constexpr int UNC_MAX_BLOCKS = UNCOVER_BLOCKS_MAX, UNC_NODES_PER_BLOCK = 4096;
+constexpr int UNC_HASH = 2048;  // open-addressing table of a chunk's candidates (at most half full)
+__device__ inline uint32_t unc_hash(int32_t key) { return (uint32_t(key) * 2654435761u) >> (32 - 11); }
+__device__ inline uint8_t uncover_type(bool anyFluid, bool anyGas) {
+    return !anyFluid ? uint8_t(T_GAS) : (anyGas ? uint8_t(T_IF) : uint8_t(T_FLUID));
+}
 __device__ inline int block_exclusive_scan(int v, int* sWave, int& total) {
     const int lane = __lane_id(), wave = threadIdx.x / 64, nw = blockDim.x / 64;
     int incl = v;
@@ -1394,57 +1405,55 @@ __device__ inline int block_exclusive_scan(int v, int* sWave, int& total) {
     return r;
 }
 
-template <typename S, int L>
-__global__ __launch_bounds__(UNCOVER_THREADS) void k_obj_uncover(Geom g, S* f, uint8_t* type,
-                                                                 typename Types<S>::M* mass,
-                                                                 typename Types<S>::C* aux, const int32_t* objList,
-                                                                 int32_t* cand, int* blockCount, DevScalars* sc,
-                                                                 const ObjLB* olb, Consts k, int32_t* changed,
-                                                                 int changedCap) {
-    using C = typename Types<S>::C;
+// pass 1: ordered compaction of the nodes of each block's slice of the list that left the sphere,
+// into the same slice of cand
+__global__ __launch_bounds__(UNCOVER_THREADS) void k_obj_uncover_find(Geom g, const int32_t* objList, int32_t* cand,
+                                                                      int* blockCount, const DevScalars* sc,
+                                                                      const ObjLB* olb) {
     __shared__ int sWave[UNCOVER_THREADS / 64 + 1];
-    __shared__ uint8_t nbT[Q - 1][UNC_CHUNK];   // neighbour base types (old values)
-    __shared__ int16_t nbC[Q - 1][UNC_CHUNK];   // neighbour's position among earlier chunk candidates, or -1
-    __shared__ uint8_t newT[UNC_CHUNK];
-    __shared__ int32_t sCand[UNC_CHUNK];
-    __shared__ int sPre[UNC_MAX_BLOCKS + 1];   // exclusive prefix of the blocks' candidate counts
-    __shared__ int sLast;
     const int tid = threadIdx.x;
     const int G = int(gridDim.x);
     const int n = sc->nObjList;
     const int per = (n + G - 1) / G;
     const ObjLB O = olb[0];
-    // pass 1 (every block): ordered compaction of the nodes of its slice that left the sphere,
-    // into the same slice of cand
-    {
-        const int beg = min(n, int(blockIdx.x) * per), end = min(n, beg + per);
-        int m = 0;
-        for (int base = beg; base < end; base += UNCOVER_THREADS) {
-            const int kk = base + tid;
-            int32_t i = 0;
-            int flag = 0;
-            if (kk < end) {
-                i = objList[kk];
-                int x, y, z;
-                cell_xyz(g, i, x, y, z);
-                flag = !inside_sphere(x, y, z, O.c[0], O.c[1], O.c[2], O.r);
-            }
-            int tot;
-            const int pos = block_exclusive_scan(flag, sWave, tot);
-            if (flag) cand[beg + m + pos] = i;
-            m += tot;
+    const int beg = min(n, int(blockIdx.x) * per), end = min(n, beg + per);
+    int m = 0;
+    for (int base = beg; base < end; base += UNCOVER_THREADS) {
+        const int kk = base + tid;
+        int32_t i = 0;
+        int flag = 0;
+        if (kk < end) {
+            i = objList[kk];
+            int x, y, z;
+            cell_xyz(g, i, x, y, z);
+            flag = !inside_sphere(x, y, z, O.c[0], O.c[1], O.c[2], O.r);
         }
-        if (tid == 0) blockCount[blockIdx.x] = m;
+        int tot;
+        const int pos = block_exclusive_scan(flag, sWave, tot);
+        if (flag) cand[beg + m + pos] = i;
+        m += tot;
     }
-    // the last block to finish pass 1 does pass 2; no block waits for another
-    __threadfence();
-    __syncthreads();
-    if (tid == 0)
-        sLast = __hip_atomic_fetch_add(&sc->uncoverDone, 1, __ATOMIC_ACQ_REL, __HIP_MEMORY_SCOPE_AGENT) == G - 1;
-    __syncthreads();
-    if (!sLast) return;
-    __threadfence();
-    if (tid == 0) sc->uncoverDone = 0;  // every block has counted
+    if (tid == 0) blockCount[blockIdx.x] = m;
+}
+
+// pass 2 (one block): the candidates of the G slices in order
+template <typename S, int L>
+__global__ __launch_bounds__(UNCOVER_THREADS) void k_obj_uncover(Geom g, S* f, uint8_t* type,
+                                                                 typename Types<S>::M* mass,
+                                                                 typename Types<S>::C* aux, const int32_t* cand,
+                                                                 const int* blockCount, int G, DevScalars* sc,
+                                                                 Consts k, int32_t* changed, int changedCap) {
+    using C = typename Types<S>::C;
+    __shared__ int sWave[UNCOVER_THREADS / 64 + 1];
+    __shared__ uint8_t nbT[Q - 1][UNC_CHUNK];   // neighbour base types (values before this pass)
+    __shared__ uint8_t newT[UNC_CHUNK];
+    __shared__ int32_t sCand[UNC_CHUNK];
+    __shared__ int32_t hKey[UNC_HASH];          // chunk candidates: node index -> position
+    __shared__ int16_t hPos[UNC_HASH];
+    __shared__ int sPre[UNC_MAX_BLOCKS + 1];   // exclusive prefix of the slices' candidate counts
+    const int tid = threadIdx.x;
+    const int n = sc->nObjList;
+    const int per = (n + G - 1) / G;
     {
         const int c = tid < G ? blockCount[tid] : 0;
         int tot;
@@ -1467,34 +1476,65 @@ __global__ __launch_bounds__(UNCOVER_THREADS) void k_obj_uncover(Geom g, S* f, u
             }
             sCand[tid] = cand[min(n, lo * per) + (q - sPre[lo])];
         }
+        for (int h = tid; h < UNC_HASH; h += UNCOVER_THREADS) hKey[h] = -1;
         __syncthreads();
+        if (tid < cnt) {  // position of every candidate, by node index
+            const int32_t key = sCand[tid];
+            for (uint32_t h = unc_hash(key);; h = (h + 1) & (UNC_HASH - 1))
+                if (atomicCAS(&hKey[h], -1, key) == -1) { hPos[h] = int16_t(tid); break; }
+        }
         if (tid < cnt) {
             const int32_t i = sCand[tid];
             int x, y, z;
             cell_xyz(g, i, x, y, z);
-            for (int j = 1; j < Q; ++j) {
-                const int32_t nb = int32_t(nbr_plain(g, x, y, z, j));
-                nbT[j - 1][tid] = type[nb] & T_BASE;
-                // earlier candidate of this chunk? (sCand ascending)
-                int lo = 0, hi = tid;
-                while (lo < hi) {
-                    const int mid = (lo + hi) >> 1;
-                    if (sCand[mid] < nb) lo = mid + 1; else hi = mid;
-                }
-                nbC[j - 1][tid] = (lo < tid && sCand[lo] == nb) ? int16_t(lo) : int16_t(-1);
-            }
+            for (int j = 1; j < Q; ++j) nbT[j - 1][tid] = type[nbr_plain(g, x, y, z, j)] & T_BASE;
         }
         __syncthreads();
-        if (tid == 0) {
-            for (int t = 0; t < cnt; ++t) {
-                bool anyFluid = false, anyGas = false;
-                for (int j = 0; j < Q - 1; ++j) {
-                    const int c = nbC[j][t];
-                    const uint8_t ty = c >= 0 ? newT[c] : nbT[j][t];
-                    anyFluid |= ty == T_FLUID;
-                    anyGas |= ty == T_GAS;
+        // The new type of candidate t: gas without a fluid neighbour, interface with fluid and gas
+        // neighbours, fluid otherwise, over the types its neighbours have when the reference
+        // reaches t (earlier candidates converted, later ones still object nodes). One wavefront
+        // takes 64 candidates at a time: types of earlier groups are final; dependencies inside
+        // the group are resolved by repeating the decision with the group's current types until
+        // none changes. On the dependency order (a candidate depends on earlier ones only) the
+        // only fixed point is the sequential result, reached after at most depth + 1 rounds.
+        if (tid < 64) {
+            const int lane = tid;
+            for (int s0 = 0; s0 < cnt; s0 += 64) {
+                const int t = s0 + lane;
+                const bool valid = t < cnt;
+                bool baseF = false, baseG = false;
+                uint64_t dep = 0;
+                if (valid) {
+                    int x, y, z;
+                    cell_xyz(g, sCand[t], x, y, z);
+                    for (int j = 1; j < Q; ++j) {
+                        const int32_t nb = int32_t(nbr_plain(g, x, y, z, j));
+                        int pos = -1;
+                        for (uint32_t h = unc_hash(nb);; h = (h + 1) & (UNC_HASH - 1)) {
+                            const int32_t kk = hKey[h];
+                            if (kk == nb) { pos = hPos[h]; break; }
+                            if (kk == -1) break;
+                        }
+                        uint8_t ty = nbT[j - 1][t];
+                        if (pos >= 0 && pos < t) {
+                            if (pos < s0) ty = newT[pos];
+                            else { dep |= 1ull << (pos - s0); continue; }
+                        }
+                        baseF |= ty == T_FLUID;
+                        baseG |= ty == T_GAS;
+                    }
                 }
-                newT[t] = !anyFluid ? uint8_t(T_GAS) : (anyGas ? uint8_t(T_IF) : uint8_t(T_FLUID));
+                uint8_t myT = uncover_type(baseF, baseG);
+                for (int round = 0; round <= 64; ++round) {
+                    const uint64_t fm = __ballot(valid && myT == T_FLUID);
+                    const uint64_t gm = __ballot(valid && myT == T_GAS);
+                    const uint8_t nt = uncover_type(baseF || (dep & fm), baseG || (dep & gm));
+                    const uint64_t ch = __ballot(valid && nt != myT);
+                    myT = nt;
+                    if (!ch) break;
+                }
+                if (valid) newT[t] = myT;
+                __threadfence_block();  // the next group of this wavefront reads these
             }
         }
         __syncthreads();
@@ -1532,7 +1572,7 @@ __global__ __launch_bounds__(UNCOVER_THREADS) void k_obj_uncover(Geom g, S* f, u
 // cover: every node inside an object sphere becomes an object node; ascending object node list.
 // Pass a covers the union box of the spheres and counts per block (blocks own consecutive
 // ranges of the box); pass b writes the list at the blocks' exclusive prefix offsets.
-constexpr int COVER_GRID = 128, COVER_BS = 256;
+constexpr int COVER_GRID = COVER_BLOCKS, COVER_BS = 256;
 struct CoverBox {
     int lo[3], n[3], tot;
 };
@@ -1557,20 +1597,23 @@ __device__ inline int inside_which(int x, int y, int z, const ObjLB* olb, int nO
     return which;
 }
 __global__ __launch_bounds__(COVER_BS) void k_obj_cover_a(Geom g, uint8_t* type, int32_t* solid, const ObjLB* olb,
-                                                          int nObj, int* blockCount, int32_t* changed, int changedCap,
-                                                          DevScalars* sc) {
+                                                          int nObj, int* blockCount, int32_t* seg, int segCap,
+                                                          int32_t* changed, int changedCap, DevScalars* sc) {
+    __shared__ int sWave[COVER_BS / 64 + 1];
     const CoverBox cb = cover_box(g, olb, nObj);
     const int per = (cb.tot + gridDim.x - 1) / gridDim.x;
-    const int beg = blockIdx.x * per, end = min(cb.tot, beg + per);
+    const int beg = min(cb.tot, int(blockIdx.x) * per), end = min(cb.tot, beg + per);
+    const bool segmented = cb.tot <= segCap;  // the block's nodes go to seg[beg ...] in order
     int cnt = 0;
     for (int c0 = beg; c0 < end; c0 += COVER_BS) {
         const int c = c0 + threadIdx.x;
         int in = 0;
+        int64_t i = 0;
         if (c < end) {
             const int x = cb.lo[0] + c % cb.n[0], y = cb.lo[1] + (c / cb.n[0]) % cb.n[1], z = cb.lo[2] + c / (cb.n[0] * cb.n[1]);
             const int which = inside_which(x, y, z, olb, nObj);
             if (which >= 0) {
-                const int64_t i = cidx(g, x, y, z);
+                i = cidx(g, x, y, z);
                 const uint8_t t = type[i];
                 // the link classes of neighbouring edge nodes hold the type and the object index
                 if ((t & T_BASE) != T_OBJ || solid[i] != which) push_changed(changed, changedCap, sc, int32_t(i));
@@ -1579,32 +1622,51 @@ __global__ __launch_bounds__(COVER_BS) void k_obj_cover_a(Geom g, uint8_t* type,
                 in = 1;
             }
         }
-        cnt += __syncthreads_count(in);
+        if (segmented) {
+            int tot;
+            const int pos = block_exclusive_scan(in, sWave, tot);
+            if (in) seg[beg + cnt + pos] = int32_t(i);
+            cnt += tot;
+        } else {
+            cnt += __syncthreads_count(in);
+        }
     }
     if (threadIdx.x == 0) blockCount[blockIdx.x] = cnt;
 }
+// the object node list: every block's nodes at the exclusive prefix of the block counts
 __global__ __launch_bounds__(COVER_BS) void k_obj_cover_b(Geom g, const uint8_t* type, const ObjLB* olb, int nObj,
-                                                          const int* blockCount, int32_t* objList, int cap,
-                                                          DevScalars* sc) {
+                                                          const int* blockCount, const int32_t* seg, int segCap,
+                                                          int32_t* objList, int cap, DevScalars* sc) {
     __shared__ int sWave[COVER_BS / 64 + 1];
-    __shared__ int sOff;
+    __shared__ int sOff, sTot;
     const CoverBox cb = cover_box(g, olb, nObj);
-    if (threadIdx.x == 0) {
-        int off = 0, tot = 0;
-        for (int b = 0; b < int(gridDim.x); ++b) {
-            if (b == int(blockIdx.x)) off = tot;
-            tot += blockCount[b];
+    {
+        int before = 0, all = 0;
+        for (int b = threadIdx.x; b < int(gridDim.x); b += COVER_BS) {
+            const int v = blockCount[b];
+            all += v;
+            if (b < int(blockIdx.x)) before += v;
         }
-        sOff = off;
-        if (blockIdx.x == 0) {
-            sc->nObjList = tot < cap ? tot : cap;
-            if (tot > cap) sc->listOverflow |= 1;
-        }
+        int t1, t2;
+        (void)block_exclusive_scan(before, sWave, t1);
+        (void)block_exclusive_scan(all, sWave, t2);
+        if (threadIdx.x == 0) { sOff = t1; sTot = t2; }
+        __syncthreads();
     }
-    __syncthreads();
+    if (blockIdx.x == 0 && threadIdx.x == 0) {
+        sc->nObjList = sTot < cap ? sTot : cap;
+        if (sTot > cap) sc->listOverflow |= 1;
+    }
     int off = sOff;
     const int per = (cb.tot + gridDim.x - 1) / gridDim.x;
-    const int beg = blockIdx.x * per, end = min(cb.tot, beg + per);
+    const int beg = min(cb.tot, int(blockIdx.x) * per), end = min(cb.tot, beg + per);
+    if (cb.tot <= segCap) {
+        const int cnt = blockCount[blockIdx.x];
+        for (int k = threadIdx.x; k < cnt; k += COVER_BS)
+            if (off + k < cap) objList[off + k] = seg[beg + k];
+        return;
+    }
+    // union box larger than the staging buffer (impactors far apart): scan the box again
     for (int c0 = beg; c0 < end; c0 += COVER_BS) {
         const int c = c0 + threadIdx.x;
         int in = 0;
@@ -1627,17 +1689,19 @@ static void obj_impl(Box& b, const Consts& k, const ObjLB* olb, DevScalars* sc, 
     using C = typename Types<S>::C;
     // the object node list holds at most objCap nodes: one block per UNC_NODES_PER_BLOCK of them
     const int grid = std::max(1, std::min(UNC_MAX_BLOCKS, (b.objCap + UNC_NODES_PER_BLOCK - 1) / UNC_NODES_PER_BLOCK));
-    int* uncCount = b.coverCount + 256;
+    int* uncCount = b.coverCount + COVER_GRID;
+    hipLaunchKernelGGL(k_obj_uncover_find, grid, UNCOVER_THREADS, 0, s, b.g, b.objList, b.objCand, uncCount, sc, olb);
     if (b.parity == 0)
-        hipLaunchKernelGGL((k_obj_uncover<S, 0>), grid, UNCOVER_THREADS, 0, s, b.g, (S*)b.f, b.type, (M*)b.mass,
-                           (C*)b.aux, b.objList, b.objCand, uncCount, sc, olb, k, b.changed, b.changedCap);
+        hipLaunchKernelGGL((k_obj_uncover<S, 0>), 1, UNCOVER_THREADS, 0, s, b.g, (S*)b.f, b.type, (M*)b.mass,
+                           (C*)b.aux, b.objCand, uncCount, grid, sc, k, b.changed, b.changedCap);
     else
-        hipLaunchKernelGGL((k_obj_uncover<S, 1>), grid, UNCOVER_THREADS, 0, s, b.g, (S*)b.f, b.type, (M*)b.mass,
-                           (C*)b.aux, b.objList, b.objCand, uncCount, sc, olb, k, b.changed, b.changedCap);
+        hipLaunchKernelGGL((k_obj_uncover<S, 1>), 1, UNCOVER_THREADS, 0, s, b.g, (S*)b.f, b.type, (M*)b.mass,
+                           (C*)b.aux, b.objCand, uncCount, grid, sc, k, b.changed, b.changedCap);
+    // the candidate buffer of the uncover pass stages the cover pass's per-block node lists
     hipLaunchKernelGGL(k_obj_cover_a, COVER_GRID, COVER_BS, 0, s, b.g, b.type, b.solid, olb, k.nObj, b.coverCount,
-                       b.changed, b.changedCap, sc);
-    hipLaunchKernelGGL(k_obj_cover_b, COVER_GRID, COVER_BS, 0, s, b.g, b.type, olb, k.nObj, b.coverCount, b.objList,
-                       b.objCap, sc);
+                       b.objCand, b.objCap, b.changed, b.changedCap, sc);
+    hipLaunchKernelGGL(k_obj_cover_b, COVER_GRID, COVER_BS, 0, s, b.g, b.type, olb, k.nObj, b.coverCount, b.objCand,
+                       b.objCap, b.objList, b.objCap, sc);
 }
 
 void lb_object_update(Box& b, const Consts& k, const ObjLB* olb, DevScalars* sc, hipStream_t s) {

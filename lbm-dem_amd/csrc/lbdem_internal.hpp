@@ -105,6 +105,7 @@ struct ObjLB {
     double u[3];     // x1 / unit.Speed (object node velocity)
     double pad;
 };
+static_assert(sizeof(ObjLB) == 8 * sizeof(double), "ObjLB is staged as 8 doubles");
 
 // Scalars shared between kernels (device memory). Doubles first.
 struct DevScalars {
@@ -130,7 +131,7 @@ struct DevScalars {
     int kminP;                 // lowest index of a particle-flagged node (-1: none)
     int needScan;              // findNewActive needs the full scan
     int kstar;                 // first particle node still covered
-    int uncoverDone;           // blocks of the object uncover pass that finished (reset by the last)
+    int pad[1];
 };
 
 template <typename Tst> struct StoreT;
@@ -170,7 +171,7 @@ struct Box {
     int32_t* flagNew;          // N, -1 = none (particle flags computed at a rebuild)
     int32_t* objList;          // object nodes, ascending
     int32_t* objCand;          // uncovered object nodes of the current step, ascending
-    int* coverCount;           // per-block counts of the object cover (256) and uncover (1024) passes
+    int* coverCount;           // per-block counts of the object cover and uncover passes
     int32_t* changed;          // nodes to re-mark (see DevScalars::nChanged)
     int changedCap;
     int ifCap, objCap;
@@ -209,6 +210,7 @@ void lb_find_new_active(const Box& b, const Consts& k, const PartLB* plb, DevSca
 void lb_object_update(Box& b, const Consts& k, const ObjLB* olb, DevScalars* sc, hipStream_t s);
 int fused_grid_blocks(const Geom& g, int fz);  // blocks of the fused kernel for z-walk length fz
 constexpr int FUSED_TILE_X = 64, FUSED_TILE_Y = 4;  // threads of a fused-kernel tile (x, y)
+constexpr int COVER_BLOCKS = 512, UNCOVER_BLOCKS_MAX = 1024;  // grids of the impactor node passes
 void lb_download_f(const Box& b, double* out, hipStream_t s);
 void lb_download_n(const Box& b, const Consts& k, double* out, hipStream_t s);
 void lb_total_mass(const Box& b, double* out, hipStream_t s);
