@@ -255,26 +255,58 @@ __device__ inline typename Types<S>::C boundary_value(const FusedArgs<S>& a, int
 // plain index offset of direction q (valid for cells without the edge bit)
 __device__ inline int32_t plain_off(int q, int32_t sy, int32_t sz) { return cxq(q) + cyq(q) * sy + czq(q) * sz; }
 
+// Offset corrections of a node next to a periodic face: a link leaving through the face reaches
+// the wrapped partner (coordinate D-2 or 1) instead of the face's boundary layer (LB.cpp:632-678).
+// m* apply to directions with a negative component on that axis, p* to positive ones; all zero
+// away from periodic faces.
+struct Wrap {
+    int32_t mx, px, my, py, mz, pz;
+    __device__ Wrap(const Geom& g, int x, int y, int z) {
+        const int32_t sy = g.X, sz = g.X * g.Y;
+        mx = (x == 1 && g.per[0]) ? g.X - 2 : 0;
+        px = (x == g.X - 2 && g.per[1]) ? -(g.X - 2) : 0;
+        my = (y == 1 && g.per[2]) ? (g.Y - 2) * sy : 0;
+        py = (y == g.Y - 2 && g.per[3]) ? -(g.Y - 2) * sy : 0;
+        mz = (z == 1 && g.per[4]) ? (g.Z - 2) * sz : 0;
+        pz = (z == g.Z - 2 && g.per[5]) ? -(g.Z - 2) * sz : 0;
+    }
+    __device__ bool any() const { return (mx | px | my | py | mz | pz) != 0; }
+    // correction of direction q (compile-time q after unrolling)
+    __device__ int32_t d(int q) const {
+        return (cxq(q) < 0 ? mx : (cxq(q) > 0 ? px : 0)) + (cyq(q) < 0 ? my : (cyq(q) > 0 ? py : 0)) +
+               (czq(q) < 0 ? mz : (czq(q) > 0 ? pz : 0));
+    }
+};
+
 // populations of node i before collision (its post-streaming values). PAR 0: the node's own
-// slots; PAR 1: the slots its neighbours wrote, at plain offsets (links that wrap through a
-// periodic face reach the face's ghost layer, filled before the step; see k_ghost)
+// slots; PAR 1: the slots its neighbours wrote, at plain offsets, wrapped through periodic faces
+// (a wavefront without any node next to a periodic face takes the plain offsets only)
 template <typename S, int PAR>
-__device__ inline void load_pop(const FusedArgs<S>& a, int32_t i, bool act, typename Types<S>::C* gq) {
+__device__ inline void load_pop(const FusedArgs<S>& a, int32_t i, bool act, int x, int y, int z,
+                                typename Types<S>::C* gq) {
     if (!act) return;
     const int32_t N = int32_t(a.g.N), sy = a.g.X, sz = a.g.X * a.g.Y;
     if (PAR == 0) {
 #pragma unroll
         for (int q = 0; q < Q; ++q) gq[q] = StoreT<S>::ld(a.f + q * N + i);
     } else {
+        const Wrap w(a.g, x, y, z);
+        if (__ballot(w.any())) {
 #pragma unroll
-        for (int q = 0; q < Q; ++q) gq[q] = StoreT<S>::ld(a.f + oppq(q) * N + i + plain_off(oppq(q), sy, sz));
+            for (int q = 0; q < Q; ++q)
+                gq[q] = StoreT<S>::ld(a.f + oppq(q) * N + i + plain_off(oppq(q), sy, sz) + w.d(oppq(q)));
+        } else {
+#pragma unroll
+            for (int q = 0; q < Q; ++q) gq[q] = StoreT<S>::ld(a.f + oppq(q) * N + i + plain_off(oppq(q), sy, sz));
+        }
     }
 }
 
 // streaming of the post-collision values (LB.cpp:1375-1389): PAR 0 into the node's own slots
-// (opposite direction), PAR 1 into the neighbours' slots
+// (opposite direction), PAR 1 into the neighbours' slots (wrapped as in load_pop)
 template <typename S, int PAR>
-__device__ inline void store_pop(const FusedArgs<S>& a, int32_t i, bool act, const typename Types<S>::C* gq) {
+__device__ inline void store_pop(const FusedArgs<S>& a, int32_t i, bool act, int x, int y, int z,
+                                 const typename Types<S>::C* gq) {
     if (!act) return;
     const int32_t N = int32_t(a.g.N), sy = a.g.X, sz = a.g.X * a.g.Y;
     StoreT<S>::st(a.f + i, gq[0]);
@@ -282,65 +314,15 @@ __device__ inline void store_pop(const FusedArgs<S>& a, int32_t i, bool act, con
 #pragma unroll
         for (int j = 1; j < Q; ++j) StoreT<S>::st(a.f + oppq(j) * N + i, gq[j]);
     } else {
+        const Wrap w(a.g, x, y, z);
+        if (__ballot(w.any())) {
 #pragma unroll
-        for (int j = 1; j < Q; ++j) StoreT<S>::st(a.f + j * N + i + plain_off(j, sy, sz), gq[j]);
-    }
-}
-
-// Ghost layers of periodic faces for the PAR 1 step. A node next to a periodic face reads and
-// writes, at plain offsets, slots of the face's boundary layer (coordinate 0 or D-1 on that
-// axis); the wrapped partner (coordinate D-2 or 1, other axes wrapped) holds the real slots
-// (LB.cpp:632-678). Before the step (PRE) the boundary layer receives the partner's slots of the
-// directions pointing out through the face (every slot a node can read there; each ghost slot has
-// one wrapped source, so overlapping faces write equal values). After the step (POST) the slots
-// that interior nodes wrote there go back to the partner; only slots whose writer is an active
-// interior node are copied, so every real slot has one source.
-__device__ inline int wrap_coord(int u, int D, bool perLo, bool perHi) {
-    return (u == 0 && perLo) ? D - 2 : ((u == D - 1 && perHi) ? 1 : u);
-}
-template <typename S, bool PRE>
-__global__ void k_ghost(Geom g, S* f, const uint8_t* type, int axis, int side) {
-    const int D[3] = {g.X, g.Y, g.Z};
-    const int a1 = (axis + 1) % 3, a2 = (axis + 2) % 3;
-    const int64_t nFace = int64_t(D[a1]) * D[a2];
-    const int64_t t = int64_t(blockIdx.x) * blockDim.x + threadIdx.x;
-    if (t >= nFace) return;
-    const int u = int(t % D[a1]), v = int(t / D[a1]);
-    int gp[3], pp[3];
-    gp[axis] = side < 0 ? 0 : D[axis] - 1;
-    pp[axis] = side < 0 ? D[axis] - 2 : 1;
-    gp[a1] = u; gp[a2] = v;
-    pp[a1] = wrap_coord(u, D[a1], g.per[2 * a1], g.per[2 * a1 + 1]);
-    pp[a2] = wrap_coord(v, D[a2], g.per[2 * a2], g.per[2 * a2 + 1]);
-    const int64_t gi = cidx(g, gp[0], gp[1], gp[2]), pi = cidx(g, pp[0], pp[1], pp[2]);
-    for (int d = 1; d < Q; ++d) {
-        const int c[3] = {cxq(d), cyq(d), czq(d)};
-        if (c[axis] != side) continue;
-        if (PRE) {
-            f[int64_t(d) * g.N + gi] = f[int64_t(d) * g.N + pi];
+            for (int j = 1; j < Q; ++j) StoreT<S>::st(a.f + j * N + i + plain_off(j, sy, sz) + w.d(j), gq[j]);
         } else {
-            // writer of ghost slot d: the node at gp - c_d, which must be an interior active node
-            // (otherwise the partner may hold a boundary value of its own in that slot)
-            int wp[3];
-            wp[axis] = gp[axis] - c[axis]; wp[a1] = gp[a1] - c[a1]; wp[a2] = gp[a2] - c[a2];
-            if (wp[a1] < 1 || wp[a1] > D[a1] - 2 || wp[a2] < 1 || wp[a2] > D[a2] - 2) continue;
-            if (!is_active(type[cidx(g, wp[0], wp[1], wp[2])])) continue;
-            f[int64_t(d) * g.N + pi] = f[int64_t(d) * g.N + gi];
+#pragma unroll
+            for (int j = 1; j < Q; ++j) StoreT<S>::st(a.f + j * N + i + plain_off(j, sy, sz), gq[j]);
         }
     }
-}
-template <typename S>
-static void launch_ghost(const Box& b, bool pre, hipStream_t s) {
-    const int D[3] = {b.g.X, b.g.Y, b.g.Z};
-    for (int axis = 0; axis < 3; ++axis)
-        for (int sd = 0; sd < 2; ++sd) {
-            if (!b.g.per[2 * axis + sd]) continue;
-            const int64_t nFace = int64_t(D[(axis + 1) % 3]) * D[(axis + 2) % 3];
-            const int side = sd == 0 ? -1 : 1;
-            const int grid = int((nFace + 255) / 256);
-            if (pre) hipLaunchKernelGGL((k_ghost<S, true>), grid, 256, 0, s, b.g, (S*)b.f, b.type, axis, side);
-            else hipLaunchKernelGGL((k_ghost<S, false>), grid, 256, 0, s, b.g, (S*)b.f, b.type, axis, side);
-        }
 }
 
 // one node's collision (node.cpp:75-207): reconstruct, interior forcing of particle nodes
@@ -531,7 +513,7 @@ __global__ __launch_bounds__(FX* FY, FusedOcc<S>::value) void k_fused(FusedArgs<
     int32_t ic = col_ok ? int32_t(cidx(g, x, y, z0)) : 0;
     uint8_t tc = type_at(0), tn = type_at(1);
     bool actc = is_active(tc);
-    load_pop<S, PAR>(a, ic, actc, gc);
+    load_pop<S, PAR>(a, ic, actc, x, y, z0, gc);
     double mc = (actc && (tc & (T_PMASK | T_EDGE))) ? double(a.mass[ic]) : 0.0;
     int32_t sc_ = (actc && (tc & T_PMASK)) ? a.solid[ic] : 0;
     uint64_t bc_ = (actc && (tc & T_EDGE)) ? a.bcm[ic] : 0;
@@ -541,7 +523,7 @@ __global__ __launch_bounds__(FX* FY, FusedOcc<S>::value) void k_fused(FusedArgs<
         const int32_t in = ic + SZ;
         const bool actn = is_active(tn);
         const uint8_t tnn = type_at(zz + 2);
-        load_pop<S, PAR>(a, in, actn, gn);
+        load_pop<S, PAR>(a, in, actn, x, y, z + 1, gn);
         const double mn = (actn && (tn & (T_PMASK | T_EDGE))) ? double(a.mass[in]) : 0.0;
         const int32_t sn = (actn && (tn & T_PMASK)) ? a.solid[in] : 0;
         const uint64_t bn = (actn && (tn & T_EDGE)) ? a.bcm[in] : 0;
@@ -552,7 +534,7 @@ __global__ __launch_bounds__(FX* FY, FusedOcc<S>::value) void k_fused(FusedArgs<
         for (int c = 0; c < 7; ++c) r.d[c] = 0;
         r.n = 1; r.sx = 0; r.sy = 0; r.sz = 0; r.usq = 0;
         if (actc) collide_cell<S>(a, sc_, tc, x, y, z, mc, gc, r);
-        store_pop<S, PAR>(a, ic, actc, gc);
+        store_pop<S, PAR>(a, ic, actc, x, y, z, gc);
 
         // boundary links of edge nodes (LB.cpp:1390-1528): the value reflected into the slot
         // this node loads next step
@@ -823,8 +805,6 @@ static void launch_fused(Box& b, const Consts& k, const PartLB* plb, const ObjLB
     a.nbx = (b.g.X + FX - 1) / FX; a.nby = (b.g.Y + FY - 1) / FY; a.nbz = (b.g.Z + fz - 1) / fz;
     const int grid = fused_grid_blocks(b.g, fz);
     const bool fs = k.freeSurface != 0;
-    const bool ghosts = b.parity == 1 && (b.g.per[0] || b.g.per[1] || b.g.per[2] || b.g.per[3] || b.g.per[4] || b.g.per[5]);
-    if (ghosts) launch_ghost<S>(b, true, s);
     if (ev0) (void)hipEventRecord(ev0, s);
     (void)diag;  // the wall diagnostics are always accumulated
 #define LBDEM_FUSED(P, F, Z) hipLaunchKernelGGL((k_fused<S, P, F, true, Z>), dim3(grid), dim3(FX, FY, 1), 0, s, a)
@@ -843,7 +823,6 @@ static void launch_fused(Box& b, const Consts& k, const PartLB* plb, const ObjLB
 #undef LBDEM_FUSED_Z
 #undef LBDEM_FUSED
     if (ev1) (void)hipEventRecord(ev1, s);
-    if (ghosts) launch_ghost<S>(b, false, s);
     b.parity ^= 1;
 }
 
@@ -864,26 +843,29 @@ void lb_fused_step(Box& b, const Consts& k, const PartLB* plb, const ObjLB* olb,
 
 // deterministic sum of the flagged per-block partials, unit conversion of all hydrodynamic results
 // (LB.cpp:1966-1977, 1536-1543)
-__global__ void k_hydro_finalize(int nblk, const double* part, const int32_t* flag, Consts k, const double* hacc, double* FHydro,
+constexpr int HF_THREADS = 1024;
+__global__ __launch_bounds__(HF_THREADS) void k_hydro_finalize(int nblk, const double* part, const int32_t* flag, Consts k, const double* hacc, double* FHydro,
                                  double* MHydro, double* fluidVol, DevScalars* sc, double* objFH, double* objMH,
                                  double* wallFH, int diag) {
-    __shared__ double red[256];
+    __shared__ double red[HF_THREADS / 64];
     const int tid = threadIdx.x;
     if (blockIdx.x < PART_STRIDE) {
-        // column c of the partial table (one block per column), summed in a fixed order
+        // column c of the partial table (one block per column), summed in a fixed order: strided
+        // per-thread sums, a wavefront butterfly, then the waves in order
         const int c = blockIdx.x;
         if (c >= 7 * MAX_OBJ_PART && !diag) return;
+        if (c < 7 * MAX_OBJ_PART && c / 7 >= k.nObj && c != 6) return;            // no such object
+        if (c >= 7 * MAX_OBJ_PART && (c - 7 * MAX_OBJ_PART) / 3 >= k.nWall) return;  // no such wall
         double acc = 0.0;
         for (int b = tid; b < nblk; b += blockDim.x)
             if (flag[b]) acc += part[int64_t(c) * nblk + b];
-        red[tid] = acc;
+#pragma unroll
+        for (int off = 32; off > 0; off >>= 1) acc += __shfl_xor(acc, off, 64);
+        if ((tid & 63) == 0) red[tid / 64] = acc;
         __syncthreads();
-        for (int w = blockDim.x / 2; w > 0; w >>= 1) {
-            if (tid < w) red[tid] += red[tid + w];
-            __syncthreads();
-        }
         if (tid == 0) {
-            const double v = red[0];
+            double v = 0.0;
+            for (int w = 0; w < HF_THREADS / 64; ++w) v += red[w];
             if (c < 7 * MAX_OBJ_PART) {
                 const int o = c / 7, comp = c % 7;
                 if (o < k.nObj) {
@@ -912,8 +894,8 @@ __global__ void k_hydro_finalize(int nblk, const double* part, const int32_t* fl
 void lb_hydro_finalize(const Box& b, const Consts& k, double* FHydro, double* MHydro, double* fluidVol,
                        DevScalars* sc, double* objFH, double* objMH, double* wallFH, bool diag, hipStream_t s) {
     const int nblk = fused_grid_blocks(b.g, b.fz);
-    const int grid = PART_STRIDE + (k.nPart + 255) / 256;
-    hipLaunchKernelGGL(k_hydro_finalize, grid, 256, 0, s, nblk, b.blockPart, b.blockFlag, k, b.hacc, FHydro, MHydro, fluidVol,
+    const int grid = PART_STRIDE + (k.nPart + HF_THREADS - 1) / HF_THREADS;
+    hipLaunchKernelGGL(k_hydro_finalize, grid, HF_THREADS, 0, s, nblk, b.blockPart, b.blockFlag, k, b.hacc, FHydro, MHydro, fluidVol,
                        sc, objFH, objMH, wallFH, diag ? 1 : 0);
 }
 
@@ -1227,34 +1209,38 @@ void lb_free_surface(Box& b, const Consts& k, DevScalars* sc, hipStream_t s) {
 // ------------------------------------------------------------------------------------------
 // particle flags (LB.cpp:683-711 via 1875-1899; findNewActive LB.cpp:1980-2029)
 // ------------------------------------------------------------------------------------------
-// one block per particle over its bounding box: active nodes inside the sphere, highest index wins
+// blocks over particles, each particle's bounding box: active nodes inside the sphere, highest
+// index wins
+constexpr int FLAGS_GRID = 2048;  // grids of the flag passes (grid-stride; most steps return early)
 __global__ void k_flags_compute(Geom g, const uint8_t* type, const PartLB* plb, int nPart, int32_t* flagNew,
                                 const DevScalars* sc) {
     if (!sc->newNeighborList) return;
-    const int p = blockIdx.x;
-    if (p >= nPart) return;
-    const PartLB P = plb[p];
-    int lo[3], hi[3];
     const int dims[3] = {g.X, g.Y, g.Z};
-    for (int a = 0; a < 3; ++a) {
-        lo[a] = int(floor(P.c[a] - P.r)) - 1;
-        hi[a] = int(ceil(P.c[a] + P.r)) + 1;
-        lo[a] = lo[a] < 0 ? 0 : lo[a];
-        hi[a] = hi[a] > dims[a] - 1 ? dims[a] - 1 : hi[a];
-    }
-    if (lo[0] > hi[0] || lo[1] > hi[1] || lo[2] > hi[2]) return;
-    const int nx = hi[0] - lo[0] + 1, ny = hi[1] - lo[1] + 1, nz = hi[2] - lo[2] + 1;
-    const int tot = nx * ny * nz;
-    for (int c = threadIdx.x; c < tot; c += blockDim.x) {
-        const int x = lo[0] + c % nx, y = lo[1] + (c / nx) % ny, z = lo[2] + c / (nx * ny);
-        const int64_t i = cidx(g, x, y, z);
-        if (!is_active(type[i])) continue;
-        if (inside_sphere(x, y, z, P.c[0], P.c[1], P.c[2], P.r)) atomicMax(&flagNew[i], p);
+    for (int p = blockIdx.x; p < nPart; p += gridDim.x) {
+        const PartLB P = plb[p];
+        int lo[3], hi[3];
+        for (int a = 0; a < 3; ++a) {
+            lo[a] = int(floor(P.c[a] - P.r)) - 1;
+            hi[a] = int(ceil(P.c[a] + P.r)) + 1;
+            lo[a] = lo[a] < 0 ? 0 : lo[a];
+            hi[a] = hi[a] > dims[a] - 1 ? dims[a] - 1 : hi[a];
+        }
+        if (lo[0] > hi[0] || lo[1] > hi[1] || lo[2] > hi[2]) continue;
+        const int nx = hi[0] - lo[0] + 1, ny = hi[1] - lo[1] + 1, nz = hi[2] - lo[2] + 1;
+        const int tot = nx * ny * nz;
+        for (int c = threadIdx.x; c < tot; c += blockDim.x) {
+            const int x = lo[0] + c % nx, y = lo[1] + (c / nx) % ny, z = lo[2] + c / (nx * ny);
+            const int64_t i = cidx(g, x, y, z);
+            if (!is_active(type[i])) continue;
+            if (inside_sphere(x, y, z, P.c[0], P.c[1], P.c[2], P.r)) atomicMax(&flagNew[i], p);
+        }
     }
 }
 
 void lb_flags_rebuild_compute(const Box& b, const Consts& k, const PartLB* plb, DevScalars* sc, hipStream_t s) {
-    if (k.nPart > 0) hipLaunchKernelGGL(k_flags_compute, k.nPart, 256, 0, s, b.g, b.type, plb, k.nPart, b.flagNew, sc);
+    // grid-stride over the particles: most steps only test the flag and return
+    if (k.nPart > 0)
+        hipLaunchKernelGGL(k_flags_compute, std::min(k.nPart, FLAGS_GRID), 256, 0, s, b.g, b.type, plb, k.nPart, b.flagNew, sc);
 }
 
 // clear every flag and solid index, apply the new flags (initializeParticleBoundaries)
@@ -1304,7 +1290,7 @@ __global__ void k_flags_apply_end(DevScalars* sc) {
 void lb_flags_rebuild_apply(const Box& b, DevScalars* sc, hipStream_t s) {
     hipLaunchKernelGGL(k_flags_apply_begin, 1, 1, 0, s, sc);
     const int64_t nb = (b.g.N + 255) / 256;
-    const int grid = int(nb < 65536 ? nb : 65536);
+    const int grid = int(nb < FLAGS_GRID ? nb : FLAGS_GRID);
     hipLaunchKernelGGL(k_flags_apply, grid, 256, 0, s, b.g.N, b.type, b.solid, b.flagNew, sc);
     hipLaunchKernelGGL(k_flags_apply_end, 1, 1, 0, s, sc);
 }
