@@ -177,6 +177,23 @@ int lbdem_set_tile(lbdem_ctx* ctx, int tileZ);
 int lbdem_get_tuning(lbdem_ctx* ctx, lbdem_tuning* tuning);
 
 /* ---------------------------------------------------------------------------------------------
+ * Batch (no reference counterpart): several initialised contexts (same precision and device)
+ * stepped together, every kernel of a step launched once for all of them (the launch's second
+ * grid dimension selects the box; each box keeps its own memory, parameters and counters). A step
+ * pair of the batch is captured into a HIP graph per phase and replayed. Members must have run
+ * past their start-up steps (the impactor wait, the tile tuner, the first LB step) and be at the
+ * same step parity; the batch uses one z-walk tile for all members (the largest box's). Results
+ * are those of each context run alone. While a context belongs to a batch it is advanced only
+ * through the batch; its downloads stay valid (they wait for the batch's work).
+ * ------------------------------------------------------------------------------------------- */
+typedef struct lbdem_batch lbdem_batch;
+int lbdem_batch_create(lbdem_ctx* const* ctxs, int n, lbdem_batch** out);
+int lbdem_batch_cycle(lbdem_batch* batch, int nsteps);  /* nsteps even */
+int lbdem_batch_synchronize(lbdem_batch* batch);
+const char* lbdem_batch_last_error(const lbdem_batch* batch);
+int lbdem_batch_destroy(lbdem_batch* batch);         /* the contexts remain, usable alone again */
+
+/* ---------------------------------------------------------------------------------------------
  * Run pool (no reference counterpart; batched and served runs of many independent boxes on one
  * GPU). A worker thread admits submitted runs in arrival order while their estimated device bytes
  * fit the HBM budget and fewer than maxConcurrent runs are active, gives every run its own context

@@ -2,6 +2,7 @@
 // reference's initialisation order, LB.cpp:340-398 and DEM.cpp:199-357), device allocation,
 // the goCycle step sequence (main.cpp:38-67) and downloads.
 #include "lbdem_internal.hpp"
+#include "launch.hpp"
 
 #include <algorithm>
 #include <array>
@@ -14,6 +15,11 @@
 #include <vector>
 
 using namespace lbdem;
+
+lbdem::Recorder*& lbdem::current_recorder() {
+    thread_local Recorder* r = nullptr;
+    return r;
+}
 
 namespace {
 
@@ -72,6 +78,7 @@ struct lbdem_ctx {
     hipGraph_t graph[4] = {nullptr, nullptr, nullptr, nullptr};
     hipGraphExec_t graphExec[4] = {nullptr, nullptr, nullptr, nullptr};
     double* xiFirst = nullptr;  // the buffer that is current at initialisation
+    lbdem_batch* batch = nullptr;  // set while the context is advanced through a batch
     // tile tuner: the first LB steps run the fused kernel once per AA parity with each candidate
     // z-walk length, timed with events; the fastest candidate is kept (lbdem_set_tile fixes one)
     static constexpr int NCAND = 4;
@@ -701,11 +708,12 @@ static void enqueue_fs(lbdem_ctx* c) { lb_free_surface(c->box, c->k, c->sc, c->s
     if (!c) return LBDEM_E_ARG;                                             \
     if (!c->initialised) return c->fail(LBDEM_E_STATE, "call lbdem_init first")
 
-extern "C" int lbdem_dem_step(lbdem_ctx* c) { NEED_INIT; enqueue_dem(c); HIPCHK(hipGetLastError()); return 0; }
-extern "C" int lbdem_coupling_step(lbdem_ctx* c) { NEED_INIT; enqueue_coupling(c); HIPCHK(hipGetLastError()); return 0; }
-extern "C" int lbdem_lb_step(lbdem_ctx* c) { NEED_INIT; enqueue_lb(c); HIPCHK(hipGetLastError()); return 0; }
+extern "C" int lbdem_dem_step(lbdem_ctx* c) { NEED_INIT; if (c->batch) return c->fail(LBDEM_E_STATE, "in a batch"); enqueue_dem(c); HIPCHK(hipGetLastError()); return 0; }
+extern "C" int lbdem_coupling_step(lbdem_ctx* c) { NEED_INIT; if (c->batch) return c->fail(LBDEM_E_STATE, "in a batch"); enqueue_coupling(c); HIPCHK(hipGetLastError()); return 0; }
+extern "C" int lbdem_lb_step(lbdem_ctx* c) { NEED_INIT; if (c->batch) return c->fail(LBDEM_E_STATE, "in a batch"); enqueue_lb(c); HIPCHK(hipGetLastError()); return 0; }
 extern "C" int lbdem_fs_step(lbdem_ctx* c) {
     NEED_INIT;
+    if (c->batch) return c->fail(LBDEM_E_STATE, "in a batch");
     if (c->freeSurface) enqueue_fs(c);
     HIPCHK(hipGetLastError());
     return 0;
@@ -790,19 +798,26 @@ static int capture_pair(lbdem_ctx* c) {
     return hipGraphLaunch(ge, c->stream) == hipSuccess ? 1 : -1;
 }
 
+// what two enqueue_step calls do to the host counters (AA parity and history buffers return to
+// their values after a pair)
+static void advance_pair(lbdem_ctx* c) {
+    for (int k = 0; k < 2; ++k) {
+        c->realTime += c->unit.Time;
+        ++c->currentTimeStep;
+        ++c->lbTime;
+        if (c->prm.demSolve) { ++c->demTimeStep; c->demTime += c->deltat; }
+    }
+}
+
 extern "C" int lbdem_cycle(lbdem_ctx* c, int nsteps) {
     NEED_INIT;
+    if (c->batch) return c->fail(LBDEM_E_STATE, "the context is advanced through its batch");
     int s = 0;
     while (s < nsteps) {
         if (nsteps - s >= 2 && graph_ready(c)) {
             if (hipGraphExec_t ge = c->graphExec[graph_phase(c)]) {
                 HIPCHK(hipGraphLaunch(ge, c->stream));
-                for (int k = 0; k < 2; ++k) {  // what enqueue_step does to the host counters
-                    c->realTime += c->unit.Time;
-                    ++c->currentTimeStep;
-                    ++c->lbTime;
-                    if (c->prm.demSolve) { ++c->demTimeStep; c->demTime += c->deltat; }
-                }
+                advance_pair(c);
                 s += 2;
                 continue;
             }
@@ -1096,3 +1111,269 @@ extern "C" int lbdem_profile_read(lbdem_ctx* c, double* fused_ms, int* fused_lau
     if (fused_launches) *fused_launches = int(c->evUsed / 2);
     return 0;
 }
+
+// ---------------------------------------------------------------------------------------------
+// batches: every kernel of a step pair launched once for all member boxes
+// ---------------------------------------------------------------------------------------------
+namespace {
+struct BatchOp {
+    void (*launch)(dim3, dim3, size_t, hipStream_t, const void*);
+    dim3 grid, block;
+    size_t shm;
+    size_t off;  // of the launch's argument table (one entry per member of the group)
+};
+// the phase's step pair, one launch list per group of members with the same launch sequence
+struct BatchPhase {
+    bool recorded = false;
+    std::vector<std::vector<BatchOp>> groups;
+    char* table = nullptr;
+    size_t tableBytes = 0;
+    bool ownTable = false;
+    hipGraph_t graph = nullptr;
+    hipGraphExec_t exec = nullptr;
+};
+}  // namespace
+
+struct lbdem_batch {
+    std::vector<lbdem_ctx*> m;
+    int device = 0;
+    hipStream_t stream = nullptr;
+    hipEvent_t done = nullptr;
+    std::vector<hipEvent_t> mev;
+    BatchPhase ph[4];
+    char* region = nullptr;  // caller-owned memory for the argument tables (else hipMalloc)
+    size_t regionSize = 0, regionUsed = 0;
+    std::string err;
+    int fail(int code, const std::string& e) { err = e; return code; }
+    ~lbdem_batch() {
+        if (stream) (void)hipStreamSynchronize(stream);
+        for (BatchPhase& P : ph) {
+            if (P.exec) (void)hipGraphExecDestroy(P.exec);
+            if (P.graph) (void)hipGraphDestroy(P.graph);
+            if (P.ownTable && P.table) (void)hipFree(P.table);
+        }
+        for (hipEvent_t e : mev) (void)hipEventDestroy(e);
+        if (done) (void)hipEventDestroy(done);
+        if (stream) (void)hipStreamDestroy(stream);
+        for (lbdem_ctx* c : m) c->batch = nullptr;
+    }
+};
+
+// the contexts' start-up is over: the launches of a step pair repeat (see graph_ready)
+static bool steady(const lbdem_ctx* c) {
+    if (!c->initialised || c->profiling || !c->tuned) return false;
+    if (c->prm.demSolve && c->demTimeStep < unsigned(std::max(c->prm.objWaitTime, 0)) + 1u) return false;
+    if (!(c->demTime >= c->prm.demInitialRepeat)) return false;
+    if (!c->forceField && !c->lbFReset) return false;
+    return true;
+}
+
+#define BATCHCHK(x)                                                                                  \
+    do {                                                                                             \
+        hipError_t e_ = (x);                                                                         \
+        if (e_ != hipSuccess) return B->fail(LBDEM_E_DEVICE, std::string("HIP error: ") + hipGetErrorString(e_) + " at " #x); \
+    } while (0)
+
+// records one step pair of every member, groups the members whose launch sequences agree
+// (kernels, blocks, argument sizes) and uploads the argument tables
+static int batch_record(lbdem_batch* B, BatchPhase& P) {
+    const size_t n = B->m.size();
+    std::vector<Recorder> rec(n);
+    std::vector<HostStepState> saved(n);
+    for (size_t i = 0; i < n; ++i) {
+        lbdem_ctx* c = B->m[i];
+        saved[i] = save_host(c);
+        current_recorder() = &rec[i];
+        enqueue_step(c);
+        enqueue_step(c);
+        current_recorder() = nullptr;
+    }
+    auto undo = [&](const std::string& why) {
+        for (size_t i = 0; i < n; ++i) restore_host(B->m[i], saved[i]);
+        return B->fail(LBDEM_E_STATE, why);
+    };
+    auto same_sequence = [&](size_t a, size_t b) {
+        if (rec[a].ops.size() != rec[b].ops.size()) return false;
+        for (size_t j = 0; j < rec[a].ops.size(); ++j) {
+            const RecordedLaunch &x = rec[a].ops[j], &y = rec[b].ops[j];
+            if (x.key != y.key || x.args.size() != y.args.size() || x.block.x != y.block.x ||
+                x.block.y != y.block.y || x.block.z != y.block.z || x.shm != y.shm)
+                return false;
+        }
+        return true;
+    };
+    std::vector<std::vector<size_t>> members;  // group -> member indices, in member order
+    for (size_t i = 0; i < n; ++i) {
+        size_t g = 0;
+        while (g < members.size() && !same_sequence(members[g][0], i)) ++g;
+        if (g == members.size()) members.emplace_back();
+        members[g].push_back(i);
+    }
+    P.groups.assign(members.size(), {});
+    size_t total = 0;
+    for (size_t g = 0; g < members.size(); ++g) {
+        const std::vector<size_t>& mg = members[g];
+        const Recorder& r0 = rec[mg[0]];
+        for (size_t j = 0; j < r0.ops.size(); ++j) {
+            BatchOp op;
+            op.launch = r0.ops[j].batch;
+            op.block = r0.ops[j].block;
+            op.shm = r0.ops[j].shm;
+            unsigned gx = 1;
+            for (size_t i : mg) {
+                const dim3 gr = rec[i].ops[j].grid;
+                if (gr.y != 1 || gr.z != 1) return undo("a kernel uses a multi-dimensional grid");
+                gx = std::max(gx, gr.x);
+            }
+            op.grid = dim3(gx, unsigned(mg.size()), 1);
+            op.off = total;
+            total += (mg.size() * r0.ops[j].args.size() + 255) & ~size_t(255);
+            P.groups[g].push_back(op);
+        }
+    }
+    std::vector<char> host(std::max<size_t>(total, 1));
+    for (size_t g = 0; g < members.size(); ++g)
+        for (size_t j = 0; j < P.groups[g].size(); ++j) {
+            const size_t sz = rec[members[g][0]].ops[j].args.size();
+            for (size_t k = 0; k < members[g].size(); ++k)
+                std::memcpy(host.data() + P.groups[g][j].off + k * sz, rec[members[g][k]].ops[j].args.data(), sz);
+        }
+    if (!P.table || P.tableBytes < total) {
+        if (P.ownTable && P.table) (void)hipFree(P.table);
+        P.table = nullptr;
+        P.ownTable = false;
+        if (B->region && B->regionUsed + total <= B->regionSize) {
+            P.table = B->region + B->regionUsed;
+            B->regionUsed += (total + 255) & ~size_t(255);
+        } else {
+            if (hipMalloc(&P.table, std::max<size_t>(total, 256)) != hipSuccess) {
+                P.table = nullptr;
+                return undo("device allocation of the batch tables failed");
+            }
+            P.ownTable = true;
+        }
+        P.tableBytes = total;
+    }
+    if (total && (hipMemcpyAsync(P.table, host.data(), total, hipMemcpyHostToDevice, B->stream) != hipSuccess ||
+                  hipStreamSynchronize(B->stream) != hipSuccess))
+        return undo("upload of the batch tables failed");
+    P.recorded = true;
+    return 0;
+}
+
+static void batch_launch_ops(lbdem_batch* B, const BatchPhase& P) {
+    for (const std::vector<BatchOp>& ops : P.groups)
+        for (const BatchOp& op : ops) op.launch(op.grid, op.block, op.shm, B->stream, P.table + op.off);
+}
+
+// the recorded pair as a graph (launched); the ops directly when capture is unavailable
+static int batch_run_pair(lbdem_batch* B, BatchPhase& P) {
+    if (!P.exec && !P.graph) {
+        if (hipStreamBeginCapture(B->stream, hipStreamCaptureModeThreadLocal) == hipSuccess) {
+            batch_launch_ops(B, P);
+            hipGraph_t g = nullptr;
+            if (hipStreamEndCapture(B->stream, &g) == hipSuccess && g &&
+                hipGraphInstantiate(&P.exec, g, nullptr, nullptr, 0) == hipSuccess) {
+                P.graph = g;
+            } else {
+                if (g) (void)hipGraphDestroy(g);
+                P.exec = nullptr;
+            }
+        }
+        (void)hipGetLastError();
+    }
+    if (P.exec) BATCHCHK(hipGraphLaunch(P.exec, B->stream));
+    else {
+        batch_launch_ops(B, P);
+        BATCHCHK(hipGetLastError());
+    }
+    return 0;
+}
+
+extern "C" int lbdem_batch_create(lbdem_ctx* const* ctxs, int n, lbdem_batch** out) {
+    if (!ctxs || n < 1 || !out) return LBDEM_E_ARG;
+    *out = nullptr;
+    lbdem_ctx* c0 = ctxs[0];
+    for (int i = 0; i < n; ++i) {
+        lbdem_ctx* c = ctxs[i];
+        if (!c || !c->initialised) return LBDEM_E_STATE;
+        if (c->batch) return c->fail(LBDEM_E_STATE, "the context is already in a batch");
+        if (!steady(c)) return c->fail(LBDEM_E_STATE, "the context has not finished its start-up steps");
+        if (c->box.prec != c0->box.prec || c->prm.device != c0->prm.device || c->freeSurface != c0->freeSurface)
+            return c->fail(LBDEM_E_ARG, "batch members need the same precision, device and free-surface setting");
+        if (graph_phase(c) != graph_phase(c0)) return c->fail(LBDEM_E_STATE, "batch members must be at the same step parity");
+        for (int k = 0; k < i; ++k)
+            if (ctxs[k] == c) return c->fail(LBDEM_E_ARG, "a context appears twice");
+    }
+    lbdem_batch* B = new lbdem_batch();
+    B->m.assign(ctxs, ctxs + n);
+    B->device = c0->prm.device;
+    if (hipSetDevice(B->device) != hipSuccess || hipStreamCreateWithFlags(&B->stream, hipStreamNonBlocking) != hipSuccess ||
+        hipEventCreateWithFlags(&B->done, hipEventDisableTiming) != hipSuccess) {
+        delete B;
+        return LBDEM_E_DEVICE;
+    }
+    B->mev.resize(n, nullptr);
+    for (hipEvent_t& e : B->mev)
+        if (hipEventCreateWithFlags(&e, hipEventDisableTiming) != hipSuccess) { delete B; return LBDEM_E_DEVICE; }
+    // one z-walk tile for all members: the one tuned for the largest box
+    const lbdem_ctx* big = c0;
+    for (lbdem_ctx* c : B->m) if (c->box.g.N > big->box.g.N) big = c;
+    const int fz = big->box.fz;
+    for (lbdem_ctx* c : B->m) {
+        (void)hipStreamSynchronize(c->stream);
+        drop_graphs(c);  // captured with the context's own tile
+        c->box.fz = fz;
+        c->batch = B;
+    }
+    *out = B;
+    return 0;
+}
+
+extern "C" int lbdem_batch_cycle(lbdem_batch* B, int nsteps) {
+    if (!B) return LBDEM_E_ARG;
+    if (nsteps < 0 || nsteps % 2) return B->fail(LBDEM_E_ARG, "a batch advances whole step pairs (nsteps even)");
+    if (nsteps == 0) return 0;
+    const size_t n = B->m.size();
+    for (size_t i = 0; i < n; ++i) {  // after each member's own pending work
+        BATCHCHK(hipEventRecord(B->mev[i], B->m[i]->stream));
+        BATCHCHK(hipStreamWaitEvent(B->stream, B->mev[i], 0));
+    }
+    for (int s = 0; s < nsteps; s += 2) {
+        BatchPhase& P = B->ph[graph_phase(B->m[0])];
+        if (!P.recorded) {
+            if (int e = batch_record(B, P)) return e;  // advances the members' counters
+        } else {
+            for (lbdem_ctx* c : B->m) advance_pair(c);
+        }
+        if (int e = batch_run_pair(B, P)) return e;
+    }
+    BATCHCHK(hipEventRecord(B->done, B->stream));
+    for (lbdem_ctx* c : B->m) BATCHCHK(hipStreamWaitEvent(c->stream, B->done, 0));
+    return 0;
+}
+
+extern "C" int lbdem_batch_synchronize(lbdem_batch* B) {
+    if (!B) return LBDEM_E_ARG;
+    BATCHCHK(hipStreamSynchronize(B->stream));
+    for (lbdem_ctx* c : B->m)
+        if (int e = check_errors(c)) return B->fail(e, c->err);
+    return 0;
+}
+
+extern "C" const char* lbdem_batch_last_error(const lbdem_batch* B) { return B ? B->err.c_str() : "null batch"; }
+
+extern "C" int lbdem_batch_destroy(lbdem_batch* B) {
+    delete B;
+    return 0;
+}
+
+// run pool hook: the batch's argument tables in caller-owned device memory
+int lbdem::batch_use_region(lbdem_batch* B, void* base, size_t bytes) {
+    if (!B) return LBDEM_E_ARG;
+    B->region = static_cast<char*>(base);
+    B->regionSize = bytes;
+    B->regionUsed = 0;
+    return 0;
+}
+hipStream_t lbdem::batch_stream(const lbdem_batch* B) { return B ? B->stream : nullptr; }

@@ -9,6 +9,7 @@
 //     is read from the previous buffer and written once, by the thread of the higher index
 //   * integration and periodic wrap (DEM.cpp:2676-2836)
 #include "lbdem_internal.hpp"
+#include "launch.hpp"
 
 #include <algorithm>
 #include <climits>
@@ -32,10 +33,11 @@ __device__ inline V sub(V a, V b) { return V{a.x - b.x, a.y - b.y, a.z - b.z}; }
 // ------------------------------------------------------------------------------------------
 // maximum |v|^2 over particles: per-block maxima into blockMax (non-negative doubles order like
 // their bit patterns), then the trigger update by one thread
-__global__ void k_maxdisp_partial(const double* x1, int n, unsigned long long* blockMax) {
+__device__ inline void k_maxdisp_partial(const double* x1, int n, int nb, unsigned long long* blockMax) {
+    if (int(blockIdx.x) >= nb) return;  // nb: the box's grid (k_maxdisp reads nb partials)
     __shared__ double red[256];
     double m = 0.0;
-    for (int p = blockIdx.x * blockDim.x + threadIdx.x; p < n; p += gridDim.x * blockDim.x) {
+    for (int p = blockIdx.x * blockDim.x + threadIdx.x; p < n; p += nb * blockDim.x) {
         const double v2 = x1[3 * p] * x1[3 * p] + x1[3 * p + 1] * x1[3 * p + 1] + x1[3 * p + 2] * x1[3 * p + 2];
         if (v2 > m) m = v2;
     }
@@ -47,7 +49,7 @@ __global__ void k_maxdisp_partial(const double* x1, int n, unsigned long long* b
     }
     if (threadIdx.x == 0) blockMax[blockIdx.x] = __double_as_longlong(red[0]);
 }
-__global__ void k_maxdisp(const unsigned long long* blockMax, int nb, double deltat, DevScalars* sc, int force) {
+__device__ inline void k_maxdisp(const unsigned long long* blockMax, int nb, double deltat, DevScalars* sc, int force) {
     if (threadIdx.x != 0) return;
     if (force) {
         sc->rebuild = 1;
@@ -90,14 +92,14 @@ struct NbrArgs {
     DevScalars* sc;
 };
 
-__global__ void k_cells_clear(NbrArgs a, int ncell) {
+__device__ inline void k_cells_clear(NbrArgs a, int ncell) {
     if (!a.sc->rebuild) return;
     for (int c = blockIdx.x * blockDim.x + threadIdx.x; c < ncell; c += gridDim.x * blockDim.x) {
         a.cellCount[c] = 0;
         a.cellFill[c] = 0;
     }
 }
-__global__ void k_cells_assign(NbrArgs a) {
+__device__ inline void k_cells_assign(NbrArgs a) {
     if (!a.sc->rebuild) return;
     const int p = blockIdx.x * blockDim.x + threadIdx.x;
     if (p >= a.p.nPart) return;
@@ -110,7 +112,7 @@ __global__ void k_cells_assign(NbrArgs a) {
     a.cellKey[p] = key;
     atomicAdd(&a.cellCount[key], 1);
 }
-__global__ void k_cells_scan(NbrArgs a, int ncell) {
+__device__ inline void k_cells_scan(NbrArgs a, int ncell) {
     if (!a.sc->rebuild) return;
     __shared__ int part[1024];
     const int per = (ncell + blockDim.x - 1) / blockDim.x;
@@ -127,7 +129,7 @@ __global__ void k_cells_scan(NbrArgs a, int ncell) {
     int acc = part[threadIdx.x];
     for (int c = b; c < e; ++c) { a.cellStart[c] = acc; acc += a.cellCount[c]; }
 }
-__global__ void k_cells_fill(NbrArgs a) {
+__device__ inline void k_cells_fill(NbrArgs a) {
     if (!a.sc->rebuild) return;
     const int p = blockIdx.x * blockDim.x + threadIdx.x;
     if (p >= a.p.nPart) return;
@@ -148,7 +150,7 @@ __device__ inline bool within_cut(const NbrArgs& a, int n1, int n2) {
 }
 
 // per-particle partner gather (pairs with minimum-image distance within sig + 2 in all axes)
-__global__ void k_nbr_build(NbrArgs a) {
+__device__ inline void k_nbr_build(NbrArgs a) {
     if (!a.sc->rebuild) return;
     const int p = blockIdx.x * blockDim.x + threadIdx.x;
     if (p >= a.p.nPart) return;
@@ -189,7 +191,7 @@ __global__ void k_nbr_build(NbrArgs a) {
 
 // slot of every entry (history lives in the list of the higher index), history carried over by
 // partner id (DEM.cpp:633-651); a partner absent from the old list starts with zero history
-__global__ void k_nbr_slots(NbrArgs a) {
+__device__ inline void k_nbr_slots(NbrArgs a) {
     if (!a.sc->rebuild) return;
     const int p = blockIdx.x * blockDim.x + threadIdx.x;
     if (p >= a.p.nPart) return;
@@ -222,7 +224,7 @@ __global__ void k_nbr_slots(NbrArgs a) {
         a.nbrSlot[p * NBR_CAP + k] = slot;
     }
 }
-__global__ void k_nbr_commit(NbrArgs a, double* xiCurMut, int* oldCount, int* oldList) {
+__device__ inline void k_nbr_commit(NbrArgs a, double* xiCurMut, int* oldCount, int* oldList) {
     if (!a.sc->rebuild) return;
     const int p = blockIdx.x * blockDim.x + threadIdx.x;
     if (p >= a.p.nPart) return;
@@ -382,7 +384,7 @@ __device__ inline V group_sum(V v) { return V{group_sum(v.x), group_sum(v.y), gr
 // Forces on particle p: DEM_LANES lanes per particle share its Verlet partners (lane l takes
 // partners l, l + DEM_LANES, ...), their partial sums are combined in a fixed order, and the
 // group's first lane adds the wall and impactor terms and writes the results.
-__global__ void k_dem_forces(ForceArgs a, double dampCoeff) {
+__device__ inline void k_dem_forces(ForceArgs a, double dampCoeff) {
 #pragma clang fp contract(off)
     const int gt = blockIdx.x * blockDim.x + threadIdx.x;
     const int lane = gt % DEM_LANES;
@@ -606,18 +608,19 @@ __global__ void k_dem_forces(ForceArgs a, double dampCoeff) {
 // Deterministic column sums of the per-particle object and wall contributions (row-major,
 // ncol columns), in two levels: block b sums rows [b*rows, (b+1)*rows) of every column
 // (threads stride over the flat range, so reads are coalesced) into part[b][c]; one block then
-// sums the block partials of each column in block order. blockIdx.y selects objects (0) or
-// walls (1).
+// sums the block partials of each column in block order. Blocks [0, RED_BLOCKS) take the
+// objects, [RED_BLOCKS, 2 RED_BLOCKS) the walls.
 constexpr int RED_BLOCKS = 64, RED_BS = 256, RED_MAXCOL = 12 * MAX_OBJECTS;
-__global__ __launch_bounds__(RED_BS) void k_dem_reduce_a(const double* oc, int nco, const double* wcn, int ncw,
-                                                        int nPart, double* part) {
+__device__ inline void k_dem_reduce_a(const double* oc, int nco, const double* wcn, int ncw, int nPart,
+                                      double* part) {
     __shared__ double red[RED_BS];
-    const double* contrib = blockIdx.y == 0 ? oc : wcn;
-    const int ncol = blockIdx.y == 0 ? nco : ncw;
+    const int role = int(blockIdx.x) / RED_BLOCKS, bx = int(blockIdx.x) % RED_BLOCKS;
+    const double* contrib = role == 0 ? oc : wcn;
+    const int ncol = role == 0 ? nco : ncw;
     if (ncol == 0) return;
     const int rows = (nPart + RED_BLOCKS - 1) / RED_BLOCKS;
-    const int64_t beg = int64_t(blockIdx.x) * rows * ncol;
-    const int64_t end = min(int64_t(nPart), int64_t(blockIdx.x + 1) * rows) * ncol;
+    const int64_t beg = int64_t(bx) * rows * ncol;
+    const int64_t end = min(int64_t(nPart), int64_t(bx + 1) * rows) * ncol;
     // thread t handles flat indices beg + t + k*step with step a multiple of ncol, so its column
     // is fixed: c = (t + beg) % ncol
     const int step = (RED_BS / ncol) * ncol;
@@ -630,10 +633,10 @@ __global__ __launch_bounds__(RED_BS) void k_dem_reduce_a(const double* oc, int n
     if (t < ncol) {
         double s = 0.0;
         for (int u = t; u < step; u += ncol) s += red[u];
-        part[(int64_t(blockIdx.y) * RED_BLOCKS + blockIdx.x) * RED_MAXCOL + t] = s;
+        part[(int64_t(role) * RED_BLOCKS + bx) * RED_MAXCOL + t] = s;
     }
 }
-__global__ void k_dem_reduce_b(const double* part, int nco, int ncw, double* objSum, double* wallSum) {
+__device__ inline void k_dem_reduce_b(const double* part, int nco, int ncw, double* objSum, double* wallSum) {
     const int t = threadIdx.x;
     if (t < nco) {
         double s = 0.0;
@@ -648,7 +651,7 @@ __global__ void k_dem_reduce_b(const double* part, int nco, int ncw, double* obj
 }
 
 // objects: totals and integration (DEM.cpp:786-790, 2742-2758); walls: totals (DEM.cpp:773-775)
-__global__ void k_dem_objects(DemParams P, DemState d, const double* objSum, const double* wallSum, int demTimeStep) {
+__device__ inline void k_dem_objects(DemParams P, DemState d, const double* objSum, const double* wallSum, int demTimeStep) {
 #pragma clang fp contract(off)
     if (threadIdx.x != 0 || blockIdx.x != 0) return;
     for (int o = 0; o < P.nObj; ++o) {
@@ -682,7 +685,7 @@ __global__ void k_dem_objects(DemParams P, DemState d, const double* objSum, con
 }
 
 // particles: totals, gravity switch-off, integration, periodic wrap, lattice view
-__global__ void k_dem_integrate(DemParams P, DemState d, int demTimeStep) {
+__device__ inline void k_dem_integrate(DemParams P, DemState d, int demTimeStep) {
 #pragma clang fp contract(off)
     const int p = blockIdx.x * blockDim.x + threadIdx.x;
     if (p >= P.nPart) return;
@@ -711,7 +714,7 @@ __global__ void k_dem_integrate(DemParams P, DemState d, int demTimeStep) {
     d.plb[p] = v;
 }
 
-__global__ void k_dem_views(DemParams P, DemState d) {
+__device__ inline void k_dem_views(DemParams P, DemState d) {
     const int p = blockIdx.x * blockDim.x + threadIdx.x;
     if (p < P.nPart) {
         PartLB v;
@@ -732,13 +735,13 @@ __global__ void k_dem_views(DemParams P, DemState d) {
     }
 }
 
-__global__ void k_zero(double* p, int n) {
+__device__ inline void k_zero(double* p, int n) {
     for (int i = threadIdx.x; i < n; i += blockDim.x) p[i] = 0.0;
 }
 
 void dem_refresh_views(const DemParams& p, DemState& d, hipStream_t s) {
     const int n = p.nPart > p.nObj ? p.nPart : p.nObj;
-    if (n > 0) hipLaunchKernelGGL(k_dem_views, (n + 255) / 256, 256, 0, s, p, d);
+    if (n > 0) launch<&k_dem_views, 1024, 1>((n + 255) / 256, 256, s, p, d);
 }
 
 static NbrArgs nbr_args(const DemParams& p, DemState& d, DevScalars* sc) {
@@ -752,21 +755,21 @@ static NbrArgs nbr_args(const DemParams& p, DemState& d, DevScalars* sc) {
 
 void dem_neighbor_list(const DemParams& p, DemState& d, DevScalars* sc, bool force, hipStream_t s) {
     const int nbm = p.nPart > 0 ? std::min(64, (p.nPart + 255) / 256) : 1;
-    hipLaunchKernelGGL(k_maxdisp_partial, nbm, 256, 0, s, d.x1, p.nPart, d.blockMax);
-    hipLaunchKernelGGL(k_maxdisp, 1, 64, 0, s, d.blockMax, nbm, p.deltat, sc, force ? 1 : 0);
+    launch<&k_maxdisp_partial, 1024, 1>(nbm, 256, s, d.x1, p.nPart, nbm, d.blockMax);
+    launch<&k_maxdisp, 1024, 1>(1, 64, s, d.blockMax, nbm, p.deltat, sc, force ? 1 : 0);
     if (p.nPart <= 0) return;
     NbrArgs a = nbr_args(p, d, sc);
     const int pb = (p.nPart + 127) / 128;
     if (p.nlx > 0) {
         const int ncell = p.nlx * p.nly * p.nlz;
-        hipLaunchKernelGGL(k_cells_clear, (ncell + 255) / 256, 256, 0, s, a, ncell);
-        hipLaunchKernelGGL(k_cells_assign, pb, 128, 0, s, a);
-        hipLaunchKernelGGL(k_cells_scan, 1, 1024, 0, s, a, ncell);
-        hipLaunchKernelGGL(k_cells_fill, pb, 128, 0, s, a);
+        launch<&k_cells_clear, 1024, 1>((ncell + 255) / 256, 256, s, a, ncell);
+        launch<&k_cells_assign, 1024, 1>(pb, 128, s, a);
+        launch<&k_cells_scan, 1024, 1>(1, 1024, s, a, ncell);
+        launch<&k_cells_fill, 1024, 1>(pb, 128, s, a);
     }
-    hipLaunchKernelGGL(k_nbr_build, pb, 128, 0, s, a);
-    hipLaunchKernelGGL(k_nbr_slots, pb, 128, 0, s, a);
-    hipLaunchKernelGGL(k_nbr_commit, pb, 128, 0, s, a, d.xiCur, d.nbrCountOld, d.nbrListOld);
+    launch<&k_nbr_build, 1024, 1>(pb, 128, s, a);
+    launch<&k_nbr_slots, 1024, 1>(pb, 128, s, a);
+    launch<&k_nbr_commit, 1024, 1>(pb, 128, s, a, d.xiCur, d.nbrCountOld, d.nbrListOld);
 }
 
 void dem_step(const DemParams& p, DemState& d, DevScalars* sc, int demTimeStep, hipStream_t s) {
@@ -778,19 +781,18 @@ void dem_step(const DemParams& p, DemState& d, DevScalars* sc, int demTimeStep, 
         f.xiOld = d.xiCur; f.xiNew = d.xiNext;
         f.ox0 = d.ox0; f.ox1 = d.ox1; f.ow1 = d.ow1; f.or_ = d.or_;
         f.objContrib = d.objContrib; f.wallContrib = d.wallContrib; f.demTimeStep = demTimeStep;
-        hipLaunchKernelGGL(k_dem_forces, (p.nPart * DEM_LANES + 255) / 256, 256, 0, s, f, p.dampCoeff);
+        launch<&k_dem_forces, 1024, 1>((p.nPart * DEM_LANES + 255) / 256, 256, s, f, p.dampCoeff);
         if (p.nObj > 0 || p.nWall > 0) {
-            hipLaunchKernelGGL(k_dem_reduce_a, dim3(RED_BLOCKS, 2), RED_BS, 0, s, d.objContrib, 12 * p.nObj, d.wallContrib,
-                               6 * p.nWall, p.nPart, d.redPart);
-            hipLaunchKernelGGL(k_dem_reduce_b, 1, 128, 0, s, d.redPart, 12 * p.nObj, 6 * p.nWall, d.objSum, d.wallSum);
+            launch<&k_dem_reduce_a, RED_BS, 1>(2 * RED_BLOCKS, RED_BS, s, d.objContrib, 12 * p.nObj, d.wallContrib, 6 * p.nWall, p.nPart, d.redPart);
+            launch<&k_dem_reduce_b, 1024, 1>(1, 128, s, d.redPart, 12 * p.nObj, 6 * p.nWall, d.objSum, d.wallSum);
         }
         double* t = d.xiCur; d.xiCur = d.xiNext; d.xiNext = t;
     } else {
-        hipLaunchKernelGGL(k_zero, 1, 128, 0, s, d.objSum, 12 * MAX_OBJECTS);
-        hipLaunchKernelGGL(k_zero, 1, 128, 0, s, d.wallSum, 6 * MAX_WALLS);
+        launch<&k_zero, 1024, 1>(1, 128, s, d.objSum, 12 * MAX_OBJECTS);
+        launch<&k_zero, 1024, 1>(1, 128, s, d.wallSum, 6 * MAX_WALLS);
     }
-    hipLaunchKernelGGL(k_dem_objects, 1, 64, 0, s, p, d, d.objSum, d.wallSum, demTimeStep);
-    if (p.nPart > 0) hipLaunchKernelGGL(k_dem_integrate, (p.nPart + 127) / 128, 128, 0, s, p, d, demTimeStep);
+    launch<&k_dem_objects, 1024, 1>(1, 64, s, p, d, d.objSum, d.wallSum, demTimeStep);
+    if (p.nPart > 0) launch<&k_dem_integrate, 1024, 1>((p.nPart + 127) / 128, 128, s, p, d, demTimeStep);
 }
 
 }  // namespace lbdem

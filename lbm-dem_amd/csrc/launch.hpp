@@ -1,0 +1,90 @@
+// Kernel launches for one box or for a batch of boxes.
+//
+// Every kernel is written once, as a __device__ body. kernel_one passes the body's arguments as
+// kernel arguments (one box); kernel_batch reads the arguments of box blockIdx.y from a device
+// table, so one launch advances every box of a batch (grid.x: the largest per-box grid; a body
+// whose work depends on its grid takes the per-box grid as an argument and lets surplus blocks
+// return). launch<> either launches kernel_one or, while a Recorder is installed on the calling
+// thread, appends the launch (kernel, grid, block, argument bytes) to it; a batch records the same
+// step on every box and replays the records with kernel_batch.
+#pragma once
+
+#include <hip/hip_runtime.h>
+
+#include <cstring>
+#include <type_traits>
+#include <utility>
+#include <vector>
+
+namespace lbdem {
+
+// the arguments of one launch: an aggregate of trivially copyable members (copied as bytes)
+template <size_t I, typename T>
+struct PackLeaf {
+    T v;
+};
+template <typename Seq, typename... P>
+struct PackImpl;
+template <size_t... I, typename... P>
+struct PackImpl<std::index_sequence<I...>, P...> : PackLeaf<I, P>... {};
+template <typename... P>
+using Pack = PackImpl<std::index_sequence_for<P...>, P...>;
+
+template <auto Body, typename... P, size_t... I>
+__device__ inline void apply_pack(const PackImpl<std::index_sequence<I...>, P...>& t) {
+    Body(static_cast<const PackLeaf<I, P>&>(t).v...);
+}
+
+template <auto Body, int MAXT, int MINW, typename... P>
+__global__ __launch_bounds__(MAXT, MINW) void kernel_one(P... args) {
+    Body(args...);
+}
+
+template <auto Body, int MAXT, int MINW, typename... P>
+__global__ __launch_bounds__(MAXT, MINW) void kernel_batch(const Pack<P...>* tab) {
+    apply_pack<Body, P...>(tab[blockIdx.y]);
+}
+
+// one recorded launch of one box
+struct RecordedLaunch {
+    const void* key;  // identity of the kernel (body and bounds)
+    void (*batch)(dim3 grid, dim3 block, size_t shm, hipStream_t s, const void* table);
+    dim3 grid, block;
+    size_t shm;
+    std::vector<unsigned char> args;  // the body's argument tuple, as bytes
+};
+struct Recorder {
+    std::vector<RecordedLaunch> ops;
+};
+Recorder*& current_recorder();  // thread-local; null: launches execute
+
+template <auto Body, int MAXT, int MINW, typename Sig = decltype(Body)>
+struct Launcher;
+template <auto Body, int MAXT, int MINW, typename... P>
+struct Launcher<Body, MAXT, MINW, void (*)(P...)> {
+    using Tuple = Pack<P...>;
+    static_assert(std::is_trivially_copyable<Tuple>::value, "kernel arguments are copied as bytes");
+    static inline const char tag = 0;
+    static void batch(dim3 g, dim3 b, size_t shm, hipStream_t s, const void* table) {
+        hipLaunchKernelGGL((kernel_batch<Body, MAXT, MINW, P...>), g, b, shm, s, static_cast<const Tuple*>(table));
+    }
+    template <typename... A>
+    static void run(dim3 g, dim3 b, size_t shm, hipStream_t s, A&&... a) {
+        if (Recorder* r = current_recorder()) {
+            const Tuple t{{static_cast<P>(std::forward<A>(a))}...};
+            RecordedLaunch op{&tag, &batch, g, b, shm, std::vector<unsigned char>(sizeof(Tuple))};
+            std::memcpy(op.args.data(), &t, sizeof(Tuple));
+            r->ops.push_back(std::move(op));
+            return;
+        }
+        hipLaunchKernelGGL((kernel_one<Body, MAXT, MINW, P...>), g, b, shm, s, static_cast<P>(std::forward<A>(a))...);
+    }
+};
+
+// launch<&body, max threads per block, min waves per SIMD>(grid, block, stream, args...)
+template <auto Body, int MAXT = 1024, int MINW = 1, typename... A>
+inline void launch(dim3 grid, dim3 block, hipStream_t s, A&&... a) {
+    Launcher<Body, MAXT, MINW>::run(grid, block, 0, s, std::forward<A>(a)...);
+}
+
+}  // namespace lbdem

@@ -6,6 +6,7 @@
 //   * particle flags and the moving-object (impactor) node update (LB.cpp:683-711, 797-943,
 //     1875-1899, 1980-2029; findNewSolid, LB.cpp:2032-2100, changes no flag)
 #include "lbdem_internal.hpp"
+#include "launch.hpp"
 
 #include <climits>
 
@@ -93,7 +94,7 @@ __device__ inline void equilibrium_offsets(C n, C ux, C uy, C uz, C* geq) {
 // initial populations (node::initialize, node.cpp:26-73) for fluid and interface nodes
 // ------------------------------------------------------------------------------------------
 template <typename S>
-__global__ void k_init_fields(Geom g, S* f, const uint8_t* type, typename Types<S>::M* mass,
+__device__ inline void k_init_fields(Geom g, S* f, const uint8_t* type, typename Types<S>::M* mass,
                               typename Types<S>::C* aux, Consts k) {
     using C = typename Types<S>::C;
     const int64_t i = int64_t(blockIdx.x) * blockDim.x + threadIdx.x;
@@ -122,11 +123,11 @@ void lb_init_fields(const Box& b, const Consts& k, hipStream_t s) {
     const int bs = 256;
     const int nb = int((b.g.N + bs - 1) / bs);
     if (b.prec == LBDEM_PREC_FP64)
-        hipLaunchKernelGGL(k_init_fields<double>, nb, bs, 0, s, b.g, (double*)b.f, b.type, (double*)b.mass, (double*)b.aux, k);
+        launch<&k_init_fields<double>, 1024, 1>(nb, bs, s, b.g, (double*)b.f, b.type, (double*)b.mass, (double*)b.aux, k);
     else if (b.prec == LBDEM_PREC_FP32)
-        hipLaunchKernelGGL(k_init_fields<float>, nb, bs, 0, s, b.g, (float*)b.f, b.type, (float*)b.mass, (float*)b.aux, k);
+        launch<&k_init_fields<float>, 1024, 1>(nb, bs, s, b.g, (float*)b.f, b.type, (float*)b.mass, (float*)b.aux, k);
     else
-        hipLaunchKernelGGL(k_init_fields<__half>, nb, bs, 0, s, b.g, (__half*)b.f, b.type, (float*)b.mass, (float*)b.aux, k);
+        launch<&k_init_fields<__half>, 1024, 1>(nb, bs, s, b.g, (__half*)b.f, b.type, (float*)b.mass, (float*)b.aux, k);
 }
 
 // ------------------------------------------------------------------------------------------
@@ -159,6 +160,7 @@ struct FusedArgs {
     const uint64_t* bcm;
     DevScalars* sc;
     int nbx, nby, nbz;
+    int gridBlocks;  // the box's grid (a batch launch covers the largest box; the rest return)
     Consts k;
 };
 
@@ -462,10 +464,11 @@ template <typename S> struct FusedOcc { static constexpr int value = 3; };
 template <> struct FusedOcc<double> { static constexpr int value = 2; };
 
 template <typename S, int PAR, bool FSURF, bool DIAG, int FZ>
-__global__ __launch_bounds__(FX* FY, FusedOcc<S>::value) void k_fused(FusedArgs<S> a) {
+__device__ inline void k_fused(FusedArgs<S> a) {
     using C = typename Types<S>::C;
     using M = typename Types<S>::M;
     constexpr int NT = FX * FY;
+    if (int(blockIdx.x) >= a.gridBlocks) return;
     __shared__ int hkey[HS];
     __shared__ double hval[HS][7];
     __shared__ double sPart[PART_STRIDE];
@@ -662,7 +665,7 @@ __global__ __launch_bounds__(FX* FY, FusedOcc<S>::value) void k_fused(FusedArgs<
     }
     // object / wall partials: written only by blocks that have any, flagged for the reduction
     const int any = __syncthreads_or(tid < PART_STRIDE && sPart[tid] != 0.0);
-    if (any && tid < PART_STRIDE) a.blockPart[int64_t(tid) * gridDim.x + b0] = sPart[tid];
+    if (any && tid < PART_STRIDE) a.blockPart[int64_t(tid) * a.gridBlocks + b0] = sPart[tid];
     if (tid == 0) a.blockFlag[b0] = any;
 }
 
@@ -688,7 +691,7 @@ __device__ inline uint32_t inactive_bytes(uint32_t w) {
     return m & 0xFu;
 }
 constexpr int MARK_GRID = 4096;
-__global__ void k_mark_full_w(Geom g, uint8_t* type, const int32_t* solid, uint64_t* bcm, const DevScalars* sc) {
+__device__ inline void k_mark_full_w(Geom g, uint8_t* type, const int32_t* solid, uint64_t* bcm, const DevScalars* sc) {
     if (!sc->fullMark) return;
     const int W = g.X / 4;
     const uint32_t* t32 = reinterpret_cast<const uint32_t*>(type);
@@ -753,14 +756,14 @@ __device__ inline void mark_node(const Geom& g, uint8_t* type, const int32_t* so
     if (nt != t) type[i] = nt;
 }
 // full pass for rows whose length is not a multiple of four: one thread per node
-__global__ void k_mark_full_b(Geom g, uint8_t* type, const int32_t* solid, uint64_t* bcm, const DevScalars* sc) {
+__device__ inline void k_mark_full_b(Geom g, uint8_t* type, const int32_t* solid, uint64_t* bcm, const DevScalars* sc) {
     if (!sc->fullMark) return;
     for (int64_t i = int64_t(blockIdx.x) * blockDim.x + threadIdx.x; i < g.N; i += int64_t(gridDim.x) * blockDim.x)
         mark_node(g, type, solid, bcm, int32_t(i));
 }
 // incremental pass: the stencil neighbourhood (wrapped) of every node on the change list. A node
 // reached from several changed nodes is marked several times with the same result.
-__global__ void k_remark(Geom g, uint8_t* type, const int32_t* solid, uint64_t* bcm, const int32_t* changed, int cap,
+__device__ inline void k_remark(Geom g, uint8_t* type, const int32_t* solid, uint64_t* bcm, const int32_t* changed, int cap,
                          const DevScalars* sc) {
     if (sc->fullMark) return;
     const int n = min(sc->nChanged, cap);
@@ -775,16 +778,16 @@ __global__ void k_remark(Geom g, uint8_t* type, const int32_t* solid, uint64_t* 
         if (p >= 0 && p < g.N) mark_node(g, type, solid, bcm, int32_t(p));  // (changed boundary-layer nodes)
     }
 }
-__global__ void k_mark_end(DevScalars* sc) {
+__device__ inline void k_mark_end(DevScalars* sc) {
     sc->fullMark = 0;
     sc->nChanged = 0;
 }
 
 void lb_mark_edges(const Box& b, DevScalars* sc, hipStream_t s) {
-    if (b.g.X % 4 == 0) hipLaunchKernelGGL(k_mark_full_w, MARK_GRID, 256, 0, s, b.g, b.type, b.solid, b.bcm, sc);
-    else hipLaunchKernelGGL(k_mark_full_b, MARK_GRID, 256, 0, s, b.g, b.type, b.solid, b.bcm, sc);
-    hipLaunchKernelGGL(k_remark, 256, 256, 0, s, b.g, b.type, b.solid, b.bcm, b.changed, b.changedCap, sc);
-    hipLaunchKernelGGL(k_mark_end, 1, 1, 0, s, sc);
+    if (b.g.X % 4 == 0) launch<&k_mark_full_w, 1024, 1>(MARK_GRID, 256, s, b.g, b.type, b.solid, b.bcm, sc);
+    else launch<&k_mark_full_b, 1024, 1>(MARK_GRID, 256, s, b.g, b.type, b.solid, b.bcm, sc);
+    launch<&k_remark, 1024, 1>(256, 256, s, b.g, b.type, b.solid, b.bcm, b.changed, b.changedCap, sc);
+    launch<&k_mark_end, 1024, 1>(1, 1, s, sc);
 }
 
 int fused_grid_blocks(const Geom& g, int fz) {
@@ -804,10 +807,11 @@ static void launch_fused(Box& b, const Consts& k, const PartLB* plb, const ObjLB
     const int fz = b.fz;
     a.nbx = (b.g.X + FX - 1) / FX; a.nby = (b.g.Y + FY - 1) / FY; a.nbz = (b.g.Z + fz - 1) / fz;
     const int grid = fused_grid_blocks(b.g, fz);
+    a.gridBlocks = grid;
     const bool fs = k.freeSurface != 0;
     if (ev0) (void)hipEventRecord(ev0, s);
     (void)diag;  // the wall diagnostics are always accumulated
-#define LBDEM_FUSED(P, F, Z) hipLaunchKernelGGL((k_fused<S, P, F, true, Z>), dim3(grid), dim3(FX, FY, 1), 0, s, a)
+#define LBDEM_FUSED(P, F, Z) launch<&k_fused<S, P, F, true, Z>, FX* FY, FusedOcc<S>::value>(dim3(grid), dim3(FX, FY, 1), s, a)
 #define LBDEM_FUSED_Z(P, F)                 \
     switch (fz) {                          \
         case 1: LBDEM_FUSED(P, F, 1); break; \
@@ -826,7 +830,7 @@ static void launch_fused(Box& b, const Consts& k, const PartLB* plb, const ObjLB
     b.parity ^= 1;
 }
 
-__global__ void k_begin_lb(DevScalars* sc, double* hacc, int nPart) {
+__device__ inline void k_begin_lb(DevScalars* sc, double* hacc, int nPart) {
     const int i = blockIdx.x * blockDim.x + threadIdx.x;
     if (i == 0) sc->nIf = 0;
     if (i < nPart * 8) hacc[i] = 0.0;
@@ -835,7 +839,7 @@ __global__ void k_begin_lb(DevScalars* sc, double* hacc, int nPart) {
 void lb_fused_step(Box& b, const Consts& k, const PartLB* plb, const ObjLB* olb, DevScalars* sc, bool diag,
                    hipStream_t s, hipEvent_t ev0, hipEvent_t ev1) {
     const int n = k.nPart * 8 + 1;
-    hipLaunchKernelGGL(k_begin_lb, (n + 255) / 256, 256, 0, s, sc, b.hacc, k.nPart);
+    launch<&k_begin_lb, 1024, 1>((n + 255) / 256, 256, s, sc, b.hacc, k.nPart);
     if (b.prec == LBDEM_PREC_FP64) launch_fused<double>(b, k, plb, olb, sc, diag, s, ev0, ev1);
     else if (b.prec == LBDEM_PREC_FP32) launch_fused<float>(b, k, plb, olb, sc, diag, s, ev0, ev1);
     else launch_fused<__half>(b, k, plb, olb, sc, diag, s, ev0, ev1);
@@ -844,7 +848,7 @@ void lb_fused_step(Box& b, const Consts& k, const PartLB* plb, const ObjLB* olb,
 // deterministic sum of the flagged per-block partials, unit conversion of all hydrodynamic results
 // (LB.cpp:1966-1977, 1536-1543)
 constexpr int HF_THREADS = 1024;
-__global__ __launch_bounds__(HF_THREADS) void k_hydro_finalize(int nblk, const double* part, const int32_t* flag, Consts k, const double* hacc, double* FHydro,
+__device__ inline void k_hydro_finalize(int nblk, const double* part, const int32_t* flag, Consts k, const double* hacc, double* FHydro,
                                  double* MHydro, double* fluidVol, DevScalars* sc, double* objFH, double* objMH,
                                  double* wallFH, int diag) {
     __shared__ double red[HF_THREADS / 64];
@@ -895,8 +899,7 @@ void lb_hydro_finalize(const Box& b, const Consts& k, double* FHydro, double* MH
                        DevScalars* sc, double* objFH, double* objMH, double* wallFH, bool diag, hipStream_t s) {
     const int nblk = fused_grid_blocks(b.g, b.fz);
     const int grid = PART_STRIDE + (k.nPart + HF_THREADS - 1) / HF_THREADS;
-    hipLaunchKernelGGL(k_hydro_finalize, grid, HF_THREADS, 0, s, nblk, b.blockPart, b.blockFlag, k, b.hacc, FHydro, MHydro, fluidVol,
-                       sc, objFH, objMH, wallFH, diag ? 1 : 0);
+    launch<&k_hydro_finalize, HF_THREADS, 1>(grid, HF_THREADS, s, nblk, b.blockPart, b.blockFlag, k, b.hacc, FHydro, MHydro, fluidVol, sc, objFH, objMH, wallFH, diag ? 1 : 0);
 }
 
 // ------------------------------------------------------------------------------------------
@@ -905,7 +908,7 @@ void lb_hydro_finalize(const Box& b, const Consts& k, double* FHydro, double* MH
 constexpr int LIST_GRID = 1024;
 
 template <typename S, int L>
-__global__ void k_fs_mass(Geom g, const S* f, const uint8_t* type, const typename Types<S>::M* mass,
+__device__ inline void k_fs_mass(Geom g, const S* f, const uint8_t* type, const typename Types<S>::M* mass,
                           const typename Types<S>::C* aux, const int32_t* ifList, typename Types<S>::M* newMass,
                           DevScalars* sc, int32_t* filledFlag) {
     using C = typename Types<S>::C;
@@ -950,7 +953,7 @@ __device__ inline void list_push(int32_t* list, int* count, int cap, int32_t v, 
 
 // assign new mass, classify filled / emptied (findInterfaceMutants, LB.cpp:1669-1698)
 template <typename S>
-__global__ void k_fs_mutants(uint8_t* type, typename Types<S>::M* mass, const typename Types<S>::C* aux, int64_t N,
+__device__ inline void k_fs_mutants(uint8_t* type, typename Types<S>::M* mass, const typename Types<S>::C* aux, int64_t N,
                              const int32_t* ifList, const typename Types<S>::M* newMass, int32_t* filled,
                              int32_t* emptied, int cap, DevScalars* sc, int32_t* marker, int32_t* changed,
                              int changedCap) {
@@ -973,7 +976,7 @@ __global__ void k_fs_mutants(uint8_t* type, typename Types<S>::M* mass, const ty
 }
 
 // gas neighbours of filled nodes: donor = highest-index filled neighbour (LB.cpp:1704-1728)
-__global__ void k_fs_donors(Geom g, const uint8_t* type, const int32_t* filled, int32_t* donor, int32_t* newIf,
+__device__ inline void k_fs_donors(Geom g, const uint8_t* type, const int32_t* filled, int32_t* donor, int32_t* newIf,
                             int cap, DevScalars* sc) {
     const int nF = sc->nFilled;
     for (int k = blockIdx.x * blockDim.x + threadIdx.x; k < nF; k += gridDim.x * blockDim.x) {
@@ -993,7 +996,7 @@ __global__ void k_fs_donors(Geom g, const uint8_t* type, const int32_t* filled, 
 // initialise the new interface nodes from their donor (node::initialize with the donor's
 // u, visc and hydroForce+lbF, mass 0.01*rho0)
 template <typename S, int L>
-__global__ void k_fs_new_from_donor(Geom g, S* f, uint8_t* type, typename Types<S>::M* mass,
+__device__ inline void k_fs_new_from_donor(Geom g, S* f, uint8_t* type, typename Types<S>::M* mass,
                                     typename Types<S>::C* aux, const int32_t* newIf, const int32_t* donor,
                                     DevScalars* sc, Consts k, int32_t* changed, int changedCap) {
     using C = typename Types<S>::C;
@@ -1027,7 +1030,7 @@ __global__ void k_fs_new_from_donor(Geom g, S* f, uint8_t* type, typename Types<
 
 // fluid neighbours of emptied nodes become interface with 99% of their density (LB.cpp:1730-1748)
 template <typename S>
-__global__ void k_fs_from_emptied(Geom g, uint8_t* type, typename Types<S>::M* mass, typename Types<S>::C* aux,
+__device__ inline void k_fs_from_emptied(Geom g, uint8_t* type, typename Types<S>::M* mass, typename Types<S>::C* aux,
                                   const int32_t* emptied, int32_t* marker, int32_t* newIf, int cap, DevScalars* sc) {
     using C = typename Types<S>::C;
     using M = typename Types<S>::M;
@@ -1060,7 +1063,7 @@ __global__ void k_fs_from_emptied(Geom g, uint8_t* type, typename Types<S>::M* m
 
 // updateMutants (LB.cpp:1751-1790)
 template <typename S>
-__global__ void k_fs_update_mutants(uint8_t* type, typename Types<S>::M* mass, const typename Types<S>::C* aux,
+__device__ inline void k_fs_update_mutants(uint8_t* type, typename Types<S>::M* mass, const typename Types<S>::C* aux,
                                     const int32_t* filled, const int32_t* emptied, DevScalars* sc) {
     const int nF = sc->nFilled, nE = sc->nEmptied;
     double acc = 0.0;
@@ -1080,7 +1083,7 @@ __global__ void k_fs_update_mutants(uint8_t* type, typename Types<S>::M* mass, c
 }
 
 // interface list after conversions: untouched old interface nodes + new ones
-__global__ void k_fs_list2(const uint8_t* type, const int32_t* ifList, const int32_t* newIf, const int32_t* marker,
+__device__ inline void k_fs_list2(const uint8_t* type, const int32_t* ifList, const int32_t* newIf, const int32_t* marker,
                            int32_t* list2, int cap, DevScalars* sc, int* count2) {
     const int nIf = sc->nIf, nN = sc->nNewIf;
     for (int k = blockIdx.x * blockDim.x + threadIdx.x; k < nIf + nN; k += gridDim.x * blockDim.x) {
@@ -1097,7 +1100,7 @@ __global__ void k_fs_list2(const uint8_t* type, const int32_t* ifList, const int
 
 // removeIsolated part 1: interface nodes without gas neighbours become fluid (LB.cpp:1796-1816)
 template <typename S>
-__global__ void k_fs_isolated_a(Geom g, uint8_t* type, typename Types<S>::M* mass, const typename Types<S>::C* aux,
+__device__ inline void k_fs_isolated_a(Geom g, uint8_t* type, typename Types<S>::M* mass, const typename Types<S>::C* aux,
                                 const int32_t* list2, const int* count2, DevScalars* sc) {
     const int n2 = *count2;
     double acc = 0.0;
@@ -1119,7 +1122,7 @@ __global__ void k_fs_isolated_a(Geom g, uint8_t* type, typename Types<S>::M* mas
 
 // removeIsolated part 2: interface nodes without fluid neighbours become gas (LB.cpp:1821-1841)
 template <typename S>
-__global__ void k_fs_isolated_b(Geom g, uint8_t* type, const typename Types<S>::M* mass, const int32_t* list2,
+__device__ inline void k_fs_isolated_b(Geom g, uint8_t* type, const typename Types<S>::M* mass, const int32_t* list2,
                                 const int* count2, DevScalars* sc, int32_t* changed, int changedCap) {
     const int n2 = *count2;
     double acc = 0.0;
@@ -1146,7 +1149,7 @@ __global__ void k_fs_isolated_b(Geom g, uint8_t* type, const typename Types<S>::
 
 // redistributeMass (LB.cpp:1844-1852) and marker reset
 template <typename S>
-__global__ void k_fs_redistribute(const uint8_t* type, typename Types<S>::M* mass, const int32_t* list2,
+__device__ inline void k_fs_redistribute(const uint8_t* type, typename Types<S>::M* mass, const int32_t* list2,
                                   const int* count2, int32_t* marker, const int32_t* filled, const int32_t* newIf,
                                   DevScalars* sc) {
     const int n2 = *count2;
@@ -1161,7 +1164,7 @@ __global__ void k_fs_redistribute(const uint8_t* type, typename Types<S>::M* mas
     if (blockIdx.x == 0 && threadIdx.x == 0) sc->lastSurplus = sc->surplus;
 }
 
-__global__ void k_fs_begin(DevScalars* sc, int* count2) {
+__device__ inline void k_fs_begin(DevScalars* sc, int* count2) {
     sc->surplus = 0.0;
     sc->nFilled = sc->nEmptied = sc->nNewIf = 0;
     sc->nIfFinal = 0;
@@ -1177,27 +1180,23 @@ static void fs_impl(Box& b, const Consts& k, DevScalars* sc, hipStream_t s) {
     C* aux = (C*)b.aux;
     int* count2 = (int*)(b.ifList2 + b.ifCap);  // counter stored after the list
     const int grid = LIST_GRID, bs = 256;
-    hipLaunchKernelGGL(k_fs_begin, 1, 1, 0, s, sc, count2);
+    launch<&k_fs_begin, 1024, 1>(1, 1, s, sc, count2);
     if (b.parity == 1)
-        hipLaunchKernelGGL((k_fs_mass<S, 1>), grid, bs, 0, s, b.g, f, b.type, mass, aux, b.ifList, (M*)b.ifNewMass, sc, b.donor);
+        launch<&k_fs_mass<S, 1>, 1024, 1>(grid, bs, s, b.g, f, b.type, mass, aux, b.ifList, (M*)b.ifNewMass, sc, b.donor);
     else
-        hipLaunchKernelGGL((k_fs_mass<S, 0>), grid, bs, 0, s, b.g, f, b.type, mass, aux, b.ifList, (M*)b.ifNewMass, sc, b.donor);
-    hipLaunchKernelGGL(k_fs_mutants<S>, grid, bs, 0, s, b.type, mass, aux, b.g.N, b.ifList, (M*)b.ifNewMass, b.filled,
-                       b.emptied, b.ifCap, sc, b.donor, b.changed, b.changedCap);
-    hipLaunchKernelGGL(k_fs_donors, grid, bs, 0, s, b.g, b.type, b.filled, b.donor, b.newIf, b.ifCap, sc);
+        launch<&k_fs_mass<S, 0>, 1024, 1>(grid, bs, s, b.g, f, b.type, mass, aux, b.ifList, (M*)b.ifNewMass, sc, b.donor);
+    launch<&k_fs_mutants<S>, 1024, 1>(grid, bs, s, b.type, mass, aux, b.g.N, b.ifList, (M*)b.ifNewMass, b.filled, b.emptied, b.ifCap, sc, b.donor, b.changed, b.changedCap);
+    launch<&k_fs_donors, 1024, 1>(grid, bs, s, b.g, b.type, b.filled, b.donor, b.newIf, b.ifCap, sc);
     if (b.parity == 1)
-        hipLaunchKernelGGL((k_fs_new_from_donor<S, 1>), grid, bs, 0, s, b.g, f, b.type, mass, aux, b.newIf, b.donor, sc, k,
-                           b.changed, b.changedCap);
+        launch<&k_fs_new_from_donor<S, 1>, 1024, 1>(grid, bs, s, b.g, f, b.type, mass, aux, b.newIf, b.donor, sc, k, b.changed, b.changedCap);
     else
-        hipLaunchKernelGGL((k_fs_new_from_donor<S, 0>), grid, bs, 0, s, b.g, f, b.type, mass, aux, b.newIf, b.donor, sc, k,
-                           b.changed, b.changedCap);
-    hipLaunchKernelGGL(k_fs_from_emptied<S>, grid, bs, 0, s, b.g, b.type, mass, aux, b.emptied, b.donor, b.newIf, b.ifCap, sc);
-    hipLaunchKernelGGL(k_fs_update_mutants<S>, grid, bs, 0, s, b.type, mass, aux, b.filled, b.emptied, sc);
-    hipLaunchKernelGGL(k_fs_list2, grid, bs, 0, s, b.type, b.ifList, b.newIf, b.donor, b.ifList2, b.ifCap, sc, count2);
-    hipLaunchKernelGGL(k_fs_isolated_a<S>, grid, bs, 0, s, b.g, b.type, mass, aux, b.ifList2, count2, sc);
-    hipLaunchKernelGGL(k_fs_isolated_b<S>, grid, bs, 0, s, b.g, b.type, mass, b.ifList2, count2, sc, b.changed,
-                       b.changedCap);
-    hipLaunchKernelGGL(k_fs_redistribute<S>, grid, bs, 0, s, b.type, mass, b.ifList2, count2, b.donor, b.filled, b.newIf, sc);
+        launch<&k_fs_new_from_donor<S, 0>, 1024, 1>(grid, bs, s, b.g, f, b.type, mass, aux, b.newIf, b.donor, sc, k, b.changed, b.changedCap);
+    launch<&k_fs_from_emptied<S>, 1024, 1>(grid, bs, s, b.g, b.type, mass, aux, b.emptied, b.donor, b.newIf, b.ifCap, sc);
+    launch<&k_fs_update_mutants<S>, 1024, 1>(grid, bs, s, b.type, mass, aux, b.filled, b.emptied, sc);
+    launch<&k_fs_list2, 1024, 1>(grid, bs, s, b.type, b.ifList, b.newIf, b.donor, b.ifList2, b.ifCap, sc, count2);
+    launch<&k_fs_isolated_a<S>, 1024, 1>(grid, bs, s, b.g, b.type, mass, aux, b.ifList2, count2, sc);
+    launch<&k_fs_isolated_b<S>, 1024, 1>(grid, bs, s, b.g, b.type, mass, b.ifList2, count2, sc, b.changed, b.changedCap);
+    launch<&k_fs_redistribute<S>, 1024, 1>(grid, bs, s, b.type, mass, b.ifList2, count2, b.donor, b.filled, b.newIf, sc);
 }
 
 void lb_free_surface(Box& b, const Consts& k, DevScalars* sc, hipStream_t s) {
@@ -1212,7 +1211,7 @@ void lb_free_surface(Box& b, const Consts& k, DevScalars* sc, hipStream_t s) {
 // blocks over particles, each particle's bounding box: active nodes inside the sphere, highest
 // index wins
 constexpr int FLAGS_GRID = 2048;  // grids of the flag passes (grid-stride; most steps return early)
-__global__ void k_flags_compute(Geom g, const uint8_t* type, const PartLB* plb, int nPart, int32_t* flagNew,
+__device__ inline void k_flags_compute(Geom g, const uint8_t* type, const PartLB* plb, int nPart, int32_t* flagNew,
                                 const DevScalars* sc) {
     if (!sc->newNeighborList) return;
     const int dims[3] = {g.X, g.Y, g.Z};
@@ -1240,11 +1239,11 @@ __global__ void k_flags_compute(Geom g, const uint8_t* type, const PartLB* plb, 
 void lb_flags_rebuild_compute(const Box& b, const Consts& k, const PartLB* plb, DevScalars* sc, hipStream_t s) {
     // grid-stride over the particles: most steps only test the flag and return
     if (k.nPart > 0)
-        hipLaunchKernelGGL(k_flags_compute, std::min(k.nPart, FLAGS_GRID), 256, 0, s, b.g, b.type, plb, k.nPart, b.flagNew, sc);
+        launch<&k_flags_compute, 1024, 1>(std::min(k.nPart, FLAGS_GRID), 256, s, b.g, b.type, plb, k.nPart, b.flagNew, sc);
 }
 
 // clear every flag and solid index, apply the new flags (initializeParticleBoundaries)
-__global__ void k_flags_apply(int64_t N, uint8_t* type, int32_t* solid, int32_t* flagNew, DevScalars* sc) {
+__device__ inline void k_flags_apply(int64_t N, uint8_t* type, int32_t* solid, int32_t* flagNew, DevScalars* sc) {
     if (!sc->newNeighborList) return;
     __shared__ int bmin;
     if (threadIdx.x == 0) bmin = INT_MAX;
@@ -1278,24 +1277,24 @@ __global__ void k_flags_apply(int64_t N, uint8_t* type, int32_t* solid, int32_t*
         atomicMin(&sc->kminP, bmin);
 }
 
-__global__ void k_flags_apply_begin(DevScalars* sc) {
+__device__ inline void k_flags_apply_begin(DevScalars* sc) {
     if (sc->newNeighborList) sc->kminP = INT_MAX;
 }
-__global__ void k_flags_apply_end(DevScalars* sc) {
+__device__ inline void k_flags_apply_end(DevScalars* sc) {
     if (!sc->newNeighborList) return;
     if (sc->kminP == INT_MAX) sc->kminP = -1;
     sc->newNeighborList = 0;
 }
 
 void lb_flags_rebuild_apply(const Box& b, DevScalars* sc, hipStream_t s) {
-    hipLaunchKernelGGL(k_flags_apply_begin, 1, 1, 0, s, sc);
+    launch<&k_flags_apply_begin, 1024, 1>(1, 1, s, sc);
     const int64_t nb = (b.g.N + 255) / 256;
     const int grid = int(nb < FLAGS_GRID ? nb : FLAGS_GRID);
-    hipLaunchKernelGGL(k_flags_apply, grid, 256, 0, s, b.g.N, b.type, b.solid, b.flagNew, sc);
-    hipLaunchKernelGGL(k_flags_apply_end, 1, 1, 0, s, sc);
+    launch<&k_flags_apply, 1024, 1>(grid, 256, s, b.g.N, b.type, b.solid, b.flagNew, sc);
+    launch<&k_flags_apply_end, 1024, 1>(1, 1, s, sc);
 }
 
-__global__ void k_fna_check(Geom g, const int32_t* solid, const PartLB* plb, int nPart, DevScalars* sc) {
+__device__ inline void k_fna_check(Geom g, const int32_t* solid, const PartLB* plb, int nPart, DevScalars* sc) {
     const int kp = sc->kminP;
     sc->needScan = 0;
     if (kp < 0) return;
@@ -1307,7 +1306,7 @@ __global__ void k_fna_check(Geom g, const int32_t* solid, const PartLB* plb, int
     if (!inside_sphere(x, y, z, P.c[0], P.c[1], P.c[2], P.r)) { sc->needScan = 1; sc->kstar = INT_MAX; }
 }
 
-__global__ void k_fna_scan(Geom g, const uint8_t* type, const int32_t* solid, const PartLB* plb, int nPart,
+__device__ inline void k_fna_scan(Geom g, const uint8_t* type, const int32_t* solid, const PartLB* plb, int nPart,
                            DevScalars* sc) {
     if (!sc->needScan) return;
     __shared__ int bmin;
@@ -1330,26 +1329,26 @@ __global__ void k_fna_scan(Geom g, const uint8_t* type, const int32_t* solid, co
         atomicMin(&sc->kstar, bmin);
 }
 
-__global__ void k_fna_unflag(Geom g, uint8_t* type, int32_t* solid, DevScalars* sc) {
+__device__ inline void k_fna_unflag(Geom g, uint8_t* type, int32_t* solid, DevScalars* sc) {
     if (!sc->needScan) return;
     const int ks = sc->kstar;
     for (int64_t i = int64_t(blockIdx.x) * blockDim.x + threadIdx.x; i < g.N && i < ks; i += int64_t(gridDim.x) * blockDim.x) {
         if (type[i] & T_PMASK) { type[i] &= uint8_t(~T_PMASK); solid[i] = 0; }
     }
 }
-__global__ void k_fna_end(DevScalars* sc) {
+__device__ inline void k_fna_end(DevScalars* sc) {
     if (!sc->needScan) return;
     sc->kminP = sc->kstar == INT_MAX ? -1 : sc->kstar;
     sc->needScan = 0;
 }
 
 void lb_find_new_active(const Box& b, const Consts& k, const PartLB* plb, DevScalars* sc, hipStream_t s) {
-    hipLaunchKernelGGL(k_fna_check, 1, 1, 0, s, b.g, b.solid, plb, k.nPart, sc);
+    launch<&k_fna_check, 1024, 1>(1, 1, s, b.g, b.solid, plb, k.nPart, sc);
     const int64_t nb = (b.g.N + 255) / 256;
     const int grid = int(nb < 16384 ? nb : 16384);
-    hipLaunchKernelGGL(k_fna_scan, grid, 256, 0, s, b.g, b.type, b.solid, plb, k.nPart, sc);
-    hipLaunchKernelGGL(k_fna_unflag, grid, 256, 0, s, b.g, b.type, b.solid, sc);
-    hipLaunchKernelGGL(k_fna_end, 1, 1, 0, s, sc);
+    launch<&k_fna_scan, 1024, 1>(grid, 256, s, b.g, b.type, b.solid, plb, k.nPart, sc);
+    launch<&k_fna_unflag, 1024, 1>(grid, 256, s, b.g, b.type, b.solid, sc);
+    launch<&k_fna_end, 1024, 1>(1, 1, s, sc);
 }
 
 // ------------------------------------------------------------------------------------------
@@ -1393,12 +1392,11 @@ __device__ inline int block_exclusive_scan(int v, int* sWave, int& total) {
 
 // pass 1: ordered compaction of the nodes of each block's slice of the list that left the sphere,
 // into the same slice of cand
-__global__ __launch_bounds__(UNCOVER_THREADS) void k_obj_uncover_find(Geom g, const int32_t* objList, int32_t* cand,
-                                                                      int* blockCount, const DevScalars* sc,
-                                                                      const ObjLB* olb) {
+__device__ inline void k_obj_uncover_find(Geom g, const int32_t* objList, int32_t* cand, int* blockCount, int G,
+                                          const DevScalars* sc, const ObjLB* olb) {
+    if (int(blockIdx.x) >= G) return;  // G: the box's grid
     __shared__ int sWave[UNCOVER_THREADS / 64 + 1];
     const int tid = threadIdx.x;
-    const int G = int(gridDim.x);
     const int n = sc->nObjList;
     const int per = (n + G - 1) / G;
     const ObjLB O = olb[0];
@@ -1424,7 +1422,7 @@ __global__ __launch_bounds__(UNCOVER_THREADS) void k_obj_uncover_find(Geom g, co
 
 // pass 2 (one block): the candidates of the G slices in order
 template <typename S, int L>
-__global__ __launch_bounds__(UNCOVER_THREADS) void k_obj_uncover(Geom g, S* f, uint8_t* type,
+__device__ inline void k_obj_uncover(Geom g, S* f, uint8_t* type,
                                                                  typename Types<S>::M* mass,
                                                                  typename Types<S>::C* aux, const int32_t* cand,
                                                                  const int* blockCount, int G, DevScalars* sc,
@@ -1582,7 +1580,7 @@ __device__ inline int inside_which(int x, int y, int z, const ObjLB* olb, int nO
         if (inside_sphere(x, y, z, olb[o].c[0], olb[o].c[1], olb[o].c[2], olb[o].r)) which = o;
     return which;
 }
-__global__ __launch_bounds__(COVER_BS) void k_obj_cover_a(Geom g, uint8_t* type, int32_t* solid, const ObjLB* olb,
+__device__ inline void k_obj_cover_a(Geom g, uint8_t* type, int32_t* solid, const ObjLB* olb,
                                                           int nObj, int* blockCount, int32_t* seg, int segCap,
                                                           int32_t* changed, int changedCap, DevScalars* sc) {
     __shared__ int sWave[COVER_BS / 64 + 1];
@@ -1620,7 +1618,7 @@ __global__ __launch_bounds__(COVER_BS) void k_obj_cover_a(Geom g, uint8_t* type,
     if (threadIdx.x == 0) blockCount[blockIdx.x] = cnt;
 }
 // the object node list: every block's nodes at the exclusive prefix of the block counts
-__global__ __launch_bounds__(COVER_BS) void k_obj_cover_b(Geom g, const uint8_t* type, const ObjLB* olb, int nObj,
+__device__ inline void k_obj_cover_b(Geom g, const uint8_t* type, const ObjLB* olb, int nObj,
                                                           const int* blockCount, const int32_t* seg, int segCap,
                                                           int32_t* objList, int cap, DevScalars* sc) {
     __shared__ int sWave[COVER_BS / 64 + 1];
@@ -1676,18 +1674,14 @@ static void obj_impl(Box& b, const Consts& k, const ObjLB* olb, DevScalars* sc, 
     // the object node list holds at most objCap nodes: one block per UNC_NODES_PER_BLOCK of them
     const int grid = std::max(1, std::min(UNC_MAX_BLOCKS, (b.objCap + UNC_NODES_PER_BLOCK - 1) / UNC_NODES_PER_BLOCK));
     int* uncCount = b.coverCount + COVER_GRID;
-    hipLaunchKernelGGL(k_obj_uncover_find, grid, UNCOVER_THREADS, 0, s, b.g, b.objList, b.objCand, uncCount, sc, olb);
+    launch<&k_obj_uncover_find, UNCOVER_THREADS, 1>(grid, UNCOVER_THREADS, s, b.g, b.objList, b.objCand, uncCount, grid, sc, olb);
     if (b.parity == 0)
-        hipLaunchKernelGGL((k_obj_uncover<S, 0>), 1, UNCOVER_THREADS, 0, s, b.g, (S*)b.f, b.type, (M*)b.mass,
-                           (C*)b.aux, b.objCand, uncCount, grid, sc, k, b.changed, b.changedCap);
+        launch<&k_obj_uncover<S, 0>, UNCOVER_THREADS, 1>(1, UNCOVER_THREADS, s, b.g, (S*)b.f, b.type, (M*)b.mass, (C*)b.aux, b.objCand, uncCount, grid, sc, k, b.changed, b.changedCap);
     else
-        hipLaunchKernelGGL((k_obj_uncover<S, 1>), 1, UNCOVER_THREADS, 0, s, b.g, (S*)b.f, b.type, (M*)b.mass,
-                           (C*)b.aux, b.objCand, uncCount, grid, sc, k, b.changed, b.changedCap);
+        launch<&k_obj_uncover<S, 1>, UNCOVER_THREADS, 1>(1, UNCOVER_THREADS, s, b.g, (S*)b.f, b.type, (M*)b.mass, (C*)b.aux, b.objCand, uncCount, grid, sc, k, b.changed, b.changedCap);
     // the candidate buffer of the uncover pass stages the cover pass's per-block node lists
-    hipLaunchKernelGGL(k_obj_cover_a, COVER_GRID, COVER_BS, 0, s, b.g, b.type, b.solid, olb, k.nObj, b.coverCount,
-                       b.objCand, b.objCap, b.changed, b.changedCap, sc);
-    hipLaunchKernelGGL(k_obj_cover_b, COVER_GRID, COVER_BS, 0, s, b.g, b.type, olb, k.nObj, b.coverCount, b.objCand,
-                       b.objCap, b.objList, b.objCap, sc);
+    launch<&k_obj_cover_a, COVER_BS, 1>(COVER_GRID, COVER_BS, s, b.g, b.type, b.solid, olb, k.nObj, b.coverCount, b.objCand, b.objCap, b.changed, b.changedCap, sc);
+    launch<&k_obj_cover_b, COVER_BS, 1>(COVER_GRID, COVER_BS, s, b.g, b.type, olb, k.nObj, b.coverCount, b.objCand, b.objCap, b.objList, b.objCap, sc);
 }
 
 void lb_object_update(Box& b, const Consts& k, const ObjLB* olb, DevScalars* sc, hipStream_t s) {
@@ -1701,7 +1695,7 @@ void lb_object_update(Box& b, const Consts& k, const ObjLB* olb, DevScalars* sc,
 // downloads
 // ------------------------------------------------------------------------------------------
 template <typename S>
-__global__ void k_download_f(Geom g, const S* f, const uint8_t* type, int L, double* out) {
+__device__ inline void k_download_f(Geom g, const S* f, const uint8_t* type, int L, double* out) {
     for (int64_t i = int64_t(blockIdx.x) * blockDim.x + threadIdx.x; i < g.N; i += int64_t(gridDim.x) * blockDim.x) {
         const bool act = is_active(type[i]);
         int x, y, z;
@@ -1714,13 +1708,13 @@ __global__ void k_download_f(Geom g, const S* f, const uint8_t* type, int L, dou
 }
 void lb_download_f(const Box& b, double* out, hipStream_t s) {
     const int grid = 4096;
-    if (b.prec == LBDEM_PREC_FP64) hipLaunchKernelGGL(k_download_f<double>, grid, 256, 0, s, b.g, (const double*)b.f, b.type, b.parity, out);
-    else if (b.prec == LBDEM_PREC_FP32) hipLaunchKernelGGL(k_download_f<float>, grid, 256, 0, s, b.g, (const float*)b.f, b.type, b.parity, out);
-    else hipLaunchKernelGGL(k_download_f<__half>, grid, 256, 0, s, b.g, (const __half*)b.f, b.type, b.parity, out);
+    if (b.prec == LBDEM_PREC_FP64) launch<&k_download_f<double>, 1024, 1>(grid, 256, s, b.g, (const double*)b.f, b.type, b.parity, out);
+    else if (b.prec == LBDEM_PREC_FP32) launch<&k_download_f<float>, 1024, 1>(grid, 256, s, b.g, (const float*)b.f, b.type, b.parity, out);
+    else launch<&k_download_f<__half>, 1024, 1>(grid, 256, s, b.g, (const __half*)b.f, b.type, b.parity, out);
 }
 
 template <typename S>
-__global__ void k_download_n(Geom g, const uint8_t* type, const typename Types<S>::M* mass,
+__device__ inline void k_download_n(Geom g, const uint8_t* type, const typename Types<S>::M* mass,
                              const typename Types<S>::C* aux, double* out) {
     for (int64_t i = int64_t(blockIdx.x) * blockDim.x + threadIdx.x; i < g.N; i += int64_t(gridDim.x) * blockDim.x) {
         const uint8_t t = type[i] & T_BASE;
@@ -1729,13 +1723,13 @@ __global__ void k_download_n(Geom g, const uint8_t* type, const typename Types<S
 }
 void lb_download_n(const Box& b, const Consts& k, double* out, hipStream_t s) {
     (void)k;
-    if (b.prec == LBDEM_PREC_FP64) hipLaunchKernelGGL(k_download_n<double>, 4096, 256, 0, s, b.g, b.type, (const double*)b.mass, (const double*)b.aux, out);
-    else hipLaunchKernelGGL(k_download_n<float>, 4096, 256, 0, s, b.g, b.type, (const float*)b.mass, (const float*)b.aux, out);
+    if (b.prec == LBDEM_PREC_FP64) launch<&k_download_n<double>, 1024, 1>(4096, 256, s, b.g, b.type, (const double*)b.mass, (const double*)b.aux, out);
+    else launch<&k_download_n<float>, 1024, 1>(4096, 256, s, b.g, b.type, (const float*)b.mass, (const float*)b.aux, out);
 }
 
 // total fluid mass: active nodes outside particles (LB.cpp:388-395)
 template <typename M>
-__global__ void k_total_mass(int64_t N, const uint8_t* type, const M* mass, double* out) {
+__device__ inline void k_total_mass(int64_t N, const uint8_t* type, const M* mass, double* out) {
     __shared__ double red[256];
     double acc = 0.0;
     for (int64_t i = int64_t(blockIdx.x) * blockDim.x + threadIdx.x; i < N; i += int64_t(gridDim.x) * blockDim.x) {
@@ -1752,11 +1746,11 @@ __global__ void k_total_mass(int64_t N, const uint8_t* type, const M* mass, doub
 }
 void lb_total_mass(const Box& b, double* out, hipStream_t s) {
     // out: 1024 per-block partials, summed on the host in block order
-    if (b.prec == LBDEM_PREC_FP64) hipLaunchKernelGGL(k_total_mass<double>, 1024, 256, 0, s, b.g.N, b.type, (const double*)b.mass, out);
-    else hipLaunchKernelGGL(k_total_mass<float>, 1024, 256, 0, s, b.g.N, b.type, (const float*)b.mass, out);
+    if (b.prec == LBDEM_PREC_FP64) launch<&k_total_mass<double>, 1024, 1>(1024, 256, s, b.g.N, b.type, (const double*)b.mass, out);
+    else launch<&k_total_mass<float>, 1024, 1>(1024, 256, s, b.g.N, b.type, (const float*)b.mass, out);
 }
 
-__global__ void k_count_types(int64_t N, const uint8_t* type, int* counts) {
+__device__ inline void k_count_types(int64_t N, const uint8_t* type, int* counts) {
     int c[5] = {0, 0, 0, 0, 0};
     for (int64_t i = int64_t(blockIdx.x) * blockDim.x + threadIdx.x; i < N; i += int64_t(gridDim.x) * blockDim.x) {
         const uint8_t t = type[i];
@@ -1767,7 +1761,7 @@ __global__ void k_count_types(int64_t N, const uint8_t* type, int* counts) {
     for (int j = 0; j < 5; ++j) if (c[j]) atomicAdd(&counts[j], c[j]);
 }
 void lb_count_types(const Box& b, int* counts, hipStream_t s) {
-    hipLaunchKernelGGL(k_count_types, 1024, 256, 0, s, b.g.N, b.type, counts);
+    launch<&k_count_types, 1024, 1>(1024, 256, s, b.g.N, b.type, counts);
 }
 
 }  // namespace lbdem

@@ -271,5 +271,8 @@ void dem_refresh_views(const DemParams& p, DemState& d, hipStream_t s);
 // them (set before lbdem_init); the stream a context enqueues on
 int ctx_use_region(lbdem_ctx* c, void* base, size_t bytes);
 hipStream_t ctx_stream(const lbdem_ctx* c);
+// a batch's argument tables in [base, base+bytes) (set before its first cycle); its stream
+int batch_use_region(lbdem_batch* b, void* base, size_t bytes);
+hipStream_t batch_stream(const lbdem_batch* b);
 
 }  // namespace lbdem

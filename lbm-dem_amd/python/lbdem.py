@@ -168,6 +168,11 @@ class Library:
             "lbdem_use_graphs": ([vp, C.c_int], C.c_int),
             "lbdem_set_tile": ([vp, C.c_int], C.c_int),
             "lbdem_get_tuning": ([vp, C.POINTER(Tuning)], C.c_int),
+            "lbdem_batch_create": ([C.POINTER(vp), C.c_int, C.POINTER(vp)], C.c_int),
+            "lbdem_batch_cycle": ([vp, C.c_int], C.c_int),
+            "lbdem_batch_synchronize": ([vp], C.c_int),
+            "lbdem_batch_last_error": ([vp], C.c_char_p),
+            "lbdem_batch_destroy": ([vp], C.c_int),
             "lbdem_pool_create": ([C.c_size_t, C.c_int, C.c_int, C.c_int, C.POINTER(vp)], C.c_int),
             "lbdem_pool_submit": ([vp, C.POINTER(RunDesc), _ip], C.c_int),
             "lbdem_pool_poll": ([vp, C.c_int, C.POINTER(RunResult)], C.c_int),
@@ -347,6 +352,42 @@ def estimate_bytes(lib: Library, params: Params, particles: "ParticleSet", objec
     if rc != 0:
         raise LbdemError(f"lbdem_estimate_bytes failed with code {rc}")
     return v.value
+
+
+class Batch:
+    """Contexts stepped together: each kernel of a step launched once for all of them."""
+
+    def __init__(self, lib: Library, sims: Sequence["Simulation"]):
+        self.lib = lib
+        self.sims = list(sims)
+        self.batch = C.c_void_p()
+        arr = (C.c_void_p * len(self.sims))(*[sm.ctx.value for sm in self.sims])
+        rc = lib.lib.lbdem_batch_create(arr, len(self.sims), C.byref(self.batch))
+        if rc != 0:
+            msgs = [sm.lib.lib.lbdem_last_error(sm.ctx).decode(errors="replace") for sm in self.sims]
+            raise LbdemError(f"lbdem_batch_create failed with code {rc}: {[m for m in msgs if m]}")
+
+    def _check(self, rc: int, what: str):
+        if rc != 0:
+            msg = self.lib.lib.lbdem_batch_last_error(self.batch) if self.batch else b""
+            raise LbdemError(f"{what} failed with code {rc}: {msg.decode(errors='replace') if msg else ''}")
+
+    def cycle(self, n: int):
+        self._check(self.lib.lib.lbdem_batch_cycle(self.batch, int(n)), "lbdem_batch_cycle")
+
+    def synchronize(self):
+        self._check(self.lib.lib.lbdem_batch_synchronize(self.batch), "lbdem_batch_synchronize")
+
+    def close(self):
+        if self.batch:
+            self.lib.lib.lbdem_batch_destroy(self.batch)
+            self.batch = C.c_void_p()
+
+    def __del__(self):
+        try:
+            self.close()
+        except Exception:
+            pass
 
 
 class Pool:

@@ -1455,6 +1455,14 @@ int lbdem_get_tuning(lbdem_ctx* c, lbdem_tuning* t) {
     std::memset(t, 0, sizeof(*t));  // no kernel tiles on the CPU
     return 0;
 }
+int lbdem_batch_create(lbdem_ctx* const*, int, lbdem_batch** out) {
+    if (out) *out = nullptr;
+    return LBDEM_E_STATE;
+}
+int lbdem_batch_cycle(lbdem_batch*, int) { return LBDEM_E_STATE; }
+int lbdem_batch_synchronize(lbdem_batch*) { return LBDEM_E_STATE; }
+const char* lbdem_batch_last_error(const lbdem_batch*) { return "the CPU oracle has no batches"; }
+int lbdem_batch_destroy(lbdem_batch*) { return 0; }
 int lbdem_pool_create(size_t, int, int, int, lbdem_pool** out) {
     if (out) *out = nullptr;
     return LBDEM_E_STATE;

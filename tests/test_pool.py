@@ -1,5 +1,6 @@
-"""Run pool of the device library: independent impact runs admitted against an HBM budget and
-stepped concurrently, each on its own stream.
+"""Batches and the run pool of the device library. A batch steps several boxes with one launch per
+kernel; the pool admits independent impact runs against an HBM budget and steps them
+concurrently.
 
 Identity: every run's result in the mix equals the same run alone (its own context, lbdem_cycle
 over all steps). The kernels add FP64 partial forces and the free-surface mass surplus with
@@ -153,3 +154,51 @@ def test_pool_destroy_with_work_outstanding(gpu_lib, runs):
                       max_concurrent=2, steps_per_chunk=8)
     submit_all(pool, runs)
     pool.close()
+
+
+WARM = 12  # start-up steps alone: the impactor wait, the tile tuner (8 LB steps), the first LB step
+
+
+def test_batch_results_equal_alone(gpu_lib, runs, alone_results):
+    """The FP32 runs (boxes of different sizes) warm up alone, advance together as one batch, and
+    finish alone: every result equals the run alone."""
+    idx = [i for i, (_, p) in enumerate(runs) if p == lbdem.PREC_FP32]
+    sims = []
+    for i in idx:
+        run, prec = runs[i]
+        sim = lbdem.Simulation(gpu_lib, lbdem.params_from_values(run.values, prec), run.particles, run.objects)
+        sim.init()
+        sim.cycle(WARM)
+        sims.append(sim)
+    common = min(runs[i][0].nsteps for i in idx) - WARM
+    common -= common % 2
+    batch = lbdem.Batch(gpu_lib, sims)
+    batch.cycle(common // 2 * 2 - 10)
+    batch.cycle(10)
+    batch.synchronize()
+    batch.close()
+    for k, i in enumerate(idx):
+        sims[k].cycle(runs[i][0].nsteps - WARM - common)
+        sims[k].synchronize()
+        res = dict(objects=sims[k].objects(), totalMass=sims[k].total_mass(), nActive=sims[k].info()["nActive"])
+        same(res, alone_results[i], f"run {i} through a batch of {len(idx)}")
+        sims[k].close()
+
+
+def test_batch_rejects_members_that_cannot_share_launches(gpu_lib, runs):
+    run32, run64 = runs[0][0], runs[1][0]
+    a = lbdem.Simulation(gpu_lib, lbdem.params_from_values(run32.values, lbdem.PREC_FP32), run32.particles, run32.objects)
+    b = lbdem.Simulation(gpu_lib, lbdem.params_from_values(run64.values, lbdem.PREC_FP64), run64.particles, run64.objects)
+    a.init()
+    b.init()
+    with pytest.raises(lbdem.LbdemError):  # start-up steps not done
+        lbdem.Batch(gpu_lib, [a, b])
+    a.cycle(WARM)
+    b.cycle(WARM)
+    with pytest.raises(lbdem.LbdemError):  # precisions differ
+        lbdem.Batch(gpu_lib, [a, b])
+    b.cycle(1)
+    with pytest.raises(lbdem.LbdemError):  # step parities differ
+        lbdem.Batch(gpu_lib, [a, a])
+    a.close()
+    b.close()
