@@ -762,7 +762,7 @@ void dem_neighbor_list(const DemParams& p, DemState& d, DevScalars* sc, bool for
     const int pb = (p.nPart + 127) / 128;
     if (p.nlx > 0) {
         const int ncell = p.nlx * p.nly * p.nlz;
-        launch<&k_cells_clear, 1024, 1>((ncell + 255) / 256, 256, s, a, ncell);
+        launch_gs<&k_cells_clear, 1024, 1>((ncell + 255) / 256, 256, s, a, ncell);
         launch<&k_cells_assign, 1024, 1>(pb, 128, s, a);
         launch<&k_cells_scan, 1024, 1>(1, 1024, s, a, ncell);
         launch<&k_cells_fill, 1024, 1>(pb, 128, s, a);

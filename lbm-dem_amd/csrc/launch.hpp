@@ -30,9 +30,18 @@ struct PackImpl<std::index_sequence<I...>, P...> : PackLeaf<I, P>... {};
 template <typename... P>
 using Pack = PackImpl<std::index_sequence_for<P...>, P...>;
 
+// The table is read through the constant address space (it does not change while a kernel runs):
+// pointers loaded from there are known to address global memory, as pointers in kernel arguments
+// are, so the bodies keep their global (not flat) memory instructions and atomic combining.
+#define LBDEM_CONST __attribute__((address_space(4)))
+template <size_t I, typename T>
+__device__ inline T leaf_value(const LBDEM_CONST PackLeaf<I, T>& l) {
+    if constexpr (std::is_class<T>::value) return *(const T*)&l.v;  // trivially copyable struct
+    else return l.v;
+}
 template <auto Body, typename... P, size_t... I>
-__device__ inline void apply_pack(const PackImpl<std::index_sequence<I...>, P...>& t) {
-    Body(static_cast<const PackLeaf<I, P>&>(t).v...);
+__device__ inline void apply_pack(const LBDEM_CONST PackImpl<std::index_sequence<I...>, P...>& t) {
+    Body(leaf_value<I, P>(t)...);
 }
 
 template <auto Body, int MAXT, int MINW, typename... P>
@@ -42,7 +51,8 @@ __global__ __launch_bounds__(MAXT, MINW) void kernel_one(P... args) {
 
 template <auto Body, int MAXT, int MINW, typename... P>
 __global__ __launch_bounds__(MAXT, MINW) void kernel_batch(const Pack<P...>* tab) {
-    apply_pack<Body, P...>(tab[blockIdx.y]);
+    const LBDEM_CONST Pack<P...>* ct = (const LBDEM_CONST Pack<P...>*)tab;
+    apply_pack<Body, P...>(ct[blockIdx.y]);
 }
 
 // one recorded launch of one box
@@ -51,6 +61,7 @@ struct RecordedLaunch {
     void (*batch)(dim3 grid, dim3 block, size_t shm, hipStream_t s, const void* table);
     dim3 grid, block;
     size_t shm;
+    bool elastic;                     // grid-stride body: correct with any number of blocks
     std::vector<unsigned char> args;  // the body's argument tuple, as bytes
 };
 struct Recorder {
@@ -69,10 +80,10 @@ struct Launcher<Body, MAXT, MINW, void (*)(P...)> {
         hipLaunchKernelGGL((kernel_batch<Body, MAXT, MINW, P...>), g, b, shm, s, static_cast<const Tuple*>(table));
     }
     template <typename... A>
-    static void run(dim3 g, dim3 b, size_t shm, hipStream_t s, A&&... a) {
+    static void run(bool elastic, dim3 g, dim3 b, size_t shm, hipStream_t s, A&&... a) {
         if (Recorder* r = current_recorder()) {
             const Tuple t{{static_cast<P>(std::forward<A>(a))}...};
-            RecordedLaunch op{&tag, &batch, g, b, shm, std::vector<unsigned char>(sizeof(Tuple))};
+            RecordedLaunch op{&tag, &batch, g, b, shm, elastic, std::vector<unsigned char>(sizeof(Tuple))};
             std::memcpy(op.args.data(), &t, sizeof(Tuple));
             r->ops.push_back(std::move(op));
             return;
@@ -84,7 +95,13 @@ struct Launcher<Body, MAXT, MINW, void (*)(P...)> {
 // launch<&body, max threads per block, min waves per SIMD>(grid, block, stream, args...)
 template <auto Body, int MAXT = 1024, int MINW = 1, typename... A>
 inline void launch(dim3 grid, dim3 block, hipStream_t s, A&&... a) {
-    Launcher<Body, MAXT, MINW>::run(grid, block, 0, s, std::forward<A>(a)...);
+    Launcher<Body, MAXT, MINW>::run(false, grid, block, 0, s, std::forward<A>(a)...);
+}
+// the same for a grid-stride body, whose grid a batch may shrink (it spreads a bounded number of
+// blocks over its boxes)
+template <auto Body, int MAXT = 1024, int MINW = 1, typename... A>
+inline void launch_gs(dim3 grid, dim3 block, hipStream_t s, A&&... a) {
+    Launcher<Body, MAXT, MINW>::run(true, grid, block, 0, s, std::forward<A>(a)...);
 }
 
 }  // namespace lbdem

@@ -784,9 +784,9 @@ __device__ inline void k_mark_end(DevScalars* sc) {
 }
 
 void lb_mark_edges(const Box& b, DevScalars* sc, hipStream_t s) {
-    if (b.g.X % 4 == 0) launch<&k_mark_full_w, 1024, 1>(MARK_GRID, 256, s, b.g, b.type, b.solid, b.bcm, sc);
-    else launch<&k_mark_full_b, 1024, 1>(MARK_GRID, 256, s, b.g, b.type, b.solid, b.bcm, sc);
-    launch<&k_remark, 1024, 1>(256, 256, s, b.g, b.type, b.solid, b.bcm, b.changed, b.changedCap, sc);
+    if (b.g.X % 4 == 0) launch_gs<&k_mark_full_w, 1024, 1>(MARK_GRID, 256, s, b.g, b.type, b.solid, b.bcm, sc);
+    else launch_gs<&k_mark_full_b, 1024, 1>(MARK_GRID, 256, s, b.g, b.type, b.solid, b.bcm, sc);
+    launch_gs<&k_remark, 1024, 1>(256, 256, s, b.g, b.type, b.solid, b.bcm, b.changed, b.changedCap, sc);
     launch<&k_mark_end, 1024, 1>(1, 1, s, sc);
 }
 
@@ -1182,21 +1182,21 @@ static void fs_impl(Box& b, const Consts& k, DevScalars* sc, hipStream_t s) {
     const int grid = LIST_GRID, bs = 256;
     launch<&k_fs_begin, 1024, 1>(1, 1, s, sc, count2);
     if (b.parity == 1)
-        launch<&k_fs_mass<S, 1>, 1024, 1>(grid, bs, s, b.g, f, b.type, mass, aux, b.ifList, (M*)b.ifNewMass, sc, b.donor);
+        launch_gs<&k_fs_mass<S, 1>, 1024, 1>(grid, bs, s, b.g, f, b.type, mass, aux, b.ifList, (M*)b.ifNewMass, sc, b.donor);
     else
-        launch<&k_fs_mass<S, 0>, 1024, 1>(grid, bs, s, b.g, f, b.type, mass, aux, b.ifList, (M*)b.ifNewMass, sc, b.donor);
-    launch<&k_fs_mutants<S>, 1024, 1>(grid, bs, s, b.type, mass, aux, b.g.N, b.ifList, (M*)b.ifNewMass, b.filled, b.emptied, b.ifCap, sc, b.donor, b.changed, b.changedCap);
-    launch<&k_fs_donors, 1024, 1>(grid, bs, s, b.g, b.type, b.filled, b.donor, b.newIf, b.ifCap, sc);
+        launch_gs<&k_fs_mass<S, 0>, 1024, 1>(grid, bs, s, b.g, f, b.type, mass, aux, b.ifList, (M*)b.ifNewMass, sc, b.donor);
+    launch_gs<&k_fs_mutants<S>, 1024, 1>(grid, bs, s, b.type, mass, aux, b.g.N, b.ifList, (M*)b.ifNewMass, b.filled, b.emptied, b.ifCap, sc, b.donor, b.changed, b.changedCap);
+    launch_gs<&k_fs_donors, 1024, 1>(grid, bs, s, b.g, b.type, b.filled, b.donor, b.newIf, b.ifCap, sc);
     if (b.parity == 1)
-        launch<&k_fs_new_from_donor<S, 1>, 1024, 1>(grid, bs, s, b.g, f, b.type, mass, aux, b.newIf, b.donor, sc, k, b.changed, b.changedCap);
+        launch_gs<&k_fs_new_from_donor<S, 1>, 1024, 1>(grid, bs, s, b.g, f, b.type, mass, aux, b.newIf, b.donor, sc, k, b.changed, b.changedCap);
     else
-        launch<&k_fs_new_from_donor<S, 0>, 1024, 1>(grid, bs, s, b.g, f, b.type, mass, aux, b.newIf, b.donor, sc, k, b.changed, b.changedCap);
-    launch<&k_fs_from_emptied<S>, 1024, 1>(grid, bs, s, b.g, b.type, mass, aux, b.emptied, b.donor, b.newIf, b.ifCap, sc);
-    launch<&k_fs_update_mutants<S>, 1024, 1>(grid, bs, s, b.type, mass, aux, b.filled, b.emptied, sc);
-    launch<&k_fs_list2, 1024, 1>(grid, bs, s, b.type, b.ifList, b.newIf, b.donor, b.ifList2, b.ifCap, sc, count2);
-    launch<&k_fs_isolated_a<S>, 1024, 1>(grid, bs, s, b.g, b.type, mass, aux, b.ifList2, count2, sc);
-    launch<&k_fs_isolated_b<S>, 1024, 1>(grid, bs, s, b.g, b.type, mass, b.ifList2, count2, sc, b.changed, b.changedCap);
-    launch<&k_fs_redistribute<S>, 1024, 1>(grid, bs, s, b.type, mass, b.ifList2, count2, b.donor, b.filled, b.newIf, sc);
+        launch_gs<&k_fs_new_from_donor<S, 0>, 1024, 1>(grid, bs, s, b.g, f, b.type, mass, aux, b.newIf, b.donor, sc, k, b.changed, b.changedCap);
+    launch_gs<&k_fs_from_emptied<S>, 1024, 1>(grid, bs, s, b.g, b.type, mass, aux, b.emptied, b.donor, b.newIf, b.ifCap, sc);
+    launch_gs<&k_fs_update_mutants<S>, 1024, 1>(grid, bs, s, b.type, mass, aux, b.filled, b.emptied, sc);
+    launch_gs<&k_fs_list2, 1024, 1>(grid, bs, s, b.type, b.ifList, b.newIf, b.donor, b.ifList2, b.ifCap, sc, count2);
+    launch_gs<&k_fs_isolated_a<S>, 1024, 1>(grid, bs, s, b.g, b.type, mass, aux, b.ifList2, count2, sc);
+    launch_gs<&k_fs_isolated_b<S>, 1024, 1>(grid, bs, s, b.g, b.type, mass, b.ifList2, count2, sc, b.changed, b.changedCap);
+    launch_gs<&k_fs_redistribute<S>, 1024, 1>(grid, bs, s, b.type, mass, b.ifList2, count2, b.donor, b.filled, b.newIf, sc);
 }
 
 void lb_free_surface(Box& b, const Consts& k, DevScalars* sc, hipStream_t s) {
@@ -1239,7 +1239,7 @@ __device__ inline void k_flags_compute(Geom g, const uint8_t* type, const PartLB
 void lb_flags_rebuild_compute(const Box& b, const Consts& k, const PartLB* plb, DevScalars* sc, hipStream_t s) {
     // grid-stride over the particles: most steps only test the flag and return
     if (k.nPart > 0)
-        launch<&k_flags_compute, 1024, 1>(std::min(k.nPart, FLAGS_GRID), 256, s, b.g, b.type, plb, k.nPart, b.flagNew, sc);
+        launch_gs<&k_flags_compute, 1024, 1>(std::min(k.nPart, FLAGS_GRID), 256, s, b.g, b.type, plb, k.nPart, b.flagNew, sc);
 }
 
 // clear every flag and solid index, apply the new flags (initializeParticleBoundaries)
@@ -1290,7 +1290,7 @@ void lb_flags_rebuild_apply(const Box& b, DevScalars* sc, hipStream_t s) {
     launch<&k_flags_apply_begin, 1024, 1>(1, 1, s, sc);
     const int64_t nb = (b.g.N + 255) / 256;
     const int grid = int(nb < FLAGS_GRID ? nb : FLAGS_GRID);
-    launch<&k_flags_apply, 1024, 1>(grid, 256, s, b.g.N, b.type, b.solid, b.flagNew, sc);
+    launch_gs<&k_flags_apply, 1024, 1>(grid, 256, s, b.g.N, b.type, b.solid, b.flagNew, sc);
     launch<&k_flags_apply_end, 1024, 1>(1, 1, s, sc);
 }
 
@@ -1346,8 +1346,8 @@ void lb_find_new_active(const Box& b, const Consts& k, const PartLB* plb, DevSca
     launch<&k_fna_check, 1024, 1>(1, 1, s, b.g, b.solid, plb, k.nPart, sc);
     const int64_t nb = (b.g.N + 255) / 256;
     const int grid = int(nb < 16384 ? nb : 16384);
-    launch<&k_fna_scan, 1024, 1>(grid, 256, s, b.g, b.type, b.solid, plb, k.nPart, sc);
-    launch<&k_fna_unflag, 1024, 1>(grid, 256, s, b.g, b.type, b.solid, sc);
+    launch_gs<&k_fna_scan, 1024, 1>(grid, 256, s, b.g, b.type, b.solid, plb, k.nPart, sc);
+    launch_gs<&k_fna_unflag, 1024, 1>(grid, 256, s, b.g, b.type, b.solid, sc);
     launch<&k_fna_end, 1024, 1>(1, 1, s, sc);
 }
 
@@ -1680,8 +1680,8 @@ static void obj_impl(Box& b, const Consts& k, const ObjLB* olb, DevScalars* sc, 
     else
         launch<&k_obj_uncover<S, 1>, UNCOVER_THREADS, 1>(1, UNCOVER_THREADS, s, b.g, (S*)b.f, b.type, (M*)b.mass, (C*)b.aux, b.objCand, uncCount, grid, sc, k, b.changed, b.changedCap);
     // the candidate buffer of the uncover pass stages the cover pass's per-block node lists
-    launch<&k_obj_cover_a, COVER_BS, 1>(COVER_GRID, COVER_BS, s, b.g, b.type, b.solid, olb, k.nObj, b.coverCount, b.objCand, b.objCap, b.changed, b.changedCap, sc);
-    launch<&k_obj_cover_b, COVER_BS, 1>(COVER_GRID, COVER_BS, s, b.g, b.type, olb, k.nObj, b.coverCount, b.objCand, b.objCap, b.objList, b.objCap, sc);
+    launch_gs<&k_obj_cover_a, COVER_BS, 1>(COVER_GRID, COVER_BS, s, b.g, b.type, b.solid, olb, k.nObj, b.coverCount, b.objCand, b.objCap, b.changed, b.changedCap, sc);
+    launch_gs<&k_obj_cover_b, COVER_BS, 1>(COVER_GRID, COVER_BS, s, b.g, b.type, olb, k.nObj, b.coverCount, b.objCand, b.objCap, b.objList, b.objCap, sc);
 }
 
 void lb_object_update(Box& b, const Consts& k, const ObjLB* olb, DevScalars* sc, hipStream_t s) {
