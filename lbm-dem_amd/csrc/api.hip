@@ -1115,8 +1115,6 @@ extern "C" int lbdem_profile_read(lbdem_ctx* c, double* fused_ms, int* fused_lau
 // ---------------------------------------------------------------------------------------------
 // batches: every kernel of a step pair launched once for all member boxes
 // ---------------------------------------------------------------------------------------------
-constexpr size_t BATCH_ELASTIC_BLOCKS = 16384;  // blocks of a grid-stride launch over all boxes of a group
-
 namespace {
 struct BatchOp {
     void (*launch)(dim3, dim3, size_t, hipStream_t, const void*);
@@ -1228,7 +1226,7 @@ static int batch_record(lbdem_batch* B, BatchPhase& P) {
                 gx = std::max(gx, gr.x);
             }
             // grid-stride kernels: a bounded number of blocks spread over the group's boxes
-            if (r0.ops[j].elastic) gx = std::min(gx, std::max(4u, unsigned(BATCH_ELASTIC_BLOCKS / mg.size())));
+            if (r0.ops[j].elastic) gx = std::min(gx, std::max(4u, unsigned(r0.ops[j].elastic / mg.size())));
             op.grid = dim3(gx, unsigned(mg.size()), 1);
             op.off = total;
             total += (mg.size() * r0.ops[j].args.size() + 255) & ~size_t(255);

@@ -61,7 +61,7 @@ struct RecordedLaunch {
     void (*batch)(dim3 grid, dim3 block, size_t shm, hipStream_t s, const void* table);
     dim3 grid, block;
     size_t shm;
-    bool elastic;                     // grid-stride body: correct with any number of blocks
+    unsigned elastic;                 // grid-stride body: blocks over all boxes of a batch (0: exact grid)
     std::vector<unsigned char> args;  // the body's argument tuple, as bytes
 };
 struct Recorder {
@@ -80,7 +80,7 @@ struct Launcher<Body, MAXT, MINW, void (*)(P...)> {
         hipLaunchKernelGGL((kernel_batch<Body, MAXT, MINW, P...>), g, b, shm, s, static_cast<const Tuple*>(table));
     }
     template <typename... A>
-    static void run(bool elastic, dim3 g, dim3 b, size_t shm, hipStream_t s, A&&... a) {
+    static void run(unsigned elastic, dim3 g, dim3 b, size_t shm, hipStream_t s, A&&... a) {
         if (Recorder* r = current_recorder()) {
             const Tuple t{{static_cast<P>(std::forward<A>(a))}...};
             RecordedLaunch op{&tag, &batch, g, b, shm, elastic, std::vector<unsigned char>(sizeof(Tuple))};
@@ -95,13 +95,13 @@ struct Launcher<Body, MAXT, MINW, void (*)(P...)> {
 // launch<&body, max threads per block, min waves per SIMD>(grid, block, stream, args...)
 template <auto Body, int MAXT = 1024, int MINW = 1, typename... A>
 inline void launch(dim3 grid, dim3 block, hipStream_t s, A&&... a) {
-    Launcher<Body, MAXT, MINW>::run(false, grid, block, 0, s, std::forward<A>(a)...);
+    Launcher<Body, MAXT, MINW>::run(0u, grid, block, 0, s, std::forward<A>(a)...);
 }
-// the same for a grid-stride body, whose grid a batch may shrink (it spreads a bounded number of
-// blocks over its boxes)
-template <auto Body, int MAXT = 1024, int MINW = 1, typename... A>
+// the same for a grid-stride body, whose grid a batch may shrink: it spreads BLOCKS blocks over
+// its boxes (kernels that usually return at once take few; kernels with work in most steps more)
+template <auto Body, int MAXT = 1024, int MINW = 1, unsigned BLOCKS = 2048, typename... A>
 inline void launch_gs(dim3 grid, dim3 block, hipStream_t s, A&&... a) {
-    Launcher<Body, MAXT, MINW>::run(true, grid, block, 0, s, std::forward<A>(a)...);
+    Launcher<Body, MAXT, MINW>::run(BLOCKS, grid, block, 0, s, std::forward<A>(a)...);
 }
 
 }  // namespace lbdem

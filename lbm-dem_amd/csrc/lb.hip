@@ -1239,7 +1239,7 @@ __device__ inline void k_flags_compute(Geom g, const uint8_t* type, const PartLB
 void lb_flags_rebuild_compute(const Box& b, const Consts& k, const PartLB* plb, DevScalars* sc, hipStream_t s) {
     // grid-stride over the particles: most steps only test the flag and return
     if (k.nPart > 0)
-        launch_gs<&k_flags_compute, 1024, 1>(std::min(k.nPart, FLAGS_GRID), 256, s, b.g, b.type, plb, k.nPart, b.flagNew, sc);
+        launch_gs<&k_flags_compute, 1024, 1, 16384>(std::min(k.nPart, FLAGS_GRID), 256, s, b.g, b.type, plb, k.nPart, b.flagNew, sc);
 }
 
 // clear every flag and solid index, apply the new flags (initializeParticleBoundaries)
@@ -1290,7 +1290,7 @@ void lb_flags_rebuild_apply(const Box& b, DevScalars* sc, hipStream_t s) {
     launch<&k_flags_apply_begin, 1024, 1>(1, 1, s, sc);
     const int64_t nb = (b.g.N + 255) / 256;
     const int grid = int(nb < FLAGS_GRID ? nb : FLAGS_GRID);
-    launch_gs<&k_flags_apply, 1024, 1>(grid, 256, s, b.g.N, b.type, b.solid, b.flagNew, sc);
+    launch_gs<&k_flags_apply, 1024, 1, 16384>(grid, 256, s, b.g.N, b.type, b.solid, b.flagNew, sc);
     launch<&k_flags_apply_end, 1024, 1>(1, 1, s, sc);
 }
 
