@@ -195,11 +195,12 @@ int lbdem_batch_destroy(lbdem_batch* batch);         /* the contexts remain, usa
 
 /* ---------------------------------------------------------------------------------------------
  * Run pool (no reference counterpart; batched and served runs of many independent boxes on one
- * GPU). A worker thread admits submitted runs in arrival order while their estimated device bytes
+ * GPU). A host thread admits submitted runs in arrival order while their estimated device bytes
  * fit the HBM budget and fewer than maxConcurrent runs are active, gives every run its own context
- * and stream, advances all active runs in chunks of stepsPerChunk coupled steps (goCycle, as
- * lbdem_cycle) and releases a run's device memory when its last step is done. Runs never
- * communicate: the result of a run does not depend on what else runs beside it.
+ * (memory carved from the pool's one allocation) and stream, advances the runs in chunks of
+ * stepsPerChunk coupled steps (goCycle, as lbdem_cycle) -- alone during their start-up steps,
+ * then together in batches -- and releases a run's device memory when its last step is done.
+ * Runs never communicate: the result of a run does not depend on what else runs beside it.
  * ------------------------------------------------------------------------------------------- */
 typedef struct lbdem_pool lbdem_pool;
 
@@ -253,8 +254,14 @@ int lbdem_pool_cancel(lbdem_pool* pool, int runId);
 typedef struct lbdem_pool_stats {
     int submitted, queued, active, done, failed, cancelled;
     int activePeak;              /* most runs active at once */
+    int batchBuilds;             /* batches formed (membership changes) */
+    int batchedPeak;             /* most runs stepped by one batch */
     size_t budget, bytesInUse, bytesPeak;
 } lbdem_pool_stats;
+/* 1 (default): runs past their start-up steps are stepped together in batches (one per storage
+ * precision; see lbdem_batch_create), re-formed between chunks as runs join and leave; 0: every
+ * run on its own stream. Set before the first submission. */
+int lbdem_pool_set_batching(lbdem_pool* pool, int enable);
 int lbdem_pool_get_stats(lbdem_pool* pool, lbdem_pool_stats* stats);
 double lbdem_pool_clock(const lbdem_pool* pool);
 const char* lbdem_pool_last_error(const lbdem_pool* pool);

@@ -1372,10 +1372,17 @@ extern "C" int lbdem_batch_destroy(lbdem_batch* B) {
 
 // run pool hook: the batch's argument tables in caller-owned device memory
 int lbdem::batch_use_region(lbdem_batch* B, void* base, size_t bytes) {
-    if (!B) return LBDEM_E_ARG;
-    B->region = static_cast<char*>(base);
-    B->regionSize = bytes;
+    if (!B || !base) return LBDEM_E_ARG;
+    // the kernels read their argument records with scalar loads: 256-byte aligned tables
+    const uintptr_t a = (reinterpret_cast<uintptr_t>(base) + 255) & ~uintptr_t(255);
+    const size_t skip = size_t(a - reinterpret_cast<uintptr_t>(base));
+    if (skip >= bytes) return LBDEM_E_ARG;
+    B->region = reinterpret_cast<char*>(a);
+    B->regionSize = (bytes - skip) & ~size_t(255);
     B->regionUsed = 0;
     return 0;
 }
 hipStream_t lbdem::batch_stream(const lbdem_batch* B) { return B ? B->stream : nullptr; }
+bool lbdem::ctx_steady(const lbdem_ctx* c) { return c && steady(c); }
+int lbdem::ctx_phase(const lbdem_ctx* c) { return c ? graph_phase(c) : -1; }
+int lbdem::ctx_precision(const lbdem_ctx* c) { return c ? c->box.prec : -1; }

@@ -274,5 +274,9 @@ hipStream_t ctx_stream(const lbdem_ctx* c);
 // a batch's argument tables in [base, base+bytes) (set before its first cycle); its stream
 int batch_use_region(lbdem_batch* b, void* base, size_t bytes);
 hipStream_t batch_stream(const lbdem_batch* b);
+// start-up over (a batch may take the context); AA/history phase; storage precision
+bool ctx_steady(const lbdem_ctx* c);
+int ctx_phase(const lbdem_ctx* c);
+int ctx_precision(const lbdem_ctx* c);
 
 }  // namespace lbdem

@@ -5,18 +5,27 @@
 // bookkeeping step and release never synchronises the device). A run is admitted, in arrival
 // order, when its block fits and fewer than maxConcurrent runs are active.
 //
-// Threads: the stepper thread enqueues chunks of coupled steps (lbdem_cycle: captured step
-// pairs replayed from HIP graphs) on every active run's own stream, at most two chunks in
-// flight per run, and polls the chunks' events; the helper thread builds contexts (host-side
-// node typing, uploads) and finishes runs (result download, release), so neither stalls the
-// enqueueing of the others. Every context is used by one thread at a time. Runs share nothing
-// on the device: a run's result is the one it has alone.
+// Stepping: a run first advances alone on its own stream (its start-up steps: the impactor wait,
+// the tile tuner, the first LB step). Runs past their start-up then advance together in a batch
+// per storage precision (lbdem_batch: every kernel launched once for all members, step pairs
+// replayed from a captured graph), in chunks of stepsPerChunk steps with at most two chunks in
+// flight. A batch is re-formed only when it has no chunk in flight: when members finish or are
+// cancelled, or when runs are waiting to join (a quarter of the batch's size, or one that has
+// waited 50 ms). A run with a single step left, or with the other step parity, takes that step
+// alone (a batch may hold a single run). The batches' argument tables live in a pool-owned
+// region, so re-forming a batch frees nothing on the device.
+//
+// Threads: the stepper thread admits runs, enqueues chunks and polls their events; the helper
+// thread builds contexts (host-side node typing, uploads) and finishes runs (result download,
+// release), so neither stalls the other's work. Every context is used by one thread at a time.
+// Runs share nothing on the device: a run's result is the one it has alone.
 #include "lbdem_internal.hpp"
 
 #include <algorithm>
 #include <atomic>
 #include <chrono>
 #include <condition_variable>
+#include <cstdlib>
 #include <cstring>
 #include <deque>
 #include <iterator>
@@ -33,6 +42,8 @@ using Clock = std::chrono::steady_clock;
 
 constexpr size_t BLOCK_ALIGN = 4096;
 constexpr int IN_FLIGHT = 2;
+constexpr size_t BATCH_REGION = size_t(64) << 20;  // argument tables of the batches (all precisions)
+constexpr double JOIN_WAIT_S = 0.05;                // longest a ready run waits for a re-formed batch
 
 size_t round_block(size_t b) { return (std::max<size_t>(b, 1) + BLOCK_ALIGN - 1) & ~(BLOCK_ALIGN - 1); }
 
@@ -101,6 +112,17 @@ struct Run {
     std::string message;
     std::deque<Chunk> inflight;
     lbdem_run_result res{};
+    // stepper-owned
+    bool batched = false;  // stepped by a batch
+    bool solo = false;     // a batch could not take it: stepped alone to the end
+    double readySince = -1.0;
+};
+
+// a batch of runs of one storage precision (stepper-owned)
+struct PoolBatch {
+    lbdem_batch* b = nullptr;
+    std::vector<Run*> members;
+    std::deque<Chunk> inflight;
 };
 
 }  // namespace
@@ -109,8 +131,10 @@ struct lbdem_pool {
     int device = 0;
     int maxConcurrent = 1;
     int chunk = 32;
+    bool batching = true;
     size_t budget = 0;
     char* base = nullptr;
+    char* batchRegion = nullptr;
     Arena arena;
     Clock::time_point t0;
 
@@ -149,6 +173,7 @@ void init_run(lbdem_pool* P, Run& r) {
     if (!rc) rc = lbdem_set_particles(r.ctx, int(r.r.size()), r.size.data(), r.r.data(), r.x0.data());
     if (!rc) rc = lbdem_set_objects(r.ctx, int(r.objR.size()), r.objR.data(), r.objX0.data(), r.objX1.data());
     if (!rc) rc = lbdem_init(r.ctx);
+    if (!rc && P->batching) rc = lbdem_use_graphs(r.ctx, 0);  // alone only for start-up steps
     if (rc) {
         r.failed = true;
         r.error = rc;
@@ -237,11 +262,55 @@ void helper_loop(lbdem_pool* P) {
     }
 }
 
+// events of chunks: reused, created on demand (stepper thread only)
+struct EventPool {
+    std::vector<hipEvent_t> spare;
+    hipEvent_t get() {
+        if (!spare.empty()) {
+            hipEvent_t e = spare.back();
+            spare.pop_back();
+            return e;
+        }
+        hipEvent_t e = nullptr;
+        return hipEventCreateWithFlags(&e, hipEventDisableTiming) == hipSuccess ? e : nullptr;
+    }
+    void put(hipEvent_t e) { spare.push_back(e); }
+    ~EventPool() {
+        for (hipEvent_t e : spare) (void)hipEventDestroy(e);
+    }
+};
+
+void fail_run(Run& r, int code, const std::string& m) {
+    if (r.failed) return;
+    r.failed = true;
+    r.error = code;
+    r.message = m;
+}
+
+// completed chunks at the front of a queue; returns the steps they held (-1 on a device error)
+int poll_chunks(std::deque<Chunk>& q, EventPool& ev, std::string& err) {
+    int steps = 0;
+    while (!q.empty()) {
+        const hipError_t e = hipEventQuery(q.front().ev);
+        if (e == hipErrorNotReady) break;
+        if (e != hipSuccess) {
+            err = std::string("HIP error: ") + hipGetErrorString(e);
+            return -1;
+        }
+        steps += q.front().steps;
+        ev.put(q.front().ev);
+        q.pop_front();
+    }
+    return steps;
+}
+
 // stepper thread: admission, chunk enqueueing and completion polling
 void stepper_loop(lbdem_pool* P) {
     (void)hipSetDevice(P->device);
-    std::vector<hipEvent_t> spare;
+    EventPool ev;
     std::vector<Run*> mine;
+    PoolBatch batches[3];  // by storage precision
+    int batchBuilds = 0, batchedPeak = 0;
     std::unique_lock<std::mutex> lk(P->mu);
     for (;;) {
         // admission in arrival order: the head waits until its block fits
@@ -262,52 +331,147 @@ void stepper_loop(lbdem_pool* P) {
         if (P->stop && P->active == 0 && P->queue.empty()) break;
         mine.clear();
         for (int id : P->stepping) mine.push_back(P->runs[id].get());
+        std::vector<char> cancelled(mine.size());
+        for (size_t k = 0; k < mine.size(); ++k) cancelled[k] = mine[k]->cancel;
+        P->stats.batchBuilds = batchBuilds;
+        P->stats.batchedPeak = batchedPeak;
         lk.unlock();
+        const double now = P->now();
 
         bool progress = false;
         std::vector<int> finished;
-        for (Run* rp : mine) {
-            Run& r = *rp;
-            while (!r.inflight.empty()) {
-                const hipError_t q = hipEventQuery(r.inflight.front().ev);
-                if (q == hipErrorNotReady) break;
-                if (q != hipSuccess) {
-                    r.failed = true;
-                    r.error = LBDEM_E_DEVICE;
-                    r.message = std::string("HIP error: ") + hipGetErrorString(q);
-                    break;
-                }
-                r.done.fetch_add(r.inflight.front().steps);
-                spare.push_back(r.inflight.front().ev);
-                r.inflight.pop_front();
+        // completed chunks of runs stepping alone, then of the batches
+        for (Run* r : mine) {
+            std::string err;
+            const int k = poll_chunks(r->inflight, ev, err);
+            if (k < 0) fail_run(*r, LBDEM_E_DEVICE, err);
+            else if (k) { r->done.fetch_add(k); progress = true; }
+        }
+        for (PoolBatch& B : batches) {
+            std::string err;
+            const int k = poll_chunks(B.inflight, ev, err);
+            if (k < 0)
+                for (Run* m : B.members) fail_run(*m, LBDEM_E_DEVICE, err);
+            else if (k) {
+                for (Run* m : B.members) m->done.fetch_add(k);
                 progress = true;
             }
-            bool cancel;
-            {
-                std::lock_guard<std::mutex> g(P->mu);
-                cancel = r.cancel;
-            }
-            while (!r.failed && !cancel && int(r.inflight.size()) < IN_FLIGHT && r.enqueued < r.nsteps) {
-                const int n = std::min(P->chunk, r.nsteps - r.enqueued);
-                int rc = lbdem_cycle(r.ctx, n);
-                hipEvent_t ev = nullptr;
-                if (!rc) {
-                    if (!spare.empty()) {
-                        ev = spare.back();
-                        spare.pop_back();
-                    } else if (hipEventCreateWithFlags(&ev, hipEventDisableTiming) != hipSuccess) {
-                        rc = LBDEM_E_DEVICE;
-                    }
+        }
+        auto is_cancelled = [&](Run* r) {
+            for (size_t k = 0; k < mine.size(); ++k)
+                if (mine[k] == r) return bool(cancelled[k]);
+            return false;
+        };
+        // runs ready to join a batch: start-up over, idle, two or more steps to go
+        auto ready = [&](Run* r) {
+            return !r->batched && !r->solo && !r->failed && !is_cancelled(r) && r->inflight.empty() &&
+                   r->nsteps - r->enqueued >= 2 && lbdem::ctx_steady(r->ctx);
+        };
+        bool reform[3] = {false, false, false};  // a batch waits for its chunks, then is re-formed
+        for (int pr = 0; P->batching && pr < 3; ++pr) {
+            PoolBatch& B = batches[pr];
+            std::vector<Run*> waiting;
+            for (Run* r : mine)
+                if (lbdem::ctx_precision(r->ctx) == pr && ready(r)) {
+                    if (r->readySince < 0) r->readySince = now;
+                    waiting.push_back(r);
                 }
-                if (!rc && hipEventRecord(ev, lbdem::ctx_stream(r.ctx)) != hipSuccess) rc = LBDEM_E_DEVICE;
+            bool leave = false;
+            for (Run* m : B.members)
+                leave = leave || m->failed || is_cancelled(m) || m->nsteps - m->enqueued < 2;
+            double oldest = 1e300;
+            for (Run* r : waiting) oldest = std::min(oldest, r->readySince);
+            const bool join = !waiting.empty() &&
+                              (waiting.size() * 4 >= std::max<size_t>(B.members.size(), 1) || now - oldest > JOIN_WAIT_S ||
+                               B.members.empty());
+            if (!(leave || join)) continue;
+            reform[pr] = true;
+            if (!B.inflight.empty()) continue;  // re-formed once its chunks are done (no new ones go out)
+            // members and joiners at one step parity (the batch's, or the first joiner's)
+            std::vector<Run*> keep;
+            for (Run* m : B.members)
+                if (!m->failed && !is_cancelled(m) && m->nsteps - m->enqueued >= 2) keep.push_back(m);
+            const int phase = !keep.empty() ? lbdem::ctx_phase(keep[0]->ctx)
+                                            : (!waiting.empty() ? lbdem::ctx_phase(waiting[0]->ctx) : -1);
+            for (Run* r : waiting)
+                if (lbdem::ctx_phase(r->ctx) == phase) keep.push_back(r);
+            if (B.b) lbdem_batch_destroy(B.b);
+            B.b = nullptr;
+            for (Run* m : B.members) m->batched = false;
+            B.members.clear();
+            if (!keep.empty()) {
+                std::vector<lbdem_ctx*> ctxs;
+                for (Run* r : keep) ctxs.push_back(r->ctx);
+                if (lbdem_batch_create(ctxs.data(), int(ctxs.size()), &B.b) == 0) {
+                    const size_t slice = (BATCH_REGION / 3) & ~size_t(255);  // tables are read as aligned records
+                    lbdem::batch_use_region(B.b, P->batchRegion + pr * slice, slice);
+                    B.members = keep;
+                    for (Run* r : keep) { r->batched = true; r->readySince = -1.0; }
+                    ++batchBuilds;
+                    batchedPeak = std::max(batchedPeak, int(keep.size()));
+                } else {
+                    B.b = nullptr;
+                    for (Run* r : keep) r->solo = true;  // step alone
+                }
+            }
+            progress = true;
+        }
+        // batch chunks: none while the batch is due to be re-formed
+        for (int pr = 0; pr < 3; ++pr) {
+            PoolBatch& B = batches[pr];
+            if (!B.b || reform[pr]) continue;
+            while (int(B.inflight.size()) < IN_FLIGHT) {
+                int n = P->chunk;
+                for (Run* m : B.members) n = std::min(n, m->nsteps - m->enqueued);
+                n -= n % 2;
+                if (n < 2) break;
+                hipEvent_t e = ev.get();
+                int rc = e ? lbdem_batch_cycle(B.b, n) : LBDEM_E_DEVICE;
+                if (!rc && hipEventRecord(e, lbdem::batch_stream(B.b)) != hipSuccess) rc = LBDEM_E_DEVICE;
                 if (rc) {
-                    if (ev) spare.push_back(ev);
-                    r.failed = true;
-                    r.error = rc;
-                    r.message = ctx_error(r.ctx);
+                    if (e) ev.put(e);
+                    const std::string m = lbdem_batch_last_error(B.b);
+                    for (Run* r : B.members) fail_run(*r, rc, "batch: " + m);
                     break;
                 }
-                r.inflight.push_back({ev, n});
+                B.inflight.push_back({e, n});
+                for (Run* m : B.members) m->enqueued += n;
+                progress = true;
+            }
+        }
+        // runs stepping alone: start-up steps, a parity step, the last single step, or everything
+        // when batching is off
+        for (Run* rp : mine) {
+            Run& r = *rp;
+            const bool cancel = is_cancelled(rp);
+            if (r.batched) continue;
+            while (!r.failed && !cancel && int(r.inflight.size()) < IN_FLIGHT && r.enqueued < r.nsteps) {
+                int n = std::min(P->chunk, r.nsteps - r.enqueued);
+                if (P->batching && !r.solo) {
+                    if (lbdem::ctx_steady(r.ctx) && r.inflight.empty()) {
+                        if (r.nsteps - r.enqueued >= 2) {
+                            // wants a batch; a run at the other parity than the batch takes one step
+                            const PoolBatch& B = batches[lbdem::ctx_precision(r.ctx)];
+                            const bool parityOff = !B.members.empty() &&
+                                                   lbdem::ctx_phase(B.members[0]->ctx) != lbdem::ctx_phase(r.ctx);
+                            if (!parityOff) break;
+                            n = 1;
+                        }
+                    } else if (lbdem::ctx_steady(r.ctx)) {
+                        break;  // wait for its chunks, then join
+                    } else {
+                        n = std::min(n, 12);  // start-up steps in small chunks
+                    }
+                }
+                hipEvent_t e = ev.get();
+                int rc = e ? lbdem_cycle(r.ctx, n) : LBDEM_E_DEVICE;
+                if (!rc && hipEventRecord(e, lbdem::ctx_stream(r.ctx)) != hipSuccess) rc = LBDEM_E_DEVICE;
+                if (rc) {
+                    if (e) ev.put(e);
+                    fail_run(r, rc, ctx_error(r.ctx));
+                    break;
+                }
+                r.inflight.push_back({e, n});
                 r.enqueued += n;
                 progress = true;
             }
@@ -326,7 +490,10 @@ void stepper_loop(lbdem_pool* P) {
         }
     }
     lk.unlock();
-    for (hipEvent_t e : spare) (void)hipEventDestroy(e);
+    for (PoolBatch& B : batches) {
+        if (B.b) lbdem_batch_destroy(B.b);
+        for (const Chunk& c : B.inflight) (void)hipEventDestroy(c.ev);
+    }
 }
 
 }  // namespace
@@ -355,6 +522,15 @@ extern "C" int lbdem_pool_create(size_t budget, int maxConcurrent, int stepsPerC
     P->arena.reset(budget);
     P->stats.budget = budget;
     P->t0 = Clock::now();
+    void* region = nullptr;
+    if (hipMalloc(&region, BATCH_REGION) != hipSuccess) {
+        (void)hipGetLastError();
+        (void)hipFree(base);
+        delete P;
+        return LBDEM_E_NOMEM;
+    }
+    P->batchRegion = static_cast<char*>(region);
+    if (const char* e = std::getenv("LBDEM_POOL_BATCH")) P->batching = std::atoi(e) != 0;
     P->stepper = std::thread(stepper_loop, P);
     P->helper = std::thread(helper_loop, P);
     *out = P;
@@ -513,7 +689,19 @@ extern "C" int lbdem_pool_destroy(lbdem_pool* P) {
     }
     P->helper.join();
     (void)hipSetDevice(P->device);
+    (void)hipFree(P->batchRegion);
     (void)hipFree(P->base);
     delete P;
+    return 0;
+}
+
+extern "C" int lbdem_pool_set_batching(lbdem_pool* P, int enable) {
+    if (!P) return LBDEM_E_ARG;
+    std::lock_guard<std::mutex> g(P->mu);
+    if (!P->runs.empty()) {
+        P->err = "batching is set before the first submission";
+        return LBDEM_E_STATE;
+    }
+    P->batching = enable != 0;
     return 0;
 }

@@ -117,7 +117,8 @@ class Tuning(C.Structure):
 class PoolStats(C.Structure):
     _fields_ = [
         ("submitted", C.c_int), ("queued", C.c_int), ("active", C.c_int), ("done", C.c_int),
-        ("failed", C.c_int), ("cancelled", C.c_int), ("activePeak", C.c_int),
+        ("failed", C.c_int), ("cancelled", C.c_int), ("activePeak", C.c_int), ("batchBuilds", C.c_int),
+        ("batchedPeak", C.c_int),
         ("budget", C.c_size_t), ("bytesInUse", C.c_size_t), ("bytesPeak", C.c_size_t),
     ]
 
@@ -180,6 +181,7 @@ class Library:
             "lbdem_pool_drain": ([vp, C.c_double], C.c_int),
             "lbdem_pool_cancel": ([vp, C.c_int], C.c_int),
             "lbdem_pool_get_stats": ([vp, C.POINTER(PoolStats)], C.c_int),
+            "lbdem_pool_set_batching": ([vp, C.c_int], C.c_int),
             "lbdem_pool_clock": ([vp], C.c_double),
             "lbdem_pool_last_error": ([vp], C.c_char_p),
             "lbdem_pool_destroy": ([vp], C.c_int),
@@ -396,13 +398,14 @@ class Pool:
     active-node count after the last step."""
 
     def __init__(self, lib: Library, budget_bytes: int = 0, max_concurrent: int = 64, steps_per_chunk: int = 32,
-                 device: int = 0):
+                 device: int = 0, batching: bool = True):
         self.lib = lib
         self.pool = C.c_void_p()
         rc = lib.lib.lbdem_pool_create(int(budget_bytes), int(max_concurrent), int(steps_per_chunk), int(device),
                                        C.byref(self.pool))
         if rc != 0:
             raise LbdemError(f"lbdem_pool_create failed with code {rc}")
+        self._check(lib.lib.lbdem_pool_set_batching(self.pool, 1 if batching else 0), "lbdem_pool_set_batching")
 
     def _check(self, rc: int, what: str):
         if rc != 0:

@@ -1473,6 +1473,7 @@ int lbdem_pool_wait(lbdem_pool*, int, double, lbdem_run_result*) { return LBDEM_
 int lbdem_pool_drain(lbdem_pool*, double) { return LBDEM_E_STATE; }
 int lbdem_pool_cancel(lbdem_pool*, int) { return LBDEM_E_STATE; }
 int lbdem_pool_get_stats(lbdem_pool*, lbdem_pool_stats*) { return LBDEM_E_STATE; }
+int lbdem_pool_set_batching(lbdem_pool*, int) { return LBDEM_E_STATE; }
 double lbdem_pool_clock(const lbdem_pool*) { return 0.0; }
 const char* lbdem_pool_last_error(const lbdem_pool*) { return "the CPU oracle has no run pool"; }
 int lbdem_pool_destroy(lbdem_pool*) { return 0; }

@@ -84,9 +84,12 @@ def test_graphs_do_not_change_results(gpu_lib, runs, alone_results):
     same(alone(gpu_lib, run, prec, graphs=False), alone_results[0], "eager vs graph replay")
 
 
-def test_pool_results_equal_alone(gpu_lib, runs, alone_results):
+@pytest.mark.parametrize("batching", [True, False], ids=["batches", "streams"])
+def test_pool_results_equal_alone(gpu_lib, runs, alone_results, batching):
+    """Runs of three precisions and six lengths (even and odd) submitted together: stepped in
+    batches after their start-up steps (or each on its own stream), every result is the run's own."""
     budget = int(1.05 * sum(estimate(gpu_lib, r, p) for r, p in runs)) + (1 << 20) * len(runs)
-    pool = lbdem.Pool(gpu_lib, budget_bytes=budget, max_concurrent=8, steps_per_chunk=16)
+    pool = lbdem.Pool(gpu_lib, budget_bytes=budget, max_concurrent=8, steps_per_chunk=16, batching=batching)
     ids = submit_all(pool, runs)
     pool.drain(300.0)
     st = pool.stats()
@@ -101,6 +104,10 @@ def test_pool_results_equal_alone(gpu_lib, runs, alone_results):
     assert st["done"] == len(runs) and st["failed"] == 0
     assert st["activePeak"] >= 2, "runs did not overlap"
     assert st["bytesPeak"] <= budget and st["bytesInUse"] == 0
+    if batching:
+        assert st["batchBuilds"] >= 1 and st["batchedPeak"] >= 2, "no batch formed"
+    else:
+        assert st["batchBuilds"] == 0
     pool.close()
 
 
