@@ -237,6 +237,70 @@ def served(prec, n_req, capacity_mlups, load=0.8, seed=20261204):
             "runs_per_hour_ratio": round(pool_out["runs_per_hour"] / seq_out["runs_per_hour"], 3)}
 
 
+def precision_sweep(vals, ps, obs, checkpoints=(100, 1000, 5000), every=100):
+    """The headline state run in FP64 (the comparison base), FP32-offset and FP16-storage
+    populations side by side: impactor depth (its drop since step 0) every ``every`` steps, total
+    fluid mass drift, and at each checkpoint the errors of the FP32/FP16 runs against FP64 over
+    the nodes active in both (mass and density fields, maximum absolute difference over the FP64
+    field's largest value) plus the largest particle position difference. Mass drift is measured
+    from the end of the first step."""
+    lib = lbdem.Library.load()
+    precs = (lbdem.PREC_FP64, lbdem.PREC_FP32, lbdem.PREC_FP16)
+    sims = {}
+    for prec in precs:
+        sm = lbdem.Simulation(lib, lbdem.params_from_values(vals, prec), ps, obs)
+        sm.init()
+        sims[prec] = sm
+    y0 = {prec: float(sm.objects()[0, lbdem.O["X0"] + 1]) for prec, sm in sims.items()}
+    # mass baseline after the first step (its coupling step takes the nodes inside spheres out of
+    # the fluid mass, LB.cpp:388-395 counts them at initialisation)
+    for sm in sims.values():
+        sm.cycle(1)
+    m0 = {prec: sm.total_mass() for prec, sm in sims.items()}
+    depth = {PREC_NAME[p]: [] for p in precs}
+    out = {"steps": [], "checkpoints": {}, "mass_baseline_step": 1}
+    done = 1
+    for target in range(every, max(checkpoints) + 1, every):
+        for sm in sims.values():
+            sm.cycle(target - done)
+        for sm in sims.values():
+            sm.synchronize()
+        done = target
+        out["steps"].append(done)
+        for prec, sm in sims.items():
+            depth[PREC_NAME[prec]].append(round(y0[prec] - float(sm.objects()[0, lbdem.O["X0"] + 1]), 6))
+        if done in checkpoints:
+            ref = sims[lbdem.PREC_FP64]
+            t_ref, _, _ = ref.types()
+            _, m_ref, n_ref = ref.fields(want_f=False)
+            p_ref = ref.particles()[:, 0:3]
+            d_ref = depth["fp64"][-1]
+            cp = {}
+            for prec in (lbdem.PREC_FP32, lbdem.PREC_FP16):
+                sm = sims[prec]
+                t, _, _ = sm.types()
+                _, m, n = sm.fields(want_f=False)
+                act = ((t_ref == 0) | (t_ref == 3)) & ((t == 0) | (t == 3))
+                d = depth[PREC_NAME[prec]][-1]
+                cp[PREC_NAME[prec]] = {
+                    "impactor_depth": d, "impactor_depth_fp64": d_ref,
+                    "impactor_depth_rel_err": abs(d - d_ref) / max(abs(d_ref), 1e-30),
+                    "mass_field_err": float(np.abs(m[act] - m_ref[act]).max() / np.abs(m_ref[act]).max()),
+                    "density_field_err": float(np.abs(n[act] - n_ref[act]).max() / np.abs(n_ref[act]).max()),
+                    "node_type_mismatch": int(np.count_nonzero(t != t_ref)),
+                    "particle_position_max_abs_diff": float(np.abs(sm.particles()[:, 0:3] - p_ref).max()),
+                    "total_mass_drift_rel": (sm.total_mass() - m0[prec]) / m0[prec],
+                }
+            cp["fp64"] = {"impactor_depth": d_ref,
+                          "total_mass_drift_rel": (ref.total_mass() - m0[lbdem.PREC_FP64]) / m0[lbdem.PREC_FP64]}
+            out["checkpoints"][str(done)] = cp
+            log(f"[bench] precision sweep step {done}: {cp}")
+    out["impactor_depth"] = depth
+    for sm in sims.values():
+        sm.close()
+    return out
+
+
 def copy_probe(nbytes=1 << 30, reps=3):
     """A plain device-to-device copy of known size (torch's copy kernel): reads and writes
     ``nbytes`` per launch, the reference for scaling the HBM byte counters of rocprofv3."""
@@ -266,6 +330,8 @@ def main():
     ap.add_argument("--batched-steps", type=int, default=2000)
     ap.add_argument("--served", type=int, default=24, help="requests in the served run (0: skip)")
     ap.add_argument("--headline-only", action="store_true", help="skip the batched and served runs")
+    ap.add_argument("--precision-sweep", type=int, default=0, metavar="STEPS",
+                    help="only the precision sweep on the headline state (checkpoints 100, 1000, 5000 up to STEPS)")
     ap.add_argument("--copy-probe", action="store_true",
                     help="only run a plain 1 GiB device copy three times (calibrates HBM byte counters)")
     args = ap.parse_args()
@@ -284,6 +350,12 @@ def main():
     vals, ps, obs = workloads.impact_box(**wl)
     phi = workloads.volume_fraction(ps, wl["size"], wl["fluid_height"])
     log(f"[bench] packing {len(ps.r)} spheres phi={phi:.3f} in {time.time() - t0:.1f} s")
+    if args.precision_sweep:
+        cps = tuple(c for c in (100, 1000, 5000) if c <= args.precision_sweep) or (args.precision_sweep,)
+        res = precision_sweep(vals, ps, obs, cps)
+        print(json.dumps({"precision_sweep": res, "workload": "headline impact run (256x256x320, 31000 spheres)"}),
+              flush=True)
+        return
     main_res = run_gpu(vals, ps, obs, prec, args.steps, args.warmup)
     fp64 = None
     if not args.no_fp64 and prec != lbdem.PREC_FP64:

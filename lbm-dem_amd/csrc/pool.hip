@@ -15,8 +15,8 @@
 // alone (a batch may hold a single run). The batches' argument tables live in a pool-owned
 // region, so re-forming a batch frees nothing on the device.
 //
-// Threads: the stepper thread admits runs, enqueues chunks and polls their events; the helper
-// thread builds contexts (host-side node typing, uploads) and finishes runs (result download,
+// Threads: the stepper thread admits runs, enqueues chunks and polls their events; HELPERS helper
+// threads build contexts (host-side node typing, uploads) and finish runs (result download,
 // release), so neither stalls the other's work. Every context is used by one thread at a time.
 // Runs share nothing on the device: a run's result is the one it has alone.
 #include "lbdem_internal.hpp"
@@ -44,6 +44,7 @@ constexpr size_t BLOCK_ALIGN = 4096;
 constexpr int IN_FLIGHT = 2;
 constexpr size_t BATCH_REGION = size_t(64) << 20;  // argument tables of the batches (all precisions)
 constexpr double JOIN_WAIT_S = 0.05;                // longest a ready run waits for a re-formed batch
+constexpr int HELPERS = 4;                          // threads building and finishing runs
 
 size_t round_block(size_t b) { return (std::max<size_t>(b, 1) + BLOCK_ALIGN - 1) & ~(BLOCK_ALIGN - 1); }
 
@@ -149,7 +150,8 @@ struct lbdem_pool {
     lbdem_pool_stats stats{};
     std::string err;
 
-    std::thread stepper, helper;
+    std::thread stepper;
+    std::vector<std::thread> helpers;
 
     double now() const { return std::chrono::duration<double>(Clock::now() - t0).count(); }
 };
@@ -532,7 +534,7 @@ extern "C" int lbdem_pool_create(size_t budget, int maxConcurrent, int stepsPerC
     P->batchRegion = static_cast<char*>(region);
     if (const char* e = std::getenv("LBDEM_POOL_BATCH")) P->batching = std::atoi(e) != 0;
     P->stepper = std::thread(stepper_loop, P);
-    P->helper = std::thread(helper_loop, P);
+    for (int h = 0; h < HELPERS; ++h) P->helpers.emplace_back(helper_loop, P);
     *out = P;
     return 0;
 }
@@ -687,7 +689,7 @@ extern "C" int lbdem_pool_destroy(lbdem_pool* P) {
         std::lock_guard<std::mutex> g(P->mu);
         P->cvHelp.notify_all();
     }
-    P->helper.join();
+    for (std::thread& h : P->helpers) h.join();
     (void)hipSetDevice(P->device);
     (void)hipFree(P->batchRegion);
     (void)hipFree(P->base);
