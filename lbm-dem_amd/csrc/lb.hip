@@ -1243,18 +1243,23 @@ void lb_flags_rebuild_compute(const Box& b, const Consts& k, const PartLB* plb, 
 }
 
 // clear every flag and solid index, apply the new flags (initializeParticleBoundaries)
-__device__ inline void k_flags_apply(int64_t N, uint8_t* type, int32_t* solid, int32_t* flagNew, DevScalars* sc) {
+__device__ inline void k_flags_apply(int64_t N, uint8_t* type, int32_t* solid, int32_t* flagNew, DevScalars* sc,
+                                     int32_t* changed, int changedCap) {
     if (!sc->newNeighborList) return;
     __shared__ int bmin;
     if (threadIdx.x == 0) bmin = INT_MAX;
     __syncthreads();
     int mymin = INT_MAX;
-    bool remark = false;  // a cleared wall or object index: link classes need a full pass
+    bool remark = false;  // a cleared wall index: link classes need a full pass
     for (int64_t i = int64_t(blockIdx.x) * blockDim.x + threadIdx.x; i < N; i += int64_t(gridDim.x) * blockDim.x) {
         const int32_t s = flagNew[i];
         const uint8_t t = type[i];
+        // an object node's index is part of its neighbours' link classes: re-mark around it
+        // (a node covered by an impactor after the flags were computed takes a particle index)
+        const bool obj = (t & T_BASE) == T_OBJ;
         if (s >= 0) {
-            type[i] = uint8_t(t | T_PMASK);  // flagged nodes are active: the edge bit stays
+            if (obj && solid[i] != s) push_changed(changed, changedCap, sc, int32_t(i));
+            type[i] = uint8_t(t | T_PMASK);  // flagged nodes were active: the edge bit stays
             solid[i] = s;
             flagNew[i] = -1;
             mymin = min(mymin, int(i));
@@ -1262,8 +1267,7 @@ __device__ inline void k_flags_apply(int64_t N, uint8_t* type, int32_t* solid, i
             // active nodes keep their edge bit; other nodes lose the flag and a wall index
             const uint8_t nt = is_active(t) ? uint8_t(t & ~T_PMASK) : uint8_t(t & T_BASE);
             if (!is_active(t) && (t & 0x70)) remark = true;
-            // an object node's index is part of its neighbours' link classes
-            if ((t & T_BASE) == T_OBJ && solid[i] != 0) remark = true;
+            if (obj && solid[i] != 0) push_changed(changed, changedCap, sc, int32_t(i));
             if (nt != t) type[i] = nt;
             solid[i] = 0;
         }
@@ -1290,7 +1294,8 @@ void lb_flags_rebuild_apply(const Box& b, DevScalars* sc, hipStream_t s) {
     launch<&k_flags_apply_begin, 1024, 1>(1, 1, s, sc);
     const int64_t nb = (b.g.N + 255) / 256;
     const int grid = int(nb < FLAGS_GRID ? nb : FLAGS_GRID);
-    launch_gs<&k_flags_apply, 1024, 1, 16384>(grid, 256, s, b.g.N, b.type, b.solid, b.flagNew, sc);
+    launch_gs<&k_flags_apply, 1024, 1, 16384>(grid, 256, s, b.g.N, b.type, b.solid, b.flagNew, sc, b.changed,
+                                              b.changedCap);
     launch<&k_flags_apply_end, 1024, 1>(1, 1, s, sc);
 }
 
