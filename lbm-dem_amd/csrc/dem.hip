@@ -781,7 +781,7 @@ void dem_step(const DemParams& p, DemState& d, DevScalars* sc, int demTimeStep, 
         f.xiOld = d.xiCur; f.xiNew = d.xiNext;
         f.ox0 = d.ox0; f.ox1 = d.ox1; f.ow1 = d.ow1; f.or_ = d.or_;
         f.objContrib = d.objContrib; f.wallContrib = d.wallContrib; f.demTimeStep = demTimeStep;
-        launch<&k_dem_forces, 1024, 1>((p.nPart * DEM_LANES + 255) / 256, 256, s, f, p.dampCoeff);
+        launch<&k_dem_forces, 256, 1>((p.nPart * DEM_LANES + 255) / 256, 256, s, f, p.dampCoeff);
         if (p.nObj > 0 || p.nWall > 0) {
             launch<&k_dem_reduce_a, RED_BS, 1>(2 * RED_BLOCKS, RED_BS, s, d.objContrib, 12 * p.nObj, d.wallContrib, 6 * p.nWall, p.nPart, d.redPart);
             launch<&k_dem_reduce_b, 1024, 1>(1, 128, s, d.redPart, 12 * p.nObj, 6 * p.nWall, d.objSum, d.wallSum);

@@ -34,23 +34,6 @@ __device__ inline void cell_xyz(const Geom& g, int64_t i, int& x, int& y, int& z
     x = int(rem - int64_t(y) * g.X);
 }
 
-// Per-axis index offsets of the wrapped neighbour rule: nb(q) = i + ox[cx+1] + oy[cy+1] + oz[cz+1]
-// (offsets fit 32 bits: initialisation rejects boxes with 19*N >= 2^31)
-struct Offs {
-    int32_t ox[3], oy[3], oz[3];
-    __device__ Offs(const Geom& g, int x, int y, int z) {
-        const int32_t sy = g.X, sz = g.X * g.Y;
-        ox[1] = 0; oy[1] = 0; oz[1] = 0;
-        ox[0] = (x - 1 == 0 && g.per[0]) ? g.X - 3 : -1;
-        ox[2] = (x + 1 == g.X - 1 && g.per[1]) ? -(g.X - 3) : 1;
-        oy[0] = (y - 1 == 0 && g.per[2]) ? (g.Y - 3) * sy : -sy;
-        oy[2] = (y + 1 == g.Y - 1 && g.per[3]) ? -(g.Y - 3) * sy : sy;
-        oz[0] = (z - 1 == 0 && g.per[4]) ? (g.Z - 3) * sz : -sz;
-        oz[2] = (z + 1 == g.Z - 1 && g.per[5]) ? -(g.Z - 3) * sz : sz;
-    }
-    __device__ int32_t d(int q) const { return ox[cxq(q) + 1] + oy[cyq(q) + 1] + oz[czq(q) + 1]; }
-};
-
 // lattice vectors for a direction index known only at run time: 2-bit fields of packed words
 constexpr uint64_t pack_dir(int axis) {
     uint64_t k = 0;
@@ -69,15 +52,34 @@ __device__ inline int cy_rt(int q) { return int((PACK_CY >> (2 * q)) & 3u) - 1; 
 __device__ inline int cz_rt(int q) { return int((PACK_CZ >> (2 * q)) & 3u) - 1; }
 __device__ inline int opp_rt(int q) { return q == 0 ? 0 : ((q - 1) ^ 1) + 1; }
 __device__ inline double w_rt(int q) { return q == 0 ? 12.0 / 36.0 : (q < 7 ? 2.0 / 36.0 : 1.0 / 36.0); }
-__device__ inline int32_t offs_rt(const Offs& o, int q) {
-    const int cx = cx_rt(q), cy = cy_rt(q), cz = cz_rt(q);
-    return (cx < 0 ? o.ox[0] : (cx > 0 ? o.ox[2] : 0)) + (cy < 0 ? o.oy[0] : (cy > 0 ? o.oy[2] : 0)) +
-           (cz < 0 ? o.oz[0] : (cz > 0 ? o.oz[2] : 0));
-}
+// Per-axis index offsets of the wrapped neighbour rule: nb(q) = i + ox[cx+1] + oy[cy+1] + oz[cz+1]
+// (offsets fit 32 bits: initialisation rejects boxes with 19*N >= 2^31)
+struct Offs {
+    int32_t ox[3], oy[3], oz[3];
+    __device__ Offs(const Geom& g, int x, int y, int z) {
+        const int32_t sy = g.X, sz = g.X * g.Y;
+        ox[1] = 0; oy[1] = 0; oz[1] = 0;
+        ox[0] = (x - 1 == 0 && g.per[0]) ? g.X - 3 : -1;
+        ox[2] = (x + 1 == g.X - 1 && g.per[1]) ? -(g.X - 3) : 1;
+        oy[0] = (y - 1 == 0 && g.per[2]) ? (g.Y - 3) * sy : -sy;
+        oy[2] = (y + 1 == g.Y - 1 && g.per[3]) ? -(g.Y - 3) * sy : sy;
+        oz[0] = (z - 1 == 0 && g.per[4]) ? (g.Z - 3) * sz : -sz;
+        oz[2] = (z + 1 == g.Z - 1 && g.per[5]) ? -(g.Z - 3) * sz : sz;
+    }
+    // selects and packed direction words, not indexed arrays: a direction known only at run
+    // time must not put tables or offsets in scratch memory
+    __device__ int32_t d(int q) const {
+        const int cx = cx_rt(q), cy = cy_rt(q), cz = cz_rt(q);
+        return (cx < 0 ? ox[0] : (cx > 0 ? ox[2] : 0)) + (cy < 0 ? oy[0] : (cy > 0 ? oy[2] : 0)) +
+               (cz < 0 ? oz[0] : (cz > 0 ? oz[2] : 0));
+    }
+};
+
+__device__ inline int32_t offs_rt(const Offs& o, int q) { return o.d(q); }
 
 // slot holding the post-streaming population q of cell i in layout L (0 natural, 1 swapped)
 __device__ inline int64_t load_slot(const Geom& g, int64_t i, const Offs& o, int q, int L) {
-    return L == 0 ? int64_t(q) * g.N + i : int64_t(oppq(q)) * g.N + i + o.d(oppq(q));
+    return L == 0 ? int64_t(q) * g.N + i : int64_t(opp_rt(q)) * g.N + i + o.d(opp_rt(q));
 }
 
 template <typename C>
@@ -691,7 +693,7 @@ __device__ inline uint32_t inactive_bytes(uint32_t w) {
     return m & 0xFu;
 }
 constexpr int MARK_GRID = 4096;
-__device__ inline void k_mark_full_w(Geom g, uint8_t* type, const int32_t* solid, uint64_t* bcm, const DevScalars* sc) {
+__device__ __attribute__((always_inline)) inline void k_mark_full_w(Geom g, uint8_t* type, const int32_t* solid, uint64_t* bcm, const DevScalars* sc) {
     if (!sc->fullMark) return;
     const int W = g.X / 4;
     const uint32_t* t32 = reinterpret_cast<const uint32_t*>(type);
@@ -730,7 +732,8 @@ __device__ inline void k_mark_full_w(Geom g, uint8_t* type, const int32_t* solid
         const uint32_t diag = inactive_bytes(t32[rowWord(ym, zm) + w]) | inactive_bytes(t32[rowWord(ym, zp) + w]) |
                               inactive_bytes(t32[rowWord(yp, zm) + w]) | inactive_bytes(t32[rowWord(yp, zp) + w]);
         uint32_t nw = cw;
-#pragma unroll
+        // rolled: one inlined copy of the link-class code keeps the kernel free of calls and scratch
+#pragma unroll 1
         for (int k = 0; k < 4; ++k) {
             const uint8_t t = uint8_t(cw >> (8 * k));
             if (!is_active(t)) continue;
@@ -784,7 +787,7 @@ __device__ inline void k_mark_end(DevScalars* sc) {
 }
 
 void lb_mark_edges(const Box& b, DevScalars* sc, hipStream_t s) {
-    if (b.g.X % 4 == 0) launch_gs<&k_mark_full_w, 1024, 1>(MARK_GRID, 256, s, b.g, b.type, b.solid, b.bcm, sc);
+    if (b.g.X % 4 == 0) launch_gs<&k_mark_full_w, 256, 1>(MARK_GRID, 256, s, b.g, b.type, b.solid, b.bcm, sc);
     else launch_gs<&k_mark_full_b, 1024, 1>(MARK_GRID, 256, s, b.g, b.type, b.solid, b.bcm, sc);
     launch_gs<&k_remark, 1024, 1>(256, 256, s, b.g, b.type, b.solid, b.bcm, b.changed, b.changedCap, sc);
     launch<&k_mark_end, 1024, 1>(1, 1, s, sc);
