@@ -59,6 +59,13 @@ struct lbdem_ctx {
     std::vector<double> pr, px0, orad, ox0, ox1;
     // device
     hipStream_t stream = nullptr;
+    // lbdem_cycle defers a step's free-surface pass to the start of the next step and runs it on
+    // fsStream beside that step's DEM step (the two touch disjoint state); the coupling step
+    // waits for both
+    hipStream_t fsStream = nullptr;
+    hipEvent_t evFork = nullptr, evJoin = nullptr;
+    bool fsDeferred = false;
+    bool streamError = false;  // an event record or wait of the overlap failed
     Box box{};
     Consts k{};
     DemParams dp{};
@@ -120,8 +127,13 @@ struct lbdem_ctx {
         for (auto& pr : tuneEv)
             for (hipEvent_t e : pr)
                 if (e) (void)hipEventDestroy(e);
+        if (fsStream) (void)hipStreamSynchronize(fsStream);
+        if (stream) (void)hipStreamSynchronize(stream);
         for (void* p : allocs) (void)hipFree(p);
         if (dTmp && !tmpPlanned) (void)hipFree(dTmp);
+        if (evFork) (void)hipEventDestroy(evFork);
+        if (evJoin) (void)hipEventDestroy(evJoin);
+        if (fsStream) (void)hipStreamDestroy(fsStream);
         if (stream) (void)hipStreamDestroy(stream);
     }
     // download scratch: the planned buffer, replaced by a hipMalloc of its own when a download
@@ -452,6 +464,9 @@ extern "C" int lbdem_init(lbdem_ctx* c) {
     if (c->initialised) return c->fail(LBDEM_E_STATE, "already initialised");
     HIPCHK(hipSetDevice(c->prm.device));
     HIPCHK(hipStreamCreateWithFlags(&c->stream, hipStreamNonBlocking));
+    HIPCHK(hipStreamCreateWithFlags(&c->fsStream, hipStreamNonBlocking));
+    HIPCHK(hipEventCreateWithFlags(&c->evFork, hipEventDisableTiming));
+    HIPCHK(hipEventCreateWithFlags(&c->evJoin, hipEventDisableTiming));
     if (const char* g = std::getenv("LBDEM_GRAPHS")) c->useGraphs = std::atoi(g) != 0;
     if (!c->tuned)
         for (auto& pr : c->tuneEv)
@@ -702,7 +717,7 @@ static void enqueue_lb(lbdem_ctx* c) {  // LB::latticeBolzmannStep (LB.cpp:400-4
     lb_hydro_finalize(c->box, c->k, c->ds.FHydro, c->ds.MHydro, c->ds.fluidVol, c->sc, c->ds.oFHydro, c->ds.oMHydro,
                       c->ds.wFHydro, true, c->stream);
 }
-static void enqueue_fs(lbdem_ctx* c) { lb_free_surface(c->box, c->k, c->sc, c->stream); }
+static void enqueue_fs(lbdem_ctx* c, hipStream_t s) { lb_free_surface(c->box, c->k, c->sc, s); }
 
 #define NEED_INIT                                                           \
     if (!c) return LBDEM_E_ARG;                                             \
@@ -714,25 +729,55 @@ extern "C" int lbdem_lb_step(lbdem_ctx* c) { NEED_INIT; if (c->batch) return c->
 extern "C" int lbdem_fs_step(lbdem_ctx* c) {
     NEED_INIT;
     if (c->batch) return c->fail(LBDEM_E_STATE, "in a batch");
-    if (c->freeSurface) enqueue_fs(c);
+    if (c->freeSurface) enqueue_fs(c, c->stream);
     HIPCHK(hipGetLastError());
     return 0;
 }
 
-// one goCycle (main.cpp:38-67) minus the output
-static void enqueue_step(lbdem_ctx* c) {
+// one goCycle (main.cpp:38-67) minus the output. With deferFs the step's free-surface pass is
+// left for the next step (or flush_fs), which runs it on fsStream beside its DEM step: the free
+// surface reads and writes the lattice (populations, types, mass, interface lists) and the DEM
+// step the particles and the hydrodynamic sums the LB step left, so the order between them does
+// not matter; the coupling step waits for both, as it follows both in goCycle.
+static void enqueue_step(lbdem_ctx* c, bool deferFs = false) {
     c->realTime += c->unit.Time;
     ++c->currentTimeStep;
     ++c->lbTime;
+    bool join = false;
+    if (c->fsDeferred) {  // the previous step's free surface (LB.cpp:422-430)
+        c->fsDeferred = false;
+        if (c->prm.demSolve) {
+            if (hipEventRecord(c->evFork, c->stream) != hipSuccess ||
+                hipStreamWaitEvent(c->fsStream, c->evFork, 0) != hipSuccess)
+                c->streamError = true;
+            enqueue_fs(c, c->fsStream);
+            if (hipEventRecord(c->evJoin, c->fsStream) != hipSuccess) c->streamError = true;
+            join = true;
+        } else {
+            enqueue_fs(c, c->stream);
+        }
+    }
     if (c->prm.demSolve) enqueue_dem(c);
+    if (join && hipStreamWaitEvent(c->stream, c->evJoin, 0) != hipSuccess) c->streamError = true;
     if (c->demTime >= c->prm.demInitialRepeat) {
         if (c->prm.demSolve) enqueue_coupling(c);
         enqueue_lb(c);
-        if (c->freeSurface) enqueue_fs(c);
+        if (c->freeSurface) {
+            if (deferFs) c->fsDeferred = true;
+            else enqueue_fs(c, c->stream);
+        }
     } else if (c->demTime <= c->prm.demInitialRepeat) {
         if (c->prm.demSolve) enqueue_coupling(c);
     }
 }
+// the deferred free-surface pass, on the context's stream (every lbdem_cycle ends with it)
+static void flush_fs(lbdem_ctx* c) {
+    if (!c->fsDeferred) return;
+    c->fsDeferred = false;
+    enqueue_fs(c, c->stream);
+}
+// lbdem_cycle overlaps a step's free surface with the next DEM step
+static bool overlap_fs(const lbdem_ctx* c) { return c->freeSurface && c->prm.demSolve; }
 
 // A pair of steps has the same launches with the same arguments as every later pair once
 //  * every step compares its DEM step number past objWaitTime (the only use of the number in
@@ -746,6 +791,7 @@ static bool graph_ready(const lbdem_ctx* c) {
     if (c->prm.demSolve && c->demTimeStep < unsigned(std::max(c->prm.objWaitTime, 0)) + 1u) return false;
     if (!(c->demTime >= c->prm.demInitialRepeat)) return false;
     if (!c->forceField && !c->lbFReset) return false;
+    if (c->fsDeferred != overlap_fs(c)) return false;  // a pair starts with the previous free surface
     return true;
 }
 
@@ -755,15 +801,17 @@ struct HostStepState {
     double* xiCur;
     double* xiNext;
     int parity;
+    bool fsDeferred;
 };
 static HostStepState save_host(const lbdem_ctx* c) {
     return {c->lbTime, c->demTimeStep, c->currentTimeStep, c->demTime, c->realTime, c->ds.xiCur, c->ds.xiNext,
-            c->box.parity};
+            c->box.parity, c->fsDeferred};
 }
 static void restore_host(lbdem_ctx* c, const HostStepState& h) {
     c->lbTime = h.lbTime; c->demTimeStep = h.demTimeStep; c->currentTimeStep = h.currentTimeStep;
     c->demTime = h.demTime; c->realTime = h.realTime;
     c->ds.xiCur = h.xiCur; c->ds.xiNext = h.xiNext; c->box.parity = h.parity;
+    c->fsDeferred = h.fsDeferred;
 }
 
 // the launches of a step pair depend on the phase: AA parity and current history buffer
@@ -780,15 +828,17 @@ static int capture_pair(lbdem_ctx* c) {
         c->graphFailed = true;
         return 0;
     }
-    enqueue_step(c);
-    enqueue_step(c);
+    const bool defer = overlap_fs(c);
+    enqueue_step(c, defer);
+    enqueue_step(c, defer);
     hipGraph_t g = nullptr;
     const hipError_t e = hipStreamEndCapture(c->stream, &g);
     hipGraphExec_t ge = nullptr;
-    if (e != hipSuccess || !g || hipGraphInstantiate(&ge, g, nullptr, nullptr, 0) != hipSuccess) {
+    if (e != hipSuccess || !g || c->streamError || hipGraphInstantiate(&ge, g, nullptr, nullptr, 0) != hipSuccess) {
         if (g) (void)hipGraphDestroy(g);
         (void)hipGetLastError();
         restore_host(c, h);
+        c->streamError = false;
         c->graphFailed = true;
         return 0;
     }
@@ -822,14 +872,22 @@ extern "C" int lbdem_cycle(lbdem_ctx* c, int nsteps) {
                 continue;
             }
             const int r = capture_pair(c);
-            if (r < 0) return c->fail(LBDEM_E_DEVICE, "launch of the captured step pair failed");
+            if (r < 0) {
+                flush_fs(c);
+                return c->fail(LBDEM_E_DEVICE, "launch of the captured step pair failed");
+            }
             if (r == 1) {
                 s += 2;
                 continue;
             }
         }
-        enqueue_step(c);
+        enqueue_step(c, overlap_fs(c));
         ++s;
+    }
+    flush_fs(c);  // every call leaves whole steps behind
+    if (c->streamError) {
+        c->streamError = false;
+        return c->fail(LBDEM_E_DEVICE, "ordering of the free-surface stream failed");
     }
     HIPCHK(hipGetLastError());
     return 0;

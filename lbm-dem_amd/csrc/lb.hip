@@ -256,76 +256,71 @@ __device__ inline typename Types<S>::C boundary_value(const FusedArgs<S>& a, int
     return gj;
 }
 
-// plain index offset of direction q (valid for cells without the edge bit)
-__device__ inline int32_t plain_off(int q, int32_t sy, int32_t sz) { return cxq(q) + cyq(q) * sy + czq(q) * sz; }
-
-// Offset corrections of a node next to a periodic face: a link leaving through the face reaches
-// the wrapped partner (coordinate D-2 or 1) instead of the face's boundary layer (LB.cpp:632-678).
-// m* apply to directions with a negative component on that axis, p* to positive ones; all zero
-// away from periodic faces.
-struct Wrap {
-    int32_t mx, px, my, py, mz, pz;
-    __device__ Wrap(const Geom& g, int x, int y, int z) {
-        const int32_t sy = g.X, sz = g.X * g.Y;
-        mx = (x == 1 && g.per[0]) ? g.X - 2 : 0;
-        px = (x == g.X - 2 && g.per[1]) ? -(g.X - 2) : 0;
-        my = (y == 1 && g.per[2]) ? (g.Y - 2) * sy : 0;
-        py = (y == g.Y - 2 && g.per[3]) ? -(g.Y - 2) * sy : 0;
-        mz = (z == 1 && g.per[4]) ? (g.Z - 2) * sz : 0;
-        pz = (z == g.Z - 2 && g.per[5]) ? -(g.Z - 2) * sz : 0;
+// Population addresses in the fused kernel: a uniform part in scalar registers (plane, row and
+// the wrapped y/z steps: a wavefront is one x-row segment, so y and z are the same on all its
+// lanes) plus a 32-bit per-lane byte offset (x, or x -+ 1 wrapped through periodic x faces) that
+// stays fixed along the z-walk. Loads and stores take the scalar-base form: no 64-bit address
+// arithmetic per access, no per-lane wrap registers.
+struct LaneX {
+    uint32_t m, o, p;  // byte offsets of x - 1, x, x + 1 within a row (wrapped, LB.cpp:632-678)
+    __device__ LaneX(const Geom& g, int x, uint32_t elt) {
+        const int xm = (x == 1 && g.per[0]) ? g.X - 2 : x - 1;
+        const int xp = (x == g.X - 2 && g.per[1]) ? 1 : x + 1;
+        m = uint32_t(xm) * elt;
+        o = uint32_t(x) * elt;
+        p = uint32_t(xp) * elt;
     }
-    __device__ bool any() const { return (mx | px | my | py | mz | pz) != 0; }
-    // correction of direction q (compile-time q after unrolling)
-    __device__ int32_t d(int q) const {
-        return (cxq(q) < 0 ? mx : (cxq(q) > 0 ? px : 0)) + (cyq(q) < 0 ? my : (cyq(q) > 0 ? py : 0)) +
-               (czq(q) < 0 ? mz : (czq(q) > 0 ? pz : 0));
+    __device__ uint32_t sel(int cx) const { return cx < 0 ? m : (cx > 0 ? p : o); }
+};
+struct RowYZ {  // uniform across a wavefront
+    int32_t row;             // index of node (0, y, z)
+    int32_t ym, yp, zm, zp;  // index steps to the wrapped rows y -+ 1 and z -+ 1
+    __device__ RowYZ(const Geom& g, int y, int z) {
+        const int32_t sy = g.X, sz = g.X * g.Y;
+        row = sy * y + sz * z;
+        ym = (y == 1 && g.per[2]) ? (g.Y - 3) * sy : -sy;
+        yp = (y == g.Y - 2 && g.per[3]) ? -(g.Y - 3) * sy : sy;
+        zm = (z == 1 && g.per[4]) ? (g.Z - 3) * sz : -sz;
+        zp = (z == g.Z - 2 && g.per[5]) ? -(g.Z - 3) * sz : sz;
+    }
+    // index of (0, y, z) of the row holding the wrapped neighbour in direction q
+    __device__ int32_t nb(int q) const {
+        return row + (cyq(q) < 0 ? ym : (cyq(q) > 0 ? yp : 0)) + (czq(q) < 0 ? zm : (czq(q) > 0 ? zp : 0));
     }
 };
+// element at a uniform element index plus a per-lane byte offset
+template <typename T>
+__device__ inline T* at(T* base, int64_t uniformIdx, uint32_t laneBytes) {
+    return reinterpret_cast<T*>(reinterpret_cast<char*>(base + uniformIdx) + laneBytes);
+}
 
-// populations of node i before collision (its post-streaming values). PAR 0: the node's own
-// slots; PAR 1: the slots its neighbours wrote, at plain offsets, wrapped through periodic faces
-// (a wavefront without any node next to a periodic face takes the plain offsets only)
+// populations of a node before collision (its post-streaming values). PAR 0: the node's own
+// slots; PAR 1: the slots its neighbours wrote (the neighbour in direction opp(q) holds slot
+// opp(q)), wrapped through periodic faces
 template <typename S, int PAR>
-__device__ inline void load_pop(const FusedArgs<S>& a, int32_t i, bool act, int x, int y, int z,
+__device__ inline void load_pop(const FusedArgs<S>& a, const RowYZ& r, const LaneX& lx, bool act,
                                 typename Types<S>::C* gq) {
     if (!act) return;
-    const int32_t N = int32_t(a.g.N), sy = a.g.X, sz = a.g.X * a.g.Y;
-    if (PAR == 0) {
+    const int64_t N = a.g.N;
 #pragma unroll
-        for (int q = 0; q < Q; ++q) gq[q] = StoreT<S>::ld(a.f + q * N + i);
-    } else {
-        const Wrap w(a.g, x, y, z);
-        if (__ballot(w.any())) {
-#pragma unroll
-            for (int q = 0; q < Q; ++q)
-                gq[q] = StoreT<S>::ld(a.f + oppq(q) * N + i + plain_off(oppq(q), sy, sz) + w.d(oppq(q)));
-        } else {
-#pragma unroll
-            for (int q = 0; q < Q; ++q) gq[q] = StoreT<S>::ld(a.f + oppq(q) * N + i + plain_off(oppq(q), sy, sz));
-        }
+    for (int q = 0; q < Q; ++q) {
+        if (PAR == 0) gq[q] = StoreT<S>::ld(at(a.f, q * N + r.row, lx.o));
+        else gq[q] = StoreT<S>::ld(at(a.f, oppq(q) * N + r.nb(oppq(q)), lx.sel(cxq(oppq(q)))));
     }
 }
 
 // streaming of the post-collision values (LB.cpp:1375-1389): PAR 0 into the node's own slots
 // (opposite direction), PAR 1 into the neighbours' slots (wrapped as in load_pop)
 template <typename S, int PAR>
-__device__ inline void store_pop(const FusedArgs<S>& a, int32_t i, bool act, int x, int y, int z,
+__device__ inline void store_pop(const FusedArgs<S>& a, const RowYZ& r, const LaneX& lx, bool act,
                                  const typename Types<S>::C* gq) {
     if (!act) return;
-    const int32_t N = int32_t(a.g.N), sy = a.g.X, sz = a.g.X * a.g.Y;
-    StoreT<S>::st(a.f + i, gq[0]);
-    if (PAR == 0) {
+    const int64_t N = a.g.N;
+    StoreT<S>::st(at(a.f, r.row, lx.o), gq[0]);
 #pragma unroll
-        for (int j = 1; j < Q; ++j) StoreT<S>::st(a.f + oppq(j) * N + i, gq[j]);
-    } else {
-        const Wrap w(a.g, x, y, z);
-        if (__ballot(w.any())) {
-#pragma unroll
-            for (int j = 1; j < Q; ++j) StoreT<S>::st(a.f + j * N + i + plain_off(j, sy, sz) + w.d(j), gq[j]);
-        } else {
-#pragma unroll
-            for (int j = 1; j < Q; ++j) StoreT<S>::st(a.f + j * N + i + plain_off(j, sy, sz), gq[j]);
-        }
+    for (int j = 1; j < Q; ++j) {
+        if (PAR == 0) StoreT<S>::st(at(a.f, oppq(j) * N + r.row, lx.o), gq[j]);
+        else StoreT<S>::st(at(a.f, j * N + r.nb(j), lx.sel(cxq(j))), gq[j]);
     }
 }
 
@@ -503,6 +498,8 @@ __device__ inline void k_fused(FusedArgs<S> a) {
     const bool tile_ok = tile < nblk;
     const int bx = tile % a.nbx, by = (tile / a.nbx) % a.nby, bz = tile / (a.nbx * a.nby);
     const int x = bx * FX + threadIdx.x, y = by * FY + threadIdx.y;
+    static_assert(FX == 64, "a wavefront is one x-row segment: y and z are uniform across it");
+    const int yw = __builtin_amdgcn_readfirstlane(y);
     const Geom& g = a.g;
     const int32_t N = int32_t(g.N), SZ = g.X * g.Y;
     const int lane = __lane_id();
@@ -518,7 +515,8 @@ __device__ inline void k_fused(FusedArgs<S> a) {
     int32_t ic = col_ok ? int32_t(cidx(g, x, y, z0)) : 0;
     uint8_t tc = type_at(0), tn = type_at(1);
     bool actc = is_active(tc);
-    load_pop<S, PAR>(a, ic, actc, x, y, z0, gc);
+    const LaneX lx(g, x, uint32_t(sizeof(S)));
+    load_pop<S, PAR>(a, RowYZ(g, yw, z0), lx, actc, gc);
     double mc = (actc && (tc & (T_PMASK | T_EDGE))) ? double(a.mass[ic]) : 0.0;
     int32_t sc_ = (actc && (tc & T_PMASK)) ? a.solid[ic] : 0;
     uint64_t bc_ = (actc && (tc & T_EDGE)) ? a.bcm[ic] : 0;
@@ -528,7 +526,7 @@ __device__ inline void k_fused(FusedArgs<S> a) {
         const int32_t in = ic + SZ;
         const bool actn = is_active(tn);
         const uint8_t tnn = type_at(zz + 2);
-        load_pop<S, PAR>(a, in, actn, x, y, z + 1, gn);
+        load_pop<S, PAR>(a, RowYZ(g, yw, z + 1), lx, actn, gn);
         const double mn = (actn && (tn & (T_PMASK | T_EDGE))) ? double(a.mass[in]) : 0.0;
         const int32_t sn = (actn && (tn & T_PMASK)) ? a.solid[in] : 0;
         const uint64_t bn = (actn && (tn & T_EDGE)) ? a.bcm[in] : 0;
@@ -539,7 +537,7 @@ __device__ inline void k_fused(FusedArgs<S> a) {
         for (int c = 0; c < 7; ++c) r.d[c] = 0;
         r.n = 1; r.sx = 0; r.sy = 0; r.sz = 0; r.usq = 0;
         if (actc) collide_cell<S>(a, sc_, tc, x, y, z, mc, gc, r);
-        store_pop<S, PAR>(a, ic, actc, x, y, z, gc);
+        store_pop<S, PAR>(a, RowYZ(g, yw, z), lx, actc, gc);
 
         // boundary links of edge nodes (LB.cpp:1390-1528): the value reflected into the slot
         // this node loads next step
