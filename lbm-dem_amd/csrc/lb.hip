@@ -148,6 +148,7 @@ template <typename S>
 struct FusedArgs {
     Geom g;
     S* f;
+    S* fq[Q];  // population planes (f + q N): uniform bases of the fused kernel's accesses
     uint8_t* type;
     const int32_t* solid;
     typename Types<S>::M* mass;
@@ -288,10 +289,11 @@ struct RowYZ {  // uniform across a wavefront
         return row + (cyq(q) < 0 ? ym : (cyq(q) > 0 ? yp : 0)) + (czq(q) < 0 ? zm : (czq(q) > 0 ? zp : 0));
     }
 };
-// element at a uniform element index plus a per-lane byte offset
+// element of a plane (uniform base) at a uniform row index plus a per-lane byte offset; a node's
+// byte offset within a plane fits 32 bits (19 N < 2^31 nodes, at most 8 bytes each)
 template <typename T>
-__device__ inline T* at(T* base, int64_t uniformIdx, uint32_t laneBytes) {
-    return reinterpret_cast<T*>(reinterpret_cast<char*>(base + uniformIdx) + laneBytes);
+__device__ inline T* at(T* plane, int32_t rowIdx, uint32_t laneBytes) {
+    return reinterpret_cast<T*>(reinterpret_cast<char*>(plane) + (uint32_t(rowIdx) * uint32_t(sizeof(T)) + laneBytes));
 }
 
 // populations of a node before collision (its post-streaming values). PAR 0: the node's own
@@ -301,11 +303,10 @@ template <typename S, int PAR>
 __device__ inline void load_pop(const FusedArgs<S>& a, const RowYZ& r, const LaneX& lx, bool act,
                                 typename Types<S>::C* gq) {
     if (!act) return;
-    const int64_t N = a.g.N;
 #pragma unroll
     for (int q = 0; q < Q; ++q) {
-        if (PAR == 0) gq[q] = StoreT<S>::ld(at(a.f, q * N + r.row, lx.o));
-        else gq[q] = StoreT<S>::ld(at(a.f, oppq(q) * N + r.nb(oppq(q)), lx.sel(cxq(oppq(q)))));
+        if (PAR == 0) gq[q] = StoreT<S>::ld(at(a.fq[q], r.row, lx.o));
+        else gq[q] = StoreT<S>::ld(at(a.fq[oppq(q)], r.nb(oppq(q)), lx.sel(cxq(oppq(q)))));
     }
 }
 
@@ -315,12 +316,11 @@ template <typename S, int PAR>
 __device__ inline void store_pop(const FusedArgs<S>& a, const RowYZ& r, const LaneX& lx, bool act,
                                  const typename Types<S>::C* gq) {
     if (!act) return;
-    const int64_t N = a.g.N;
-    StoreT<S>::st(at(a.f, r.row, lx.o), gq[0]);
+    StoreT<S>::st(at(a.fq[0], r.row, lx.o), gq[0]);
 #pragma unroll
     for (int j = 1; j < Q; ++j) {
-        if (PAR == 0) StoreT<S>::st(at(a.f, oppq(j) * N + r.row, lx.o), gq[j]);
-        else StoreT<S>::st(at(a.f, j * N + r.nb(j), lx.sel(cxq(j))), gq[j]);
+        if (PAR == 0) StoreT<S>::st(at(a.fq[oppq(j)], r.row, lx.o), gq[j]);
+        else StoreT<S>::st(at(a.fq[j], r.nb(j), lx.sel(cxq(j))), gq[j]);
     }
 }
 
@@ -455,10 +455,35 @@ __device__ inline uint64_t link_classes(const Geom& g, const uint8_t* type, cons
 // z+1). Edge nodes are handled by the same waves as their row neighbours, so every population
 // access stays coalesced; their boundary links run in a rolled loop over the precomputed link
 // classes, without further loads for gas and wall links.
-// waves per SIMD the fused kernel is compiled for: 3 for 4- and 2-byte storage (at most 168
-// VGPRs), 2 for FP64
-template <typename S> struct FusedOcc { static constexpr int value = 3; };
-template <> struct FusedOcc<double> { static constexpr int value = 2; };
+// blocks per CU the fused kernel is compiled for (launch bounds): two, with up to 256 VGPRs
+// per lane. Measured on the 256x320x256 impact run (FP32): two blocks 0.69 ms per launch, three
+// blocks (168 VGPRs, the LDS limit) 0.73 ms.
+template <typename S> struct FusedOcc { static constexpr int value = 2; };
+// nodes whose bundle is loaded ahead of the one being collided. Measured as above: none 0.71 ms,
+// one 0.69 ms, two 0.78 ms (more loads in flight per CU slow the memory system down).
+template <typename S> struct FusedAhead { static constexpr int value = 1; };
+
+// per-node bundle: type byte, populations, mass (particle or edge node), solid index (particle
+// node), boundary-link classes (edge node)
+template <typename S>
+struct Bundle {
+    typename Types<S>::C g[Q];
+    double m;
+    uint64_t b;
+    int32_t s;
+    uint8_t t;
+    bool act;
+};
+template <typename S, int PAR>
+__device__ inline void load_bundle(const FusedArgs<S>& a, const RowYZ& r, const LaneX& lx, int32_t i, uint8_t t,
+                                   Bundle<S>& n) {
+    n.t = t;
+    n.act = is_active(t);
+    load_pop<S, PAR>(a, r, lx, n.act, n.g);
+    n.m = (n.act && (t & (T_PMASK | T_EDGE))) ? double(a.mass[i]) : 0.0;
+    n.s = (n.act && (t & T_PMASK)) ? a.solid[i] : 0;
+    n.b = (n.act && (t & T_EDGE)) ? a.bcm[i] : 0;
+}
 
 template <typename S, int PAR, bool FSURF, bool DIAG, int FZ>
 __device__ inline void k_fused(FusedArgs<S> a) {
@@ -498,7 +523,7 @@ __device__ inline void k_fused(FusedArgs<S> a) {
     const bool tile_ok = tile < nblk;
     const int bx = tile % a.nbx, by = (tile / a.nbx) % a.nby, bz = tile / (a.nbx * a.nby);
     const int x = bx * FX + threadIdx.x, y = by * FY + threadIdx.y;
-    static_assert(FX == 64, "a wavefront is one x-row segment: y and z are uniform across it");
+    static_assert(FX % 64 == 0, "a wavefront is one x-row segment: y and z are uniform across it");
     const int yw = __builtin_amdgcn_readfirstlane(y);
     const Geom& g = a.g;
     const int32_t N = int32_t(g.N), SZ = g.X * g.Y;
@@ -509,27 +534,27 @@ __device__ inline void k_fused(FusedArgs<S> a) {
         return (col_ok && zz < FZ && z0 + zz < g.Z) ? a.type[int32_t(cidx(g, x, y, z0 + zz))] : uint8_t(T_WALL);
     };
 
-    // per-node bundle: populations, mass (particle or edge node), solid index (particle node),
-    // boundary-link classes (edge node)
-    C gc[Q], gn[Q];
+    // Bundles of the current node and of the AH nodes after it are in registers; per node the
+    // thread loads the bundle AH nodes ahead and the type byte one further before it collides and
+    // stores the current node. In either AA parity a node loads and stores only its own slot set,
+    // so no load of a node ahead reads a slot that a store of the current node writes.
+    constexpr int AH = FusedAhead<S>::value;
+    Bundle<S> nb[AH + 1];  // nb[0]: current node, nb[k]: k nodes ahead
     int32_t ic = col_ok ? int32_t(cidx(g, x, y, z0)) : 0;
-    uint8_t tc = type_at(0), tn = type_at(1);
-    bool actc = is_active(tc);
     const LaneX lx(g, x, uint32_t(sizeof(S)));
-    load_pop<S, PAR>(a, RowYZ(g, yw, z0), lx, actc, gc);
-    double mc = (actc && (tc & (T_PMASK | T_EDGE))) ? double(a.mass[ic]) : 0.0;
-    int32_t sc_ = (actc && (tc & T_PMASK)) ? a.solid[ic] : 0;
-    uint64_t bc_ = (actc && (tc & T_EDGE)) ? a.bcm[ic] : 0;
+#pragma unroll
+    for (int k = 0; k < AH; ++k) load_bundle<S, PAR>(a, RowYZ(g, yw, z0 + k), lx, ic + k * SZ, type_at(k), nb[k]);
+    uint8_t tAhead = type_at(AH);  // type byte of the node AH ahead
     for (int zz = 0; zz < FZ; ++zz) {
         const int z = z0 + zz;
-        // next node's bundle and the type byte of the node after it
-        const int32_t in = ic + SZ;
-        const bool actn = is_active(tn);
-        const uint8_t tnn = type_at(zz + 2);
-        load_pop<S, PAR>(a, RowYZ(g, yw, z + 1), lx, actn, gn);
-        const double mn = (actn && (tn & (T_PMASK | T_EDGE))) ? double(a.mass[in]) : 0.0;
-        const int32_t sn = (actn && (tn & T_PMASK)) ? a.solid[in] : 0;
-        const uint64_t bn = (actn && (tn & T_EDGE)) ? a.bcm[in] : 0;
+        const uint8_t tFar = type_at(zz + AH + 1);
+        load_bundle<S, PAR>(a, RowYZ(g, yw, z + AH), lx, ic + AH * SZ, tAhead, nb[AH]);
+        C* gc = nb[0].g;
+        const bool actc = nb[0].act;
+        const uint8_t tc = nb[0].t;
+        const double mc = nb[0].m;
+        const int32_t sc_ = nb[0].s;
+        const uint64_t bc_ = nb[0].b;
 
         CellOut<S> r;
         r.key = -1;
@@ -612,15 +637,10 @@ __device__ inline void k_fused(FusedArgs<S> a) {
         }
         accumulate_particles<C>(r.d, r.key, lane, hkey, hval, a.hacc);
         // advance the column
-        actc = actn;
-        ic = in;
-        tc = tn;
-        tn = tnn;
-        mc = mn;
-        sc_ = sn;
-        bc_ = bn;
+        ic += SZ;
+        tAhead = tFar;
 #pragma unroll
-        for (int q = 0; q < Q; ++q) gc[q] = gn[q];
+        for (int k = 0; k < AH; ++k) nb[k] = nb[k + 1];
     }
     __syncthreads();
     flush_particles(tid, NT, hkey, hval, a.hacc);
@@ -801,6 +821,7 @@ static void launch_fused(Box& b, const Consts& k, const PartLB* plb, const ObjLB
                          hipStream_t s, hipEvent_t ev0, hipEvent_t ev1) {
     FusedArgs<S> a;
     a.g = b.g; a.f = (S*)b.f; a.type = b.type; a.solid = b.solid;
+    for (int q = 0; q < Q; ++q) a.fq[q] = a.f + int64_t(q) * b.g.N;
     a.mass = (typename Types<S>::M*)b.mass; a.aux = (typename Types<S>::C*)b.aux;
     a.ifList = b.ifList; a.ifCap = b.ifCap;
     a.plb = plb; a.olb = olb; a.hacc = b.hacc; a.blockPart = b.blockPart; a.blockFlag = b.blockFlag; a.bcm = b.bcm; a.sc = sc;
