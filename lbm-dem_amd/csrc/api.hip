@@ -553,6 +553,8 @@ extern "C" int lbdem_init(lbdem_ctx* c) {
     hipStream_t st = c->stream;
     HIPCHK(hipMemsetAsync(b.f, 0, size_t(Q) * g.N * selt, st));
     HIPCHK(hipMemsetAsync(b.mass, 0, g.N * celt, st));
+    // particle hydrodynamic sums: zero before the first LB step (k_hydro_finalize re-zeroes them)
+    HIPCHK(hipMemsetAsync(b.hacc, 0, size_t(std::max(int(c->pr.size()), 1)) * 8 * sizeof(double), st));
     HIPCHK(hipMemsetAsync(b.aux, 0, 7 * g.N * celt, st));
     HIPCHK(hipMemsetAsync(b.donor, 0xff, g.N * 4, st));
     HIPCHK(hipMemsetAsync(b.flagNew, 0xff, g.N * 4, st));

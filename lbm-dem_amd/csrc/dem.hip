@@ -636,24 +636,36 @@ __device__ inline void k_dem_reduce_a(const double* oc, int nco, const double* w
         part[(int64_t(role) * RED_BLOCKS + bx) * RED_MAXCOL + t] = s;
     }
 }
-__device__ inline void k_dem_reduce_b(const double* part, int nco, int ncw, double* objSum, double* wallSum) {
+// One block: the second level of the sums (the block partials of each column in block order; or
+// zeros when no particle contributes), then, on thread 0, objects: totals and integration
+// (DEM.cpp:786-790, 2742-2758) and walls: totals (DEM.cpp:773-775).
+constexpr int OBJ_THREADS = 128;
+static_assert(OBJ_THREADS >= 12 * MAX_OBJECTS && OBJ_THREADS >= 6 * MAX_WALLS, "one thread per column");
+__device__ inline void k_dem_objects(DemParams P, DemState d, const double* part, int sumsFromParts, double* objSum,
+                                     double* wallSum, int demTimeStep) {
+#pragma clang fp contract(off)
+    __shared__ double sObj[12 * MAX_OBJECTS], sWall[6 * MAX_WALLS];
+    if (blockIdx.x != 0) return;
     const int t = threadIdx.x;
-    if (t < nco) {
+    const int nco = 12 * P.nObj, ncw = 6 * P.nWall;
+    if (t < 12 * MAX_OBJECTS) {
         double s = 0.0;
-        for (int b = 0; b < RED_BLOCKS; ++b) s += part[int64_t(b) * RED_MAXCOL + t];
+        if (sumsFromParts && t < nco)
+            for (int b = 0; b < RED_BLOCKS; ++b) s += part[int64_t(b) * RED_MAXCOL + t];
+        sObj[t] = s;
         objSum[t] = s;
     }
-    if (t < ncw) {
+    if (t < 6 * MAX_WALLS) {
         double s = 0.0;
-        for (int b = 0; b < RED_BLOCKS; ++b) s += part[(int64_t(RED_BLOCKS) + b) * RED_MAXCOL + t];
+        if (sumsFromParts && t < ncw)
+            for (int b = 0; b < RED_BLOCKS; ++b) s += part[(int64_t(RED_BLOCKS) + b) * RED_MAXCOL + t];
+        sWall[t] = s;
         wallSum[t] = s;
     }
-}
-
-// objects: totals and integration (DEM.cpp:786-790, 2742-2758); walls: totals (DEM.cpp:773-775)
-__device__ inline void k_dem_objects(DemParams P, DemState d, const double* objSum, const double* wallSum, int demTimeStep) {
-#pragma clang fp contract(off)
-    if (threadIdx.x != 0 || blockIdx.x != 0) return;
+    __syncthreads();
+    if (t != 0) return;
+    objSum = sObj;
+    wallSum = sWall;
     for (int o = 0; o < P.nObj; ++o) {
         const double* s = objSum + 12 * o;
         const V Fc{s[0], s[1], s[2]}, Mc{s[3], s[4], s[5]}, Fl{s[6], s[7], s[8]}, Ml{s[9], s[10], s[11]};
@@ -735,9 +747,6 @@ __device__ inline void k_dem_views(DemParams P, DemState d) {
     }
 }
 
-__device__ inline void k_zero(double* p, int n) {
-    for (int i = threadIdx.x; i < n; i += blockDim.x) p[i] = 0.0;
-}
 
 void dem_refresh_views(const DemParams& p, DemState& d, hipStream_t s) {
     const int n = p.nPart > p.nObj ? p.nPart : p.nObj;
@@ -782,16 +791,12 @@ void dem_step(const DemParams& p, DemState& d, DevScalars* sc, int demTimeStep, 
         f.ox0 = d.ox0; f.ox1 = d.ox1; f.ow1 = d.ow1; f.or_ = d.or_;
         f.objContrib = d.objContrib; f.wallContrib = d.wallContrib; f.demTimeStep = demTimeStep;
         launch<&k_dem_forces, 256, 1>((p.nPart * DEM_LANES + 255) / 256, 256, s, f, p.dampCoeff);
-        if (p.nObj > 0 || p.nWall > 0) {
+        if (p.nObj > 0 || p.nWall > 0)
             launch<&k_dem_reduce_a, RED_BS, 1>(2 * RED_BLOCKS, RED_BS, s, d.objContrib, 12 * p.nObj, d.wallContrib, 6 * p.nWall, p.nPart, d.redPart);
-            launch<&k_dem_reduce_b, 1024, 1>(1, 128, s, d.redPart, 12 * p.nObj, 6 * p.nWall, d.objSum, d.wallSum);
-        }
         double* t = d.xiCur; d.xiCur = d.xiNext; d.xiNext = t;
-    } else {
-        launch<&k_zero, 1024, 1>(1, 128, s, d.objSum, 12 * MAX_OBJECTS);
-        launch<&k_zero, 1024, 1>(1, 128, s, d.wallSum, 6 * MAX_WALLS);
     }
-    launch<&k_dem_objects, 1024, 1>(1, 64, s, p, d, d.objSum, d.wallSum, demTimeStep);
+    const int fromParts = (p.nPart > 0 && (p.nObj > 0 || p.nWall > 0)) ? 1 : 0;  // else zero sums
+    launch<&k_dem_objects, OBJ_THREADS, 1>(1, OBJ_THREADS, s, p, d, (const double*)d.redPart, fromParts, d.objSum, d.wallSum, demTimeStep);
     if (p.nPart > 0) launch<&k_dem_integrate, 1024, 1>((p.nPart + 127) / 128, 128, s, p, d, demTimeStep);
 }
 

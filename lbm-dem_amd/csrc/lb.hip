@@ -799,16 +799,12 @@ __device__ inline void k_remark(Geom g, uint8_t* type, const int32_t* solid, uin
         if (p >= 0 && p < g.N) mark_node(g, type, solid, bcm, int32_t(p));  // (changed boundary-layer nodes)
     }
 }
-__device__ inline void k_mark_end(DevScalars* sc) {
-    sc->fullMark = 0;
-    sc->nChanged = 0;
-}
 
 void lb_mark_edges(const Box& b, DevScalars* sc, hipStream_t s) {
     if (b.g.X % 4 == 0) launch_gs<&k_mark_full_w, 256, 1>(MARK_GRID, 256, s, b.g, b.type, b.solid, b.bcm, sc);
     else launch_gs<&k_mark_full_b, 1024, 1>(MARK_GRID, 256, s, b.g, b.type, b.solid, b.bcm, sc);
     launch_gs<&k_remark, 1024, 1>(256, 256, s, b.g, b.type, b.solid, b.bcm, b.changed, b.changedCap, sc);
-    launch<&k_mark_end, 1024, 1>(1, 1, s, sc);
+    // the list and the full-pass flag are reset by the next kernel (k_begin_lb)
 }
 
 int fused_grid_blocks(const Geom& g, int fz) {
@@ -852,16 +848,18 @@ static void launch_fused(Box& b, const Consts& k, const PartLB* plb, const ObjLB
     b.parity ^= 1;
 }
 
-__device__ inline void k_begin_lb(DevScalars* sc, double* hacc, int nPart) {
-    const int i = blockIdx.x * blockDim.x + threadIdx.x;
-    if (i == 0) sc->nIf = 0;
-    if (i < nPart * 8) hacc[i] = 0.0;
+// after the edge marking, before the fused kernel: the change list is consumed, the interface
+// list starts empty (the particle sums were zeroed by the previous step's k_hydro_finalize, or
+// at initialisation)
+__device__ inline void k_begin_lb(DevScalars* sc) {
+    sc->fullMark = 0;
+    sc->nChanged = 0;
+    sc->nIf = 0;
 }
 
 void lb_fused_step(Box& b, const Consts& k, const PartLB* plb, const ObjLB* olb, DevScalars* sc, bool diag,
                    hipStream_t s, hipEvent_t ev0, hipEvent_t ev1) {
-    const int n = k.nPart * 8 + 1;
-    launch<&k_begin_lb, 1024, 1>((n + 255) / 256, 256, s, sc, b.hacc, k.nPart);
+    launch<&k_begin_lb, 1024, 1>(1, 1, s, sc);
     if (b.prec == LBDEM_PREC_FP64) launch_fused<double>(b, k, plb, olb, sc, diag, s, ev0, ev1);
     else if (b.prec == LBDEM_PREC_FP32) launch_fused<float>(b, k, plb, olb, sc, diag, s, ev0, ev1);
     else launch_fused<__half>(b, k, plb, olb, sc, diag, s, ev0, ev1);
@@ -870,11 +868,17 @@ void lb_fused_step(Box& b, const Consts& k, const PartLB* plb, const ObjLB* olb,
 // deterministic sum of the flagged per-block partials, unit conversion of all hydrodynamic results
 // (LB.cpp:1966-1977, 1536-1543)
 constexpr int HF_THREADS = 1024;
-__device__ inline void k_hydro_finalize(int nblk, const double* part, const int32_t* flag, Consts k, const double* hacc, double* FHydro,
+__device__ inline void k_hydro_finalize(int nblk, const double* part, const int32_t* flag, Consts k, double* hacc, double* FHydro,
                                  double* MHydro, double* fluidVol, DevScalars* sc, double* objFH, double* objMH,
-                                 double* wallFH, int diag) {
+                                 double* wallFH, int diag, int* count2) {
     __shared__ double red[HF_THREADS / 64];
     const int tid = threadIdx.x;
+    if (blockIdx.x == 0 && tid == 0) {  // counters of the free-surface pass that follows (LB.cpp:422-430)
+        sc->surplus = 0.0;
+        sc->nFilled = sc->nEmptied = sc->nNewIf = 0;
+        sc->nIfFinal = 0;
+        *count2 = 0;
+    }
     if (blockIdx.x < PART_STRIDE) {
         // column c of the partial table (one block per column), summed in a fixed order: strided
         // per-thread sums, a wavefront butterfly, then the waves in order
@@ -907,12 +911,13 @@ __device__ inline void k_hydro_finalize(int nblk, const double* part, const int3
     } else {
         const int p = (blockIdx.x - PART_STRIDE) * blockDim.x + tid;
         if (p < k.nPart) {
-            const double* h = hacc + int64_t(p) * 8;
+            double* h = hacc + int64_t(p) * 8;
             for (int c = 0; c < 3; ++c) {
                 FHydro[3 * p + c] = h[c] * k.unitForce;
                 MHydro[3 * p + c] = h[3 + c] * k.unitTorque;
             }
             fluidVol[p] = h[6] * k.unitVolume;
+            for (int c = 0; c < 8; ++c) h[c] = 0.0;  // zero for the next LB step's sums
         }
     }
 }
@@ -921,7 +926,8 @@ void lb_hydro_finalize(const Box& b, const Consts& k, double* FHydro, double* MH
                        DevScalars* sc, double* objFH, double* objMH, double* wallFH, bool diag, hipStream_t s) {
     const int nblk = fused_grid_blocks(b.g, b.fz);
     const int grid = PART_STRIDE + (k.nPart + HF_THREADS - 1) / HF_THREADS;
-    launch<&k_hydro_finalize, HF_THREADS, 1>(grid, HF_THREADS, s, nblk, b.blockPart, b.blockFlag, k, b.hacc, FHydro, MHydro, fluidVol, sc, objFH, objMH, wallFH, diag ? 1 : 0);
+    int* count2 = (int*)(b.ifList2 + b.ifCap);  // free-surface counter stored after the list
+    launch<&k_hydro_finalize, HF_THREADS, 1>(grid, HF_THREADS, s, nblk, b.blockPart, b.blockFlag, k, b.hacc, FHydro, MHydro, fluidVol, sc, objFH, objMH, wallFH, diag ? 1 : 0, count2);
 }
 
 // ------------------------------------------------------------------------------------------
@@ -1186,12 +1192,6 @@ __device__ inline void k_fs_redistribute(const uint8_t* type, typename Types<S>:
     if (blockIdx.x == 0 && threadIdx.x == 0) sc->lastSurplus = sc->surplus;
 }
 
-__device__ inline void k_fs_begin(DevScalars* sc, int* count2) {
-    sc->surplus = 0.0;
-    sc->nFilled = sc->nEmptied = sc->nNewIf = 0;
-    sc->nIfFinal = 0;
-    *count2 = 0;
-}
 
 template <typename S>
 static void fs_impl(Box& b, const Consts& k, DevScalars* sc, hipStream_t s) {
@@ -1202,7 +1202,6 @@ static void fs_impl(Box& b, const Consts& k, DevScalars* sc, hipStream_t s) {
     C* aux = (C*)b.aux;
     int* count2 = (int*)(b.ifList2 + b.ifCap);  // counter stored after the list
     const int grid = LIST_GRID, bs = 256;
-    launch<&k_fs_begin, 1024, 1>(1, 1, s, sc, count2);
     if (b.parity == 1)
         launch_gs<&k_fs_mass<S, 1>, 1024, 1>(grid, bs, s, b.g, f, b.type, mass, aux, b.ifList, (M*)b.ifNewMass, sc, b.donor);
     else
@@ -1234,8 +1233,9 @@ void lb_free_surface(Box& b, const Consts& k, DevScalars* sc, hipStream_t s) {
 // index wins
 constexpr int FLAGS_GRID = 2048;  // grids of the flag passes (grid-stride; most steps return early)
 __device__ inline void k_flags_compute(Geom g, const uint8_t* type, const PartLB* plb, int nPart, int32_t* flagNew,
-                                const DevScalars* sc) {
+                                DevScalars* sc) {
     if (!sc->newNeighborList) return;
+    if (blockIdx.x == 0 && threadIdx.x == 0) sc->kminP = INT_MAX;  // lowest flagged index, from k_flags_apply
     const int dims[3] = {g.X, g.Y, g.Z};
     for (int p = blockIdx.x; p < nPart; p += gridDim.x) {
         const PartLB P = plb[p];
@@ -1303,25 +1303,19 @@ __device__ inline void k_flags_apply(int64_t N, uint8_t* type, int32_t* solid, i
         atomicMin(&sc->kminP, bmin);
 }
 
-__device__ inline void k_flags_apply_begin(DevScalars* sc) {
-    if (sc->newNeighborList) sc->kminP = INT_MAX;
-}
-__device__ inline void k_flags_apply_end(DevScalars* sc) {
-    if (!sc->newNeighborList) return;
-    if (sc->kminP == INT_MAX) sc->kminP = -1;
-    sc->newNeighborList = 0;
-}
 
 void lb_flags_rebuild_apply(const Box& b, DevScalars* sc, hipStream_t s) {
-    launch<&k_flags_apply_begin, 1024, 1>(1, 1, s, sc);
     const int64_t nb = (b.g.N + 255) / 256;
     const int grid = int(nb < FLAGS_GRID ? nb : FLAGS_GRID);
     launch_gs<&k_flags_apply, 1024, 1, 16384>(grid, 256, s, b.g.N, b.type, b.solid, b.flagNew, sc, b.changed,
                                               b.changedCap);
-    launch<&k_flags_apply_end, 1024, 1>(1, 1, s, sc);
 }
 
 __device__ inline void k_fna_check(Geom g, const int32_t* solid, const PartLB* plb, int nPart, DevScalars* sc) {
+    if (sc->newNeighborList) {  // end of the flag pass (k_flags_compute set kminP to INT_MAX)
+        if (sc->kminP == INT_MAX) sc->kminP = -1;
+        sc->newNeighborList = 0;
+    }
     const int kp = sc->kminP;
     sc->needScan = 0;
     if (kp < 0) return;
