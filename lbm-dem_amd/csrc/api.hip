@@ -555,6 +555,8 @@ extern "C" int lbdem_init(lbdem_ctx* c) {
     HIPCHK(hipMemsetAsync(b.mass, 0, g.N * celt, st));
     // particle hydrodynamic sums: zero before the first LB step (k_hydro_finalize re-zeroes them)
     HIPCHK(hipMemsetAsync(b.hacc, 0, size_t(std::max(int(c->pr.size()), 1)) * 8 * sizeof(double), st));
+    // largest particle |v|^2 of the step, for the neighbour-list trigger: velocities start at zero
+    HIPCHK(hipMemsetAsync(c->ds.blockMax, 0, sizeof(unsigned long long), st));
     HIPCHK(hipMemsetAsync(b.aux, 0, 7 * g.N * celt, st));
     HIPCHK(hipMemsetAsync(b.donor, 0xff, g.N * 4, st));
     HIPCHK(hipMemsetAsync(b.flagNew, 0xff, g.N * 4, st));

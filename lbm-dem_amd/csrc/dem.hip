@@ -31,33 +31,19 @@ __device__ inline V sub(V a, V b) { return V{a.x - b.x, a.y - b.y, a.z - b.z}; }
 // ------------------------------------------------------------------------------------------
 // neighbour-list trigger (DEM.cpp:366-378, 460-470)
 // ------------------------------------------------------------------------------------------
-// maximum |v|^2 over particles: per-block maxima into blockMax (non-negative doubles order like
-// their bit patterns), then the trigger update by one thread
-__device__ inline void k_maxdisp_partial(const double* x1, int n, int nb, unsigned long long* blockMax) {
-    if (int(blockIdx.x) >= nb) return;  // nb: the box's grid (k_maxdisp reads nb partials)
-    __shared__ double red[256];
-    double m = 0.0;
-    for (int p = blockIdx.x * blockDim.x + threadIdx.x; p < n; p += nb * blockDim.x) {
-        const double v2 = x1[3 * p] * x1[3 * p] + x1[3 * p + 1] * x1[3 * p + 1] + x1[3 * p + 2] * x1[3 * p + 2];
-        if (v2 > m) m = v2;
-    }
-    red[threadIdx.x] = m;
-    __syncthreads();
-    for (int w = 128; w > 0; w >>= 1) {
-        if (threadIdx.x < w && red[threadIdx.x + w] > red[threadIdx.x]) red[threadIdx.x] = red[threadIdx.x + w];
-        __syncthreads();
-    }
-    if (threadIdx.x == 0) blockMax[blockIdx.x] = __double_as_longlong(red[0]);
-}
-__device__ inline void k_maxdisp(const unsigned long long* blockMax, int nb, double deltat, DevScalars* sc, int force) {
+// The trigger needs the largest |v|^2 over the particles at the start of the step: the
+// velocities the previous step's k_dem_integrate wrote, which folds them into velMax with an
+// atomic maximum on the bit patterns (non-negative doubles order like their bit patterns; zero at
+// initialisation, when every velocity is zero). k_maxdisp reads and clears it.
+__device__ inline void k_maxdisp(unsigned long long* velMax, double deltat, DevScalars* sc, int force) {
     if (threadIdx.x != 0) return;
+    const double m = __longlong_as_double(*velMax);
+    *velMax = 0ull;
     if (force) {
         sc->rebuild = 1;
         sc->builds += 1;
         return;
     }
-    double m = 0.0;
-    for (int b = 0; b < nb; ++b) { const double v = __longlong_as_double(blockMax[b]); if (v > m) m = v; }
     const double maxVel = sqrt(m);
     sc->maxDisp += maxVel * deltat;
     if (sc->maxDisp > 0.05) {
@@ -696,11 +682,9 @@ __device__ inline void k_dem_objects(DemParams P, DemState d, const double* part
     }
 }
 
-// particles: totals, gravity switch-off, integration, periodic wrap, lattice view
-__device__ inline void k_dem_integrate(DemParams P, DemState d, int demTimeStep) {
+// particle p: totals, gravity switch-off, integration, periodic wrap, lattice view; returns |v|^2
+__device__ inline double integrate_particle(const DemParams& P, const DemState& d, int demTimeStep, int p) {
 #pragma clang fp contract(off)
-    const int p = blockIdx.x * blockDim.x + threadIdx.x;
-    if (p >= P.nPart) return;
     const V FT = add(add(add(add(ld3(d.FHydro, p), ld3(d.FContact, p)), ld3(d.FLub, p)), ld3(d.FGrav, p)), ld3(d.FRep, p));
     const V MT = add(add(ld3(d.MHydro, p), ld3(d.MContact, p)), ld3(d.MLub, p));
     st3(d.FTotal, p, FT); st3(d.MTotal, p, MT);
@@ -724,6 +708,19 @@ __device__ inline void k_dem_integrate(DemParams P, DemState d, int demTimeStep)
     }
     v.r = d.r[p] / P.unitLength;
     d.plb[p] = v;
+    return x1[0] * x1[0] + x1[1] * x1[1] + x1[2] * x1[2];
+}
+__device__ inline void k_dem_integrate(DemParams P, DemState d, int demTimeStep) {
+#pragma clang fp contract(off)
+    const int p = blockIdx.x * blockDim.x + threadIdx.x;
+    double v2 = 0.0;  // |v|^2 after the update, for the next step's neighbour-list trigger
+    if (p < P.nPart) v2 = integrate_particle(P, d, demTimeStep, p);
+#pragma unroll
+    for (int off = 32; off > 0; off >>= 1) {
+        const double o = __shfl_xor(v2, off, 64);
+        v2 = o > v2 ? o : v2;
+    }
+    if (__lane_id() == 0 && v2 > 0.0) atomicMax(d.blockMax, (unsigned long long)__double_as_longlong(v2));
 }
 
 __device__ inline void k_dem_views(DemParams P, DemState d) {
@@ -763,9 +760,7 @@ static NbrArgs nbr_args(const DemParams& p, DemState& d, DevScalars* sc) {
 }
 
 void dem_neighbor_list(const DemParams& p, DemState& d, DevScalars* sc, bool force, hipStream_t s) {
-    const int nbm = p.nPart > 0 ? std::min(64, (p.nPart + 255) / 256) : 1;
-    launch<&k_maxdisp_partial, 1024, 1>(nbm, 256, s, d.x1, p.nPart, nbm, d.blockMax);
-    launch<&k_maxdisp, 1024, 1>(1, 64, s, d.blockMax, nbm, p.deltat, sc, force ? 1 : 0);
+    launch<&k_maxdisp, 1024, 1>(1, 64, s, d.blockMax, p.deltat, sc, force ? 1 : 0);
     if (p.nPart <= 0) return;
     NbrArgs a = nbr_args(p, d, sc);
     const int pb = (p.nPart + 127) / 128;

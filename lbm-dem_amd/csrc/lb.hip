@@ -1110,10 +1110,17 @@ __device__ inline void k_fs_update_mutants(uint8_t* type, typename Types<S>::M* 
     if (acc != 0.0) atomicAdd(&sc->surplus, acc);
 }
 
-// interface list after conversions: untouched old interface nodes + new ones
-__device__ inline void k_fs_list2(const uint8_t* type, const int32_t* ifList, const int32_t* newIf, const int32_t* marker,
-                           int32_t* list2, int cap, DevScalars* sc, int* count2) {
+// removeIsolated part 1: interface nodes without gas neighbours become fluid (LB.cpp:1796-1816).
+// The interface list after the conversions (the old interface nodes left untouched, then the new
+// ones) is visited here and compacted into list2 for the two passes that follow. A node's filter
+// depends only on its own type and marker, which no other thread changes in this pass, and a
+// conversion to fluid changes no node's gas test, so the pass equals the sequential loop.
+template <typename S>
+__device__ inline void k_fs_isolated_a(Geom g, uint8_t* type, typename Types<S>::M* mass, const typename Types<S>::C* aux,
+                                const int32_t* ifList, const int32_t* newIf, const int32_t* marker, int32_t* list2,
+                                int cap, int* count2, DevScalars* sc) {
     const int nIf = sc->nIf, nN = sc->nNewIf;
+    double acc = 0.0;
     for (int k = blockIdx.x * blockDim.x + threadIdx.x; k < nIf + nN; k += gridDim.x * blockDim.x) {
         int64_t i;
         if (k < nIf) {
@@ -1123,17 +1130,6 @@ __device__ inline void k_fs_list2(const uint8_t* type, const int32_t* ifList, co
             i = newIf[k - nIf];
         }
         list_push(list2, count2, cap, int32_t(i), &sc->listOverflow);
-    }
-}
-
-// removeIsolated part 1: interface nodes without gas neighbours become fluid (LB.cpp:1796-1816)
-template <typename S>
-__device__ inline void k_fs_isolated_a(Geom g, uint8_t* type, typename Types<S>::M* mass, const typename Types<S>::C* aux,
-                                const int32_t* list2, const int* count2, DevScalars* sc) {
-    const int n2 = *count2;
-    double acc = 0.0;
-    for (int k = blockIdx.x * blockDim.x + threadIdx.x; k < n2; k += gridDim.x * blockDim.x) {
-        const int64_t i = list2[k];
         int x, y, z;
         cell_xyz(g, i, x, y, z);
         const Offs o(g, x, y, z);
@@ -1214,8 +1210,8 @@ static void fs_impl(Box& b, const Consts& k, DevScalars* sc, hipStream_t s) {
         launch_gs<&k_fs_new_from_donor<S, 0>, 1024, 1>(grid, bs, s, b.g, f, b.type, mass, aux, b.newIf, b.donor, sc, k, b.changed, b.changedCap);
     launch_gs<&k_fs_from_emptied<S>, 1024, 1>(grid, bs, s, b.g, b.type, mass, aux, b.emptied, b.donor, b.newIf, b.ifCap, sc);
     launch_gs<&k_fs_update_mutants<S>, 1024, 1>(grid, bs, s, b.type, mass, aux, b.filled, b.emptied, sc);
-    launch_gs<&k_fs_list2, 1024, 1>(grid, bs, s, b.type, b.ifList, b.newIf, b.donor, b.ifList2, b.ifCap, sc, count2);
-    launch_gs<&k_fs_isolated_a<S>, 1024, 1>(grid, bs, s, b.g, b.type, mass, aux, b.ifList2, count2, sc);
+    launch_gs<&k_fs_isolated_a<S>, 1024, 1>(grid, bs, s, b.g, b.type, mass, aux, b.ifList, b.newIf, b.donor, b.ifList2,
+                                            b.ifCap, count2, sc);
     launch_gs<&k_fs_isolated_b<S>, 1024, 1>(grid, bs, s, b.g, b.type, mass, b.ifList2, count2, sc, b.changed, b.changedCap);
     launch_gs<&k_fs_redistribute<S>, 1024, 1>(grid, bs, s, b.type, mass, b.ifList2, count2, b.donor, b.filled, b.newIf, sc);
 }

@@ -254,7 +254,7 @@ struct DemState {
     double* objSum;            // 12*MAX_OBJECTS
     double* redPart;           // 2*64*12*MAX_OBJECTS block partials of the object/wall reduction
     double* wallSum;           // 6*MAX_WALLS
-    unsigned long long* blockMax;  // per-block maxima of |v|^2
+    unsigned long long* blockMax;  // [0]: largest particle |v|^2 of the last integration (bit pattern)
     // objects
     double *ox0, *ox1, *ow1, *or_, *om, *oI, *oim, *oiI;
     double *oFHydro, *oMHydro, *oFContact, *oMContact, *oFLub, *oMLub, *oFTotal, *oMTotal, *oFGrav;
