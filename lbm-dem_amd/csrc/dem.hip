@@ -382,28 +382,27 @@ __device__ inline void k_dem_forces(ForceArgs a, double dampCoeff) {
     V Fc{0, 0, 0}, Mc{0, 0, 0}, Fl{0, 0, 0}, Ml{0, 0, 0}, Fr{0, 0, 0};
     const int cnt = valid ? a.nbrCount[p] : 0;
     const bool waited = a.demTimeStep >= P.objWaitTime;
-    if (waited) {
-        // particle lubrication (DEM.cpp:2302-2521)
-        for (int k = lane; k < cnt; k += DEM_LANES) {
-            const int q = a.nbrList[p * NBR_CAP + k];
-            const int n1 = p > q ? p : q, n2 = p > q ? q : p;
-            const LubOut o = pair_lub(P, ld3(a.x0, n1), ld3(a.x0, n2), a.r[n1], a.r[n2], ld3(a.x1, n1), ld3(a.x1, n2),
-                                      ld3(a.w1, n1), ld3(a.w1, n2), a.r[n2], a.r[n1]);
-            if (!o.act) continue;
-            if (p == n1) { Fl = add(Fl, o.F); Ml = add(Ml, o.T1); }
-            else { Fl = sub(Fl, o.F); Ml = add(Ml, o.T2); }
-        }
-    }
-    // particle-particle contact (DEM.cpp:1487-1791)
+    // One pass over the lane's partners: the pair's data are gathered once and the three pair
+    // terms follow in the reference's order; each accumulator sees its terms in partner order, as
+    // three separate loops would give them.
     for (int k = lane; k < cnt; k += DEM_LANES) {
         const int q = a.nbrList[p * NBR_CAP + k];
         const int n1 = p > q ? p : q, n2 = p > q ? q : p;
         const int slot = a.nbrSlot[p * NBR_CAP + k];
-        V xi{0, 0, 0};
-        if (slot >= 0) xi = ld3(a.xiOld, slot);
         const V A = ld3(a.x0, n1), B = ld3(a.x0, n2), vA = ld3(a.x1, n1), vB = ld3(a.x1, n2);
         const V wA = ld3(a.w1, n1), wB = ld3(a.w1, n2);
         const double rA = a.r[n1], rB = a.r[n2];
+        V xi{0, 0, 0};
+        if (slot >= 0) xi = ld3(a.xiOld, slot);
+        // particle lubrication (DEM.cpp:2302-2521)
+        if (waited) {
+            const LubOut o = pair_lub(P, A, B, rA, rB, vA, vB, wA, wB, rB, rA);
+            if (o.act) {
+                if (p == n1) { Fl = add(Fl, o.F); Ml = add(Ml, o.T1); }
+                else { Fl = sub(Fl, o.F); Ml = add(Ml, o.T2); }
+            }
+        }
+        // particle-particle contact (DEM.cpp:1487-1791)
         const double rad = 0.5 * (rA + rB);
         const double x12 = A.x - B.x, y12 = A.y - B.y, z12 = A.z - B.z;
         const double r12 = sqrt(x12 * x12 + y12 * y12 + z12 * z12);
@@ -447,25 +446,17 @@ __device__ inline void k_dem_forces(ForceArgs a, double dampCoeff) {
         const double dtx = fty * nz - ftz * ny, dty = ftz * nx - ftx * nz, dtz = ftx * ny - fty * nx;
         if (p == n1) { Fc = add(Fc, V{dfx, dfy, dfz}); Mc = add(Mc, V{rA * dtx, rA * dty, rA * dtz}); }
         else { Fc = sub(Fc, V{dfx, dfy, dfz}); Mc = add(Mc, V{rB * dtx, rB * dty, rB * dtz}); }
-    }
-    // electrostatic repulsion over every listed pair (DEM.cpp:1404-1484)
-    for (int k = lane; k < cnt; k += DEM_LANES) {
-        const int q = a.nbrList[p * NBR_CAP + k];
-        const int n1 = p > q ? p : q, n2 = p > q ? q : p;
-        const V A = ld3(a.x0, n1), B = ld3(a.x0, n2);
-        const double rad = 0.5 * (a.r[n1] + a.r[n2]);
-        const double x12 = A.x - B.x, y12 = A.y - B.y, z12 = A.z - B.z;
-        const double r12 = sqrt(x12 * x12 + y12 * y12 + z12 * z12);
-        const double nx = x12 / r12, ny = y12 / r12, nz = z12 / r12;
-        const double del = 2 * rad - r12;
-        double fx, fy, fz;
-        if (del <= 0.0) {
-            const double e = P.electroForce * exp(del / P.debyeLength);
-            fx = e * nx; fy = e * ny; fz = e * nz;
-        } else {
-            fx = P.electroForce * nx; fy = P.electroForce * ny; fz = P.electroForce * nz;
+        // electrostatic repulsion over every listed pair (DEM.cpp:1404-1484)
+        {
+            double fx, fy, fz;
+            if (del <= 0.0) {
+                const double e = P.electroForce * exp(del / P.debyeLength);
+                fx = e * nx; fy = e * ny; fz = e * nz;
+            } else {
+                fx = P.electroForce * nx; fy = P.electroForce * ny; fz = P.electroForce * nz;
+            }
+            if (p == n1) Fr = add(Fr, V{fx, fy, fz}); else Fr = sub(Fr, V{fx, fy, fz});
         }
-        if (p == n1) Fr = add(Fr, V{fx, fy, fz}); else Fr = sub(Fr, V{fx, fy, fz});
     }
     Fc = group_sum(Fc); Mc = group_sum(Mc); Fl = group_sum(Fl); Ml = group_sum(Ml); Fr = group_sum(Fr);
     if (!valid || lane != 0) return;
