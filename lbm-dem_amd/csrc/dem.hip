@@ -357,7 +357,7 @@ __device__ void wall_like_contact(const DemParams& P, double dampCoeff, double r
     Mout = V{r * dtx, r * dty, r * dtz};
 }
 
-constexpr int DEM_LANES = 8;  // lanes per particle in the force kernel
+constexpr int DEM_LANES = 4;  // lanes per particle in the force kernel (all wavefronts resident at once on the 31 000-sphere run)
 
 // sum over the DEM_LANES lanes of a particle's group, in a fixed butterfly order
 __device__ inline double group_sum(double v) {
