@@ -949,16 +949,25 @@ __device__ inline void k_fs_mass(Geom g, const S* f, const uint8_t* type, const 
         cell_xyz(g, i, x, y, z);
         const Offs o(g, x, y, z);
         const C mi = C(double(mass[i]) + addE);
-        C dm = 0;
+        // every neighbour's type, populations and mass are loaded up front (all addresses are
+        // nodes of the box), so the loads overlap; the sum then runs in direction order
+        uint8_t ty[Q];
+        C a1[Q], a2[Q], mj[Q];
+#pragma unroll
         for (int j = 1; j < Q; ++j) {
             const int64_t yj = i + o.d(j);
-            const uint8_t ty = type[yj] & T_BASE;
-            if (ty != T_IF && ty != T_FLUID && ty != T_OBJ) continue;
-            const C a1 = StoreT<S>::ld(f + int64_t(j) * g.N + yj);
-            const C a2 = StoreT<S>::ld(f + int64_t(oppq(j)) * g.N + i);
-            const C ms = L == 1 ? (a1 - a2) : (a2 - a1);  // incoming(opp j) - outgoing(j), offsets
-            if (ty == T_IF) dm += C(0.5) * (C(double(mass[yj]) + addE) + mi) * ms;
-            else if (ty == T_FLUID) dm += ms;
+            ty[j] = type[yj] & T_BASE;
+            a1[j] = StoreT<S>::ld(f + int64_t(j) * g.N + yj);
+            a2[j] = StoreT<S>::ld(f + int64_t(oppq(j)) * g.N + i);
+            mj[j] = C(mass[yj]);
+        }
+        C dm = 0;
+#pragma unroll
+        for (int j = 1; j < Q; ++j) {
+            if (ty[j] != T_IF && ty[j] != T_FLUID && ty[j] != T_OBJ) continue;
+            const C ms = L == 1 ? (a1[j] - a2[j]) : (a2[j] - a1[j]);  // incoming(opp j) - outgoing(j), offsets
+            if (ty[j] == T_IF) dm += C(0.5) * (C(double(mj[j]) + addE) + mi) * ms;
+            else if (ty[j] == T_FLUID) dm += ms;
             else dm += mi * ms;
         }
         newMass[k] = M(mi + dm);
@@ -1012,9 +1021,12 @@ __device__ inline void k_fs_donors(Geom g, const uint8_t* type, const int32_t* f
         int x, y, z;
         cell_xyz(g, i, x, y, z);
         const Offs o(g, x, y, z);
-        for (int j = 1; j < Q; ++j) {
+        uint32_t gasMask = 0;  // neighbour types loaded up front so the loads overlap
+#pragma unroll
+        for (int j = 1; j < Q; ++j) gasMask |= uint32_t((type[i + o.d(j)] & T_BASE) == T_GAS) << j;
+        for (uint32_t m = gasMask; m; m &= m - 1u) {  // gas neighbours in direction order
+            const int j = __ffs(m) - 1;
             const int64_t yj = i + o.d(j);
-            if ((type[yj] & T_BASE) != T_GAS) continue;
             const int old = atomicMax(&donor[yj], int32_t(i));
             if (old < 0) list_push(newIf, &sc->nNewIf, cap, int32_t(yj), &sc->listOverflow);
         }
@@ -1068,9 +1080,14 @@ __device__ inline void k_fs_from_emptied(Geom g, uint8_t* type, typename Types<S
         int x, y, z;
         cell_xyz(g, i, x, y, z);
         const Offs o(g, x, y, z);
-        for (int j = 0; j < Q; ++j) {
+        // neighbour types loaded up front so the loads overlap; the marker CAS below decides which
+        // thread converts a node, so a type read early changes nothing
+        uint32_t fluidMask = 0;
+#pragma unroll
+        for (int j = 0; j < Q; ++j) fluidMask |= uint32_t((type[i + o.d(j)] & T_BASE) == T_FLUID) << j;
+        for (uint32_t m = fluidMask; m; m &= m - 1u) {  // fluid neighbours in direction order
+            const int j = __ffs(m) - 1;
             const int64_t l = i + o.d(j);
-            if ((type[l] & T_BASE) != T_FLUID) continue;
             double nl;
             int old = atomicCAS(&marker[l], -1, -2);
             if (old == -1) {
@@ -1133,8 +1150,9 @@ __device__ inline void k_fs_isolated_a(Geom g, uint8_t* type, typename Types<S>:
         int x, y, z;
         cell_xyz(g, i, x, y, z);
         const Offs o(g, x, y, z);
-        bool gasNb = false;
-        for (int j = 1; j < Q && !gasNb; ++j) gasNb = (type[i + o.d(j)] & T_BASE) == T_GAS;
+        bool gasNb = false;  // the 18 type loads overlap (no early exit)
+#pragma unroll
+        for (int j = 1; j < Q; ++j) gasNb |= (type[i + o.d(j)] & T_BASE) == T_GAS;
         if (!gasNb) {
             acc += double(mass[i]) - double(aux[i]);
             mass[i] = typename Types<S>::M(aux[i]);
@@ -1157,8 +1175,9 @@ __device__ inline void k_fs_isolated_b(Geom g, uint8_t* type, const typename Typ
         int x, y, z;
         cell_xyz(g, i, x, y, z);
         const Offs o(g, x, y, z);
-        bool fluidNb = false;
-        for (int j = 1; j < Q && !fluidNb; ++j) fluidNb = (type[i + o.d(j)] & T_BASE) == T_FLUID;
+        bool fluidNb = false;  // the 18 type loads overlap (no early exit)
+#pragma unroll
+        for (int j = 1; j < Q; ++j) fluidNb |= (type[i + o.d(j)] & T_BASE) == T_FLUID;
         if (!fluidNb) {
             acc += double(mass[i]);
             type[i] = (type[i] & T_PMASK) | T_GAS;
@@ -1199,21 +1218,21 @@ static void fs_impl(Box& b, const Consts& k, DevScalars* sc, hipStream_t s) {
     int* count2 = (int*)(b.ifList2 + b.ifCap);  // counter stored after the list
     const int grid = LIST_GRID, bs = 256;
     if (b.parity == 1)
-        launch_gs<&k_fs_mass<S, 1>, 1024, 1>(grid, bs, s, b.g, f, b.type, mass, aux, b.ifList, (M*)b.ifNewMass, sc, b.donor);
+        launch_gs<&k_fs_mass<S, 1>, 256, 1>(grid, bs, s, b.g, f, b.type, mass, aux, b.ifList, (M*)b.ifNewMass, sc, b.donor);
     else
-        launch_gs<&k_fs_mass<S, 0>, 1024, 1>(grid, bs, s, b.g, f, b.type, mass, aux, b.ifList, (M*)b.ifNewMass, sc, b.donor);
-    launch_gs<&k_fs_mutants<S>, 1024, 1>(grid, bs, s, b.type, mass, aux, b.g.N, b.ifList, (M*)b.ifNewMass, b.filled, b.emptied, b.ifCap, sc, b.donor, b.changed, b.changedCap);
-    launch_gs<&k_fs_donors, 1024, 1>(grid, bs, s, b.g, b.type, b.filled, b.donor, b.newIf, b.ifCap, sc);
+        launch_gs<&k_fs_mass<S, 0>, 256, 1>(grid, bs, s, b.g, f, b.type, mass, aux, b.ifList, (M*)b.ifNewMass, sc, b.donor);
+    launch_gs<&k_fs_mutants<S>, 256, 1>(grid, bs, s, b.type, mass, aux, b.g.N, b.ifList, (M*)b.ifNewMass, b.filled, b.emptied, b.ifCap, sc, b.donor, b.changed, b.changedCap);
+    launch_gs<&k_fs_donors, 256, 1>(grid, bs, s, b.g, b.type, b.filled, b.donor, b.newIf, b.ifCap, sc);
     if (b.parity == 1)
-        launch_gs<&k_fs_new_from_donor<S, 1>, 1024, 1>(grid, bs, s, b.g, f, b.type, mass, aux, b.newIf, b.donor, sc, k, b.changed, b.changedCap);
+        launch_gs<&k_fs_new_from_donor<S, 1>, 256, 1>(grid, bs, s, b.g, f, b.type, mass, aux, b.newIf, b.donor, sc, k, b.changed, b.changedCap);
     else
-        launch_gs<&k_fs_new_from_donor<S, 0>, 1024, 1>(grid, bs, s, b.g, f, b.type, mass, aux, b.newIf, b.donor, sc, k, b.changed, b.changedCap);
-    launch_gs<&k_fs_from_emptied<S>, 1024, 1>(grid, bs, s, b.g, b.type, mass, aux, b.emptied, b.donor, b.newIf, b.ifCap, sc);
-    launch_gs<&k_fs_update_mutants<S>, 1024, 1>(grid, bs, s, b.type, mass, aux, b.filled, b.emptied, sc);
-    launch_gs<&k_fs_isolated_a<S>, 1024, 1>(grid, bs, s, b.g, b.type, mass, aux, b.ifList, b.newIf, b.donor, b.ifList2,
+        launch_gs<&k_fs_new_from_donor<S, 0>, 256, 1>(grid, bs, s, b.g, f, b.type, mass, aux, b.newIf, b.donor, sc, k, b.changed, b.changedCap);
+    launch_gs<&k_fs_from_emptied<S>, 256, 1>(grid, bs, s, b.g, b.type, mass, aux, b.emptied, b.donor, b.newIf, b.ifCap, sc);
+    launch_gs<&k_fs_update_mutants<S>, 256, 1>(grid, bs, s, b.type, mass, aux, b.filled, b.emptied, sc);
+    launch_gs<&k_fs_isolated_a<S>, 256, 1>(grid, bs, s, b.g, b.type, mass, aux, b.ifList, b.newIf, b.donor, b.ifList2,
                                             b.ifCap, count2, sc);
-    launch_gs<&k_fs_isolated_b<S>, 1024, 1>(grid, bs, s, b.g, b.type, mass, b.ifList2, count2, sc, b.changed, b.changedCap);
-    launch_gs<&k_fs_redistribute<S>, 1024, 1>(grid, bs, s, b.type, mass, b.ifList2, count2, b.donor, b.filled, b.newIf, sc);
+    launch_gs<&k_fs_isolated_b<S>, 256, 1>(grid, bs, s, b.g, b.type, mass, b.ifList2, count2, sc, b.changed, b.changedCap);
+    launch_gs<&k_fs_redistribute<S>, 256, 1>(grid, bs, s, b.type, mass, b.ifList2, count2, b.donor, b.filled, b.newIf, sc);
 }
 
 void lb_free_surface(Box& b, const Consts& k, DevScalars* sc, hipStream_t s) {
