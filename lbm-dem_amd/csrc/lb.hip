@@ -167,28 +167,6 @@ struct FusedArgs {
     Consts k;
 };
 
-// first lane of the run of equal keys that contains this lane
-__device__ inline int run_start(int key, int lane) {
-    const int prev = __shfl_up(key, 1, 64);
-    int start = (lane == 0 || prev != key) ? lane : 0;
-#pragma unroll
-    for (int off = 1; off < 64; off <<= 1) {
-        const int o = __shfl_up(start, off, 64);
-        if (lane >= off && o > start) start = o;
-    }
-    return start;
-}
-// segmented inclusive sum over the lanes [start, lane] of the wavefront
-template <typename C>
-__device__ inline C seg_sum(C v, int start, int lane) {
-#pragma unroll
-    for (int off = 1; off < 64; off <<= 1) {
-        const C o = __shfl_up(v, off, 64);
-        if (lane - off >= start) v += o;
-    }
-    return v;
-}
-
 // moving-object link (LB.cpp:1427-1453): bounce-back with the velocity of the object whose sphere
 // holds the neighbour (the last one, as the reference scans objects in order), momentum exchange
 // into the partials of the object named by the neighbour's solid index oi
@@ -377,20 +355,10 @@ __device__ inline void collide_cell(const FusedArgs<S>& a, int32_t sp, uint8_t t
     r.n = n; r.sx = sx; r.sy = sy; r.sz = sz; r.usq = usq; r.hx = hx; r.hy = hy; r.hz = hz;
 }
 
-// particle accumulators of one wavefront: segmented sums over runs of equal particle index,
-// one LDS hash update per run (global atomics when the block's table is full)
+// one thread's running sums of particle key into the block's table (LDS hash; global atomics
+// when the table is full)
 template <typename C>
-__device__ inline void accumulate_particles(const C* d, int key, int lane, int* hkey, double (*hval)[7],
-                                            double* hacc) {
-    const unsigned long long pm = __ballot(key >= 0);
-    if (!pm) return;
-    const int st0 = run_start(key, lane);
-    C s[7];
-#pragma unroll
-    for (int c = 0; c < 7; ++c) s[c] = seg_sum(d[c], st0, lane);
-    const int nextKey = __shfl_down(key, 1, 64);
-    const bool tail = key >= 0 && (lane == 63 || nextKey != key);
-    if (!tail) return;
+__device__ inline void add_particle(int key, const C* s, int* hkey, double (*hval)[7], double* hacc) {
     int slot = key & (HS - 1);
     int probes = 0;
     for (; probes < HS; ++probes) {
@@ -545,6 +513,10 @@ __device__ inline void k_fused(FusedArgs<S> a) {
 #pragma unroll
     for (int k = 0; k < AH; ++k) load_bundle<S, PAR>(a, RowYZ(g, yw, z0 + k), lx, ic + k * SZ, type_at(k), nb[k]);
     uint8_t tAhead = type_at(AH);  // type byte of the node AH ahead
+    int accKey = -1;  // particle whose contributions acc holds
+    C acc[7];
+#pragma unroll
+    for (int c = 0; c < 7; ++c) acc[c] = 0;
     for (int zz = 0; zz < FZ; ++zz) {
         const int z = z0 + zz;
         const uint8_t tFar = type_at(zz + AH + 1);
@@ -635,13 +607,25 @@ __device__ inline void k_fused(FusedArgs<S> a) {
                 if (isIf) sIf[base + __popcll(m & ((1ull << lane) - 1ull))] = ic;
             }
         }
-        accumulate_particles<C>(r.d, r.key, lane, hkey, hval, a.hacc);
+        // particle sums: the thread's z-walk usually stays in one sphere for several nodes, so the
+        // contributions are summed in registers while the particle index is unchanged and added
+        // to the block's table when it changes (and after the walk)
+        if (r.key != accKey) {
+            if (accKey >= 0) add_particle<C>(accKey, acc, hkey, hval, a.hacc);
+            accKey = r.key;
+#pragma unroll
+            for (int c = 0; c < 7; ++c) acc[c] = r.d[c];
+        } else if (r.key >= 0) {
+#pragma unroll
+            for (int c = 0; c < 7; ++c) acc[c] += r.d[c];
+        }
         // advance the column
         ic += SZ;
         tAhead = tFar;
 #pragma unroll
         for (int k = 0; k < AH; ++k) nb[k] = nb[k + 1];
     }
+    if (accKey >= 0) add_particle<C>(accKey, acc, hkey, hval, a.hacc);
     __syncthreads();
     flush_particles(tid, NT, hkey, hval, a.hacc);
     if (FSURF) {
