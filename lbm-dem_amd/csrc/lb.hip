@@ -423,9 +423,11 @@ __device__ inline uint64_t link_classes(const Geom& g, const uint8_t* type, cons
 // z+1). Edge nodes are handled by the same waves as their row neighbours, so every population
 // access stays coalesced; their boundary links run in a rolled loop over the precomputed link
 // classes, without further loads for gas and wall links.
-// blocks per CU the fused kernel is compiled for (launch bounds): two, with up to 256 VGPRs
-// per lane. Measured on the 256x320x256 impact run (FP32): two blocks 0.69 ms per launch, three
-// blocks (168 VGPRs, the LDS limit) 0.73 ms.
+// Launch bounds of the fused kernel: two blocks per CU, i.e. up to 256 VGPRs per lane, so the
+// compiler never spills (a 168-VGPR bound spilled the odd-parity kernel). FP32 needs about 153,
+// so LDS (about 52 KB per block) sets residency at three blocks per CU. Measured on the
+// 256x320x256 impact run (FP32): three blocks resident 0.69 ms per launch; LDS padded to allow
+// only two 0.80 ms.
 template <typename S> struct FusedOcc { static constexpr int value = 2; };
 // nodes whose bundle is loaded ahead of the one being collided. Measured as above: none 0.71 ms,
 // one 0.69 ms, two 0.78 ms (more loads in flight per CU slow the memory system down).
