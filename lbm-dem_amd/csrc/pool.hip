@@ -176,6 +176,11 @@ void init_run(lbdem_pool* P, Run& r) {
     if (!rc) rc = lbdem_set_objects(r.ctx, int(r.objR.size()), r.objR.data(), r.objX0.data(), r.objX1.data());
     if (!rc) rc = lbdem_init(r.ctx);
     if (!rc && P->batching) rc = lbdem_use_graphs(r.ctx, 0);  // alone only for start-up steps
+    // A batch runs every member with one z-walk length (the largest member's), so the tile
+    // tuner's timed start-up steps (and their wait on the host) would be spent for nothing: runs
+    // that will be batched take the 8-node walk the tuner picks on large boxes and can join a
+    // batch after their first steps.
+    if (!rc && P->batching) rc = lbdem_set_tile(r.ctx, 8);
     if (rc) {
         r.failed = true;
         r.error = rc;
