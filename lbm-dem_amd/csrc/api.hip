@@ -557,6 +557,9 @@ extern "C" int lbdem_init(lbdem_ctx* c) {
     HIPCHK(hipMemsetAsync(b.hacc, 0, size_t(std::max(int(c->pr.size()), 1)) * 8 * sizeof(double), st));
     // largest particle |v|^2 of the step, for the neighbour-list trigger: velocities start at zero
     HIPCHK(hipMemsetAsync(c->ds.blockMax, 0, sizeof(unsigned long long), st));
+    // DEM cell list: each rebuild clears it after use (k_nbr_slots), the first one finds it zero
+    HIPCHK(hipMemsetAsync(c->ds.cellCount, 0, size_t(plan.ncell) * sizeof(int), st));
+    HIPCHK(hipMemsetAsync(c->ds.cellFill, 0, size_t(plan.ncell) * sizeof(int), st));
     HIPCHK(hipMemsetAsync(b.aux, 0, 7 * g.N * celt, st));
     HIPCHK(hipMemsetAsync(b.donor, 0xff, g.N * 4, st));
     HIPCHK(hipMemsetAsync(b.flagNew, 0xff, g.N * 4, st));

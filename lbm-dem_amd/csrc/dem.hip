@@ -78,13 +78,6 @@ struct NbrArgs {
     DevScalars* sc;
 };
 
-__device__ inline void k_cells_clear(NbrArgs a, int ncell) {
-    if (!a.sc->rebuild) return;
-    for (int c = blockIdx.x * blockDim.x + threadIdx.x; c < ncell; c += gridDim.x * blockDim.x) {
-        a.cellCount[c] = 0;
-        a.cellFill[c] = 0;
-    }
-}
 __device__ inline void k_cells_assign(NbrArgs a) {
     if (!a.sc->rebuild) return;
     const int p = blockIdx.x * blockDim.x + threadIdx.x;
@@ -179,6 +172,13 @@ __device__ inline void k_nbr_build(NbrArgs a) {
 // partner id (DEM.cpp:633-651); a partner absent from the old list starts with zero history
 __device__ inline void k_nbr_slots(NbrArgs a) {
     if (!a.sc->rebuild) return;
+    // k_nbr_build was the cell list's last reader: clear it for the next rebuild (zero at
+    // initialisation)
+    const int ncell = a.p.nlx * a.p.nly * a.p.nlz;
+    for (int c = blockIdx.x * blockDim.x + threadIdx.x; c < ncell; c += gridDim.x * blockDim.x) {
+        a.cellCount[c] = 0;
+        a.cellFill[c] = 0;
+    }
     const int p = blockIdx.x * blockDim.x + threadIdx.x;
     if (p >= a.p.nPart) return;
     const int cnt = a.nbrCount[p];
@@ -757,7 +757,6 @@ void dem_neighbor_list(const DemParams& p, DemState& d, DevScalars* sc, bool for
     const int pb = (p.nPart + 127) / 128;
     if (p.nlx > 0) {
         const int ncell = p.nlx * p.nly * p.nlz;
-        launch_gs<&k_cells_clear, 1024, 1>((ncell + 255) / 256, 256, s, a, ncell);
         launch<&k_cells_assign, 1024, 1>(pb, 128, s, a);
         launch<&k_cells_scan, 1024, 1>(1, 1024, s, a, ncell);
         launch<&k_cells_fill, 1024, 1>(pb, 128, s, a);

@@ -1493,7 +1493,11 @@ __device__ inline void k_obj_uncover(Geom g, S* f, uint8_t* type,
             const int32_t i = sCand[tid];
             int x, y, z;
             cell_xyz(g, i, x, y, z);
-            for (int j = 1; j < Q; ++j) nbT[j - 1][tid] = type[nbr_plain(g, x, y, z, j)] & T_BASE;
+            uint8_t nt[Q];  // the 18 loads are issued together, then written to LDS
+#pragma unroll
+            for (int j = 1; j < Q; ++j) nt[j] = type[nbr_plain(g, x, y, z, j)] & T_BASE;
+#pragma unroll
+            for (int j = 1; j < Q; ++j) nbT[j - 1][tid] = nt[j];
         }
         __syncthreads();
         // The new type of candidate t: gas without a fluid neighbour, interface with fluid and gas
