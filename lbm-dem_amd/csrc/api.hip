@@ -12,6 +12,7 @@
 #include <cstring>
 #include <string>
 #include <type_traits>
+#include <map>
 #include <vector>
 
 using namespace lbdem;
@@ -1185,8 +1186,15 @@ struct BatchOp {
     void (*launch)(dim3, dim3, size_t, hipStream_t, const void*);
     dim3 grid, block;
     size_t shm;
-    size_t off;  // of the launch's argument table (one entry per member of the group)
+    size_t off;               // of the launch's argument table (one entry per box of the launch)
+    size_t j;                 // index of the launch in the recorded step pair
+    std::vector<size_t> who;  // the batch members it runs, in table order
 };
+// A kernel with an exact grid runs every box of a launch on the launch's largest grid (the
+// surplus blocks return at once). When the boxes' grids differ by more than 2x and the largest
+// is big, the boxes are launched in classes of power-of-two grid size instead, so few blocks
+// are launched for nothing.
+constexpr unsigned SPLIT_MIN_GRID = 256;
 // the phase's step pair, one launch list per group of members with the same launch sequence
 struct BatchPhase {
     bool recorded = false;
@@ -1280,30 +1288,49 @@ static int batch_record(lbdem_batch* B, BatchPhase& P) {
         const std::vector<size_t>& mg = members[g];
         const Recorder& r0 = rec[mg[0]];
         for (size_t j = 0; j < r0.ops.size(); ++j) {
-            BatchOp op;
-            op.launch = r0.ops[j].batch;
-            op.block = r0.ops[j].block;
-            op.shm = r0.ops[j].shm;
-            unsigned gx = 1;
+            unsigned gmax = 1, gmin = ~0u;
             for (size_t i : mg) {
                 const dim3 gr = rec[i].ops[j].grid;
                 if (gr.y != 1 || gr.z != 1) return undo("a kernel uses a multi-dimensional grid");
-                gx = std::max(gx, gr.x);
+                gmax = std::max(gmax, gr.x);
+                gmin = std::min(gmin, gr.x);
             }
-            // grid-stride kernels: a bounded number of blocks spread over the group's boxes
-            if (r0.ops[j].elastic) gx = std::min(gx, std::max(4u, unsigned(r0.ops[j].elastic / mg.size())));
-            op.grid = dim3(gx, unsigned(mg.size()), 1);
-            op.off = total;
-            total += (mg.size() * r0.ops[j].args.size() + 255) & ~size_t(255);
-            P.groups[g].push_back(op);
+            std::vector<std::vector<size_t>> parts;
+            if (!r0.ops[j].elastic && gmax >= SPLIT_MIN_GRID && gmax > 2 * gmin) {
+                std::map<int, std::vector<size_t>> cls;  // by ceil(log2(grid))
+                for (size_t i : mg) {
+                    int k = 0;
+                    while ((1u << k) < rec[i].ops[j].grid.x) ++k;
+                    cls[k].push_back(i);
+                }
+                for (auto& kv : cls) parts.push_back(std::move(kv.second));
+            } else {
+                parts.push_back(mg);
+            }
+            for (std::vector<size_t>& part : parts) {
+                BatchOp op;
+                op.launch = r0.ops[j].batch;
+                op.block = r0.ops[j].block;
+                op.shm = r0.ops[j].shm;
+                op.j = j;
+                unsigned gx = 1;
+                for (size_t i : part) gx = std::max(gx, rec[i].ops[j].grid.x);
+                // grid-stride kernels: a bounded number of blocks spread over the launch's boxes
+                if (r0.ops[j].elastic) gx = std::min(gx, std::max(4u, unsigned(r0.ops[j].elastic / part.size())));
+                op.grid = dim3(gx, unsigned(part.size()), 1);
+                op.off = total;
+                total += (part.size() * r0.ops[j].args.size() + 255) & ~size_t(255);
+                op.who = std::move(part);
+                P.groups[g].push_back(std::move(op));
+            }
         }
     }
     std::vector<char> host(std::max<size_t>(total, 1));
-    for (size_t g = 0; g < members.size(); ++g)
-        for (size_t j = 0; j < P.groups[g].size(); ++j) {
-            const size_t sz = rec[members[g][0]].ops[j].args.size();
-            for (size_t k = 0; k < members[g].size(); ++k)
-                std::memcpy(host.data() + P.groups[g][j].off + k * sz, rec[members[g][k]].ops[j].args.data(), sz);
+    for (const std::vector<BatchOp>& ops : P.groups)
+        for (const BatchOp& op : ops) {
+            const size_t sz = rec[op.who[0]].ops[op.j].args.size();
+            for (size_t k = 0; k < op.who.size(); ++k)
+                std::memcpy(host.data() + op.off + k * sz, rec[op.who[k]].ops[op.j].args.data(), sz);
         }
     if (!P.table || P.tableBytes < total) {
         if (P.ownTable && P.table) (void)hipFree(P.table);

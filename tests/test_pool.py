@@ -222,3 +222,27 @@ def test_batch_rejects_members_that_cannot_share_launches(gpu_lib, runs):
         lbdem.Batch(gpu_lib, [a, a])
     a.close()
     b.close()
+
+
+def test_batch_of_mixed_sizes_equals_alone(gpu_lib):
+    """Boxes whose fused-kernel grids differ by more than 2x (48^3: 72 blocks, 96^3: 576) are
+    launched in grid-size classes; every result still equals the run alone (same z-walk)."""
+    rng = np.random.default_rng(11)
+    runs = [workloads.random_impact_run(rng, (e, e, e), 60, relax=100) for e in (48, 96, 48)]
+    prec = lbdem.PREC_FP32
+    ref = [alone(gpu_lib, run, prec, tile=BATCH_TILE) for run in runs]
+    sims = []
+    for run in runs:
+        sim = lbdem.Simulation(gpu_lib, lbdem.params_from_values(run.values, prec), run.particles, run.objects)
+        sim.init()
+        sim.set_tile(BATCH_TILE)
+        sim.cycle(WARM)
+        sims.append(sim)
+    batch = lbdem.Batch(gpu_lib, sims)
+    batch.cycle(60 - WARM)
+    batch.synchronize()
+    batch.close()
+    for k, sim in enumerate(sims):
+        res = dict(objects=sim.objects(), totalMass=sim.total_mass(), nActive=sim.info()["nActive"])
+        same(res, ref[k], f"box {runs[k].edge} through a mixed-size batch")
+        sim.close()
