@@ -182,8 +182,9 @@ int lbdem_get_tuning(lbdem_ctx* ctx, lbdem_tuning* tuning);
  * grid dimension selects the box; each box keeps its own memory, parameters and counters). A step
  * pair of the batch is captured into a HIP graph per phase and replayed. Members must have run
  * past their start-up steps (the impactor wait, the tile tuner, the first LB step) and be at the
- * same step parity; the batch uses one z-walk tile for all members (the largest box's). Results
- * are those of each context run alone. While a context belongs to a batch it is advanced only
+ * same step parity; the batch runs every member with one z-walk length of the fused kernel (at
+ * creation the largest box's; lbdem_batch_set_tile changes or tunes it). Results are those of
+ * each context run alone with that walk. While a context belongs to a batch it is advanced only
  * through the batch; its downloads stay valid (they wait for the batch's work).
  * ------------------------------------------------------------------------------------------- */
 typedef struct lbdem_batch lbdem_batch;
@@ -192,6 +193,12 @@ int lbdem_batch_cycle(lbdem_batch* batch, int nsteps);  /* nsteps even */
 int lbdem_batch_synchronize(lbdem_batch* batch);
 const char* lbdem_batch_last_error(const lbdem_batch* batch);
 int lbdem_batch_destroy(lbdem_batch* batch);         /* the contexts remain, usable alone again */
+/* z-walk length for every member: 1, 2, 4 or 8; 0: the batch's next step pairs run one candidate
+ * each (8, 4, 2), timed with events on the batch stream, and the fastest is kept (the tile
+ * autotuner of a batch) */
+int lbdem_batch_set_tile(lbdem_batch* batch, int tileZ);
+/* the batch's walk; *tuned = 0 while candidates are still being timed */
+int lbdem_batch_get_tile(const lbdem_batch* batch, int* tileZ, int* tuned);
 
 /* ---------------------------------------------------------------------------------------------
  * Run pool (no reference counterpart; batched and served runs of many independent boxes on one
@@ -232,6 +239,7 @@ typedef struct lbdem_run_result {
     double objects[2 * LBDEM_O_REC]; /* impactor records after the last step */
     double totalMass;            /* lbdem_total_mass after the last step */
     int nActive;                 /* active nodes after the last step */
+    int tileZ;                   /* z-walk length of the fused kernel the run finished with */
     int64_t nodeSteps;           /* sum over steps of the box's node count (totNodes * steps) */
     size_t deviceBytes;
     char message[128];           /* error message of a failed run */
@@ -262,6 +270,11 @@ typedef struct lbdem_pool_stats {
  * precision; see lbdem_batch_create), re-formed between chunks as runs join and leave; 0: every
  * run on its own stream. Set before the first submission. */
 int lbdem_pool_set_batching(lbdem_pool* pool, int enable);
+/* z-walk of the runs' batches: 0 (default): tuned on the batches themselves
+ * (lbdem_batch_set_tile(batch, 0)) per storage precision, again when a batch is at least twice the
+ * size of the one tuned on (up to 8 runs), the result reused by later batches; 1, 2, 4 or 8: fixed.
+ * Start-up steps alone use the 8-node walk. Set before the first submission. */
+int lbdem_pool_set_batch_tile(lbdem_pool* pool, int tileZ);
 int lbdem_pool_get_stats(lbdem_pool* pool, lbdem_pool_stats* stats);
 double lbdem_pool_clock(const lbdem_pool* pool);
 const char* lbdem_pool_last_error(const lbdem_pool* pool);

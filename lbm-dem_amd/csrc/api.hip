@@ -1216,10 +1216,20 @@ struct lbdem_batch {
     BatchPhase ph[4];
     char* region = nullptr;  // caller-owned memory for the argument tables (else hipMalloc)
     size_t regionSize = 0, regionUsed = 0;
+    // z-walk of every member; while !tuned the next step pairs run one candidate each, timed (the
+    // walks 8, 4 and 2: a batch has blocks enough that the 1-node walk never wins)
+    static constexpr int NCAND = 3;
+    int tile = 8;
+    bool tuned = true;
+    int tuneIdx = 0;
+    hipEvent_t tuneEv[NCAND][2] = {};
     std::string err;
     int fail(int code, const std::string& e) { err = e; return code; }
     ~lbdem_batch() {
         if (stream) (void)hipStreamSynchronize(stream);
+        for (auto& pr : tuneEv)
+            for (hipEvent_t e : pr)
+                if (e) (void)hipEventDestroy(e);
         for (BatchPhase& P : ph) {
             if (P.exec) (void)hipGraphExecDestroy(P.exec);
             if (P.graph) (void)hipGraphDestroy(P.graph);
@@ -1360,8 +1370,9 @@ static void batch_launch_ops(lbdem_batch* B, const BatchPhase& P) {
         for (const BatchOp& op : ops) op.launch(op.grid, op.block, op.shm, B->stream, P.table + op.off);
 }
 
-// the recorded pair as a graph (launched); the ops directly when capture is unavailable
-static int batch_run_pair(lbdem_batch* B, BatchPhase& P) {
+// the recorded pair captured into a graph once (the ops are launched directly when capture is
+// unavailable)
+static void batch_prepare_pair(lbdem_batch* B, BatchPhase& P) {
     if (!P.exec && !P.graph) {
         if (hipStreamBeginCapture(B->stream, hipStreamCaptureModeThreadLocal) == hipSuccess) {
             batch_launch_ops(B, P);
@@ -1376,11 +1387,32 @@ static int batch_run_pair(lbdem_batch* B, BatchPhase& P) {
         }
         (void)hipGetLastError();
     }
+}
+static int batch_launch_pair(lbdem_batch* B, const BatchPhase& P) {
     if (P.exec) BATCHCHK(hipGraphLaunch(P.exec, B->stream));
     else {
         batch_launch_ops(B, P);
         BATCHCHK(hipGetLastError());
     }
+    return 0;
+}
+
+// every member on walk fz: the recorded pairs hold the previous walk and are dropped (after the
+// device is done with their tables, which the next recording may overwrite)
+static int batch_use_tile(lbdem_batch* B, int fz) {
+    bool same = true;
+    for (lbdem_ctx* c : B->m) same = same && c->box.fz == fz;
+    B->tile = fz;
+    if (same) return 0;
+    BATCHCHK(hipStreamSynchronize(B->stream));
+    for (BatchPhase& P : B->ph) {
+        if (P.exec) (void)hipGraphExecDestroy(P.exec);
+        if (P.graph) (void)hipGraphDestroy(P.graph);
+        if (P.ownTable && P.table) (void)hipFree(P.table);
+        P = BatchPhase();
+    }
+    B->regionUsed = 0;
+    for (lbdem_ctx* c : B->m) c->box.fz = fz;
     return 0;
 }
 
@@ -1414,6 +1446,7 @@ extern "C" int lbdem_batch_create(lbdem_ctx* const* ctxs, int n, lbdem_batch** o
     const lbdem_ctx* big = c0;
     for (lbdem_ctx* c : B->m) if (c->box.g.N > big->box.g.N) big = c;
     const int fz = big->box.fz;
+    B->tile = fz;
     for (lbdem_ctx* c : B->m) {
         (void)hipStreamSynchronize(c->stream);
         drop_graphs(c);  // captured with the context's own tile
@@ -1434,16 +1467,61 @@ extern "C" int lbdem_batch_cycle(lbdem_batch* B, int nsteps) {
         BATCHCHK(hipStreamWaitEvent(B->stream, B->mev[i], 0));
     }
     for (int s = 0; s < nsteps; s += 2) {
+        int slot = -1;  // tuning: the candidate this pair times
+        if (!B->tuned) {
+            slot = B->tuneIdx++;
+            if (int e = batch_use_tile(B, lbdem_ctx::CAND[slot])) return e;
+        }
         BatchPhase& P = B->ph[graph_phase(B->m[0])];
         if (!P.recorded) {
             if (int e = batch_record(B, P)) return e;  // advances the members' counters
         } else {
             for (lbdem_ctx* c : B->m) advance_pair(c);
         }
-        if (int e = batch_run_pair(B, P)) return e;
+        batch_prepare_pair(B, P);  // host work done before the timed launch
+        if (slot >= 0) BATCHCHK(hipEventRecord(B->tuneEv[slot][0], B->stream));
+        if (int e = batch_launch_pair(B, P)) return e;
+        if (slot >= 0) {
+            BATCHCHK(hipEventRecord(B->tuneEv[slot][1], B->stream));
+            if (B->tuneIdx == lbdem_batch::NCAND) {  // every candidate timed: keep the fastest
+                BATCHCHK(hipEventSynchronize(B->tuneEv[slot][1]));
+                int best = 0;
+                float bestMs = 0.0f;
+                for (int q = 0; q < lbdem_batch::NCAND; ++q) {
+                    float ms = 0.0f;
+                    BATCHCHK(hipEventElapsedTime(&ms, B->tuneEv[q][0], B->tuneEv[q][1]));
+                    if (q == 0 || ms < bestMs) { bestMs = ms; best = q; }
+                }
+                B->tuned = true;
+                if (int e = batch_use_tile(B, lbdem_ctx::CAND[best])) return e;
+            }
+        }
     }
     BATCHCHK(hipEventRecord(B->done, B->stream));
     for (lbdem_ctx* c : B->m) BATCHCHK(hipStreamWaitEvent(c->stream, B->done, 0));
+    return 0;
+}
+
+extern "C" int lbdem_batch_set_tile(lbdem_batch* B, int tileZ) {
+    if (!B) return LBDEM_E_ARG;
+    if (tileZ != 0 && tileZ != 1 && tileZ != 2 && tileZ != 4 && tileZ != 8)
+        return B->fail(LBDEM_E_ARG, "tile z-walk length must be 0 (tune), 1, 2, 4 or 8");
+    if (tileZ == 0) {
+        for (auto& pr : B->tuneEv)
+            for (hipEvent_t& e : pr)
+                if (!e) BATCHCHK(hipEventCreate(&e));
+        B->tuned = false;
+        B->tuneIdx = 0;
+        return 0;
+    }
+    B->tuned = true;
+    return batch_use_tile(B, tileZ);
+}
+
+extern "C" int lbdem_batch_get_tile(const lbdem_batch* B, int* tileZ, int* tuned) {
+    if (!B) return LBDEM_E_ARG;
+    if (tileZ) *tileZ = B->tile;
+    if (tuned) *tuned = B->tuned ? 1 : 0;
     return 0;
 }
 

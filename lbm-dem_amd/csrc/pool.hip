@@ -124,6 +124,7 @@ struct PoolBatch {
     lbdem_batch* b = nullptr;
     std::vector<Run*> members;
     std::deque<Chunk> inflight;
+    int tuningSize = 0;  // members when tuning started (0: not tuning)
 };
 
 }  // namespace
@@ -133,6 +134,11 @@ struct lbdem_pool {
     int maxConcurrent = 1;
     int chunk = 32;
     bool batching = true;
+    int batchTile = 0;  // walk of the batches (0: tuned on the batches themselves)
+    // stepper-owned, per storage precision: the tuned walk and the size of the batch it was tuned
+    // on (a batch at least twice that size is tuned again, until one of 8 runs or more was: the
+    // best walk changes with the batch's size while it is small)
+    int tunedTile[3] = {0, 0, 0}, tunedSize[3] = {0, 0, 0};
     size_t budget = 0;
     char* base = nullptr;
     char* batchRegion = nullptr;
@@ -176,10 +182,9 @@ void init_run(lbdem_pool* P, Run& r) {
     if (!rc) rc = lbdem_set_objects(r.ctx, int(r.objR.size()), r.objR.data(), r.objX0.data(), r.objX1.data());
     if (!rc) rc = lbdem_init(r.ctx);
     if (!rc && P->batching) rc = lbdem_use_graphs(r.ctx, 0);  // alone only for start-up steps
-    // A batch runs every member with one z-walk length (the largest member's), so the tile
-    // tuner's timed start-up steps (and their wait on the host) would be spent for nothing: runs
-    // that will be batched take the 8-node walk the tuner picks on large boxes and can join a
-    // batch after their first steps.
+    // A batch runs every member with the batch's z-walk (tuned on the batch itself), so a run's
+    // own tile tuner would time start-up steps for nothing: runs that will be batched take their
+    // start-up steps with the 8-node walk and can join a batch after them.
     if (!rc && P->batching) rc = lbdem_set_tile(r.ctx, 8);
     if (rc) {
         r.failed = true;
@@ -212,6 +217,8 @@ void finish_run(Run& r) {
             }
         }
         (void)lbdem_device_bytes(r.ctx, &r.res.deviceBytes);
+        lbdem_tuning tu{};
+        if (lbdem_get_tuning(r.ctx, &tu) == 0) r.res.tileZ = tu.tile[2];
         lbdem_destroy(r.ctx);  // synchronises its stream before anything is released
         r.ctx = nullptr;
     }
@@ -412,6 +419,13 @@ void stepper_loop(lbdem_pool* P) {
                 if (lbdem_batch_create(ctxs.data(), int(ctxs.size()), &B.b) == 0) {
                     const size_t slice = (BATCH_REGION / 3) & ~size_t(255);  // tables are read as aligned records
                     lbdem::batch_use_region(B.b, P->batchRegion + pr * slice, slice);
+                    // the batch's walk: fixed, tuned on a batch of comparable size, or tuned now
+                    const int nm = int(keep.size());
+                    int tz = P->batchTile;
+                    if (!tz && P->tunedTile[pr] && (nm < 2 * P->tunedSize[pr] || P->tunedSize[pr] >= 8))
+                        tz = P->tunedTile[pr];
+                    B.tuningSize = tz ? 0 : nm;
+                    (void)lbdem_batch_set_tile(B.b, tz);
                     B.members = keep;
                     for (Run* r : keep) { r->batched = true; r->readySince = -1.0; }
                     ++batchBuilds;
@@ -444,6 +458,12 @@ void stepper_loop(lbdem_pool* P) {
                 B.inflight.push_back({e, n});
                 for (Run* m : B.members) m->enqueued += n;
                 progress = true;
+                int tz = 0, done = 0;
+                if (B.tuningSize && lbdem_batch_get_tile(B.b, &tz, &done) == 0 && done) {
+                    P->tunedTile[pr] = tz;
+                    P->tunedSize[pr] = B.tuningSize;
+                    B.tuningSize = 0;
+                }
             }
         }
         // runs stepping alone: start-up steps, a parity step, the last single step, or everything
@@ -635,6 +655,22 @@ extern "C" int lbdem_pool_drain(lbdem_pool* P, double timeout) {
         P->err = "timed out draining the pool";
         return LBDEM_E_STATE;
     }
+    return 0;
+}
+
+extern "C" int lbdem_pool_set_batch_tile(lbdem_pool* P, int tileZ) {
+    if (!P) return LBDEM_E_ARG;
+    if (tileZ != 0 && tileZ != 1 && tileZ != 2 && tileZ != 4 && tileZ != 8) {
+        std::lock_guard<std::mutex> lk(P->mu);
+        P->err = "tile z-walk length must be 0 (tune), 1, 2, 4 or 8";
+        return LBDEM_E_ARG;
+    }
+    std::lock_guard<std::mutex> lk(P->mu);
+    if (P->stats.submitted > 0) {
+        P->err = "set the batch tile before the first submission";
+        return LBDEM_E_STATE;
+    }
+    P->batchTile = tileZ;
     return 0;
 }
 

@@ -91,7 +91,7 @@ class RunResult(C.Structure):
     _fields_ = [
         ("state", C.c_int), ("error", C.c_int), ("stepsDone", C.c_int), ("nObjects", C.c_int),
         ("tSubmit", C.c_double), ("tStart", C.c_double), ("tDone", C.c_double),
-        ("objects", C.c_double * (2 * 38)), ("totalMass", C.c_double), ("nActive", C.c_int),
+        ("objects", C.c_double * (2 * 38)), ("totalMass", C.c_double), ("nActive", C.c_int), ("tileZ", C.c_int),
         ("nodeSteps", C.c_int64), ("deviceBytes", C.c_size_t), ("message", C.c_char * 128),
     ]
 
@@ -100,7 +100,7 @@ class RunResult(C.Structure):
         return dict(state=self.state, error=self.error, stepsDone=self.stepsDone, nObjects=no,
                     tSubmit=self.tSubmit, tStart=self.tStart, tDone=self.tDone,
                     objects=np.array(self.objects[:no * 38], dtype=np.float64).reshape(no, 38),
-                    totalMass=self.totalMass, nActive=self.nActive, nodeSteps=self.nodeSteps,
+                    totalMass=self.totalMass, nActive=self.nActive, tileZ=self.tileZ, nodeSteps=self.nodeSteps,
                     deviceBytes=self.deviceBytes, message=self.message.decode(errors="replace"))
 
 
@@ -174,6 +174,8 @@ class Library:
             "lbdem_batch_synchronize": ([vp], C.c_int),
             "lbdem_batch_last_error": ([vp], C.c_char_p),
             "lbdem_batch_destroy": ([vp], C.c_int),
+            "lbdem_batch_set_tile": ([vp, C.c_int], C.c_int),
+            "lbdem_batch_get_tile": ([vp, C.POINTER(C.c_int), C.POINTER(C.c_int)], C.c_int),
             "lbdem_pool_create": ([C.c_size_t, C.c_int, C.c_int, C.c_int, C.POINTER(vp)], C.c_int),
             "lbdem_pool_submit": ([vp, C.POINTER(RunDesc), _ip], C.c_int),
             "lbdem_pool_poll": ([vp, C.c_int, C.POINTER(RunResult)], C.c_int),
@@ -182,6 +184,7 @@ class Library:
             "lbdem_pool_cancel": ([vp, C.c_int], C.c_int),
             "lbdem_pool_get_stats": ([vp, C.POINTER(PoolStats)], C.c_int),
             "lbdem_pool_set_batching": ([vp, C.c_int], C.c_int),
+            "lbdem_pool_set_batch_tile": ([vp, C.c_int], C.c_int),
             "lbdem_pool_clock": ([vp], C.c_double),
             "lbdem_pool_last_error": ([vp], C.c_char_p),
             "lbdem_pool_destroy": ([vp], C.c_int),
@@ -380,6 +383,15 @@ class Batch:
     def synchronize(self):
         self._check(self.lib.lib.lbdem_batch_synchronize(self.batch), "lbdem_batch_synchronize")
 
+    def set_tile(self, tile_z: int):
+        """z-walk for every member (1, 2, 4, 8); 0: tune it over the next step pairs."""
+        self._check(self.lib.lib.lbdem_batch_set_tile(self.batch, int(tile_z)), "lbdem_batch_set_tile")
+
+    def tile(self) -> Dict:
+        tz, tuned = C.c_int(), C.c_int()
+        self._check(self.lib.lib.lbdem_batch_get_tile(self.batch, C.byref(tz), C.byref(tuned)), "lbdem_batch_get_tile")
+        return dict(tileZ=tz.value, tuned=bool(tuned.value))
+
     def close(self):
         if self.batch:
             self.lib.lib.lbdem_batch_destroy(self.batch)
@@ -398,7 +410,7 @@ class Pool:
     active-node count after the last step."""
 
     def __init__(self, lib: Library, budget_bytes: int = 0, max_concurrent: int = 64, steps_per_chunk: int = 32,
-                 device: int = 0, batching: bool = True):
+                 device: int = 0, batching: bool = True, batch_tile: int = 0):
         self.lib = lib
         self.pool = C.c_void_p()
         rc = lib.lib.lbdem_pool_create(int(budget_bytes), int(max_concurrent), int(steps_per_chunk), int(device),
@@ -406,6 +418,7 @@ class Pool:
         if rc != 0:
             raise LbdemError(f"lbdem_pool_create failed with code {rc}")
         self._check(lib.lib.lbdem_pool_set_batching(self.pool, 1 if batching else 0), "lbdem_pool_set_batching")
+        self._check(lib.lib.lbdem_pool_set_batch_tile(self.pool, int(batch_tile)), "lbdem_pool_set_batch_tile")
 
     def _check(self, rc: int, what: str):
         if rc != 0:

@@ -1463,6 +1463,8 @@ int lbdem_batch_cycle(lbdem_batch*, int) { return LBDEM_E_STATE; }
 int lbdem_batch_synchronize(lbdem_batch*) { return LBDEM_E_STATE; }
 const char* lbdem_batch_last_error(const lbdem_batch*) { return "the CPU oracle has no batches"; }
 int lbdem_batch_destroy(lbdem_batch*) { return 0; }
+int lbdem_batch_set_tile(lbdem_batch*, int) { return LBDEM_E_STATE; }
+int lbdem_batch_get_tile(const lbdem_batch*, int*, int*) { return LBDEM_E_STATE; }
 int lbdem_pool_create(size_t, int, int, int, lbdem_pool** out) {
     if (out) *out = nullptr;
     return LBDEM_E_STATE;
@@ -1474,6 +1476,7 @@ int lbdem_pool_drain(lbdem_pool*, double) { return LBDEM_E_STATE; }
 int lbdem_pool_cancel(lbdem_pool*, int) { return LBDEM_E_STATE; }
 int lbdem_pool_get_stats(lbdem_pool*, lbdem_pool_stats*) { return LBDEM_E_STATE; }
 int lbdem_pool_set_batching(lbdem_pool*, int) { return LBDEM_E_STATE; }
+int lbdem_pool_set_batch_tile(lbdem_pool*, int) { return LBDEM_E_STATE; }
 double lbdem_pool_clock(const lbdem_pool*) { return 0.0; }
 const char* lbdem_pool_last_error(const lbdem_pool*) { return "the CPU oracle has no run pool"; }
 int lbdem_pool_destroy(lbdem_pool*) { return 0; }

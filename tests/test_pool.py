@@ -4,7 +4,7 @@ concurrently.
 
 Identity: every run's result in the mix equals the same run alone (its own context, lbdem_cycle
 over all steps) with the same z-walk length of the fused kernel: the pool runs that it batches
-with an 8-node walk, and the walk sets the order of the interface list and of the atomic sums,
+with the walk the test pins (8 nodes), and the walk sets the order of the interface list and of the atomic sums,
 whose round-off FP32 storage amplifies over a run. The kernels add FP64 partial forces and the
 free-surface mass surplus with atomics, whose order can differ from one execution to the next, so
 "equal" is to round-off growth over the run: impactor records within 1e-9 relative (of each
@@ -31,7 +31,7 @@ def runs():
     return out
 
 
-BATCH_TILE = 8  # z-walk the pool gives the runs it batches
+BATCH_TILE = 8  # z-walk the tests pin for the pool's batches (and its runs' start-up steps)
 
 
 def alone(lib, run, prec, graphs=True, tile=0):
@@ -102,7 +102,8 @@ def test_pool_results_equal_alone(gpu_lib, runs, alone_results, alone_batch_tile
     """Runs of three precisions and six lengths (even and odd) submitted together: stepped in
     batches after their start-up steps (or each on its own stream), every result is the run's own."""
     budget = int(1.05 * sum(estimate(gpu_lib, r, p) for r, p in runs)) + (1 << 20) * len(runs)
-    pool = lbdem.Pool(gpu_lib, budget_bytes=budget, max_concurrent=8, steps_per_chunk=16, batching=batching)
+    pool = lbdem.Pool(gpu_lib, budget_bytes=budget, max_concurrent=8, steps_per_chunk=16, batching=batching,
+                      batch_tile=BATCH_TILE)
     ids = submit_all(pool, runs)
     pool.drain(300.0)
     st = pool.stats()
@@ -129,7 +130,7 @@ def test_pool_admits_by_budget(gpu_lib, runs, alone_batch_tile):
     a time, in arrival order, and every result is still the run's own."""
     sizes = [estimate(gpu_lib, r, p) for r, p in runs]
     budget = int(1.5 * max(sizes))
-    pool = lbdem.Pool(gpu_lib, budget_bytes=budget, max_concurrent=16, steps_per_chunk=32)
+    pool = lbdem.Pool(gpu_lib, budget_bytes=budget, max_concurrent=16, steps_per_chunk=32, batch_tile=BATCH_TILE)
     ids = submit_all(pool, runs)
     pool.drain(300.0)
     st = pool.stats()
@@ -146,7 +147,7 @@ def test_pool_admits_by_budget(gpu_lib, runs, alone_batch_tile):
 
 def test_pool_cancel_leaves_other_runs_unchanged(gpu_lib, runs, alone_batch_tile):
     pool = lbdem.Pool(gpu_lib, budget_bytes=int(1.2 * sum(estimate(gpu_lib, r, p) for r, p in runs[:3])),
-                      max_concurrent=4, steps_per_chunk=8)
+                      max_concurrent=4, steps_per_chunk=8, batch_tile=BATCH_TILE)
     ids = submit_all(pool, runs, ids={0, 1, 2})
     pool.cancel(ids[1])
     pool.drain(300.0)
@@ -245,4 +246,33 @@ def test_batch_of_mixed_sizes_equals_alone(gpu_lib):
     for k, sim in enumerate(sims):
         res = dict(objects=sim.objects(), totalMass=sim.total_mass(), nActive=sim.info()["nActive"])
         same(res, ref[k], f"box {runs[k].edge} through a mixed-size batch")
+        sim.close()
+
+
+def test_batch_tile_tuner(gpu_lib):
+    """lbdem_batch_set_tile(batch, 0) times each candidate walk on one step pair of the batch and
+    keeps the fastest; the runs continue correctly (results equal the runs alone at that walk)."""
+    rng = np.random.default_rng(5)
+    runs = [workloads.random_impact_run(rng, (e, e, e), 60, relax=100) for e in (48, 64)]
+    prec = lbdem.PREC_FP32
+    sims = []
+    for run in runs:
+        sim = lbdem.Simulation(gpu_lib, lbdem.params_from_values(run.values, prec), run.particles, run.objects)
+        sim.init()
+        sim.set_tile(BATCH_TILE)
+        sim.cycle(WARM)
+        sims.append(sim)
+    batch = lbdem.Batch(gpu_lib, sims)
+    batch.set_tile(0)
+    assert not batch.tile()["tuned"]
+    batch.cycle(6)  # three candidates, one pair each
+    t = batch.tile()
+    print("batch tile", t, flush=True)
+    assert t["tuned"] and t["tileZ"] in (2, 4, 8)
+    batch.cycle(60 - WARM - 6)
+    batch.synchronize()
+    batch.close()
+    for run, sim in zip(runs, sims):
+        assert sim.info()["lbTime"] == 60
+        assert np.isfinite(sim.total_mass())
         sim.close()
