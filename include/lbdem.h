@@ -272,8 +272,9 @@ typedef struct lbdem_pool_stats {
 int lbdem_pool_set_batching(lbdem_pool* pool, int enable);
 /* z-walk of the runs' batches: 0 (default): tuned on the batches themselves
  * (lbdem_batch_set_tile(batch, 0)) per storage precision, again when a batch is at least twice the
- * size of the one tuned on (up to 8 runs), the result reused by later batches; 1, 2, 4 or 8: fixed.
- * Start-up steps alone use the 8-node walk. Set before the first submission. */
+ * size of the one tuned on (up to 8 runs), the result reused by later batches, and start-up steps
+ * alone use the 8-node walk (with batching off each run tunes its own); 1, 2, 4 or 8: fixed for
+ * every step of every run, batched or not. Set before the first submission. */
 int lbdem_pool_set_batch_tile(lbdem_pool* pool, int tileZ);
 int lbdem_pool_get_stats(lbdem_pool* pool, lbdem_pool_stats* stats);
 double lbdem_pool_clock(const lbdem_pool* pool);

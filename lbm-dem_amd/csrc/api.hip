@@ -412,7 +412,7 @@ static bool plan_buffers(const Plan& p, Box& b, DemState& d, DevScalars*& sc, do
     A(d.objSum, 12 * MAX_OBJECTS);
     A(d.redPart, 2 * 64 * 12 * MAX_OBJECTS);
     A(d.wallSum, 6 * MAX_WALLS);
-    A(d.blockMax, 64);
+    A(d.blockMax, VEL_SLOTS);
     double** ovecs[] = {&d.ox0, &d.ox1, &d.ow1, &d.oFHydro, &d.oMHydro, &d.oFContact, &d.oMContact, &d.oFLub,
                         &d.oMLub, &d.oFTotal, &d.oMTotal, &d.oFGrav};
     for (double** v : ovecs) A(*v, 3 * no1);
@@ -557,7 +557,7 @@ extern "C" int lbdem_init(lbdem_ctx* c) {
     // particle hydrodynamic sums: zero before the first LB step (k_hydro_finalize re-zeroes them)
     HIPCHK(hipMemsetAsync(b.hacc, 0, size_t(std::max(int(c->pr.size()), 1)) * 8 * sizeof(double), st));
     // largest particle |v|^2 of the step, for the neighbour-list trigger: velocities start at zero
-    HIPCHK(hipMemsetAsync(c->ds.blockMax, 0, sizeof(unsigned long long), st));
+    HIPCHK(hipMemsetAsync(c->ds.blockMax, 0, VEL_SLOTS * sizeof(unsigned long long), st));
     // DEM cell list: each rebuild clears it after use (k_nbr_slots), the first one finds it zero
     HIPCHK(hipMemsetAsync(c->ds.cellCount, 0, size_t(plan.ncell) * sizeof(int), st));
     HIPCHK(hipMemsetAsync(c->ds.cellFill, 0, size_t(plan.ncell) * sizeof(int), st));

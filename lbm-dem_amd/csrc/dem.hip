@@ -32,13 +32,24 @@ __device__ inline V sub(V a, V b) { return V{a.x - b.x, a.y - b.y, a.z - b.z}; }
 // neighbour-list trigger (DEM.cpp:366-378, 460-470)
 // ------------------------------------------------------------------------------------------
 // The trigger needs the largest |v|^2 over the particles at the start of the step: the
-// velocities the previous step's k_dem_integrate wrote, which folds them into velMax with an
-// atomic maximum on the bit patterns (non-negative doubles order like their bit patterns; zero at
-// initialisation, when every velocity is zero). k_maxdisp reads and clears it.
+// velocities the previous step's k_dem_integrate wrote, which folds them into the VEL_SLOTS slots
+// of velMax with atomic maxima on the bit patterns (block b into slot b % VEL_SLOTS, so no slot
+// takes more than a few atomics; non-negative doubles order like their bit patterns; zero at
+// initialisation, when every velocity is zero). k_maxdisp (one wavefront) reads and clears them.
+static_assert(VEL_SLOTS == 64, "k_maxdisp reduces the slots in one wavefront");
 __device__ inline void k_maxdisp(unsigned long long* velMax, double deltat, DevScalars* sc, int force) {
-    if (threadIdx.x != 0) return;
-    const double m = __longlong_as_double(*velMax);
-    *velMax = 0ull;
+    const int t = threadIdx.x;
+    double m = 0.0;
+    if (t < VEL_SLOTS) {
+        m = __longlong_as_double(velMax[t]);
+        velMax[t] = 0ull;
+    }
+#pragma unroll
+    for (int off = 32; off > 0; off >>= 1) {
+        const double o = __shfl_xor(m, off, 64);
+        m = o > m ? o : m;
+    }
+    if (t != 0) return;
     if (force) {
         sc->rebuild = 1;
         sc->builds += 1;
@@ -603,8 +614,18 @@ __device__ inline void k_dem_reduce_a(const double* oc, int nco, const double* w
     const int step = (RED_BS / ncol) * ncol;
     double acc = 0.0;
     const int t = threadIdx.x;
-    if (t < step)
-        for (int64_t i = beg + t; i < end; i += step) acc += contrib[i];
+    if (t < step) {
+        // eight loads in flight, added in index order (the same sum as one at a time)
+        int64_t i = beg + t;
+        for (; i + 7 * int64_t(step) < end; i += 8 * int64_t(step)) {
+            double v[8];
+#pragma unroll
+            for (int u = 0; u < 8; ++u) v[u] = contrib[i + u * int64_t(step)];
+#pragma unroll
+            for (int u = 0; u < 8; ++u) acc += v[u];
+        }
+        for (; i < end; i += step) acc += contrib[i];
+    }
     red[t] = t < step ? acc : 0.0;
     __syncthreads();
     if (t < ncol) {
@@ -613,29 +634,42 @@ __device__ inline void k_dem_reduce_a(const double* oc, int nco, const double* w
         part[(int64_t(role) * RED_BLOCKS + bx) * RED_MAXCOL + t] = s;
     }
 }
-// One block: the second level of the sums (the block partials of each column in block order; or
-// zeros when no particle contributes), then, on thread 0, objects: totals and integration
-// (DEM.cpp:786-790, 2742-2758) and walls: totals (DEM.cpp:773-775).
+// One block (the last block of the integration launch, which runs beside the particles' blocks:
+// neither reads what the other writes): the second level of the sums (the block partials of each
+// column in block order; or zeros when no particle contributes), then, on thread 0, objects:
+// totals and integration (DEM.cpp:786-790, 2742-2758) and walls: totals (DEM.cpp:773-775).
 constexpr int OBJ_THREADS = 128;
+static_assert(RED_BLOCKS % 16 == 0, "column_sum loads sixteen partials at a time");
+// the RED_BLOCKS block partials of one column (stride RED_MAXCOL), added in block order; the loads
+// are issued sixteen at a time
+__device__ inline double column_sum(const double* col) {
+    double s = 0.0;
+#pragma unroll
+    for (int b0 = 0; b0 < RED_BLOCKS; b0 += 16) {
+        double v[16];
+#pragma unroll
+        for (int u = 0; u < 16; ++u) v[u] = col[int64_t(b0 + u) * RED_MAXCOL];
+#pragma unroll
+        for (int u = 0; u < 16; ++u) s += v[u];
+    }
+    return s;
+}
 static_assert(OBJ_THREADS >= 12 * MAX_OBJECTS && OBJ_THREADS >= 6 * MAX_WALLS, "one thread per column");
-__device__ inline void k_dem_objects(DemParams P, DemState d, const double* part, int sumsFromParts, double* objSum,
-                                     double* wallSum, int demTimeStep) {
+__device__ inline void dem_objects(DemParams P, DemState d, const double* part, int sumsFromParts, double* objSum,
+                                   double* wallSum, int demTimeStep) {
 #pragma clang fp contract(off)
     __shared__ double sObj[12 * MAX_OBJECTS], sWall[6 * MAX_WALLS];
-    if (blockIdx.x != 0) return;
     const int t = threadIdx.x;
     const int nco = 12 * P.nObj, ncw = 6 * P.nWall;
     if (t < 12 * MAX_OBJECTS) {
         double s = 0.0;
-        if (sumsFromParts && t < nco)
-            for (int b = 0; b < RED_BLOCKS; ++b) s += part[int64_t(b) * RED_MAXCOL + t];
+        if (sumsFromParts && t < nco) s = column_sum(part + t);
         sObj[t] = s;
         objSum[t] = s;
     }
     if (t < 6 * MAX_WALLS) {
         double s = 0.0;
-        if (sumsFromParts && t < ncw)
-            for (int b = 0; b < RED_BLOCKS; ++b) s += part[(int64_t(RED_BLOCKS) + b) * RED_MAXCOL + t];
+        if (sumsFromParts && t < ncw) s = column_sum(part + int64_t(RED_BLOCKS) * RED_MAXCOL + t);
         sWall[t] = s;
         wallSum[t] = s;
     }
@@ -701,8 +735,15 @@ __device__ inline double integrate_particle(const DemParams& P, const DemState& 
     d.plb[p] = v;
     return x1[0] * x1[0] + x1[1] * x1[1] + x1[2] * x1[2];
 }
-__device__ inline void k_dem_integrate(DemParams P, DemState d, int demTimeStep) {
+// Blocks [0, nb) integrate the particles (OBJ_THREADS per block), block nb does the objects and
+// walls (dem_objects); later blocks (a batch launch sized for a larger box) return.
+__device__ inline void k_dem_integrate(DemParams P, DemState d, const double* part, int sumsFromParts, int demTimeStep,
+                                       int nb) {
 #pragma clang fp contract(off)
+    if (int(blockIdx.x) >= nb) {
+        if (int(blockIdx.x) == nb) dem_objects(P, d, part, sumsFromParts, d.objSum, d.wallSum, demTimeStep);
+        return;
+    }
     const int p = blockIdx.x * blockDim.x + threadIdx.x;
     double v2 = 0.0;  // |v|^2 after the update, for the next step's neighbour-list trigger
     if (p < P.nPart) v2 = integrate_particle(P, d, demTimeStep, p);
@@ -711,7 +752,8 @@ __device__ inline void k_dem_integrate(DemParams P, DemState d, int demTimeStep)
         const double o = __shfl_xor(v2, off, 64);
         v2 = o > v2 ? o : v2;
     }
-    if (__lane_id() == 0 && v2 > 0.0) atomicMax(d.blockMax, (unsigned long long)__double_as_longlong(v2));
+    if (__lane_id() == 0 && v2 > 0.0)
+        atomicMax(d.blockMax + blockIdx.x % VEL_SLOTS, (unsigned long long)__double_as_longlong(v2));
 }
 
 __device__ inline void k_dem_views(DemParams P, DemState d) {
@@ -751,7 +793,7 @@ static NbrArgs nbr_args(const DemParams& p, DemState& d, DevScalars* sc) {
 }
 
 void dem_neighbor_list(const DemParams& p, DemState& d, DevScalars* sc, bool force, hipStream_t s) {
-    launch<&k_maxdisp, 1024, 1>(1, 64, s, d.blockMax, p.deltat, sc, force ? 1 : 0);
+    launch<&k_maxdisp, 1024, 1>(1, VEL_SLOTS, s, d.blockMax, p.deltat, sc, force ? 1 : 0);
     if (p.nPart <= 0) return;
     NbrArgs a = nbr_args(p, d, sc);
     const int pb = (p.nPart + 127) / 128;
@@ -781,8 +823,9 @@ void dem_step(const DemParams& p, DemState& d, DevScalars* sc, int demTimeStep, 
         double* t = d.xiCur; d.xiCur = d.xiNext; d.xiNext = t;
     }
     const int fromParts = (p.nPart > 0 && (p.nObj > 0 || p.nWall > 0)) ? 1 : 0;  // else zero sums
-    launch<&k_dem_objects, OBJ_THREADS, 1>(1, OBJ_THREADS, s, p, d, (const double*)d.redPart, fromParts, d.objSum, d.wallSum, demTimeStep);
-    if (p.nPart > 0) launch<&k_dem_integrate, 1024, 1>((p.nPart + 127) / 128, 128, s, p, d, demTimeStep);
+    const int nb = (p.nPart + OBJ_THREADS - 1) / OBJ_THREADS;
+    launch<&k_dem_integrate, OBJ_THREADS, 1>(nb + 1, OBJ_THREADS, s, p, d, (const double*)d.redPart, fromParts,
+                                             demTimeStep, nb);
 }
 
 }  // namespace lbdem

@@ -231,6 +231,7 @@ struct DemParams {
     double wallP[MAX_WALLS][3], wallN[MAX_WALLS][3];
     double unitLength, unitSpeed, unitAngVel;
 };
+constexpr int VEL_SLOTS = 64;  // slots of the particle speed maximum (DemState::blockMax)
 struct DemState {
     // particles (FP64, vectors stored as 3 consecutive doubles)
     double *x0, *x1, *w1, *r, *m, *I, *im, *iI;
@@ -254,7 +255,7 @@ struct DemState {
     double* objSum;            // 12*MAX_OBJECTS
     double* redPart;           // 2*64*12*MAX_OBJECTS block partials of the object/wall reduction
     double* wallSum;           // 6*MAX_WALLS
-    unsigned long long* blockMax;  // [0]: largest particle |v|^2 of the last integration (bit pattern)
+    unsigned long long* blockMax;  // [VEL_SLOTS]: largest particle |v|^2 of the last integration, per slot (bit patterns)
     // objects
     double *ox0, *ox1, *ow1, *or_, *om, *oI, *oim, *oiI;
     double *oFHydro, *oMHydro, *oFContact, *oMContact, *oFLub, *oMLub, *oFTotal, *oMTotal, *oFGrav;

@@ -182,10 +182,11 @@ void init_run(lbdem_pool* P, Run& r) {
     if (!rc) rc = lbdem_set_objects(r.ctx, int(r.objR.size()), r.objR.data(), r.objX0.data(), r.objX1.data());
     if (!rc) rc = lbdem_init(r.ctx);
     if (!rc && P->batching) rc = lbdem_use_graphs(r.ctx, 0);  // alone only for start-up steps
-    // A batch runs every member with the batch's z-walk (tuned on the batch itself), so a run's
-    // own tile tuner would time start-up steps for nothing: runs that will be batched take their
-    // start-up steps with the 8-node walk and can join a batch after them.
-    if (!rc && P->batching) rc = lbdem_set_tile(r.ctx, 8);
+    // A pinned walk (lbdem_pool_set_batch_tile) is every step's, batched or not. Otherwise a batch
+    // runs every member with the batch's z-walk (tuned on the batch itself), so a run's own tile
+    // tuner would time start-up steps for nothing: runs that will be batched take their start-up
+    // steps with the 8-node walk and can join a batch after them.
+    if (!rc && (P->batchTile || P->batching)) rc = lbdem_set_tile(r.ctx, P->batchTile ? P->batchTile : 8);
     if (rc) {
         r.failed = true;
         r.error = rc;

@@ -3,9 +3,9 @@ kernel; the pool admits independent impact runs against an HBM budget and steps 
 concurrently.
 
 Identity: every run's result in the mix equals the same run alone (its own context, lbdem_cycle
-over all steps) with the same z-walk length of the fused kernel: the pool runs that it batches
-with the walk the test pins (8 nodes), and the walk sets the order of the interface list and of the atomic sums,
-whose round-off FP32 storage amplifies over a run. The kernels add FP64 partial forces and the
+over all steps) with the same z-walk length of the fused kernel: the tests pin the walk (8 nodes)
+on both sides, since the walk sets the order of the interface list and of the atomic sums, whose
+round-off FP32 storage amplifies over a run, and a tuned walk depends on timings. The kernels add FP64 partial forces and the
 free-surface mass surplus with atomics, whose order can differ from one execution to the next, so
 "equal" is to round-off growth over the run: impactor records within 1e-9 relative (of each
 record's largest entry), total fluid mass within 1e-12 relative, active node count exact."""
@@ -31,7 +31,7 @@ def runs():
     return out
 
 
-BATCH_TILE = 8  # z-walk the tests pin for the pool's batches (and its runs' start-up steps)
+BATCH_TILE = 8  # z-walk the tests pin: the pool's runs, the batches' members and the runs alone
 
 
 def alone(lib, run, prec, graphs=True, tile=0):
@@ -51,12 +51,7 @@ def alone(lib, run, prec, graphs=True, tile=0):
 
 @pytest.fixture(scope="module")
 def alone_results(gpu_lib, runs):
-    return [alone(gpu_lib, run, prec) for run, prec in runs]
-
-
-@pytest.fixture(scope="module")
-def alone_batch_tile(gpu_lib, runs):
-    """The runs alone with the z-walk the pool gives the runs it batches."""
+    """The runs alone with the pinned z-walk."""
     return [alone(gpu_lib, run, prec, tile=BATCH_TILE) for run, prec in runs]
 
 
@@ -94,11 +89,11 @@ def test_estimate_equals_allocation(gpu_lib, runs, alone_results):
 
 def test_graphs_do_not_change_results(gpu_lib, runs, alone_results):
     run, prec = runs[0]
-    same(alone(gpu_lib, run, prec, graphs=False), alone_results[0], "eager vs graph replay")
+    same(alone(gpu_lib, run, prec, graphs=False, tile=BATCH_TILE), alone_results[0], "eager vs graph replay")
 
 
 @pytest.mark.parametrize("batching", [True, False], ids=["batches", "streams"])
-def test_pool_results_equal_alone(gpu_lib, runs, alone_results, alone_batch_tile, batching):
+def test_pool_results_equal_alone(gpu_lib, runs, alone_results, batching):
     """Runs of three precisions and six lengths (even and odd) submitted together: stepped in
     batches after their start-up steps (or each on its own stream), every result is the run's own."""
     budget = int(1.05 * sum(estimate(gpu_lib, r, p) for r, p in runs)) + (1 << 20) * len(runs)
@@ -114,7 +109,8 @@ def test_pool_results_equal_alone(gpu_lib, runs, alone_results, alone_batch_tile
         assert res["stepsDone"] == runs[i][0].nsteps
         assert res["nodeSteps"] == runs[i][0].nodes * runs[i][0].nsteps
         assert res["tSubmit"] <= res["tStart"] <= res["tDone"]
-        same(res, (alone_batch_tile if batching else alone_results)[i], f"run {i} in the mix")
+        assert res["tileZ"] == BATCH_TILE
+        same(res, alone_results[i], f"run {i} in the mix")
     assert st["done"] == len(runs) and st["failed"] == 0
     assert st["activePeak"] >= 2, "runs did not overlap"
     assert st["bytesPeak"] <= budget and st["bytesInUse"] == 0
@@ -125,7 +121,7 @@ def test_pool_results_equal_alone(gpu_lib, runs, alone_results, alone_batch_tile
     pool.close()
 
 
-def test_pool_admits_by_budget(gpu_lib, runs, alone_batch_tile):
+def test_pool_admits_by_budget(gpu_lib, runs, alone_results):
     """A budget that holds the largest run but not two of them: runs are admitted one or two at
     a time, in arrival order, and every result is still the run's own."""
     sizes = [estimate(gpu_lib, r, p) for r, p in runs]
@@ -141,11 +137,11 @@ def test_pool_admits_by_budget(gpu_lib, runs, alone_batch_tile):
     for i, rid in enumerate(ids):
         res = pool.poll(rid)
         assert res["state"] == lbdem.RUN_DONE, res["message"]
-        same(res, alone_batch_tile[i], f"run {i} under a tight budget")
+        same(res, alone_results[i], f"run {i} under a tight budget")
     pool.close()
 
 
-def test_pool_cancel_leaves_other_runs_unchanged(gpu_lib, runs, alone_batch_tile):
+def test_pool_cancel_leaves_other_runs_unchanged(gpu_lib, runs, alone_results):
     pool = lbdem.Pool(gpu_lib, budget_bytes=int(1.2 * sum(estimate(gpu_lib, r, p) for r, p in runs[:3])),
                       max_concurrent=4, steps_per_chunk=8, batch_tile=BATCH_TILE)
     ids = submit_all(pool, runs, ids={0, 1, 2})
@@ -155,7 +151,7 @@ def test_pool_cancel_leaves_other_runs_unchanged(gpu_lib, runs, alone_batch_tile
     for k in (0, 2):
         res = pool.poll(ids[k])
         assert res["state"] == lbdem.RUN_DONE, res["message"]
-        same(res, alone_batch_tile[k], f"run {k} beside a cancelled run")
+        same(res, alone_results[k], f"run {k} beside a cancelled run")
     st = pool.stats()
     assert st["cancelled"] == 1 and st["bytesInUse"] == 0
     pool.close()
@@ -189,6 +185,7 @@ def test_batch_results_equal_alone(gpu_lib, runs, alone_results):
         run, prec = runs[i]
         sim = lbdem.Simulation(gpu_lib, lbdem.params_from_values(run.values, prec), run.particles, run.objects)
         sim.init()
+        sim.set_tile(BATCH_TILE)
         sim.cycle(WARM)
         sims.append(sim)
     common = min(runs[i][0].nsteps for i in idx) - WARM
