@@ -696,9 +696,8 @@ __device__ inline uint32_t inactive_bytes(uint32_t w) {
     m |= m >> 14;
     return m & 0xFu;
 }
-constexpr int MARK_GRID = 4096;
-__device__ __attribute__((always_inline)) inline void k_mark_full_w(Geom g, uint8_t* type, const int32_t* solid, uint64_t* bcm, const DevScalars* sc) {
-    if (!sc->fullMark) return;
+constexpr int MARK_GRID = 1024;  // blocks of k_mark (grid-stride; the full pass is rare)
+__device__ __attribute__((always_inline)) inline void mark_full_w(Geom g, uint8_t* type, const int32_t* solid, uint64_t* bcm) {
     const int W = g.X / 4;
     const uint32_t* t32 = reinterpret_cast<const uint32_t*>(type);
     const int64_t total = int64_t(g.Y) * g.Z * W;
@@ -763,16 +762,21 @@ __device__ inline void mark_node(const Geom& g, uint8_t* type, const int32_t* so
     if (nt != t) type[i] = nt;
 }
 // full pass for rows whose length is not a multiple of four: one thread per node
-__device__ inline void k_mark_full_b(Geom g, uint8_t* type, const int32_t* solid, uint64_t* bcm, const DevScalars* sc) {
-    if (!sc->fullMark) return;
+__device__ inline void mark_full_b(Geom g, uint8_t* type, const int32_t* solid, uint64_t* bcm) {
     for (int64_t i = int64_t(blockIdx.x) * blockDim.x + threadIdx.x; i < g.N; i += int64_t(gridDim.x) * blockDim.x)
         mark_node(g, type, solid, bcm, int32_t(i));
 }
-// incremental pass: the stencil neighbourhood (wrapped) of every node on the change list. A node
-// reached from several changed nodes is marked several times with the same result.
-__device__ inline void k_remark(Geom g, uint8_t* type, const int32_t* solid, uint64_t* bcm, const int32_t* changed, int cap,
-                         const DevScalars* sc) {
-    if (sc->fullMark) return;
+// One launch: the full pass when the full-pass flag is set (initialisation, a flag rebuild that
+// clears wall indices, change-list overflow), else the incremental pass: the stencil
+// neighbourhood (wrapped) of every node on the change list. A node reached from several changed
+// nodes is marked several times with the same result.
+__device__ inline void k_mark(Geom g, uint8_t* type, const int32_t* solid, uint64_t* bcm, const int32_t* changed, int cap,
+                              const DevScalars* sc) {
+    if (sc->fullMark) {
+        if (g.X % 4 == 0) mark_full_w(g, type, solid, bcm);
+        else mark_full_b(g, type, solid, bcm);
+        return;
+    }
     const int n = min(sc->nChanged, cap);
     for (int64_t t = int64_t(blockIdx.x) * blockDim.x + threadIdx.x; t < int64_t(n) * Q;
          t += int64_t(gridDim.x) * blockDim.x) {
@@ -787,9 +791,7 @@ __device__ inline void k_remark(Geom g, uint8_t* type, const int32_t* solid, uin
 }
 
 void lb_mark_edges(const Box& b, DevScalars* sc, hipStream_t s) {
-    if (b.g.X % 4 == 0) launch_gs<&k_mark_full_w, 256, 1>(MARK_GRID, 256, s, b.g, b.type, b.solid, b.bcm, sc);
-    else launch_gs<&k_mark_full_b, 1024, 1>(MARK_GRID, 256, s, b.g, b.type, b.solid, b.bcm, sc);
-    launch_gs<&k_remark, 1024, 1>(256, 256, s, b.g, b.type, b.solid, b.bcm, b.changed, b.changedCap, sc);
+    launch_gs<&k_mark, 256, 1>(MARK_GRID, 256, s, b.g, b.type, b.solid, b.bcm, b.changed, b.changedCap, sc);
     // the list and the full-pass flag are reset by the next kernel (k_begin_lb)
 }
 
@@ -834,10 +836,19 @@ static void launch_fused(Box& b, const Consts& k, const PartLB* plb, const ObjLB
     b.parity ^= 1;
 }
 
-// after the edge marking, before the fused kernel: the change list is consumed, the interface
-// list starts empty (the particle sums were zeroed by the previous step's k_hydro_finalize, or
+// end of findNewActive's scan (LB.cpp:1980-2029): the lowest node still inside its particle
+// becomes the next step's starting point (-1: none)
+__device__ inline void fna_finish(DevScalars* sc) {
+    if (!sc->needScan) return;
+    sc->kminP = sc->kstar == INT_MAX ? -1 : sc->kstar;
+    sc->needScan = 0;
+}
+
+// after the edge marking, before the fused kernel: the findNewActive scan's result is taken, the
+// change list is consumed, the interface list starts empty (the particle sums were zeroed by the previous step's k_hydro_finalize, or
 // at initialisation)
 __device__ inline void k_begin_lb(DevScalars* sc) {
+    fna_finish(sc);
     sc->fullMark = 0;
     sc->nChanged = 0;
     sc->nIf = 0;
@@ -1235,6 +1246,9 @@ void lb_free_surface(Box& b, const Consts& k, DevScalars* sc, hipStream_t s) {
 constexpr int FLAGS_GRID = 2048;  // grids of the flag passes (grid-stride; most steps return early)
 __device__ inline void k_flags_compute(Geom g, const uint8_t* type, const PartLB* plb, int nPart, int32_t* flagNew,
                                 DevScalars* sc) {
+    // a coupling step that follows a coupling step without an LB step between them (the DEM
+    // start-up repeat) finishes the previous findNewActive here (no other block touches kminP)
+    if (blockIdx.x == 0 && threadIdx.x == 0) fna_finish(sc);
     if (!sc->newNeighborList) return;
     if (blockIdx.x == 0 && threadIdx.x == 0) sc->kminP = INT_MAX;  // lowest flagged index, from k_flags_apply
     const int dims[3] = {g.X, g.Y, g.Z};
@@ -1313,6 +1327,7 @@ void lb_flags_rebuild_apply(const Box& b, DevScalars* sc, hipStream_t s) {
 }
 
 __device__ inline void k_fna_check(Geom g, const int32_t* solid, const PartLB* plb, int nPart, DevScalars* sc) {
+    fna_finish(sc);  // a scan still pending (boxes without particles have no k_flags_compute)
     if (sc->newNeighborList) {  // end of the flag pass (k_flags_compute set kminP to INT_MAX)
         if (sc->kminP == INT_MAX) sc->kminP = -1;
         sc->newNeighborList = 0;
@@ -1358,19 +1373,15 @@ __device__ inline void k_fna_unflag(Geom g, uint8_t* type, int32_t* solid, DevSc
         if (type[i] & T_PMASK) { type[i] &= uint8_t(~T_PMASK); solid[i] = 0; }
     }
 }
-__device__ inline void k_fna_end(DevScalars* sc) {
-    if (!sc->needScan) return;
-    sc->kminP = sc->kstar == INT_MAX ? -1 : sc->kstar;
-    sc->needScan = 0;
-}
 
 void lb_find_new_active(const Box& b, const Consts& k, const PartLB* plb, DevScalars* sc, hipStream_t s) {
     launch<&k_fna_check, 1024, 1>(1, 1, s, b.g, b.solid, plb, k.nPart, sc);
     const int64_t nb = (b.g.N + 255) / 256;
-    const int grid = int(nb < 16384 ? nb : 16384);
+    const int grid = int(nb < FLAGS_GRID ? nb : FLAGS_GRID);  // grid-stride; most steps return early
     launch_gs<&k_fna_scan, 1024, 1>(grid, 256, s, b.g, b.type, b.solid, plb, k.nPart, sc);
     launch_gs<&k_fna_unflag, 1024, 1>(grid, 256, s, b.g, b.type, b.solid, sc);
-    launch<&k_fna_end, 1024, 1>(1, 1, s, sc);
+    // the scan's result (fna_finish) is taken by the next single-thread pass: k_begin_lb, or the
+    // next coupling step's k_flags_compute
 }
 
 // ------------------------------------------------------------------------------------------
