@@ -49,6 +49,12 @@ def _case_values(name: str) -> Tuple[Dict[str, object], np.ndarray, np.ndarray, 
         pc = np.array([[8.05, 11.25, 8.05], [8.05, 6.95, 8.05], [3.4, 2.5, 3.5], [12.6, 5.0, 12.2],
                        [12.3, 12.0, 3.8], [4.0, 9.0, 12.0]])
         return v, pr, pc, np.array([3.0]), np.array([[8.0, 16.55, 8.0]]), np.array([[0.0, -0.06, 0.0]])
+    if name == "startup":
+        # the suspension case with a DEM start-up repeat: while demTime < demInitialRepeat a step is
+        # a DEM step and a coupling step only (main.cpp:49-63)
+        v, pr, pc, orr, oc, ov = _case_values("suspension")
+        v.update(demInitialRepeat=6.0)
+        return v, pr, pc, orr, oc, ov
     if name == "periodic":
         # x and z periodic, floor and lid walls, spheres away from the periodic seams
         v.update(lbSizeX=16.0, lbSizeY=18.0, lbSizeZ=16.0, fluidHeight=12.0,
@@ -79,7 +85,7 @@ def _case_values(name: str) -> Tuple[Dict[str, object], np.ndarray, np.ndarray, 
     raise KeyError(name)
 
 
-CASES = ("fs_impactor", "suspension", "periodic", "units", "lb_closed", "moving")
+CASES = ("fs_impactor", "suspension", "periodic", "units", "lb_closed", "moving", "startup")
 
 # initial sphere velocities (x1, w1) of the cases that have them
 VELOCITIES = {

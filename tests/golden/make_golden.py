@@ -9,7 +9,8 @@ the total fluid mass. Also runs the reference's own main loop on the host-progra
 (cases.HOST_RUNS) and keeps the result files it writes (objPosition.dat, objForce.dat,
 wallForce.dat) under host_<case>/. Only data is stored.
 
-usage: python tests/golden/make_golden.py [--host-only]
+usage: python tests/golden/make_golden.py [--host-only] [--cases name,name,...]
+(--cases: only the named cases of the plan, merged into the existing manifest)
 """
 from __future__ import annotations
 
@@ -27,7 +28,7 @@ import cases  # noqa: E402
 
 DRIVER = os.path.join(cases.REPO, "oracle", "_ref", "ref_driver")
 PLAN = {"lb_closed": [1, 12], "fs_impactor": [1, 40], "suspension": [1, 40, 120], "periodic": [40], "units": [40],
-        "moving": [1, 30]}
+        "moving": [1, 30], "startup": [5, 30]}
 
 
 def host_outputs():
@@ -52,12 +53,24 @@ def host_outputs():
 def main():
     if not os.path.exists(DRIVER):
         sys.exit(f"{DRIVER} missing: run `make -C oracle ref` where the reference sources are present")
-    host_outputs()
+    only = None
+    if "--cases" in sys.argv:
+        only = sys.argv[sys.argv.index("--cases") + 1].split(",")
+        unknown = [n for n in only if n not in PLAN]
+        if unknown:
+            sys.exit(f"not in the plan: {unknown}")
+    else:
+        host_outputs()
     if "--host-only" in sys.argv:
         return
     manifest = {}
+    if only is not None:
+        with open(os.path.join(HERE, "manifest.json")) as fh:
+            manifest = json.load(fh)
     with tempfile.TemporaryDirectory() as tmp:
         for name, steps in PLAN.items():
+            if only is not None and name not in only:
+                continue
             d = cases.make_case(tmp, name)
             env = dict(os.environ, OMP_NUM_THREADS="1")
             r = subprocess.run([DRIVER, "config.cfg", "out", str(max(steps)), "dump"] + [str(s) for s in steps],
