@@ -1468,6 +1468,8 @@ __device__ inline void k_obj_uncover(Geom g, S* f, uint8_t* type,
     __shared__ int32_t hKey[UNC_HASH];          // chunk candidates: node index -> position
     __shared__ int16_t hPos[UNC_HASH];
     __shared__ int sPre[UNC_MAX_BLOCKS + 1];   // exclusive prefix of the slices' candidate counts
+    __shared__ uint8_t ownT[UNC_CHUNK];        // the candidates' own types (staged with the neighbours')
+    __shared__ int sChBase;                    // the chunk's first slot on the change list
     const int tid = threadIdx.x;
     const int n = sc->nObjList;
     const int per = (n + G - 1) / G;
@@ -1500,13 +1502,16 @@ __device__ inline void k_obj_uncover(Geom g, S* f, uint8_t* type,
             for (uint32_t h = unc_hash(key);; h = (h + 1) & (UNC_HASH - 1))
                 if (atomicCAS(&hKey[h], -1, key) == -1) { hPos[h] = int16_t(tid); break; }
         }
+        if (tid == 0) sChBase = atomicAdd(&sc->nChanged, cnt);  // every candidate goes on the change list
         if (tid < cnt) {
             const int32_t i = sCand[tid];
             int x, y, z;
             cell_xyz(g, i, x, y, z);
-            uint8_t nt[Q];  // the 18 loads are issued together, then written to LDS
+            uint8_t nt[Q];  // the 19 loads are issued together, then written to LDS
+            nt[0] = type[i];
 #pragma unroll
             for (int j = 1; j < Q; ++j) nt[j] = type[nbr_plain(g, x, y, z, j)] & T_BASE;
+            ownT[tid] = nt[0];
 #pragma unroll
             for (int j = 1; j < Q; ++j) nbT[j - 1][tid] = nt[j];
         }
@@ -1562,8 +1567,10 @@ __device__ inline void k_obj_uncover(Geom g, S* f, uint8_t* type,
         if (tid < cnt) {
             const int32_t i = sCand[tid];
             const uint8_t nt = newT[tid];
-            type[i] = uint8_t((type[i] & T_PMASK) | nt);
-            push_changed(changed, changedCap, sc, i);
+            type[i] = uint8_t((ownT[tid] & T_PMASK) | nt);  // no other thread writes a candidate's type
+            const int pos = sChBase + tid;
+            if (pos < changedCap) changed[pos] = i;
+            else atomicOr(&sc->fullMark, 1);
             if (nt != T_GAS) {
                 const bool isIf = nt == T_IF;
                 int x, y, z;
