@@ -64,7 +64,7 @@ struct lbdem_ctx {
     // fsStream beside that step's DEM step (the two touch disjoint state); the coupling step
     // waits for both
     hipStream_t fsStream = nullptr;
-    hipEvent_t evFork = nullptr, evJoin = nullptr;
+    hipEvent_t evFork = nullptr, evJoin = nullptr, evHydro = nullptr;
     bool fsDeferred = false;
     bool streamError = false;  // an event record or wait of the overlap failed
     Box box{};
@@ -134,6 +134,7 @@ struct lbdem_ctx {
         if (dTmp && !tmpPlanned) (void)hipFree(dTmp);
         if (evFork) (void)hipEventDestroy(evFork);
         if (evJoin) (void)hipEventDestroy(evJoin);
+        if (evHydro) (void)hipEventDestroy(evHydro);
         if (fsStream) (void)hipStreamDestroy(fsStream);
         if (stream) (void)hipStreamDestroy(stream);
     }
@@ -468,6 +469,7 @@ extern "C" int lbdem_init(lbdem_ctx* c) {
     HIPCHK(hipStreamCreateWithFlags(&c->fsStream, hipStreamNonBlocking));
     HIPCHK(hipEventCreateWithFlags(&c->evFork, hipEventDisableTiming));
     HIPCHK(hipEventCreateWithFlags(&c->evJoin, hipEventDisableTiming));
+    HIPCHK(hipEventCreateWithFlags(&c->evHydro, hipEventDisableTiming));
     if (const char* g = std::getenv("LBDEM_GRAPHS")) c->useGraphs = std::atoi(g) != 0;
     if (!c->tuned)
         for (auto& pr : c->tuneEv)
@@ -664,11 +666,13 @@ static int check_errors(lbdem_ctx* c) {
 }
 
 // --------------------------- steps ---------------------------
-static void enqueue_dem(lbdem_ctx* c) {  // DEM::discreteElementStep (DEM.cpp:359-401)
+// hydroReady: the previous LB step's hydrodynamic sums are finalised on another stream; the
+// integration (the first DEM launch that reads them) waits for it
+static void enqueue_dem(lbdem_ctx* c, hipEvent_t hydroReady = nullptr) {  // DEM::discreteElementStep (DEM.cpp:359-401)
     c->demTimeStep++;
     c->demTime += c->deltat;
     dem_neighbor_list(c->dp, c->ds, c->sc, false, c->stream);
-    dem_step(c->dp, c->ds, c->sc, int(c->demTimeStep), c->stream);
+    if (!dem_step(c->dp, c->ds, c->sc, int(c->demTimeStep), c->stream, hydroReady)) c->streamError = true;
 }
 static void enqueue_coupling(lbdem_ctx* c) {  // LB::latticeBoltzmannCouplingStep (LB.cpp:432-467)
     lb_flags_rebuild_compute(c->box, c->k, c->ds.plb, c->sc, c->stream);
@@ -696,7 +700,15 @@ static void finish_tuning(lbdem_ctx* c) {
     c->tuned = true;
 }
 
-static void enqueue_lb(lbdem_ctx* c) {  // LB::latticeBolzmannStep (LB.cpp:400-418)
+// the particle, impactor and wall sums of the fused kernel (and the counters of the free-surface
+// pass that follows)
+static void enqueue_hydro(lbdem_ctx* c, hipStream_t s) {
+    lb_hydro_finalize(c->box, c->k, c->ds.FHydro, c->ds.MHydro, c->ds.fluidVol, c->sc, c->ds.oFHydro, c->ds.oMHydro,
+                      c->ds.wFHydro, true, s);
+}
+// finalize = false: the hydrodynamic finalisation is left to the next step's fork (it goes with
+// the deferred free surface)
+static void enqueue_lb(lbdem_ctx* c, bool finalize = true) {  // LB::latticeBolzmannStep (LB.cpp:400-418)
     if (!c->forceField && !c->lbFReset) {  // LB.cpp:1303-1305
         c->k.lbF[0] = c->k.lbF[1] = c->k.lbF[2] = 0.0;
         c->lbFReset = true;
@@ -722,8 +734,7 @@ static void enqueue_lb(lbdem_ctx* c) {  // LB::latticeBolzmannStep (LB.cpp:400-4
     }
     lb_mark_edges(c->box, c->sc, c->stream);
     lb_fused_step(c->box, c->k, c->ds.plb, c->ds.olb, c->sc, true, c->stream, e0, e1);
-    lb_hydro_finalize(c->box, c->k, c->ds.FHydro, c->ds.MHydro, c->ds.fluidVol, c->sc, c->ds.oFHydro, c->ds.oMHydro,
-                      c->ds.wFHydro, true, c->stream);
+    if (finalize) enqueue_hydro(c, c->stream);
 }
 static void enqueue_fs(lbdem_ctx* c, hipStream_t s) { lb_free_surface(c->box, c->k, c->sc, s); }
 
@@ -752,36 +763,44 @@ static void enqueue_step(lbdem_ctx* c, bool deferFs = false) {
     ++c->currentTimeStep;
     ++c->lbTime;
     bool join = false;
-    if (c->fsDeferred) {  // the previous step's free surface (LB.cpp:422-430)
+    hipEvent_t hydroReady = nullptr;
+    if (c->fsDeferred) {  // the previous step's hydrodynamic sums and free surface (LB.cpp:422-430)
         c->fsDeferred = false;
         if (c->prm.demSolve) {
             if (hipEventRecord(c->evFork, c->stream) != hipSuccess ||
                 hipStreamWaitEvent(c->fsStream, c->evFork, 0) != hipSuccess)
                 c->streamError = true;
+            enqueue_hydro(c, c->fsStream);
+            if (hipEventRecord(c->evHydro, c->fsStream) != hipSuccess) c->streamError = true;
+            hydroReady = c->evHydro;
             enqueue_fs(c, c->fsStream);
             if (hipEventRecord(c->evJoin, c->fsStream) != hipSuccess) c->streamError = true;
             join = true;
         } else {
+            enqueue_hydro(c, c->stream);
             enqueue_fs(c, c->stream);
         }
     }
-    if (c->prm.demSolve) enqueue_dem(c);
+    if (c->prm.demSolve) enqueue_dem(c, hydroReady);
     if (join && hipStreamWaitEvent(c->stream, c->evJoin, 0) != hipSuccess) c->streamError = true;
     if (c->demTime >= c->prm.demInitialRepeat) {
         if (c->prm.demSolve) enqueue_coupling(c);
-        enqueue_lb(c);
+        const bool defer = deferFs && c->freeSurface;
+        enqueue_lb(c, !defer);
         if (c->freeSurface) {
-            if (deferFs) c->fsDeferred = true;
+            if (defer) c->fsDeferred = true;
             else enqueue_fs(c, c->stream);
         }
     } else if (c->demTime <= c->prm.demInitialRepeat) {
         if (c->prm.demSolve) enqueue_coupling(c);
     }
 }
-// the deferred free-surface pass, on the context's stream (every lbdem_cycle ends with it)
+// the deferred hydrodynamic finalisation and free-surface pass, on the context's stream (every
+// lbdem_cycle ends with them)
 static void flush_fs(lbdem_ctx* c) {
     if (!c->fsDeferred) return;
     c->fsDeferred = false;
+    enqueue_hydro(c, c->stream);
     enqueue_fs(c, c->stream);
 }
 // lbdem_cycle overlaps a step's free surface with the next DEM step

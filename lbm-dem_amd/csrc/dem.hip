@@ -808,7 +808,7 @@ void dem_neighbor_list(const DemParams& p, DemState& d, DevScalars* sc, bool for
     launch<&k_nbr_commit, 1024, 1>(pb, 128, s, a, d.xiCur, d.nbrCountOld, d.nbrListOld);
 }
 
-void dem_step(const DemParams& p, DemState& d, DevScalars* sc, int demTimeStep, hipStream_t s) {
+bool dem_step(const DemParams& p, DemState& d, DevScalars* sc, int demTimeStep, hipStream_t s, hipEvent_t hydroReady) {
     if (p.nPart > 0) {
         ForceArgs f;
         f.p = p; f.x0 = d.x0; f.x1 = d.x1; f.w1 = d.w1; f.r = d.r;
@@ -824,8 +824,11 @@ void dem_step(const DemParams& p, DemState& d, DevScalars* sc, int demTimeStep, 
     }
     const int fromParts = (p.nPart > 0 && (p.nObj > 0 || p.nWall > 0)) ? 1 : 0;  // else zero sums
     const int nb = (p.nPart + OBJ_THREADS - 1) / OBJ_THREADS;
+    // the integration reads the hydrodynamic sums (particles, impactors, walls)
+    const bool ok = !hydroReady || hipStreamWaitEvent(s, hydroReady, 0) == hipSuccess;
     launch<&k_dem_integrate, OBJ_THREADS, 1>(nb + 1, OBJ_THREADS, s, p, d, (const double*)d.redPart, fromParts,
                                              demTimeStep, nb);
+    return ok;
 }
 
 }  // namespace lbdem

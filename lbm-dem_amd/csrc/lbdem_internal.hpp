@@ -265,7 +265,10 @@ struct DemState {
     ObjLB* olb;
 };
 void dem_neighbor_list(const DemParams& p, DemState& d, DevScalars* sc, bool force, hipStream_t s);
-void dem_step(const DemParams& p, DemState& d, DevScalars* sc, int demTimeStep, hipStream_t s);
+// hydroReady (may be null): an event the integration waits for (the hydrodynamic sums it reads);
+// false if the wait could not be enqueued
+bool dem_step(const DemParams& p, DemState& d, DevScalars* sc, int demTimeStep, hipStream_t s,
+              hipEvent_t hydroReady = nullptr);
 void dem_refresh_views(const DemParams& p, DemState& d, hipStream_t s);
 
 // run pool hooks: a context carves its buffers from [base, base+bytes) instead of allocating
