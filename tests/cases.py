@@ -49,6 +49,12 @@ def _case_values(name: str) -> Tuple[Dict[str, object], np.ndarray, np.ndarray, 
         pc = np.array([[8.05, 11.25, 8.05], [8.05, 6.95, 8.05], [3.4, 2.5, 3.5], [12.6, 5.0, 12.2],
                        [12.3, 12.0, 3.8], [4.0, 9.0, 12.0]])
         return v, pr, pc, np.array([3.0]), np.array([[8.0, 16.55, 8.0]]), np.array([[0.0, -0.06, 0.0]])
+    if name == "two_impactors":
+        # closed box, free surface, two impactors entering the surface side by side; the reference
+        # uncovers every object node against the first object's sphere (LB.cpp:870)
+        v.update(lbSizeX=16.0, lbSizeY=20.0, lbSizeZ=14.0, fluidHeight=11.0, objWaitTime=0)
+        return (v, np.zeros(0), empty, np.array([2.5, 2.2]), np.array([[4.6, 14.4, 7.1], [11.3, 15.2, 6.8]]),
+                np.array([[0.0, -0.07, 0.0], [0.01, -0.05, 0.0]]))
     if name == "startup":
         # the suspension case with a DEM start-up repeat: while demTime < demInitialRepeat a step is
         # a DEM step and a coupling step only (main.cpp:49-63)
@@ -85,7 +91,7 @@ def _case_values(name: str) -> Tuple[Dict[str, object], np.ndarray, np.ndarray, 
     raise KeyError(name)
 
 
-CASES = ("fs_impactor", "suspension", "periodic", "units", "lb_closed", "moving", "startup")
+CASES = ("fs_impactor", "suspension", "periodic", "units", "lb_closed", "moving", "startup", "two_impactors")
 
 # initial sphere velocities (x1, w1) of the cases that have them
 VELOCITIES = {

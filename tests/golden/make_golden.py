@@ -28,7 +28,7 @@ import cases  # noqa: E402
 
 DRIVER = os.path.join(cases.REPO, "oracle", "_ref", "ref_driver")
 PLAN = {"lb_closed": [1, 12], "fs_impactor": [1, 40], "suspension": [1, 40, 120], "periodic": [40], "units": [40],
-        "moving": [1, 30], "startup": [5, 30]}
+        "moving": [1, 30], "startup": [5, 30], "two_impactors": [1, 40]}
 
 
 def host_outputs():

@@ -82,7 +82,7 @@ def test_fp64_long_window(oracle_lib, gpu_lib, case_root):
     run_pair(oracle_lib, gpu_lib, case_root, "suspension", lbdem.PREC_FP64, 300, every=25)
 
 
-@pytest.mark.parametrize("name", ["fs_impactor", "periodic", "moving", "startup"])
+@pytest.mark.parametrize("name", ["fs_impactor", "periodic", "moving", "startup", "two_impactors"])
 def test_graph_replays_match_oracle(oracle_lib, gpu_lib, case_root, name):
     """cycle(n) with n >= 2 replays a captured pair of steady-state steps (HIP graph): 10-step
     chunks with downloads and synchronisation between replays, every chunk against the oracle."""
@@ -134,7 +134,7 @@ def test_fp16_storage_within_tolerance(oracle_lib, gpu_lib, case_root):
 GOLD = os.path.join(os.path.dirname(os.path.abspath(__file__)), "golden")
 
 
-@pytest.mark.parametrize("name", ["suspension", "fs_impactor", "periodic", "units", "moving", "startup"])
+@pytest.mark.parametrize("name", ["suspension", "fs_impactor", "periodic", "units", "moving", "startup", "two_impactors"])
 def test_fp64_matches_reference_fixtures(gpu_lib, case_root, name):
     with open(os.path.join(GOLD, "manifest.json")) as fh:
         steps = sorted(m["step"] for m in json.load(fh).values() if m["case"] == name)
