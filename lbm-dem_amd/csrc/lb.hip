@@ -522,7 +522,13 @@ __device__ inline void k_fused(FusedArgs<S> a) {
     for (int zz = 0; zz < FZ; ++zz) {
         const int z = z0 + zz;
         const uint8_t tFar = type_at(zz + AH + 1);
+        // The wavefront issues its memory operations at raised priority, so that with three
+        // wavefronts per SIMD the next node's loads (and the current node's stores) leave ahead of
+        // the other wavefronts' arithmetic: measured 1 % faster (0.688 -> 0.679 ms per launch on
+        // the 256x320x256 impact box, FP32) than at the default priority throughout.
+        __builtin_amdgcn_s_setprio(2);
         load_bundle<S, PAR>(a, RowYZ(g, yw, z + AH), lx, ic + AH * SZ, tAhead, nb[AH]);
+        __builtin_amdgcn_s_setprio(0);
         C* gc = nb[0].g;
         const bool actc = nb[0].act;
         const uint8_t tc = nb[0].t;
@@ -536,7 +542,9 @@ __device__ inline void k_fused(FusedArgs<S> a) {
         for (int c = 0; c < 7; ++c) r.d[c] = 0;
         r.n = 1; r.sx = 0; r.sy = 0; r.sz = 0; r.usq = 0;
         if (actc) collide_cell<S>(a, sc_, tc, x, y, z, mc, gc, r);
+        __builtin_amdgcn_s_setprio(1);
         store_pop<S, PAR>(a, RowYZ(g, yw, z), lx, actc, gc);
+        __builtin_amdgcn_s_setprio(0);
 
         // boundary links of edge nodes (LB.cpp:1390-1528): the value reflected into the slot
         // this node loads next step
