@@ -44,15 +44,54 @@ def log(msg):
     print(msg, file=sys.stderr, flush=True)
 
 
-def run_gpu(vals, ps, obs, prec, steps, warmup):
+def run_gpu(vals, ps, obs, prec, steps, warmup, run_steps=5000, timed_from=1500, every=100):
+    """One headline run of ``run_steps`` coupled steps in precision ``prec``.
+
+    Untimed before the timed window: the ``warmup`` steps asked for, then step pairs until the
+    context's start-up is over (the fused kernel's tile tuner has picked its walk and the steady
+    step pair is captured as a HIP graph), then steps up to ``timed_from`` (the impactor meets the
+    surface between steps 1000 and 1500, so the window lies in the impact phase: splash
+    conversions, impactor uncover/cover). Timed: exactly ``steps`` steps, bracketed by device
+    synchronisation. The run then continues to ``run_steps``; the whole run (first step to last,
+    window included) is timed as well. The impactor depth of the reference's objPosition.dat
+    (fluidHeight + r - y, IO.cpp:1323) is sampled every ``every`` steps."""
     lib = lbdem.Library.load()
     sim = lbdem.Simulation(lib, lbdem.params_from_values(vals, prec), ps, obs)
     t0 = time.time()
     sim.init()
     sim.synchronize()
-    log(f"[bench] {PREC_NAME[prec]}: init {time.time() - t0:.2f} s")
-    sim.cycle(warmup)
+    name = PREC_NAME[prec]
+    log(f"[bench] {name}: init {time.time() - t0:.2f} s")
+    fh, r_imp = float(vals["fluidHeight"]), float(obs.r[0])
+    state = {"done": 0}
+    depth_steps, depth, active_samples = [], [], []
+
+    def advance(n):
+        # in chunks that end on multiples of ``every``, sampling depth and active nodes there
+        while n > 0:
+            k = min(n, every - state["done"] % every)
+            sim.cycle(k)
+            state["done"] += k
+            n -= k
+            if state["done"] % every == 0:
+                o = sim.objects()
+                depth_steps.append(state["done"])
+                depth.append(fh + r_imp - float(o[0, lbdem.O["X0"] + 1]))
+                active_samples.append(sim.info()["nActive"])
+
+    t_run0 = time.perf_counter()
+    advance(warmup)
+    if state["done"] < timed_from:
+        advance(timed_from - state["done"])
+    extra = 0
+    while True:
+        tu = sim.tuning()
+        if (tu["tuned"] and tu["pair_captured"]) or extra >= 64:
+            break
+        advance(2)
+        extra += 2
     sim.synchronize()
+    start_step = state["done"]
     a0 = sim.info()["nActive"]
     mass0 = sim.total_mass()
     sim.synchronize()
@@ -60,9 +99,14 @@ def run_gpu(vals, ps, obs, prec, steps, warmup):
     sim.cycle(steps)   # steady-state step pairs replay a captured HIP graph
     sim.synchronize()
     t1 = time.perf_counter()
+    state["done"] += steps
     inf = sim.info()
     a1 = inf["nActive"]
     mass = sim.total_mass()
+    advance(max(0, run_steps - state["done"]))
+    sim.synchronize()
+    t_run1 = time.perf_counter()
+    run_done = state["done"]
     # the fused collide-stream kernel, timed with HIP events on the library stream in a short
     # pass of launches outside the graph
     sim.profile(True)
@@ -71,22 +115,44 @@ def run_gpu(vals, ps, obs, prec, steps, warmup):
     fused_ms, launches = sim.profile_read()
     launches = max(launches, 1)
     sim.profile(False)
-    obj = sim.objects()
+    tuning = sim.tuning()
     sim.close()
     wall = t1 - t0
     active = 0.5 * (a0 + a1)
-    res = dict(prec=PREC_NAME[prec], steps=steps, seconds=wall, ms_per_step=1e3 * wall / steps,
+    run_wall = t_run1 - t_run0
+    res = dict(prec=name, steps=steps, seconds=wall, ms_per_step=1e3 * wall / steps,
                active_cells=active, mlups_active=active * steps / wall / 1e6,
                mlups_box=inf["totNodes"] * steps / wall / 1e6,
                particle_steps_per_s=len(ps.r) * steps / wall,
                fused_ms_avg=fused_ms / launches, fused_launches=launches,
-               total_mass=mass, total_mass_start=mass0,
-               impactor_y=float(obj[0, 1]) if len(obj) else None, builds=inf["neighborListBuilds"])
+               total_mass=mass, total_mass_start=mass0, builds=inf["neighborListBuilds"],
+               window_start_step=start_step, startup_extra_steps=extra, tile_z=tuning["tile"][2],
+               run_steps=run_done, run_seconds=run_wall, run_ms_per_step=1e3 * run_wall / run_done,
+               run_mlups_active=float(np.mean(active_samples)) * run_done / run_wall / 1e6 if active_samples else None,
+               depth_steps=depth_steps, depth=depth)
     b = BYTES_PER_CELL[prec]
     res["fused_achieved_gbs"] = active * b / (res["fused_ms_avg"] * 1e-3) / 1e9
-    log(f"[bench] {PREC_NAME[prec]}: {res['ms_per_step']:.3f} ms/step, fused collide-stream {res['fused_ms_avg']:.3f} ms "
-        f"= {res['fused_achieved_gbs']:.0f} GB/s, {res['mlups_active']:.0f} MLUPS active")
+    log(f"[bench] {name}: window steps {start_step}-{start_step + steps}: {res['ms_per_step']:.3f} ms/step, "
+        f"{res['mlups_active']:.0f} MLUPS active; fused collide-stream {res['fused_ms_avg']:.3f} ms = "
+        f"{res['fused_achieved_gbs']:.0f} GB/s; whole run of {run_done} steps {res['run_ms_per_step']:.3f} ms/step; "
+        f"final depth {depth[-1] if depth else float('nan'):.4f}")
     return res
+
+
+def depth_error(res, ref):
+    """Impactor depth of ``res`` against ``ref`` (the FP64 run): at the last common sample and at
+    the deepest point of the FP64 curve, as relative errors (depth changes sign when the impactor
+    meets the surface, so the error is taken where the FP64 depth is furthest from zero)."""
+    n = min(len(res["depth"]), len(ref["depth"]))
+    if n == 0:
+        return None
+    d, r = np.asarray(res["depth"][:n]), np.asarray(ref["depth"][:n])
+    k = int(np.argmax(np.abs(r)))
+    return {"final_step": int(ref["depth_steps"][n - 1]), "final_depth": round(float(d[-1]), 6),
+            "final_depth_fp64": round(float(r[-1]), 6),
+            "final_rel_err": float(abs(d[-1] - r[-1]) / max(abs(r[-1]), 1e-30)),
+            "max_depth_step": int(ref["depth_steps"][k]), "max_depth_rel_err": float(abs(d[k] - r[k]) / max(abs(r[k]), 1e-30)),
+            "max_abs_diff_over_run": float(np.abs(d - r).max())}
 
 
 def cpu_baseline(seconds_budget=25.0):
@@ -335,6 +401,9 @@ def main():
     ap.add_argument("--steps", type=int, default=200)
     ap.add_argument("--warmup", type=int, default=20)
     ap.add_argument("--precision", default="fp32", choices=["fp32", "fp64", "fp16"])
+    ap.add_argument("--run-steps", type=int, default=5000, help="length of each headline run (the config's 5000)")
+    ap.add_argument("--timed-from", type=int, default=1500,
+                    help="first step of the timed window (untimed steps before it; 1500 is in the impact phase)")
     ap.add_argument("--no-fp64", action="store_true", help="skip the FP64 comparison run")
     ap.add_argument("--no-cpu-baseline", action="store_true")
     ap.add_argument("--small", action="store_true", help="quarter-size box (for quick checks)")
@@ -368,12 +437,15 @@ def main():
         print(json.dumps({"precision_sweep": res, "workload": "headline impact run (256x256x320, 31000 spheres)"}),
               flush=True)
         return
-    main_res = run_gpu(vals, ps, obs, prec, args.steps, args.warmup)
+    rs = args.run_steps
+    main_res = run_gpu(vals, ps, obs, prec, args.steps, args.warmup, run_steps=rs, timed_from=min(args.timed_from, rs))
     fp64 = fp16 = None
     if not args.no_fp64 and prec != lbdem.PREC_FP64:
-        fp64 = run_gpu(vals, ps, obs, lbdem.PREC_FP64, max(20, args.steps // 4), args.warmup)
+        fp64 = run_gpu(vals, ps, obs, lbdem.PREC_FP64, args.steps, args.warmup, run_steps=rs,
+                       timed_from=min(args.timed_from, rs))
     if not args.no_fp64 and prec == lbdem.PREC_FP32:  # the FP16-storage variant, measured beside it
-        fp16 = run_gpu(vals, ps, obs, lbdem.PREC_FP16, max(20, args.steps // 4), args.warmup)
+        fp16 = run_gpu(vals, ps, obs, lbdem.PREC_FP16, args.steps, args.warmup, run_steps=rs,
+                       timed_from=min(args.timed_from, rs))
     bat = srv = None
     if not args.headline_only and args.batched > 0:
         bat = batched(prec, args.batched, args.batched_steps)
@@ -418,6 +490,19 @@ def main():
                              "speedup_vs_fp64_plain": round(fp16["mlups_active"] / fp64["mlups_active"], 3) if fp64 else None}
                             if fp16 else None,
             "fused_kernel_ms": round(main_res["fused_ms_avg"], 4),
+            "timed_window": {"first_step": main_res["window_start_step"], "steps": args.steps,
+                             "untimed_steps_before": main_res["window_start_step"],
+                             "startup_extra_steps": main_res["startup_extra_steps"],
+                             "note": "untimed: --warmup steps, steps to the impact phase, then step pairs until the "
+                                     "tile tuner is done and the step pair is captured"},
+            "tile_z": main_res["tile_z"],
+            "whole_run": {"steps": main_res["run_steps"], "seconds": round(main_res["run_seconds"], 3),
+                          "ms_per_step": round(main_res["run_ms_per_step"], 4),
+                          "mlups_active": round(main_res["run_mlups_active"], 1) if main_res["run_mlups_active"] else None,
+                          "fp64_ms_per_step": round(fp64["run_ms_per_step"], 4) if fp64 else None,
+                          "fp16_ms_per_step": round(fp16["run_ms_per_step"], 4) if fp16 else None},
+            "impactor_depth_vs_fp64": depth_error(main_res, fp64) if fp64 else None,
+            "fp16_impactor_depth_vs_fp64": depth_error(fp16, fp64) if (fp16 and fp64) else None,
             "neighbor_list_builds": main_res["builds"],
             "total_mass_drift_rel": (main_res["total_mass"] - main_res["total_mass_start"]) / main_res["total_mass_start"],
         },

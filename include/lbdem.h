@@ -138,7 +138,9 @@ int lbdem_get_fields(lbdem_ctx* ctx, double* f, double* mass, double* n);
 int lbdem_get_particles(lbdem_ctx* ctx, double* rec /* nParticles*LBDEM_P_REC */);
 int lbdem_get_objects(lbdem_ctx* ctx, double* rec /* nObjects*LBDEM_O_REC */);
 int lbdem_get_walls(lbdem_ctx* ctx, double* rec /* nWalls*LBDEM_W_REC */);
-/* Verlet list of every particle, partners sorted ascending: count[n], list[n*cap + k]. */
+/* Verlet list of every particle: count[n], list[n*cap + k] (DEM.cpp:473-652). The MI355X library
+ * lists partners in ascending order, the oracle in the reference's traversal order; a pair that
+ * the reference lists more than once (fewer than 3 link cells along z) appears as often. */
 int lbdem_get_neighbors(lbdem_ctx* ctx, int cap, int* count, int* list);
 /* Total fluid mass: sum of mass over active nodes not inside a particle (LB.cpp:388-395). */
 int lbdem_total_mass(lbdem_ctx* ctx, double* mass);
@@ -172,6 +174,8 @@ typedef struct lbdem_tuning {
     int nCandidates;
     int candidateZ[4];
     double candidateMs[4];       /* fused kernel time, both parities, per candidate */
+    int pairCaptured;            /* 1 when lbdem_cycle's next step pair replays a captured graph,
+                                    i.e. the context's start-up (tuning, capture) is over */
 } lbdem_tuning;
 int lbdem_set_tile(lbdem_ctx* ctx, int tileZ);
 int lbdem_get_tuning(lbdem_ctx* ctx, lbdem_tuning* tuning);

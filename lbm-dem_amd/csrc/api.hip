@@ -317,13 +317,15 @@ static int wall_count(const lbdem_ctx* c) {
     return n;
 }
 
-// DEM linked cells (DEM.cpp:515-554): cell edge 2*rmax+2 over the DEM box; direct sum (no
-// cells) when an axis has fewer than 3 cells
+// DEM linked cells (DEM.cpp:505-554): cell edge 2*rmax+2 over the DEM box; the direct sum (all
+// zero here) when x or y has fewer than 3 cells. z is not tested (DEM.cpp:515): with 1 or 2
+// z-cells the wrapped 27-cell walk visits a z-cell more than once and lists such pairs once per
+// visit, and with 0 z-cells no list is built at all; both are kept.
 static void dem_cells(const lbdem_ctx* c, double rmax, int nl[3]) {
     const double demSize[3] = {c->physSize[0], c->freeSurface ? c->fluidHeight : c->physSize[1], c->physSize[2]};
     const double cut = 2.0 * rmax + 2.0;
     for (int a = 0; a < 3; ++a) nl[a] = int(demSize[a] / cut);
-    if (std::min(nl[0], nl[1]) < 3 || nl[2] < 3) nl[0] = nl[1] = nl[2] = 0;
+    if (std::min(nl[0], nl[1]) < 3) nl[0] = nl[1] = nl[2] = 0;
 }
 
 static int make_plan(lbdem_ctx* c, Plan& p) {
@@ -961,6 +963,7 @@ extern "C" int lbdem_get_tuning(lbdem_ctx* c, lbdem_tuning* t) {
         t->candidateZ[q] = lbdem_ctx::CAND[q];
         t->candidateMs[q] = c->tuneMs[q];
     }
+    t->pairCaptured = graph_ready(c) && c->graphExec[graph_phase(c)] ? 1 : 0;
     return 0;
 }
 

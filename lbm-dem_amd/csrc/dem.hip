@@ -155,7 +155,9 @@ __device__ inline void k_nbr_build(NbrArgs a) {
     };
     if (a.p.nlx == 0) {
         for (int q = 0; q < a.p.nPart; ++q) consider(q);
-    } else {
+    } else if (a.p.nlz > 0) {
+        // with nlz < 3 the wrapped z offsets repeat a cell, and the partners in it are taken once
+        // per visit (DEM.cpp:585-627); with nlz == 0 the reference builds no list
         const int nlx = a.p.nlx, nly = a.p.nly, nlz = a.p.nlz;
         const int key = a.cellKey[p];
         const int lx = key / (nly * nlz), ly = (key / nlz) % nly, lz = key % nlz;
@@ -167,7 +169,7 @@ __device__ inline void k_nbr_build(NbrArgs a) {
                     for (int t = b; t < e; ++t) consider(a.cellItems[t]);
                 }
     }
-    // ascending partners (insertion sort, lists are short)
+    // ascending partners (insertion sort, lists are short; a pair listed twice stays twice)
     for (int i = 1; i < cnt; ++i) {
         const int v = lst[i];
         int j = i - 1;
@@ -538,7 +540,9 @@ __device__ inline void k_dem_forces(ForceArgs a, double dampCoeff) {
                     const double xin = xio.x * n1wx + xio.y * n1wy + xio.z * n1wz;
                     xio.x -= xin * n1wx; xio.y -= xin * n1wy; xio.z -= xin * n1wz;
                     const double kt = 0.2 * k;
-                    const double dct = P.kinVisc == 0.0 ? dampCoeff : 0.0;
+                    // dampCoeffTanw: declared without an initialiser and never assigned
+                    // (DEM.cpp:916); taken as zero
+                    const double dct = 0.0;
                     ftx = -kt * xio.x; fty = -kt * xio.y; ftz = -kt * xio.z;
                     const double fnn = sqrt(fnx * fnx + fny * fny + fnz * fnz);
                     const double ftt = sqrt(ftx * ftx + fty * fty + ftz * ftz);
@@ -797,7 +801,7 @@ void dem_neighbor_list(const DemParams& p, DemState& d, DevScalars* sc, bool for
     if (p.nPart <= 0) return;
     NbrArgs a = nbr_args(p, d, sc);
     const int pb = (p.nPart + 127) / 128;
-    if (p.nlx > 0) {
+    if (p.nlx > 0 && p.nlz > 0) {
         const int ncell = p.nlx * p.nly * p.nlz;
         launch<&k_cells_assign, 1024, 1>(pb, 128, s, a);
         launch<&k_cells_scan, 1024, 1>(1, 1024, s, a, ncell);

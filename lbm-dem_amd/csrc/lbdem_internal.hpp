@@ -226,7 +226,7 @@ struct DemParams {
     double rmax;
     int objWaitTime;
     int periodic[3];           // both faces periodic along axis
-    int nlx, nly, nlz;         // cell grid (0 when the direct sum is used)
+    int nlx, nly, nlz;         // cell grid: all 0 for the direct sum; else nlx, nly >= 3, nlz >= 0
     int nPart, nObj, nWall;
     double wallP[MAX_WALLS][3], wallN[MAX_WALLS][3];
     double unitLength, unitSpeed, unitAngVel;

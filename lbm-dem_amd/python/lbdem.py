@@ -106,12 +106,13 @@ class RunResult(C.Structure):
 
 class Tuning(C.Structure):
     _fields_ = [("tile", C.c_int * 3), ("tuned", C.c_int), ("nCandidates", C.c_int), ("candidateZ", C.c_int * 4),
-                ("candidateMs", C.c_double * 4)]
+                ("candidateMs", C.c_double * 4), ("pairCaptured", C.c_int)]
 
     def as_dict(self) -> Dict:
         n = self.nCandidates
         return dict(tile=list(self.tile), tuned=bool(self.tuned),
-                    candidates={int(self.candidateZ[i]): float(self.candidateMs[i]) for i in range(n)})
+                    candidates={int(self.candidateZ[i]): float(self.candidateMs[i]) for i in range(n)},
+                    pair_captured=bool(self.pairCaptured))
 
 
 class PoolStats(C.Structure):

@@ -11,8 +11,9 @@
 //
 // Pinned behaviour where the reference is undefined (each also used by the GPU library):
 //  * xPbcs/yPbcs/zPbcs loop index starts at 0 (DEM.cpp:2771, 2792, 2817 leave it uninitialised).
-//  * object-particle tangential damping dampCoeffTanw = dampCoeff if kinVisc==0 else 0
-//    (DEM.cpp:916 reads it uninitialised; rule taken from DEM.cpp:2015-2020).
+//  * object-particle tangential damping dampCoeffTanw = 0 (DEM.cpp:916 declares it without an
+//    initialiser and never assigns it; the reference program of oracle/Makefile is compiled with
+//    every such local zero-initialised, which gives the same value).
 //  * wall bounce-back force goes to walls[solidIndex] when solidIndex < nWalls, else walls[0]
 //    (LB.cpp:1425-1429 would index walls[] with a particle index).
 //  * object bounce-back force and torque are dropped when the object node's solidIndex is not an
@@ -426,13 +427,40 @@ struct lbdem_ctx {
     }
 
     // ================= coupling (LB.cpp:432-467, 683-711, 797-943, 1875-1899, 1980-2100) =================
+    // The reference tests every active node against every particle in particle order, so a node
+    // inside several spheres keeps the highest index. Here each particle visits only the nodes of
+    // its bounding box [floor(c - r), ceil(c + r)] that are on the active list, in the same
+    // particle order: a node outside the box has one coordinate difference of at least r, so
+    // its rounded squared distance is at least r*r and the strict test fails there anyway. The
+    // flags are therefore the reference's (the fixtures compare them bit for bit).
     void initializeParticleBoundaries() {  // LB.cpp:683-711
         for (int64_t it = 0; it < N; ++it) { p[it] = 0; solid[it] = 0; }
+        std::vector<uint8_t> onList(N, 0);
+        for (uint32_t index : activeNodes) onList[index] = 1;
+        const int64_t size[3] = {lbSize[0], lbSize[1], lbSize[2]};
         for (auto& P : particles) {
             const V3 c = P.x0 / unit.Length; const double rr = P.r / unit.Length;
             const unsigned idx = P.particleIndex;
-            for (uint32_t index : activeNodes)
-                if (getPosition(index).insideSphere(c, rr)) { p[index] = 1; solid[index] = uint16_t(idx); }
+            const double cc[3] = {c.x, c.y, c.z};
+            int64_t lo[3], hi[3];
+            bool empty = !(rr > 0.0);
+            for (int a = 0; a < 3 && !empty; ++a) {
+                if (!std::isfinite(cc[a] - rr) || !std::isfinite(cc[a] + rr)) { empty = true; break; }
+                const double l = std::floor(cc[a] - rr), h = std::ceil(cc[a] + rr);
+                if (h < 0.0 || l > double(size[a] - 1)) { empty = true; break; }
+                lo[a] = l < 0.0 ? 0 : int64_t(l);
+                hi[a] = h > double(size[a] - 1) ? size[a] - 1 : int64_t(h);
+            }
+            if (empty) continue;
+            for (int64_t z = lo[2]; z <= hi[2]; ++z)
+                for (int64_t y = lo[1]; y <= hi[1]; ++y)
+                    for (int64_t x = lo[0]; x <= hi[0]; ++x) {
+                        const int64_t index = x + y * size[0] + z * size[0] * size[1];
+                        if (onList[index] && getPosition(index).insideSphere(c, rr)) {
+                            p[index] = 1;
+                            solid[index] = uint16_t(idx);
+                        }
+                    }
         }
     }
     void findNewActiveObject() {  // LB.cpp:846-943
@@ -799,7 +827,13 @@ struct lbdem_ctx {
                     A.xiz[A.list[0]] = B.xiz[B.list[0]]; B.xiz[B.list[0]] = A.xiz[A.list[0]];
                 }
         } else {
+            // nlz may be 0, 1 or 2 here (only nlx and nly are tested, DEM.cpp:515). With 1 or 2
+            // z-cells the wrapped walk below visits a z-cell more than once and lists a pair
+            // once per visit, as the reference does; with 0 its loops never run and every list
+            // stays empty (the reference then files every particle under cell 0, which nothing
+            // reads).
             const int ncell = nlx * nly * nlz;
+            if (ncell == 0) return 0;  // every list was emptied above
             std::vector<std::vector<int>> cell(ncell);
             for (auto& P : particles)
                 if (!std::isfinite(P.x0.x) || !std::isfinite(P.x0.y) || !std::isfinite(P.x0.z)) {
@@ -1055,7 +1089,7 @@ struct lbdem_ctx {
                         const double xi1w = P.xio.x * n1wx + P.xio.y * n1wy + P.xio.z * n1wz;
                         P.xio.x -= xi1w * n1wx; P.xio.y -= xi1w * n1wy; P.xio.z -= xi1w * n1wz;
                         const double ktw = 0.2 * k;
-                        const double dct = prm.kinVisc == 0.0 ? dampCoeff : 0.0;  // pinned (see header)
+                        const double dct = 0.0;  // dampCoeffTanw, zero-initialised (see header)
                         ftx = -ktw * P.xio.x; fty = -ktw * P.xio.y; ftz = -ktw * P.xio.z;
                         const double fnnw = std::sqrt(fnx * fnx + fny * fny + fnz * fnz);
                         const double fttw = std::sqrt(ftx * ftx + fty * fty + ftz * ftz);

@@ -33,11 +33,53 @@ BASE: Dict[str, object] = {
 }
 
 
+GOLDEN = os.path.join(os.path.dirname(os.path.abspath(__file__)), "golden")
+
+# DEM parameters of the lattice-unit impact workloads (bench.py's): stiffer contacts, denser
+# spheres, lubrication with the fluid's kinematic viscosity
+IMPACT_DEM = {"density": 2.5, "linearStiff": 2.0, "frictionCoefPart": 0.3, "lubNormal": 1.0, "lubTangential": 0.5,
+              "lubRotation": 0.4, "lubCutoff": 0.2, "kinVisc": 0.1, "electroForce": 0.01, "debyeLength": 0.5,
+              "objWaitTime": 0, "lbFY": -1.0e-5}
+
+# Cases whose sphere centres come from a seeded packing (tests/golden/make_golden.py --inputs
+# writes them once into tests/golden/inputs_<case>.npy, so every machine reads the same values):
+# name -> (lattice size, fluid height, sphere radius, sphere count, packing seed, impactor r,
+# impactor centre, impactor velocity). The DEM link cells have edge 2r + 2 over the DEM box
+# (lbSizeX, fluidHeight, lbSizeZ); x and z are periodic.
+PACKED = {
+    # 6 x 6 x 6 link cells; four extra spheres above the surface, pushed apart by their overlap,
+    # cross the x and the z seam at step 25
+    "cells": ((64, 80, 64), 64.0, 4.0, 452, 7, 8.0, (32.0, 74.0, 32.0), (0.0, -0.01, 0.0)),
+    # 6 x 4 link cells in x, y and 2, 1, 0 in z: the reference tests only x and y (DEM.cpp:515)
+    "thin_z2": ((40, 30, 12), 24.0, 2.0, 50, 11, 3.0, (20.0, 27.0, 6.0), (0.0, -0.05, 0.0)),
+    "thin_z1": ((40, 30, 8), 24.0, 2.0, 34, 12, 3.0, (20.0, 27.0, 4.0), (0.0, -0.05, 0.0)),
+    "thin_z0": ((40, 30, 5), 24.0, 2.0, 20, 13, 2.0, (20.0, 27.0, 2.5), (0.0, -0.05, 0.0)),
+}
+# spheres added to the "cells" packing (centres as in particles.dat, before the reader's -0.05):
+# two overlapping pairs in the gas above the surface, each pushing one sphere across a periodic
+# seam (a sphere that crosses a seam inside the fluid keeps, until the next re-flag, particle
+# nodes a box length away, and the reference's interior forcing then diverges)
+SEAM_PAIRS = np.array([[63.8, 72.05, 12.05], [56.8, 72.05, 12.05], [12.05, 72.05, 0.3], [12.05, 72.05, 7.3]])
+
+
+def packed_values(name: str):
+    size, fh, r, n, seed, r_imp, c_imp, v_imp = PACKED[name]
+    v = dict(BASE)
+    v.update(IMPACT_DEM)
+    v.update(lbSizeX=float(size[0]), lbSizeY=float(size[1]), lbSizeZ=float(size[2]), fluidHeight=fh,
+             boundary0=4, boundary1=4, boundary4=4, boundary5=4)
+    return v, size, fh, r, n, seed, r_imp, c_imp, v_imp
+
+
 def _case_values(name: str) -> Tuple[Dict[str, object], np.ndarray, np.ndarray, np.ndarray, np.ndarray, np.ndarray]:
     """Returns (config values, particle radii, particle centres as written to particles.dat,
     object radii, object centres, object velocities)."""
     v = dict(BASE)
     empty = np.zeros((0, 3))
+    if name in PACKED:
+        v, size, fh, r, n, seed, r_imp, c_imp, v_imp = packed_values(name)
+        pc = np.load(os.path.join(GOLDEN, f"inputs_{name}.npy"), allow_pickle=False)
+        return v, np.full(len(pc), r), pc, np.array([r_imp]), np.array([c_imp]), np.array([v_imp])
     if name == "fs_impactor":
         # closed box, free surface, impactor entering the surface; no suspended spheres
         v.update(lbSizeX=14.0, lbSizeY=20.0, lbSizeZ=14.0, fluidHeight=11.0, objWaitTime=0)
@@ -92,6 +134,8 @@ def _case_values(name: str) -> Tuple[Dict[str, object], np.ndarray, np.ndarray, 
 
 
 CASES = ("fs_impactor", "suspension", "periodic", "units", "lb_closed", "moving", "startup", "two_impactors")
+# cases that take the DEM link-cell branch of the neighbour list (DEM.cpp:556-631)
+CELL_CASES = ("cells", "thin_z2", "thin_z1", "thin_z0")
 
 # initial sphere velocities (x1, w1) of the cases that have them
 VELOCITIES = {
@@ -173,3 +217,62 @@ def state(sim: "lbdem.Simulation", with_f: bool = True) -> Dict[str, object]:
     cnt, lst = sim.neighbors()
     return dict(type=t, pflag=p, solid=s, f=f, mass=m, n=n, particles=sim.particles(), objects=sim.objects(),
                 walls=sim.walls(), ncount=cnt, nlist=lst, totalMass=sim.total_mass(), info=sim.info())
+
+
+def _n_int(x):
+    return np.where(x >= 0, np.trunc(x + 0.5), np.trunc(x - 0.5))
+
+
+def _s_int(x):
+    return np.where(x >= 0, np.trunc(x), np.trunc(x - 1.0))
+
+
+def brute_neighbours(x0: np.ndarray, r: np.ndarray, dem_size) -> list:
+    """Partner multisets of the reference's Verlet list (DEM.cpp:473-631) by brute force over all
+    pairs, independent of any cell walk: a pair is listed when its minimum-image distance in every
+    axis of the DEM box is within r1 + r2 + 2 (DEM.cpp:521-531, 601-609). With fewer than 3 link
+    cells along x or y every pair is visited once (the direct sum). Otherwise the pair is listed once
+    per offset d in {-1, 0, 1}^3 for which the wrapped cell of one sphere plus d is the other's cell:
+    per axis with n cells, the count of d with (l1 + d) mod n == l2, which is 1 for neighbouring cells
+    when n >= 3, 1 or 2 when n == 2, 3 when n == 1, and the product of the three; no pair at all when
+    n == 0 along z (the walk has no cells). Returns sorted partner lists."""
+    x0 = np.asarray(x0, dtype=np.float64)
+    r = np.asarray(r, dtype=np.float64)
+    n = len(r)
+    L = np.asarray(dem_size, dtype=np.float64)
+    rmax = float(r.max()) if n else 0.0
+    cut = 2.0 * rmax + 2.0
+    nl = [int(L[a] / cut) for a in range(3)]
+    direct = min(nl[0], nl[1]) < 3
+    cell = None
+    if not direct:
+        u = x0 / L
+        b = u - _s_int(u / 1.0) * 1.0
+        cell = np.stack([(b[:, a] * nl[a]).astype(np.int64) if nl[a] > 0 else np.zeros(n, np.int64)
+                         for a in range(3)], axis=1)
+    out = [[] for _ in range(n)]
+    for a in range(1, n):
+        d = x0[a] - x0[:a]
+        d = d - _n_int(d / L) * L
+        r12 = d[:, 0] * d[:, 0] + d[:, 1] * d[:, 1] + d[:, 2] * d[:, 2]
+        c = r[a] + r[:a] + 2.0
+        for b in np.nonzero(r12 <= c * c)[0]:
+            mult = 1
+            if not direct:
+                for ax in range(3):
+                    k = nl[ax]
+                    if k == 0:
+                        mult = 0
+                        break
+                    mult *= sum(1 for dd in (-1, 0, 1) if (cell[a, ax] + dd) % k == cell[b, ax])
+            for _ in range(mult):
+                out[a].append(int(b))
+                out[int(b)].append(a)
+    return [sorted(p) for p in out]
+
+
+def dem_size(values: Dict[str, object]):
+    """DEM box (DEM.cpp:201-214): lbSize in physical units, y up to fluidHeight with a free surface."""
+    fs = int(values["freeSurfaceSolve"]) != 0
+    return (float(values["lbSizeX"]), float(values["fluidHeight"]) if fs else float(values["lbSizeY"]),
+            float(values["lbSizeZ"]))

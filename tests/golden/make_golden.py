@@ -9,7 +9,15 @@ the total fluid mass. Also runs the reference's own main loop on the host-progra
 (cases.HOST_RUNS) and keeps the result files it writes (objPosition.dat, objForce.dat,
 wallForce.dat) under host_<case>/. Only data is stored.
 
-usage: python tests/golden/make_golden.py [--host-only] [--cases name,name,...]
+Cases larger than SAMPLE_ABOVE nodes keep the populations, density and mass of a seeded sample
+of active nodes (``sample`` holds their indices) instead of every active node; everything else
+is stored whole.
+
+``--inputs`` writes the sphere centres of the packed cases (cases.PACKED) to inputs_<case>.npy
+from the seeded packing tool (lbm-dem_amd/lib/liblbdem_tools.so); they are committed, so the
+cases do not depend on the tool.
+
+usage: python tests/golden/make_golden.py [--inputs] [--host-only] [--cases name,name,...]
 (--cases: only the named cases of the plan, merged into the existing manifest)
 """
 from __future__ import annotations
@@ -28,7 +36,32 @@ import cases  # noqa: E402
 
 DRIVER = os.path.join(cases.REPO, "oracle", "_ref", "ref_driver")
 PLAN = {"lb_closed": [1, 12], "fs_impactor": [1, 40], "suspension": [1, 40, 120], "periodic": [40], "units": [40],
-        "moving": [1, 30], "startup": [5, 30], "two_impactors": [1, 40]}
+        "moving": [1, 30], "startup": [5, 30], "two_impactors": [1, 40],
+        "cells": [1, 26, 60], "thin_z2": [1, 40], "thin_z1": [40], "thin_z0": [20]}
+SAMPLE_ABOVE = 8_000   # nodes
+SAMPLE_NODES = 1536
+
+
+def make_inputs():
+    """Sphere centres (particles.dat columns, i.e. before the reader's -0.05) of the packed cases."""
+    sys.path.insert(0, os.path.join(cases.REPO, "lbm-dem_amd", "python"))
+    import workloads
+    for name in cases.PACKED:
+        v, size, fh, r, n, seed, *_ = cases.packed_values(name)
+        X, Y, Z = size
+        if name == "cells":
+            x0 = workloads.pack(n, r, (1.0, 1.0, 1.0), (X - 1.0, fh, Z - 1.0), seed, relax=300)
+            # move the packing by 30 lattice units along x and z (periodic) so spheres sit on both
+            # sides of the seams, then add the seam-crossing pairs
+            x0[:, 0] = np.mod(x0[:, 0] + 30.0, float(X))
+            x0[:, 2] = np.mod(x0[:, 2] + 30.0, float(Z))
+            pc = np.concatenate([x0 + 0.05, cases.SEAM_PAIRS])
+        else:
+            x0 = workloads.pack(n, r, (1.0, 1.0, 0.5), (X - 1.0, fh - 1.0, Z - 0.5), seed, relax=200)
+            pc = x0 + 0.05
+        out = os.path.join(HERE, f"inputs_{name}.npy")
+        np.save(out, pc)
+        print("inputs of", name, pc.shape, "->", out)
 
 
 def host_outputs():
@@ -53,6 +86,9 @@ def host_outputs():
 def main():
     if not os.path.exists(DRIVER):
         sys.exit(f"{DRIVER} missing: run `make -C oracle ref` where the reference sources are present")
+    if "--inputs" in sys.argv:
+        make_inputs()
+        return
     only = None
     if "--cases" in sys.argv:
         only = sys.argv[sys.argv.index("--cases") + 1].split(",")
@@ -81,9 +117,14 @@ def main():
                 dump = cases.read_dump(os.path.join(d, f"dump_{s}.bin"))
                 act = (dump["type"] == 0) | (dump["type"] == 3)
                 out = os.path.join(HERE, f"{name}_s{s}.npz")
+                sel = np.nonzero(act)[0]
+                if dump["N"] > SAMPLE_ABOVE:
+                    rng = np.random.default_rng(1000 + s)
+                    sel = np.sort(rng.choice(sel, size=min(SAMPLE_NODES, sel.size), replace=False))
                 np.savez_compressed(
-                    out, type=dump["type"], pflag=dump["pflag"], solid=dump["solid"], active=np.nonzero(act)[0],
-                    f_active=dump["f"][:, act], n_active=dump["n"][act], mass_active=dump["mass"][act],
+                    out, type=dump["type"], pflag=dump["pflag"], solid=dump["solid"], active=sel,
+                    sampled=np.array(dump["N"] > SAMPLE_ABOVE),
+                    f_active=dump["f"][:, sel], n_active=dump["n"][sel], mass_active=dump["mass"][sel],
                     particles=dump["particles"], objects=dump["objects"], walls=dump["walls"],
                     ncount=dump["ncount"], nlist=dump["nlist"], totalMass=dump["totalMass"],
                     totalMassInit=dump["totalMassInit"], size=np.array(dump["size"]))

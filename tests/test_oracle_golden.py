@@ -65,3 +65,26 @@ def test_fixtures_exercise_every_event(oracle_lib, case_root):
         ev["wallc"] += int((np.abs(w[:, lbdem.W["FCONTACT"]:lbdem.W["FCONTACT"] + 3]).sum(1) > 0).sum())
         ev["conv"] += int(inf["lastSurplus"] != 0.0)
     assert all(v > 0 for v in ev.values()), ev
+
+
+def test_cell_fixtures_cross_the_seams():
+    """The "cells" fixtures cover what the link-cell branch must get right: 6 cells per axis, a
+    sphere that crossed the x seam and one that crossed the z seam between steps 1 and 26 (the
+    DEM periodic wrap, DEM.cpp:2769-2836), and listed pairs whose raw separation spans the box (the
+    wrapped walk, DEM.cpp:585-603); the thin-z fixtures hold repeated partners (2 and 1 z-cells)
+    and empty lists (0 z-cells)."""
+    import lbdem
+    s1 = np.load(os.path.join(GOLD, "cells_s1.npz"))
+    s26 = np.load(os.path.join(GOLD, "cells_s26.npz"))
+    x1, x26 = s1["particles"][:, 0:3], s26["particles"][:, 0:3]
+    assert x1[452, 0] > 63.0 and x26[452, 0] < 1.0          # across x = 64 -> 0
+    assert x1[454, 2] < 1.0 and x26[454, 2] > 63.0          # across z = 0 -> 64
+    assert np.all(s1["particles"][:, lbdem.P["R"]] == 4.0)   # cut 10 over a 64^3 DEM box: 6 cells
+    cnt, lst = s26["ncount"], s26["nlist"]
+    span = [abs(x26[a] - x26[b]).max() for a in range(len(cnt)) for b in lst[a, :cnt[a]]]
+    assert max(span) > 32.0
+    for name, dup in (("thin_z2_s40", True), ("thin_z1_s40", True)):
+        g = np.load(os.path.join(GOLD, f"{name}.npz"))
+        rows = [g["nlist"][a, :g["ncount"][a]] for a in range(len(g["ncount"]))]
+        assert any(len(r) != len(set(r.tolist())) for r in rows) == dup, name
+    assert np.all(np.load(os.path.join(GOLD, "thin_z0_s20.npz"))["ncount"] == 0)

@@ -2,6 +2,7 @@
 D3Q19 equilibrium moments, exact mass conservation of BGK + Guo forcing with half-way
 bounce-back in a closed box, a flat free surface at rest, and the symmetric Verlet list."""
 import numpy as np
+import pytest
 
 import cases
 import lbdem
@@ -93,3 +94,38 @@ def test_verlet_list_is_symmetric_and_complete(oracle_lib, case_root):
             inside = (dx @ dx) <= (r[a] + r[b] + 2.0) ** 2
             assert (b in partners) == inside
             assert (a in set(lst[b, :cnt[b]].tolist())) == (b in partners)
+
+
+
+@pytest.mark.parametrize("name", cases.CELL_CASES)
+def test_cell_branch_lists_equal_brute_force(oracle_lib, case_root, name):
+    """The oracle's link-cell neighbour list (DEM.cpp:556-631) against a brute-force pair search
+    with the multiplicity of the wrapped 27-cell walk stated in closed form
+    (cases.brute_neighbours): equal partner multisets at initialisation and after every rebuild of
+    a 30-step window (for "cells" the window has spheres on both sides of both periodic seams, and
+    the four spheres above the surface are listed with bed spheres through the y image of the DEM
+    box)."""
+    vals, *_ = cases._case_values(name)
+    d = cases.make_case(case_root, name)
+    sim = cases.open_sim(oracle_lib, d)
+    L = cases.dem_size(vals)
+    R = sim.particles()[:, lbdem.P["R"]]
+    # a rebuild happens inside a step's DEM pass, from the positions the previous step left
+    x_before = sim.particles()[:, 0:3]
+    builds = sim.info()["neighborListBuilds"]
+    checked = 0
+    for step in range(31):
+        if step:
+            sim.cycle(1)
+            now = sim.info()["neighborListBuilds"]
+            if now == builds:
+                x_before = sim.particles()[:, 0:3]
+                continue
+            builds = now
+        expect = cases.brute_neighbours(x_before, R, L)
+        cnt, lst = sim.neighbors()
+        for a in range(len(cnt)):
+            assert sorted(lst[a, :cnt[a]].tolist()) == expect[a], f"{name} step {step}: partners of {a}"
+        checked += 1
+        x_before = sim.particles()[:, 0:3]
+    assert checked >= 2, checked
