@@ -88,8 +88,10 @@ def run_gpu(vals, ps, obs, prec, steps, warmup, run_steps=5000, timed_from=1500,
         tu = sim.tuning()
         if (tu["tuned"] and tu["pair_captured"]) or extra >= 64:
             break
-        advance(2)
-        extra += 2
+        advance(4)   # a call runs its first step alone, then captures or replays step pairs
+        extra += 4
+    if not (tu["tuned"] and tu["pair_captured"]):
+        log(f"[bench] {name}: start-up not finished after {extra} extra steps: {tu}")
     sim.synchronize()
     start_step = state["done"]
     a0 = sim.info()["nActive"]

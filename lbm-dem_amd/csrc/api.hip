@@ -815,11 +815,15 @@ static bool overlap_fs(const lbdem_ctx* c) { return c->freeSurface && c->prm.dem
 //  * the one-off reset of the lattice force (LB.cpp:1303-1305) is done;
 // the AA parity and the tangential-history buffers alternate with period two. Every
 // data-dependent branch is a device flag, so the pair can be captured once and replayed.
-static bool graph_ready(const lbdem_ctx* c) {
+static bool steady_pairs(const lbdem_ctx* c) {
     if (!c->useGraphs || c->graphFailed || c->profiling || !c->tuned) return false;
     if (c->prm.demSolve && c->demTimeStep < unsigned(std::max(c->prm.objWaitTime, 0)) + 1u) return false;
     if (!(c->demTime >= c->prm.demInitialRepeat)) return false;
     if (!c->forceField && !c->lbFReset) return false;
+    return true;
+}
+static bool graph_ready(const lbdem_ctx* c) {
+    if (!steady_pairs(c)) return false;
     if (c->fsDeferred != overlap_fs(c)) return false;  // a pair starts with the previous free surface
     return true;
 }
@@ -963,7 +967,12 @@ extern "C" int lbdem_get_tuning(lbdem_ctx* c, lbdem_tuning* t) {
         t->candidateZ[q] = lbdem_ctx::CAND[q];
         t->candidateMs[q] = c->tuneMs[q];
     }
-    t->pairCaptured = graph_ready(c) && c->graphExec[graph_phase(c)] ? 1 : 0;
+    // Between calls the deferred free-surface pass has been flushed, so an lbdem_cycle call takes
+    // its first step alone and replays pairs from the phase one step later (AA parity and, with
+    // particles, the history buffer flip every step).
+    int next = graph_phase(c);
+    if (!c->fsDeferred && overlap_fs(c)) next ^= 2 | ((c->prm.demSolve && !c->pr.empty()) ? 1 : 0);
+    t->pairCaptured = steady_pairs(c) && c->graphExec[next] ? 1 : 0;
     return 0;
 }
 

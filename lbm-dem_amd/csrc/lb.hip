@@ -140,9 +140,9 @@ constexpr int FX = FUSED_TILE_X, FY = FUSED_TILE_Y;  // tile: 64x4 threads, each
 constexpr int HS = 64;                  // per-block particle hash slots
 constexpr int PART_STRIDE = PART_STRIDE_;  // per-block partials: obj (7*MAX_OBJ_PART) + walls 18
 constexpr int MAX_OBJ_PART = 2;
-// per-thread partial sums of the fused kernel: object 0 force and torque, the bounce-back mass
-// term, and the force on the thread's first wall
-constexpr int ACC_OBJ = 0, ACC_BB = 6, ACC_WALL = 7, NACC = 10;
+// block partials of the fused kernel (sPart, doubles): object o's force and torque at 7 o, the
+// bounce-back mass term of all object links at 6, wall w's force at 7 MAX_OBJ_PART + 3 w
+constexpr int PART_BB = 6, PART_WALL = 7 * MAX_OBJ_PART;
 
 template <typename S>
 struct FusedArgs {
@@ -173,8 +173,7 @@ struct FusedArgs {
 template <typename S>
 __device__ inline typename Types<S>::C object_value(const FusedArgs<S>& a, int j, int32_t yj, int oi,
                                                    typename Types<S>::C gj, typename Types<S>::C n, double mi,
-                                                   double* sPart, double (*sAcc)[FX * FY], int tid,
-                                                   const ObjLB* sObj) {
+                                                   double* sPart, const ObjLB* sObj) {
     using C = typename Types<S>::C;
     const int nObj = a.k.nObj;
     int yx, yy, yz;
@@ -191,16 +190,11 @@ __device__ inline typename Types<S>::C object_value(const FusedArgs<S>& a, int j
         const double Fb[3] = {fm * cx_rt(j), fm * cy_rt(j), fm * cz_rt(j)};
         const double rr[3] = {yx - sObj[oi].c[0], yy - sObj[oi].c[1], yz - sObj[oi].c[2]};
         const double M[3] = {rr[1] * Fb[2] - rr[2] * Fb[1], rr[2] * Fb[0] - rr[0] * Fb[2], rr[0] * Fb[1] - rr[1] * Fb[0]};
-        if (oi == 0) {  // the thread's own accumulators
+        double* sp = &sPart[7 * oi];
 #pragma unroll
-            for (int c = 0; c < 3; ++c) { sAcc[ACC_OBJ + c][tid] += Fb[c]; sAcc[ACC_OBJ + 3 + c][tid] += M[c]; }
-        } else {
-            double* sp = &sPart[7 * oi];
-#pragma unroll
-            for (int c = 0; c < 3; ++c) { atomicAdd(sp + c, Fb[c]); atomicAdd(sp + 3 + c, M[c]); }
-        }
+        for (int c = 0; c < 3; ++c) { atomicAdd(sp + c, Fb[c]); atomicAdd(sp + 3 + c, M[c]); }
     }
-    sAcc[ACC_BB][tid] += double(BBi) * mi;
+    atomicAdd(&sPart[PART_BB], double(BBi) * mi);
     return gj - BBi;
 }
 
@@ -211,8 +205,7 @@ __device__ inline typename Types<S>::C boundary_value(const FusedArgs<S>& a, int
                                                      typename Types<S>::C gj, typename Types<S>::C n,
                                                      typename Types<S>::C sx, typename Types<S>::C sy,
                                                      typename Types<S>::C sz, typename Types<S>::C usq, double mi,
-                                                     double* sPart, double (*sAcc)[FX * FY], int tid,
-                                                     const ObjLB* sObj) {
+                                                     double* sPart, const ObjLB* sObj) {
     using C = typename Types<S>::C;
     const Consts& k = a.k;
     if (ty == T_GAS) {
@@ -224,13 +217,13 @@ __device__ inline typename Types<S>::C boundary_value(const FusedArgs<S>& a, int
             int wi = a.solid[yj];
             wi = (wi >= 0 && wi < k.nWall) ? wi : 0;
             const double v2 = 2.0 * double(gj);
-            if (cx_rt(j)) atomicAdd(&sPart[7 * MAX_OBJ_PART + 3 * wi + 0], v2 * cx_rt(j));
-            if (cy_rt(j)) atomicAdd(&sPart[7 * MAX_OBJ_PART + 3 * wi + 1], v2 * cy_rt(j));
-            if (cz_rt(j)) atomicAdd(&sPart[7 * MAX_OBJ_PART + 3 * wi + 2], v2 * cz_rt(j));
+            if (cx_rt(j)) atomicAdd(&sPart[PART_WALL + 3 * wi + 0], v2 * cx_rt(j));
+            if (cy_rt(j)) atomicAdd(&sPart[PART_WALL + 3 * wi + 1], v2 * cy_rt(j));
+            if (cz_rt(j)) atomicAdd(&sPart[PART_WALL + 3 * wi + 2], v2 * cz_rt(j));
         }
         return gj;
     }
-    if (ty == T_OBJ) return object_value<S>(a, j, yj, a.solid[yj], gj, n, mi, sPart, sAcc, tid, sObj);
+    if (ty == T_OBJ) return object_value<S>(a, j, yj, a.solid[yj], gj, n, mi, sPart, sObj);
     atomicOr(&a.sc->typeError, 1);
     return gj;
 }
@@ -428,18 +421,17 @@ __device__ inline uint64_t link_classes(const Geom& g, const uint8_t* type, cons
 // so LDS (about 52 KB per block) sets residency at three blocks per CU. Measured on the
 // 256x320x256 impact run (FP32): three blocks resident 0.69 ms per launch; LDS padded to allow
 // only two 0.80 ms.
-template <typename S> struct FusedOcc { static constexpr int value = 2; };
+template <typename S> struct FusedOcc { static constexpr int value = sizeof(S) == 8 ? 2 : 4; };
 // nodes whose bundle is loaded ahead of the one being collided. Measured as above: none 0.71 ms,
 // one 0.69 ms, two 0.78 ms (more loads in flight per CU slow the memory system down).
 template <typename S> struct FusedAhead { static constexpr int value = 1; };
 
 // per-node bundle: type byte, populations, mass (particle or edge node), solid index (particle
-// node), boundary-link classes (edge node)
+// node); the boundary-link classes of edge nodes (rare) are loaded where they are used
 template <typename S>
 struct Bundle {
     typename Types<S>::C g[Q];
-    double m;
-    uint64_t b;
+    typename Types<S>::M m;
     int32_t s;
     uint8_t t;
     bool act;
@@ -447,12 +439,12 @@ struct Bundle {
 template <typename S, int PAR>
 __device__ inline void load_bundle(const FusedArgs<S>& a, const RowYZ& r, const LaneX& lx, int32_t i, uint8_t t,
                                    Bundle<S>& n) {
+    using M = typename Types<S>::M;
     n.t = t;
     n.act = is_active(t);
     load_pop<S, PAR>(a, r, lx, n.act, n.g);
-    n.m = (n.act && (t & (T_PMASK | T_EDGE))) ? double(a.mass[i]) : 0.0;
+    n.m = (n.act && (t & (T_PMASK | T_EDGE))) ? a.mass[i] : M(0);
     n.s = (n.act && (t & T_PMASK)) ? a.solid[i] : 0;
-    n.b = (n.act && (t & T_EDGE)) ? a.bcm[i] : 0;
 }
 
 template <typename S, int PAR, bool FSURF, bool DIAG, int FZ>
@@ -463,13 +455,9 @@ __device__ inline void k_fused(FusedArgs<S> a) {
     if (int(blockIdx.x) >= a.gridBlocks) return;
     __shared__ int hkey[HS];
     __shared__ double hval[HS][7];
-    __shared__ double sPart[PART_STRIDE];
-    __shared__ C sg[Q - 1][NT];           // post-collision values of edge nodes (boundary loop)
+    __shared__ double sPart[PART_STRIDE]; // object / wall partials of the block (FP64 LDS atomics)
     __shared__ int32_t sIf[NT * FZ];      // interface nodes of the block
     __shared__ int sIfCount, sIfBase;
-    __shared__ double sAcc[NACC][NT];     // per-thread object / wall partials (no atomics)
-    __shared__ int8_t sWtag[NT];          // wall whose force the thread accumulates (-1: none yet)
-    __shared__ double sRed[7 + 3 * MAX_WALLS][NT / 64];
     __shared__ ObjLB sObj[MAX_OBJ_PART];  // the impactors' centres, radii and velocities
     const int tid = threadIdx.x + FX * threadIdx.y;
     if (tid < MAX_OBJ_PART * 8 && tid / 8 < a.k.nObj)
@@ -479,9 +467,6 @@ __device__ inline void k_fused(FusedArgs<S> a) {
         for (int c = 0; c < 7; ++c) hval[s][c] = 0.0;
     }
     if (tid < PART_STRIDE) sPart[tid] = 0.0;
-#pragma unroll
-    for (int c = 0; c < NACC; ++c) sAcc[c][tid] = 0.0;
-    sWtag[tid] = -1;
     if (tid == 0) sIfCount = 0;
     __syncthreads();
 
@@ -532,9 +517,8 @@ __device__ inline void k_fused(FusedArgs<S> a) {
         C* gc = nb[0].g;
         const bool actc = nb[0].act;
         const uint8_t tc = nb[0].t;
-        const double mc = nb[0].m;
+        const double mc = double(nb[0].m);
         const int32_t sc_ = nb[0].s;
-        const uint64_t bc_ = nb[0].b;
 
         CellOut<S> r;
         r.key = -1;
@@ -548,15 +532,18 @@ __device__ inline void k_fused(FusedArgs<S> a) {
 
         // boundary links of edge nodes (LB.cpp:1390-1528): the value reflected into the slot
         // this node loads next step
-        const bool edge = actc && (tc & T_EDGE) && bc_ != 0;
+        // The post-collision value of link j is read back from where store_pop just put it
+        // (PAR 0: the node's own slot opp(j); PAR 1: slot j of the neighbour), which the same
+        // lane wrote, so it needs no staging; the reflected value then goes to the slot the node
+        // loads next step.
+        const bool edge = actc && (tc & T_EDGE);
         if (__ballot(edge)) {
+            const uint64_t bc_ = edge ? a.bcm[ic] : 0;
             uint32_t lm = 0;
             if (edge) {
 #pragma unroll
-                for (int j = 1; j < Q; ++j) {
-                    sg[j - 1][tid] = gc[j];
+                for (int j = 1; j < Q; ++j)
                     if ((bc_ >> (BC_BITS * (j - 1))) & 7u) lm |= 1u << j;
-                }
             }
             const Offs o(g, x, y, z);
             const int wprim = int(bc_ >> BC_WALL_SHIFT) & 7;
@@ -564,8 +551,10 @@ __device__ inline void k_fused(FusedArgs<S> a) {
                 const int j = __ffs(lm) - 1;
                 lm &= lm - 1u;
                 const int cls = int(bc_ >> (BC_BITS * (j - 1))) & 7;
-                const C gj = sg[j - 1][tid];
                 const int32_t yj = ic + offs_rt(o, j);
+                S* const own = a.f + int64_t(opp_rt(j)) * N + ic;  // slot opp(j) of this node
+                S* const nbs = a.f + int64_t(j) * N + yj;          // slot j of the neighbour
+                const C gj = StoreT<S>::ld(PAR == 0 ? own : nbs);
                 C bcv;
                 if (cls == BC_GAS) {
                     const C cu = C(cx_rt(j)) * r.sx + C(cy_rt(j)) * r.sy + C(cz_rt(j)) * r.sz;
@@ -573,30 +562,20 @@ __device__ inline void k_fused(FusedArgs<S> a) {
                 } else if (cls == BC_WALL) {
                     if (DIAG && a.k.nWall > 0) {
                         const double v2 = 2.0 * double(gj);
-                        const int tag = sWtag[tid];
-                        if (tag < 0 || tag == wprim) {
-                            sWtag[tid] = int8_t(wprim);
-                            sAcc[ACC_WALL + 0][tid] += v2 * cx_rt(j);
-                            sAcc[ACC_WALL + 1][tid] += v2 * cy_rt(j);
-                            sAcc[ACC_WALL + 2][tid] += v2 * cz_rt(j);
-                        } else {  // a second wall along this column (corners of closed boxes)
-                            double* wp = &sPart[7 * MAX_OBJ_PART + 3 * wprim];
-                            if (cx_rt(j)) atomicAdd(wp + 0, v2 * cx_rt(j));
-                            if (cy_rt(j)) atomicAdd(wp + 1, v2 * cy_rt(j));
-                            if (cz_rt(j)) atomicAdd(wp + 2, v2 * cz_rt(j));
-                        }
+                        double* wp = &sPart[PART_WALL + 3 * wprim];
+                        if (cx_rt(j)) atomicAdd(wp + 0, v2 * cx_rt(j));
+                        if (cy_rt(j)) atomicAdd(wp + 1, v2 * cy_rt(j));
+                        if (cz_rt(j)) atomicAdd(wp + 2, v2 * cz_rt(j));
                     }
                     bcv = gj;
                 } else if (cls == BC_OBJ0 || cls == BC_OBJ1) {
-                    bcv = object_value<S>(a, j, yj, cls - BC_OBJ0, gj, r.n, mc, sPart, sAcc, tid, sObj);
+                    bcv = object_value<S>(a, j, yj, cls - BC_OBJ0, gj, r.n, mc, sPart, sObj);
                 } else {
                     const uint8_t ty = a.type[yj] & T_BASE;
                     if (ty == T_FLUID || ty == T_IF) continue;  // an object node uncovered since marking
-                    bcv = boundary_value<S, DIAG>(a, j, yj, ty, gj, r.n, r.sx, r.sy, r.sz, r.usq, mc, sPart, sAcc, tid,
-                                                  sObj);
+                    bcv = boundary_value<S, DIAG>(a, j, yj, ty, gj, r.n, r.sx, r.sy, r.sz, r.usq, mc, sPart, sObj);
                 }
-                if (PAR == 0) StoreT<S>::st(a.f + j * N + yj, bcv);
-                else StoreT<S>::st(a.f + opp_rt(j) * N + ic, bcv);
+                StoreT<S>::st(PAR == 0 ? nbs : own, bcv);
             }
         }
         if (FSURF) {
@@ -649,32 +628,6 @@ __device__ inline void k_fused(FusedArgs<S> a) {
                 if (base + k < a.ifCap) a.ifList[base + k] = sIf[k];
                 else atomicOr(&a.sc->listOverflow, 1);
             }
-        }
-    }
-    // per-thread accumulators into the block partials, in a fixed order (wavefront butterfly,
-    // then the waves in order); skipped by blocks without object or wall links
-    if (__syncthreads_or(sWtag[tid] >= 0 || sAcc[ACC_BB][tid] != 0.0 || sAcc[ACC_OBJ][tid] != 0.0 ||
-                         sAcc[ACC_OBJ + 1][tid] != 0.0 || sAcc[ACC_OBJ + 2][tid] != 0.0)) {
-        const int wv = tid / 64;
-        const int nv = 7 + (DIAG ? 3 * a.k.nWall : 0);  // object 0 (6), BB term, walls (3 each)
-        for (int c = 0; c < nv; ++c) {
-            double v;
-            if (c < 7) v = sAcc[c][tid];
-            else {
-                const int w = (c - 7) / 3;
-                v = sWtag[tid] == w ? sAcc[ACC_WALL + (c - 7) % 3][tid] : 0.0;
-            }
-#pragma unroll
-            for (int off = 32; off > 0; off >>= 1) v += __shfl_xor(v, off, 64);
-            if (lane == 0) sRed[c][wv] = v;
-        }
-        __syncthreads();
-        if (tid < nv) {
-            double t = 0.0;
-#pragma unroll
-            for (int w = 0; w < NT / 64; ++w) t += sRed[tid][w];
-            // partial layout: object o at 7 o (F, M, BB term at 6), wall w at 7 MAX_OBJ_PART + 3 w
-            sPart[tid < 7 ? tid : 7 * MAX_OBJ_PART + (tid - 7)] += t;
         }
     }
     // object / wall partials: written only by blocks that have any, flagged for the reduction
