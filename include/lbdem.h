@@ -135,6 +135,13 @@ int lbdem_get_types(lbdem_ctx* ctx, uint8_t* type, uint8_t* pflag, int32_t* soli
 /* f: 19*N post-streaming populations, SoA (f[q*N+i]), 0 where the node does not exist;
  * mass, n: N values (0 where the node does not exist). Any pointer may be NULL. */
 int lbdem_get_fields(lbdem_ctx* ctx, double* f, double* mass, double* n);
+/* lbdem_get_fields without any allocation (no counterpart in the reference; for callers that
+ * cannot allocate: code under HIP graph capture, tight serving loops, harnesses that forbid
+ * hipMalloc). Same arguments and values. The conversion goes through a 16 MiB device array that
+ * each GPU holds from the library's load until the process ends, one field or population plane at
+ * a time, so boxes of more than 2^21 (2,097,152) nodes are rejected with LBDEM_E_CAPACITY. Calls
+ * on one GPU share that array: calls that overlap on one GPU race. */
+int lbdem_get_fields_noalloc(lbdem_ctx* ctx, double* f, double* mass, double* n);
 int lbdem_get_particles(lbdem_ctx* ctx, double* rec /* nParticles*LBDEM_P_REC */);
 int lbdem_get_objects(lbdem_ctx* ctx, double* rec /* nObjects*LBDEM_O_REC */);
 int lbdem_get_walls(lbdem_ctx* ctx, double* rec /* nWalls*LBDEM_W_REC */);

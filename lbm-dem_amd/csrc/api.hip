@@ -13,6 +13,7 @@
 #include <string>
 #include <type_traits>
 #include <map>
+#include <mutex>
 #include <vector>
 
 using namespace lbdem;
@@ -103,7 +104,9 @@ struct lbdem_ctx {
     unsigned lbTime = 0, demTimeStep = 0, currentTimeStep = 0;
     double demTime = 0, realTime = 0, deltat = 1.0, totalMass = 0;
 
-    int fail(int code, const std::string& m) { err = m; return code; }
+    const char* errFixed = nullptr;  // a message with static storage (set without allocating)
+    int fail(int code, const std::string& m) { err = m; errFixed = nullptr; return code; }
+    int fail_fixed(int code, const char* m) { errFixed = m; return code; }
     // 256-byte granules; from the region when one is set, else one hipMalloc per buffer
     void* raw_alloc(size_t bytes) {
         bytes = (std::max<size_t>(bytes, 1) + 255) & ~size_t(255);
@@ -209,7 +212,9 @@ extern "C" int lbdem_destroy(lbdem_ctx* ctx) {
     return 0;
 }
 
-extern "C" const char* lbdem_last_error(const lbdem_ctx* ctx) { return ctx ? ctx->err.c_str() : "null context"; }
+extern "C" const char* lbdem_last_error(const lbdem_ctx* ctx) {
+    return ctx ? (ctx->errFixed ? ctx->errFixed : ctx->err.c_str()) : "null context";
+}
 
 extern "C" int lbdem_set_particles(lbdem_ctx* c, int n, const int* size, const double* r, const double* x0) {
     if (!c || n < 0 || (n > 0 && (!r || !x0))) return LBDEM_E_ARG;
@@ -463,10 +468,43 @@ extern "C" int lbdem_device_bytes(lbdem_ctx* c, size_t* bytes) {
     return 0;
 }
 
+// ---------------------------------------------------------------------------------------------
+// Download scratch of lbdem_get_fields_noalloc: a file-scope device array, so that entry point
+// calls no allocation. Each GPU holds its own copy from the library's load until the process
+// ends. In host code the array's name is only a stand-in for it: its device address is looked up
+// once per device (with that device current, in lbdem_init) and kept in scratchPtr; every host-side
+// use (kernel arguments, copies) takes that pointer.
+// ---------------------------------------------------------------------------------------------
+constexpr size_t NOALLOC_DOUBLES = size_t(1) << 21;  // 16 MiB: one field of a box of up to 2^21 nodes
+static __device__ __align__(256) double scratch[NOALLOC_DOUBLES];
+constexpr int NOALLOC_DEVICES = 64;
+static std::mutex scratchMu;
+static double* scratchPtr[NOALLOC_DEVICES];
+
+// the calling thread's current device must be ``dev``; returns a hipError_t
+static int noalloc_scratch_setup(int dev) {
+    if (dev < 0 || dev >= NOALLOC_DEVICES) return int(hipErrorInvalidDevice);
+    std::lock_guard<std::mutex> lk(scratchMu);
+    if (scratchPtr[dev]) return 0;
+    void* ptr = nullptr;
+    const hipError_t e = hipGetSymbolAddress(&ptr, HIP_SYMBOL(scratch));
+    if (e != hipSuccess || !ptr) return int(e != hipSuccess ? e : hipErrorInvalidSymbol);
+    scratchPtr[dev] = static_cast<double*>(ptr);
+    return 0;
+}
+static double* noalloc_scratch(int dev) {
+    if (dev < 0 || dev >= NOALLOC_DEVICES) return nullptr;
+    std::lock_guard<std::mutex> lk(scratchMu);
+    return scratchPtr[dev];
+}
+
 extern "C" int lbdem_init(lbdem_ctx* c) {
     if (!c) return LBDEM_E_ARG;
     if (c->initialised) return c->fail(LBDEM_E_STATE, "already initialised");
     HIPCHK(hipSetDevice(c->prm.device));
+    if (const int e = noalloc_scratch_setup(c->prm.device); e != 0)
+        return c->fail(LBDEM_E_DEVICE, std::string("hipGetSymbolAddress of the download scratch failed: ") +
+                                           hipGetErrorString(hipError_t(e)));
     HIPCHK(hipStreamCreateWithFlags(&c->stream, hipStreamNonBlocking));
     HIPCHK(hipStreamCreateWithFlags(&c->fsStream, hipStreamNonBlocking));
     HIPCHK(hipEventCreateWithFlags(&c->evFork, hipEventDisableTiming));
@@ -1077,6 +1115,44 @@ extern "C" int lbdem_get_fields(lbdem_ctx* c, double* f, double* mass, double* n
     HIPCHK(hipStreamSynchronize(c->stream));
     return 0;
 }
+
+// lbdem_get_fields through the file-scope scratch, one field (or population plane) at a time:
+// the same kernels and values, no allocation on the host or the device. Calls on one GPU share
+// its scratch and must not overlap.
+// (error messages are string literals: even the error paths allocate nothing)
+#define NOALLOC_CHK(x)                                                                        \
+    do {                                                                                      \
+        if ((x) != hipSuccess) return c->fail_fixed(LBDEM_E_DEVICE, "HIP error at " #x);      \
+    } while (0)
+extern "C" int lbdem_get_fields_noalloc(lbdem_ctx* c, double* f, double* mass, double* n) {
+    if (!c) return LBDEM_E_ARG;
+    if (!c->initialised) return c->fail_fixed(LBDEM_E_STATE, "call lbdem_init first");
+    const int64_t N = c->box.g.N;
+    if (N > int64_t(NOALLOC_DOUBLES))
+        return c->fail_fixed(LBDEM_E_CAPACITY,
+                             "lbdem_get_fields_noalloc: the box has more than 2097152 (2^21) nodes, the most its "
+                             "scratch holds; use lbdem_get_fields");
+    double* d = noalloc_scratch(c->prm.device);
+    if (!d) return c->fail_fixed(LBDEM_E_STATE, "lbdem_get_fields_noalloc: scratch not set up on this device");
+    // stream order: each copy out of the scratch completes before the next kernel overwrites it
+    if (f)
+        for (int q = 0; q < Q; ++q) {
+            lb_download_f(c->box, d, c->stream, q, q + 1);
+            NOALLOC_CHK(hipMemcpyAsync(f + q * N, d, size_t(N) * 8, hipMemcpyDeviceToHost, c->stream));
+        }
+    if (mass) {
+        lb_download_mass(c->box, d, c->stream);
+        NOALLOC_CHK(hipMemcpyAsync(mass, d, size_t(N) * 8, hipMemcpyDeviceToHost, c->stream));
+    }
+    if (n) {
+        lb_download_n(c->box, c->k, d, c->stream);
+        NOALLOC_CHK(hipMemcpyAsync(n, d, size_t(N) * 8, hipMemcpyDeviceToHost, c->stream));
+    }
+    NOALLOC_CHK(hipGetLastError());
+    NOALLOC_CHK(hipStreamSynchronize(c->stream));
+    return 0;
+}
+#undef NOALLOC_CHK
 
 static int fetch(lbdem_ctx* c, double* dst, const double* src, size_t n) {
     HIPCHK(hipMemcpyAsync(dst, src, n * 8, hipMemcpyDeviceToHost, c->stream));

@@ -1699,23 +1699,36 @@ void lb_object_update(Box& b, const Consts& k, const ObjLB* olb, DevScalars* sc,
 // ------------------------------------------------------------------------------------------
 // downloads
 // ------------------------------------------------------------------------------------------
+// post-streaming populations q0 <= q < q1 of every node in FP64 (plane q at out + (q - q0) N),
+// zero where the node is not active
 template <typename S>
-__device__ inline void k_download_f(Geom g, const S* f, const uint8_t* type, int L, double* out) {
+__device__ inline void k_download_f(Geom g, const S* f, const uint8_t* type, int L, int q0, int q1, double* out) {
     for (int64_t i = int64_t(blockIdx.x) * blockDim.x + threadIdx.x; i < g.N; i += int64_t(gridDim.x) * blockDim.x) {
         const bool act = is_active(type[i]);
         int x, y, z;
         cell_xyz(g, i, x, y, z);
-        if (!act) { for (int q = 0; q < Q; ++q) out[int64_t(q) * g.N + i] = 0.0; continue; }
+        if (!act) { for (int q = q0; q < q1; ++q) out[int64_t(q - q0) * g.N + i] = 0.0; continue; }
         const Offs o(g, x, y, z);
-        for (int q = 0; q < Q; ++q)
-            out[int64_t(q) * g.N + i] = wq(q) + double(StoreT<S>::ld(f + load_slot(g, i, o, q, L)));
+        for (int q = q0; q < q1; ++q)
+            out[int64_t(q - q0) * g.N + i] = wq(q) + double(StoreT<S>::ld(f + load_slot(g, i, o, q, L)));
     }
 }
-void lb_download_f(const Box& b, double* out, hipStream_t s) {
+void lb_download_f(const Box& b, double* out, hipStream_t s, int q0, int q1) {
     const int grid = 4096;
-    if (b.prec == LBDEM_PREC_FP64) launch<&k_download_f<double>, 1024, 1>(grid, 256, s, b.g, (const double*)b.f, b.type, b.parity, out);
-    else if (b.prec == LBDEM_PREC_FP32) launch<&k_download_f<float>, 1024, 1>(grid, 256, s, b.g, (const float*)b.f, b.type, b.parity, out);
-    else launch<&k_download_f<__half>, 1024, 1>(grid, 256, s, b.g, (const __half*)b.f, b.type, b.parity, out);
+    if (b.prec == LBDEM_PREC_FP64) launch<&k_download_f<double>, 1024, 1>(grid, 256, s, b.g, (const double*)b.f, b.type, b.parity, q0, q1, out);
+    else if (b.prec == LBDEM_PREC_FP32) launch<&k_download_f<float>, 1024, 1>(grid, 256, s, b.g, (const float*)b.f, b.type, b.parity, q0, q1, out);
+    else launch<&k_download_f<__half>, 1024, 1>(grid, 256, s, b.g, (const __half*)b.f, b.type, b.parity, q0, q1, out);
+}
+
+// mass field in FP64, zero where the node is not active
+template <typename M>
+__device__ inline void k_download_mass(int64_t N, const uint8_t* type, const M* mass, double* out) {
+    for (int64_t i = int64_t(blockIdx.x) * blockDim.x + threadIdx.x; i < N; i += int64_t(gridDim.x) * blockDim.x)
+        out[i] = is_active(type[i]) ? double(mass[i]) : 0.0;
+}
+void lb_download_mass(const Box& b, double* out, hipStream_t s) {
+    if (b.prec == LBDEM_PREC_FP64) launch<&k_download_mass<double>, 1024, 1>(4096, 256, s, b.g.N, b.type, (const double*)b.mass, out);
+    else launch<&k_download_mass<float>, 1024, 1>(4096, 256, s, b.g.N, b.type, (const float*)b.mass, out);
 }
 
 template <typename S>

@@ -211,7 +211,8 @@ void lb_object_update(Box& b, const Consts& k, const ObjLB* olb, DevScalars* sc,
 int fused_grid_blocks(const Geom& g, int fz);  // blocks of the fused kernel for z-walk length fz
 constexpr int FUSED_TILE_X = 64, FUSED_TILE_Y = 4;  // threads of a fused-kernel tile (x, y)
 constexpr int COVER_BLOCKS = 512, UNCOVER_BLOCKS_MAX = 1024;  // grids of the impactor node passes
-void lb_download_f(const Box& b, double* out, hipStream_t s);
+void lb_download_f(const Box& b, double* out, hipStream_t s, int q0 = 0, int q1 = Q);
+void lb_download_mass(const Box& b, double* out, hipStream_t s);
 void lb_download_n(const Box& b, const Consts& k, double* out, hipStream_t s);
 void lb_total_mass(const Box& b, double* out, hipStream_t s);
 void lb_count_types(const Box& b, int* counts, hipStream_t s);

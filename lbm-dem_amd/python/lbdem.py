@@ -158,6 +158,7 @@ class Library:
             "lbdem_get_info": ([vp, C.POINTER(Info)], C.c_int),
             "lbdem_get_types": ([vp, C.c_void_p, C.c_void_p, C.c_void_p], C.c_int),
             "lbdem_get_fields": ([vp, C.c_void_p, C.c_void_p, C.c_void_p], C.c_int),
+            "lbdem_get_fields_noalloc": ([vp, C.c_void_p, C.c_void_p, C.c_void_p], C.c_int),
             "lbdem_get_particles": ([vp, _dp], C.c_int),
             "lbdem_get_objects": ([vp, _dp], C.c_int),
             "lbdem_get_walls": ([vp, _dp], C.c_int),
@@ -281,13 +282,16 @@ class Simulation:
                     "lbdem_get_types")
         return t, p, s
 
-    def fields(self, want_f: bool = True):
+    def fields(self, want_f: bool = True, noalloc: bool = False):
+        """Populations (19 x N, or None), mass and density fields; ``noalloc`` goes through
+        lbdem_get_fields_noalloc (the library's static scratch; boxes of at most 2^21 nodes)."""
         N = self.info()["totNodes"]
         f = np.zeros(Q * N, np.float64) if want_f else None
         m = np.zeros(N, np.float64)
         n = np.zeros(N, np.float64)
-        self._check(self.lib.lib.lbdem_get_fields(self.ctx, f.ctypes.data if want_f else None, m.ctypes.data,
-                                                  n.ctypes.data), "lbdem_get_fields")
+        fn = self.lib.lib.lbdem_get_fields_noalloc if noalloc else self.lib.lib.lbdem_get_fields
+        self._check(fn(self.ctx, f.ctypes.data if want_f else None, m.ctypes.data, n.ctypes.data),
+                    "lbdem_get_fields_noalloc" if noalloc else "lbdem_get_fields")
         return (f.reshape(Q, N) if want_f else None), m, n
 
     def particles(self) -> np.ndarray:

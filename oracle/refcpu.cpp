@@ -1409,6 +1409,8 @@ int lbdem_get_fields(lbdem_ctx* c, double* f, double* mass, double* n) {
     }
     return 0;
 }
+// the CPU oracle allocates nothing per call either way: the same values as lbdem_get_fields
+int lbdem_get_fields_noalloc(lbdem_ctx* c, double* f, double* mass, double* n) { return lbdem_get_fields(c, f, mass, n); }
 static void put3(double* dst, const V3& v) { dst[0] = v.x; dst[1] = v.y; dst[2] = v.z; }
 int lbdem_get_particles(lbdem_ctx* c, double* rec) {
     NEED_INIT;
