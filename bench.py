@@ -109,16 +109,9 @@ def run_gpu(vals, ps, obs, prec, steps, warmup, run_steps=5000, timed_from=1500,
     sim.synchronize()
     t_run1 = time.perf_counter()
     run_done = state["done"]
-    # the fused collide-stream kernel, timed with HIP events on the library stream in a short
-    # pass of launches outside the graph
-    sim.profile(True)
-    sim.cycle(min(steps, 20))
-    sim.synchronize()
-    fused_ms, launches = sim.profile_read()
-    launches = max(launches, 1)
-    sim.profile(False)
     tuning = sim.tuning()
     sim.close()
+    fused_ms, launches, fused_active = fused_kernel_run(vals, ps, obs, prec, run_done, tuning["tile"][2], every)
     wall = t1 - t0
     active = 0.5 * (a0 + a1)
     run_wall = t_run1 - t_run0
@@ -133,12 +126,37 @@ def run_gpu(vals, ps, obs, prec, steps, warmup, run_steps=5000, timed_from=1500,
                run_mlups_active=float(np.mean(active_samples)) * run_done / run_wall / 1e6 if active_samples else None,
                depth_steps=depth_steps, depth=depth)
     b = BYTES_PER_CELL[prec]
-    res["fused_achieved_gbs"] = active * b / (res["fused_ms_avg"] * 1e-3) / 1e9
+    res["fused_active_cells"] = fused_active
+    res["fused_achieved_gbs"] = fused_active * b / (res["fused_ms_avg"] * 1e-3) / 1e9
     log(f"[bench] {name}: window steps {start_step}-{start_step + steps}: {res['ms_per_step']:.3f} ms/step, "
         f"{res['mlups_active']:.0f} MLUPS active; fused collide-stream {res['fused_ms_avg']:.3f} ms = "
         f"{res['fused_achieved_gbs']:.0f} GB/s; whole run of {run_done} steps {res['run_ms_per_step']:.3f} ms/step; "
         f"final depth {depth[-1] if depth else float('nan'):.4f}")
     return res
+
+
+def fused_kernel_run(vals, ps, obs, prec, run_steps, tile_z, every):
+    """The fused collide-stream kernel over a whole run: the same run again, every fused launch
+    bracketed by HIP events on the library stream (launches outside graphs), with the z-walk the
+    first run's tuner chose. Returns the summed kernel time, the launch count and the mean number of
+    active nodes (sampled every ``every`` steps)."""
+    lib = lbdem.Library.load()
+    sim = lbdem.Simulation(lib, lbdem.params_from_values(vals, prec), ps, obs)
+    sim.init()
+    sim.set_tile(tile_z)
+    sim.profile(True)
+    act = []
+    done = 0
+    while done < run_steps:
+        k = min(every, run_steps - done)
+        sim.cycle(k)
+        done += k
+        act.append(sim.info()["nActive"])
+    sim.synchronize()
+    ms, n = sim.profile_read()
+    sim.profile(False)
+    sim.close()
+    return ms, max(n, 1), float(np.mean(act))
 
 
 def depth_error(res, ref):
@@ -157,20 +175,16 @@ def depth_error(res, ref):
             "max_abs_diff_over_run": float(np.abs(d - r).max())}
 
 
-def cpu_baseline(seconds_budget=25.0):
-    """The reference program (oracle/_ref/ref_driver) on a bounded sample of the workload: a
-    64 x 80 x 64 box (x/z periodic) with the same sphere radius and volume fraction, 100 timed
-    steps, up to 16 host threads."""
+def cpu_baseline(steps=203, warm=3):
+    """The reference program (oracle/_ref/ref_driver, compiled from the reference's sources) on a
+    bounded sample of the reference-scale run (workloads.CONFIG0: 64^3 box, 480 spheres of radius
+    3, impactor r 8): its first ``steps`` steps (of 2000), the first ``warm`` untimed, with as many
+    OpenMP threads as the host gives one GPU (at most 16)."""
     drv = os.path.join(HERE, "oracle", "_ref", "ref_driver")
-    kind = "reference"
     if not os.path.exists(drv):
         return None
     with tempfile.TemporaryDirectory() as tmp:
-        size = (64, 80, 64)
-        n = int(round(0.5 * (size[0] - 2) * (64 - 1) * (size[2] - 2) / (4.0 / 3.0 * np.pi * 64)))
-        vals, ps, obs = workloads.impact_box(size=size, fluid_height=64.0, n_spheres=n, r=4.0, r_imp=8.0,
-                                             y_imp=74.0, speed=0.01, seed=7, relax=300,
-                                             periodic=True)
+        vals, ps, obs = workloads.impact_box(**workloads.CONFIG0)
         d = os.path.join(tmp, "case")
         os.makedirs(os.path.join(d, "input"))
         lbdem.write_getpot(os.path.join(d, "config.cfg"), vals)
@@ -178,10 +192,9 @@ def cpu_baseline(seconds_budget=25.0):
         lbdem.write_objects(os.path.join(d, "input", "objects.dat"), obs.r, obs.x0, obs.x1)
         with open(os.path.join(d, "input", "body.dat"), "w") as fh:
             fh.write("0\n")
-        # at most 16 threads: the host's share per GPU
-        threads = max(1, min(os.cpu_count() or 1, 16))
-        env = dict(os.environ, OMP_NUM_THREADS=str(threads), LBDEM_WARMUP="3")
-        steps = 103
+        nproc = os.cpu_count() or 1
+        threads = max(1, min(nproc, 16))
+        env = dict(os.environ, OMP_NUM_THREADS=str(threads), LBDEM_WARMUP=str(warm))
         t0 = time.time()
         r = subprocess.run([drv, "config.cfg", "out", str(steps), "dump"], cwd=d, env=env, capture_output=True,
                            text=True, timeout=600)
@@ -194,9 +207,12 @@ def cpu_baseline(seconds_budget=25.0):
         nst = int(kv["steps"])
         active = int(kv["active"])
         log(f"[bench] cpu baseline: {nst} steps in {sec:.2f} s ({time.time() - t0:.1f} s total), {active} active cells")
-        return {"value": active * nst / sec / 1e6, "unit": "MLUPS", "cores": int(kv["threads"]), "kind": kind,
-                "sample": f"reference program (g++ -O2 -fopenmp), 64x80x64 box x/z periodic, {n} spheres r=4 (phi~0.5), impactor r=8, "
-                          f"{nst} timed steps after 3 warm-up steps, {active} active cells"}
+        return {"value": active * nst / sec / 1e6, "unit": "MLUPS", "cores": int(kv["threads"]), "kind": "reference",
+                "host_cpus": nproc, "particle_steps_per_s": len(ps.r) * nst / sec,
+                "sample": f"reference program (its sources compiled with g++ -O2 -fopenmp) on the reference-scale "
+                          f"run: 64^3 box x/z periodic, fluid to 48, {len(ps.r)} spheres r=3 (phi~0.3), impactor r=8 "
+                          f"at 56, vy=-0.005; steps {warm + 1}-{warm + nst} of 2000 timed ({warm} warm-up), "
+                          f"{active} active cells, {kv['threads']} OpenMP threads of {nproc} host CPUs"}
 
 
 def run_pool(lib, runs, prec, budget, max_concurrent, chunk, arrivals=None):
@@ -233,6 +249,18 @@ def run_pool(lib, runs, prec, budget, max_concurrent, chunk, arrivals=None):
     return res, st
 
 
+def rates(res_like, span, prec):
+    """Whole-leg rates of a set of runs finished in ``span`` seconds: box node updates, active node
+    updates (each run's active nodes after its last step times its steps), particle steps, and the
+    active rate as a fraction of the HBM roofline at the algorithmic bytes per active node."""
+    node_steps = sum(x["nodeSteps"] for x, _ in res_like)
+    active_steps = sum(float(x["nActive"]) * r.nsteps for x, r in res_like)
+    mla = active_steps / span / 1e6
+    return {"mlups_box": round(node_steps / span / 1e6, 1), "mlups_active": round(mla, 1),
+            "particle_steps_per_s": round(sum(len(r.particles.r) * r.nsteps for _, r in res_like) / span, 1),
+            "roofline_frac": round(mla * 1e6 * BYTES_PER_CELL[prec] / (PEAK_HBM_GBS * 1e9), 4)}
+
+
 def batched(prec, n_boxes=64, nsteps=2000):
     """Many independent boxes of uneven size submitted together to the run pool: aggregate
     lattice updates per second over the whole batch."""
@@ -245,21 +273,21 @@ def batched(prec, n_boxes=64, nsteps=2000):
     res, st = run_pool(lib, runs, prec, budget, max_concurrent=n_boxes, chunk=32)
     t_first = min(x["tSubmit"] for x in res)
     span = max(x["tDone"] for x in res) - t_first
-    node_steps = sum(x["nodeSteps"] for x in res)
     out = {"boxes": n_boxes, "steps_each": nsteps, "edges": sorted({r.edge for r in runs}),
-           "total_nodes": int(sum(r.nodes for r in runs)), "seconds": round(span, 3),
-           "mlups_box": round(node_steps / span / 1e6, 1),
-           "particle_steps_per_s": round(sum(len(r.particles.r) * r.nsteps for r in runs) / span, 1),
-           "hbm_budget_bytes": budget, "bytes_peak": st["bytesPeak"], "active_peak": st["activePeak"],
-           "type_error_runs": st["type_error_runs"]}
+           "total_nodes": int(sum(r.nodes for r in runs)), "seconds": round(span, 3)}
+    out.update(rates(list(zip(res, runs)), span, prec))
+    out.update({"hbm_budget_bytes": budget, "bytes_peak": st["bytesPeak"], "active_peak": st["activePeak"],
+                "type_error_runs": st["type_error_runs"]})
     log(f"[bench] batched: {out}")
     return out
 
 
 def served(prec, n_req, capacity_mlups, load=0.8, seed=20261204):
     """Poisson arrivals of impact runs at ``load`` x the capacity measured on the batched
-    workload, served by the run pool; then the same requests served one at a time with plain
-    per-run launches (no pool, no graph replay). Latency is arrival to result."""
+    workload, served by the run pool; then the same requests, at the same arrival times, served one
+    at a time with plain per-run launches (no pool, no graph replay) by a server that takes each
+    request in arrival order once it has arrived and the previous one is done. Latency is arrival
+    to result, measured on the host clock in both legs."""
     lib = lbdem.Library.load()
     mean_nodes = float(np.mean([e ** 3 for e in workloads.SERVE_EDGES]))
     mean_steps = 0.5 * (1000 + 8000)
@@ -274,15 +302,20 @@ def served(prec, n_req, capacity_mlups, load=0.8, seed=20261204):
     span = max(x["tDone"] for x in res) - t_first
     node_steps = sum(x["nodeSteps"] for x in res)
     pool_out = {"p50_s": round(float(np.percentile(lat, 50)), 3), "p99_s": round(float(np.percentile(lat, 99)), 3),
-                "mean_s": round(float(lat.mean()), 3), "makespan_s": round(span, 3),
-                "mlups_box": round(node_steps / span / 1e6, 1), "runs_per_hour": round(3600.0 * n_req / span, 1),
-                "active_peak": st["activePeak"], "bytes_peak": st["bytesPeak"], "type_error_runs": st["type_error_runs"]}
+                "max_s": round(float(lat.max()), 3), "mean_s": round(float(lat.mean()), 3), "makespan_s": round(span, 3),
+                "runs_per_hour": round(3600.0 * n_req / span, 1)}
+    pool_out.update(rates(list(zip(res, runs)), span, prec))
+    pool_out.update({"active_peak": st["activePeak"], "bytes_peak": st["bytesPeak"],
+                     "type_error_runs": st["type_error_runs"]})
     log(f"[bench] served (pool): {pool_out}")
-    # plain sequential service of the same requests: each run alone, kernels launched one by one;
-    # a single server in arrival order (start = max(arrival, previous finish))
-    service = []
+    # plain sequential service, in real time
     seq_type_errors = 0
+    lat_seq, service, seq_res = [], [], []
+    t_base = time.perf_counter()
     for i, r in enumerate(runs):
+        wait = float(arrivals[i]) - (time.perf_counter() - t_base)
+        if wait > 0:
+            time.sleep(wait)
         sim = lbdem.Simulation(lib, lbdem.params_from_values(r.values, prec), r.particles, r.objects)
         sim.use_graphs(False)
         t = time.perf_counter()
@@ -291,30 +324,32 @@ def served(prec, n_req, capacity_mlups, load=0.8, seed=20261204):
         try:
             sim.synchronize()
             sim.objects()
+            n_act = sim.info()["nActive"]
         except lbdem.LbdemError as e:  # the reference's TYPE ERROR (see run_pool): timed all the same
             if "TYPE ERROR" not in str(e):
                 raise
             seq_type_errors += 1
-        service.append(time.perf_counter() - t)
+            n_act = res[i]["nActive"]
+        done = time.perf_counter()
+        service.append(done - t)
+        lat_seq.append(done - t_base - float(arrivals[i]))
+        seq_res.append({"nodeSteps": res[i]["nodeSteps"], "nActive": n_act})
         sim.close()
         if (i + 1) % 8 == 0:
             log(f"[bench] served (plain sequential): {i + 1}/{n_req} runs")
-    done = 0.0
-    lat_seq = []
-    for a, s_ in zip(arrivals, service):
-        done = max(float(a), done) + s_
-        lat_seq.append(done - float(a))
     lat_seq = np.array(lat_seq)
-    span_seq = done - float(arrivals[0])
+    span_seq = time.perf_counter() - t_base - float(arrivals[0])
     seq_out = {"p50_s": round(float(np.percentile(lat_seq, 50)), 3), "p99_s": round(float(np.percentile(lat_seq, 99)), 3),
-               "mean_s": round(float(lat_seq.mean()), 3), "makespan_s": round(span_seq, 3),
-               "mlups_box": round(node_steps / span_seq / 1e6, 1), "runs_per_hour": round(3600.0 * n_req / span_seq, 1),
+               "max_s": round(float(lat_seq.max()), 3), "mean_s": round(float(lat_seq.mean()), 3),
+               "makespan_s": round(span_seq, 3), "runs_per_hour": round(3600.0 * n_req / span_seq, 1),
                "service_mlups_box": round(node_steps / sum(service) / 1e6, 1), "type_error_runs": seq_type_errors}
+    seq_out.update(rates(list(zip(seq_res, runs)), span_seq, prec))
     log(f"[bench] served (plain sequential): {seq_out}")
     return {"requests": n_req, "rate_per_s": round(rate, 4), "offered_load": load,
             "capacity_mlups_box": round(capacity_mlups, 1), "edges": list(workloads.SERVE_EDGES),
             "steps_range": [1000, 8000], "hbm_budget_bytes": budget, "pool": pool_out, "plain_sequential": seq_out,
-            "runs_per_hour_ratio": round(pool_out["runs_per_hour"] / seq_out["runs_per_hour"], 3)}
+            "runs_per_hour_ratio": round(pool_out["runs_per_hour"] / seq_out["runs_per_hour"], 3),
+            "p99_ratio_plain_over_pool": round(seq_out["p99_s"] / pool_out["p99_s"], 3)}
 
 
 def precision_sweep(vals, ps, obs, checkpoints=(100, 1000, 5000), every=100):
@@ -411,7 +446,9 @@ def main():
     ap.add_argument("--small", action="store_true", help="quarter-size box (for quick checks)")
     ap.add_argument("--batched", type=int, default=64, help="boxes in the batched run (0: skip)")
     ap.add_argument("--batched-steps", type=int, default=2000)
-    ap.add_argument("--served", type=int, default=24, help="requests in the served run (0: skip)")
+    ap.add_argument("--served", type=int, default=48,
+                    help="requests in the served run (0: skip); 48 puts the 99th percentile between the two "
+                         "slowest results rather than on the slowest")
     ap.add_argument("--headline-only", action="store_true", help="skip the batched and served runs")
     ap.add_argument("--precision-sweep", type=int, default=0, metavar="STEPS",
                     help="only the precision sweep on the headline state (checkpoints 100, 1000, 5000 up to STEPS)")
@@ -510,8 +547,11 @@ def main():
         },
         "roofline": {"bound": "hbm", "achieved": round(ach, 1), "peak": PEAK_HBM_GBS, "unit": "GB/s",
                      "frac": round(ach / PEAK_HBM_GBS, 4), "traffic": traffic,
-                     "algorithmic_bytes_per_launch": int(main_res["active_cells"] * b),
-                     "kernel": "k_fused", "bytes_per_active_node": b},
+                     "algorithmic_bytes_per_launch": int(main_res["fused_active_cells"] * b),
+                     "kernel": "k_fused", "bytes_per_active_node": b,
+                     "kernel_ms_avg": round(main_res["fused_ms_avg"], 4), "launches": main_res["fused_launches"],
+                     "timing": "HIP events around every fused launch of a second, event-timed run of the whole "
+                               "headline run (same z-walk); active nodes averaged over that run"},
         "cpu_baseline": cpu,
         "batched": bat,
         "served": srv,

@@ -379,7 +379,7 @@ static bool plan_buffers(const Plan& p, Box& b, DemState& d, DevScalars*& sc, do
     b.ifCap = p.ifCap;
     b.objCap = p.objCap;
     b.changedCap = b.ifCap + b.objCap;
-    b.blockPartCap = fused_grid_blocks(p.g, 1) * PART_STRIDE_;  // the largest grid (z-walk 1)
+    b.blockPartCap = std::max(fused_grid_blocks(p.g, 1, LBDEM_PREC_FP32), fused_grid_blocks(p.g, 1, LBDEM_PREC_FP16)) * PART_STRIDE_;  // the largest grid (z-walk 1)
     A(b.f, size_t(Q) * N * p.selt);
     A(b.type, N);
     A(b.solid, N);
@@ -398,7 +398,7 @@ static bool plan_buffers(const Plan& p, Box& b, DemState& d, DevScalars*& sc, do
     A(b.coverCount, COVER_BLOCKS + UNCOVER_BLOCKS_MAX);
     A(b.changed, size_t(b.changedCap));
     A(b.blockPart, size_t(b.blockPartCap));
-    A(b.blockFlag, size_t(fused_grid_blocks(p.g, 1)));
+    A(b.blockFlag, size_t(std::max(fused_grid_blocks(p.g, 1, LBDEM_PREC_FP32), fused_grid_blocks(p.g, 1, LBDEM_PREC_FP16))));
     A(b.bcm, N);
     A(b.hacc, np1 * 8);
     A(sc, 1);
@@ -715,7 +715,9 @@ static void enqueue_dem(lbdem_ctx* c, hipEvent_t hydroReady = nullptr) {  // DEM
     if (!dem_step(c->dp, c->ds, c->sc, int(c->demTimeStep), c->stream, hydroReady)) c->streamError = true;
 }
 static void enqueue_coupling(lbdem_ctx* c) {  // LB::latticeBoltzmannCouplingStep (LB.cpp:432-467)
-    lb_flags_rebuild_compute(c->box, c->k, c->ds.plb, c->sc, c->stream);
+    // the lists of the rebuild resolve overlaps unless the reference builds none (no z-cell)
+    const bool lists = c->dp.nlx == 0 || c->dp.nlz > 0;
+    lb_flags_rebuild_compute(c->box, c->k, c->ds.plb, c->sc, c->ds.nbrCount, c->ds.nbrList, lists, c->stream);
     lb_object_update(c->box, c->k, c->ds.olb, c->sc, c->stream);
     lb_flags_rebuild_apply(c->box, c->sc, c->stream);
     lb_find_new_active(c->box, c->k, c->ds.plb, c->sc, c->stream);

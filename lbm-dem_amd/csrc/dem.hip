@@ -102,21 +102,27 @@ __device__ inline void k_cells_assign(NbrArgs a) {
     a.cellKey[p] = key;
     atomicAdd(&a.cellCount[key], 1);
 }
+// exclusive prefix sum of the cell counts (one block of 1024 threads): each thread sums a range of
+// cells, the ranges are scanned by wavefront shuffles and the wavefront totals in LDS
 __device__ inline void k_cells_scan(NbrArgs a, int ncell) {
     if (!a.sc->rebuild) return;
-    __shared__ int part[1024];
+    __shared__ int wtot[1024 / 64];
     const int per = (ncell + blockDim.x - 1) / blockDim.x;
     const int b = threadIdx.x * per, e = min(ncell, b + per);
     int s = 0;
     for (int c = b; c < e; ++c) s += a.cellCount[c];
-    part[threadIdx.x] = s;
-    __syncthreads();
-    if (threadIdx.x == 0) {
-        int acc = 0;
-        for (int t = 0; t < int(blockDim.x); ++t) { const int v = part[t]; part[t] = acc; acc += v; }
+    const int lane = threadIdx.x & 63, wv = threadIdx.x >> 6;
+    int incl = s;  // inclusive scan over the wavefront
+#pragma unroll
+    for (int off = 1; off < 64; off <<= 1) {
+        const int o = __shfl_up(incl, off, 64);
+        if (lane >= off) incl += o;
     }
+    if (lane == 63) wtot[wv] = incl;
     __syncthreads();
-    int acc = part[threadIdx.x];
+    int base = 0;
+    for (int w = 0; w < wv; ++w) base += wtot[w];
+    int acc = base + incl - s;
     for (int c = b; c < e; ++c) { a.cellStart[c] = acc; acc += a.cellCount[c]; }
 }
 __device__ inline void k_cells_fill(NbrArgs a) {
@@ -139,50 +145,69 @@ __device__ inline bool within_cut(const NbrArgs& a, int n1, int n2) {
     return r12 <= cut * cut;
 }
 
-// per-particle partner gather (pairs with minimum-image distance within sig + 2 in all axes)
+// Partner gather (pairs with minimum-image distance within sig + 2 in all axes), NBR_LANES lanes
+// per particle: with link cells, lane l < 27 takes the wrapped neighbour cell offset
+// (l / 9 - 1, l / 3 % 3 - 1, l % 3 - 1) (a cell that two offsets reach, with fewer than 3 cells
+// along z, is walked by both, so its partners are listed twice, as the reference lists them); in
+// the direct sum lane l takes every NBR_LANES-th particle. Partners collect in LDS, then the
+// group's first lane sorts them ascending and the group writes the list.
+constexpr int NBR_LANES = 32, NBR_BLOCK = 256, NBR_GROUPS = NBR_BLOCK / NBR_LANES;
 __device__ inline void k_nbr_build(NbrArgs a) {
     if (!a.sc->rebuild) return;
-    const int p = blockIdx.x * blockDim.x + threadIdx.x;
-    if (p >= a.p.nPart) return;
-    int lst[NBR_CAP];
-    int cnt = 0;
-    bool over = false;
+    __shared__ int sl[NBR_GROUPS][NBR_CAP];
+    __shared__ int sc[NBR_GROUPS];
+    __shared__ int sover[NBR_GROUPS];
+    const int grp = threadIdx.x / NBR_LANES, l = threadIdx.x % NBR_LANES;
+    const int p = blockIdx.x * NBR_GROUPS + grp;
+    const bool valid = p < a.p.nPart;
+    if (l == 0) { sc[grp] = 0; sover[grp] = 0; }
+    __syncthreads();
     auto consider = [&](int q) {
         if (q == p) return;
         const int n1 = p > q ? p : q, n2 = p > q ? q : p;
         if (!within_cut(a, n1, n2)) return;
-        if (cnt < NBR_CAP) lst[cnt++] = q; else over = true;
+        const int k = atomicAdd(&sc[grp], 1);
+        if (k < NBR_CAP - 1) sl[grp][k] = q; else sover[grp] = 1;  // 63 partners at most
     };
-    if (a.p.nlx == 0) {
-        for (int q = 0; q < a.p.nPart; ++q) consider(q);
-    } else if (a.p.nlz > 0) {
-        // with nlz < 3 the wrapped z offsets repeat a cell, and the partners in it are taken once
-        // per visit (DEM.cpp:585-627); with nlz == 0 the reference builds no list
-        const int nlx = a.p.nlx, nly = a.p.nly, nlz = a.p.nlz;
-        const int key = a.cellKey[p];
-        const int lx = key / (nly * nlz), ly = (key / nlz) % nly, lz = key % nlz;
-        for (int dx = -1; dx <= 1; ++dx)
-            for (int dy = -1; dy <= 1; ++dy)
-                for (int dz = -1; dz <= 1; ++dz) {
-                    const int c = ((lx + dx + nlx) % nlx) * nly * nlz + ((ly + dy + nly) % nly) * nlz + (lz + dz + nlz) % nlz;
-                    const int b = a.cellStart[c], e = b + a.cellCount[c];
-                    for (int t = b; t < e; ++t) consider(a.cellItems[t]);
-                }
+    if (valid) {
+        if (a.p.nlx == 0) {
+            for (int q = l; q < a.p.nPart; q += NBR_LANES) consider(q);
+        } else if (a.p.nlz > 0 && l < 27) {
+            // with nlz < 3 the wrapped z offsets repeat a cell, and the partners in it are taken once
+            // per visit (DEM.cpp:585-627); with nlz == 0 the reference builds no list
+            const int nlx = a.p.nlx, nly = a.p.nly, nlz = a.p.nlz;
+            const int key = a.cellKey[p];
+            const int lx = key / (nly * nlz), ly = (key / nlz) % nly, lz = key % nlz;
+            const int dx = l / 9 - 1, dy = (l / 3) % 3 - 1, dz = l % 3 - 1;
+            const int c = ((lx + dx + nlx) % nlx) * nly * nlz + ((ly + dy + nly) % nly) * nlz + (lz + dz + nlz) % nlz;
+            const int b = a.cellStart[c], e = b + a.cellCount[c];
+            for (int t = b; t < e; ++t) consider(a.cellItems[t]);
+        }
     }
-    // ascending partners (insertion sort, lists are short; a pair listed twice stays twice)
-    for (int i = 1; i < cnt; ++i) {
-        const int v = lst[i];
-        int j = i - 1;
-        while (j >= 0 && lst[j] > v) { lst[j + 1] = lst[j]; --j; }
-        lst[j + 1] = v;
+    __syncthreads();
+    const int cnt = sc[grp] < NBR_CAP - 1 ? sc[grp] : NBR_CAP - 1;
+    if (valid && l == 0) {
+        // ascending partners (insertion sort, lists are short; a pair listed twice stays twice)
+        int* v = sl[grp];
+        for (int i = 1; i < cnt; ++i) {
+            const int x = v[i];
+            int j = i - 1;
+            while (j >= 0 && v[j] > x) { v[j + 1] = v[j]; --j; }
+            v[j + 1] = x;
+        }
+        a.nbrCount[p] = cnt;
+        if (sover[grp]) atomicOr(&a.sc->nbrOverflow, 1);
     }
-    a.nbrCount[p] = cnt;
-    for (int i = 0; i < cnt; ++i) a.nbrList[p * NBR_CAP + i] = lst[i];
-    if (over) atomicOr(&a.sc->nbrOverflow, 1);
+    __syncthreads();
+    if (valid)
+        for (int k = l; k < cnt; k += NBR_LANES) a.nbrList[p * NBR_CAP + k] = sl[grp][k];
 }
 
-// slot of every entry (history lives in the list of the higher index), history carried over by
-// partner id (DEM.cpp:633-651); a partner absent from the old list starts with zero history
+// Slot of every entry, SLOT_LANES lanes per particle: the pair's tangential history lives in
+// the entry of the higher index, carried over by partner id from the old list (DEM.cpp:633-651;
+// the last matching old entry wins, as in the reference's loop); a partner absent from the old
+// list starts with zero history. The lower index finds its entry in the partner's (sorted) list.
+constexpr int SLOT_LANES = 16;
 __device__ inline void k_nbr_slots(NbrArgs a) {
     if (!a.sc->rebuild) return;
     // k_nbr_build was the cell list's last reader: clear it for the next rebuild (zero at
@@ -192,48 +217,50 @@ __device__ inline void k_nbr_slots(NbrArgs a) {
         a.cellCount[c] = 0;
         a.cellFill[c] = 0;
     }
-    const int p = blockIdx.x * blockDim.x + threadIdx.x;
+    const int e = blockIdx.x * blockDim.x + threadIdx.x;
+    const int p = e / SLOT_LANES;
     if (p >= a.p.nPart) return;
     const int cnt = a.nbrCount[p];
-    for (int k = 0; k < cnt; ++k) {
-        const int q = a.nbrList[p * NBR_CAP + k];
-        int slot;
-        if (p > q) {
-            slot = p * NBR_CAP + k;
-            double h0 = 0.0, h1 = 0.0, h2 = 0.0;
-            const int oc = a.oldCount[p];
-            for (int l = 0; l < oc; ++l)
-                if (a.oldList[p * NBR_CAP + l] == q) {
-                    const int os = p * NBR_CAP + l;
-                    h0 = a.xiCur[3 * os]; h1 = a.xiCur[3 * os + 1]; h2 = a.xiCur[3 * os + 2];
-                }
-            a.xiTmp[3 * slot] = h0; a.xiTmp[3 * slot + 1] = h1; a.xiTmp[3 * slot + 2] = h2;
-        } else {
-            // binary search p in q's (sorted) list
-            const int qc = a.nbrCount[q];
-            int lo = 0, hi = qc - 1, pos = -1;
-            while (lo <= hi) {
-                const int mid = (lo + hi) >> 1;
-                const int v = a.nbrList[q * NBR_CAP + mid];
-                if (v == p) { pos = mid; break; }
-                if (v < p) lo = mid + 1; else hi = mid - 1;
+    for (int k = e % SLOT_LANES; k < cnt; k += SLOT_LANES) {
+    const int q = a.nbrList[p * NBR_CAP + k];
+    int slot;
+    if (p > q) {
+        slot = p * NBR_CAP + k;
+        double h0 = 0.0, h1 = 0.0, h2 = 0.0;
+        const int oc = a.oldCount[p];
+        for (int l = 0; l < oc; ++l)
+            if (a.oldList[p * NBR_CAP + l] == q) {
+                const int os = p * NBR_CAP + l;
+                h0 = a.xiCur[3 * os]; h1 = a.xiCur[3 * os + 1]; h2 = a.xiCur[3 * os + 2];
             }
-            slot = pos >= 0 ? q * NBR_CAP + pos : -1;
+        a.xiTmp[3 * slot] = h0; a.xiTmp[3 * slot + 1] = h1; a.xiTmp[3 * slot + 2] = h2;
+    } else {
+        const int qc = a.nbrCount[q];
+        int lo = 0, hi = qc - 1, pos = -1;
+        while (lo <= hi) {
+            const int mid = (lo + hi) >> 1;
+            const int v = a.nbrList[q * NBR_CAP + mid];
+            if (v == p) { pos = mid; break; }
+            if (v < p) lo = mid + 1; else hi = mid - 1;
         }
-        a.nbrSlot[p * NBR_CAP + k] = slot;
+        slot = pos >= 0 ? q * NBR_CAP + pos : -1;
+    }
+    a.nbrSlot[p * NBR_CAP + k] = slot;
     }
 }
+// the new list and its carried history become the current ones, SLOT_LANES lanes per particle
 __device__ inline void k_nbr_commit(NbrArgs a, double* xiCurMut, int* oldCount, int* oldList) {
     if (!a.sc->rebuild) return;
-    const int p = blockIdx.x * blockDim.x + threadIdx.x;
+    const int e = blockIdx.x * blockDim.x + threadIdx.x;
+    const int p = e / SLOT_LANES;
     if (p >= a.p.nPart) return;
     const int cnt = a.nbrCount[p];
-    for (int k = 0; k < cnt; ++k) {
+    if (e % SLOT_LANES == 0) oldCount[p] = cnt;
+    for (int k = e % SLOT_LANES; k < cnt; k += SLOT_LANES) {
         const int s = p * NBR_CAP + k;
         xiCurMut[3 * s] = a.xiTmp[3 * s]; xiCurMut[3 * s + 1] = a.xiTmp[3 * s + 1]; xiCurMut[3 * s + 2] = a.xiTmp[3 * s + 2];
         oldList[s] = a.nbrList[s];
     }
-    oldCount[p] = cnt;
 }
 
 // ------------------------------------------------------------------------------------------
@@ -807,9 +834,10 @@ void dem_neighbor_list(const DemParams& p, DemState& d, DevScalars* sc, bool for
         launch<&k_cells_scan, 1024, 1>(1, 1024, s, a, ncell);
         launch<&k_cells_fill, 1024, 1>(pb, 128, s, a);
     }
-    launch<&k_nbr_build, 1024, 1>(pb, 128, s, a);
-    launch<&k_nbr_slots, 1024, 1>(pb, 128, s, a);
-    launch<&k_nbr_commit, 1024, 1>(pb, 128, s, a, d.xiCur, d.nbrCountOld, d.nbrListOld);
+    launch<&k_nbr_build, NBR_BLOCK, 1>((p.nPart + NBR_GROUPS - 1) / NBR_GROUPS, NBR_BLOCK, s, a);
+    const int eb = int((int64_t(p.nPart) * SLOT_LANES + 255) / 256);  // SLOT_LANES threads per particle
+    launch<&k_nbr_slots, 256, 1>(eb, 256, s, a);
+    launch<&k_nbr_commit, 256, 1>(eb, 256, s, a, d.xiCur, d.nbrCountOld, d.nbrListOld);
 }
 
 bool dem_step(const DemParams& p, DemState& d, DevScalars* sc, int demTimeStep, hipStream_t s, hipEvent_t hydroReady) {

@@ -135,7 +135,7 @@ void lb_init_fields(const Box& b, const Consts& k, hipStream_t s) {
 // ------------------------------------------------------------------------------------------
 // fused collide + stream
 // ------------------------------------------------------------------------------------------
-constexpr int FX = FUSED_TILE_X, FY = FUSED_TILE_Y;  // tile: 64x4 threads, each walks FZ cells
+constexpr int FX = FUSED_TILE_X, FY = FUSED_TILE_Y;  // 64x4 threads: 4 row segments, each lane walks FZ cells
                                                      // along z (FZ 1, 2, 4 or 8, chosen per box)
 constexpr int HS = 64;                  // per-block particle hash slots
 constexpr int PART_STRIDE = PART_STRIDE_;  // per-block partials: obj (7*MAX_OBJ_PART) + walls 18
@@ -162,7 +162,7 @@ struct FusedArgs {
     int32_t* blockFlag;
     const uint64_t* bcm;
     DevScalars* sc;
-    int nbx, nby, nbz;
+    int nseg, nby, nbz;  // 64-node segments per row (tiles per row with 2-byte storage), row groups, z-slabs
     int gridBlocks;  // the box's grid (a batch launch covers the largest box; the rest return)
     Consts k;
 };
@@ -416,12 +416,17 @@ __device__ inline uint64_t link_classes(const Geom& g, const uint8_t* type, cons
 // z+1). Edge nodes are handled by the same waves as their row neighbours, so every population
 // access stays coalesced; their boundary links run in a rolled loop over the precomputed link
 // classes, without further loads for gas and wall links.
-// Launch bounds of the fused kernel: two blocks per CU, i.e. up to 256 VGPRs per lane, so the
-// compiler never spills (a 168-VGPR bound spilled the odd-parity kernel). FP32 needs about 153,
-// so LDS (about 52 KB per block) sets residency at three blocks per CU. Measured on the
-// 256x320x256 impact run (FP32): three blocks resident 0.69 ms per launch; LDS padded to allow
-// only two 0.80 ms.
+// Launch bounds of the fused kernel, in blocks (one wavefront per SIMD each) per CU: four for
+// 4- and 2-byte storage (128 VGPRs: the free-surface FP32 and FP16 kernels fit without spilling,
+// those without a free surface spill up to 4 words; LDS is at most 13 KB per block), two for
+// FP64 (its populations alone take 38 VGPRs per node, and two nodes are live).
 template <typename S> struct FusedOcc { static constexpr int value = sizeof(S) == 8 ? 2 : 4; };
+// Block-to-node mapping. Row order (see k_fused) for 4- and 8-byte storage; measured on the
+// 256x320x256 impact box in the impact phase: FP32 0.726 -> 0.631 ms per launch, FP64
+// 1.310 -> 1.198 ms against 64 x 4 tiles spread over the XCDs (a probe of the bare AA accesses:
+// 5.2 TB/s in either parity in row order, 3.8 TB/s odd / 4.6 TB/s even with the tiles). 2-byte
+// storage keeps the tiles (0.521 ms there against 0.656 ms in row order).
+template <typename S> struct FusedRowOrder { static constexpr bool value = sizeof(S) >= 4; };
 // nodes whose bundle is loaded ahead of the one being collided. Measured as above: none 0.71 ms,
 // one 0.69 ms, two 0.78 ms (more loads in flight per CU slow the memory system down).
 template <typename S> struct FusedAhead { static constexpr int value = 1; };
@@ -470,20 +475,37 @@ __device__ inline void k_fused(FusedArgs<S> a) {
     if (tid == 0) sIfCount = 0;
     __syncthreads();
 
-    // XCD-aware tile order: consecutive tiles of the box go to the same XCD
-    const int nblk = a.nbx * a.nby * a.nbz;
-    const int b0 = blockIdx.x;
-    const int per = (nblk + 7) / 8;
-    const int tile = (b0 % 8) * per + b0 / 8;
-    const bool tile_ok = tile < nblk;
-    const int bx = tile % a.nbx, by = (tile / a.nbx) % a.nby, bz = tile / (a.nbx * a.nby);
-    const int x = bx * FX + threadIdx.x, y = by * FY + threadIdx.y;
-    static_assert(FX % 64 == 0, "a wavefront is one x-row segment: y and z are uniform across it");
-    const int yw = __builtin_amdgcn_readfirstlane(y);
+    static_assert(FX == 64, "a wavefront is one x-row segment: y and z are uniform across it");
     const Geom& g = a.g;
+    const int b0 = blockIdx.x;
+    int x, y, bz;
+    bool col_ok;
+    if constexpr (FusedRowOrder<S>::value) {
+        // Row order: wavefront threadIdx.y of block b takes row segment s = FY b + threadIdx.y, i.e.
+        // segment seg (64 nodes along x) of row y of z-slab bz, s = (bz Y + y) nseg + seg, and walks
+        // the slab's FZ nodes in z. The blocks resident at a time then cover a few consecutive
+        // z-planes, and the x -+ 1 accesses of a segment's edge lanes meet the neighbouring
+        // segment's in the same or the next block while both are in flight.
+        const int sid = b0 * FY + int(threadIdx.y);
+        const int seg = sid % a.nseg, rest = sid / a.nseg;
+        y = rest % g.Y;
+        bz = rest / g.Y;
+        x = seg * FX + threadIdx.x;
+        col_ok = bz < a.nbz && x < g.X;
+    } else {
+        // 64 x 4 tiles, consecutive tiles on one XCD (tile = (b % 8) per + b / 8)
+        const int nblk = a.nseg * a.nby * a.nbz;
+        const int per = (nblk + 7) / 8;
+        const int tile = (b0 % 8) * per + b0 / 8;
+        const int bx = tile % a.nseg, by = (tile / a.nseg) % a.nby;
+        bz = tile / (a.nseg * a.nby);
+        x = bx * FX + threadIdx.x;
+        y = by * FY + threadIdx.y;
+        col_ok = tile < nblk && x < g.X && y < g.Y;
+    }
+    const int yw = __builtin_amdgcn_readfirstlane(y);
     const int32_t N = int32_t(g.N), SZ = g.X * g.Y;
     const int lane = __lane_id();
-    const bool col_ok = tile_ok && x < g.X && y < g.Y;
     const int z0 = bz * FZ;
     auto type_at = [&](int zz) -> uint8_t {
         return (col_ok && zz < FZ && z0 + zz < g.Z) ? a.type[int32_t(cidx(g, x, y, z0 + zz))] : uint8_t(T_WALL);
@@ -756,9 +778,10 @@ void lb_mark_edges(const Box& b, DevScalars* sc, hipStream_t s) {
     // the list and the full-pass flag are reset by the next kernel (k_begin_lb)
 }
 
-int fused_grid_blocks(const Geom& g, int fz) {
-    const int nbx = (g.X + FX - 1) / FX, nby = (g.Y + FY - 1) / FY, nbz = (g.Z + fz - 1) / fz;
-    return ((nbx * nby * nbz + 7) / 8) * 8;
+int fused_grid_blocks(const Geom& g, int fz, int prec) {
+    const int nseg = (g.X + FX - 1) / FX, nby = (g.Y + FY - 1) / FY, nbz = (g.Z + fz - 1) / fz;
+    if (prec != LBDEM_PREC_FP16) return (nseg * g.Y * nbz + FY - 1) / FY;  // row order
+    return ((nseg * nby * nbz + 7) / 8) * 8;                              // XCD-spread tiles
 }
 
 template <typename S>
@@ -772,13 +795,14 @@ static void launch_fused(Box& b, const Consts& k, const PartLB* plb, const ObjLB
     a.plb = plb; a.olb = olb; a.hacc = b.hacc; a.blockPart = b.blockPart; a.blockFlag = b.blockFlag; a.bcm = b.bcm; a.sc = sc;
     a.k = k;
     const int fz = b.fz;
-    a.nbx = (b.g.X + FX - 1) / FX; a.nby = (b.g.Y + FY - 1) / FY; a.nbz = (b.g.Z + fz - 1) / fz;
-    const int grid = fused_grid_blocks(b.g, fz);
+    a.nseg = (b.g.X + FX - 1) / FX; a.nby = (b.g.Y + FY - 1) / FY; a.nbz = (b.g.Z + fz - 1) / fz;
+    const int grid = fused_grid_blocks(b.g, fz, b.prec);
     a.gridBlocks = grid;
     const bool fs = k.freeSurface != 0;
     if (ev0) (void)hipEventRecord(ev0, s);
     (void)diag;  // the wall diagnostics are always accumulated
-#define LBDEM_FUSED(P, F, Z) launch<&k_fused<S, P, F, true, Z>, FX* FY, FusedOcc<S>::value>(dim3(grid), dim3(FX, FY, 1), s, a)
+#define LBDEM_FUSED(P, F, Z) \
+    launch<&k_fused<S, P, F, true, Z>, FX * FY, FusedOcc<S>::value>(dim3(grid), dim3(FX, FY, 1), s, a)
 #define LBDEM_FUSED_Z(P, F)                 \
     switch (fz) {                          \
         case 1: LBDEM_FUSED(P, F, 1); break; \
@@ -882,7 +906,7 @@ __device__ inline void k_hydro_finalize(int nblk, const double* part, const int3
 
 void lb_hydro_finalize(const Box& b, const Consts& k, double* FHydro, double* MHydro, double* fluidVol,
                        DevScalars* sc, double* objFH, double* objMH, double* wallFH, bool diag, hipStream_t s) {
-    const int nblk = fused_grid_blocks(b.g, b.fz);
+    const int nblk = fused_grid_blocks(b.g, b.fz, b.prec);
     const int grid = PART_STRIDE + (k.nPart + HF_THREADS - 1) / HF_THREADS;
     int* count2 = (int*)(b.ifList2 + b.ifCap);  // free-surface counter stored after the list
     launch<&k_hydro_finalize, HF_THREADS, 1>(grid, HF_THREADS, s, nblk, b.blockPart, b.blockFlag, k, b.hacc, FHydro, MHydro, fluidVol, sc, objFH, objMH, wallFH, diag ? 1 : 0, count2);
@@ -891,7 +915,7 @@ void lb_hydro_finalize(const Box& b, const Consts& k, double* FHydro, double* MH
 // ------------------------------------------------------------------------------------------
 // free surface (LB.cpp:1548-1852)
 // ------------------------------------------------------------------------------------------
-constexpr int LIST_GRID = 1024;
+constexpr int LIST_GRID = 256;  // blocks of the free-surface list passes (few items: launch ramp dominates)
 
 template <typename S, int L>
 __device__ inline void k_fs_mass(Geom g, const S* f, const uint8_t* type, const typename Types<S>::M* mass,
@@ -932,18 +956,43 @@ __device__ inline void k_fs_mass(Geom g, const S* f, const uint8_t* type, const 
     }
 }
 
+// Position in a list for each lane of the wavefront that calls this now (any subset of its
+// lanes, in divergent code too): one atomic reservation per wavefront, lanes in lane order.
+__device__ inline int wave_reserve(int* count) {
+    const unsigned long long m = __ballot(1);
+    const int lane = __lane_id();
+    const int leader = __ffsll(m) - 1;
+    int base = 0;
+    if (lane == leader) base = atomicAdd(count, __popcll(m));
+    base = __shfl(base, leader, 64);
+    return base + __popcll(m & ((1ull << lane) - 1ull));
+}
+
 // a node changed between active and non-active (or among non-active types): its neighbourhood is
 // re-marked before the next LB step; an overflowing list asks for a full re-mark instead
 __device__ inline void push_changed(int32_t* changed, int cap, DevScalars* sc, int32_t i) {
-    const int pos = atomicAdd(&sc->nChanged, 1);
+    const int pos = wave_reserve(&sc->nChanged);
     if (pos < cap) changed[pos] = i;
     else atomicOr(&sc->fullMark, 1);
 }
 
 __device__ inline void list_push(int32_t* list, int* count, int cap, int32_t v, int* overflow) {
-    const int pos = atomicAdd(count, 1);
+    const int pos = wave_reserve(count);
     if (pos < cap) list[pos] = v;
     else atomicOr(overflow, 1);
+}
+
+// per-thread partial sums into one global total: summed over the wavefront first (every lane of
+// the wavefront must call this), one atomic per wavefront that has a nonzero sum
+__device__ inline void wave_add(double* dst, double v) {
+#pragma unroll
+    for (int off = 32; off > 0; off >>= 1) v += __shfl_xor(v, off, 64);
+    if (__lane_id() == 0 && v != 0.0) atomicAdd(dst, v);
+}
+__device__ inline void wave_add(int* dst, int v) {
+#pragma unroll
+    for (int off = 32; off > 0; off >>= 1) v += __shfl_xor(v, off, 64);
+    if (__lane_id() == 0 && v != 0) atomicAdd(dst, v);
 }
 
 // assign new mass, classify filled / emptied (findInterfaceMutants, LB.cpp:1669-1698)
@@ -1033,6 +1082,7 @@ __device__ inline void k_fs_from_emptied(Geom g, uint8_t* type, typename Types<S
     using C = typename Types<S>::C;
     using M = typename Types<S>::M;
     const int nE = sc->nEmptied;
+    double gained = 0.0;
     for (int k = blockIdx.x * blockDim.x + threadIdx.x; k < nE; k += gridDim.x * blockDim.x) {
         const int64_t i = emptied[k];
         int x, y, z;
@@ -1058,10 +1108,11 @@ __device__ inline void k_fs_from_emptied(Geom g, uint8_t* type, typename Types<S
             type[l] = (type[l] & (T_PMASK | T_EDGE)) | T_IF;  // active to active: edge bit stays
             mass[l] = M(0.99 * nl);
             aux[l] = C(nl);
-            atomicAdd(&sc->surplus, 0.01 * nl);
+            gained += 0.01 * nl;
             list_push(newIf, &sc->nNewIf, cap, int32_t(l), &sc->listOverflow);
         }
     }
+    wave_add(&sc->surplus, gained);
 }
 
 // updateMutants (LB.cpp:1751-1790)
@@ -1082,7 +1133,7 @@ __device__ inline void k_fs_update_mutants(uint8_t* type, typename Types<S>::M* 
             if ((type[i] & T_BASE) == T_GAS) acc += double(mass[i]);
         }
     }
-    if (acc != 0.0) atomicAdd(&sc->surplus, acc);
+    wave_add(&sc->surplus, acc);
 }
 
 // removeIsolated part 1: interface nodes without gas neighbours become fluid (LB.cpp:1796-1816).
@@ -1117,7 +1168,7 @@ __device__ inline void k_fs_isolated_a(Geom g, uint8_t* type, typename Types<S>:
             type[i] = (type[i] & (T_PMASK | T_EDGE)) | T_FLUID;  // active to active: edge bit stays
         }
     }
-    if (acc != 0.0) atomicAdd(&sc->surplus, acc);
+    wave_add(&sc->surplus, acc);
 }
 
 // removeIsolated part 2: interface nodes without fluid neighbours become gas (LB.cpp:1821-1841)
@@ -1144,8 +1195,8 @@ __device__ inline void k_fs_isolated_b(Geom g, uint8_t* type, const typename Typ
             keep++;
         }
     }
-    if (acc != 0.0) atomicAdd(&sc->surplus, acc);
-    if (keep) atomicAdd(&sc->nIfFinal, keep);
+    wave_add(&sc->surplus, acc);
+    wave_add(&sc->nIfFinal, keep);
 }
 
 // redistributeMass (LB.cpp:1844-1852) and marker reset
@@ -1202,19 +1253,44 @@ void lb_free_surface(Box& b, const Consts& k, DevScalars* sc, hipStream_t s) {
 // ------------------------------------------------------------------------------------------
 // particle flags (LB.cpp:683-711 via 1875-1899; findNewActive LB.cpp:1980-2029)
 // ------------------------------------------------------------------------------------------
-// blocks over particles, each particle's bounding box: active nodes inside the sphere, highest
-// index wins
+// Blocks over particles, each particle's bounding box: active nodes inside the sphere, highest
+// index wins. A node inside particle p and inside a higher-index particle q has |c_p - c_q| <
+// r_p + r_q, so q is on p's Verlet list, which this rebuild step built from positions one step
+// earlier with a skin of two length units (DEM.cpp:585-627). So p first gathers the higher-index
+// partners whose spheres overlap its own, then writes every node inside its sphere and inside none
+// of them with a plain store: each node gets exactly one writer, the highest index (the atomic
+// maximum the list-free path uses gives the same). Without a list (no link cell along z, where
+// the reference builds none) the atomic maximum is used.
 constexpr int FLAGS_GRID = 2048;  // grids of the flag passes (grid-stride; most steps return early)
 __device__ inline void k_flags_compute(Geom g, const uint8_t* type, const PartLB* plb, int nPart, int32_t* flagNew,
-                                DevScalars* sc) {
+                                DevScalars* sc, const int* nbrCount, const int* nbrList, int useList) {
     // a coupling step that follows a coupling step without an LB step between them (the DEM
     // start-up repeat) finishes the previous findNewActive here (no other block touches kminP)
     if (blockIdx.x == 0 && threadIdx.x == 0) fna_finish(sc);
     if (!sc->newNeighborList) return;
     if (blockIdx.x == 0 && threadIdx.x == 0) sc->kminP = INT_MAX;  // lowest flagged index, from k_flags_apply
+    __shared__ double sq[NBR_CAP][4];  // centre and radius of overlapping higher-index partners
+    __shared__ int nq;
     const int dims[3] = {g.X, g.Y, g.Z};
     for (int p = blockIdx.x; p < nPart; p += gridDim.x) {
         const PartLB P = plb[p];
+        if (useList) {
+            if (threadIdx.x == 0) nq = 0;
+            __syncthreads();
+            const int cnt = nbrCount[p];
+            for (int k = threadIdx.x; k < cnt; k += blockDim.x) {
+                const int q = nbrList[p * NBR_CAP + k];
+                if (q <= p) continue;
+                const PartLB Qp = plb[q];
+                const double dx = Qp.c[0] - P.c[0], dy = Qp.c[1] - P.c[1], dz = Qp.c[2] - P.c[2];
+                const double rs = P.r + Qp.r + 1e-9;  // a margin: a spare candidate only costs a test
+                if (dx * dx + dy * dy + dz * dz < rs * rs) {
+                    const int slot = atomicAdd(&nq, 1);
+                    sq[slot][0] = Qp.c[0]; sq[slot][1] = Qp.c[1]; sq[slot][2] = Qp.c[2]; sq[slot][3] = Qp.r;
+                }
+            }
+            __syncthreads();
+        }
         int lo[3], hi[3];
         for (int a = 0; a < 3; ++a) {
             lo[a] = int(floor(P.c[a] - P.r)) - 1;
@@ -1222,22 +1298,34 @@ __device__ inline void k_flags_compute(Geom g, const uint8_t* type, const PartLB
             lo[a] = lo[a] < 0 ? 0 : lo[a];
             hi[a] = hi[a] > dims[a] - 1 ? dims[a] - 1 : hi[a];
         }
-        if (lo[0] > hi[0] || lo[1] > hi[1] || lo[2] > hi[2]) continue;
         const int nx = hi[0] - lo[0] + 1, ny = hi[1] - lo[1] + 1, nz = hi[2] - lo[2] + 1;
-        const int tot = nx * ny * nz;
+        const int tot = (lo[0] > hi[0] || lo[1] > hi[1] || lo[2] > hi[2]) ? 0 : nx * ny * nz;
+        const int nHigher = useList ? nq : 0;
+        // the sphere tests first (arithmetic only): the type byte is loaded for the nodes kept
         for (int c = threadIdx.x; c < tot; c += blockDim.x) {
             const int x = lo[0] + c % nx, y = lo[1] + (c / nx) % ny, z = lo[2] + c / (nx * ny);
+            if (!inside_sphere(x, y, z, P.c[0], P.c[1], P.c[2], P.r)) continue;
+            if (useList) {
+                bool higher = false;
+                for (int j = 0; j < nHigher && !higher; ++j)
+                    higher = inside_sphere(x, y, z, sq[j][0], sq[j][1], sq[j][2], sq[j][3]);
+                if (higher) continue;
+            }
             const int64_t i = cidx(g, x, y, z);
             if (!is_active(type[i])) continue;
-            if (inside_sphere(x, y, z, P.c[0], P.c[1], P.c[2], P.r)) atomicMax(&flagNew[i], p);
+            if (useList) flagNew[i] = p;
+            else atomicMax(&flagNew[i], p);
         }
+        if (useList) __syncthreads();  // sq is refilled for the next particle
     }
 }
 
-void lb_flags_rebuild_compute(const Box& b, const Consts& k, const PartLB* plb, DevScalars* sc, hipStream_t s) {
+void lb_flags_rebuild_compute(const Box& b, const Consts& k, const PartLB* plb, DevScalars* sc, const int* nbrCount,
+                              const int* nbrList, bool useList, hipStream_t s) {
     // grid-stride over the particles: most steps only test the flag and return
     if (k.nPart > 0)
-        launch_gs<&k_flags_compute, 1024, 1, 16384>(std::min(k.nPart, FLAGS_GRID), 256, s, b.g, b.type, plb, k.nPart, b.flagNew, sc);
+        launch_gs<&k_flags_compute, 1024, 1, 16384>(std::min(k.nPart, FLAGS_GRID), 256, s, b.g, b.type, plb, k.nPart,
+                                                    b.flagNew, sc, nbrCount, nbrList, useList ? 1 : 0);
 }
 
 // clear every flag and solid index, apply the new flags (initializeParticleBoundaries)

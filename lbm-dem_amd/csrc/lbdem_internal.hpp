@@ -203,12 +203,15 @@ void lb_fused_step(Box& b, const Consts& k, const PartLB* plb, const ObjLB* olb,
 void lb_hydro_finalize(const Box& b, const Consts& k, double* FHydro, double* MHydro, double* fluidVol,
                        DevScalars* sc, double* objFH, double* objMH, double* wallFH, bool diag, hipStream_t s);
 void lb_free_surface(Box& b, const Consts& k, DevScalars* sc, hipStream_t s);
-void lb_flags_rebuild_compute(const Box& b, const Consts& k, const PartLB* plb, DevScalars* sc, hipStream_t s);
+// particle flags of a neighbour-list rebuild; with useList the Verlet lists of that rebuild
+// resolve overlapping spheres (highest index wins) without atomics
+void lb_flags_rebuild_compute(const Box& b, const Consts& k, const PartLB* plb, DevScalars* sc, const int* nbrCount,
+                              const int* nbrList, bool useList, hipStream_t s);
 constexpr int PART_STRIDE_ = 32;
 void lb_flags_rebuild_apply(const Box& b, DevScalars* sc, hipStream_t s);
 void lb_find_new_active(const Box& b, const Consts& k, const PartLB* plb, DevScalars* sc, hipStream_t s);
 void lb_object_update(Box& b, const Consts& k, const ObjLB* olb, DevScalars* sc, hipStream_t s);
-int fused_grid_blocks(const Geom& g, int fz);  // blocks of the fused kernel for z-walk length fz
+int fused_grid_blocks(const Geom& g, int fz, int prec);  // blocks of the fused kernel for z-walk length fz
 constexpr int FUSED_TILE_X = 64, FUSED_TILE_Y = 4;  // threads of a fused-kernel tile (x, y)
 constexpr int COVER_BLOCKS = 512, UNCOVER_BLOCKS_MAX = 1024;  // grids of the impactor node passes
 void lb_download_f(const Box& b, double* out, hipStream_t s, int q0 = 0, int q1 = Q);

@@ -81,6 +81,15 @@ def volume_fraction(ps: lbdem.ParticleSet, size, fluid_height) -> float:
 HEADLINE = dict(size=(256, 320, 256), fluid_height=256.0, n_spheres=31000, r=4.0, r_imp=16.0, y_imp=290.0,
                 speed=0.01, seed=20261202, periodic=True)
 
+# The reference-scale run: a 64^3 box (x, z periodic, floor and lid walls), fluid to 48, an
+# impactor of radius 8 at height 56 falling at 0.005, and spheres of radius 3. The run is specified
+# both as 1000 spheres and as phi ~ 0.3 in the fluid, which disagree (1000 spheres of radius 3 fill
+# 0.63 of the 62 x 47 x 62 fluid region, above what a random packing reaches); phi = 0.3 is kept:
+# 480 spheres by seeded random sequential addition (below the jamming limit, no relaxation needed).
+CONFIG0 = dict(size=(64, 64, 64), fluid_height=48.0, n_spheres=480, r=3.0, r_imp=8.0, y_imp=56.0, speed=0.005,
+               seed=20261201, relax=100, periodic=True)
+CONFIG0_STEPS = 2000
+
 
 # ----------------------------------------------------------------------------------------------
 # Many independent impact runs (batched and served workloads)
