@@ -412,7 +412,7 @@ static bool plan_buffers(const Plan& p, Box& b, DemState& d, DevScalars*& sc, do
     const size_t slots = np1 * NBR_CAP;
     A(d.nbrCount, np1); A(d.nbrCountOld, np1);
     A(d.nbrList, slots); A(d.nbrListOld, slots); A(d.nbrSlot, slots);
-    A(d.xiCur, 3 * slots); A(d.xiNext, 3 * slots); A(d.xiTmp, 3 * slots);
+    A(d.xiCur, 3 * slots); A(d.xiNext, 3 * slots);
     A(d.cellCount, size_t(p.ncell)); A(d.cellStart, size_t(p.ncell)); A(d.cellFill, size_t(p.ncell));
     A(d.cellItems, np1); A(d.cellKey, np1);
     A(d.objContrib, np1 * 12 * no1);
@@ -637,7 +637,6 @@ extern "C" int lbdem_init(lbdem_ctx* c) {
     HIPCHK(hipMemsetAsync(d.nbrCountOld, 0, std::max(np, 1) * 4, st));
     HIPCHK(hipMemsetAsync(d.xiCur, 0, 3 * slots * 8, st));
     HIPCHK(hipMemsetAsync(d.xiNext, 0, 3 * slots * 8, st));
-    HIPCHK(hipMemsetAsync(d.xiTmp, 0, 3 * slots * 8, st));
     if (np > 0) {
         HIPCHK(hipMemcpyAsync(d.x0, c->px0.data(), 3 * np * 8, hipMemcpyHostToDevice, st));
         HIPCHK(hipMemcpyAsync(d.r, c->pr.data(), np * 8, hipMemcpyHostToDevice, st));
@@ -717,7 +716,9 @@ static void enqueue_dem(lbdem_ctx* c, hipEvent_t hydroReady = nullptr) {  // DEM
 static void enqueue_coupling(lbdem_ctx* c) {  // LB::latticeBoltzmannCouplingStep (LB.cpp:432-467)
     // the lists of the rebuild resolve overlaps unless the reference builds none (no z-cell)
     const bool lists = c->dp.nlx == 0 || c->dp.nlz > 0;
-    lb_flags_rebuild_compute(c->box, c->k, c->ds.plb, c->sc, c->ds.nbrCount, c->ds.nbrList, lists, c->stream);
+    const bool cells = c->dp.nlx > 0 && c->dp.nlz > 0;  // the rebuild sorted the particles by link cell
+    lb_flags_rebuild_compute(c->box, c->k, c->ds.plb, c->sc, c->ds.nbrCount, c->ds.nbrList, lists,
+                             cells ? c->ds.cellItems : nullptr, c->stream);
     lb_object_update(c->box, c->k, c->ds.olb, c->sc, c->stream);
     lb_flags_rebuild_apply(c->box, c->sc, c->stream);
     lb_find_new_active(c->box, c->k, c->ds.plb, c->sc, c->stream);

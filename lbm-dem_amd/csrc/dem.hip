@@ -82,10 +82,6 @@ struct NbrArgs {
     int* nbrCount;
     int* nbrList;
     int* nbrSlot;
-    const int* oldCount;
-    const int* oldList;
-    const double* xiCur;
-    double* xiTmp;
     DevScalars* sc;
 };
 
@@ -102,28 +98,45 @@ __device__ inline void k_cells_assign(NbrArgs a) {
     a.cellKey[p] = key;
     atomicAdd(&a.cellCount[key], 1);
 }
-// exclusive prefix sum of the cell counts (one block of 1024 threads): each thread sums a range of
-// cells, the ranges are scanned by wavefront shuffles and the wavefront totals in LDS
+// exclusive prefix sum of the cell counts by one block of 1024 threads, 4096 cells at a time: each
+// thread takes 4 consecutive counts (coalesced over the block), the block scans the thread sums
+// (wavefront shuffles, wavefront totals in LDS) and the chunk's total carries into the next chunk
+constexpr int SCAN_THREADS = 1024, SCAN_PER = 4;
 __device__ inline void k_cells_scan(NbrArgs a, int ncell) {
     if (!a.sc->rebuild) return;
-    __shared__ int wtot[1024 / 64];
-    const int per = (ncell + blockDim.x - 1) / blockDim.x;
-    const int b = threadIdx.x * per, e = min(ncell, b + per);
-    int s = 0;
-    for (int c = b; c < e; ++c) s += a.cellCount[c];
-    const int lane = threadIdx.x & 63, wv = threadIdx.x >> 6;
-    int incl = s;  // inclusive scan over the wavefront
-#pragma unroll
-    for (int off = 1; off < 64; off <<= 1) {
-        const int o = __shfl_up(incl, off, 64);
-        if (lane >= off) incl += o;
-    }
-    if (lane == 63) wtot[wv] = incl;
+    __shared__ int wtot[SCAN_THREADS / 64];
+    __shared__ int carry;
+    if (threadIdx.x == 0) carry = 0;
     __syncthreads();
-    int base = 0;
-    for (int w = 0; w < wv; ++w) base += wtot[w];
-    int acc = base + incl - s;
-    for (int c = b; c < e; ++c) { a.cellStart[c] = acc; acc += a.cellCount[c]; }
+    const int lane = threadIdx.x & 63, wv = threadIdx.x >> 6;
+    for (int c0 = 0; c0 < ncell; c0 += SCAN_THREADS * SCAN_PER) {
+        const int b = c0 + SCAN_PER * int(threadIdx.x);
+        int v[SCAN_PER];
+        int s = 0;
+#pragma unroll
+        for (int j = 0; j < SCAN_PER; ++j) {
+            v[j] = b + j < ncell ? a.cellCount[b + j] : 0;
+            s += v[j];
+        }
+        int incl = s;  // inclusive scan over the wavefront
+#pragma unroll
+        for (int off = 1; off < 64; off <<= 1) {
+            const int o = __shfl_up(incl, off, 64);
+            if (lane >= off) incl += o;
+        }
+        if (lane == 63) wtot[wv] = incl;
+        __syncthreads();
+        int acc = carry + incl - s;
+        for (int w = 0; w < wv; ++w) acc += wtot[w];
+#pragma unroll
+        for (int j = 0; j < SCAN_PER; ++j) {
+            if (b + j < ncell) a.cellStart[b + j] = acc;
+            acc += v[j];
+        }
+        __syncthreads();  // carry and wtot are read
+        if (threadIdx.x == SCAN_THREADS - 1) carry = acc;
+        __syncthreads();
+    }
 }
 __device__ inline void k_cells_fill(NbrArgs a) {
     if (!a.sc->rebuild) return;
@@ -134,13 +147,15 @@ __device__ inline void k_cells_fill(NbrArgs a) {
     a.cellItems[a.cellStart[key] + pos] = p;
 }
 
-__device__ inline bool within_cut(const NbrArgs& a, int n1, int n2) {
+// the pair (n1, n2) = (higher, lower index) within sig + 2 (DEM.cpp:601-608); positions and radii
+// passed in, differences taken higher minus lower as in the reference
+__device__ inline bool within_cut(const DemParams& P, const double* x1, double r1, const double* x2, double r2) {
 #pragma clang fp contract(off)
-    const double x12 = n_image(a.x0[3 * n1] - a.x0[3 * n2], a.p.demSize[0]);
-    const double y12 = n_image(a.x0[3 * n1 + 1] - a.x0[3 * n2 + 1], a.p.demSize[1]);
-    const double z12 = n_image(a.x0[3 * n1 + 2] - a.x0[3 * n2 + 2], a.p.demSize[2]);
+    const double x12 = n_image(x1[0] - x2[0], P.demSize[0]);
+    const double y12 = n_image(x1[1] - x2[1], P.demSize[1]);
+    const double z12 = n_image(x1[2] - x2[2], P.demSize[2]);
     const double r12 = x12 * x12 + y12 * y12 + z12 * z12;
-    const double sig = a.r[n1] + a.r[n2];
+    const double sig = r1 + r2;
     const double cut = sig + 2.0;
     return r12 <= cut * cut;
 }
@@ -149,8 +164,9 @@ __device__ inline bool within_cut(const NbrArgs& a, int n1, int n2) {
 // per particle: with link cells, lane l < 27 takes the wrapped neighbour cell offset
 // (l / 9 - 1, l / 3 % 3 - 1, l % 3 - 1) (a cell that two offsets reach, with fewer than 3 cells
 // along z, is walked by both, so its partners are listed twice, as the reference lists them); in
-// the direct sum lane l takes every NBR_LANES-th particle. Partners collect in LDS, then the
-// group's first lane sorts them ascending and the group writes the list.
+// the direct sum lane l takes every NBR_LANES-th particle. Partners collect in LDS; each lane then
+// ranks its entries among the group's (ascending, equal ids by collection order) and writes them
+// to their place in the list.
 constexpr int NBR_LANES = 32, NBR_BLOCK = 256, NBR_GROUPS = NBR_BLOCK / NBR_LANES;
 __device__ inline void k_nbr_build(NbrArgs a) {
     if (!a.sc->rebuild) return;
@@ -162,10 +178,13 @@ __device__ inline void k_nbr_build(NbrArgs a) {
     const bool valid = p < a.p.nPart;
     if (l == 0) { sc[grp] = 0; sover[grp] = 0; }
     __syncthreads();
+    double xp[3] = {0.0, 0.0, 0.0}, rp = 0.0;
+    if (valid) { xp[0] = a.x0[3 * p]; xp[1] = a.x0[3 * p + 1]; xp[2] = a.x0[3 * p + 2]; rp = a.r[p]; }
     auto consider = [&](int q) {
         if (q == p) return;
-        const int n1 = p > q ? p : q, n2 = p > q ? q : p;
-        if (!within_cut(a, n1, n2)) return;
+        const double xq[3] = {a.x0[3 * q], a.x0[3 * q + 1], a.x0[3 * q + 2]};
+        const double rq = a.r[q];
+        if (!(p > q ? within_cut(a.p, xp, rp, xq, rq) : within_cut(a.p, xq, rq, xp, rp))) return;
         const int k = atomicAdd(&sc[grp], 1);
         if (k < NBR_CAP - 1) sl[grp][k] = q; else sover[grp] = 1;  // 63 partners at most
     };
@@ -185,30 +204,35 @@ __device__ inline void k_nbr_build(NbrArgs a) {
         }
     }
     __syncthreads();
+    if (!valid) return;
     const int cnt = sc[grp] < NBR_CAP - 1 ? sc[grp] : NBR_CAP - 1;
-    if (valid && l == 0) {
-        // ascending partners (insertion sort, lists are short; a pair listed twice stays twice)
-        int* v = sl[grp];
-        for (int i = 1; i < cnt; ++i) {
-            const int x = v[i];
-            int j = i - 1;
-            while (j >= 0 && v[j] > x) { v[j + 1] = v[j]; --j; }
-            v[j + 1] = x;
+    const int* v = sl[grp];
+    for (int k = l; k < cnt; k += NBR_LANES) {
+        const int x = v[k];
+        int rank = 0;
+        for (int j = 0; j < cnt; ++j) {
+            const int y = v[j];
+            rank += (y < x || (y == x && j < k)) ? 1 : 0;
         }
+        a.nbrList[p * NBR_CAP + rank] = x;
+    }
+    if (l == 0) {
         a.nbrCount[p] = cnt;
         if (sover[grp]) atomicOr(&a.sc->nbrOverflow, 1);
     }
-    __syncthreads();
-    if (valid)
-        for (int k = l; k < cnt; k += NBR_LANES) a.nbrList[p * NBR_CAP + k] = sl[grp][k];
 }
 
-// Slot of every entry, SLOT_LANES lanes per particle: the pair's tangential history lives in
-// the entry of the higher index, carried over by partner id from the old list (DEM.cpp:633-651;
-// the last matching old entry wins, as in the reference's loop); a partner absent from the old
-// list starts with zero history. The lower index finds its entry in the partner's (sorted) list.
-constexpr int SLOT_LANES = 16;
-__device__ inline void k_nbr_slots(NbrArgs a) {
+// Slot and carried tangential history of every entry, SLOT_LANES lanes per particle (a
+// particle's lanes lie in one wavefront): the pair's history lives in the entry of the higher
+// index, carried over by partner id from the old list (DEM.cpp:633-651; the last matching old
+// entry wins, as in the reference's loop); a partner absent from the old list starts with zero
+// history. The lower index finds its entry in the partner's (ascending) list. Every lane first
+// reads what it needs of its particle's old list and history, then the lanes write the new list,
+// history and slots over the old ones in place: a particle's old entries are read and written only
+// by its own lanes, and the memory operations of a wavefront are performed in order.
+constexpr int SLOT_LANES = 16, SLOT_PER = NBR_CAP / SLOT_LANES;
+static_assert(64 % SLOT_LANES == 0, "a particle's lanes lie in one wavefront");
+__device__ inline void k_nbr_slots(NbrArgs a, double* xi, int* oldCount, int* oldList) {
     if (!a.sc->rebuild) return;
     // k_nbr_build was the cell list's last reader: clear it for the next rebuild (zero at
     // initialisation)
@@ -218,48 +242,53 @@ __device__ inline void k_nbr_slots(NbrArgs a) {
         a.cellFill[c] = 0;
     }
     const int e = blockIdx.x * blockDim.x + threadIdx.x;
-    const int p = e / SLOT_LANES;
-    if (p >= a.p.nPart) return;
-    const int cnt = a.nbrCount[p];
-    for (int k = e % SLOT_LANES; k < cnt; k += SLOT_LANES) {
-    const int q = a.nbrList[p * NBR_CAP + k];
-    int slot;
-    if (p > q) {
-        slot = p * NBR_CAP + k;
-        double h0 = 0.0, h1 = 0.0, h2 = 0.0;
-        const int oc = a.oldCount[p];
-        for (int l = 0; l < oc; ++l)
-            if (a.oldList[p * NBR_CAP + l] == q) {
-                const int os = p * NBR_CAP + l;
-                h0 = a.xiCur[3 * os]; h1 = a.xiCur[3 * os + 1]; h2 = a.xiCur[3 * os + 2];
+    const int p = e / SLOT_LANES, l = e % SLOT_LANES;
+    const bool valid = p < a.p.nPart;
+    const int cnt = valid ? a.nbrCount[p] : 0;
+    const int oc = valid ? oldCount[p] : 0;
+    int q[SLOT_PER], slot[SLOT_PER];
+    double h[SLOT_PER][3];
+#pragma unroll
+    for (int i = 0; i < SLOT_PER; ++i) {
+        const int k = l + SLOT_LANES * i;
+        q[i] = -1;
+        slot[i] = -1;
+        h[i][0] = h[i][1] = h[i][2] = 0.0;
+        if (k >= cnt) continue;
+        q[i] = a.nbrList[p * NBR_CAP + k];
+        if (p > q[i]) {
+            slot[i] = p * NBR_CAP + k;
+            int last = -1;
+            for (int m = 0; m < oc; ++m)
+                if (oldList[p * NBR_CAP + m] == q[i]) last = m;
+            if (last >= 0) {
+                const int os = p * NBR_CAP + last;
+                h[i][0] = xi[3 * os]; h[i][1] = xi[3 * os + 1]; h[i][2] = xi[3 * os + 2];
             }
-        a.xiTmp[3 * slot] = h0; a.xiTmp[3 * slot + 1] = h1; a.xiTmp[3 * slot + 2] = h2;
-    } else {
-        const int qc = a.nbrCount[q];
-        int lo = 0, hi = qc - 1, pos = -1;
-        while (lo <= hi) {
-            const int mid = (lo + hi) >> 1;
-            const int v = a.nbrList[q * NBR_CAP + mid];
-            if (v == p) { pos = mid; break; }
-            if (v < p) lo = mid + 1; else hi = mid - 1;
+        } else {
+            const int qq = q[i], qc = a.nbrCount[qq];
+            int lo = 0, hi = qc - 1;
+            while (lo <= hi) {
+                const int mid = (lo + hi) >> 1;
+                const int v = a.nbrList[qq * NBR_CAP + mid];
+                if (v == p) { slot[i] = qq * NBR_CAP + mid; break; }
+                if (v < p) lo = mid + 1; else hi = mid - 1;
+            }
         }
-        slot = pos >= 0 ? q * NBR_CAP + pos : -1;
     }
-    a.nbrSlot[p * NBR_CAP + k] = slot;
-    }
-}
-// the new list and its carried history become the current ones, SLOT_LANES lanes per particle
-__device__ inline void k_nbr_commit(NbrArgs a, double* xiCurMut, int* oldCount, int* oldList) {
-    if (!a.sc->rebuild) return;
-    const int e = blockIdx.x * blockDim.x + threadIdx.x;
-    const int p = e / SLOT_LANES;
-    if (p >= a.p.nPart) return;
-    const int cnt = a.nbrCount[p];
-    if (e % SLOT_LANES == 0) oldCount[p] = cnt;
-    for (int k = e % SLOT_LANES; k < cnt; k += SLOT_LANES) {
+    // every lane's reads of the old entries are issued before any lane overwrites them
+    __builtin_amdgcn_fence(__ATOMIC_ACQ_REL, "wavefront");
+    __builtin_amdgcn_wave_barrier();
+    if (!valid) return;
+    if (l == 0) oldCount[p] = cnt;
+#pragma unroll
+    for (int i = 0; i < SLOT_PER; ++i) {
+        const int k = l + SLOT_LANES * i;
+        if (k >= cnt) continue;
         const int s = p * NBR_CAP + k;
-        xiCurMut[3 * s] = a.xiTmp[3 * s]; xiCurMut[3 * s + 1] = a.xiTmp[3 * s + 1]; xiCurMut[3 * s + 2] = a.xiTmp[3 * s + 2];
-        oldList[s] = a.nbrList[s];
+        a.nbrSlot[s] = slot[i];
+        oldList[s] = q[i];
+        if (p > q[i]) { xi[3 * s] = h[i][0]; xi[3 * s + 1] = h[i][1]; xi[3 * s + 2] = h[i][2]; }
     }
 }
 
@@ -819,7 +848,7 @@ static NbrArgs nbr_args(const DemParams& p, DemState& d, DevScalars* sc) {
     a.p = p; a.x0 = d.x0; a.r = d.r;
     a.cellCount = d.cellCount; a.cellStart = d.cellStart; a.cellFill = d.cellFill; a.cellItems = d.cellItems;
     a.cellKey = d.cellKey; a.nbrCount = d.nbrCount; a.nbrList = d.nbrList; a.nbrSlot = d.nbrSlot;
-    a.oldCount = d.nbrCountOld; a.oldList = d.nbrListOld; a.xiCur = d.xiCur; a.xiTmp = d.xiTmp; a.sc = sc;
+    a.sc = sc;
     return a;
 }
 
@@ -831,13 +860,12 @@ void dem_neighbor_list(const DemParams& p, DemState& d, DevScalars* sc, bool for
     if (p.nlx > 0 && p.nlz > 0) {
         const int ncell = p.nlx * p.nly * p.nlz;
         launch<&k_cells_assign, 1024, 1>(pb, 128, s, a);
-        launch<&k_cells_scan, 1024, 1>(1, 1024, s, a, ncell);
+        launch<&k_cells_scan, SCAN_THREADS, 1>(1, SCAN_THREADS, s, a, ncell);
         launch<&k_cells_fill, 1024, 1>(pb, 128, s, a);
     }
     launch<&k_nbr_build, NBR_BLOCK, 1>((p.nPart + NBR_GROUPS - 1) / NBR_GROUPS, NBR_BLOCK, s, a);
     const int eb = int((int64_t(p.nPart) * SLOT_LANES + 255) / 256);  // SLOT_LANES threads per particle
-    launch<&k_nbr_slots, 256, 1>(eb, 256, s, a);
-    launch<&k_nbr_commit, 256, 1>(eb, 256, s, a, d.xiCur, d.nbrCountOld, d.nbrListOld);
+    launch<&k_nbr_slots, 256, 1>(eb, 256, s, a, d.xiCur, d.nbrCountOld, d.nbrListOld);
 }
 
 bool dem_step(const DemParams& p, DemState& d, DevScalars* sc, int demTimeStep, hipStream_t s, hipEvent_t hydroReady) {

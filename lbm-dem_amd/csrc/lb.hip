@@ -28,6 +28,14 @@ __device__ inline bool inside_sphere(double px, double py, double pz, double cx,
 }
 
 __device__ inline void cell_xyz(const Geom& g, int64_t i, int& x, int& y, int& z) {
+    if ((uint64_t(i) | uint64_t(g.X) * uint64_t(g.Y)) <= 0xffffffffull) {  // 32-bit quotients
+        const uint32_t u = uint32_t(i), sxy = uint32_t(g.X) * uint32_t(g.Y);
+        const uint32_t zz = u / sxy, rem = u - zz * sxy, yy = rem / uint32_t(g.X);
+        z = int(zz);
+        y = int(yy);
+        x = int(rem - yy * uint32_t(g.X));
+        return;
+    }
     z = int(i / (int64_t(g.X) * g.Y));
     const int64_t rem = i - int64_t(z) * g.X * g.Y;
     y = int(rem / g.X);
@@ -316,7 +324,8 @@ __device__ inline void collide_cell(const FusedArgs<S>& a, int32_t sp, uint8_t t
         jx += C(cxq(q)) * gc[q]; jy += C(cyq(q)) * gc[q]; jz += C(czq(q)) * gc[q];
     }
     const C n = C(1) + sum;
-    const C ux = jx / n, uy = jy / n, uz = jz / n;
+    const C inv = C(1) / n;  // one division per node: the quotients below differ by rounding only
+    const C ux = jx * inv, uy = jy * inv, uz = jz * inv;
     C hx = 0, hy = 0, hz = 0;
     if (t & T_PMASK) {
         const PartLB P = a.plb[sp];
@@ -325,7 +334,7 @@ __device__ inline void collide_cell(const FusedArgs<S>& a, int32_t sp, uint8_t t
         const C lvx = C(P.v[0]) + (wy * rz - wz * ry);
         const C lvy = C(P.v[1]) + (wz * rx - wx * rz);
         const C lvz = C(P.v[2]) + (wx * ry - wy * rx);
-        const C lf = C(mi) / n;
+        const C lf = C(mi) * inv;
         const C dfx = lf * (ux - lvx), dfy = lf * (uy - lvy), dfz = lf * (uz - lvz);
         hx = -dfx; hy = -dfy; hz = -dfz;
         r.d[0] = dfx; r.d[1] = dfy; r.d[2] = dfz;
@@ -334,7 +343,7 @@ __device__ inline void collide_cell(const FusedArgs<S>& a, int32_t sp, uint8_t t
         r.key = sp;
     }
     const C Fx = C(k.lbF[0]) + hx, Fy = C(k.lbF[1]) + hy, Fz = C(k.lbF[2]) + hz;
-    const C sx = ux + C(0.5) * Fx / n, sy = uy + C(0.5) * Fy / n, sz = uz + C(0.5) * Fz / n;
+    const C sx = ux + C(0.5) * Fx * inv, sy = uy + C(0.5) * Fy * inv, sz = uz + C(0.5) * Fz * inv;
     const C usq = sx * sx + sy * sy + sz * sz;
     const C uF = sx * Fx + sy * Fy + sz * Fz;
     const C om = C(k.omega), omf = C(k.omegaf);
@@ -1155,13 +1164,13 @@ __device__ inline void k_fs_isolated_a(Geom g, uint8_t* type, typename Types<S>:
         } else {
             i = newIf[k - nIf];
         }
-        list_push(list2, count2, cap, int32_t(i), &sc->listOverflow);
         int x, y, z;
         cell_xyz(g, i, x, y, z);
         const Offs o(g, x, y, z);
         bool gasNb = false;  // the 18 type loads overlap (no early exit)
 #pragma unroll
         for (int j = 1; j < Q; ++j) gasNb |= (type[i + o.d(j)] & T_BASE) == T_GAS;
+        list_push(list2, count2, cap, int32_t(i), &sc->listOverflow);
         if (!gasNb) {
             acc += double(mass[i]) - double(aux[i]);
             mass[i] = typename Types<S>::M(aux[i]);
@@ -1253,79 +1262,123 @@ void lb_free_surface(Box& b, const Consts& k, DevScalars* sc, hipStream_t s) {
 // ------------------------------------------------------------------------------------------
 // particle flags (LB.cpp:683-711 via 1875-1899; findNewActive LB.cpp:1980-2029)
 // ------------------------------------------------------------------------------------------
-// Blocks over particles, each particle's bounding box: active nodes inside the sphere, highest
-// index wins. A node inside particle p and inside a higher-index particle q has |c_p - c_q| <
-// r_p + r_q, so q is on p's Verlet list, which this rebuild step built from positions one step
-// earlier with a skin of two length units (DEM.cpp:585-627). So p first gathers the higher-index
-// partners whose spheres overlap its own, then writes every node inside its sphere and inside none
-// of them with a plain store: each node gets exactly one writer, the highest index (the atomic
-// maximum the list-free path uses gives the same). Without a list (no link cell along z, where
-// the reference builds none) the atomic maximum is used.
-constexpr int FLAGS_GRID = 2048;  // grids of the flag passes (grid-stride; most steps return early)
+// One wavefront per particle, over the particle's bounding box: active nodes inside the sphere,
+// highest index wins. A node inside particle p and inside a higher-index particle q has
+// |c_p - c_q| < r_p + r_q, so q is on p's Verlet list, which this rebuild step built from positions
+// one step earlier with a skin of two length units (DEM.cpp:585-627). So p first gathers the
+// higher-index partners whose spheres overlap its own (one list entry per lane), then writes every
+// node inside its sphere and inside none of them with a plain store: each node gets exactly one
+// writer, the highest index (the atomic maximum the list-free path uses gives the same). Without a
+// list (no link cell along z, where the reference builds none) the atomic maximum is used.
+// Particles are taken in link-cell order when the rebuild sorted them (neighbouring particles
+// then share the type lines they read), in one contiguous share of that order per XCD (blocks
+// b, b + 8, ... run on one XCD, whose L2 then holds one region). A wavefront walks its box 64 nodes
+// per lane step, FLAGS_UNROLL steps at a time so that their type loads are in flight together; the
+// sphere test keeps the reference's operations (LB.cpp:705, myvector.cpp:171-175).
+constexpr int FLAGS_GRID = 2048;   // blocks of the flag pass (grid-stride; most steps return early)
+constexpr int FLAGS_WAVES = 4;     // wavefronts (particles at a time) per block
+constexpr int FLAGS_UNROLL = 6;
 __device__ inline void k_flags_compute(Geom g, const uint8_t* type, const PartLB* plb, int nPart, int32_t* flagNew,
-                                DevScalars* sc, const int* nbrCount, const int* nbrList, int useList) {
+                                DevScalars* sc, const int* nbrCount, const int* nbrList, int useList,
+                                const int* order) {
     // a coupling step that follows a coupling step without an LB step between them (the DEM
     // start-up repeat) finishes the previous findNewActive here (no other block touches kminP)
     if (blockIdx.x == 0 && threadIdx.x == 0) fna_finish(sc);
     if (!sc->newNeighborList) return;
     if (blockIdx.x == 0 && threadIdx.x == 0) sc->kminP = INT_MAX;  // lowest flagged index, from k_flags_apply
-    __shared__ double sq[NBR_CAP][4];  // centre and radius of overlapping higher-index partners
-    __shared__ int nq;
+    __shared__ double sq[FLAGS_WAVES][NBR_CAP][4];  // centre and radius of overlapping higher-index partners
+    const int wv = __builtin_amdgcn_readfirstlane(int(threadIdx.x) >> 6), lane = int(threadIdx.x) & 63;
+    double(*hq)[4] = sq[wv];
     const int dims[3] = {g.X, g.Y, g.Z};
-    for (int p = blockIdx.x; p < nPart; p += gridDim.x) {
+    // this wavefront's particles: positions k0, k0 + step, ... below k1 of the (sorted) order
+    int k0 = int(blockIdx.x) * FLAGS_WAVES + wv, k1 = nPart, step = int(gridDim.x) * FLAGS_WAVES;
+    if (gridDim.x % 8 == 0) {
+        const int share = (nPart + 7) / 8, xcd = int(blockIdx.x) % 8;
+        k0 = xcd * share + (int(blockIdx.x) / 8) * FLAGS_WAVES + wv;
+        k1 = min(nPart, (xcd + 1) * share);
+        step = int(gridDim.x / 8) * FLAGS_WAVES;
+    }
+    for (int k = k0; k < k1; k += step) {
+        const int p = order ? order[k] : k;
         const PartLB P = plb[p];
+        int nHigher = 0;
         if (useList) {
-            if (threadIdx.x == 0) nq = 0;
-            __syncthreads();
-            const int cnt = nbrCount[p];
-            for (int k = threadIdx.x; k < cnt; k += blockDim.x) {
-                const int q = nbrList[p * NBR_CAP + k];
-                if (q <= p) continue;
-                const PartLB Qp = plb[q];
-                const double dx = Qp.c[0] - P.c[0], dy = Qp.c[1] - P.c[1], dz = Qp.c[2] - P.c[2];
-                const double rs = P.r + Qp.r + 1e-9;  // a margin: a spare candidate only costs a test
-                if (dx * dx + dy * dy + dz * dz < rs * rs) {
-                    const int slot = atomicAdd(&nq, 1);
-                    sq[slot][0] = Qp.c[0]; sq[slot][1] = Qp.c[1]; sq[slot][2] = Qp.c[2]; sq[slot][3] = Qp.r;
+            const int cnt = nbrCount[p];  // at most NBR_CAP - 1 entries: one lane each
+            bool keep = false;
+            double qc[4] = {0.0, 0.0, 0.0, 0.0};
+            if (lane < cnt) {
+                const int q = nbrList[p * NBR_CAP + lane];
+                if (q > p) {
+                    const PartLB Qp = plb[q];
+                    const double dx = Qp.c[0] - P.c[0], dy = Qp.c[1] - P.c[1], dz = Qp.c[2] - P.c[2];
+                    const double rs = P.r + Qp.r + 1e-9;  // a margin: a spare candidate only costs a test
+                    keep = dx * dx + dy * dy + dz * dz < rs * rs;
+                    qc[0] = Qp.c[0]; qc[1] = Qp.c[1]; qc[2] = Qp.c[2]; qc[3] = Qp.r;
                 }
             }
-            __syncthreads();
+            const uint64_t m = __ballot(keep);
+            // the previous particle's reads of hq are done (LDS requests of a wavefront complete in order)
+            __builtin_amdgcn_fence(__ATOMIC_RELEASE, "wavefront");
+            __builtin_amdgcn_wave_barrier();
+            if (keep) {
+                const int slot = __popcll(m & ((uint64_t(1) << lane) - 1));
+                hq[slot][0] = qc[0]; hq[slot][1] = qc[1]; hq[slot][2] = qc[2]; hq[slot][3] = qc[3];
+            }
+            __builtin_amdgcn_fence(__ATOMIC_ACQ_REL, "wavefront");
+            __builtin_amdgcn_wave_barrier();
+            nHigher = __popcll(m);
         }
         int lo[3], hi[3];
-        for (int a = 0; a < 3; ++a) {
-            lo[a] = int(floor(P.c[a] - P.r)) - 1;
-            hi[a] = int(ceil(P.c[a] + P.r)) + 1;
-            lo[a] = lo[a] < 0 ? 0 : lo[a];
-            hi[a] = hi[a] > dims[a] - 1 ? dims[a] - 1 : hi[a];
+        for (int ax = 0; ax < 3; ++ax) {
+            lo[ax] = int(floor(P.c[ax] - P.r)) - 1;
+            hi[ax] = int(ceil(P.c[ax] + P.r)) + 1;
+            lo[ax] = lo[ax] < 0 ? 0 : lo[ax];
+            hi[ax] = hi[ax] > dims[ax] - 1 ? dims[ax] - 1 : hi[ax];
         }
-        const int nx = hi[0] - lo[0] + 1, ny = hi[1] - lo[1] + 1, nz = hi[2] - lo[2] + 1;
-        const int tot = (lo[0] > hi[0] || lo[1] > hi[1] || lo[2] > hi[2]) ? 0 : nx * ny * nz;
-        const int nHigher = useList ? nq : 0;
-        // the sphere tests first (arithmetic only): the type byte is loaded for the nodes kept
-        for (int c = threadIdx.x; c < tot; c += blockDim.x) {
-            const int x = lo[0] + c % nx, y = lo[1] + (c / nx) % ny, z = lo[2] + c / (nx * ny);
-            if (!inside_sphere(x, y, z, P.c[0], P.c[1], P.c[2], P.r)) continue;
-            if (useList) {
-                bool higher = false;
-                for (int j = 0; j < nHigher && !higher; ++j)
-                    higher = inside_sphere(x, y, z, sq[j][0], sq[j][1], sq[j][2], sq[j][3]);
-                if (higher) continue;
+        if (lo[0] > hi[0] || lo[1] > hi[1] || lo[2] > hi[2]) continue;
+        const int nx = hi[0] - lo[0] + 1, ny = hi[1] - lo[1] + 1, nxy = nx * ny, tot = nxy * (hi[2] - lo[2] + 1);
+        // node c = lane + 64 j of the box, as (x, y, z) offsets advanced by 64 nodes per step
+        const int sx = 64 % nx, sy = (64 / nx) % ny, sz = 64 / nxy;
+        int ox = lane % nx, oy = (lane / nx) % ny, oz = lane / nxy;
+        for (int c0 = 0; c0 < tot; c0 += 64 * FLAGS_UNROLL) {
+            int64_t idx[FLAGS_UNROLL];
+            bool in[FLAGS_UNROLL];
+            uint8_t t[FLAGS_UNROLL];
+#pragma unroll
+            for (int u = 0; u < FLAGS_UNROLL; ++u) {
+                const int x = lo[0] + ox, y = lo[1] + oy, z = lo[2] + oz;
+                const bool ok = c0 + 64 * u + lane < tot;
+                idx[u] = cidx(g, ok ? x : lo[0], ok ? y : lo[1], ok ? z : lo[2]);
+                t[u] = type[idx[u]];
+                in[u] = ok && inside_sphere(x, y, z, P.c[0], P.c[1], P.c[2], P.r);
+                if (in[u] && useList)
+                    for (int j = 0; j < nHigher && in[u]; ++j)
+                        in[u] = !inside_sphere(x, y, z, hq[j][0], hq[j][1], hq[j][2], hq[j][3]);
+                ox += sx;
+                const int cy = ox >= nx ? 1 : 0;
+                ox -= cy * nx;
+                oy += sy + cy;
+                const int cz = oy >= ny ? 1 : 0;
+                oy -= cz * ny;
+                oz += sz + cz;
             }
-            const int64_t i = cidx(g, x, y, z);
-            if (!is_active(type[i])) continue;
-            if (useList) flagNew[i] = p;
-            else atomicMax(&flagNew[i], p);
+#pragma unroll
+            for (int u = 0; u < FLAGS_UNROLL; ++u) {
+                if (!in[u] || !is_active(t[u])) continue;
+                if (useList) flagNew[idx[u]] = p;
+                else atomicMax(&flagNew[idx[u]], p);
+            }
         }
-        if (useList) __syncthreads();  // sq is refilled for the next particle
     }
 }
 
 void lb_flags_rebuild_compute(const Box& b, const Consts& k, const PartLB* plb, DevScalars* sc, const int* nbrCount,
-                              const int* nbrList, bool useList, hipStream_t s) {
+                              const int* nbrList, bool useList, const int* order, hipStream_t s) {
     // grid-stride over the particles: most steps only test the flag and return
     if (k.nPart > 0)
-        launch_gs<&k_flags_compute, 1024, 1, 16384>(std::min(k.nPart, FLAGS_GRID), 256, s, b.g, b.type, plb, k.nPart,
-                                                    b.flagNew, sc, nbrCount, nbrList, useList ? 1 : 0);
+        launch_gs<&k_flags_compute, 64 * FLAGS_WAVES, 8, 16384>(
+            std::min((k.nPart + FLAGS_WAVES - 1) / FLAGS_WAVES, FLAGS_GRID), 64 * FLAGS_WAVES, s, b.g, b.type, plb,
+            k.nPart, b.flagNew, sc, nbrCount, nbrList, useList ? 1 : 0, order);
 }
 
 // clear every flag and solid index, apply the new flags (initializeParticleBoundaries)

@@ -205,8 +205,9 @@ void lb_hydro_finalize(const Box& b, const Consts& k, double* FHydro, double* MH
 void lb_free_surface(Box& b, const Consts& k, DevScalars* sc, hipStream_t s);
 // particle flags of a neighbour-list rebuild; with useList the Verlet lists of that rebuild
 // resolve overlapping spheres (highest index wins) without atomics
+// order: the particles sorted by link cell (the rebuild's cell list), or null
 void lb_flags_rebuild_compute(const Box& b, const Consts& k, const PartLB* plb, DevScalars* sc, const int* nbrCount,
-                              const int* nbrList, bool useList, hipStream_t s);
+                              const int* nbrList, bool useList, const int* order, hipStream_t s);
 constexpr int PART_STRIDE_ = 32;
 void lb_flags_rebuild_apply(const Box& b, DevScalars* sc, hipStream_t s);
 void lb_find_new_active(const Box& b, const Consts& k, const PartLB* plb, DevScalars* sc, hipStream_t s);
@@ -248,7 +249,6 @@ struct DemState {
     int* nbrListOld;
     double* xiCur;             // tangential history per slot (3 per slot), read by the force kernel
     double* xiNext;            // written by the force kernel (buffers swap every step)
-    double* xiTmp;             // remap target during a rebuild
     int* cellCount;
     int* cellStart;
     int* cellFill;
