@@ -119,7 +119,29 @@ int lbdem_set_particles(lbdem_ctx* ctx, int n, const int* size, const double* r,
 /* objects.dat rows (DEM.cpp:138-165) after scaling: r, x0 (3n), x1 (3n). */
 int lbdem_set_objects(lbdem_ctx* ctx, int n, const double* r, const double* x0, const double* x1);
 
+/* Fluid restart from a fluidRecycle file (LB::latticeBolzmannInit with lbRestart, LB.cpp:352-367;
+ * restartInterface LB.cpp:994-1009; restartVariables LB.cpp:1098-1138), between lbdem_set_objects
+ * and lbdem_init; the host parses the file (IO.cpp:973-997 writes it). size: the lattice size the
+ * file states (must equal the box's); types: one type code per node (index x + X (y + Y z)),
+ * taken at every node whose initial type is fluid (codes 0, 2 and 3 are accepted there); records:
+ * LBDEM_RESTART_REC values per record, n - 1, u (3), mass - 1, visc, f (19), consumed in node
+ * index order by the active nodes of the restarted box (extra records are ignored, as the
+ * reference ignores them). Every record's visc must equal the run's viscosity (initVisc in
+ * lattice units): the library relaxes every node with that one viscosity. */
+#define LBDEM_RESTART_REC 25
+int lbdem_set_restart_fluid(lbdem_ctx* ctx, const int size[3], const uint8_t* types, int nRecords,
+                            const double* records);
+
 int lbdem_init(lbdem_ctx* ctx);
+/* Velocity and viscosity of every active node, as IO reads node->u and node->visc on an output
+ * step (IO.cpp:973-997): the force-shifted velocity of the node's last collision (node.cpp:95-98,
+ * LB.cpp:1310-1330), or the one the free surface gave a new interface node (LB.cpp:1705-1725).
+ * The LB step after lbdem_store_velocity stores it for every active node (other steps keep it for
+ * interface nodes only); lbdem_get_velocity returns u, 3 N values (x, y, z per node), and visc,
+ * N values (0 where not active; either may be NULL), while no other LB step has run since (after
+ * lbdem_init, the initial velocities), LBDEM_E_STATE otherwise. Not for contexts in a batch. */
+int lbdem_store_velocity(lbdem_ctx* ctx);
+int lbdem_get_velocity(lbdem_ctx* ctx, double* u, double* visc);
 int lbdem_dem_step(lbdem_ctx* ctx);
 int lbdem_coupling_step(lbdem_ctx* ctx);
 int lbdem_lb_step(lbdem_ctx* ctx);

@@ -104,6 +104,15 @@ struct lbdem_ctx {
     unsigned lbTime = 0, demTimeStep = 0, currentTimeStep = 0;
     double demTime = 0, realTime = 0, deltat = 1.0, totalMass = 0;
 
+    // fluid restart (lbdem_set_restart_fluid): the file's types and records, used by lbdem_init
+    bool restart = false;
+    std::vector<uint8_t> rTypes;
+    std::vector<double> rRec;
+    int rCount = 0;
+    // node velocities (lbdem_store_velocity / lbdem_get_velocity): 0 not held, 1 the next LB step
+    // stores them for every active node, 2 held (stored by the last LB step or by lbdem_init)
+    int velState = 0;
+
     const char* errFixed = nullptr;  // a message with static storage (set without allocating)
     int fail(int code, const std::string& m) { err = m; errFixed = nullptr; return code; }
     int fail_fixed(int code, const char* m) { errFixed = m; return code; }
@@ -237,11 +246,33 @@ extern "C" int lbdem_set_objects(lbdem_ctx* c, int n, const double* r, const dou
     return 0;
 }
 
+// LB::latticeBolzmannInit's restart branch (LB.cpp:352-367): the file's size must be the
+// lattice's (the reference asserts it); the types and records are kept for lbdem_init
+extern "C" int lbdem_set_restart_fluid(lbdem_ctx* c, const int size[3], const uint8_t* types, int nRecords,
+                                       const double* records) {
+    if (!c || !size || !types || nRecords < 0 || (nRecords > 0 && !records)) return LBDEM_E_ARG;
+    if (c->initialised) return c->fail(LBDEM_E_STATE, "the restart must be set before lbdem_init");
+    for (int a = 0; a < 3; ++a)
+        if (size[a] != c->lbSize[a]) return c->fail(LBDEM_E_ARG, "the restart file's lattice size differs from the box's");
+    const size_t n = size_t(c->lbSize[0]) * size_t(c->lbSize[1]) * size_t(c->lbSize[2]);
+    c->rTypes.assign(types, types + n);
+    c->rRec.assign(records, records + size_t(nRecords) * LBDEM_RESTART_REC);
+    c->rCount = nRecords;
+    c->restart = true;
+    return 0;
+}
+
 // initial node types on the host: lattice boundaries (LB.cpp:538-564), DEM walls
 // (LB.cpp:713-748), objects (LB.cpp:751-795), gas above the fluid height (LB.cpp:974-992) and
 // interface nodes next to gas (LB.cpp:1058-1068)
-static void host_types(lbdem_ctx* c, const Geom& g, std::vector<uint8_t>& t, std::vector<int32_t>& s,
-                       const std::vector<std::array<double, 6>>& walls) {
+// initial node types (initializeLatticeBoundaries, initializeTypes, initializeInterface;
+// LB.cpp:340-398) and the impactor node list (initializeObjectBoundaries, LB.cpp:751-795). With a
+// restart the file's type replaces every fluid node's (restartInterface, LB.cpp:994-1009) instead
+// of the free-surface initialisation: fluid, gas, interface, or an impactor node (setObjectWall,
+// node.cpp:405), which keeps solid index 0 and, not on the impactor list, is never uncovered (the
+// file holds the impactor where it was when written). Returns false for any other type there.
+static bool host_types(lbdem_ctx* c, const Geom& g, std::vector<uint8_t>& t, std::vector<int32_t>& s,
+                       const std::vector<std::array<double, 6>>& walls, std::vector<int32_t>& objList) {
     const int X = g.X, Y = g.Y, Z = g.Z;
     t.assign(g.N, T_FLUID);
     s.assign(g.N, 0);
@@ -276,23 +307,34 @@ static void host_types(lbdem_ctx* c, const Geom& g, std::vector<uint8_t>& t, std
                     if (dx * dx + dy * dy + dz * dz < r * r) { const int64_t i = cidx(g, x, y, z); s[i] = int32_t(o); t[i] = T_OBJ; }
                 }
     }
-    if (c->freeSurface) {
-        const double h0 = c->fluidHeight / c->unit.Length;
+    objList.clear();
+    for (int64_t i = 0; i < g.N; ++i) if (t[i] == T_OBJ) objList.push_back(int32_t(i));
+    if (c->restart) {
+        for (int64_t i = 0; i < g.N; ++i) {
+            if (t[i] != T_FLUID) continue;
+            const uint8_t v = c->rTypes[size_t(i)];
+            if (v != T_FLUID && v != T_GAS && v != T_IF && v != T_OBJ) return false;
+            t[i] = v;
+        }
+    } else {
+        if (c->freeSurface) {
+            const double h0 = c->fluidHeight / c->unit.Length;
+            for (int z = 0; z < Z; ++z)
+                for (int y = 0; y < Y; ++y)
+                    for (int x = 0; x < X; ++x) {
+                        const int64_t i = cidx(g, x, y, z);
+                        if (unsigned(y) > h0 && t[i] == T_FLUID) t[i] = T_GAS;
+                    }
+        }
         for (int z = 0; z < Z; ++z)
             for (int y = 0; y < Y; ++y)
                 for (int x = 0; x < X; ++x) {
                     const int64_t i = cidx(g, x, y, z);
-                    if (unsigned(y) > h0 && t[i] == T_FLUID) t[i] = T_GAS;
+                    if (t[i] != T_FLUID) continue;
+                    for (int q = 1; q < Q; ++q)
+                        if (t[nbr_wrapped(g, x, y, z, q)] == T_GAS) { t[i] = T_IF; break; }
                 }
     }
-    for (int z = 0; z < Z; ++z)
-        for (int y = 0; y < Y; ++y)
-            for (int x = 0; x < X; ++x) {
-                const int64_t i = cidx(g, x, y, z);
-                if (t[i] != T_FLUID) continue;
-                for (int q = 1; q < Q; ++q)
-                    if (t[nbr_wrapped(g, x, y, z, q)] == T_GAS) { t[i] = T_IF; break; }
-            }
     // wall nodes carry the wall that receives their link forces (LB.cpp:1416-1420)
     const int nw = int(walls.size());
     for (int64_t i = 0; i < g.N; ++i)
@@ -300,6 +342,7 @@ static void host_types(lbdem_ctx* c, const Geom& g, std::vector<uint8_t>& t, std
             const int w = (s[i] >= 0 && s[i] < nw) ? s[i] : 0;
             t[i] = uint8_t(T_WALL | (w << T_WSHIFT));
         }
+    return true;
 }
 
 // ---------------------------------------------------------------------------------------------
@@ -578,8 +621,23 @@ extern "C" int lbdem_init(lbdem_ctx* c) {
     // ---------------- lattice ----------------
     const Geom g = plan.g;
     std::vector<uint8_t> t;
-    std::vector<int32_t> s;
-    host_types(c, g, t, s, walls);
+    std::vector<int32_t> s, objList;
+    if (c->restart && (c->rTypes.size() != size_t(g.N)))
+        return c->fail(LBDEM_E_ARG, "the restart file's lattice size differs from the box's");
+    if (!host_types(c, g, t, s, walls, objList))
+        return c->fail(LBDEM_E_ARG, "the restart file puts a type other than fluid, gas, interface or impactor at a fluid node");
+    // restartVariables (LB.cpp:1098-1138): one record per active node, in node index order
+    std::vector<int32_t> rNode;
+    if (c->restart) {
+        for (int64_t i = 0; i < g.N; ++i)
+            if (t[i] == T_FLUID || t[i] == T_IF) rNode.push_back(int32_t(i));
+        if (int64_t(rNode.size()) > c->rCount)
+            return c->fail(LBDEM_E_ARG, "the restart file holds " + std::to_string(c->rCount) + " records for " +
+                                            std::to_string(rNode.size()) + " active nodes");
+        for (size_t r = 0; r < rNode.size(); ++r)
+            if (c->rRec[r * LBDEM_RESTART_REC + 5] != c->initDynVisc)
+                return c->fail(LBDEM_E_ARG, "a restart record's viscosity differs from the run's (initVisc)");
+    }
 
     Box& b = c->box;
     b.g = g;
@@ -608,8 +666,6 @@ extern "C" int lbdem_init(lbdem_ctx* c) {
     HIPCHK(hipMemsetAsync(b.flagNew, 0xff, g.N * 4, st));
     HIPCHK(hipMemcpyAsync(b.type, t.data(), g.N, hipMemcpyHostToDevice, st));
     HIPCHK(hipMemcpyAsync(b.solid, s.data(), g.N * 4, hipMemcpyHostToDevice, st));
-    std::vector<int32_t> objList;
-    for (int64_t i = 0; i < g.N; ++i) if (t[i] == T_OBJ) objList.push_back(int32_t(i));
     if (int64_t(objList.size()) > b.objCap) return c->fail(LBDEM_E_CAPACITY, "object node list capacity");
     if (!objList.empty()) HIPCHK(hipMemcpyAsync(b.objList, objList.data(), objList.size() * 4, hipMemcpyHostToDevice, st));
     DevScalars hs;
@@ -673,9 +729,32 @@ extern "C" int lbdem_init(lbdem_ctx* c) {
     c->xiFirst = d.xiCur;
     dem_refresh_views(P, d, st);
     dem_neighbor_list(P, d, c->sc, true, st);   // DEM.cpp:294
-    lb_init_fields(b, k, st);
+    if (c->restart) {
+        const size_t nr = rNode.size();
+        double* dRec = nullptr;
+        int32_t* dNode = nullptr;
+        if (nr > 0) {
+            if (hipMalloc(&dRec, nr * LBDEM_RESTART_REC * sizeof(double)) != hipSuccess ||
+                hipMalloc(&dNode, nr * sizeof(int32_t)) != hipSuccess) {
+                if (dRec) (void)hipFree(dRec);
+                return c->fail(LBDEM_E_NOMEM, "device allocation of the restart records failed");
+            }
+            HIPCHK(hipMemcpyAsync(dRec, c->rRec.data(), nr * LBDEM_RESTART_REC * sizeof(double), hipMemcpyHostToDevice, st));
+            HIPCHK(hipMemcpyAsync(dNode, rNode.data(), nr * sizeof(int32_t), hipMemcpyHostToDevice, st));
+            lb_restart_fields(b, dRec, dNode, int(nr), st);
+        }
+        HIPCHK(hipGetLastError());
+        HIPCHK(hipStreamSynchronize(st));
+        if (dRec) (void)hipFree(dRec);
+        if (dNode) (void)hipFree(dNode);
+        c->rTypes.clear(); c->rTypes.shrink_to_fit();
+        c->rRec.clear(); c->rRec.shrink_to_fit();
+    } else {
+        lb_init_fields(b, k, st);
+    }
     HIPCHK(hipGetLastError());
     HIPCHK(hipStreamSynchronize(st));
+    c->velState = 2;  // the initial (or restarted) velocity of every active node is in aux
     // total mass (LB.cpp:388-395): no node is flagged yet, all active nodes count
     double total = 0.0;
     {
@@ -776,7 +855,9 @@ static void enqueue_lb(lbdem_ctx* c, bool finalize = true) {  // LB::latticeBolz
         }
     }
     lb_mark_edges(c->box, c->sc, c->stream);
-    lb_fused_step(c->box, c->k, c->ds.plb, c->ds.olb, c->sc, true, c->stream, e0, e1);
+    const bool storeVel = c->velState == 1;  // lbdem_store_velocity asked for this step
+    c->velState = storeVel ? 2 : 0;
+    lb_fused_step(c->box, c->k, c->ds.plb, c->ds.olb, c->sc, true, storeVel, c->stream, e0, e1);
     if (finalize) enqueue_hydro(c, c->stream);
 }
 static void enqueue_fs(lbdem_ctx* c, hipStream_t s) { lb_free_surface(c->box, c->k, c->sc, s); }
@@ -858,6 +939,7 @@ static bool overlap_fs(const lbdem_ctx* c) { return c->freeSurface && c->prm.dem
 // data-dependent branch is a device flag, so the pair can be captured once and replayed.
 static bool steady_pairs(const lbdem_ctx* c) {
     if (!c->useGraphs || c->graphFailed || c->profiling || !c->tuned) return false;
+    if (c->velState == 1) return false;  // the next LB step stores every node's velocity
     if (c->prm.demSolve && c->demTimeStep < unsigned(std::max(c->prm.objWaitTime, 0)) + 1u) return false;
     if (!(c->demTime >= c->prm.demInitialRepeat)) return false;
     if (!c->forceField && !c->lbFReset) return false;
@@ -925,6 +1007,7 @@ static int capture_pair(lbdem_ctx* c) {
 // what two enqueue_step calls do to the host counters (AA parity and history buffers return to
 // their values after a pair)
 static void advance_pair(lbdem_ctx* c) {
+    c->velState = 0;  // two LB steps without the velocity store
     for (int k = 0; k < 2; ++k) {
         c->realTime += c->unit.Time;
         ++c->currentTimeStep;
@@ -1078,6 +1161,37 @@ extern "C" int lbdem_get_types(lbdem_ctx* c, uint8_t* type, uint8_t* pflag, int3
     for (int64_t i = 0; i < N; ++i) {
         if (type) type[i] = t[i] & T_BASE;
         if (pflag) pflag[i] = (t[i] & T_PMASK) ? 1 : 0;
+    }
+    return 0;
+}
+
+extern "C" int lbdem_store_velocity(lbdem_ctx* c) {
+    NEED_INIT;
+    if (c->batch) return c->fail(LBDEM_E_STATE, "the context is advanced through its batch");
+    c->velState = 1;
+    return 0;
+}
+
+extern "C" int lbdem_get_velocity(lbdem_ctx* c, double* u, double* visc) {
+    NEED_INIT;
+    if (c->velState != 2)
+        return c->fail(LBDEM_E_STATE, "no velocity held: call lbdem_store_velocity before the LB step to read");
+    const int64_t N = c->box.g.N;
+    if (u) {
+        double* d = c->tmp(size_t(3) * N * 8);
+        if (!d) return c->fail(LBDEM_E_NOMEM, "device allocation failed");
+        lb_download_velocity(c->box, d, c->stream);
+        HIPCHK(hipMemcpyAsync(u, d, size_t(3) * N * 8, hipMemcpyDeviceToHost, c->stream));
+        HIPCHK(hipStreamSynchronize(c->stream));
+    }
+    if (visc) {  // every active node relaxes with the run's viscosity
+        std::vector<uint8_t> t(N);
+        HIPCHK(hipMemcpyAsync(t.data(), c->box.type, N, hipMemcpyDeviceToHost, c->stream));
+        HIPCHK(hipStreamSynchronize(c->stream));
+        for (int64_t i = 0; i < N; ++i) {
+            const uint8_t b = t[i] & T_BASE;
+            visc[i] = (b == T_FLUID || b == T_IF) ? c->k.visc : 0.0;
+        }
     }
     return 0;
 }
@@ -1314,6 +1428,7 @@ struct BatchPhase {
     bool ownTable = false;
     hipGraph_t graph = nullptr;
     hipGraphExec_t exec = nullptr;
+    bool captureFailed = false;  // launched op by op from then on
 };
 }  // namespace
 
@@ -1483,7 +1598,7 @@ static void batch_launch_ops(lbdem_batch* B, const BatchPhase& P) {
 // the recorded pair captured into a graph once (the ops are launched directly when capture is
 // unavailable)
 static void batch_prepare_pair(lbdem_batch* B, BatchPhase& P) {
-    if (!P.exec && !P.graph) {
+    if (!P.exec && !P.graph && !P.captureFailed) {
         if (hipStreamBeginCapture(B->stream, hipStreamCaptureModeThreadLocal) == hipSuccess) {
             batch_launch_ops(B, P);
             hipGraph_t g = nullptr;
@@ -1495,6 +1610,7 @@ static void batch_prepare_pair(lbdem_batch* B, BatchPhase& P) {
                 P.exec = nullptr;
             }
         }
+        P.captureFailed = !P.exec;
         (void)hipGetLastError();
     }
 }
@@ -1562,6 +1678,7 @@ extern "C" int lbdem_batch_create(lbdem_ctx* const* ctxs, int n, lbdem_batch** o
         drop_graphs(c);  // captured with the context's own tile
         c->box.fz = fz;
         c->batch = B;
+        c->velState = 0;  // batched steps keep no velocity
     }
     *out = B;
     return 0;

@@ -122,11 +122,51 @@ __device__ inline void k_init_fields(Geom g, S* f, const uint8_t* type, typename
         StoreT<S>::st(f + int64_t(q) * g.N + i, C(feq - wq(q)));
     }
     mass[i] = typename Types<S>::M(t == T_FLUID ? k.rho0 : 0.5 * k.rho0);
-    if (t == T_IF) {
-        aux[i] = C(n);
-        aux[g.N + i] = C(ux); aux[2 * g.N + i] = C(uy); aux[3 * g.N + i] = C(uz);
-        aux[4 * g.N + i] = C(0); aux[5 * g.N + i] = C(0); aux[6 * g.N + i] = C(0);
+    // density and velocity of every active node (the interface nodes' for the free surface, all
+    // of them for a velocity read-back before the first LB step)
+    aux[i] = C(n);
+    aux[g.N + i] = C(ux); aux[2 * g.N + i] = C(uy); aux[3 * g.N + i] = C(uz);
+    aux[4 * g.N + i] = C(0); aux[5 * g.N + i] = C(0); aux[6 * g.N + i] = C(0);
+}
+
+// restartVariables (LB.cpp:1098-1138) for record r of the restart file, at active node node[r]:
+// populations f = restF, mass = restMass + 1, density n = restN + 1 and velocity u (node::restart,
+// node.cpp:38-51, with initDensity 1); hydrodynamic force zero (a fresh node)
+template <typename S>
+__device__ inline void k_restart_fields(Geom g, S* f, typename Types<S>::M* mass, typename Types<S>::C* aux,
+                                        const double* rec, const int32_t* node, int nrec) {
+    using C = typename Types<S>::C;
+    const int r = int(blockIdx.x * blockDim.x + threadIdx.x);
+    if (r >= nrec) return;
+    const int64_t i = node[r];
+    const double* v = rec + int64_t(r) * LBDEM_RESTART_REC;
+    for (int q = 0; q < Q; ++q) StoreT<S>::st(f + int64_t(q) * g.N + i, C(v[6 + q] - wq(q)));
+    mass[i] = typename Types<S>::M(v[4] + 1.0);
+    aux[i] = C(v[0] + 1.0);
+    aux[g.N + i] = C(v[1]); aux[2 * g.N + i] = C(v[2]); aux[3 * g.N + i] = C(v[3]);
+    aux[4 * g.N + i] = C(0); aux[5 * g.N + i] = C(0); aux[6 * g.N + i] = C(0);
+}
+void lb_restart_fields(const Box& b, const double* rec, const int32_t* node, int nrec, hipStream_t s) {
+    const int bs = 256, nb = (nrec + bs - 1) / bs;
+    if (b.prec == LBDEM_PREC_FP64)
+        launch<&k_restart_fields<double>, 1024, 1>(nb, bs, s, b.g, (double*)b.f, (double*)b.mass, (double*)b.aux, rec, node, nrec);
+    else if (b.prec == LBDEM_PREC_FP32)
+        launch<&k_restart_fields<float>, 1024, 1>(nb, bs, s, b.g, (float*)b.f, (float*)b.mass, (float*)b.aux, rec, node, nrec);
+    else
+        launch<&k_restart_fields<__half>, 1024, 1>(nb, bs, s, b.g, (__half*)b.f, (float*)b.mass, (float*)b.aux, rec, node, nrec);
+}
+
+// velocity of every active node (aux planes 1-3), x y z per node, zero elsewhere
+template <typename C>
+__device__ inline void k_download_vel(int64_t N, const uint8_t* type, const C* aux, double* out) {
+    for (int64_t i = int64_t(blockIdx.x) * blockDim.x + threadIdx.x; i < N; i += int64_t(gridDim.x) * blockDim.x) {
+        const bool act = is_active(type[i]);
+        for (int a = 0; a < 3; ++a) out[3 * i + a] = act ? double(aux[(1 + a) * N + i]) : 0.0;
     }
+}
+void lb_download_velocity(const Box& b, double* out, hipStream_t s) {
+    if (b.prec == LBDEM_PREC_FP64) launch<&k_download_vel<double>, 1024, 1>(4096, 256, s, b.g.N, b.type, (const double*)b.aux, out);
+    else launch<&k_download_vel<float>, 1024, 1>(4096, 256, s, b.g.N, b.type, (const float*)b.aux, out);
 }
 
 void lb_init_fields(const Box& b, const Consts& k, hipStream_t s) {
@@ -172,6 +212,7 @@ struct FusedArgs {
     DevScalars* sc;
     int nseg, nby, nbz;  // 64-node segments per row (tiles per row with 2-byte storage), row groups, z-slabs
     int gridBlocks;  // the box's grid (a batch launch covers the largest box; the rest return)
+    int storeVel;    // every active node's density and velocity into aux (else interface nodes only)
     Consts k;
 };
 
@@ -186,12 +227,15 @@ __device__ inline typename Types<S>::C object_value(const FusedArgs<S>& a, int j
     const int nObj = a.k.nObj;
     int yx, yy, yz;
     cell_xyz(a.g, yj, yx, yy, yz);
-    int ov = 0;
-    if (nObj > 1)
-        for (int oo = 0; oo < nObj; ++oo)
-            if (inside_sphere(yx, yy, yz, sObj[oo].c[0], sObj[oo].c[1], sObj[oo].c[2], sObj[oo].r)) ov = oo;
-    const ObjLB& OV = sObj[ov];
-    const double cuo = OV.u[0] * cx_rt(j) + OV.u[1] * cy_rt(j) + OV.u[2] * cz_rt(j);
+    // the impactor node's velocity (LB.cpp:1440): updateObjectBoundaries (LB.cpp:822-837) gives
+    // every impactor node a fresh node each step and the velocity of the last impactor whose
+    // sphere holds it, or none (zero) when no sphere holds it (a node a restart file typed as
+    // impactor away from where the impactor now is)
+    int ov = -1;
+    for (int oo = 0; oo < nObj; ++oo)
+        if (inside_sphere(yx, yy, yz, sObj[oo].c[0], sObj[oo].c[1], sObj[oo].c[2], sObj[oo].r)) ov = oo;
+    const double cuo =
+        ov < 0 ? 0.0 : sObj[ov].u[0] * cx_rt(j) + sObj[ov].u[1] * cy_rt(j) + sObj[ov].u[2] * cz_rt(j);
     const C BBi = C(6.0) * n * C(w_rt(j)) * C(cuo);
     if (oi >= 0 && oi < nObj && oi < MAX_OBJ_PART) {
         const double fm = 2.0 * double(gj) - double(BBi);
@@ -612,7 +656,7 @@ __device__ inline void k_fused(FusedArgs<S> a) {
         if (FSURF) {
             const bool isIf = actc && (tc & T_BASE) == T_IF;
             if (actc && !isIf) a.mass[ic] = M(r.n);
-            if (isIf) {
+            if (isIf || (actc && a.storeVel)) {
                 a.aux[ic] = r.n;
                 a.aux[N + ic] = r.sx; a.aux[2 * N + ic] = r.sy; a.aux[3 * N + ic] = r.sz;
                 a.aux[4 * N + ic] = r.hx; a.aux[5 * N + ic] = r.hy; a.aux[6 * N + ic] = r.hz;
@@ -626,6 +670,10 @@ __device__ inline void k_fused(FusedArgs<S> a) {
                 base = __shfl(base, leader, 64);
                 if (isIf) sIf[base + __popcll(m & ((1ull << lane) - 1ull))] = ic;
             }
+        } else if (actc && a.storeVel) {
+            a.aux[ic] = r.n;
+            a.aux[N + ic] = r.sx; a.aux[2 * N + ic] = r.sy; a.aux[3 * N + ic] = r.sz;
+            a.aux[4 * N + ic] = r.hx; a.aux[5 * N + ic] = r.hy; a.aux[6 * N + ic] = r.hz;
         }
         // particle sums: the thread's z-walk usually stays in one sphere for several nodes, so the
         // contributions are summed in registers while the particle index is unchanged and added
@@ -795,7 +843,7 @@ int fused_grid_blocks(const Geom& g, int fz, int prec) {
 
 template <typename S>
 static void launch_fused(Box& b, const Consts& k, const PartLB* plb, const ObjLB* olb, DevScalars* sc, bool diag,
-                         hipStream_t s, hipEvent_t ev0, hipEvent_t ev1) {
+                         bool storeVel, hipStream_t s, hipEvent_t ev0, hipEvent_t ev1) {
     FusedArgs<S> a;
     a.g = b.g; a.f = (S*)b.f; a.type = b.type; a.solid = b.solid;
     for (int q = 0; q < Q; ++q) a.fq[q] = a.f + int64_t(q) * b.g.N;
@@ -807,6 +855,7 @@ static void launch_fused(Box& b, const Consts& k, const PartLB* plb, const ObjLB
     a.nseg = (b.g.X + FX - 1) / FX; a.nby = (b.g.Y + FY - 1) / FY; a.nbz = (b.g.Z + fz - 1) / fz;
     const int grid = fused_grid_blocks(b.g, fz, b.prec);
     a.gridBlocks = grid;
+    a.storeVel = storeVel ? 1 : 0;
     const bool fs = k.freeSurface != 0;
     if (ev0) (void)hipEventRecord(ev0, s);
     (void)diag;  // the wall diagnostics are always accumulated
@@ -849,11 +898,11 @@ __device__ inline void k_begin_lb(DevScalars* sc) {
 }
 
 void lb_fused_step(Box& b, const Consts& k, const PartLB* plb, const ObjLB* olb, DevScalars* sc, bool diag,
-                   hipStream_t s, hipEvent_t ev0, hipEvent_t ev1) {
+                   bool storeVel, hipStream_t s, hipEvent_t ev0, hipEvent_t ev1) {
     launch<&k_begin_lb, 1024, 1>(1, 1, s, sc);
-    if (b.prec == LBDEM_PREC_FP64) launch_fused<double>(b, k, plb, olb, sc, diag, s, ev0, ev1);
-    else if (b.prec == LBDEM_PREC_FP32) launch_fused<float>(b, k, plb, olb, sc, diag, s, ev0, ev1);
-    else launch_fused<__half>(b, k, plb, olb, sc, diag, s, ev0, ev1);
+    if (b.prec == LBDEM_PREC_FP64) launch_fused<double>(b, k, plb, olb, sc, diag, storeVel, s, ev0, ev1);
+    else if (b.prec == LBDEM_PREC_FP32) launch_fused<float>(b, k, plb, olb, sc, diag, storeVel, s, ev0, ev1);
+    else launch_fused<__half>(b, k, plb, olb, sc, diag, storeVel, s, ev0, ev1);
 }
 
 // deterministic sum of the flagged per-block partials, unit conversion of all hydrodynamic results

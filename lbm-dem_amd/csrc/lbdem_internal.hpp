@@ -197,9 +197,14 @@ struct Consts {
 
 // ---------------- launchers (lb.hip) ----------------
 void lb_init_fields(const Box& b, const Consts& k, hipStream_t s);
+// rec: LBDEM_RESTART_REC values per record, written at active node node[r]
+void lb_restart_fields(const Box& b, const double* rec, const int32_t* node, int nrec, hipStream_t s);
+// 3 N values: the velocity held in aux for every active node, zero elsewhere
+void lb_download_velocity(const Box& b, double* out, hipStream_t s);
 void lb_mark_edges(const Box& b, DevScalars* sc, hipStream_t s);
+// storeVel: every active node's density and velocity into aux (else the interface nodes' only)
 void lb_fused_step(Box& b, const Consts& k, const PartLB* plb, const ObjLB* olb, DevScalars* sc, bool diag,
-                   hipStream_t s, hipEvent_t ev0 = nullptr, hipEvent_t ev1 = nullptr);
+                   bool storeVel, hipStream_t s, hipEvent_t ev0 = nullptr, hipEvent_t ev1 = nullptr);
 void lb_hydro_finalize(const Box& b, const Consts& k, double* FHydro, double* MHydro, double* fluidVol,
                        DevScalars* sc, double* objFH, double* objMH, double* wallFH, bool diag, hipStream_t s);
 void lb_free_surface(Box& b, const Consts& k, DevScalars* sc, hipStream_t s);

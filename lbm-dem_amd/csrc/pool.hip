@@ -707,7 +707,15 @@ extern "C" int lbdem_pool_get_stats(lbdem_pool* P, lbdem_pool_stats* s) {
 
 extern "C" double lbdem_pool_clock(const lbdem_pool* P) { return P ? P->now() : 0.0; }
 
-extern "C" const char* lbdem_pool_last_error(const lbdem_pool* P) { return P ? P->err.c_str() : "null pool"; }
+// the message is copied under the pool's lock into storage of the calling thread (valid until the
+// thread's next call)
+extern "C" const char* lbdem_pool_last_error(const lbdem_pool* P) {
+    if (!P) return "null pool";
+    thread_local std::string copy;
+    std::lock_guard<std::mutex> g(const_cast<lbdem_pool*>(P)->mu);
+    copy = P->err;
+    return copy.c_str();
+}
 
 extern "C" int lbdem_pool_destroy(lbdem_pool* P) {
     if (!P) return 0;

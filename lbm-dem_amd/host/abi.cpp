@@ -38,6 +38,9 @@ Api Api::load(const std::string& path) {
     bind(a.handle, path, "lbdem_get_walls", a.get_walls);
     bind(a.handle, path, "lbdem_get_neighbors", a.get_neighbors);
     bind(a.handle, path, "lbdem_total_mass", a.total_mass);
+    bind(a.handle, path, "lbdem_set_restart_fluid", a.set_restart_fluid);
+    bind(a.handle, path, "lbdem_store_velocity", a.store_velocity);
+    bind(a.handle, path, "lbdem_get_velocity", a.get_velocity);
     return a;
 }
 

@@ -30,6 +30,9 @@ struct Api {
     int (*get_walls)(lbdem_ctx*, double*) = nullptr;
     int (*get_neighbors)(lbdem_ctx*, int, int*, int*) = nullptr;
     int (*total_mass)(lbdem_ctx*, double*) = nullptr;
+    int (*set_restart_fluid)(lbdem_ctx*, const int*, const uint8_t*, int, const double*) = nullptr;
+    int (*store_velocity)(lbdem_ctx*) = nullptr;
+    int (*get_velocity)(lbdem_ctx*, double*, double*) = nullptr;
 
     void* handle = nullptr;
     std::string path;

@@ -3,6 +3,7 @@
 #include <sys/stat.h>
 
 #include <cmath>
+#include <cstdio>
 #include <cstdlib>
 #include <fstream>
 #include <iostream>
@@ -54,6 +55,7 @@ void parseConfigFile(IO& io, DEM& dem, LB& lb, ConfigFile& cfg, CommandLine& cl)
     lb.lbmInitialRepeat = parse_value(cfg, cl, "lbmInitialRepeat", 0);
     io.maxTime = parse_value(cfg, cl, "maxTime", 0.0);
     io.screenExpTime = parse_value(cfg, cl, "screenExpTime", 0.0);
+    io.fluidRecycleExpTime = parse_value(cfg, cl, "fluidRecycleExpTime", 0.0);
     io.demSolve = parse_value(cfg, cl, "demSolve", 0);
     io.lbmSolve = parse_value(cfg, cl, "lbSolve", 0);
     lb.freeSurface = parse_value(cfg, cl, "freeSurfaceSolve", 0);
@@ -73,7 +75,7 @@ void parseConfigFile(IO& io, DEM& dem, LB& lb, ConfigFile& cfg, CommandLine& cl)
 // LB
 // ------------------------------------------------------------------------------------------
 void LB::latticeBoltzmannGet(ConfigFile& cfg, CommandLine& cl) {
-    // keys and defaults of LB.cpp:137-336 (Newtonian, no restart, no topography, no turbulence)
+    // keys and defaults of LB.cpp:137-336 (Newtonian, no topography, no turbulence)
     lbdem_params& p = c_.params;
     p.unitLength = parse_value(cfg, cl, "unitLength", 1.0);
     p.unitTime = parse_value(cfg, cl, "unitTime", 1.0);
@@ -82,11 +84,9 @@ void LB::latticeBoltzmannGet(ConfigFile& cfg, CommandLine& cl) {
     unitTime = p.unitTime;
     const char* ax[3] = {"X", "Y", "Z"};
     for (int a = 0; a < 3; ++a) p.lbSize[a] = parse_value(cfg, cl, std::string("lbSize") + ax[a], 0.0);
-    const std::string restart = parse_value(cfg, cl, "lbRestart", "false");
-    if (to_int(restart, 0)) {
-        std::cout << "ERROR: lbRestart is not supported" << std::endl;
-        std::exit(1);
-    }
+    lbRestart = to_int(parse_value(cfg, cl, "lbRestart", "false"), 0);
+    lbRestartFile = parse_value(cfg, cl, "lbRestartFile", "");
+    if (lbRestart) std::cout << "Computation from LB restart file " << lbRestartFile << std::endl;
     for (int a = 0; a < 3; ++a) p.initVelocity[a] = parse_value(cfg, cl, std::string("initVelocity") + ax[a], 0.0);
     for (int a = 0; a < 3; ++a) p.lbF[a] = parse_value(cfg, cl, std::string("lbF") + ax[a], 0.0);
     for (int b = 0; b < 6; ++b) p.boundary[b] = parse_value(cfg, cl, "boundary" + std::to_string(b), 1);
@@ -117,6 +117,32 @@ void LB::LBShow() const {
 }
 
 void LB::latticeBolzmannInit(wallList&, particleList&, objectList&) {
+    if (lbRestart && !c_.initialised()) {
+        // the fluidRecycle file (IO.cpp:973-997): lattice size, every node's type, then one
+        // record per active node (LB.cpp:352-367, 994-1009, 1098-1138 read it back)
+        std::ifstream in(lbRestartFile);
+        if (!in.is_open()) {
+            std::cout << "ERROR: Condition fluidFileID.is_open() not fulfilled!" << std::endl;
+            std::exit(1);
+        }
+        unsigned sz[3] = {0, 0, 0};
+        in >> sz[0] >> sz[1] >> sz[2];
+        const size_t nodes = size_t(sz[0]) * sz[1] * sz[2];
+        std::vector<uint8_t> types(nodes, 0);
+        for (size_t i = 0; i < nodes; ++i) {
+            unsigned v = 0;
+            in >> v;
+            types[i] = uint8_t(v);
+        }
+        std::vector<double> rec;
+        double v = 0.0;
+        while (in >> v) rec.push_back(v);
+        const int size[3] = {int(sz[0]), int(sz[1]), int(sz[2])};
+        c_.create();
+        c_.check(c_.api().set_restart_fluid(c_.ctx(), size, types.data(), int(rec.size() / LBDEM_RESTART_REC),
+                                            rec.data()),
+                 "lbdem_set_restart_fluid");
+    }
     c_.initialise();
     lbdem_info in;
     c_.check(c_.api().get_info(c_.ctx(), &in), "lbdem_get_info");
@@ -137,6 +163,11 @@ void LB::latticeBoltzmannCouplingStep(bool& newNeighborList, particleList&, obje
 void LB::latticeBoltzmannFreeSurfaceStep() {
     c_.initialise();
     c_.check(c_.api().fs_step(c_.ctx()), "lbdem_fs_step");
+}
+
+void LB::storeVelocity() {
+    c_.initialise();
+    c_.check(c_.api().store_velocity(c_.ctx()), "lbdem_store_velocity");
 }
 
 double LB::currentMass() const {
@@ -289,6 +320,17 @@ static void append_line(const std::string& path, const char* text) {
 }
 
 void IO::initialize() {
+    partDirectory = workDirectory + "/particleData";
+    fluidDirectory = workDirectory + "/fluidData";
+    if (demSolve) {
+        const int a = ::mkdir(partDirectory.c_str(), 0777);
+        std::cout << "Work directory created = " << partDirectory << ". Result: " << a << "\n";
+    }
+    if (lbmSolve) {
+        const int a = ::mkdir(fluidDirectory.c_str(), 0777);
+        std::cout << "Work directory created = " << fluidDirectory << ". Result: " << a << "\n";
+    }
+    fluidRecycleFileFormat = fluidDirectory + "/fluidRecycle%010u.dat";
     exportFileName = workDirectory + "/export.dat";
     if (demSolve) {
         objectPositionFileName = workDirectory + "/objPosition.dat";
@@ -310,27 +352,76 @@ void IO::initialize() {
         append_line(massFileName, "step time totalFluidMass\n");
     }
     lastScreenExp = 0;
+    lastFluidRecycleExp = 0;
 }
 
 void IO::outputStep(LB& lb, DEM& dem) {
     const unsigned screenExpCounter = screenExpTime > 0 ? unsigned(realTime / screenExpTime) + 1 : 0;
-    if (screenExpCounter <= lastScreenExp) return;
-    lastScreenExp = screenExpCounter;
-    std::ofstream exportFile(exportFileName, std::ios::app);
-    std::cout << currentTimeStep << "; time=" << realTime << "\t";
-    exportFile << currentTimeStep << "; time=" << realTime << "\t";
-    dem.refresh();
-    if (demSolve) {
-        if (!dem.objects.empty()) {
-            exportObjectPosition(dem.objects, lb);
-            exportObjectForce(dem.objects, lb);
+    if (screenExpCounter > lastScreenExp) {
+        lastScreenExp = screenExpCounter;
+        std::ofstream exportFile(exportFileName, std::ios::app);
+        std::cout << currentTimeStep << "; time=" << realTime << "\t";
+        exportFile << currentTimeStep << "; time=" << realTime << "\t";
+        dem.refresh();
+        if (demSolve) {
+            if (!dem.objects.empty()) {
+                exportObjectPosition(dem.objects, lb);
+                exportObjectForce(dem.objects, lb);
+            }
+            if (!dem.walls.empty()) exportWallForce(dem.walls);
         }
-        if (!dem.walls.empty()) exportWallForce(dem.walls);
+        if (lbmSolve && c_.initialised()) exportMass(lb);
+        std::cout << "\n";
+        exportFile << "\n";
+        std::cout.flush();
     }
-    if (lbmSolve && c_.initialised()) exportMass(lb);
-    std::cout << "\n";
-    exportFile << "\n";
-    std::cout.flush();
+    createFiles(lb, dem);
+}
+
+bool IO::fluidVelocityDue() const {
+    const unsigned counter = fluidRecycleExpTime > 0 ? unsigned(realTime / fluidRecycleExpTime) + 1 : 0;
+    return lbmSolve && counter > lastFluidRecycleExp;
+}
+
+void IO::createFiles(LB& lb, DEM&) {
+    if (!lbmSolve || !c_.initialised()) return;
+    const unsigned counter = fluidRecycleExpTime > 0 ? unsigned(realTime / fluidRecycleExpTime) + 1 : 0;
+    if (counter > lastFluidRecycleExp) {
+        lastFluidRecycleExp = counter;
+        char path[1024];
+        std::snprintf(path, sizeof(path), fluidRecycleFileFormat.c_str(), currentTimeStep);
+        exportRecycleFluid(lb, path);
+    }
+}
+
+// sizes, every node's type, then per active node (ascending index, as LB::cleanLists sorts
+// activeNodes): n - 1, u.x, u.y, u.z (u.dot of the unit vectors), mass - 1, visc, f[0..18], each
+// followed by a space, with the stream's default formatting (initDensity is 1, LB.cpp:330)
+void IO::exportRecycleFluid(const LB& lb, const std::string& file) {
+    Context& c = lb.context();
+    lbdem_info in;
+    c.check(c.api().get_info(c.ctx(), &in), "lbdem_get_info");
+    const int64_t N = in.totNodes;
+    std::vector<uint8_t> t(N);
+    std::vector<double> f(size_t(Q_DIRECTIONS) * N), mass(N), n(N), u(3 * size_t(N)), visc(N);
+    c.check(c.api().get_types(c.ctx(), t.data(), nullptr, nullptr), "lbdem_get_types");
+    c.check(c.api().get_fields(c.ctx(), f.data(), mass.data(), n.data()), "lbdem_get_fields");
+    c.check(c.api().get_velocity(c.ctx(), u.data(), visc.data()), "lbdem_get_velocity");
+    std::ofstream out(file);
+    out << lb.lbSize[0] << " " << lb.lbSize[1] << " " << lb.lbSize[2] << "\n";
+    for (int64_t i = 0; i < N; ++i) out << unsigned(t[i]) << " ";
+    out << std::endl;
+    std::cout << std::endl;
+    const double initDensity = 1.0;
+    const tVect Xp(1.0, 0.0, 0.0), Yp(0.0, 1.0, 0.0), Zp(0.0, 0.0, 1.0);
+    for (int64_t i = 0; i < N; ++i) {
+        if (t[i] != 0 && t[i] != 3) continue;
+        const tVect ui(u[3 * i], u[3 * i + 1], u[3 * i + 2]);
+        out << n[i] - initDensity << " " << ui.dot(Xp) << " " << ui.dot(Yp) << " " << ui.dot(Zp) << " "
+            << mass[i] - initDensity << " " << visc[i] << " ";
+        for (int j = 0; j < Q_DIRECTIONS; ++j) out << f[size_t(j) * N + i] << " ";
+        out << std::endl;
+    }
 }
 
 void IO::exportObjectPosition(const objectList& objects, const LB& lb) {
@@ -392,6 +483,7 @@ void goCycle(IO& io, DEM& dem, LB& lb) {
     if (io.demSolve) dem.discreteElementStep(boundary);
     if (io.lbmSolve && dem.demTime >= dem.demInitialRepeat) {
         if (io.demSolve) lb.latticeBoltzmannCouplingStep(dem.newNeighborList, dem.particles, dem.objects);
+        if (io.fluidVelocityDue()) lb.storeVelocity();
         lb.latticeBolzmannStep(dem.particles, dem.walls, dem.objects);
         if (lb.freeSurface) lb.latticeBoltzmannFreeSurfaceStep();
     } else if (io.lbmSolve && dem.demTime <= dem.demInitialRepeat) {

@@ -14,6 +14,8 @@
 
 namespace lbdem {
 
+constexpr int Q_DIRECTIONS = 19;  // D3Q19 (lattice.h)
+
 struct tVect {
     double x = 0.0, y = 0.0, z = 0.0;
     tVect() = default;
@@ -22,6 +24,7 @@ struct tVect {
     tVect operator-(const tVect& o) const { return {x - o.x, y - o.y, z - o.z}; }
     tVect operator*(double s) const { return {x * s, y * s, z * s}; }
     tVect operator/(double s) const { return {x / s, y / s, z / s}; }
+    double dot(const tVect& o) const { return x * o.x + y * o.y + z * o.z; }
     // "x y z " with the stream's current formatting (the reference's tVect::print)
     void print(std::ostream& os) const;
 };
@@ -89,9 +92,13 @@ public:
     void latticeBoltzmannFreeSurfaceStep();                                           // LB.cpp:422-430
     // total fluid mass over active nodes outside particles (LB.cpp:388-395)
     double currentMass() const;
+    // the next LB step keeps every active node's velocity for the output (IO reads node->u)
+    void storeVelocity();
 
     unsigned int time = 0;
     int freeSurface = 0, forceField = 0, lbmInitialRepeat = 0;
+    int lbRestart = 0;              // LB.cpp:138-139
+    std::string lbRestartFile;
     double fluidHeight = 0.0;   // physical, as configured
     double unitTime = 1.0, unitLength = 1.0;
     unsigned lbSize[3] = {0, 0, 0};
@@ -132,10 +139,13 @@ class IO {
 public:
     explicit IO(Context& c) : c_(c) {}
     void initialize();                             // IO.cpp:9-112 (the result files of the path)
-    void outputStep(LB& lb, DEM& dem);             // IO.cpp:113-168 (screen interval)
+    void outputStep(LB& lb, DEM& dem);             // IO.cpp:113-168 (screen interval, then createFiles)
+    // whether this step's createFiles writes a fluid file that needs node->u (IO.cpp:203-210)
+    bool fluidVelocityDue() const;
 
     std::string workDirectory = "data", configFileName = "config.cfg";
     double realTime = 0.0, maxTime = 0.0, screenExpTime = 0.0;
+    double fluidRecycleExpTime = 0.0;              // main.cpp:157
     unsigned int currentTimeStep = 0;
     int demSolve = 0, lbmSolve = 0;
 
@@ -144,9 +154,12 @@ private:
     void exportObjectForce(const objectList& objects, const LB& lb);      // IO.cpp:1239-1293
     void exportWallForce(const wallList& walls);                          // IO.cpp:1295-1314
     void exportMass(const LB& lb);                                        // step, time, total fluid mass
+    void createFiles(LB& lb, DEM& dem);                                   // IO.cpp:181-289 (fluid recycle)
+    void exportRecycleFluid(const LB& lb, const std::string& file);       // IO.cpp:973-997
     Context& c_;
-    unsigned lastScreenExp = 0;
+    unsigned lastScreenExp = 0, lastFluidRecycleExp = 0;
     std::string exportFileName, objectPositionFileName, objectForceFileName, wallForceFileName, massFileName;
+    std::string partDirectory, fluidDirectory, fluidRecycleFileFormat;
 };
 
 // parseConfigFile (main.cpp:128-185) and one goCycle (main.cpp:38-67)

@@ -148,6 +148,9 @@ class Library:
             "lbdem_last_error": ([vp], C.c_char_p),
             "lbdem_set_particles": ([vp, C.c_int, _ip, _dp, _dp], C.c_int),
             "lbdem_set_objects": ([vp, C.c_int, _dp, _dp, _dp], C.c_int),
+            "lbdem_set_restart_fluid": ([vp, _ip, C.c_void_p, C.c_int, _dp], C.c_int),
+            "lbdem_store_velocity": ([vp], C.c_int),
+            "lbdem_get_velocity": ([vp, C.c_void_p, C.c_void_p], C.c_int),
             "lbdem_init": ([vp], C.c_int),
             "lbdem_dem_step": ([vp], C.c_int),
             "lbdem_coupling_step": ([vp], C.c_int),
@@ -247,8 +250,28 @@ class Simulation:
         except Exception:
             pass
 
+    def set_restart(self, size, types: np.ndarray, records: np.ndarray):
+        """The contents of a fluidRecycle file (read_recycle) for lbdem_init to restart from."""
+        sz = np.ascontiguousarray(size, dtype=np.int32)
+        t = np.ascontiguousarray(types, dtype=np.uint8)
+        rec = np.ascontiguousarray(records, dtype=np.float64).reshape(-1, RESTART_REC)
+        self._check(self.lib.lib.lbdem_set_restart_fluid(self.ctx, _ptr(sz, C.c_int), t.ctypes.data, len(rec),
+                                                         _ptr(rec, C.c_double)), "lbdem_set_restart_fluid")
+
     def init(self):
         self._check(self.lib.lib.lbdem_init(self.ctx), "lbdem_init")
+
+    def store_velocity(self):
+        """The next LB step keeps every active node's velocity (read with velocity())."""
+        self._check(self.lib.lib.lbdem_store_velocity(self.ctx), "lbdem_store_velocity")
+
+    def velocity(self):
+        """(u as N x 3, viscosity as N) of every active node, zero elsewhere."""
+        N = self.info()["totNodes"]
+        u = np.zeros(3 * N, np.float64)
+        v = np.zeros(N, np.float64)
+        self._check(self.lib.lib.lbdem_get_velocity(self.ctx, u.ctypes.data, v.ctypes.data), "lbdem_get_velocity")
+        return u.reshape(N, 3), v
 
     def dem_step(self):
         self._check(self.lib.lib.lbdem_dem_step(self.ctx), "lbdem_dem_step")
@@ -663,6 +686,23 @@ def write_objects(path: str, r: Sequence[float], x0: np.ndarray, x1: np.ndarray)
         for i in range(len(r)):
             a = [float(c) for c in (*x0[i], *x1[i])]
             fh.write(f"{i} {float(r[i])!r} " + " ".join(repr(c) for c in a) + "\n")
+
+
+RESTART_REC = 25  # include/lbdem.h LBDEM_RESTART_REC: n - 1, u (3), mass - 1, visc, f (19)
+
+
+def read_recycle(path: str):
+    """A fluidRecycle file (IO.cpp:973-997): (lattice size, type of every node, records as
+    n_records x 25), parsed as the reference's restart reads it (LB.cpp:352-367, 994-1009,
+    1098-1138): whitespace-separated numbers."""
+    with open(path) as fh:
+        tok = fh.read().split()
+    size = [int(t) for t in tok[:3]]
+    n = size[0] * size[1] * size[2]
+    types = np.array([int(t) for t in tok[3:3 + n]], np.uint8)
+    vals = np.array([float(t) for t in tok[3 + n:]], np.float64)
+    m = len(vals) // RESTART_REC
+    return size, types, vals[:m * RESTART_REC].reshape(m, RESTART_REC)
 
 
 def load_case(case_dir: str, cfg_name: str = "config.cfg", precision: int = PREC_FP64, overrides=None):

@@ -132,6 +132,10 @@ struct lbdem_ctx {
     // ---------------- IO counters (main.cpp:40-43) ----------------
     double realTime = 0;
     unsigned currentTimeStep = 0;
+    // ---------------- fluid restart (lbdem_set_restart_fluid) ----------------
+    bool lbRestart = false, restartShort = false;
+    std::vector<uint8_t> restartTypes;
+    std::vector<double> restartRecords;
     int fail(int code, const std::string& m) { err = m; return code; }
 
     // ================= index helpers (LB.cpp:2113-2169) =================
@@ -364,23 +368,45 @@ struct lbdem_ctx {
                 objectNodes.push_back(uint32_t(ip));
             }
         }
-        // initializeInterface (LB.cpp:974-992)
-        if (freeSurface) {
-            const double h0 = fluidHeight / unit.Length;
+        if (lbRestart) {
+            // restartInterface (LB.cpp:994-1009): the file's type at every fluid node
             for (int64_t it = 0; it < N; ++it)
-                if (getY(it) > h0 && t[it] == 0) t[it] = 2;
-        }
-        // initializeVariables (LB.cpp:1042-1096)
-        for (int64_t it = 0; it < N; ++it) {
-            if (t[it] == 0) {
+                if (t[it] == 0) t[it] = restartTypes[size_t(it)];
+            // restartVariables (LB.cpp:1098-1138): the next record at every active node, in index
+            // order; node::restart (node.cpp:38-51) with initDensity added back
+            size_t r = 0;
+            const size_t nrec = restartRecords.size() / LBDEM_RESTART_REC;
+            for (int64_t it = 0; it < N; ++it) {
+                if (!isActive(it)) continue;
+                if (r >= nrec) { restartShort = true; return; }
+                const double* v = &restartRecords[r * LBDEM_RESTART_REC];
+                ++r;
                 createNode(it);
-                for (int j = 1; j < Q; ++j) {
-                    const uint32_t l = d[size_t(it) * Q + j];
-                    if (inRange(l) && t[l] == 2) { t[it] = 3; break; }
-                }
+                n[it] = v[0] + initDensity;
+                setU(it, V3(v[1], v[2], v[3]));
+                mass[it] = v[4] + initDensity;
+                visc[it] = v[5];
+                for (int j = 0; j < Q; ++j) F(it)[j] = FS(it)[j] = v[6 + j];
             }
-            if (t[it] == 0) nodeInitialize(it, initDensity, initVelocity, initDensity, initDynVisc, lbF);
-            else if (t[it] == 3) nodeInitialize(it, initDensity, initVelocity, 0.5 * initDensity, initDynVisc, lbF);
+        } else {
+            // initializeInterface (LB.cpp:974-992)
+            if (freeSurface) {
+                const double h0 = fluidHeight / unit.Length;
+                for (int64_t it = 0; it < N; ++it)
+                    if (getY(it) > h0 && t[it] == 0) t[it] = 2;
+            }
+            // initializeVariables (LB.cpp:1042-1096)
+            for (int64_t it = 0; it < N; ++it) {
+                if (t[it] == 0) {
+                    createNode(it);
+                    for (int j = 1; j < Q; ++j) {
+                        const uint32_t l = d[size_t(it) * Q + j];
+                        if (inRange(l) && t[l] == 2) { t[it] = 3; break; }
+                    }
+                }
+                if (t[it] == 0) nodeInitialize(it, initDensity, initVelocity, initDensity, initDynVisc, lbF);
+                else if (t[it] == 3) nodeInitialize(it, initDensity, initVelocity, 0.5 * initDensity, initDynVisc, lbF);
+            }
         }
         // initializeWalls (LB.cpp:1140-1176)
         wallNodes.clear();
@@ -1347,11 +1373,24 @@ int lbdem_set_objects(lbdem_ctx* c, int n, const double* r, const double* x0, co
     }
     return 0;
 }
+// the fluidRecycle file's contents (LB.cpp:352-367): kept for latticeBolzmannInit
+int lbdem_set_restart_fluid(lbdem_ctx* c, const int size[3], const uint8_t* types, int nRecords, const double* records) {
+    if (!c || !size || !types || nRecords < 0 || (nRecords > 0 && !records)) return LBDEM_E_ARG;
+    if (c->initialised) return c->fail(LBDEM_E_STATE, "the restart must be set before lbdem_init");
+    for (int a = 0; a < 3; ++a)
+        if (unsigned(size[a]) != c->lbSize[a]) return c->fail(LBDEM_E_ARG, "the restart file's lattice size differs");
+    const size_t n = size_t(c->lbSize[0]) * c->lbSize[1] * c->lbSize[2];
+    c->restartTypes.assign(types, types + n);
+    c->restartRecords.assign(records, records + size_t(nRecords) * LBDEM_RESTART_REC);
+    c->lbRestart = true;
+    return 0;
+}
 int lbdem_init(lbdem_ctx* c) {
     if (!c) return LBDEM_E_ARG;
     c->latticeDefinition();
     c->discreteElementInit();
     c->latticeBolzmannInit();
+    if (c->restartShort) return c->fail(LBDEM_E_ARG, "the restart file holds fewer records than active nodes");
     c->lbTime = 0; c->demTime = 0.0; c->realTime = 0.0; c->demTimeStep = 0;
     c->initialised = true;
     return c->err.empty() ? 0 : LBDEM_E_CAPACITY;
@@ -1406,6 +1445,17 @@ int lbdem_get_fields(lbdem_ctx* c, double* f, double* mass, double* n) {
         if (f) for (int q = 0; q < Q; ++q) f[q * N + i] = e ? c->f[size_t(i) * Q + q] : 0.0;
         if (mass) mass[i] = e ? c->mass[i] : 0.0;
         if (n) n[i] = e ? c->n[i] : 0.0;
+    }
+    return 0;
+}
+// node->u of every active node (IO.cpp:973-997 reads it); the oracle keeps it for every node
+int lbdem_store_velocity(lbdem_ctx* c) { NEED_INIT; return 0; }
+int lbdem_get_velocity(lbdem_ctx* c, double* uo, double* vo) {
+    NEED_INIT;
+    for (int64_t i = 0; i < c->N; ++i) {
+        const bool a = c->exists[i] && c->isActive(i);
+        if (uo) for (int k = 0; k < 3; ++k) uo[3 * i + k] = a ? c->u[3 * i + k] : 0.0;
+        if (vo) vo[i] = a ? c->visc[i] : 0.0;
     }
     return 0;
 }

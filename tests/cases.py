@@ -97,6 +97,12 @@ def _case_values(name: str) -> Tuple[Dict[str, object], np.ndarray, np.ndarray, 
         v.update(lbSizeX=16.0, lbSizeY=20.0, lbSizeZ=14.0, fluidHeight=11.0, objWaitTime=0)
         return (v, np.zeros(0), empty, np.array([2.5, 2.2]), np.array([[4.6, 14.4, 7.1], [11.3, 15.2, 6.8]]),
                 np.array([[0.0, -0.07, 0.0], [0.01, -0.05, 0.0]]))
+    if name == "restart":
+        # the suspension case restarted from the reference's fluidRecycle file of its step 15
+        # (LB.cpp:352-367); make_case writes the file into the case directory
+        v, pr, pc, orr, oc, ov = _case_values("suspension")
+        v.update(lbRestart="true", lbRestartFile=RESTART_FILE)
+        return v, pr, pc, orr, oc, ov
     if name == "startup":
         # the suspension case with a DEM start-up repeat: while demTime < demInitialRepeat a step is
         # a DEM step and a coupling step only (main.cpp:49-63)
@@ -147,6 +153,19 @@ VELOCITIES = {
 
 # runs of the host program compared with the reference program's result files: case, config
 # overrides (time window and screen interval)
+# the fluid restart: the reference writes fluidRecycle files of the suspension case at steps 0,
+# 15 and 30 (IO.cpp:203-210; kept gzip-compressed under golden/recycle_suspension/), and the
+# "restart" case reads the step-15 file back
+RECYCLE_RUN = ("suspension", {"maxTime": 30.0, "screenExpTime": 10.0, "fluidRecycleExpTime": 15.0})
+RECYCLE_STEPS = (0, 15, 30)
+RESTART_FILE = "restart.dat"
+RESTART_FROM = 15
+
+
+def recycle_name(step: int) -> str:
+    return "fluidRecycle%010u.dat" % step
+
+
 HOST_RUNS = {"suspension": {"maxTime": 39.0, "screenExpTime": 4.0},
              "fs_impactor": {"maxTime": 29.0, "screenExpTime": 3.0},
              "moving": {"maxTime": 29.0, "screenExpTime": 2.0}}
@@ -167,6 +186,11 @@ def make_case(root: str, name: str, overrides: Dict[str, object] = None, subdir:
     lbdem.write_objects(os.path.join(d, "input", "objects.dat"), orr, ox0, ox1)
     with open(os.path.join(d, "input", "body.dat"), "w") as fh:
         fh.write("0\n")
+    if name == "restart":
+        import gzip
+        src = os.path.join(GOLDEN, "recycle_suspension", recycle_name(RESTART_FROM) + ".gz")
+        with gzip.open(src, "rb") as a, open(os.path.join(d, RESTART_FILE), "wb") as b:
+            b.write(a.read())
     return d
 
 
@@ -207,6 +231,8 @@ def read_dump(path: str) -> Dict[str, object]:
 def open_sim(lib: "lbdem.Library", case_dir: str, precision: int = lbdem.PREC_FP64) -> "lbdem.Simulation":
     vals, prm, ps, obs = lbdem.load_case(case_dir, precision=precision)
     sim = lbdem.Simulation(lib, prm, ps, obs)
+    if int(vals.get("lbRestart", 0)):
+        sim.set_restart(*lbdem.read_recycle(os.path.join(case_dir, str(vals["lbRestartFile"]))))
     sim.init()
     return sim
 

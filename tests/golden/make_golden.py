@@ -7,7 +7,9 @@ compressed .npz files: node types / particle flags / solid indices, post-streami
 of active nodes, density and mass fields, particle / object / wall records, Verlet lists and
 the total fluid mass. Also runs the reference's own main loop on the host-program cases
 (cases.HOST_RUNS) and keeps the result files it writes (objPosition.dat, objForce.dat,
-wallForce.dat) under host_<case>/. Only data is stored.
+wallForce.dat) under host_<case>/, and the fluidRecycle files of the recycle run
+(cases.RECYCLE_RUN) under recycle_<case>/ (gzip), which the "restart" case reads back. Only data
+is stored.
 
 Cases larger than SAMPLE_ABOVE nodes keep the populations, density and mass of a seeded sample
 of active nodes (``sample`` holds their indices) instead of every active node; everything else
@@ -37,7 +39,7 @@ import cases  # noqa: E402
 DRIVER = os.path.join(cases.REPO, "oracle", "_ref", "ref_driver")
 PLAN = {"lb_closed": [1, 12], "fs_impactor": [1, 40], "suspension": [1, 40, 120], "periodic": [40], "units": [40],
         "moving": [1, 30], "startup": [5, 30], "two_impactors": [1, 40],
-        "cells": [1, 26, 60], "thin_z2": [1, 40], "thin_z1": [40], "thin_z0": [20]}
+        "cells": [1, 26, 60], "thin_z2": [1, 40], "thin_z1": [40], "thin_z0": [20], "restart": [0, 1, 20]}
 SAMPLE_ABOVE = 8_000   # nodes
 SAMPLE_NODES = 1536
 
@@ -83,6 +85,28 @@ def host_outputs():
             print("host outputs of", name, "->", dst)
 
 
+def recycle_outputs():
+    """The fluidRecycle files the reference writes on the recycle run (IO.cpp:203-210,
+    973-997), gzip-compressed under recycle_<case>/; the restart case reads one back."""
+    import gzip
+    name, over = cases.RECYCLE_RUN
+    with tempfile.TemporaryDirectory() as tmp:
+        d = cases.make_case(tmp, name, over)
+        env = dict(os.environ, OMP_NUM_THREADS="1")
+        r = subprocess.run([DRIVER, "config.cfg", "out", str(int(over["maxTime"])), "dump"], cwd=d, env=env,
+                           capture_output=True, text=True)
+        if r.returncode:
+            sys.exit(f"reference run failed for the recycle run:\n{r.stdout[-2000:]}\n{r.stderr[-2000:]}")
+        dst = os.path.join(HERE, f"recycle_{name}")
+        os.makedirs(dst, exist_ok=True)
+        for s in cases.RECYCLE_STEPS:
+            src = os.path.join(d, "out", "fluidData", cases.recycle_name(s))
+            with open(src, "rb") as a, gzip.GzipFile(os.path.join(dst, cases.recycle_name(s) + ".gz"), "wb",
+                                                     mtime=0) as b:
+                b.write(a.read())
+        print("recycle files of", name, "->", dst)
+
+
 def main():
     if not os.path.exists(DRIVER):
         sys.exit(f"{DRIVER} missing: run `make -C oracle ref` where the reference sources are present")
@@ -97,6 +121,9 @@ def main():
             sys.exit(f"not in the plan: {unknown}")
     else:
         host_outputs()
+        recycle_outputs()
+    if only is not None and "restart" in only:
+        recycle_outputs()
     if "--host-only" in sys.argv:
         return
     manifest = {}

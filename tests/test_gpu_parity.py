@@ -151,7 +151,7 @@ GOLD = os.path.join(os.path.dirname(os.path.abspath(__file__)), "golden")
 
 
 @pytest.mark.parametrize("name", ["suspension", "fs_impactor", "periodic", "units", "moving", "startup", "two_impactors",
-                                  "cells", "thin_z2", "thin_z1", "thin_z0"])
+                                  "cells", "thin_z2", "thin_z1", "thin_z0", "restart"])
 def test_fp64_matches_reference_fixtures(gpu_lib, case_root, name):
     with open(os.path.join(GOLD, "manifest.json")) as fh:
         steps = sorted(m["step"] for m in json.load(fh).values() if m["case"] == name)
@@ -176,6 +176,35 @@ def test_fp64_matches_reference_fixtures(gpu_lib, case_root, name):
             c = g["ncount"][p]
             assert np.array_equal(st["nlist"][p, :c], np.sort(g["nlist"][p, :c])), f"{name} step {step}: list of {p}"
     sim.close()
+
+
+def test_node_velocity_matches_oracle(oracle_lib, gpu_lib, case_root):
+    """lbdem_store_velocity / lbdem_get_velocity (what IO reads as node->u and node->visc): after
+    initialisation, after a restart and after a stored LB step, every active node's velocity and
+    viscosity equal the oracle's (FP64); an LB step without the store leaves none to read."""
+    for name, warm in (("suspension", 12), ("restart", 5)):
+        d = cases.make_case(case_root, name)
+        a = cases.open_sim(oracle_lib, d)
+        b = cases.open_sim(gpu_lib, d, precision=lbdem.PREC_FP64)
+        print(f"case={name}", flush=True)
+        ua, va = a.velocity()
+        ub, vb = b.velocity()
+        assert np.abs(ua - ub).max() <= 1e-15 and np.array_equal(va, vb), f"{name}: initial velocity"
+        a.cycle(warm)
+        b.cycle(warm)
+        for sim in (a, b):
+            sim.store_velocity()
+            sim.cycle(1)
+        ua, va = a.velocity()
+        ub, vb = b.velocity()
+        scale = max(np.abs(ua).max(), 1e-30)
+        assert np.abs(ua - ub).max() <= 1e-12 * scale, f"{name}: stored velocity"
+        assert np.array_equal(va, vb), f"{name}: viscosity"
+        b.cycle(1)
+        with pytest.raises(lbdem.LbdemError):
+            b.velocity()
+        a.close()
+        b.close()
 
 
 def test_device_errors_are_reported(gpu_lib, case_root):

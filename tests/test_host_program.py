@@ -73,3 +73,90 @@ def test_host_program_on_the_gpu(host_program, gpu_lib, tmp_path, name):
         scale = np.maximum(np.abs(b).max(0), 1e-30)
         err = np.abs(a - b) / scale
         assert err.max() <= 2e-5, f"{name}: {f} differs by {err.max()} of the column scale"
+
+
+# ---------------------------------------------------------------------------------------------
+# fluid restart files (IO.cpp:973-997 writes them, LB.cpp:352-367 reads them back)
+# ---------------------------------------------------------------------------------------------
+def recycle_text(step):
+    import gzip
+    with gzip.open(os.path.join(GOLD, "recycle_suspension", cases.recycle_name(step) + ".gz"), "rt") as fh:
+        return fh.read()
+
+
+def run_recycle(prog, root, library, extra=()):
+    name, over = cases.RECYCLE_RUN
+    d = cases.make_case(str(root), name, over)
+    r = subprocess.run([prog, "-c", "config.cfg", "-d", "out", "--library", library, *extra], cwd=d,
+                       capture_output=True, text=True, timeout=600)
+    assert r.returncode == 0, r.stdout[-3000:] + r.stderr[-2000:]
+    return os.path.join(d, "out", "fluidData")
+
+
+def recycle_table(text):
+    """(size, types, records) of a fluidRecycle file's text, records as rows of number strings."""
+    lines = text.splitlines()
+    size = [int(t) for t in lines[0].split()]
+    types = np.array([int(t) for t in lines[1].split()], np.int64)
+    return size, types, [l.split() for l in lines[2:]]
+
+
+def test_host_program_writes_the_reference_recycle_files(host_program, oracle_lib, tmp_path):
+    """fluidRecycleExpTime: with the FP64 oracle behind the ABI the fluidRecycle files are
+    byte-identical to the reference's, at steps 0, 15 and 30 (every node's type, then per active
+    node n - 1, u, mass - 1, viscosity and the 19 populations)."""
+    out = run_recycle(host_program, tmp_path, ORACLE_LIB)
+    assert sorted(os.listdir(out)) == [cases.recycle_name(s) for s in cases.RECYCLE_STEPS]
+    for s in cases.RECYCLE_STEPS:
+        with open(os.path.join(out, cases.recycle_name(s))) as fh:
+            assert fh.read() == recycle_text(s), f"step {s}"
+
+
+def restarted_file(prog, root, library, extra=()):
+    """The recycle file the host program writes right after restarting from the reference's
+    step-15 file (the restart case, one step, a recycle file at step 0 only)."""
+    d = cases.make_case(str(root), "restart", {"maxTime": 0.0, "fluidRecycleExpTime": 100.0})
+    r = subprocess.run([prog, "-c", "config.cfg", "-d", "out", "--library", library, *extra], cwd=d,
+                       capture_output=True, text=True, timeout=600)
+    assert r.returncode == 0, r.stdout[-3000:] + r.stderr[-2000:]
+    with open(os.path.join(d, "out", "fluidData", cases.recycle_name(0))) as fh:
+        return recycle_table(fh.read())
+
+
+def test_host_program_restarts_from_the_reference_file(host_program, oracle_lib, tmp_path):
+    """lbRestart: the host program parses the reference's file and restarts from it; what it then
+    writes holds the reference's restarted types (tests/golden/restart_s0, from the reference's own
+    restart) and, as the reference prints them, its densities, masses and populations."""
+    size, types, rows = restarted_file(host_program, tmp_path, ORACLE_LIB)
+    g = np.load(os.path.join(GOLD, "restart_s0.npz"), allow_pickle=False)
+    assert size == list(g["size"]) and np.array_equal(types, g["type"])
+    act = g["active"]
+    assert len(rows) == len(act)
+    fmt = np.vectorize(lambda v: "%g" % v)
+    assert [r[0] for r in rows] == list(fmt(g["n_active"] - 1.0))
+    assert [r[4] for r in rows] == list(fmt(g["mass_active"] - 1.0))
+    for q in range(19):
+        assert [r[6 + q] for r in rows] == list(fmt(g["f_active"][q])), f"population {q}"
+
+
+@pytest.mark.gpu
+def test_host_program_recycle_files_on_the_gpu(host_program, gpu_lib, tmp_path):
+    """The MI355X library behind the host program (FP64) writes the reference's fluidRecycle
+    files: the same types and records, the numbers to the printed precision (relative to the
+    column's scale), and restarts from the reference's file to its restarted state."""
+    out = run_recycle(host_program, tmp_path / "w", GPU_LIB, extra=("--precision", "fp64"))
+    for s in cases.RECYCLE_STEPS:
+        with open(os.path.join(out, cases.recycle_name(s))) as fh:
+            size, types, rows = recycle_table(fh.read())
+        gs, gt, grows = recycle_table(recycle_text(s))
+        assert size == gs and np.array_equal(types, gt) and len(rows) == len(grows), f"step {s}"
+        a = np.array(rows, np.float64)
+        b = np.array(grows, np.float64)
+        scale = np.maximum(np.abs(b).max(0), 1e-30)
+        assert (np.abs(a - b) / scale).max() <= 2e-5, f"step {s}"
+    size, types, rows = restarted_file(host_program, tmp_path / "r", GPU_LIB, extra=("--precision", "fp64"))
+    g = np.load(os.path.join(GOLD, "restart_s0.npz"), allow_pickle=False)
+    assert np.array_equal(types, g["type"])
+    a = np.array(rows, np.float64)
+    assert np.abs(a[:, 6:] - g["f_active"].T).max() <= 1e-6 * np.abs(g["f_active"]).max()
+    assert np.abs(a[:, 4] - (g["mass_active"] - 1.0)).max() <= 1e-6
