@@ -55,6 +55,8 @@ void parseConfigFile(IO& io, DEM& dem, LB& lb, ConfigFile& cfg, CommandLine& cl)
     lb.lbmInitialRepeat = parse_value(cfg, cl, "lbmInitialRepeat", 0);
     io.maxTime = parse_value(cfg, cl, "maxTime", 0.0);
     io.screenExpTime = parse_value(cfg, cl, "screenExpTime", 0.0);
+    io.fluidExpTime = parse_value(cfg, cl, "fluidExpTime", 0.0);
+    io.fluidLagrangianExpTime = parse_value(cfg, cl, "fluidLagrangianExpTime", 0.0);
     io.fluidRecycleExpTime = parse_value(cfg, cl, "fluidRecycleExpTime", 0.0);
     io.demSolve = parse_value(cfg, cl, "demSolve", 0);
     io.lbmSolve = parse_value(cfg, cl, "lbSolve", 0);
@@ -82,6 +84,7 @@ void LB::latticeBoltzmannGet(ConfigFile& cfg, CommandLine& cl) {
     p.unitDensity = parse_value(cfg, cl, "unitDensity", 1.0);
     unitLength = p.unitLength;
     unitTime = p.unitTime;
+    unitDensity = p.unitDensity;
     const char* ax[3] = {"X", "Y", "Z"};
     for (int a = 0; a < 3; ++a) p.lbSize[a] = parse_value(cfg, cl, std::string("lbSize") + ax[a], 0.0);
     lbRestart = to_int(parse_value(cfg, cl, "lbRestart", "false"), 0);
@@ -330,6 +333,8 @@ void IO::initialize() {
         const int a = ::mkdir(fluidDirectory.c_str(), 0777);
         std::cout << "Work directory created = " << fluidDirectory << ". Result: " << a << "\n";
     }
+    fluidFileFormat = fluidDirectory + "/fluid%010u.vti";
+    fluidLagrangianFileFormat = fluidDirectory + "/fluidLag%010u.vtu";
     fluidRecycleFileFormat = fluidDirectory + "/fluidRecycle%010u.dat";
     exportFileName = workDirectory + "/export.dat";
     if (demSolve) {
@@ -352,7 +357,7 @@ void IO::initialize() {
         append_line(massFileName, "step time totalFluidMass\n");
     }
     lastScreenExp = 0;
-    lastFluidRecycleExp = 0;
+    lastFluidRecycleExp = lastFluidExp = lastFluidLagrangianExp = 0;
 }
 
 void IO::outputStep(LB& lb, DEM& dem) {
@@ -378,35 +383,160 @@ void IO::outputStep(LB& lb, DEM& dem) {
     createFiles(lb, dem);
 }
 
+// an output interval's counter (IO.cpp:117, 186-207)
+static unsigned exp_counter(double realTime, double expTime) {
+    return expTime > 0 ? unsigned(realTime / expTime) + 1 : 0;
+}
+
 bool IO::fluidVelocityDue() const {
-    const unsigned counter = fluidRecycleExpTime > 0 ? unsigned(realTime / fluidRecycleExpTime) + 1 : 0;
-    return lbmSolve && counter > lastFluidRecycleExp;
+    return lbmSolve && (exp_counter(realTime, fluidLagrangianExpTime) > lastFluidLagrangianExp ||
+                        exp_counter(realTime, fluidRecycleExpTime) > lastFluidRecycleExp);
 }
 
 void IO::createFiles(LB& lb, DEM&) {
     if (!lbmSolve || !c_.initialised()) return;
-    const unsigned counter = fluidRecycleExpTime > 0 ? unsigned(realTime / fluidRecycleExpTime) + 1 : 0;
+    char path[1024];
+    unsigned counter = exp_counter(realTime, fluidExpTime);
+    if (counter > lastFluidExp) {
+        lastFluidExp = counter;
+        std::snprintf(path, sizeof(path), fluidFileFormat.c_str(), currentTimeStep);
+        exportEulerianParaviewFluid(lb, path);
+    }
+    counter = exp_counter(realTime, fluidLagrangianExpTime);
+    if (counter > lastFluidLagrangianExp) {
+        lastFluidLagrangianExp = counter;
+        std::snprintf(path, sizeof(path), fluidLagrangianFileFormat.c_str(), currentTimeStep);
+        exportLagrangianParaviewFluid(lb, path);
+    }
+    counter = exp_counter(realTime, fluidRecycleExpTime);
     if (counter > lastFluidRecycleExp) {
         lastFluidRecycleExp = counter;
-        char path[1024];
         std::snprintf(path, sizeof(path), fluidRecycleFileFormat.c_str(), currentTimeStep);
         exportRecycleFluid(lb, path);
     }
+}
+
+const IO::FluidState& IO::fluidState(const LB& lb, bool populations, bool velocity) {
+    if (fsStep_ != currentTimeStep) {
+        fsStep_ = currentTimeStep;
+        fsF_ = fsU_ = false;
+        Context& c = lb.context();
+        lbdem_info in;
+        c.check(c.api().get_info(c.ctx(), &in), "lbdem_get_info");
+        fs_.N = in.totNodes;
+        fs_.t.assign(fs_.N, 0);
+        fs_.p.assign(fs_.N, 0);
+        fs_.mass.assign(fs_.N, 0.0);
+        fs_.n.assign(fs_.N, 0.0);
+        c.check(c.api().get_types(c.ctx(), fs_.t.data(), fs_.p.data(), nullptr), "lbdem_get_types");
+        c.check(c.api().get_fields(c.ctx(), nullptr, fs_.mass.data(), fs_.n.data()), "lbdem_get_fields");
+    }
+    Context& c = lb.context();
+    if (populations && !fsF_) {
+        fs_.f.assign(size_t(Q_DIRECTIONS) * fs_.N, 0.0);
+        c.check(c.api().get_fields(c.ctx(), fs_.f.data(), nullptr, nullptr), "lbdem_get_fields");
+        fsF_ = true;
+    }
+    if (velocity && !fsU_) {
+        fs_.u.assign(3 * size_t(fs_.N), 0.0);
+        fs_.visc.assign(fs_.N, 0.0);
+        c.check(c.api().get_velocity(c.ctx(), fs_.u.data(), fs_.visc.data()), "lbdem_get_velocity");
+        fsU_ = true;
+    }
+    return fs_;
+}
+
+// every node's type (1 inside a particle) and physical mass (0 where no node holds fluid), as
+// IO.cpp:1139-1235 writes them, the closing tags straight after the last value
+void IO::exportEulerianParaviewFluid(const LB& lb, const std::string& file) {
+    const FluidState& st = fluidState(lb, false, false);
+    const double zero = 0.0;
+    std::ofstream out(file);
+    out << "<VTKFile type=\"ImageData\" version=\"0.1\">\n";
+    out << " <ImageData WholeExtent=\"0 " << lb.lbSize[0] - 1 << " 0 " << lb.lbSize[1] - 1 << " 0 " << lb.lbSize[2] - 1
+        << "\" " << "Origin=\"0.0 0.0 0.0\" Spacing=\"" << lb.unitLength << " " << lb.unitLength << " " << lb.unitLength
+        << "\">\n";
+    out << "  <Piece Extent=\"0 " << lb.lbSize[0] - 1 << " 0 " << lb.lbSize[1] - 1 << " 0 " << lb.lbSize[2] - 1 << "\">\n";
+    out << "   <PointData>\n";
+    out << "    <DataArray type=\"Int8\" Name=\"type\" format=\"ascii\" RangeMin=\"0\" RangeMax=\"8\">\n";
+    for (int64_t i = 0; i < st.N; ++i) out << (st.p[i] ? 1u : unsigned(st.t[i])) << " ";
+    out << "    </DataArray>\n";
+    out << "    <DataArray type=\"Float64\" Name=\"amass\" format=\"ascii\" RangeMin=\"0\" RangeMax=\"2\">\n";
+    for (int64_t i = 0; i < st.N; ++i) {
+        if (st.t[i] == 0 || st.t[i] == 3) out << st.mass[i] * lb.unitDensity << " ";
+        else out << zero << " ";
+    }
+    out << "    </DataArray>\n";
+    out << "   </PointData>\n";
+    out << "   <CellData>\n";
+    out << "   </CellData>\n";
+    out << "  </Piece>\n";
+    out << " </ImageData>\n";
+    out << "</VTKFile>\n";
+}
+
+// the active nodes (ascending index) as points: velocity, pressure, density, viscosity, mass, type
+// and position in physical units, as IO.cpp:1038-1137 writes them
+void IO::exportLagrangianParaviewFluid(const LB& lb, const std::string& file) {
+    const FluidState& st = fluidState(lb, false, true);
+    std::vector<int64_t> act;
+    for (int64_t i = 0; i < st.N; ++i)
+        if (st.t[i] == 0 || st.t[i] == 3) act.push_back(i);
+    const double speed = lb.unitSpeed(), pressure = lb.unitPressure(), initDensity = 1.0;
+    const unsigned X = lb.lbSize[0], Y = lb.lbSize[1];
+    std::ofstream out(file);
+    out << "<?xml version=\"1.0\"?>\n";
+    out << "<VTKFile type=\"UnstructuredGrid\" version=\"0.1\" byte_order=\"LittleEndian\" "
+           "compressor=\"vtkZLibDataCompressor\">\n";
+    out << " <UnstructuredGrid GhostLevel=\"0\">\n";
+    out << "  <Piece NumberOfPoints=\"" << act.size() << "\" NumberOfCells=\"" << act.size() << "\">\n";
+    out << "   <PointData>\n";
+    out << "    <DataArray type=\"Float64\" Name=\"v\" NumberOfComponents=\"3\" format=\"ascii\"/>\n";
+    for (int64_t i : act) out << st.u[3 * i] * speed << " " << st.u[3 * i + 1] * speed << " " << st.u[3 * i + 2] * speed << "\n";
+    out << "    <DataArray type=\"Float64\" Name=\"pressure\" format=\"ascii\" RangeMin=\"0\" RangeMax=\"2\"/>\n";
+    for (int64_t i : act) out << 0.3333333 * (st.n[i] - initDensity) * pressure << "\n";
+    out << "    <DataArray type=\"Float64\" Name=\"n\" format=\"ascii\" RangeMin=\"0\" RangeMax=\"2\"/>\n";
+    for (int64_t i : act) out << st.n[i] << "\n";
+    out << "    <DataArray type=\"Float64\" Name=\"visc\" format=\"ascii\" RangeMin=\"0\" RangeMax=\"2\"/>\n";
+    for (int64_t i : act) out << st.visc[i] << "\n";
+    out << "    <DataArray type=\"Float64\" Name=\"mass\" format=\"ascii\" RangeMin=\"0\" RangeMax=\"2\"/>\n";
+    for (int64_t i : act) out << st.mass[i] * lb.unitDensity << "\n";
+    out << "    <DataArray type=\"Int8\" Name=\"type\" format=\"ascii\" RangeMin=\"0\" RangeMax=\"8\"/>\n";
+    for (int64_t i : act) out << unsigned(st.t[i]) << "\n";
+    out << "   </PointData>\n";
+    out << "   <CellData>\n";
+    out << "   </CellData>\n";
+    out << "   <Points>\n";
+    out << "    <DataArray type=\"Float64\" Name=\"Points\" NumberOfComponents=\"3\"/>\n";
+    for (int64_t i : act) {
+        const double x = double(unsigned(i % X)), y = double(unsigned((i / X) % Y)), z = double(unsigned(i / (int64_t(X) * Y)));
+        out << x * lb.unitLength << " " << y * lb.unitLength << " " << z * lb.unitLength << "\n";
+    }
+    out << "   </Points>\n";
+    out << "   <Cells>\n";
+    out << "    <DataArray type=\"Int64\" Name=\"connectivity\" format=\"ascii\">\n";
+    for (size_t i = 1; i < act.size() + 1; ++i) out << i - 1 << "\n";
+    out << "    </DataArray>\n";
+    out << "    <DataArray type=\"Int64\" Name=\"offsets\" format=\"ascii\">\n";
+    for (size_t i = 1; i < act.size() + 1; ++i) out << i << "\n";
+    out << "    </DataArray>\n";
+    out << "    <DataArray type=\"Int64\" Name=\"types\" format=\"ascii\">\n";
+    for (size_t i = 0; i < act.size(); ++i) out << 1 << "\n";
+    out << "    </DataArray>\n";
+    out << "   </Cells>\n";
+    out << "  </Piece>\n";
+    out << " </UnstructuredGrid>\n";
+    out << "</VTKFile>";
 }
 
 // sizes, every node's type, then per active node (ascending index, as LB::cleanLists sorts
 // activeNodes): n - 1, u.x, u.y, u.z (u.dot of the unit vectors), mass - 1, visc, f[0..18], each
 // followed by a space, with the stream's default formatting (initDensity is 1, LB.cpp:330)
 void IO::exportRecycleFluid(const LB& lb, const std::string& file) {
-    Context& c = lb.context();
-    lbdem_info in;
-    c.check(c.api().get_info(c.ctx(), &in), "lbdem_get_info");
-    const int64_t N = in.totNodes;
-    std::vector<uint8_t> t(N);
-    std::vector<double> f(size_t(Q_DIRECTIONS) * N), mass(N), n(N), u(3 * size_t(N)), visc(N);
-    c.check(c.api().get_types(c.ctx(), t.data(), nullptr, nullptr), "lbdem_get_types");
-    c.check(c.api().get_fields(c.ctx(), f.data(), mass.data(), n.data()), "lbdem_get_fields");
-    c.check(c.api().get_velocity(c.ctx(), u.data(), visc.data()), "lbdem_get_velocity");
+    const FluidState& st = fluidState(lb, true, true);
+    const int64_t N = st.N;
+    const std::vector<uint8_t>& t = st.t;
+    const std::vector<double>&f = st.f, &mass = st.mass, &n = st.n, &u = st.u, &visc = st.visc;
     std::ofstream out(file);
     out << lb.lbSize[0] << " " << lb.lbSize[1] << " " << lb.lbSize[2] << "\n";
     for (int64_t i = 0; i < N; ++i) out << unsigned(t[i]) << " ";

@@ -100,7 +100,10 @@ public:
     int lbRestart = 0;              // LB.cpp:138-139
     std::string lbRestartFile;
     double fluidHeight = 0.0;   // physical, as configured
-    double unitTime = 1.0, unitLength = 1.0;
+    double unitTime = 1.0, unitLength = 1.0, unitDensity = 1.0;
+    // composite units (measureUnits::setComposite, node.cpp:512-525)
+    double unitSpeed() const { return unitLength / unitTime; }
+    double unitPressure() const { return unitDensity * unitLength * unitLength / unitTime / unitTime; }
     unsigned lbSize[3] = {0, 0, 0};
     double totalMass = 0.0;     // at initialisation (LB.cpp:388-395)
     Context& context() const { return c_; }
@@ -146,6 +149,7 @@ public:
     std::string workDirectory = "data", configFileName = "config.cfg";
     double realTime = 0.0, maxTime = 0.0, screenExpTime = 0.0;
     double fluidRecycleExpTime = 0.0;              // main.cpp:157
+    double fluidExpTime = 0.0, fluidLagrangianExpTime = 0.0;  // main.cpp:148-151
     unsigned int currentTimeStep = 0;
     int demSolve = 0, lbmSolve = 0;
 
@@ -154,12 +158,24 @@ private:
     void exportObjectForce(const objectList& objects, const LB& lb);      // IO.cpp:1239-1293
     void exportWallForce(const wallList& walls);                          // IO.cpp:1295-1314
     void exportMass(const LB& lb);                                        // step, time, total fluid mass
-    void createFiles(LB& lb, DEM& dem);                                   // IO.cpp:181-289 (fluid recycle)
+    void createFiles(LB& lb, DEM& dem);                                   // IO.cpp:181-289 (the fluid files)
     void exportRecycleFluid(const LB& lb, const std::string& file);       // IO.cpp:973-997
+    void exportEulerianParaviewFluid(const LB& lb, const std::string& file);    // IO.cpp:1139-1235
+    void exportLagrangianParaviewFluid(const LB& lb, const std::string& file);  // IO.cpp:1038-1137
+    // the lattice fields the fluid files read, downloaded once per output step
+    struct FluidState {
+        int64_t N = 0;
+        std::vector<uint8_t> t, p;
+        std::vector<double> f, mass, n, u, visc;
+    };
+    const FluidState& fluidState(const LB& lb, bool populations, bool velocity);
     Context& c_;
-    unsigned lastScreenExp = 0, lastFluidRecycleExp = 0;
+    unsigned lastScreenExp = 0, lastFluidRecycleExp = 0, lastFluidExp = 0, lastFluidLagrangianExp = 0;
     std::string exportFileName, objectPositionFileName, objectForceFileName, wallForceFileName, massFileName;
-    std::string partDirectory, fluidDirectory, fluidRecycleFileFormat;
+    std::string partDirectory, fluidDirectory, fluidRecycleFileFormat, fluidFileFormat, fluidLagrangianFileFormat;
+    FluidState fs_;
+    unsigned fsStep_ = ~0u;
+    bool fsF_ = false, fsU_ = false;
 };
 
 // parseConfigFile (main.cpp:128-185) and one goCycle (main.cpp:38-67)

@@ -166,6 +166,21 @@ def recycle_name(step: int) -> str:
     return "fluidRecycle%010u.dat" % step
 
 
+# the fluid writers in physical units: the reference's Eulerian (.vti) and Lagrangian (.vtu) files
+# of the units case (IO.cpp:186-202, 1038-1235; kept gzip-compressed under golden/fluid_units/)
+FLUID_RUN = ("units", {"maxTime": 0.6, "screenExpTime": 0.2, "fluidExpTime": 0.3, "fluidLagrangianExpTime": 0.3})
+
+
+def host_steps(values: Dict[str, object]) -> int:
+    """Steps the reference's main loop runs (main.cpp:288-308: until realTime > maxTime, realTime
+    advancing by unitTime per step)."""
+    t, n = 0.0, 0
+    while not (t != 0.0 and t > float(values["maxTime"])):
+        t += float(values["unitTime"])
+        n += 1
+    return n
+
+
 HOST_RUNS = {"suspension": {"maxTime": 39.0, "screenExpTime": 4.0},
              "fs_impactor": {"maxTime": 29.0, "screenExpTime": 3.0},
              "moving": {"maxTime": 29.0, "screenExpTime": 2.0}}

@@ -7,9 +7,9 @@ compressed .npz files: node types / particle flags / solid indices, post-streami
 of active nodes, density and mass fields, particle / object / wall records, Verlet lists and
 the total fluid mass. Also runs the reference's own main loop on the host-program cases
 (cases.HOST_RUNS) and keeps the result files it writes (objPosition.dat, objForce.dat,
-wallForce.dat) under host_<case>/, and the fluidRecycle files of the recycle run
-(cases.RECYCLE_RUN) under recycle_<case>/ (gzip), which the "restart" case reads back. Only data
-is stored.
+wallForce.dat) under host_<case>/, the fluidRecycle files of the recycle run
+(cases.RECYCLE_RUN) under recycle_<case>/ (gzip), which the "restart" case reads back, and the
+.vti / .vtu files of the fluid run (cases.FLUID_RUN) under fluid_<case>/. Only data is stored.
 
 Cases larger than SAMPLE_ABOVE nodes keep the populations, density and mass of a seeded sample
 of active nodes (``sample`` holds their indices) instead of every active node; everything else
@@ -85,26 +85,38 @@ def host_outputs():
             print("host outputs of", name, "->", dst)
 
 
-def recycle_outputs():
-    """The fluidRecycle files the reference writes on the recycle run (IO.cpp:203-210,
-    973-997), gzip-compressed under recycle_<case>/; the restart case reads one back."""
+def fluid_files(run, prefix, only=None):
+    """Runs the reference's main loop on a file run (case, overrides) for as many steps as its own
+    loop would take, and keeps the files it writes into fluidData/ (all of them, or those named in
+    ``only``) gzip-compressed under <prefix>_<case>/."""
     import gzip
-    name, over = cases.RECYCLE_RUN
+    name, over = run
     with tempfile.TemporaryDirectory() as tmp:
         d = cases.make_case(tmp, name, over)
+        vals, *_ = cases._case_values(name)
+        vals.update(over)
         env = dict(os.environ, OMP_NUM_THREADS="1")
-        r = subprocess.run([DRIVER, "config.cfg", "out", str(int(over["maxTime"])), "dump"], cwd=d, env=env,
+        r = subprocess.run([DRIVER, "config.cfg", "out", str(cases.host_steps(vals)), "dump"], cwd=d, env=env,
                            capture_output=True, text=True)
         if r.returncode:
-            sys.exit(f"reference run failed for the recycle run:\n{r.stdout[-2000:]}\n{r.stderr[-2000:]}")
-        dst = os.path.join(HERE, f"recycle_{name}")
+            sys.exit(f"reference run failed for {prefix}_{name}:\n{r.stdout[-2000:]}\n{r.stderr[-2000:]}")
+        src_dir = os.path.join(d, "out", "fluidData")
+        dst = os.path.join(HERE, f"{prefix}_{name}")
         os.makedirs(dst, exist_ok=True)
-        for s in cases.RECYCLE_STEPS:
-            src = os.path.join(d, "out", "fluidData", cases.recycle_name(s))
-            with open(src, "rb") as a, gzip.GzipFile(os.path.join(dst, cases.recycle_name(s) + ".gz"), "wb",
-                                                     mtime=0) as b:
+        names = sorted(os.listdir(src_dir)) if only is None else list(only)
+        for f in names:
+            with open(os.path.join(src_dir, f), "rb") as a, gzip.GzipFile(os.path.join(dst, f + ".gz"), "wb",
+                                                                         mtime=0) as b:
                 b.write(a.read())
-        print("recycle files of", name, "->", dst)
+        print(f"{prefix} files of", name, "->", dst, names)
+
+
+def recycle_outputs():
+    """The fluidRecycle files the reference writes on the recycle run (IO.cpp:203-210,
+    973-997), gzip-compressed under recycle_<case>/ (the restart case reads one back), and the
+    Eulerian / Lagrangian fluid files of the fluid run under fluid_<case>/."""
+    fluid_files(cases.RECYCLE_RUN, "recycle", [cases.recycle_name(s) for s in cases.RECYCLE_STEPS])
+    fluid_files(cases.FLUID_RUN, "fluid")
 
 
 def main():

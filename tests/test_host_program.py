@@ -84,8 +84,14 @@ def recycle_text(step):
         return fh.read()
 
 
-def run_recycle(prog, root, library, extra=()):
-    name, over = cases.RECYCLE_RUN
+def gold_text(folder, fname):
+    import gzip
+    with gzip.open(os.path.join(GOLD, folder, fname + ".gz"), "rt") as fh:
+        return fh.read()
+
+
+def run_recycle(prog, root, library, extra=(), run=None):
+    name, over = run or cases.RECYCLE_RUN
     d = cases.make_case(str(root), name, over)
     r = subprocess.run([prog, "-c", "config.cfg", "-d", "out", "--library", library, *extra], cwd=d,
                        capture_output=True, text=True, timeout=600)
@@ -160,3 +166,46 @@ def test_host_program_recycle_files_on_the_gpu(host_program, gpu_lib, tmp_path):
     a = np.array(rows, np.float64)
     assert np.abs(a[:, 6:] - g["f_active"].T).max() <= 1e-6 * np.abs(g["f_active"]).max()
     assert np.abs(a[:, 4] - (g["mass_active"] - 1.0)).max() <= 1e-6
+
+
+# ---------------------------------------------------------------------------------------------
+# Eulerian (.vti) and Lagrangian (.vtu) fluid files (IO.cpp:186-202, 1038-1235)
+# ---------------------------------------------------------------------------------------------
+FLUID_DIR = "fluid_" + cases.FLUID_RUN[0]
+
+
+def fluid_gold_names():
+    return sorted(f[:-3] for f in os.listdir(os.path.join(GOLD, FLUID_DIR)))
+
+
+def test_host_program_writes_the_reference_fluid_files(host_program, oracle_lib, tmp_path):
+    """fluidExpTime / fluidLagrangianExpTime in physical units (the units case): with the FP64
+    oracle behind the ABI the .vti (every node's type and physical mass) and .vtu files (the active
+    nodes' velocity, pressure, density, viscosity, mass, type and position) are byte-identical to
+    the reference's."""
+    out = run_recycle(host_program, tmp_path, ORACLE_LIB, run=cases.FLUID_RUN)
+    assert sorted(os.listdir(out)) == fluid_gold_names()
+    for f in fluid_gold_names():
+        with open(os.path.join(out, f)) as fh:
+            assert fh.read() == gold_text(FLUID_DIR, f), f
+
+
+@pytest.mark.gpu
+def test_host_program_fluid_files_on_the_gpu(host_program, gpu_lib, tmp_path):
+    """The MI355X library behind the host program (FP64) writes the reference's .vti and .vtu
+    files: the same tags, node sets and types, the numbers to the printed precision."""
+    import re
+    out = run_recycle(host_program, tmp_path, GPU_LIB, extra=("--precision", "fp64"), run=cases.FLUID_RUN)
+    assert sorted(os.listdir(out)) == fluid_gold_names()
+    for f in fluid_gold_names():
+        with open(os.path.join(out, f)) as fh:
+            a_text = fh.read()
+        b_text = gold_text(FLUID_DIR, f)
+        assert re.findall(r"<[^>]*>", a_text) == re.findall(r"<[^>]*>", b_text), f
+        # section by section (the text between tags: one data array each), relative to its scale
+        for k, (sa, sb) in enumerate(zip(re.split(r"<[^>]*>", a_text), re.split(r"<[^>]*>", b_text))):
+            a = np.array([float(t) for t in sa.split()], np.float64)
+            b = np.array([float(t) for t in sb.split()], np.float64)
+            assert a.shape == b.shape, f"{f} section {k}"
+            if b.size:
+                assert np.abs(a - b).max() <= 2e-5 * max(np.abs(b).max(), 1e-30), f"{f} section {k}"
