@@ -432,6 +432,43 @@ def copy_probe(nbytes=1 << 30, reps=3):
                       "gbs_read_plus_write": round(2 * nbytes / dt / 1e9, 1)}), flush=True)
 
 
+def summarize_profile(paths):
+    """Reduce rocprofv3 CSV output (kernel trace or counter collection) to per-kernel means, written
+    beside each CSV as <csv>.sum.json ("summary": kernel -> counter -> [launches, mean]; "per_launch":
+    every value of the fused and copy kernels); the CSV itself is removed, since a long run's
+    per-dispatch rows run to hundreds of MB."""
+    import collections
+    import csv
+    for path in paths:
+        agg = collections.defaultdict(lambda: collections.defaultdict(lambda: [0, 0.0]))
+        keep = collections.defaultdict(list)
+        trace = "kernel_trace" in os.path.basename(path)
+        with open(path) as fh:
+            for r in csv.DictReader(fh):
+                kn = r["Kernel_Name"]
+                if trace:
+                    v, cn = (int(r["End_Timestamp"]) - int(r["Start_Timestamp"])) / 1e3, "dur_us"
+                else:
+                    v, cn = float(r["Counter_Value"]), r["Counter_Name"]
+                a = agg[kn][cn]
+                a[0] += 1
+                a[1] += v
+                if "k_fused" in kn or "copy" in kn.lower():
+                    keep[kn + "|" + cn].append(v)
+        names = list(agg)
+        try:
+            dm = subprocess.run(["c++filt"], input="\n".join(names), capture_output=True, text=True).stdout.split("\n")
+        except OSError:
+            dm = []
+        out = {"summary": {(dm[i] if i < len(dm) and dm[i] else n)[:200]: {c: [v[0], v[1] / v[0]] for c, v in agg[n].items()}
+                           for i, n in enumerate(names)},
+               "per_launch": {k[:40] + ".." + k[-60:]: v for k, v in keep.items()}}
+        with open(path + ".sum.json", "w") as fh:
+            json.dump(out, fh)
+        os.remove(path)
+        print("summarized", path, len(names), "kernels", flush=True)
+
+
 def main():
     ap = argparse.ArgumentParser()
     ap.add_argument("--gpus", type=int, default=1)
@@ -454,7 +491,12 @@ def main():
                     help="only the precision sweep on the headline state (checkpoints 100, 1000, 5000 up to STEPS)")
     ap.add_argument("--copy-probe", action="store_true",
                     help="only run a plain 1 GiB device copy three times (calibrates HBM byte counters)")
+    ap.add_argument("--summarize-profile", nargs="+", metavar="CSV",
+                    help="only reduce rocprofv3 CSV files to per-kernel means (<csv>.sum.json; removes the CSV)")
     args = ap.parse_args()
+    if args.summarize_profile:
+        summarize_profile(args.summarize_profile)
+        return
     if args.copy_probe:
         copy_probe()
         return
