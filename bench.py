@@ -432,6 +432,58 @@ def copy_probe(nbytes=1 << 30, reps=3):
                       "gbs_read_plus_write": round(2 * nbytes / dt / 1e9, 1)}), flush=True)
 
 
+def batch_profile(prec, n_boxes=64, steps=400, tile=8):
+    """The batched workload's boxes (64, edges 48-128) as one batch of the batch API (not the pool):
+    ``steps`` steps replayed from graphs and timed on the host, then the same number launched one by
+    one with a HIP event after each launch (lbdem_batch_profile_enable): the time of every batch
+    kernel, summed over its launches, beside the replayed step."""
+    lib = lbdem.Library.load()
+    runs = workloads.batched_runs(n_boxes, 2000)
+    sims = []
+    for run in runs:
+        s = lbdem.Simulation(lib, lbdem.params_from_values(run.values, prec), run.particles, run.objects)
+        s.init()
+        s.set_tile(tile)
+        s.cycle(4)  # start-up steps alone
+        sims.append(s)
+    for s in sims:
+        s.synchronize()
+    nodes = sum(s.info()["totNodes"] for s in sims)
+    batch = lbdem.Batch(lib, sims)
+    batch.set_tile(tile)
+    batch.cycle(20)  # records and captures the pair of each phase
+    batch.synchronize()
+    t0 = time.perf_counter()
+    batch.cycle(steps)
+    batch.synchronize()
+    replay = (time.perf_counter() - t0) / steps
+    active = sum(s.info()["nActive"] for s in sims)
+    log(f"[bench] batch profile: graph replay {replay * 1e3:.3f} ms per step; timing launches")
+    batch.profile(True)
+    t0 = time.perf_counter()
+    batch.cycle(steps)
+    batch.synchronize()
+    timed = (time.perf_counter() - t0) / steps
+    rows = batch.profile_read()
+    batch.profile(False)
+    batch.close()
+    for s in sims:
+        s.close()
+    tot = sum(r["ms"] for r in rows)
+    rows.sort(key=lambda r: -r["ms"])
+    out = {"batch_profile": {
+        "boxes": n_boxes, "box_nodes": int(nodes), "active_nodes_end": int(active), "steps": steps, "tile_z": tile,
+        "precision": PREC_NAME[prec],
+        "ms_per_step_graph_replay": round(replay * 1e3, 4),
+        "mlups_box_graph_replay": round(nodes / replay / 1e6, 1),
+        "ms_per_step_launch_by_launch": round(timed * 1e3, 4),
+        "kernel_ms_per_step_sum": round(tot / steps, 4),
+        "kernels": [{"kernel": r["kernel"], "ms_per_step": round(r["ms"] / steps, 4),
+                     "launches_per_step": round(r["launches"] / steps, 2),
+                     "us_per_launch": round(1e3 * r["ms"] / max(r["launches"], 1), 2)} for r in rows]}}
+    print(json.dumps(out), flush=True)
+
+
 def summarize_profile(paths):
     """Reduce rocprofv3 CSV output (kernel trace or counter collection) to per-kernel means, written
     beside each CSV as <csv>.sum.json ("summary": kernel -> counter -> [launches, mean]; "per_launch":
@@ -491,11 +543,17 @@ def main():
                     help="only the precision sweep on the headline state (checkpoints 100, 1000, 5000 up to STEPS)")
     ap.add_argument("--copy-probe", action="store_true",
                     help="only run a plain 1 GiB device copy three times (calibrates HBM byte counters)")
+    ap.add_argument("--batch-profile", type=int, default=0, metavar="STEPS",
+                    help="only time every kernel of the batched workload's boxes in one batch (HIP events)")
     ap.add_argument("--summarize-profile", nargs="+", metavar="CSV",
                     help="only reduce rocprofv3 CSV files to per-kernel means (<csv>.sum.json; removes the CSV)")
     args = ap.parse_args()
     if args.summarize_profile:
         summarize_profile(args.summarize_profile)
+        return
+    if args.batch_profile:
+        batch_profile({"fp32": lbdem.PREC_FP32, "fp64": lbdem.PREC_FP64, "fp16": lbdem.PREC_FP16}[args.precision],
+                      steps=args.batch_profile)
         return
     if args.copy_probe:
         copy_probe()

@@ -232,6 +232,14 @@ int lbdem_batch_destroy(lbdem_batch* batch);         /* the contexts remain, usa
 int lbdem_batch_set_tile(lbdem_batch* batch, int tileZ);
 /* the batch's walk; *tuned = 0 while candidates are still being timed */
 int lbdem_batch_get_tile(const lbdem_batch* batch, int* tileZ, int* tuned);
+/* Kernel timing of a batch (no reference counterpart; used by benchmarks): while enabled, each step
+ * pair's batch launches run one by one with a HIP event after each (no graph replay; the host waits
+ * for every pair). lbdem_batch_profile_read reports kernel i in order of first launch: its name
+ * (valid until the next enable), summed milliseconds and launches since the last enable, and the
+ * number of kernels in *count; an i past the end sets only *count and *kernel = NULL. */
+int lbdem_batch_profile_enable(lbdem_batch* batch, int enable);
+int lbdem_batch_profile_read(const lbdem_batch* batch, int i, const char** kernel, double* ms, int* launches,
+                             int* count);
 
 /* ---------------------------------------------------------------------------------------------
  * Run pool (no reference counterpart; batched and served runs of many independent boxes on one

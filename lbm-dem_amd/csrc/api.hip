@@ -1413,12 +1413,35 @@ struct BatchOp {
     size_t off;               // of the launch's argument table (one entry per box of the launch)
     size_t j;                 // index of the launch in the recorded step pair
     std::vector<size_t> who;  // the batch members it runs, in table order
+    std::string name;         // the kernel body with its template arguments
 };
-// A kernel with an exact grid runs every box of a launch on the launch's largest grid (the
-// surplus blocks return at once). When the boxes' grids differ by more than 2x and the largest
-// is big, the boxes are launched in classes of power-of-two grid size instead, so few blocks
-// are launched for nothing.
-constexpr unsigned SPLIT_MIN_GRID = 256;
+// "k_fused<float, 0, true, true, 4>" from a launcher's demangled type name, whose first template
+// argument is the body: "&lbdem::k_mark" or, for a function template,
+// "&(void lbdem::k_fused<float, 0, true, true, 4>(lbdem::FusedArgs<float>))"
+static std::string body_name(const char* sig) {
+    const char* p = sig ? std::strstr(sig, "Launcher<&") : nullptr;
+    if (!p) return sig ? sig : "?";
+    p += 10;
+    if (*p == '(') {  // skip "(void "
+        const char* sp = std::strchr(p, ' ');
+        p = sp ? sp + 1 : p + 1;
+    }
+    if (std::strncmp(p, "lbdem::", 7) == 0) p += 7;
+    std::string out;
+    int depth = 0;
+    for (; *p; ++p) {
+        if (*p == '<') ++depth;
+        else if (*p == '>') {
+            if (depth == 0) break;
+            --depth;
+        } else if ((*p == ',' || *p == '(' || *p == ')') && depth == 0) break;
+        out += *p;
+    }
+    return out;
+}
+// A kernel with an exact grid runs every box of a launch on the launch's largest grid: the
+// surplus blocks of smaller boxes return at once, which costs less than launching boxes of
+// different sizes separately (measured on boxes of edge 48-128: 4 % per step).
 // the phase's step pair, one launch list per group of members with the same launch sequence
 struct BatchPhase {
     bool recorded = false;
@@ -1448,6 +1471,11 @@ struct lbdem_batch {
     bool tuned = true;
     int tuneIdx = 0;
     hipEvent_t tuneEv[NCAND][2] = {};
+    // kernel timing (lbdem_batch_profile_enable): every launch of a pair bracketed by events
+    bool prof = false;
+    std::vector<hipEvent_t> pev;
+    std::vector<std::string> profName;  // kernels in order of first launch
+    std::map<std::string, std::pair<double, int>> profTime;  // ms summed, launches
     std::string err;
     int fail(int code, const std::string& e) { err = e; return code; }
     ~lbdem_batch() {
@@ -1455,6 +1483,7 @@ struct lbdem_batch {
         for (auto& pr : tuneEv)
             for (hipEvent_t e : pr)
                 if (e) (void)hipEventDestroy(e);
+        for (hipEvent_t e : pev) (void)hipEventDestroy(e);
         for (BatchPhase& P : ph) {
             if (P.exec) (void)hipGraphExecDestroy(P.exec);
             if (P.graph) (void)hipGraphDestroy(P.graph);
@@ -1523,41 +1552,25 @@ static int batch_record(lbdem_batch* B, BatchPhase& P) {
         const std::vector<size_t>& mg = members[g];
         const Recorder& r0 = rec[mg[0]];
         for (size_t j = 0; j < r0.ops.size(); ++j) {
-            unsigned gmax = 1, gmin = ~0u;
+            BatchOp op;
+            op.launch = r0.ops[j].batch;
+            op.block = r0.ops[j].block;
+            op.shm = r0.ops[j].shm;
+            op.j = j;
+            op.name = body_name(r0.ops[j].name);
+            unsigned gx = 1;
             for (size_t i : mg) {
                 const dim3 gr = rec[i].ops[j].grid;
                 if (gr.y != 1 || gr.z != 1) return undo("a kernel uses a multi-dimensional grid");
-                gmax = std::max(gmax, gr.x);
-                gmin = std::min(gmin, gr.x);
+                gx = std::max(gx, gr.x);
             }
-            std::vector<std::vector<size_t>> parts;
-            if (!r0.ops[j].elastic && gmax >= SPLIT_MIN_GRID && gmax > 2 * gmin) {
-                std::map<int, std::vector<size_t>> cls;  // by ceil(log2(grid))
-                for (size_t i : mg) {
-                    int k = 0;
-                    while ((1u << k) < rec[i].ops[j].grid.x) ++k;
-                    cls[k].push_back(i);
-                }
-                for (auto& kv : cls) parts.push_back(std::move(kv.second));
-            } else {
-                parts.push_back(mg);
-            }
-            for (std::vector<size_t>& part : parts) {
-                BatchOp op;
-                op.launch = r0.ops[j].batch;
-                op.block = r0.ops[j].block;
-                op.shm = r0.ops[j].shm;
-                op.j = j;
-                unsigned gx = 1;
-                for (size_t i : part) gx = std::max(gx, rec[i].ops[j].grid.x);
-                // grid-stride kernels: a bounded number of blocks spread over the launch's boxes
-                if (r0.ops[j].elastic) gx = std::min(gx, std::max(4u, unsigned(r0.ops[j].elastic / part.size())));
-                op.grid = dim3(gx, unsigned(part.size()), 1);
-                op.off = total;
-                total += (part.size() * r0.ops[j].args.size() + 255) & ~size_t(255);
-                op.who = std::move(part);
-                P.groups[g].push_back(std::move(op));
-            }
+            // grid-stride kernels: a bounded number of blocks spread over the launch's boxes
+            if (r0.ops[j].elastic) gx = std::min(gx, std::max(4u, unsigned(r0.ops[j].elastic / mg.size())));
+            op.grid = dim3(gx, unsigned(mg.size()), 1);
+            op.off = total;
+            total += (mg.size() * r0.ops[j].args.size() + 255) & ~size_t(255);
+            op.who = mg;
+            P.groups[g].push_back(std::move(op));
         }
     }
     std::vector<char> host(std::max<size_t>(total, 1));
@@ -1614,7 +1627,43 @@ static void batch_prepare_pair(lbdem_batch* B, BatchPhase& P) {
         (void)hipGetLastError();
     }
 }
+// profiling: the pair's launches one by one, an event after each, times read back at once (the
+// host waits for the pair)
+static int batch_launch_pair_timed(lbdem_batch* B, const BatchPhase& P) {
+    size_t nops = 0;
+    for (const std::vector<BatchOp>& ops : P.groups) nops += ops.size();
+    while (B->pev.size() < nops + 1) {
+        hipEvent_t e = nullptr;
+        BATCHCHK(hipEventCreate(&e));
+        B->pev.push_back(e);
+    }
+    BATCHCHK(hipEventRecord(B->pev[0], B->stream));
+    size_t k = 0;
+    for (const std::vector<BatchOp>& ops : P.groups)
+        for (const BatchOp& op : ops) {
+            op.launch(op.grid, op.block, op.shm, B->stream, P.table + op.off);
+            BATCHCHK(hipGetLastError());
+            BATCHCHK(hipEventRecord(B->pev[++k], B->stream));
+        }
+    BATCHCHK(hipEventSynchronize(B->pev[nops]));
+    k = 0;
+    for (const std::vector<BatchOp>& ops : P.groups)
+        for (const BatchOp& op : ops) {
+            float ms = 0.0f;
+            BATCHCHK(hipEventElapsedTime(&ms, B->pev[k], B->pev[k + 1]));
+            ++k;
+            auto it = B->profTime.find(op.name);
+            if (it == B->profTime.end()) {
+                B->profName.push_back(op.name);
+                it = B->profTime.emplace(op.name, std::make_pair(0.0, 0)).first;
+            }
+            it->second.first += ms;
+            it->second.second += 1;
+        }
+    return 0;
+}
 static int batch_launch_pair(lbdem_batch* B, const BatchPhase& P) {
+    if (B->prof) return batch_launch_pair_timed(B, P);
     if (P.exec) BATCHCHK(hipGraphLaunch(P.exec, B->stream));
     else {
         batch_launch_ops(B, P);
@@ -1757,6 +1806,28 @@ extern "C" int lbdem_batch_synchronize(lbdem_batch* B) {
     BATCHCHK(hipStreamSynchronize(B->stream));
     for (lbdem_ctx* c : B->m)
         if (int e = check_errors(c)) return B->fail(e, c->err);
+    return 0;
+}
+
+extern "C" int lbdem_batch_profile_enable(lbdem_batch* B, int enable) {
+    if (!B) return LBDEM_E_ARG;
+    BATCHCHK(hipStreamSynchronize(B->stream));
+    B->prof = enable != 0;
+    B->profName.clear();
+    B->profTime.clear();
+    return 0;
+}
+
+extern "C" int lbdem_batch_profile_read(const lbdem_batch* B, int i, const char** kernel, double* ms, int* launches,
+                                        int* count) {
+    if (!B || i < 0) return LBDEM_E_ARG;
+    if (count) *count = int(B->profName.size());
+    if (kernel) *kernel = nullptr;
+    if (size_t(i) >= B->profName.size()) return 0;
+    const auto& v = B->profTime.at(B->profName[size_t(i)]);
+    if (kernel) *kernel = B->profName[size_t(i)].c_str();
+    if (ms) *ms = v.first;
+    if (launches) *launches = v.second;
     return 0;
 }
 

@@ -11,8 +11,13 @@
 
 #include <hip/hip_runtime.h>
 
+#include <cxxabi.h>
+
+#include <cstdlib>
 #include <cstring>
+#include <string>
 #include <type_traits>
+#include <typeinfo>
 #include <utility>
 #include <vector>
 
@@ -58,6 +63,7 @@ __global__ __launch_bounds__(MAXT, MINW) void kernel_batch(const Pack<P...>* tab
 // one recorded launch of one box
 struct RecordedLaunch {
     const void* key;  // identity of the kernel (body and bounds)
+    const char* name; // the launcher's signature, naming the body (for timing tables)
     void (*batch)(dim3 grid, dim3 block, size_t shm, hipStream_t s, const void* table);
     dim3 grid, block;
     size_t shm;
@@ -76,6 +82,16 @@ struct Launcher<Body, MAXT, MINW, void (*)(P...)> {
     using Tuple = Pack<P...>;
     static_assert(std::is_trivially_copyable<Tuple>::value, "kernel arguments are copied as bytes");
     static inline const char tag = 0;
+    static const char* signature() {  // "lbdem::Launcher<&lbdem::k_fused<float, 0, true, true, 8>, 256, 4, ...>"
+        static const std::string s = [] {
+            int st = 0;
+            char* d = abi::__cxa_demangle(typeid(Launcher).name(), nullptr, nullptr, &st);
+            std::string r = (st == 0 && d) ? d : typeid(Launcher).name();
+            std::free(d);
+            return r;
+        }();
+        return s.c_str();
+    }
     static void batch(dim3 g, dim3 b, size_t shm, hipStream_t s, const void* table) {
         hipLaunchKernelGGL((kernel_batch<Body, MAXT, MINW, P...>), g, b, shm, s, static_cast<const Tuple*>(table));
     }
@@ -83,7 +99,7 @@ struct Launcher<Body, MAXT, MINW, void (*)(P...)> {
     static void run(unsigned elastic, dim3 g, dim3 b, size_t shm, hipStream_t s, A&&... a) {
         if (Recorder* r = current_recorder()) {
             const Tuple t{{static_cast<P>(std::forward<A>(a))}...};
-            RecordedLaunch op{&tag, &batch, g, b, shm, elastic, std::vector<unsigned char>(sizeof(Tuple))};
+            RecordedLaunch op{&tag, signature(), &batch, g, b, shm, elastic, std::vector<unsigned char>(sizeof(Tuple))};
             std::memcpy(op.args.data(), &t, sizeof(Tuple));
             r->ops.push_back(std::move(op));
             return;

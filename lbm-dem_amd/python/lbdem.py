@@ -181,6 +181,9 @@ class Library:
             "lbdem_batch_destroy": ([vp], C.c_int),
             "lbdem_batch_set_tile": ([vp, C.c_int], C.c_int),
             "lbdem_batch_get_tile": ([vp, C.POINTER(C.c_int), C.POINTER(C.c_int)], C.c_int),
+            "lbdem_batch_profile_enable": ([vp, C.c_int], C.c_int),
+            "lbdem_batch_profile_read": ([vp, C.c_int, C.POINTER(C.c_char_p), C.POINTER(C.c_double),
+                                          C.POINTER(C.c_int), C.POINTER(C.c_int)], C.c_int),
             "lbdem_pool_create": ([C.c_size_t, C.c_int, C.c_int, C.c_int, C.POINTER(vp)], C.c_int),
             "lbdem_pool_submit": ([vp, C.POINTER(RunDesc), _ip], C.c_int),
             "lbdem_pool_poll": ([vp, C.c_int, C.POINTER(RunResult)], C.c_int),
@@ -419,6 +422,23 @@ class Batch:
         tz, tuned = C.c_int(), C.c_int()
         self._check(self.lib.lib.lbdem_batch_get_tile(self.batch, C.byref(tz), C.byref(tuned)), "lbdem_batch_get_tile")
         return dict(tileZ=tz.value, tuned=bool(tuned.value))
+
+    def profile(self, enable: bool = True):
+        """Time every batch launch with HIP events (pairs run launch by launch, no graph)."""
+        self._check(self.lib.lib.lbdem_batch_profile_enable(self.batch, 1 if enable else 0),
+                    "lbdem_batch_profile_enable")
+
+    def profile_read(self) -> List[Dict]:
+        """[{kernel, ms, launches}] since the last enable, in order of first launch."""
+        out, i = [], 0
+        while True:
+            name, ms, n, cnt = C.c_char_p(), C.c_double(), C.c_int(), C.c_int()
+            self._check(self.lib.lib.lbdem_batch_profile_read(self.batch, i, C.byref(name), C.byref(ms), C.byref(n),
+                                                              C.byref(cnt)), "lbdem_batch_profile_read")
+            if i >= cnt.value:
+                return out
+            out.append(dict(kernel=name.value.decode(), ms=ms.value, launches=n.value))
+            i += 1
 
     def close(self):
         if self.batch:

@@ -1551,6 +1551,12 @@ const char* lbdem_batch_last_error(const lbdem_batch*) { return "the CPU oracle 
 int lbdem_batch_destroy(lbdem_batch*) { return 0; }
 int lbdem_batch_set_tile(lbdem_batch*, int) { return LBDEM_E_STATE; }
 int lbdem_batch_get_tile(const lbdem_batch*, int*, int*) { return LBDEM_E_STATE; }
+int lbdem_batch_profile_enable(lbdem_batch*, int) { return LBDEM_E_STATE; }
+int lbdem_batch_profile_read(const lbdem_batch*, int, const char** kernel, double*, int*, int* count) {
+    if (kernel) *kernel = nullptr;
+    if (count) *count = 0;
+    return LBDEM_E_STATE;
+}
 int lbdem_pool_create(size_t, int, int, int, lbdem_pool** out) {
     if (out) *out = nullptr;
     return LBDEM_E_STATE;

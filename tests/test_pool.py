@@ -223,8 +223,9 @@ def test_batch_rejects_members_that_cannot_share_launches(gpu_lib, runs):
 
 
 def test_batch_of_mixed_sizes_equals_alone(gpu_lib):
-    """Boxes whose fused-kernel grids differ by more than 2x (48^3: 72 blocks, 96^3: 576) are
-    launched in grid-size classes; every result still equals the run alone (same z-walk)."""
+    """Boxes whose fused-kernel grids differ by 8x (48^3: 72 blocks, 96^3: 576) share each launch,
+    whose grid is the largest box's (the surplus blocks of the small boxes return at once); every
+    result still equals the run alone (same z-walk)."""
     rng = np.random.default_rng(11)
     runs = [workloads.random_impact_run(rng, (e, e, e), 60, relax=100) for e in (48, 96, 48)]
     prec = lbdem.PREC_FP32
@@ -272,4 +273,40 @@ def test_batch_tile_tuner(gpu_lib):
     for run, sim in zip(runs, sims):
         assert sim.info()["lbTime"] == 60
         assert np.isfinite(sim.total_mass())
+        sim.close()
+
+
+def test_batch_kernel_timing(gpu_lib):
+    """lbdem_batch_profile_enable / _read: with timing on, every launch of the mixed-size batch is
+    timed (the fused kernel once per step for all three boxes) and the runs still equal the runs
+    alone."""
+    rng = np.random.default_rng(11)
+    runs = [workloads.random_impact_run(rng, (e, e, e), 60, relax=100) for e in (48, 96, 48)]
+    prec = lbdem.PREC_FP32
+    ref = [alone(gpu_lib, run, prec, tile=BATCH_TILE) for run in runs]
+    sims = []
+    for run in runs:
+        sim = lbdem.Simulation(gpu_lib, lbdem.params_from_values(run.values, prec), run.particles, run.objects)
+        sim.init()
+        sim.set_tile(BATCH_TILE)
+        sim.cycle(WARM)
+        sims.append(sim)
+    batch = lbdem.Batch(gpu_lib, sims)
+    batch.cycle(10)
+    batch.profile(True)
+    assert batch.profile_read() == []
+    batch.cycle(20)
+    rows = batch.profile_read()
+    print("batch kernels", [(r["kernel"], r["launches"]) for r in rows[:6]], flush=True)
+    fused = [r for r in rows if r["kernel"].startswith("k_fused<float")]
+    assert fused and sum(r["launches"] for r in fused) == 20
+    assert all(r["ms"] > 0 and r["launches"] > 0 for r in rows)
+    assert len({r["kernel"] for r in rows}) == len(rows)
+    batch.profile(False)
+    batch.cycle(60 - WARM - 30)
+    batch.synchronize()
+    batch.close()
+    for k, sim in enumerate(sims):
+        res = dict(objects=sim.objects(), totalMass=sim.total_mass(), nActive=sim.info()["nActive"])
+        same(res, ref[k], f"box {runs[k].edge} through a timed batch")
         sim.close()
