@@ -60,7 +60,16 @@ typedef struct lbdem_params {
     double lbF[3];               /* physical acceleration */
     int boundary[6];             /* boundary0..5: 4 periodic, 7 no-slip static wall */
     double fluidHeight;          /* physical */
-    double initVisc;             /* physical dynamic viscosity (NEWTONIAN) */
+    double initVisc;             /* physical dynamic viscosity (NEWTONIAN, FRICTIONAL, VOELLMY; the
+                                    reference keeps its default (1.0 - 0.5) / 3.0 for BINGHAM) */
+    /* rheology and Smagorinsky turbulence (LB.cpp:197-245, node.cpp:113-168) */
+    int rheologyModel;           /* LBDEM_RHEO_*: 0 NEWTONIAN, 1 BINGHAM, 2 FRICTIONAL, 3 VOELLMY */
+    double plasticVisc;          /* physical dynamic viscosity (BINGHAM) */
+    double yieldStress;          /* physical stress (BINGHAM) */
+    double frictionCoefFluid;    /* FRICTIONAL, VOELLMY */
+    double voellmyConstant;      /* physical acceleration (VOELLMY) */
+    int turbulenceSolve;         /* Smagorinsky term added to the apparent viscosity */
+    double turbConst;            /* lattice units (the reference does not scale it) */
     /* DEM.cpp:19-75 */
     double density, objectDensity, bodyDensity, linearStiff, frictionCoefPart;
     double lubNormal, lubTangential, lubRotation, lubCutoff, kinVisc;
@@ -70,6 +79,9 @@ typedef struct lbdem_params {
     int precision;               /* LBDEM_PREC_* (the CPU oracle always uses FP64) */
     int device;                  /* HIP device ordinal */
 } lbdem_params;
+
+/* rheologyModel values (node.h:17) */
+enum { LBDEM_RHEO_NEWTONIAN = 0, LBDEM_RHEO_BINGHAM = 1, LBDEM_RHEO_FRICTIONAL = 2, LBDEM_RHEO_VOELLMY = 3 };
 
 /* Per-particle record layout used by lbdem_get_particles (doubles). */
 enum {
@@ -126,8 +138,9 @@ int lbdem_set_objects(lbdem_ctx* ctx, int n, const double* r, const double* x0, 
  * taken at every node whose initial type is fluid (codes 0, 2 and 3 are accepted there); records:
  * LBDEM_RESTART_REC values per record, n - 1, u (3), mass - 1, visc, f (19), consumed in node
  * index order by the active nodes of the restarted box (extra records are ignored, as the
- * reference ignores them). Every record's visc must equal the run's viscosity (initVisc in
- * lattice units): the library relaxes every node with that one viscosity. */
+ * reference ignores them). Each record's visc becomes the node's viscosity (node.cpp:40-50); a
+ * NEWTONIAN run without turbulence keeps per-node viscosities only when some record's visc
+ * differs from initVisc (in lattice units). */
 #define LBDEM_RESTART_REC 25
 int lbdem_set_restart_fluid(lbdem_ctx* ctx, const int size[3], const uint8_t* types, int nRecords,
                             const double* records);

@@ -27,7 +27,7 @@ namespace {
 
 struct Units {  // conversion units (node.cpp:512-525)
     double Length = 1, Time = 1, Density = 1, Volume = 1, Speed = 1, Accel = 1, AngVel = 1, KinVisc = 1, DynVisc = 1,
-           Force = 1, Torque = 1, Mass = 1;
+           Force = 1, Torque = 1, Mass = 1, Stress = 1;
     void setComposite() {
         Volume = Length * Length * Length;
         Speed = Length / Time;
@@ -38,6 +38,7 @@ struct Units {  // conversion units (node.cpp:512-525)
         Force = Density * Length * Length * Length * Length / Time / Time;
         Torque = Density * Length * Length * Length * Length * Length / Time / Time;
         Mass = Density * Length * Length * Length;
+        Stress = Density * Length * Length / Time / Time;
     }
 };
 
@@ -55,6 +56,11 @@ struct lbdem_ctx {
     double bLoc[6][3];
     double lbF[3] = {0, 0, 0}, initVel[3] = {0, 0, 0};
     double initDynVisc = 0, rho0 = 1.0, fluidHeight = 0;
+    // rheology in lattice units (LB.cpp:320-323); viscUpdate: the collision recomputes each node's
+    // viscosity (non-Newtonian model or turbulence); viscPlane: nodes keep their own viscosity
+    // (viscUpdate, or restart records with viscosities other than initDynVisc)
+    double plasticVisc = 0, yieldStress = 0, voellmyCoef = 0;
+    bool viscUpdate = false, viscPlane = false;
     bool freeSurface = false, forceField = false, lbFReset = false;
     // inputs
     std::vector<int> psize;
@@ -197,6 +203,13 @@ static int read_params(lbdem_ctx* c) {
             return c->fail(LBDEM_E_ARG, "only boundary types 4 (periodic) and 7 (static no-slip wall) are supported");
     c->fluidHeight = P.fluidHeight;
     c->initDynVisc = P.initVisc / u.DynVisc;
+    if (P.rheologyModel < LBDEM_RHEO_NEWTONIAN || P.rheologyModel > LBDEM_RHEO_VOELLMY)
+        return c->fail(LBDEM_E_ARG, "unknown rheologyModel");
+    c->plasticVisc = P.plasticVisc / u.DynVisc;
+    c->yieldStress = P.yieldStress / u.Stress;
+    c->voellmyCoef = P.voellmyConstant / u.Accel;
+    c->viscUpdate = P.rheologyModel != LBDEM_RHEO_NEWTONIAN || P.turbulenceSolve != 0;
+    c->viscPlane = c->viscUpdate;
     c->freeSurface = P.freeSurface != 0;
     c->forceField = P.forceField != 0;
     if (!P.lbSolve) return c->fail(LBDEM_E_ARG, "lbSolve=0 is not supported");
@@ -357,6 +370,7 @@ struct Plan {
     int nl[3] = {0, 0, 0};
     size_t selt = 8, celt = 8;
     int ifCap = 0, objCap = 0;
+    bool visc = false;  // per-node viscosity plane
 };
 
 static int wall_count(const lbdem_ctx* c) {
@@ -399,6 +413,7 @@ static int make_plan(lbdem_ctx* c, Plan& p) {
         objCells += int64_t(std::ceil(8.0 * r * r * r));
     }
     p.objCap = int(std::min<int64_t>(std::max<int64_t>(objCells, 1024), g.N));
+    p.visc = c->viscPlane;
     return 0;
 }
 
@@ -443,6 +458,8 @@ static bool plan_buffers(const Plan& p, Box& b, DemState& d, DevScalars*& sc, do
     A(b.blockPart, size_t(b.blockPartCap));
     A(b.blockFlag, size_t(std::max(fused_grid_blocks(p.g, 1, LBDEM_PREC_FP32), fused_grid_blocks(p.g, 1, LBDEM_PREC_FP16))));
     A(b.bcm, N);
+    if (p.visc) A(b.visc, N * p.celt);
+    else b.visc = nullptr;
     A(b.hacc, np1 * 8);
     A(sc, 1);
     // DEM
@@ -634,9 +651,10 @@ extern "C" int lbdem_init(lbdem_ctx* c) {
         if (int64_t(rNode.size()) > c->rCount)
             return c->fail(LBDEM_E_ARG, "the restart file holds " + std::to_string(c->rCount) + " records for " +
                                             std::to_string(rNode.size()) + " active nodes");
-        for (size_t r = 0; r < rNode.size(); ++r)
-            if (c->rRec[r * LBDEM_RESTART_REC + 5] != c->initDynVisc)
-                return c->fail(LBDEM_E_ARG, "a restart record's viscosity differs from the run's (initVisc)");
+        // a record with another viscosity than the run's: nodes keep their own (node.cpp:40-50)
+        for (size_t r = 0; r < rNode.size() && !c->viscPlane; ++r)
+            if (c->rRec[r * LBDEM_RESTART_REC + 5] != c->initDynVisc) c->viscPlane = true;
+        plan.visc = c->viscPlane;
     }
 
     Box& b = c->box;
@@ -721,6 +739,16 @@ extern "C" int lbdem_init(lbdem_ctx* c) {
     k.rho0 = c->rho0;
     k.omega = 1.0 / (0.5 + 3.0 * k.visc);
     k.omegaf = 1.0 - 1.0 / (1.0 + 6.0 * k.visc);
+    k.rheo = c->prm.rheologyModel;
+    k.turb = c->prm.turbulenceSolve != 0 ? 1 : 0;
+    k.viscUpdate = c->viscUpdate ? 1 : 0;
+    k.plasticVisc = c->plasticVisc;
+    k.yieldStress = c->yieldStress;
+    k.frictionCoef = c->prm.frictionCoefFluid;
+    k.voellmyCoef = c->voellmyCoef;
+    k.turbConst = c->prm.turbConst;
+    k.minVisc = (0.500001 - 0.5) / 3.0;  // lattice.h:26-27 (minTau, maxTau), node.cpp:116-117
+    k.maxVisc = (1.8 - 0.5) / 3.0;
     k.freeSurface = c->freeSurface ? 1 : 0;
     k.nPart = np; k.nObj = no; k.nWall = int(walls.size());
     k.unitLength = u.Length; k.unitSpeed = u.Speed; k.unitAngVel = u.AngVel;
@@ -1184,7 +1212,13 @@ extern "C" int lbdem_get_velocity(lbdem_ctx* c, double* u, double* visc) {
         HIPCHK(hipMemcpyAsync(u, d, size_t(3) * N * 8, hipMemcpyDeviceToHost, c->stream));
         HIPCHK(hipStreamSynchronize(c->stream));
     }
-    if (visc) {  // every active node relaxes with the run's viscosity
+    if (visc && c->box.visc) {
+        double* d = c->tmp(size_t(N) * 8);
+        if (!d) return c->fail(LBDEM_E_NOMEM, "device allocation failed");
+        lb_download_visc(c->box, d, c->stream);
+        HIPCHK(hipMemcpyAsync(visc, d, size_t(N) * 8, hipMemcpyDeviceToHost, c->stream));
+        HIPCHK(hipStreamSynchronize(c->stream));
+    } else if (visc) {  // every active node relaxes with the run's viscosity
         std::vector<uint8_t> t(N);
         HIPCHK(hipMemcpyAsync(t.data(), c->box.type, N, hipMemcpyDeviceToHost, c->stream));
         HIPCHK(hipStreamSynchronize(c->stream));

@@ -105,7 +105,7 @@ __device__ inline void equilibrium_offsets(C n, C ux, C uy, C uz, C* geq) {
 // ------------------------------------------------------------------------------------------
 template <typename S>
 __device__ inline void k_init_fields(Geom g, S* f, const uint8_t* type, typename Types<S>::M* mass,
-                              typename Types<S>::C* aux, Consts k) {
+                              typename Types<S>::C* aux, typename Types<S>::C* visc, Consts k) {
     using C = typename Types<S>::C;
     const int64_t i = int64_t(blockIdx.x) * blockDim.x + threadIdx.x;
     if (i >= g.N) return;
@@ -127,6 +127,7 @@ __device__ inline void k_init_fields(Geom g, S* f, const uint8_t* type, typename
     aux[i] = C(n);
     aux[g.N + i] = C(ux); aux[2 * g.N + i] = C(uy); aux[3 * g.N + i] = C(uz);
     aux[4 * g.N + i] = C(0); aux[5 * g.N + i] = C(0); aux[6 * g.N + i] = C(0);
+    if (visc) visc[i] = C(k.visc);  // initDynVisc (LB.cpp:1083-1090)
 }
 
 // restartVariables (LB.cpp:1098-1138) for record r of the restart file, at active node node[r]:
@@ -134,7 +135,7 @@ __device__ inline void k_init_fields(Geom g, S* f, const uint8_t* type, typename
 // node.cpp:38-51, with initDensity 1); hydrodynamic force zero (a fresh node)
 template <typename S>
 __device__ inline void k_restart_fields(Geom g, S* f, typename Types<S>::M* mass, typename Types<S>::C* aux,
-                                        const double* rec, const int32_t* node, int nrec) {
+                                        typename Types<S>::C* visc, const double* rec, const int32_t* node, int nrec) {
     using C = typename Types<S>::C;
     const int r = int(blockIdx.x * blockDim.x + threadIdx.x);
     if (r >= nrec) return;
@@ -145,15 +146,30 @@ __device__ inline void k_restart_fields(Geom g, S* f, typename Types<S>::M* mass
     aux[i] = C(v[0] + 1.0);
     aux[g.N + i] = C(v[1]); aux[2 * g.N + i] = C(v[2]); aux[3 * g.N + i] = C(v[3]);
     aux[4 * g.N + i] = C(0); aux[5 * g.N + i] = C(0); aux[6 * g.N + i] = C(0);
+    if (visc) visc[i] = C(v[5]);
 }
 void lb_restart_fields(const Box& b, const double* rec, const int32_t* node, int nrec, hipStream_t s) {
     const int bs = 256, nb = (nrec + bs - 1) / bs;
     if (b.prec == LBDEM_PREC_FP64)
-        launch<&k_restart_fields<double>, 1024, 1>(nb, bs, s, b.g, (double*)b.f, (double*)b.mass, (double*)b.aux, rec, node, nrec);
+        launch<&k_restart_fields<double>, 1024, 1>(nb, bs, s, b.g, (double*)b.f, (double*)b.mass, (double*)b.aux,
+                                                   (double*)b.visc, rec, node, nrec);
     else if (b.prec == LBDEM_PREC_FP32)
-        launch<&k_restart_fields<float>, 1024, 1>(nb, bs, s, b.g, (float*)b.f, (float*)b.mass, (float*)b.aux, rec, node, nrec);
+        launch<&k_restart_fields<float>, 1024, 1>(nb, bs, s, b.g, (float*)b.f, (float*)b.mass, (float*)b.aux,
+                                                  (float*)b.visc, rec, node, nrec);
     else
-        launch<&k_restart_fields<__half>, 1024, 1>(nb, bs, s, b.g, (__half*)b.f, (float*)b.mass, (float*)b.aux, rec, node, nrec);
+        launch<&k_restart_fields<__half>, 1024, 1>(nb, bs, s, b.g, (__half*)b.f, (float*)b.mass, (float*)b.aux,
+                                                   (float*)b.visc, rec, node, nrec);
+}
+
+// node viscosities (the per-node plane), zero at nodes that are not active
+template <typename C>
+__device__ inline void k_download_visc(int64_t N, const uint8_t* type, const C* visc, double* out) {
+    for (int64_t i = int64_t(blockIdx.x) * blockDim.x + threadIdx.x; i < N; i += int64_t(gridDim.x) * blockDim.x)
+        out[i] = is_active(type[i]) ? double(visc[i]) : 0.0;
+}
+void lb_download_visc(const Box& b, double* out, hipStream_t s) {
+    if (b.prec == LBDEM_PREC_FP64) launch<&k_download_visc<double>, 1024, 1>(4096, 256, s, b.g.N, b.type, (const double*)b.visc, out);
+    else launch<&k_download_visc<float>, 1024, 1>(4096, 256, s, b.g.N, b.type, (const float*)b.visc, out);
 }
 
 // velocity of every active node (aux planes 1-3), x y z per node, zero elsewhere
@@ -173,11 +189,14 @@ void lb_init_fields(const Box& b, const Consts& k, hipStream_t s) {
     const int bs = 256;
     const int nb = int((b.g.N + bs - 1) / bs);
     if (b.prec == LBDEM_PREC_FP64)
-        launch<&k_init_fields<double>, 1024, 1>(nb, bs, s, b.g, (double*)b.f, b.type, (double*)b.mass, (double*)b.aux, k);
+        launch<&k_init_fields<double>, 1024, 1>(nb, bs, s, b.g, (double*)b.f, b.type, (double*)b.mass, (double*)b.aux,
+                                                (double*)b.visc, k);
     else if (b.prec == LBDEM_PREC_FP32)
-        launch<&k_init_fields<float>, 1024, 1>(nb, bs, s, b.g, (float*)b.f, b.type, (float*)b.mass, (float*)b.aux, k);
+        launch<&k_init_fields<float>, 1024, 1>(nb, bs, s, b.g, (float*)b.f, b.type, (float*)b.mass, (float*)b.aux,
+                                               (float*)b.visc, k);
     else
-        launch<&k_init_fields<__half>, 1024, 1>(nb, bs, s, b.g, (__half*)b.f, b.type, (float*)b.mass, (float*)b.aux, k);
+        launch<&k_init_fields<__half>, 1024, 1>(nb, bs, s, b.g, (__half*)b.f, b.type, (float*)b.mass, (float*)b.aux,
+                                                (float*)b.visc, k);
 }
 
 // ------------------------------------------------------------------------------------------
@@ -201,6 +220,7 @@ struct FusedArgs {
     const int32_t* solid;
     typename Types<S>::M* mass;
     typename Types<S>::C* aux;
+    typename Types<S>::C* visc;  // node viscosities (the NN variant of the kernel only)
     int32_t* ifList;
     int ifCap;
     const PartLB* plb;
@@ -354,11 +374,53 @@ template <typename S>
 struct CellOut {
     typename Types<S>::C n, sx, sy, sz, usq, hx, hy, hz;
     typename Types<S>::C d[7];  // particle contribution: F(3), M(3), fluid volume
+    typename Types<S>::C visc;  // the node's viscosity after the collision (NN variant)
     int key;                    // particle index or -1
 };
-template <typename S>
+
+// apparent viscosity (node.cpp:113-168): the shear-rate tensor from the non-equilibrium part of the
+// post-streaming populations, relaxed with the node's previous viscosity nu, its magnitude
+// (tinyMatrix::magnitude, myvector.cpp:546-550), the rheology model's viscosity plus the
+// Smagorinsky term, clamped as std::max(minVisc, std::min(maxVisc, .)) clamps it (a NaN from 0 / 0
+// becomes maxVisc). geq is the equilibrium in offset form, as the collision below uses it.
+template <typename C>
+__device__ inline C apparent_viscosity(const Consts& k, const C* gc, C n, C sx, C sy, C sz, C usq, C nu) {
+    C xx = 0, yy = 0, zz = 0, xy = 0, xz = 0, yz = 0;
+#pragma unroll
+    for (int q = 0; q < Q; ++q) {
+        const C cu = C(cxq(q)) * sx + C(cyq(q)) * sy + C(czq(q)) * sz;
+        const C geq = C(wq(q)) * ((n - C(1)) + n * (C(3) * cu + C(4.5) * cu * cu - C(1.5) * usq));
+        const C d = gc[q] - geq;
+        if (cxq(q)) xx += d;
+        if (cyq(q)) yy += d;
+        if (czq(q)) zz += d;
+        if (cxq(q) && cyq(q)) xy += C(cxq(q) * cyq(q)) * d;
+        if (cxq(q) && czq(q)) xz += C(cxq(q) * czq(q)) * d;
+        if (cyq(q) && czq(q)) yz += C(cyq(q) * czq(q)) * d;
+    }
+    const C tau = C(0.5) + C(3) * nu;
+    const C sc = C(1.5) / (tau * n);
+    xx *= sc; yy *= sc; zz *= sc; xy *= sc; xz *= sc; yz *= sc;
+    const C shear = C(2) * sqrt(C(0.5) * (xx * xx + yy * yy + zz * zz + C(2) * (xy * xy + xz * xz + yz * yz)));
+    C app;
+    if (k.rheo == LBDEM_RHEO_BINGHAM) {
+        app = C(k.plasticVisc) + C(k.yieldStress) / shear;
+    } else if (k.rheo == LBDEM_RHEO_FRICTIONAL || k.rheo == LBDEM_RHEO_VOELLMY) {
+        const C pr = C(0.33333333) * (n - C(1));
+        const C pressure = C(0) < pr ? pr : C(0);  // std::max(0.0, .)
+        app = k.rheo == LBDEM_RHEO_FRICTIONAL ? C(k.visc) + C(k.frictionCoef) * pressure / shear
+                                              : (C(k.frictionCoef) * pressure + usq / C(k.voellmyCoef)) / shear;
+    } else {
+        app = C(k.visc);
+    }
+    if (k.turb) app += C(k.turbConst) * shear;
+    const C lo = C(k.minVisc), hi = C(k.maxVisc);
+    const C m = app < hi ? app : hi;  // std::min(maxVisc, app)
+    return lo < m ? m : lo;           // std::max(minVisc, .)
+}
+template <typename S, bool NN>
 __device__ inline void collide_cell(const FusedArgs<S>& a, int32_t sp, uint8_t t, int x, int y, int z, double mi,
-                                    typename Types<S>::C* gc, CellOut<S>& r) {
+                                    typename Types<S>::C* gc, typename Types<S>::C nuIn, CellOut<S>& r) {
     using C = typename Types<S>::C;
     const Consts& k = a.k;
     C sum = 0, jx = 0, jy = 0, jz = 0;
@@ -390,7 +452,13 @@ __device__ inline void collide_cell(const FusedArgs<S>& a, int32_t sp, uint8_t t
     const C sx = ux + C(0.5) * Fx * inv, sy = uy + C(0.5) * Fy * inv, sz = uz + C(0.5) * Fz * inv;
     const C usq = sx * sx + sy * sy + sz * sz;
     const C uF = sx * Fx + sy * Fy + sz * Fz;
-    const C om = C(k.omega), omf = C(k.omegaf);
+    C om = C(k.omega), omf = C(k.omegaf);
+    if constexpr (NN) {  // the node's own viscosity, updated first when the model asks for it
+        const C nu = k.viscUpdate ? apparent_viscosity<C>(k, gc, n, sx, sy, sz, usq, nuIn) : nuIn;
+        om = C(1) / (C(0.5) + C(3) * nu);
+        omf = C(1) - C(1) / (C(1) + C(6) * nu);
+        r.visc = nu;
+    }
 #pragma unroll
     for (int q = 0; q < Q; ++q) {
         const C cu = C(cxq(q)) * sx + C(cyq(q)) * sy + C(czq(q)) * sz;
@@ -505,7 +573,7 @@ __device__ inline void load_bundle(const FusedArgs<S>& a, const RowYZ& r, const 
     n.s = (n.act && (t & T_PMASK)) ? a.solid[i] : 0;
 }
 
-template <typename S, int PAR, bool FSURF, bool DIAG, int FZ>
+template <typename S, int PAR, bool FSURF, bool DIAG, int FZ, bool NN>
 __device__ inline void k_fused(FusedArgs<S> a) {
     using C = typename Types<S>::C;
     using M = typename Types<S>::M;
@@ -600,7 +668,14 @@ __device__ inline void k_fused(FusedArgs<S> a) {
 #pragma unroll
         for (int c = 0; c < 7; ++c) r.d[c] = 0;
         r.n = 1; r.sx = 0; r.sy = 0; r.sz = 0; r.usq = 0;
-        if (actc) collide_cell<S>(a, sc_, tc, x, y, z, mc, gc, r);
+        C nu = 0;
+        if constexpr (NN) {
+            if (actc) nu = a.visc[ic];
+        }
+        if (actc) collide_cell<S, NN>(a, sc_, tc, x, y, z, mc, gc, nu, r);
+        if constexpr (NN) {
+            if (actc && a.k.viscUpdate) a.visc[ic] = r.visc;
+        }
         __builtin_amdgcn_s_setprio(1);
         store_pop<S, PAR>(a, RowYZ(g, yw, z), lx, actc, gc);
         __builtin_amdgcn_s_setprio(0);
@@ -848,6 +923,7 @@ static void launch_fused(Box& b, const Consts& k, const PartLB* plb, const ObjLB
     a.g = b.g; a.f = (S*)b.f; a.type = b.type; a.solid = b.solid;
     for (int q = 0; q < Q; ++q) a.fq[q] = a.f + int64_t(q) * b.g.N;
     a.mass = (typename Types<S>::M*)b.mass; a.aux = (typename Types<S>::C*)b.aux;
+    a.visc = (typename Types<S>::C*)b.visc;
     a.ifList = b.ifList; a.ifCap = b.ifCap;
     a.plb = plb; a.olb = olb; a.hacc = b.hacc; a.blockPart = b.blockPart; a.blockFlag = b.blockFlag; a.bcm = b.bcm; a.sc = sc;
     a.k = k;
@@ -859,20 +935,24 @@ static void launch_fused(Box& b, const Consts& k, const PartLB* plb, const ObjLB
     const bool fs = k.freeSurface != 0;
     if (ev0) (void)hipEventRecord(ev0, s);
     (void)diag;  // the wall diagnostics are always accumulated
-#define LBDEM_FUSED(P, F, Z) \
-    launch<&k_fused<S, P, F, true, Z>, FX * FY, FusedOcc<S>::value>(dim3(grid), dim3(FX, FY, 1), s, a)
-#define LBDEM_FUSED_Z(P, F)                 \
-    switch (fz) {                          \
-        case 1: LBDEM_FUSED(P, F, 1); break; \
-        case 2: LBDEM_FUSED(P, F, 2); break; \
-        case 4: LBDEM_FUSED(P, F, 4); break; \
-        default: LBDEM_FUSED(P, F, 8); break; \
+    const bool nn = b.visc != nullptr;
+#define LBDEM_FUSED(P, F, Z, V) \
+    launch<&k_fused<S, P, F, true, Z, V>, FX * FY, FusedOcc<S>::value>(dim3(grid), dim3(FX, FY, 1), s, a)
+#define LBDEM_FUSED_Z(P, F, V)                 \
+    switch (fz) {                             \
+        case 1: LBDEM_FUSED(P, F, 1, V); break; \
+        case 2: LBDEM_FUSED(P, F, 2, V); break; \
+        case 4: LBDEM_FUSED(P, F, 4, V); break; \
+        default: LBDEM_FUSED(P, F, 8, V); break; \
     }
-    if (b.parity == 0) {
-        if (fs) { LBDEM_FUSED_Z(0, true); } else { LBDEM_FUSED_Z(0, false); }
+#define LBDEM_FUSED_FS(P, V) \
+    if (fs) { LBDEM_FUSED_Z(P, true, V); } else { LBDEM_FUSED_Z(P, false, V); }
+    if (nn) {
+        if (b.parity == 0) { LBDEM_FUSED_FS(0, true) } else { LBDEM_FUSED_FS(1, true) }
     } else {
-        if (fs) { LBDEM_FUSED_Z(1, true); } else { LBDEM_FUSED_Z(1, false); }
+        if (b.parity == 0) { LBDEM_FUSED_FS(0, false) } else { LBDEM_FUSED_FS(1, false) }
     }
+#undef LBDEM_FUSED_FS
 #undef LBDEM_FUSED_Z
 #undef LBDEM_FUSED
     if (ev1) (void)hipEventRecord(ev1, s);
@@ -1102,8 +1182,8 @@ __device__ inline void k_fs_donors(Geom g, const uint8_t* type, const int32_t* f
 // u, visc and hydroForce+lbF, mass 0.01*rho0)
 template <typename S, int L>
 __device__ inline void k_fs_new_from_donor(Geom g, S* f, uint8_t* type, typename Types<S>::M* mass,
-                                    typename Types<S>::C* aux, const int32_t* newIf, const int32_t* donor,
-                                    DevScalars* sc, Consts k, int32_t* changed, int changedCap) {
+                                    typename Types<S>::C* aux, typename Types<S>::C* visc, const int32_t* newIf,
+                                    const int32_t* donor, DevScalars* sc, Consts k, int32_t* changed, int changedCap) {
     using C = typename Types<S>::C;
     const int nN = sc->nNewIf;
     for (int kk = blockIdx.x * blockDim.x + threadIdx.x; kk < nN; kk += gridDim.x * blockDim.x) {
@@ -1129,6 +1209,7 @@ __device__ inline void k_fs_new_from_donor(Geom g, S* f, uint8_t* type, typename
         aux[gi] = C(n);
         aux[N + gi] = C(ux); aux[2 * N + gi] = C(uy); aux[3 * N + gi] = C(uz);
         aux[4 * N + gi] = C(0); aux[5 * N + gi] = C(0); aux[6 * N + gi] = C(0);
+        if (visc) visc[gi] = visc[d];  // the donor's viscosity (LB.cpp:1719)
     }
     if (blockIdx.x == 0 && threadIdx.x == 0) atomicAdd(&sc->surplus, -0.01 * k.rho0 * double(nN));
 }
@@ -1291,9 +1372,11 @@ static void fs_impl(Box& b, const Consts& k, DevScalars* sc, hipStream_t s) {
     launch_gs<&k_fs_mutants<S>, 256, 1>(grid, bs, s, b.type, mass, aux, b.g.N, b.ifList, (M*)b.ifNewMass, b.filled, b.emptied, b.ifCap, sc, b.donor, b.changed, b.changedCap);
     launch_gs<&k_fs_donors, 256, 1>(grid, bs, s, b.g, b.type, b.filled, b.donor, b.newIf, b.ifCap, sc);
     if (b.parity == 1)
-        launch_gs<&k_fs_new_from_donor<S, 1>, 256, 1>(grid, bs, s, b.g, f, b.type, mass, aux, b.newIf, b.donor, sc, k, b.changed, b.changedCap);
+        launch_gs<&k_fs_new_from_donor<S, 1>, 256, 1>(grid, bs, s, b.g, f, b.type, mass, aux, (C*)b.visc, b.newIf, b.donor, sc, k,
+                                                      b.changed, b.changedCap);
     else
-        launch_gs<&k_fs_new_from_donor<S, 0>, 256, 1>(grid, bs, s, b.g, f, b.type, mass, aux, b.newIf, b.donor, sc, k, b.changed, b.changedCap);
+        launch_gs<&k_fs_new_from_donor<S, 0>, 256, 1>(grid, bs, s, b.g, f, b.type, mass, aux, (C*)b.visc, b.newIf, b.donor, sc, k,
+                                                      b.changed, b.changedCap);
     launch_gs<&k_fs_from_emptied<S>, 256, 1>(grid, bs, s, b.g, b.type, mass, aux, b.emptied, b.donor, b.newIf, b.ifCap, sc);
     launch_gs<&k_fs_update_mutants<S>, 256, 1>(grid, bs, s, b.type, mass, aux, b.filled, b.emptied, sc);
     launch_gs<&k_fs_isolated_a<S>, 256, 1>(grid, bs, s, b.g, b.type, mass, aux, b.ifList, b.newIf, b.donor, b.ifList2,
@@ -1608,9 +1691,9 @@ __device__ inline void k_obj_uncover_find(Geom g, const int32_t* objList, int32_
 template <typename S, int L>
 __device__ inline void k_obj_uncover(Geom g, S* f, uint8_t* type,
                                                                  typename Types<S>::M* mass,
-                                                                 typename Types<S>::C* aux, const int32_t* cand,
-                                                                 const int* blockCount, int G, DevScalars* sc,
-                                                                 Consts k, int32_t* changed, int changedCap) {
+                                                                 typename Types<S>::C* aux, typename Types<S>::C* visc,
+                                                                 const int32_t* cand, const int* blockCount, int G,
+                                                                 DevScalars* sc, Consts k, int32_t* changed, int changedCap) {
     using C = typename Types<S>::C;
     __shared__ int sWave[UNCOVER_THREADS / 64 + 1];
     __shared__ uint8_t nbT[Q - 1][UNC_CHUNK];   // neighbour base types (values before this pass)
@@ -1741,6 +1824,7 @@ __device__ inline void k_obj_uncover(Geom g, S* f, uint8_t* type,
                 aux[i] = C(nn);
                 aux[g.N + i] = C(ux); aux[2 * g.N + i] = C(uy); aux[3 * g.N + i] = C(uz);
                 aux[4 * g.N + i] = C(0); aux[5 * g.N + i] = C(0); aux[6 * g.N + i] = C(0);
+                if (visc) visc[i] = C(k.visc);  // initDynVisc (LB.cpp:928-934)
             }
         }
         __threadfence_block();
@@ -1871,9 +1955,11 @@ static void obj_impl(Box& b, const Consts& k, const ObjLB* olb, DevScalars* sc, 
     int* uncCount = b.coverCount + COVER_GRID;
     launch<&k_obj_uncover_find, UNCOVER_THREADS, 1>(grid, UNCOVER_THREADS, s, b.g, b.objList, b.objCand, uncCount, grid, sc, olb);
     if (b.parity == 0)
-        launch<&k_obj_uncover<S, 0>, UNCOVER_THREADS, 1>(1, UNCOVER_THREADS, s, b.g, (S*)b.f, b.type, (M*)b.mass, (C*)b.aux, b.objCand, uncCount, grid, sc, k, b.changed, b.changedCap);
+        launch<&k_obj_uncover<S, 0>, UNCOVER_THREADS, 1>(1, UNCOVER_THREADS, s, b.g, (S*)b.f, b.type, (M*)b.mass, (C*)b.aux,
+                                                         (C*)b.visc, b.objCand, uncCount, grid, sc, k, b.changed, b.changedCap);
     else
-        launch<&k_obj_uncover<S, 1>, UNCOVER_THREADS, 1>(1, UNCOVER_THREADS, s, b.g, (S*)b.f, b.type, (M*)b.mass, (C*)b.aux, b.objCand, uncCount, grid, sc, k, b.changed, b.changedCap);
+        launch<&k_obj_uncover<S, 1>, UNCOVER_THREADS, 1>(1, UNCOVER_THREADS, s, b.g, (S*)b.f, b.type, (M*)b.mass, (C*)b.aux,
+                                                         (C*)b.visc, b.objCand, uncCount, grid, sc, k, b.changed, b.changedCap);
     // the candidate buffer of the uncover pass stages the cover pass's per-block node lists
     launch_gs<&k_obj_cover_a, COVER_BS, 1>(COVER_GRID, COVER_BS, s, b.g, b.type, b.solid, olb, k.nObj, b.coverCount, b.objCand, b.objCap, b.changed, b.changedCap, sc);
     launch_gs<&k_obj_cover_b, COVER_BS, 1>(COVER_GRID, COVER_BS, s, b.g, b.type, olb, k.nObj, b.coverCount, b.objCand, b.objCap, b.objList, b.objCap, sc);

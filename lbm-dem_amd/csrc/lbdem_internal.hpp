@@ -178,6 +178,8 @@ struct Box {
     double* blockPart;         // per-block partials of the fused kernel (object F/M, extra mass, walls)
     int32_t* blockFlag;        // per block: partials written
     uint64_t* bcm;             // N, boundary-link classes of edge nodes (valid where T_EDGE is set)
+    void* visc;                // N compute-type node viscosities (lattice units), or null: every node
+                               // has the run's viscosity (Newtonian, no turbulence, no restart values)
     int blockPartCap;
     double* hacc;              // per-particle hydro accumulators: F3 M3 vol (lattice units)
     int parity;                // 0: populations in natural layout, 1: swapped
@@ -190,6 +192,10 @@ struct Consts {
     double visc;                // lattice kinematic viscosity (initDynVisc)
     double rho0;
     double omega, omegaf;       // 1/(0.5+3 nu), 1 - 1/(1+6 nu)
+    // apparent viscosity (node.cpp:113-168; with per-node viscosities only): rheology model
+    // (LBDEM_RHEO_*), its parameters in lattice units, the Smagorinsky constant, the clamp range
+    int rheo, turb, viscUpdate;
+    double plasticVisc, yieldStress, frictionCoef, voellmyCoef, turbConst, minVisc, maxVisc;
     int freeSurface;
     int nPart, nObj, nWall;
     double unitLength, unitSpeed, unitAngVel, unitForce, unitTorque, unitVolume;
@@ -201,6 +207,8 @@ void lb_init_fields(const Box& b, const Consts& k, hipStream_t s);
 void lb_restart_fields(const Box& b, const double* rec, const int32_t* node, int nrec, hipStream_t s);
 // 3 N values: the velocity held in aux for every active node, zero elsewhere
 void lb_download_velocity(const Box& b, double* out, hipStream_t s);
+// N values: the node viscosity plane (b.visc must be set) at active nodes, zero elsewhere
+void lb_download_visc(const Box& b, double* out, hipStream_t s);
 void lb_mark_edges(const Box& b, DevScalars* sc, hipStream_t s);
 // storeVel: every active node's density and velocity into aux (else the interface nodes' only)
 void lb_fused_step(Box& b, const Consts& k, const PartLB* plb, const ObjLB* olb, DevScalars* sc, bool diag,

@@ -77,7 +77,7 @@ void parseConfigFile(IO& io, DEM& dem, LB& lb, ConfigFile& cfg, CommandLine& cl)
 // LB
 // ------------------------------------------------------------------------------------------
 void LB::latticeBoltzmannGet(ConfigFile& cfg, CommandLine& cl) {
-    // keys and defaults of LB.cpp:137-336 (Newtonian, no topography, no turbulence)
+    // keys and defaults of LB.cpp:137-336 (no topography)
     lbdem_params& p = c_.params;
     p.unitLength = parse_value(cfg, cl, "unitLength", 1.0);
     p.unitTime = parse_value(cfg, cl, "unitTime", 1.0);
@@ -93,16 +93,22 @@ void LB::latticeBoltzmannGet(ConfigFile& cfg, CommandLine& cl) {
     for (int a = 0; a < 3; ++a) p.initVelocity[a] = parse_value(cfg, cl, std::string("initVelocity") + ax[a], 0.0);
     for (int a = 0; a < 3; ++a) p.lbF[a] = parse_value(cfg, cl, std::string("lbF") + ax[a], 0.0);
     for (int b = 0; b < 6; ++b) p.boundary[b] = parse_value(cfg, cl, "boundary" + std::to_string(b), 1);
+    // rheology (LB.cpp:197-245): an unknown name keeps NEWTONIAN; BINGHAM reads no initVisc and
+    // keeps the default (1.0 - 0.5) / 3.0, scaled at LB.cpp:320 like a physical viscosity
     const std::string rheology = parse_value(cfg, cl, "rheologyModel", "none");
-    if (rheology != "NEWTONIAN") {
-        std::cout << "ERROR: rheologyModel " << rheology << " is not supported (NEWTONIAN only)" << std::endl;
-        std::exit(1);
+    p.rheologyModel = rheology == "BINGHAM" ? LBDEM_RHEO_BINGHAM
+                    : rheology == "FRICTIONAL" ? LBDEM_RHEO_FRICTIONAL
+                    : rheology == "VOELLMY" ? LBDEM_RHEO_VOELLMY : LBDEM_RHEO_NEWTONIAN;
+    p.initVisc = p.rheologyModel == LBDEM_RHEO_BINGHAM ? (1.0 - 0.5) / 3.0 : parse_value(cfg, cl, "initVisc", 1.0);
+    if (p.rheologyModel == LBDEM_RHEO_BINGHAM) {
+        p.plasticVisc = parse_value(cfg, cl, "plasticVisc", 0.0);
+        p.yieldStress = parse_value(cfg, cl, "yieldStress", 0.0);
     }
-    p.initVisc = parse_value(cfg, cl, "initVisc", 1.0);
-    if (parse_value(cfg, cl, "turbulenceSolve", 0)) {
-        std::cout << "ERROR: turbulenceSolve is not supported" << std::endl;
-        std::exit(1);
-    }
+    if (p.rheologyModel == LBDEM_RHEO_FRICTIONAL || p.rheologyModel == LBDEM_RHEO_VOELLMY)
+        p.frictionCoefFluid = parse_value(cfg, cl, "frictionCoefFluid", 0.0);
+    if (p.rheologyModel == LBDEM_RHEO_VOELLMY) p.voellmyConstant = parse_value(cfg, cl, "voellmyConstant", 1.0);
+    p.turbulenceSolve = to_int(parse_value(cfg, cl, "turbulenceSolve", "0"), 0);
+    p.turbConst = parse_value(cfg, cl, "turbConst", 0.0);
     p.fluidHeight = parse_value(cfg, cl, "fluidHeight", 0.0);
     fluidHeight = p.fluidHeight;
 }

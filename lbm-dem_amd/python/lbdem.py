@@ -27,6 +27,7 @@ REPO = os.path.dirname(PKG)
 LIB_PATH = os.path.join(PKG, "lib", "liblbdem.so")
 
 PREC_FP64, PREC_FP32, PREC_FP16 = 0, 1, 2
+RHEO_NEWTONIAN, RHEO_BINGHAM, RHEO_FRICTIONAL, RHEO_VOELLMY = 0, 1, 2, 3  # include/lbdem.h LBDEM_RHEO_*
 Q = 19
 
 # record layouts (include/lbdem.h)
@@ -44,6 +45,9 @@ class Params(C.Structure):
         ("unitLength", C.c_double), ("unitTime", C.c_double), ("unitDensity", C.c_double),
         ("lbSize", C.c_double * 3), ("initVelocity", C.c_double * 3), ("lbF", C.c_double * 3),
         ("boundary", C.c_int * 6), ("fluidHeight", C.c_double), ("initVisc", C.c_double),
+        ("rheologyModel", C.c_int), ("plasticVisc", C.c_double), ("yieldStress", C.c_double),
+        ("frictionCoefFluid", C.c_double), ("voellmyConstant", C.c_double), ("turbulenceSolve", C.c_int),
+        ("turbConst", C.c_double),
         ("density", C.c_double), ("objectDensity", C.c_double), ("bodyDensity", C.c_double),
         ("linearStiff", C.c_double), ("frictionCoefPart", C.c_double),
         ("lubNormal", C.c_double), ("lubTangential", C.c_double), ("lubRotation", C.c_double),
@@ -539,6 +543,7 @@ DEFAULTS: Dict[str, object] = {
     # LB.cpp:137-336
     "lbRestart": 0, "lbTopography": 0, "unitLength": 1.0, "unitTime": 1.0, "unitDensity": 1.0,
     "lbSizeX": 0.0, "lbSizeY": 0.0, "lbSizeZ": 0.0, "rheologyModel": "none", "initVisc": 1.0,
+    "plasticVisc": 0.0, "yieldStress": 0.0, "frictionCoefFluid": 0.0, "voellmyConstant": 1.0,
     "turbulenceSolve": 0, "turbConst": 0.0,
     "initVelocityX": 0.0, "initVelocityY": 0.0, "initVelocityZ": 0.0,
     "lbFX": 0.0, "lbFY": 0.0, "lbFZ": 0.0,
@@ -594,12 +599,26 @@ def config_values(text: str, overrides: Optional[Dict[str, object]] = None) -> D
     return vals
 
 
+RHEOLOGY = {"NEWTONIAN": RHEO_NEWTONIAN, "BINGHAM": RHEO_BINGHAM, "FRICTIONAL": RHEO_FRICTIONAL,
+            "VOELLMY": RHEO_VOELLMY}
+
+
 def params_from_values(v: Dict[str, object], precision: int = PREC_FP64, device: int = 0) -> Params:
-    if str(v.get("rheologyModel", "NEWTONIAN")) not in ("NEWTONIAN",):
-        raise LbdemError("only rheologyModel = NEWTONIAN is supported")
-    if int(v.get("turbulenceSolve", 0)):
-        raise LbdemError("turbulenceSolve is not supported")
+    """lbdem_params from getpot values. The rheology keys are read as LB.cpp:197-245 reads them: an
+    unknown model name keeps NEWTONIAN; a BINGHAM run reads no initVisc and keeps the default
+    (1.0 - 0.5) / 3.0, which LB.cpp:320 then scales like a physical viscosity."""
     p = Params()
+    model = RHEOLOGY.get(str(v.get("rheologyModel", "none")), RHEO_NEWTONIAN)
+    p.rheologyModel = model
+    p.initVisc = (1.0 - 0.5) / 3.0 if model == RHEO_BINGHAM else float(v["initVisc"])
+    if model == RHEO_BINGHAM:
+        p.plasticVisc, p.yieldStress = float(v.get("plasticVisc", 0.0)), float(v.get("yieldStress", 0.0))
+    if model in (RHEO_FRICTIONAL, RHEO_VOELLMY):
+        p.frictionCoefFluid = float(v.get("frictionCoefFluid", 0.0))
+    if model == RHEO_VOELLMY:
+        p.voellmyConstant = float(v.get("voellmyConstant", 1.0))
+    p.turbulenceSolve = int(v.get("turbulenceSolve", 0))
+    p.turbConst = float(v.get("turbConst", 0.0))
     p.demInitialRepeat = float(v["demInitialRepeat"])
     p.maxTime = float(v["maxTime"])
     p.demSolve, p.lbSolve = int(v["demSolve"]), int(v["lbSolve"])
@@ -610,7 +629,6 @@ def params_from_values(v: Dict[str, object], precision: int = PREC_FP64, device:
     p.lbF[:] = [float(v["lbFX"]), float(v["lbFY"]), float(v["lbFZ"])]
     p.boundary[:] = [int(v[f"boundary{i}"]) for i in range(6)]
     p.fluidHeight = float(v["fluidHeight"])
-    p.initVisc = float(v["initVisc"])
     for k in ("density", "objectDensity", "bodyDensity", "linearStiff", "frictionCoefPart", "lubNormal",
               "lubTangential", "lubRotation", "lubCutoff", "kinVisc", "electroForce", "debyeLength"):
         setattr(p, k, float(v[k]))

@@ -101,6 +101,10 @@ struct lbdem_ctx {
     int boundary[6];
     double fluidHeight = 0;
     double initDynVisc = 0, initDensity = 1.0;
+    // FluidMaterial (node.h:17-49): rheology model and its parameters in lattice units
+    int rheologyModel = LBDEM_RHEO_NEWTONIAN;
+    double plasticVisc = 0, yieldStress = 0, frictionCoefFluid = 0, voellmyCoefficient = 0, turbConst = 0;
+    bool turbulenceOn = false;
     V3 initVelocity, lbF;
     bool freeSurface = false, forceField = false;
     unsigned lbTime = 0;
@@ -251,7 +255,19 @@ struct lbdem_ctx {
             if (boundary[i] != 4 && boundary[i] != 7) return fail(LBDEM_E_ARG, "only boundary types 4 (periodic) and 7 (static no-slip wall) are supported");
         }
         fluidHeight = P.fluidHeight;
-        initDynVisc /= unit.DynVisc;
+        rheologyModel = P.rheologyModel;
+        if (rheologyModel < LBDEM_RHEO_NEWTONIAN || rheologyModel > LBDEM_RHEO_VOELLMY)
+            return fail(LBDEM_E_ARG, "unknown rheologyModel");
+        plasticVisc = P.plasticVisc;
+        yieldStress = P.yieldStress;
+        frictionCoefFluid = P.frictionCoefFluid;
+        voellmyCoefficient = P.voellmyConstant;
+        turbulenceOn = P.turbulenceSolve != 0;
+        turbConst = P.turbConst;
+        initDynVisc /= unit.DynVisc;  // LB.cpp:320-323
+        plasticVisc /= unit.DynVisc;
+        yieldStress /= unit.Stress;
+        voellmyCoefficient /= unit.Accel;
         initDensity = 1.0;
         freeSurface = P.freeSurface != 0;
         forceField = P.forceField != 0;
@@ -588,6 +604,44 @@ struct lbdem_ctx {
         }
         for (auto& P : particles) { P.FHydro *= unit.Force; P.MHydro *= unit.Torque; P.fluidVolume *= unit.Volume; }
     }
+    // node.cpp:113-168: shear rate from the non-equilibrium part relaxed with the node's previous
+    // viscosity, apparent viscosity of the rheology model plus the Smagorinsky term, clamped to the
+    // viscosities of tau in [minTau, maxTau] (lattice.h:26-27) with std::min / std::max
+    void apparentViscosity(uint32_t index, const double* fi, const double* feq, const V3& uu) {
+        const double minVisc = (0.500001 - 0.5) / 3.0, maxVisc = (1.8 - 0.5) / 3.0;
+        const double tau = 0.5 + 3.0 * visc[index];
+        double m[3][3] = {{0, 0, 0}, {0, 0, 0}, {0, 0, 0}};
+        for (int j = 0; j < Q; ++j) {
+            const V3 vj = vdir(j);
+            const double c[3] = {vj.x, vj.y, vj.z};
+            const double d = fi[j] - feq[j];
+            for (int a = 0; a < 3; ++a)
+                for (int b = 0; b < 3; ++b) m[a][b] += d * (c[a] * c[b]);
+        }
+        const double sc = 1.5 / (tau * n[index]);
+        for (int a = 0; a < 3; ++a)
+            for (int b = 0; b < 3; ++b) m[a][b] *= sc;
+        const double mag = std::sqrt(0.5 * (m[0][0] * m[0][0] + m[1][1] * m[1][1] + m[2][2] * m[2][2] +
+                                            2.0 * (m[0][1] * m[1][0] + m[2][0] * m[0][2] + m[1][2] * m[2][1])));
+        const double shearRate = 2.0 * mag;
+        double nuApp = 0.0;
+        switch (rheologyModel) {
+            case LBDEM_RHEO_BINGHAM: nuApp = plasticVisc + yieldStress / shearRate; break;
+            case LBDEM_RHEO_FRICTIONAL: {
+                const double pressure = std::max(0.0, 0.33333333 * (n[index] - 1.0));
+                nuApp = initDynVisc + frictionCoefFluid * pressure / shearRate;
+                break;
+            }
+            case LBDEM_RHEO_VOELLMY: {
+                const double pressure = std::max(0.0, 0.33333333 * (n[index] - 1.0));
+                nuApp = (frictionCoefFluid * pressure + uu.norm2() / voellmyCoefficient) / shearRate;
+                break;
+            }
+            default: nuApp = initDynVisc; break;
+        }
+        if (turbulenceOn) nuApp += turbConst * shearRate;
+        visc[index] = std::max(minVisc, std::min(maxVisc, nuApp));
+    }
     void collision() {  // LB.cpp:1301-1332 with node.cpp:95-98, 181-207
         if (!forceField) lbF.reset();
         const int64_t na = int64_t(activeNodes.size());
@@ -601,6 +655,7 @@ struct lbdem_ctx {
             setU(index, uu);
             double feq[Q];
             equilibrium(n[index], uu, feq);
+            if (rheologyModel != LBDEM_RHEO_NEWTONIAN || turbulenceOn) apparentViscosity(index, fi, feq, uu);
             const double omega = 1.0 / (0.5 + 3.0 * visc[index]);
             for (int j = 0; j < Q; j++) fi[j] += omega * (feq[j] - fi[j]);
             const double omegaf = 1.0 - 1.0 / (1.0 + 6.0 * visc[index]);

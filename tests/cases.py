@@ -132,6 +132,12 @@ def _case_values(name: str) -> Tuple[Dict[str, object], np.ndarray, np.ndarray, 
         pc = np.array([[8.05, 11.25, 8.05], [8.05, 6.95, 8.05], [3.4, 2.5, 3.5], [12.6, 5.0, 12.2],
                        [12.3, 12.0, 3.8], [4.0, 9.0, 12.0]])
         return v, pr, pc, np.array([3.0]), np.array([[8.0, 16.55, 8.0]]), np.array([[0.0, -0.06, 0.0]])
+    if name in RHEOLOGY_CASES:
+        # the collision with per-node apparent viscosity (node.cpp:113-168) on an existing geometry
+        base, extra = RHEOLOGY_CASES[name]
+        v, pr, pc, orr, oc, ov = _case_values(base)
+        v.update(extra)
+        return v, pr, pc, orr, oc, ov
     if name == "lb_closed":
         # no free surface, no spheres, no impactor: gravity-driven closed box
         v.update(lbSizeX=10.0, lbSizeY=12.0, lbSizeZ=9.0, freeSurfaceSolve=0, fluidHeight=0.0, lbFX=1.0e-4)
@@ -140,6 +146,16 @@ def _case_values(name: str) -> Tuple[Dict[str, object], np.ndarray, np.ndarray, 
 
 
 CASES = ("fs_impactor", "suspension", "periodic", "units", "lb_closed", "moving", "startup", "two_impactors")
+# rheology cases: name -> (geometry case, config values). BINGHAM reads no initVisc (LB.cpp:211-218)
+# and keeps the default 1/6; the Smagorinsky term is added to the model's viscosity (LB.cpp:242-245)
+RHEOLOGY_CASES = {
+    "bingham": ("suspension", {"rheologyModel": "BINGHAM", "plasticVisc": 0.08, "yieldStress": 2.0e-5,
+                               "turbulenceSolve": 1, "turbConst": 0.05}),
+    "frictional": ("periodic", {"rheologyModel": "FRICTIONAL", "frictionCoefFluid": 0.02, "initVisc": 0.1}),
+    "voellmy": ("fs_impactor", {"rheologyModel": "VOELLMY", "frictionCoefFluid": 0.2, "voellmyConstant": 0.5,
+                                "initVisc": 0.1}),
+    "smagorinsky": ("lb_closed", {"turbulenceSolve": 1, "turbConst": 2.0}),
+}
 # cases that take the DEM link-cell branch of the neighbour list (DEM.cpp:556-631)
 CELL_CASES = ("cells", "thin_z2", "thin_z1", "thin_z0")
 

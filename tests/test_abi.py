@@ -71,6 +71,34 @@ def test_getpot_parser_and_defaults():
     assert "c" not in v
 
 
+def test_rheology_keys(oracle_lib):
+    """LB.cpp:197-245: the model name picks the keys read; BINGHAM reads no initVisc and keeps the
+    default (1.0 - 0.5) / 3.0; an unknown name keeps NEWTONIAN; an unknown model code is rejected."""
+    base = lbdem.config_values("initVisc = 0.3\nplasticVisc = 0.2\nyieldStress = 1e-3\nfrictionCoefFluid = 0.4\n"
+                               "voellmyConstant = 2.0\nturbulenceSolve = 1\nturbConst = 0.05\n")
+    for name, model in (("BINGHAM", lbdem.RHEO_BINGHAM), ("FRICTIONAL", lbdem.RHEO_FRICTIONAL),
+                        ("VOELLMY", lbdem.RHEO_VOELLMY), ("NEWTONIAN", lbdem.RHEO_NEWTONIAN),
+                        ("none", lbdem.RHEO_NEWTONIAN)):
+        p = lbdem.params_from_values(dict(base, rheologyModel=name))
+        assert p.rheologyModel == model and p.turbulenceSolve == 1 and p.turbConst == 0.05
+        assert p.initVisc == ((1.0 - 0.5) / 3.0 if model == lbdem.RHEO_BINGHAM else 0.3)
+        assert (p.plasticVisc, p.yieldStress) == ((0.2, 1e-3) if model == lbdem.RHEO_BINGHAM else (0.0, 0.0))
+        assert p.frictionCoefFluid == (0.4 if model in (lbdem.RHEO_FRICTIONAL, lbdem.RHEO_VOELLMY) else 0.0)
+        assert p.voellmyConstant == (2.0 if model == lbdem.RHEO_VOELLMY else 0.0)
+    p = lbdem.Params()
+    p.unitLength = p.unitTime = p.unitDensity = 1.0
+    p.lbSize[:] = [8.0, 8.0, 8.0]
+    p.initVisc = 0.1
+    p.boundary[:] = [7] * 6
+    p.lbSolve = 1
+    p.density = p.objectDensity = 1.0
+    p.rheologyModel = 4
+    ctx = ctypes.c_void_p()
+    assert oracle_lib.lib.lbdem_create(ctypes.byref(p), ctypes.byref(ctx)) == 1
+    assert b"rheologyModel" in oracle_lib.lib.lbdem_last_error(ctx)
+    oracle_lib.lib.lbdem_destroy(ctx)
+
+
 def test_particle_and_object_readers(tmp_path):
     lbdem.write_particles(str(tmp_path / "p.dat"), [1.0, 1.2], np.array([[1.0, 2.0, 3.0], [4.0, 5.0, 6.0]]))
     ps = lbdem.read_particles(str(tmp_path / "p.dat"), scale=2.0)

@@ -88,3 +88,24 @@ def test_cell_fixtures_cross_the_seams():
         rows = [g["nlist"][a, :g["ncount"][a]] for a in range(len(g["ncount"]))]
         assert any(len(r) != len(set(r.tolist())) for r in rows) == dup, name
     assert np.all(np.load(os.path.join(GOLD, "thin_z0_s20.npz"))["ncount"] == 0)
+
+
+def test_rheology_fixtures_cover_the_viscosity_range(oracle_lib, case_root):
+    """The rheology fixtures exercise the apparent viscosity on both sides of its clamp: after 40
+    steps the node viscosities vary with the shear rate, and the BINGHAM, FRICTIONAL and VOELLMY
+    cases hold nodes both below and at maxVisc = (1.8 - 0.5) / 3 (lattice.h:27); the Smagorinsky
+    case stays above its Newtonian viscosity 0.1 by the turbulent term."""
+    top = (1.8 - 0.5) / 3.0
+    for name in cases.RHEOLOGY_CASES:
+        sim = cases.open_sim(oracle_lib, cases.make_case(case_root, name))
+        sim.cycle(39)
+        sim.store_velocity()
+        sim.cycle(1)
+        _, visc = sim.velocity()
+        v = visc[visc > 0]
+        assert v.size > 100 and v.max() - v.min() > 1e-4, name
+        if name == "smagorinsky":
+            assert v.min() >= 0.1 and v.max() < top, name
+        else:
+            assert (v >= top).any() and (v < 0.99 * top).sum() > 10, name
+        sim.close()
