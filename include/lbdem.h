@@ -75,6 +75,9 @@ typedef struct lbdem_params {
     double lubNormal, lubTangential, lubRotation, lubCutoff, kinVisc;
     double electroForce, debyeLength;
     int objWaitTime;
+    /* body-spring impactor (DEM.cpp:63-73; bodyObjectInteraction DEM.cpp:798-869) */
+    double springDamp, epsilon, k13;
+    int nonLinType;              /* 0 spring, 1 + dashpot, 2 + cubic, 3 + repulsive LJ, 4 + WCA */
     /* implementation options (no reference counterpart) */
     int precision;               /* LBDEM_PREC_* (the CPU oracle always uses FP64) */
     int device;                  /* HIP device ordinal */
@@ -98,6 +101,12 @@ enum {
     LBDEM_O_FTOTAL = 27, LBDEM_O_MTOTAL = 30, LBDEM_O_FGRAV = 33, LBDEM_O_R = 36, LBDEM_O_M = 37,
     LBDEM_O_REC = 38
 };
+/* Per-body record layout used by lbdem_get_bodies (doubles; body, elmt.h, DEM.cpp:168-196, 255-266). */
+enum {
+    LBDEM_B_X0 = 0, LBDEM_B_X1 = 3, LBDEM_B_FSPRING = 6, LBDEM_B_FTOTAL = 9, LBDEM_B_FGRAV = 12,
+    LBDEM_B_L = 15, LBDEM_B_L0 = 16, LBDEM_B_K = 17, LBDEM_B_R = 18, LBDEM_B_M = 19,
+    LBDEM_B_REC = 20
+};
 /* Per-wall record layout used by lbdem_get_walls (doubles). */
 enum {
     LBDEM_W_FHYDRO = 0, LBDEM_W_FCONTACT = 3, LBDEM_W_FLUB = 6, LBDEM_W_FTOTAL = 9,
@@ -118,6 +127,7 @@ typedef struct lbdem_info {
     double lastExtraMass;        /* streaming extraMass of the last LB step (LB.cpp:1533) */
     double lastSurplus;          /* massSurplus of the last free-surface step (LB.cpp:1644) */
     int precision;
+    int nBodies;
 } lbdem_info;
 
 typedef struct lbdem_ctx lbdem_ctx;
@@ -130,6 +140,12 @@ const char* lbdem_last_error(const lbdem_ctx* ctx);
 int lbdem_set_particles(lbdem_ctx* ctx, int n, const int* size, const double* r, const double* x0);
 /* objects.dat rows (DEM.cpp:138-165) after scaling: r, x0 (3n), x1 (3n). */
 int lbdem_set_objects(lbdem_ctx* ctx, int n, const double* r, const double* x0, const double* x1);
+/* body.dat rows (DEM.cpp:168-196) after scaling: r, x0*scale (3n), x1 (3n), spring stiffness k,
+ * rest length L0; at most LBDEM_MAX_BODIES. Each body pulls every impactor along y (DEM.cpp:798-869)
+ * from the impactor wait on, and moves under its gravity and spring forces (DEM.cpp:2728-2740). */
+#define LBDEM_MAX_BODIES 8
+int lbdem_set_bodies(lbdem_ctx* ctx, int n, const double* r, const double* x0, const double* x1, const double* k,
+                     const double* L0);
 
 /* Fluid restart from a fluidRecycle file (LB::latticeBolzmannInit with lbRestart, LB.cpp:352-367;
  * restartInterface LB.cpp:994-1009; restartVariables LB.cpp:1098-1138), between lbdem_set_objects
@@ -180,6 +196,7 @@ int lbdem_get_fields_noalloc(lbdem_ctx* ctx, double* f, double* mass, double* n)
 int lbdem_get_particles(lbdem_ctx* ctx, double* rec /* nParticles*LBDEM_P_REC */);
 int lbdem_get_objects(lbdem_ctx* ctx, double* rec /* nObjects*LBDEM_O_REC */);
 int lbdem_get_walls(lbdem_ctx* ctx, double* rec /* nWalls*LBDEM_W_REC */);
+int lbdem_get_bodies(lbdem_ctx* ctx, double* rec /* nBodies*LBDEM_B_REC */);
 /* Verlet list of every particle: count[n], list[n*cap + k] (DEM.cpp:473-652). The MI355X library
  * lists partners in ascending order, the oracle in the reference's traversal order; a pair that
  * the reference lists more than once (fewer than 3 link cells along z) appears as often. */

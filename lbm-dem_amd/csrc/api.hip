@@ -65,6 +65,7 @@ struct lbdem_ctx {
     // inputs
     std::vector<int> psize;
     std::vector<double> pr, px0, orad, ox0, ox1;
+    std::vector<double> brad, bx0, bx1, bk, bL0;  // bodies (lbdem_set_bodies)
     // device
     hipStream_t stream = nullptr;
     // lbdem_cycle defers a step's free-surface pass to the start of the next step and runs it on
@@ -256,6 +257,22 @@ extern "C" int lbdem_set_objects(lbdem_ctx* c, int n, const double* r, const dou
     c->orad.assign(r, r + n);
     c->ox0.assign(x0, x0 + 3 * size_t(n));
     c->ox1.assign(x1, x1 + 3 * size_t(n));
+    return 0;
+}
+
+extern "C" int lbdem_set_bodies(lbdem_ctx* c, int n, const double* r, const double* x0, const double* x1,
+                                const double* k, const double* L0) {
+    if (!c || n < 0 || (n > 0 && (!r || !x0 || !x1 || !k || !L0))) return LBDEM_E_ARG;
+    if (n > LBDEM_MAX_BODIES) return c->fail(LBDEM_E_ARG, "too many bodies");
+    if (c->initialised) return c->fail(LBDEM_E_STATE, "bodies must be set before lbdem_init");
+    // nonLinType outside 0-4 leaves the reference's spring force undefined (DEM.cpp:801-860)
+    if (n > 0 && (c->prm.nonLinType < 0 || c->prm.nonLinType > 4))
+        return c->fail(LBDEM_E_ARG, "nonLinType must be 0-4 with bodies");
+    c->brad.assign(r, r + n);
+    c->bx0.assign(x0, x0 + 3 * size_t(n));
+    c->bx1.assign(x1, x1 + 3 * size_t(n));
+    c->bk.assign(k, k + n);
+    c->bL0.assign(L0, L0 + n);
     return 0;
 }
 
@@ -488,6 +505,7 @@ static bool plan_buffers(const Plan& p, Box& b, DemState& d, DevScalars*& sc, do
     for (double** v : opvecs) A(*v, no1);
     double** wvecs[] = {&d.wFHydro, &d.wFContact, &d.wFLub, &d.wFTotal};
     for (double** v : wvecs) A(*v, 3 * size_t(MAX_WALLS));
+    A(d.body, size_t(LBDEM_MAX_BODIES) * LBDEM_B_REC);
     A(d.plb, np1);
     A(d.olb, no1);
     A(tmp, TMP_PLANNED / sizeof(double));
@@ -609,6 +627,21 @@ extern "C" int lbdem_init(lbdem_ctx* c) {
         oim[o] = 1.0 / om[o];
         oiI[o] = 1.0 / oI[o];
     }
+    // bodies (DEM.cpp:257-266): mass, gravity, L = L0; the records as lbdem_get_bodies returns them
+    const int nbody = int(c->brad.size());
+    std::vector<double> brec(size_t(std::max(nbody, 1)) * LBDEM_B_REC, 0.0);
+    for (int b = 0; b < nbody; ++b) {
+        double* R = &brec[size_t(b) * LBDEM_B_REC];
+        const double r = c->brad[b];
+        const double m = c->prm.bodyDensity * (4.0 / 3.0) * M_PI * (r * r * r);
+        for (int a = 0; a < 3; ++a) {
+            R[LBDEM_B_X0 + a] = c->bx0[3 * b + a];
+            R[LBDEM_B_X1 + a] = c->bx1[3 * b + a];
+            R[LBDEM_B_FGRAV + a] = demF[a] * m;
+        }
+        R[LBDEM_B_L] = c->bL0[b]; R[LBDEM_B_L0] = c->bL0[b]; R[LBDEM_B_K] = c->bk[b];
+        R[LBDEM_B_R] = r; R[LBDEM_B_M] = m;
+    }
     std::vector<std::array<double, 6>> walls;
     const double dirs[6][3] = {{1, 0, 0}, {-1, 0, 0}, {0, 1, 0}, {0, -1, 0}, {0, 0, 1}, {0, 0, -1}};
     for (int i = 0; i < 6; ++i)
@@ -623,6 +656,9 @@ extern "C" int lbdem_init(lbdem_ctx* c) {
     P.kinVisc = c->prm.kinVisc;
     P.lubNormal = c->prm.lubNormal; P.lubTangential = c->prm.lubTangential;
     P.lubRotation = c->prm.lubRotation; P.lubCutoff = c->prm.lubCutoff;
+    P.nBody = nbody;
+    P.nonLinType = c->prm.nonLinType;
+    P.springDamp = c->prm.springDamp; P.epsilon = c->prm.epsilon; P.k13 = c->prm.k13;
     P.electroForce = c->prm.electroForce; P.debyeLength = c->prm.debyeLength;
     P.rmax = rmax;
     P.objWaitTime = c->prm.objWaitTime;
@@ -730,6 +766,7 @@ extern "C" int lbdem_init(lbdem_ctx* c) {
         HIPCHK(hipMemcpyAsync(d.oiI, oiI.data(), no * 8, hipMemcpyHostToDevice, st));
         HIPCHK(hipMemcpyAsync(d.oFGrav, ofg.data(), 3 * no * 8, hipMemcpyHostToDevice, st));
     }
+    HIPCHK(hipMemcpyAsync(d.body, brec.data(), brec.size() * 8, hipMemcpyHostToDevice, st));
     HIPCHK(hipMemcpyAsync(c->sc, &hs, sizeof(hs), hipMemcpyHostToDevice, st));
 
     // constants of the lattice kernels
@@ -1158,6 +1195,7 @@ extern "C" int lbdem_get_info(lbdem_ctx* c, lbdem_info* in) {
     for (int a = 0; a < 3; ++a) in->size[a] = c->lbSize[a];
     in->totNodes = c->box.g.N;
     in->nParticles = c->dp.nPart; in->nObjects = c->dp.nObj; in->nWalls = c->dp.nWall;
+    in->nBodies = c->dp.nBody;
     int* cnt = (int*)c->tmp(64);
     if (!cnt) return c->fail(LBDEM_E_NOMEM, "device allocation failed");
     HIPCHK(hipMemsetAsync(cnt, 0, 5 * sizeof(int), c->stream));
@@ -1365,6 +1403,15 @@ extern "C" int lbdem_get_objects(lbdem_ctx* c, double* rec) {
         HIPCHK(hipStreamSynchronize(c->stream));
         for (int i = 0; i < no; ++i) rec[size_t(i) * LBDEM_O_REC + o1[k]] = buf[i];
     }
+    return 0;
+}
+
+extern "C" int lbdem_get_bodies(lbdem_ctx* c, double* rec) {
+    NEED_INIT;
+    const int nb = c->dp.nBody;
+    if (nb == 0) return 0;
+    if (fetch(c, rec, c->ds.body, nb * LBDEM_B_REC)) return LBDEM_E_DEVICE;
+    HIPCHK(hipStreamSynchronize(c->stream));
     return 0;
 }
 

@@ -737,12 +737,62 @@ __device__ inline void dem_objects(DemParams P, DemState d, const double* part, 
     if (t != 0) return;
     objSum = sObj;
     wallSum = sWall;
+    // body-spring forces (DEM.cpp:798-869), on the positions before this step's integration
+    double spring[MAX_OBJECTS];  // y component of each object's FSpring
+    for (int o = 0; o < MAX_OBJECTS; ++o) spring[o] = 0.0;
+    const bool waited = demTimeStep >= P.objWaitTime;
+    for (int b = 0; b < P.nBody; ++b) {
+        double* B = d.body + b * LBDEM_B_REC;
+        double fsB = 0.0;
+        if (waited) {
+            for (int o = 0; o < P.nObj; ++o) {
+                const double L = B[LBDEM_B_X0 + 1] - d.ox0[3 * o + 1];
+                B[LBDEM_B_L] = L;
+                const double rx = B[LBDEM_B_X0] - d.ox0[3 * o], ry = L, rz = B[LBDEM_B_X0 + 2] - d.ox0[3 * o + 2];
+                const double rn = sqrt(rx * rx + ry * ry + rz * rz);
+                const double ny = ry / rn;
+                const double k = B[LBDEM_B_K], del = B[LBDEM_B_L0] - L, sigma = B[LBDEM_B_R];
+                const double bodyDamp = 2.0 * sqrt(B[LBDEM_B_M] * k) * P.springDamp;
+                const double objDamp = 2.0 * sqrt(d.om[o] * k) * P.springDamp;
+                double fs, fdo = -d.ox1[3 * o + 1] * objDamp, fdb = -B[LBDEM_B_X1 + 1] * bodyDamp;
+                if (P.nonLinType == 0) {
+                    fs = k * del * ny;
+                    fdo = 0.0;
+                    fdb = 0.0;
+                } else if (P.nonLinType == 1) {
+                    fs = (k * del) * ny;
+                } else if (P.nonLinType == 2) {
+                    fs = (k * del + P.k13 * k * (del * del * del)) * ny;
+                } else if (P.nonLinType == 3) {
+                    fs = (k * del + (24.0 * P.epsilon / rn) * 2.0 * (pow((sigma / rn), 12.0))) * ny;
+                } else {  // 4: WCA repulsion inside 2^(1/6) sigma
+                    if (rn <= pow(2.0, 1.0 / 6.0) * sigma)
+                        fs = (k * del + (24.0 * P.epsilon / rn) * ((2.0 * pow(sigma / rn, 12.0)) - (pow(sigma / rn, 6.0)))) * ny;
+                    else
+                        fs = (k * del) * ny;
+                }
+                spring[o] -= fs;
+                fsB += fs;
+                spring[o] += fdo;
+                fsB += fdb;
+            }
+        }
+        // totals (DEM.cpp:792-794): FTotal = FGrav + FSpring; integration (DEM.cpp:2730-2740)
+        const double FT[3] = {B[LBDEM_B_FGRAV] + 0.0, B[LBDEM_B_FGRAV + 1] + fsB, B[LBDEM_B_FGRAV + 2] + 0.0};
+        B[LBDEM_B_FSPRING] = 0.0; B[LBDEM_B_FSPRING + 1] = fsB; B[LBDEM_B_FSPRING + 2] = 0.0;
+        for (int c = 0; c < 3; ++c) B[LBDEM_B_FTOTAL + c] = FT[c];
+        if (waited) {
+            const double dt = P.deltat, im = 1.0 / B[LBDEM_B_M];
+            for (int c = 0; c < 3; ++c) B[LBDEM_B_X0 + c] += dt * B[LBDEM_B_X1 + c] + dt * dt * FT[c] * im;
+            for (int c = 0; c < 3; ++c) B[LBDEM_B_X1 + c] += dt * FT[c] * im;
+        }
+    }
     for (int o = 0; o < P.nObj; ++o) {
         const double* s = objSum + 12 * o;
         const V Fc{s[0], s[1], s[2]}, Mc{s[3], s[4], s[5]}, Fl{s[6], s[7], s[8]}, Ml{s[9], s[10], s[11]};
         st3(d.oFContact, o, Fc); st3(d.oMContact, o, Mc); st3(d.oFLub, o, Fl); st3(d.oMLub, o, Ml);
         const V FH = ld3(d.oFHydro, o), MH = ld3(d.oMHydro, o), FG = ld3(d.oFGrav, o);
-        const V FT = add(add(add(add(FH, Fc), Fl), FG), V{0, 0, 0});
+        const V FT = add(add(add(add(FH, Fc), Fl), FG), V{0, spring[o], 0});
         const V MT = add(add(MH, Mc), Ml);
         st3(d.oFTotal, o, FT); st3(d.oMTotal, o, MT);
         if (demTimeStep >= P.objWaitTime) {

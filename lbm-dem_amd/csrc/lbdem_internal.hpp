@@ -246,6 +246,8 @@ struct DemParams {
     int periodic[3];           // both faces periodic along axis
     int nlx, nly, nlz;         // cell grid: all 0 for the direct sum; else nlx, nly >= 3, nlz >= 0
     int nPart, nObj, nWall;
+    int nBody, nonLinType;     // body-spring impactor (DEM.cpp:63-73, 798-869)
+    double springDamp, epsilon, k13;
     double wallP[MAX_WALLS][3], wallN[MAX_WALLS][3];
     double unitLength, unitSpeed, unitAngVel;
 };
@@ -276,6 +278,8 @@ struct DemState {
     // objects
     double *ox0, *ox1, *ow1, *or_, *om, *oI, *oim, *oiI;
     double *oFHydro, *oMHydro, *oFContact, *oMContact, *oFLub, *oMLub, *oFTotal, *oMTotal, *oFGrav;
+    // bodies: LBDEM_B_REC doubles each (include/lbdem.h layout)
+    double* body;
     // walls
     double *wFHydro, *wFContact, *wFLub, *wFTotal;
     PartLB* plb;

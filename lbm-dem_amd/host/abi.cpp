@@ -23,6 +23,7 @@ Api Api::load(const std::string& path) {
     bind(a.handle, path, "lbdem_last_error", a.last_error);
     bind(a.handle, path, "lbdem_set_particles", a.set_particles);
     bind(a.handle, path, "lbdem_set_objects", a.set_objects);
+    bind(a.handle, path, "lbdem_set_bodies", a.set_bodies);
     bind(a.handle, path, "lbdem_init", a.init);
     bind(a.handle, path, "lbdem_dem_step", a.dem_step);
     bind(a.handle, path, "lbdem_coupling_step", a.coupling_step);
@@ -36,6 +37,7 @@ Api Api::load(const std::string& path) {
     bind(a.handle, path, "lbdem_get_particles", a.get_particles);
     bind(a.handle, path, "lbdem_get_objects", a.get_objects);
     bind(a.handle, path, "lbdem_get_walls", a.get_walls);
+    bind(a.handle, path, "lbdem_get_bodies", a.get_bodies);
     bind(a.handle, path, "lbdem_get_neighbors", a.get_neighbors);
     bind(a.handle, path, "lbdem_total_mass", a.total_mass);
     bind(a.handle, path, "lbdem_set_restart_fluid", a.set_restart_fluid);

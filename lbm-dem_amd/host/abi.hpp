@@ -15,6 +15,8 @@ struct Api {
     const char* (*last_error)(const lbdem_ctx*) = nullptr;
     int (*set_particles)(lbdem_ctx*, int, const int*, const double*, const double*) = nullptr;
     int (*set_objects)(lbdem_ctx*, int, const double*, const double*, const double*) = nullptr;
+    int (*set_bodies)(lbdem_ctx*, int, const double*, const double*, const double*, const double*,
+                      const double*) = nullptr;
     int (*init)(lbdem_ctx*) = nullptr;
     int (*dem_step)(lbdem_ctx*) = nullptr;
     int (*coupling_step)(lbdem_ctx*) = nullptr;
@@ -28,6 +30,7 @@ struct Api {
     int (*get_particles)(lbdem_ctx*, double*) = nullptr;
     int (*get_objects)(lbdem_ctx*, double*) = nullptr;
     int (*get_walls)(lbdem_ctx*, double*) = nullptr;
+    int (*get_bodies)(lbdem_ctx*, double*) = nullptr;
     int (*get_neighbors)(lbdem_ctx*, int, int*, int*) = nullptr;
     int (*total_mass)(lbdem_ctx*, double*) = nullptr;
     int (*set_restart_fluid)(lbdem_ctx*, const int*, const uint8_t*, int, const double*) = nullptr;

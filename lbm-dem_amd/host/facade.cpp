@@ -38,6 +38,9 @@ void Context::create() {
     const int np = int(pr.size()), no = int(orad.size());
     check(api_.set_particles(ctx_, np, psize.data(), pr.data(), px0.data()), "lbdem_set_particles");
     check(api_.set_objects(ctx_, no, orad.data(), ox0.data(), ox1.data()), "lbdem_set_objects");
+    const int nb = int(brad.size());
+    if (nb > 0)
+        check(api_.set_bodies(ctx_, nb, brad.data(), bx0.data(), bx1.data(), bk.data(), bL0.data()), "lbdem_set_bodies");
 }
 
 void Context::initialise() {
@@ -214,6 +217,10 @@ void DEM::discreteElementGet(ConfigFile& cfg, CommandLine& cl) {
     p.electroForce = parse_value(cfg, cl, "electroForce", 0.0);
     p.debyeLength = parse_value(cfg, cl, "debyeLength", 0.0);
     p.objWaitTime = parse_value(cfg, cl, "objWaitTime", 0);
+    p.springDamp = parse_value(cfg, cl, "springDamp", 0.0);
+    p.epsilon = parse_value(cfg, cl, "epsilon", 0.0);
+    p.nonLinType = parse_value(cfg, cl, "nonLinType", 0);
+    p.k13 = parse_value(cfg, cl, "k13", 0.0);
     const double scale = parse_value(cfg, cl, "scale", 1.0);
 
     // input/particles.dat: count, then index size r x y z vx vy vz wx wy wz q0(4) q1(4) (DEM.cpp:81-134);
@@ -252,6 +259,26 @@ void DEM::discreteElementGet(ConfigFile& cfg, CommandLine& cl) {
             for (int a = 0; a < 3; ++a) {
                 c_.ox0[3 * i + a] = x[a] * scale;
                 c_.ox1[3 * i + a] = v[a];
+            }
+        }
+    }
+    // input/body.dat: count, then index r x y z vx vy vz k L0 (DEM.cpp:168-196)
+    {
+        std::ifstream in = open_input("./input/body.dat");
+        unsigned n = 0;
+        in >> n;
+        c_.brad.assign(n, 0.0); c_.bk.assign(n, 0.0); c_.bL0.assign(n, 0.0);
+        c_.bx0.assign(3 * size_t(n), 0.0);
+        c_.bx1.assign(3 * size_t(n), 0.0);
+        for (unsigned i = 0; i < n; ++i) {
+            double index, r, x[3], v[3], k, L0;
+            in >> index >> r >> x[0] >> x[1] >> x[2] >> v[0] >> v[1] >> v[2] >> k >> L0;
+            c_.brad[i] = r;
+            c_.bk[i] = k;
+            c_.bL0[i] = L0;
+            for (int a = 0; a < 3; ++a) {
+                c_.bx0[3 * i + a] = x[a] * scale;
+                c_.bx1[3 * i + a] = v[a];
             }
         }
     }
@@ -308,6 +335,17 @@ void DEM::refresh() {
         O.FRep = tVect();  // no electrostatic force acts on objects (DEM.cpp:786-790)
         O.r = r[LBDEM_O_R]; O.m = r[LBDEM_O_M];
     }
+    rec.assign(size_t(std::max(in.nBodies, 1)) * LBDEM_B_REC, 0.0);
+    c_.check(c_.api().get_bodies(c_.ctx(), rec.data()), "lbdem_get_bodies");
+    bodies.resize(in.nBodies);
+    for (int i = 0; i < in.nBodies; ++i) {
+        const double* r = &rec[size_t(i) * LBDEM_B_REC];
+        body& B = bodies[i];
+        B.index = unsigned(i);
+        B.x0 = rec3(r, LBDEM_B_X0); B.x1 = rec3(r, LBDEM_B_X1); B.FSpring = rec3(r, LBDEM_B_FSPRING);
+        B.FTotal = rec3(r, LBDEM_B_FTOTAL); B.FGrav = rec3(r, LBDEM_B_FGRAV);
+        B.L = r[LBDEM_B_L]; B.L0 = r[LBDEM_B_L0]; B.k = r[LBDEM_B_K]; B.r = r[LBDEM_B_R]; B.m = r[LBDEM_B_M];
+    }
     rec.assign(size_t(std::max(in.nWalls, 1)) * LBDEM_W_REC, 0.0);
     c_.check(c_.api().get_walls(c_.ctx(), rec.data()), "lbdem_get_walls");
     walls.resize(in.nWalls);
@@ -347,6 +385,9 @@ void IO::initialize() {
         objectPositionFileName = workDirectory + "/objPosition.dat";
         append_line(objectPositionFileName,
                     "position(x) position(y) position(z) velocity(x) velocity(y) velocity(z) depth(y)  \n");
+        bodyPositionFileName = workDirectory + "/bodyPosition.dat";
+        append_line(bodyPositionFileName,
+                    "position(x) position(y) position(z) velocity(x) velocity(y) velocity(z) Length Delta  \n");
         objectForceFileName = workDirectory + "/objForce.dat";
         append_line(objectForceFileName,
                     "FExerted(x) FExerted(y) FExerted(z) SExerted(x) SExerted(y) SExerted(z) FTotal(x) FTotal(y) "
@@ -379,6 +420,7 @@ void IO::outputStep(LB& lb, DEM& dem) {
                 exportObjectPosition(dem.objects, lb);
                 exportObjectForce(dem.objects, lb);
             }
+            if (!dem.bodies.empty()) exportBodyPosition(dem.bodies);
             if (!dem.walls.empty()) exportWallForce(dem.walls);
         }
         if (lbmSolve && c_.initialised()) exportMass(lb);
@@ -568,6 +610,17 @@ void IO::exportObjectPosition(const objectList& objects, const LB& lb) {
         f << "\t";
         o.x1.print(f);
         f << "\t" << depth;
+        f << std::endl;
+    }
+}
+
+void IO::exportBodyPosition(const bodyList& bodies) {
+    std::ofstream f(bodyPositionFileName, std::ios::app);
+    for (const body& b : bodies) {
+        b.x0.print(f);
+        f << "\t";
+        b.x1.print(f);
+        f << "\t" << b.L;
         f << std::endl;
     }
 }

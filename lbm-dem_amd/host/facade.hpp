@@ -45,8 +45,14 @@ struct wall {
     unsigned index = 0;
     tVect FHydro, FContact, FLub, FTotal;
 };
+struct body {
+    unsigned index = 0;
+    double r = 0.0, m = 0.0, k = 0.0, L0 = 0.0, L = 0.0;
+    tVect x0, x1, FSpring, FTotal, FGrav;
+};
 using particleList = std::vector<particle>;
 using objectList = std::vector<object>;
+using bodyList = std::vector<body>;
 using wallList = std::vector<wall>;
 using unsIntList = std::vector<unsigned>;
 using doubleList = std::vector<double>;
@@ -65,6 +71,7 @@ public:
     lbdem_params params{};
     std::vector<int> psize;
     std::vector<double> pr, px0, orad, ox0, ox1;
+    std::vector<double> brad, bx0, bx1, bk, bL0;  // input/body.dat (DEM.cpp:168-196)
 
     lbdem_ctx* ctx() const { return ctx_; }
     void create();                    // lbdem_create + inputs (idempotent)
@@ -131,6 +138,7 @@ public:
     bool newNeighborList = false;
     particleList particles;
     objectList objects;
+    bodyList bodies;
     wallList walls;
     Context& context() const { return c_; }
 
@@ -157,6 +165,7 @@ private:
     void exportObjectPosition(const objectList& objects, const LB& lb);   // IO.cpp:1316-1338
     void exportObjectForce(const objectList& objects, const LB& lb);      // IO.cpp:1239-1293
     void exportWallForce(const wallList& walls);                          // IO.cpp:1295-1314
+    void exportBodyPosition(const bodyList& bodies);                      // IO.cpp:1340-1353
     void exportMass(const LB& lb);                                        // step, time, total fluid mass
     void createFiles(LB& lb, DEM& dem);                                   // IO.cpp:181-289 (the fluid files)
     void exportRecycleFluid(const LB& lb, const std::string& file);       // IO.cpp:973-997
@@ -172,6 +181,7 @@ private:
     Context& c_;
     unsigned lastScreenExp = 0, lastFluidRecycleExp = 0, lastFluidExp = 0, lastFluidLagrangianExp = 0;
     std::string exportFileName, objectPositionFileName, objectForceFileName, wallForceFileName, massFileName;
+    std::string bodyPositionFileName;
     std::string partDirectory, fluidDirectory, fluidRecycleFileFormat, fluidFileFormat, fluidLagrangianFileFormat;
     FluidState fs_;
     unsigned fsStep_ = ~0u;

@@ -36,6 +36,8 @@ P = dict(X0=0, X1=3, W1=6, FHYDRO=9, MHYDRO=12, FCONTACT=15, MCONTACT=18, FLUB=2
 O = dict(X0=0, X1=3, W1=6, FHYDRO=9, MHYDRO=12, FCONTACT=15, MCONTACT=18, FLUB=21, MLUB=24, FTOTAL=27,
          MTOTAL=30, FGRAV=33, R=36, M=37, REC=38)
 W = dict(FHYDRO=0, FCONTACT=3, FLUB=6, FTOTAL=9, REC=12)
+B = dict(X0=0, X1=3, FSPRING=6, FTOTAL=9, FGRAV=12, L=15, L0=16, K=17, R=18, M=19, REC=20)  # LBDEM_B_*
+MAX_BODIES = 8
 
 
 class Params(C.Structure):
@@ -54,6 +56,7 @@ class Params(C.Structure):
         ("lubCutoff", C.c_double), ("kinVisc", C.c_double),
         ("electroForce", C.c_double), ("debyeLength", C.c_double),
         ("objWaitTime", C.c_int),
+        ("springDamp", C.c_double), ("epsilon", C.c_double), ("k13", C.c_double), ("nonLinType", C.c_int),
         ("precision", C.c_int), ("device", C.c_int),
     ]
 
@@ -67,7 +70,7 @@ class Info(C.Structure):
         ("lbTime", C.c_uint), ("demTimeStep", C.c_uint), ("demTime", C.c_double), ("realTime", C.c_double),
         ("neighborListBuilds", C.c_int), ("newNeighborList", C.c_int),
         ("totalMassInit", C.c_double), ("lastExtraMass", C.c_double), ("lastSurplus", C.c_double),
-        ("precision", C.c_int),
+        ("precision", C.c_int), ("nBodies", C.c_int),
     ]
 
     def as_dict(self) -> Dict:
@@ -168,6 +171,8 @@ class Library:
             "lbdem_get_fields_noalloc": ([vp, C.c_void_p, C.c_void_p, C.c_void_p], C.c_int),
             "lbdem_get_particles": ([vp, _dp], C.c_int),
             "lbdem_get_objects": ([vp, _dp], C.c_int),
+            "lbdem_set_bodies": ([vp, C.c_int, _dp, _dp, _dp, _dp, _dp], C.c_int),
+            "lbdem_get_bodies": ([vp, _dp], C.c_int),
             "lbdem_get_walls": ([vp, _dp], C.c_int),
             "lbdem_get_neighbors": ([vp, C.c_int, _ip, _ip], C.c_int),
             "lbdem_total_mass": ([vp, _dp], C.c_int),
@@ -240,6 +245,11 @@ class Simulation:
         ox1 = np.ascontiguousarray(objects.x1, dtype=np.float64).reshape(-1)
         self._check(lib.lib.lbdem_set_objects(self.ctx, m, _ptr(orr, C.c_double), _ptr(ox0, C.c_double),
                                               _ptr(ox1, C.c_double)), "lbdem_set_objects")
+        bs = objects.bodies
+        if bs is not None and len(bs.r):
+            arrs = [np.ascontiguousarray(a, dtype=np.float64).reshape(-1) for a in (bs.r, bs.x0, bs.x1, bs.k, bs.L0)]
+            self._check(lib.lib.lbdem_set_bodies(self.ctx, len(bs.r), *[_ptr(a, C.c_double) for a in arrs]),
+                        "lbdem_set_bodies")
 
     def _check(self, rc: int, what: str):
         if rc != 0:
@@ -334,6 +344,12 @@ class Simulation:
         n = self.info()["nObjects"]
         rec = np.zeros((max(n, 1), O["REC"]), np.float64)
         self._check(self.lib.lib.lbdem_get_objects(self.ctx, _ptr(rec, C.c_double)), "lbdem_get_objects")
+        return rec[:n]
+
+    def bodies(self) -> np.ndarray:
+        n = self.info()["nBodies"]
+        rec = np.zeros((max(n, 1), B["REC"]), np.float64)
+        self._check(self.lib.lib.lbdem_get_bodies(self.ctx, _ptr(rec, C.c_double)), "lbdem_get_bodies")
         return rec[:n]
 
     def walls(self) -> np.ndarray:
@@ -633,6 +649,8 @@ def params_from_values(v: Dict[str, object], precision: int = PREC_FP64, device:
               "lubTangential", "lubRotation", "lubCutoff", "kinVisc", "electroForce", "debyeLength"):
         setattr(p, k, float(v[k]))
     p.objWaitTime = int(v["objWaitTime"])
+    p.springDamp, p.epsilon, p.k13 = float(v["springDamp"]), float(v["epsilon"]), float(v["k13"])
+    p.nonLinType = int(v["nonLinType"])
     p.precision = int(precision)
     p.device = int(device)
     return p
@@ -662,10 +680,22 @@ class ParticleSet:
 
 
 @dataclass
+class BodySet:
+    """input/body.dat rows (DEM.cpp:168-196): radius, centre (scaled), velocity, spring stiffness k
+    and rest length L0."""
+    r: np.ndarray = field(default_factory=lambda: np.zeros(0))
+    x0: np.ndarray = field(default_factory=lambda: np.zeros((0, 3)))
+    x1: np.ndarray = field(default_factory=lambda: np.zeros((0, 3)))
+    k: np.ndarray = field(default_factory=lambda: np.zeros(0))
+    L0: np.ndarray = field(default_factory=lambda: np.zeros(0))
+
+
+@dataclass
 class ObjectSet:
     r: np.ndarray = field(default_factory=lambda: np.zeros(0))
     x0: np.ndarray = field(default_factory=lambda: np.zeros((0, 3)))
     x1: np.ndarray = field(default_factory=lambda: np.zeros((0, 3)))
+    bodies: Optional[BodySet] = None  # the spring-connected bodies of input/body.dat, if any
 
 
 def _tokens(path: str) -> List[str]:
@@ -743,11 +773,34 @@ def read_recycle(path: str):
     return size, types, vals[:m * RESTART_REC].reshape(m, RESTART_REC)
 
 
+def read_bodies(path: str, scale: float = 1.0) -> BodySet:
+    """input/body.dat as DEM::discreteElementGet reads it (DEM.cpp:168-196): a count, then index, r,
+    x0 (times scale), x1, k, L0 per body."""
+    tok = _tokens(path)
+    n = int(tok[0])
+    rows = np.array([float(x) for x in tok[1:1 + 10 * n]], dtype=np.float64).reshape(n, 10) if n else np.zeros((0, 10))
+    return BodySet(r=rows[:, 1].copy(), x0=rows[:, 2:5] * scale, x1=rows[:, 5:8].copy(), k=rows[:, 8].copy(),
+                   L0=rows[:, 9].copy())
+
+
+def write_bodies(path: str, r: Sequence[float], x0: np.ndarray, x1: np.ndarray, k: Sequence[float],
+                 L0: Sequence[float]):
+    with open(path, "w") as fh:
+        fh.write(f"{len(r)}\n")
+        for i in range(len(r)):
+            fh.write(f"{i} {r[i]!r} {x0[i][0]!r} {x0[i][1]!r} {x0[i][2]!r} {x1[i][0]!r} {x1[i][1]!r} {x1[i][2]!r} "
+                     f"{k[i]!r} {L0[i]!r}\n")
+
+
 def load_case(case_dir: str, cfg_name: str = "config.cfg", precision: int = PREC_FP64, overrides=None):
-    """Reads a reference-style case directory: cfg plus input/particles.dat and input/objects.dat."""
+    """Reads a reference-style case directory: cfg plus input/particles.dat, input/objects.dat and
+    input/body.dat (when present)."""
     with open(os.path.join(case_dir, cfg_name)) as fh:
         vals = config_values(fh.read(), overrides)
     scale = float(vals.get("scale", 1.0))
     ps = read_particles(os.path.join(case_dir, "input", "particles.dat"), scale)
     obs = read_objects(os.path.join(case_dir, "input", "objects.dat"), scale)
+    body = os.path.join(case_dir, "input", "body.dat")
+    if os.path.exists(body):
+        obs.bodies = read_bodies(body, scale)
     return vals, params_from_values(vals, precision), ps, obs

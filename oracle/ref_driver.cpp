@@ -113,6 +113,18 @@ static void dump(const std::string& path, LB& lb, DEM& dem, IO& io) {
     }
     double sc[6] = {lb.totalMass, tm, dem.demTime, io.realTime, double(lb.time), double(dem.demTimeStep)};
     std::fwrite(sc, 8, 6, fp);
+    // bodies (elmt.h body): count, then LBDEM_B_REC values each
+    const int32_t nb = int32_t(dem.bodies.size());
+    std::fwrite(&nb, 4, 1, fp);
+    for (size_t i = 0; i < dem.bodies.size(); ++i) {
+        const body& B = dem.bodies[i];
+        double r[LBDEM_B_REC] = {};
+        const tVect* v[5] = {&B.x0, &B.x1, &B.FSpring, &B.FTotal, &B.FGrav};
+        const int at[5] = {LBDEM_B_X0, LBDEM_B_X1, LBDEM_B_FSPRING, LBDEM_B_FTOTAL, LBDEM_B_FGRAV};
+        for (int q = 0; q < 5; ++q) { r[at[q]] = v[q]->x; r[at[q] + 1] = v[q]->y; r[at[q] + 2] = v[q]->z; }
+        r[LBDEM_B_L] = B.L; r[LBDEM_B_L0] = B.L0; r[LBDEM_B_K] = B.k; r[LBDEM_B_R] = B.r; r[LBDEM_B_M] = B.m;
+        std::fwrite(r, 8, LBDEM_B_REC, fp);
+    }
     std::fclose(fp);
 }
 

@@ -81,6 +81,11 @@ struct Object {  // object (elmt.h:160-228)
     V3 FContact, FHydro, FLub, FGrav, FSpring, FTotal, MContact, MHydro, MTotal, MLub;
 };
 
+struct Body {  // body (elmt.h), spring-connected to the impactors (DEM.cpp:168-196, 255-266)
+    double r = 0, m = 1, _m = 1, k = 0, L0 = 0, L = 0;
+    V3 x0, x1, FGrav, FSpring, FTotal;
+};
+
 struct Wall {  // wall (utils.h:19-101)
     unsigned index = 0;
     V3 n = V3(1, 0, 0), p;
@@ -124,6 +129,7 @@ struct lbdem_ctx {
     double demSize[3] = {1, 1, 1};
     std::vector<Particle> particles;
     std::vector<Object> objects;
+    std::vector<Body> bodies;
     std::vector<Wall> walls;
     double demInitialRepeat = 0;
     unsigned demTimeStep = 0;
@@ -315,6 +321,12 @@ struct lbdem_ctx {
             O._m = 1.0 / O.m;
             O._I = 1.0 / O.I;
             O.rho = objDensity;
+        }
+        for (auto& B : bodies) {  // DEM.cpp:257-266
+            B.m = prm.bodyDensity * (4.0 / 3.0) * M_PI * (B.r * B.r * B.r);
+            B._m = 1.0 / B.m;
+            B.FGrav = demF * B.m;
+            B.L = B.L0;
         }
         // DEM::initializeWalls (DEM.cpp:405-458); walls are static no-slip (type 7 only)
         walls.clear();
@@ -1299,7 +1311,9 @@ struct lbdem_ctx {
             O.MTotal.reset(); O.MContact.reset(); O.MLub.reset();
         }
         for (auto& W : walls) { W.FTotal.reset(); W.FContact.reset(); W.FLub.reset(); W.FRep.reset(); }
+        for (auto& B : bodies) { B.FTotal.reset(); B.FSpring.reset(); }
         if (int(demTimeStep) >= prm.objWaitTime) {
+            if (!bodies.empty()) bodyObjectInteraction();
             particleLubrication();
             wallLubrication();
             objectParticleContacts();
@@ -1321,6 +1335,49 @@ struct lbdem_ctx {
             O.FTotal += O.FHydro + O.FContact + O.FLub + O.FGrav + O.FSpring;
             O.MTotal += O.MHydro + O.MContact + O.MLub;
         }
+        for (auto& B : bodies) B.FTotal += B.FGrav + B.FSpring;
+    }
+    // DEM.cpp:798-869: every body pulls every impactor along y with a spring of rest length L0
+    // (L = the y distance), with dashpots and cubic / Lennard-Jones / WCA terms by nonLinType
+    void bodyObjectInteraction() {
+        const double springDamp = prm.springDamp, k13 = prm.k13, epsilon = prm.epsilon;
+        for (auto& B : bodies)
+            for (auto& O : objects) {
+                double fs = 0.0, fdo = 0.0, fdb = 0.0;
+                B.L = B.x0.y - O.x0.y;
+                const V3 r = B.x0 - O.x0;
+                const V3 n = V3(r.x / r.norm(), r.y / r.norm(), r.z / r.norm());
+                const double del = B.L0 - B.L;
+                const double bodyDamp = 2.0 * std::sqrt(B.m * B.k) * springDamp;
+                const double objDamp = 2.0 * std::sqrt(O.m * B.k) * springDamp;
+                const double sigma = B.r;
+                switch (prm.nonLinType) {
+                    case 0: fs = B.k * del * n.y; break;
+                    case 1: fs = (B.k * del) * n.y; fdo = -O.x1.y * objDamp; fdb = -B.x1.y * bodyDamp; break;
+                    case 2:
+                        fs = (B.k * del + k13 * B.k * (del * del * del)) * n.y;
+                        fdo = -O.x1.y * objDamp; fdb = -B.x1.y * bodyDamp;
+                        break;
+                    case 3:
+                        fs = (B.k * del + (24.0 * epsilon / r.norm()) * 2.0 * (std::pow((sigma / r.norm()), 12.0))) * n.y;
+                        fdo = -O.x1.y * objDamp; fdb = -B.x1.y * bodyDamp;
+                        break;
+                    default: {
+                        const double wcacut = std::pow(2.0, 1.0 / 6.0) * sigma;
+                        if (r.norm() <= wcacut)
+                            fs = (B.k * del + (24.0 * epsilon / r.norm()) *
+                                                  ((2.0 * std::pow(sigma / r.norm(), 12.0)) - (std::pow(sigma / r.norm(), 6.0)))) * n.y;
+                        else
+                            fs = (B.k * del) * n.y;
+                        fdo = -O.x1.y * objDamp; fdb = -B.x1.y * bodyDamp;
+                        break;
+                    }
+                }
+                O.FSpring.y -= fs;
+                B.FSpring.y += fs;
+                O.FSpring.y += fdo;
+                B.FSpring.y += fdb;
+            }
     }
     void integration() {  // DEM.cpp:2676-2765
         for (auto& P : particles) {
@@ -1334,6 +1391,15 @@ struct lbdem_ctx {
             P.w1.y += deltat * P.MTotal.y * P._I;
             P.w1.z += deltat * P.MTotal.z * P._I;
         }
+        if (int(demTimeStep) >= prm.objWaitTime)
+            for (auto& B : bodies) {  // DEM.cpp:2730-2740
+                B.x0.x += deltat * B.x1.x + deltat * deltat * B.FTotal.x * B._m;
+                B.x0.y += deltat * B.x1.y + deltat * deltat * B.FTotal.y * B._m;
+                B.x0.z += deltat * B.x1.z + deltat * deltat * B.FTotal.z * B._m;
+                B.x1.x += deltat * B.FTotal.x * B._m;
+                B.x1.y += deltat * B.FTotal.y * B._m;
+                B.x1.z += deltat * B.FTotal.z * B._m;
+            }
         if (int(demTimeStep) >= prm.objWaitTime)
             for (auto& O : objects) {
                 O.x0.x += deltat * O.x1.x + deltat * deltat * O.FTotal.x * O._m;
@@ -1428,6 +1494,22 @@ int lbdem_set_objects(lbdem_ctx* c, int n, const double* r, const double* x0, co
     }
     return 0;
 }
+int lbdem_set_bodies(lbdem_ctx* c, int n, const double* r, const double* x0, const double* x1, const double* k,
+                     const double* L0) {
+    if (!c || n < 0 || n > LBDEM_MAX_BODIES || (n > 0 && (!r || !x0 || !x1 || !k || !L0))) return LBDEM_E_ARG;
+    if (n > 0 && (c->prm.nonLinType < 0 || c->prm.nonLinType > 4))
+        return c->fail(LBDEM_E_ARG, "nonLinType must be 0-4 with bodies");
+    c->bodies.assign(n, Body());
+    for (int i = 0; i < n; ++i) {
+        Body& B = c->bodies[i];
+        B.r = r[i];
+        B.x0 = V3(x0[3 * i], x0[3 * i + 1], x0[3 * i + 2]);
+        B.x1 = V3(x1[3 * i], x1[3 * i + 1], x1[3 * i + 2]);
+        B.k = k[i];
+        B.L0 = L0[i];
+    }
+    return 0;
+}
 // the fluidRecycle file's contents (LB.cpp:352-367): kept for latticeBolzmannInit
 int lbdem_set_restart_fluid(lbdem_ctx* c, const int size[3], const uint8_t* types, int nRecords, const double* records) {
     if (!c || !size || !types || nRecords < 0 || (nRecords > 0 && !records)) return LBDEM_E_ARG;
@@ -1481,6 +1563,7 @@ int lbdem_get_info(lbdem_ctx* c, lbdem_info* in) {
     in->neighborListBuilds = c->nlBuilds; in->newNeighborList = c->newNeighborList ? 1 : 0;
     in->totalMassInit = c->totalMass; in->lastExtraMass = c->lastExtraMass; in->lastSurplus = c->lastSurplus;
     in->precision = LBDEM_PREC_FP64;
+    in->nBodies = int(c->bodies.size());
     return 0;
 }
 int lbdem_get_types(lbdem_ctx* c, uint8_t* type, uint8_t* pflag, int32_t* solid) {
@@ -1544,6 +1627,17 @@ int lbdem_get_objects(lbdem_ctx* c, double* rec) {
         put3(r + LBDEM_O_FLUB, O.FLub); put3(r + LBDEM_O_MLUB, O.MLub);
         put3(r + LBDEM_O_FTOTAL, O.FTotal); put3(r + LBDEM_O_MTOTAL, O.MTotal); put3(r + LBDEM_O_FGRAV, O.FGrav);
         r[LBDEM_O_R] = O.r; r[LBDEM_O_M] = O.m;
+    }
+    return 0;
+}
+int lbdem_get_bodies(lbdem_ctx* c, double* rec) {
+    NEED_INIT;
+    for (size_t i = 0; i < c->bodies.size(); ++i) {
+        const Body& B = c->bodies[i];
+        double* r = rec + i * LBDEM_B_REC;
+        put3(r + LBDEM_B_X0, B.x0); put3(r + LBDEM_B_X1, B.x1); put3(r + LBDEM_B_FSPRING, B.FSpring);
+        put3(r + LBDEM_B_FTOTAL, B.FTotal); put3(r + LBDEM_B_FGRAV, B.FGrav);
+        r[LBDEM_B_L] = B.L; r[LBDEM_B_L0] = B.L0; r[LBDEM_B_K] = B.k; r[LBDEM_B_R] = B.r; r[LBDEM_B_M] = B.m;
     }
     return 0;
 }

@@ -132,6 +132,10 @@ def _case_values(name: str) -> Tuple[Dict[str, object], np.ndarray, np.ndarray, 
         pc = np.array([[8.05, 11.25, 8.05], [8.05, 6.95, 8.05], [3.4, 2.5, 3.5], [12.6, 5.0, 12.2],
                        [12.3, 12.0, 3.8], [4.0, 9.0, 12.0]])
         return v, pr, pc, np.array([3.0]), np.array([[8.0, 16.55, 8.0]]), np.array([[0.0, -0.06, 0.0]])
+    if name in BODY_CASES:
+        v, pr, pc, orr, oc, ov = _case_values("suspension")
+        v.update(BODY_CASES[name][0])
+        return v, pr, pc, orr, oc, ov
     if name in RHEOLOGY_CASES:
         # the collision with per-node apparent viscosity (node.cpp:113-168) on an existing geometry
         base, extra = RHEOLOGY_CASES[name]
@@ -146,6 +150,16 @@ def _case_values(name: str) -> Tuple[Dict[str, object], np.ndarray, np.ndarray, 
 
 
 CASES = ("fs_impactor", "suspension", "periodic", "units", "lb_closed", "moving", "startup", "two_impactors")
+# body-spring cases (DEM.cpp:798-869) on the suspension geometry: name -> (config values, body.dat
+# rows r, x0, x1, k, L0). body_wca: nonLinType 4 with a body close enough for the WCA term (within
+# 2^(1/6) r of the impactor centre); body_cubic: nonLinType 2, two bodies pulling one impactor.
+BODY_CASES = {
+    "body_wca": ({"nonLinType": 4, "springDamp": 0.3, "epsilon": 0.002},
+                 [(5.0, (8.0, 21.0, 8.0), (0.0, 0.0, 0.0), 0.05, 3.0)]),
+    "body_cubic": ({"nonLinType": 2, "springDamp": 0.2, "k13": 0.5},
+                   [(1.0, (8.0, 20.5, 8.0), (0.0, 0.01, 0.0), 0.05, 3.0),
+                    (0.8, (6.0, 21.0, 9.0), (0.0, 0.0, 0.0), 0.02, 4.5)]),
+}
 # rheology cases: name -> (geometry case, config values). BINGHAM reads no initVisc (LB.cpp:211-218)
 # and keeps the default 1/6; the Smagorinsky term is added to the model's viscosity (LB.cpp:242-245)
 RHEOLOGY_CASES = {
@@ -199,8 +213,9 @@ def host_steps(values: Dict[str, object]) -> int:
 
 HOST_RUNS = {"suspension": {"maxTime": 39.0, "screenExpTime": 4.0},
              "fs_impactor": {"maxTime": 29.0, "screenExpTime": 3.0},
-             "moving": {"maxTime": 29.0, "screenExpTime": 2.0}}
-HOST_FILES = ("objPosition.dat", "objForce.dat", "wallForce.dat")
+             "moving": {"maxTime": 29.0, "screenExpTime": 2.0},
+             "body_cubic": {"maxTime": 39.0, "screenExpTime": 4.0}}
+HOST_FILES = ("objPosition.dat", "objForce.dat", "wallForce.dat", "bodyPosition.dat")
 
 
 def make_case(root: str, name: str, overrides: Dict[str, object] = None, subdir: str = None) -> str:
@@ -215,8 +230,13 @@ def make_case(root: str, name: str, overrides: Dict[str, object] = None, subdir:
     x1, w1 = VELOCITIES.get(name, (None, None))
     lbdem.write_particles(os.path.join(d, "input", "particles.dat"), pr, pc, x1=x1, w1=w1)
     lbdem.write_objects(os.path.join(d, "input", "objects.dat"), orr, ox0, ox1)
-    with open(os.path.join(d, "input", "body.dat"), "w") as fh:
-        fh.write("0\n")
+    if name in BODY_CASES:
+        rows = BODY_CASES[name][1]
+        lbdem.write_bodies(os.path.join(d, "input", "body.dat"), [b[0] for b in rows], [b[1] for b in rows],
+                           [b[2] for b in rows], [b[3] for b in rows], [b[4] for b in rows])
+    else:
+        with open(os.path.join(d, "input", "body.dat"), "w") as fh:
+            fh.write("0\n")
     if name == "restart":
         import gzip
         src = os.path.join(GOLDEN, "recycle_suspension", recycle_name(RESTART_FROM) + ".gz")
@@ -256,6 +276,8 @@ def read_dump(path: str) -> Dict[str, object]:
     out["nlist"] = take(np.int32, npart * 64).reshape(npart, 64)
     sc = take(np.float64, 6)
     out["totalMassInit"], out["totalMass"], out["demTime"], out["realTime"] = sc[0], sc[1], sc[2], sc[3]
+    nb = int(take(np.int32, 1)[0]) if off < len(buf) else 0  # dumps of older drivers end here
+    out["bodies"] = take(np.float64, nb * lbdem.B["REC"]).reshape(nb, lbdem.B["REC"])
     return out
 
 
@@ -273,7 +295,8 @@ def state(sim: "lbdem.Simulation", with_f: bool = True) -> Dict[str, object]:
     f, m, n = sim.fields(with_f)
     cnt, lst = sim.neighbors()
     return dict(type=t, pflag=p, solid=s, f=f, mass=m, n=n, particles=sim.particles(), objects=sim.objects(),
-                walls=sim.walls(), ncount=cnt, nlist=lst, totalMass=sim.total_mass(), info=sim.info())
+                walls=sim.walls(), bodies=sim.bodies(), ncount=cnt, nlist=lst, totalMass=sim.total_mass(),
+                info=sim.info())
 
 
 def _n_int(x):

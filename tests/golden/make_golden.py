@@ -40,7 +40,8 @@ DRIVER = os.path.join(cases.REPO, "oracle", "_ref", "ref_driver")
 PLAN = {"lb_closed": [1, 12], "fs_impactor": [1, 40], "suspension": [1, 40, 120], "periodic": [40], "units": [40],
         "moving": [1, 30], "startup": [5, 30], "two_impactors": [1, 40],
         "cells": [1, 26, 60], "thin_z2": [1, 40], "thin_z1": [40], "thin_z0": [20], "restart": [0, 1, 20],
-        "bingham": [1, 40], "frictional": [1, 40], "voellmy": [1, 40], "smagorinsky": [1, 12]}
+        "bingham": [1, 40], "frictional": [1, 40], "voellmy": [1, 40], "smagorinsky": [1, 12],
+        "body_wca": [1, 40], "body_cubic": [1, 40]}
 SAMPLE_ABOVE = 8_000   # nodes
 SAMPLE_NODES = 1536
 
@@ -166,7 +167,7 @@ def main():
                     sampled=np.array(dump["N"] > SAMPLE_ABOVE),
                     f_active=dump["f"][:, sel], n_active=dump["n"][sel], mass_active=dump["mass"][sel],
                     particles=dump["particles"], objects=dump["objects"], walls=dump["walls"],
-                    ncount=dump["ncount"], nlist=dump["nlist"], totalMass=dump["totalMass"],
+                    ncount=dump["ncount"], nlist=dump["nlist"], totalMass=dump["totalMass"], bodies=dump["bodies"],
                     totalMassInit=dump["totalMassInit"], size=np.array(dump["size"]))
                 manifest[f"{name}_s{s}"] = dict(case=name, step=s, size=list(dump["size"]),
                                                  active=int(act.sum()), bytes=os.path.getsize(out))

@@ -48,6 +48,7 @@ def compare(A, B, prec, where, free_surface=True):
     assert rel(A["particles"], B["particles"]) <= t["part"], f"{where}: particle records"
     assert rel(A["objects"], B["objects"]) <= t["obj"], f"{where}: impactor records"
     assert rel(A["walls"], B["walls"]) <= t["wall"], f"{where}: wall records"
+    assert rel(A["bodies"], B["bodies"]) <= t["obj"], f"{where}: body records"
     assert np.array_equal(A["ncount"], B["ncount"]), f"{where}: neighbour counts"
     # partner multisets: the reference (and the oracle) list partners in cell-walk order, the
     # library in ascending order
@@ -155,6 +156,14 @@ def test_rheology_matches_oracle(oracle_lib, gpu_lib, case_root, name, prec):
     run_pair(oracle_lib, gpu_lib, case_root, name, prec, 40, every=5)
 
 
+@pytest.mark.parametrize("prec", [lbdem.PREC_FP64, lbdem.PREC_FP32])
+@pytest.mark.parametrize("name", sorted(cases.BODY_CASES))
+def test_bodies_match_oracle(oracle_lib, gpu_lib, case_root, name, prec):
+    """Spring-connected bodies (DEM.cpp:798-869, 2730-2740): WCA and cubic springs with dashpots
+    pulling the impactor, 60 steps compared every 10 (body and impactor records included)."""
+    run_pair(oracle_lib, gpu_lib, case_root, name, prec, 60, every=10)
+
+
 def test_restart_viscosities_match_oracle(oracle_lib, gpu_lib, case_root):
     """A Newtonian restart whose records carry viscosities other than the run's: every node relaxes
     with its own (node.cpp:40-50, 181-207), new interface nodes take their donor's (LB.cpp:1719) and
@@ -195,7 +204,7 @@ GOLD = os.path.join(os.path.dirname(os.path.abspath(__file__)), "golden")
 
 @pytest.mark.parametrize("name", ["suspension", "fs_impactor", "periodic", "units", "moving", "startup", "two_impactors",
                                   "cells", "thin_z2", "thin_z1", "thin_z0", "restart", "bingham", "frictional",
-                                  "voellmy"])
+                                  "voellmy", "body_wca", "body_cubic"])
 def test_fp64_matches_reference_fixtures(gpu_lib, case_root, name):
     with open(os.path.join(GOLD, "manifest.json")) as fh:
         steps = sorted(m["step"] for m in json.load(fh).values() if m["case"] == name)
@@ -214,6 +223,8 @@ def test_fp64_matches_reference_fixtures(gpu_lib, case_root, name):
         assert np.abs(st["mass"][act] - g["mass_active"]).max() <= 1e-11
         assert rel(g["particles"], st["particles"]) <= 1e-10
         assert rel(g["objects"], st["objects"]) <= 1e-10
+        if "bodies" in g:
+            assert rel(g["bodies"], st["bodies"]) <= 1e-10
         assert abs(st["totalMass"] - float(g["totalMass"])) <= 1e-11 * abs(float(g["totalMass"]))
         assert np.array_equal(st["ncount"], g["ncount"])
         for p in range(len(g["ncount"])):

@@ -70,6 +70,8 @@ def test_host_program_on_the_gpu(host_program, gpu_lib, tmp_path, name):
         a = table(os.path.join(out, f))
         b = table(os.path.join(GOLD, f"host_{name}", f))
         assert a.shape == b.shape, f
+        if b.size == 0:  # a header only (bodyPosition.dat of a run without bodies)
+            continue
         scale = np.maximum(np.abs(b).max(0), 1e-30)
         err = np.abs(a - b) / scale
         assert err.max() <= 2e-5, f"{name}: {f} differs by {err.max()} of the column scale"

@@ -39,6 +39,8 @@ def test_oracle_matches_reference_fixtures(oracle_lib, case_root, name):
         assert np.array_equal(st["particles"], g["particles"]), f"{name} step {step}: particles"
         assert np.array_equal(st["objects"], g["objects"]), f"{name} step {step}: objects"
         assert np.array_equal(st["walls"], g["walls"]), f"{name} step {step}: walls"
+        if "bodies" in g:
+            assert np.array_equal(st["bodies"], g["bodies"]), f"{name} step {step}: bodies"
         assert np.array_equal(st["ncount"], g["ncount"]), f"{name} step {step}: neighbour counts"
         cnt = g["ncount"]
         for p in range(len(cnt)):
