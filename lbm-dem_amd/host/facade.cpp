@@ -6,12 +6,16 @@
 #include <cstdio>
 #include <cstdlib>
 #include <fstream>
+#include <iomanip>
 #include <iostream>
 #include <stdexcept>
 
 namespace lbdem {
 
 void tVect::print(std::ostream& os) const { os << x << " " << y << " " << z << " "; }
+void tVect::printFixedLine(std::ostream& os) const {
+    os << std::setprecision(8) << std::fixed << x << " " << y << " " << z << "\n";
+}
 
 // ------------------------------------------------------------------------------------------
 // Context
@@ -61,6 +65,7 @@ void parseConfigFile(IO& io, DEM& dem, LB& lb, ConfigFile& cfg, CommandLine& cl)
     io.fluidExpTime = parse_value(cfg, cl, "fluidExpTime", 0.0);
     io.fluidLagrangianExpTime = parse_value(cfg, cl, "fluidLagrangianExpTime", 0.0);
     io.fluidRecycleExpTime = parse_value(cfg, cl, "fluidRecycleExpTime", 0.0);
+    io.partExpTime = parse_value(cfg, cl, "partExpTime", 0.0);
     io.demSolve = parse_value(cfg, cl, "demSolve", 0);
     io.lbmSolve = parse_value(cfg, cl, "lbSolve", 0);
     lb.freeSurface = parse_value(cfg, cl, "freeSurfaceSolve", 0);
@@ -380,6 +385,11 @@ void IO::initialize() {
     fluidFileFormat = fluidDirectory + "/fluid%010u.vti";
     fluidLagrangianFileFormat = fluidDirectory + "/fluidLag%010u.vtu";
     fluidRecycleFileFormat = fluidDirectory + "/fluidRecycle%010u.dat";
+    velFileFormat = partDirectory + "/vel%04u.vtk";
+    boundFileFormat = partDirectory + "/bound%04u.vtk";
+    objectFileFormat = partDirectory + "/object%04u.vtk";
+    bodyFileFormat = partDirectory + "/body%04u.vtk";
+    springFileFormat = partDirectory + "/spring%04u.vtk";
     exportFileName = workDirectory + "/export.dat";
     if (demSolve) {
         objectPositionFileName = workDirectory + "/objPosition.dat";
@@ -405,6 +415,7 @@ void IO::initialize() {
     }
     lastScreenExp = 0;
     lastFluidRecycleExp = lastFluidExp = lastFluidLagrangianExp = 0;
+    lastVelExp = lastBoundExp = lastObjectExp = lastBodyExp = lastSpringExp = 0;
 }
 
 void IO::outputStep(LB& lb, DEM& dem) {
@@ -441,9 +452,43 @@ bool IO::fluidVelocityDue() const {
                         exp_counter(realTime, fluidRecycleExpTime) > lastFluidRecycleExp);
 }
 
-void IO::createFiles(LB& lb, DEM&) {
-    if (!lbmSolve || !c_.initialised()) return;
+void IO::createFiles(LB& lb, DEM& dem) {
+    if (!c_.initialised()) return;
     char path[1024];
+    if (demSolve) {
+        // IO.cpp:212-256 (one interval, partExpTime, for all of them); the particle, force-chain and
+        // particle-recycle files carry stress diagnostics this build does not compute
+        const unsigned cv = exp_counter(realTime, partExpTime);
+        const bool due = cv > lastVelExp || cv > lastBoundExp || (!dem.objects.empty() && cv > lastObjectExp) ||
+                         cv > lastBodyExp || cv > lastSpringExp;
+        if (due) dem.refresh();
+        if (cv > lastVelExp) {
+            lastVelExp = cv;
+            std::snprintf(path, sizeof(path), velFileFormat.c_str(), currentTimeStep);
+            exportParaviewParticleVelocities(dem.particles, path);
+        }
+        if (cv > lastBoundExp) {
+            lastBoundExp = cv;
+            std::snprintf(path, sizeof(path), boundFileFormat.c_str(), currentTimeStep);
+            exportParaviewBoundaries(lb, path);
+        }
+        if (!dem.objects.empty() && cv > lastObjectExp) {
+            lastObjectExp = cv;
+            std::snprintf(path, sizeof(path), objectFileFormat.c_str(), currentTimeStep);
+            exportParaviewObjects(dem.objects, path);
+        }
+        if (cv > lastBodyExp) {
+            lastBodyExp = cv;
+            std::snprintf(path, sizeof(path), bodyFileFormat.c_str(), currentTimeStep);
+            exportParaviewBodies(dem.bodies, path);
+        }
+        if (cv > lastSpringExp) {
+            lastSpringExp = cv;
+            std::snprintf(path, sizeof(path), springFileFormat.c_str(), currentTimeStep);
+            exportParaviewSprings(dem.bodies, dem.objects, path);
+        }
+    }
+    if (!lbmSolve) return;
     unsigned counter = exp_counter(realTime, fluidExpTime);
     if (counter > lastFluidExp) {
         lastFluidExp = counter;
@@ -612,6 +657,103 @@ void IO::exportObjectPosition(const objectList& objects, const LB& lb) {
         f << "\t" << depth;
         f << std::endl;
     }
+}
+
+// legacy VTK polydata of the particle centres with their velocities
+void IO::exportParaviewParticleVelocities(const particleList& particles, const std::string& file) {
+    std::ofstream out(file);
+    const int n = int(particles.size());
+    out << "# vtk DataFile Version 3.0\n" << "Data.vtk\n" << "ASCII\n" << "DATASET POLYDATA\n";
+    out << "POINTS " << n << " float\n";
+    for (const particle& p : particles) p.x0.printFixedLine(out);
+    out << "\n";
+    out << " POINT_DATA " << n << "\n";
+    out << "VECTORS velocity float\n";
+    for (const particle& p : particles) p.x1.printFixedLine(out);
+    out << "\n";
+}
+
+// the 12 edges of the DEM box (x, fluid height with a free surface, z) as line cells
+void IO::exportParaviewBoundaries(const LB& lb, const std::string& file) {
+    std::ofstream out(file);
+    const double L = lb.unitLength;
+    const double X = double(lb.lbSize[0]) * L, Z = double(lb.lbSize[2]) * L;
+    const double Y = lb.freeSurface ? lb.fluidHeight : double(lb.lbSize[1]) * L;
+    out << "# vtk DataFile Version 3.0\n" << "DataFC.vtk\n" << "ASCII\n" << "DATASET POLYDATA\n" << "\n";
+    out << "POINTS " << 24 << " float\n";
+    const double corner[24][3] = {{0.0, 0.0, 0.0}, {X, 0.0, 0.0}, {X, 0.0, 0.0}, {X, 0.0, Z}, {X, 0.0, Z}, {0.0, 0.0, Z},
+                                  {0.0, 0.0, Z}, {0.0, 0.0, 0.0},
+                                  {0.0, 0.0, 0.0}, {0.0, Y, 0.0}, {X, 0.0, 0.0}, {X, Y, 0.0}, {X, 0.0, Z}, {X, Y, Z},
+                                  {0.0, 0.0, Z}, {0.0, Y, Z},
+                                  {0.0, Y, 0.0}, {X, Y, 0.0}, {X, Y, 0.0}, {X, Y, Z}, {X, Y, Z}, {0.0, Y, Z},
+                                  {0.0, Y, Z}, {0.0, Y, 0.0}};
+    for (const auto& c : corner) out << c[0] << " " << c[1] << " " << c[2] << "\n";
+    out << "\n";
+    out << "LINES " << 12 << " " << 36 << "\n";
+    for (int i = 0; i < 12; i++) out << "2 " << 2 * i << " " << 2 * i + 1 << "\n";
+    out << "\n";
+    out << "POINT_DATA " << 24 << "\n";
+    out << "SCALARS radius float\n";
+    out << "LOOKUP_TABLE default \n";
+    for (int i = 0; i < 24; ++i) out << "1.0\n";
+}
+
+// impactor centres as vertex cells with their diameters
+void IO::exportParaviewObjects(const objectList& objects, const std::string& file) {
+    std::ofstream out(file);
+    const int n = int(objects.size());
+    out << "# vtk DataFile Version 3.0\n" << "Data.vtk\n" << "ASCII\n" << "DATASET UNSTRUCTURED_GRID\n" << "  \n";
+    out << "POINTS " << n << " float\n";
+    for (const object& o : objects) o.x0.printFixedLine(out);
+    out << "\n";
+    out << "CELL_TYPES " << n << "\n";
+    for (int i = 0; i < n; ++i) out << "1\n";
+    out << "\n";
+    out << "POINT_DATA " << n << "\n";
+    out << "SCALARS diameter float\n";
+    out << "LOOKUP_TABLE default \n";
+    for (const object& o : objects) out << 2.0 * o.r << "\n";
+    out << "\n";
+}
+
+// body centres with their diameters
+void IO::exportParaviewBodies(const bodyList& bodies, const std::string& file) {
+    std::ofstream out(file);
+    out << "# vtk DataFile Version 3.0\n" << "Data.vtk\n" << "ASCII\n" << "DATASET POLYDATA\n" << "\n";
+    out << "POINTS " << bodies.size() << " float\n";
+    for (const body& b : bodies) b.x0.printFixedLine(out);
+    out << "\n";
+    out << "POINT_DATA " << bodies.size() << "\n";
+    out << "SCALARS diameter float\n";
+    out << "LOOKUP_TABLE default \n";
+    for (const body& b : bodies) out << 2.0 * b.r << "\n";
+    out << "\n";
+}
+
+// the spring as one line from the (first) body to the (first) impactor, with its compression
+// L - L0 at both ends; the point count is 2 whatever the body and impactor counts, as written by
+// the reference
+void IO::exportParaviewSprings(const bodyList& bodies, const objectList& objects, const std::string& file) {
+    std::ofstream out(file);
+    out << "# vtk DataFile Version 3.0\n" << "Data.vtk\n" << "ASCII\n" << "DATASET POLYDATA\n" << "\n";
+    out << "POINTS " << 2 << " float\n";
+    for (const body& b : bodies) b.x0.printFixedLine(out);
+    for (const object& o : objects) o.x0.printFixedLine(out);
+    out << "\n";
+    out << "LINES " << 1 << " " << 3 << "\n";
+    out << "2 " << 0 << " " << 1 << "\n";
+    out << "\n";
+    out << "POINT_DATA " << 2 << "\n";
+    out << "SCALARS radius float\n";
+    out << "LOOKUP_TABLE default \n";
+    out << "1.0\n" << "1.0\n" << "\n";
+    out << "SCALARS compression float\n";
+    out << "LOOKUP_TABLE default \n";
+    for (const body& b : bodies) {
+        out << b.L - b.L0 << "\n";
+        out << b.L - b.L0 << "\n";
+    }
+    out << "\n";
 }
 
 void IO::exportBodyPosition(const bodyList& bodies) {

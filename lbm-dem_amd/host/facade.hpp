@@ -27,6 +27,8 @@ struct tVect {
     double dot(const tVect& o) const { return x * o.x + y * o.y + z * o.z; }
     // "x y z " with the stream's current formatting (the reference's tVect::print)
     void print(std::ostream& os) const;
+    // "x y z\n" fixed with 8 digits, left set on the stream (tinyVector::printFixedLine, myvector.cpp:25-27)
+    void printFixedLine(std::ostream& os) const;
 };
 
 // host mirrors of the element records (fields on the path; elmt.h:15-228, utils.h:19-101)
@@ -158,6 +160,8 @@ public:
     double realTime = 0.0, maxTime = 0.0, screenExpTime = 0.0;
     double fluidRecycleExpTime = 0.0;              // main.cpp:157
     double fluidExpTime = 0.0, fluidLagrangianExpTime = 0.0;  // main.cpp:148-151
+    // main.cpp:155-162: every DEM file's interval is partExpTime
+    double partExpTime = 0.0;
     unsigned int currentTimeStep = 0;
     int demSolve = 0, lbmSolve = 0;
 
@@ -166,6 +170,12 @@ private:
     void exportObjectForce(const objectList& objects, const LB& lb);      // IO.cpp:1239-1293
     void exportWallForce(const wallList& walls);                          // IO.cpp:1295-1314
     void exportBodyPosition(const bodyList& bodies);                      // IO.cpp:1340-1353
+    // the DEM VTK files of createFiles that need no stress diagnostics (IO.cpp:212-256)
+    void exportParaviewParticleVelocities(const particleList& particles, const std::string& file);  // IO.cpp:651-700
+    void exportParaviewBoundaries(const LB& lb, const std::string& file);                          // IO.cpp:829-912
+    void exportParaviewObjects(const objectList& objects, const std::string& file);                // IO.cpp:702-751
+    void exportParaviewBodies(const bodyList& bodies, const std::string& file);                    // IO.cpp:753-784
+    void exportParaviewSprings(const bodyList& bodies, const objectList& objects, const std::string& file);  // IO.cpp:786-827
     void exportMass(const LB& lb);                                        // step, time, total fluid mass
     void createFiles(LB& lb, DEM& dem);                                   // IO.cpp:181-289 (the fluid files)
     void exportRecycleFluid(const LB& lb, const std::string& file);       // IO.cpp:973-997
@@ -180,6 +190,8 @@ private:
     const FluidState& fluidState(const LB& lb, bool populations, bool velocity);
     Context& c_;
     unsigned lastScreenExp = 0, lastFluidRecycleExp = 0, lastFluidExp = 0, lastFluidLagrangianExp = 0;
+    unsigned lastVelExp = 0, lastBoundExp = 0, lastObjectExp = 0, lastBodyExp = 0, lastSpringExp = 0;
+    std::string velFileFormat, boundFileFormat, objectFileFormat, bodyFileFormat, springFileFormat;
     std::string exportFileName, objectPositionFileName, objectForceFileName, wallForceFileName, massFileName;
     std::string bodyPositionFileName;
     std::string partDirectory, fluidDirectory, fluidRecycleFileFormat, fluidFileFormat, fluidLagrangianFileFormat;

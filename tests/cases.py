@@ -201,6 +201,13 @@ def recycle_name(step: int) -> str:
 FLUID_RUN = ("units", {"maxTime": 0.6, "screenExpTime": 0.2, "fluidExpTime": 0.3, "fluidLagrangianExpTime": 0.3})
 
 
+# the DEM VTK files (IO.cpp:212-256, every one on partExpTime) that need no stress diagnostics:
+# particle velocities, box edges, impactors, bodies, springs (kept gzip-compressed under
+# golden/dem_body_cubic/)
+DEM_RUN = ("body_cubic", {"maxTime": 20.0, "screenExpTime": 10.0, "partExpTime": 10.0})
+DEM_FILES = ("vel", "bound", "object", "body", "spring")
+
+
 def host_steps(values: Dict[str, object]) -> int:
     """Steps the reference's main loop runs (main.cpp:288-308: until realTime > maxTime, realTime
     advancing by unitTime per step)."""

@@ -87,10 +87,10 @@ def host_outputs():
             print("host outputs of", name, "->", dst)
 
 
-def fluid_files(run, prefix, only=None):
+def fluid_files(run, prefix, only=None, sub="fluidData", starts=None):
     """Runs the reference's main loop on a file run (case, overrides) for as many steps as its own
-    loop would take, and keeps the files it writes into fluidData/ (all of them, or those named in
-    ``only``) gzip-compressed under <prefix>_<case>/."""
+    loop would take, and keeps the files it writes into <sub>/ (all of them, those named in ``only``
+    or those whose names start with one of ``starts``) gzip-compressed under <prefix>_<case>/."""
     import gzip
     name, over = run
     with tempfile.TemporaryDirectory() as tmp:
@@ -102,10 +102,12 @@ def fluid_files(run, prefix, only=None):
                            capture_output=True, text=True)
         if r.returncode:
             sys.exit(f"reference run failed for {prefix}_{name}:\n{r.stdout[-2000:]}\n{r.stderr[-2000:]}")
-        src_dir = os.path.join(d, "out", "fluidData")
+        src_dir = os.path.join(d, "out", sub)
         dst = os.path.join(HERE, f"{prefix}_{name}")
         os.makedirs(dst, exist_ok=True)
         names = sorted(os.listdir(src_dir)) if only is None else list(only)
+        if starts is not None:
+            names = [f for f in names if f.startswith(tuple(starts))]
         for f in names:
             with open(os.path.join(src_dir, f), "rb") as a, gzip.GzipFile(os.path.join(dst, f + ".gz"), "wb",
                                                                          mtime=0) as b:
@@ -115,9 +117,11 @@ def fluid_files(run, prefix, only=None):
 
 def recycle_outputs():
     """The fluidRecycle files the reference writes on the recycle run (IO.cpp:203-210,
-    973-997), gzip-compressed under recycle_<case>/ (the restart case reads one back), and the
-    Eulerian / Lagrangian fluid files of the fluid run under fluid_<case>/."""
+    973-997), gzip-compressed under recycle_<case>/ (the restart case reads one back), the DEM VTK
+    files of the DEM run under dem_<case>/, and the Eulerian / Lagrangian fluid files of the fluid
+    run under fluid_<case>/."""
     fluid_files(cases.RECYCLE_RUN, "recycle", [cases.recycle_name(s) for s in cases.RECYCLE_STEPS])
+    fluid_files(cases.DEM_RUN, "dem", sub="particleData", starts=cases.DEM_FILES)
     fluid_files(cases.FLUID_RUN, "fluid")
 
 

@@ -211,3 +211,60 @@ def test_host_program_fluid_files_on_the_gpu(host_program, gpu_lib, tmp_path):
             assert a.shape == b.shape, f"{f} section {k}"
             if b.size:
                 assert np.abs(a - b).max() <= 2e-5 * max(np.abs(b).max(), 1e-30), f"{f} section {k}"
+
+
+# ---------------------------------------------------------------------------------------------
+# DEM VTK files (IO.cpp:212-256, 651-912): particle velocities, box edges, impactors, bodies and
+# springs, every one on partExpTime
+# ---------------------------------------------------------------------------------------------
+DEM_DIR = "dem_" + cases.DEM_RUN[0]
+
+
+def dem_gold_names():
+    return sorted(f[:-3] for f in os.listdir(os.path.join(GOLD, DEM_DIR)))
+
+
+def run_dem(prog, root, library, extra=()):
+    name, over = cases.DEM_RUN
+    d = cases.make_case(str(root), name, over)
+    r = subprocess.run([prog, "-c", "config.cfg", "-d", "out", "--library", library, *extra], cwd=d,
+                       capture_output=True, text=True, timeout=600)
+    assert r.returncode == 0, r.stdout[-3000:] + r.stderr[-2000:]
+    out = os.path.join(d, "out", "particleData")
+    return out, sorted(f for f in os.listdir(out) if f.startswith(cases.DEM_FILES))
+
+
+def test_host_program_writes_the_reference_dem_files(host_program, oracle_lib, tmp_path):
+    """With the FP64 oracle behind the ABI the velocity, boundary, impactor, body and spring VTK
+    files are byte-identical to the reference's (fixed 8-digit coordinates, the box of the fluid
+    height, the spring file's two points and compressions as the reference writes them)."""
+    out, names = run_dem(host_program, tmp_path, ORACLE_LIB)
+    assert names == dem_gold_names()
+    for f in names:
+        with open(os.path.join(out, f)) as fh:
+            assert fh.read() == gold_text(DEM_DIR, f), f
+
+
+@pytest.mark.gpu
+def test_host_program_dem_files_on_the_gpu(host_program, gpu_lib, tmp_path):
+    """The MI355X library behind the host program (FP64) writes the same DEM VTK files: the same
+    words, the numbers to the printed precision."""
+    out, names = run_dem(host_program, tmp_path, GPU_LIB, extra=("--precision", "fp64"))
+    assert names == dem_gold_names()
+    for f in names:
+        with open(os.path.join(out, f)) as fh:
+            a_tok = fh.read().split()
+        b_tok = gold_text(DEM_DIR, f).split()
+        assert len(a_tok) == len(b_tok), f
+
+        def num(t):
+            try:
+                return float(t)
+            except ValueError:
+                return None
+        an, bn = [num(t) for t in a_tok], [num(t) for t in b_tok]
+        assert [t for t, v in zip(a_tok, an) if v is None] == [t for t, v in zip(b_tok, bn) if v is None], f
+        a = np.array([v for v in an if v is not None])
+        b = np.array([v for v in bn if v is not None])
+        if b.size:
+            assert np.abs(a - b).max() <= 1e-6 * max(np.abs(b).max(), 1.0), f
