@@ -58,7 +58,9 @@ typedef struct lbdem_params {
     double lbSize[3];            /* lbSizeX/Y/Z (physical) */
     double initVelocity[3];      /* physical */
     double lbF[3];               /* physical acceleration */
-    int boundary[6];             /* boundary0..5: 4 periodic, 7 no-slip static wall */
+    int boundary[6];             /* boundary0..5: 4 periodic, 5-8 wall; every wall is of boundary0's
+                                    kind (DEM.cpp:430-446): 7 static or (boundary0 8) moving with
+                                    zero velocity (type 8); boundary0 5 or 6 (slip) is rejected */
     double fluidHeight;          /* physical */
     double initVisc;             /* physical dynamic viscosity (NEWTONIAN, FRICTIONAL, VOELLMY; the
                                     reference keeps its default (1.0 - 0.5) / 3.0 for BINGHAM) */

@@ -61,6 +61,7 @@ struct lbdem_ctx {
     // (viscUpdate, or restart records with viscosities other than initDynVisc)
     double plasticVisc = 0, yieldStress = 0, voellmyCoef = 0;
     bool viscUpdate = false, viscPlane = false;
+    bool dynWalls = false;  // wall nodes are reported as moving walls (type 8)
     bool freeSurface = false, forceField = false, lbFReset = false;
     // inputs
     std::vector<int> psize;
@@ -200,8 +201,14 @@ static int read_params(lbdem_ctx* c) {
         c->lbF[a] = P.lbF[a] / u.Accel;
     }
     for (int i = 0; i < 6; ++i)
-        if (P.boundary[i] != 4 && P.boundary[i] != 7)
-            return c->fail(LBDEM_E_ARG, "only boundary types 4 (periodic) and 7 (static no-slip wall) are supported");
+        if (P.boundary[i] != 4 && (P.boundary[i] < 5 || P.boundary[i] > 8))
+            return c->fail(LBDEM_E_ARG, "boundary codes must be 4 (periodic) or 5-8 (walls)");
+    // every wall takes boundary0's kind (DEM.cpp:430-446 test externalBoundary[0] for all walls):
+    // slip walls are not supported; a moving (8) wall keeps the zero velocity its nodes are given
+    // (LB.cpp:1162-1169), so it reflects as a static wall and only its type code differs
+    if (P.boundary[0] == 5 || P.boundary[0] == 6)
+        return c->fail(LBDEM_E_ARG, "slip walls (boundary0 5 or 6) are not supported");
+    c->dynWalls = P.boundary[0] == 8;
     c->fluidHeight = P.fluidHeight;
     c->initDynVisc = P.initVisc / u.DynVisc;
     if (P.rheologyModel < LBDEM_RHEO_NEWTONIAN || P.rheologyModel > LBDEM_RHEO_VOELLMY)
@@ -1224,8 +1231,9 @@ extern "C" int lbdem_get_types(lbdem_ctx* c, uint8_t* type, uint8_t* pflag, int3
     HIPCHK(hipMemcpyAsync(t.data(), c->box.type, N, hipMemcpyDeviceToHost, c->stream));
     if (solid) HIPCHK(hipMemcpyAsync(solid, c->box.solid, N * 4, hipMemcpyDeviceToHost, c->stream));
     HIPCHK(hipStreamSynchronize(c->stream));
+    const uint8_t wallCode = c->dynWalls ? 8 : T_WALL;
     for (int64_t i = 0; i < N; ++i) {
-        if (type) type[i] = t[i] & T_BASE;
+        if (type) type[i] = (t[i] & T_BASE) == T_WALL ? wallCode : uint8_t(t[i] & T_BASE);
         if (pflag) pflag[i] = (t[i] & T_PMASK) ? 1 : 0;
     }
     return 0;

@@ -104,6 +104,7 @@ struct lbdem_ctx {
     double lbPhysicalSize[3];
     V3 lbBoundaryLocation[6];
     int boundary[6];
+    int wallType = 7;  // node type of every wall: 7 static or 8 dynamic (moving, with zero velocity)
     double fluidHeight = 0;
     double initDynVisc = 0, initDensity = 1.0;
     // FluidMaterial (node.h:17-49): rheology model and its parameters in lattice units
@@ -258,7 +259,8 @@ struct lbdem_ctx {
         lbF /= unit.Accel;
         for (int i = 0; i < 6; ++i) {
             boundary[i] = P.boundary[i];
-            if (boundary[i] != 4 && boundary[i] != 7) return fail(LBDEM_E_ARG, "only boundary types 4 (periodic) and 7 (static no-slip wall) are supported");
+            if (boundary[i] != 4 && (boundary[i] < 5 || boundary[i] > 8))
+                return fail(LBDEM_E_ARG, "boundary codes must be 4 (periodic) or 5-8 (walls)");
         }
         fluidHeight = P.fluidHeight;
         rheologyModel = P.rheologyModel;
@@ -278,6 +280,10 @@ struct lbdem_ctx {
         freeSurface = P.freeSurface != 0;
         forceField = P.forceField != 0;
         demInitialRepeat = P.demInitialRepeat;
+        // every wall takes boundary0's kind (DEM.cpp:430-446 test externalBoundary[0]); slip walls
+        // (5, 6) are not restated
+        if (boundary[0] == 5 || boundary[0] == 6) return fail(LBDEM_E_ARG, "slip walls (boundary0 5 or 6) are not supported");
+        wallType = boundary[0] == 8 ? 8 : 7;
         if (!P.lbSolve) return fail(LBDEM_E_ARG, "lbSolve=0 is not supported");
         if (!(P.density > 0) || !(P.objectDensity > 0)) return fail(LBDEM_E_ARG, "density/objectDensity must be > 0");
         return 0;
@@ -378,7 +384,7 @@ struct lbdem_ctx {
         for (auto& w : walls) {
             const V3 wp = w.p / unit.Length;
             for (int64_t it = 0; it < N; ++it)
-                if (getPosition(it).insidePlane(wp, w.n)) { solid[it] = uint16_t(w.index); t[it] = 7; }
+                if (getPosition(it).insidePlane(wp, w.n)) { solid[it] = uint16_t(w.index); t[it] = uint8_t(wallType); }
         }
         // initializeObjectBoundaries (LB.cpp:751-795)
         objectNodes.clear();
@@ -441,7 +447,9 @@ struct lbdem_ctx {
         for (int64_t it = 0; it < N; ++it) {
             if (isWall(it)) {
                 createNode(it);
-                if (t[it] == 7) nodeInitialize(it, initDensity, V3(), 0.0, 0.0, V3());
+                // static and dynamic wall nodes alike: velocity zero (the dynamic one's solidVelocity
+                // is never set, LB.cpp:1162-1169)
+                if (t[it] == 7 || t[it] == 8) nodeInitialize(it, initDensity, V3(), 0.0, 0.0, V3());
                 wallNodes.push_back(uint32_t(it));
             }
         }
@@ -706,7 +714,7 @@ struct lbdem_ctx {
                     const double usq = uu.norm2(), vuj = uu.dot(vdir(j));
                     fi[OPP[j]] = -fsi[j] + coeff(j) * initDensity * (2.0 + 9.0 * (vuj * vuj) - 3.0 * usq);
                 } else if (tl == 7) fi[OPP[j]] = fsi[j];
-                else if (tl == 12) {
+                else if (tl == 8 || tl == 12) {  // moving wall or object (LB.cpp:1403-1421, 1435-1453)
                     const V3 vel = U(l);
                     const double BBi = 6.0 * n[it] * coeff(j) * vel.dot(vdir(j));
                     fi[OPP[j]] = fsi[j] - BBi;
@@ -726,6 +734,13 @@ struct lbdem_ctx {
                     const V3 BBforce = (2.0 * (fsi[j] - staticPres[j]) - 0.0) * vdir(j);
                     const unsigned si = solid[l];
                     walls[si < walls.size() ? si : 0].FHydro += BBforce;
+                } else if (t[l] == 8) {  // moving wall (LB.cpp:1403-1421): the wall node's velocity
+                    const unsigned si = solid[l];
+                    const V3 vel = U(l);
+                    const double BBi = 6.0 * n[it] * coeff(j) * vel.dot(vdir(j));
+                    const V3 BBforce = (2.0 * (fsi[j] - staticPres[j]) - BBi) * vdir(j);
+                    walls[si < walls.size() ? si : 0].FHydro += BBforce;
+                    extraMass += BBi * mass[it];
                 } else if (t[l] == 12) {
                     const unsigned si = solid[l];
                     const V3 vel = U(l);

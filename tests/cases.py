@@ -132,6 +132,17 @@ def _case_values(name: str) -> Tuple[Dict[str, object], np.ndarray, np.ndarray, 
         pc = np.array([[8.05, 11.25, 8.05], [8.05, 6.95, 8.05], [3.4, 2.5, 3.5], [12.6, 5.0, 12.2],
                        [12.3, 12.0, 3.8], [4.0, 9.0, 12.0]])
         return v, pr, pc, np.array([3.0]), np.array([[8.0, 16.55, 8.0]]), np.array([[0.0, -0.06, 0.0]])
+    if name == "dyn_walls":
+        # every face a moving wall (boundary 8): wall nodes of type 8 with zero velocity
+        v, pr, pc, orr, oc, ov = _case_values("suspension")
+        v.update({f"boundary{i}": 8 for i in range(6)})
+        return v, pr, pc, orr, oc, ov
+    if name == "wall_codes":
+        # x and z periodic with a slip floor (5) and a moving lid (8): with boundary0 = 4 both walls
+        # are static no-slip walls (type 7), as every wall takes boundary0's kind (DEM.cpp:430-446)
+        v, pr, pc, orr, oc, ov = _case_values("periodic")
+        v.update(boundary2=5, boundary3=8)
+        return v, pr, pc, orr, oc, ov
     if name in BODY_CASES:
         v, pr, pc, orr, oc, ov = _case_values("suspension")
         v.update(BODY_CASES[name][0])

@@ -39,7 +39,7 @@ def test_create_rejects_bad_configuration(oracle_lib):
     p.unitLength = p.unitTime = p.unitDensity = 1.0
     p.lbSize[:] = [8.0, 8.0, 8.0]
     p.initVisc = 0.1
-    p.boundary[:] = [7, 7, 7, 7, 7, 5]  # slip wall: not supported
+    p.boundary[:] = [5, 7, 7, 7, 7, 7]  # boundary0 5: every wall slips, not supported
     p.lbSolve = 1
     p.density = p.objectDensity = 1.0
     ctx = ctypes.c_void_p()

@@ -164,6 +164,15 @@ def test_bodies_match_oracle(oracle_lib, gpu_lib, case_root, name, prec):
     run_pair(oracle_lib, gpu_lib, case_root, name, prec, 60, every=10)
 
 
+@pytest.mark.parametrize("prec", [lbdem.PREC_FP64, lbdem.PREC_FP32])
+@pytest.mark.parametrize("name", ["dyn_walls", "wall_codes"])
+def test_wall_codes_match_oracle(oracle_lib, gpu_lib, case_root, name, prec):
+    """Boundary codes 5-8 (every wall of boundary0's kind, DEM.cpp:430-446): all faces moving walls
+    (type 8, zero velocity), and a slip floor with a moving lid under a periodic boundary0 (both
+    static walls); 40 steps, every 10."""
+    run_pair(oracle_lib, gpu_lib, case_root, name, prec, 40, every=10)
+
+
 def test_restart_viscosities_match_oracle(oracle_lib, gpu_lib, case_root):
     """A Newtonian restart whose records carry viscosities other than the run's: every node relaxes
     with its own (node.cpp:40-50, 181-207), new interface nodes take their donor's (LB.cpp:1719) and
@@ -204,7 +213,7 @@ GOLD = os.path.join(os.path.dirname(os.path.abspath(__file__)), "golden")
 
 @pytest.mark.parametrize("name", ["suspension", "fs_impactor", "periodic", "units", "moving", "startup", "two_impactors",
                                   "cells", "thin_z2", "thin_z1", "thin_z0", "restart", "bingham", "frictional",
-                                  "voellmy", "body_wca", "body_cubic"])
+                                  "voellmy", "body_wca", "body_cubic", "dyn_walls", "wall_codes"])
 def test_fp64_matches_reference_fixtures(gpu_lib, case_root, name):
     with open(os.path.join(GOLD, "manifest.json")) as fh:
         steps = sorted(m["step"] for m in json.load(fh).values() if m["case"] == name)
