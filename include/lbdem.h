@@ -59,8 +59,9 @@ typedef struct lbdem_params {
     double initVelocity[3];      /* physical */
     double lbF[3];               /* physical acceleration */
     int boundary[6];             /* boundary0..5: 4 periodic, 5-8 wall; every wall is of boundary0's
-                                    kind (DEM.cpp:430-446): 7 static or (boundary0 8) moving with
-                                    zero velocity (type 8); boundary0 5 or 6 (slip) is rejected */
+                                    kind (DEM.cpp:430-446): 7 static, 8 moving with zero velocity,
+                                    5 static slip (slipCoefficient); boundary0 6 is rejected */
+    double slipCoefficient;      /* LB.cpp:313 */
     double fluidHeight;          /* physical */
     double initVisc;             /* physical dynamic viscosity (NEWTONIAN, FRICTIONAL, VOELLMY; the
                                     reference keeps its default (1.0 - 0.5) / 3.0 for BINGHAM) */

@@ -62,6 +62,8 @@ struct lbdem_ctx {
     double plasticVisc = 0, yieldStress = 0, voellmyCoef = 0;
     bool viscUpdate = false, viscPlane = false;
     bool dynWalls = false;  // wall nodes are reported as moving walls (type 8)
+    bool slipWalls = false; // every wall slips (type 5): wall links get the slip pass
+    std::vector<int32_t> slipNodes;  // nodes with a diagonal link into a wall (init)
     bool freeSurface = false, forceField = false, lbFReset = false;
     // inputs
     std::vector<int> psize;
@@ -206,9 +208,11 @@ static int read_params(lbdem_ctx* c) {
     // every wall takes boundary0's kind (DEM.cpp:430-446 test externalBoundary[0] for all walls):
     // slip walls are not supported; a moving (8) wall keeps the zero velocity its nodes are given
     // (LB.cpp:1162-1169), so it reflects as a static wall and only its type code differs
-    if (P.boundary[0] == 5 || P.boundary[0] == 6)
-        return c->fail(LBDEM_E_ARG, "slip walls (boundary0 5 or 6) are not supported");
+    if (P.boundary[0] == 6)
+        return c->fail(LBDEM_E_ARG, "moving slip walls (boundary0 6) are not supported");
     c->dynWalls = P.boundary[0] == 8;
+    c->slipWalls = P.boundary[0] == 5;
+    if (c->slipWalls && !(P.slipCoefficient >= 0.0)) return c->fail(LBDEM_E_ARG, "slipCoefficient must be >= 0");
     c->fluidHeight = P.fluidHeight;
     c->initDynVisc = P.initVisc / u.DynVisc;
     if (P.rheologyModel < LBDEM_RHEO_NEWTONIAN || P.rheologyModel > LBDEM_RHEO_VOELLMY)
@@ -395,6 +399,7 @@ struct Plan {
     size_t selt = 8, celt = 8;
     int ifCap = 0, objCap = 0;
     bool visc = false;  // per-node viscosity plane
+    int nSlip = 0;      // slip-pass nodes
 };
 
 static int wall_count(const lbdem_ctx* c) {
@@ -438,6 +443,13 @@ static int make_plan(lbdem_ctx* c, Plan& p) {
     }
     p.objCap = int(std::min<int64_t>(std::max<int64_t>(objCells, 1024), g.N));
     p.visc = c->viscPlane;
+    if (c->slipWalls) {  // at most the two node layers next to each wall face
+        const int64_t area[3] = {int64_t(g.Y) * g.Z, int64_t(g.X) * g.Z, int64_t(g.X) * g.Y};
+        int64_t cnt = 0;
+        for (int i = 0; i < 6; ++i)
+            if (c->prm.boundary[i] >= 5 && c->prm.boundary[i] <= 8) cnt += 2 * area[i / 2];
+        p.nSlip = int(std::min<int64_t>(std::max<int64_t>(cnt, 1), g.N));
+    }
     return 0;
 }
 
@@ -484,6 +496,9 @@ static bool plan_buffers(const Plan& p, Box& b, DemState& d, DevScalars*& sc, do
     A(b.bcm, N);
     if (p.visc) A(b.visc, N * p.celt);
     else b.visc = nullptr;
+    if (p.nSlip > 0) A(b.slipList, size_t(p.nSlip));
+    else b.slipList = nullptr;
+    b.nSlip = p.nSlip;
     A(b.hacc, np1 * 8);
     A(sc, 1);
     // DEM
@@ -686,6 +701,21 @@ extern "C" int lbdem_init(lbdem_ctx* c) {
         return c->fail(LBDEM_E_ARG, "the restart file's lattice size differs from the box's");
     if (!host_types(c, g, t, s, walls, objList))
         return c->fail(LBDEM_E_ARG, "the restart file puts a type other than fluid, gas, interface or impactor at a fluid node");
+    // slip walls: every node that is neither wall nor periodic layer with a diagonal link into a
+    // wall (walls do not change; the slip pass checks activity every step)
+    c->slipNodes.clear();
+    if (c->slipWalls) {
+        for (int z = 0; z < g.Z; ++z)
+            for (int y = 0; y < g.Y; ++y)
+                for (int x = 0; x < g.X; ++x) {
+                    const int64_t i = cidx(g, x, y, z);
+                    const int base = t[i] & T_BASE;
+                    if (base == T_WALL || base == T_PER) continue;
+                    for (int j = 7; j < Q; ++j)
+                        if ((t[nbr_wrapped(g, x, y, z, j)] & T_BASE) == T_WALL) { c->slipNodes.push_back(int32_t(i)); break; }
+                }
+        if (int64_t(c->slipNodes.size()) > plan.nSlip) return c->fail(LBDEM_E_STATE, "slip node list exceeds its plan");
+    }
     // restartVariables (LB.cpp:1098-1138): one record per active node, in node index order
     std::vector<int32_t> rNode;
     if (c->restart) {
@@ -774,6 +804,9 @@ extern "C" int lbdem_init(lbdem_ctx* c) {
         HIPCHK(hipMemcpyAsync(d.oFGrav, ofg.data(), 3 * no * 8, hipMemcpyHostToDevice, st));
     }
     HIPCHK(hipMemcpyAsync(d.body, brec.data(), brec.size() * 8, hipMemcpyHostToDevice, st));
+    b.nSlip = int(c->slipNodes.size());
+    if (b.nSlip > 0)
+        HIPCHK(hipMemcpyAsync(b.slipList, c->slipNodes.data(), size_t(b.nSlip) * 4, hipMemcpyHostToDevice, st));
     HIPCHK(hipMemcpyAsync(c->sc, &hs, sizeof(hs), hipMemcpyHostToDevice, st));
 
     // constants of the lattice kernels
@@ -791,6 +824,8 @@ extern "C" int lbdem_init(lbdem_ctx* c) {
     k.frictionCoef = c->prm.frictionCoefFluid;
     k.voellmyCoef = c->voellmyCoef;
     k.turbConst = c->prm.turbConst;
+    k.slip = c->slipWalls ? 1 : 0;
+    k.slipCoef = c->prm.slipCoefficient;
     k.minVisc = (0.500001 - 0.5) / 3.0;  // lattice.h:26-27 (minTau, maxTau), node.cpp:116-117
     k.maxVisc = (1.8 - 0.5) / 3.0;
     k.freeSurface = c->freeSurface ? 1 : 0;
@@ -1231,7 +1266,7 @@ extern "C" int lbdem_get_types(lbdem_ctx* c, uint8_t* type, uint8_t* pflag, int3
     HIPCHK(hipMemcpyAsync(t.data(), c->box.type, N, hipMemcpyDeviceToHost, c->stream));
     if (solid) HIPCHK(hipMemcpyAsync(solid, c->box.solid, N * 4, hipMemcpyDeviceToHost, c->stream));
     HIPCHK(hipStreamSynchronize(c->stream));
-    const uint8_t wallCode = c->dynWalls ? 8 : T_WALL;
+    const uint8_t wallCode = c->dynWalls ? 8 : (c->slipWalls ? 5 : T_WALL);
     for (int64_t i = 0; i < N; ++i) {
         if (type) type[i] = (t[i] & T_BASE) == T_WALL ? wallCode : uint8_t(t[i] & T_BASE);
         if (pflag) pflag[i] = (t[i] & T_PMASK) ? 1 : 0;

@@ -285,7 +285,7 @@ __device__ inline typename Types<S>::C boundary_value(const FusedArgs<S>& a, int
         return -gj + C(w_rt(j)) * (C(9) * (cu * cu) - C(3) * usq);
     }
     if (ty == T_WALL) {
-        if (DIAG && k.nWall > 0) {
+        if (DIAG && k.nWall > 0 && !k.slip) {
             int wi = a.solid[yj];
             wi = (wi >= 0 && wi < k.nWall) ? wi : 0;
             const double v2 = 2.0 * double(gj);
@@ -710,7 +710,7 @@ __device__ inline void k_fused(FusedArgs<S> a) {
                     const C cu = C(cx_rt(j)) * r.sx + C(cy_rt(j)) * r.sy + C(cz_rt(j)) * r.sz;
                     bcv = -gj + C(w_rt(j)) * (C(9) * (cu * cu) - C(3) * r.usq);
                 } else if (cls == BC_WALL) {
-                    if (DIAG && a.k.nWall > 0) {
+                    if (DIAG && a.k.nWall > 0 && !a.k.slip) {  // slip walls take no force (LB.cpp:1454-1482)
                         const double v2 = 2.0 * double(gj);
                         double* wp = &sPart[PART_WALL + 3 * wprim];
                         if (cx_rt(j)) atomicAdd(wp + 0, v2 * cx_rt(j));
@@ -916,6 +916,76 @@ int fused_grid_blocks(const Geom& g, int fz, int prec) {
     return ((nseg * nby * nbz + 7) / 8) * 8;                              // XCD-spread tiles
 }
 
+// Slip walls (LB.cpp:1454-1482, slip static walls): a diagonal link j (j > 6) of an active node A
+// into a wall, with exactly one of its two check nodes A + c(slip1Check[j]), A + c(slip2Check[j])
+// active, receives S1 times that active check node B's post-collision population k = slip1[j]
+// (or slip2[j]) plus S2 = 1 - S1 times A's own post-collision population j; every other wall link
+// bounces back, as the fused kernel has already written. Runs right after the fused kernel of
+// parity PAR, over the nodes next to a wall: it reads post-collision values where that kernel
+// stored them (PAR 0: slot opp(k) of the node itself; PAR 1: slot k of the node's neighbour in
+// direction k, here always the inactive check node or the wall) and overwrites the slot A loads
+// next step (PAR 0: slot j of the wall node; PAR 1: A's own slot opp(j)), none of which is read here.
+__host__ __device__ constexpr int slip1Check(int j) {
+    constexpr int c[Q] = {0, 0, 0, 0, 0, 0, 0, 1, 2, 3, 4, 3, 4, 5, 6, 1, 2, 6, 5};
+    return c[j];
+}
+__host__ __device__ constexpr int slip1Dir(int j) {
+    constexpr int c[Q] = {0, 0, 0, 0, 0, 0, 0, 9, 10, 8, 7, 13, 14, 12, 11, 18, 17, 15, 16};
+    return c[j];
+}
+__host__ __device__ constexpr int slip2Check(int j) {
+    constexpr int c[Q] = {0, 0, 0, 0, 0, 0, 0, 3, 4, 2, 1, 5, 6, 4, 3, 5, 6, 1, 2};
+    return c[j];
+}
+__host__ __device__ constexpr int slip2Dir(int j) {
+    constexpr int c[Q] = {0, 0, 0, 0, 0, 0, 0, 10, 9, 7, 8, 14, 13, 11, 12, 17, 18, 16, 15};
+    return c[j];
+}
+template <typename S, int PAR>
+__device__ inline void k_slip(Geom g, S* f, const uint8_t* type, const int32_t* list, int n, double s1) {
+    using C = typename StoreT<S>::C;
+    const int i = int(blockIdx.x * blockDim.x + threadIdx.x);
+    if (i >= n) return;
+    const int32_t a = list[i];
+    if (!is_active(type[a])) return;
+    int x, y, z;
+    cell_xyz(g, a, x, y, z);
+    const Offs oa(g, x, y, z);
+    const int64_t N = g.N;
+#pragma unroll
+    for (int j = 7; j < Q; ++j) {
+        const int32_t w = a + oa.d(j);
+        if ((type[w] & T_BASE) != T_WALL) continue;
+        const bool act1 = is_active(type[a + oa.d(slip1Check(j))]);
+        const bool act2 = is_active(type[a + oa.d(slip2Check(j))]);
+        if (act1 == act2) continue;  // bounce-back, written by the fused kernel
+        const int32_t b = a + oa.d(act1 ? slip1Check(j) : slip2Check(j));
+        const int kk = act1 ? slip1Dir(j) : slip2Dir(j);
+        C fb, fa;
+        if (PAR == 0) {
+            fb = StoreT<S>::ld(f + int64_t(oppq(kk)) * N + b);
+            fa = StoreT<S>::ld(f + int64_t(oppq(j)) * N + a);
+        } else {
+            int bx, by, bz;
+            cell_xyz(g, b, bx, by, bz);
+            const Offs ob(g, bx, by, bz);
+            fb = StoreT<S>::ld(f + int64_t(kk) * N + b + ob.d(kk));
+            fa = StoreT<S>::ld(f + int64_t(j) * N + w);
+        }
+        // offset storage: S1 (fb + w) + S2 (fa + w) - w = S1 fb + S2 fa (the weights of j and kk agree)
+        const C v = C(s1) * fb + (C(1) - C(s1)) * fa;
+        if (PAR == 0) StoreT<S>::st(f + int64_t(j) * N + w, v);
+        else StoreT<S>::st(f + int64_t(oppq(j)) * N + a, v);
+    }
+}
+
+template <typename S>
+static void launch_slip(const Box& b, double s1, hipStream_t s) {
+    const int bs = 256, nb = (b.nSlip + bs - 1) / bs;
+    if (b.parity == 0) launch<&k_slip<S, 0>, 256, 1>(nb, bs, s, b.g, (S*)b.f, b.type, b.slipList, b.nSlip, s1);
+    else launch<&k_slip<S, 1>, 256, 1>(nb, bs, s, b.g, (S*)b.f, b.type, b.slipList, b.nSlip, s1);
+}
+
 template <typename S>
 static void launch_fused(Box& b, const Consts& k, const PartLB* plb, const ObjLB* olb, DevScalars* sc, bool diag,
                          bool storeVel, hipStream_t s, hipEvent_t ev0, hipEvent_t ev1) {
@@ -956,6 +1026,7 @@ static void launch_fused(Box& b, const Consts& k, const PartLB* plb, const ObjLB
 #undef LBDEM_FUSED_Z
 #undef LBDEM_FUSED
     if (ev1) (void)hipEventRecord(ev1, s);
+    if (b.slipList && b.nSlip > 0) launch_slip<S>(b, k.slipCoef, s);
     b.parity ^= 1;
 }
 

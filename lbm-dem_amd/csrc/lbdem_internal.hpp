@@ -178,6 +178,8 @@ struct Box {
     double* blockPart;         // per-block partials of the fused kernel (object F/M, extra mass, walls)
     int32_t* blockFlag;        // per block: partials written
     uint64_t* bcm;             // N, boundary-link classes of edge nodes (valid where T_EDGE is set)
+    int32_t* slipList;         // nodes with a diagonal link into a wall, when the walls slip (else null)
+    int nSlip;
     void* visc;                // N compute-type node viscosities (lattice units), or null: every node
                                // has the run's viscosity (Newtonian, no turbulence, no restart values)
     int blockPartCap;
@@ -195,6 +197,8 @@ struct Consts {
     // apparent viscosity (node.cpp:113-168; with per-node viscosities only): rheology model
     // (LBDEM_RHEO_*), its parameters in lattice units, the Smagorinsky constant, the clamp range
     int rheo, turb, viscUpdate;
+    int slip;                   // walls slip (boundary0 = 5): no wall forces; the slip pass follows
+    double slipCoef;            // slipCoefficient (LB.cpp:313, 1344-1345)
     double plasticVisc, yieldStress, frictionCoef, voellmyCoef, turbConst, minVisc, maxVisc;
     int freeSurface;
     int nPart, nObj, nWall;

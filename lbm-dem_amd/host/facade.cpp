@@ -101,6 +101,7 @@ void LB::latticeBoltzmannGet(ConfigFile& cfg, CommandLine& cl) {
     for (int a = 0; a < 3; ++a) p.initVelocity[a] = parse_value(cfg, cl, std::string("initVelocity") + ax[a], 0.0);
     for (int a = 0; a < 3; ++a) p.lbF[a] = parse_value(cfg, cl, std::string("lbF") + ax[a], 0.0);
     for (int b = 0; b < 6; ++b) p.boundary[b] = parse_value(cfg, cl, "boundary" + std::to_string(b), 1);
+    p.slipCoefficient = parse_value(cfg, cl, "slipCoefficient", 0.0);
     // rheology (LB.cpp:197-245): an unknown name keeps NEWTONIAN; BINGHAM reads no initVisc and
     // keeps the default (1.0 - 0.5) / 3.0, scaled at LB.cpp:320 like a physical viscosity
     const std::string rheology = parse_value(cfg, cl, "rheologyModel", "none");

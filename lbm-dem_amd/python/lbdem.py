@@ -46,7 +46,8 @@ class Params(C.Structure):
         ("demSolve", C.c_int), ("lbSolve", C.c_int), ("freeSurface", C.c_int), ("forceField", C.c_int),
         ("unitLength", C.c_double), ("unitTime", C.c_double), ("unitDensity", C.c_double),
         ("lbSize", C.c_double * 3), ("initVelocity", C.c_double * 3), ("lbF", C.c_double * 3),
-        ("boundary", C.c_int * 6), ("fluidHeight", C.c_double), ("initVisc", C.c_double),
+        ("boundary", C.c_int * 6), ("slipCoefficient", C.c_double), ("fluidHeight", C.c_double),
+        ("initVisc", C.c_double),
         ("rheologyModel", C.c_int), ("plasticVisc", C.c_double), ("yieldStress", C.c_double),
         ("frictionCoefFluid", C.c_double), ("voellmyConstant", C.c_double), ("turbulenceSolve", C.c_int),
         ("turbConst", C.c_double),
@@ -645,6 +646,7 @@ def params_from_values(v: Dict[str, object], precision: int = PREC_FP64, device:
     p.lbF[:] = [float(v["lbFX"]), float(v["lbFY"]), float(v["lbFZ"])]
     p.boundary[:] = [int(v[f"boundary{i}"]) for i in range(6)]
     p.fluidHeight = float(v["fluidHeight"])
+    p.slipCoefficient = float(v["slipCoefficient"])
     for k in ("density", "objectDensity", "bodyDensity", "linearStiff", "frictionCoefPart", "lubNormal",
               "lubTangential", "lubRotation", "lubCutoff", "kinVisc", "electroForce", "debyeLength"):
         setattr(p, k, float(v[k]))

@@ -104,7 +104,8 @@ struct lbdem_ctx {
     double lbPhysicalSize[3];
     V3 lbBoundaryLocation[6];
     int boundary[6];
-    int wallType = 7;  // node type of every wall: 7 static or 8 dynamic (moving, with zero velocity)
+    int wallType = 7;  // node type of every wall: 7 static, 8 dynamic (zero velocity) or 5 static slip
+    double slipCoefficient = 0.0;
     double fluidHeight = 0;
     double initDynVisc = 0, initDensity = 1.0;
     // FluidMaterial (node.h:17-49): rheology model and its parameters in lattice units
@@ -282,8 +283,9 @@ struct lbdem_ctx {
         demInitialRepeat = P.demInitialRepeat;
         // every wall takes boundary0's kind (DEM.cpp:430-446 test externalBoundary[0]); slip walls
         // (5, 6) are not restated
-        if (boundary[0] == 5 || boundary[0] == 6) return fail(LBDEM_E_ARG, "slip walls (boundary0 5 or 6) are not supported");
-        wallType = boundary[0] == 8 ? 8 : 7;
+        if (boundary[0] == 6) return fail(LBDEM_E_ARG, "moving slip walls (boundary0 6) are not supported");
+        wallType = boundary[0] == 8 ? 8 : (boundary[0] == 5 ? 5 : 7);
+        slipCoefficient = P.slipCoefficient;
         if (!P.lbSolve) return fail(LBDEM_E_ARG, "lbSolve=0 is not supported");
         if (!(P.density > 0) || !(P.objectDensity > 0)) return fail(LBDEM_E_ARG, "density/objectDensity must be > 0");
         return 0;
@@ -449,7 +451,7 @@ struct lbdem_ctx {
                 createNode(it);
                 // static and dynamic wall nodes alike: velocity zero (the dynamic one's solidVelocity
                 // is never set, LB.cpp:1162-1169)
-                if (t[it] == 7 || t[it] == 8) nodeInitialize(it, initDensity, V3(), 0.0, 0.0, V3());
+                if (t[it] == 5 || t[it] == 7 || t[it] == 8) nodeInitialize(it, initDensity, V3(), 0.0, 0.0, V3());
                 wallNodes.push_back(uint32_t(it));
             }
         }
@@ -714,7 +716,18 @@ struct lbdem_ctx {
                     const double usq = uu.norm2(), vuj = uu.dot(vdir(j));
                     fi[OPP[j]] = -fsi[j] + coeff(j) * initDensity * (2.0 + 9.0 * (vuj * vuj) - 3.0 * usq);
                 } else if (tl == 7) fi[OPP[j]] = fsi[j];
-                else if (tl == 8 || tl == 12) {  // moving wall or object (LB.cpp:1403-1421, 1435-1453)
+                else if (tl == 5) {  // static slip wall (LB.cpp:1454-1482)
+                    const double S1 = slipCoefficient, S2 = (1.0 - slipCoefficient);
+                    if (j > 6) {
+                        const uint32_t c1 = d[size_t(it) * Q + SLIP1_CHECK[j]], c2 = d[size_t(it) * Q + SLIP2_CHECK[j]];
+                        const bool a1 = isActive(c1), a2 = isActive(c2);
+                        if (a1 && !a2) fi[OPP[j]] = S1 * FS(c1)[SLIP1[j]] + S2 * fsi[j];
+                        else if (!a1 && a2) fi[OPP[j]] = S1 * FS(c2)[SLIP2[j]] + S2 * fsi[j];
+                        else fi[OPP[j]] = fsi[j];
+                    } else {
+                        fi[OPP[j]] = fsi[j];
+                    }
+                } else if (tl == 8 || tl == 12) {  // moving wall or object (LB.cpp:1403-1421, 1435-1453)
                     const V3 vel = U(l);
                     const double BBi = 6.0 * n[it] * coeff(j) * vel.dot(vdir(j));
                     fi[OPP[j]] = fsi[j] - BBi;

@@ -41,6 +41,11 @@ constexpr int CX[Q] = {0, 1, -1, 0, 0, 0, 0, 1, -1, -1, 1, 0, 0, 0, 0, 1, -1, 1,
 constexpr int CY[Q] = {0, 0, 0, 1, -1, 0, 0, 1, -1, 1, -1, 1, -1, -1, 1, 0, 0, 0, 0};
 constexpr int CZ[Q] = {0, 0, 0, 0, 0, 1, -1, 0, 0, 0, 0, 1, -1, 1, -1, 1, -1, -1, 1};
 constexpr int OPP[Q] = {0, 2, 1, 4, 3, 6, 5, 8, 7, 10, 9, 12, 11, 14, 13, 16, 15, 18, 17};
+// slip directions (lattice.h:137-140)
+constexpr int SLIP1_CHECK[Q] = {0, 0, 0, 0, 0, 0, 0, 1, 2, 3, 4, 3, 4, 5, 6, 1, 2, 6, 5};
+constexpr int SLIP1[Q] = {0, 0, 0, 0, 0, 0, 0, 9, 10, 8, 7, 13, 14, 12, 11, 18, 17, 15, 16};
+constexpr int SLIP2_CHECK[Q] = {0, 0, 0, 0, 0, 0, 0, 3, 4, 2, 1, 5, 6, 4, 3, 5, 6, 1, 2};
+constexpr int SLIP2[Q] = {0, 0, 0, 0, 0, 0, 0, 10, 9, 7, 8, 14, 13, 11, 12, 17, 18, 16, 15};
 inline V3 vdir(int j) { return V3(double(CX[j]), double(CY[j]), double(CZ[j])); }
 inline double coeff(int j) { return j == 0 ? 12.0 / 36.0 : (j < 7 ? 2.0 / 36.0 : 1.0 / 36.0); }
 

@@ -137,6 +137,13 @@ def _case_values(name: str) -> Tuple[Dict[str, object], np.ndarray, np.ndarray, 
         v, pr, pc, orr, oc, ov = _case_values("suspension")
         v.update({f"boundary{i}": 8 for i in range(6)})
         return v, pr, pc, orr, oc, ov
+    if name == "slip_walls":
+        # slip walls (boundary0 5: every wall slips, LB.cpp:1454-1482) on x and y, z periodic, a
+        # force along z driving flow past them
+        v, pr, pc, orr, oc, ov = _case_values("suspension")
+        v.update(boundary0=5, boundary1=5, boundary2=5, boundary3=5, boundary4=4, boundary5=4, slipCoefficient=0.6,
+                 lbFZ=1.0e-4)
+        return v, pr, pc, orr, oc, ov
     if name == "wall_codes":
         # x and z periodic with a slip floor (5) and a moving lid (8): with boundary0 = 4 both walls
         # are static no-slip walls (type 7), as every wall takes boundary0's kind (DEM.cpp:430-446)

@@ -111,3 +111,16 @@ def test_rheology_fixtures_cover_the_viscosity_range(oracle_lib, case_root):
         else:
             assert (v >= top).any() and (v < 0.99 * top).sum() > 10, name
         sim.close()
+
+
+def test_slip_fixture_pins_the_slip_term(oracle_lib, case_root):
+    """The slip-wall fixture depends on the slip term: the same case with slipCoefficient 0 (every
+    wall link bounces back) ends 40 steps later with populations well apart from the fixture's."""
+    g = np.load(os.path.join(GOLD, "slip_walls_s40.npz"), allow_pickle=False)
+    d = cases.make_case(case_root, "slip_walls", overrides={"slipCoefficient": 0.0}, subdir="slip_zero")
+    sim = cases.open_sim(oracle_lib, d)
+    sim.cycle(40)
+    st = cases.state(sim)
+    act = g["active"]
+    assert np.abs(st["f"][:, act] - g["f_active"]).max() > 1e-6
+    sim.close()
