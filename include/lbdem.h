@@ -60,7 +60,7 @@ typedef struct lbdem_params {
     double lbF[3];               /* physical acceleration */
     int boundary[6];             /* boundary0..5: 4 periodic, 5-8 wall; every wall is of boundary0's
                                     kind (DEM.cpp:430-446): 7 static, 8 moving with zero velocity,
-                                    5 static slip (slipCoefficient); boundary0 6 is rejected */
+                                    5 static slip, 6 moving slip (slipCoefficient) */
     double slipCoefficient;      /* LB.cpp:313 */
     double fluidHeight;          /* physical */
     double initVisc;             /* physical dynamic viscosity (NEWTONIAN, FRICTIONAL, VOELLMY; the

@@ -62,7 +62,8 @@ struct lbdem_ctx {
     double plasticVisc = 0, yieldStress = 0, voellmyCoef = 0;
     bool viscUpdate = false, viscPlane = false;
     bool dynWalls = false;  // wall nodes are reported as moving walls (type 8)
-    bool slipWalls = false; // every wall slips (type 5): wall links get the slip pass
+    bool slipWalls = false; // every wall slips (type 5 or 6): wall links get the slip pass
+    bool slipDyn = false;   // moving slip walls (6): zero velocity, but in the free-surface mass exchange
     std::vector<int32_t> slipNodes;  // nodes with a diagonal link into a wall (init)
     bool freeSurface = false, forceField = false, lbFReset = false;
     // inputs
@@ -208,10 +209,9 @@ static int read_params(lbdem_ctx* c) {
     // every wall takes boundary0's kind (DEM.cpp:430-446 test externalBoundary[0] for all walls):
     // slip walls are not supported; a moving (8) wall keeps the zero velocity its nodes are given
     // (LB.cpp:1162-1169), so it reflects as a static wall and only its type code differs
-    if (P.boundary[0] == 6)
-        return c->fail(LBDEM_E_ARG, "moving slip walls (boundary0 6) are not supported");
     c->dynWalls = P.boundary[0] == 8;
-    c->slipWalls = P.boundary[0] == 5;
+    c->slipWalls = P.boundary[0] == 5 || P.boundary[0] == 6;
+    c->slipDyn = P.boundary[0] == 6;
     if (c->slipWalls && !(P.slipCoefficient >= 0.0)) return c->fail(LBDEM_E_ARG, "slipCoefficient must be >= 0");
     c->fluidHeight = P.fluidHeight;
     c->initDynVisc = P.initVisc / u.DynVisc;
@@ -824,7 +824,7 @@ extern "C" int lbdem_init(lbdem_ctx* c) {
     k.frictionCoef = c->prm.frictionCoefFluid;
     k.voellmyCoef = c->voellmyCoef;
     k.turbConst = c->prm.turbConst;
-    k.slip = c->slipWalls ? 1 : 0;
+    k.slip = c->slipDyn ? 2 : (c->slipWalls ? 1 : 0);
     k.slipCoef = c->prm.slipCoefficient;
     k.minVisc = (0.500001 - 0.5) / 3.0;  // lattice.h:26-27 (minTau, maxTau), node.cpp:116-117
     k.maxVisc = (1.8 - 0.5) / 3.0;
@@ -1266,7 +1266,7 @@ extern "C" int lbdem_get_types(lbdem_ctx* c, uint8_t* type, uint8_t* pflag, int3
     HIPCHK(hipMemcpyAsync(t.data(), c->box.type, N, hipMemcpyDeviceToHost, c->stream));
     if (solid) HIPCHK(hipMemcpyAsync(solid, c->box.solid, N * 4, hipMemcpyDeviceToHost, c->stream));
     HIPCHK(hipStreamSynchronize(c->stream));
-    const uint8_t wallCode = c->dynWalls ? 8 : (c->slipWalls ? 5 : T_WALL);
+    const uint8_t wallCode = c->dynWalls ? 8 : (c->slipDyn ? 6 : (c->slipWalls ? 5 : T_WALL));
     for (int64_t i = 0; i < N; ++i) {
         if (type) type[i] = (t[i] & T_BASE) == T_WALL ? wallCode : uint8_t(t[i] & T_BASE);
         if (pflag) pflag[i] = (t[i] & T_PMASK) ? 1 : 0;

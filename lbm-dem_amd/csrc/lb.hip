@@ -1129,7 +1129,7 @@ constexpr int LIST_GRID = 256;  // blocks of the free-surface list passes (few i
 template <typename S, int L>
 __device__ inline void k_fs_mass(Geom g, const S* f, const uint8_t* type, const typename Types<S>::M* mass,
                           const typename Types<S>::C* aux, const int32_t* ifList, typename Types<S>::M* newMass,
-                          DevScalars* sc, int32_t* filledFlag) {
+                          DevScalars* sc, int32_t* filledFlag, int slipDyn, double s1) {
     using C = typename Types<S>::C;
     using M = typename Types<S>::M;
     const int nIf = sc->nIf;
@@ -1155,11 +1155,21 @@ __device__ inline void k_fs_mass(Geom g, const S* f, const uint8_t* type, const 
         C dm = 0;
 #pragma unroll
         for (int j = 1; j < Q; ++j) {
-            if (ty[j] != T_IF && ty[j] != T_FLUID && ty[j] != T_OBJ) continue;
+            const bool slipLink = slipDyn && ty[j] == T_WALL;
+            if (ty[j] != T_IF && ty[j] != T_FLUID && ty[j] != T_OBJ && !slipLink) continue;
             const C ms = L == 1 ? (a1[j] - a2[j]) : (a2[j] - a1[j]);  // incoming(opp j) - outgoing(j), offsets
             if (ty[j] == T_IF) dm += C(0.5) * (C(double(mj[j]) + addE) + mi) * ms;
             else if (ty[j] == T_FLUID) dm += ms;
-            else dm += mi * ms;
+            else if (slipLink) {  // moving slip wall (LB.cpp:1595-1625): (1 - S) of the mass on a slip link
+                bool slipped = false;
+                if (j > 6) {
+                    const uint8_t t1 = ty[slip1Check(j)], t2 = ty[slip2Check(j)];
+                    slipped = (t1 == T_FLUID || t1 == T_IF) != (t2 == T_FLUID || t2 == T_IF);
+                }
+                dm += (slipped ? C(1.0 - s1) * mi : mi) * ms;
+            } else {
+                dm += mi * ms;
+            }
         }
         newMass[k] = M(mi + dm);
     }
@@ -1437,9 +1447,11 @@ static void fs_impl(Box& b, const Consts& k, DevScalars* sc, hipStream_t s) {
     int* count2 = (int*)(b.ifList2 + b.ifCap);  // counter stored after the list
     const int grid = LIST_GRID, bs = 256;
     if (b.parity == 1)
-        launch_gs<&k_fs_mass<S, 1>, 256, 1>(grid, bs, s, b.g, f, b.type, mass, aux, b.ifList, (M*)b.ifNewMass, sc, b.donor);
+        launch_gs<&k_fs_mass<S, 1>, 256, 1>(grid, bs, s, b.g, f, b.type, mass, aux, b.ifList, (M*)b.ifNewMass, sc, b.donor,
+                                            k.slip == 2 ? 1 : 0, k.slipCoef);
     else
-        launch_gs<&k_fs_mass<S, 0>, 256, 1>(grid, bs, s, b.g, f, b.type, mass, aux, b.ifList, (M*)b.ifNewMass, sc, b.donor);
+        launch_gs<&k_fs_mass<S, 0>, 256, 1>(grid, bs, s, b.g, f, b.type, mass, aux, b.ifList, (M*)b.ifNewMass, sc, b.donor,
+                                            k.slip == 2 ? 1 : 0, k.slipCoef);
     launch_gs<&k_fs_mutants<S>, 256, 1>(grid, bs, s, b.type, mass, aux, b.g.N, b.ifList, (M*)b.ifNewMass, b.filled, b.emptied, b.ifCap, sc, b.donor, b.changed, b.changedCap);
     launch_gs<&k_fs_donors, 256, 1>(grid, bs, s, b.g, b.type, b.filled, b.donor, b.newIf, b.ifCap, sc);
     if (b.parity == 1)

@@ -197,7 +197,9 @@ struct Consts {
     // apparent viscosity (node.cpp:113-168; with per-node viscosities only): rheology model
     // (LBDEM_RHEO_*), its parameters in lattice units, the Smagorinsky constant, the clamp range
     int rheo, turb, viscUpdate;
-    int slip;                   // walls slip (boundary0 = 5): no wall forces; the slip pass follows
+    int slip;                   // walls slip (1: boundary0 = 5, 2: boundary0 = 6, moving with zero velocity):
+                                // no wall forces; the slip pass follows the fused kernel; 2 also
+                                // enters the free-surface mass exchange
     double slipCoef;            // slipCoefficient (LB.cpp:313, 1344-1345)
     double plasticVisc, yieldStress, frictionCoef, voellmyCoef, turbConst, minVisc, maxVisc;
     int freeSurface;

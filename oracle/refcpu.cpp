@@ -104,7 +104,7 @@ struct lbdem_ctx {
     double lbPhysicalSize[3];
     V3 lbBoundaryLocation[6];
     int boundary[6];
-    int wallType = 7;  // node type of every wall: 7 static, 8 dynamic (zero velocity) or 5 static slip
+    int wallType = 7;  // node type of every wall: 7 static, 8 moving (zero velocity), 5 static slip, 6 moving slip
     double slipCoefficient = 0.0;
     double fluidHeight = 0;
     double initDynVisc = 0, initDensity = 1.0;
@@ -283,8 +283,7 @@ struct lbdem_ctx {
         demInitialRepeat = P.demInitialRepeat;
         // every wall takes boundary0's kind (DEM.cpp:430-446 test externalBoundary[0]); slip walls
         // (5, 6) are not restated
-        if (boundary[0] == 6) return fail(LBDEM_E_ARG, "moving slip walls (boundary0 6) are not supported");
-        wallType = boundary[0] == 8 ? 8 : (boundary[0] == 5 ? 5 : 7);
+        wallType = (boundary[0] >= 5 && boundary[0] <= 8) ? boundary[0] : 7;
         slipCoefficient = P.slipCoefficient;
         if (!P.lbSolve) return fail(LBDEM_E_ARG, "lbSolve=0 is not supported");
         if (!(P.density > 0) || !(P.objectDensity > 0)) return fail(LBDEM_E_ARG, "density/objectDensity must be > 0");
@@ -451,7 +450,7 @@ struct lbdem_ctx {
                 createNode(it);
                 // static and dynamic wall nodes alike: velocity zero (the dynamic one's solidVelocity
                 // is never set, LB.cpp:1162-1169)
-                if (t[it] == 5 || t[it] == 7 || t[it] == 8) nodeInitialize(it, initDensity, V3(), 0.0, 0.0, V3());
+                if (t[it] >= 5 && t[it] <= 8) nodeInitialize(it, initDensity, V3(), 0.0, 0.0, V3());
                 wallNodes.push_back(uint32_t(it));
             }
         }
@@ -727,6 +726,18 @@ struct lbdem_ctx {
                     } else {
                         fi[OPP[j]] = fsi[j];
                     }
+                } else if (tl == 6) {  // moving slip wall (LB.cpp:1483-1523): the wall node's velocity
+                    const double S1 = slipCoefficient, S2 = (1.0 - slipCoefficient);
+                    const double BBi = 6.0 * n[it] * coeff(j) * U(l).dot(vdir(j));
+                    if (j > 6) {
+                        const uint32_t c1 = d[size_t(it) * Q + SLIP1_CHECK[j]], c2 = d[size_t(it) * Q + SLIP2_CHECK[j]];
+                        const bool a1 = isActive(c1), a2 = isActive(c2);
+                        if (a1 && !a2) fi[OPP[j]] = S1 * FS(c1)[SLIP1[j]] + S2 * (fsi[j] - BBi);
+                        else if (!a1 && a2) fi[OPP[j]] = S1 * FS(c2)[SLIP2[j]] + S2 * (fsi[j] - BBi);
+                        else fi[OPP[j]] = fsi[j] - BBi;
+                    } else {
+                        fi[OPP[j]] = fsi[j] - BBi;
+                    }
                 } else if (tl == 8 || tl == 12) {  // moving wall or object (LB.cpp:1403-1421, 1435-1453)
                     const V3 vel = U(l);
                     const double BBi = 6.0 * n[it] * coeff(j) * vel.dot(vdir(j));
@@ -747,6 +758,17 @@ struct lbdem_ctx {
                     const V3 BBforce = (2.0 * (fsi[j] - staticPres[j]) - 0.0) * vdir(j);
                     const unsigned si = solid[l];
                     walls[si < walls.size() ? si : 0].FHydro += BBforce;
+                } else if (t[l] == 6) {  // moving slip wall: extra mass only (LB.cpp:1483-1523)
+                    const double S2 = (1.0 - slipCoefficient);
+                    const double BBi = 6.0 * n[it] * coeff(j) * U(l).dot(vdir(j));
+                    bool slip = false;
+                    if (j > 6) {
+                        const bool a1 = isActive(d[size_t(it) * Q + SLIP1_CHECK[j]]);
+                        const bool a2 = isActive(d[size_t(it) * Q + SLIP2_CHECK[j]]);
+                        slip = a1 != a2;
+                    }
+                    if (slip) extraMass += S2 * mass[it] * BBi;
+                    else extraMass += mass[it] * BBi;
                 } else if (t[l] == 8) {  // moving wall (LB.cpp:1403-1421): the wall node's velocity
                     const unsigned si = solid[l];
                     const V3 vel = U(l);
@@ -812,6 +834,18 @@ struct lbdem_ctx {
                 if (t[l] == 3) deltaMass += 0.5 * (mass[l] + mass[index]) * ms;
                 else if (t[l] == 0) deltaMass += 1.0 * ms;
                 else if (t[l] == 8 || t[l] == 12) deltaMass += 1.0 * mass[index] * ms;
+                else if (t[l] == 6) {  // moving slip wall (LB.cpp:1595-1625)
+                    double averageMass = 0.0;
+                    if (j > 6) {
+                        const bool a1 = isActive(d[size_t(index) * Q + SLIP1_CHECK[j]]);
+                        const bool a2 = isActive(d[size_t(index) * Q + SLIP2_CHECK[j]]);
+                        if (a1 != a2) averageMass += 1.0 * (1.0 - slipCoefficient) * mass[index];
+                        else averageMass += 1.0 * mass[index];
+                    } else {
+                        averageMass = 1.0 * mass[index];
+                    }
+                    deltaMass += averageMass * ms;
+                }
             }
             newMass[index] += deltaMass;
         }

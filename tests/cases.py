@@ -144,6 +144,12 @@ def _case_values(name: str) -> Tuple[Dict[str, object], np.ndarray, np.ndarray, 
         v.update(boundary0=5, boundary1=5, boundary2=5, boundary3=5, boundary4=4, boundary5=4, slipCoefficient=0.6,
                  lbFZ=1.0e-4)
         return v, pr, pc, orr, oc, ov
+    if name == "slip_dyn_walls":
+        # moving slip walls (boundary0 6): the slip reflection with zero wall velocity, and the
+        # interface nodes along them exchange (1 - slipCoefficient) of their mass on slip links
+        v, pr, pc, orr, oc, ov = _case_values("slip_walls")
+        v.update(boundary0=6, boundary1=6, boundary2=6, boundary3=6)
+        return v, pr, pc, orr, oc, ov
     if name == "wall_codes":
         # x and z periodic with a slip floor (5) and a moving lid (8): with boundary0 = 4 both walls
         # are static no-slip walls (type 7), as every wall takes boundary0's kind (DEM.cpp:430-446)

@@ -42,7 +42,7 @@ PLAN = {"lb_closed": [1, 12], "fs_impactor": [1, 40], "suspension": [1, 40, 120]
         "cells": [1, 26, 60], "thin_z2": [1, 40], "thin_z1": [40], "thin_z0": [20], "restart": [0, 1, 20],
         "bingham": [1, 40], "frictional": [1, 40], "voellmy": [1, 40], "smagorinsky": [1, 12],
         "body_wca": [1, 40], "body_cubic": [1, 40], "dyn_walls": [1, 40], "wall_codes": [1, 40],
-        "slip_walls": [1, 40]}
+        "slip_walls": [1, 40], "slip_dyn_walls": [1, 40]}
 SAMPLE_ABOVE = 8_000   # nodes
 SAMPLE_NODES = 1536
 

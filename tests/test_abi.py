@@ -39,7 +39,7 @@ def test_create_rejects_bad_configuration(oracle_lib):
     p.unitLength = p.unitTime = p.unitDensity = 1.0
     p.lbSize[:] = [8.0, 8.0, 8.0]
     p.initVisc = 0.1
-    p.boundary[:] = [6, 7, 7, 7, 7, 7]  # boundary0 6: every wall a moving slip wall, not supported
+    p.boundary[:] = [3, 7, 7, 7, 7, 7]  # boundary code 3: neither periodic nor a wall
     p.lbSolve = 1
     p.density = p.objectDensity = 1.0
     ctx = ctypes.c_void_p()
@@ -141,6 +141,6 @@ def test_device_byte_estimate_is_host_side():
     more = lbdem.ParticleSet(size=np.ones(1000, np.int32), r=np.full(1000, 3.0), x0=np.zeros((1000, 3)))
     assert lbdem.estimate_bytes(lib, _box_params(96, lbdem.PREC_FP32), more, obs) > est[lbdem.PREC_FP32]
     bad = _box_params(96, lbdem.PREC_FP32)
-    bad.boundary[0] = 6
+    bad.boundary[0] = 3
     with pytest.raises(lbdem.LbdemError):
         lbdem.estimate_bytes(lib, bad, ps, obs)

@@ -165,7 +165,7 @@ def test_bodies_match_oracle(oracle_lib, gpu_lib, case_root, name, prec):
 
 
 @pytest.mark.parametrize("prec", [lbdem.PREC_FP64, lbdem.PREC_FP32])
-@pytest.mark.parametrize("name", ["dyn_walls", "wall_codes", "slip_walls"])
+@pytest.mark.parametrize("name", ["dyn_walls", "wall_codes", "slip_walls", "slip_dyn_walls"])
 def test_wall_codes_match_oracle(oracle_lib, gpu_lib, case_root, name, prec):
     """Boundary codes 5-8 (every wall of boundary0's kind, DEM.cpp:430-446): all faces moving walls
     (type 8, zero velocity); a slip floor with a moving lid under a periodic boundary0 (both static
@@ -214,7 +214,8 @@ GOLD = os.path.join(os.path.dirname(os.path.abspath(__file__)), "golden")
 
 @pytest.mark.parametrize("name", ["suspension", "fs_impactor", "periodic", "units", "moving", "startup", "two_impactors",
                                   "cells", "thin_z2", "thin_z1", "thin_z0", "restart", "bingham", "frictional",
-                                  "voellmy", "body_wca", "body_cubic", "dyn_walls", "wall_codes", "slip_walls"])
+                                  "voellmy", "body_wca", "body_cubic", "dyn_walls", "wall_codes", "slip_walls",
+                                  "slip_dyn_walls"])
 def test_fp64_matches_reference_fixtures(gpu_lib, case_root, name):
     with open(os.path.join(GOLD, "manifest.json")) as fh:
         steps = sorted(m["step"] for m in json.load(fh).values() if m["case"] == name)
@@ -276,6 +277,6 @@ def test_device_errors_are_reported(gpu_lib, case_root):
     """An unsupported configuration fails at create time with a message, before any launch."""
     d = cases.make_case(case_root, "suspension")
     vals, prm, ps, obs = lbdem.load_case(d)
-    prm.boundary[0] = 6
+    prm.boundary[0] = 3
     with pytest.raises(lbdem.LbdemError, match="boundary"):
         lbdem.Simulation(gpu_lib, prm, ps, obs)

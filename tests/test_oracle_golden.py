@@ -124,3 +124,8 @@ def test_slip_fixture_pins_the_slip_term(oracle_lib, case_root):
     act = g["active"]
     assert np.abs(st["f"][:, act] - g["f_active"]).max() > 1e-6
     sim.close()
+    # moving slip walls differ from static ones only in the free-surface mass exchange of their
+    # slip links (LB.cpp:1595-1625): the two fixtures share the populations' reflection and part
+    # in the interface masses
+    h = np.load(os.path.join(GOLD, "slip_dyn_walls_s40.npz"), allow_pickle=False)
+    assert float(h["totalMass"]) != float(g["totalMass"])
