@@ -1201,6 +1201,45 @@ __device__ inline void list_push(int32_t* list, int* count, int cap, int32_t v, 
     else atomicOr(overflow, 1);
 }
 
+// Block-wide forms of the two above for the passes over the whole interface list, where one
+// atomic per wavefront on one counter serialises (every thread of the block must call them, the
+// same number of times; sTmp: blockDim.x / 64 + 1 ints or doubles of LDS, reused by the next call):
+// positions in a list for the threads with `want`, one reservation per block, block-major then
+// wave then lane order; and a sum into one global total, one atomic per block.
+__device__ inline int block_reserve(int* count, bool want, int* sTmp) {
+    const unsigned long long m = __ballot(want);
+    const int lane = __lane_id(), wave = threadIdx.x / 64, nw = blockDim.x / 64;
+    if (lane == 0) sTmp[wave] = __popcll(m);
+    __syncthreads();
+    if (threadIdx.x == 0) {
+        int acc = 0;
+        for (int w = 0; w < nw; ++w) {
+            const int c = sTmp[w];
+            sTmp[w] = acc;
+            acc += c;
+        }
+        sTmp[nw] = acc ? atomicAdd(count, acc) : 0;
+    }
+    __syncthreads();
+    const int r = sTmp[nw] + sTmp[wave] + __popcll(m & ((1ull << lane) - 1ull));
+    __syncthreads();
+    return r;
+}
+template <typename T>
+__device__ inline void block_add(T* dst, T v, T* sTmp) {
+#pragma unroll
+    for (int off = 32; off > 0; off >>= 1) v += __shfl_xor(v, off, 64);
+    const int wave = threadIdx.x / 64, nw = blockDim.x / 64;
+    if (__lane_id() == 0) sTmp[wave] = v;
+    __syncthreads();
+    if (threadIdx.x == 0) {
+        T t = 0;
+        for (int w = 0; w < nw; ++w) t += sTmp[w];
+        if (t != T(0)) atomicAdd(dst, t);
+    }
+    __syncthreads();
+}
+
 // per-thread partial sums into one global total: summed over the wavefront first (every lane of
 // the wavefront must call this), one atomic per wavefront that has a nonzero sum
 __device__ inline void wave_add(double* dst, double v) {
@@ -1366,29 +1405,39 @@ __device__ inline void k_fs_isolated_a(Geom g, uint8_t* type, typename Types<S>:
                                 const int32_t* ifList, const int32_t* newIf, const int32_t* marker, int32_t* list2,
                                 int cap, int* count2, DevScalars* sc) {
     const int nIf = sc->nIf, nN = sc->nNewIf;
+    __shared__ int sPos[8];
+    __shared__ double sSum[8];
     double acc = 0.0;
-    for (int k = blockIdx.x * blockDim.x + threadIdx.x; k < nIf + nN; k += gridDim.x * blockDim.x) {
-        int64_t i;
+    // the loop runs the same number of times on every thread of a block (block_reserve)
+    for (int k0 = blockIdx.x * blockDim.x; k0 < nIf + nN; k0 += gridDim.x * blockDim.x) {
+        const int k = k0 + int(threadIdx.x);
+        int64_t i = -1;
         if (k < nIf) {
             i = ifList[k];
-            if ((type[i] & T_BASE) != T_IF || marker[i] != -1) continue;
-        } else {
+            if ((type[i] & T_BASE) != T_IF || marker[i] != -1) i = -1;
+        } else if (k < nIf + nN) {
             i = newIf[k - nIf];
         }
-        int x, y, z;
-        cell_xyz(g, i, x, y, z);
-        const Offs o(g, x, y, z);
-        bool gasNb = false;  // the 18 type loads overlap (no early exit)
+        bool gasNb = false;
+        if (i >= 0) {
+            int x, y, z;
+            cell_xyz(g, i, x, y, z);
+            const Offs o(g, x, y, z);
 #pragma unroll
-        for (int j = 1; j < Q; ++j) gasNb |= (type[i + o.d(j)] & T_BASE) == T_GAS;
-        list_push(list2, count2, cap, int32_t(i), &sc->listOverflow);
-        if (!gasNb) {
-            acc += double(mass[i]) - double(aux[i]);
-            mass[i] = typename Types<S>::M(aux[i]);
-            type[i] = (type[i] & (T_PMASK | T_EDGE)) | T_FLUID;  // active to active: edge bit stays
+            for (int j = 1; j < Q; ++j) gasNb |= (type[i + o.d(j)] & T_BASE) == T_GAS;  // the 18 loads overlap
+        }
+        const int pos = block_reserve(count2, i >= 0, sPos);
+        if (i >= 0) {
+            if (pos < cap) list2[pos] = int32_t(i);
+            else atomicOr(&sc->listOverflow, 1);
+            if (!gasNb) {
+                acc += double(mass[i]) - double(aux[i]);
+                mass[i] = typename Types<S>::M(aux[i]);
+                type[i] = (type[i] & (T_PMASK | T_EDGE)) | T_FLUID;  // active to active: edge bit stays
+            }
         }
     }
-    wave_add(&sc->surplus, acc);
+    block_add(&sc->surplus, acc, sSum);
 }
 
 // removeIsolated part 2: interface nodes without fluid neighbours become gas (LB.cpp:1821-1841)
@@ -1396,6 +1445,8 @@ template <typename S>
 __device__ inline void k_fs_isolated_b(Geom g, uint8_t* type, const typename Types<S>::M* mass, const int32_t* list2,
                                 const int* count2, DevScalars* sc, int32_t* changed, int changedCap) {
     const int n2 = *count2;
+    __shared__ double sSum[8];
+    __shared__ int sCnt[8];
     double acc = 0.0;
     int keep = 0;
     for (int k = blockIdx.x * blockDim.x + threadIdx.x; k < n2; k += gridDim.x * blockDim.x) {
@@ -1415,8 +1466,8 @@ __device__ inline void k_fs_isolated_b(Geom g, uint8_t* type, const typename Typ
             keep++;
         }
     }
-    wave_add(&sc->surplus, acc);
-    wave_add(&sc->nIfFinal, keep);
+    block_add(&sc->surplus, acc, sSum);
+    block_add(&sc->nIfFinal, keep, sCnt);
 }
 
 // redistributeMass (LB.cpp:1844-1852) and marker reset
