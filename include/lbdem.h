@@ -200,6 +200,20 @@ int lbdem_get_particles(lbdem_ctx* ctx, double* rec /* nParticles*LBDEM_P_REC */
 int lbdem_get_objects(lbdem_ctx* ctx, double* rec /* nObjects*LBDEM_O_REC */);
 int lbdem_get_walls(lbdem_ctx* ctx, double* rec /* nWalls*LBDEM_W_REC */);
 int lbdem_get_bodies(lbdem_ctx* ctx, double* rec /* nBodies*LBDEM_B_REC */);
+/* Stress diagnostics (the DEM members totalStress, hydroStress, lubStress, contactStress,
+ * electroStress and the particles' STotal and disp, DEM.cpp:655-771, 2705-2725, with the particle
+ * stresslet SHydro of LB::computeHydroForces, LB.cpp:1951-1959). The reference computes them on
+ * every step; here they are computed only when enabled, which lbdem_set_stress does before
+ * lbdem_init (LBDEM_E_STATE after it). Once enabled, every LB step also stores the velocity of
+ * every active node and runs one more pass over the particle nodes, and every DEM step
+ * accumulates the pair stresses. Not available in batches. */
+int lbdem_set_stress(lbdem_ctx* ctx, int enable);
+/* The values of the last DEM step (as the reference's IO reads them after goCycle): global = 45
+ * doubles, the tensors totalStress, hydroStress, lubStress, contactStress, electroStress in that
+ * order, each m00 m01 m02 m10 ... m22, already divided by the volume (DEM.cpp:766-771);
+ * stotal = nParticles*9 (STotal of each particle); disp = nParticles*3 (x_i - x0, DEM.cpp:2705).
+ * Any pointer may be NULL. LBDEM_E_STATE unless enabled. */
+int lbdem_get_stress(lbdem_ctx* ctx, double* global, double* stotal, double* disp);
 /* Verlet list of every particle: count[n], list[n*cap + k] (DEM.cpp:473-652). The MI355X library
  * lists partners in ascending order, the oracle in the reference's traversal order; a pair that
  * the reference lists more than once (fewer than 3 link cells along z) appears as often. */

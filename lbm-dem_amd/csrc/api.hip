@@ -123,6 +123,7 @@ struct lbdem_ctx {
     // node velocities (lbdem_store_velocity / lbdem_get_velocity): 0 not held, 1 the next LB step
     // stores them for every active node, 2 held (stored by the last LB step or by lbdem_init)
     int velState = 0;
+    bool stress = false;  // stress diagnostics (lbdem_set_stress)
 
     const char* errFixed = nullptr;  // a message with static storage (set without allocating)
     int fail(int code, const std::string& m) { err = m; errFixed = nullptr; return code; }
@@ -400,6 +401,7 @@ struct Plan {
     int ifCap = 0, objCap = 0;
     bool visc = false;  // per-node viscosity plane
     int nSlip = 0;      // slip-pass nodes
+    bool stress = false;  // stress diagnostic buffers
 };
 
 static int wall_count(const lbdem_ctx* c) {
@@ -443,6 +445,7 @@ static int make_plan(lbdem_ctx* c, Plan& p) {
     }
     p.objCap = int(std::min<int64_t>(std::max<int64_t>(objCells, 1024), g.N));
     p.visc = c->viscPlane;
+    p.stress = c->stress;
     if (c->slipWalls) {  // at most the two node layers next to each wall face
         const int64_t area[3] = {int64_t(g.Y) * g.Z, int64_t(g.X) * g.Z, int64_t(g.X) * g.Y};
         int64_t cnt = 0;
@@ -528,6 +531,11 @@ static bool plan_buffers(const Plan& p, Box& b, DemState& d, DevScalars*& sc, do
     double** wvecs[] = {&d.wFHydro, &d.wFContact, &d.wFLub, &d.wFTotal};
     for (double** v : wvecs) A(*v, 3 * size_t(MAX_WALLS));
     A(d.body, size_t(LBDEM_MAX_BODIES) * LBDEM_B_REC);
+    if (p.stress) {
+        A(d.stressPart, np1 * SP_REC); A(d.shydro, np1 * 9); A(d.xi0, n3); A(d.disp, n3);
+    } else {
+        d.stressPart = d.shydro = d.xi0 = d.disp = nullptr;
+    }
     A(d.plb, np1);
     A(d.olb, no1);
     A(tmp, TMP_PLANNED / sizeof(double));
@@ -692,6 +700,7 @@ extern "C" int lbdem_init(lbdem_ctx* c) {
     for (size_t w = 0; w < walls.size(); ++w)
         for (int a = 0; a < 3; ++a) { P.wallP[w][a] = walls[w][a]; P.wallN[w][a] = walls[w][3 + a]; }
     P.unitLength = u.Length; P.unitSpeed = u.Speed; P.unitAngVel = u.AngVel;
+    P.stress = c->stress ? 1 : 0;
 
     // ---------------- lattice ----------------
     const Geom g = plan.g;
@@ -804,6 +813,13 @@ extern "C" int lbdem_init(lbdem_ctx* c) {
         HIPCHK(hipMemcpyAsync(d.oFGrav, ofg.data(), 3 * no * 8, hipMemcpyHostToDevice, st));
     }
     HIPCHK(hipMemcpyAsync(d.body, brec.data(), brec.size() * 8, hipMemcpyHostToDevice, st));
+    if (c->stress) {  // x_i = x0 at the start (DEM.cpp:288); SHydro zero until the first LB step
+        HIPCHK(hipMemsetAsync(d.stressPart, 0, size_t(std::max(np, 1)) * SP_REC * 8, st));
+        HIPCHK(hipMemsetAsync(d.shydro, 0, size_t(std::max(np, 1)) * 9 * 8, st));
+        HIPCHK(hipMemsetAsync(d.disp, 0, n3 * 8, st));
+        HIPCHK(hipMemsetAsync(d.xi0, 0, n3 * 8, st));
+        if (np > 0) HIPCHK(hipMemcpyAsync(d.xi0, c->px0.data(), 3 * np * 8, hipMemcpyHostToDevice, st));
+    }
     b.nSlip = int(c->slipNodes.size());
     if (b.nSlip > 0)
         HIPCHK(hipMemcpyAsync(b.slipList, c->slipNodes.data(), size_t(b.nSlip) * 4, hipMemcpyHostToDevice, st));
@@ -964,7 +980,10 @@ static void enqueue_lb(lbdem_ctx* c, bool finalize = true) {  // LB::latticeBolz
     lb_mark_edges(c->box, c->sc, c->stream);
     const bool storeVel = c->velState == 1;  // lbdem_store_velocity asked for this step
     c->velState = storeVel ? 2 : 0;
-    lb_fused_step(c->box, c->k, c->ds.plb, c->ds.olb, c->sc, true, storeVel, c->stream, e0, e1);
+    // the stress diagnostics read every active node's hydrodynamic force from aux
+    lb_fused_step(c->box, c->k, c->ds.plb, c->ds.olb, c->sc, true, storeVel || c->stress, c->stream, e0, e1);
+    if (c->stress)
+        lb_hydro_stress(c->box, c->ds.plb, c->dp.nPart, c->unit.Length, c->unit.Force, c->ds.shydro, c->stream);
     if (finalize) enqueue_hydro(c, c->stream);
 }
 static void enqueue_fs(lbdem_ctx* c, hipStream_t s) { lb_free_surface(c->box, c->k, c->sc, s); }
@@ -1458,6 +1477,52 @@ extern "C" int lbdem_get_bodies(lbdem_ctx* c, double* rec) {
     return 0;
 }
 
+extern "C" int lbdem_set_stress(lbdem_ctx* c, int enable) {
+    if (!c) return LBDEM_E_ARG;
+    if (c->initialised) return c->fail(LBDEM_E_STATE, "lbdem_set_stress after lbdem_init");
+    c->stress = enable != 0;
+    return 0;
+}
+
+// The per-particle records of the last DEM step, combined in the reference's order on the host:
+// STotal = SHydro + SLub + SContact + SRep (DEM.cpp:748), the sums over the particles divided by
+// the volume (DEM.cpp:745-771). Each particle pair added its contact and electrostatic tensor to
+// both of its spheres, so the pair parts of contactStress and electroStress are half the sums.
+extern "C" int lbdem_get_stress(lbdem_ctx* c, double* global, double* stotal, double* disp) {
+    NEED_INIT;
+    if (!c->stress) return c->fail(LBDEM_E_STATE, "the stress diagnostics are off (lbdem_set_stress before lbdem_init)");
+    const int np = c->dp.nPart;
+    std::vector<double> rec(size_t(std::max(np, 1)) * SP_REC, 0.0);
+    if (np > 0) {
+        if (fetch(c, rec.data(), c->ds.stressPart, np * SP_REC)) return LBDEM_E_DEVICE;
+        HIPCHK(hipStreamSynchronize(c->stream));
+        if (disp) {
+            if (fetch(c, disp, c->ds.disp, 3 * np)) return LBDEM_E_DEVICE;
+            HIPCHK(hipStreamSynchronize(c->stream));
+        }
+    }
+    double hydro[9] = {}, lub[9] = {}, cpair[9] = {}, cwall[9] = {}, rep[9] = {};
+    for (int i = 0; i < np; ++i) {
+        const double* R = &rec[size_t(i) * SP_REC];
+        for (int e = 0; e < 9; ++e) {
+            hydro[e] += R[SP_HYDRO + e]; lub[e] += R[SP_LUB + e];
+            cpair[e] += R[SP_CPAIR + e]; cwall[e] += R[SP_CWALL + e]; rep[e] += R[SP_REP + e];
+            if (stotal)
+                stotal[size_t(i) * 9 + e] = R[SP_HYDRO + e] + R[SP_LUB + e] + (R[SP_CPAIR + e] + R[SP_CWALL + e]) + R[SP_REP + e];
+        }
+    }
+    if (global) {
+        const double vol = c->dp.demSize[0] * c->dp.demSize[1] * c->dp.demSize[2];
+        for (int e = 0; e < 9; ++e) {
+            const double h = hydro[e] / vol, l = lub[e] / vol;
+            const double ct = (0.5 * cpair[e] + cwall[e]) / vol, el = 0.5 * rep[e] / vol;
+            global[e] = ct + l + el + h;
+            global[9 + e] = h; global[18 + e] = l; global[27 + e] = ct; global[36 + e] = el;
+        }
+    }
+    return 0;
+}
+
 extern "C" int lbdem_get_walls(lbdem_ctx* c, double* rec) {
     NEED_INIT;
     const int nw = c->dp.nWall;
@@ -1823,6 +1888,7 @@ extern "C" int lbdem_batch_create(lbdem_ctx* const* ctxs, int n, lbdem_batch** o
         lbdem_ctx* c = ctxs[i];
         if (!c || !c->initialised) return LBDEM_E_STATE;
         if (c->batch) return c->fail(LBDEM_E_STATE, "the context is already in a batch");
+        if (c->stress) return c->fail(LBDEM_E_ARG, "the stress diagnostics are not available in batches");
         if (!steady(c)) return c->fail(LBDEM_E_STATE, "the context has not finished its start-up steps");
         if (c->box.prec != c0->box.prec || c->prm.device != c0->prm.device || c->freeSurface != c0->freeSurface)
             return c->fail(LBDEM_E_ARG, "batch members need the same precision, device and free-surface setting");

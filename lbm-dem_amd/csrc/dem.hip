@@ -309,11 +309,15 @@ struct ForceArgs {
     double* objContrib;   // per particle, per object: Fc(3) Mc(3) Fl(3) Ml(3)
     double* wallContrib;  // per particle, per wall: Fc(3) Fl(3)
     int demTimeStep;
+    double* stressPart;   // SP_REC per particle (with the stress diagnostics)
+    const double* shydro; // 9 per particle
 };
 
-struct LubOut { V F, T1, T2; bool act; };
+// S1, S2: the two spheres' stresslets as xx yy zz yz zx xy (with STRESS only)
+struct LubOut { V F, T1, T2; bool act; double S1[6], S2[6]; };
 
 // pair lubrication (DEM.cpp:1140-1340, 2302-2521) for the ordered pair (a = n1, b = n2)
+template <bool STRESS>
 __device__ LubOut pair_lub(const DemParams& P, V xa, V xb, double ra, double rb, V va, V vb, V wa, V wb, double bnum,
                            double bden) {
 #pragma clang fp contract(off)
@@ -328,15 +332,22 @@ __device__ LubOut pair_lub(const DemParams& P, V xa, V xb, double ra, double rb,
     o.act = dr < dmaxd(cut, dmaxd(lub_N, dmaxd(lub_T, lub_R)));
     if (!o.act) return o;
     double Xa, Ya, Yb1, Yb2, Yc11, Yc12, Yc21, Yc22, h_c, beta;
+    double Xg1 = 0, Xg2 = 0, Yg1 = 0, Yg2 = 0, Yh11 = 0, Yh12 = 0, Yh21 = 0, Yh22 = 0;  // stresslet (DEM.cpp:2362-2406)
     const double alpha = M_PI * P.kinVisc * rad * P.deltat;
     const double beta1 = bnum / bden;
     const double beta2 = beta1 * beta1, beta3 = beta1 * beta2;
     const double beta4 = pow((1 + beta1), 4.0);
+    const double beta5 = STRESS ? pow((1 + beta1), 5.0) : 1.0;
     if (dr < dmaxd(cut, lub_N)) {
         Xa = 4.0 * beta2 / beta4;
         h_c = dmaxd(dr, lub_N);
         Xa *= rad / h - rad / h_c;
         Xa *= 6.0 * alpha;
+        if constexpr (STRESS) {
+            Xg1 = 12.0 * beta2 / beta5; Xg2 = -12.0 * beta3 / beta5;
+            Xg1 *= rad / h - rad / h_c; Xg2 *= rad / h - rad / h_c;
+            Xg1 *= 4.0 * alpha * rad; Xg2 *= 4.0 * alpha * rad;
+        }
     } else Xa = 0.0;
     if (dr < dmaxd(cut, lub_T)) {
         Ya = 8.0 * beta1 * (2.0 + beta1 + 2.0 * beta2) / (15.0 * beta4);
@@ -345,6 +356,11 @@ __device__ LubOut pair_lub(const DemParams& P, V xa, V xb, double ra, double rb,
         h_c = dmaxd(dr, lub_T);
         beta = alpha * log(h_c / h);
         Ya *= 6.0 * beta; Yb1 *= 4.0 * beta * rad; Yb2 *= 4.0 * beta * rad;
+        if constexpr (STRESS) {
+            Yg1 = 2.0 * beta1 * (4.0 - beta1 + 7.0 * beta2) / (5.0 * beta5);
+            Yg2 = -2.0 * beta2 * (4.0 * beta2 - beta1 + 7.0) / (5.0 * beta5);
+            Yg1 *= 4.0 * beta * rad; Yg2 *= 4.0 * beta * rad;
+        }
     } else Ya = Yb1 = Yb2 = 0.0;
     if (dr < dmaxd(cut, lub_R)) {
         Yc11 = 16.0 * beta1 / (5.0 * beta4);
@@ -355,6 +371,14 @@ __device__ LubOut pair_lub(const DemParams& P, V xa, V xb, double ra, double rb,
         beta = alpha * log(h_c / h);
         Yc11 *= 8.0 * beta * rad * rad; Yc12 *= 8.0 * beta * rad * rad;
         Yc21 *= 8.0 * beta * rad * rad; Yc22 *= 8.0 * beta * rad * rad;
+        if constexpr (STRESS) {
+            Yh11 = 4.0 * beta1 * (2.0 - beta1) / (5.0 * beta5);
+            Yh12 = 2.0 * beta2 * (1.0 + 7.0 * beta1) / (5.0 * beta5);
+            Yh21 = 2.0 * beta2 * (beta1 + 7.0) / (5.0 * beta5);
+            Yh22 = 4.0 * beta2 * (2.0 * beta2 - beta1) / (5.0 * beta5);
+            Yh11 *= 8.0 * beta * rad * rad; Yh21 *= 8.0 * beta * rad * rad;
+            Yh12 *= 8.0 * beta * rad * rad; Yh22 *= 8.0 * beta * rad * rad;
+        }
     } else Yc11 = Yc12 = Yc21 = Yc22 = 0.0;
     x12 /= r12; y12 /= r12; z12 /= r12;
     const double w1x = wa.x, w1y = wa.y, w1z = wa.z, w2x = wb.x, w2y = wb.y, w2z = wb.z;
@@ -379,7 +403,30 @@ __device__ LubOut pair_lub(const DemParams& P, V xa, V xb, double ra, double rb,
     T2y += -Yc21 * (w1y - w1_r * y12) - Yc22 * (w2y - w2_r * y12);
     T2z += -Yc21 * (w1z - w1_r * z12) - Yc22 * (w2z - w2_r * z12);
     o.F = V{Fx, Fy, Fz}; o.T1 = V{T1x, T1y, T1z}; o.T2 = V{T2x, T2y, T2z};
+    if constexpr (STRESS) {  // DEM.cpp:2454-2477
+        const double xs[3] = {x12, y12, z12}, us[3] = {umx, umy, umz};
+        const double w1s[3] = {w1xrx, w1xry, w1xrz}, w2s[3] = {w2xrx, w2xry, w2xrz};
+        // components xx yy zz yz zx xy as (i, j) pairs
+        const int ci[6] = {0, 1, 2, 1, 2, 0}, cj[6] = {0, 1, 2, 2, 0, 1};
+#pragma unroll
+        for (int e = 0; e < 6; ++e) {
+            const int i = ci[e], j = cj[e];
+            double s1 = Xg1 * um_r * xs[i] * xs[j] + Yg1 * (xs[i] * us[j] + us[i] * xs[j]);
+            double s2 = -Xg2 * um_r * xs[i] * xs[j] - Yg2 * (xs[i] * us[j] + us[i] * xs[j]);
+            s1 += -Yh11 * (xs[i] * w1s[j] + w1s[i] * xs[j]) - Yh12 * (xs[i] * w2s[j] + w2s[i] * xs[j]);
+            s2 += -Yh21 * (xs[i] * w1s[j] + w1s[i] * xs[j]) - Yh22 * (xs[i] * w2s[j] + w2s[i] * xs[j]);
+            o.S1[e] = s1; o.S2[e] = s2;
+        }
+    }
     return o;
+}
+
+// S += -a (x) f, row-major (the contact and electrostatic stresses, DEM.cpp:1715-1783, 1451-1479)
+__device__ inline void add_arm(double* S, double ax, double ay, double az, double fx, double fy, double fz) {
+#pragma clang fp contract(off)
+    S[0] += -ax * fx; S[1] += -ax * fy; S[2] += -ax * fz;
+    S[3] += -ay * fx; S[4] += -ay * fy; S[5] += -ay * fz;
+    S[6] += -az * fx; S[7] += -az * fy; S[8] += -az * fz;
 }
 
 // wall-type contact with the shared per-particle history xiw (DEM.cpp:1946-2153, 2157-2296)
@@ -438,7 +485,9 @@ __device__ inline V group_sum(V v) { return V{group_sum(v.x), group_sum(v.y), gr
 
 // Forces on particle p: DEM_LANES lanes per particle share its Verlet partners (lane l takes
 // partners l, l + DEM_LANES, ...), their partial sums are combined in a fixed order, and the
-// group's first lane adds the wall and impactor terms and writes the results.
+// group's first lane adds the wall and impactor terms and writes the results. STRESS also
+// accumulates the particle's stress tensors (DEM.cpp:655-771) into its SP_REC record.
+template <bool STRESS>
 __device__ inline void k_dem_forces(ForceArgs a, double dampCoeff) {
 #pragma clang fp contract(off)
     const int gt = blockIdx.x * blockDim.x + threadIdx.x;
@@ -449,6 +498,8 @@ __device__ inline void k_dem_forces(ForceArgs a, double dampCoeff) {
     const V xp = ld3(a.x0, p), vp = ld3(a.x1, p), wp = ld3(a.w1, p);
     const double rp = a.r[p];
     V Fc{0, 0, 0}, Mc{0, 0, 0}, Fl{0, 0, 0}, Ml{0, 0, 0}, Fr{0, 0, 0};
+    double Sl[6] = {0, 0, 0, 0, 0, 0};  // SLub as xx yy zz yz zx xy
+    double Sc[9] = {0, 0, 0, 0, 0, 0, 0, 0, 0}, Sr[9] = {0, 0, 0, 0, 0, 0, 0, 0, 0};
     const int cnt = valid ? a.nbrCount[p] : 0;
     const bool waited = a.demTimeStep >= P.objWaitTime;
     // One pass over the lane's partners: the pair's data are gathered once and the three pair
@@ -465,10 +516,12 @@ __device__ inline void k_dem_forces(ForceArgs a, double dampCoeff) {
         if (slot >= 0) xi = ld3(a.xiOld, slot);
         // particle lubrication (DEM.cpp:2302-2521)
         if (waited) {
-            const LubOut o = pair_lub(P, A, B, rA, rB, vA, vB, wA, wB, rB, rA);
+            const LubOut o = pair_lub<STRESS>(P, A, B, rA, rB, vA, vB, wA, wB, rB, rA);
             if (o.act) {
                 if (p == n1) { Fl = add(Fl, o.F); Ml = add(Ml, o.T1); }
                 else { Fl = sub(Fl, o.F); Ml = add(Ml, o.T2); }
+                if constexpr (STRESS)
+                    for (int e = 0; e < 6; ++e) Sl[e] += p == n1 ? o.S1[e] : o.S2[e];
             }
         }
         // particle-particle contact (DEM.cpp:1487-1791)
@@ -515,6 +568,7 @@ __device__ inline void k_dem_forces(ForceArgs a, double dampCoeff) {
         const double dtx = fty * nz - ftz * ny, dty = ftz * nx - ftx * nz, dtz = ftx * ny - fty * nx;
         if (p == n1) { Fc = add(Fc, V{dfx, dfy, dfz}); Mc = add(Mc, V{rA * dtx, rA * dty, rA * dtz}); }
         else { Fc = sub(Fc, V{dfx, dfy, dfz}); Mc = add(Mc, V{rB * dtx, rB * dty, rB * dtz}); }
+        if constexpr (STRESS) add_arm(Sc, x12, y12, z12, dfx, dfy, dfz);  // the same tensor for n1 and n2
         // electrostatic repulsion over every listed pair (DEM.cpp:1404-1484)
         {
             double fx, fy, fz;
@@ -525,9 +579,15 @@ __device__ inline void k_dem_forces(ForceArgs a, double dampCoeff) {
                 fx = P.electroForce * nx; fy = P.electroForce * ny; fz = P.electroForce * nz;
             }
             if (p == n1) Fr = add(Fr, V{fx, fy, fz}); else Fr = sub(Fr, V{fx, fy, fz});
+            if constexpr (STRESS) add_arm(Sr, x12, y12, z12, fx, fy, fz);
         }
     }
     Fc = group_sum(Fc); Mc = group_sum(Mc); Fl = group_sum(Fl); Ml = group_sum(Ml); Fr = group_sum(Fr);
+    double Sw[9] = {0, 0, 0, 0, 0, 0, 0, 0, 0};  // contact stress of walls and impactors
+    if constexpr (STRESS) {
+        for (int e = 0; e < 6; ++e) Sl[e] = group_sum(Sl[e]);
+        for (int e = 0; e < 9; ++e) { Sc[e] = group_sum(Sc[e]); Sr[e] = group_sum(Sr[e]); }
+    }
     if (!valid || lane != 0) return;
     V xiw = ld3(a.xiw, p), xio = ld3(a.xio, p);
     double* oc = a.objContrib + int64_t(p) * 12 * P.nObj;
@@ -547,15 +607,20 @@ __device__ inline void k_dem_forces(ForceArgs a, double dampCoeff) {
             const double h = dmaxd(dr, cut);
             if (!(dr < dmaxd(cut, dmaxd(lub_N, dmaxd(lub_T, lub_R))))) continue;
             const double alpha = M_PI * P.kinVisc * rad * P.deltat;
-            double Xa, Ya, Yb1, Yc11, h_c, beta;
-            if (dr < dmaxd(cut, lub_N)) { Xa = 1.0; h_c = dmaxd(dr, lub_N); Xa *= rad / h - rad / h_c; Xa *= 6.0 * alpha; }
-            else Xa = 0.0;
+            double Xa, Ya, Yb1, Yc11, h_c, beta, Xg1 = 0, Yg1 = 0, Yh11 = 0;
+            if (dr < dmaxd(cut, lub_N)) {
+                Xa = 1.0; h_c = dmaxd(dr, lub_N); Xa *= rad / h - rad / h_c; Xa *= 6.0 * alpha;
+                if constexpr (STRESS) { Xg1 = 3.0 / 2.0; Xg1 *= rad / h - rad / h_c; Xg1 *= 4.0 * alpha * rad; }
+            } else Xa = 0.0;
             if (dr < dmaxd(cut, lub_T)) {
                 Ya = 8.0 / 15.0; Yb1 = -1.0 / 5.0; h_c = dmaxd(dr, lub_T); beta = alpha * log(h_c / h);
                 Ya *= 6.0 * beta; Yb1 *= 4.0 * beta * rad;
+                if constexpr (STRESS) { Yg1 = 7.0 / 10.0; Yg1 *= 4.0 * beta * rad; }
             } else Ya = Yb1 = 0.0;
-            if (dr < dmaxd(cut, lub_R)) { Yc11 = 2.0 / 5.0; h_c = dmaxd(dr, lub_R); beta = alpha * log(h_c / h); Yc11 *= 8.0 * beta * rad * rad; }
-            else Yc11 = 0.0;
+            if (dr < dmaxd(cut, lub_R)) {
+                Yc11 = 2.0 / 5.0; h_c = dmaxd(dr, lub_R); beta = alpha * log(h_c / h); Yc11 *= 8.0 * beta * rad * rad;
+                if constexpr (STRESS) { Yh11 = -1.0 / 10.0; Yh11 *= 8.0 * beta * rad * rad; }
+            } else Yc11 = 0.0;
             x12 /= r12; y12 /= r12; z12 /= r12;
             const double umx = vp.x, umy = vp.y, umz = vp.z, w1x = wp.x, w1y = wp.y, w1z = wp.z;
             const double um_r = umx * x12 + umy * y12 + umz * z12;
@@ -570,6 +635,16 @@ __device__ inline void k_dem_forces(ForceArgs a, double dampCoeff) {
             T1x += -Yc11 * (w1x - w1_r * x12); T1y += -Yc11 * (w1y - w1_r * y12); T1z += -Yc11 * (w1z - w1_r * z12);
             Fl = add(Fl, V{Fx, Fy, Fz}); Ml = add(Ml, V{T1x, T1y, T1z});
             wc[6 * w + 3] -= Fx; wc[6 * w + 4] -= Fy; wc[6 * w + 5] -= Fz;
+            if constexpr (STRESS) {  // DEM.cpp:2623-2634 (the wall's own bracketing of the Xg1 term)
+                const double xs[3] = {x12, y12, z12}, us[3] = {umx, umy, umz}, w1s[3] = {w1xrx, w1xry, w1xrz};
+                const int ci[6] = {0, 1, 2, 1, 2, 0}, cj[6] = {0, 1, 2, 2, 0, 1};
+                for (int e = 0; e < 6; ++e) {
+                    const int i = ci[e], j = cj[e];
+                    double s1 = Xg1 * (um_r * xs[i] * xs[j]) + Yg1 * (xs[i] * us[j] + us[i] * xs[j]);
+                    s1 += -Yh11 * (xs[i] * w1s[j] + w1s[i] * xs[j]);
+                    Sl[e] += s1;
+                }
+            }
         }
         // object-particle contact (DEM.cpp:912-1138); history xio shared by all objects
         for (int o = 0; o < P.nObj; ++o) {
@@ -620,15 +695,18 @@ __device__ inline void k_dem_forces(ForceArgs a, double dampCoeff) {
             const double dtx = fty * n1wz - ftz * n1wy, dty = ftz * n1wx - ftx * n1wz, dtz = ftx * n1wy - fty * n1wx;
             Fc = add(Fc, df);
             Mc = add(Mc, V{rp * dtx, rp * dty, rp * dtz});
+            if constexpr (STRESS) add_arm(Sw, x1w, y1w, z1w, df.x, df.y, df.z);  // DEM.cpp:1097-1135
             double* c = oc + 12 * o;
             c[0] = -df.x; c[1] = -df.y; c[2] = -df.z;
             c[3] = orr * dtx; c[4] = orr * dty; c[5] = orr * dtz;
         }
         // object-particle lubrication (DEM.cpp:1140-1340)
         for (int o = 0; o < P.nObj; ++o) {
-            const LubOut lo = pair_lub(P, xp, ld3(a.ox0, o), rp, a.or_[o], vp, ld3(a.ox1, o), wp, ld3(a.ow1, o), a.or_[o], rp);
+            const LubOut lo = pair_lub<STRESS>(P, xp, ld3(a.ox0, o), rp, a.or_[o], vp, ld3(a.ox1, o), wp, ld3(a.ow1, o), a.or_[o], rp);
             if (!lo.act) continue;
             Fl = add(Fl, lo.F); Ml = add(Ml, lo.T1);
+            if constexpr (STRESS)
+                for (int e = 0; e < 6; ++e) Sl[e] += lo.S1[e];
             double* c = oc + 12 * o;
             c[6] = -lo.F.x; c[7] = -lo.F.y; c[8] = -lo.F.z;
             c[9] = lo.T2.x; c[10] = lo.T2.y; c[11] = lo.T2.z;
@@ -648,6 +726,19 @@ __device__ inline void k_dem_forces(ForceArgs a, double dampCoeff) {
         wall_like_contact(P, dampCoeff, rp, vp, wp, xiw, x1w, y1w, z1w, F, M);
         Fc = add(Fc, F); Mc = add(Mc, M);
         wc[6 * w + 0] -= F.x; wc[6 * w + 1] -= F.y; wc[6 * w + 2] -= F.z;
+        if constexpr (STRESS) add_arm(Sw, x1w, y1w, z1w, F.x, F.y, F.z);  // DEM.cpp:2110-2148
+    }
+    if constexpr (STRESS) {
+        double* rec = a.stressPart + int64_t(p) * SP_REC;
+        for (int e = 0; e < 9; ++e) rec[SP_HYDRO + e] = a.shydro[9 * p + e];
+        // SLub in full (m01 = m10 = xy, m02 = m20 = zx, m12 = m21 = yz)
+        const double full[9] = {Sl[0], Sl[5], Sl[4], Sl[5], Sl[1], Sl[3], Sl[4], Sl[3], Sl[2]};
+        for (int e = 0; e < 9; ++e) {
+            rec[SP_LUB + e] = full[e];
+            rec[SP_CPAIR + e] = Sc[e];
+            rec[SP_CWALL + e] = Sw[e];
+            rec[SP_REP + e] = Sr[e];
+        }
     }
     st3(a.FContact, p, Fc); st3(a.MContact, p, Mc); st3(a.FLub, p, Fl); st3(a.MLub, p, Ml); st3(a.FRep, p, Fr);
     st3(a.xiw, p, xiw); st3(a.xio, p, xio);
@@ -830,6 +921,11 @@ __device__ inline double integrate_particle(const DemParams& P, const DemState& 
     for (int c = 0; c < 3; ++c) x0[c] += dt * x1[c] + dt * dt * F[c] * im;
     for (int c = 0; c < 3; ++c) x1[c] += dt * F[c] * im;
     for (int c = 0; c < 3; ++c) w1[c] += dt * M[c] * iI;
+    if (P.stress) {  // DEM.cpp:2705, 2721-2725 (before the periodic wrap, which follows the integration)
+        for (int c = 0; c < 3; ++c) d.disp[3 * p + c] = d.xi0[3 * p + c] - x0[c];
+        if (demTimeStep == P.objWaitTime)
+            for (int c = 0; c < 3; ++c) d.xi0[3 * p + c] = x0[c];
+    }
     for (int c = 0; c < 3; ++c)
         if (P.periodic[c]) {
             if (x0[c] > P.demSize[c]) x0[c] -= P.demSize[c];
@@ -927,7 +1023,9 @@ bool dem_step(const DemParams& p, DemState& d, DevScalars* sc, int demTimeStep, 
         f.xiOld = d.xiCur; f.xiNew = d.xiNext;
         f.ox0 = d.ox0; f.ox1 = d.ox1; f.ow1 = d.ow1; f.or_ = d.or_;
         f.objContrib = d.objContrib; f.wallContrib = d.wallContrib; f.demTimeStep = demTimeStep;
-        launch<&k_dem_forces, 256, 1>((p.nPart * DEM_LANES + 255) / 256, 256, s, f, p.dampCoeff);
+        f.stressPart = d.stressPart; f.shydro = d.shydro;
+        if (p.stress) launch<&k_dem_forces<true>, 256, 1>((p.nPart * DEM_LANES + 255) / 256, 256, s, f, p.dampCoeff);
+        else launch<&k_dem_forces<false>, 256, 1>((p.nPart * DEM_LANES + 255) / 256, 256, s, f, p.dampCoeff);
         if (p.nObj > 0 || p.nWall > 0)
             launch<&k_dem_reduce_a, RED_BS, 1>(2 * RED_BLOCKS, RED_BS, s, d.objContrib, 12 * p.nObj, d.wallContrib, 6 * p.nWall, p.nPart, d.redPart);
         double* t = d.xiCur; d.xiCur = d.xiNext; d.xiNext = t;

@@ -185,6 +185,57 @@ void lb_download_velocity(const Box& b, double* out, hipStream_t s) {
     else launch<&k_download_vel<float>, 1024, 1>(4096, 256, s, b.g.N, b.type, (const float*)b.aux, out);
 }
 
+// SHydro of every particle (LB.cpp:1951-1959): the sum over the active nodes inside it of
+// radius (x) diffVel in physical units, with diffVel = -(the node's hydrodynamic force), which the
+// LB step just stored in aux planes 4-6 (it runs with every active node's values stored). One
+// thread per (x, y) column walks z; its sums go out, 9 atomics, whenever the particle changes.
+__device__ inline void k_zero_doubles(double* p, int64_t n) {
+    for (int64_t i = int64_t(blockIdx.x) * blockDim.x + threadIdx.x; i < n; i += int64_t(gridDim.x) * blockDim.x)
+        p[i] = 0.0;
+}
+template <typename M>
+__device__ inline void k_hydro_stress(Geom g, const uint8_t* type, const int32_t* solid, const M* aux,
+                                      const PartLB* plb, double L, double F, double* shydro) {
+#pragma clang fp contract(off)
+    const int64_t SZ = int64_t(g.X) * g.Y, N = g.N;
+    const int64_t col = int64_t(blockIdx.x) * blockDim.x + threadIdx.x;
+    if (col >= SZ) return;
+    const int x = int(col % g.X), y = int(col / g.X);
+    int key = -1;
+    double acc[9];
+    for (int e = 0; e < 9; ++e) acc[e] = 0.0;
+    for (int z = 0; z < g.Z; ++z) {
+        const int64_t i = col + SZ * z;
+        const uint8_t t = type[i];
+        const int k = (is_active(t) && (t & T_PMASK)) ? solid[i] : -1;
+        if (k != key) {
+            if (key >= 0)
+                for (int e = 0; e < 9; ++e) atomicAdd(shydro + 9 * int64_t(key) + e, acc[e]);
+            key = k;
+            for (int e = 0; e < 9; ++e) acc[e] = 0.0;
+        }
+        if (k < 0) continue;
+        const PartLB P = plb[k];
+        const double r[3] = {double(x) - P.c[0], double(y) - P.c[1], double(z) - P.c[2]};
+        const double d[3] = {-double(aux[4 * N + i]), -double(aux[5 * N + i]), -double(aux[6 * N + i])};
+        for (int a = 0; a < 3; ++a)
+            for (int c = 0; c < 3; ++c) acc[3 * a + c] += r[a] * L * d[c] * F;
+    }
+    if (key >= 0)
+        for (int e = 0; e < 9; ++e) atomicAdd(shydro + 9 * int64_t(key) + e, acc[e]);
+}
+void lb_hydro_stress(const Box& b, const PartLB* plb, int nPart, double L, double F, double* shydro, hipStream_t s) {
+    const int64_t n9 = 9 * int64_t(nPart > 0 ? nPart : 1);
+    launch<&k_zero_doubles, 1024, 1>(int((n9 + 255) / 256), 256, s, shydro, n9);
+    if (nPart <= 0) return;
+    const int64_t cols = int64_t(b.g.X) * b.g.Y;
+    const int nb = int((cols + 255) / 256);
+    if (b.prec == LBDEM_PREC_FP64)
+        launch<&k_hydro_stress<double>, 1024, 1>(nb, 256, s, b.g, b.type, b.solid, (const double*)b.aux, plb, L, F, shydro);
+    else
+        launch<&k_hydro_stress<float>, 1024, 1>(nb, 256, s, b.g, b.type, b.solid, (const float*)b.aux, plb, L, F, shydro);
+}
+
 void lb_init_fields(const Box& b, const Consts& k, hipStream_t s) {
     const int bs = 256;
     const int nb = int((b.g.N + bs - 1) / bs);

@@ -215,6 +215,9 @@ void lb_restart_fields(const Box& b, const double* rec, const int32_t* node, int
 void lb_download_velocity(const Box& b, double* out, hipStream_t s);
 // N values: the node viscosity plane (b.visc must be set) at active nodes, zero elsewhere
 void lb_download_visc(const Box& b, double* out, hipStream_t s);
+// SHydro of every particle (9 per particle, physical units: L, F = unit.Length, unit.Force) from
+// the aux planes an LB step with storeVel left
+void lb_hydro_stress(const Box& b, const PartLB* plb, int nPart, double L, double F, double* shydro, hipStream_t s);
 void lb_mark_edges(const Box& b, DevScalars* sc, hipStream_t s);
 // storeVel: every active node's density and velocity into aux (else the interface nodes' only)
 void lb_fused_step(Box& b, const Consts& k, const PartLB* plb, const ObjLB* olb, DevScalars* sc, bool diag,
@@ -256,7 +259,13 @@ struct DemParams {
     double springDamp, epsilon, k13;
     double wallP[MAX_WALLS][3], wallN[MAX_WALLS][3];
     double unitLength, unitSpeed, unitAngVel;
+    int stress;                // accumulate the stress diagnostics (DemState::stressPart, disp)
 };
+// per-particle stress record of the last DEM step (doubles): the stresslet SHydro the force kernel
+// copied from DemState::shydro, SLub, the contact stress of the particle pairs (each pair adds the
+// same tensor to both of its spheres), the contact stress of walls and impactors, SRep; each
+// m00 m01 m02 m10 ... m22
+enum { SP_HYDRO = 0, SP_LUB = 9, SP_CPAIR = 18, SP_CWALL = 27, SP_REP = 36, SP_REC = 45 };
 constexpr int VEL_SLOTS = 64;  // slots of the particle speed maximum (DemState::blockMax)
 struct DemState {
     // particles (FP64, vectors stored as 3 consecutive doubles)
@@ -286,6 +295,11 @@ struct DemState {
     double *oFHydro, *oMHydro, *oFContact, *oMContact, *oFLub, *oMLub, *oFTotal, *oMTotal, *oFGrav;
     // bodies: LBDEM_B_REC doubles each (include/lbdem.h layout)
     double* body;
+    // stress diagnostics (null unless DemParams::stress)
+    double* stressPart;        // SP_REC per particle
+    double* shydro;            // 9 per particle: SHydro of the last LB step (physical units)
+    double* xi0;               // 3 per particle: x_i, the position at the start or at objWaitTime
+    double* disp;              // 3 per particle: x_i - x0 after the last integration
     // walls
     double *wFHydro, *wFContact, *wFLub, *wFTotal;
     PartLB* plb;

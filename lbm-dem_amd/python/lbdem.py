@@ -174,6 +174,8 @@ class Library:
             "lbdem_get_objects": ([vp, _dp], C.c_int),
             "lbdem_set_bodies": ([vp, C.c_int, _dp, _dp, _dp, _dp, _dp], C.c_int),
             "lbdem_get_bodies": ([vp, _dp], C.c_int),
+            "lbdem_set_stress": ([vp, C.c_int], C.c_int),
+            "lbdem_get_stress": ([vp, C.c_void_p, C.c_void_p, C.c_void_p], C.c_int),
             "lbdem_get_walls": ([vp, _dp], C.c_int),
             "lbdem_get_neighbors": ([vp, C.c_int, _ip, _ip], C.c_int),
             "lbdem_total_mass": ([vp, _dp], C.c_int),
@@ -278,6 +280,21 @@ class Simulation:
 
     def init(self):
         self._check(self.lib.lib.lbdem_init(self.ctx), "lbdem_init")
+
+    def set_stress(self, enable: bool = True):
+        """Compute the stress diagnostics (before init)."""
+        self._check(self.lib.lib.lbdem_set_stress(self.ctx, 1 if enable else 0), "lbdem_set_stress")
+
+    def stress(self):
+        """(global 5 x 3 x 3: totalStress, hydroStress, lubStress, contactStress, electroStress; STotal
+        of every particle, n x 3 x 3; disp, n x 3) of the last DEM step."""
+        n = self.info()["nParticles"]
+        g = np.zeros(45, np.float64)
+        st = np.zeros(max(n, 1) * 9, np.float64)
+        d = np.zeros(max(n, 1) * 3, np.float64)
+        self._check(self.lib.lib.lbdem_get_stress(self.ctx, g.ctypes.data, st.ctypes.data, d.ctypes.data),
+                    "lbdem_get_stress")
+        return g.reshape(5, 3, 3), st[:9 * n].reshape(n, 3, 3), d[:3 * n].reshape(n, 3)
 
     def store_velocity(self):
         """The next LB step keeps every active node's velocity (read with velocity())."""

@@ -125,6 +125,23 @@ static void dump(const std::string& path, LB& lb, DEM& dem, IO& io) {
         r[LBDEM_B_L] = B.L; r[LBDEM_B_L0] = B.L0; r[LBDEM_B_K] = B.k; r[LBDEM_B_R] = B.r; r[LBDEM_B_M] = B.m;
         std::fwrite(r, 8, LBDEM_B_REC, fp);
     }
+    // stress diagnostics in the layout of lbdem_get_stress: 45 global values, then STotal (9) and
+    // disp (3) of every particle
+    const tMat* g[5] = {&dem.totalStress, &dem.hydroStress, &dem.lubStress, &dem.contactStress, &dem.electroStress};
+    for (const tMat* t : g) {
+        const double m[9] = {t->m00, t->m01, t->m02, t->m10, t->m11, t->m12, t->m20, t->m21, t->m22};
+        std::fwrite(m, 8, 9, fp);
+    }
+    for (size_t i = 0; i < dem.particles.size(); ++i) {
+        const tMat& t = dem.particles[i].STotal;
+        const double m[9] = {t.m00, t.m01, t.m02, t.m10, t.m11, t.m12, t.m20, t.m21, t.m22};
+        std::fwrite(m, 8, 9, fp);
+    }
+    for (size_t i = 0; i < dem.particles.size(); ++i) {
+        const tVect& d = dem.particles[i].disp;
+        const double v[3] = {d.x, d.y, d.z};
+        std::fwrite(v, 8, 3, fp);
+    }
     std::fclose(fp);
 }
 

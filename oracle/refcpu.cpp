@@ -58,10 +58,29 @@ struct Units {  // measureUnits (node.h:192-225, node.cpp:512-525)
     }
 };
 
+struct T9 {  // tinyMatrix (myvector.h), m00 m01 m02 m10 ... m22
+    double m[9] = {0, 0, 0, 0, 0, 0, 0, 0, 0};
+    void reset() { for (double& v : m) v = 0.0; }
+    // += -a (x) f, the stress of a force f at arm a (DEM.cpp:1097-1105 and the other pair loops)
+    void addArm(double ax, double ay, double az, double fx, double fy, double fz) {
+        m[0] += -ax * fx; m[1] += -ax * fy; m[2] += -ax * fz;
+        m[3] += -ay * fx; m[4] += -ay * fy; m[5] += -ay * fz;
+        m[6] += -az * fx; m[7] += -az * fy; m[8] += -az * fz;
+    }
+    // += a symmetric stresslet given as xx yy zz yz zx xy (DEM.cpp:1325-1333)
+    void addSym(const double* s) {
+        m[0] += s[0]; m[1] += s[5]; m[2] += s[4];
+        m[3] += s[5]; m[4] += s[1]; m[5] += s[3];
+        m[6] += s[4]; m[7] += s[3]; m[8] += s[2];
+    }
+};
+
 struct Particle {  // particle (elmt.h:15-158), fields used on the path
     unsigned size = 1, particleIndex = 0;
     double r = 1, m = 1, I = 1, _m = 1, _I = 1, dens = 1;
     V3 x0, x1, w1, x0_lst;
+    V3 x_i, disp;                             // position at the start (or at objWaitTime) and x_i - x0
+    T9 SHydro, SLub, SContact, SRep, STotal;  // stress diagnostics (DEM.cpp:655-771)
     V3 FHydro, FContact, FTotal, FLub, FGrav, FRep, MHydro, MContact, MTotal, MLub;
     double fluidVolume = 0;
     int list[MAXL], _list[MAXL];
@@ -129,6 +148,8 @@ struct lbdem_ctx {
     double lastExtraMass = 0, lastSurplus = 0;
     // ---------------- DEM (DEM.h:22-118) ----------------
     double demSize[3] = {1, 1, 1};
+    double vol = 1;  // volume the stresses are divided by (DEM.cpp:212-216)
+    T9 totalStress, contactStress, lubStress, electroStress, hydroStress;  // DEM.cpp:655-771
     std::vector<Particle> particles;
     std::vector<Object> objects;
     std::vector<Body> bodies;
@@ -306,6 +327,7 @@ struct lbdem_ctx {
         demSize[0] = lbPhysicalSize[0];
         demSize[1] = freeSurface ? fluidHeight : lbPhysicalSize[1];
         demSize[2] = lbPhysicalSize[2];
+        vol = freeSurface ? demSize[0] * fluidHeight * demSize[2] : demSize[0] * demSize[1] * demSize[2];
         const double partDensity = prm.density, objDensity = prm.objectDensity;
         demF = lbF * unit.Accel;
         unsigned globalIndex = 0;
@@ -319,6 +341,7 @@ struct lbdem_ctx {
             P.FTotal.reset(); P.FRep.reset(); P.FHydro.reset(); P.FContact.reset(); P.FLub.reset();
             P.FGrav = demF * P.m;
             P.MTotal.reset(); P.MHydro.reset(); P.MContact.reset(); P.MLub.reset();
+            P.x_i = P.x0;
             globalIndex++;
         }
         for (auto& O : objects) {
@@ -610,7 +633,7 @@ struct lbdem_ctx {
 
     // ================= LB step (LB.cpp:400-418, 1261-1544, 1901-1978) =================
     void computeHydroForces() {  // LB.cpp:1901-1978
-        for (auto& P : particles) { P.FHydro = V3(); P.MHydro = V3(); P.fluidVolume = 0.0; }
+        for (auto& P : particles) { P.FHydro = V3(); P.MHydro = V3(); P.fluidVolume = 0.0; P.SHydro.reset(); }
         for (uint32_t index : activeNodes) {
             setH(index, V3());
             if (!p[index]) continue;
@@ -622,6 +645,9 @@ struct lbdem_ctx {
             P.fluidVolume += mass[index];
             P.FHydro += 1.0 * diffVel;
             P.MHydro += 1.0 * radius.cross(diffVel);
+            const double rr[3] = {radius.x, radius.y, radius.z}, dd[3] = {diffVel.x, diffVel.y, diffVel.z};
+            for (int a = 0; a < 3; ++a)  // LB.cpp:1951-1959
+                for (int b = 0; b < 3; ++b) P.SHydro.m[3 * a + b] += 1.0 * rr[a] * unit.Length * dd[b] * unit.Force;
         }
         for (auto& P : particles) { P.FHydro *= unit.Force; P.MHydro *= unit.Torque; P.fluidVolume *= unit.Volume; }
     }
@@ -1028,7 +1054,7 @@ struct lbdem_ctx {
 
     // Jeffrey-Onishi / Nguyen-Ladd lubrication coefficients shared by the particle-particle
     // and object-particle routines (DEM.cpp:1181-1257, 2349-2416)
-    struct LubOut { V3 F, T1, T2; };
+    struct LubOut { V3 F, T1, T2; double S1[6] = {0, 0, 0, 0, 0, 0}, S2[6] = {0, 0, 0, 0, 0, 0}; };  // S: xx yy zz yz zx xy
     LubOut pairLub(const V3& xa, const V3& xb, double ra, double rb, const V3& va, const V3& vb, const V3& wa, const V3& wb,
                    double beta1_num, double beta1_den, bool& active) const {
         LubOut o;
@@ -1044,39 +1070,59 @@ struct lbdem_ctx {
         active = dr < dmax(cut, dmax(lub_N, dmax(lub_T, lub_R)));
         if (!active) return o;
         double Xa, Ya, Yb1, Yb2, Yc11, Yc12, Yc21, Yc22, h_c, beta;
+        double Xg1, Xg2, Yg1, Yg2, Yh11, Yh12, Yh21, Yh22;  // stresslet coefficients (DEM.cpp:2362-2406)
         const double nu = prm.kinVisc;
         const double alpha = M_PI * nu * rad * deltat;
         const double beta1 = beta1_num / beta1_den;
         const double beta2 = beta1 * beta1, beta3 = beta1 * beta2;
         const double beta4 = std::pow((1 + beta1), 4.0);
+        const double beta5 = std::pow((1 + beta1), 5.0);
         if (dr < dmax(cut, lub_N)) {
             Xa = 4.0 * beta2 / beta4;
+            Xg1 = 12.0 * beta2 / beta5;
+            Xg2 = -12.0 * beta3 / beta5;
             h_c = dmax(dr, lub_N);
             Xa *= rad / h - rad / h_c;
+            Xg1 *= rad / h - rad / h_c;
+            Xg2 *= rad / h - rad / h_c;
             Xa *= 6.0 * alpha;
-        } else Xa = 0.0;
+            Xg1 *= 4.0 * alpha * rad;
+            Xg2 *= 4.0 * alpha * rad;
+        } else Xa = Xg1 = Xg2 = 0.0;
         if (dr < dmax(cut, lub_T)) {
             Ya = 8.0 * beta1 * (2.0 + beta1 + 2.0 * beta2) / (15.0 * beta4);
             Yb1 = -4.0 * beta1 * (4.0 + beta1) / (5.0 * beta4);
             Yb2 = 4.0 * beta1 * (4.0 * beta2 + beta1) / (5.0 * beta4);
+            Yg1 = 2.0 * beta1 * (4.0 - beta1 + 7.0 * beta2) / (5.0 * beta5);
+            Yg2 = -2.0 * beta2 * (4.0 * beta2 - beta1 + 7.0) / (5.0 * beta5);
             h_c = dmax(dr, lub_T);
             beta = alpha * std::log(h_c / h);
             Ya *= 6.0 * beta;
             Yb1 *= 4.0 * beta * rad;
             Yb2 *= 4.0 * beta * rad;
-        } else Ya = Yb1 = Yb2 = 0.0;
+            Yg1 *= 4.0 * beta * rad;
+            Yg2 *= 4.0 * beta * rad;
+        } else Ya = Yb1 = Yb2 = Yg1 = Yg2 = 0.0;
         if (dr < dmax(cut, lub_R)) {
             Yc11 = 16.0 * beta1 / (5.0 * beta4);
             Yc12 = 4.0 * beta2 / (5.0 * beta4);
             Yc21 = 4.0 * beta2 / (5.0 * beta4);
             Yc22 = 16.0 * beta3 / (5.0 * beta4);
+            Yh11 = 4.0 * beta1 * (2.0 - beta1) / (5.0 * beta5);
+            Yh12 = 2.0 * beta2 * (1.0 + 7.0 * beta1) / (5.0 * beta5);
+            Yh21 = 2.0 * beta2 * (beta1 + 7.0) / (5.0 * beta5);
+            Yh22 = 4.0 * beta2 * (2.0 * beta2 - beta1) / (5.0 * beta5);
             h_c = dmax(dr, lub_R);
             beta = alpha * std::log(h_c / h);
             Yc11 *= 8.0 * beta * rad * rad;
             Yc12 *= 8.0 * beta * rad * rad;
             Yc21 *= 8.0 * beta * rad * rad;
             Yc22 *= 8.0 * beta * rad * rad;
-        } else Yc11 = Yc12 = Yc21 = Yc22 = 0.0;
+            Yh11 *= 8.0 * beta * rad * rad;
+            Yh21 *= 8.0 * beta * rad * rad;
+            Yh12 *= 8.0 * beta * rad * rad;
+            Yh22 *= 8.0 * beta * rad * rad;
+        } else Yc11 = Yc12 = Yc21 = Yc22 = Yh11 = Yh12 = Yh21 = Yh22 = 0.0;
         x12 /= r12; y12 /= r12; z12 /= r12;
         const double um_r = umx * x12 + umy * y12 + umz * z12;
         const double w1_r = w1x * x12 + w1y * y12 + w1z * z12;
@@ -1099,6 +1145,33 @@ struct lbdem_ctx {
         T2y += -Yc21 * (w1y - w1_r * y12) - Yc22 * (w2y - w2_r * y12);
         T2z += -Yc21 * (w1z - w1_r * z12) - Yc22 * (w2z - w2_r * z12);
         o.F = V3(Fx, Fy, Fz); o.T1 = V3(T1x, T1y, T1z); o.T2 = V3(T2x, T2y, T2z);
+        // stresslets of the two spheres (DEM.cpp:2454-2477), as xx yy zz yz zx xy
+        double* S1 = o.S1;
+        double* S2 = o.S2;
+        S1[0] = Xg1 * um_r * x12 * x12 + Yg1 * (x12 * umx + umx * x12);
+        S1[1] = Xg1 * um_r * y12 * y12 + Yg1 * (y12 * umy + umy * y12);
+        S1[2] = Xg1 * um_r * z12 * z12 + Yg1 * (z12 * umz + umz * z12);
+        S1[3] = Xg1 * um_r * y12 * z12 + Yg1 * (y12 * umz + umy * z12);
+        S1[4] = Xg1 * um_r * z12 * x12 + Yg1 * (z12 * umx + umz * x12);
+        S1[5] = Xg1 * um_r * x12 * y12 + Yg1 * (x12 * umy + umx * y12);
+        S2[0] = -Xg2 * um_r * x12 * x12 - Yg2 * (x12 * umx + umx * x12);
+        S2[1] = -Xg2 * um_r * y12 * y12 - Yg2 * (y12 * umy + umy * y12);
+        S2[2] = -Xg2 * um_r * z12 * z12 - Yg2 * (z12 * umz + umz * z12);
+        S2[3] = -Xg2 * um_r * y12 * z12 - Yg2 * (y12 * umz + umy * z12);
+        S2[4] = -Xg2 * um_r * z12 * x12 - Yg2 * (z12 * umx + umz * x12);
+        S2[5] = -Xg2 * um_r * x12 * y12 - Yg2 * (x12 * umy + umx * y12);
+        S1[0] += -Yh11 * (x12 * w1xrx + w1xrx * x12) - Yh12 * (x12 * w2xrx + w2xrx * x12);
+        S1[1] += -Yh11 * (y12 * w1xry + w1xry * y12) - Yh12 * (y12 * w2xry + w2xry * y12);
+        S1[2] += -Yh11 * (z12 * w1xrz + w1xrz * z12) - Yh12 * (z12 * w2xrz + w2xrz * z12);
+        S1[3] += -Yh11 * (y12 * w1xrz + w1xry * z12) - Yh12 * (y12 * w2xrz + w2xry * z12);
+        S1[4] += -Yh11 * (z12 * w1xrx + w1xrz * x12) - Yh12 * (z12 * w2xrx + w2xrz * x12);
+        S1[5] += -Yh11 * (x12 * w1xry + w1xrx * y12) - Yh12 * (x12 * w2xry + w2xrx * y12);
+        S2[0] += -Yh21 * (x12 * w1xrx + w1xrx * x12) - Yh22 * (x12 * w2xrx + w2xrx * x12);
+        S2[1] += -Yh21 * (y12 * w1xry + w1xry * y12) - Yh22 * (y12 * w2xry + w2xry * y12);
+        S2[2] += -Yh21 * (z12 * w1xrz + w1xrz * z12) - Yh22 * (z12 * w2xrz + w2xrz * z12);
+        S2[3] += -Yh21 * (y12 * w1xrz + w1xry * z12) - Yh22 * (y12 * w2xrz + w2xry * z12);
+        S2[4] += -Yh21 * (z12 * w1xrx + w1xrz * x12) - Yh22 * (z12 * w2xrx + w2xrz * x12);
+        S2[5] += -Yh21 * (x12 * w1xry + w1xrx * y12) - Yh22 * (x12 * w2xry + w2xrx * y12);
         return o;
     }
     void particleLubrication() {  // DEM.cpp:2302-2521
@@ -1112,6 +1185,7 @@ struct lbdem_ctx {
                 LubOut o = pairLub(A.x0, B.x0, A.r, B.r, A.x1, B.x1, A.w1, B.w1, B.r, A.r, act);
                 if (!act) continue;
                 A.FLub += o.F; B.FLub -= o.F; A.MLub += o.T1; B.MLub += o.T2;
+                A.SLub.addSym(o.S1); B.SLub.addSym(o.S2);
             }
         }
     }
@@ -1130,15 +1204,20 @@ struct lbdem_ctx {
                 const double h = dmax(dr, cut);
                 if (!(dr < dmax(cut, dmax(lub_N, dmax(lub_T, lub_R))))) continue;
                 const double alpha = M_PI * prm.kinVisc * rad * deltat;
-                double Xa, Ya, Yb1, Yc11, h_c, beta;
-                if (dr < dmax(cut, lub_N)) { Xa = 1.0; h_c = dmax(dr, lub_N); Xa *= rad / h - rad / h_c; Xa *= 6.0 * alpha; }
-                else Xa = 0.0;
+                double Xa, Ya, Yb1, Yc11, h_c, beta, Xg1, Yg1, Yh11;
+                if (dr < dmax(cut, lub_N)) {
+                    Xa = 1.0; Xg1 = 3.0 / 2.0; h_c = dmax(dr, lub_N);
+                    Xa *= rad / h - rad / h_c; Xg1 *= rad / h - rad / h_c;
+                    Xa *= 6.0 * alpha; Xg1 *= 4.0 * alpha * rad;
+                } else Xa = Xg1 = 0.0;
                 if (dr < dmax(cut, lub_T)) {
-                    Ya = 8.0 / 15.0; Yb1 = -1.0 / 5.0; h_c = dmax(dr, lub_T); beta = alpha * std::log(h_c / h);
-                    Ya *= 6.0 * beta; Yb1 *= 4.0 * beta * rad;
-                } else Ya = Yb1 = 0.0;
-                if (dr < dmax(cut, lub_R)) { Yc11 = 2.0 / 5.0; h_c = dmax(dr, lub_R); beta = alpha * std::log(h_c / h); Yc11 *= 8.0 * beta * rad * rad; }
-                else Yc11 = 0.0;
+                    Ya = 8.0 / 15.0; Yb1 = -1.0 / 5.0; Yg1 = 7.0 / 10.0; h_c = dmax(dr, lub_T); beta = alpha * std::log(h_c / h);
+                    Ya *= 6.0 * beta; Yb1 *= 4.0 * beta * rad; Yg1 *= 4.0 * beta * rad;
+                } else Ya = Yb1 = Yg1 = 0.0;
+                if (dr < dmax(cut, lub_R)) {
+                    Yc11 = 2.0 / 5.0; Yh11 = -1.0 / 10.0; h_c = dmax(dr, lub_R); beta = alpha * std::log(h_c / h);
+                    Yc11 *= 8.0 * beta * rad * rad; Yh11 *= 8.0 * beta * rad * rad;
+                } else Yc11 = Yh11 = 0.0;
                 x12 /= r12; y12 /= r12; z12 /= r12;
                 const double um_r = umx * x12 + umy * y12 + umz * z12;
                 const double w1_r = w1x * x12 + w1y * y12 + w1z * z12;
@@ -1151,6 +1230,20 @@ struct lbdem_ctx {
                 double T1x = Yb1 * umxrx, T1y = Yb1 * umxry, T1z = Yb1 * umxrz;
                 T1x += -Yc11 * (w1x - w1_r * x12); T1y += -Yc11 * (w1y - w1_r * y12); T1z += -Yc11 * (w1z - w1_r * z12);
                 P.FLub += V3(Fx, Fy, Fz); P.MLub += V3(T1x, T1y, T1z); W.FLub -= V3(Fx, Fy, Fz);
+                double S1[6];  // DEM.cpp:2623-2634 (the wall's own bracketing of the Xg1 term)
+                S1[0] = Xg1 * (um_r * x12 * x12) + Yg1 * (x12 * umx + umx * x12);
+                S1[1] = Xg1 * (um_r * y12 * y12) + Yg1 * (y12 * umy + umy * y12);
+                S1[2] = Xg1 * (um_r * z12 * z12) + Yg1 * (z12 * umz + umz * z12);
+                S1[3] = Xg1 * (um_r * y12 * z12) + Yg1 * (y12 * umz + umy * z12);
+                S1[4] = Xg1 * (um_r * z12 * x12) + Yg1 * (z12 * umx + umz * x12);
+                S1[5] = Xg1 * (um_r * x12 * y12) + Yg1 * (x12 * umy + umx * y12);
+                S1[0] += -Yh11 * (x12 * w1xrx + w1xrx * x12);
+                S1[1] += -Yh11 * (y12 * w1xry + w1xry * y12);
+                S1[2] += -Yh11 * (z12 * w1xrz + w1xrz * z12);
+                S1[3] += -Yh11 * (y12 * w1xrz + w1xry * z12);
+                S1[4] += -Yh11 * (z12 * w1xrx + w1xrz * x12);
+                S1[5] += -Yh11 * (x12 * w1xry + w1xrx * y12);
+                P.SLub.addSym(S1);
             }
     }
     // tangential spring-dashpot with Coulomb cap, shared by the wall and fictional-lid contacts
@@ -1209,6 +1302,8 @@ struct lbdem_ctx {
                 V3 Fc, Mc;
                 wallLikeContact(P, x1w, y1w, z1w, Fc, Mc);
                 P.FContact += Fc; W.FContact -= Fc; P.MContact += Mc;
+                P.SContact.addArm(x1w, y1w, z1w, Fc.x, Fc.y, Fc.z);  // DEM.cpp:2110-2148
+                contactStress.addArm(x1w, y1w, z1w, Fc.x, Fc.y, Fc.z);
             }
     }
     void fictionalWallContacts() {  // DEM.cpp:2157-2296
@@ -1268,6 +1363,8 @@ struct lbdem_ctx {
                 O.FContact -= V3(dfx, dfy, dfz);
                 P.MContact += V3(P.r * dtx, P.r * dty, P.r * dtz);
                 O.MContact += V3(O.r * dtx, O.r * dty, O.r * dtz);
+                P.SContact.addArm(x1w, y1w, z1w, dfx, dfy, dfz);  // DEM.cpp:1097-1135
+                contactStress.addArm(x1w, y1w, z1w, dfx, dfy, dfz);
             }
     }
     void objectParticleLubrication() {  // DEM.cpp:1140-1340
@@ -1277,6 +1374,7 @@ struct lbdem_ctx {
                 LubOut o = pairLub(P.x0, O.x0, P.r, O.r, P.x1, O.x1, P.w1, O.w1, O.r, P.r, act);
                 if (!act) continue;
                 P.FLub += o.F; O.FLub -= o.F; P.MLub += o.T1; O.MLub += o.T2;
+                P.SLub.addSym(o.S1);
             }
     }
     void particleContacts() {  // DEM.cpp:1487-1791
@@ -1334,6 +1432,9 @@ struct lbdem_ctx {
                 B.FContact -= V3(dfx, dfy, dfz);
                 A.MContact += V3(A.r * dtx, A.r * dty, A.r * dtz);
                 B.MContact += V3(B.r * dtx, B.r * dty, B.r * dtz);
+                A.SContact.addArm(x12, y12, z12, dfx, dfy, dfz);  // DEM.cpp:1715-1783
+                B.SContact.addArm(x12, y12, z12, dfx, dfy, dfz);
+                contactStress.addArm(x12, y12, z12, dfx, dfy, dfz);
             }
         }
     }
@@ -1360,13 +1461,18 @@ struct lbdem_ctx {
                 }
                 A.FRep += V3(frx, fry, frz);
                 B.FRep -= V3(frx, fry, frz);
+                A.SRep.addArm(x12, y12, z12, frx, fry, frz);  // DEM.cpp:1451-1479
+                B.SRep.addArm(x12, y12, z12, frx, fry, frz);
+                electroStress.addArm(x12, y12, z12, frx, fry, frz);
             }
         }
     }
     void evaluateForces() {  // DEM.cpp:655-796
+        totalStress.reset(); contactStress.reset(); lubStress.reset(); electroStress.reset(); hydroStress.reset();
         for (auto& P : particles) {
             P.FTotal.reset(); P.MTotal.reset(); P.FContact.reset(); P.MContact.reset();
             P.FLub.reset(); P.MLub.reset(); P.FRep.reset();
+            P.SLub.reset(); P.SContact.reset(); P.SRep.reset(); P.STotal.reset();
         }
         for (auto& O : objects) {
             O.FTotal.reset(); O.FContact.reset(); O.FLub.reset(); O.FSpring.reset();
@@ -1386,9 +1492,18 @@ struct lbdem_ctx {
         particleContacts();
         particleElectro();
         wallContacts();
-        for (auto& P : particles) {
+        for (auto& P : particles) {  // DEM.cpp:745-764
+            for (int e = 0; e < 9; ++e) {
+                lubStress.m[e] += P.SLub.m[e];
+                hydroStress.m[e] += P.SHydro.m[e];
+                P.STotal.m[e] += P.SHydro.m[e] + P.SLub.m[e] + P.SContact.m[e] + P.SRep.m[e];
+            }
             P.FTotal += P.FHydro + P.FContact + P.FLub + P.FGrav + P.FRep;
             P.MTotal += P.MHydro + P.MContact + P.MLub;
+        }
+        for (int e = 0; e < 9; ++e) {  // DEM.cpp:766-771
+            lubStress.m[e] /= vol; contactStress.m[e] /= vol; electroStress.m[e] /= vol; hydroStress.m[e] /= vol;
+            totalStress.m[e] += contactStress.m[e] + lubStress.m[e] + electroStress.m[e] + hydroStress.m[e];
         }
         for (auto& W : walls) W.FTotal += W.FHydro + W.FContact + W.FLub + W.FRep;
         if (int(demTimeStep) <= prm.objWaitTime)
@@ -1452,7 +1567,10 @@ struct lbdem_ctx {
             P.w1.x += deltat * P.MTotal.x * P._I;
             P.w1.y += deltat * P.MTotal.y * P._I;
             P.w1.z += deltat * P.MTotal.z * P._I;
+            P.disp = P.x_i - P.x0;  // DEM.cpp:2705
         }
+        if (int(demTimeStep) == prm.objWaitTime)  // DEM.cpp:2721-2725
+            for (auto& P : particles) P.x_i = P.x0;
         if (int(demTimeStep) >= prm.objWaitTime)
             for (auto& B : bodies) {  // DEM.cpp:2730-2740
                 B.x0.x += deltat * B.x1.x + deltat * deltat * B.FTotal.x * B._m;
@@ -1700,6 +1818,23 @@ int lbdem_get_bodies(lbdem_ctx* c, double* rec) {
         put3(r + LBDEM_B_X0, B.x0); put3(r + LBDEM_B_X1, B.x1); put3(r + LBDEM_B_FSPRING, B.FSpring);
         put3(r + LBDEM_B_FTOTAL, B.FTotal); put3(r + LBDEM_B_FGRAV, B.FGrav);
         r[LBDEM_B_L] = B.L; r[LBDEM_B_L0] = B.L0; r[LBDEM_B_K] = B.k; r[LBDEM_B_R] = B.r; r[LBDEM_B_M] = B.m;
+    }
+    return 0;
+}
+int lbdem_set_stress(lbdem_ctx* c, int) {  // the oracle computes the stresses on every step
+    if (!c) return LBDEM_E_ARG;
+    return c->initialised ? c->fail(LBDEM_E_STATE, "lbdem_set_stress after lbdem_init") : 0;
+}
+int lbdem_get_stress(lbdem_ctx* c, double* global, double* stotal, double* disp) {
+    NEED_INIT;
+    if (global) {
+        const T9* t[5] = {&c->totalStress, &c->hydroStress, &c->lubStress, &c->contactStress, &c->electroStress};
+        for (int k = 0; k < 5; ++k) for (int e = 0; e < 9; ++e) global[9 * k + e] = t[k]->m[e];
+    }
+    for (size_t i = 0; i < c->particles.size(); ++i) {
+        const Particle& P = c->particles[i];
+        if (stotal) for (int e = 0; e < 9; ++e) stotal[9 * i + e] = P.STotal.m[e];
+        if (disp) put3(disp + 3 * i, P.disp);
     }
     return 0;
 }

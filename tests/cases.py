@@ -309,25 +309,37 @@ def read_dump(path: str) -> Dict[str, object]:
     out["totalMassInit"], out["totalMass"], out["demTime"], out["realTime"] = sc[0], sc[1], sc[2], sc[3]
     nb = int(take(np.int32, 1)[0]) if off < len(buf) else 0  # dumps of older drivers end here
     out["bodies"] = take(np.float64, nb * lbdem.B["REC"]).reshape(nb, lbdem.B["REC"])
+    if off < len(buf):  # stress diagnostics (drivers that write them)
+        npart = out["particles"].shape[0]
+        out["stress"] = take(np.float64, 45)
+        out["stotal"] = take(np.float64, npart * 9).reshape(npart, 9)
+        out["disp"] = take(np.float64, npart * 3).reshape(npart, 3)
     return out
 
 
-def open_sim(lib: "lbdem.Library", case_dir: str, precision: int = lbdem.PREC_FP64) -> "lbdem.Simulation":
+def open_sim(lib: "lbdem.Library", case_dir: str, precision: int = lbdem.PREC_FP64,
+             stress: bool = False) -> "lbdem.Simulation":
     vals, prm, ps, obs = lbdem.load_case(case_dir, precision=precision)
     sim = lbdem.Simulation(lib, prm, ps, obs)
     if int(vals.get("lbRestart", 0)):
         sim.set_restart(*lbdem.read_recycle(os.path.join(case_dir, str(vals["lbRestartFile"]))))
+    if stress:
+        sim.set_stress()
     sim.init()
     return sim
 
 
-def state(sim: "lbdem.Simulation", with_f: bool = True) -> Dict[str, object]:
+def state(sim: "lbdem.Simulation", with_f: bool = True, stress: bool = False) -> Dict[str, object]:
     t, p, s = sim.types()
     f, m, n = sim.fields(with_f)
     cnt, lst = sim.neighbors()
-    return dict(type=t, pflag=p, solid=s, f=f, mass=m, n=n, particles=sim.particles(), objects=sim.objects(),
-                walls=sim.walls(), bodies=sim.bodies(), ncount=cnt, nlist=lst, totalMass=sim.total_mass(),
-                info=sim.info())
+    out = dict(type=t, pflag=p, solid=s, f=f, mass=m, n=n, particles=sim.particles(), objects=sim.objects(),
+               walls=sim.walls(), bodies=sim.bodies(), ncount=cnt, nlist=lst, totalMass=sim.total_mass(),
+               info=sim.info())
+    if stress:  # in the layout of the reference dumps
+        g, st, d = sim.stress()
+        out.update(stress=g.reshape(-1), stotal=st.reshape(-1, 9), disp=d)
+    return out
 
 
 def _n_int(x):

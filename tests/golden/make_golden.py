@@ -167,8 +167,9 @@ def main():
                 if dump["N"] > SAMPLE_ABOVE:
                     rng = np.random.default_rng(1000 + s)
                     sel = np.sort(rng.choice(sel, size=min(SAMPLE_NODES, sel.size), replace=False))
+                extra = {k: dump[k] for k in ("stress", "stotal", "disp") if k in dump}
                 np.savez_compressed(
-                    out, type=dump["type"], pflag=dump["pflag"], solid=dump["solid"], active=sel,
+                    out, **extra, type=dump["type"], pflag=dump["pflag"], solid=dump["solid"], active=sel,
                     sampled=np.array(dump["N"] > SAMPLE_ABOVE),
                     f_active=dump["f"][:, sel], n_active=dump["n"][sel], mass_active=dump["mass"][sel],
                     particles=dump["particles"], objects=dump["objects"], walls=dump["walls"],

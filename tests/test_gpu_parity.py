@@ -7,6 +7,8 @@ Tolerances (absolute unless marked relative; populations are O(1e-1), masses O(1
                   particle records 1e-4 relative, impactor 1e-6 relative
   FP16 storage  : flags identical over the window; populations 5e-5, mass 2e-4,
                   particle records 1e-3 relative
+  Stress diagnostics (each of the five global tensors, STotal, relative to their largest entry):
+                  FP64 1e-9, FP32 2e-2; disp 1e-10 (FP64) and 1e-4 (FP32) absolute
 The reference's own golden fixtures are also compared with the FP64 library directly."""
 import json
 import os
@@ -209,6 +211,49 @@ def test_restart_viscosities_match_oracle(oracle_lib, gpu_lib, case_root):
     b.close()
 
 
+STRESS_TOL = {lbdem.PREC_FP64: (1e-9, 1e-10), lbdem.PREC_FP32: (2e-2, 1e-4)}
+STRESS_NAMES = ("totalStress", "hydroStress", "lubStress", "contactStress", "electroStress")
+
+
+def compare_stress(A, B, prec, where):
+    """A, B: (global 5x3x3, STotal n x 3 x 3, disp n x 3), A the oracle or the reference."""
+    tol, dtol = STRESS_TOL[prec]
+    for k, nm in enumerate(STRESS_NAMES):
+        assert rel(np.asarray(A[0][k]), np.asarray(B[0][k])) <= tol, f"{where}: {nm} {A[0][k]} vs {B[0][k]}"
+    assert rel(np.asarray(A[1]), np.asarray(B[1])) <= tol, f"{where}: STotal"
+    if np.asarray(A[2]).size:
+        assert np.abs(np.asarray(A[2]) - np.asarray(B[2])).max() <= dtol, f"{where}: disp"
+
+
+@pytest.mark.parametrize("prec", [lbdem.PREC_FP64, lbdem.PREC_FP32])
+@pytest.mark.parametrize("name", ["suspension", "cells", "startup", "units", "body_cubic", "periodic"])
+def test_stress_diagnostics_match_oracle(oracle_lib, gpu_lib, case_root, name, prec):
+    """SHydro (the stresslet pass after the fused kernel), the pair, wall and impactor stresses of
+    the force kernel and the displacement of the integration, against the oracle every 10 steps."""
+    d = cases.make_case(case_root, name)
+    a = cases.open_sim(oracle_lib, d, stress=True)
+    b = cases.open_sim(gpu_lib, d, precision=prec, stress=True)
+    print(f"case={name} prec={prec} stress", flush=True)
+    for step in range(10, 41, 10):
+        a.cycle(10)
+        b.cycle(10)
+        b.synchronize()
+        A, B = a.stress(), b.stress()
+        assert np.abs(A[0][1]).max() > 0 or name == "periodic", "the window has hydrodynamic stress"
+        compare_stress(A, B, prec, f"{name} step {step}")
+    a.close()
+    b.close()
+
+
+def test_stress_needs_enabling(gpu_lib, case_root):
+    sim = cases.open_sim(gpu_lib, cases.make_case(case_root, "suspension"))
+    with pytest.raises(lbdem.LbdemError):
+        sim.stress()
+    with pytest.raises(lbdem.LbdemError):
+        sim.set_stress()
+    sim.close()
+
+
 GOLD = os.path.join(os.path.dirname(os.path.abspath(__file__)), "golden")
 
 
@@ -220,13 +265,19 @@ def test_fp64_matches_reference_fixtures(gpu_lib, case_root, name):
     with open(os.path.join(GOLD, "manifest.json")) as fh:
         steps = sorted(m["step"] for m in json.load(fh).values() if m["case"] == name)
     d = cases.make_case(case_root, name)
-    sim = cases.open_sim(gpu_lib, d, precision=lbdem.PREC_FP64)
+    with np.load(os.path.join(GOLD, f"{name}_s{steps[0]}.npz"), allow_pickle=False) as g0:
+        stress = "stress" in g0.files
+    sim = cases.open_sim(gpu_lib, d, precision=lbdem.PREC_FP64, stress=stress)
     done = 0
     for step in steps:
         sim.cycle(step - done)
         done = step
         g = np.load(os.path.join(GOLD, f"{name}_s{step}.npz"), allow_pickle=False)
         st = cases.state(sim)
+        if stress:
+            n = g["particles"].shape[0]
+            compare_stress((g["stress"].reshape(5, 3, 3), g["stotal"].reshape(n, 3, 3), g["disp"]), sim.stress(),
+                           lbdem.PREC_FP64, f"{name} step {step} (reference)")
         assert np.array_equal(st["type"], g["type"]) and np.array_equal(st["pflag"], g["pflag"])
         assert np.array_equal(st["solid"], g["solid"])
         act = g["active"]

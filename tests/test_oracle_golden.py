@@ -1,6 +1,7 @@
 """The FP64 CPU oracle against golden fixtures written by the reference program itself
 (tests/golden/make_golden.py): node types, particle flags, solid indices, populations, masses,
-particle / impactor / wall records, Verlet lists and total fluid mass must agree bit for bit."""
+particle / impactor / wall records, Verlet lists, total fluid mass and, where the fixture holds
+them, the stress diagnostics must agree bit for bit."""
 import json
 import os
 
@@ -22,13 +23,13 @@ for key, meta in MANIFEST.items():
 @pytest.mark.parametrize("name", sorted(BY_CASE))
 def test_oracle_matches_reference_fixtures(oracle_lib, case_root, name):
     d = cases.make_case(case_root, name)
-    sim = cases.open_sim(oracle_lib, d)
+    sim = cases.open_sim(oracle_lib, d, stress=True)
     done = 0
     for step in sorted(BY_CASE[name]):
         sim.cycle(step - done)
         done = step
         g = np.load(os.path.join(GOLD, f"{name}_s{step}.npz"), allow_pickle=False)
-        st = cases.state(sim)
+        st = cases.state(sim, stress=True)
         assert np.array_equal(st["type"], g["type"]), f"{name} step {step}: node types"
         assert np.array_equal(st["pflag"], g["pflag"]), f"{name} step {step}: particle flags"
         assert np.array_equal(st["solid"], g["solid"]), f"{name} step {step}: solid indices"
@@ -47,6 +48,9 @@ def test_oracle_matches_reference_fixtures(oracle_lib, case_root, name):
             assert np.array_equal(st["nlist"][p, :cnt[p]], g["nlist"][p, :cnt[p]]), f"{name} step {step}: list of {p}"
         assert st["totalMass"] == float(g["totalMass"]), f"{name} step {step}: total fluid mass"
         assert st["info"]["totalMassInit"] == float(g["totalMassInit"])
+        if "stress" in g:
+            for k in ("stress", "stotal", "disp"):
+                assert np.array_equal(st[k], g[k]), f"{name} step {step}: {k}"
 
 
 def test_fixtures_exercise_every_event(oracle_lib, case_root):
