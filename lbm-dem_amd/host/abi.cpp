@@ -42,6 +42,8 @@ Api Api::load(const std::string& path) {
     bind(a.handle, path, "lbdem_total_mass", a.total_mass);
     bind(a.handle, path, "lbdem_set_restart_fluid", a.set_restart_fluid);
     bind(a.handle, path, "lbdem_store_velocity", a.store_velocity);
+    bind(a.handle, path, "lbdem_set_stress", a.set_stress);
+    bind(a.handle, path, "lbdem_get_stress", a.get_stress);
     bind(a.handle, path, "lbdem_get_velocity", a.get_velocity);
     return a;
 }

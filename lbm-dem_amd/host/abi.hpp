@@ -36,6 +36,8 @@ struct Api {
     int (*set_restart_fluid)(lbdem_ctx*, const int*, const uint8_t*, int, const double*) = nullptr;
     int (*store_velocity)(lbdem_ctx*) = nullptr;
     int (*get_velocity)(lbdem_ctx*, double*, double*) = nullptr;
+    int (*set_stress)(lbdem_ctx*, int) = nullptr;
+    int (*get_stress)(lbdem_ctx*, double*, double*, double*) = nullptr;
 
     void* handle = nullptr;
     std::string path;

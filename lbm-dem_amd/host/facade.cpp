@@ -13,6 +13,14 @@
 namespace lbdem {
 
 void tVect::print(std::ostream& os) const { os << x << " " << y << " " << z << " "; }
+void tMat::print(std::ostream& os) const {
+    os << m00 << " " << m11 << " " << m22 << " " << (m01 + m10) / 2.0 << " " << (m12 + m21) / 2.0 << " "
+       << (m02 + m20) / 2.0 << " ";
+}
+void tMat::printNormal(std::ostream& os) const { os << m00 << " " << m11 << " " << m22 << " "; }
+void tMat::printShear(std::ostream& os) const {
+    os << (m01 + m10) / 2.0 << " " << (m12 + m21) / 2.0 << " " << (m02 + m20) / 2.0 << " ";
+}
 void tVect::printFixedLine(std::ostream& os) const {
     os << std::setprecision(8) << std::fixed << x << " " << y << " " << z << "\n";
 }
@@ -50,6 +58,7 @@ void Context::create() {
 void Context::initialise() {
     create();
     if (initialised_) return;
+    if (stress) check(api_.set_stress(ctx_, 1), "lbdem_set_stress");
     check(api_.init(ctx_), "lbdem_init");
     initialised_ = true;
 }
@@ -66,6 +75,7 @@ void parseConfigFile(IO& io, DEM& dem, LB& lb, ConfigFile& cfg, CommandLine& cl)
     io.fluidLagrangianExpTime = parse_value(cfg, cl, "fluidLagrangianExpTime", 0.0);
     io.fluidRecycleExpTime = parse_value(cfg, cl, "fluidRecycleExpTime", 0.0);
     io.partExpTime = parse_value(cfg, cl, "partExpTime", 0.0);
+    io.partRecycleExpTime = parse_value(cfg, cl, "partRecycleExpTime", 0.0);
     io.demSolve = parse_value(cfg, cl, "demSolve", 0);
     io.lbmSolve = parse_value(cfg, cl, "lbSolve", 0);
     lb.freeSurface = parse_value(cfg, cl, "freeSurfaceSolve", 0);
@@ -76,6 +86,9 @@ void parseConfigFile(IO& io, DEM& dem, LB& lb, ConfigFile& cfg, CommandLine& cl)
     p.demInitialRepeat = dem.demInitialRepeat;
     p.maxTime = io.maxTime;
     p.demSolve = io.demSolve;
+    // the stress files of the screen interval and the particle-recycle files need the stress
+    // diagnostics, which the reference computes on every DEM step
+    lb.context().stress = io.demSolve && (io.screenExpTime > 0 || io.partRecycleExpTime > 0);
     p.lbSolve = io.lbmSolve;
     p.freeSurface = lb.freeSurface;
     p.forceField = lb.forceField;
@@ -326,6 +339,16 @@ void DEM::refresh() {
         P.fluidVolume = r[LBDEM_P_FLUIDVOL]; P.xiw = rec3(r, LBDEM_P_XIW); P.xio = rec3(r, LBDEM_P_XIO);
         P.r = r[LBDEM_P_R]; P.m = r[LBDEM_P_M];
     }
+    if (c_.stress) {  // DEM.cpp:745-771, 2705
+        std::vector<double> g(45), st(size_t(std::max(in.nParticles, 1)) * 9), dp(size_t(std::max(in.nParticles, 1)) * 3);
+        c_.check(c_.api().get_stress(c_.ctx(), g.data(), st.data(), dp.data()), "lbdem_get_stress");
+        tMat* t[5] = {&totalStress, &hydroStress, &lubStress, &contactStress, &electroStress};
+        for (int k = 0; k < 5; ++k) t[k]->set(&g[9 * size_t(k)]);
+        for (int i = 0; i < in.nParticles; ++i) {
+            particles[i].STotal.set(&st[9 * size_t(i)]);
+            particles[i].disp = tVect(dp[3 * size_t(i)], dp[3 * size_t(i) + 1], dp[3 * size_t(i) + 2]);
+        }
+    }
     rec.assign(size_t(std::max(in.nObjects, 1)) * LBDEM_O_REC, 0.0);
     c_.check(c_.api().get_objects(c_.ctx(), rec.data()), "lbdem_get_objects");
     objects.resize(in.nObjects);
@@ -391,6 +414,7 @@ void IO::initialize() {
     objectFileFormat = partDirectory + "/object%04u.vtk";
     bodyFileFormat = partDirectory + "/body%04u.vtk";
     springFileFormat = partDirectory + "/spring%04u.vtk";
+    partRecycleFileFormat = partDirectory + "/partRecycle%04u.dat";
     exportFileName = workDirectory + "/export.dat";
     if (demSolve) {
         objectPositionFileName = workDirectory + "/objPosition.dat";
@@ -409,6 +433,16 @@ void IO::initialize() {
                     "0-FTotal(x) 0-FTotal(y) 0-FTotal(z) 1-FTotal(x) 1-FTotal(y) 1-FTotal(z) 2-FTotal(x) "
                     "2-FTotal(y) 2-FTotal(z) 3-FTotal(x) 3-FTotal(y) 3-FTotal(z) 4-FTotal(x) 4-FTotal(y) "
                     "4-FTotal(z) 5-FTotal(x) 5-FTotal(y) 5-FTotal(z) \n");
+        stressTensorFileName = workDirectory + "/stressTensor.dat";
+        append_line(stressTensorFileName, "xx yy zz xy yz zx \n");
+        normalStressContFileName = workDirectory + "/normalStressCont.dat";
+        append_line(normalStressContFileName,
+                    "hydro-xx hydro-yy hydro-zz lub-xx lub-yy lub-zz contact-xx contact-yy contact-zz rep-xx rep-yy "
+                    "rep-zz \n");
+        shearStressContFileName = workDirectory + "/shearStressCont.dat";
+        append_line(shearStressContFileName,
+                    "hydro-xy hydro-yz hydro-zx lub-xy lub-yz lub-zx contact-xy contact-yz contact-zx rep-xy rep-yz "
+                    "rep-zx \n");
     }
     if (lbmSolve) {
         massFileName = workDirectory + "/mass.dat";
@@ -417,6 +451,7 @@ void IO::initialize() {
     lastScreenExp = 0;
     lastFluidRecycleExp = lastFluidExp = lastFluidLagrangianExp = 0;
     lastVelExp = lastBoundExp = lastObjectExp = lastBodyExp = lastSpringExp = 0;
+    lastPartRecycleExp = 0;
 }
 
 void IO::outputStep(LB& lb, DEM& dem) {
@@ -434,6 +469,9 @@ void IO::outputStep(LB& lb, DEM& dem) {
             }
             if (!dem.bodies.empty()) exportBodyPosition(dem.bodies);
             if (!dem.walls.empty()) exportWallForce(dem.walls);
+            exportStressTensor(dem);
+            exportNormalStressContributions(dem);
+            exportShearStressContributions(dem);
         }
         if (lbmSolve && c_.initialised()) exportMass(lb);
         std::cout << "\n";
@@ -457,11 +495,12 @@ void IO::createFiles(LB& lb, DEM& dem) {
     if (!c_.initialised()) return;
     char path[1024];
     if (demSolve) {
-        // IO.cpp:212-256 (one interval, partExpTime, for all of them); the particle, force-chain and
-        // particle-recycle files carry stress diagnostics this build does not compute
-        const unsigned cv = exp_counter(realTime, partExpTime);
+        // IO.cpp:212-283 (one interval, partExpTime, for the VTK files; partRecycleExpTime for the
+        // particle-recycle file, numbered by its counter); the particle and force-chain files are not
+        // written
+        const unsigned cv = exp_counter(realTime, partExpTime), cr = exp_counter(realTime, partRecycleExpTime);
         const bool due = cv > lastVelExp || cv > lastBoundExp || (!dem.objects.empty() && cv > lastObjectExp) ||
-                         cv > lastBodyExp || cv > lastSpringExp;
+                         cv > lastBodyExp || cv > lastSpringExp || cr > lastPartRecycleExp;
         if (due) dem.refresh();
         if (cv > lastVelExp) {
             lastVelExp = cv;
@@ -487,6 +526,11 @@ void IO::createFiles(LB& lb, DEM& dem) {
             lastSpringExp = cv;
             std::snprintf(path, sizeof(path), springFileFormat.c_str(), currentTimeStep);
             exportParaviewSprings(dem.bodies, dem.objects, path);
+        }
+        if (cr > lastPartRecycleExp) {
+            lastPartRecycleExp = cr;
+            std::snprintf(path, sizeof(path), partRecycleFileFormat.c_str(), cr);
+            exportRecycleParticles(dem.particles, path);
         }
     }
     if (!lbmSolve) return;
@@ -796,6 +840,43 @@ void IO::exportWallForce(const wallList& walls) {
         w.FContact.print(f); f << "\t";
     }
     f << std::endl;
+}
+
+void IO::exportStressTensor(const DEM& dem) {
+    std::ofstream f(stressTensorFileName, std::ios::app);
+    dem.totalStress.print(f);
+    f << std::endl;
+}
+
+void IO::exportNormalStressContributions(const DEM& dem) {
+    std::ofstream f(normalStressContFileName, std::ios::app);
+    dem.hydroStress.printNormal(f); f << "\t";
+    dem.lubStress.printNormal(f); f << "\t";
+    dem.contactStress.printNormal(f); f << "\t";
+    dem.electroStress.printNormal(f);
+    f << std::endl;
+}
+
+void IO::exportShearStressContributions(const DEM& dem) {
+    std::ofstream f(shearStressContFileName, std::ios::app);
+    dem.hydroStress.printShear(f); f << "\t";
+    dem.lubStress.printShear(f); f << "\t";
+    dem.contactStress.printShear(f); f << "\t";
+    dem.electroStress.printShear(f);
+    f << std::endl;
+}
+
+// radius, position, velocity, the negated symmetrised total stress and -disp.y of every particle
+void IO::exportRecycleParticles(const particleList& particles, const std::string& file) {
+    std::ofstream f(file);
+    for (const particle& P : particles) {
+        f << P.r << " ";
+        P.x0.print(f);
+        P.x1.print(f);
+        const tMat& S = P.STotal;
+        f << -S.m00 << "\t" << -S.m11 << "\t" << -S.m22 << "\t" << -(S.m01 + S.m10) * 0.5 << "\t"
+          << -(S.m12 + S.m21) * 0.5 << "\t" << -(S.m02 + S.m20) * 0.5 << "\t" << -P.disp.y << "\n";
+    }
 }
 
 void IO::exportMass(const LB& lb) {

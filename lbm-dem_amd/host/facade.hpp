@@ -31,12 +31,25 @@ struct tVect {
     void printFixedLine(std::ostream& os) const;
 };
 
+// tinyMatrix (myvector.h) with the reference's three printers (myvector.cpp:382-392)
+struct tMat {
+    double m00 = 0, m01 = 0, m02 = 0, m10 = 0, m11 = 0, m12 = 0, m20 = 0, m21 = 0, m22 = 0;
+    void set(const double* v) {
+        m00 = v[0]; m01 = v[1]; m02 = v[2]; m10 = v[3]; m11 = v[4]; m12 = v[5]; m20 = v[6]; m21 = v[7]; m22 = v[8];
+    }
+    void print(std::ostream& os) const;        // "xx yy zz xy yz zx " (symmetrised shear)
+    void printNormal(std::ostream& os) const;  // "xx yy zz "
+    void printShear(std::ostream& os) const;   // "xy yz zx "
+};
+
 // host mirrors of the element records (fields on the path; elmt.h:15-228, utils.h:19-101)
 struct particle {
     unsigned index = 0;
     int size = 1;
     double r = 0.0, m = 0.0, fluidVolume = 0.0;
     tVect x0, x1, w1, FHydro, MHydro, FContact, MContact, FLub, MLub, FRep, FTotal, MTotal, FGrav, xiw, xio;
+    tMat STotal;  // with the stress diagnostics on
+    tVect disp;
 };
 struct object {
     unsigned index = 0;
@@ -74,6 +87,7 @@ public:
     std::vector<int> psize;
     std::vector<double> pr, px0, orad, ox0, ox1;
     std::vector<double> brad, bx0, bx1, bk, bL0;  // input/body.dat (DEM.cpp:168-196)
+    bool stress = false;  // compute the stress diagnostics (lbdem_set_stress before lbdem_init)
 
     lbdem_ctx* ctx() const { return ctx_; }
     void create();                    // lbdem_create + inputs (idempotent)
@@ -142,6 +156,8 @@ public:
     objectList objects;
     bodyList bodies;
     wallList walls;
+    // stress diagnostics of the last DEM step (DEM.h:82-86; filled by refresh when they are on)
+    tMat totalStress, contactStress, lubStress, electroStress, hydroStress;
     Context& context() const { return c_; }
 
 private:
@@ -162,6 +178,7 @@ public:
     double fluidExpTime = 0.0, fluidLagrangianExpTime = 0.0;  // main.cpp:148-151
     // main.cpp:155-162: every DEM file's interval is partExpTime
     double partExpTime = 0.0;
+    double partRecycleExpTime = 0.0;               // main.cpp:159
     unsigned int currentTimeStep = 0;
     int demSolve = 0, lbmSolve = 0;
 
@@ -177,6 +194,10 @@ private:
     void exportParaviewBodies(const bodyList& bodies, const std::string& file);                    // IO.cpp:753-784
     void exportParaviewSprings(const bodyList& bodies, const objectList& objects, const std::string& file);  // IO.cpp:786-827
     void exportMass(const LB& lb);                                        // step, time, total fluid mass
+    void exportStressTensor(const DEM& dem);                              // IO.cpp:1355-1363
+    void exportNormalStressContributions(const DEM& dem);                 // IO.cpp:1365-1386
+    void exportShearStressContributions(const DEM& dem);                  // IO.cpp:1388-1409
+    void exportRecycleParticles(const particleList& particles, const std::string& file);  // IO.cpp:999-1036
     void createFiles(LB& lb, DEM& dem);                                   // IO.cpp:181-289 (the fluid files)
     void exportRecycleFluid(const LB& lb, const std::string& file);       // IO.cpp:973-997
     void exportEulerianParaviewFluid(const LB& lb, const std::string& file);    // IO.cpp:1139-1235
@@ -191,6 +212,8 @@ private:
     Context& c_;
     unsigned lastScreenExp = 0, lastFluidRecycleExp = 0, lastFluidExp = 0, lastFluidLagrangianExp = 0;
     unsigned lastVelExp = 0, lastBoundExp = 0, lastObjectExp = 0, lastBodyExp = 0, lastSpringExp = 0;
+    unsigned lastPartRecycleExp = 0;
+    std::string partRecycleFileFormat, stressTensorFileName, normalStressContFileName, shearStressContFileName;
     std::string velFileFormat, boundFileFormat, objectFileFormat, bodyFileFormat, springFileFormat;
     std::string exportFileName, objectPositionFileName, objectForceFileName, wallForceFileName, massFileName;
     std::string bodyPositionFileName;

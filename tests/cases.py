@@ -228,8 +228,8 @@ FLUID_RUN = ("units", {"maxTime": 0.6, "screenExpTime": 0.2, "fluidExpTime": 0.3
 # the DEM VTK files (IO.cpp:212-256, every one on partExpTime) that need no stress diagnostics:
 # particle velocities, box edges, impactors, bodies, springs (kept gzip-compressed under
 # golden/dem_body_cubic/)
-DEM_RUN = ("body_cubic", {"maxTime": 20.0, "screenExpTime": 10.0, "partExpTime": 10.0})
-DEM_FILES = ("vel", "bound", "object", "body", "spring")
+DEM_RUN = ("body_cubic", {"maxTime": 20.0, "screenExpTime": 10.0, "partExpTime": 10.0, "partRecycleExpTime": 10.0})
+DEM_FILES = ("vel", "bound", "object", "body", "spring", "partRecycle")
 
 
 def host_steps(values: Dict[str, object]) -> int:
@@ -246,7 +246,8 @@ HOST_RUNS = {"suspension": {"maxTime": 39.0, "screenExpTime": 4.0},
              "fs_impactor": {"maxTime": 29.0, "screenExpTime": 3.0},
              "moving": {"maxTime": 29.0, "screenExpTime": 2.0},
              "body_cubic": {"maxTime": 39.0, "screenExpTime": 4.0}}
-HOST_FILES = ("objPosition.dat", "objForce.dat", "wallForce.dat", "bodyPosition.dat")
+HOST_FILES = ("objPosition.dat", "objForce.dat", "wallForce.dat", "bodyPosition.dat", "stressTensor.dat",
+              "normalStressCont.dat", "shearStressCont.dat")
 
 
 def make_case(root: str, name: str, overrides: Dict[str, object] = None, subdir: str = None) -> str:
