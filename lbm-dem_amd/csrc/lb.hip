@@ -1128,9 +1128,20 @@ __device__ inline void k_hydro_finalize(int nblk, const double* part, const int3
         if (c >= 7 * MAX_OBJ_PART && !diag) return;
         if (c < 7 * MAX_OBJ_PART && c / 7 >= k.nObj && c != 6) return;            // no such object
         if (c >= 7 * MAX_OBJ_PART && (c - 7 * MAX_OBJ_PART) / 3 >= k.nWall) return;  // no such wall
+        // four flag loads (and then four value loads) in flight per thread; the additions keep the
+        // order of one strided pass (a skipped block adds +0.0, which leaves the sum unchanged)
         double acc = 0.0;
-        for (int b = tid; b < nblk; b += blockDim.x)
-            if (flag[b]) acc += part[int64_t(c) * nblk + b];
+        const int bs = int(blockDim.x);
+        const double* col = part + int64_t(c) * nblk;
+        int b = tid;
+        for (; b + 3 * bs < nblk; b += 4 * bs) {
+            const int32_t f0 = flag[b], f1 = flag[b + bs], f2 = flag[b + 2 * bs], f3 = flag[b + 3 * bs];
+            const double v0 = f0 ? col[b] : 0.0, v1 = f1 ? col[b + bs] : 0.0;
+            const double v2 = f2 ? col[b + 2 * bs] : 0.0, v3 = f3 ? col[b + 3 * bs] : 0.0;
+            acc += v0; acc += v1; acc += v2; acc += v3;
+        }
+        for (; b < nblk; b += bs)
+            if (flag[b]) acc += col[b];
 #pragma unroll
         for (int off = 32; off > 0; off >>= 1) acc += __shfl_xor(acc, off, 64);
         if ((tid & 63) == 0) red[tid / 64] = acc;
