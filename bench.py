@@ -484,14 +484,40 @@ def batch_profile(prec, n_boxes=64, steps=400, tile=8):
     print(json.dumps(out), flush=True)
 
 
-def summarize_profile(paths):
+def step_chains(rows, first, last):
+    """The dispatches between fused launches `first` and `last` of a kernel trace (rows: start ns,
+    end ns, kernel name), one entry per step: the fused kernel's duration, the time to the next
+    fused launch, and for each chain of the step (free surface with the hydrodynamic finalisation;
+    DEM; coupling and edge marking) the span from its first start to its last end and its launch
+    count."""
+    rows = sorted(rows)
+    fused = [r for r in rows if "k_fused" in r[2]]
+    chain_of = (("free_surface", ("k_fs_", "k_hydro_finalize")),
+                ("dem", ("k_dem_", "k_maxdisp", "k_cells_", "k_nbr_")))
+    steps = []
+    for i in range(first, min(last, len(fused) - 1)):
+        f0, f1 = fused[i], fused[i + 1]
+        spans = {}
+        for r in rows:
+            if f0[1] <= r[0] < f1[0]:
+                ch = next((c for c, pre in chain_of if any(p in r[2] for p in pre)), "coupling_marking")
+                s = spans.setdefault(ch, [r[0], r[1], 0])
+                s[0], s[1], s[2] = min(s[0], r[0]), max(s[1], r[1]), s[2] + 1
+        steps.append({"fused_us": (f0[1] - f0[0]) / 1e3, "to_next_fused_us": (f1[0] - f0[1]) / 1e3,
+                      **{c: [(v[1] - v[0]) / 1e3, v[2]] for c, v in spans.items()}})
+    return steps
+
+
+def summarize_profile(paths, window=(1600, 1606)):
     """Reduce rocprofv3 CSV output (kernel trace or counter collection) to per-kernel means, written
     beside each CSV as <csv>.sum.json ("summary": kernel -> counter -> [launches, mean]; "per_launch":
-    every value of the fused and copy kernels); the CSV itself is removed, since a long run's
-    per-dispatch rows run to hundreds of MB."""
+    every value of the fused and copy kernels; for a kernel trace also "steps", the chains of the
+    steps between fused launches window[0] and window[1] of the process); the CSV itself is removed,
+    since a long run's per-dispatch rows run to hundreds of MB."""
     import collections
     import csv
     for path in paths:
+        rows = []
         agg = collections.defaultdict(lambda: collections.defaultdict(lambda: [0, 0.0]))
         keep = collections.defaultdict(list)
         trace = "kernel_trace" in os.path.basename(path)
@@ -499,6 +525,7 @@ def summarize_profile(paths):
             for r in csv.DictReader(fh):
                 kn = r["Kernel_Name"]
                 if trace:
+                    rows.append((int(r["Start_Timestamp"]), int(r["End_Timestamp"]), kn))
                     v, cn = (int(r["End_Timestamp"]) - int(r["Start_Timestamp"])) / 1e3, "dur_us"
                 else:
                     v, cn = float(r["Counter_Value"]), r["Counter_Name"]
@@ -515,10 +542,57 @@ def summarize_profile(paths):
         out = {"summary": {(dm[i] if i < len(dm) and dm[i] else n)[:200]: {c: [v[0], v[1] / v[0]] for c, v in agg[n].items()}
                            for i, n in enumerate(names)},
                "per_launch": {k[:40] + ".." + k[-60:]: v for k, v in keep.items()}}
+        if trace:
+            out["steps"] = step_chains(rows, *window)
         with open(path + ".sum.json", "w") as fh:
             json.dump(out, fh)
         os.remove(path)
         print("summarized", path, len(names), "kernels", flush=True)
+        if trace:
+            print(trace_tables(out))
+
+
+def kernel_label(name):
+    """k_name<args> from a (possibly mangled, truncated) kernel name: float / double / integer /
+    bool template arguments in order."""
+    import re
+    m = re.search(r"(k_[a-z0-9_]+)", name)
+    if not m:
+        return name[:60]
+    rest, args, i = name[m.end():], [], 1
+    if rest.startswith("<"):
+        return m.group(1) + rest[:rest.find(">") + 1]
+    if not rest.startswith("I"):
+        return m.group(1)
+    while i < len(rest) and rest[i] != "E":
+        if rest[i] in "fd":
+            args.append("float" if rest[i] == "f" else "double")
+            i += 1
+        elif rest.startswith(("Li", "Lb"), i) and "E" in rest[i:]:
+            j = rest.index("E", i)
+            v = rest[i + 2:j]
+            args.append(v if rest[i + 1] == "i" else ("true" if v == "1" else "false"))
+            i = j + 1
+        else:
+            break
+    return f"{m.group(1)}<{', '.join(args)}>"
+
+
+def trace_tables(out, top=36):
+    """Markdown tables of a summarised kernel trace: kernels by total time, and the steps' chains."""
+    rows = sorted(((v["dur_us"][0] * v["dur_us"][1], k, v["dur_us"][0], v["dur_us"][1])
+                   for k, v in out["summary"].items() if "dur_us" in v), reverse=True)
+    tot = sum(r[0] for r in rows) or 1.0
+    lines = ["| kernel | calls | total ms | avg us | % of GPU time |", "|---|---:|---:|---:|---:|"]
+    for t, k, c, a in rows[:top]:
+        lines.append(f"| `{kernel_label(k)}` | {c} | {t / 1e3:.2f} | {a:.1f} | {100 * t / tot:.1f} |")
+    lines += ["", "| step (after fused launch) | fused kernel us | until the next fused launch us | free-surface chain us "
+              "(launches) | DEM chain us (launches) | coupling + marking us (launches) |", "|---|---:|---:|---:|---:|---:|"]
+    for i, st in enumerate(out.get("steps", [])):
+        f, d, c = (st.get(k, [0.0, 0]) for k in ("free_surface", "dem", "coupling_marking"))
+        lines.append(f"| {i + 1} | {st['fused_us']:.1f} | {st['to_next_fused_us']:.1f} | {f[0]:.1f} ({f[1]}) | "
+                     f"{d[0]:.1f} ({d[1]}) | {c[0]:.1f} ({c[1]}) |")
+    return "\n".join(lines)
 
 
 def main():
@@ -547,9 +621,11 @@ def main():
                     help="only time every kernel of the batched workload's boxes in one batch (HIP events)")
     ap.add_argument("--summarize-profile", nargs="+", metavar="CSV",
                     help="only reduce rocprofv3 CSV files to per-kernel means (<csv>.sum.json; removes the CSV)")
+    ap.add_argument("--profile-window", type=int, nargs=2, default=[1600, 1606], metavar=("FIRST", "LAST"),
+                    help="fused launches of a kernel trace whose steps --summarize-profile breaks down by chain")
     args = ap.parse_args()
     if args.summarize_profile:
-        summarize_profile(args.summarize_profile)
+        summarize_profile(args.summarize_profile, tuple(args.profile_window))
         return
     if args.batch_profile:
         batch_profile({"fp32": lbdem.PREC_FP32, "fp64": lbdem.PREC_FP64, "fp16": lbdem.PREC_FP16}[args.precision],
