@@ -252,6 +252,11 @@ def test_stress_needs_enabling(gpu_lib, case_root):
     with pytest.raises(lbdem.LbdemError):
         sim.set_stress()
     sim.close()
+    # a context with the stress diagnostics cannot join a batch
+    sim = cases.open_sim(gpu_lib, cases.make_case(case_root, "suspension"), stress=True)
+    with pytest.raises(lbdem.LbdemError, match="stress"):
+        lbdem.Batch(gpu_lib, [sim])
+    sim.close()
 
 
 GOLD = os.path.join(os.path.dirname(os.path.abspath(__file__)), "golden")
