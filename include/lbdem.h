@@ -214,6 +214,16 @@ int lbdem_set_stress(lbdem_ctx* ctx, int enable);
  * stotal = nParticles*9 (STotal of each particle); disp = nParticles*3 (x_i - x0, DEM.cpp:2705).
  * Any pointer may be NULL. LBDEM_E_STATE unless enabled. */
 int lbdem_get_stress(lbdem_ctx* ctx, double* global, double* stotal, double* disp);
+/* Per-particle record of lbdem_get_particle_stress (doubles): the tensors of the particle VTK file
+ * (STotal, SElas and SVisc with SVisc_h, DEM.cpp:745-761, 1091-1125, 1702-1773, 2103-2138), the
+ * overlap FElas / linearStiff, disp, and the contact and normal-lubrication counts (DEM.cpp:1668,
+ * 2362). */
+enum {
+    LBDEM_S_STOTAL = 0, LBDEM_S_SELAS = 9, LBDEM_S_SVISC = 18, LBDEM_S_SVISC_H = 27, LBDEM_S_OVERLAP = 36,
+    LBDEM_S_DISP = 39, LBDEM_S_COORD = 42, LBDEM_S_LUB_COORD = 43, LBDEM_S_REC = 44
+};
+/* The last DEM step's per-particle stress records, nParticles*LBDEM_S_REC (stress diagnostics on). */
+int lbdem_get_particle_stress(lbdem_ctx* ctx, double* rec);
 /* Verlet list of every particle: count[n], list[n*cap + k] (DEM.cpp:473-652). The MI355X library
  * lists partners in ascending order, the oracle in the reference's traversal order; a pair that
  * the reference lists more than once (fewer than 3 link cells along z) appears as often. */

@@ -1523,6 +1523,38 @@ extern "C" int lbdem_get_stress(lbdem_ctx* c, double* global, double* stotal, do
     return 0;
 }
 
+// STotal as in lbdem_get_stress; SVisc_h = SHydro + SLub and SVisc = its contact part + SVisc_h
+// (DEM.cpp:749-750); overlap = FElas / linearStiff (DEM.cpp:761)
+extern "C" int lbdem_get_particle_stress(lbdem_ctx* c, double* out) {
+    NEED_INIT;
+    if (!c->stress) return c->fail(LBDEM_E_STATE, "the stress diagnostics are off (lbdem_set_stress before lbdem_init)");
+    const int np = c->dp.nPart;
+    if (np == 0) return 0;
+    std::vector<double> rec(size_t(np) * SP_REC), disp(3 * size_t(np));
+    if (fetch(c, rec.data(), c->ds.stressPart, size_t(np) * SP_REC)) return LBDEM_E_DEVICE;
+    HIPCHK(hipStreamSynchronize(c->stream));
+    if (fetch(c, disp.data(), c->ds.disp, 3 * size_t(np))) return LBDEM_E_DEVICE;
+    HIPCHK(hipStreamSynchronize(c->stream));
+    for (int i = 0; i < np; ++i) {
+        const double* R = &rec[size_t(i) * SP_REC];
+        double* O = out + size_t(i) * LBDEM_S_REC;
+        for (int e = 0; e < 9; ++e) {
+            O[LBDEM_S_STOTAL + e] = R[SP_HYDRO + e] + R[SP_LUB + e] + (R[SP_CPAIR + e] + R[SP_CWALL + e]) + R[SP_REP + e];
+            O[LBDEM_S_SELAS + e] = R[SP_ELAS + e];
+            const double vh = R[SP_HYDRO + e] + R[SP_LUB + e];
+            O[LBDEM_S_SVISC_H + e] = vh;
+            O[LBDEM_S_SVISC + e] = R[SP_VISC + e] + vh;
+        }
+        for (int a = 0; a < 3; ++a) {
+            O[LBDEM_S_OVERLAP + a] = R[SP_FELAS + a] / c->dp.linearStiff;
+            O[LBDEM_S_DISP + a] = disp[3 * size_t(i) + a];
+        }
+        O[LBDEM_S_COORD] = R[SP_COORD];
+        O[LBDEM_S_LUB_COORD] = R[SP_LUBC];
+    }
+    return 0;
+}
+
 extern "C" int lbdem_get_walls(lbdem_ctx* c, double* rec) {
     NEED_INIT;
     const int nw = c->dp.nWall;

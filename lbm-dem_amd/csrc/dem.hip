@@ -314,7 +314,8 @@ struct ForceArgs {
 };
 
 // S1, S2: the two spheres' stresslets as xx yy zz yz zx xy (with STRESS only)
-struct LubOut { V F, T1, T2; bool act; double S1[6], S2[6]; };
+// normal: within the normal range (the pair counts in lub_coord, DEM.cpp:2360-2363)
+struct LubOut { V F, T1, T2; bool act; double S1[6], S2[6]; bool normal; };
 
 // pair lubrication (DEM.cpp:1140-1340, 2302-2521) for the ordered pair (a = n1, b = n2)
 template <bool STRESS>
@@ -339,6 +340,7 @@ __device__ LubOut pair_lub(const DemParams& P, V xa, V xb, double ra, double rb,
     const double beta4 = pow((1 + beta1), 4.0);
     const double beta5 = STRESS ? pow((1 + beta1), 5.0) : 1.0;
     if (dr < dmaxd(cut, lub_N)) {
+        o.normal = true;
         Xa = 4.0 * beta2 / beta4;
         h_c = dmaxd(dr, lub_N);
         Xa *= rad / h - rad / h_c;
@@ -431,8 +433,10 @@ __device__ inline void add_arm(double* S, double ax, double ay, double az, doubl
 
 // wall-type contact with the shared per-particle history xiw (DEM.cpp:1946-2153, 2157-2296)
 __device__ void wall_like_contact(const DemParams& P, double dampCoeff, double r, V x1, V w1, V& xi, double x1w,
-                                  double y1w, double z1w, V& Fout, V& Mout) {
+                                  double y1w, double z1w, V& Fout, V& Mout, V* felOut = nullptr,
+                                  V* fvisOut = nullptr) {
 #pragma clang fp contract(off)
+    V fel{0, 0, 0}, fvis{0, 0, 0};  // elastic and viscous parts (DEM.cpp:2001-2079)
     const double r1w = sqrt(x1w * x1w + y1w * y1w + z1w * z1w);
     const double delw = r - r1w;
     double fnx = 0, fny = 0, fnz = 0, ftx = 0, fty = 0, ftz = 0, n1wx = 0, n1wy = 0, n1wz = 0;
@@ -443,6 +447,8 @@ __device__ void wall_like_contact(const DemParams& P, double dampCoeff, double r
         fnx = k * delw * n1wx - dampCoeff * gn * n1wx;
         fny = k * delw * n1wy - dampCoeff * gn * n1wy;
         fnz = k * delw * n1wz - dampCoeff * gn * n1wz;
+        fel = V{k * delw * n1wx, k * delw * n1wy, k * delw * n1wz};
+        fvis = V{-dampCoeff * gn * n1wx, -dampCoeff * gn * n1wy, -dampCoeff * gn * n1wz};
         if (mu != 0.0) {
             const double ox = r * w1.x, oy = r * w1.y, oz = r * w1.z;
             const double vx = x1.x + n1wy * oz - n1wz * oy - gn * n1wx;
@@ -464,10 +470,13 @@ __device__ void wall_like_contact(const DemParams& P, double dampCoeff, double r
                 xi.y = -(1 / kt) * (fty + dct * vy);
                 xi.z = -(1 / kt) * (ftz + dct * vz);
             }
+            fel = add(fel, V{ftx, fty, ftz});
         }
     } else {
         xi = V{0, 0, 0};
     }
+    if (felOut) *felOut = fel;
+    if (fvisOut) *fvisOut = fvis;
     Fout = V{fnx + ftx, fny + fty, fnz + ftz};
     const double dtx = fty * n1wz - ftz * n1wy, dty = ftz * n1wx - ftx * n1wz, dtz = ftx * n1wy - fty * n1wx;
     Mout = V{r * dtx, r * dty, r * dtz};
@@ -500,6 +509,9 @@ __device__ inline void k_dem_forces(ForceArgs a, double dampCoeff) {
     V Fc{0, 0, 0}, Mc{0, 0, 0}, Fl{0, 0, 0}, Ml{0, 0, 0}, Fr{0, 0, 0};
     double Sl[6] = {0, 0, 0, 0, 0, 0};  // SLub as xx yy zz yz zx xy
     double Sc[9] = {0, 0, 0, 0, 0, 0, 0, 0, 0}, Sr[9] = {0, 0, 0, 0, 0, 0, 0, 0, 0};
+    double Se[9] = {0, 0, 0, 0, 0, 0, 0, 0, 0}, Sv[9] = {0, 0, 0, 0, 0, 0, 0, 0, 0};  // SElas, SVisc
+    V Fe{0, 0, 0};
+    double coord = 0.0, lubc = 0.0;
     const int cnt = valid ? a.nbrCount[p] : 0;
     const bool waited = a.demTimeStep >= P.objWaitTime;
     // One pass over the lane's partners: the pair's data are gathered once and the three pair
@@ -520,8 +532,10 @@ __device__ inline void k_dem_forces(ForceArgs a, double dampCoeff) {
             if (o.act) {
                 if (p == n1) { Fl = add(Fl, o.F); Ml = add(Ml, o.T1); }
                 else { Fl = sub(Fl, o.F); Ml = add(Ml, o.T2); }
-                if constexpr (STRESS)
+                if constexpr (STRESS) {
                     for (int e = 0; e < 6; ++e) Sl[e] += p == n1 ? o.S1[e] : o.S2[e];
+                    if (o.normal) lubc += 1.0;
+                }
             }
         }
         // particle-particle contact (DEM.cpp:1487-1791)
@@ -533,11 +547,21 @@ __device__ inline void k_dem_forces(ForceArgs a, double dampCoeff) {
         const double gn = urx * nx + ury * ny + urz * nz;
         const double del = 2 * rad - r12;
         double fnx = 0, fny = 0, fnz = 0, ftx = 0, fty = 0, ftz = 0;
+        V fel{0, 0, 0}, fvis{0, 0, 0};  // DEM.cpp:1555-1679
         if (del > 0.0) {
             const double k_ = P.linearStiff, mu = P.frictionCoefPart;
             fnx = k_ * del * nx - dampCoeff * gn * nx;
             fny = k_ * del * ny - dampCoeff * gn * ny;
             fnz = k_ * del * nz - dampCoeff * gn * nz;
+            if constexpr (STRESS) {
+                fel = V{k_ * del * nx, k_ * del * ny, k_ * del * nz};
+                fvis = V{-dampCoeff * gn * nx, -dampCoeff * gn * ny, -dampCoeff * gn * nz};
+                // a contact counts where the box-wrapped centres are within 2 rmax (DEM.cpp:1647-1670)
+                const double x1 = dbox(A.x, P.demSize[0]), y1 = dbox(A.y, P.demSize[1]), z1 = dbox(A.z, P.demSize[2]);
+                const double x2 = dbox(B.x, P.demSize[0]), y2 = dbox(B.y, P.demSize[1]), z2 = dbox(B.z, P.demSize[2]);
+                const double zij = z1 - z2, xij = x1 - x2, yij = y1 - y2;
+                if (sqrt(zij * zij + yij * yij + xij * xij) <= 2.0 * (P.rmax)) coord += 1.0;
+            }
             if (mu != 0.0) {
                 const double ox = rA * wA.x + rB * wB.x, oy = rA * wA.y + rB * wB.y, oz = rA * wA.z + rB * wB.z;
                 const double vx = urx + ny * oz - nz * oy - gn * nx;
@@ -559,6 +583,7 @@ __device__ inline void k_dem_forces(ForceArgs a, double dampCoeff) {
                     xi.y = -(1 / kt) * (fty + dct * vy);
                     xi.z = -(1 / kt) * (ftz + dct * vz);
                 }
+                if constexpr (STRESS) fel = add(fel, V{ftx, fty, ftz});
             }
         } else {
             xi = V{0, 0, 0};
@@ -568,7 +593,12 @@ __device__ inline void k_dem_forces(ForceArgs a, double dampCoeff) {
         const double dtx = fty * nz - ftz * ny, dty = ftz * nx - ftx * nz, dtz = ftx * ny - fty * nx;
         if (p == n1) { Fc = add(Fc, V{dfx, dfy, dfz}); Mc = add(Mc, V{rA * dtx, rA * dty, rA * dtz}); }
         else { Fc = sub(Fc, V{dfx, dfy, dfz}); Mc = add(Mc, V{rB * dtx, rB * dty, rB * dtz}); }
-        if constexpr (STRESS) add_arm(Sc, x12, y12, z12, dfx, dfy, dfz);  // the same tensor for n1 and n2
+        if constexpr (STRESS) {  // the same tensors for n1 and n2; FElas with the side's sign
+            add_arm(Sc, x12, y12, z12, dfx, dfy, dfz);
+            add_arm(Se, x12, y12, z12, fel.x, fel.y, fel.z);
+            add_arm(Sv, x12, y12, z12, fvis.x, fvis.y, fvis.z);
+            Fe = p == n1 ? add(Fe, fel) : sub(Fe, fel);
+        }
         // electrostatic repulsion over every listed pair (DEM.cpp:1404-1484)
         {
             double fx, fy, fz;
@@ -586,7 +616,12 @@ __device__ inline void k_dem_forces(ForceArgs a, double dampCoeff) {
     double Sw[9] = {0, 0, 0, 0, 0, 0, 0, 0, 0};  // contact stress of walls and impactors
     if constexpr (STRESS) {
         for (int e = 0; e < 6; ++e) Sl[e] = group_sum(Sl[e]);
-        for (int e = 0; e < 9; ++e) { Sc[e] = group_sum(Sc[e]); Sr[e] = group_sum(Sr[e]); }
+        for (int e = 0; e < 9; ++e) {
+            Sc[e] = group_sum(Sc[e]); Sr[e] = group_sum(Sr[e]); Se[e] = group_sum(Se[e]); Sv[e] = group_sum(Sv[e]);
+        }
+        Fe = group_sum(Fe);
+        coord = group_sum(coord);
+        lubc = group_sum(lubc);
     }
     if (!valid || lane != 0) return;
     V xiw = ld3(a.xiw, p), xio = ld3(a.xio, p);
@@ -657,12 +692,15 @@ __device__ inline void k_dem_forces(ForceArgs a, double dampCoeff) {
             const double urx = vp.x - ov.x, ury = vp.y - ov.y, urz = vp.z - ov.z;
             const double delw = (2 * radw) - r1w;
             double fnx = 0, fny = 0, fnz = 0, ftx = 0, fty = 0, ftz = 0;
+            double felx = 0, fely = 0, felz = 0, fvisx = 0, fvisy = 0, fvisz = 0;  // DEM.cpp:970-1056
             if (delw > 0.0) {
                 const double gn = urx * n1wx + ury * n1wy + urz * n1wz;
                 const double k = P.linearStiff, mu = P.frictionCoefPart;
                 fnx = k * delw * n1wx - dampCoeff * gn * n1wx;
                 fny = k * delw * n1wy - dampCoeff * gn * n1wy;
                 fnz = k * delw * n1wz - dampCoeff * gn * n1wz;
+                felx = k * delw * n1wx; fely = k * delw * n1wy; felz = k * delw * n1wz;
+                fvisx = -dampCoeff * gn * n1wx; fvisy = -dampCoeff * gn * n1wy; fvisz = -dampCoeff * gn * n1wz;
                 if (mu != 0.0) {
                     const double o1x = rp * wp.x + orr * ow.x, o1y = rp * wp.y + orr * ow.y, o1z = rp * wp.z + orr * ow.z;
                     const double vx = urx + n1wy * o1z - n1wz * o1y - gn * n1wx;
@@ -681,10 +719,18 @@ __device__ inline void k_dem_forces(ForceArgs a, double dampCoeff) {
                     const double tx = ftx / ftt, ty = fty / ftt, tz = ftz / ftt;
                     if (fri < ftt) {
                         ftx = fri * tx; fty = fri * ty; ftz = fri * tz;
+                        // the capped split as written (DEM.cpp:1000-1007): the z term of fel goes
+                        // to y, and all three terms of fvis go to x
+                        const double fne = sqrt(felx * felx + fely * fely + felz * felz);
+                        const double fnv = sqrt(fvisx * fvisx + fvisy * fvisy + fvisz * fvisz);
+                        felx += mu * fne * tx; fely += mu * fne * ty; fely += mu * fne * tz;
+                        fvisx += mu * fnv * tx; fvisx += mu * fnv * ty; fvisx += mu * fnv * tz;
                         xio.x = -(1 / kt) * (ftx + dct * vx);
                         xio.y = -(1 / kt) * (fty + dct * vy);
                         xio.z = -(1 / kt) * (ftz + dct * vz);
                     } else {
+                        felx += -kt * xio.x; fely += -kt * xio.y; felz += -kt * xio.z;
+                        fvisx += -dct * vx; fvisy += -dct * vy; fvisz += -dct * vz;
                         xio.x += vx * P.deltat; xio.y += vy * P.deltat; xio.z += vz * P.deltat;
                     }
                 }
@@ -695,7 +741,12 @@ __device__ inline void k_dem_forces(ForceArgs a, double dampCoeff) {
             const double dtx = fty * n1wz - ftz * n1wy, dty = ftz * n1wx - ftx * n1wz, dtz = ftx * n1wy - fty * n1wx;
             Fc = add(Fc, df);
             Mc = add(Mc, V{rp * dtx, rp * dty, rp * dtz});
-            if constexpr (STRESS) add_arm(Sw, x1w, y1w, z1w, df.x, df.y, df.z);  // DEM.cpp:1097-1135
+            if constexpr (STRESS) {  // DEM.cpp:1091-1135
+                add_arm(Sw, x1w, y1w, z1w, df.x, df.y, df.z);
+                add_arm(Se, x1w, y1w, z1w, felx, fely, felz);
+                add_arm(Sv, x1w, y1w, z1w, fvisx, fvisy, fvisz);
+                Fe = add(Fe, V{felx, fely, felz});
+            }
             double* c = oc + 12 * o;
             c[0] = -df.x; c[1] = -df.y; c[2] = -df.z;
             c[3] = orr * dtx; c[4] = orr * dty; c[5] = orr * dtz;
@@ -722,11 +773,16 @@ __device__ inline void k_dem_forces(ForceArgs a, double dampCoeff) {
         const double x1w = (xp.x - P.wallP[w][0]) * P.wallN[w][0] * P.wallN[w][0];
         const double y1w = (xp.y - P.wallP[w][1]) * P.wallN[w][1] * P.wallN[w][1];
         const double z1w = (xp.z - P.wallP[w][2]) * P.wallN[w][2] * P.wallN[w][2];
-        V F, M;
-        wall_like_contact(P, dampCoeff, rp, vp, wp, xiw, x1w, y1w, z1w, F, M);
+        V F, M, fel, fvis;
+        wall_like_contact(P, dampCoeff, rp, vp, wp, xiw, x1w, y1w, z1w, F, M, &fel, &fvis);
         Fc = add(Fc, F); Mc = add(Mc, M);
         wc[6 * w + 0] -= F.x; wc[6 * w + 1] -= F.y; wc[6 * w + 2] -= F.z;
-        if constexpr (STRESS) add_arm(Sw, x1w, y1w, z1w, F.x, F.y, F.z);  // DEM.cpp:2110-2148
+        if constexpr (STRESS) {  // DEM.cpp:2103-2148
+            add_arm(Sw, x1w, y1w, z1w, F.x, F.y, F.z);
+            add_arm(Se, x1w, y1w, z1w, fel.x, fel.y, fel.z);
+            add_arm(Sv, x1w, y1w, z1w, fvis.x, fvis.y, fvis.z);
+            Fe = add(Fe, fel);
+        }
     }
     if constexpr (STRESS) {
         double* rec = a.stressPart + int64_t(p) * SP_REC;
@@ -738,7 +794,11 @@ __device__ inline void k_dem_forces(ForceArgs a, double dampCoeff) {
             rec[SP_CPAIR + e] = Sc[e];
             rec[SP_CWALL + e] = Sw[e];
             rec[SP_REP + e] = Sr[e];
+            rec[SP_ELAS + e] = Se[e];
+            rec[SP_VISC + e] = Sv[e];
         }
+        rec[SP_FELAS] = Fe.x; rec[SP_FELAS + 1] = Fe.y; rec[SP_FELAS + 2] = Fe.z;
+        rec[SP_COORD] = coord; rec[SP_LUBC] = lubc;
     }
     st3(a.FContact, p, Fc); st3(a.MContact, p, Mc); st3(a.FLub, p, Fl); st3(a.MLub, p, Ml); st3(a.FRep, p, Fr);
     st3(a.xiw, p, xiw); st3(a.xio, p, xio);

@@ -263,9 +263,11 @@ struct DemParams {
 };
 // per-particle stress record of the last DEM step (doubles): the stresslet SHydro the force kernel
 // copied from DemState::shydro, SLub, the contact stress of the particle pairs (each pair adds the
-// same tensor to both of its spheres), the contact stress of walls and impactors, SRep; each
-// m00 m01 m02 m10 ... m22
-enum { SP_HYDRO = 0, SP_LUB = 9, SP_CPAIR = 18, SP_CWALL = 27, SP_REP = 36, SP_REC = 45 };
+// same tensor to both of its spheres), the contact stress of walls and impactors, SRep, the
+// elastic and viscous parts of the contact stress (SElas, SVisc before SVisc_h is added), FElas,
+// the contact and normal-lubrication counts; tensors m00 m01 m02 m10 ... m22
+enum { SP_HYDRO = 0, SP_LUB = 9, SP_CPAIR = 18, SP_CWALL = 27, SP_REP = 36, SP_ELAS = 45, SP_VISC = 54,
+       SP_FELAS = 63, SP_COORD = 66, SP_LUBC = 67, SP_REC = 68 };
 constexpr int VEL_SLOTS = 64;  // slots of the particle speed maximum (DemState::blockMax)
 struct DemState {
     // particles (FP64, vectors stored as 3 consecutive doubles)

@@ -44,6 +44,7 @@ Api Api::load(const std::string& path) {
     bind(a.handle, path, "lbdem_store_velocity", a.store_velocity);
     bind(a.handle, path, "lbdem_set_stress", a.set_stress);
     bind(a.handle, path, "lbdem_get_stress", a.get_stress);
+    bind(a.handle, path, "lbdem_get_particle_stress", a.get_particle_stress);
     bind(a.handle, path, "lbdem_get_velocity", a.get_velocity);
     return a;
 }

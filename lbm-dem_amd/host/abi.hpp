@@ -38,6 +38,7 @@ struct Api {
     int (*get_velocity)(lbdem_ctx*, double*, double*) = nullptr;
     int (*set_stress)(lbdem_ctx*, int) = nullptr;
     int (*get_stress)(lbdem_ctx*, double*, double*, double*) = nullptr;
+    int (*get_particle_stress)(lbdem_ctx*, double*) = nullptr;
 
     void* handle = nullptr;
     std::string path;

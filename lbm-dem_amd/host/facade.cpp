@@ -3,6 +3,7 @@
 #include <sys/stat.h>
 
 #include <cmath>
+#include <utility>
 #include <cstdio>
 #include <cstdlib>
 #include <fstream>
@@ -88,7 +89,7 @@ void parseConfigFile(IO& io, DEM& dem, LB& lb, ConfigFile& cfg, CommandLine& cl)
     p.demSolve = io.demSolve;
     // the stress files of the screen interval and the particle-recycle files need the stress
     // diagnostics, which the reference computes on every DEM step
-    lb.context().stress = io.demSolve && (io.screenExpTime > 0 || io.partRecycleExpTime > 0);
+    lb.context().stress = io.demSolve && (io.screenExpTime > 0 || io.partRecycleExpTime > 0 || io.partExpTime > 0);
     p.lbSolve = io.lbmSolve;
     p.freeSurface = lb.freeSurface;
     p.forceField = lb.forceField;
@@ -344,9 +345,19 @@ void DEM::refresh() {
         c_.check(c_.api().get_stress(c_.ctx(), g.data(), st.data(), dp.data()), "lbdem_get_stress");
         tMat* t[5] = {&totalStress, &hydroStress, &lubStress, &contactStress, &electroStress};
         for (int k = 0; k < 5; ++k) t[k]->set(&g[9 * size_t(k)]);
+        std::vector<double> ps(size_t(std::max(in.nParticles, 1)) * LBDEM_S_REC);
+        c_.check(c_.api().get_particle_stress(c_.ctx(), ps.data()), "lbdem_get_particle_stress");
         for (int i = 0; i < in.nParticles; ++i) {
-            particles[i].STotal.set(&st[9 * size_t(i)]);
-            particles[i].disp = tVect(dp[3 * size_t(i)], dp[3 * size_t(i) + 1], dp[3 * size_t(i) + 2]);
+            particle& P = particles[i];
+            const double* R = &ps[size_t(i) * LBDEM_S_REC];
+            P.STotal.set(R + LBDEM_S_STOTAL); P.SElas.set(R + LBDEM_S_SELAS);
+            P.SVisc.set(R + LBDEM_S_SVISC); P.SVisc_h.set(R + LBDEM_S_SVISC_H);
+            P.overlap = rec3(R, LBDEM_S_OVERLAP);
+            P.disp = rec3(R, LBDEM_S_DISP);
+            P.coord = R[LBDEM_S_COORD]; P.lub_coord = R[LBDEM_S_LUB_COORD];
+            // DEM.cpp:751, computed by every DEM step (0 until the first)
+            P.pressure = in.demTimeStep > 0 ? -1.0 / 3.0 * (P.STotal.m00 + P.STotal.m11 + P.STotal.m22) : 0.0;
+            P.disp_r = std::sqrt(P.disp.x * P.disp.x + P.disp.z * P.disp.z);       // DEM.cpp:2712
         }
     }
     rec.assign(size_t(std::max(in.nObjects, 1)) * LBDEM_O_REC, 0.0);
@@ -415,6 +426,7 @@ void IO::initialize() {
     bodyFileFormat = partDirectory + "/body%04u.vtk";
     springFileFormat = partDirectory + "/spring%04u.vtk";
     partRecycleFileFormat = partDirectory + "/partRecycle%04u.dat";
+    partFileFormat = partDirectory + "/part%04u.vtk";
     exportFileName = workDirectory + "/export.dat";
     if (demSolve) {
         objectPositionFileName = workDirectory + "/objPosition.dat";
@@ -451,7 +463,7 @@ void IO::initialize() {
     lastScreenExp = 0;
     lastFluidRecycleExp = lastFluidExp = lastFluidLagrangianExp = 0;
     lastVelExp = lastBoundExp = lastObjectExp = lastBodyExp = lastSpringExp = 0;
-    lastPartRecycleExp = 0;
+    lastPartRecycleExp = lastPartExp = 0;
 }
 
 void IO::outputStep(LB& lb, DEM& dem) {
@@ -499,13 +511,19 @@ void IO::createFiles(LB& lb, DEM& dem) {
         // particle-recycle file, numbered by its counter); the particle and force-chain files are not
         // written
         const unsigned cv = exp_counter(realTime, partExpTime), cr = exp_counter(realTime, partRecycleExpTime);
-        const bool due = cv > lastVelExp || cv > lastBoundExp || (!dem.objects.empty() && cv > lastObjectExp) ||
-                         cv > lastBodyExp || cv > lastSpringExp || cr > lastPartRecycleExp;
+        const bool due = cv > lastVelExp || cv > lastPartExp || cv > lastBoundExp ||
+                         (!dem.objects.empty() && cv > lastObjectExp) || cv > lastBodyExp || cv > lastSpringExp ||
+                         cr > lastPartRecycleExp;
         if (due) dem.refresh();
         if (cv > lastVelExp) {
             lastVelExp = cv;
             std::snprintf(path, sizeof(path), velFileFormat.c_str(), currentTimeStep);
             exportParaviewParticleVelocities(dem.particles, path);
+        }
+        if (cv > lastPartExp) {
+            lastPartExp = cv;
+            std::snprintf(path, sizeof(path), partFileFormat.c_str(), currentTimeStep);
+            exportParaviewParticles(dem.particles, path);
         }
         if (cv > lastBoundExp) {
             lastBoundExp = cv;
@@ -716,6 +734,59 @@ void IO::exportParaviewParticleVelocities(const particleList& particles, const s
     out << "VECTORS velocity float\n";
     for (const particle& p : particles) p.x1.printFixedLine(out);
     out << "\n";
+}
+
+// the particles with their diameters, displacements, stresses, overlaps and contact counts, one
+// scalar array each in the reference's order and signs (y is the vertical: "z" in the names)
+void IO::exportParaviewParticles(const particleList& particles, const std::string& file) {
+    std::ofstream out(file);
+    const int n = int(particles.size());
+    out << "# vtk DataFile Version 3.0\n" << "Data.vtk\n" << "ASCII\n" << "DATASET POLYDATA\n" << "  \n";
+    out << "POINTS " << n << " float\n";
+    for (const particle& p : particles) p.x0.printFixedLine(out);
+    out << "\n";
+    out << "POINT_DATA " << n << "\n";
+    using F = double (*)(const particle&);
+    static const std::pair<const char*, F> scalars[] = {
+        {"diameter", [](const particle& p) { return 2.0 * p.r; }},
+        {"z-displacement", [](const particle& p) { return -p.disp.y; }},
+        {"x-displacement", [](const particle& p) { return -p.disp.x; }},
+        {"y-displacement", [](const particle& p) { return -p.disp.z; }},
+        {"z-vel", [](const particle& p) { return -p.x1.y; }},
+        {"r-displacement", [](const particle& p) { return p.disp_r; }},
+        {"sig-xx", [](const particle& p) { return p.STotal.m00; }},
+        {"sig-yy", [](const particle& p) { return p.STotal.m22; }},
+        {"pressure", [](const particle& p) { return p.pressure; }},
+        {"zz-stress", [](const particle& p) { return -p.STotal.m11; }},
+        {"xz-stress", [](const particle& p) { return -(p.STotal.m01 + p.STotal.m10) * 0.5; }},
+        {"xx-stress-el", [](const particle& p) { return -p.SElas.m00; }},
+        {"yy-stress-el", [](const particle& p) { return -p.SElas.m22; }},
+        {"zz-stress-el", [](const particle& p) { return -p.SElas.m11; }},
+        {"xz-stress-el", [](const particle& p) { return -(p.SElas.m01 + p.SElas.m10) * 0.5; }},
+        {"yz-stress-el", [](const particle& p) { return -(p.SElas.m12 + p.SElas.m21) * 0.5; }},
+        {"xy-stress-el", [](const particle& p) { return -(p.SElas.m02 + p.SElas.m20) * 0.5; }},
+        {"xx-stress-vis", [](const particle& p) { return -p.SVisc.m00; }},
+        {"yy-stress-vis", [](const particle& p) { return -p.SVisc.m22; }},
+        {"zz-stress-vis", [](const particle& p) { return -p.SVisc.m11; }},
+        {"xz-stress-vis", [](const particle& p) { return -(p.SVisc.m01 + p.SVisc.m10) * 0.5; }},
+        {"xy-stress-vis", [](const particle& p) { return -(p.SVisc.m02 + p.SVisc.m20) * 0.5; }},
+        {"yz-stress-vis", [](const particle& p) { return -(p.SVisc.m12 + p.SVisc.m21) * 0.5; }},
+        {"zz-stress-vis-h", [](const particle& p) { return -p.SVisc_h.m11; }},
+        {"xz-stress-vis-h", [](const particle& p) { return -(p.SVisc_h.m01 + p.SVisc_h.m10) * 0.5; }},
+        {"xy-stress", [](const particle& p) { return -(p.STotal.m02 + p.STotal.m20) * 0.5; }},
+        {"yz-stress", [](const particle& p) { return -(p.STotal.m12 + p.STotal.m21) * 0.5; }},
+        {"delta-z", [](const particle& p) { return p.overlap.y; }},
+        {"delta-x", [](const particle& p) { return p.overlap.x; }},
+        {"delta-y", [](const particle& p) { return p.overlap.z; }},
+        {"coord", [](const particle& p) { return p.coord; }},
+        {"lub_coord", [](const particle& p) { return p.lub_coord; }},
+    };
+    for (const auto& s : scalars) {
+        out << "SCALARS " << s.first << " float\n";
+        out << "LOOKUP_TABLE default \n";
+        for (const particle& p : particles) out << s.second(p) << "\n";
+        out << "\n";
+    }
 }
 
 // the 12 edges of the DEM box (x, fluid height with a free surface, z) as line cells

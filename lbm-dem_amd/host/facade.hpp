@@ -48,8 +48,9 @@ struct particle {
     int size = 1;
     double r = 0.0, m = 0.0, fluidVolume = 0.0;
     tVect x0, x1, w1, FHydro, MHydro, FContact, MContact, FLub, MLub, FRep, FTotal, MTotal, FGrav, xiw, xio;
-    tMat STotal;  // with the stress diagnostics on
-    tVect disp;
+    tMat STotal, SElas, SVisc, SVisc_h;  // with the stress diagnostics on
+    tVect disp, overlap;
+    double coord = 0.0, lub_coord = 0.0, pressure = 0.0, disp_r = 0.0;
 };
 struct object {
     unsigned index = 0;
@@ -189,6 +190,7 @@ private:
     void exportBodyPosition(const bodyList& bodies);                      // IO.cpp:1340-1353
     // the DEM VTK files of createFiles that need no stress diagnostics (IO.cpp:212-256)
     void exportParaviewParticleVelocities(const particleList& particles, const std::string& file);  // IO.cpp:651-700
+    void exportParaviewParticles(const particleList& particles, const std::string& file);           // IO.cpp:292-649
     void exportParaviewBoundaries(const LB& lb, const std::string& file);                          // IO.cpp:829-912
     void exportParaviewObjects(const objectList& objects, const std::string& file);                // IO.cpp:702-751
     void exportParaviewBodies(const bodyList& bodies, const std::string& file);                    // IO.cpp:753-784
@@ -212,8 +214,8 @@ private:
     Context& c_;
     unsigned lastScreenExp = 0, lastFluidRecycleExp = 0, lastFluidExp = 0, lastFluidLagrangianExp = 0;
     unsigned lastVelExp = 0, lastBoundExp = 0, lastObjectExp = 0, lastBodyExp = 0, lastSpringExp = 0;
-    unsigned lastPartRecycleExp = 0;
-    std::string partRecycleFileFormat, stressTensorFileName, normalStressContFileName, shearStressContFileName;
+    unsigned lastPartRecycleExp = 0, lastPartExp = 0;
+    std::string partFileFormat, partRecycleFileFormat, stressTensorFileName, normalStressContFileName, shearStressContFileName;
     std::string velFileFormat, boundFileFormat, objectFileFormat, bodyFileFormat, springFileFormat;
     std::string exportFileName, objectPositionFileName, objectForceFileName, wallForceFileName, massFileName;
     std::string bodyPositionFileName;

@@ -176,6 +176,7 @@ class Library:
             "lbdem_get_bodies": ([vp, _dp], C.c_int),
             "lbdem_set_stress": ([vp, C.c_int], C.c_int),
             "lbdem_get_stress": ([vp, C.c_void_p, C.c_void_p, C.c_void_p], C.c_int),
+            "lbdem_get_particle_stress": ([vp, _dp], C.c_int),
             "lbdem_get_walls": ([vp, _dp], C.c_int),
             "lbdem_get_neighbors": ([vp, C.c_int, _ip, _ip], C.c_int),
             "lbdem_total_mass": ([vp, _dp], C.c_int),
@@ -295,6 +296,15 @@ class Simulation:
         self._check(self.lib.lib.lbdem_get_stress(self.ctx, g.ctypes.data, st.ctypes.data, d.ctypes.data),
                     "lbdem_get_stress")
         return g.reshape(5, 3, 3), st[:9 * n].reshape(n, 3, 3), d[:3 * n].reshape(n, 3)
+
+    def particle_stress(self) -> np.ndarray:
+        """n x 44 records of lbdem_get_particle_stress (STotal, SElas, SVisc, SVisc_h, overlap, disp,
+        coord, lub_coord) of the last DEM step."""
+        n = self.info()["nParticles"]
+        rec = np.zeros((max(n, 1), 44), np.float64)
+        self._check(self.lib.lib.lbdem_get_particle_stress(self.ctx, _ptr(rec, C.c_double)),
+                    "lbdem_get_particle_stress")
+        return rec[:n]
 
     def store_velocity(self):
         """The next LB step keeps every active node's velocity (read with velocity())."""

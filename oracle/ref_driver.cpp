@@ -142,6 +142,20 @@ static void dump(const std::string& path, LB& lb, DEM& dem, IO& io) {
         const double v[3] = {d.x, d.y, d.z};
         std::fwrite(v, 8, 3, fp);
     }
+    // the rest of the particle VTK file's values, in the layout of lbdem_get_particle_stress
+    for (size_t i = 0; i < dem.particles.size(); ++i) {
+        const particle& P = dem.particles[i];
+        double r[LBDEM_S_REC] = {};
+        const tMat* t[4] = {&P.STotal, &P.SElas, &P.SVisc, &P.SVisc_h};
+        for (int k = 0; k < 4; ++k) {
+            const double m[9] = {t[k]->m00, t[k]->m01, t[k]->m02, t[k]->m10, t[k]->m11, t[k]->m12, t[k]->m20, t[k]->m21, t[k]->m22};
+            for (int e = 0; e < 9; ++e) r[9 * k + e] = m[e];
+        }
+        r[LBDEM_S_OVERLAP] = P.overlap.x; r[LBDEM_S_OVERLAP + 1] = P.overlap.y; r[LBDEM_S_OVERLAP + 2] = P.overlap.z;
+        r[LBDEM_S_DISP] = P.disp.x; r[LBDEM_S_DISP + 1] = P.disp.y; r[LBDEM_S_DISP + 2] = P.disp.z;
+        r[LBDEM_S_COORD] = P.coord; r[LBDEM_S_LUB_COORD] = P.lub_coord;
+        std::fwrite(r, 8, LBDEM_S_REC, fp);
+    }
     std::fclose(fp);
 }
 

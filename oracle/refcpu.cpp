@@ -81,6 +81,9 @@ struct Particle {  // particle (elmt.h:15-158), fields used on the path
     V3 x0, x1, w1, x0_lst;
     V3 x_i, disp;                             // position at the start (or at objWaitTime) and x_i - x0
     T9 SHydro, SLub, SContact, SRep, STotal;  // stress diagnostics (DEM.cpp:655-771)
+    T9 SElas, SVisc, SVisc_h;                 // elastic and viscous parts of the contact stress
+    V3 FElas, overlap;
+    double coord = 0, lub_coord = 0;          // contacts and normal-lubrication pairs
     V3 FHydro, FContact, FTotal, FLub, FGrav, FRep, MHydro, MContact, MTotal, MLub;
     double fluidVolume = 0;
     int list[MAXL], _list[MAXL];
@@ -1054,7 +1057,8 @@ struct lbdem_ctx {
 
     // Jeffrey-Onishi / Nguyen-Ladd lubrication coefficients shared by the particle-particle
     // and object-particle routines (DEM.cpp:1181-1257, 2349-2416)
-    struct LubOut { V3 F, T1, T2; double S1[6] = {0, 0, 0, 0, 0, 0}, S2[6] = {0, 0, 0, 0, 0, 0}; };  // S: xx yy zz yz zx xy
+    // S: xx yy zz yz zx xy; normal: within the normal range (DEM.cpp:2360-2363 counts the pair)
+    struct LubOut { V3 F, T1, T2; double S1[6] = {0, 0, 0, 0, 0, 0}, S2[6] = {0, 0, 0, 0, 0, 0}; bool normal = false; };
     LubOut pairLub(const V3& xa, const V3& xb, double ra, double rb, const V3& va, const V3& vb, const V3& wa, const V3& wb,
                    double beta1_num, double beta1_den, bool& active) const {
         LubOut o;
@@ -1078,6 +1082,7 @@ struct lbdem_ctx {
         const double beta4 = std::pow((1 + beta1), 4.0);
         const double beta5 = std::pow((1 + beta1), 5.0);
         if (dr < dmax(cut, lub_N)) {
+            o.normal = true;
             Xa = 4.0 * beta2 / beta4;
             Xg1 = 12.0 * beta2 / beta5;
             Xg2 = -12.0 * beta3 / beta5;
@@ -1186,6 +1191,7 @@ struct lbdem_ctx {
                 if (!act) continue;
                 A.FLub += o.F; B.FLub -= o.F; A.MLub += o.T1; B.MLub += o.T2;
                 A.SLub.addSym(o.S1); B.SLub.addSym(o.S2);
+                if (o.normal) { A.lub_coord += 1.0; B.lub_coord += 1.0; }
             }
         }
     }
@@ -1248,7 +1254,9 @@ struct lbdem_ctx {
     }
     // tangential spring-dashpot with Coulomb cap, shared by the wall and fictional-lid contacts
     // (DEM.cpp:1962-2063, 2175-2254); xi is the per-particle wall history xiw.
-    void wallLikeContact(Particle& P, double x1w, double y1w, double z1w, V3& Fout, V3& Mout) {
+    void wallLikeContact(Particle& P, double x1w, double y1w, double z1w, V3& Fout, V3& Mout, V3* felOut = nullptr,
+                         V3* fvisOut = nullptr) {
+        double felx = 0, fely = 0, felz = 0, fvisx = 0, fvisy = 0, fvisz = 0;  // DEM.cpp:2001-2079
         const double radw = P.r;
         const double r1w = std::sqrt(x1w * x1w + y1w * y1w + z1w * z1w);
         const double urxw = P.x1.x, uryw = P.x1.y, urzw = P.x1.z;
@@ -1261,6 +1269,8 @@ struct lbdem_ctx {
             fnx = k * delw * n1wx - dampCoeff * gn1w * n1wx;
             fny = k * delw * n1wy - dampCoeff * gn1w * n1wy;
             fnz = k * delw * n1wz - dampCoeff * gn1w * n1wz;
+            felx = k * delw * n1wx; fely = k * delw * n1wy; felz = k * delw * n1wz;
+            fvisx = -dampCoeff * gn1w * n1wx; fvisy = -dampCoeff * gn1w * n1wy; fvisz = -dampCoeff * gn1w * n1wz;
             if (muw != 0.0) {
                 const double o1wx = P.r * P.w1.x, o1wy = P.r * P.w1.y, o1wz = P.r * P.w1.z;
                 const double v1wx = urxw + n1wy * o1wz - n1wz * o1wy - gn1w * n1wx;
@@ -1284,10 +1294,13 @@ struct lbdem_ctx {
                     P.xiw.y = -(1 / ktw) * (fty + dct * v1wy);
                     P.xiw.z = -(1 / ktw) * (ftz + dct * v1wz);
                 }
+                felx += ftx; fely += fty; felz += ftz;
             }
         } else {
             P.xiw = V3();
         }
+        if (felOut) *felOut = V3(felx, fely, felz);
+        if (fvisOut) *fvisOut = V3(fvisx, fvisy, fvisz);
         const double dfx = fnx + ftx, dfy = fny + fty, dfz = fnz + ftz;
         const double dtx = fty * n1wz - ftz * n1wy, dty = ftz * n1wx - ftx * n1wz, dtz = ftx * n1wy - fty * n1wx;
         Fout = V3(dfx, dfy, dfz);
@@ -1300,8 +1313,12 @@ struct lbdem_ctx {
                 const double y1w = (P.x0.y - W.p.y) * W.n.y * W.n.y;
                 const double z1w = (P.x0.z - W.p.z) * W.n.z * W.n.z;
                 V3 Fc, Mc;
-                wallLikeContact(P, x1w, y1w, z1w, Fc, Mc);
+                V3 fel, fvis;
+                wallLikeContact(P, x1w, y1w, z1w, Fc, Mc, &fel, &fvis);
                 P.FContact += Fc; W.FContact -= Fc; P.MContact += Mc;
+                P.FElas = P.FElas + fel;  // DEM.cpp:2103-2138
+                P.SElas.addArm(x1w, y1w, z1w, fel.x, fel.y, fel.z);
+                P.SVisc.addArm(x1w, y1w, z1w, fvis.x, fvis.y, fvis.z);
                 P.SContact.addArm(x1w, y1w, z1w, Fc.x, Fc.y, Fc.z);  // DEM.cpp:2110-2148
                 contactStress.addArm(x1w, y1w, z1w, Fc.x, Fc.y, Fc.z);
             }
@@ -1323,12 +1340,15 @@ struct lbdem_ctx {
                 const double urxw = P.x1.x - O.x1.x, uryw = P.x1.y - O.x1.y, urzw = P.x1.z - O.x1.z;
                 const double delw = (2 * radw) - r1w;
                 double fnx = 0, fny = 0, fnz = 0, ftx = 0, fty = 0, ftz = 0;
+                double felx = 0, fely = 0, felz = 0, fvisx = 0, fvisy = 0, fvisz = 0;  // DEM.cpp:970-1056
                 if (delw > 0.0) {
                     const double gn1w = urxw * n1wx + uryw * n1wy + urzw * n1wz;
                     const double muw = prm.frictionCoefPart, k = prm.linearStiff;
                     fnx = k * delw * n1wx - dampCoeff * gn1w * n1wx;
                     fny = k * delw * n1wy - dampCoeff * gn1w * n1wy;
                     fnz = k * delw * n1wz - dampCoeff * gn1w * n1wz;
+                    felx = k * delw * n1wx; fely = k * delw * n1wy; felz = k * delw * n1wz;
+                    fvisx = -dampCoeff * gn1w * n1wx; fvisy = -dampCoeff * gn1w * n1wy; fvisz = -dampCoeff * gn1w * n1wz;
                     if (muw != 0.0) {
                         const double o1wx = P.r * P.w1.x + O.r * O.w1.x;
                         const double o1wy = P.r * P.w1.y + O.r * O.w1.y;
@@ -1347,10 +1367,18 @@ struct lbdem_ctx {
                         const double t1wx = ftx / fttw, t1wy = fty / fttw, t1wz = ftz / fttw;
                         if (friw < fttw) {
                             ftx = friw * t1wx; fty = friw * t1wy; ftz = friw * t1wz;
+                            // the capped split as written (DEM.cpp:1000-1007): the z term of fel
+                            // goes to y, and all three terms of fvis go to x
+                            const double fne = std::sqrt(felx * felx + fely * fely + felz * felz);
+                            const double fnv = std::sqrt(fvisx * fvisx + fvisy * fvisy + fvisz * fvisz);
+                            felx += muw * fne * t1wx; fely += muw * fne * t1wy; fely += muw * fne * t1wz;
+                            fvisx += muw * fnv * t1wx; fvisx += muw * fnv * t1wy; fvisx += muw * fnv * t1wz;
                             P.xio.x = -(1 / ktw) * (ftx + dct * v1wx);
                             P.xio.y = -(1 / ktw) * (fty + dct * v1wy);
                             P.xio.z = -(1 / ktw) * (ftz + dct * v1wz);
                         } else {
+                            felx += -ktw * P.xio.x; fely += -ktw * P.xio.y; felz += -ktw * P.xio.z;
+                            fvisx += -dct * v1wx; fvisy += -dct * v1wy; fvisz += -dct * v1wz;
                             P.xio.x += v1wx * deltat; P.xio.y += v1wy * deltat; P.xio.z += v1wz * deltat;
                         }
                     }
@@ -1364,6 +1392,9 @@ struct lbdem_ctx {
                 P.MContact += V3(P.r * dtx, P.r * dty, P.r * dtz);
                 O.MContact += V3(O.r * dtx, O.r * dty, O.r * dtz);
                 P.SContact.addArm(x1w, y1w, z1w, dfx, dfy, dfz);  // DEM.cpp:1097-1135
+                P.FElas = P.FElas + V3(felx, fely, felz);
+                P.SElas.addArm(x1w, y1w, z1w, felx, fely, felz);
+                P.SVisc.addArm(x1w, y1w, z1w, fvisx, fvisy, fvisz);
                 contactStress.addArm(x1w, y1w, z1w, dfx, dfy, dfz);
             }
     }
@@ -1393,10 +1424,13 @@ struct lbdem_ctx {
                 const double gn12 = urx * n12x + ury * n12y + urz * n12z;
                 const double del = 2 * rad - r12;
                 double fnx = 0, fny = 0, fnz = 0, ftx = 0, fty = 0, ftz = 0;
+                double felx = 0, fely = 0, felz = 0, fvisx = 0, fvisy = 0, fvisz = 0;  // DEM.cpp:1555-1679
                 if (del > 0.0) {
                     fnx = k * del * n12x - dampCoeff * gn12 * n12x;
                     fny = k * del * n12y - dampCoeff * gn12 * n12y;
                     fnz = k * del * n12z - dampCoeff * gn12 * n12z;
+                    felx = k * del * n12x; fely = k * del * n12y; felz = k * del * n12z;
+                    fvisx = -dampCoeff * gn12 * n12x; fvisy = -dampCoeff * gn12 * n12y; fvisz = -dampCoeff * gn12 * n12z;
                     if (mu != 0.0) {
                         const double o12x = A.r * A.w1.x + B.r * B.w1.x;
                         const double o12y = A.r * A.w1.y + B.r * B.w1.y;
@@ -1422,7 +1456,15 @@ struct lbdem_ctx {
                             A.xiy[nn] = -(1 / kt) * (fty + dct * v12y);
                             A.xiz[nn] = -(1 / kt) * (ftz + dct * v12z);
                         }
+                        felx += ftx; fely += fty; felz += ftz;
                     }
+                    // a contact counts where the box-wrapped centres are within 2 maxPartRadius
+                    // (DEM.cpp:1647-1670)
+                    const double x1 = dbox(A.x0.x, demSize[0]), y1 = dbox(A.x0.y, demSize[1]), z1 = dbox(A.x0.z, demSize[2]);
+                    const double x2 = dbox(B.x0.x, demSize[0]), y2 = dbox(B.x0.y, demSize[1]), z2 = dbox(B.x0.z, demSize[2]);
+                    const double zij = z1 - z2, xij = x1 - x2, yij = y1 - y2;
+                    const double rij = std::sqrt(zij * zij + yij * yij + xij * xij);
+                    if (rij <= 2.0 * (maxPartRadius)) { A.coord += 1.0; B.coord += 1.0; }
                 } else {
                     A.xix[nn] = A.xiy[nn] = A.xiz[nn] = 0.0;
                 }
@@ -1434,6 +1476,10 @@ struct lbdem_ctx {
                 B.MContact += V3(B.r * dtx, B.r * dty, B.r * dtz);
                 A.SContact.addArm(x12, y12, z12, dfx, dfy, dfz);  // DEM.cpp:1715-1783
                 B.SContact.addArm(x12, y12, z12, dfx, dfy, dfz);
+                A.FElas = A.FElas + V3(felx, fely, felz);
+                B.FElas = B.FElas - V3(felx, fely, felz);
+                A.SElas.addArm(x12, y12, z12, felx, fely, felz); A.SVisc.addArm(x12, y12, z12, fvisx, fvisy, fvisz);
+                B.SElas.addArm(x12, y12, z12, felx, fely, felz); B.SVisc.addArm(x12, y12, z12, fvisx, fvisy, fvisz);
                 contactStress.addArm(x12, y12, z12, dfx, dfy, dfz);
             }
         }
@@ -1473,6 +1519,7 @@ struct lbdem_ctx {
             P.FTotal.reset(); P.MTotal.reset(); P.FContact.reset(); P.MContact.reset();
             P.FLub.reset(); P.MLub.reset(); P.FRep.reset();
             P.SLub.reset(); P.SContact.reset(); P.SRep.reset(); P.STotal.reset();
+            P.FElas.reset(); P.SElas.reset(); P.SVisc.reset(); P.SVisc_h.reset(); P.coord = 0.0; P.lub_coord = 0.0;
         }
         for (auto& O : objects) {
             O.FTotal.reset(); O.FContact.reset(); O.FLub.reset(); O.FSpring.reset();
@@ -1497,7 +1544,10 @@ struct lbdem_ctx {
                 lubStress.m[e] += P.SLub.m[e];
                 hydroStress.m[e] += P.SHydro.m[e];
                 P.STotal.m[e] += P.SHydro.m[e] + P.SLub.m[e] + P.SContact.m[e] + P.SRep.m[e];
+                P.SVisc_h.m[e] += P.SHydro.m[e] + P.SLub.m[e];
+                P.SVisc.m[e] += P.SVisc_h.m[e];
             }
+            P.overlap = P.FElas / prm.linearStiff;  // DEM.cpp:761
             P.FTotal += P.FHydro + P.FContact + P.FLub + P.FGrav + P.FRep;
             P.MTotal += P.MHydro + P.MContact + P.MLub;
         }
@@ -1835,6 +1885,21 @@ int lbdem_get_stress(lbdem_ctx* c, double* global, double* stotal, double* disp)
         const Particle& P = c->particles[i];
         if (stotal) for (int e = 0; e < 9; ++e) stotal[9 * i + e] = P.STotal.m[e];
         if (disp) put3(disp + 3 * i, P.disp);
+    }
+    return 0;
+}
+int lbdem_get_particle_stress(lbdem_ctx* c, double* rec) {
+    NEED_INIT;
+    for (size_t i = 0; i < c->particles.size(); ++i) {
+        const Particle& P = c->particles[i];
+        double* R = rec + i * LBDEM_S_REC;
+        for (int e = 0; e < 9; ++e) {
+            R[LBDEM_S_STOTAL + e] = P.STotal.m[e]; R[LBDEM_S_SELAS + e] = P.SElas.m[e];
+            R[LBDEM_S_SVISC + e] = P.SVisc.m[e]; R[LBDEM_S_SVISC_H + e] = P.SVisc_h.m[e];
+        }
+        put3(R + LBDEM_S_OVERLAP, P.overlap);
+        put3(R + LBDEM_S_DISP, P.disp);
+        R[LBDEM_S_COORD] = P.coord; R[LBDEM_S_LUB_COORD] = P.lub_coord;
     }
     return 0;
 }

@@ -229,7 +229,7 @@ FLUID_RUN = ("units", {"maxTime": 0.6, "screenExpTime": 0.2, "fluidExpTime": 0.3
 # particle velocities, box edges, impactors, bodies, springs (kept gzip-compressed under
 # golden/dem_body_cubic/)
 DEM_RUN = ("body_cubic", {"maxTime": 20.0, "screenExpTime": 10.0, "partExpTime": 10.0, "partRecycleExpTime": 10.0})
-DEM_FILES = ("vel", "bound", "object", "body", "spring", "partRecycle")
+DEM_FILES = ("vel", "bound", "object", "body", "spring", "partRecycle", "part0")
 
 
 def host_steps(values: Dict[str, object]) -> int:
@@ -315,6 +315,9 @@ def read_dump(path: str) -> Dict[str, object]:
         out["stress"] = take(np.float64, 45)
         out["stotal"] = take(np.float64, npart * 9).reshape(npart, 9)
         out["disp"] = take(np.float64, npart * 3).reshape(npart, 3)
+    if off < len(buf):  # the particle VTK values (drivers that write them)
+        npart = out["particles"].shape[0]
+        out["pstress"] = take(np.float64, npart * 44).reshape(npart, 44)
     return out
 
 
@@ -340,6 +343,7 @@ def state(sim: "lbdem.Simulation", with_f: bool = True, stress: bool = False) ->
     if stress:  # in the layout of the reference dumps
         g, st, d = sim.stress()
         out.update(stress=g.reshape(-1), stotal=st.reshape(-1, 9), disp=d)
+        out.update(pstress=sim.particle_stress())
     return out
 
 

@@ -167,7 +167,7 @@ def main():
                 if dump["N"] > SAMPLE_ABOVE:
                     rng = np.random.default_rng(1000 + s)
                     sel = np.sort(rng.choice(sel, size=min(SAMPLE_NODES, sel.size), replace=False))
-                extra = {k: dump[k] for k in ("stress", "stotal", "disp") if k in dump}
+                extra = {k: dump[k] for k in ("stress", "stotal", "disp", "pstress") if k in dump}
                 np.savez_compressed(
                     out, **extra, type=dump["type"], pflag=dump["pflag"], solid=dump["solid"], active=sel,
                     sampled=np.array(dump["N"] > SAMPLE_ABOVE),

@@ -215,6 +215,21 @@ STRESS_TOL = {lbdem.PREC_FP64: (1e-9, 1e-10), lbdem.PREC_FP32: (2e-2, 1e-4)}
 STRESS_NAMES = ("totalStress", "hydroStress", "lubStress", "contactStress", "electroStress")
 
 
+def compare_particle_stress(A, B, prec, where):
+    """n x 44 records of lbdem_get_particle_stress (A the oracle or the reference): each tensor
+    relative to its largest entry, overlap and disp absolutely, the counts exactly (FP64)."""
+    tol, dtol = STRESS_TOL[prec]
+    A, B = np.asarray(A), np.asarray(B)
+    assert A.shape == B.shape, f"{where}: particle stress shape"
+    if A.size == 0:
+        return
+    for k, nm in enumerate(("STotal", "SElas", "SVisc", "SVisc_h")):
+        assert rel(A[:, 9 * k:9 * k + 9], B[:, 9 * k:9 * k + 9]) <= tol, f"{where}: {nm}"
+    assert np.abs(A[:, 36:42] - B[:, 36:42]).max() <= max(dtol, tol * np.abs(A[:, 36:39]).max()), f"{where}: overlap, disp"
+    if prec == lbdem.PREC_FP64:
+        assert np.array_equal(A[:, 42:44], B[:, 42:44]), f"{where}: contact and lubrication counts"
+
+
 def compare_stress(A, B, prec, where):
     """A, B: (global 5x3x3, STotal n x 3 x 3, disp n x 3), A the oracle or the reference."""
     tol, dtol = STRESS_TOL[prec]
@@ -241,6 +256,7 @@ def test_stress_diagnostics_match_oracle(oracle_lib, gpu_lib, case_root, name, p
         A, B = a.stress(), b.stress()
         assert np.abs(A[0][1]).max() > 0 or name == "periodic", "the window has hydrodynamic stress"
         compare_stress(A, B, prec, f"{name} step {step}")
+        compare_particle_stress(a.particle_stress(), b.particle_stress(), prec, f"{name} step {step}")
     a.close()
     b.close()
 
@@ -283,6 +299,9 @@ def test_fp64_matches_reference_fixtures(gpu_lib, case_root, name):
             n = g["particles"].shape[0]
             compare_stress((g["stress"].reshape(5, 3, 3), g["stotal"].reshape(n, 3, 3), g["disp"]), sim.stress(),
                            lbdem.PREC_FP64, f"{name} step {step} (reference)")
+            if "pstress" in g:
+                compare_particle_stress(g["pstress"], sim.particle_stress(), lbdem.PREC_FP64,
+                                        f"{name} step {step} (reference)")
         assert np.array_equal(st["type"], g["type"]) and np.array_equal(st["pflag"], g["pflag"])
         assert np.array_equal(st["solid"], g["solid"])
         act = g["active"]

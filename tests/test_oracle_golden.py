@@ -49,8 +49,9 @@ def test_oracle_matches_reference_fixtures(oracle_lib, case_root, name):
         assert st["totalMass"] == float(g["totalMass"]), f"{name} step {step}: total fluid mass"
         assert st["info"]["totalMassInit"] == float(g["totalMassInit"])
         if "stress" in g:
-            for k in ("stress", "stotal", "disp"):
-                assert np.array_equal(st[k], g[k]), f"{name} step {step}: {k}"
+            for k in ("stress", "stotal", "disp", "pstress"):
+                if k in g:
+                    assert np.array_equal(st[k], g[k]), f"{name} step {step}: {k}"
 
 
 def test_fixtures_exercise_every_event(oracle_lib, case_root):
