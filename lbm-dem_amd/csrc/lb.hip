@@ -188,29 +188,51 @@ void lb_download_velocity(const Box& b, double* out, hipStream_t s) {
 // SHydro of every particle (LB.cpp:1951-1959): the sum over the active nodes inside it of
 // radius (x) diffVel in physical units, with diffVel = -(the node's hydrodynamic force), which the
 // LB step just stored in aux planes 4-6 (it runs with every active node's values stored). One
-// thread per (x, y) column walks z; its sums go out, 9 atomics, whenever the particle changes.
+// thread per (x, y) column and run of HS_ZRUN nodes in z walks its run and sums in registers while
+// the particle is unchanged; the sums go to the block's LDS table (global atomics when it is
+// full), which adds them to the particles once per block.
 __device__ inline void k_zero_doubles(double* p, int64_t n) {
     for (int64_t i = int64_t(blockIdx.x) * blockDim.x + threadIdx.x; i < n; i += int64_t(gridDim.x) * blockDim.x)
         p[i] = 0.0;
+}
+constexpr int HS_ZRUN = 16, HS_SLOTS = 128;
+__device__ inline void hydro_stress_add(int key, const double* acc, int* hk, double (*hv)[9], double* shydro) {
+    int slot = key & (HS_SLOTS - 1), probes = 0;
+    for (; probes < HS_SLOTS; ++probes) {
+        const int old = atomicCAS(&hk[slot], -1, key);
+        if (old == -1 || old == key) break;
+        slot = (slot + 1) & (HS_SLOTS - 1);
+    }
+    if (probes < HS_SLOTS)
+        for (int e = 0; e < 9; ++e) atomicAdd(&hv[slot][e], acc[e]);
+    else
+        for (int e = 0; e < 9; ++e) atomicAdd(shydro + 9 * int64_t(key) + e, acc[e]);
 }
 template <typename M>
 __device__ inline void k_hydro_stress(Geom g, const uint8_t* type, const int32_t* solid, const M* aux,
                                       const PartLB* plb, double L, double F, double* shydro) {
 #pragma clang fp contract(off)
+    __shared__ int hk[HS_SLOTS];
+    __shared__ double hv[HS_SLOTS][9];
+    for (int s = threadIdx.x; s < HS_SLOTS; s += blockDim.x) {
+        hk[s] = -1;
+        for (int e = 0; e < 9; ++e) hv[s][e] = 0.0;
+    }
+    __syncthreads();
     const int64_t SZ = int64_t(g.X) * g.Y, N = g.N;
-    const int64_t col = int64_t(blockIdx.x) * blockDim.x + threadIdx.x;
-    if (col >= SZ) return;
+    const int64_t t = int64_t(blockIdx.x) * blockDim.x + threadIdx.x;
+    const int64_t col = t % SZ, run = t / SZ;
     const int x = int(col % g.X), y = int(col / g.X);
+    const int z0 = int(run) * HS_ZRUN, z1 = min(g.Z, z0 + HS_ZRUN);  // empty past the last run
     int key = -1;
     double acc[9];
     for (int e = 0; e < 9; ++e) acc[e] = 0.0;
-    for (int z = 0; z < g.Z; ++z) {
+    for (int z = z0; z < z1; ++z) {
         const int64_t i = col + SZ * z;
-        const uint8_t t = type[i];
-        const int k = (is_active(t) && (t & T_PMASK)) ? solid[i] : -1;
+        const uint8_t ty = type[i];
+        const int k = (is_active(ty) && (ty & T_PMASK)) ? solid[i] : -1;
         if (k != key) {
-            if (key >= 0)
-                for (int e = 0; e < 9; ++e) atomicAdd(shydro + 9 * int64_t(key) + e, acc[e]);
+            if (key >= 0) hydro_stress_add(key, acc, hk, hv, shydro);
             key = k;
             for (int e = 0; e < 9; ++e) acc[e] = 0.0;
         }
@@ -221,15 +243,18 @@ __device__ inline void k_hydro_stress(Geom g, const uint8_t* type, const int32_t
         for (int a = 0; a < 3; ++a)
             for (int c = 0; c < 3; ++c) acc[3 * a + c] += r[a] * L * d[c] * F;
     }
-    if (key >= 0)
-        for (int e = 0; e < 9; ++e) atomicAdd(shydro + 9 * int64_t(key) + e, acc[e]);
+    if (key >= 0) hydro_stress_add(key, acc, hk, hv, shydro);
+    __syncthreads();
+    for (int s = threadIdx.x; s < HS_SLOTS; s += blockDim.x)
+        if (hk[s] >= 0)
+            for (int e = 0; e < 9; ++e) atomicAdd(shydro + 9 * int64_t(hk[s]) + e, hv[s][e]);
 }
 void lb_hydro_stress(const Box& b, const PartLB* plb, int nPart, double L, double F, double* shydro, hipStream_t s) {
     const int64_t n9 = 9 * int64_t(nPart > 0 ? nPart : 1);
     launch<&k_zero_doubles, 1024, 1>(int((n9 + 255) / 256), 256, s, shydro, n9);
     if (nPart <= 0) return;
-    const int64_t cols = int64_t(b.g.X) * b.g.Y;
-    const int nb = int((cols + 255) / 256);
+    const int64_t threads = int64_t(b.g.X) * b.g.Y * ((b.g.Z + HS_ZRUN - 1) / HS_ZRUN);
+    const int nb = int((threads + 255) / 256);
     if (b.prec == LBDEM_PREC_FP64)
         launch<&k_hydro_stress<double>, 1024, 1>(nb, 256, s, b.g, b.type, b.solid, (const double*)b.aux, plb, L, F, shydro);
     else
